@@ -1,0 +1,1272 @@
+// cp_oracle — FP64 CPU restatement of ConstraintPhysics' per-step contact
+// pipeline.  TEST INFRASTRUCTURE ONLY (see cp_oracle.h).  Scalar code that
+// keeps the reference's data flow and floating-point operation order; compiled
+// with -ffp-contract=off.  Reference citations are file:line in
+// NathanMacLeod/ConstraintPhysics.
+//
+// Parity status (pinned by tests/test_oracle.py against reference fixtures):
+//   world geometry / AABBs, broadphase set, SAT poly-poly + clipping + cull,
+//   contact persistence, island order, row build, PGS, integration: pinned
+//   bit-for-bit on C0-pattern box piles.
+#include "cp_oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <set>
+#include <vector>
+
+namespace {
+
+// ---------------------------------------------------------------- math (Math/src)
+struct V3 {
+	double x = 0, y = 0, z = 0;
+	V3() {}
+	V3(double a, double b, double c) : x(a), y(b), z(c) {}
+	V3 operator+(V3 v) const { return V3(x + v.x, y + v.y, z + v.z); }
+	V3 operator-(V3 v) const { return V3(x - v.x, y - v.y, z - v.z); }
+	V3 operator-() const { return V3(-x, -y, -z); }
+	V3 operator*(double d) const { return V3(x * d, y * d, z * d); }  // Vec3.cpp:4-9
+	V3 operator/(double d) const { return V3(x / d, y / d, z / d); }
+	void operator+=(V3 v) { x += v.x; y += v.y; z += v.z; }
+	void operator-=(V3 v) { x -= v.x; y -= v.y; z -= v.z; }
+	double dot(V3 v) const { return x * v.x + y * v.y + z * v.z; }
+	V3 cross(V3 v) const { return V3(y * v.z - z * v.y, z * v.x - x * v.z, x * v.y - y * v.x); }
+	double magSqrd() const { return x * x + y * y + z * z; }
+	double mag() const { return std::sqrt(x * x + y * y + z * z); }
+	V3 normalize() const {  // Vec3.h:38-41
+		double m = mag();
+		return (m == 0) ? V3(0, 0, 0) : V3(x / m, y / m, z / m);
+	}
+	bool operator!=(V3 v) const { return x != v.x || y != v.y || z != v.z; }
+	// Vec3.h:53-57
+	void perpBasis(V3* u, V3* w) const {
+		V3 a1(1, 0, 0), a2(0, 1, 0);
+		*u = (std::abs(dot(a1)) < std::abs(dot(a2))) ? cross(a1).normalize() : cross(a2).normalize();
+		*w = cross(*u).normalize();
+	}
+};
+static inline V3 operator*(double d, V3 v) { return V3(v.x * d, v.y * d, v.z * d); }
+
+struct M3 {
+	double v[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+	V3 operator*(V3 w) const {  // Mat3.h:70-74
+		return V3(v[0][0] * w.x + v[0][1] * w.y + v[0][2] * w.z, v[1][0] * w.x + v[1][1] * w.y + v[1][2] * w.z,
+		          v[2][0] * w.x + v[2][1] * w.y + v[2][2] * w.z);
+	}
+	M3 operator*(const M3& m) const {  // Mat3.h:95-106
+		M3 o;
+		for (int i = 0; i < 3; i++)
+			for (int j = 0; j < 3; j++)
+				for (int k = 0; k < 3; k++) o.v[i][j] += v[i][k] * m.v[k][j];
+		return o;
+	}
+	M3 operator+(const M3& m) const {
+		M3 o;
+		for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) o.v[i][j] = v[i][j] + m.v[i][j];
+		return o;
+	}
+	M3 operator-(const M3& m) const {
+		M3 o;
+		for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) o.v[i][j] = v[i][j] - m.v[i][j];
+		return o;
+	}
+	double det() const {  // Mat3.h:52-55
+		double a = v[0][0], b = v[0][1], c = v[0][2], d = v[1][0], e = v[1][1], f = v[1][2], g = v[2][0], h = v[2][1], i = v[2][2];
+		return a * (e * i - f * h) - b * (d * i - f * g) + c * (d * h - e * g);
+	}
+	M3 inverse() const {  // Mat3.h:57-68
+		M3 o;
+		double a = v[0][0], b = v[0][1], c = v[0][2], d = v[1][0], e = v[1][1], f = v[1][2], g = v[2][0], h = v[2][1], i = v[2][2];
+		double dt = a * (e * i - f * h) - b * (d * i - f * g) + c * (d * h - e * g);
+		o.v[0][0] = (e * i - f * h); o.v[0][1] = (c * h - b * i); o.v[0][2] = (b * f - c * e);
+		o.v[1][0] = (f * g - d * i); o.v[1][1] = (a * i - c * g); o.v[1][2] = (c * d - a * f);
+		o.v[2][0] = (d * h - e * g); o.v[2][1] = (b * g - a * h); o.v[2][2] = (a * e - b * d);
+		for (int r = 0; r < 3; r++) for (int s = 0; s < 3; s++) o.v[r][s] /= dt;
+		return o;
+	}
+	static M3 cross_mat(V3 w) {  // Mat3.h:43-49
+		M3 o;
+		o.v[0][0] = 0; o.v[0][1] = -w.z; o.v[0][2] = w.y;
+		o.v[1][0] = w.z; o.v[1][1] = 0; o.v[1][2] = -w.x;
+		o.v[2][0] = -w.y; o.v[2][1] = w.x; o.v[2][2] = 0;
+		return o;
+	}
+};
+static inline M3 operator*(double d, const M3& m) {  // Mat3.cpp:4-12
+	M3 o;
+	for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) o.v[i][j] = d * m.v[i][j];
+	return o;
+}
+
+struct Q {
+	double r = 1, i = 0, j = 0, k = 0;
+	Q() {}
+	Q(double a, double b, double c, double d) : r(a), i(b), j(c), k(d) {}
+	Q(double theta, V3 axis) {  // Quaternion.h:26-33
+		axis = axis.normalize();
+		r = std::cos(theta / 2.0);
+		double s = std::sin(theta / 2.0);
+		i = axis.x * s; j = axis.y * s; k = axis.z * s;
+	}
+	V3 rotate(V3 v) const {  // Quaternion.h:51-55
+		return V3(v.x * (r * r + i * i - j * j - k * k) + v.y * (2 * i * j - 2 * r * k) + v.z * (2 * i * k + 2 * r * j),
+		          v.x * (2 * i * j + 2 * r * k) + v.y * (r * r - i * i + j * j - k * k) + v.z * (2 * j * k - 2 * r * i),
+		          v.x * (2 * i * k - 2 * r * j) + v.y * (2 * j * k + 2 * r * i) + v.z * (r * r - i * i - j * j + k * k));
+	}
+	M3 rot() const {  // Quaternion.h:61-67
+		M3 m;
+		m.v[0][0] = r * r + i * i - j * j - k * k; m.v[0][1] = 2 * i * j - 2 * r * k; m.v[0][2] = 2 * i * k + 2 * r * j;
+		m.v[1][0] = 2 * i * j + 2 * r * k; m.v[1][1] = r * r - i * i + j * j - k * k; m.v[1][2] = 2 * j * k - 2 * r * i;
+		m.v[2][0] = 2 * i * k - 2 * r * j; m.v[2][1] = 2 * j * k + 2 * r * i; m.v[2][2] = r * r - i * i - j * j + k * k;
+		return m;
+	}
+	Q conj() const { return Q(r, -i, -j, -k); }
+	Q operator*(Q q) const {  // Quaternion.h:73-78
+		return Q(r * q.r - i * q.i - j * q.j - k * q.k, r * q.i + i * q.r + j * q.k - k * q.j, r * q.j - i * q.k + j * q.r + k * q.i,
+		         r * q.k + i * q.j - j * q.i + k * q.r);
+	}
+	Q normalize() const {  // Quaternion.h:88-91
+		double m = std::sqrt(r * r + i * i + j * j + k * k);
+		return Q(r / m, i / m, j / m, k / m);
+	}
+	bool operator!=(Q q) const { return r != q.r || i != q.i || j != q.j || k != q.k; }
+};
+
+// Fixed-size row algebra with NMat's accumulation order (NMat.h:50-60, :91-103).
+template <int R, int C>
+struct NM {
+	double v[R][C] = {};
+};
+template <int R, int C, int X>
+static NM<R, X> mul(const NM<R, C>& a, const NM<C, X>& b) {
+	NM<R, X> o;
+	for (int i = 0; i < R; i++)
+		for (int j = 0; j < X; j++) {
+			o.v[i][j] = 0;
+			for (int k = 0; k < C; k++) o.v[i][j] += a.v[i][k] * b.v[k][j];
+		}
+	return o;
+}
+template <int R, int C>
+static void mulv(const NM<R, C>& a, const double* x, double* out) {
+	for (int i = 0; i < R; i++) {
+		out[i] = 0;
+		for (int j = 0; j < C; j++) out[i] += x[j] * a.v[i][j];
+	}
+}
+template <int R, int C>
+static NM<C, R> tr(const NM<R, C>& a) {
+	NM<C, R> o;
+	for (int i = 0; i < R; i++) for (int j = 0; j < C; j++) o.v[j][i] = a.v[i][j];
+	return o;
+}
+template <int R, int C>
+static NM<R, C> neg(const NM<R, C>& a) {
+	NM<R, C> o;
+	for (int i = 0; i < R; i++) for (int j = 0; j < C; j++) o.v[i][j] = -a.v[i][j];
+	return o;
+}
+static NM<3, 3> crossNM(V3 v) {  // NMat.cpp:62-70 (untouched entries stay 0)
+	NM<3, 3> o;
+	o.v[0][1] = -v.z; o.v[0][2] = v.y;
+	o.v[1][0] = v.z; o.v[1][2] = -v.x;
+	o.v[2][0] = -v.y; o.v[2][1] = v.x;
+	return o;
+}
+// Constraint::aInvMass (ConstraintSolver.cpp:215-225)
+static NM<6, 6> invMass6(double im, const M3& it) {
+	NM<6, 6> o;
+	for (int i = 0; i < 3; i++) o.v[i][i] = 1.0 * im;
+	for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) o.v[3 + i][3 + j] = it.v[i][j];
+	// idenMat<3>() * inv_mass multiplies the zeros too; keep them exact zeros.
+	for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) if (i != j) o.v[i][j] = 0.0 * im;
+	return o;
+}
+static bool isZero(const double* v, int n) {  // NVec.h:12-19
+	for (int i = 0; i < n; i++) if (std::abs(v[i]) > 0.00000001) return false;
+	return true;
+}
+
+// Reference constants (CollisionDetect.h:20-26).
+static const double REF_PI = 3.14159265358979323846;
+static const double TOL_ANG = 1 * REF_PI / 180;
+static const double COS_TOL = 1 - std::cos(TOL_ANG);
+static const double SIN_TOL = std::sin(TOL_ANG);
+
+// ---------------------------------------------------------------- geometry
+struct Hull {
+	std::vector<V3> ref_pts;
+	std::vector<std::vector<int>> faces;
+	std::vector<int> surf_id, ncalc, ndir;
+	std::vector<V3> gnorm;  // body frame
+	std::vector<int> gred, gmin_pid, gmax_pid, gref_idx;
+	std::vector<double> gmin, gmax, gref_val;
+	std::vector<std::pair<int, int>> arcs, edges;
+	std::vector<std::vector<int>> adj_faces, adj_edges;
+	// world copy
+	std::vector<V3> pts;
+	std::vector<V3> wnorm;
+};
+
+struct Body {
+	int id;
+	int movement;  // 0 dynamic, 1 fixed
+	double mass, inv_mass;
+	M3 I_ref, I_ref_inv, I, I_inv;
+	double restitution, mu_k, mu_s;
+	V3 com, vel, ang_vel, pvel, pang;
+	Q q;
+	Hull hull;
+	double aabb[6];
+	V3 prev_com;
+	Q prev_q;
+	bool dynamic() const { return movement == 0; }
+	double invMass() const { return movement == 0 ? 1.0 / mass : 0; }       // RigidBody.cpp:162-164
+	M3 invTensor() const { return movement == 0 ? I_inv : M3(); }          // RigidBody.cpp:170-172
+	V3 getVel() const { return movement == 1 ? V3() : vel; }
+	V3 getAngVel() const { return movement == 1 ? V3() : ang_vel; }
+};
+
+// Surface::normal (ConvexPrimitive.cpp:815-821), evaluated on world points.
+static V3 faceNormal(const Hull& h, int f) {
+	const std::vector<int>& pi = h.faces[f];
+	V3 p1 = h.pts[pi[0]], p2 = h.pts[pi[1]], p3 = h.pts[pi[h.ncalc[f]]];
+	return (p2 - p1).cross(p3 - p2).normalize() * (double)h.ndir[f];
+}
+
+// RigidBody::updateGeometry -> Polyhedron::recomputeFromReference + gen_AABB
+// (RigidBody.cpp:281-296, ConvexPrimitive.cpp:660-676, :715-729).
+static void worldGeometry(Body& b) {
+	M3 R = b.q.rot();
+	Hull& h = b.hull;
+	h.pts.resize(h.ref_pts.size());
+	for (size_t i = 0; i < h.ref_pts.size(); i++) h.pts[i] = b.com + R * h.ref_pts[i];
+	h.wnorm.resize(h.gnorm.size());
+	for (size_t i = 0; i < h.gnorm.size(); i++) h.wnorm[i] = R * h.gnorm[i];
+	const double inf = std::numeric_limits<double>::infinity();
+	double mn[3] = {inf, inf, inf}, mx[3] = {-inf, -inf, -inf};
+	for (const V3& p : h.pts) {
+		mn[0] = std::min<double>(mn[0], p.x); mn[1] = std::min<double>(mn[1], p.y); mn[2] = std::min<double>(mn[2], p.z);
+		mx[0] = std::max<double>(mx[0], p.x); mx[1] = std::max<double>(mx[1], p.y); mx[2] = std::max<double>(mx[2], p.z);
+	}
+	for (int i = 0; i < 3; i++) { b.aabb[i] = mn[i]; b.aabb[3 + i] = mx[i]; }
+}
+
+// RigidBody::updateInertiaTensor (RigidBody.cpp:270-279).
+static void updateInertia(Body& b) {
+	b.q = b.q.normalize();
+	M3 r = b.q.rot(), rc = b.q.conj().rot();
+	b.I = r * b.I_ref * rc;
+	b.I_inv = r * b.I_ref_inv * rc;
+}
+
+// ---------------------------------------------------------------- narrowphase
+struct Extrema {  // CollisionDetect.h:57-70
+	int min_pid = -1, max_pid = -1;
+	double min_val = std::numeric_limits<double>::infinity(), max_val = -std::numeric_limits<double>::infinity();
+};
+static Extrema findExtrema(const Hull& h, V3 axis) {  // CollisionDetect.cpp:627-644
+	Extrema e;
+	for (size_t i = 0; i < h.pts.size(); i++) {
+		double val = h.pts[i].dot(axis);
+		if (val < e.min_val) { e.min_pid = (int)i; e.min_val = val; }
+		if (val > e.max_val) { e.max_pid = (int)i; e.max_val = val; }
+	}
+	return e;
+}
+struct CheckNorm {
+	int a_pid, b_pid;
+	V3 n;
+	double pen;
+};
+static CheckNorm satCheck(const Extrema& a, const Extrema& b, V3 n) {  // CollisionDetect.cpp:705-715
+	double f = a.max_val - b.min_val, r = b.max_val - a.min_val;
+	if (f < r) return {a.max_pid, b.min_pid, n, f};
+	return {a.min_pid, b.max_pid, -n, r};
+}
+static Extrema recentered(const Hull& h, int g) {  // CollisionDetect.cpp:622-625, :726
+	double nv = h.wnorm[g].dot(h.pts[h.gref_idx[g]]);
+	double diff = nv - h.gref_val[g];
+	Extrema e;
+	e.min_pid = h.gmin_pid[g]; e.max_pid = h.gmax_pid[g];
+	e.min_val = h.gmin[g] + diff; e.max_val = h.gmax[g] + diff;
+	return e;
+}
+
+enum Origin { FACE, EDGE, VERTEX };
+struct P2 { double x, y; };
+static P2 sub2(P2 a, P2 b) { return {a.x - b.x, a.y - b.y}; }
+static double dot2(P2 a, P2 b) { double s = 0; s += a.x * b.x; s += a.y * b.y; return s; }
+struct Area {
+	std::vector<P2> ps;
+	std::vector<int> ids;
+	int surface;
+	Origin origin;
+};
+
+// findContactArea (CollisionDetect.cpp:237-280) with projectFace/projectEdge (:215-235).
+static Area contactArea(const Hull& h, V3 n, V3 p, int pid, V3 u, V3 w) {
+	int best_f = -1;
+	double best_ft = 1.0;
+	for (int f = 0; f < (int)h.adj_faces[pid].size(); f++) {
+		int fi = h.adj_faces[pid][f];
+		double c = faceNormal(h, fi).dot(n);
+		if (1 - c < best_ft) { best_ft = 1 - c; best_f = fi; }
+	}
+	if (best_ft <= COS_TOL) {
+		Area a;
+		a.surface = h.surf_id[best_f];
+		a.origin = FACE;
+		for (int vi : h.faces[best_f]) {
+			V3 v = h.pts[vi];
+			a.ps.push_back({v.dot(u), v.dot(w)});
+			a.ids.push_back(vi);
+		}
+		return a;
+	}
+	int best_e = -1;
+	double best_et = 1.0;
+	for (int ei : h.adj_edges[pid]) {
+		V3 e1 = h.pts[h.edges[ei].first], e2 = h.pts[h.edges[ei].second];
+		double s = std::abs((e2 - e1).normalize().dot(n));
+		if (s < best_et) { best_et = s; best_e = ei; }
+	}
+	if (best_et <= SIN_TOL) {
+		V3 e1 = h.pts[h.edges[best_e].first], e2 = h.pts[h.edges[best_e].second];
+		Area a;
+		a.ps = {{e1.dot(u), e1.dot(w)}, {e2.dot(u), e2.dot(w)}};
+		a.ids = {h.edges[best_e].first, h.edges[best_e].second};
+		a.surface = -1;
+		a.origin = EDGE;
+		return a;
+	}
+	Area a;
+	a.ps = {{p.dot(u), p.dot(w)}};
+	a.ids = {pid};
+	a.surface = -1;
+	a.origin = VERTEX;
+	return a;
+}
+
+static uint32_t edgeID(uint16_t a, uint16_t b) {  // CollisionDetect.cpp:9-21
+	int mn, mx;
+	if (a < b) { mn = a; mx = b; } else { mn = b; mx = a; }
+	return (0x0000FFFF & mn) + (0xFFFF0000 & ((uint32_t)mx << 16));
+}
+static P2 inDir(P2 p1, P2 p2) {  // CollisionDetect.cpp:392-396, NVec.h:50-52
+	P2 d{p1.y - p2.y, p2.x - p1.x};
+	double s = 0; s += d.x * d.x; s += d.y * d.y;
+	double k = 1.0 / std::sqrt(s);
+	return {d.x * k, d.y * k};
+}
+struct CEP {
+	P2 pos;
+	int32_t src;
+	uint64_t magic;
+	bool skip;
+};
+static bool ccw(const Area& c) {  // CollisionDetect.cpp:398-409
+	P2 in = inDir(c.ps[0], c.ps[1]);
+	for (size_t i = 2; i < c.ps.size(); i++) {
+		double d = dot2(sub2(c.ps[i], c.ps[0]), in);
+		if (d > 0.0000000001) return true;
+		else if (d < -0.0000000001) return false;
+	}
+	return false;
+}
+static std::vector<CEP> clipPoly(const Area& c, uint32_t other_sid, bool flip) {  // CollisionDetect.cpp:423-442
+	std::vector<CEP> out;
+	for (size_t i = 0; i < c.ps.size(); i++) {
+		uint64_t m = 0;
+		if (flip) {
+			m |= 0x00000000FFFFFFFFull & other_sid;
+			m |= 0xFFFFFFFF00000000ull & (uint64_t(c.ids[i]) << 32);
+		} else {
+			m |= 0x00000000FFFFFFFFull & (uint64_t)(int64_t)c.ids[i];
+			m |= 0xFFFFFFFF00000000ull & (uint64_t(other_sid) << 32);
+		}
+		out.push_back({c.ps[i], c.ids[i], m, false});
+	}
+	if (c.ps.size() > 2 && !ccw(c)) std::reverse(out.begin(), out.end());
+	return out;
+}
+static CEP intersect(CEP e1, CEP e2, P2 cn, P2 sp, int32_t ceid, bool flip) {  // CollisionDetect.cpp:444-467
+	P2 d = sub2(e2.pos, e1.pos);
+	double t = dot2(cn, sub2(sp, e1.pos)) / dot2(cn, d);
+	P2 ip{e1.pos.x + d.x * t, e1.pos.y + d.y * t};
+	bool e2p = dot2(d, cn) > 0;
+	int32_t ps = e2p ? e1.src : e2.src;
+	uint64_t m = 0;
+	if (flip) {
+		m |= 0x00000000FFFFFFFFull & (uint64_t)(int64_t)ceid;
+		m |= 0xFFFFFFFF00000000ull & (uint64_t(edgeID((uint16_t)e1.src, (uint16_t)e2.src)) << 32);
+	} else {
+		m |= 0x00000000FFFFFFFFull & edgeID((uint16_t)e1.src, (uint16_t)e2.src);
+		m |= 0xFFFFFFFF00000000ull & (uint64_t)((uint64_t)(int64_t)ceid << 32);
+	}
+	return {ip, ps, m, !e2p};
+}
+enum ClipR { BOTH, P1ON_P2C, P2ON_P1C, NEITHER, ONLY_P1, ONLY_P2 };
+static ClipR checkSeg(double a, double b, double c) {  // CollisionDetect.cpp:473-480
+	if (a >= c && b >= c) return NEITHER;
+	if (a == c && b < c) return P1ON_P2C;
+	if (a < c && b == c) return P2ON_P1C;
+	if (a < c && b >= c) return ONLY_P1;
+	if (a >= c && b < c) return ONLY_P2;
+	return BOTH;
+}
+static std::vector<CEP> clipByEdge(const std::vector<CEP>& c, P2 side, P2 sp, int32_t eid, bool flip) {  // :482-562
+	std::vector<CEP> out;
+	double cv = dot2(sp, side);
+	if (c.size() >= 3) {
+		double p1v = 0, p2v = 0;
+		CEP p1, p2;
+		for (size_t i = 0; i < c.size(); i++) {
+			if (i > 0) { p1v = p2v; p1 = p2; }
+			else { p1 = c[0]; p1v = dot2(p1.pos, side); }
+			p2 = c[(i + 1) % c.size()];
+			p2v = dot2(p2.pos, side);
+			ClipR r = checkSeg(p1v, p2v, cv);
+			if (r == NEITHER || r == P1ON_P2C) out.push_back(p1);
+			else if (r == ONLY_P2) { out.push_back(p1); out.push_back(intersect(p1, p2, side, sp, eid, flip)); }
+			else if (r == ONLY_P1) out.push_back(intersect(p1, p2, side, sp, eid, flip));
+		}
+		return out;
+	} else if (c.size() == 2) {
+		double a = dot2(c[0].pos, side), b = dot2(c[1].pos, side);
+		ClipR r = checkSeg(a, b, cv);
+		if (r == NEITHER) return c;
+		if (r == P1ON_P2C) return {c[0]};
+		if (r == P2ON_P1C) return {c[1]};
+		if (r == ONLY_P1) return {intersect(c[0], c[1], side, sp, eid, flip), c[1]};
+		if (r == ONLY_P2) return {c[0], intersect(c[0], c[1], side, sp, eid, flip)};
+		return out;
+	} else {
+		if (dot2(c[0].pos, side) >= cv) return c;
+		return out;
+	}
+}
+static std::vector<CEP> clipAll(std::vector<CEP> c1, const std::vector<CEP>& c2, bool flip) {  // :564-576
+	for (size_t i = 0; i < c2.size(); i++) {
+		CEP e1 = c2[i];
+		if (e1.skip) continue;
+		CEP e2 = c2[(i + 1) % c2.size()];
+		P2 dir = inDir(e1.pos, e2.pos);
+		int32_t eid = (int32_t)edgeID((uint16_t)e1.src, (uint16_t)e2.src);
+		c1 = clipByEdge(c1, dir, e1.pos, eid, flip);
+		if (c1.empty()) return c1;
+	}
+	return c1;
+}
+static std::vector<CEP> clipContacts(const Area& c1, const Area& c2) {  // :578-620
+	std::vector<CEP> p1 = clipPoly(c1, (uint32_t)c2.surface, false);
+	std::vector<CEP> p2 = clipPoly(c2, (uint32_t)c1.surface, true);
+	if (p1.size() == 1) return p1;
+	if (p2.size() == 1) return p2;
+	if (p1.size() == 2 && p2.size() == 2) {
+		P2 nn = inDir(p2[0].pos, p2[1].pos);
+		return {intersect(p1[0], p1[1], nn, p2[0].pos, (int32_t)edgeID((uint16_t)p2[0].src, (uint16_t)p2[1].src), false)};
+	}
+	if (p1.size() == 2) return clipAll(p1, p2, false);
+	if (p2.size() == 2) return clipAll(p2, p1, true);
+	p1 = clipAll(p1, p2, false);
+	return clipAll(p2, p1, true);
+}
+
+struct CP {
+	V3 pos;
+	double pen;
+	uint64_t cid, bid;
+	double restitution, mu_k, mu_s;
+};
+struct Manifold {
+	std::vector<CP> pts;
+	V3 n;
+	double max_pen = -1;
+};
+
+// SAT_PolyPoly (CollisionDetect.cpp:717-835).
+static Manifold satPolyPoly(const Body& A, uint64_t a_key, const Body& B, uint64_t b_key) {
+	Manifold out;
+	const Hull& a = A.hull;
+	const Hull& b = B.hull;
+	CheckNorm mp{-1, -1, V3(), std::numeric_limits<double>::infinity()};
+	for (size_t g = 0; g < a.wnorm.size(); g++) {
+		if (a.gred[g]) continue;
+		CheckNorm x = satCheck(recentered(a, (int)g), findExtrema(b, a.wnorm[g]), a.wnorm[g]);
+		if (x.pen < 0) { out.max_pen = -1; return out; }
+		if (x.pen < mp.pen) mp = x;
+	}
+	for (size_t g = 0; g < b.wnorm.size(); g++) {
+		if (b.gred[g]) continue;
+		CheckNorm x = satCheck(findExtrema(a, b.wnorm[g]), recentered(b, (int)g), b.wnorm[g]);
+		if (x.pen < 0) { out.max_pen = -1; return out; }
+		if (x.pen < mp.pen) mp = x;
+	}
+	for (auto arc1 : a.arcs)
+		for (auto arc2 : b.arcs) {
+			V3 a1 = a.wnorm[arc1.first], a2 = a.wnorm[arc1.second];
+			V3 b1 = -b.wnorm[arc2.first], b2 = -b.wnorm[arc2.second];
+			V3 aa = a1 + a2;
+			if (aa.dot(b1) + aa.dot(b2) <= 0) continue;
+			V3 ap = a1.cross(a2), bp = b1.cross(b2);
+			if (ap.dot(b1) * ap.dot(b2) > 0 || bp.dot(a1) * bp.dot(a2) > 0) continue;
+			V3 n = ap.cross(bp);
+			if (n.magSqrd() == 0) continue;
+			n = n.normalize();
+			if (aa.dot(n) < 0) n = n * -1.0;
+			CheckNorm x = satCheck(findExtrema(a, n), findExtrema(b, n), n);
+			if (x.pen < 0) { out.max_pen = -1; return out; }
+			if (x.pen < mp.pen) mp = x;
+		}
+	out.n = mp.n;
+	V3 n = mp.n;
+	V3 amax = a.pts[mp.a_pid], bmax = b.pts[mp.b_pid];
+	V3 u, w;
+	n.perpBasis(&u, &w);
+	Area ac = contactArea(a, n, amax, mp.a_pid, u, w);
+	Area bc = contactArea(b, n * -1.0, bmax, mp.b_pid, u, w);
+	std::vector<CEP> pool = clipContacts(ac, bc);
+	double a_dot = amax.dot(n);
+	V3 noff = n * a_dot;
+	uint64_t cid = 0;
+	cid |= 0x00000000FFFFFFFFull & a_key;
+	cid |= 0xFFFFFFFF00000000ull & (b_key << 32);
+	for (const CEP& p : pool) {
+		CP c;
+		c.pos = u * p.pos.x + w * p.pos.y + noff;
+		c.pen = c.pos.dot(n) - a_dot + mp.pen;
+		c.restitution = std::max<double>(A.restitution, B.restitution);
+		c.mu_k = (A.mu_k + B.mu_k) / 2.0;
+		c.mu_s = (A.mu_s + B.mu_s) / 2.0;
+		c.cid = cid;
+		c.bid = p.magic;
+		out.pts.push_back(c);
+	}
+	out.max_pen = mp.pen;
+	return out;
+}
+
+// cull_manifold (CollisionDetect.cpp:2795-2830).
+static Manifold cullManifold(const Manifold& m, int ns) {
+	if (ns >= (int)m.pts.size()) return m;
+	Manifold out;
+	out.n = m.n;
+	out.max_pen = 0;
+	out.pts.resize(ns);
+	std::vector<bool> avail(m.pts.size(), true);
+	V3 u, w;
+	m.n.perpBasis(&u, &w);
+	for (int i = 0; i < ns; i++) {
+		double vu = std::cos(2 * REF_PI * i / ns), vw = std::sin(2 * REF_PI * i / ns);
+		V3 t = u * vu + w * vw;
+		CP best;
+		int bi = -1;
+		double bv = -std::numeric_limits<double>::infinity();
+		for (size_t j = 0; j < m.pts.size(); j++) {
+			if (!avail[j]) continue;
+			double v = m.pts[j].pos.dot(t);
+			if (v > bv) { bv = v; best = m.pts[j]; bi = (int)j; }
+		}
+		out.pts[i] = best;
+		out.max_pen = std::max<double>(out.max_pen, best.pen);
+		avail[bi] = false;
+	}
+	return out;
+}
+
+// ---------------------------------------------------------------- constraints
+struct ContactRow {  // ContactConstraint (ConstraintSolver.cpp:240-272)
+	int a, b;
+	NM<1, 6> ja, jb;
+	NM<6, 1> ita, itb;
+	double inv;  // impulse_to_value_inverse
+	double target, ptarget, imp, pimp, bounce, cutoff;
+};
+struct FrictionRow {  // FrictionConstraint (ConstraintSolver.cpp:277-343)
+	int a, b;
+	NM<2, 6> ja, jb;
+	NM<6, 2> ita, itb;
+	NM<2, 2> inv;
+	double target[2], imp[2], pimp[2];
+	double mu;
+	V3 u, w;
+	bool static_ready;
+};
+struct Contact {
+	int b1, b2;  // body indices
+	uint64_t cid, bid;
+	V3 lp1, lp2, ln1;
+	double pen, bounce, mu, hardness, cutoff;
+	int life;
+	bool live;
+	ContactRow c;
+	FrictionRow f;
+	uint64_t seq;  // insertion order within the edge
+};
+
+static double posCorrectCoeff(double s, double dt) { return std::min<double>(s * dt, 1.0 / dt); }  // PersistentConstraint.cpp:199
+
+struct World {
+	orc_config cfg;
+	double cutoff_vel;
+	std::vector<Body> bodies;
+	std::vector<Contact> cache;  // grouped by edge (b1 < b2 by id), edge order = insertion
+	uint64_t seq = 0;
+	std::vector<int64_t> last_rows, last_offs;
+	std::vector<Contact> last_mid;
+};
+
+static void applyCFMInv1(double itv, double cfm, double* inv) { *inv = 1.0 / (itv * (1 + cfm)); }
+
+// Contact::updateSolverConstraints (PersistentConstraint.cpp:201-215) building
+// ContactConstraint and FrictionConstraint.
+static void buildRows(World& W, Contact& c, double sub_dt) {
+	Body& A = W.bodies[c.b1];
+	Body& B = W.bodies[c.b2];
+	V3 p1 = A.q.rotate(c.lp1) + A.com;
+	V3 p2 = B.q.rotate(c.lp2) + B.com;
+	V3 n = A.q.rotate(c.ln1);
+	double adj = (p2 - p1).dot(n) + c.pen;
+	double cimp = c.c.imp * 1.0;
+	double fimp[2] = {c.f.imp[0] * 1.0, c.f.imp[1] * 1.0};
+	double pcc = posCorrectCoeff(c.hardness, sub_dt);
+	double cfm = W.cfg.cfm;
+	double ima = A.invMass(), imb = B.invMass();
+	M3 ia = A.invTensor(), ib = B.invTensor();
+	NM<6, 6> Ma = invMass6(ima, ia), Mb = invMass6(imb, ib);
+	// contact row
+	{
+		ContactRow& r = c.c;
+		r.a = c.b1; r.b = c.b2;
+		V3 rA = p1 - A.com, rB = p1 - B.com;
+		NM<3, 3> sa = crossNM(rA), sb = crossNM(rB);
+		NM<1, 3> nd;
+		nd.v[0][0] = n.x; nd.v[0][1] = n.y; nd.v[0][2] = n.z;
+		NM<1, 3> ndn = neg(nd);
+		NM<1, 3> ja_ang = mul(nd, sa), jb_ang = mul(ndn, sb);
+		for (int i = 0; i < 3; i++) {
+			r.ja.v[0][i] = ndn.v[0][i]; r.ja.v[0][3 + i] = ja_ang.v[0][i];
+			r.jb.v[0][i] = nd.v[0][i]; r.jb.v[0][3 + i] = jb_ang.v[0][i];
+		}
+		r.ita = mul(Ma, tr(r.ja));
+		r.itb = mul(Mb, tr(r.jb));
+		NM<1, 1> t1 = mul(r.ja, r.ita), t2 = mul(r.jb, r.itb);
+		double itv = t1.v[0][0] + t2.v[0][0];
+		applyCFMInv1(itv, cfm, &r.inv);
+		double slop = 0.05;
+		double sp = std::max<double>(0.0, adj - slop);
+		r.ptarget = sp * pcc;
+		r.imp = cimp;
+		r.pimp = 0;
+		r.bounce = c.bounce;
+		r.cutoff = c.cutoff;
+	}
+	// friction rows
+	{
+		FrictionRow& r = c.f;
+		V3 su = r.u, sw = r.w;
+		r.a = c.b1; r.b = c.b2;
+		V3 rA = p1 - A.com, rB = p1 - B.com;
+		n.perpBasis(&r.u, &r.w);
+		V3 u = r.u, w = r.w;
+		r.imp[0] = u.dot(su) * fimp[0] + u.dot(sw) * fimp[1];
+		r.imp[1] = w.dot(su) * fimp[0] + w.dot(sw) * fimp[1];
+		NM<1, 3> ud, wd;
+		ud.v[0][0] = u.x; ud.v[0][1] = u.y; ud.v[0][2] = u.z;
+		wd.v[0][0] = w.x; wd.v[0][1] = w.y; wd.v[0][2] = w.z;
+		NM<3, 3> sa = crossNM(rA), sb = crossNM(rB);
+		NM<1, 3> a_u = mul(neg(ud), sa), b_u = mul(ud, sb), a_w = mul(neg(wd), sa), b_w = mul(wd, sb);
+		NM<1, 3> nu = neg(ud), nw = neg(wd);
+		for (int i = 0; i < 3; i++) {
+			r.ja.v[0][i] = ud.v[0][i]; r.ja.v[0][3 + i] = a_u.v[0][i];
+			r.jb.v[0][i] = nu.v[0][i]; r.jb.v[0][3 + i] = b_u.v[0][i];
+			r.ja.v[1][i] = wd.v[0][i]; r.ja.v[1][3 + i] = a_w.v[0][i];
+			r.jb.v[1][i] = nw.v[0][i]; r.jb.v[1][3 + i] = b_w.v[0][i];
+		}
+		r.ita = mul(Ma, tr(r.ja));
+		r.itb = mul(Mb, tr(r.jb));
+		NM<2, 2> t1 = mul(r.ja, r.ita), t2 = mul(r.jb, r.itb), itv;
+		for (int i = 0; i < 2; i++) for (int j = 0; j < 2; j++) itv.v[i][j] = t1.v[i][j] + t2.v[i][j];
+		for (int i = 0; i < 2; i++) itv.v[i][i] *= (1 + cfm);
+		// rowMajorOrderInverse<2> (NMat.cpp:11-21)
+		double a = itv.v[0][0], bq = itv.v[0][1], cq = itv.v[1][0], d = itv.v[1][1];
+		double det = a * d - bq * cq;
+		double di = 1.0 / det;
+		r.inv.v[0][0] = d * di; r.inv.v[0][1] = -bq * di;
+		r.inv.v[1][0] = -cq * di; r.inv.v[1][1] = a * di;
+		r.pimp[0] = r.pimp[1] = 0;
+		r.mu = c.mu;
+		r.static_ready = false;
+	}
+}
+
+// ---------------------------------------------------------------- PGS (ConstraintSolver.cpp:80-213)
+struct DV {
+	double v[6] = {0, 0, 0, 0, 0, 0};
+	double p[6] = {0, 0, 0, 0, 0, 0};
+};
+static void addv(double* acc, const double* d) { for (int i = 0; i < 6; i++) acc[i] += d[i]; }
+
+static void pgsIsland(World& W, const std::vector<std::pair<Contact*, int>>& rows) {
+	std::map<int, DV> dv;
+	for (auto& r : rows) { dv[r.first->b1]; dv[r.first->b2]; }
+	// targets (ConstraintSolver.cpp:265-268 / ConstraintSolver.h:78)
+	for (auto& r : rows) {
+		Contact* c = r.first;
+		const Body& A = W.bodies[c->b1];
+		const Body& B = W.bodies[c->b2];
+		double va[6] = {A.getVel().x, A.getVel().y, A.getVel().z, A.getAngVel().x, A.getAngVel().y, A.getAngVel().z};
+		double vb[6] = {B.getVel().x, B.getVel().y, B.getVel().z, B.getAngVel().x, B.getAngVel().y, B.getAngVel().z};
+		if (r.second == 0) {
+			double x[1], y[1];
+			mulv(c->c.ja, va, x); mulv(c->c.jb, vb, y);
+			double cur = x[0] + y[0];
+			c->c.target = (cur < -c->c.cutoff) ? -(1 + c->c.bounce) * cur : -cur;
+		} else {
+			double x[2], y[2];
+			mulv(c->f.ja, va, x); mulv(c->f.jb, vb, y);
+			c->f.target[0] = -(x[0] + y[0]);
+			c->f.target[1] = -(x[1] + y[1]);
+		}
+	}
+	// warm start
+	for (auto& r : rows) {
+		Contact* c = r.first;
+		DV& a = dv[c->b1];
+		DV& b = dv[c->b2];
+		if (r.second == 0) {
+			if (isZero(&c->c.imp, 1)) continue;
+			double da[6], db[6];
+			mulv(c->c.ita, &c->c.imp, da); mulv(c->c.itb, &c->c.imp, db);
+			addv(a.v, da); addv(b.v, db);
+		} else {
+			if (isZero(c->f.imp, 2)) continue;
+			double da[6], db[6];
+			mulv(c->f.ita, c->f.imp, da); mulv(c->f.itb, c->f.imp, db);
+			addv(a.v, da); addv(b.v, db);
+		}
+	}
+	auto step = [&](Contact* c, int kind, bool pseudo) {
+		DV& A = dv[c->b1];
+		DV& B = dv[c->b2];
+		double* va = pseudo ? A.p : A.v;
+		double* vb = pseudo ? B.p : B.v;
+		if (kind == 0) {
+			ContactRow& r = c->c;
+			double* acc = pseudo ? &r.pimp : &r.imp;
+			double tgt = pseudo ? r.ptarget : r.target;
+			double x[1], y[1];
+			mulv(r.ja, va, x); mulv(r.jb, vb, y);
+			double cur = x[0] + y[0];
+			double delta = tgt - cur;
+			double m = 0; m += delta * r.inv;
+			double nw = *acc + m;
+			double diff = std::max<double>(nw, 0) - *acc;
+			if (!isZero(&diff, 1)) {
+				double da[6], db[6];
+				mulv(r.ita, &diff, da); mulv(r.itb, &diff, db);
+				addv(va, da); addv(vb, db);
+				*acc += diff;
+			}
+		} else {
+			FrictionRow& r = c->f;
+			double x[2], y[2];
+			mulv(r.ja, va, x); mulv(r.jb, vb, y);
+			double cur[2] = {x[0] + y[0], x[1] + y[1]};
+			double delta[2] = {r.target[0] - cur[0], r.target[1] - cur[1]};
+			double m[2];
+			mulv(r.inv, delta, m);
+			double nw[2] = {r.imp[0] + m[0], r.imp[1] + m[1]};
+			// FrictionConstraint::projectValidImpulse (ConstraintSolver.cpp:331-343)
+			double maxm = r.mu * std::min<double>(std::numeric_limits<double>::infinity(), c->c.imp);
+			double cm = std::sqrt(nw[0] * nw[0] + nw[1] * nw[1]);
+			double pr[2];
+			if (cm > maxm) {
+				r.static_ready = false;
+				double q = maxm / cm;
+				pr[0] = nw[0] * q; pr[1] = nw[1] * q;
+			} else {
+				r.static_ready = true;
+				pr[0] = nw[0]; pr[1] = nw[1];
+			}
+			double diff[2] = {pr[0] - r.imp[0], pr[1] - r.imp[1]};
+			if (!isZero(diff, 2)) {
+				double da[6], db[6];
+				mulv(r.ita, diff, da); mulv(r.itb, diff, db);
+				addv(va, da); addv(vb, db);
+				r.imp[0] += diff[0]; r.imp[1] += diff[1];
+			}
+		}
+	};
+	for (int it = 0; it < W.cfg.vel_iters; it++)
+		for (auto& r : rows) step(r.first, r.second, false);
+	for (int it = 0; it < W.cfg.pos_iters; it++)
+		for (auto& r : rows)
+			if (r.second == 0) step(r.first, 0, true);
+	// applyVelocityChange (PhysicsEngine.cpp:1113-1136)
+	for (auto& kv : dv) {
+		Body& b = W.bodies[kv.first];
+		if (!b.dynamic()) continue;
+		b.vel += V3(kv.second.v[0], kv.second.v[1], kv.second.v[2]);
+		b.ang_vel += V3(kv.second.v[3], kv.second.v[4], kv.second.v[5]);
+		b.pvel += V3(kv.second.p[0], kv.second.p[1], kv.second.p[2]);
+		b.pang += V3(kv.second.p[3], kv.second.p[4], kv.second.p[5]);
+	}
+}
+
+// ---------------------------------------------------------------- integration
+// RigidBody::rotateWhileApplyingGyroAccel (RigidBody.cpp:207-255), n_itr = 4.
+static void gyroRotate(Body& b, double dt, int n_itr) {
+	if (b.ang_vel.magSqrd() == 0) return;
+	M3 T = b.I;
+	Q o = b.q;
+	V3 av = b.ang_vel;
+	for (int i = 0; i < n_itr; i++) {
+		V3 w = av;
+		double ts = dt / n_itr;
+		for (int s = 0; s < 2; s++) {
+			V3 f = (w.cross(T * w)) * ts - T * (av - w);
+			M3 jac = T + ts * (M3::cross_mat(w) * T - M3::cross_mat(T * w));
+			if (std::abs(jac.det()) > 0.00000000001) w -= jac.inverse() * f;
+		}
+		av = w;
+		o = Q(ts * av.mag(), av) * o;
+		if (i + 1 < n_itr) {
+			M3 r = o.rot(), rc = o.conj().rot();
+			T = r * b.I_ref * rc;
+		}
+	}
+	b.q = o;
+	b.ang_vel = av;
+}
+
+// ---------------------------------------------------------------- step
+static void refreshAll(World& W) {
+	for (Body& b : W.bodies) {
+		updateInertia(b);
+		worldGeometry(b);
+	}
+}
+
+static bool aabbOverlap(const double* a, const double* b) {  // AABB.h:10-14
+	return !(a[3] < b[0] || a[0] > b[3] || a[4] < b[1] || a[1] > b[4] || a[5] < b[2] || a[2] > b[5]);
+}
+
+static std::vector<std::pair<int, int>> broadphase(const World& W) {
+	std::vector<std::pair<int, int>> v;
+	for (size_t i = 0; i < W.bodies.size(); i++)
+		for (size_t j = i + 1; j < W.bodies.size(); j++)
+			if (aabbOverlap(W.bodies[i].aabb, W.bodies[j].aabb)) v.push_back({(int)i, (int)j});
+	return v;
+}
+
+static uint64_t primKey(const Body& b) { return (uint64_t)b.id * 16; }
+
+static Manifold narrow(const World& W, int i, int j) {
+	const Body& A = W.bodies[i];
+	const Body& B = W.bodies[j];
+	Manifold m = satPolyPoly(A, primKey(A), B, primKey(B));
+	if (!(m.max_pen > 0)) return Manifold();
+	return cullManifold(m, 4);
+}
+
+// addContact (PhysicsEngine.cpp:1062-1111).
+static void addContact(World& W, int i, int j, const CP& p, V3 n) {
+	Body& A = W.bodies[i];
+	Body& B = W.bodies[j];
+	M3 ia = A.q.conj().rot(), ib = B.q.conj().rot();
+	V3 nl = ia * n, pl1 = ia * (p.pos - A.com), pl2 = ib * (p.pos - B.com);
+	for (Contact& c : W.cache) {
+		if (c.b1 != i || c.b2 != j) continue;
+		if (c.cid == p.cid && c.bid == p.bid) {
+			double fr = c.f.static_ready ? p.mu_s : p.mu_k;
+			c.mu = fr;
+			c.pen = p.pen;
+			c.ln1 = nl;
+			c.lp1 = pl1;
+			c.lp2 = pl2;
+			c.life = W.cfg.contact_life;
+			c.live = true;
+			return;
+		}
+	}
+	Contact c{};
+	c.b1 = i; c.b2 = j;
+	c.cid = p.cid; c.bid = p.bid;
+	c.lp1 = pl1; c.lp2 = pl2; c.ln1 = nl;
+	c.pen = p.pen; c.bounce = p.restitution; c.hardness = W.cfg.contact_hardness; c.mu = p.mu_k;
+	c.cutoff = W.cutoff_vel;
+	c.life = W.cfg.contact_life;
+	c.live = true;
+	c.c = ContactRow{};
+	c.f = FrictionRow{};
+	c.f.static_ready = false;
+	c.seq = W.seq++;
+	W.cache.push_back(c);
+}
+
+static void sortCache(World& W) {
+	std::stable_sort(W.cache.begin(), W.cache.end(), [&](const Contact& a, const Contact& b) {
+		int ai = W.bodies[a.b1].id, aj = W.bodies[a.b2].id, bi = W.bodies[b.b1].id, bj = W.bodies[b.b2].id;
+		if (ai != bi) return ai < bi;
+		if (aj != bj) return aj < bj;
+		return a.seq < b.seq;
+	});
+}
+
+static void step(World& W, const int64_t* node_order) {
+	const int N = (int)W.bodies.size();
+	double dt = W.cfg.dt;
+	V3 g(W.cfg.gravity[0], W.cfg.gravity[1], W.cfg.gravity[2]);
+	// PhysicsEngine.cpp:65-87 (gravity once per step)
+	for (Body& b : W.bodies)
+		if (b.dynamic()) b.vel += g * dt;
+	// broadphase + narrowphase (PhysicsEngine.cpp:102-335)
+	auto pairs = broadphase(W);
+	for (auto& pr : pairs) {
+		const Body& A = W.bodies[pr.first];
+		const Body& B = W.bodies[pr.second];
+		if (!(A.dynamic() || B.dynamic())) continue;
+		Manifold m = narrow(W, pr.first, pr.second);
+		for (const CP& p : m.pts) addContact(W, pr.first, pr.second, p, m.n);
+	}
+	sortCache(W);
+	W.last_mid = W.cache;
+	// island search (PhysicsEngine.cpp:1536-1613, :1263-1283)
+	std::vector<std::vector<std::pair<int, int>>> adj(N);  // (other, first cache index) per edge, sorted by other id
+	{
+		std::map<std::pair<int, int>, int> first;
+		for (size_t k = 0; k < W.cache.size(); k++) {
+			auto key = std::make_pair(W.cache[k].b1, W.cache[k].b2);
+			if (!first.count(key)) first[key] = (int)k;
+		}
+		for (auto& kv : first) {
+			adj[kv.first.first].push_back({kv.first.second, kv.second});
+			adj[kv.first.second].push_back({kv.first.first, kv.second});
+		}
+		for (auto& a : adj)
+			std::sort(a.begin(), a.end(), [&](const std::pair<int, int>& x, const std::pair<int, int>& y) {
+				return W.bodies[x.first].id < W.bodies[y.first].id;
+			});
+	}
+	std::vector<std::vector<std::pair<Contact*, int>>> islands;
+	W.last_rows.clear();
+	W.last_offs.assign(1, 0);
+	{
+		std::vector<char> visited(N, 0);
+		std::vector<int> roots;
+		if (node_order) for (int k = 0; k < N; k++) roots.push_back((int)node_order[k] - 1);
+		else for (int k = 0; k < N; k++) roots.push_back(k);
+		for (int root : roots) {
+			if (!W.bodies[root].dynamic() || visited[root]) continue;
+			std::vector<std::pair<Contact*, int>> rows;
+			std::vector<int> st{root};
+			while (!st.empty()) {
+				int cur = st.back();
+				st.pop_back();
+				if (visited[cur]) continue;
+				visited[cur] = 1;
+				if (W.bodies[cur].dynamic()) {
+					for (auto& e : adj[cur]) {
+						if (visited[e.first]) continue;
+						int lo = std::min(cur, e.first), hi = std::max(cur, e.first);
+						for (size_t k = e.second; k < W.cache.size() && W.cache[k].b1 == lo && W.cache[k].b2 == hi; k++) {
+							if (!W.cache[k].live) continue;
+							rows.push_back({&W.cache[k], 0});
+							rows.push_back({&W.cache[k], 1});
+							W.last_rows.push_back((int64_t)k);
+							W.last_rows.push_back(0);
+							W.last_rows.push_back((int64_t)k);
+							W.last_rows.push_back(1);
+						}
+					}
+				}
+				for (auto& e : adj[cur])
+					if (W.bodies[e.first].dynamic() && !visited[e.first]) st.push_back(e.first);
+			}
+			if (!rows.empty()) {
+				islands.push_back(rows);
+				W.last_offs.push_back((int64_t)W.last_rows.size() / 2);
+			}
+		}
+	}
+	// substeps (PhysicsEngine.cpp:368-420)
+	int S = W.cfg.substeps;
+	for (int s = 0; s < S; s++) {
+		double sdt = dt / S;
+		bool first = s == 0, last = s == S - 1;
+		// maintainConstraintGraphApplyPoweredConstraints (PhysicsEngine.cpp:1378-1425)
+		for (Contact& c : W.cache) {
+			bool skip = !c.live;
+			if (last) { c.life--; c.live = false; }
+			if (skip) continue;
+			buildRows(W, c, sdt);
+		}
+		if (first) {
+			// removeExpiredContactConstraints (PhysicsEngine.cpp:1434-1446); island pointers stay valid
+			// because expired contacts are never live and the vector is compacted only after the solve.
+		}
+		for (auto& isl : islands) pgsIsland(W, isl);
+		// update_positions (PhysicsEngine.cpp:390-410)
+		for (Body& b : W.bodies) {
+			if (!b.dynamic()) continue;
+			b.prev_com = b.com;
+			b.prev_q = b.q;
+			b.com += (b.vel + b.pvel) * sdt;
+			if (b.ang_vel.magSqrd() != 0) {
+				gyroRotate(b, sdt, 4);
+				V3 pr = b.pang;
+				b.q = Q(sdt * pr.mag(), pr) * b.q;
+			}
+			b.pvel = V3();
+			b.pang = V3();
+			updateInertia(b);
+		}
+		if (first) {
+			std::vector<Contact> keep;
+			for (Contact& c : W.cache)
+				if (!(c.life < 0)) keep.push_back(c);
+			// re-point island rows is unnecessary after this substep only if no later substep
+			// uses them; rebuild the island vectors against the compacted cache instead.
+			std::map<const Contact*, size_t> remap;
+			size_t k = 0;
+			for (Contact& c : W.cache) {
+				if (!(c.life < 0)) remap[&c] = k++;
+			}
+			for (auto& isl : islands)
+				for (auto& r : isl) r.first = (Contact*)(uintptr_t)(remap[r.first] + 1);
+			W.cache = keep;
+			for (auto& isl : islands)
+				for (auto& r : isl) r.first = &W.cache[(size_t)(uintptr_t)r.first - 1];
+		}
+	}
+	// update_geometry (PhysicsEngine.cpp:424-438)
+	for (Body& b : W.bodies) {
+		if (!b.dynamic()) continue;
+		if (b.com != b.prev_com || b.q != b.prev_q) {
+			b.q = b.q.normalize();
+			worldGeometry(b);
+		}
+	}
+}
+
+}  // namespace
+
+struct orc_world : World {};
+
+extern "C" {
+
+orc_world* orc_create(const orc_config* cfg) {
+	orc_world* w = new orc_world();
+	w->cfg = *cfg;
+	V3 g(cfg->gravity[0], cfg->gravity[1], cfg->gravity[2]);
+	w->cutoff_vel = 2 * g.mag() * cfg->dt;  // PhysicsEngine.h:193
+	return w;
+}
+
+void orc_destroy(orc_world* w) { delete w; }
+
+int orc_add_hull_body(orc_world* w, int movement, const double* mp, const double* mat, int n_pts, const double* pts, int n_faces,
+                      const int64_t* face_off, const int64_t* face_verts, const int64_t* surface_id, const int64_t* normal_calc,
+                      const int64_t* normal_dir, const double* gn, const int64_t* gred, const int64_t* gint, const double* gval,
+                      int n_arcs, const int64_t* arcs, int n_edges, const int64_t* edges) {
+	Body b{};
+	b.id = (int)w->bodies.size() + 1;
+	b.movement = movement;
+	b.mass = mp[0];
+	for (int r = 0; r < 3; r++)
+		for (int c = 0; c < 3; c++) {
+			b.I_ref.v[r][c] = mp[1 + 3 * r + c];
+			b.I_ref_inv.v[r][c] = mp[10 + 3 * r + c];
+		}
+	b.restitution = mat[1];
+	b.mu_k = mat[2];
+	b.mu_s = mat[3];
+	Hull& h = b.hull;
+	for (int i = 0; i < n_pts; i++) h.ref_pts.push_back(V3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]));
+	h.adj_faces.resize(n_pts);
+	h.adj_edges.resize(n_pts);
+	for (int f = 0; f < n_faces; f++) {
+		std::vector<int> fv;
+		for (int64_t k = face_off[f]; k < face_off[f + 1]; k++) fv.push_back((int)face_verts[k]);
+		h.faces.push_back(fv);
+		h.surf_id.push_back((int)surface_id[f]);
+		h.ncalc.push_back((int)normal_calc[f]);
+		h.ndir.push_back((int)normal_dir[f]);
+		for (int v : fv) h.adj_faces[v].push_back(f);  // Polyhedron ctor, ConvexPrimitive.cpp:597-604
+		h.gnorm.push_back(V3(gn[3 * f], gn[3 * f + 1], gn[3 * f + 2]));
+		h.gred.push_back((int)gred[f]);
+		h.gmin_pid.push_back((int)gint[3 * f]);
+		h.gmax_pid.push_back((int)gint[3 * f + 1]);
+		h.gref_idx.push_back((int)gint[3 * f + 2]);
+		h.gmin.push_back(gval[3 * f]);
+		h.gmax.push_back(gval[3 * f + 1]);
+		h.gref_val.push_back(gval[3 * f + 2]);
+	}
+	for (int a = 0; a < n_arcs; a++) h.arcs.push_back({(int)arcs[2 * a], (int)arcs[2 * a + 1]});
+	for (int e = 0; e < n_edges; e++) {
+		h.edges.push_back({(int)edges[2 * e], (int)edges[2 * e + 1]});
+		h.adj_edges[edges[2 * e]].push_back(e);
+		h.adj_edges[edges[2 * e + 1]].push_back(e);
+	}
+	w->bodies.push_back(b);
+	return (int)w->bodies.size() - 1;
+}
+
+void orc_set_state(orc_world* w, const double* s) {
+	for (size_t i = 0; i < w->bodies.size(); i++) {
+		Body& b = w->bodies[i];
+		const double* r = s + 13 * i;
+		b.com = V3(r[0], r[1], r[2]);
+		b.q = Q(r[3], r[4], r[5], r[6]);
+		b.vel = V3(r[7], r[8], r[9]);
+		b.ang_vel = V3(r[10], r[11], r[12]);
+		b.pvel = V3();
+		b.pang = V3();
+	}
+}
+
+void orc_get_state(const orc_world* w, double* s) {
+	for (size_t i = 0; i < w->bodies.size(); i++) {
+		const Body& b = w->bodies[i];
+		double* r = s + 13 * i;
+		r[0] = b.com.x; r[1] = b.com.y; r[2] = b.com.z;
+		r[3] = b.q.r; r[4] = b.q.i; r[5] = b.q.j; r[6] = b.q.k;
+		r[7] = b.vel.x; r[8] = b.vel.y; r[9] = b.vel.z;
+		r[10] = b.ang_vel.x; r[11] = b.ang_vel.y; r[12] = b.ang_vel.z;
+	}
+}
+
+// The stored orientation is already normalised by the previous step's
+// updateGeometry; the inertia tensor follows it (updateInertiaTensor normalises
+// again, which the reference also does every substep).
+void orc_refresh(orc_world* w) {
+	for (Body& b : w->bodies) {
+		Q keep = b.q;
+		updateInertia(b);
+		b.q = keep;
+		worldGeometry(b);
+	}
+}
+
+void orc_set_inertia(orc_world* w, const double* in) {
+	for (size_t i = 0; i < w->bodies.size(); i++) {
+		Body& b = w->bodies[i];
+		for (int r = 0; r < 3; r++)
+			for (int c = 0; c < 3; c++) {
+				b.I.v[r][c] = in[18 * i + 3 * r + c];
+				b.I_inv.v[r][c] = in[18 * i + 9 + 3 * r + c];
+			}
+	}
+}
+
+void orc_set_cache(orc_world* w, int n, const int64_t* ints, const uint64_t* magic, const double* f) {
+	w->cache.clear();
+	for (int k = 0; k < n; k++) {
+		const int64_t* ii = ints + 6 * k;
+		const double* ff = f + 21 * k;
+		Contact c{};
+		c.b1 = (int)ii[0] - 1;
+		c.b2 = (int)ii[1] - 1;
+		c.life = (int)ii[2];
+		c.live = ii[3] != 0;
+		c.f.static_ready = ii[4] != 0;
+		c.cid = magic[2 * k];
+		c.bid = magic[2 * k + 1];
+		c.lp1 = V3(ff[0], ff[1], ff[2]);
+		c.lp2 = V3(ff[3], ff[4], ff[5]);
+		c.ln1 = V3(ff[6], ff[7], ff[8]);
+		c.pen = ff[9];
+		c.bounce = ff[10];
+		c.mu = ff[11];
+		c.c.imp = ff[12];
+		c.f.imp[0] = ff[13];
+		c.f.imp[1] = ff[14];
+		c.f.u = V3(ff[15], ff[16], ff[17]);
+		c.f.w = V3(ff[18], ff[19], ff[20]);
+		c.hardness = w->cfg.contact_hardness;
+		c.cutoff = w->cutoff_vel;
+		c.seq = w->seq++;
+		w->cache.push_back(c);
+	}
+}
+
+static void dumpCache(const orc_world* w, const std::vector<Contact>& cache, int64_t* ints, uint64_t* magic, double* f) {
+	int pos = 0;
+	for (size_t k = 0; k < cache.size(); k++) {
+		const Contact& c = cache[k];
+		if (k > 0 && (cache[k - 1].b1 != c.b1 || cache[k - 1].b2 != c.b2)) pos = 0;
+		int64_t* ii = ints + 6 * k;
+		ii[0] = w->bodies[c.b1].id;
+		ii[1] = w->bodies[c.b2].id;
+		ii[2] = c.life;
+		ii[3] = c.live;
+		ii[4] = c.f.static_ready;
+		ii[5] = pos++;
+		magic[2 * k] = c.cid;
+		magic[2 * k + 1] = c.bid;
+		double* ff = f + 21 * k;
+		double v[21] = {c.lp1.x, c.lp1.y, c.lp1.z, c.lp2.x, c.lp2.y, c.lp2.z, c.ln1.x, c.ln1.y, c.ln1.z, c.pen, c.bounce,
+		                c.mu, c.c.imp, c.f.imp[0], c.f.imp[1], c.f.u.x, c.f.u.y, c.f.u.z, c.f.w.x, c.f.w.y, c.f.w.z};
+		memcpy(ff, v, sizeof v);
+	}
+}
+
+int orc_cache_size(const orc_world* w) { return (int)w->cache.size(); }
+void orc_get_cache(const orc_world* w, int64_t* ints, uint64_t* magic, double* f) { dumpCache(w, w->cache, ints, magic, f); }
+int orc_last_mid_cache_size(const orc_world* w) { return (int)w->last_mid.size(); }
+void orc_last_mid_cache(const orc_world* w, int64_t* ints, uint64_t* magic, double* f) { dumpCache(w, w->last_mid, ints, magic, f); }
+
+int orc_broadphase(const orc_world* w, int64_t* out, int cap) {
+	auto v = broadphase(*w);
+	int n = 0;
+	for (auto& p : v) {
+		if (n < cap) {
+			out[2 * n] = w->bodies[p.first].id;
+			out[2 * n + 1] = w->bodies[p.second].id;
+		}
+		n++;
+	}
+	return n;
+}
+
+void orc_get_aabbs(const orc_world* w, double* a) {
+	for (size_t i = 0; i < w->bodies.size(); i++) memcpy(a + 6 * i, w->bodies[i].aabb, 6 * sizeof(double));
+}
+
+void orc_get_world_points(const orc_world* w, int body, double* p) {
+	const Hull& h = w->bodies[body].hull;
+	for (size_t i = 0; i < h.pts.size(); i++) {
+		p[3 * i] = h.pts[i].x; p[3 * i + 1] = h.pts[i].y; p[3 * i + 2] = h.pts[i].z;
+	}
+}
+
+int orc_narrowphase(const orc_world* w, int id_lo, int id_hi, double* head, double* pts, uint64_t* magic) {
+	Manifold m = narrow(*w, id_lo - 1, id_hi - 1);
+	if (m.pts.empty()) return 0;
+	head[0] = m.n.x; head[1] = m.n.y; head[2] = m.n.z; head[3] = m.max_pen;
+	for (size_t k = 0; k < m.pts.size(); k++) {
+		pts[4 * k] = m.pts[k].pos.x; pts[4 * k + 1] = m.pts[k].pos.y; pts[4 * k + 2] = m.pts[k].pos.z; pts[4 * k + 3] = m.pts[k].pen;
+		magic[2 * k] = m.pts[k].cid;
+		magic[2 * k + 1] = m.pts[k].bid;
+	}
+	return (int)m.pts.size();
+}
+
+void orc_step(orc_world* w, const int64_t* node_order) { step(*w, node_order); }
+
+int orc_last_islands(const orc_world* w, int64_t* rows, int64_t* offs, int cap) {
+	int n = (int)w->last_rows.size() / 2;
+	if (n <= cap) {
+		memcpy(rows, w->last_rows.data(), w->last_rows.size() * sizeof(int64_t));
+		memcpy(offs, w->last_offs.data(), w->last_offs.size() * sizeof(int64_t));
+	}
+	return (int)w->last_offs.size();
+}
+
+}  // extern "C"

@@ -1,0 +1,85 @@
+// cp_oracle — FP64 CPU restatement of ConstraintPhysics' per-step contact
+// pipeline.  TEST INFRASTRUCTURE: only tests/, __graft_entry__.smoke() and the
+// bench's cpu_baseline leg may load it, and only as the checker.  The product
+// (constraintphysics_amd) never links or calls it.
+//
+// Every function follows the reference's operation order so that, fed the same
+// per-step state, it reproduces the reference's FP64 results bit for bit (the
+// reference is compiled with -ffp-contract=off; so is this file).  Each
+// function cites the reference file:line it restates.  Parity of each part is
+// pinned against golden vectors dumped from the compiled reference
+// (tests/golden/, produced by oracle/ref_harness.cpp) — see tests/test_oracle.py.
+#pragma once
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_world orc_world;
+
+// Engine parameters (PhysicsEngine.h:174-216 defaults unless set).
+typedef struct {
+	double dt;            // step time
+	double gravity[3];
+	int vel_iters, pos_iters, substeps;
+	double cfm;           // global constraint-force mixing
+	double contact_hardness;
+	int contact_life;
+} orc_config;
+
+orc_world* orc_create(const orc_config* cfg);
+void orc_destroy(orc_world* w);
+
+// Add a body.  movement: 0 dynamic, 1 fixed.  mass_props: mass, I_ref[9], I_ref_inv[9].
+// material: density, restitution, kinetic mu, static mu.
+// Polyhedron reference geometry in the body frame: n_pts points, n_faces faces
+// (face_off[n_faces+1] into face_verts), per-face normal_calc_index and
+// normal_direction (ConvexPrimitive.cpp:791-812), Gauss map (normals, redundant
+// flags, cached extrema ints (min_pid, max_pid, ref_idx) and vals (min, max,
+// ref_val)), arcs, edges.  Returns body index (ids are index+1).
+int orc_add_hull_body(orc_world* w, int movement, const double* mass_props, const double* material,
+                      int n_pts, const double* pts, int n_faces, const int64_t* face_off, const int64_t* face_verts,
+                      const int64_t* surface_id, const int64_t* normal_calc, const int64_t* normal_dir,
+                      const double* gauss_normal, const int64_t* gauss_red, const int64_t* gauss_ints, const double* gauss_vals,
+                      int n_arcs, const int64_t* arcs, int n_edges, const int64_t* edges);
+
+// Body state rows of 13 doubles: com 3, q (r,i,j,k), vel 3, ang_vel 3.
+void orc_set_state(orc_world* w, const double* state);
+void orc_get_state(const orc_world* w, double* state);
+// World inertia rows of 18 doubles: tensor 9, inverse tensor 9 (RigidBody.h:116-117).
+void orc_set_inertia(orc_world* w, const double* inertia);
+// Recompute world geometry/AABBs/inertia from the current state (RigidBody::updateGeometry
+// and updateInertiaTensor, RigidBody.cpp:270-312) without changing the state.
+void orc_refresh(orc_world* w);
+
+// Contact cache rows: ints (b1, b2, memory_life, is_live, static_ready, pos_in_edge),
+// magic (cid, bid), 21 doubles (local_b1 3, local_b2 3, normal_local_b1 3, pen, bounce,
+// mu, imp_n, imp_t0, imp_t1, u 3, w 3).  Rows are grouped per edge in edge order.
+void orc_set_cache(orc_world* w, int n, const int64_t* ints, const uint64_t* magic, const double* f);
+int orc_cache_size(const orc_world* w);
+void orc_get_cache(const orc_world* w, int64_t* ints, uint64_t* magic, double* f);
+
+// Broadphase: all (lo, hi) body-id pairs with overlapping closed AABBs, sorted.
+int orc_broadphase(const orc_world* w, int64_t* pairs, int cap);
+void orc_get_aabbs(const orc_world* w, double* aabbs);
+void orc_get_world_points(const orc_world* w, int body, double* pts);
+
+// Narrowphase of one body pair (ids lo < hi): culled manifold as the engine
+// would add it (PhysicsEngine.cpp:167-277).  Returns number of points (0 if no
+// collision); head = normal 3, max_pen; pts rows = pos 3, pen; magic rows = cid, bid.
+int orc_narrowphase(const orc_world* w, int id_lo, int id_hi, double* head, double* pts, uint64_t* magic);
+
+// One full step (PhysicsEngine::timeStep for the configuration used here:
+// no sleeping, no actions, contacts only).  node_order: body ids in the order
+// the island search visits roots (PhysicsEngine.cpp:1540); NULL = ascending id.
+void orc_step(orc_world* w, const int64_t* node_order);
+
+// Island rows of the last step: (cache index into the post-narrowphase cache, kind 0 normal / 1 friction).
+int orc_last_islands(const orc_world* w, int64_t* rows, int64_t* offs, int cap);
+int orc_last_mid_cache_size(const orc_world* w);
+void orc_last_mid_cache(const orc_world* w, int64_t* ints, uint64_t* magic, double* f);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,543 @@
+// Golden-fixture dumper and CPU-baseline timer over the compiled reference
+// engine (NathanMacLeod/ConstraintPhysics).  TEST INFRASTRUCTURE ONLY: this
+// program is the checker's source of truth, never part of the product path.
+//
+// It is compiled as ONE translation unit together with the reference's own
+// engine sources (the recipe in oracle/Makefile passes the reference root with
+// -I).  Member access is opened with `#define private public` so that the
+// per-step state (contact cache, constraint graph, island lists) can be read
+// without restating any of the reference's orchestration: the only hook into a
+// step is a collision action registered through the public API
+// (PhysicsEngine::registerCollisionAction, PhysicsEngine.h:110), which the
+// reference invokes after its narrowphase and before its island build
+// (PhysicsEngine.cpp:340-358 then :364).
+//
+// Output: a directory holding raw little-endian arrays (<name>.bin) and a
+// manifest.json describing dtype/shape of each, plus scalar metadata.
+
+#include <algorithm>
+#include <atomic>
+#include <cassert>
+#include <chrono>
+#include <cinttypes>
+#include <climits>
+#include <cmath>
+#include <condition_variable>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <functional>
+#include <limits>
+#include <map>
+#include <mutex>
+#include <set>
+#include <stack>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+#include <sys/stat.h>
+
+#define private public
+#define protected public
+#include "Math/src/Vec3.cpp"
+#include "Math/src/Mat3.cpp"
+#include "Math/src/NMat.cpp"
+#include "ConstraintPhysics/src/CFM.cpp"
+#include "ConstraintPhysics/src/CollisionDetect.cpp"
+#include "ConstraintPhysics/src/ConstraintSolver.cpp"
+#include "ConstraintPhysics/src/ConvexPrimitive.cpp"
+#include "ConstraintPhysics/src/Geometry.cpp"
+#include "ConstraintPhysics/src/HolonomicBlockSolver.cpp"
+#include "ConstraintPhysics/src/Octree.cpp"
+#include "ConstraintPhysics/src/PersistentConstraint.cpp"
+#include "ConstraintPhysics/src/PhysicsEngine.cpp"
+#include "ConstraintPhysics/src/RigidBody.cpp"
+#undef private
+#undef protected
+
+// Out-of-class definition the reference ODR-uses but never defines.
+const int phyz::HolonomicSystem::BLOCK_EMPTY;
+
+using phyz::PhysicsEngine;
+using phyz::RigidBody;
+
+// ---------------------------------------------------------------------------
+// Seeded generator shared bit-for-bit with the Python/C++ scene builders
+// (SplitMix64; u = (x >> 11) * 2^-53).
+struct SplitMix64 {
+	uint64_t s;
+	explicit SplitMix64(uint64_t seed) : s(seed) {}
+	uint64_t next() {
+		uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+		z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+		z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+		return z ^ (z >> 31);
+	}
+	double uniform() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); }
+	double uniform(double a, double b) { return a + (b - a) * uniform(); }
+};
+
+// ---------------------------------------------------------------------------
+struct Out {
+	std::string dir;
+	std::vector<std::string> entries;
+	std::vector<std::string> metas;
+
+	explicit Out(const std::string& d) : dir(d) { mkdir(d.c_str(), 0755); }
+
+	template <class T>
+	void put(const std::string& name, const std::vector<T>& v, std::vector<size_t> shape, const char* dtype) {
+		std::string path = dir + "/" + name + ".bin";
+		FILE* f = fopen(path.c_str(), "wb");
+		if (!f) { perror(path.c_str()); exit(2); }
+		if (!v.empty()) fwrite(v.data(), sizeof(T), v.size(), f);
+		fclose(f);
+		size_t n = 1;
+		for (size_t s : shape) n *= s;
+		if (n != v.size()) { fprintf(stderr, "shape mismatch for %s\n", name.c_str()); exit(2); }
+		std::string e = "\"" + name + "\": {\"dtype\": \"" + dtype + "\", \"shape\": [";
+		for (size_t i = 0; i < shape.size(); i++) e += (i ? ", " : "") + std::to_string(shape[i]);
+		e += "]}";
+		entries.push_back(e);
+	}
+	void f64(const std::string& n, const std::vector<double>& v, std::vector<size_t> s) { put(n, v, s, "f64"); }
+	void i64(const std::string& n, const std::vector<int64_t>& v, std::vector<size_t> s) { put(n, v, s, "i64"); }
+	void u64(const std::string& n, const std::vector<uint64_t>& v, std::vector<size_t> s) { put(n, v, s, "u64"); }
+	void meta(const std::string& k, double v) {
+		char b[64];
+		snprintf(b, sizeof b, "%.17g", v);
+		metas.push_back("\"" + k + "\": " + b);
+	}
+	void meta_s(const std::string& k, const std::string& v) { metas.push_back("\"" + k + "\": \"" + v + "\""); }
+	void finish() {
+		FILE* f = fopen((dir + "/manifest.json").c_str(), "w");
+		fprintf(f, "{\n\"arrays\": {\n");
+		for (size_t i = 0; i < entries.size(); i++) fprintf(f, "  %s%s\n", entries[i].c_str(), i + 1 < entries.size() ? "," : "");
+		fprintf(f, "},\n\"meta\": {\n");
+		for (size_t i = 0; i < metas.size(); i++) fprintf(f, "  %s%s\n", metas[i].c_str(), i + 1 < metas.size() ? "," : "");
+		fprintf(f, "}\n}\n");
+		fclose(f);
+	}
+};
+
+// ---------------------------------------------------------------------------
+// Engine configuration pins (public API only): sleeping off, explicit step
+// parameters, every body collidable.  See RigidBody.h:122 / :89 — both flags
+// are otherwise left uninitialised by the reference constructors.
+struct SceneCfg {
+	double dt = 1.0 / 120.0;
+	mthz::Vec3 gravity = mthz::Vec3(0, -6, 0);
+	int vel_iters = 10, pos_iters = 1, hol_iters = 1;
+	int substeps = 8;
+};
+
+static void configure(PhysicsEngine& E, const SceneCfg& c) {
+	E.setStep_time(c.dt);
+	E.setGravity(c.gravity);
+	E.setPGSIterations(c.vel_iters, c.pos_iters, c.hol_iters);
+	E.setSubstepCount(c.substeps);
+	E.setSleepingEnabled(false);
+}
+
+static void pin_body(RigidBody* r) {
+	r->setNoCollision(false);
+	r->setSleepDisabled(true);
+}
+
+static phyz::Material c0_material() {
+	phyz::Material m = phyz::Material::default_material();
+	m.density = 1.0;
+	m.restitution = 0.0;
+	m.kinetic_friction_coeff = 0.5;
+	m.static_friction_coeff = 0.5;
+	return m;
+}
+
+// Ground: FIXED box((-100,-1,-100), 200, 1, 200) -> top face at y = 0.
+// Cubes: n^3 unit cubes, centre (1.1 i, 0.6 + 1.1 j, 1.1 k) + U[-j,j]^3 and yaw
+// U[0, 2 pi) about +y, drawn in loop order j (y), i (x), k (z), draws (dx, dy, dz, yaw).
+static void build_pile(PhysicsEngine& E, int nx, int ny, int nz, uint64_t seed, double spacing, double jitter) {
+	phyz::Material m = c0_material();
+	RigidBody* g = E.createRigidBody(phyz::ConvexUnionGeometry::box(mthz::Vec3(-100, -1, -100), 200, 1, 200, m), RigidBody::FIXED);
+	pin_body(g);
+	SplitMix64 rng(seed);
+	for (int j = 0; j < ny; j++)
+		for (int i = 0; i < nx; i++)
+			for (int k = 0; k < nz; k++) {
+				double cx = spacing * i + rng.uniform(-jitter, jitter);
+				double cy = 0.6 + spacing * j + rng.uniform(-jitter, jitter);
+				double cz = spacing * k + rng.uniform(-jitter, jitter);
+				double yaw = rng.uniform(0.0, 2.0 * 3.14159265358979323846);
+				RigidBody* r = E.createRigidBody(phyz::ConvexUnionGeometry::box(mthz::Vec3(cx - 0.5, cy - 0.5, cz - 0.5), 1, 1, 1, m));
+				r->setOrientation(mthz::Quaternion(yaw, mthz::Vec3(0, 1, 0)));
+				pin_body(r);
+			}
+}
+
+// ---------------------------------------------------------------------------
+// Snapshot helpers.
+
+static std::map<int, std::pair<int, int>> prim_owner(PhysicsEngine& E) {
+	std::map<int, std::pair<int, int>> own;
+	for (RigidBody* b : E.bodies)
+		for (int i = 0; i < (int)b->geometry.size(); i++) own[b->geometry[i].getID()] = {(int)b->getID(), i};
+	return own;
+}
+
+// cID packs two global primitive ids (CollisionDetect.cpp:827-829); remap each
+// half to body_id * 16 + primitive index so that it is stable across runs.
+static uint64_t remap_cid(uint64_t cid, const std::map<int, std::pair<int, int>>& own) {
+	int a = (int)(cid & 0xFFFFFFFFull), b = (int)(cid >> 32);
+	auto fa = own.find(a), fb = own.find(b);
+	uint64_t ra = fa == own.end() ? 0xFFFFFFFFull : (uint64_t)(fa->second.first * 16 + fa->second.second);
+	uint64_t rb = fb == own.end() ? 0xFFFFFFFFull : (uint64_t)(fb->second.first * 16 + fb->second.second);
+	return ra | (rb << 32);
+}
+
+static void body_state(PhysicsEngine& E, std::vector<double>& s) {
+	for (RigidBody* b : E.bodies) {
+		mthz::Vec3 c = b->com;
+		mthz::Quaternion q = b->orientation;
+		s.insert(s.end(), {c.x, c.y, c.z, q.r, q.i, q.j, q.k, b->vel.x, b->vel.y, b->vel.z, b->ang_vel.x, b->ang_vel.y, b->ang_vel.z});
+	}
+}
+
+static void body_inertia(PhysicsEngine& E, std::vector<double>& s) {
+	for (RigidBody* b : E.bodies) {
+		for (int r = 0; r < 3; r++) for (int c = 0; c < 3; c++) s.push_back(b->tensor.v[r][c]);
+		for (int r = 0; r < 3; r++) for (int c = 0; c < 3; c++) s.push_back(b->invTensor.v[r][c]);
+	}
+}
+
+struct CacheDump {
+	std::vector<int64_t> ints;   // b1, b2, memory_life, is_live, static_ready, pos_in_edge
+	std::vector<uint64_t> magic; // cid(remapped), bid
+	std::vector<double> f;       // local_b1 3, local_b2 3, normal_local_b1 3, pen, bounce, mu, imp_n, imp_t 2, u 3, w 3
+	std::map<const phyz::Constraint*, std::pair<int, int>> rowmap; // solver row -> (cache index, kind)
+	int n = 0;
+};
+static const int CACHE_I = 6, CACHE_F = 21;
+
+static void dump_cache(PhysicsEngine& E, CacheDump& d) {
+	auto own = prim_owner(E);
+	std::vector<unsigned> ids;
+	for (auto& kv : E.constraint_graph_nodes) ids.push_back(kv.first);
+	std::sort(ids.begin(), ids.end());
+	for (unsigned id : ids) {
+		auto* n = E.constraint_graph_nodes[id];
+		for (auto* e : n->constraints) {
+			auto* other = e->other(n);
+			if (other->b->getID() < id) continue;
+			int pos = 0;
+			for (phyz::PersistentConstraint* pc : e->constraints) {
+				if (pc->getId().getType() != phyz::ConstraintID::CONTACT) { pos++; continue; }
+				auto* c = static_cast<phyz::Contact*>(pc);
+				d.ints.insert(d.ints.end(), {(int64_t)c->b1->getID(), (int64_t)c->b2->getID(), c->memory_life, c->is_live_contact ? 1 : 0,
+				                             c->friction.static_ready ? 1 : 0, pos});
+				d.magic.push_back(remap_cid(c->magic.cID, own));
+				d.magic.push_back(c->magic.bID);
+				d.f.insert(d.f.end(), {c->contact_pos_local_b1.x, c->contact_pos_local_b1.y, c->contact_pos_local_b1.z,
+				                       c->contact_pos_local_b2.x, c->contact_pos_local_b2.y, c->contact_pos_local_b2.z,
+				                       c->normal_local_b1.x, c->normal_local_b1.y, c->normal_local_b1.z, c->pen_depth, c->bounce,
+				                       c->friction_coeff, c->contact.impulse.v[0], c->friction.impulse.v[0], c->friction.impulse.v[1],
+				                       c->friction.u.x, c->friction.u.y, c->friction.u.z, c->friction.w.x, c->friction.w.y, c->friction.w.z});
+				d.rowmap[&c->contact] = {d.n, 0};
+				d.rowmap[&c->friction] = {d.n, 1};
+				d.n++;
+				pos++;
+			}
+		}
+	}
+}
+
+struct ManifoldDump {
+	std::vector<int64_t> pair;   // b1, b2, npoints
+	std::vector<double> head;    // normal 3, max_pen
+	std::vector<double> pts;     // pos 3, pen
+	std::vector<uint64_t> magic; // cid(remapped), bid
+};
+
+struct StepDump {
+	std::vector<double> pre_state, post_state, pre_inertia, post_inertia;
+	CacheDump pre_cache, mid_cache, post_cache;
+	ManifoldDump man;
+	std::vector<int64_t> pairs;        // broadphase candidate set, (lo, hi) sorted
+	std::vector<int64_t> island_rows;  // (cache index, kind) per row
+	std::vector<int64_t> island_off;   // row offsets per island
+	std::vector<int64_t> node_order;   // constraint_graph_nodes iteration order
+	bool mid_done = false;
+};
+
+static StepDump* g_cur = nullptr;
+static bool g_lite = false;  // omit the mid/post contact caches (large scenes)
+static PhysicsEngine* g_eng = nullptr;
+
+static void on_collision(RigidBody* b1, RigidBody* b2, const std::vector<phyz::Manifold>& ms) {
+	StepDump* d = g_cur;
+	if (!d) return;
+	auto own = prim_owner(*g_eng);
+	if (!d->mid_done) {
+		d->mid_done = true;
+		dump_cache(*g_eng, d->mid_cache);
+		for (auto& kv : g_eng->constraint_graph_nodes) d->node_order.push_back(kv.first);
+		PhysicsEngine::ActiveConstraintData a = g_eng->sleepOrSolveIslands();
+		d->island_off.push_back(0);
+		for (auto& isl : a.island_systems) {
+			for (phyz::Constraint* c : isl.constraints) {
+				auto it = d->mid_cache.rowmap.find(c);
+				if (it == d->mid_cache.rowmap.end()) d->island_rows.insert(d->island_rows.end(), {-1, -1});
+				else d->island_rows.insert(d->island_rows.end(), {it->second.first, it->second.second});
+			}
+			d->island_off.push_back((int64_t)d->island_rows.size() / 2);
+		}
+	}
+	for (const phyz::Manifold& m : ms) {
+		d->man.pair.insert(d->man.pair.end(), {(int64_t)b1->getID(), (int64_t)b2->getID(), (int64_t)m.points.size()});
+		d->man.head.insert(d->man.head.end(), {m.normal.x, m.normal.y, m.normal.z, m.max_pen_depth});
+		for (const phyz::ContactP& p : m.points) {
+			d->man.pts.insert(d->man.pts.end(), {p.pos.x, p.pos.y, p.pos.z, p.pen_depth});
+			d->man.magic.push_back(remap_cid(p.magicID.cID, own));
+			d->man.magic.push_back(p.magicID.bID);
+		}
+	}
+}
+
+static void broadphase_pairs(PhysicsEngine& E, std::vector<int64_t>& out) {
+	std::vector<std::pair<int64_t, int64_t>> v;
+	for (auto& p : E.aabb_tree.getAllCollisionCandidates()) {
+		int64_t a = p.t1->getID(), b = p.t2->getID();
+		if (a > b) std::swap(a, b);
+		v.push_back({a, b});
+	}
+	std::sort(v.begin(), v.end());
+	for (auto& p : v) out.insert(out.end(), {p.first, p.second});
+}
+
+// Static per-body data: mass properties and, for polyhedra, the body-frame hull
+// and its Gauss map exactly as the reference stores them.
+static void dump_static(PhysicsEngine& E, Out& o) {
+	std::vector<double> mp;
+	std::vector<int64_t> bi; // id, movement type, geometry type, n primitives, primitive type
+	std::vector<double> pts, gv, gx;
+	std::vector<int64_t> poff{0}, foff{0}, fidx{0}, fverts, gred, gint, arcs, aoff{0}, edges, eoff{0}, surfid, ncalc, ndir;
+	std::vector<double> mat;
+	for (RigidBody* b : E.bodies) {
+		mp.push_back(b->mass);
+		for (int r = 0; r < 3; r++) for (int c = 0; c < 3; c++) mp.push_back(b->reference_tensor.v[r][c]);
+		for (int r = 0; r < 3; r++) for (int c = 0; c < 3; c++) mp.push_back(b->reference_invTensor.v[r][c]);
+		int ptype = b->geometry.empty() ? -1 : (int)b->geometry[0].getType();
+		bi.insert(bi.end(), {(int64_t)b->getID(), (int64_t)b->movement_type, (int64_t)b->geometry_type, (int64_t)b->geometry.size(), ptype});
+		if (!b->geometry.empty()) {
+			const phyz::Material& m = b->geometry[0].material;
+			mat.insert(mat.end(), {m.density, m.restitution, m.kinetic_friction_coeff, m.static_friction_coeff});
+		} else mat.insert(mat.end(), {0, 0, 0, 0});
+		if (ptype == phyz::POLYHEDRON && b->geometry.size() == 1) {
+			const phyz::Polyhedron& ref = (const phyz::Polyhedron&)*b->reference_geometry[0].getGeometry();
+			const phyz::Polyhedron& wrl = (const phyz::Polyhedron&)*b->geometry[0].getGeometry();
+			for (auto& p : ref.points) pts.insert(pts.end(), {p.x, p.y, p.z});
+			for (auto& s : ref.surfaces) {
+				for (uint32_t ix : s.point_indexes) fverts.push_back(ix);
+				fidx.push_back((int64_t)fverts.size());
+				surfid.push_back(s.getSurfaceID());
+				ncalc.push_back(s.normal_calc_index);
+				ndir.push_back(s.normalDirection);
+			}
+			const phyz::GaussMap& g = wrl.gauss_map; // cached extrema live in the world copy
+			for (size_t i = 0; i < g.face_verts.size(); i++) {
+				const phyz::GaussVert& v = g.face_verts[i];
+				const phyz::GaussVert& rv = ref.gauss_map.face_verts[i];
+				gv.insert(gv.end(), {rv.v.x, rv.v.y, rv.v.z});
+				gred.push_back(v.SAT_redundant ? 1 : 0);
+				gint.insert(gint.end(), {v.cached_SAT_query.min_pID, v.cached_SAT_query.max_pID, (int64_t)v.SAT_reference_point_index});
+				gx.insert(gx.end(), {v.cached_SAT_query.min_val, v.cached_SAT_query.max_val, v.SAT_reference_point_value});
+			}
+			for (auto& a : g.arcs) arcs.insert(arcs.end(), {(int64_t)a.v1_indx, (int64_t)a.v2_indx});
+			for (auto& e : ref.edges) edges.insert(edges.end(), {e.p1_indx, e.p2_indx});
+		}
+		poff.push_back((int64_t)pts.size() / 3);
+		foff.push_back((int64_t)fidx.size() - 1);
+		aoff.push_back((int64_t)arcs.size() / 2);
+		eoff.push_back((int64_t)edges.size() / 2);
+	}
+	size_t N = E.bodies.size();
+	o.f64("mass_props", mp, {N, 19});
+	o.i64("body_info", bi, {N, 5});
+	o.f64("material", mat, {N, 4});
+	o.f64("hull_points", pts, {pts.size() / 3, 3});
+	o.i64("hull_point_off", poff, {N + 1});
+	o.i64("hull_face_off", foff, {N + 1});
+	o.i64("hull_face_idx", fidx, {fidx.size()});
+	o.i64("hull_face_verts", fverts, {fverts.size()});
+	o.i64("hull_surface_id", surfid, {surfid.size()});
+	o.i64("hull_normal_calc", ncalc, {ncalc.size()});
+	o.i64("hull_normal_dir", ndir, {ndir.size()});
+	o.f64("gauss_normal", gv, {gv.size() / 3, 3});
+	o.i64("gauss_redundant", gred, {gred.size()});
+	o.i64("gauss_ints", gint, {gint.size() / 3, 3});
+	o.f64("gauss_vals", gx, {gx.size() / 3, 3});
+	o.i64("gauss_arcs", arcs, {arcs.size() / 2, 2});
+	o.i64("gauss_arc_off", aoff, {N + 1});
+	o.i64("hull_edges", edges, {edges.size() / 2, 2});
+	o.i64("hull_edge_off", eoff, {N + 1});
+}
+
+static void put_cache(Out& o, const std::string& p, const CacheDump& c) {
+	o.i64(p + "_i", c.ints, {(size_t)c.n, (size_t)CACHE_I});
+	o.u64(p + "_m", c.magic, {(size_t)c.n, 2});
+	o.f64(p + "_f", c.f, {(size_t)c.n, (size_t)CACHE_F});
+}
+
+// Run `steps` steps; record full dumps at the listed step indices.
+static void run_trace(PhysicsEngine& E, const SceneCfg& cfg, int steps, const std::set<int>& record, Out& o) {
+	size_t N = E.bodies.size();
+	dump_static(E, o);
+	{
+		std::vector<double> s;
+		body_state(E, s);
+		o.f64("initial_state", s, {N, 13});
+	}
+	o.meta("n_bodies", (double)N);
+	o.meta("dt", cfg.dt);
+	o.meta("gravity_y", cfg.gravity.y);
+	o.meta("vel_iters", cfg.vel_iters);
+	o.meta("pos_iters", cfg.pos_iters);
+	o.meta("substeps", cfg.substeps);
+	o.meta("steps", steps);
+	std::vector<int64_t> rec_list;
+	g_eng = &E;
+	E.registerCollisionAction(phyz::CollisionTarget::all(), phyz::CollisionTarget::all(), on_collision);
+	for (int k = 0; k < steps; k++) {
+		bool rec = record.count(k) > 0;
+		StepDump d;
+		if (rec) {
+			body_state(E, d.pre_state);
+			body_inertia(E, d.pre_inertia);
+			dump_cache(E, d.pre_cache);
+			g_cur = &d;
+		}
+		E.timeStep();
+		g_cur = nullptr;
+		if (!rec) continue;
+		rec_list.push_back(k);
+		broadphase_pairs(E, d.pairs);
+		body_state(E, d.post_state);
+		body_inertia(E, d.post_inertia);
+		dump_cache(E, d.post_cache);
+		std::string p = "s" + std::to_string(k);
+		o.f64(p + "_pre", d.pre_state, {N, 13});
+		o.f64(p + "_post", d.post_state, {N, 13});
+		o.f64(p + "_pre_inertia", d.pre_inertia, {N, 18});
+		o.f64(p + "_post_inertia", d.post_inertia, {N, 18});
+		put_cache(o, p + "_cache_pre", d.pre_cache);
+		if (!g_lite) {
+			put_cache(o, p + "_cache_mid", d.mid_cache);
+			put_cache(o, p + "_cache_post", d.post_cache);
+		}
+		o.i64(p + "_pairs", d.pairs, {d.pairs.size() / 2, 2});
+		o.i64(p + "_man_pair", d.man.pair, {d.man.pair.size() / 3, 3});
+		o.f64(p + "_man_head", d.man.head, {d.man.head.size() / 4, 4});
+		o.f64(p + "_man_pts", d.man.pts, {d.man.pts.size() / 4, 4});
+		o.u64(p + "_man_magic", d.man.magic, {d.man.magic.size() / 2, 2});
+		o.i64(p + "_island_rows", d.island_rows, {d.island_rows.size() / 2, 2});
+		if (d.island_off.empty()) d.island_off.push_back(0);
+		o.i64(p + "_island_off", d.island_off, {d.island_off.size()});
+		o.i64(p + "_node_order", d.node_order, {d.node_order.size()});
+	}
+	o.i64("recorded_steps", rec_list, {rec_list.size()});
+	{
+		std::vector<double> s;
+		body_state(E, s);
+		o.f64("final_state", s, {N, 13});
+	}
+}
+
+// ---------------------------------------------------------------------------
+static double arg_d(int argc, char** argv, const char* key, double def) {
+	for (int i = 1; i + 1 < argc; i++) if (!strcmp(argv[i], key)) return atof(argv[i + 1]);
+	return def;
+}
+static std::string arg_s(int argc, char** argv, const char* key, const char* def) {
+	for (int i = 1; i + 1 < argc; i++) if (!strcmp(argv[i], key)) return argv[i + 1];
+	return def;
+}
+static std::set<int> arg_steps(int argc, char** argv, const char* key) {
+	std::set<int> s;
+	std::string v = arg_s(argc, argv, key, "");
+	size_t pos = 0;
+	while (pos < v.size()) {
+		size_t c = v.find(',', pos);
+		if (c == std::string::npos) c = v.size();
+		s.insert(atoi(v.substr(pos, c - pos).c_str()));
+		pos = c + 1;
+	}
+	return s;
+}
+
+// pile: C0-pattern box pile (n x n x n), trace dump.
+static int cmd_pile(int argc, char** argv) {
+	int n = (int)arg_d(argc, argv, "--n", 3);
+	int ny = (int)arg_d(argc, argv, "--ny", n);
+	int steps = (int)arg_d(argc, argv, "--steps", 10);
+	uint64_t seed = (uint64_t)arg_d(argc, argv, "--seed", 12345);
+	SceneCfg cfg;
+	g_lite = arg_d(argc, argv, "--lite", 0) != 0;
+	cfg.vel_iters = (int)arg_d(argc, argv, "--vel", 10);
+	cfg.substeps = (int)arg_d(argc, argv, "--substeps", 8);
+	PhysicsEngine E;
+	configure(E, cfg);
+	build_pile(E, n, ny, n, seed, arg_d(argc, argv, "--spacing", 1.1), arg_d(argc, argv, "--jitter", 0.05));
+	Out o(arg_s(argc, argv, "--out", "out"));
+	o.meta("seed", (double)seed);
+	o.meta("nx", n);
+	o.meta("ny", ny);
+	run_trace(E, cfg, steps, arg_steps(argc, argv, "--record"), o);
+	o.finish();
+	return 0;
+}
+
+// time: single-threaded wall time of the reference's own timeStep on a C0 pile.
+static int cmd_time(int argc, char** argv) {
+	int n = (int)arg_d(argc, argv, "--n", 10);
+	int steps = (int)arg_d(argc, argv, "--steps", 300);
+	int warm = (int)arg_d(argc, argv, "--warmup", 10);
+	double budget = arg_d(argc, argv, "--budget-s", 1e30);
+	uint64_t seed = (uint64_t)arg_d(argc, argv, "--seed", 12345);
+	SceneCfg cfg;
+	PhysicsEngine E;
+	configure(E, cfg);
+	build_pile(E, n, n, n, seed, 1.1, 0.05);
+	int dyn = 0;
+	for (RigidBody* b : E.bodies) dyn += b->getMovementType() == RigidBody::DYNAMIC;
+	for (int k = 0; k < warm; k++) E.timeStep();
+	std::vector<double> t;
+	double total = 0;
+	for (int k = 0; k < steps; k++) {
+		auto a = std::chrono::steady_clock::now();
+		E.timeStep();
+		auto b = std::chrono::steady_clock::now();
+		double s = std::chrono::duration<double>(b - a).count();
+		t.push_back(s);
+		total += s;
+		if (total > budget) break;
+	}
+	std::vector<double> st = t;
+	std::sort(st.begin(), st.end());
+	double p50 = st[st.size() / 2], p99 = st[std::min(st.size() - 1, (size_t)(0.99 * st.size()))];
+	printf("{\"bodies\": %d, \"steps\": %zu, \"warmup\": %d, \"wall_s\": %.6f, \"body_steps_per_s\": %.3f, \"p50_ms\": %.3f, \"p99_ms\": %.3f}\n",
+	       dyn, t.size(), warm, total, dyn * (double)t.size() / total, p50 * 1e3, p99 * 1e3);
+	return 0;
+}
+
+int main(int argc, char** argv) {
+	if (argc < 2) {
+		fprintf(stderr, "usage: cp_ref_harness pile|time [options]\n");
+		return 2;
+	}
+	std::string cmd = argv[1];
+	if (cmd == "pile") return cmd_pile(argc, argv);
+	if (cmd == "time") return cmd_time(argc, argv);
+	fprintf(stderr, "unknown command %s\n", cmd.c_str());
+	return 2;
+}

@@ -1,0 +1,224 @@
+"""Golden-fixture loading and the ctypes binding of the CPU oracle.
+
+A fixture directory holds raw little-endian arrays (<name>.bin) described by
+manifest.json (dtype + shape) and scalar metadata; they are written by
+oracle/ref_harness.cpp from the compiled reference engine.  Nothing here
+executes code from a fixture.
+"""
+import ctypes
+import json
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+ORACLE_SO = os.path.join(ROOT, "oracle", "_build", "libcp_oracle.so")
+
+_DT = {"f64": np.float64, "i64": np.int64, "u64": np.uint64}
+
+
+class Fixture:
+    def __init__(self, path):
+        self.path = path
+        with open(os.path.join(path, "manifest.json")) as f:
+            m = json.load(f)
+        self.arrays = m["arrays"]
+        self.meta = m["meta"]
+        self._cache = {}
+
+    def __contains__(self, name):
+        return name in self.arrays
+
+    def __getitem__(self, name):
+        if name not in self._cache:
+            spec = self.arrays[name]
+            a = np.fromfile(os.path.join(self.path, name + ".bin"), dtype=_DT[spec["dtype"]])
+            self._cache[name] = a.reshape(spec["shape"])
+        return self._cache[name]
+
+    @property
+    def recorded(self):
+        return [int(k) for k in self["recorded_steps"]]
+
+    def hull(self, b):
+        """Body-frame hull and Gauss map of body index b (as the reference stores them)."""
+        po, fo, ao, eo = self["hull_point_off"], self["hull_face_off"], self["gauss_arc_off"], self["hull_edge_off"]
+        f0, f1 = int(fo[b]), int(fo[b + 1])
+        fidx = self["hull_face_idx"]
+        face_off = fidx[f0:f1 + 1] - fidx[f0]
+        return dict(
+            pts=self["hull_points"][po[b]:po[b + 1]],
+            face_off=face_off,
+            face_verts=self["hull_face_verts"][fidx[f0]:fidx[f1]],
+            surface_id=self["hull_surface_id"][f0:f1],
+            normal_calc=self["hull_normal_calc"][f0:f1],
+            normal_dir=self["hull_normal_dir"][f0:f1],
+            gauss_normal=self["gauss_normal"][f0:f1],
+            gauss_red=self["gauss_redundant"][f0:f1],
+            gauss_ints=self["gauss_ints"][f0:f1],
+            gauss_vals=self["gauss_vals"][f0:f1],
+            arcs=self["gauss_arcs"][ao[b]:ao[b + 1]],
+            edges=self["hull_edges"][eo[b]:eo[b + 1]],
+        )
+
+    def cache(self, step, which):
+        p = f"s{step}_cache_{which}"
+        return self[p + "_i"], self[p + "_m"], self[p + "_f"]
+
+
+def golden(name):
+    return Fixture(os.path.join(GOLDEN, name))
+
+
+# ----------------------------------------------------------------------------- oracle
+class OrcConfig(ctypes.Structure):
+    _fields_ = [
+        ("dt", ctypes.c_double),
+        ("gravity", ctypes.c_double * 3),
+        ("vel_iters", ctypes.c_int),
+        ("pos_iters", ctypes.c_int),
+        ("substeps", ctypes.c_int),
+        ("cfm", ctypes.c_double),
+        ("contact_hardness", ctypes.c_double),
+        ("contact_life", ctypes.c_int),
+    ]
+
+
+def _p(a, t):
+    return a.ctypes.data_as(ctypes.POINTER(t))
+
+
+_lib = None
+
+
+def oracle_lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError(f"oracle library missing: {ORACLE_SO} (run make -C oracle oracle)")
+        lib = ctypes.CDLL(ORACLE_SO)
+        lib.orc_create.restype = ctypes.c_void_p
+        lib.orc_create.argtypes = [ctypes.POINTER(OrcConfig)]
+        lib.orc_destroy.argtypes = [ctypes.c_void_p]
+        for fn in ("orc_set_state", "orc_get_state", "orc_set_inertia", "orc_get_aabbs"):
+            getattr(lib, fn).argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        lib.orc_refresh.argtypes = [ctypes.c_void_p]
+        lib.orc_add_hull_body.restype = ctypes.c_int
+        lib.orc_add_hull_body.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                          ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 9 + [ctypes.c_int, ctypes.c_void_p,
+                                                                                                    ctypes.c_int, ctypes.c_void_p]
+        lib.orc_set_cache.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        lib.orc_cache_size.restype = ctypes.c_int
+        lib.orc_cache_size.argtypes = [ctypes.c_void_p]
+        lib.orc_get_cache.argtypes = [ctypes.c_void_p] * 4
+        lib.orc_last_mid_cache_size.restype = ctypes.c_int
+        lib.orc_last_mid_cache_size.argtypes = [ctypes.c_void_p]
+        lib.orc_last_mid_cache.argtypes = [ctypes.c_void_p] * 4
+        lib.orc_broadphase.restype = ctypes.c_int
+        lib.orc_broadphase.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+        lib.orc_get_world_points.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+        lib.orc_narrowphase.restype = ctypes.c_int
+        lib.orc_narrowphase.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        lib.orc_step.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        lib.orc_last_islands.restype = ctypes.c_int
+        lib.orc_last_islands.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+        _lib = lib
+    return _lib
+
+
+def _c(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+class Oracle:
+    """FP64 CPU restatement of the reference step, loaded with a fixture's scene."""
+
+    def __init__(self, fx, cfm=0.0000025, hardness=1000.0, life=6):
+        lib = oracle_lib()
+        self.lib = lib
+        m = fx.meta
+        cfg = OrcConfig(m["dt"], (ctypes.c_double * 3)(0.0, m["gravity_y"], 0.0), int(m["vel_iters"]), int(m["pos_iters"]),
+                        int(m["substeps"]), cfm, hardness, life)
+        self.w = ctypes.c_void_p(lib.orc_create(ctypes.byref(cfg)))
+        self.n = int(m["n_bodies"])
+        info = fx["body_info"]
+        mp = fx["mass_props"]
+        mat = fx["material"]
+        self._keep = []
+        for b in range(self.n):
+            h = fx.hull(b)
+            arrs = [_c(mp[b], np.float64), _c(mat[b], np.float64), _c(h["pts"], np.float64), _c(h["face_off"], np.int64),
+                    _c(h["face_verts"], np.int64), _c(h["surface_id"], np.int64), _c(h["normal_calc"], np.int64),
+                    _c(h["normal_dir"], np.int64), _c(h["gauss_normal"], np.float64), _c(h["gauss_red"], np.int64),
+                    _c(h["gauss_ints"], np.int64), _c(h["gauss_vals"], np.float64), _c(h["arcs"], np.int64), _c(h["edges"], np.int64)]
+            self._keep.append(arrs)
+            lib.orc_add_hull_body(self.w, int(info[b, 1]), arrs[0].ctypes.data, arrs[1].ctypes.data, len(h["pts"]), arrs[2].ctypes.data,
+                                  len(h["face_off"]) - 1, *[a.ctypes.data for a in arrs[3:12]], len(h["arcs"]), arrs[12].ctypes.data,
+                                  len(h["edges"]), arrs[13].ctypes.data)
+
+    def __del__(self):
+        try:
+            self.lib.orc_destroy(self.w)
+        except Exception:
+            pass
+
+    def load(self, state, inertia=None, cache=None):
+        st = _c(state, np.float64)
+        self.lib.orc_set_state(self.w, st.ctypes.data)
+        if inertia is not None:
+            self.lib.orc_refresh(self.w)
+            it = _c(inertia, np.float64)
+            self.lib.orc_set_inertia(self.w, it.ctypes.data)
+        else:
+            self.lib.orc_refresh(self.w)
+        if cache is not None:
+            ci, cm, cf = (_c(cache[0], np.int64), _c(cache[1], np.uint64), _c(cache[2], np.float64))
+            self.lib.orc_set_cache(self.w, len(ci), ci.ctypes.data, cm.ctypes.data, cf.ctypes.data)
+
+    def state(self):
+        s = np.zeros((self.n, 13))
+        self.lib.orc_get_state(self.w, s.ctypes.data)
+        return s
+
+    def aabbs(self):
+        a = np.zeros((self.n, 6))
+        self.lib.orc_get_aabbs(self.w, a.ctypes.data)
+        return a
+
+    def broadphase(self):
+        cap = self.n * 64
+        out = np.zeros((cap, 2), np.int64)
+        k = self.lib.orc_broadphase(self.w, out.ctypes.data, cap)
+        assert k <= cap
+        return out[:k]
+
+    def narrowphase(self, lo, hi):
+        head = np.zeros(4)
+        pts = np.zeros((64, 4))
+        mg = np.zeros((64, 2), np.uint64)
+        k = self.lib.orc_narrowphase(self.w, lo, hi, head.ctypes.data, pts.ctypes.data, mg.ctypes.data)
+        return head, pts[:k], mg[:k]
+
+    def step(self, node_order=None):
+        if node_order is not None:
+            no = _c(node_order, np.int64)
+            self.lib.orc_step(self.w, no.ctypes.data)
+        else:
+            self.lib.orc_step(self.w, None)
+
+    def cache(self, mid=False):
+        n = self.lib.orc_last_mid_cache_size(self.w) if mid else self.lib.orc_cache_size(self.w)
+        ci = np.zeros((n, 6), np.int64)
+        cm = np.zeros((n, 2), np.uint64)
+        cf = np.zeros((n, 21))
+        (self.lib.orc_last_mid_cache if mid else self.lib.orc_get_cache)(self.w, ci.ctypes.data, cm.ctypes.data, cf.ctypes.data)
+        return ci, cm, cf
+
+    def islands(self):
+        cap = 1 << 20
+        rows = np.zeros((cap, 2), np.int64)
+        offs = np.zeros(cap, np.int64)
+        no = self.lib.orc_last_islands(self.w, rows.ctypes.data, offs.ctypes.data, cap)
+        offs = offs[:no]
+        return rows[:offs[-1]], offs

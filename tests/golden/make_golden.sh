@@ -1,0 +1,15 @@
+#!/bin/sh
+# Regenerates the golden fixtures from the compiled reference engine.
+# Runs only where the reference checkout exists (REF, default /root/reference);
+# the GPU box uses the committed fixtures.
+set -e
+HERE=$(cd "$(dirname "$0")" && pwd)
+ROOT=$(cd "$HERE/../.." && pwd)
+make -C "$ROOT/oracle" ref REF="${REF:-/root/reference}"
+H="$ROOT/oracle/_ref/cp_ref_harness"
+cd "$HERE"
+rm -rf pile3 c0_full
+# C0 pattern, 3x3x3 cubes: full per-step dumps (state, caches, manifolds, islands).
+"$H" pile --n 3 --steps 40 --record 0,1,2,10,39 --out pile3
+# C0 in full (10x10x10 cubes): first step and a settled step (no mid/post caches).
+"$H" pile --n 10 --steps 121 --record 0,120 --lite 1 --out c0_full

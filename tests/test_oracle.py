@@ -1,0 +1,71 @@
+"""Pins the CPU oracle (oracle/cp_oracle.cpp) against golden vectors dumped
+from the compiled reference engine: every compared quantity is bit-exact.
+
+The only excluded field is `static_ready` of contacts created in the step
+itself: the reference's FrictionConstraint default constructor leaves it
+uninitialised (ConstraintSolver.h, `FrictionConstraint() : normal_impulse(nullptr) {}`),
+and it is overwritten before any read."""
+import numpy as np
+import pytest
+
+from fixtures import Oracle, golden
+
+
+def _load(o, fx, k):
+    o.load(fx[f"s{k}_pre"], fx[f"s{k}_pre_inertia"], fx.cache(k, "pre"))
+
+
+@pytest.fixture(scope="module")
+def pile3():
+    return golden("pile3")
+
+
+@pytest.mark.parametrize("k", [0, 1, 2, 10, 39])
+def test_broadphase_and_narrowphase_pile(pile3, k):
+    o = Oracle(pile3)
+    _load(o, pile3, k)
+    assert np.array_equal(o.broadphase(), pile3[f"s{k}_pairs"])
+    mp, mh, mpts, mm = (pile3[f"s{k}_man_{x}"] for x in ("pair", "head", "pts", "magic"))
+    off = 0
+    for i, (a, b, n) in enumerate(mp):
+        head, pts, mg = o.narrowphase(int(a), int(b))
+        assert len(pts) == n
+        assert np.array_equal(head, mh[i])
+        assert np.array_equal(pts, mpts[off:off + n])
+        assert np.array_equal(mg, mm[off:off + n])
+        off += n
+
+
+@pytest.mark.parametrize("k", [0, 1, 2, 10, 39])
+def test_full_step_pile(pile3, k):
+    o = Oracle(pile3)
+    _load(o, pile3, k)
+    o.step(pile3[f"s{k}_node_order"])
+    mi, mm, mf = o.cache(mid=True)
+    fi, fm, ff = pile3.cache(k, "mid")
+    pre_keys = {(int(a), int(b), int(c), int(d)) for (a, b, *_), (c, d) in zip(pile3.cache(k, "pre")[0], pile3.cache(k, "pre")[1])}
+    known = np.array([(int(a), int(b), int(c), int(d)) in pre_keys for (a, b, *_), (c, d) in zip(fi, fm)], bool)
+    cols = [0, 1, 2, 3, 5]
+    assert np.array_equal(mi[:, cols], fi[:, cols])
+    assert np.array_equal(mi[known, 4], fi[known, 4])
+    assert np.array_equal(mm, fm)
+    assert np.array_equal(mf, ff)
+    rows, offs = o.islands()
+    assert np.array_equal(rows, pile3[f"s{k}_island_rows"])
+    assert np.array_equal(offs, pile3[f"s{k}_island_off"])
+    assert np.array_equal(o.state(), pile3[f"s{k}_post"])
+    pi, pm, pf = o.cache()
+    gi, gm, gf = pile3.cache(k, "post")
+    assert np.array_equal(pi, gi) and np.array_equal(pm, gm) and np.array_equal(pf, gf)
+
+
+def test_c0_full_first_and_settled_step():
+    fx = golden("c0_full")
+    o = Oracle(fx)
+    for k in fx.recorded:
+        _load(o, fx, k)
+        assert np.array_equal(o.broadphase(), fx[f"s{k}_pairs"])
+        o.step(fx[f"s{k}_node_order"])
+        rows, offs = o.islands()
+        assert np.array_equal(rows, fx[f"s{k}_island_rows"])
+        assert np.array_equal(o.state(), fx[f"s{k}_post"])
