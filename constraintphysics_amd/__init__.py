@@ -1,0 +1,278 @@
+"""MI355X-native per-step contact pipeline of ConstraintPhysics.
+
+Python access to libcp_hip.so through its C ABI (include/cp_abi.h).  The
+library is built in-tree by ``python -m constraintphysics_amd.build``; loading
+fails loudly when it is missing — there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcp_hip.so")
+
+CP_PGS_COLORED = 0
+CP_PGS_REFERENCE_ORDER = 1
+
+
+class CpError(RuntimeError):
+    pass
+
+
+class Config(ctypes.Structure):
+    _fields_ = [
+        ("dt", ctypes.c_double),
+        ("gravity", ctypes.c_double * 3),
+        ("vel_iters", ctypes.c_int),
+        ("pos_iters", ctypes.c_int),
+        ("hol_iters", ctypes.c_int),
+        ("substeps", ctypes.c_int),
+        ("cfm", ctypes.c_double),
+        ("contact_hardness", ctypes.c_double),
+        ("contact_life", ctypes.c_int),
+        ("gyro_ticks", ctypes.c_int),
+        ("pgs_mode", ctypes.c_int),
+        ("solver_threads", ctypes.c_int),
+    ]
+
+
+_lib = None
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+
+
+def lib():
+    """Load libcp_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise CpError(f"{LIB_PATH} is missing: build it with `python -m constraintphysics_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    sig = {
+        "cp_config_default": (None, [_P]),
+        "cp_status_string": (ctypes.c_char_p, [_I]),
+        "cp_world_create": (_I, [_P, _P]),
+        "cp_world_destroy": (_I, [_P]),
+        "cp_world_set_config": (_I, [_P, _P]),
+        "cp_shape_add_hull_exact": (_I, [_P, _I, _P, _I] + [_P] * 9 + [_I, _P, _I, _P, _P]),
+        "cp_shape_add_polyhedron": (_I, [_P, _I, _P, _I, _P, _P, _P]),
+        "cp_polyhedron_mass_properties": (_I, [_I, _P, _I, _P, _P, ctypes.c_double, _P]),
+        "cp_body_add": (_I, [_P, _I, _I, _P, _P, _I, _P]),
+        "cp_world_body_count": (_I, [_P, _P]),
+        "cp_world_set_state": (_I, [_P, _P, _P]),
+        "cp_world_get_state": (_I, [_P, _P]),
+        "cp_world_get_aabbs": (_I, [_P, _P]),
+        "cp_world_set_contacts": (_I, [_P, _I, _P, _P, _P]),
+        "cp_world_get_contact_count": (_I, [_P, _P]),
+        "cp_world_get_contacts": (_I, [_P, _I, _P, _P, _P, _P]),
+        "cp_world_set_node_order": (_I, [_P, _I, _P]),
+        "cp_reference_node_order": (_I, [_I, _P]),
+        "cp_world_step": (_I, [_P, _I]),
+        "cp_broadphase": (_I, [_P, _I, _P, _P]),
+        "cp_narrowphase": (_I, [_P, _I, _P, _P, _P, _P, _P]),
+        "cp_world_last_islands": (_I, [_P, _I, _P, _I, _P, _P, _P]),
+        "cp_world_last_timing": (_I, [_P, _P]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def exported_symbols():
+    return [ln.split("(")[0].split()[-1].lstrip("*") for ln in _abi_decls()]
+
+
+def _abi_decls():
+    path = os.path.join(os.path.dirname(HERE), "include", "cp_abi.h")
+    out = []
+    buf = ""
+    for line in open(path):
+        s = line.strip()
+        if buf or s.startswith(("int cp_", "void cp_", "const char* cp_")):
+            buf += " " + s
+            if s.endswith(";"):
+                out.append(buf.strip())
+                buf = ""
+    return out
+
+
+def _chk(st, what=""):
+    if st != 0:
+        msg = lib().cp_status_string(st).decode()
+        raise CpError(f"{what}: {msg} ({st})")
+
+
+def _a(x, dt):
+    return np.ascontiguousarray(x, dtype=dt)
+
+
+def default_config(**kw):
+    c = Config()
+    lib().cp_config_default(ctypes.byref(c))
+    for k, v in kw.items():
+        if k == "gravity":
+            c.gravity = (ctypes.c_double * 3)(*v)
+        else:
+            setattr(c, k, v)
+    return c
+
+
+def reference_node_order(n):
+    out = np.zeros(n, np.int64)
+    _chk(lib().cp_reference_node_order(n, out.ctypes.data), "cp_reference_node_order")
+    return out
+
+
+def polyhedron_mass_properties(pts, faces, density):
+    pts = _a(pts, np.float64)
+    off = np.zeros(len(faces) + 1, np.int64)
+    off[1:] = np.cumsum([len(f) for f in faces])
+    fv = _a(np.concatenate([np.asarray(f) for f in faces]), np.int64)
+    out = np.zeros(13)
+    _chk(lib().cp_polyhedron_mass_properties(len(pts), pts.ctypes.data, len(faces), off.ctypes.data, fv.ctypes.data, float(density),
+                                             out.ctypes.data), "cp_polyhedron_mass_properties")
+    return out[0], out[1:10].reshape(3, 3), out[10:13]
+
+
+class World:
+    """A device-resident world (one or many scenes)."""
+
+    def __init__(self, config=None, **kw):
+        self.L = lib()
+        self.cfg = config if config is not None else default_config(**kw)
+        w = ctypes.c_void_p()
+        _chk(self.L.cp_world_create(ctypes.byref(self.cfg), ctypes.byref(w)), "cp_world_create")
+        self.w = w
+        self.n = 0
+
+    def close(self):
+        if self.w:
+            self.L.cp_world_destroy(self.w)
+            self.w = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_config(self, **kw):
+        for k, v in kw.items():
+            setattr(self.cfg, k, v)
+        _chk(self.L.cp_world_set_config(self.w, ctypes.byref(self.cfg)), "cp_world_set_config")
+
+    # ---- shapes / bodies
+    def add_hull_exact(self, h):
+        arr = [_a(h["pts"], np.float64), _a(h["face_off"], np.int64), _a(h["face_verts"], np.int64), _a(h["surface_id"], np.int64),
+               _a(h["normal_calc"], np.int64), _a(h["normal_dir"], np.int64), _a(h["gauss_normal"], np.float64),
+               _a(h["gauss_red"], np.int64), _a(h["gauss_ints"], np.int64), _a(h["gauss_vals"], np.float64), _a(h["arcs"], np.int64),
+               _a(h["edges"], np.int64)]
+        sid = ctypes.c_int()
+        _chk(self.L.cp_shape_add_hull_exact(self.w, len(arr[0]), arr[0].ctypes.data, len(arr[1]) - 1, *[a.ctypes.data for a in arr[1:10]],
+                                            len(arr[10]), arr[10].ctypes.data, len(arr[11]), arr[11].ctypes.data, ctypes.byref(sid)),
+             "cp_shape_add_hull_exact")
+        return sid.value
+
+    def add_polyhedron(self, pts, faces):
+        pts = _a(pts, np.float64)
+        off = np.zeros(len(faces) + 1, np.int64)
+        off[1:] = np.cumsum([len(f) for f in faces])
+        fv = _a(np.concatenate([np.asarray(f) for f in faces]), np.int64)
+        sid = ctypes.c_int()
+        _chk(self.L.cp_shape_add_polyhedron(self.w, len(pts), pts.ctypes.data, len(faces), off.ctypes.data, fv.ctypes.data,
+                                            ctypes.byref(sid)), "cp_shape_add_polyhedron")
+        return sid.value
+
+    def add_body(self, shape, movement, mass_props, material, scene=0):
+        mp = _a(mass_props, np.float64)
+        mt = _a(material, np.float64)
+        bi = ctypes.c_int()
+        _chk(self.L.cp_body_add(self.w, shape, movement, mp.ctypes.data, mt.ctypes.data, scene, ctypes.byref(bi)), "cp_body_add")
+        self.n += 1
+        return bi.value
+
+    # ---- state
+    def set_state(self, state, inertia=None):
+        st = _a(state, np.float64)
+        assert st.shape == (self.n, 13)
+        if inertia is not None:
+            it = _a(inertia, np.float64)
+            assert it.shape == (self.n, 18)
+            _chk(self.L.cp_world_set_state(self.w, st.ctypes.data, it.ctypes.data), "cp_world_set_state")
+        else:
+            _chk(self.L.cp_world_set_state(self.w, st.ctypes.data, None), "cp_world_set_state")
+
+    def state(self):
+        s = np.zeros((self.n, 13))
+        _chk(self.L.cp_world_get_state(self.w, s.ctypes.data), "cp_world_get_state")
+        return s
+
+    def aabbs(self):
+        a = np.zeros((self.n, 6))
+        _chk(self.L.cp_world_get_aabbs(self.w, a.ctypes.data), "cp_world_get_aabbs")
+        return a
+
+    def set_contacts(self, ints, magic, f):
+        ii, mg, ff = _a(ints, np.int64), _a(magic, np.uint64), _a(f, np.float64)
+        _chk(self.L.cp_world_set_contacts(self.w, len(ii), ii.ctypes.data, mg.ctypes.data, ff.ctypes.data), "cp_world_set_contacts")
+
+    def contacts(self):
+        n = ctypes.c_int()
+        _chk(self.L.cp_world_get_contact_count(self.w, ctypes.byref(n)), "cp_world_get_contact_count")
+        k = n.value
+        ii = np.zeros((k, 6), np.int64)
+        mg = np.zeros((k, 2), np.uint64)
+        ff = np.zeros((k, 21))
+        _chk(self.L.cp_world_get_contacts(self.w, k, ii.ctypes.data, mg.ctypes.data, ff.ctypes.data, ctypes.byref(n)),
+             "cp_world_get_contacts")
+        return ii[:n.value], mg[:n.value], ff[:n.value]
+
+    def set_node_order(self, ids):
+        ids = _a(ids, np.int64)
+        _chk(self.L.cp_world_set_node_order(self.w, len(ids), ids.ctypes.data), "cp_world_set_node_order")
+
+    def step(self, n=1):
+        _chk(self.L.cp_world_step(self.w, n), "cp_world_step")
+
+    def broadphase(self):
+        cap = max(64, self.n * 16)
+        for _ in range(4):
+            out = np.zeros((cap, 2), np.int64)
+            n = ctypes.c_int()
+            st = self.L.cp_broadphase(self.w, cap, out.ctypes.data, ctypes.byref(n))
+            if st == 4:
+                cap = n.value
+                continue
+            _chk(st, "cp_broadphase")
+            return out[:n.value]
+        raise CpError("cp_broadphase: capacity")
+
+    def narrowphase(self, pairs):
+        pairs = _a(pairs, np.int64).reshape(-1, 2)
+        n = len(pairs)
+        counts = np.zeros(n, np.int32)
+        heads = np.zeros((n, 4))
+        pts = np.zeros((n, 4, 4))
+        mg = np.zeros((n, 4, 2), np.uint64)
+        _chk(self.L.cp_narrowphase(self.w, n, pairs.ctypes.data, counts.ctypes.data, heads.ctypes.data, pts.ctypes.data, mg.ctypes.data),
+             "cp_narrowphase")
+        return counts, heads, pts, mg
+
+    def last_islands(self):
+        nr, ni = ctypes.c_int(), ctypes.c_int()
+        st = self.L.cp_world_last_islands(self.w, 0, None, 0, None, ctypes.byref(nr), ctypes.byref(ni))
+        rows = np.zeros((max(nr.value, 1), 2), np.int64)
+        offs = np.zeros(ni.value + 1, np.int64)
+        _chk(self.L.cp_world_last_islands(self.w, nr.value, rows.ctypes.data, ni.value + 1, offs.ctypes.data, ctypes.byref(nr),
+                                          ctypes.byref(ni)), "cp_world_last_islands")
+        return rows[:nr.value], offs
+
+    def last_timing(self):
+        ms = np.zeros(8)
+        _chk(self.L.cp_world_last_timing(self.w, ms.ctypes.data), "cp_world_last_timing")
+        return ms

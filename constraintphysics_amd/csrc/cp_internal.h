@@ -1,0 +1,136 @@
+// Device-side data layout of a world (one or many independent scenes) and the
+// kernel launchers shared by the translation units of libcp_hip.so.
+//
+// Layout (all in HBM, structure-of-arrays, component-major so that a thread
+// per body/contact reads coalesced):
+//   bodies   com[3][nb] q[4][nb] vel/angv/pvel/pang[3][nb] (FP64), inertia
+//            I/Iinv world and reference [9][nb], mass, movement, shape, scene.
+//   shapes   body-frame hull: points, faces (vertex lists), Gauss map (face
+//            normals, redundancy flags, cached self-extrema), Gauss arcs,
+//            edges, per-vertex face/edge adjacency.  One shape may be shared.
+//   world    per-body world hull points and world Gauss normals, AABBs.
+//   pairs    broadphase output: u64 key lo<<32|hi (global body indices).
+//   contacts the persistent cache, sorted by (pair key, insertion order), with
+//            per-substep solver rows stored beside it.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "cp_math.h"
+
+namespace cpx {
+
+enum ShapeType { SHAPE_HULL = 0, SHAPE_SPHERE = 1, SHAPE_CYLINDER = 2 };
+
+struct Shapes {
+	int n;
+	int* type;
+	int *pt_off, *face_off, *arc_off, *edge_off;  // [n+1] into the flat arrays below
+	double* ref_pts;                              // [3*P] xyz interleaved
+	int *fv_off, *fv;                             // per global face: vertex list offsets [F+1], vertices
+	int *f_ncalc, *f_ndir, *f_sid;                // Surface normal_calc_index / direction / surface id
+	double* g_norm;                               // [3*F] body-frame Gauss normals
+	int *g_red, *g_minpid, *g_maxpid, *g_refidx;  // redundancy + cached extrema (per face)
+	double *g_min, *g_max, *g_refval;
+	int* arcs;                                    // [2*A] face-index pairs (shape-local)
+	int* edges;                                   // [2*E] vertex-index pairs (shape-local)
+	int *va_face_off, *va_faces;                  // per global vertex: adjacent faces (shape-local ids)
+	int *va_edge_off, *va_edges;                  // per global vertex: adjacent edges (shape-local ids)
+	double* radius;                               // sphere / cylinder radius
+	double* height;                               // cylinder height
+};
+
+struct Bodies {
+	int n;
+	double *com, *q, *vel, *angv, *pvel, *pang, *prev_com, *prev_q;
+	double *I, *Iinv, *Iref, *Irefinv;
+	double* mass;
+	int *movement, *shape, *scene;
+	double* mat;      // [3][n]: restitution, mu_kinetic, mu_static
+	double* aabb;     // [6][n]: min xyz, max xyz
+	int* wpt_off;     // [n+1] world point offsets
+	double* wpts;     // [3*WP] world points (xyz interleaved)
+	int* wgv_off;     // [n+1] world Gauss normal offsets
+	double* wnorm;    // [3*WG]
+	double* dv;       // [12][n] PGS velocity / pseudo-velocity deltas
+	int* color_mask;  // scratch for colouring (2 x u64 per body, as 4 ints)
+};
+
+struct Params {
+	double dt, sub_dt, gx, gy, gz, cfm, hardness, cutoff_vel;
+	double cos_tol, sin_tol;
+	double cull_c[4], cull_s[4];  // cos/sin(2*pi*i/4), host libm values
+	int vel_iters, pos_iters, substeps, contact_life;
+	int gyro_ticks;
+};
+
+// Narrowphase output, one slot per candidate pair.
+struct Manifolds {
+	int* count;       // [cap] points in the culled manifold (0 = none)
+	double* normal;   // [3][cap]
+	double* maxpen;   // [cap]
+	double* pts;      // [cap*4*4]: pos xyz, pen
+	uint64_t* magic;  // [cap*4*2]: cid, bid
+	double* mat;      // [3][cap]: restitution, mu_k, mu_s of the pair
+};
+
+// Contact cache (SoA).  Row data is rebuilt each substep for live contacts.
+struct Contacts {
+	int cap;
+	uint64_t *key, *cid, *bid, *seq;
+	double *lp1, *lp2, *ln1;  // [3][cap]
+	double *pen, *bounce, *mu;
+	int *life, *live, *sready;
+	double *imp_n, *imp_t;  // imp_t [2][cap]
+	double *fu, *fw;        // [3][cap] friction basis of the last row build
+	// rows
+	double *ja, *jb, *ita, *itb;  // normal row: [6][cap] each
+	double *inv_n, *tgt_n, *ptgt_n, *pimp_n;
+	double *fja, *fjb, *fita, *fitb;  // friction rows: [12][cap], index r*6+c
+	double *finv;                     // [4][cap]
+	double *ftgt;                     // [2][cap]
+};
+
+// A step's manifold points, compacted and sorted by pair key.
+struct NewPts {
+	uint64_t *key, *cid, *bid;
+	double *pos, *nrm;  // [3][cap]
+	double *pen, *mat;  // mat [3][cap]: restitution, mu_k, mu_s
+	int *is_new, *src;
+};
+
+#define CP_MAXC 128  // colours per scene (plus one serial bucket)
+
+// Launchers (defined in the kernel translation units).
+void launch_gravity(const Bodies& b, const Params& p, hipStream_t s);
+void launch_refresh_geometry(const Bodies& b, const Shapes& sh, int only_moved, hipStream_t s);
+void launch_update_inertia(const Bodies& b, hipStream_t s);
+void launch_integrate(const Bodies& b, const Params& p, hipStream_t s);
+void launch_bp_keys(const Bodies& b, uint64_t* keys, int* vals, hipStream_t s);
+void launch_bp_gather(const Bodies& b, const int* sorted, double* sb, hipStream_t s);
+void launch_bp_sweep(int n, const int* sorted, const double* sb, const int* scene_of_body, const int* scene_body_off, int* counts,
+                     const int* offs, uint64_t* out, int write, hipStream_t s);
+void launch_narrowphase(const Bodies& b, const Shapes& sh, const Params& p, const uint64_t* pairs, int npairs, const Manifolds& m,
+                        int cap, hipStream_t s);
+void launch_np_gather(const Manifolds& M, int mcap, const uint64_t* pairs, int npairs, const int* off, const NewPts& np, int cap,
+                      hipStream_t s);
+void launch_np_match(const NewPts& np, int nnew, int cap, const Contacts& C, int nold, const Bodies& b, const Params& P,
+                     hipStream_t s);
+void launch_old_keep(const Contacts& C, int nold, int* keep, hipStream_t s);
+void launch_new_keys(const NewPts& np, int nnew, const int* rank, uint64_t* nk, hipStream_t s);
+void launch_merge(const Contacts& S, int nold, const int* keep_scan, const uint64_t* nk, int nnewc, const NewPts& np, int nnew, int cap,
+                  const int* rank, const Contacts& D, const Bodies& b, const Params& P, uint64_t seq0, hipStream_t s);
+void launch_age(const Contacts& C, int n, hipStream_t s);
+void launch_edge_heads(const Contacts& C, int n, int* head, hipStream_t s);
+void launch_edge_build(const Contacts& C, int n, const int* head, const int* eidx, uint64_t* ekey, int* efirst, int* ecnt, int* elive,
+                       hipStream_t s);
+void launch_color(const Bodies& b, const int* scene_body_off, int nscenes, const uint64_t* ekey, const int* elive, int nedges,
+                  int* ecolor, int* order, int* color_off, int* ncolors, int* edge_range, hipStream_t s);
+void launch_build_rows(const Bodies& b, const Contacts& C, int nc, const Params& P, hipStream_t s);
+void launch_pgs_ordered(const Bodies& b, const Contacts& C, const int* rows, const int* isl_off, int nisl, const Params& P, int* applied,
+                        hipStream_t s);
+void launch_pgs_colored(const Bodies& b, const Contacts& C, const int* scene_body_off, int nscenes, const int* edge_range, const int* order,
+                        const int* color_off, const int* ncolors, const int* efirst, const int* ecnt, const Params& P, int threads,
+                        hipStream_t s);
+
+}  // namespace cpx

@@ -1,0 +1,175 @@
+// Per-body kernels: gravity, substep integration (implicit gyroscopic
+// rotation), inertia update and the end-of-step world-geometry refresh.
+// One thread per body; all arithmetic FP64 in the reference's operation order
+// (this file is compiled with -ffp-contract=off).
+//
+// Roofline: HBM-bound streaming over the body SoA.  Integrate reads and writes
+// com/q/vel/angv/pvel/pang + world and reference inertia: 2*(3+4+3+3+3+3)*8 +
+// 36*8 B = 592 B per body per substep (FP64).
+#include "cp_internal.h"
+
+namespace cpx {
+
+#define LD3(a, i, n) V3{(a)[(i)], (a)[(n) + (i)], (a)[2 * (n) + (i)]}
+#define ST3(a, i, n, v) do { (a)[(i)] = (v).x; (a)[(n) + (i)] = (v).y; (a)[2 * (n) + (i)] = (v).z; } while (0)
+
+__device__ __forceinline__ Q ldq(const double* a, int i, int n) { return Q{a[i], a[n + i], a[2 * n + i], a[3 * n + i]}; }
+__device__ __forceinline__ void stq(double* a, int i, int n, Q q) {
+	a[i] = q.r; a[n + i] = q.i; a[2 * n + i] = q.j; a[3 * n + i] = q.k;
+}
+__device__ __forceinline__ M3 ldm(const double* a, int i, int n) {
+	M3 m;
+	for (int r = 0; r < 3; r++) for (int c = 0; c < 3; c++) m.m[r][c] = a[(3 * r + c) * n + i];
+	return m;
+}
+__device__ __forceinline__ void stm(double* a, int i, int n, const M3& m) {
+	for (int r = 0; r < 3; r++) for (int c = 0; c < 3; c++) a[(3 * r + c) * n + i] = m.m[r][c];
+}
+
+// PhysicsEngine.cpp:84-86: v += g * dt once per step for awake dynamic bodies.
+__global__ void k_gravity(Bodies b, Params p) {
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= b.n || b.movement[i] != 0) return;
+	V3 v = LD3(b.vel, i, b.n);
+	v += V3{p.gx, p.gy, p.gz} * p.dt;
+	ST3(b.vel, i, b.n, v);
+}
+
+// RigidBody::rotateWhileApplyingGyroAccel (RigidBody.cpp:207-255), implicit
+// Euler on the gyroscopic term with two Newton iterations per sub-tick.
+__device__ void gyro_rotate(V3& angv, Q& q, const M3& I_world, const M3& I_ref, double dt, int n_itr) {
+	if (mag2(angv) == 0) return;
+	M3 T = I_world;
+	Q o = q;
+	V3 av = angv;
+	for (int it = 0; it < n_itr; it++) {
+		V3 w = av;
+		double ts = dt / n_itr;
+		for (int s = 0; s < 2; s++) {
+			V3 f = ts * cross(w, T * w) - T * (av - w);
+			M3 jac = T + ts * (cross_mat(w) * T - cross_mat(T * w));
+			if (fabs(det(jac)) > 0.00000000001) w -= inverse(jac) * f;
+		}
+		av = w;
+		o = q_mul(q_axis_angle(ts * mag(av), av), o);
+		if (it + 1 < n_itr) {
+			M3 r = q_rot(o), rc = q_rot(q_conj(o));
+			T = r * I_ref * rc;
+		}
+	}
+	q = o;
+	angv = av;
+}
+
+// update_positions (PhysicsEngine.cpp:390-410) followed by
+// RigidBody::updateInertiaTensor (RigidBody.cpp:270-279).
+__global__ void k_integrate(Bodies b, Params p) {
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	const int n = b.n;
+	if (i >= n || b.movement[i] != 0) return;
+	V3 com = LD3(b.com, i, n), vel = LD3(b.vel, i, n), angv = LD3(b.angv, i, n);
+	V3 pvel = LD3(b.pvel, i, n), pang = LD3(b.pang, i, n);
+	Q q = ldq(b.q, i, n);
+	ST3(b.prev_com, i, n, com);
+	stq(b.prev_q, i, n, q);
+	com += (vel + pvel) * p.sub_dt;
+	if (mag2(angv) != 0) {
+		M3 Iw = ldm(b.I, i, n), Ir = ldm(b.Iref, i, n);
+		gyro_rotate(angv, q, Iw, Ir, p.sub_dt, p.gyro_ticks);
+		q = q_mul(q_axis_angle(p.sub_dt * mag(pang), pang), q);
+	}
+	V3 z{0, 0, 0};
+	ST3(b.pvel, i, n, z);
+	ST3(b.pang, i, n, z);
+	q = q_normalize(q);
+	M3 r = q_rot(q), rc = q_rot(q_conj(q));
+	M3 Iw = r * ldm(b.Iref, i, n) * rc;
+	M3 Iwi = r * ldm(b.Irefinv, i, n) * rc;
+	ST3(b.com, i, n, com);
+	ST3(b.angv, i, n, angv);
+	stq(b.q, i, n, q);
+	stm(b.I, i, n, Iw);
+	stm(b.Iinv, i, n, Iwi);
+}
+
+// updateInertiaTensor for every body without touching the stored orientation
+// (used when a state is uploaded without its world inertia).
+__global__ void k_update_inertia(Bodies b) {
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	const int n = b.n;
+	if (i >= n) return;
+	Q q = q_normalize(ldq(b.q, i, n));
+	M3 r = q_rot(q), rc = q_rot(q_conj(q));
+	stm(b.I, i, n, r * ldm(b.Iref, i, n) * rc);
+	stm(b.Iinv, i, n, r * ldm(b.Irefinv, i, n) * rc);
+}
+
+__device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b : a; }  // std::min<double>
+__device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b : a; }  // std::max<double>
+
+// RigidBody::updateGeometry (RigidBody.cpp:281-296): world hull points and
+// Gauss normals (Polyhedron::recomputeFromReference, ConvexPrimitive.cpp:715-729)
+// and the AABB (ConvexPrimitive.cpp:660-676).  only_moved: the end-of-step
+// refresh (PhysicsEngine.cpp:424-438), which normalises the orientation first
+// and skips bodies whose pose did not change in the last substep.
+__global__ void k_refresh_geometry(Bodies b, Shapes sh, int only_moved) {
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	const int n = b.n;
+	if (i >= n) return;
+	Q q = ldq(b.q, i, n);
+	V3 com = LD3(b.com, i, n);
+	if (only_moved) {
+		if (b.movement[i] != 0) return;
+		if (!(neq(com, LD3(b.prev_com, i, n)) || q_neq(q, ldq(b.prev_q, i, n)))) return;
+		q = q_normalize(q);
+		stq(b.q, i, n, q);
+	}
+	int s = b.shape[i];
+	M3 R = q_rot(q);
+	const double inf = __builtin_inf();
+	double mn0 = inf, mn1 = inf, mn2 = inf, mx0 = -inf, mx1 = -inf, mx2 = -inf;
+	if (sh.type[s] == SHAPE_HULL) {
+		int p0 = sh.pt_off[s], np = sh.pt_off[s + 1] - p0;
+		int w0 = b.wpt_off[i];
+		for (int k = 0; k < np; k++) {
+			const double* rp = sh.ref_pts + 3 * (p0 + k);
+			V3 w = com + R * V3{rp[0], rp[1], rp[2]};
+			double* wp = b.wpts + 3 * (w0 + k);
+			wp[0] = w.x; wp[1] = w.y; wp[2] = w.z;
+			mn0 = dmin(mn0, w.x); mn1 = dmin(mn1, w.y); mn2 = dmin(mn2, w.z);
+			mx0 = dmax(mx0, w.x); mx1 = dmax(mx1, w.y); mx2 = dmax(mx2, w.z);
+		}
+		int f0 = sh.face_off[s], nf = sh.face_off[s + 1] - f0;
+		int g0 = b.wgv_off[i];
+		for (int k = 0; k < nf; k++) {
+			const double* gn = sh.g_norm + 3 * (f0 + k);
+			V3 w = R * V3{gn[0], gn[1], gn[2]};
+			double* wp = b.wnorm + 3 * (g0 + k);
+			wp[0] = w.x; wp[1] = w.y; wp[2] = w.z;
+		}
+	} else if (sh.type[s] == SHAPE_SPHERE) {
+		// Sphere::gen_AABB (ConvexPrimitive.cpp:439-443): centre = com for a single-sphere body.
+		double r = sh.radius[s];
+		mn0 = com.x - r; mn1 = com.y - r; mn2 = com.z - r;
+		mx0 = com.x + r; mx1 = com.y + r; mx2 = com.z + r;
+	}
+	b.aabb[i] = mn0; b.aabb[n + i] = mn1; b.aabb[2 * n + i] = mn2;
+	b.aabb[3 * n + i] = mx0; b.aabb[4 * n + i] = mx1; b.aabb[5 * n + i] = mx2;
+}
+
+static inline int nblk(int n, int t) { return (n + t - 1) / t; }
+
+void launch_gravity(const Bodies& b, const Params& p, hipStream_t s) {
+	if (b.n) hipLaunchKernelGGL(k_gravity, dim3(nblk(b.n, 256)), dim3(256), 0, s, b, p);
+}
+void launch_integrate(const Bodies& b, const Params& p, hipStream_t s) {
+	if (b.n) hipLaunchKernelGGL(k_integrate, dim3(nblk(b.n, 128)), dim3(128), 0, s, b, p);
+}
+void launch_update_inertia(const Bodies& b, hipStream_t s) {
+	if (b.n) hipLaunchKernelGGL(k_update_inertia, dim3(nblk(b.n, 128)), dim3(128), 0, s, b);
+}
+void launch_refresh_geometry(const Bodies& b, const Shapes& sh, int only_moved, hipStream_t s) {
+	if (b.n) hipLaunchKernelGGL(k_refresh_geometry, dim3(nblk(b.n, 128)), dim3(128), 0, s, b, sh, only_moved);
+}
+
+}  // namespace cpx

@@ -1,0 +1,303 @@
+// Contact persistence, constraint-graph edges and edge colouring.
+//
+// Persistence restates PhysicsEngine::addContact (PhysicsEngine.cpp:1062-1111)
+// and the contact life cycle (:1378-1446) over a cache kept sorted by
+// (pair key, insertion order) — the per-edge vectors of the reference laid end
+// to end.  A step's new manifold points (sorted by pair key) are matched to the
+// cache by binary search on the key and a MagicID compare; matched contacts are
+// refreshed in place, unmatched ones are merged in after the existing contacts
+// of their pair.  The merge needs no sort: each element's output position is
+// its rank in its own list plus a binary-search count in the other list.
+//
+// Removal of expired contacts (memory_life < 0 at the first substep) is
+// deferred to the next step's merge: the reference removes them before any
+// later read, so dropping entries whose life fell below -1 after the end-of-
+// step decrement yields the same cache.
+//
+// Colouring: one thread per scene greedily colours the scene's edges (body
+// pairs with at least one live contact) so that no two edges of a colour share
+// a dynamic body; the PGS kernel then sweeps colour by colour.
+//
+// Roofline: HBM-bound gathers/scatters; per contact ~240 B read + written per
+// step (cache fields), per new point ~100 B.
+#include "cp_internal.h"
+
+namespace cpx {
+
+__global__ void k_np_gather(Manifolds M, int mcap, const uint64_t* pairs, int npairs, const int* off, NewPts np, int cap) {
+	int s = blockIdx.x * blockDim.x + threadIdx.x;
+	if (s >= npairs) return;
+	int c = M.count[s];
+	int o = off[s];
+	for (int k = 0; k < c; k++) {
+		int j = o + k;
+		np.key[j] = pairs[s];
+		np.cid[j] = M.magic[(size_t)s * 8 + 2 * k];
+		np.bid[j] = M.magic[(size_t)s * 8 + 2 * k + 1];
+		for (int d = 0; d < 3; d++) {
+			np.pos[d * cap + j] = M.pts[(size_t)s * 16 + 4 * k + d];
+			np.nrm[d * cap + j] = M.normal[d * mcap + s];
+			np.mat[d * cap + j] = M.mat[d * mcap + s];
+		}
+		np.pen[j] = M.pts[(size_t)s * 16 + 4 * k + 3];
+	}
+}
+
+__device__ __forceinline__ int lower_bound_u64(const uint64_t* a, int n, uint64_t v) {
+	int lo = 0, hi = n;
+	while (lo < hi) {
+		int m = (lo + hi) >> 1;
+		if (a[m] < v) lo = m + 1;
+		else hi = m;
+	}
+	return lo;
+}
+__device__ __forceinline__ int upper_bound_u64(const uint64_t* a, int n, uint64_t v) {
+	int lo = 0, hi = n;
+	while (lo < hi) {
+		int m = (lo + hi) >> 1;
+		if (a[m] <= v) lo = m + 1;
+		else hi = m;
+	}
+	return lo;
+}
+
+// Local frames of a new point (PhysicsEngine.cpp:1067-1071).
+__device__ void local_frames(const Bodies& b, int i1, int i2, V3 p, V3 n, V3* lp1, V3* lp2, V3* ln1) {
+	const int N = b.n;
+	Q q1{b.q[i1], b.q[N + i1], b.q[2 * N + i1], b.q[3 * N + i1]};
+	Q q2{b.q[i2], b.q[N + i2], b.q[2 * N + i2], b.q[3 * N + i2]};
+	V3 c1{b.com[i1], b.com[N + i1], b.com[2 * N + i1]};
+	V3 c2{b.com[i2], b.com[N + i2], b.com[2 * N + i2]};
+	M3 m1 = q_rot(q_conj(q1)), m2 = q_rot(q_conj(q2));
+	*ln1 = m1 * n;
+	*lp1 = m1 * (p - c1);
+	*lp2 = m2 * (p - c2);
+}
+
+// Match each new point against the cache; refresh matched contacts in place.
+// Within one pair, repeated MagicIDs behave as sequential addContact calls:
+// the first occurrence owns the slot, the last occurrence's data is kept.
+__global__ void k_np_match(NewPts np, int nnew, int cap, Contacts C, int nold, Bodies b, Params P) {
+	int j = blockIdx.x * blockDim.x + threadIdx.x;
+	if (j >= nnew) return;
+	uint64_t key = np.key[j], cid = np.cid[j], bid = np.bid[j];
+	bool first = true, last = true;
+	int lastj = j;
+	for (int k = j - 1; k >= 0 && np.key[k] == key; k--)
+		if (np.cid[k] == cid && np.bid[k] == bid) first = false;
+	for (int k = j + 1; k < nnew && np.key[k] == key; k++)
+		if (np.cid[k] == cid && np.bid[k] == bid) { last = false; lastj = k; }
+	int m = -1;
+	for (int k = lower_bound_u64(C.key, nold, key); k < nold && C.key[k] == key; k++) {
+		if (C.life[k] < -1) continue;  // removed at the previous step's first substep
+		if (C.cid[k] == cid && C.bid[k] == bid) { m = k; break; }
+	}
+	np.is_new[j] = (m < 0 && first) ? 1 : 0;
+	np.src[j] = lastj;
+	if (m >= 0 && last) {
+		int i1 = (int)(key >> 32), i2 = (int)(key & 0xFFFFFFFFu);
+		V3 p{np.pos[j], np.pos[cap + j], np.pos[2 * cap + j]};
+		V3 n{np.nrm[j], np.nrm[cap + j], np.nrm[2 * cap + j]};
+		V3 lp1, lp2, ln1;
+		local_frames(b, i1, i2, p, n, &lp1, &lp2, &ln1);
+		int K = C.cap;
+		C.mu[m] = C.sready[m] ? np.mat[2 * cap + j] : np.mat[cap + j];
+		C.pen[m] = np.pen[j];
+		C.ln1[m] = ln1.x; C.ln1[K + m] = ln1.y; C.ln1[2 * K + m] = ln1.z;
+		C.lp1[m] = lp1.x; C.lp1[K + m] = lp1.y; C.lp1[2 * K + m] = lp1.z;
+		C.lp2[m] = lp2.x; C.lp2[K + m] = lp2.y; C.lp2[2 * K + m] = lp2.z;
+		C.life[m] = P.contact_life;
+		C.live[m] = 1;
+	}
+}
+
+__global__ void k_old_keep(const Contacts C, int nold, int* keep) {
+	int k = blockIdx.x * blockDim.x + threadIdx.x;
+	if (k < nold) keep[k] = C.life[k] >= -1 ? 1 : 0;
+	if (k == nold) keep[k] = 0;
+}
+
+__global__ void k_new_keys(NewPts np, int nnew, const int* rank, uint64_t* nk) {
+	int j = blockIdx.x * blockDim.x + threadIdx.x;
+	if (j < nnew && np.is_new[j]) nk[rank[j]] = np.key[j];
+}
+
+__device__ void copy_contact(const Contacts& S, int i, Contacts& D, int o) {
+	int KS = S.cap, KD = D.cap;
+	D.key[o] = S.key[i]; D.cid[o] = S.cid[i]; D.bid[o] = S.bid[i]; D.seq[o] = S.seq[i];
+	for (int d = 0; d < 3; d++) {
+		D.lp1[d * KD + o] = S.lp1[d * KS + i];
+		D.lp2[d * KD + o] = S.lp2[d * KS + i];
+		D.ln1[d * KD + o] = S.ln1[d * KS + i];
+		D.fu[d * KD + o] = S.fu[d * KS + i];
+		D.fw[d * KD + o] = S.fw[d * KS + i];
+	}
+	D.pen[o] = S.pen[i]; D.bounce[o] = S.bounce[i]; D.mu[o] = S.mu[i];
+	D.life[o] = S.life[i]; D.live[o] = S.live[i]; D.sready[o] = S.sready[i];
+	D.imp_n[o] = S.imp_n[i];
+	D.imp_t[o] = S.imp_t[i];
+	D.imp_t[KD + o] = S.imp_t[KS + i];
+}
+
+__global__ void k_merge_old(Contacts S, int nold, const int* keep_scan, const uint64_t* nk, int nnewc, Contacts D) {
+	int k = blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= nold || S.life[k] < -1) return;
+	int o = keep_scan[k] + lower_bound_u64(nk, nnewc, S.key[k]);
+	copy_contact(S, k, D, o);
+}
+
+// New contact (PhysicsEngine.cpp:1096-1110 / PersistentConstraint.h:51-56).
+__global__ void k_merge_new(NewPts np, int nnew, int cap, const int* rank, Contacts S, int nold, const int* keep_scan, Contacts D,
+                            Bodies b, Params P, uint64_t seq0) {
+	int j = blockIdx.x * blockDim.x + threadIdx.x;
+	if (j >= nnew || !np.is_new[j]) return;
+	uint64_t key = np.key[j];
+	int u = upper_bound_u64(S.key, nold, key);
+	int o = rank[j] + keep_scan[u];
+	int s = np.src[j];
+	int i1 = (int)(key >> 32), i2 = (int)(key & 0xFFFFFFFFu);
+	V3 p{np.pos[s], np.pos[cap + s], np.pos[2 * cap + s]};
+	V3 n{np.nrm[s], np.nrm[cap + s], np.nrm[2 * cap + s]};
+	V3 lp1, lp2, ln1;
+	local_frames(b, i1, i2, p, n, &lp1, &lp2, &ln1);
+	int K = D.cap;
+	D.key[o] = key; D.cid[o] = np.cid[j]; D.bid[o] = np.bid[j]; D.seq[o] = seq0 + (uint64_t)j;
+	D.lp1[o] = lp1.x; D.lp1[K + o] = lp1.y; D.lp1[2 * K + o] = lp1.z;
+	D.lp2[o] = lp2.x; D.lp2[K + o] = lp2.y; D.lp2[2 * K + o] = lp2.z;
+	D.ln1[o] = ln1.x; D.ln1[K + o] = ln1.y; D.ln1[2 * K + o] = ln1.z;
+	for (int d = 0; d < 3; d++) { D.fu[d * K + o] = 0; D.fw[d * K + o] = 0; }
+	D.pen[o] = np.pen[s];
+	D.bounce[o] = np.mat[s];
+	D.mu[o] = np.mat[cap + s];  // new contacts take the kinetic coefficient
+	D.life[o] = P.contact_life;
+	D.live[o] = 1;
+	D.sready[o] = 0;
+	D.imp_n[o] = 0;
+	D.imp_t[o] = 0;
+	D.imp_t[K + o] = 0;
+}
+
+// Last substep (PhysicsEngine.cpp:1387-1392): memory_life--, is_live = false.
+__global__ void k_age(Contacts C, int n) {
+	int k = blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= n) return;
+	C.life[k] -= 1;
+	C.live[k] = 0;
+}
+
+// ------------------------------------------------------------- edges
+__global__ void k_edge_heads(const Contacts C, int n, int* head) {
+	int k = blockIdx.x * blockDim.x + threadIdx.x;
+	if (k < n) head[k] = (k == 0 || C.key[k] != C.key[k - 1]) ? 1 : 0;
+}
+__global__ void k_edge_build(const Contacts C, int n, const int* head, const int* eidx, uint64_t* ekey, int* efirst, int* ecnt,
+                             int* elive) {
+	int k = blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= n || !head[k]) return;
+	int e = eidx[k];
+	int c = 0, live = 0;
+	for (int t = k; t < n && C.key[t] == C.key[k]; t++) { c++; live |= C.live[t]; }
+	ekey[e] = C.key[k];
+	efirst[e] = k;
+	ecnt[e] = c;
+	elive[e] = live;
+}
+
+#define MAXC CP_MAXC
+
+// One thread per scene: greedy edge colouring, then a counting sort of the
+// scene's live edges by colour.  Colour MAXC collects edges that found no free
+// colour; the solver runs that bucket on a single thread.
+__global__ void k_color(Bodies b, const int* scene_body_off, int nscenes, const uint64_t* ekey, const int* elive, int nedges,
+                        int* ecolor, int* order, int* color_off, int* ncolors, int* edge_range) {
+	int s = blockIdx.x * blockDim.x + threadIdx.x;
+	if (s >= nscenes) return;
+	int b0 = scene_body_off[s], b1 = scene_body_off[s + 1];
+	uint64_t* mask = (uint64_t*)b.color_mask;
+	for (int i = b0; i < b1; i++) { mask[2 * i] = 0; mask[2 * i + 1] = 0; }
+	int e0 = lower_bound_u64(ekey, nedges, (uint64_t)b0 << 32), e1 = lower_bound_u64(ekey, nedges, (uint64_t)b1 << 32);
+	edge_range[2 * s] = e0;
+	edge_range[2 * s + 1] = e1;
+	int cnt[MAXC + 1];
+	for (int c = 0; c <= MAXC; c++) cnt[c] = 0;
+	int maxc = -1;
+	for (int e = e0; e < e1; e++) {
+		if (!elive[e]) { ecolor[e] = -1; continue; }
+		int lo = (int)(ekey[e] >> 32), hi = (int)(ekey[e] & 0xFFFFFFFFu);
+		bool dl = b.movement[lo] == 0, dh = b.movement[hi] == 0;
+		uint64_t m0 = (dl ? mask[2 * lo] : 0) | (dh ? mask[2 * hi] : 0);
+		uint64_t m1 = (dl ? mask[2 * lo + 1] : 0) | (dh ? mask[2 * hi + 1] : 0);
+		int c;
+		if (~m0) c = __builtin_ctzll(~m0);
+		else if (~m1) c = 64 + __builtin_ctzll(~m1);
+		else c = MAXC;
+		if (c < MAXC) {
+			uint64_t bit = 1ull << (c & 63);
+			if (c < 64) {
+				if (dl) mask[2 * lo] |= bit;
+				if (dh) mask[2 * hi] |= bit;
+			} else {
+				if (dl) mask[2 * lo + 1] |= bit;
+				if (dh) mask[2 * hi + 1] |= bit;
+			}
+		}
+		ecolor[e] = c;
+		cnt[c]++;
+		if (c > maxc) maxc = c;
+	}
+	int* co = color_off + (size_t)s * (MAXC + 2);
+	int acc = 0;
+	for (int c = 0; c <= MAXC; c++) {
+		co[c] = acc;
+		acc += cnt[c];
+		cnt[c] = co[c];
+	}
+	co[MAXC + 1] = acc;
+	for (int e = e0; e < e1; e++) {
+		int c = ecolor[e];
+		if (c < 0) continue;
+		order[e0 + cnt[c]++] = e;
+	}
+	ncolors[s] = maxc + 1;
+}
+
+static inline int nblk(int n, int t) { return (n + t - 1) / t; }
+
+void launch_np_gather(const Manifolds& M, int mcap, const uint64_t* pairs, int npairs, const int* off, const NewPts& np, int cap,
+                      hipStream_t s) {
+	if (npairs) hipLaunchKernelGGL(k_np_gather, dim3(nblk(npairs, 256)), dim3(256), 0, s, M, mcap, pairs, npairs, off, np, cap);
+}
+void launch_np_match(const NewPts& np, int nnew, int cap, const Contacts& C, int nold, const Bodies& b, const Params& P,
+                     hipStream_t s) {
+	if (nnew) hipLaunchKernelGGL(k_np_match, dim3(nblk(nnew, 128)), dim3(128), 0, s, np, nnew, cap, C, nold, b, P);
+}
+void launch_old_keep(const Contacts& C, int nold, int* keep, hipStream_t s) {
+	hipLaunchKernelGGL(k_old_keep, dim3(nblk(nold + 1, 256)), dim3(256), 0, s, C, nold, keep);
+}
+void launch_new_keys(const NewPts& np, int nnew, const int* rank, uint64_t* nk, hipStream_t s) {
+	if (nnew) hipLaunchKernelGGL(k_new_keys, dim3(nblk(nnew, 256)), dim3(256), 0, s, np, nnew, rank, nk);
+}
+void launch_merge(const Contacts& S, int nold, const int* keep_scan, const uint64_t* nk, int nnewc, const NewPts& np, int nnew, int cap,
+                  const int* rank, const Contacts& D, const Bodies& b, const Params& P, uint64_t seq0, hipStream_t s) {
+	if (nold) hipLaunchKernelGGL(k_merge_old, dim3(nblk(nold, 128)), dim3(128), 0, s, S, nold, keep_scan, nk, nnewc, D);
+	if (nnew)
+		hipLaunchKernelGGL(k_merge_new, dim3(nblk(nnew, 128)), dim3(128), 0, s, np, nnew, cap, rank, S, nold, keep_scan, D, b, P, seq0);
+}
+void launch_age(const Contacts& C, int n, hipStream_t s) {
+	if (n) hipLaunchKernelGGL(k_age, dim3(nblk(n, 256)), dim3(256), 0, s, C, n);
+}
+void launch_edge_heads(const Contacts& C, int n, int* head, hipStream_t s) {
+	if (n) hipLaunchKernelGGL(k_edge_heads, dim3(nblk(n, 256)), dim3(256), 0, s, C, n, head);
+}
+void launch_edge_build(const Contacts& C, int n, const int* head, const int* eidx, uint64_t* ekey, int* efirst, int* ecnt, int* elive,
+                       hipStream_t s) {
+	if (n) hipLaunchKernelGGL(k_edge_build, dim3(nblk(n, 256)), dim3(256), 0, s, C, n, head, eidx, ekey, efirst, ecnt, elive);
+}
+void launch_color(const Bodies& b, const int* scene_body_off, int nscenes, const uint64_t* ekey, const int* elive, int nedges,
+                  int* ecolor, int* order, int* color_off, int* ncolors, int* edge_range, hipStream_t s) {
+	hipLaunchKernelGGL(k_color, dim3(nblk(nscenes, 64)), dim3(64), 0, s, b, scene_body_off, nscenes, ekey, elive, nedges, ecolor, order,
+	                   color_off, ncolors, edge_range);
+}
+
+}  // namespace cpx

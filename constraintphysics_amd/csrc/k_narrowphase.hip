@@ -1,0 +1,484 @@
+// Narrowphase: one 64-lane wavefront per candidate pair.
+//
+// Convex hull vs convex hull follows SAT_PolyPoly (CollisionDetect.cpp:717-835):
+// the candidate axes are A's non-redundant face normals (A's own extrema come
+// from the cached self-query, recentred), B's, then the Gauss-map arc pairs
+// that pass the hemisphere (:760) and mutual-crossing (:767) tests, with
+// n = (a1 x a2) x (b1 x b2).  Lanes evaluate axes in parallel; the reference's
+// sequential rule "first strict minimum wins, any separating axis ends the
+// test" is reproduced exactly by a wave ballot for separation and a
+// lexicographic (depth, axis index) minimum across lanes.
+//
+// The contact feature search, 2-D Sutherland-Hodgman clipping with feature-id
+// (MagicID) bookkeeping, lift to 3-D and cull to four points (CollisionDetect.cpp
+// :237-280, :392-620, :809-831, :2795-2830) are inherently serial and short;
+// lane 0 runs them on polygons staged in LDS.
+//
+// Roofline: per pair the algorithmic traffic is 2 poses + pair key + the
+// manifold written (~150 B), but hull vertices are re-read once per tested
+// axis, so this kernel is VALU/latency bound rather than HBM bound.
+#include "cp_internal.h"
+
+namespace cpx {
+
+#define MAXP 40
+#define WAVES_PER_BLOCK 4
+
+struct CEP {
+	double x, y;
+	uint64_t magic;
+	int src;
+	int skip;
+};
+
+struct Hull {
+	const double* wp;  // world points (xyz)
+	const double* wn;  // world Gauss normals (xyz)
+	int np, nf, na, ne;
+	const int *fv_off, *fv;  // indexed by global face id (f0 + local)
+	int f0, p0;
+	const int *ncalc, *ndir, *sid, *red, *minpid, *maxpid, *refidx;
+	const double *gmin, *gmax, *grefval;
+	const int *arcs, *edges;
+	const int *vaf_off, *vaf, *vae_off, *vae;  // indexed by global vertex id (p0 + local)
+};
+
+__device__ __forceinline__ V3 ldv(const double* a, int k) { return V3{a[3 * k], a[3 * k + 1], a[3 * k + 2]}; }
+
+__device__ Hull make_hull(const Bodies& b, const Shapes& sh, int body) {
+	Hull h;
+	int s = b.shape[body];
+	h.wp = b.wpts + 3 * b.wpt_off[body];
+	h.wn = b.wnorm + 3 * b.wgv_off[body];
+	h.p0 = sh.pt_off[s];
+	h.np = sh.pt_off[s + 1] - h.p0;
+	h.f0 = sh.face_off[s];
+	h.nf = sh.face_off[s + 1] - h.f0;
+	int a0 = sh.arc_off[s], e0 = sh.edge_off[s];
+	h.na = sh.arc_off[s + 1] - a0;
+	h.ne = sh.edge_off[s + 1] - e0;
+	h.fv_off = sh.fv_off;
+	h.fv = sh.fv;
+	h.ncalc = sh.f_ncalc + h.f0;
+	h.ndir = sh.f_ndir + h.f0;
+	h.sid = sh.f_sid + h.f0;
+	h.red = sh.g_red + h.f0;
+	h.minpid = sh.g_minpid + h.f0;
+	h.maxpid = sh.g_maxpid + h.f0;
+	h.refidx = sh.g_refidx + h.f0;
+	h.gmin = sh.g_min + h.f0;
+	h.gmax = sh.g_max + h.f0;
+	h.grefval = sh.g_refval + h.f0;
+	h.arcs = sh.arcs + 2 * a0;
+	h.edges = sh.edges + 2 * e0;
+	h.vaf_off = sh.va_face_off;
+	h.vaf = sh.va_faces;
+	h.vae_off = sh.va_edge_off;
+	h.vae = sh.va_edges;
+	return h;
+}
+
+struct Ext {
+	int min_pid, max_pid;
+	double min_val, max_val;
+};
+
+__device__ Ext find_extrema(const Hull& h, V3 axis) {  // CollisionDetect.cpp:627-644
+	Ext e{-1, -1, __builtin_inf(), -__builtin_inf()};
+	for (int i = 0; i < h.np; i++) {
+		double v = dot(ldv(h.wp, i), axis);
+		if (v < e.min_val) { e.min_pid = i; e.min_val = v; }
+		if (v > e.max_val) { e.max_pid = i; e.max_val = v; }
+	}
+	return e;
+}
+
+__device__ Ext recentered(const Hull& h, int g) {  // CollisionDetect.cpp:622-625 and :726
+	double nv = dot(ldv(h.wn, g), ldv(h.wp, h.refidx[g]));
+	double diff = nv - h.grefval[g];
+	return Ext{h.minpid[g], h.maxpid[g], h.gmin[g] + diff, h.gmax[g] + diff};
+}
+
+struct Check {
+	int a_pid, b_pid;
+	V3 n;
+	double pen;
+};
+__device__ __forceinline__ Check sat_check(const Ext& a, const Ext& b, V3 n) {  // CollisionDetect.cpp:705-715
+	double f = a.max_val - b.min_val, r = b.max_val - a.min_val;
+	if (f < r) return Check{a.max_pid, b.min_pid, n, f};
+	return Check{a.min_pid, b.max_pid, -n, r};
+}
+
+// Axis t of the pair's sequential candidate order; valid=false when the arc
+// pair is pruned.  Mirrors the three loops of SAT_PolyPoly.
+__device__ Check eval_axis(const Hull& A, const Hull& B, int t, bool* valid) {
+	*valid = true;
+	if (t < A.nf) {
+		if (A.red[t]) { *valid = false; return Check{}; }
+		V3 n = ldv(A.wn, t);
+		return sat_check(recentered(A, t), find_extrema(B, n), n);
+	}
+	t -= A.nf;
+	if (t < B.nf) {
+		if (B.red[t]) { *valid = false; return Check{}; }
+		V3 n = ldv(B.wn, t);
+		return sat_check(find_extrema(A, n), recentered(B, t), n);
+	}
+	t -= B.nf;
+	int i1 = t / B.na, i2 = t - i1 * B.na;
+	V3 a1 = ldv(A.wn, A.arcs[2 * i1]), a2 = ldv(A.wn, A.arcs[2 * i1 + 1]);
+	V3 b1 = -ldv(B.wn, B.arcs[2 * i2]), b2 = -ldv(B.wn, B.arcs[2 * i2 + 1]);
+	V3 aa = a1 + a2;
+	if (dot(aa, b1) + dot(aa, b2) <= 0) { *valid = false; return Check{}; }
+	V3 ap = cross(a1, a2), bp = cross(b1, b2);
+	if (dot(ap, b1) * dot(ap, b2) > 0 || dot(bp, a1) * dot(bp, a2) > 0) { *valid = false; return Check{}; }
+	V3 n = cross(ap, bp);
+	if (mag2(n) == 0) { *valid = false; return Check{}; }
+	n = normalize(n);
+	if (dot(aa, n) < 0) n = n * -1.0;
+	return sat_check(find_extrema(A, n), find_extrema(B, n), n);
+}
+
+// ------------------------------------------------------------- contact area
+__device__ V3 face_normal(const Hull& h, int f) {  // Surface::normal, ConvexPrimitive.cpp:815-821
+	int o = h.fv_off[h.f0 + f];
+	V3 p1 = ldv(h.wp, h.fv[o]), p2 = ldv(h.wp, h.fv[o + 1]), p3 = ldv(h.wp, h.fv[o + h.ncalc[f]]);
+	return normalize(cross(p2 - p1, p3 - p2)) * (double)h.ndir[f];
+}
+
+// findContactArea (CollisionDetect.cpp:237-280): fills out[].x/y/src, returns count; *surface = id or -1.
+__device__ int contact_area(const Hull& h, V3 n, V3 p, int pid, V3 u, V3 w, double cos_tol, double sin_tol, CEP* out, int* surface) {
+	int gv = h.p0 + pid;
+	int best_f = -1;
+	double best_ft = 1.0;
+	for (int k = h.vaf_off[gv]; k < h.vaf_off[gv + 1]; k++) {
+		int f = h.vaf[k];
+		double c = dot(face_normal(h, f), n);
+		if (1 - c < best_ft) { best_ft = 1 - c; best_f = f; }
+	}
+	if (best_ft <= cos_tol) {
+		int o = h.fv_off[h.f0 + best_f], e = h.fv_off[h.f0 + best_f + 1];
+		int cnt = 0;
+		for (int k = o; k < e && cnt < MAXP; k++, cnt++) {
+			int vi = h.fv[k];
+			V3 v = ldv(h.wp, vi);
+			out[cnt].x = dot(v, u);
+			out[cnt].y = dot(v, w);
+			out[cnt].src = vi;
+		}
+		*surface = h.sid[best_f];
+		return cnt;
+	}
+	int best_e = -1;
+	double best_et = 1.0;
+	for (int k = h.vae_off[gv]; k < h.vae_off[gv + 1]; k++) {
+		int e = h.vae[k];
+		V3 e1 = ldv(h.wp, h.edges[2 * e]), e2 = ldv(h.wp, h.edges[2 * e + 1]);
+		double s = fabs(dot(normalize(e2 - e1), n));
+		if (s < best_et) { best_et = s; best_e = e; }
+	}
+	*surface = -1;
+	if (best_et <= sin_tol) {
+		int i1 = h.edges[2 * best_e], i2 = h.edges[2 * best_e + 1];
+		V3 e1 = ldv(h.wp, i1), e2 = ldv(h.wp, i2);
+		out[0].x = dot(e1, u); out[0].y = dot(e1, w); out[0].src = i1;
+		out[1].x = dot(e2, u); out[1].y = dot(e2, w); out[1].src = i2;
+		return 2;
+	}
+	out[0].x = dot(p, u);
+	out[0].y = dot(p, w);
+	out[0].src = pid;
+	return 1;
+}
+
+// ------------------------------------------------------------- clipping
+__device__ __forceinline__ uint32_t edge_id(uint16_t a, uint16_t b) {  // CollisionDetect.cpp:9-21
+	int mn, mx;
+	if (a < b) { mn = a; mx = b; } else { mn = b; mx = a; }
+	return (0x0000FFFFu & (uint32_t)mn) + (0xFFFF0000u & ((uint32_t)mx << 16));
+}
+__device__ __forceinline__ double dot2(double ax, double ay, double bx, double by) {
+	double s = 0;
+	s += ax * bx;
+	s += ay * by;
+	return s;
+}
+__device__ __forceinline__ void in_dir(double p1x, double p1y, double p2x, double p2y, double* ox, double* oy) {  // :392-396
+	double dx = p1y - p2y, dy = p2x - p1x;
+	double s = 0;
+	s += dx * dx;
+	s += dy * dy;
+	double k = 1.0 / sqrt(s);
+	*ox = dx * k;
+	*oy = dy * k;
+}
+__device__ bool winding_ccw(const CEP* c, int n) {  // :398-409
+	double ix, iy;
+	in_dir(c[0].x, c[0].y, c[1].x, c[1].y, &ix, &iy);
+	for (int i = 2; i < n; i++) {
+		double d = dot2(c[i].x - c[0].x, c[i].y - c[0].y, ix, iy);
+		if (d > 0.0000000001) return true;
+		else if (d < -0.0000000001) return false;
+	}
+	return false;
+}
+// createClipEvaluationPoly (:423-442), in place.
+__device__ void clip_poly(CEP* c, int n, uint32_t other_sid, bool flip) {
+	for (int i = 0; i < n; i++) {
+		uint64_t m = 0;
+		if (flip) {
+			m |= 0x00000000FFFFFFFFull & (uint64_t)other_sid;
+			m |= 0xFFFFFFFF00000000ull & ((uint64_t)(int64_t)c[i].src << 32);
+		} else {
+			m |= 0x00000000FFFFFFFFull & (uint64_t)(int64_t)c[i].src;
+			m |= 0xFFFFFFFF00000000ull & ((uint64_t)other_sid << 32);
+		}
+		c[i].magic = m;
+		c[i].skip = 0;
+	}
+	if (n > 2 && !winding_ccw(c, n)) {
+		for (int i = 0; i < n / 2; i++) {
+			CEP t = c[i];
+			c[i] = c[n - 1 - i];
+			c[n - 1 - i] = t;
+		}
+	}
+}
+// getEdgeIntersectionWithClippingEdge (:444-467).
+__device__ CEP intersect(const CEP& e1, const CEP& e2, double cnx, double cny, double spx, double spy, int32_t ceid, bool flip) {
+	double dx = e2.x - e1.x, dy = e2.y - e1.y;
+	double t = dot2(cnx, cny, spx - e1.x, spy - e1.y) / dot2(cnx, cny, dx, dy);
+	CEP o;
+	o.x = e1.x + dx * t;
+	o.y = e1.y + dy * t;
+	bool e2p = dot2(dx, dy, cnx, cny) > 0;
+	o.src = e2p ? e1.src : e2.src;
+	uint64_t m = 0;
+	uint32_t eid = edge_id((uint16_t)e1.src, (uint16_t)e2.src);
+	if (flip) {
+		m |= 0x00000000FFFFFFFFull & (uint64_t)(int64_t)ceid;
+		m |= 0xFFFFFFFF00000000ull & ((uint64_t)eid << 32);
+	} else {
+		m |= 0x00000000FFFFFFFFull & (uint64_t)eid;
+		m |= 0xFFFFFFFF00000000ull & ((uint64_t)(int64_t)ceid << 32);
+	}
+	o.magic = m;
+	o.skip = !e2p;
+	return o;
+}
+enum { BOTH, P1ON_P2C, P2ON_P1C, NEITHER, ONLY_P1, ONLY_P2 };
+__device__ __forceinline__ int check_seg(double a, double b, double c) {  // :473-480
+	if (a >= c && b >= c) return NEITHER;
+	if (a == c && b < c) return P1ON_P2C;
+	if (a < c && b == c) return P2ON_P1C;
+	if (a < c && b >= c) return ONLY_P1;
+	if (a >= c && b < c) return ONLY_P2;
+	return BOTH;
+}
+// getClipEvaluatinPolyAfterClippingByEdge (:482-562): src[0..n) -> dst, returns count.
+__device__ int clip_by_edge(const CEP* c, int n, double sx, double sy, double spx, double spy, int32_t eid, bool flip, CEP* out) {
+	double cv = dot2(spx, spy, sx, sy);
+	int k = 0;
+	if (n >= 3) {
+		double p1v = 0, p2v = 0;
+		CEP p1, p2;
+		for (int i = 0; i < n; i++) {
+			if (i > 0) { p1v = p2v; p1 = p2; }
+			else { p1 = c[0]; p1v = dot2(p1.x, p1.y, sx, sy); }
+			p2 = c[(i + 1) % n];
+			p2v = dot2(p2.x, p2.y, sx, sy);
+			int r = check_seg(p1v, p2v, cv);
+			if (r == NEITHER || r == P1ON_P2C) {
+				if (k < MAXP) out[k++] = p1;
+			} else if (r == ONLY_P2) {
+				if (k < MAXP) out[k++] = p1;
+				if (k < MAXP) out[k++] = intersect(p1, p2, sx, sy, spx, spy, eid, flip);
+			} else if (r == ONLY_P1) {
+				if (k < MAXP) out[k++] = intersect(p1, p2, sx, sy, spx, spy, eid, flip);
+			}
+		}
+		return k;
+	} else if (n == 2) {
+		double a = dot2(c[0].x, c[0].y, sx, sy), b = dot2(c[1].x, c[1].y, sx, sy);
+		int r = check_seg(a, b, cv);
+		if (r == NEITHER) { out[0] = c[0]; out[1] = c[1]; return 2; }
+		if (r == P1ON_P2C) { out[0] = c[0]; return 1; }
+		if (r == P2ON_P1C) { out[0] = c[1]; return 1; }
+		if (r == ONLY_P1) { CEP t = intersect(c[0], c[1], sx, sy, spx, spy, eid, flip); out[1] = c[1]; out[0] = t; return 2; }
+		if (r == ONLY_P2) { CEP t = intersect(c[0], c[1], sx, sy, spx, spy, eid, flip); out[0] = c[0]; out[1] = t; return 2; }
+		return 0;
+	} else if (n == 1) {
+		if (dot2(c[0].x, c[0].y, sx, sy) >= cv) { out[0] = c[0]; return 1; }
+		return 0;
+	}
+	return 0;
+}
+// clipC1ByAllEdgesOfC2 (:564-576).  c1 lives in bufA (scratch bufB); returns
+// the buffer holding the result and its count.
+__device__ CEP* clip_all(CEP* bufA, CEP* bufB, int n1, const CEP* c2, int n2, bool flip, int* nout) {
+	CEP* cur = bufA;
+	CEP* nxt = bufB;
+	for (int i = 0; i < n2; i++) {
+		CEP e1 = c2[i];
+		if (e1.skip) continue;
+		CEP e2 = c2[(i + 1) % n2];
+		double dx, dy;
+		in_dir(e1.x, e1.y, e2.x, e2.y, &dx, &dy);
+		int32_t eid = (int32_t)edge_id((uint16_t)e1.src, (uint16_t)e2.src);
+		n1 = clip_by_edge(cur, n1, dx, dy, e1.x, e1.y, eid, flip, nxt);
+		CEP* t = cur;
+		cur = nxt;
+		nxt = t;
+		if (n1 == 0) break;
+	}
+	*nout = n1;
+	return cur;
+}
+
+// Serial contact generation for a hull pair after SAT (lane 0).
+__device__ void hull_contacts(const Hull& A, const Hull& B, const Check& mp, const Params& P, uint64_t cid, CEP* lds, int slot,
+                              Manifolds& M) {
+	CEP* b0 = lds;
+	CEP* b1 = lds + MAXP;
+	CEP* b2 = lds + 2 * MAXP;
+	CEP* b3 = lds + 3 * MAXP;
+	V3 n = mp.n;
+	V3 amax = ldv(A.wp, mp.a_pid), bmax = ldv(B.wp, mp.b_pid);
+	V3 u, w;
+	perp_basis(n, &u, &w);
+	int sa, sb;
+	int na = contact_area(A, n, amax, mp.a_pid, u, w, P.cos_tol, P.sin_tol, b0, &sa);
+	int nb = contact_area(B, n * -1.0, bmax, mp.b_pid, u, w, P.cos_tol, P.sin_tol, b1, &sb);
+	clip_poly(b0, na, (uint32_t)sb, false);
+	clip_poly(b1, nb, (uint32_t)sa, true);
+	CEP* res;
+	int nr;
+	if (na == 1) { res = b0; nr = 1; }
+	else if (nb == 1) { res = b1; nr = 1; }
+	else if (na == 2 && nb == 2) {
+		double dx, dy;
+		in_dir(b1[0].x, b1[0].y, b1[1].x, b1[1].y, &dx, &dy);
+		b2[0] = intersect(b0[0], b0[1], dx, dy, b1[0].x, b1[0].y, (int32_t)edge_id((uint16_t)b1[0].src, (uint16_t)b1[1].src), false);
+		res = b2;
+		nr = 1;
+	} else if (na == 2) {
+		res = clip_all(b0, b2, na, b1, nb, false, &nr);
+	} else if (nb == 2) {
+		res = clip_all(b1, b2, nb, b0, na, true, &nr);
+	} else {
+		// poly v poly: clip A by B's edges, then B by the clipped A's edges
+		// (an empty clipped A leaves B unclipped, as in the reference).
+		int n1;
+		CEP* r1 = clip_all(b0, b2, na, b1, nb, false, &n1);
+		CEP* scratch = (r1 == b0) ? b2 : b0;
+		res = clip_all(b1, scratch, nb, r1, n1, true, &nr);
+	}
+	double a_dot = dot(amax, n);
+	V3 noff = n * a_dot;
+	double* mpts = M.pts + (size_t)slot * 16;
+	uint64_t* mmag = M.magic + (size_t)slot * 8;
+	if (nr <= 4) {
+		for (int k = 0; k < nr; k++) {
+			V3 pos = u * res[k].x + w * res[k].y + noff;
+			mpts[4 * k] = pos.x; mpts[4 * k + 1] = pos.y; mpts[4 * k + 2] = pos.z;
+			mpts[4 * k + 3] = dot(pos, n) - a_dot + mp.pen;
+			mmag[2 * k] = cid;
+			mmag[2 * k + 1] = res[k].magic;
+		}
+		M.count[slot] = nr;
+		M.maxpen[slot] = mp.pen;
+	} else {
+		// cull_manifold (:2795-2830) with the reference's own cos/sin of 2*pi*i/4.
+		uint64_t avail = (nr >= 64) ? ~0ull : ((1ull << nr) - 1);
+		V3 cu, cw;
+		perp_basis(n, &cu, &cw);
+		double maxpen = 0;
+		for (int i = 0; i < 4; i++) {
+			V3 t = cu * P.cull_c[i] + cw * P.cull_s[i];
+			double bv = -__builtin_inf();
+			int bi = -1;
+			for (int j = 0; j < nr; j++) {
+				if (!((avail >> j) & 1)) continue;
+				V3 pos = u * res[j].x + w * res[j].y + noff;
+				double v = dot(pos, t);
+				if (v > bv) { bv = v; bi = j; }
+			}
+			if (bi < 0) bi = 0;
+			V3 pos = u * res[bi].x + w * res[bi].y + noff;
+			double pen = dot(pos, n) - a_dot + mp.pen;
+			mpts[4 * i] = pos.x; mpts[4 * i + 1] = pos.y; mpts[4 * i + 2] = pos.z; mpts[4 * i + 3] = pen;
+			mmag[2 * i] = cid;
+			mmag[2 * i + 1] = res[bi].magic;
+			maxpen = (maxpen < pen) ? pen : maxpen;
+			avail &= ~(1ull << bi);
+		}
+		M.count[slot] = 4;
+		M.maxpen[slot] = maxpen;
+	}
+}
+
+// Grid-stride over pair slots, one wave per pair.
+__global__ __launch_bounds__(64 * WAVES_PER_BLOCK) void k_narrowphase(Bodies b, Shapes sh, Params P, const uint64_t* pairs, int npairs,
+                                                                      Manifolds M, int cap) {
+	__shared__ CEP lds[WAVES_PER_BLOCK][4 * MAXP];
+	const int lane = threadIdx.x & 63;
+	const int wib = threadIdx.x >> 6;
+	const int nwaves = gridDim.x * WAVES_PER_BLOCK;
+	for (int slot = blockIdx.x * WAVES_PER_BLOCK + wib; slot < npairs; slot += nwaves) {
+		uint64_t key = pairs[slot];
+		int ia = (int)(key >> 32), ib = (int)(key & 0xFFFFFFFFu);
+		if (lane == 0) {
+			M.count[slot] = 0;
+			M.maxpen[slot] = -1;
+		}
+		// PhysicsEngine.cpp:174: at least one dynamic body
+		if (b.movement[ia] != 0 && b.movement[ib] != 0) continue;
+		int sa = b.shape[ia], sbb = b.shape[ib];
+		if (sh.type[sa] != SHAPE_HULL || sh.type[sbb] != SHAPE_HULL) continue;
+		Hull A = make_hull(b, sh, ia), B = make_hull(b, sh, ib);
+		int naxes = A.nf + B.nf + A.na * B.na;
+		double bp = __builtin_inf();
+		int bi = 0x7FFFFFFF;
+		int sep = 0;
+		for (int t = lane; t < naxes; t += 64) {
+			bool valid;
+			Check x = eval_axis(A, B, t, &valid);
+			if (!valid) continue;
+			if (x.pen < 0) sep = 1;
+			if (x.pen < bp) { bp = x.pen; bi = t; }
+		}
+		if (__ballot(sep) != 0) continue;
+		for (int off = 32; off >= 1; off >>= 1) {
+			double obp = __shfl_xor(bp, off);
+			int obi = __shfl_xor(bi, off);
+			if (obp < bp || (obp == bp && obi < bi)) { bp = obp; bi = obi; }
+		}
+		if (!(bp > 0) || bi == 0x7FFFFFFF) continue;  // PhysicsEngine.cpp:213: keep only max_pen_depth > 0
+		if (lane == 0) {
+			bool valid;
+			Check mp = eval_axis(A, B, bi, &valid);
+			uint64_t ka = (uint64_t)(ia + 1) * 16, kb = (uint64_t)(ib + 1) * 16;
+			uint64_t cid = (0x00000000FFFFFFFFull & ka) | (0xFFFFFFFF00000000ull & (kb << 32));
+			// material mixing (CollisionDetect.cpp:820-824)
+			double ra = b.mat[ia], rb = b.mat[ib];
+			M.mat[slot] = (ra < rb) ? rb : ra;
+			M.mat[cap + slot] = (b.mat[b.n + ia] + b.mat[b.n + ib]) / 2.0;
+			M.mat[2 * cap + slot] = (b.mat[2 * b.n + ia] + b.mat[2 * b.n + ib]) / 2.0;
+			M.normal[slot] = mp.n.x;
+			M.normal[cap + slot] = mp.n.y;
+			M.normal[2 * cap + slot] = mp.n.z;
+			hull_contacts(A, B, mp, P, cid, lds[wib], slot, M);
+		}
+	}
+}
+
+void launch_narrowphase(const Bodies& b, const Shapes& sh, const Params& p, const uint64_t* pairs, int npairs, const Manifolds& m,
+                        int cap, hipStream_t s) {
+	if (npairs <= 0) return;
+	int blocks = (npairs + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
+	if (blocks > 8192) blocks = 8192;
+	hipLaunchKernelGGL(k_narrowphase, dim3(blocks), dim3(64 * WAVES_PER_BLOCK), 0, s, b, sh, p, pairs, npairs, m, cap);
+}
+
+}  // namespace cpx

@@ -1,0 +1,1016 @@
+// Host runtime behind the C ABI (include/cp_abi.h): owns every device buffer
+// of a world, uploads scenes, and issues the per-step launch sequence that
+// replaces PhysicsEngine::timeStep (PhysicsEngine.cpp:50-471):
+//
+//   gravity -> broadphase (segmented radix sort by AABB min.x, sweep, pair sort)
+//   -> narrowphase (wave per pair) -> point compaction -> contact persistence
+//   (match + merge) -> constraint-graph edges -> schedule (edge colouring, or the
+//   reference's island order built on the host) -> substeps x (row build, PGS,
+//   integrate) -> contact ageing -> world-geometry refresh.
+//
+// Sizes that drive later launches (pair, point and contact counts) are read
+// back with one small synchronous copy each; everything else stays on the
+// device.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/cp_abi.h"
+#include "cp_internal.h"
+#include "shapes.h"
+
+using namespace cpx;
+
+#define HIPCHK(x)                                  \
+	do {                                           \
+		hipError_t e_ = (x);                       \
+		if (e_ != hipSuccess) throw DevErr{e_, #x}; \
+	} while (0)
+
+namespace {
+
+struct DevErr {
+	hipError_t e;
+	const char* what;
+};
+struct ApiErr {
+	int code;
+};
+
+// Grow-only device array.
+template <class T>
+struct DArr {
+	T* p = nullptr;
+	size_t cap = 0;
+	void reserve(size_t n) {
+		if (n <= cap && p) return;
+		size_t m = std::max<size_t>(n, 16);
+		if (p) HIPCHK(hipFree(p));
+		p = nullptr;
+		if (hipMalloc(&p, m * sizeof(T)) != hipSuccess) {
+			p = nullptr;
+			cap = 0;
+			throw ApiErr{CP_ERR_OOM};
+		}
+		cap = m;
+	}
+	void free_() {
+		if (p) (void)hipFree(p);
+		p = nullptr;
+		cap = 0;
+	}
+};
+
+struct ContactBuf {
+	int cap = 0;
+	std::vector<void*> allocs;
+	Contacts c{};
+	void release() {
+		for (void* a : allocs) (void)hipFree(a);
+		allocs.clear();
+		cap = 0;
+		c = Contacts{};
+	}
+	template <class T>
+	T* al(size_t n) {
+		T* p = nullptr;
+		if (hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) throw ApiErr{CP_ERR_OOM};
+		allocs.push_back(p);
+		return p;
+	}
+	void reserve(int n) {
+		if (n <= cap) return;
+		release();
+		int K = std::max(n + n / 2, 256);
+		cap = K;
+		c.cap = K;
+		c.key = al<uint64_t>(K); c.cid = al<uint64_t>(K); c.bid = al<uint64_t>(K); c.seq = al<uint64_t>(K);
+		c.lp1 = al<double>(3 * K); c.lp2 = al<double>(3 * K); c.ln1 = al<double>(3 * K);
+		c.pen = al<double>(K); c.bounce = al<double>(K); c.mu = al<double>(K);
+		c.life = al<int>(K); c.live = al<int>(K); c.sready = al<int>(K);
+		c.imp_n = al<double>(K); c.imp_t = al<double>(2 * K);
+		c.fu = al<double>(3 * K); c.fw = al<double>(3 * K);
+		c.ja = al<double>(6 * K); c.jb = al<double>(6 * K); c.ita = al<double>(6 * K); c.itb = al<double>(6 * K);
+		c.inv_n = al<double>(K); c.tgt_n = al<double>(K); c.ptgt_n = al<double>(K); c.pimp_n = al<double>(K);
+		c.fja = al<double>(12 * K); c.fjb = al<double>(12 * K); c.fita = al<double>(12 * K); c.fitb = al<double>(12 * K);
+		c.finv = al<double>(4 * K); c.ftgt = al<double>(2 * K);
+	}
+};
+
+template <class T>
+void up(DArr<T>& d, const std::vector<T>& h) {
+	d.reserve(h.size());
+	if (!h.empty()) HIPCHK(hipMemcpy(d.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+}
+template <class T>
+void down(std::vector<T>& h, const T* d, size_t n) {
+	h.resize(n);
+	if (n) HIPCHK(hipMemcpy(h.data(), d, n * sizeof(T), hipMemcpyDeviceToHost));
+}
+template <class T>
+T read1(const T* d) {
+	T v;
+	HIPCHK(hipMemcpy(&v, d, sizeof(T), hipMemcpyDeviceToHost));
+	return v;
+}
+
+}  // namespace
+
+struct cp_world {
+	cp_config cfg;
+	hipStream_t stream = nullptr;
+	int device = 0;
+	// host copies of static data
+	HostShapes hs;
+	std::vector<int> b_shape, b_move, b_scene;
+	std::vector<double> b_massprops, b_mat;
+	std::vector<int64_t> node_order;
+	// device: shapes
+	DArr<int> s_type, s_pt_off, s_face_off, s_arc_off, s_edge_off, s_fv_off, s_fv, s_ncalc, s_ndir, s_sid, s_red, s_minpid, s_maxpid,
+	    s_refidx, s_arcs, s_edges, s_vaf_off, s_vaf, s_vae_off, s_vae;
+	DArr<double> s_ref_pts, s_gnorm, s_gmin, s_gmax, s_grefval, s_radius, s_height;
+	// device: bodies
+	DArr<double> com, q, vel, angv, pvel, pang, prev_com, prev_q, I, Iinv, Iref, Irefinv, mass, mat, aabb, wpts, wnorm, dv;
+	DArr<int> movement, shape, scene, wpt_off, wgv_off, color_mask, scene_body_off, scene_of_body;
+	int nb = 0, nscenes = 0;
+	bool uploaded = false;
+	// step temporaries
+	DArr<uint64_t> bp_keys, bp_keys2, pairs, pairs2, nk;
+	DArr<int> bp_vals, bp_vals2, bp_counts, bp_offs;
+	DArr<double> bp_sb;
+	DArr<char> tmp;
+	int mcap = 0;
+	Manifolds man{};
+	std::vector<void*> man_allocs;
+	DArr<int> pt_cnt, pt_off;
+	int npcap = 0;
+	NewPts np{};
+	std::vector<void*> np_allocs;
+	DArr<int> keep, keep_scan, rank;
+	ContactBuf cb[2];
+	int cur = 0;
+	int ncontacts = 0;
+	uint64_t seq = 1ull << 32;
+	DArr<int> e_head, e_idx, e_first, e_cnt, e_live, e_color, e_order, color_off, ncolors, edge_range, applied, isl_rows, isl_off;
+	DArr<uint64_t> e_key;
+	int nedges = 0;
+	std::vector<int64_t> last_rows, last_offs;
+	hipEvent_t ev[10];
+	double last_ms[8] = {0};
+	bool timing = true;
+
+	Bodies bodies() {
+		Bodies b{};
+		b.n = nb;
+		b.com = com.p; b.q = q.p; b.vel = vel.p; b.angv = angv.p; b.pvel = pvel.p; b.pang = pang.p;
+		b.prev_com = prev_com.p; b.prev_q = prev_q.p;
+		b.I = I.p; b.Iinv = Iinv.p; b.Iref = Iref.p; b.Irefinv = Irefinv.p;
+		b.mass = mass.p;
+		b.movement = movement.p; b.shape = shape.p; b.scene = scene.p;
+		b.mat = mat.p; b.aabb = aabb.p;
+		b.wpt_off = wpt_off.p; b.wpts = wpts.p; b.wgv_off = wgv_off.p; b.wnorm = wnorm.p;
+		b.dv = dv.p; b.color_mask = color_mask.p;
+		return b;
+	}
+	Shapes shapes() {
+		Shapes s{};
+		s.n = (int)hs.type.size();
+		s.type = s_type.p; s.pt_off = s_pt_off.p; s.face_off = s_face_off.p; s.arc_off = s_arc_off.p; s.edge_off = s_edge_off.p;
+		s.ref_pts = s_ref_pts.p; s.fv_off = s_fv_off.p; s.fv = s_fv.p;
+		s.f_ncalc = s_ncalc.p; s.f_ndir = s_ndir.p; s.f_sid = s_sid.p; s.g_norm = s_gnorm.p;
+		s.g_red = s_red.p; s.g_minpid = s_minpid.p; s.g_maxpid = s_maxpid.p; s.g_refidx = s_refidx.p;
+		s.g_min = s_gmin.p; s.g_max = s_gmax.p; s.g_refval = s_grefval.p;
+		s.arcs = s_arcs.p; s.edges = s_edges.p;
+		s.va_face_off = s_vaf_off.p; s.va_faces = s_vaf.p; s.va_edge_off = s_vae_off.p; s.va_edges = s_vae.p;
+		s.radius = s_radius.p; s.height = s_height.p;
+		return s;
+	}
+	Params params() {
+		Params p{};
+		p.dt = cfg.dt;
+		p.sub_dt = cfg.dt / cfg.substeps;
+		p.gx = cfg.gravity[0]; p.gy = cfg.gravity[1]; p.gz = cfg.gravity[2];
+		p.cfm = cfg.cfm;
+		p.hardness = cfg.contact_hardness;
+		double gm = std::sqrt(p.gx * p.gx + p.gy * p.gy + p.gz * p.gz);
+		p.cutoff_vel = 2 * gm * cfg.dt;  // PhysicsEngine.h:193
+		const double ref_pi = 3.14159265358979323846;  // CollisionDetect.h:20
+		double tol = 1 * ref_pi / 180;
+		p.cos_tol = 1 - std::cos(tol);
+		p.sin_tol = std::sin(tol);
+		for (int i = 0; i < 4; i++) {
+			p.cull_c[i] = std::cos(2 * ref_pi * i / 4);
+			p.cull_s[i] = std::sin(2 * ref_pi * i / 4);
+		}
+		p.vel_iters = cfg.vel_iters;
+		p.pos_iters = cfg.pos_iters;
+		p.substeps = cfg.substeps;
+		p.contact_life = cfg.contact_life;
+		p.gyro_ticks = cfg.gyro_ticks;
+		return p;
+	}
+
+	void free_all();
+	void upload();
+	void ensure_manifolds(int n);
+	void ensure_points(int n);
+	void scan(const int* in, int* out, int n);
+	int broadphase();
+	int narrowphase(int npairs);
+	void persist(int npairs);
+	void edges();
+	void schedule_reference_order();
+	void step();
+};
+
+void cp_world::free_all() {
+	DArr<int>* ints[] = {&s_type, &s_pt_off, &s_face_off, &s_arc_off, &s_edge_off, &s_fv_off, &s_fv, &s_ncalc, &s_ndir, &s_sid, &s_red,
+	                     &s_minpid, &s_maxpid, &s_refidx, &s_arcs, &s_edges, &s_vaf_off, &s_vaf, &s_vae_off, &s_vae, &movement, &shape,
+	                     &scene, &wpt_off, &wgv_off, &color_mask, &scene_body_off, &scene_of_body, &bp_vals, &bp_vals2, &bp_counts,
+	                     &bp_offs, &pt_cnt, &pt_off, &keep, &keep_scan, &rank, &e_head, &e_idx, &e_first, &e_cnt, &e_live, &e_color,
+	                     &e_order, &color_off, &ncolors, &edge_range, &applied, &isl_rows, &isl_off};
+	for (auto* a : ints) a->free_();
+	DArr<double>* dbl[] = {&s_ref_pts, &s_gnorm, &s_gmin, &s_gmax, &s_grefval, &s_radius, &s_height, &com, &q, &vel, &angv, &pvel, &pang,
+	                       &prev_com, &prev_q, &I, &Iinv, &Iref, &Irefinv, &mass, &mat, &aabb, &wpts, &wnorm, &dv, &bp_sb};
+	for (auto* a : dbl) a->free_();
+	DArr<uint64_t>* u64s[] = {&bp_keys, &bp_keys2, &pairs, &pairs2, &nk, &e_key};
+	for (auto* a : u64s) a->free_();
+	tmp.free_();
+	for (void* a : man_allocs) (void)hipFree(a);
+	man_allocs.clear();
+	for (void* a : np_allocs) (void)hipFree(a);
+	np_allocs.clear();
+	cb[0].release();
+	cb[1].release();
+}
+
+// Upload static scene data (shapes, bodies) and size per-body buffers.
+void cp_world::upload() {
+	const HostShapes& h = hs;
+	up(s_type, h.type); up(s_pt_off, h.pt_off); up(s_face_off, h.face_off); up(s_arc_off, h.arc_off); up(s_edge_off, h.edge_off);
+	up(s_ref_pts, h.ref_pts); up(s_fv_off, h.fv_off); up(s_fv, h.fv); up(s_ncalc, h.ncalc); up(s_ndir, h.ndir); up(s_sid, h.sid);
+	up(s_gnorm, h.gnorm); up(s_red, h.red); up(s_minpid, h.minpid); up(s_maxpid, h.maxpid); up(s_refidx, h.refidx);
+	up(s_gmin, h.gmin); up(s_gmax, h.gmax); up(s_grefval, h.grefval); up(s_arcs, h.arcs); up(s_edges, h.edges);
+	up(s_vaf_off, h.vaf_off); up(s_vaf, h.vaf); up(s_vae_off, h.vae_off); up(s_vae, h.vae);
+	up(s_radius, h.radius); up(s_height, h.height);
+	int n = nb;
+	std::vector<double> Ir(9 * (size_t)n), Iri(9 * (size_t)n), ms(n), mt(3 * (size_t)n);
+	std::vector<int> wpo(n + 1), wgo(n + 1);
+	for (int i = 0; i < n; i++) {
+		const double* mp = &b_massprops[19 * (size_t)i];
+		ms[i] = mp[0];
+		for (int c = 0; c < 9; c++) {
+			Ir[c * (size_t)n + i] = mp[1 + c];
+			Iri[c * (size_t)n + i] = mp[10 + c];
+		}
+		for (int c = 0; c < 3; c++) mt[c * (size_t)n + i] = b_mat[4 * (size_t)i + 1 + c];
+		int s = b_shape[i];
+		bool hull = h.type[s] == SHAPE_HULL;
+		wpo[i + 1] = wpo[i] + (hull ? h.pt_off[s + 1] - h.pt_off[s] : 0);
+		wgo[i + 1] = wgo[i] + (hull ? h.face_off[s + 1] - h.face_off[s] : 0);
+	}
+	up(Iref, Ir); up(Irefinv, Iri); up(mass, ms); up(mat, mt);
+	up(movement, b_move); up(shape, b_shape); up(scene, b_scene);
+	up(wpt_off, wpo); up(wgv_off, wgo);
+	wpts.reserve(3 * (size_t)wpo[n]);
+	wnorm.reserve(3 * (size_t)wgo[n]);
+	// scenes
+	std::vector<int> sbo;
+	nscenes = 0;
+	for (int i = 0; i < n; i++) {
+		if (i == 0 || b_scene[i] != b_scene[i - 1]) {
+			sbo.push_back(i);
+			nscenes++;
+		}
+	}
+	sbo.push_back(n);
+	std::vector<int> sob(n);
+	for (int s = 0; s < nscenes; s++)
+		for (int i = sbo[s]; i < sbo[s + 1]; i++) sob[i] = s;
+	up(scene_body_off, sbo);
+	up(scene_of_body, sob);
+	DArr<double>* per3[] = {&com, &vel, &angv, &pvel, &pang, &prev_com};
+	for (auto* a : per3) {
+		a->reserve(3 * (size_t)n);
+		HIPCHK(hipMemset(a->p, 0, 3 * (size_t)n * sizeof(double)));
+	}
+	q.reserve(4 * (size_t)n);
+	prev_q.reserve(4 * (size_t)n);
+	I.reserve(9 * (size_t)n);
+	Iinv.reserve(9 * (size_t)n);
+	aabb.reserve(6 * (size_t)n);
+	dv.reserve(12 * (size_t)n);
+	color_mask.reserve(4 * (size_t)n);
+	applied.reserve(n);
+	uploaded = true;
+}
+
+void cp_world::scan(const int* in, int* out, int n) {
+	size_t bytes = 0;
+	HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, n, stream));
+	tmp.reserve(bytes + 16);
+	HIPCHK(hipcub::DeviceScan::ExclusiveSum(tmp.p, bytes, in, out, n, stream));
+}
+
+void cp_world::ensure_manifolds(int n) {
+	if (n <= mcap) return;
+	for (void* a : man_allocs) (void)hipFree(a);
+	man_allocs.clear();
+	int K = std::max(n + n / 2, 1024);
+	auto al = [&](size_t bytes) {
+		void* p = nullptr;
+		if (hipMalloc(&p, bytes) != hipSuccess) throw ApiErr{CP_ERR_OOM};
+		man_allocs.push_back(p);
+		return p;
+	};
+	man.count = (int*)al(sizeof(int) * K);
+	man.normal = (double*)al(sizeof(double) * 3 * K);
+	man.maxpen = (double*)al(sizeof(double) * K);
+	man.pts = (double*)al(sizeof(double) * 16 * K);
+	man.magic = (uint64_t*)al(sizeof(uint64_t) * 8 * K);
+	man.mat = (double*)al(sizeof(double) * 3 * K);
+	mcap = K;
+}
+
+void cp_world::ensure_points(int n) {
+	if (n <= npcap) return;
+	for (void* a : np_allocs) (void)hipFree(a);
+	np_allocs.clear();
+	int K = std::max(n + n / 2, 1024);
+	auto al = [&](size_t bytes) {
+		void* p = nullptr;
+		if (hipMalloc(&p, bytes) != hipSuccess) throw ApiErr{CP_ERR_OOM};
+		np_allocs.push_back(p);
+		return p;
+	};
+	np.key = (uint64_t*)al(8 * (size_t)K); np.cid = (uint64_t*)al(8 * (size_t)K); np.bid = (uint64_t*)al(8 * (size_t)K);
+	np.pos = (double*)al(24 * (size_t)K); np.nrm = (double*)al(24 * (size_t)K);
+	np.pen = (double*)al(8 * (size_t)K); np.mat = (double*)al(24 * (size_t)K);
+	np.is_new = (int*)al(4 * (size_t)K); np.src = (int*)al(4 * (size_t)K);
+	npcap = K;
+}
+
+// Returns the number of candidate pairs now in `pairs` (sorted keys).
+int cp_world::broadphase() {
+	Bodies b = bodies();
+	int n = nb;
+	bp_keys.reserve(n); bp_keys2.reserve(n); bp_vals.reserve(n); bp_vals2.reserve(n);
+	bp_sb.reserve(6 * (size_t)n);
+	bp_counts.reserve(n + 1); bp_offs.reserve(n + 1);
+	launch_bp_keys(b, bp_keys.p, bp_vals.p, stream);
+	size_t bytes = 0;
+	if (nscenes == 1) {
+		HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, bp_keys.p, bp_keys2.p, bp_vals.p, bp_vals2.p, n, 0, 64, stream));
+		tmp.reserve(bytes + 16);
+		HIPCHK(hipcub::DeviceRadixSort::SortPairs(tmp.p, bytes, bp_keys.p, bp_keys2.p, bp_vals.p, bp_vals2.p, n, 0, 64, stream));
+	} else {
+		HIPCHK(hipcub::DeviceSegmentedRadixSort::SortPairs(nullptr, bytes, bp_keys.p, bp_keys2.p, bp_vals.p, bp_vals2.p, n, nscenes,
+		                                                   scene_body_off.p, scene_body_off.p + 1, 0, 64, stream));
+		tmp.reserve(bytes + 16);
+		HIPCHK(hipcub::DeviceSegmentedRadixSort::SortPairs(tmp.p, bytes, bp_keys.p, bp_keys2.p, bp_vals.p, bp_vals2.p, n, nscenes,
+		                                                   scene_body_off.p, scene_body_off.p + 1, 0, 64, stream));
+	}
+	launch_bp_gather(b, bp_vals2.p, bp_sb.p, stream);
+	HIPCHK(hipMemsetAsync(bp_counts.p + n, 0, sizeof(int), stream));
+	launch_bp_sweep(n, bp_vals2.p, bp_sb.p, scene_of_body.p, scene_body_off.p, bp_counts.p, nullptr, nullptr, 0, stream);
+	scan(bp_counts.p, bp_offs.p, n + 1);
+	int np_ = read1(bp_offs.p + n);
+	pairs.reserve(np_); pairs2.reserve(np_);
+	launch_bp_sweep(n, bp_vals2.p, bp_sb.p, scene_of_body.p, scene_body_off.p, bp_counts.p, bp_offs.p, pairs2.p, 1, stream);
+	if (np_ > 0) {
+		bytes = 0;
+		HIPCHK(hipcub::DeviceRadixSort::SortKeys(nullptr, bytes, pairs2.p, pairs.p, np_, 0, 64, stream));
+		tmp.reserve(bytes + 16);
+		HIPCHK(hipcub::DeviceRadixSort::SortKeys(tmp.p, bytes, pairs2.p, pairs.p, np_, 0, 64, stream));
+	}
+	return np_;
+}
+
+int cp_world::narrowphase(int npairs) {
+	ensure_manifolds(npairs);
+	launch_narrowphase(bodies(), shapes(), params(), pairs.p, npairs, man, mcap, stream);
+	return npairs;
+}
+
+// Contact persistence: compaction of manifold points, match, merge.
+void cp_world::persist(int npairs) {
+	Bodies b = bodies();
+	Params P = params();
+	pt_off.reserve(npairs + 1);
+	pt_cnt.reserve(npairs + 1);
+	if (npairs) HIPCHK(hipMemcpyAsync(pt_cnt.p, man.count, npairs * sizeof(int), hipMemcpyDeviceToDevice, stream));
+	HIPCHK(hipMemsetAsync(pt_cnt.p + npairs, 0, sizeof(int), stream));
+	scan(pt_cnt.p, pt_off.p, npairs + 1);
+	int nnew = read1(pt_off.p + npairs);
+	ensure_points(nnew);
+	launch_np_gather(man, mcap, pairs.p, npairs, pt_off.p, np, npcap, stream);
+	int nold = ncontacts;
+	if (nold == 0) cb[cur].reserve(1);
+	launch_np_match(np, nnew, npcap, cb[cur].c, nold, b, P, stream);
+	keep.reserve(nold + 1); keep_scan.reserve(nold + 1);
+	launch_old_keep(cb[cur].c, nold, keep.p, stream);
+	scan(keep.p, keep_scan.p, nold + 1);
+	// is_new -> rank (exclusive scan over nnew+1 with a zero tail)
+	rank.reserve(nnew + 1);
+	pt_cnt.reserve(nnew + 1);
+	if (nnew) HIPCHK(hipMemcpyAsync(pt_cnt.p, np.is_new, nnew * sizeof(int), hipMemcpyDeviceToDevice, stream));
+	HIPCHK(hipMemsetAsync(pt_cnt.p + nnew, 0, sizeof(int), stream));
+	scan(pt_cnt.p, rank.p, nnew + 1);
+	int nkept = read1(keep_scan.p + nold);
+	int nnc = read1(rank.p + nnew);
+	nk.reserve(nnc + 1);
+	launch_new_keys(np, nnew, rank.p, nk.p, stream);
+	int other = 1 - cur;
+	int total = nkept + nnc;
+	cb[other].reserve(total);
+	launch_merge(cb[cur].c, nold, keep_scan.p, nk.p, nnc, np, nnew, npcap, rank.p, cb[other].c, b, P, seq, stream);
+	seq += (uint64_t)nnew + 1;
+	cur = other;
+	ncontacts = total;
+}
+
+void cp_world::edges() {
+	int n = ncontacts;
+	e_head.reserve(n + 1); e_idx.reserve(n + 1);
+	HIPCHK(hipMemsetAsync(e_head.p + n, 0, sizeof(int), stream));
+	launch_edge_heads(cb[cur].c, n, e_head.p, stream);
+	scan(e_head.p, e_idx.p, n + 1);
+	nedges = read1(e_idx.p + n);
+	e_key.reserve(nedges); e_first.reserve(nedges); e_cnt.reserve(nedges); e_live.reserve(nedges);
+	e_color.reserve(nedges); e_order.reserve(nedges);
+	launch_edge_build(cb[cur].c, n, e_head.p, e_idx.p, e_key.p, e_first.p, e_cnt.p, e_live.p, stream);
+}
+
+// Island search of PhysicsEngine::sleepOrSolveIslands (PhysicsEngine.cpp:1536-1613)
+// with bfsVisitAll (:1263-1283), on the host over the post-narrowphase cache.
+void cp_world::schedule_reference_order() {
+	int n = ncontacts;
+	std::vector<uint64_t> key;
+	std::vector<int> live;
+	down(key, cb[cur].c.key, n);
+	down(live, cb[cur].c.live, n);
+	std::vector<std::vector<std::pair<int, int>>> adj(nb);  // (other, first contact index), by other id
+	for (int k = 0; k < n; k++) {
+		if (k > 0 && key[k] == key[k - 1]) continue;
+		int lo = (int)(key[k] >> 32), hi = (int)(key[k] & 0xFFFFFFFFu);
+		adj[lo].push_back({hi, k});
+		adj[hi].push_back({lo, k});
+	}
+	for (auto& a : adj) std::sort(a.begin(), a.end());
+	std::vector<char> visited(nb, 0);
+	std::vector<int> roots;
+	if ((int)node_order.size() == nb)
+		for (int64_t id : node_order) roots.push_back((int)id - 1);
+	else {
+		std::vector<int64_t> ord(nb);
+		cp_reference_node_order(nb, ord.data());
+		for (int64_t id : ord) roots.push_back((int)id - 1);
+	}
+	last_rows.clear();
+	last_offs.assign(1, 0);
+	for (int root : roots) {
+		if (b_move[root] != 0 || visited[root]) continue;
+		size_t start = last_rows.size();
+		std::vector<int> st{root};
+		while (!st.empty()) {
+			int c = st.back();
+			st.pop_back();
+			if (visited[c]) continue;
+			visited[c] = 1;
+			if (b_move[c] == 0) {
+				for (auto& e : adj[c]) {
+					if (visited[e.first]) continue;
+					for (int k = e.second; k < n && key[k] == key[e.second]; k++) {
+						if (!live[k]) continue;
+						last_rows.push_back(k); last_rows.push_back(0);
+						last_rows.push_back(k); last_rows.push_back(1);
+					}
+				}
+			}
+			for (auto& e : adj[c])
+				if (b_move[e.first] == 0 && !visited[e.first]) st.push_back(e.first);
+		}
+		if (last_rows.size() > start) last_offs.push_back((int64_t)last_rows.size() / 2);
+	}
+	std::vector<int> rows(last_rows.begin(), last_rows.end()), offs(last_offs.begin(), last_offs.end());
+	up(isl_rows, rows);
+	up(isl_off, offs);
+}
+
+void cp_world::step() {
+	Params P = params();
+	Bodies b = bodies();
+	if (timing) HIPCHK(hipEventRecord(ev[0], stream));
+	launch_gravity(b, P, stream);
+	int npairs = broadphase();
+	if (timing) HIPCHK(hipEventRecord(ev[1], stream));
+	narrowphase(npairs);
+	if (timing) HIPCHK(hipEventRecord(ev[2], stream));
+	persist(npairs);
+	edges();
+	bool ordered = cfg.pgs_mode == CP_PGS_REFERENCE_ORDER;
+	int nisl = 0;
+	if (ordered) {
+		schedule_reference_order();
+		nisl = (int)last_offs.size() - 1;
+	} else {
+		color_off.reserve((size_t)nscenes * (CP_MAXC + 2));
+		ncolors.reserve(nscenes);
+		edge_range.reserve(2 * (size_t)nscenes);
+		launch_color(b, scene_body_off.p, nscenes, e_key.p, e_live.p, nedges, e_color.p, e_order.p, color_off.p, ncolors.p, edge_range.p,
+		             stream);
+	}
+	if (timing) HIPCHK(hipEventRecord(ev[3], stream));
+	for (int s = 0; s < cfg.substeps; s++) {
+		launch_build_rows(b, cb[cur].c, ncontacts, P, stream);
+		if (ordered) {
+			HIPCHK(hipMemsetAsync(dv.p, 0, 12 * (size_t)nb * sizeof(double), stream));
+			HIPCHK(hipMemsetAsync(applied.p, 0, (size_t)nb * sizeof(int), stream));
+			launch_pgs_ordered(b, cb[cur].c, isl_rows.p, isl_off.p, nisl, P, applied.p, stream);
+		} else {
+			launch_pgs_colored(b, cb[cur].c, scene_body_off.p, nscenes, edge_range.p, e_order.p, color_off.p, ncolors.p, e_first.p, e_cnt.p,
+			                   P, cfg.solver_threads, stream);
+		}
+		if (s == cfg.substeps - 1) launch_age(cb[cur].c, ncontacts, stream);
+		launch_integrate(b, P, stream);
+	}
+	if (timing) HIPCHK(hipEventRecord(ev[4], stream));
+	launch_refresh_geometry(b, shapes(), 1, stream);
+	if (timing) {
+		HIPCHK(hipEventRecord(ev[5], stream));
+		HIPCHK(hipEventSynchronize(ev[5]));
+		for (int i = 0; i < 5; i++) {
+			float ms = 0;
+			HIPCHK(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
+			last_ms[i] = ms;
+		}
+	}
+}
+
+// ============================================================================ C ABI
+static int guard(cp_world* w, const std::function<void()>& f) {
+	try {
+		if (w) HIPCHK(hipSetDevice(w->device));
+		f();
+		return CP_OK;
+	} catch (const ApiErr& e) {
+		return e.code;
+	} catch (const DevErr& e) {
+		fprintf(stderr, "cp_hip: %s failed: %s\n", e.what, hipGetErrorString(e.e));
+		return CP_ERR_DEVICE;
+	} catch (const std::bad_alloc&) {
+		return CP_ERR_OOM;
+	}
+}
+
+extern "C" {
+
+void cp_config_default(cp_config* c) {
+	c->dt = 1.0 / 90;
+	c->gravity[0] = 0; c->gravity[1] = -6; c->gravity[2] = 0;
+	c->vel_iters = 4; c->pos_iters = 1; c->hol_iters = 1;
+	c->substeps = 8;
+	c->cfm = 0.0000025;
+	c->contact_hardness = 1000;
+	c->contact_life = 6;
+	c->gyro_ticks = 4;
+	c->pgs_mode = CP_PGS_COLORED;
+	c->solver_threads = 256;
+}
+
+const char* cp_status_string(int s) {
+	switch (s) {
+	case CP_OK: return "ok";
+	case CP_ERR_INVALID: return "invalid argument";
+	case CP_ERR_OOM: return "device out of memory";
+	case CP_ERR_DEVICE: return "HIP runtime error";
+	case CP_ERR_CAPACITY: return "buffer too small";
+	case CP_ERR_STATE: return "invalid world state";
+	}
+	return "unknown";
+}
+
+static bool valid_cfg(const cp_config* c) {
+	return c && c->dt > 0 && c->substeps >= 1 && c->vel_iters >= 0 && c->pos_iters >= 0 && c->gyro_ticks >= 1 && c->solver_threads >= 64 &&
+	       c->solver_threads <= 1024 && (c->solver_threads % 64) == 0 && (c->pgs_mode == 0 || c->pgs_mode == 1);
+}
+
+int cp_world_create(const cp_config* cfg, cp_world** out) {
+	if (!out || !valid_cfg(cfg)) return CP_ERR_INVALID;
+	*out = nullptr;
+	int ndev = 0;
+	if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return CP_ERR_DEVICE;
+	cp_world* w = new cp_world();
+	w->cfg = *cfg;
+	int st = guard(w, [&] {
+		HIPCHK(hipGetDevice(&w->device));
+		HIPCHK(hipStreamCreate(&w->stream));
+		for (auto& e : w->ev) HIPCHK(hipEventCreate(&e));
+	});
+	if (st != CP_OK) {
+		delete w;
+		return st;
+	}
+	*out = w;
+	return CP_OK;
+}
+
+int cp_world_destroy(cp_world* w) {
+	if (!w) return CP_ERR_INVALID;
+	(void)hipSetDevice(w->device);
+	(void)hipStreamSynchronize(w->stream);
+	w->free_all();
+	for (auto& e : w->ev) (void)hipEventDestroy(e);
+	(void)hipStreamDestroy(w->stream);
+	delete w;
+	return CP_OK;
+}
+
+int cp_world_set_config(cp_world* w, const cp_config* cfg) {
+	if (!w || !valid_cfg(cfg)) return CP_ERR_INVALID;
+	w->cfg = *cfg;
+	return CP_OK;
+}
+
+int cp_shape_add_hull_exact(cp_world* w, int n_pts, const double* pts, int n_faces, const int64_t* face_off, const int64_t* face_verts,
+                            const int64_t* surface_id, const int64_t* normal_calc, const int64_t* normal_dir, const double* gauss_normal,
+                            const int64_t* gauss_red, const int64_t* gauss_ints, const double* gauss_vals, int n_arcs, const int64_t* arcs,
+                            int n_edges, const int64_t* edges, int* shape_id) {
+	if (!w || !shape_id || n_pts < 4 || n_faces < 4 || !pts || !face_off || !face_verts) return CP_ERR_INVALID;
+	HullData d;
+	d.pts.assign(pts, pts + 3 * (size_t)n_pts);
+	for (int f = 0; f < n_faces; f++) {
+		std::vector<int> fv;
+		for (int64_t k = face_off[f]; k < face_off[f + 1]; k++) {
+			if (face_verts[k] < 0 || face_verts[k] >= n_pts) return CP_ERR_INVALID;
+			fv.push_back((int)face_verts[k]);
+		}
+		if (fv.size() < 3) return CP_ERR_INVALID;
+		d.faces.push_back(fv);
+		d.sid.push_back((int)surface_id[f]);
+		d.ncalc.push_back((int)normal_calc[f]);
+		d.ndir.push_back((int)normal_dir[f]);
+		for (int c = 0; c < 3; c++) d.gnorm.push_back(gauss_normal[3 * f + c]);
+		d.red.push_back((int)gauss_red[f]);
+		d.minpid.push_back((int)gauss_ints[3 * f]);
+		d.maxpid.push_back((int)gauss_ints[3 * f + 1]);
+		d.refidx.push_back((int)gauss_ints[3 * f + 2]);
+		d.gmin.push_back(gauss_vals[3 * f]);
+		d.gmax.push_back(gauss_vals[3 * f + 1]);
+		d.grefval.push_back(gauss_vals[3 * f + 2]);
+	}
+	for (int a = 0; a < n_arcs; a++) d.arcs.push_back({(int)arcs[2 * a], (int)arcs[2 * a + 1]});
+	for (int e = 0; e < n_edges; e++) d.edges.push_back({(int)edges[2 * e], (int)edges[2 * e + 1]});
+	*shape_id = w->hs.add_hull(d);
+	w->uploaded = false;
+	return CP_OK;
+}
+
+int cp_shape_add_polyhedron(cp_world* w, int n_pts, const double* pts, int n_faces, const int64_t* face_off, const int64_t* face_verts,
+                            int* shape_id) {
+	if (!w || !shape_id || n_pts < 4 || n_faces < 4 || !pts || !face_off || !face_verts) return CP_ERR_INVALID;
+	std::vector<std::vector<int>> faces;
+	for (int f = 0; f < n_faces; f++) {
+		std::vector<int> fv;
+		for (int64_t k = face_off[f]; k < face_off[f + 1]; k++) {
+			if (face_verts[k] < 0 || face_verts[k] >= n_pts) return CP_ERR_INVALID;
+			fv.push_back((int)face_verts[k]);
+		}
+		if (fv.size() < 3) return CP_ERR_INVALID;
+		faces.push_back(fv);
+	}
+	HullData d = build_polyhedron(std::vector<double>(pts, pts + 3 * (size_t)n_pts), faces);
+	*shape_id = w->hs.add_hull(d);
+	w->uploaded = false;
+	return CP_OK;
+}
+
+int cp_polyhedron_mass_properties(int n_pts, const double* pts, int n_faces, const int64_t* face_off, const int64_t* face_verts,
+                                  double density, double* out13) {
+	if (n_pts < 4 || n_faces < 4 || !pts || !face_off || !face_verts || !out13 || !(density > 0)) return CP_ERR_INVALID;
+	std::vector<std::vector<int>> faces;
+	for (int f = 0; f < n_faces; f++) {
+		std::vector<int> fv;
+		for (int64_t k = face_off[f]; k < face_off[f + 1]; k++) {
+			if (face_verts[k] < 0 || face_verts[k] >= n_pts) return CP_ERR_INVALID;
+			fv.push_back((int)face_verts[k]);
+		}
+		if (fv.size() < 3) return CP_ERR_INVALID;
+		faces.push_back(fv);
+	}
+	polyhedron_mass_properties(std::vector<double>(pts, pts + 3 * (size_t)n_pts), faces, density, out13, out13 + 1, out13 + 10);
+	return CP_OK;
+}
+
+int cp_body_add(cp_world* w, int shape_id, int movement, const double* mp, const double* material, int scene, int* body_index) {
+	if (!w || !mp || !material || !body_index) return CP_ERR_INVALID;
+	if (shape_id < 0 || shape_id >= (int)w->hs.type.size() || (movement != 0 && movement != 1)) return CP_ERR_INVALID;
+	if (!w->b_scene.empty() && scene < w->b_scene.back()) return CP_ERR_INVALID;  // scenes are contiguous and ascending
+	if (w->uploaded && w->ncontacts > 0) return CP_ERR_STATE;
+	w->b_shape.push_back(shape_id);
+	w->b_move.push_back(movement);
+	w->b_scene.push_back(scene);
+	w->b_massprops.insert(w->b_massprops.end(), mp, mp + 19);
+	w->b_mat.insert(w->b_mat.end(), material, material + 4);
+	*body_index = w->nb++;
+	w->uploaded = false;
+	return CP_OK;
+}
+
+int cp_world_body_count(cp_world* w, int* n) {
+	if (!w || !n) return CP_ERR_INVALID;
+	*n = w->nb;
+	return CP_OK;
+}
+
+int cp_world_set_state(cp_world* w, const double* s, const double* inertia) {
+	if (!w || !s || w->nb == 0) return CP_ERR_INVALID;
+	return guard(w, [&] {
+		if (!w->uploaded) w->upload();
+		int n = w->nb;
+		std::vector<double> com(3 * (size_t)n), q(4 * (size_t)n), v(3 * (size_t)n), av(3 * (size_t)n);
+		for (int i = 0; i < n; i++) {
+			const double* r = s + 13 * (size_t)i;
+			for (int c = 0; c < 3; c++) {
+				com[c * (size_t)n + i] = r[c];
+				v[c * (size_t)n + i] = r[7 + c];
+				av[c * (size_t)n + i] = r[10 + c];
+			}
+			for (int c = 0; c < 4; c++) q[c * (size_t)n + i] = r[3 + c];
+		}
+		HIPCHK(hipMemcpy(w->com.p, com.data(), com.size() * 8, hipMemcpyHostToDevice));
+		HIPCHK(hipMemcpy(w->q.p, q.data(), q.size() * 8, hipMemcpyHostToDevice));
+		HIPCHK(hipMemcpy(w->vel.p, v.data(), v.size() * 8, hipMemcpyHostToDevice));
+		HIPCHK(hipMemcpy(w->angv.p, av.data(), av.size() * 8, hipMemcpyHostToDevice));
+		HIPCHK(hipMemcpy(w->prev_com.p, com.data(), com.size() * 8, hipMemcpyHostToDevice));
+		HIPCHK(hipMemcpy(w->prev_q.p, q.data(), q.size() * 8, hipMemcpyHostToDevice));
+		HIPCHK(hipMemset(w->pvel.p, 0, 3 * (size_t)n * 8));
+		HIPCHK(hipMemset(w->pang.p, 0, 3 * (size_t)n * 8));
+		if (inertia) {
+			std::vector<double> I(9 * (size_t)n), Ii(9 * (size_t)n);
+			for (int i = 0; i < n; i++)
+				for (int c = 0; c < 9; c++) {
+					I[c * (size_t)n + i] = inertia[18 * (size_t)i + c];
+					Ii[c * (size_t)n + i] = inertia[18 * (size_t)i + 9 + c];
+				}
+			HIPCHK(hipMemcpy(w->I.p, I.data(), I.size() * 8, hipMemcpyHostToDevice));
+			HIPCHK(hipMemcpy(w->Iinv.p, Ii.data(), Ii.size() * 8, hipMemcpyHostToDevice));
+		} else {
+			launch_update_inertia(w->bodies(), w->stream);
+		}
+		launch_refresh_geometry(w->bodies(), w->shapes(), 0, w->stream);
+		HIPCHK(hipStreamSynchronize(w->stream));
+	});
+}
+
+int cp_world_get_state(cp_world* w, double* s) {
+	if (!w || !s || !w->uploaded) return CP_ERR_INVALID;
+	return guard(w, [&] {
+		HIPCHK(hipStreamSynchronize(w->stream));
+		int n = w->nb;
+		std::vector<double> com, q, v, av;
+		down(com, w->com.p, 3 * (size_t)n);
+		down(q, w->q.p, 4 * (size_t)n);
+		down(v, w->vel.p, 3 * (size_t)n);
+		down(av, w->angv.p, 3 * (size_t)n);
+		for (int i = 0; i < n; i++) {
+			double* r = s + 13 * (size_t)i;
+			for (int c = 0; c < 3; c++) {
+				r[c] = com[c * (size_t)n + i];
+				r[7 + c] = v[c * (size_t)n + i];
+				r[10 + c] = av[c * (size_t)n + i];
+			}
+			for (int c = 0; c < 4; c++) r[3 + c] = q[c * (size_t)n + i];
+		}
+	});
+}
+
+int cp_world_get_aabbs(cp_world* w, double* a) {
+	if (!w || !a || !w->uploaded) return CP_ERR_INVALID;
+	return guard(w, [&] {
+		HIPCHK(hipStreamSynchronize(w->stream));
+		int n = w->nb;
+		std::vector<double> h;
+		down(h, w->aabb.p, 6 * (size_t)n);
+		for (int i = 0; i < n; i++)
+			for (int c = 0; c < 6; c++) a[6 * (size_t)i + c] = h[c * (size_t)n + i];
+	});
+}
+
+int cp_world_set_contacts(cp_world* w, int n, const int64_t* ii, const uint64_t* mg, const double* ff) {
+	if (!w || n < 0 || (n > 0 && (!ii || !mg || !ff)) || !w->uploaded) return CP_ERR_INVALID;
+	for (int k = 0; k < n; k++) {
+		const int64_t* r = ii + 6 * (size_t)k;
+		if (r[0] < 1 || r[1] <= r[0] || r[1] > w->nb) return CP_ERR_INVALID;
+		if (k > 0) {
+			const int64_t* p = ii + 6 * (size_t)(k - 1);
+			if (p[0] > r[0] || (p[0] == r[0] && p[1] > r[1])) return CP_ERR_INVALID;  // grouped by sorted pair
+		}
+	}
+	return guard(w, [&] {
+		ContactBuf& B = w->cb[w->cur];
+		B.reserve(std::max(n, 1));
+		int K = B.cap;
+		std::vector<uint64_t> key(n), cid(n), bid(n), seq(n);
+		std::vector<double> lp1(3 * (size_t)K), lp2(3 * (size_t)K), ln1(3 * (size_t)K), fu(3 * (size_t)K), fw(3 * (size_t)K), imp_t(2 * (size_t)K);
+		std::vector<double> pen(n), bounce(n), mu(n), imp_n(n);
+		std::vector<int> life(n), live(n), sr(n);
+		for (int k = 0; k < n; k++) {
+			const int64_t* r = ii + 6 * (size_t)k;
+			const double* f = ff + 21 * (size_t)k;
+			key[k] = ((uint64_t)(r[0] - 1) << 32) | (uint64_t)(r[1] - 1);
+			cid[k] = mg[2 * k];
+			bid[k] = mg[2 * k + 1];
+			seq[k] = (uint64_t)k;
+			life[k] = (int)r[2];
+			live[k] = (int)r[3];
+			sr[k] = (int)r[4];
+			for (int c = 0; c < 3; c++) {
+				lp1[c * (size_t)K + k] = f[c];
+				lp2[c * (size_t)K + k] = f[3 + c];
+				ln1[c * (size_t)K + k] = f[6 + c];
+				fu[c * (size_t)K + k] = f[15 + c];
+				fw[c * (size_t)K + k] = f[18 + c];
+			}
+			pen[k] = f[9]; bounce[k] = f[10]; mu[k] = f[11]; imp_n[k] = f[12];
+			imp_t[k] = f[13];
+			imp_t[K + k] = f[14];
+		}
+		auto cp = [&](void* d, const void* h, size_t bytes) {
+			if (bytes) HIPCHK(hipMemcpy(d, h, bytes, hipMemcpyHostToDevice));
+		};
+		Contacts& C = B.c;
+		cp(C.key, key.data(), 8 * (size_t)n); cp(C.cid, cid.data(), 8 * (size_t)n); cp(C.bid, bid.data(), 8 * (size_t)n);
+		cp(C.seq, seq.data(), 8 * (size_t)n);
+		cp(C.lp1, lp1.data(), lp1.size() * 8); cp(C.lp2, lp2.data(), lp2.size() * 8); cp(C.ln1, ln1.data(), ln1.size() * 8);
+		cp(C.fu, fu.data(), fu.size() * 8); cp(C.fw, fw.data(), fw.size() * 8); cp(C.imp_t, imp_t.data(), imp_t.size() * 8);
+		cp(C.pen, pen.data(), 8 * (size_t)n); cp(C.bounce, bounce.data(), 8 * (size_t)n); cp(C.mu, mu.data(), 8 * (size_t)n);
+		cp(C.imp_n, imp_n.data(), 8 * (size_t)n);
+		cp(C.life, life.data(), 4 * (size_t)n); cp(C.live, live.data(), 4 * (size_t)n); cp(C.sready, sr.data(), 4 * (size_t)n);
+		w->ncontacts = n;
+	});
+}
+
+int cp_world_get_contact_count(cp_world* w, int* n) {
+	if (!w || !n) return CP_ERR_INVALID;
+	return guard(w, [&] {
+		HIPCHK(hipStreamSynchronize(w->stream));
+		std::vector<int> life;
+		down(life, w->cb[w->cur].c.life, w->ncontacts);
+		int c = 0;
+		for (int v : life) c += v >= -1;
+		*n = c;
+	});
+}
+
+// Contacts that the reference would still hold (memory_life >= -1), in cache order.
+int cp_world_get_contacts(cp_world* w, int cap, int64_t* ii, uint64_t* mg, double* ff, int* nout) {
+	if (!w || !nout) return CP_ERR_INVALID;
+	int status = CP_OK;
+	int g = guard(w, [&] {
+		HIPCHK(hipStreamSynchronize(w->stream));
+		const Contacts& C = w->cb[w->cur].c;
+		int n = w->ncontacts, K = C.cap;
+		std::vector<uint64_t> key, cid, bid;
+		std::vector<double> lp1, lp2, ln1, fu, fw, imp_t, pen, bounce, mu, imp_n;
+		std::vector<int> life, live, sr;
+		down(key, C.key, n); down(cid, C.cid, n); down(bid, C.bid, n);
+		down(lp1, C.lp1, 3 * (size_t)K); down(lp2, C.lp2, 3 * (size_t)K); down(ln1, C.ln1, 3 * (size_t)K);
+		down(fu, C.fu, 3 * (size_t)K); down(fw, C.fw, 3 * (size_t)K); down(imp_t, C.imp_t, 2 * (size_t)K);
+		down(pen, C.pen, n); down(bounce, C.bounce, n); down(mu, C.mu, n); down(imp_n, C.imp_n, n);
+		down(life, C.life, n); down(live, C.live, n); down(sr, C.sready, n);
+		int o = 0, pos = 0;
+		uint64_t prev = ~0ull;
+		for (int k = 0; k < n; k++) {
+			if (life[k] < -1) continue;
+			if (key[k] != prev) pos = 0;
+			prev = key[k];
+			if (o < cap && ii && mg && ff) {
+				int64_t* r = ii + 6 * (size_t)o;
+				r[0] = (int64_t)(key[k] >> 32) + 1;
+				r[1] = (int64_t)(key[k] & 0xFFFFFFFFu) + 1;
+				r[2] = life[k]; r[3] = live[k]; r[4] = sr[k]; r[5] = pos;
+				mg[2 * o] = cid[k];
+				mg[2 * o + 1] = bid[k];
+				double* f = ff + 21 * (size_t)o;
+				for (int c = 0; c < 3; c++) {
+					f[c] = lp1[c * (size_t)K + k];
+					f[3 + c] = lp2[c * (size_t)K + k];
+					f[6 + c] = ln1[c * (size_t)K + k];
+					f[15 + c] = fu[c * (size_t)K + k];
+					f[18 + c] = fw[c * (size_t)K + k];
+				}
+				f[9] = pen[k]; f[10] = bounce[k]; f[11] = mu[k]; f[12] = imp_n[k]; f[13] = imp_t[k]; f[14] = imp_t[K + k];
+			}
+			pos++;
+			o++;
+		}
+		*nout = o;
+		if (o > cap) status = CP_ERR_CAPACITY;
+	});
+	return g != CP_OK ? g : status;
+}
+
+int cp_world_set_node_order(cp_world* w, int n, const int64_t* ids) {
+	if (!w || n < 0 || (n > 0 && !ids)) return CP_ERR_INVALID;
+	for (int k = 0; k < n; k++)
+		if (ids[k] < 1 || ids[k] > w->nb) return CP_ERR_INVALID;
+	w->node_order.assign(ids, ids + n);
+	return CP_OK;
+}
+
+// Iteration order of std::unordered_map<unsigned, T> after inserting 1..n in
+// sequence — the container type the reference keeps its constraint-graph nodes
+// in (PhysicsEngine.h:230); produced by the same standard library.
+int cp_reference_node_order(int n, int64_t* ids) {
+	if (n < 0 || (n > 0 && !ids)) return CP_ERR_INVALID;
+	std::unordered_map<unsigned int, int> m;
+	for (int i = 1; i <= n; i++) m[(unsigned)i] = i;
+	int k = 0;
+	for (auto& kv : m) ids[k++] = kv.first;
+	return CP_OK;
+}
+
+int cp_world_step(cp_world* w, int n_steps) {
+	if (!w || n_steps < 0 || w->nb == 0) return CP_ERR_INVALID;
+	if (!w->uploaded) return CP_ERR_STATE;
+	return guard(w, [&] {
+		for (int i = 0; i < n_steps; i++) w->step();
+		HIPCHK(hipStreamSynchronize(w->stream));
+	});
+}
+
+int cp_broadphase(cp_world* w, int cap, int64_t* out, int* n) {
+	if (!w || !n || !w->uploaded) return CP_ERR_INVALID;
+	int status = CP_OK;
+	int g = guard(w, [&] {
+		int np_ = w->broadphase();
+		std::vector<uint64_t> h;
+		down(h, w->pairs.p, np_);
+		*n = np_;
+		if (np_ > cap || !out) {
+			status = CP_ERR_CAPACITY;
+			return;
+		}
+		for (int k = 0; k < np_; k++) {
+			out[2 * k] = (int64_t)(h[k] >> 32) + 1;
+			out[2 * k + 1] = (int64_t)(h[k] & 0xFFFFFFFFu) + 1;
+		}
+	});
+	return g != CP_OK ? g : status;
+}
+
+int cp_narrowphase(cp_world* w, int n, const int64_t* pairs2, int* counts, double* heads, double* pts, uint64_t* magic) {
+	if (!w || n < 0 || !w->uploaded || (n > 0 && (!pairs2 || !counts || !heads || !pts || !magic))) return CP_ERR_INVALID;
+	for (int k = 0; k < n; k++)
+		if (pairs2[2 * k] < 1 || pairs2[2 * k + 1] <= pairs2[2 * k] || pairs2[2 * k + 1] > w->nb) return CP_ERR_INVALID;
+	return guard(w, [&] {
+		std::vector<uint64_t> keys(n);
+		for (int k = 0; k < n; k++) keys[k] = ((uint64_t)(pairs2[2 * k] - 1) << 32) | (uint64_t)(pairs2[2 * k + 1] - 1);
+		up(w->pairs, keys);
+		w->narrowphase(n);
+		HIPCHK(hipStreamSynchronize(w->stream));
+		int K = w->mcap;
+		std::vector<int> cnt;
+		std::vector<double> nrm, mp, pp;
+		std::vector<uint64_t> mg;
+		down(cnt, w->man.count, n);
+		down(nrm, w->man.normal, 3 * (size_t)K);
+		down(mp, w->man.maxpen, n);
+		down(pp, w->man.pts, 16 * (size_t)n);
+		down(mg, w->man.magic, 8 * (size_t)n);
+		for (int k = 0; k < n; k++) {
+			counts[k] = cnt[k];
+			for (int c = 0; c < 3; c++) heads[4 * k + c] = cnt[k] ? nrm[c * (size_t)K + k] : 0;
+			heads[4 * k + 3] = mp[k];
+			for (int j = 0; j < 16; j++) pts[16 * (size_t)k + j] = pp[16 * (size_t)k + j];
+			for (int j = 0; j < 8; j++) magic[8 * (size_t)k + j] = mg[8 * (size_t)k + j];
+		}
+	});
+}
+
+int cp_world_last_islands(cp_world* w, int cap_rows, int64_t* rows2, int cap_offs, int64_t* offs, int* n_rows, int* n_islands) {
+	if (!w || !n_rows || !n_islands) return CP_ERR_INVALID;
+	int nr = (int)w->last_rows.size() / 2, no = (int)w->last_offs.size();
+	*n_rows = nr;
+	*n_islands = no - 1;
+	if (nr > cap_rows || no > cap_offs || !rows2 || !offs) return CP_ERR_CAPACITY;
+	memcpy(rows2, w->last_rows.data(), w->last_rows.size() * sizeof(int64_t));
+	memcpy(offs, w->last_offs.data(), w->last_offs.size() * sizeof(int64_t));
+	return CP_OK;
+}
+
+int cp_world_last_timing(cp_world* w, double* ms) {
+	if (!w || !ms) return CP_ERR_INVALID;
+	for (int i = 0; i < 8; i++) ms[i] = w->last_ms[i];
+	return CP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,139 @@
+/* cp_abi.h — C ABI of the MI355X (gfx950) contact-pipeline library
+ * (constraintphysics_amd/libcp_hip.so).
+ *
+ * The boundary sits where NathanMacLeod/ConstraintPhysics' PhysicsEngine
+ * (ConstraintPhysics/src/PhysicsEngine.h:60-165) hands work to its per-step
+ * pipeline.  No C++ or torch types cross it: plain pointers, sizes and int
+ * status codes.  Each entry point names the reference interface it replaces.
+ *
+ * Conventions
+ *   - Every function returns cp_status (0 = OK).  No exceptions cross the ABI.
+ *   - Bodies are indexed densely from 0 in creation order; the reference body
+ *     id is index + 1 (PhysicsEngine.cpp:487, next_id starts at 1).
+ *   - Body state rows are 13 doubles: com xyz, orientation (r, i, j, k),
+ *     velocity xyz, angular velocity xyz.
+ *   - Contact cache rows: 6 int64 (id1, id2, memory_life, is_live,
+ *     static_ready, position in edge), 2 uint64 MagicID (cID, bID), 21 doubles
+ *     (contact point in b1 and b2 frames, normal in b1 frame, pen depth,
+ *     restitution, friction coefficient, normal impulse, 2 friction impulses,
+ *     friction basis u and w).  cID halves encode primitives as
+ *     body_id * 16 + primitive index.
+ *   - Bodies of one scene must be added contiguously; scenes never interact.
+ */
+#ifndef CP_ABI_H
+#define CP_ABI_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+	CP_OK = 0,
+	CP_ERR_INVALID = 1,   /* bad argument */
+	CP_ERR_OOM = 2,       /* device allocation failed */
+	CP_ERR_DEVICE = 3,    /* HIP runtime error */
+	CP_ERR_CAPACITY = 4,  /* caller buffer too small (required size returned) */
+	CP_ERR_STATE = 5      /* call not valid in the current world state */
+} cp_status;
+
+typedef enum { CP_PGS_COLORED = 0, CP_PGS_REFERENCE_ORDER = 1 } cp_pgs_mode;
+
+/* Engine parameters; defaults of PhysicsEngine.h:174-216 via cp_config_default. */
+typedef struct {
+	double dt;               /* setStep_time (PhysicsEngine.h:98) */
+	double gravity[3];       /* setGravity */
+	int vel_iters;           /* setPGSIterations(n_vel, n_pos, n_holonomic) */
+	int pos_iters;
+	int hol_iters;
+	int substeps;            /* setSubstepCount */
+	double cfm;              /* setGlobalConstraintForceMixing */
+	double contact_hardness; /* contact_pos_correct_hardness (PhysicsEngine.h:216) */
+	int contact_life;        /* contact_life (PhysicsEngine.h:215) */
+	int gyro_ticks;          /* setAngleVelUpdateTickCount */
+	int pgs_mode;            /* cp_pgs_mode */
+	int solver_threads;      /* threads per scene workgroup in colored mode (64..1024) */
+} cp_config;
+
+typedef struct cp_world cp_world;
+
+void cp_config_default(cp_config* cfg);
+const char* cp_status_string(int status);
+
+/* World lifetime.  Replaces constructing a phyz::PhysicsEngine. */
+int cp_world_create(const cp_config* cfg, cp_world** out);
+int cp_world_destroy(cp_world* w);
+int cp_world_set_config(cp_world* w, const cp_config* cfg);
+
+/* Shapes (body-frame reference geometry, device resident).
+ * cp_shape_add_hull_exact takes a polyhedron exactly as the reference stores
+ * it after construction (Polyhedron ctor, ConvexPrimitive.cpp:582-658, and
+ * computeGaussMap :861-896): faces with winding fixed, per-face
+ * normal_calc_index / normalDirection (:783-812), Gauss normals, redundancy
+ * flags, cached self-extrema (min_pid, max_pid, ref_idx) and (min, max,
+ * ref_value), arcs and edges.  Used to replay reference states bit for bit. */
+int cp_shape_add_hull_exact(cp_world* w, int n_pts, const double* pts, int n_faces, const int64_t* face_off,
+                            const int64_t* face_verts, const int64_t* surface_id, const int64_t* normal_calc,
+                            const int64_t* normal_dir, const double* gauss_normal, const int64_t* gauss_red,
+                            const int64_t* gauss_ints, const double* gauss_vals, int n_arcs, const int64_t* arcs,
+                            int n_edges, const int64_t* edges, int* shape_id);
+/* Convex polyhedron from points and face vertex lists (Polyhedron(points,
+ * surface_vertex_indices), ConvexPrimitive.h:216): builds surfaces, edges and
+ * the Gauss map on the host the way the reference constructor does.  Points
+ * are given in the body frame (centre of mass at the origin). */
+int cp_shape_add_polyhedron(cp_world* w, int n_pts, const double* pts, int n_faces, const int64_t* face_off,
+                            const int64_t* face_verts, int* shape_id);
+/* Mass properties of a closed polyhedron of uniform density:
+ * out = mass, inertia tensor about the centre of mass [9], centre of mass [3]
+ * (calculateMassProperties, RigidBody.cpp:427-640). */
+int cp_polyhedron_mass_properties(int n_pts, const double* pts, int n_faces, const int64_t* face_off,
+                                  const int64_t* face_verts, double density, double* out13);
+
+/* Bodies.  Replaces PhysicsEngine::createRigidBody (PhysicsEngine.cpp:486-496).
+ * movement: 0 DYNAMIC, 1 FIXED (RigidBody.h:22).  mass_props: mass,
+ * reference tensor[9], reference inverse tensor[9].  material: density,
+ * restitution, kinetic and static friction (ConvexPrimitive.h:17-34). */
+int cp_body_add(cp_world* w, int shape_id, int movement, const double* mass_props, const double* material, int scene,
+                int* body_index);
+int cp_world_body_count(cp_world* w, int* n);
+
+/* State transfer (host <-> HBM).  inertia (nullable): world tensor[9] and
+ * inverse[9] per body (RigidBody::tensor / invTensor); when null they are
+ * recomputed from the orientation (RigidBody::updateInertiaTensor). */
+int cp_world_set_state(cp_world* w, const double* state13, const double* inertia18);
+int cp_world_get_state(cp_world* w, double* state13);
+int cp_world_get_aabbs(cp_world* w, double* aabb6);
+int cp_world_set_contacts(cp_world* w, int n, const int64_t* ints6, const uint64_t* magic2, const double* f21);
+int cp_world_get_contact_count(cp_world* w, int* n);
+int cp_world_get_contacts(cp_world* w, int cap, int64_t* ints6, uint64_t* magic2, double* f21, int* n);
+
+/* Island search root order for CP_PGS_REFERENCE_ORDER: body ids (index + 1)
+ * in the order PhysicsEngine::sleepOrSolveIslands visits its
+ * unordered_map (PhysicsEngine.cpp:1540).  cp_reference_node_order fills the
+ * order libstdc++ produces for ids 1..n inserted in sequence. */
+int cp_world_set_node_order(cp_world* w, int n, const int64_t* ids);
+int cp_reference_node_order(int n, int64_t* ids);
+
+/* One or more full steps.  Replaces PhysicsEngine::timeStep (PhysicsEngine.cpp:50-471). */
+int cp_world_step(cp_world* w, int n_steps);
+
+/* Stage entry points over the current state (parity seams of PhysicsEngine.cpp:91-335). */
+/* Broadphase candidate set (AABBTree::getAllCollisionCandidates, AABB_Tree.h:153-224):
+ * sorted (id_lo, id_hi) pairs. */
+int cp_broadphase(cp_world* w, int cap, int64_t* pairs2, int* n);
+/* Narrowphase of given (id_lo, id_hi) pairs: per pair the culled manifold
+ * (detectCollision + cull_manifold, CollisionDetect.cpp:57-152, :2795-2830).
+ * counts[n]; heads[n][4] = normal xyz, max depth; pts[n][4][4] = pos xyz, depth;
+ * magic[n][4][2]. */
+int cp_narrowphase(cp_world* w, int n, const int64_t* pairs2, int* counts, double* heads, double* pts, uint64_t* magic);
+/* Island rows of the last step in CP_PGS_REFERENCE_ORDER mode: (contact index
+ * in the post-narrowphase cache, kind 0 normal / 1 friction), and offsets. */
+int cp_world_last_islands(cp_world* w, int cap_rows, int64_t* rows2, int cap_offs, int64_t* offs, int* n_rows, int* n_islands);
+
+/* Timing of the last cp_world_step call, per phase, in milliseconds (HIP events). */
+int cp_world_last_timing(cp_world* w, double* ms8);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

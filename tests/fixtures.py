@@ -222,3 +222,20 @@ class Oracle:
         no = self.lib.orc_last_islands(self.w, rows.ctypes.data, offs.ctypes.data, cap)
         offs = offs[:no]
         return rows[:offs[-1]], offs
+
+
+# ----------------------------------------------------------------------------- product world from a fixture
+def world_from_fixture(fx, pgs_mode=1, **cfg):
+    """Device world holding exactly the reference scene of a fixture
+    (body-frame hulls, Gauss maps and mass properties as the reference stores them)."""
+    import constraintphysics_amd as cpa
+    m = fx.meta
+    kw = dict(dt=m["dt"], gravity=(0.0, m["gravity_y"], 0.0), vel_iters=int(m["vel_iters"]), pos_iters=int(m["pos_iters"]),
+              substeps=int(m["substeps"]), pgs_mode=pgs_mode)
+    kw.update(cfg)
+    w = cpa.World(**kw)
+    info, mp, mat = fx["body_info"], fx["mass_props"], fx["material"]
+    for b in range(int(m["n_bodies"])):
+        sid = w.add_hull_exact(fx.hull(b))
+        w.add_body(sid, int(info[b, 1]), mp[b], mat[b], 0)
+    return w
