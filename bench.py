@@ -1,0 +1,210 @@
+"""Benchmark: batched contact-pipeline steps on one MI355X.
+
+Workload (BASELINE metric "body-steps/s vs plain GPU path at stated FP64
+residual; p50/p99 step latency", batched-scenes configuration): independent
+scenes with body counts log-uniform in [8, 2048], each in its own fixed box
+container, seeded per scene (base + i); 10 PGS velocity sweeps, 1 position
+sweep, 8 substeps, dt = 1/120 s.  The primitive kinds drawn are the ones the
+narrowphase supports (reported in config.primitive_mix).
+
+Prints ONE JSON line.  `value` = dynamic bodies x timed steps / wall time of
+the timed steps, with every scene resident in HBM before timing starts.
+Extra fields: p50/p99 step latency, the plain single-scene GPU path on a
+sample of the same scenes (each scene its own world, stepped alone), the
+roofline of the dominant kernel (PGS sweeps) from HIP events on the library's
+stream, and the CPU baseline (the FP64 oracle, a bit-exact restatement of the
+reference, single thread, on a bounded sample of the same scenes).
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def pgs_bytes_per_contact(vel_iters, pos_iters):
+    """Algorithmic FP64 bytes one PGS launch moves per live contact point:
+    warm start (M^-1 J^T of 3 rows + 3 impulses), per velocity sweep the normal
+    row (J 12, M^-1 J^T 12, inverse, target, impulse r/w = 28 doubles) and the
+    friction pair (J 24, M^-1 J^T 24, inverse 4, targets 2, impulses r/w 4,
+    mu, normal impulse = 60 doubles), per position sweep the normal row again
+    (28 doubles)."""
+    return 8 * (27 + vel_iters * (28 + 60) + pos_iters * 28)
+
+
+def build_world(sizes, seed, pgs_mode=0, vel_iters=10, threads=256, scene_indices=None):
+    import constraintphysics_amd as cpa
+    from constraintphysics_amd import scenes
+    w = cpa.World(dt=1.0 / 120.0, gravity=(0.0, -6.0, 0.0), vel_iters=vel_iters, pos_iters=1, substeps=8, pgs_mode=pgs_mode,
+                  solver_threads=threads)
+    info = scenes.build_batched(w, sizes, seed, scene_indices=scene_indices)
+    return w, info
+
+
+def time_steps(w, k):
+    lat = []
+    for _ in range(k):
+        t0 = time.perf_counter()
+        w.step(1)
+        lat.append(time.perf_counter() - t0)
+    return np.array(lat)
+
+
+def cpu_baseline(sizes, seed, budget_s, warmup, steps):
+    """Oracle (single thread, FP64, reference order) on a bounded sample of the
+    scenes: the same initial state and warm-up as the GPU run, then timed steps."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from fixtures import Oracle  # the checker, used here only for the reported CPU baseline
+    from constraintphysics_amd import scenes
+    order = np.argsort(sizes)
+    picks = [int(order[int(q * (len(order) - 1))]) for q in (0.5, 0.3, 0.7, 0.1, 0.9)]
+    bodies, spent, used = 0, 0.0, []
+    t_all = time.perf_counter()
+    for s in picks:
+        desc = scenes.scene_description(sizes, seed, s)
+        o = Oracle.from_description(desc, dt=1.0 / 120.0, gravity_y=-6.0, vel_iters=10, pos_iters=1, substeps=8)
+        n_dyn = int(np.sum(desc["movement"] == 0))
+        for _ in range(warmup):
+            o.step()
+        t0 = time.perf_counter()
+        k = 0
+        while k < steps and time.perf_counter() - t0 < budget_s / 2:
+            o.step()
+            k += 1
+        dt = time.perf_counter() - t0
+        spent += dt
+        bodies += n_dyn * k
+        used.append((s, int(sizes[s]), k))
+        if spent > budget_s or time.perf_counter() - t_all > 4 * budget_s:
+            break
+    return bodies / spent, used, spent
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=40)
+    ap.add_argument("--scenes", type=int, default=4096)
+    ap.add_argument("--seed", type=int, default=20240)
+    ap.add_argument("--plain-scenes", type=int, default=16)
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only: --gpus {args.gpus} is not supported (scenes would shard by HBM footprint "
+              f"across GPUs with no per-step exchange; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from constraintphysics_amd import scenes
+    sizes = scenes.c3_sizes(args.scenes, args.seed)
+    t_build = time.perf_counter()
+    w, info = build_world(sizes, args.seed)
+    t_build = time.perf_counter() - t_build
+    n_dyn = int(info["dynamic_bodies"])
+    print(f"[bench] built {args.scenes} scenes, {n_dyn} dynamic bodies in {t_build:.1f}s", file=sys.stderr, flush=True)
+    for i in range(args.warmup):
+        w.step(1)
+        if (i + 1) % 10 == 0:
+            print(f"[bench] warmup {i + 1}/{args.warmup}", file=sys.stderr, flush=True)
+    lat = []
+    tim = []
+    for i in range(args.steps):
+        t0 = time.perf_counter()
+        w.step(1)
+        lat.append(time.perf_counter() - t0)
+        tim.append(w.last_timing())
+    lat = np.array(lat)
+    tim = np.array(tim)
+    total = float(lat.sum())
+    value = n_dyn * args.steps / total
+    ms_step = 1e3 * total / args.steps
+    pgs_ms_per_launch = float(tim[:, 5].sum() / max(1.0, tim[:, 6].sum()))
+    live = float(tim[:, 7].mean())
+    bpc = pgs_bytes_per_contact(10, 1)
+    achieved = live * bpc / (pgs_ms_per_launch * 1e-3) / 1e9 if pgs_ms_per_launch > 0 else 0.0
+    phases = dict(zip(["broadphase_ms", "narrowphase_ms", "persist_schedule_ms", "substeps_ms", "geometry_ms", "pgs_ms"],
+                      [round(float(x), 3) for x in tim[:, :6].mean(0)]))
+    print(f"[bench] timed {args.steps} steps: {ms_step:.2f} ms/step, {value:.0f} body-steps/s; phases {phases}", file=sys.stderr,
+          flush=True)
+    w.close()
+
+    plain = None
+    if not args.no_plain:
+        order = np.argsort(sizes)
+        pick = [int(order[int(q * (len(order) - 1))]) for q in np.linspace(0.02, 0.98, args.plain_scenes)]
+        pb, pt = 0, 0.0
+        for s in pick:
+            ws, inf = build_world(sizes, args.seed, scene_indices=[s])
+            for _ in range(args.warmup):
+                ws.step(1)
+            l = time_steps(ws, args.steps)
+            pb += int(inf["dynamic_bodies"]) * args.steps
+            pt += float(l.sum())
+            ws.close()
+        plain = dict(value=pb / pt, scenes=len(pick), note="each sampled scene stepped alone in its own world, same steps")
+        print(f"[bench] plain single-scene path: {plain['value']:.0f} body-steps/s", file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu:
+        v, used, spent = cpu_baseline(sizes, args.seed, args.cpu_budget, args.warmup, args.steps)
+        cpu = dict(value=round(v, 1), unit="body-steps/s", cores=1, kind="port",
+                   sample=f"FP64 oracle (bit-exact restatement of the reference step, single thread), same initial state and "
+                          f"{args.warmup} warm-up steps, timed steps on scenes (index, bodies, steps) {used}: {spent:.1f}s of CPU; "
+                          f"host has {os.cpu_count()} cores")
+        print(f"[bench] cpu baseline: {v:.0f} body-steps/s", file=sys.stderr, flush=True)
+
+    out = {
+        "metric": "body-steps/s vs plain GPU path at stated FP64 residual; p50/p99 step latency",
+        "value": round(value, 1),
+        "unit": "body-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "p50_step_ms": round(1e3 * float(np.percentile(lat, 50)), 3),
+        "p99_step_ms": round(1e3 * float(np.percentile(lat, 99)), 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"batched scenes: {args.scenes} scenes, bodies log-uniform [8, 2048] (seed {args.seed}, per-scene seed base+i), "
+                        f"box containers at 20k/60^3 bodies per m^3, v ~ N(0,1); dt 1/120, 8 substeps, PGS 10 vel + 1 pos sweeps, "
+                        f"graph-coloured PGS, FP64",
+            "scenes": args.scenes,
+            "dynamic_bodies": n_dyn,
+            "primitive_mix": info["mix"],
+            "phases_ms": phases,
+        },
+        "plain_gpu_path": plain,
+        "speedup_vs_plain": round(value / plain["value"], 2) if plain else None,
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_pgs_colored",
+            "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5),
+            "traffic": None,
+            "bytes_per_contact_per_launch": bpc,
+            "live_contacts": live,
+            "avg_launch_ms": round(pgs_ms_per_launch, 4),
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -59,6 +59,8 @@ def lib():
         "cp_shape_add_hull_exact": (_I, [_P, _I, _P, _I] + [_P] * 9 + [_I, _P, _I, _P, _P]),
         "cp_shape_add_polyhedron": (_I, [_P, _I, _P, _I, _P, _P, _P]),
         "cp_polyhedron_mass_properties": (_I, [_I, _P, _I, _P, _P, ctypes.c_double, _P]),
+        "cp_polyhedron_build": (_I, [_I, _P, _I, _P, _P] + [_P] * 8 + [_I, _P, _P, _P, _P]),
+        "cp_body_add_many": (_I, [_P, _I, _P, _P, _P, _P, _P, _P]),
         "cp_body_add": (_I, [_P, _I, _I, _P, _P, _I, _P]),
         "cp_world_body_count": (_I, [_P, _P]),
         "cp_world_set_state": (_I, [_P, _P, _P]),
@@ -137,6 +139,30 @@ def polyhedron_mass_properties(pts, faces, density):
     _chk(lib().cp_polyhedron_mass_properties(len(pts), pts.ctypes.data, len(faces), off.ctypes.data, fv.ctypes.data, float(density),
                                              out.ctypes.data), "cp_polyhedron_mass_properties")
     return out[0], out[1:10].reshape(3, 3), out[10:13]
+
+
+def polyhedron_build(pts, faces):
+    """Host-built polyhedron (surfaces, winding, Gauss map) in the exact-hull layout."""
+    pts = _a(pts, np.float64)
+    nf = len(faces)
+    off = np.zeros(nf + 1, np.int64)
+    off[1:] = np.cumsum([len(f) for f in faces])
+    fv = _a(np.concatenate([np.asarray(f) for f in faces]), np.int64)
+    fvo = np.zeros_like(fv)
+    sid, ncalc, ndir, gred = (np.zeros(nf, np.int64) for _ in range(4))
+    gn = np.zeros((nf, 3))
+    gint = np.zeros((nf, 3), np.int64)
+    gval = np.zeros((nf, 3))
+    cap = int(off[-1])
+    arcs = np.zeros((cap, 2), np.int64)
+    edges = np.zeros((cap, 2), np.int64)
+    na, ne = ctypes.c_int(), ctypes.c_int()
+    _chk(lib().cp_polyhedron_build(len(pts), pts.ctypes.data, nf, off.ctypes.data, fv.ctypes.data, fvo.ctypes.data, sid.ctypes.data,
+                                   ncalc.ctypes.data, ndir.ctypes.data, gn.ctypes.data, gred.ctypes.data, gint.ctypes.data,
+                                   gval.ctypes.data, cap, arcs.ctypes.data, ctypes.byref(na), edges.ctypes.data, ctypes.byref(ne)),
+         "cp_polyhedron_build")
+    return dict(pts=pts, face_off=off, face_verts=fvo, surface_id=sid, normal_calc=ncalc, normal_dir=ndir, gauss_normal=gn,
+                gauss_red=gred, gauss_ints=gint, gauss_vals=gval, arcs=arcs[:na.value], edges=edges[:ne.value])
 
 
 class World:

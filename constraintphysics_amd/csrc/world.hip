@@ -161,8 +161,10 @@ struct cp_world {
 	DArr<int> e_head, e_idx, e_first, e_cnt, e_live, e_color, e_order, color_off, ncolors, edge_range, applied, isl_rows, isl_off;
 	DArr<uint64_t> e_key;
 	int nedges = 0;
+	int live_contacts = 0;
 	std::vector<int64_t> last_rows, last_offs;
 	hipEvent_t ev[10];
+	hipEvent_t pgs_ev[2 * 64];
 	double last_ms[8] = {0};
 	bool timing = true;
 
@@ -409,6 +411,7 @@ void cp_world::persist(int npairs) {
 	HIPCHK(hipMemsetAsync(pt_cnt.p + npairs, 0, sizeof(int), stream));
 	scan(pt_cnt.p, pt_off.p, npairs + 1);
 	int nnew = read1(pt_off.p + npairs);
+	live_contacts = nnew;
 	ensure_points(nnew);
 	launch_np_gather(man, mcap, pairs.p, npairs, pt_off.p, np, npcap, stream);
 	int nold = ncontacts;
@@ -528,8 +531,11 @@ void cp_world::step() {
 		             stream);
 	}
 	if (timing) HIPCHK(hipEventRecord(ev[3], stream));
+	int npgs = 0;
 	for (int s = 0; s < cfg.substeps; s++) {
 		launch_build_rows(b, cb[cur].c, ncontacts, P, stream);
+		bool tev = timing && npgs < 64;
+		if (tev) HIPCHK(hipEventRecord(pgs_ev[2 * npgs], stream));
 		if (ordered) {
 			HIPCHK(hipMemsetAsync(dv.p, 0, 12 * (size_t)nb * sizeof(double), stream));
 			HIPCHK(hipMemsetAsync(applied.p, 0, (size_t)nb * sizeof(int), stream));
@@ -537,6 +543,10 @@ void cp_world::step() {
 		} else {
 			launch_pgs_colored(b, cb[cur].c, scene_body_off.p, nscenes, edge_range.p, e_order.p, color_off.p, ncolors.p, e_first.p, e_cnt.p,
 			                   P, cfg.solver_threads, stream);
+		}
+		if (tev) {
+			HIPCHK(hipEventRecord(pgs_ev[2 * npgs + 1], stream));
+			npgs++;
 		}
 		if (s == cfg.substeps - 1) launch_age(cb[cur].c, ncontacts, stream);
 		launch_integrate(b, P, stream);
@@ -551,6 +561,15 @@ void cp_world::step() {
 			HIPCHK(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
 			last_ms[i] = ms;
 		}
+		double pg = 0;
+		for (int i = 0; i < npgs; i++) {
+			float ms = 0;
+			HIPCHK(hipEventElapsedTime(&ms, pgs_ev[2 * i], pgs_ev[2 * i + 1]));
+			pg += ms;
+		}
+		last_ms[5] = pg;
+		last_ms[6] = npgs;
+		last_ms[7] = live_contacts;
 	}
 }
 
@@ -613,6 +632,7 @@ int cp_world_create(const cp_config* cfg, cp_world** out) {
 		HIPCHK(hipGetDevice(&w->device));
 		HIPCHK(hipStreamCreate(&w->stream));
 		for (auto& e : w->ev) HIPCHK(hipEventCreate(&e));
+		for (auto& e : w->pgs_ev) HIPCHK(hipEventCreate(&e));
 	});
 	if (st != CP_OK) {
 		delete w;
@@ -628,6 +648,7 @@ int cp_world_destroy(cp_world* w) {
 	(void)hipStreamSynchronize(w->stream);
 	w->free_all();
 	for (auto& e : w->ev) (void)hipEventDestroy(e);
+	for (auto& e : w->pgs_ev) (void)hipEventDestroy(e);
 	(void)hipStreamDestroy(w->stream);
 	delete w;
 	return CP_OK;
@@ -692,6 +713,39 @@ int cp_shape_add_polyhedron(cp_world* w, int n_pts, const double* pts, int n_fac
 	return CP_OK;
 }
 
+int cp_polyhedron_build(int n_pts, const double* pts, int n_faces, const int64_t* face_off, const int64_t* face_verts,
+                        int64_t* fvo, int64_t* sid, int64_t* ncalc, int64_t* ndir, double* gn, int64_t* gred, int64_t* gint, double* gval,
+                        int cap_pairs, int64_t* arcs_out, int* n_arcs, int64_t* edges_out, int* n_edges) {
+	if (n_pts < 4 || n_faces < 4 || !pts || !face_off || !face_verts || !fvo || !sid || !ncalc || !ndir || !gn || !gred || !gint || !gval ||
+	    !n_arcs || !n_edges)
+		return CP_ERR_INVALID;
+	std::vector<std::vector<int>> faces;
+	for (int f = 0; f < n_faces; f++) {
+		std::vector<int> fv;
+		for (int64_t k = face_off[f]; k < face_off[f + 1]; k++) {
+			if (face_verts[k] < 0 || face_verts[k] >= n_pts) return CP_ERR_INVALID;
+			fv.push_back((int)face_verts[k]);
+		}
+		if (fv.size() < 3) return CP_ERR_INVALID;
+		faces.push_back(fv);
+	}
+	HullData d = build_polyhedron(std::vector<double>(pts, pts + 3 * (size_t)n_pts), faces);
+	size_t o = 0;
+	for (int f = 0; f < n_faces; f++) {
+		for (int v : d.faces[f]) fvo[o++] = v;
+		sid[f] = d.sid[f]; ncalc[f] = d.ncalc[f]; ndir[f] = d.ndir[f]; gred[f] = d.red[f];
+		for (int c = 0; c < 3; c++) gn[3 * f + c] = d.gnorm[3 * f + c];
+		gint[3 * f] = d.minpid[f]; gint[3 * f + 1] = d.maxpid[f]; gint[3 * f + 2] = d.refidx[f];
+		gval[3 * f] = d.gmin[f]; gval[3 * f + 1] = d.gmax[f]; gval[3 * f + 2] = d.grefval[f];
+	}
+	*n_arcs = (int)d.arcs.size();
+	*n_edges = (int)d.edges.size();
+	if ((int)d.arcs.size() > cap_pairs || (int)d.edges.size() > cap_pairs || !arcs_out || !edges_out) return CP_ERR_CAPACITY;
+	for (size_t a = 0; a < d.arcs.size(); a++) { arcs_out[2 * a] = d.arcs[a].first; arcs_out[2 * a + 1] = d.arcs[a].second; }
+	for (size_t e = 0; e < d.edges.size(); e++) { edges_out[2 * e] = d.edges[e].first; edges_out[2 * e + 1] = d.edges[e].second; }
+	return CP_OK;
+}
+
 int cp_polyhedron_mass_properties(int n_pts, const double* pts, int n_faces, const int64_t* face_off, const int64_t* face_verts,
                                   double density, double* out13) {
 	if (n_pts < 4 || n_faces < 4 || !pts || !face_off || !face_verts || !out13 || !(density > 0)) return CP_ERR_INVALID;
@@ -720,6 +774,27 @@ int cp_body_add(cp_world* w, int shape_id, int movement, const double* mp, const
 	w->b_massprops.insert(w->b_massprops.end(), mp, mp + 19);
 	w->b_mat.insert(w->b_mat.end(), material, material + 4);
 	*body_index = w->nb++;
+	w->uploaded = false;
+	return CP_OK;
+}
+
+int cp_body_add_many(cp_world* w, int n, const int* shape_ids, const int* movement, const double* mp, const double* material,
+                     const int* scene, int* first_index) {
+	if (!w || n < 0 || !first_index || (n > 0 && (!shape_ids || !movement || !mp || !material || !scene))) return CP_ERR_INVALID;
+	if (w->uploaded && w->ncontacts > 0) return CP_ERR_STATE;
+	int last = w->b_scene.empty() ? -1 : w->b_scene.back();
+	for (int i = 0; i < n; i++) {
+		if (shape_ids[i] < 0 || shape_ids[i] >= (int)w->hs.type.size() || (movement[i] != 0 && movement[i] != 1)) return CP_ERR_INVALID;
+		if (scene[i] < last) return CP_ERR_INVALID;
+		last = scene[i];
+	}
+	*first_index = w->nb;
+	w->b_shape.insert(w->b_shape.end(), shape_ids, shape_ids + n);
+	w->b_move.insert(w->b_move.end(), movement, movement + n);
+	w->b_scene.insert(w->b_scene.end(), scene, scene + n);
+	w->b_massprops.insert(w->b_massprops.end(), mp, mp + 19 * (size_t)n);
+	w->b_mat.insert(w->b_mat.end(), material, material + 4 * (size_t)n);
+	w->nb += n;
 	w->uploaded = false;
 	return CP_OK;
 }

@@ -1,0 +1,287 @@
+"""Seeded synthetic scenes for the benchmark configurations.
+
+C0: unit cubes (density 1, restitution 0, friction 0.5) in an n x n x n grid
+    with 1.1 m spacing, U[-0.05, 0.05] jitter and random yaw, on a FIXED
+    ground box (-100,-1,-100)+(200,1,200) — the same generator (SplitMix64) as
+    the reference fixture harness, so scenes match body for body.
+C3: independent scenes with body counts log-uniform in [8, 2048], each in its
+    own FIXED box container sized to C1's body density (20k bodies in 60^3 m^3),
+    bodies placed without initial overlap, velocities N(0, 1) m/s.
+"""
+import math
+
+import numpy as np
+
+import constraintphysics_amd as cpa
+
+# ConvexUnionGeometry::box surface order (Geometry.cpp:14-37), corner-origin points.
+BOX_FACES = [[0, 3, 2, 1], [0, 1, 5, 4], [1, 2, 6, 5], [2, 3, 7, 6], [3, 0, 4, 7], [4, 5, 6, 7]]
+DEFAULT_MATERIAL = (1.0, 0.3, 0.6, 1.1)  # Material::default_material (ConvexPrimitive.h:18)
+C0_MATERIAL = (1.0, 0.0, 0.5, 0.5)
+
+
+class SplitMix64:
+    def __init__(self, seed):
+        self.s = seed & 0xFFFFFFFFFFFFFFFF
+
+    def next(self):
+        self.s = (self.s + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+        z = self.s
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+        return z ^ (z >> 31)
+
+    def uniform(self, a=0.0, b=1.0):
+        u = (self.next() >> 11) * (1.0 / 9007199254740992.0)
+        return a + (b - a) * u
+
+
+def box_points(dx, dy, dz):
+    """Corner-origin box points in the reference's vertex order, centred on the centre of mass."""
+    p = np.array([[0, 0, 0], [dx, 0, 0], [dx, dy, 0], [0, dy, 0], [0, 0, dz], [dx, 0, dz], [dx, dy, dz], [0, dy, dz]], float)
+    return p - np.array([dx, dy, dz]) / 2.0
+
+
+def inv3(m):
+    return np.linalg.inv(m)
+
+
+def mass_props_row(mass, tensor):
+    return np.concatenate([[mass], np.asarray(tensor).ravel(), inv3(tensor).ravel()])
+
+
+def quat_axis_angle(theta, axis):
+    axis = np.asarray(axis, float)
+    axis = axis / np.linalg.norm(axis)
+    s = math.sin(theta / 2.0)
+    return np.array([math.cos(theta / 2.0), axis[0] * s, axis[1] * s, axis[2] * s])
+
+
+class SceneBuilder:
+    """Collects shapes and bodies for one World; bodies are added scene by scene."""
+
+    def __init__(self, world):
+        self.w = world
+        self.shape_cache = {}
+        self.rows_shape, self.rows_move, self.rows_mp, self.rows_mat, self.rows_scene = [], [], [], [], []
+        self.state = []
+
+    def box_shape(self, dx, dy, dz, density):
+        key = ("box", dx, dy, dz, density)
+        if key not in self.shape_cache:
+            pts = box_points(dx, dy, dz)
+            sid = self.w.add_polyhedron(pts, BOX_FACES)
+            m, t, _ = cpa.polyhedron_mass_properties(pts, BOX_FACES, density)
+            self.shape_cache[key] = (sid, mass_props_row(m, t))
+        return self.shape_cache[key]
+
+    def hull_shape(self, pts, faces, density):
+        m, t, com = cpa.polyhedron_mass_properties(pts, faces, density)
+        pts = np.asarray(pts, float) - com
+        m, t, _ = cpa.polyhedron_mass_properties(pts, faces, density)
+        sid = self.w.add_polyhedron(pts, faces)
+        return sid, mass_props_row(m, t)
+
+    def add(self, shape, mp, movement, material, scene, com, q=(1, 0, 0, 0), vel=(0, 0, 0), angv=(0, 0, 0)):
+        self.add_many([shape], [mp], [movement], [material], [scene], [com], [q], [vel], [angv])
+
+    def add_many(self, shapes, mps, moves, mats, scenes, coms, qs, vels, angvs):
+        n = len(shapes)
+        self.rows_shape.append(np.asarray(shapes, np.int32).reshape(n))
+        self.rows_move.append(np.asarray(moves, np.int32).reshape(n))
+        self.rows_mp.append(np.asarray(mps, float).reshape(n, 19))
+        self.rows_mat.append(np.asarray(mats, float).reshape(n, 4))
+        self.rows_scene.append(np.asarray(scenes, np.int32).reshape(n))
+        self.state.append(np.concatenate([np.asarray(coms, float).reshape(n, 3), np.asarray(qs, float).reshape(n, 4),
+                                          np.asarray(vels, float).reshape(n, 3), np.asarray(angvs, float).reshape(n, 3)], axis=1))
+
+    def finish(self):
+        import ctypes
+        sh = np.ascontiguousarray(np.concatenate(self.rows_shape), np.int32)
+        n = len(sh)
+        mv = np.ascontiguousarray(np.concatenate(self.rows_move), np.int32)
+        mp = np.ascontiguousarray(np.concatenate(self.rows_mp))
+        mt = np.ascontiguousarray(np.concatenate(self.rows_mat))
+        sc = np.ascontiguousarray(np.concatenate(self.rows_scene), np.int32)
+        first = ctypes.c_int()
+        cpa._chk(self.w.L.cp_body_add_many(self.w.w, n, sh.ctypes.data, mv.ctypes.data, mp.ctypes.data, mt.ctypes.data, sc.ctypes.data,
+                                           ctypes.byref(first)), "cp_body_add_many")
+        self.w.n += n
+        st = np.ascontiguousarray(np.concatenate(self.state))
+        self.w.set_state(st)
+        return st
+
+
+def build_c0(world, n=10, seed=12345, scene=0, builder=None, offset=(0.0, 0.0, 0.0)):
+    """C0 pile: n^3 unit cubes on a ground box (same draws as oracle/ref_harness.cpp build_pile)."""
+    b = builder or SceneBuilder(world)
+    gs, gmp = b.box_shape(200.0, 1.0, 200.0, 1.0)
+    ox, oy, oz = offset
+    b.add(gs, gmp, 1, C0_MATERIAL, scene, np.array([0.0 + ox, -0.5 + oy, 0.0 + oz]))
+    cs, cmp_ = b.box_shape(1.0, 1.0, 1.0, 1.0)
+    rng = SplitMix64(seed)
+    for j in range(n):
+        for i in range(n):
+            for k in range(n):
+                cx = 1.1 * i + rng.uniform(-0.05, 0.05)
+                cy = 0.6 + 1.1 * j + rng.uniform(-0.05, 0.05)
+                cz = 1.1 * k + rng.uniform(-0.05, 0.05)
+                yaw = rng.uniform(0.0, 2.0 * 3.14159265358979323846)
+                b.add(cs, cmp_, 0, C0_MATERIAL, scene, np.array([cx + ox, cy + oy, cz + oz]), quat_axis_angle(yaw, (0, 1, 0)))
+    if builder is None:
+        return b.finish()
+    return None
+
+
+# ----------------------------------------------------------------------------- C1 / C3 mix
+C1_DENSITY = 20000.0 / 60.0 ** 3
+
+
+def random_hull(rng, n_min=8, n_max=32):
+    """Convex hull of 8..32 seeded points on an ellipsoid with semi-axes U[0.3, 0.8]."""
+    from scipy.spatial import ConvexHull
+    n = int(rng.integers(n_min, n_max + 1))
+    axes = rng.uniform(0.3, 0.8, 3)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    pts = d * axes
+    h = ConvexHull(pts)
+    used = np.unique(h.simplices)
+    remap = -np.ones(n, int)
+    remap[used] = np.arange(len(used))
+    faces = [list(remap[s]) for s in h.simplices]
+    return pts[used], faces
+
+
+def random_quat(rng):
+    q = rng.normal(size=4)
+    return q / np.linalg.norm(q)
+
+
+class HullLibrary:
+    """A seeded library of convex hull shapes (body frame, centre of mass at
+    the origin) instanced by many bodies; geometry is independent of any world."""
+
+    def __init__(self, count, seed, density=1.0):
+        rng = np.random.default_rng(seed)
+        self.items = []
+        for _ in range(count):
+            pts, faces = random_hull(rng)
+            m, t, com = cpa.polyhedron_mass_properties(pts, faces, density)
+            pts = np.asarray(pts, float) - com
+            m, t, _ = cpa.polyhedron_mass_properties(pts, faces, density)
+            r = float(np.max(np.linalg.norm(pts, axis=1))) + 0.05
+            self.items.append(dict(pts=pts, faces=faces, mp=mass_props_row(m, t), r=r))
+        self.r = np.array([it["r"] for it in self.items])
+        self.mp = np.array([it["mp"] for it in self.items])
+
+
+def add_container(b, scene, L, material=DEFAULT_MATERIAL, origin=(0.0, 0.0, 0.0)):
+    t = 1.0
+    ox, oy, oz = origin
+    walls = [((L + 2 * t, t, L + 2 * t), (L / 2, -t / 2, L / 2)), ((L + 2 * t, t, L + 2 * t), (L / 2, L + t / 2, L / 2)),
+             ((t, L, L), (-t / 2, L / 2, L / 2)), ((t, L, L), (L + t / 2, L / 2, L / 2)),
+             ((L, L, t), (L / 2, L / 2, -t / 2)), ((L, L, t), (L / 2, L / 2, L + t / 2))]
+    for (dx, dy, dz), c in walls:
+        s, mp = b.box_shape(dx, dy, dz, 1.0)
+        b.add(s, mp, 1, material, scene, np.array([c[0] + ox, c[1] + oy, c[2] + oz]))
+
+
+def place_bodies(rng, radii, L):
+    """Positions without overlap: distinct random cells of a lattice of pitch
+    2 * max radius, jittered uniformly inside each cell's slack."""
+    n = len(radii)
+    pitch = 2.0 * float(radii.max())
+    m = max(1, int(L // pitch))
+    while m ** 3 < n:
+        m += 1
+    cells = rng.choice(m ** 3, size=n, replace=False)
+    ijk = np.stack([cells // (m * m), (cells // m) % m, cells % m], axis=1).astype(float)
+    slack = (pitch / 2.0 - radii)[:, None]
+    return (ijk + 0.5) * pitch + rng.uniform(-1.0, 1.0, (n, 3)) * slack
+
+
+def c3_sizes(n_scenes, seed, lo=8, hi=2048):
+    rng = np.random.default_rng(seed)
+    return np.rint(np.exp(rng.uniform(math.log(lo), math.log(hi), n_scenes))).astype(int)
+
+
+_LIBS = {}
+
+
+def hull_library(seed, count=256):
+    key = (seed, count)
+    if key not in _LIBS:
+        _LIBS[key] = HullLibrary(count, seed ^ 0x5EED)
+    return _LIBS[key]
+
+
+def scene_bodies(sizes, seed, s, lib):
+    """Dynamic bodies of scene s: (library picks, container side, positions, orientations, velocities)."""
+    n = int(sizes[s])
+    rng = np.random.default_rng(seed + s)
+    picks = rng.integers(0, len(lib.items), n)
+    radii = lib.r[picks]
+    pitch = 2.0 * float(radii.max())
+    L = max((n / C1_DENSITY) ** (1.0 / 3.0), pitch * math.ceil(n ** (1.0 / 3.0)))
+    pos = place_bodies(rng, radii, L)
+    vel = rng.normal(size=(n, 3))
+    q = rng.normal(size=(n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    return picks, L, pos, q, vel
+
+
+CONTAINER_WALLS = 6
+
+
+def container_walls(L, t=1.0):
+    return [((L + 2 * t, t, L + 2 * t), (L / 2, -t / 2, L / 2)), ((L + 2 * t, t, L + 2 * t), (L / 2, L + t / 2, L / 2)),
+            ((t, L, L), (-t / 2, L / 2, L / 2)), ((t, L, L), (L + t / 2, L / 2, L / 2)),
+            ((L, L, t), (L / 2, L / 2, -t / 2)), ((L, L, t), (L / 2, L / 2, L + t / 2))]
+
+
+def build_batched(world, sizes, seed, scene_indices=None, hull_lib_size=256):
+    """Batched scenes (hull mix): scene_indices picks which scenes of the
+    seeded sequence to build (default all), each becoming one world scene."""
+    b = SceneBuilder(world)
+    lib = hull_library(seed, hull_lib_size)
+    lib_sid = np.array([world.add_polyhedron(it["pts"], it["faces"]) for it in lib.items])
+    idx = range(len(sizes)) if scene_indices is None else scene_indices
+    total = 0
+    for out_s, s in enumerate(idx):
+        picks, L, pos, q, vel = scene_bodies(sizes, seed, s, lib)
+        for (dx, dy, dz), c in container_walls(L):
+            sid, mp = b.box_shape(dx, dy, dz, 1.0)
+            b.add(sid, mp, 1, DEFAULT_MATERIAL, out_s, np.array(c))
+        n = len(picks)
+        b.add_many(lib_sid[picks], lib.mp[picks], np.zeros(n, np.int32), np.tile(DEFAULT_MATERIAL, (n, 1)), np.full(n, out_s), pos, q, vel,
+                   np.zeros((n, 3)))
+        total += n
+    b.finish()
+    return dict(dynamic_bodies=total, mix="convex hulls of 8-32 points on ellipsoids (256-shape seeded library) + box walls")
+
+
+def scene_description(sizes, seed, s, hull_lib_size=256):
+    """Scene s as plain arrays (for the CPU oracle): per body the hull in the
+    exact-hull layout, movement, mass properties, material, and the state."""
+    lib = hull_library(seed, hull_lib_size)
+    picks, L, pos, q, vel = scene_bodies(sizes, seed, s, lib)
+    hulls, move, mps, mats, state = [], [], [], [], []
+    cache = {}
+    for (dx, dy, dz), c in container_walls(L):
+        pts = box_points(dx, dy, dz)
+        hulls.append(cpa.polyhedron_build(pts, BOX_FACES))
+        m, t, _ = cpa.polyhedron_mass_properties(pts, BOX_FACES, 1.0)
+        move.append(1)
+        mps.append(mass_props_row(m, t))
+        mats.append(DEFAULT_MATERIAL)
+        state.append(np.concatenate([c, [1, 0, 0, 0], [0, 0, 0], [0, 0, 0]]))
+    for i, p in enumerate(picks):
+        if p not in cache:
+            cache[p] = cpa.polyhedron_build(lib.items[p]["pts"], lib.items[p]["faces"])
+        hulls.append(cache[p])
+        move.append(0)
+        mps.append(lib.mp[p])
+        mats.append(DEFAULT_MATERIAL)
+        state.append(np.concatenate([pos[i], q[i], vel[i], [0, 0, 0]]))
+    return dict(hulls=hulls, movement=np.array(move), mass_props=np.array(mps), material=np.array(mats), state=np.array(state))

@@ -83,6 +83,15 @@ int cp_shape_add_hull_exact(cp_world* w, int n_pts, const double* pts, int n_fac
  * are given in the body frame (centre of mass at the origin). */
 int cp_shape_add_polyhedron(cp_world* w, int n_pts, const double* pts, int n_faces, const int64_t* face_off,
                             const int64_t* face_verts, int* shape_id);
+/* Host-only: the polyhedron cp_shape_add_polyhedron would build, returned in
+ * the cp_shape_add_hull_exact layout.  Caller buffers: face_verts_out[total
+ * face vertices] (winding fixed), per face surface_id/normal_calc/normal_dir/
+ * gauss_red [n_faces], gauss_normal [3 n_faces], gauss_ints/gauss_vals
+ * [3 n_faces]; arcs_out/edges_out [2 * cap_pairs] with counts returned. */
+int cp_polyhedron_build(int n_pts, const double* pts, int n_faces, const int64_t* face_off, const int64_t* face_verts,
+                        int64_t* face_verts_out, int64_t* surface_id, int64_t* normal_calc, int64_t* normal_dir,
+                        double* gauss_normal, int64_t* gauss_red, int64_t* gauss_ints, double* gauss_vals, int cap_pairs,
+                        int64_t* arcs_out, int* n_arcs, int64_t* edges_out, int* n_edges);
 /* Mass properties of a closed polyhedron of uniform density:
  * out = mass, inertia tensor about the centre of mass [9], centre of mass [3]
  * (calculateMassProperties, RigidBody.cpp:427-640). */
@@ -95,6 +104,9 @@ int cp_polyhedron_mass_properties(int n_pts, const double* pts, int n_faces, con
  * restitution, kinetic and static friction (ConvexPrimitive.h:17-34). */
 int cp_body_add(cp_world* w, int shape_id, int movement, const double* mass_props, const double* material, int scene,
                 int* body_index);
+/* Bulk form of cp_body_add for n bodies (arrays of n, n, 19n, 4n, n). */
+int cp_body_add_many(cp_world* w, int n, const int* shape_ids, const int* movement, const double* mass_props,
+                     const double* material, const int* scene, int* first_index);
 int cp_world_body_count(cp_world* w, int* n);
 
 /* State transfer (host <-> HBM).  inertia (nullable): world tensor[9] and
@@ -130,7 +142,10 @@ int cp_narrowphase(cp_world* w, int n, const int64_t* pairs2, int* counts, doubl
  * in the post-narrowphase cache, kind 0 normal / 1 friction), and offsets. */
 int cp_world_last_islands(cp_world* w, int cap_rows, int64_t* rows2, int cap_offs, int64_t* offs, int* n_rows, int* n_islands);
 
-/* Timing of the last cp_world_step call, per phase, in milliseconds (HIP events). */
+/* Timing of the last cp_world_step call, per phase, in milliseconds (HIP events):
+ * [0] gravity + broadphase, [1] narrowphase, [2] persistence + schedule,
+ * [3] substeps (rows + PGS + integrate), [4] geometry refresh,
+ * [5] PGS kernels only (sum over substeps), [6] PGS launches, [7] live contacts. */
 int cp_world_last_timing(cp_world* w, double* ms8);
 
 #ifdef __cplusplus

@@ -134,28 +134,40 @@ def _c(a, dt):
 class Oracle:
     """FP64 CPU restatement of the reference step, loaded with a fixture's scene."""
 
-    def __init__(self, fx, cfm=0.0000025, hardness=1000.0, life=6):
+    def __init__(self, fx=None, cfm=0.0000025, hardness=1000.0, life=6, desc=None, dt=None, gravity_y=None, vel_iters=None,
+                 pos_iters=None, substeps=None):
         lib = oracle_lib()
         self.lib = lib
-        m = fx.meta
-        cfg = OrcConfig(m["dt"], (ctypes.c_double * 3)(0.0, m["gravity_y"], 0.0), int(m["vel_iters"]), int(m["pos_iters"]),
-                        int(m["substeps"]), cfm, hardness, life)
+        if fx is not None:
+            m = fx.meta
+            dt, gravity_y, vel_iters, pos_iters, substeps = m["dt"], m["gravity_y"], m["vel_iters"], m["pos_iters"], m["substeps"]
+            n = int(m["n_bodies"])
+            movement = fx["body_info"][:, 1]
+            mp, mat = fx["mass_props"], fx["material"]
+            hulls = [fx.hull(b) for b in range(n)]
+        else:
+            n = len(desc["hulls"])
+            movement, mp, mat, hulls = desc["movement"], desc["mass_props"], desc["material"], desc["hulls"]
+        cfg = OrcConfig(dt, (ctypes.c_double * 3)(0.0, gravity_y, 0.0), int(vel_iters), int(pos_iters), int(substeps), cfm, hardness, life)
         self.w = ctypes.c_void_p(lib.orc_create(ctypes.byref(cfg)))
-        self.n = int(m["n_bodies"])
-        info = fx["body_info"]
-        mp = fx["mass_props"]
-        mat = fx["material"]
+        self.n = n
         self._keep = []
         for b in range(self.n):
-            h = fx.hull(b)
+            h = hulls[b]
             arrs = [_c(mp[b], np.float64), _c(mat[b], np.float64), _c(h["pts"], np.float64), _c(h["face_off"], np.int64),
                     _c(h["face_verts"], np.int64), _c(h["surface_id"], np.int64), _c(h["normal_calc"], np.int64),
                     _c(h["normal_dir"], np.int64), _c(h["gauss_normal"], np.float64), _c(h["gauss_red"], np.int64),
                     _c(h["gauss_ints"], np.int64), _c(h["gauss_vals"], np.float64), _c(h["arcs"], np.int64), _c(h["edges"], np.int64)]
             self._keep.append(arrs)
-            lib.orc_add_hull_body(self.w, int(info[b, 1]), arrs[0].ctypes.data, arrs[1].ctypes.data, len(h["pts"]), arrs[2].ctypes.data,
+            lib.orc_add_hull_body(self.w, int(movement[b]), arrs[0].ctypes.data, arrs[1].ctypes.data, len(h["pts"]), arrs[2].ctypes.data,
                                   len(h["face_off"]) - 1, *[a.ctypes.data for a in arrs[3:12]], len(h["arcs"]), arrs[12].ctypes.data,
                                   len(h["edges"]), arrs[13].ctypes.data)
+
+    @classmethod
+    def from_description(cls, desc, **kw):
+        o = cls(desc=desc, **kw)
+        o.load(desc["state"])
+        return o
 
     def __del__(self):
         try:
