@@ -34,6 +34,8 @@ class Config(ctypes.Structure):
         ("gyro_ticks", ctypes.c_int),
         ("pgs_mode", ctypes.c_int),
         ("solver_threads", ctypes.c_int),
+        ("diagnostics", ctypes.c_int),
+        ("defect_fixes", ctypes.c_int),
     ]
 
 
@@ -58,6 +60,7 @@ def lib():
         "cp_world_set_config": (_I, [_P, _P]),
         "cp_shape_add_hull_exact": (_I, [_P, _I, _P, _I] + [_P] * 9 + [_I, _P, _I, _P, _P]),
         "cp_shape_add_polyhedron": (_I, [_P, _I, _P, _I, _P, _P, _P]),
+        "cp_shape_add_sphere": (_I, [_P, _P, ctypes.c_double, _P]),
         "cp_polyhedron_mass_properties": (_I, [_I, _P, _I, _P, _P, ctypes.c_double, _P]),
         "cp_polyhedron_build": (_I, [_I, _P, _I, _P, _P] + [_P] * 8 + [_I, _P, _P, _P, _P]),
         "cp_body_add_many": (_I, [_P, _I, _P, _P, _P, _P, _P, _P]),
@@ -76,6 +79,7 @@ def lib():
         "cp_narrowphase": (_I, [_P, _I, _P, _P, _P, _P, _P]),
         "cp_world_last_islands": (_I, [_P, _I, _P, _I, _P, _P, _P]),
         "cp_world_last_timing": (_I, [_P, _P]),
+        "cp_world_last_residual": (_I, [_P, _P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -204,6 +208,12 @@ class World:
              "cp_shape_add_hull_exact")
         return sid.value
 
+    def add_sphere(self, center, radius):
+        c = _a(center, np.float64)
+        sid = ctypes.c_int()
+        _chk(self.L.cp_shape_add_sphere(self.w, c.ctypes.data, float(radius), ctypes.byref(sid)), "cp_shape_add_sphere")
+        return sid.value
+
     def add_polyhedron(self, pts, faces):
         pts = _a(pts, np.float64)
         off = np.zeros(len(faces) + 1, np.int64)
@@ -297,6 +307,11 @@ class World:
         _chk(self.L.cp_world_last_islands(self.w, nr.value, rows.ctypes.data, ni.value + 1, offs.ctypes.data, ctypes.byref(nr),
                                           ctypes.byref(ni)), "cp_world_last_islands")
         return rows[:nr.value], offs
+
+    def last_residual(self):
+        r = np.zeros(4)
+        _chk(self.L.cp_world_last_residual(self.w, r.ctypes.data), "cp_world_last_residual")
+        return r
 
     def last_timing(self):
         ms = np.zeros(8)

@@ -62,6 +62,7 @@ struct Params {
 	double cull_c[4], cull_s[4];  // cos/sin(2*pi*i/4), host libm values
 	int vel_iters, pos_iters, substeps, contact_life;
 	int gyro_ticks;
+	int defect_fixes;
 };
 
 // Narrowphase output, one slot per candidate pair.
@@ -127,6 +128,7 @@ void launch_edge_build(const Contacts& C, int n, const int* head, const int* eid
 void launch_color(const Bodies& b, const int* scene_body_off, int nscenes, const uint64_t* ekey, const int* elive, int nedges,
                   int* ecolor, int* order, int* color_off, int* ncolors, int* edge_range, hipStream_t s);
 void launch_build_rows(const Bodies& b, const Contacts& C, int nc, const Params& P, hipStream_t s);
+void launch_pgs_residual(const Bodies& b, const Contacts& C, int nc, double* acc, hipStream_t s);
 void launch_pgs_ordered(const Bodies& b, const Contacts& C, const int* rows, const int* isl_off, int nisl, const Params& P, int* applied,
                         hipStream_t s);
 void launch_pgs_colored(const Bodies& b, const Contacts& C, const int* scene_body_off, int nscenes, const int* edge_range, const int* order,

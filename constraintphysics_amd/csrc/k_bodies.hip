@@ -148,10 +148,15 @@ __global__ void k_refresh_geometry(Bodies b, Shapes sh, int only_moved) {
 			wp[0] = w.x; wp[1] = w.y; wp[2] = w.z;
 		}
 	} else if (sh.type[s] == SHAPE_SPHERE) {
-		// Sphere::gen_AABB (ConvexPrimitive.cpp:439-443): centre = com for a single-sphere body.
+		// Sphere::recomputeFromReference and gen_AABB (ConvexPrimitive.cpp:433-443)
+		const double* rp = sh.ref_pts + 3 * sh.pt_off[s];
+		V3 c = com + R * V3{rp[0], rp[1], rp[2]};
+		double* wp = b.wpts + 3 * b.wpt_off[i];
+		wp[0] = c.x; wp[1] = c.y; wp[2] = c.z;
 		double r = sh.radius[s];
-		mn0 = com.x - r; mn1 = com.y - r; mn2 = com.z - r;
-		mx0 = com.x + r; mx1 = com.y + r; mx2 = com.z + r;
+		V3 mn = c - V3{r, r, r}, mx = c + V3{r, r, r};
+		mn0 = mn.x; mn1 = mn.y; mn2 = mn.z;
+		mx0 = mx.x; mx1 = mx.y; mx2 = mx.z;
 	}
 	b.aabb[i] = mn0; b.aabb[n + i] = mn1; b.aabb[2 * n + i] = mn2;
 	b.aabb[3 * n + i] = mx0; b.aabb[4 * n + i] = mx1; b.aabb[5 * n + i] = mx2;

@@ -418,6 +418,147 @@ __device__ void hull_contacts(const Hull& A, const Hull& B, const Check& mp, con
 	}
 }
 
+// ------------------------------------------------------------- spheres
+__device__ __forceinline__ Ext sphere_extrema(V3 c, double r, V3 dir) {  // CollisionDetect.cpp:646-656
+	double cv = dot(dir, c);
+	return Ext{-1, -1, cv - r, cv + r};
+}
+
+// Candidate axis t of SAT_PolySphere (CollisionDetect.cpp:1336-1385): A's
+// non-redundant face normals, then vertex directions, then edge normals.
+__device__ Check eval_axis_sphere(const Hull& A, V3 c, double r, int t, bool* valid, uint32_t* feat) {
+	*valid = true;
+	if (t < A.nf) {
+		if (A.red[t]) { *valid = false; return Check{}; }
+		V3 n = ldv(A.wn, t);
+		*feat = (uint32_t)(A.np + t);
+		return sat_check(recentered(A, t), sphere_extrema(c, r, n), n);
+	}
+	t -= A.nf;
+	if (t < A.np) {
+		V3 n = normalize(ldv(A.wp, t) - c);
+		*feat = (uint32_t)t;
+		return sat_check(find_extrema(A, n), sphere_extrema(c, r, n), n);
+	}
+	t -= A.np;
+	int i1 = A.edges[2 * t], i2 = A.edges[2 * t + 1];
+	V3 e1 = ldv(A.wp, i1), e2 = ldv(A.wp, i2);
+	V3 dir = normalize(e2 - e1);
+	V3 sample = e1 - c;
+	V3 n = normalize(sample - dir * dot(dir, sample));
+	*feat = edge_id((uint16_t)i1, (uint16_t)i2);
+	return sat_check(find_extrema(A, n), sphere_extrema(c, r, n), n);
+}
+
+__device__ __forceinline__ uint64_t swap_halves(uint64_t v) {  // swapOrder (CollisionDetect.cpp:28-40)
+	uint64_t lo = v & 0x00000000FFFFFFFFull, hi = (v & 0xFFFFFFFF00000000ull) >> 32;
+	return (0x00000000FFFFFFFFull & hi) + (0xFFFFFFFF00000000ull & (lo << 32));
+}
+
+__device__ void write_mat(const Bodies& b, int ia, int ib, Manifolds& M, int slot, int cap) {  // CollisionDetect.cpp:820-824
+	double ra = b.mat[ia], rb = b.mat[ib];
+	M.mat[slot] = (ra < rb) ? rb : ra;
+	M.mat[cap + slot] = (b.mat[b.n + ia] + b.mat[b.n + ib]) / 2.0;
+	M.mat[2 * cap + slot] = (b.mat[2 * b.n + ia] + b.mat[2 * b.n + ib]) / 2.0;
+}
+
+// Wave-parallel SAT_PolySphere; poly = body ip, sphere = body is_.  When the
+// sphere is the pair's first body the manifold is flipped as detectCollision
+// does (CollisionDetect.cpp:91-99): normal negated, MagicID halves swapped.
+__device__ void poly_sphere(const Bodies& b, const Shapes& sh, int ip, int is_, bool swapped, int lane, int slot, Manifolds& M, int cap) {
+	Hull A = make_hull(b, sh, ip);
+	V3 c = ldv(b.wpts + 3 * b.wpt_off[is_], 0);
+	double r = sh.radius[b.shape[is_]];
+	int naxes = A.nf + A.np + A.ne;
+	double bp = __builtin_inf();
+	int bi = 0x7FFFFFFF;
+	int sep = 0;
+	for (int base = 0; base < naxes; base += 64) {
+		int t = base + lane;
+		if (t < naxes) {
+			bool valid;
+			uint32_t f;
+			Check x = eval_axis_sphere(A, c, r, t, &valid, &f);
+			if (valid) {
+				if (x.pen < 0) sep = 1;
+				if (x.pen < bp) { bp = x.pen; bi = t; }
+			}
+		}
+		if (__ballot(sep) != 0) break;
+	}
+	if (__ballot(sep) != 0) return;
+	for (int off = 32; off >= 1; off >>= 1) {
+		double obp = __shfl_xor(bp, off);
+		int obi = __shfl_xor(bi, off);
+		if (obp < bp || (obp == bp && obi < bi)) { bp = obp; bi = obi; }
+	}
+	if (!(bp > 0) || bi == 0x7FFFFFFF || lane != 0) return;
+	bool valid;
+	uint32_t feat = 0;
+	Check mp = eval_axis_sphere(A, c, r, bi, &valid, &feat);
+	uint64_t kp = (uint64_t)(ip + 1) * 16, ks = (uint64_t)(is_ + 1) * 16;
+	uint64_t cid = (0x00000000FFFFFFFFull & kp) | (0xFFFFFFFF00000000ull & (ks << 32));
+	uint64_t bid = (uint64_t)feat;
+	V3 n = mp.n;
+	V3 pos = c - n * r;
+	if (swapped) {
+		n = -n;
+		cid = swap_halves(cid);
+		bid = swap_halves(bid);
+	}
+	M.normal[slot] = n.x; M.normal[cap + slot] = n.y; M.normal[2 * cap + slot] = n.z;
+	double* mpts = M.pts + (size_t)slot * 16;
+	mpts[0] = pos.x; mpts[1] = pos.y; mpts[2] = pos.z; mpts[3] = mp.pen;
+	M.magic[(size_t)slot * 8] = cid;
+	M.magic[(size_t)slot * 8 + 1] = bid;
+	M.maxpen[slot] = mp.pen;
+	M.count[slot] = 1;
+	write_mat(b, ip, is_, M, slot, cap);
+}
+
+// detectSphereSphere (CollisionDetect.cpp:837-866), lane 0 only.
+__device__ void sphere_sphere(const Bodies& b, const Shapes& sh, int ia, int ib, int slot, Manifolds& M, int cap) {
+	V3 ca = ldv(b.wpts + 3 * b.wpt_off[ia], 0), cb = ldv(b.wpts + 3 * b.wpt_off[ib], 0);
+	double ra = sh.radius[b.shape[ia]], rb = sh.radius[b.shape[ib]];
+	V3 diff = cb - ca;
+	double d = mag(diff);
+	double pen = ra + rb - d;
+	if (!(pen > 0)) return;
+	V3 n = normalize(diff);
+	V3 pos = ca + n * ra;
+	uint64_t ka = (uint64_t)(ia + 1) * 16, kb = (uint64_t)(ib + 1) * 16;
+	M.normal[slot] = n.x; M.normal[cap + slot] = n.y; M.normal[2 * cap + slot] = n.z;
+	double* mpts = M.pts + (size_t)slot * 16;
+	mpts[0] = pos.x; mpts[1] = pos.y; mpts[2] = pos.z; mpts[3] = pen;
+	M.magic[(size_t)slot * 8] = (0x00000000FFFFFFFFull & ka) | (0xFFFFFFFF00000000ull & (kb << 32));
+	M.magic[(size_t)slot * 8 + 1] = 0;
+	M.maxpen[slot] = pen;
+	M.count[slot] = 1;
+	write_mat(b, ia, ib, M, slot, cap);
+}
+
+// Removes non-finite points from a written manifold slot (lane 0); a
+// non-finite normal drops the whole manifold.
+__device__ void sanitize_slot(Manifolds& M, int slot, int cap) {
+	int n = M.count[slot];
+	if (n <= 0) return;
+	double nx = M.normal[slot], ny = M.normal[cap + slot], nz = M.normal[2 * cap + slot];
+	if (!isfinite(nx) || !isfinite(ny) || !isfinite(nz)) { M.count[slot] = 0; return; }
+	double* p = M.pts + (size_t)slot * 16;
+	uint64_t* g = M.magic + (size_t)slot * 8;
+	int o = 0;
+	for (int k = 0; k < n; k++) {
+		if (!(isfinite(p[4 * k]) && isfinite(p[4 * k + 1]) && isfinite(p[4 * k + 2]) && isfinite(p[4 * k + 3]))) continue;
+		if (o != k) {
+			for (int c = 0; c < 4; c++) p[4 * o + c] = p[4 * k + c];
+			g[2 * o] = g[2 * k];
+			g[2 * o + 1] = g[2 * k + 1];
+		}
+		o++;
+	}
+	M.count[slot] = o;
+}
+
 // Grid-stride over pair slots, one wave per pair.
 __global__ __launch_bounds__(64 * WAVES_PER_BLOCK) void k_narrowphase(Bodies b, Shapes sh, Params P, const uint64_t* pairs, int npairs,
                                                                       Manifolds M, int cap) {
@@ -434,19 +575,42 @@ __global__ __launch_bounds__(64 * WAVES_PER_BLOCK) void k_narrowphase(Bodies b, 
 		}
 		// PhysicsEngine.cpp:174: at least one dynamic body
 		if (b.movement[ia] != 0 && b.movement[ib] != 0) continue;
-		int sa = b.shape[ia], sbb = b.shape[ib];
-		if (sh.type[sa] != SHAPE_HULL || sh.type[sbb] != SHAPE_HULL) continue;
+		int ta = sh.type[b.shape[ia]], tb = sh.type[b.shape[ib]];
+		if (ta == SHAPE_HULL && tb == SHAPE_SPHERE) {
+			poly_sphere(b, sh, ia, ib, false, lane, slot, M, cap);
+			if (lane == 0 && (P.defect_fixes & 1)) sanitize_slot(M, slot, cap);
+			continue;
+		}
+		if (ta == SHAPE_SPHERE && tb == SHAPE_HULL) {
+			poly_sphere(b, sh, ib, ia, true, lane, slot, M, cap);
+			if (lane == 0 && (P.defect_fixes & 1)) sanitize_slot(M, slot, cap);
+			continue;
+		}
+		if (ta == SHAPE_SPHERE && tb == SHAPE_SPHERE) {
+			if (lane == 0) sphere_sphere(b, sh, ia, ib, slot, M, cap);
+			if (lane == 0 && (P.defect_fixes & 1)) sanitize_slot(M, slot, cap);
+			continue;
+		}
+		if (ta != SHAPE_HULL || tb != SHAPE_HULL) continue;
 		Hull A = make_hull(b, sh, ia), B = make_hull(b, sh, ib);
+		// Stages in the reference's axis order (A faces, B faces, then arc pairs in
+		// chunks of one axis per lane); a separating axis anywhere means "no
+		// contact", so the wave stops at the first chunk that finds one.
 		int naxes = A.nf + B.nf + A.na * B.na;
 		double bp = __builtin_inf();
 		int bi = 0x7FFFFFFF;
 		int sep = 0;
-		for (int t = lane; t < naxes; t += 64) {
-			bool valid;
-			Check x = eval_axis(A, B, t, &valid);
-			if (!valid) continue;
-			if (x.pen < 0) sep = 1;
-			if (x.pen < bp) { bp = x.pen; bi = t; }
+		for (int base = 0; base < naxes; base += 64) {
+			int t = base + lane;
+			if (t < naxes) {
+				bool valid;
+				Check x = eval_axis(A, B, t, &valid);
+				if (valid) {
+					if (x.pen < 0) sep = 1;
+					if (x.pen < bp) { bp = x.pen; bi = t; }
+				}
+			}
+			if (__ballot(sep) != 0) break;
 		}
 		if (__ballot(sep) != 0) continue;
 		for (int off = 32; off >= 1; off >>= 1) {
@@ -469,6 +633,7 @@ __global__ __launch_bounds__(64 * WAVES_PER_BLOCK) void k_narrowphase(Bodies b, 
 			M.normal[cap + slot] = mp.n.y;
 			M.normal[2 * cap + slot] = mp.n.z;
 			hull_contacts(A, B, mp, P, cid, lds[wib], slot, M);
+			if (P.defect_fixes & 1) sanitize_slot(M, slot, cap);
 		}
 	}
 }

@@ -410,7 +410,62 @@ __global__ void k_pgs_colored(Bodies b, Contacts C, const int* scene_body_off, c
 		if (b.movement[i] == 0) apply_dv(b, i);
 }
 
+// FP64 natural-map residual of the velocity rows after the sweeps; acc[0] +=
+// sum r^2, acc[1] += sum lambda^2, acc[2] += rows.  Reads the final velocity
+// deltas (b.dv) of the solve, so it runs right after the PGS launch.
+__global__ void k_pgs_residual(Bodies b, Contacts C, int nc, double* acc) {
+	int k = blockIdx.x * blockDim.x + threadIdx.x;
+	double r2 = 0, l2 = 0, rows = 0;
+	if (k < nc && C.live[k]) {
+		const int K = C.cap, N = b.n;
+		int i1 = (int)(C.key[k] >> 32), i2 = (int)(C.key[k] & 0xFFFFFFFFu);
+		double x = 0, y = 0;
+		for (int q = 0; q < 6; q++) {
+			x += b.dv[q * N + i1] * C.ja[q * K + k];
+			y += b.dv[q * N + i2] * C.jb[q * K + k];
+		}
+		double ln = C.imp_n[k];
+		double z = ln + C.inv_n[k] * (C.tgt_n[k] - (x + y));
+		double rn = ln - (z < 0 ? 0 : z);
+		double cur[2];
+		for (int r = 0; r < 2; r++) {
+			double a = 0, c = 0;
+			for (int q = 0; q < 6; q++) {
+				a += b.dv[q * N + i1] * C.fja[(r * 6 + q) * K + k];
+				c += b.dv[q * N + i2] * C.fjb[(r * 6 + q) * K + k];
+			}
+			cur[r] = a + c;
+		}
+		double d0 = C.ftgt[k] - cur[0], d1 = C.ftgt[K + k] - cur[1];
+		double t0 = C.imp_t[k], t1 = C.imp_t[K + k];
+		double z0 = t0 + C.finv[k] * d0 + C.finv[K + k] * d1;
+		double z1 = t1 + C.finv[2 * K + k] * d0 + C.finv[3 * K + k] * d1;
+		double lim = C.mu[k] * (ln > 0 ? ln : 0);
+		double m = sqrt(z0 * z0 + z1 * z1);
+		if (m > lim && m > 0) { z0 *= lim / m; z1 *= lim / m; }
+		double r0 = t0 - z0, r1 = t1 - z1;
+		r2 = rn * rn + r0 * r0 + r1 * r1;
+		l2 = ln * ln + t0 * t0 + t1 * t1;
+		rows = 3;
+	}
+	// wave reduction, one atomic per wave
+	for (int off = 32; off >= 1; off >>= 1) {
+		r2 += __shfl_xor(r2, off);
+		l2 += __shfl_xor(l2, off);
+		rows += __shfl_xor(rows, off);
+	}
+	if ((threadIdx.x & 63) == 0 && rows > 0) {
+		atomicAdd(&acc[0], r2);
+		atomicAdd(&acc[1], l2);
+		atomicAdd(&acc[2], rows);
+	}
+}
+
 static inline int nblk(int n, int t) { return (n + t - 1) / t; }
+
+void launch_pgs_residual(const Bodies& b, const Contacts& C, int nc, double* acc, hipStream_t s) {
+	if (nc) hipLaunchKernelGGL(k_pgs_residual, dim3(nblk(nc, 256)), dim3(256), 0, s, b, C, nc, acc);
+}
 
 void launch_build_rows(const Bodies& b, const Contacts& C, int nc, const Params& P, hipStream_t s) {
 	if (nc) hipLaunchKernelGGL(k_build_rows, dim3(nblk(nc, 64)), dim3(64), 0, s, b, C, nc, P);

@@ -58,10 +58,14 @@ int HostShapes::add_hull(const HullData& d) {
 	return id;
 }
 
-int HostShapes::add_sphere(double r) {
+int HostShapes::add_sphere(const double* center, double r) {
 	int id = (int)type.size();
 	type.push_back(1);
-	pt_off.push_back(pt_off.back());
+	// the centre in the body frame is the shape's single reference point
+	ref_pts.insert(ref_pts.end(), center, center + 3);
+	pt_off.push_back(pt_off.back() + 1);
+	vaf_off.push_back((int)vaf.size());
+	vae_off.push_back((int)vae.size());
 	face_off.push_back(face_off.back());
 	arc_off.push_back(arc_off.back());
 	edge_off.push_back(edge_off.back());

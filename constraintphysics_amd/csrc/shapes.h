@@ -28,7 +28,7 @@ struct HostShapes {
 	std::vector<double> radius, height;
 
 	int add_hull(const HullData& d);
-	int add_sphere(double r);
+	int add_sphere(const double* center, double r);
 };
 
 // Polyhedron(points, surface_vertex_indices) (ConvexPrimitive.cpp:582-658)

@@ -162,6 +162,8 @@ struct cp_world {
 	DArr<uint64_t> e_key;
 	int nedges = 0;
 	int live_contacts = 0;
+	DArr<double> resid_acc;
+	double last_resid[4] = {0, 0, 0, 0};
 	std::vector<int64_t> last_rows, last_offs;
 	hipEvent_t ev[10];
 	hipEvent_t pgs_ev[2 * 64];
@@ -216,6 +218,7 @@ struct cp_world {
 		p.substeps = cfg.substeps;
 		p.contact_life = cfg.contact_life;
 		p.gyro_ticks = cfg.gyro_ticks;
+		p.defect_fixes = cfg.defect_fixes;
 		return p;
 	}
 
@@ -229,6 +232,7 @@ struct cp_world {
 	void persist(int npairs);
 	void edges();
 	void schedule_reference_order();
+	void record_colored_schedule();
 	void step();
 };
 
@@ -242,6 +246,7 @@ void cp_world::free_all() {
 	DArr<double>* dbl[] = {&s_ref_pts, &s_gnorm, &s_gmin, &s_gmax, &s_grefval, &s_radius, &s_height, &com, &q, &vel, &angv, &pvel, &pang,
 	                       &prev_com, &prev_q, &I, &Iinv, &Iref, &Irefinv, &mass, &mat, &aabb, &wpts, &wnorm, &dv, &bp_sb};
 	for (auto* a : dbl) a->free_();
+	resid_acc.free_();
 	DArr<uint64_t>* u64s[] = {&bp_keys, &bp_keys2, &pairs, &pairs2, &nk, &e_key};
 	for (auto* a : u64s) a->free_();
 	tmp.free_();
@@ -275,7 +280,7 @@ void cp_world::upload() {
 		for (int c = 0; c < 3; c++) mt[c * (size_t)n + i] = b_mat[4 * (size_t)i + 1 + c];
 		int s = b_shape[i];
 		bool hull = h.type[s] == SHAPE_HULL;
-		wpo[i + 1] = wpo[i] + (hull ? h.pt_off[s + 1] - h.pt_off[s] : 0);
+		wpo[i + 1] = wpo[i] + (h.pt_off[s + 1] - h.pt_off[s]);
 		wgo[i + 1] = wgo[i] + (hull ? h.face_off[s + 1] - h.face_off[s] : 0);
 	}
 	up(Iref, Ir); up(Irefinv, Iri); up(mass, ms); up(mat, mt);
@@ -507,6 +512,37 @@ void cp_world::schedule_reference_order() {
 	up(isl_off, offs);
 }
 
+// The colour schedule as one sequential row list per scene (colours in order,
+// edges of a colour in order, contacts of an edge in cache order, normal row
+// then friction row): edges of one colour share no dynamic body, so the
+// parallel sweep equals this sequential one.
+void cp_world::record_colored_schedule() {
+	std::vector<int> order, co, nc, er, ef, ecnt, live;
+	down(order, e_order.p, nedges);
+	down(co, color_off.p, (size_t)nscenes * (CP_MAXC + 2));
+	down(nc, ncolors.p, nscenes);
+	down(er, edge_range.p, 2 * (size_t)nscenes);
+	down(ef, e_first.p, nedges);
+	down(ecnt, e_cnt.p, nedges);
+	down(live, cb[cur].c.live, ncontacts);
+	last_rows.clear();
+	last_offs.assign(1, 0);
+	for (int s = 0; s < nscenes; s++) {
+		for (int c = 0; c < nc[s]; c++) {
+			const int* o = &co[(size_t)s * (CP_MAXC + 2)];
+			for (int x = o[c]; x < o[c + 1]; x++) {
+				int e = order[er[2 * s] + x];
+				for (int k = ef[e]; k < ef[e] + ecnt[e]; k++) {
+					if (!live[k]) continue;
+					last_rows.push_back(k); last_rows.push_back(0);
+					last_rows.push_back(k); last_rows.push_back(1);
+				}
+			}
+		}
+		last_offs.push_back((int64_t)last_rows.size() / 2);
+	}
+}
+
 void cp_world::step() {
 	Params P = params();
 	Bodies b = bodies();
@@ -529,6 +565,7 @@ void cp_world::step() {
 		edge_range.reserve(2 * (size_t)nscenes);
 		launch_color(b, scene_body_off.p, nscenes, e_key.p, e_live.p, nedges, e_color.p, e_order.p, color_off.p, ncolors.p, edge_range.p,
 		             stream);
+		if (cfg.diagnostics & 2) record_colored_schedule();
 	}
 	if (timing) HIPCHK(hipEventRecord(ev[3], stream));
 	int npgs = 0;
@@ -548,11 +585,24 @@ void cp_world::step() {
 			HIPCHK(hipEventRecord(pgs_ev[2 * npgs + 1], stream));
 			npgs++;
 		}
+		if (s == cfg.substeps - 1 && (cfg.diagnostics & 1)) {
+			resid_acc.reserve(4);
+			HIPCHK(hipMemsetAsync(resid_acc.p, 0, 4 * sizeof(double), stream));
+			launch_pgs_residual(b, cb[cur].c, ncontacts, resid_acc.p, stream);
+		}
 		if (s == cfg.substeps - 1) launch_age(cb[cur].c, ncontacts, stream);
 		launch_integrate(b, P, stream);
 	}
 	if (timing) HIPCHK(hipEventRecord(ev[4], stream));
 	launch_refresh_geometry(b, shapes(), 1, stream);
+	if (cfg.diagnostics & 1) {
+		double h[4];
+		HIPCHK(hipMemcpy(h, resid_acc.p, 3 * sizeof(double), hipMemcpyDeviceToHost));
+		last_resid[0] = std::sqrt(h[0]);
+		last_resid[1] = std::sqrt(h[1]);
+		last_resid[2] = h[1] > 0 ? std::sqrt(h[0] / h[1]) : 0.0;
+		last_resid[3] = h[2];
+	}
 	if (timing) {
 		HIPCHK(hipEventRecord(ev[5], stream));
 		HIPCHK(hipEventSynchronize(ev[5]));
@@ -602,6 +652,8 @@ void cp_config_default(cp_config* c) {
 	c->gyro_ticks = 4;
 	c->pgs_mode = CP_PGS_COLORED;
 	c->solver_threads = 256;
+	c->diagnostics = 0;
+	c->defect_fixes = 1;
 }
 
 const char* cp_status_string(int s) {
@@ -690,6 +742,13 @@ int cp_shape_add_hull_exact(cp_world* w, int n_pts, const double* pts, int n_fac
 	for (int a = 0; a < n_arcs; a++) d.arcs.push_back({(int)arcs[2 * a], (int)arcs[2 * a + 1]});
 	for (int e = 0; e < n_edges; e++) d.edges.push_back({(int)edges[2 * e], (int)edges[2 * e + 1]});
 	*shape_id = w->hs.add_hull(d);
+	w->uploaded = false;
+	return CP_OK;
+}
+
+int cp_shape_add_sphere(cp_world* w, const double* center, double radius, int* shape_id) {
+	if (!w || !center || !shape_id || !(radius > 0)) return CP_ERR_INVALID;
+	*shape_id = w->hs.add_sphere(center, radius);
 	w->uploaded = false;
 	return CP_OK;
 }
@@ -1079,6 +1138,13 @@ int cp_world_last_islands(cp_world* w, int cap_rows, int64_t* rows2, int cap_off
 	if (nr > cap_rows || no > cap_offs || !rows2 || !offs) return CP_ERR_CAPACITY;
 	memcpy(rows2, w->last_rows.data(), w->last_rows.size() * sizeof(int64_t));
 	memcpy(offs, w->last_offs.data(), w->last_offs.size() * sizeof(int64_t));
+	return CP_OK;
+}
+
+int cp_world_last_residual(cp_world* w, double* out) {
+	if (!w || !out) return CP_ERR_INVALID;
+	if (!(w->cfg.diagnostics & 1)) return CP_ERR_STATE;
+	for (int i = 0; i < 4; i++) out[i] = w->last_resid[i];
 	return CP_OK;
 }
 

@@ -208,6 +208,13 @@ def c3_sizes(n_scenes, seed, lo=8, hi=2048):
 
 _LIBS = {}
 
+SPHERE_PI = 3.1415926535  # the reference's PI macro (Quaternion.h:4), used for sphere mass (RigidBody.cpp:556)
+
+# Primitive mix of the mixed-primitive configuration: 40% spheres, 30% cylinders,
+# 30% hulls.  Kinds the narrowphase does not support yet are left out and the
+# remaining weights renormalised (reported in the bench line).
+MIX = (("sphere", 0.4), ("hull", 0.3))
+
 
 def hull_library(seed, count=256):
     key = (seed, count)
@@ -216,19 +223,33 @@ def hull_library(seed, count=256):
     return _LIBS[key]
 
 
-def scene_bodies(sizes, seed, s, lib):
-    """Dynamic bodies of scene s: (library picks, container side, positions, orientations, velocities)."""
+def sphere_library(seed, count=256, density=1.0):
+    """Seeded sphere radii U[0.3, 0.7] with mass properties (RigidBody.cpp:552-563)."""
+    rng = np.random.default_rng(seed ^ 0x5FE4E)
+    r = rng.uniform(0.3, 0.7, count)
+    m = (4.0 / 3.0) * SPHERE_PI * r * r * r * density
+    k = 2.0 / 5.0 * m * r * r
+    mp = np.array([mass_props_row(mi, np.eye(3) * ki) for mi, ki in zip(m, k)])
+    return r, mp
+
+
+def scene_bodies(sizes, seed, s, lib, sph_r):
+    """Dynamic bodies of scene s: kind (0 hull, 1 sphere), library pick,
+    container side, positions, orientations, velocities."""
     n = int(sizes[s])
     rng = np.random.default_rng(seed + s)
+    w = np.array([m[1] for m in MIX])
+    kinds = rng.choice(len(MIX), size=n, p=w / w.sum())
+    kind = np.array([0 if MIX[k][0] == "hull" else 1 for k in kinds])
     picks = rng.integers(0, len(lib.items), n)
-    radii = lib.r[picks]
+    radii = np.where(kind == 0, lib.r[picks], sph_r[picks] + 0.05)
     pitch = 2.0 * float(radii.max())
     L = max((n / C1_DENSITY) ** (1.0 / 3.0), pitch * math.ceil(n ** (1.0 / 3.0)))
     pos = place_bodies(rng, radii, L)
     vel = rng.normal(size=(n, 3))
     q = rng.normal(size=(n, 4))
     q /= np.linalg.norm(q, axis=1, keepdims=True)
-    return picks, L, pos, q, vel
+    return kind, picks, L, pos, q, vel
 
 
 CONTAINER_WALLS = 6
@@ -240,32 +261,44 @@ def container_walls(L, t=1.0):
             ((L, L, t), (L / 2, L / 2, -t / 2)), ((L, L, t), (L / 2, L / 2, L + t / 2))]
 
 
+def mix_description():
+    tot = sum(m[1] for m in MIX)
+    parts = ", ".join(f"{100 * m[1] / tot:.0f}% {m[0]}s" for m in MIX)
+    return (f"{parts} (cylinders pending); hulls: 8-32 points on ellipsoids, semi-axes U[0.3,0.8] (256-shape seeded library); "
+            f"spheres r~U[0.3,0.7] (256-radius seeded library); box walls")
+
+
 def build_batched(world, sizes, seed, scene_indices=None, hull_lib_size=256):
-    """Batched scenes (hull mix): scene_indices picks which scenes of the
-    seeded sequence to build (default all), each becoming one world scene."""
+    """Batched scenes of the mixed-primitive distribution: scene_indices picks
+    which scenes of the seeded sequence to build (default all), each becoming
+    one world scene."""
     b = SceneBuilder(world)
     lib = hull_library(seed, hull_lib_size)
+    sph_r, sph_mp = sphere_library(seed, hull_lib_size)
     lib_sid = np.array([world.add_polyhedron(it["pts"], it["faces"]) for it in lib.items])
+    sph_sid = np.array([world.add_sphere((0.0, 0.0, 0.0), r) for r in sph_r])
     idx = range(len(sizes)) if scene_indices is None else scene_indices
     total = 0
     for out_s, s in enumerate(idx):
-        picks, L, pos, q, vel = scene_bodies(sizes, seed, s, lib)
+        kind, picks, L, pos, q, vel = scene_bodies(sizes, seed, s, lib, sph_r)
         for (dx, dy, dz), c in container_walls(L):
             sid, mp = b.box_shape(dx, dy, dz, 1.0)
             b.add(sid, mp, 1, DEFAULT_MATERIAL, out_s, np.array(c))
         n = len(picks)
-        b.add_many(lib_sid[picks], lib.mp[picks], np.zeros(n, np.int32), np.tile(DEFAULT_MATERIAL, (n, 1)), np.full(n, out_s), pos, q, vel,
-                   np.zeros((n, 3)))
+        shp = np.where(kind == 0, lib_sid[picks], sph_sid[picks])
+        mps = np.where((kind == 0)[:, None], lib.mp[picks], sph_mp[picks])
+        b.add_many(shp, mps, np.zeros(n, np.int32), np.tile(DEFAULT_MATERIAL, (n, 1)), np.full(n, out_s), pos, q, vel, np.zeros((n, 3)))
         total += n
     b.finish()
-    return dict(dynamic_bodies=total, mix="convex hulls of 8-32 points on ellipsoids (256-shape seeded library) + box walls")
+    return dict(dynamic_bodies=total, mix=mix_description())
 
 
 def scene_description(sizes, seed, s, hull_lib_size=256):
     """Scene s as plain arrays (for the CPU oracle): per body the hull in the
     exact-hull layout, movement, mass properties, material, and the state."""
     lib = hull_library(seed, hull_lib_size)
-    picks, L, pos, q, vel = scene_bodies(sizes, seed, s, lib)
+    sph_r, sph_mp = sphere_library(seed, hull_lib_size)
+    kind, picks, L, pos, q, vel = scene_bodies(sizes, seed, s, lib, sph_r)
     hulls, move, mps, mats, state = [], [], [], [], []
     cache = {}
     for (dx, dy, dz), c in container_walls(L):
@@ -277,6 +310,13 @@ def scene_description(sizes, seed, s, hull_lib_size=256):
         mats.append(DEFAULT_MATERIAL)
         state.append(np.concatenate([c, [1, 0, 0, 0], [0, 0, 0], [0, 0, 0]]))
     for i, p in enumerate(picks):
+        if kind[i] == 1:
+            hulls.append(dict(sphere=np.array([0.0, 0.0, 0.0, sph_r[p]])))
+            move.append(0)
+            mps.append(sph_mp[p])
+            mats.append(DEFAULT_MATERIAL)
+            state.append(np.concatenate([pos[i], q[i], vel[i], [0, 0, 0]]))
+            continue
         if p not in cache:
             cache[p] = cpa.polyhedron_build(lib.items[p]["pts"], lib.items[p]["faces"])
         hulls.append(cache[p])

@@ -53,6 +53,11 @@ typedef struct {
 	int gyro_ticks;          /* setAngleVelUpdateTickCount */
 	int pgs_mode;            /* cp_pgs_mode */
 	int solver_threads;      /* threads per scene workgroup in colored mode (64..1024) */
+	int diagnostics;         /* bit 0: measure the FP64 PGS residual of every step's last substep;
+	                            bit 1: record the coloured sweep schedule (cp_world_last_islands) */
+	int defect_fixes;        /* bit 0: drop non-finite manifold points (the reference's clip
+	                            intersection divides by n.d, which rounds to 0 for an edge parallel to
+	                            the clipping edge, CollisionDetect.cpp:444-467); 0 = bug-compatible */
 } cp_config;
 
 typedef struct cp_world cp_world;
@@ -77,6 +82,9 @@ int cp_shape_add_hull_exact(cp_world* w, int n_pts, const double* pts, int n_fac
                             const int64_t* normal_dir, const double* gauss_normal, const int64_t* gauss_red,
                             const int64_t* gauss_ints, const double* gauss_vals, int n_arcs, const int64_t* arcs,
                             int n_edges, const int64_t* edges, int* shape_id);
+/* Sphere (Sphere(center, radius), ConvexPrimitive.h:76-102): centre in the body
+ * frame (the reference keeps the primitive's offset from the centre of mass). */
+int cp_shape_add_sphere(cp_world* w, const double* center, double radius, int* shape_id);
 /* Convex polyhedron from points and face vertex lists (Polyhedron(points,
  * surface_vertex_indices), ConvexPrimitive.h:216): builds surfaces, edges and
  * the Gauss map on the host the way the reference constructor does.  Points
@@ -138,9 +146,17 @@ int cp_broadphase(cp_world* w, int cap, int64_t* pairs2, int* n);
  * counts[n]; heads[n][4] = normal xyz, max depth; pts[n][4][4] = pos xyz, depth;
  * magic[n][4][2]. */
 int cp_narrowphase(cp_world* w, int n, const int64_t* pairs2, int* counts, double* heads, double* pts, uint64_t* magic);
-/* Island rows of the last step in CP_PGS_REFERENCE_ORDER mode: (contact index
- * in the post-narrowphase cache, kind 0 normal / 1 friction), and offsets. */
+/* Sweep schedule of the last step: in CP_PGS_REFERENCE_ORDER mode the island
+ * rows; in CP_PGS_COLORED mode (with diagnostics bit 1) one row list per scene
+ * in colour order.  Rows are (contact index in the post-narrowphase cache,
+ * kind 0 normal / 1 friction), with group offsets. */
 int cp_world_last_islands(cp_world* w, int cap_rows, int64_t* rows2, int cap_offs, int64_t* offs, int* n_rows, int* n_islands);
+
+/* FP64 residual of the last substep's velocity solve (needs diagnostics bit 0):
+ * out[0] = ||r|| over all live rows of the natural map r = lambda - proj(lambda +
+ * A^-1 (target - J dv)) (contact: lambda >= 0; friction: disk of radius
+ * mu * lambda_n), out[1] = ||lambda||, out[2] = out[0] / out[1], out[3] = rows. */
+int cp_world_last_residual(cp_world* w, double* out4);
 
 /* Timing of the last cp_world_step call, per phase, in milliseconds (HIP events):
  * [0] gravity + broadphase, [1] narrowphase, [2] persistence + schedule,

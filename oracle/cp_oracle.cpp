@@ -221,6 +221,10 @@ struct Body {
 	V3 com, vel, ang_vel, pvel, pang;
 	Q q;
 	Hull hull;
+	int type = 0;        // 0 polyhedron, 1 sphere
+	V3 sph_ref;          // sphere centre in the body frame (Sphere::recomputeFromReference)
+	double radius = 0;
+	V3 sph_c;            // world centre
 	double aabb[6];
 	V3 prev_com;
 	Q prev_q;
@@ -242,6 +246,14 @@ static V3 faceNormal(const Hull& h, int f) {
 // (RigidBody.cpp:281-296, ConvexPrimitive.cpp:660-676, :715-729).
 static void worldGeometry(Body& b) {
 	M3 R = b.q.rot();
+	if (b.type == 1) {  // Sphere::recomputeFromReference / gen_AABB (ConvexPrimitive.cpp:433-443)
+		b.sph_c = b.com + R * b.sph_ref;
+		V3 r(b.radius, b.radius, b.radius);
+		V3 mn = b.sph_c - r, mx = b.sph_c + r;
+		b.aabb[0] = mn.x; b.aabb[1] = mn.y; b.aabb[2] = mn.z;
+		b.aabb[3] = mx.x; b.aabb[4] = mx.y; b.aabb[5] = mx.z;
+		return;
+	}
 	Hull& h = b.hull;
 	h.pts.resize(h.ref_pts.size());
 	for (size_t i = 0; i < h.ref_pts.size(); i++) h.pts[i] = b.com + R * h.ref_pts[i];
@@ -549,6 +561,84 @@ static Manifold satPolyPoly(const Body& A, uint64_t a_key, const Body& B, uint64
 	}
 	out.max_pen = mp.pen;
 	return out;
+}
+
+static Extrema sphereExtrema(const Body& s, V3 dir) {  // CollisionDetect.cpp:646-656
+	Extrema e;
+	double cv = dir.dot(s.sph_c);
+	e.max_val = cv + s.radius;
+	e.min_val = cv - s.radius;
+	e.min_pid = -1;
+	e.max_pid = -1;
+	return e;
+}
+
+// detectSphereSphere (CollisionDetect.cpp:837-866).
+static Manifold sphereSphere(const Body& A, uint64_t a_key, const Body& B, uint64_t b_key) {
+	Manifold out;
+	V3 diff = B.sph_c - A.sph_c;
+	double d = diff.mag();
+	out.max_pen = A.radius + B.radius - d;
+	if (out.max_pen < 0) return out;
+	out.n = diff.normalize();
+	CP c;
+	c.pos = A.sph_c + out.n * A.radius;
+	c.pen = out.max_pen;
+	c.cid = (0x00000000FFFFFFFFull & a_key) | (0xFFFFFFFF00000000ull & (b_key << 32));
+	c.bid = 0;
+	c.restitution = std::max<double>(A.restitution, B.restitution);
+	c.mu_k = (A.mu_k + B.mu_k) / 2.0;
+	c.mu_s = (A.mu_s + B.mu_s) / 2.0;
+	out.pts.push_back(c);
+	return out;
+}
+
+// SAT_PolySphere (CollisionDetect.cpp:1336-1411).
+static Manifold satPolySphere(const Body& A, uint64_t a_key, const Body& B, uint64_t b_key) {
+	Manifold out;
+	out.max_pen = -1;
+	const Hull& a = A.hull;
+	CheckNorm mp{-1, -1, V3(), std::numeric_limits<double>::infinity()};
+	uint32_t feat = 0;
+	uint32_t sid = (uint32_t)a.pts.size();
+	for (size_t g = 0; g < a.wnorm.size(); g++, sid++) {
+		if (a.gred[g]) continue;
+		CheckNorm x = satCheck(recentered(a, (int)g), sphereExtrema(B, a.wnorm[g]), a.wnorm[g]);
+		if (x.pen < 0) { out.max_pen = -1; return out; }
+		if (x.pen < mp.pen) { feat = sid; mp = x; }
+	}
+	for (size_t p = 0; p < a.pts.size(); p++) {
+		V3 n = (a.pts[p] - B.sph_c).normalize();
+		CheckNorm x = satCheck(findExtrema(a, n), sphereExtrema(B, n), n);
+		if (x.pen < 0) { out.max_pen = -1; return out; }
+		if (x.pen < mp.pen) { mp = x; feat = (uint32_t)p; }
+	}
+	for (auto& e : a.edges) {
+		V3 e1 = a.pts[e.first], e2 = a.pts[e.second];
+		V3 dir = (e2 - e1).normalize();
+		V3 sample = e1 - B.sph_c;
+		V3 n = (sample - dir * dir.dot(sample)).normalize();
+		CheckNorm x = satCheck(findExtrema(a, n), sphereExtrema(B, n), n);
+		if (x.pen < 0) { out.max_pen = -1; return out; }
+		if (x.pen < mp.pen) { mp = x; feat = edgeID((uint16_t)e.first, (uint16_t)e.second); }
+	}
+	out.n = mp.n;
+	CP c;
+	c.pos = B.sph_c - out.n * B.radius;
+	c.pen = mp.pen;
+	c.restitution = std::max<double>(A.restitution, B.restitution);
+	c.mu_k = (A.mu_k + B.mu_k) / 2.0;
+	c.mu_s = (A.mu_s + B.mu_s) / 2.0;
+	c.cid = (0x00000000FFFFFFFFull & a_key) | (0xFFFFFFFF00000000ull & (b_key << 32));
+	c.bid = feat;
+	out.pts.push_back(c);
+	out.max_pen = mp.pen;
+	return out;
+}
+
+static uint64_t swapHalves(uint64_t v) {  // swapOrder (CollisionDetect.cpp:28-40)
+	uint64_t lo = v & 0x00000000FFFFFFFFull, hi = (v & 0xFFFFFFFF00000000ull) >> 32;
+	return (0x00000000FFFFFFFFull & hi) + (0xFFFFFFFF00000000ull & (lo << 32));
 }
 
 // cull_manifold (CollisionDetect.cpp:2795-2830).
@@ -867,10 +957,30 @@ static std::vector<std::pair<int, int>> broadphase(const World& W) {
 
 static uint64_t primKey(const Body& b) { return (uint64_t)b.id * 16; }
 
+// detectCollision dispatch (CollisionDetect.cpp:57-152).
+static Manifold narrowRaw(const World& W, int i, int j);
 static Manifold narrow(const World& W, int i, int j) {
+	Manifold m = narrowRaw(W, i, j);
+	if (W.cfg.drop_nonfinite && !m.pts.empty()) {
+		if (!std::isfinite(m.n.x) || !std::isfinite(m.n.y) || !std::isfinite(m.n.z)) return Manifold();
+		std::vector<CP> keep;
+		for (const CP& c : m.pts)
+			if (std::isfinite(c.pos.x) && std::isfinite(c.pos.y) && std::isfinite(c.pos.z) && std::isfinite(c.pen)) keep.push_back(c);
+		m.pts = keep;
+	}
+	return m;
+}
+static Manifold narrowRaw(const World& W, int i, int j) {
 	const Body& A = W.bodies[i];
 	const Body& B = W.bodies[j];
-	Manifold m = satPolyPoly(A, primKey(A), B, primKey(B));
+	Manifold m;
+	if (A.type == 0 && B.type == 0) m = satPolyPoly(A, primKey(A), B, primKey(B));
+	else if (A.type == 0 && B.type == 1) m = satPolySphere(A, primKey(A), B, primKey(B));
+	else if (A.type == 1 && B.type == 0) {
+		m = satPolySphere(B, primKey(B), A, primKey(A));
+		m.n = -m.n;
+		for (CP& c : m.pts) { c.cid = swapHalves(c.cid); c.bid = swapHalves(c.bid); }
+	} else m = sphereSphere(A, primKey(A), B, primKey(B));
 	if (!(m.max_pen > 0)) return Manifold();
 	return cullManifold(m, 4);
 }
@@ -919,7 +1029,8 @@ static void sortCache(World& W) {
 	});
 }
 
-static void step(World& W, const int64_t* node_order) {
+static void step(World& W, const int64_t* node_order, const int64_t* sched_rows = nullptr, const int64_t* sched_offs = nullptr,
+                 int sched_n = 0) {
 	const int N = (int)W.bodies.size();
 	double dt = W.cfg.dt;
 	V3 g(W.cfg.gravity[0], W.cfg.gravity[1], W.cfg.gravity[2]);
@@ -957,7 +1068,14 @@ static void step(World& W, const int64_t* node_order) {
 	std::vector<std::vector<std::pair<Contact*, int>>> islands;
 	W.last_rows.clear();
 	W.last_offs.assign(1, 0);
-	{
+	if (sched_rows) {
+		// explicit sweep order over the post-narrowphase cache (one row list per group)
+		for (int g = 0; g < sched_n; g++) {
+			std::vector<std::pair<Contact*, int>> rows;
+			for (int64_t r = sched_offs[g]; r < sched_offs[g + 1]; r++) rows.push_back({&W.cache[sched_rows[2 * r]], (int)sched_rows[2 * r + 1]});
+			if (!rows.empty()) islands.push_back(rows);
+		}
+	} else {
 		std::vector<char> visited(N, 0);
 		std::vector<int> roots;
 		if (node_order) for (int k = 0; k < N; k++) roots.push_back((int)node_order[k] - 1);
@@ -1118,6 +1236,26 @@ int orc_add_hull_body(orc_world* w, int movement, const double* mp, const double
 	return (int)w->bodies.size() - 1;
 }
 
+int orc_add_sphere_body(orc_world* w, int movement, const double* mp, const double* mat, const double* center, double radius) {
+	Body b{};
+	b.id = (int)w->bodies.size() + 1;
+	b.movement = movement;
+	b.type = 1;
+	b.mass = mp[0];
+	for (int r = 0; r < 3; r++)
+		for (int c = 0; c < 3; c++) {
+			b.I_ref.v[r][c] = mp[1 + 3 * r + c];
+			b.I_ref_inv.v[r][c] = mp[10 + 3 * r + c];
+		}
+	b.restitution = mat[1];
+	b.mu_k = mat[2];
+	b.mu_s = mat[3];
+	b.sph_ref = V3(center[0], center[1], center[2]);
+	b.radius = radius;
+	w->bodies.push_back(b);
+	return (int)w->bodies.size() - 1;
+}
+
 void orc_set_state(orc_world* w, const double* s) {
 	for (size_t i = 0; i < w->bodies.size(); i++) {
 		Body& b = w->bodies[i];
@@ -1259,6 +1397,8 @@ int orc_narrowphase(const orc_world* w, int id_lo, int id_hi, double* head, doub
 }
 
 void orc_step(orc_world* w, const int64_t* node_order) { step(*w, node_order); }
+
+void orc_step_scheduled(orc_world* w, int n_groups, const int64_t* rows, const int64_t* offs) { step(*w, nullptr, rows, offs, n_groups); }
 
 int orc_last_islands(const orc_world* w, int64_t* rows, int64_t* offs, int cap) {
 	int n = (int)w->last_rows.size() / 2;

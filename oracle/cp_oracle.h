@@ -26,6 +26,7 @@ typedef struct {
 	double cfm;           // global constraint-force mixing
 	double contact_hardness;
 	int contact_life;
+	int drop_nonfinite;   // 1: drop non-finite manifold points (the product's defect fix); 0: as the reference
 } orc_config;
 
 orc_world* orc_create(const orc_config* cfg);
@@ -43,6 +44,10 @@ int orc_add_hull_body(orc_world* w, int movement, const double* mass_props, cons
                       const int64_t* surface_id, const int64_t* normal_calc, const int64_t* normal_dir,
                       const double* gauss_normal, const int64_t* gauss_red, const int64_t* gauss_ints, const double* gauss_vals,
                       int n_arcs, const int64_t* arcs, int n_edges, const int64_t* edges);
+
+// Sphere body: centre in the body frame and radius (Sphere, ConvexPrimitive.h:76-102).
+int orc_add_sphere_body(orc_world* w, int movement, const double* mass_props, const double* material, const double* center,
+                        double radius);
 
 // Body state rows of 13 doubles: com 3, q (r,i,j,k), vel 3, ang_vel 3.
 void orc_set_state(orc_world* w, const double* state);
@@ -74,6 +79,11 @@ int orc_narrowphase(const orc_world* w, int id_lo, int id_hi, double* head, doub
 // no sleeping, no actions, contacts only).  node_order: body ids in the order
 // the island search visits roots (PhysicsEngine.cpp:1540); NULL = ascending id.
 void orc_step(orc_world* w, const int64_t* node_order);
+
+// One full step with an explicit sweep schedule: n_groups row lists
+// (rows[2*r] = index into the post-narrowphase cache, rows[2*r+1] = 0 normal /
+// 1 friction; offs[n_groups+1]), solved as PGS_solve solves one island each.
+void orc_step_scheduled(orc_world* w, int n_groups, const int64_t* rows, const int64_t* offs);
 
 // Island rows of the last step: (cache index into the post-narrowphase cache, kind 0 normal / 1 friction).
 int orc_last_islands(const orc_world* w, int64_t* rows, int64_t* offs, int cap);

@@ -176,6 +176,34 @@ static void build_pile(PhysicsEngine& E, int nx, int ny, int nz, uint64_t seed, 
 			}
 }
 
+// Mixed pile: like build_pile but every other body is a sphere of radius
+// U[0.3, 0.5] (draw after the yaw), so sphere-sphere, sphere-box, box-box and
+// body-ground pairs all occur.
+static void build_mixed(PhysicsEngine& E, int nx, int ny, int nz, uint64_t seed, double spacing, double jitter) {
+	phyz::Material m = c0_material();
+	RigidBody* g = E.createRigidBody(phyz::ConvexUnionGeometry::box(mthz::Vec3(-100, -1, -100), 200, 1, 200, m), RigidBody::FIXED);
+	pin_body(g);
+	SplitMix64 rng(seed);
+	int idx = 0;
+	for (int j = 0; j < ny; j++)
+		for (int i = 0; i < nx; i++)
+			for (int k = 0; k < nz; k++) {
+				double cx = spacing * i + rng.uniform(-jitter, jitter);
+				double cy = 0.6 + spacing * j + rng.uniform(-jitter, jitter);
+				double cz = spacing * k + rng.uniform(-jitter, jitter);
+				double yaw = rng.uniform(0.0, 2.0 * 3.14159265358979323846);
+				double r = rng.uniform(0.3, 0.5);
+				RigidBody* b;
+				if ((idx++ & 1) == 0) {
+					b = E.createRigidBody(phyz::ConvexUnionGeometry::box(mthz::Vec3(cx - 0.5, cy - 0.5, cz - 0.5), 1, 1, 1, m));
+					b->setOrientation(mthz::Quaternion(yaw, mthz::Vec3(0, 1, 0)));
+				} else {
+					b = E.createRigidBody(phyz::ConvexUnionGeometry::sphere(mthz::Vec3(cx, cy, cz), r, m));
+				}
+				pin_body(b);
+			}
+}
+
 // ---------------------------------------------------------------------------
 // Snapshot helpers.
 
@@ -322,7 +350,7 @@ static void dump_static(PhysicsEngine& E, Out& o) {
 	std::vector<int64_t> bi; // id, movement type, geometry type, n primitives, primitive type
 	std::vector<double> pts, gv, gx;
 	std::vector<int64_t> poff{0}, foff{0}, fidx{0}, fverts, gred, gint, arcs, aoff{0}, edges, eoff{0}, surfid, ncalc, ndir;
-	std::vector<double> mat;
+	std::vector<double> mat, sph;  // sphere: body-frame centre xyz, radius
 	for (RigidBody* b : E.bodies) {
 		mp.push_back(b->mass);
 		for (int r = 0; r < 3; r++) for (int c = 0; c < 3; c++) mp.push_back(b->reference_tensor.v[r][c]);
@@ -333,6 +361,10 @@ static void dump_static(PhysicsEngine& E, Out& o) {
 			const phyz::Material& m = b->geometry[0].material;
 			mat.insert(mat.end(), {m.density, m.restitution, m.kinetic_friction_coeff, m.static_friction_coeff});
 		} else mat.insert(mat.end(), {0, 0, 0, 0});
+		if (ptype == phyz::SPHERE && b->geometry.size() == 1) {
+			const phyz::Sphere& rs = (const phyz::Sphere&)*b->reference_geometry[0].getGeometry();
+			sph.insert(sph.end(), {rs.center.x, rs.center.y, rs.center.z, rs.radius});
+		} else sph.insert(sph.end(), {0, 0, 0, 0});
 		if (ptype == phyz::POLYHEDRON && b->geometry.size() == 1) {
 			const phyz::Polyhedron& ref = (const phyz::Polyhedron&)*b->reference_geometry[0].getGeometry();
 			const phyz::Polyhedron& wrl = (const phyz::Polyhedron&)*b->geometry[0].getGeometry();
@@ -365,6 +397,7 @@ static void dump_static(PhysicsEngine& E, Out& o) {
 	o.f64("mass_props", mp, {N, 19});
 	o.i64("body_info", bi, {N, 5});
 	o.f64("material", mat, {N, 4});
+	o.f64("sphere", sph, {N, 4});
 	o.f64("hull_points", pts, {pts.size() / 3, 3});
 	o.i64("hull_point_off", poff, {N + 1});
 	o.i64("hull_face_off", foff, {N + 1});
@@ -487,7 +520,10 @@ static int cmd_pile(int argc, char** argv) {
 	cfg.substeps = (int)arg_d(argc, argv, "--substeps", 8);
 	PhysicsEngine E;
 	configure(E, cfg);
-	build_pile(E, n, ny, n, seed, arg_d(argc, argv, "--spacing", 1.1), arg_d(argc, argv, "--jitter", 0.05));
+	if (arg_d(argc, argv, "--mixed", 0) != 0)
+		build_mixed(E, n, ny, n, seed, arg_d(argc, argv, "--spacing", 1.1), arg_d(argc, argv, "--jitter", 0.05));
+	else
+		build_pile(E, n, ny, n, seed, arg_d(argc, argv, "--spacing", 1.1), arg_d(argc, argv, "--jitter", 0.05));
 	Out o(arg_s(argc, argv, "--out", "out"));
 	o.meta("seed", (double)seed);
 	o.meta("nx", n);

@@ -41,6 +41,12 @@ class Fixture:
     def recorded(self):
         return [int(k) for k in self["recorded_steps"]]
 
+    def shape(self, b):
+        """Hull dict for polyhedra, dict(sphere=[cx, cy, cz, r]) for spheres."""
+        if int(self["body_info"][b, 4]) == 1:
+            return dict(sphere=self["sphere"][b])
+        return self.hull(b)
+
     def hull(self, b):
         """Body-frame hull and Gauss map of body index b (as the reference stores them)."""
         po, fo, ao, eo = self["hull_point_off"], self["hull_face_off"], self["gauss_arc_off"], self["hull_edge_off"]
@@ -82,6 +88,7 @@ class OrcConfig(ctypes.Structure):
         ("cfm", ctypes.c_double),
         ("contact_hardness", ctypes.c_double),
         ("contact_life", ctypes.c_int),
+        ("drop_nonfinite", ctypes.c_int),
     ]
 
 
@@ -108,6 +115,8 @@ def oracle_lib():
         lib.orc_add_hull_body.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                                           ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 9 + [ctypes.c_int, ctypes.c_void_p,
                                                                                                     ctypes.c_int, ctypes.c_void_p]
+        lib.orc_add_sphere_body.restype = ctypes.c_int
+        lib.orc_add_sphere_body.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double]
         lib.orc_set_cache.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         lib.orc_cache_size.restype = ctypes.c_int
         lib.orc_cache_size.argtypes = [ctypes.c_void_p]
@@ -121,6 +130,7 @@ def oracle_lib():
         lib.orc_narrowphase.restype = ctypes.c_int
         lib.orc_narrowphase.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         lib.orc_step.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        lib.orc_step_scheduled.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
         lib.orc_last_islands.restype = ctypes.c_int
         lib.orc_last_islands.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
         _lib = lib
@@ -135,7 +145,7 @@ class Oracle:
     """FP64 CPU restatement of the reference step, loaded with a fixture's scene."""
 
     def __init__(self, fx=None, cfm=0.0000025, hardness=1000.0, life=6, desc=None, dt=None, gravity_y=None, vel_iters=None,
-                 pos_iters=None, substeps=None):
+                 pos_iters=None, substeps=None, drop_nonfinite=0):
         lib = oracle_lib()
         self.lib = lib
         if fx is not None:
@@ -144,16 +154,23 @@ class Oracle:
             n = int(m["n_bodies"])
             movement = fx["body_info"][:, 1]
             mp, mat = fx["mass_props"], fx["material"]
-            hulls = [fx.hull(b) for b in range(n)]
+            kinds = fx["body_info"][:, 4]
+            hulls = [fx.hull(b) if kinds[b] == 0 else dict(sphere=fx["sphere"][b]) for b in range(n)]
         else:
             n = len(desc["hulls"])
             movement, mp, mat, hulls = desc["movement"], desc["mass_props"], desc["material"], desc["hulls"]
-        cfg = OrcConfig(dt, (ctypes.c_double * 3)(0.0, gravity_y, 0.0), int(vel_iters), int(pos_iters), int(substeps), cfm, hardness, life)
+        cfg = OrcConfig(dt, (ctypes.c_double * 3)(0.0, gravity_y, 0.0), int(vel_iters), int(pos_iters), int(substeps), cfm, hardness, life, drop_nonfinite)
         self.w = ctypes.c_void_p(lib.orc_create(ctypes.byref(cfg)))
         self.n = n
         self._keep = []
         for b in range(self.n):
             h = hulls[b]
+            if "sphere" in h:
+                arrs = [_c(mp[b], np.float64), _c(mat[b], np.float64), _c(h["sphere"][:3], np.float64)]
+                self._keep.append(arrs)
+                lib.orc_add_sphere_body(self.w, int(movement[b]), arrs[0].ctypes.data, arrs[1].ctypes.data, arrs[2].ctypes.data,
+                                        float(h["sphere"][3]))
+                continue
             arrs = [_c(mp[b], np.float64), _c(mat[b], np.float64), _c(h["pts"], np.float64), _c(h["face_off"], np.int64),
                     _c(h["face_verts"], np.int64), _c(h["surface_id"], np.int64), _c(h["normal_calc"], np.int64),
                     _c(h["normal_dir"], np.int64), _c(h["gauss_normal"], np.float64), _c(h["gauss_red"], np.int64),
@@ -219,6 +236,11 @@ class Oracle:
         else:
             self.lib.orc_step(self.w, None)
 
+    def step_scheduled(self, rows, offs):
+        r = _c(rows, np.int64)
+        o = _c(offs, np.int64)
+        self.lib.orc_step_scheduled(self.w, len(o) - 1, r.ctypes.data, o.ctypes.data)
+
     def cache(self, mid=False):
         n = self.lib.orc_last_mid_cache_size(self.w) if mid else self.lib.orc_cache_size(self.w)
         ci = np.zeros((n, 6), np.int64)
@@ -248,6 +270,7 @@ def world_from_fixture(fx, pgs_mode=1, **cfg):
     w = cpa.World(**kw)
     info, mp, mat = fx["body_info"], fx["mass_props"], fx["material"]
     for b in range(int(m["n_bodies"])):
-        sid = w.add_hull_exact(fx.hull(b))
+        sh = fx.shape(b)
+        sid = w.add_sphere(sh["sphere"][:3], sh["sphere"][3]) if "sphere" in sh else w.add_hull_exact(sh)
         w.add_body(sid, int(info[b, 1]), mp[b], mat[b], 0)
     return w

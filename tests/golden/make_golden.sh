@@ -8,8 +8,10 @@ ROOT=$(cd "$HERE/../.." && pwd)
 make -C "$ROOT/oracle" ref REF="${REF:-/root/reference}"
 H="$ROOT/oracle/_ref/cp_ref_harness"
 cd "$HERE"
-rm -rf pile3 c0_full
+rm -rf pile3 c0_full mix3
 # C0 pattern, 3x3x3 cubes: full per-step dumps (state, caches, manifolds, islands).
 "$H" pile --n 3 --steps 40 --record 0,1,2,10,39 --out pile3
 # C0 in full (10x10x10 cubes): first step and a settled step (no mid/post caches).
 "$H" pile --n 10 --steps 121 --record 0,120 --lite 1 --out c0_full
+# Mixed pile (cubes and spheres alternating, 3x3x3): sphere-sphere, sphere-box, box-box, ground pairs.
+"$H" pile --mixed 1 --n 3 --steps 60 --record 0,5,30,59 --out mix3

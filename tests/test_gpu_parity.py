@@ -39,11 +39,15 @@ def c0():
     return golden("c0_full")
 
 
-@pytest.mark.parametrize("k", PILE_STEPS)
-def test_broadphase_bit_exact_pile(pile3, k):
-    w = world_from_fixture(pile3)
-    _load(w, pile3, k)
-    assert np.array_equal(w.broadphase(), pile3[f"s{k}_pairs"])
+CASES = [("pile3", k) for k in PILE_STEPS] + [("mix3", k) for k in (0, 5, 30, 59)]
+
+
+@pytest.mark.parametrize("name,k", CASES)
+def test_broadphase_bit_exact(name, k):
+    fx = golden(name)
+    w = world_from_fixture(fx)
+    _load(w, fx, k)
+    assert np.array_equal(w.broadphase(), fx[f"s{k}_pairs"])
 
 
 def test_broadphase_bit_exact_c0(c0):
@@ -65,11 +69,12 @@ def _check_manifolds(w, fx, k):
         off += n
 
 
-@pytest.mark.parametrize("k", PILE_STEPS)
-def test_narrowphase_bit_exact_pile(pile3, k):
-    w = world_from_fixture(pile3)
-    _load(w, pile3, k)
-    _check_manifolds(w, pile3, k)
+@pytest.mark.parametrize("name,k", CASES)
+def test_narrowphase_bit_exact(name, k):
+    fx = golden(name)
+    w = world_from_fixture(fx)
+    _load(w, fx, k)
+    _check_manifolds(w, fx, k)
 
 
 def test_narrowphase_bit_exact_c0(c0):
@@ -95,18 +100,19 @@ def _rel_close(a, b, rtol):
     return np.max(np.abs(a - b) / scale)
 
 
-@pytest.mark.parametrize("k", PILE_STEPS)
-def test_full_step_reference_order_pile(pile3, k):
-    w = world_from_fixture(pile3, pgs_mode=1)
-    _load(w, pile3, k)
-    w.set_node_order(pile3[f"s{k}_node_order"])
+@pytest.mark.parametrize("name,k", CASES)
+def test_full_step_reference_order(name, k):
+    fx = golden(name)
+    w = world_from_fixture(fx, pgs_mode=1)
+    _load(w, fx, k)
+    w.set_node_order(fx[f"s{k}_node_order"])
     w.step(1)
     rows, offs = w.last_islands()
-    assert np.array_equal(rows, pile3[f"s{k}_island_rows"])
-    assert np.array_equal(offs, pile3[f"s{k}_island_off"])
-    err = _rel_close(w.state(), pile3[f"s{k}_post"], 1e-12)
+    assert np.array_equal(rows, fx[f"s{k}_island_rows"])
+    assert np.array_equal(offs, fx[f"s{k}_island_off"])
+    err = _rel_close(w.state(), fx[f"s{k}_post"], 1e-12)
     assert err <= 1e-12, err
-    gi, gm, gf = pile3.cache(k, "post")
+    gi, gm, gf = fx.cache(k, "post")
     ii, mg, ff = w.contacts()
     assert np.array_equal(ii[:, [0, 1, 2, 3, 5]], gi[:, [0, 1, 2, 3, 5]])
     assert np.array_equal(mg, gm)
@@ -151,3 +157,26 @@ def test_colored_mode_converges_to_reference_order_solution(pile3):
     d_lo = np.max(np.abs(out[0, 10][:, 7:] - out[1, 10][:, 7:]))
     assert np.all(np.isfinite(out[0, 10])) and np.all(np.isfinite(out[0, 400]))
     assert d_hi < 0.25 * d_lo, (d_hi, d_lo)
+
+
+@pytest.mark.parametrize("name,k", [("pile3", 10), ("pile3", 39), ("mix3", 30), ("mix3", 59)])
+def test_colored_schedule_replayed_by_oracle(name, k):
+    """The coloured sweep is a sequential Gauss-Seidel order in disguise (edges
+    of one colour share no dynamic body).  The oracle replaying the GPU's
+    colour schedule as an explicit row order reproduces the coloured step."""
+    from fixtures import Oracle
+    fx = golden(name)
+    w = world_from_fixture(fx, pgs_mode=0, diagnostics=2)
+    _load(w, fx, k)
+    w.step(1)
+    rows, offs = w.last_islands()
+    o = Oracle(fx)
+    o.load(fx[f"s{k}_pre"], fx[f"s{k}_pre_inertia"], fx.cache(k, "pre"))
+    o.step_scheduled(rows, offs)
+    a, b = w.state(), o.state()
+    err = float(np.max(np.abs(a - b) / np.maximum(1.0, np.abs(b))))
+    assert err <= 1e-12, err
+    ii, mg, ff = w.contacts()
+    oi, om, of = o.cache()
+    assert np.array_equal(ii, oi) and np.array_equal(mg, om)
+    assert _rel_close(ff, of, 1e-10) <= 1e-10

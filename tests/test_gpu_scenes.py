@@ -1,0 +1,107 @@
+"""Simulation-outcome tests restated from the reference's test suite, on scenes
+built through the product's own host path (shape builder, mass properties).
+
+Box vs Ground (Demo/src/Demos/Tests/CollisionTests.h:77-165): a unit box
+dropped from y=5 onto a fixed ground box at 60 Hz with the engine defaults
+rests after 5 s with its centre 0.5 +- 0.1 above the ground top and contact
+normals within 0.9999 of +y.  The reference test also asks |v|, |w| <= 1e-5,
+which it reaches by putting the resting box to sleep (RigidBody::sleep zeroes
+the velocities); without sleeping the box keeps a ~2e-3 m/s creep, which the
+FP64 oracle (the reference restated, sleeping off) reproduces to the bit —
+the test checks the GPU against that oracle over all 300 steps instead.
+"""
+import os
+import sys
+import numpy as np
+import pytest
+
+import constraintphysics_amd as cpa
+from constraintphysics_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _box_on_ground(mode):
+    w = cpa.World(dt=1.0 / 60.0, pgs_mode=mode)  # reference defaults otherwise (PhysicsEngine.h:174-216)
+    b = scenes.SceneBuilder(w)
+    gs, gmp = b.box_shape(100.0, 0.5, 100.0, 1.0)
+    b.add(gs, gmp, 1, scenes.DEFAULT_MATERIAL, 0, np.array([0.0, -0.25, 0.0]))
+    bs, bmp = b.box_shape(1.0, 1.0, 1.0, 1.0)
+    b.add(bs, bmp, 0, scenes.DEFAULT_MATERIAL, 0, np.array([0.5, 5.5, 0.5]))
+    b.finish()
+    return w
+
+
+def _oracle_box_on_ground():
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from fixtures import Oracle
+    hulls, mps = [], []
+    for dims in ((100.0, 0.5, 100.0), (1.0, 1.0, 1.0)):
+        p = scenes.box_points(*dims)
+        hulls.append(cpa.polyhedron_build(p, scenes.BOX_FACES))
+        m, t, _ = cpa.polyhedron_mass_properties(p, scenes.BOX_FACES, 1.0)
+        mps.append(scenes.mass_props_row(m, t))
+    st = np.array([[0, -0.25, 0, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0], [0.5, 5.5, 0.5, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0]], float)
+    desc = dict(hulls=hulls, movement=np.array([1, 0]), mass_props=np.array(mps), material=np.array([scenes.DEFAULT_MATERIAL] * 2), state=st)
+    o = Oracle(desc=desc, dt=1.0 / 60.0, gravity_y=-6.0, vel_iters=4, pos_iters=1, substeps=8)
+    o.load(st)
+    return o
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_box_vs_ground_rests(mode):
+    w = _box_on_ground(mode)
+    o = _oracle_box_on_ground()
+    worst = 0.0
+    for _ in range(300):
+        w.step(1)
+        o.step()
+        a, b = w.state(), o.state()
+        worst = max(worst, float(np.max(np.abs(a - b) / np.maximum(1.0, np.abs(b)))))
+    assert worst <= 1e-9, worst
+    s = w.state()[1]
+    assert np.linalg.norm(s[7:10]) <= 5e-3
+    assert np.linalg.norm(s[10:13]) <= 5e-3
+    assert abs(s[1] - 0.5) <= 0.1
+    ii, mg, ff = w.contacts()
+    assert len(ii) > 0 and np.all(ii[:, 0] == 1) and np.all(ii[:, 1] == 2)
+    # contact normals (ground frame = world frame for the fixed ground) within 0.9999 of +y
+    assert np.all(np.abs(ff[:, 7]) >= 0.9999)
+
+
+def test_box_vs_ground_modes_agree():
+    """One body pair = one edge: the coloured schedule visits rows in the reference order."""
+    a, b = _box_on_ground(0), _box_on_ground(1)
+    for _ in range(120):
+        a.step(1)
+        b.step(1)
+    assert np.array_equal(a.state(), b.state())
+
+
+def test_c0_pile_settles_coloured():
+    w = cpa.World(dt=1.0 / 120.0, vel_iters=10, pos_iters=1, substeps=8, pgs_mode=0)
+    scenes.build_c0(w, n=4, seed=7)
+    for _ in range(360):
+        w.step(1)
+    s = w.state()[1:]
+    assert np.all(np.isfinite(s))
+    assert np.max(np.linalg.norm(s[:, 7:10], axis=1)) < 0.05
+    assert np.min(s[:, 1]) > 0.4
+
+
+def test_batched_scenes_match_individual_scenes():
+    """Scenes never interact: a scene stepped inside a batch equals the same
+    scene stepped alone (coloured mode colours each scene independently)."""
+    sizes = scenes.c3_sizes(6, 11, lo=8, hi=60)
+    wb = cpa.World(dt=1.0 / 120.0, vel_iters=10, pos_iters=1, substeps=8, pgs_mode=0)
+    scenes.build_batched(wb, sizes, 11, hull_lib_size=16)
+    for _ in range(60):
+        wb.step(1)
+    sb = wb.state()
+    off = np.concatenate([[0], np.cumsum(np.asarray(sizes) + scenes.CONTAINER_WALLS)])
+    for s in (0, 3, 5):
+        ws = cpa.World(dt=1.0 / 120.0, vel_iters=10, pos_iters=1, substeps=8, pgs_mode=0)
+        scenes.build_batched(ws, sizes, 11, scene_indices=[s], hull_lib_size=16)
+        for _ in range(60):
+            ws.step(1)
+        assert np.array_equal(ws.state(), sb[off[s]:off[s + 1]])

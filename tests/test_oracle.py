@@ -69,3 +69,25 @@ def test_c0_full_first_and_settled_step():
         rows, offs = o.islands()
         assert np.array_equal(rows, fx[f"s{k}_island_rows"])
         assert np.array_equal(o.state(), fx[f"s{k}_post"])
+
+
+@pytest.mark.parametrize("k", [0, 5, 30, 59])
+def test_mixed_spheres_boxes_full_step(k):
+    fx = golden("mix3")
+    o = Oracle(fx)
+    _load(o, fx, k)
+    assert np.array_equal(o.broadphase(), fx[f"s{k}_pairs"])
+    mp, mh, mpts, mm = (fx[f"s{k}_man_{x}"] for x in ("pair", "head", "pts", "magic"))
+    off = 0
+    for i, (a, b, n) in enumerate(mp):
+        head, pts, mg = o.narrowphase(int(a), int(b))
+        assert len(pts) == n and np.array_equal(head, mh[i])
+        assert np.array_equal(pts, mpts[off:off + n]) and np.array_equal(mg, mm[off:off + n])
+        off += n
+    o.step(fx[f"s{k}_node_order"])
+    rows, offs = o.islands()
+    assert np.array_equal(rows, fx[f"s{k}_island_rows"])
+    assert np.array_equal(o.state(), fx[f"s{k}_post"])
+    pi, pm, pf = o.cache()
+    gi, gm, gf = fx.cache(k, "post")
+    assert np.array_equal(pi, gi) and np.array_equal(pm, gm) and np.array_equal(pf, gf)
