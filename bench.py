@@ -97,7 +97,7 @@ def cpu_baseline(sizes, seed, budget_s, steps, state, contacts, off):
         n_dyn = int(np.sum(desc["movement"] == 0))
         t0 = time.perf_counter()
         k = 0
-        while k < steps and time.perf_counter() - t0 < budget_s / 2:
+        while time.perf_counter() - t0 < budget_s / len(picks) or k < 2:
             o.step()
             k += 1
         dt = time.perf_counter() - t0

@@ -82,14 +82,33 @@ struct Contacts {
 	double *lp1, *lp2, *ln1;  // [3][cap]
 	double *pen, *bounce, *mu;
 	int *life, *live, *sready;
-	double *imp_n, *imp_t;  // imp_t [2][cap]
+	double *imp_n, *imp_t;  // imp_t [2][cap]; persistent warm-start impulses
 	double *fu, *fw;        // [3][cap] friction basis of the last row build
-	// rows
-	double *ja, *jb, *ita, *itb;  // normal row: [6][cap] each
-	double *inv_n, *tgt_n, *ptgt_n, *pimp_n;
-	double *fja, *fjb, *fita, *fitb;  // friction rows: [12][cap], index r*6+c
-	double *finv;                     // [4][cap]
-	double *ftgt;                     // [2][cap]
+	// Solver rows, one contiguous record of CP_RS doubles per contact (the
+	// coloured sweep visits contacts out of memory order, so a record keeps a
+	// row update within a few cache lines).  Layout: CP_R_* offsets below.
+	double* row;
+};
+
+#define CP_RS 88
+enum RowField {
+	CP_R_JA = 0,     // normal row J_a [6]
+	CP_R_JB = 6,     // J_b [6]
+	CP_R_ITA = 12,   // M_a^-1 J_a^T [6]
+	CP_R_ITB = 18,   // M_b^-1 J_b^T [6]
+	CP_R_INV = 24,   // (J M^-1 J^T (1+cfm))^-1
+	CP_R_TGT = 25,   // velocity target
+	CP_R_PTGT = 26,  // pseudo-velocity target
+	CP_R_IMP = 27,   // accumulated normal impulse
+	CP_R_PIMP = 28,  // accumulated pseudo impulse
+	CP_R_MU = 29,    // friction coefficient
+	CP_R_FJA = 30,   // friction J_a [2][6]
+	CP_R_FJB = 42,   // friction J_b [2][6]
+	CP_R_FITA = 54,  // M_a^-1 J_a^T [6][2]
+	CP_R_FITB = 66,  // M_b^-1 J_b^T [6][2]
+	CP_R_FINV = 78,  // 2x2 inverse
+	CP_R_FTGT = 82,  // targets [2]
+	CP_R_FIMP = 84   // accumulated friction impulses [2]
 };
 
 // A step's manifold points, compacted and sorted by pair key.
@@ -128,11 +147,15 @@ void launch_edge_build(const Contacts& C, int n, const int* head, const int* eid
 void launch_color(const Bodies& b, const int* scene_body_off, int nscenes, const uint64_t* ekey, const int* elive, int nedges,
                   int* ecolor, int* order, int* color_off, int* ncolors, int* edge_range, hipStream_t s);
 void launch_build_rows(const Bodies& b, const Contacts& C, int nc, const Params& P, hipStream_t s);
+void launch_pgs_colored_lds(const Bodies& b, const Contacts& C, const int* scene_list, int nlist, int max_bodies, const int* scene_body_off,
+                            const int* edge_range, const int* order, const int* color_off, const int* ncolors, const int* efirst,
+                            const int* ecnt, const Params& P, int threads, hipStream_t s);
+void launch_rows_writeback(const Contacts& C, int nc, hipStream_t s);
 void launch_pgs_residual(const Bodies& b, const Contacts& C, int nc, double* acc, hipStream_t s);
 void launch_pgs_ordered(const Bodies& b, const Contacts& C, const int* rows, const int* isl_off, int nisl, const Params& P, int* applied,
                         hipStream_t s);
 void launch_pgs_colored(const Bodies& b, const Contacts& C, const int* scene_body_off, int nscenes, const int* edge_range, const int* order,
                         const int* color_off, const int* ncolors, const int* efirst, const int* ecnt, const Params& P, int threads,
-                        hipStream_t s);
+                        hipStream_t s, const int* scene_list = nullptr, int nlist = 0);
 
 }  // namespace cpx

@@ -5,18 +5,23 @@
 // 240-272) and FrictionConstraint (:277-343), plus the per-solve target values
 // (:265-268 and ConstraintSolver.h:78).  Matrix products keep NMat's
 // accumulation order (NMat.h:50-60, :91-103) so FP64 rows match bit for bit.
+// Each contact's rows are one contiguous record of CP_RS doubles, so a row
+// update reads a few whole cache lines instead of one line per component.
 //
-// Two sweep schedules over the same rows (ConstraintSolver.cpp:80-213):
+// Sweep schedules over the same rows (ConstraintSolver.cpp:80-213):
 //  * k_pgs_ordered: the reference's sequential order (one thread per island,
 //    rows in the island-search order supplied by the host) — the parity mode.
-//  * k_pgs_colored: one workgroup per scene; edges (body pairs) are coloured
-//    so that no two edges of a colour share a dynamic body, each thread runs
-//    one edge's contacts in cache order, and colours are separated by a
-//    workgroup barrier.  Gauss-Seidel per body, in a graph-coloured order.
+//  * k_pgs_colored_lds / k_pgs_colored: one workgroup per scene; edges (body
+//    pairs) are coloured so that no two edges of a colour share a dynamic
+//    body, each thread runs one edge's contacts in cache order, and colours
+//    are separated by a workgroup barrier.  Gauss-Seidel per body, in a
+//    graph-coloured order.  The LDS variant keeps the scene's velocity deltas
+//    in LDS; the global one serves scenes too large for LDS.
 //
-// Roofline: HBM/L2-bound.  Per contact per sweep the algorithmic traffic is the
-// row (~350 B FP64: J, M^-1 J^T, effective masses, targets, impulses) plus the
-// gather/scatter of two bodies' velocity deltas.
+// Roofline: per contact per launch the algorithmic traffic is the row record
+// (warm start, then per velocity sweep the normal and friction rows with
+// their impulses, per position sweep the normal row), plus two bodies'
+// velocity deltas when those are not in LDS.
 #include "cp_internal.h"
 
 namespace cpx {
@@ -29,6 +34,7 @@ __global__ void k_build_rows(Bodies b, Contacts C, int nc, Params P) {
 	int k = blockIdx.x * blockDim.x + threadIdx.x;
 	if (k >= nc || !C.live[k]) return;
 	const int N = b.n, K = C.cap;
+	double* R = C.row + (size_t)k * CP_RS;
 	int i1 = (int)(C.key[k] >> 32), i2 = (int)(C.key[k] & 0xFFFFFFFFu);
 	Q q1{b.q[i1], b.q[N + i1], b.q[2 * N + i1], b.q[3 * N + i1]};
 	Q q2{b.q[i2], b.q[N + i2], b.q[2 * N + i2], b.q[3 * N + i2]};
@@ -59,23 +65,15 @@ __global__ void k_build_rows(Bodies b, Contacts C, int nc, Params P) {
 	// crossMat (NMat.cpp:62-70)
 	double sa[3][3] = {{0, -rA.z, rA.y}, {rA.z, 0, -rA.x}, {-rA.y, rA.x, 0}};
 	double sb[3][3] = {{0, -rB.z, rB.y}, {rB.z, 0, -rB.x}, {-rB.y, rB.x, 0}};
+	// getVel()/getAngVel() are zero only for FIXED bodies (RigidBody.cpp:174-181)
 	double va[6], vb[6];
 	{
-		V3 v = d1 ? ld3(b.vel, i1, N) : V3{0, 0, 0}, w = d1 ? ld3(b.angv, i1, N) : V3{0, 0, 0};
+		bool x1 = b.movement[i1] == 1, x2 = b.movement[i2] == 1;
+		V3 v = x1 ? V3{0, 0, 0} : ld3(b.vel, i1, N), w = x1 ? V3{0, 0, 0} : ld3(b.angv, i1, N);
 		va[0] = v.x; va[1] = v.y; va[2] = v.z; va[3] = w.x; va[4] = w.y; va[5] = w.z;
-		// getVel()/getAngVel() are zero only for FIXED bodies (RigidBody.cpp:174-181)
-		if (b.movement[i1] != 1 && !d1) {
-			V3 v2 = ld3(b.vel, i1, N), w2 = ld3(b.angv, i1, N);
-			va[0] = v2.x; va[1] = v2.y; va[2] = v2.z; va[3] = w2.x; va[4] = w2.y; va[5] = w2.z;
-		}
-	}
-	{
-		V3 v = d2 ? ld3(b.vel, i2, N) : V3{0, 0, 0}, w = d2 ? ld3(b.angv, i2, N) : V3{0, 0, 0};
+		v = x2 ? V3{0, 0, 0} : ld3(b.vel, i2, N);
+		w = x2 ? V3{0, 0, 0} : ld3(b.angv, i2, N);
 		vb[0] = v.x; vb[1] = v.y; vb[2] = v.z; vb[3] = w.x; vb[4] = w.y; vb[5] = w.z;
-		if (b.movement[i2] != 1 && !d2) {
-			V3 v2 = ld3(b.vel, i2, N), w2 = ld3(b.angv, i2, N);
-			vb[0] = v2.x; vb[1] = v2.y; vb[2] = v2.z; vb[3] = w2.x; vb[4] = w2.y; vb[5] = w2.z;
-		}
 	}
 	// ---- ContactConstraint
 	{
@@ -98,19 +96,20 @@ __global__ void k_build_rows(Bodies b, Contacts C, int nc, Params P) {
 		double itv = t1 + t2;
 		itv *= (1 + P.cfm);
 		for (int r = 0; r < 6; r++) {
-			C.ja[r * K + k] = ja[r]; C.jb[r * K + k] = jb[r];
-			C.ita[r * K + k] = ita[r]; C.itb[r * K + k] = itb[r];
+			R[CP_R_JA + r] = ja[r]; R[CP_R_JB + r] = jb[r];
+			R[CP_R_ITA + r] = ita[r]; R[CP_R_ITB + r] = itb[r];
 		}
-		C.inv_n[k] = 1.0 / itv;
+		R[CP_R_INV] = 1.0 / itv;
 		double sp = adj - 0.05;
 		sp = (sp < 0.0) ? 0.0 : sp;  // std::max<double>(0.0, x)
-		C.ptgt_n[k] = sp * pcc;
-		C.pimp_n[k] = 0;
-		C.imp_n[k] = cimp;
+		R[CP_R_PTGT] = sp * pcc;
+		R[CP_R_PIMP] = 0;
+		R[CP_R_IMP] = cimp;
+		R[CP_R_MU] = C.mu[k];
 		double x = 0, y = 0;
 		for (int q = 0; q < 6; q++) { x += va[q] * ja[q]; y += vb[q] * jb[q]; }
 		double cur = x + y;
-		C.tgt_n[k] = (cur < -P.cutoff_vel) ? -(1 + C.bounce[k]) * cur : -cur;
+		R[CP_R_TGT] = (cur < -P.cutoff_vel) ? -(1 + C.bounce[k]) * cur : -cur;
 	}
 	// ---- FrictionConstraint
 	{
@@ -151,29 +150,32 @@ __global__ void k_build_rows(Bodies b, Contacts C, int nc, Params P) {
 			}
 		itv[0][0] *= (1 + P.cfm);
 		itv[1][1] *= (1 + P.cfm);
+		// rowMajorOrderInverse<2> (NMat.cpp:11-21)
 		double a = itv[0][0], bq = itv[0][1], cq = itv[1][0], d = itv[1][1];
 		double dt = a * d - bq * cq;
 		double di = 1.0 / dt;
-		C.finv[k] = d * di; C.finv[K + k] = -bq * di; C.finv[2 * K + k] = -cq * di; C.finv[3 * K + k] = a * di;
+		R[CP_R_FINV] = d * di; R[CP_R_FINV + 1] = -bq * di; R[CP_R_FINV + 2] = -cq * di; R[CP_R_FINV + 3] = a * di;
 		for (int r = 0; r < 2; r++)
-			for (int c = 0; c < 6; c++) { C.fja[(r * 6 + c) * K + k] = ja[r][c]; C.fjb[(r * 6 + c) * K + k] = jb[r][c]; }
+			for (int c = 0; c < 6; c++) { R[CP_R_FJA + r * 6 + c] = ja[r][c]; R[CP_R_FJB + r * 6 + c] = jb[r][c]; }
 		for (int r = 0; r < 6; r++)
-			for (int c = 0; c < 2; c++) { C.fita[(r * 2 + c) * K + k] = ita[r][c]; C.fitb[(r * 2 + c) * K + k] = itb[r][c]; }
-		C.imp_t[k] = i0;
-		C.imp_t[K + k] = i1v;
+			for (int c = 0; c < 2; c++) { R[CP_R_FITA + r * 2 + c] = ita[r][c]; R[CP_R_FITB + r * 2 + c] = itb[r][c]; }
+		R[CP_R_FIMP] = i0;
+		R[CP_R_FIMP + 1] = i1v;
 		C.fu[k] = u.x; C.fu[K + k] = u.y; C.fu[2 * K + k] = u.z;
 		C.fw[k] = w.x; C.fw[K + k] = w.y; C.fw[2 * K + k] = w.z;
 		C.sready[k] = 0;
 		for (int r = 0; r < 2; r++) {
 			double x = 0, y = 0;
 			for (int q = 0; q < 6; q++) { x += va[q] * ja[r][q]; y += vb[q] * jb[r][q]; }
-			C.ftgt[r * K + k] = -(x + y);
+			R[CP_R_FTGT + r] = -(x + y);
 		}
 	}
 }
 
-// ------------------------------------------------------------- PGS row updates
-struct DvRef {
+// ------------------------------------------------------------- row updates
+// Velocity-delta accessors: global memory [12][n] (components 0-5 velocity,
+// 6-11 pseudo-velocity) or LDS [6][nbs] holding one half at a time.
+struct DvGlobal {
 	double* base;
 	int i, n;
 	bool dyn;
@@ -182,27 +184,39 @@ struct DvRef {
 		if (dyn) base[c * n + i] += v;
 	}
 };
+struct DvLds {
+	double* s;
+	int i, nbs;  // scene-local body index
+	bool dyn;
+	__device__ double get(int c) const { return s[(c % 6) * nbs + i]; }
+	__device__ void add(int c, double v) const {
+		if (dyn) s[(c % 6) * nbs + i] += v;
+	}
+};
 
-__device__ __forceinline__ void warm_start(const Contacts& C, int k, const DvRef& A, const DvRef& B) {
-	const int K = C.cap;
-	double imp = C.imp_n[k];
-	if (!is_zero1(imp)) {
+// applyWarmStartVelocityChange of the contact row (kind 0) or the friction
+// pair (kind 1) (ConstraintSolver.cpp:121-126).
+template <class D>
+__device__ __forceinline__ void warm_start(const double* R, const D& A, const D& B, int kind) {
+	if (kind == 0) {
+		double imp = R[CP_R_IMP];
+		if (is_zero1(imp)) return;
 		for (int r = 0; r < 6; r++) {
 			double da = 0, db = 0;
-			da += imp * C.ita[r * K + k];
-			db += imp * C.itb[r * K + k];
+			da += imp * R[CP_R_ITA + r];
+			db += imp * R[CP_R_ITB + r];
 			A.add(r, da);
 			B.add(r, db);
 		}
-	}
-	double t0 = C.imp_t[k], t1 = C.imp_t[K + k];
-	if (!is_zero2(t0, t1)) {
+	} else {
+		double t0 = R[CP_R_FIMP], t1 = R[CP_R_FIMP + 1];
+		if (is_zero2(t0, t1)) return;
 		for (int r = 0; r < 6; r++) {
 			double da = 0, db = 0;
-			da += t0 * C.fita[(r * 2) * K + k];
-			da += t1 * C.fita[(r * 2 + 1) * K + k];
-			db += t0 * C.fitb[(r * 2) * K + k];
-			db += t1 * C.fitb[(r * 2 + 1) * K + k];
+			da += t0 * R[CP_R_FITA + r * 2];
+			da += t1 * R[CP_R_FITA + r * 2 + 1];
+			db += t0 * R[CP_R_FITB + r * 2];
+			db += t1 * R[CP_R_FITB + r * 2 + 1];
 			A.add(r, da);
 			B.add(r, db);
 		}
@@ -210,68 +224,67 @@ __device__ __forceinline__ void warm_start(const Contacts& C, int k, const DvRef
 }
 
 // PGSConstraintStep<1> for the contact row (ConstraintSolver.cpp:12-40, :270-272).
-__device__ __forceinline__ void normal_step(Contacts& C, int k, const DvRef& A, const DvRef& B, bool pseudo) {
-	const int K = C.cap;
+template <class D>
+__device__ __forceinline__ void normal_step(double* R, const D& A, const D& B, bool pseudo) {
 	int off = pseudo ? 6 : 0;
 	double x = 0, y = 0;
 	for (int q = 0; q < 6; q++) {
-		x += A.get(off + q) * C.ja[q * K + k];
-		y += B.get(off + q) * C.jb[q * K + k];
+		x += A.get(off + q) * R[CP_R_JA + q];
+		y += B.get(off + q) * R[CP_R_JB + q];
 	}
 	double cur = x + y;
-	double tgt = pseudo ? C.ptgt_n[k] : C.tgt_n[k];
-	double acc = pseudo ? C.pimp_n[k] : C.imp_n[k];
+	double tgt = pseudo ? R[CP_R_PTGT] : R[CP_R_TGT];
+	double acc = pseudo ? R[CP_R_PIMP] : R[CP_R_IMP];
 	double delta = tgt - cur;
 	double m = 0;
-	m += delta * C.inv_n[k];
+	m += delta * R[CP_R_INV];
 	double nw = acc + m;
 	double pr = (nw < 0) ? 0 : nw;  // std::max<double>(nw, 0)
 	double diff = pr - acc;
 	if (!is_zero1(diff)) {
 		for (int r = 0; r < 6; r++) {
 			double da = 0, db = 0;
-			da += diff * C.ita[r * K + k];
-			db += diff * C.itb[r * K + k];
+			da += diff * R[CP_R_ITA + r];
+			db += diff * R[CP_R_ITB + r];
 			A.add(off + r, da);
 			B.add(off + r, db);
 		}
-		if (pseudo) C.pimp_n[k] = acc + diff;
-		else C.imp_n[k] = acc + diff;
+		R[pseudo ? CP_R_PIMP : CP_R_IMP] = acc + diff;
 	}
 }
 
 // PGSConstraintStep<2> with FrictionConstraint::projectValidImpulse (ConstraintSolver.cpp:331-343).
-__device__ __forceinline__ void friction_step(Contacts& C, int k, const DvRef& A, const DvRef& B) {
-	const int K = C.cap;
+template <class D>
+__device__ __forceinline__ void friction_step(double* R, int* sready, const D& A, const D& B) {
 	double cur[2];
 	for (int r = 0; r < 2; r++) {
 		double x = 0, y = 0;
 		for (int q = 0; q < 6; q++) {
-			x += A.get(q) * C.fja[(r * 6 + q) * K + k];
-			y += B.get(q) * C.fjb[(r * 6 + q) * K + k];
+			x += A.get(q) * R[CP_R_FJA + r * 6 + q];
+			y += B.get(q) * R[CP_R_FJB + r * 6 + q];
 		}
 		cur[r] = x + y;
 	}
-	double d0 = C.ftgt[k] - cur[0], d1 = C.ftgt[K + k] - cur[1];
+	double d0 = R[CP_R_FTGT] - cur[0], d1 = R[CP_R_FTGT + 1] - cur[1];
 	double m0 = 0, m1 = 0;
-	m0 += d0 * C.finv[k];
-	m0 += d1 * C.finv[K + k];
-	m1 += d0 * C.finv[2 * K + k];
-	m1 += d1 * C.finv[3 * K + k];
-	double i0 = C.imp_t[k], i1 = C.imp_t[K + k];
+	m0 += d0 * R[CP_R_FINV];
+	m0 += d1 * R[CP_R_FINV + 1];
+	m1 += d0 * R[CP_R_FINV + 2];
+	m1 += d1 * R[CP_R_FINV + 3];
+	double i0 = R[CP_R_FIMP], i1 = R[CP_R_FIMP + 1];
 	double n0 = i0 + m0, n1 = i1 + m1;
-	double ni = C.imp_n[k];
+	double ni = R[CP_R_IMP];
 	double lim = (ni < __builtin_inf()) ? ni : __builtin_inf();
-	double maxm = C.mu[k] * lim;
+	double maxm = R[CP_R_MU] * lim;
 	double cm = sqrt(n0 * n0 + n1 * n1);
 	double p0, p1;
 	if (cm > maxm) {
-		C.sready[k] = 0;
+		*sready = 0;
 		double q = maxm / cm;
 		p0 = n0 * q;
 		p1 = n1 * q;
 	} else {
-		C.sready[k] = 1;
+		*sready = 1;
 		p0 = n0;
 		p1 = n1;
 	}
@@ -279,15 +292,15 @@ __device__ __forceinline__ void friction_step(Contacts& C, int k, const DvRef& A
 	if (!is_zero2(e0, e1)) {
 		for (int r = 0; r < 6; r++) {
 			double da = 0, db = 0;
-			da += e0 * C.fita[(r * 2) * K + k];
-			da += e1 * C.fita[(r * 2 + 1) * K + k];
-			db += e0 * C.fitb[(r * 2) * K + k];
-			db += e1 * C.fitb[(r * 2 + 1) * K + k];
+			da += e0 * R[CP_R_FITA + r * 2];
+			da += e1 * R[CP_R_FITA + r * 2 + 1];
+			db += e0 * R[CP_R_FITB + r * 2];
+			db += e1 * R[CP_R_FITB + r * 2 + 1];
 			A.add(r, da);
 			B.add(r, db);
 		}
-		C.imp_t[k] = i0 + e0;
-		C.imp_t[K + k] = i1 + e1;
+		R[CP_R_FIMP] = i0 + e0;
+		R[CP_R_FIMP + 1] = i1 + e1;
 	}
 }
 
@@ -306,55 +319,33 @@ __global__ void k_pgs_ordered(Bodies b, Contacts C, const int* rows, const int* 
 	int t = blockIdx.x * blockDim.x + threadIdx.x;
 	if (t >= nisl) return;
 	int r0 = isl_off[t], r1 = isl_off[t + 1];
-	auto refs = [&](int k, DvRef& A, DvRef& B) {
+	auto refs = [&](int k, DvGlobal& A, DvGlobal& B) {
 		int i1 = (int)(C.key[k] >> 32), i2 = (int)(C.key[k] & 0xFFFFFFFFu);
-		A = DvRef{b.dv, i1, b.n, b.movement[i1] == 0};
-		B = DvRef{b.dv, i2, b.n, b.movement[i2] == 0};
+		A = DvGlobal{b.dv, i1, b.n, b.movement[i1] == 0};
+		B = DvGlobal{b.dv, i2, b.n, b.movement[i2] == 0};
 	};
 	for (int r = r0; r < r1; r++) {
 		int k = rows[2 * r];
-		DvRef A, B;
+		DvGlobal A, B;
 		refs(k, A, B);
-		const int K = C.cap;
-		if (rows[2 * r + 1] == 0) {
-			double imp = C.imp_n[k];
-			if (!is_zero1(imp))
-				for (int q = 0; q < 6; q++) {
-					double da = 0, db = 0;
-					da += imp * C.ita[q * K + k];
-					db += imp * C.itb[q * K + k];
-					A.add(q, da);
-					B.add(q, db);
-				}
-		} else {
-			double t0 = C.imp_t[k], t1 = C.imp_t[K + k];
-			if (!is_zero2(t0, t1))
-				for (int q = 0; q < 6; q++) {
-					double da = 0, db = 0;
-					da += t0 * C.fita[(q * 2) * K + k];
-					da += t1 * C.fita[(q * 2 + 1) * K + k];
-					db += t0 * C.fitb[(q * 2) * K + k];
-					db += t1 * C.fitb[(q * 2 + 1) * K + k];
-					A.add(q, da);
-					B.add(q, db);
-				}
-		}
+		warm_start(C.row + (size_t)k * CP_RS, A, B, rows[2 * r + 1]);
 	}
 	for (int it = 0; it < P.vel_iters; it++)
 		for (int r = r0; r < r1; r++) {
 			int k = rows[2 * r];
-			DvRef A, B;
+			DvGlobal A, B;
 			refs(k, A, B);
-			if (rows[2 * r + 1] == 0) normal_step(C, k, A, B, false);
-			else friction_step(C, k, A, B);
+			double* R = C.row + (size_t)k * CP_RS;
+			if (rows[2 * r + 1] == 0) normal_step(R, A, B, false);
+			else friction_step(R, &C.sready[k], A, B);
 		}
 	for (int it = 0; it < P.pos_iters; it++)
 		for (int r = r0; r < r1; r++) {
 			if (rows[2 * r + 1] != 0) continue;
 			int k = rows[2 * r];
-			DvRef A, B;
+			DvGlobal A, B;
 			refs(k, A, B);
-			normal_step(C, k, A, B, true);
+			normal_step(C.row + (size_t)k * CP_RS, A, B, true);
 		}
 	for (int r = r0; r < r1; r++) {
 		int k = rows[2 * r];
@@ -368,10 +359,37 @@ __global__ void k_pgs_ordered(Bodies b, Contacts C, const int* rows, const int* 
 	}
 }
 
-// One workgroup per scene; colour-ordered sweeps.
+// One colour of one sweep: each thread runs the contacts of its edges in cache
+// order; the colour index MAXC holds edges that did not fit a colour and runs
+// on thread 0 alone.  kind 0 warm start, 1 velocity sweep, 2 position sweep.
+template <class D, class Make>
+__device__ __forceinline__ void colour_phase(Contacts& C, const int* order, int e0, int beg, int end, bool serial, const int* efirst,
+                                             const int* ecnt, int kind, Make make) {
+	int stride = serial ? 1 : blockDim.x;
+	int start = serial ? (threadIdx.x == 0 ? beg : end) : beg + threadIdx.x;
+	for (int x = start; x < end; x += stride) {
+		int e = order[e0 + x];
+		int k0 = efirst[e], k1 = k0 + ecnt[e];
+		int i1 = (int)(C.key[k0] >> 32), i2 = (int)(C.key[k0] & 0xFFFFFFFFu);
+		D A = make(i1), B = make(i2);
+		for (int k = k0; k < k1; k++) {
+			if (!C.live[k]) continue;
+			double* R = C.row + (size_t)k * CP_RS;
+			if (kind == 0) {
+				warm_start(R, A, B, 0);
+				warm_start(R, A, B, 1);
+			} else if (kind == 1) {
+				normal_step(R, A, B, false);
+				friction_step(R, &C.sready[k], A, B);
+			} else normal_step(R, A, B, true);
+		}
+	}
+}
+
+// Global-memory variant (scenes whose velocity deltas do not fit LDS).
 __global__ void k_pgs_colored(Bodies b, Contacts C, const int* scene_body_off, const int* edge_range, const int* order,
-                              const int* color_off, const int* ncolors, const int* efirst, const int* ecnt, Params P) {
-	const int s = blockIdx.x;
+                              const int* color_off, const int* ncolors, const int* efirst, const int* ecnt, Params P, const int* scene_list) {
+	const int s = scene_list ? scene_list[blockIdx.x] : blockIdx.x;
 	const int b0 = scene_body_off[s], b1 = scene_body_off[s + 1];
 	const int N = b.n;
 	for (int i = b0 + threadIdx.x; i < b1; i += blockDim.x)
@@ -380,34 +398,72 @@ __global__ void k_pgs_colored(Bodies b, Contacts C, const int* scene_body_off, c
 	const int e0 = edge_range[2 * s];
 	const int* co = color_off + (size_t)s * (MAXC + 2);
 	const int nc = ncolors[s];
-	// phase 0: warm start, 1: velocity sweeps, 2: pseudo-velocity sweeps
+	auto make = [&](int i) { return DvGlobal{b.dv, i, N, b.movement[i] == 0}; };
 	int total = 1 + P.vel_iters + P.pos_iters;
 	for (int ph = 0; ph < total; ph++) {
 		int kind = ph == 0 ? 0 : (ph <= P.vel_iters ? 1 : 2);
 		for (int c = 0; c < nc; c++) {
-			int beg = co[c], end = co[c + 1];
-			bool serial = (c == MAXC);
-			int stride = serial ? 1 : blockDim.x;
-			int start = serial ? (threadIdx.x == 0 ? beg : end) : beg + threadIdx.x;
-			for (int x = start; x < end; x += stride) {
-				int e = order[e0 + x];
-				int k0 = efirst[e], k1 = k0 + ecnt[e];
-				int i1 = (int)(C.key[k0] >> 32), i2 = (int)(C.key[k0] & 0xFFFFFFFFu);
-				DvRef A{b.dv, i1, N, b.movement[i1] == 0}, B{b.dv, i2, N, b.movement[i2] == 0};
-				for (int k = k0; k < k1; k++) {
-					if (!C.live[k]) continue;
-					if (kind == 0) warm_start(C, k, A, B);
-					else if (kind == 1) {
-						normal_step(C, k, A, B, false);
-						friction_step(C, k, A, B);
-					} else normal_step(C, k, A, B, true);
-				}
-			}
+			colour_phase<DvGlobal>(C, order, e0, co[c], co[c + 1], c == MAXC, efirst, ecnt, kind, make);
 			__syncthreads();
 		}
 	}
 	for (int i = b0 + threadIdx.x; i < b1; i += blockDim.x)
 		if (b.movement[i] == 0) apply_dv(b, i);
+}
+
+// LDS variant: 6 doubles per body, the velocity half during warm start and
+// velocity sweeps, then the pseudo-velocity half during position sweeps.
+// One workgroup per scene of the launch's size class (scene_list).
+__global__ void k_pgs_colored_lds(Bodies b, Contacts C, const int* scene_list, const int* scene_body_off, const int* edge_range,
+                                  const int* order, const int* color_off, const int* ncolors, const int* efirst, const int* ecnt, Params P) {
+	extern __shared__ double sdv[];
+	const int s = scene_list[blockIdx.x];
+	const int b0 = scene_body_off[s], nbs = scene_body_off[s + 1] - b0;
+	const int N = b.n;
+	const int e0 = edge_range[2 * s];
+	const int* co = color_off + (size_t)s * (MAXC + 2);
+	const int nc = ncolors[s];
+	auto make = [&](int i) { return DvLds{sdv, i - b0, nbs, b.movement[i] == 0}; };
+	for (int half = 0; half < 2; half++) {
+		for (int x = threadIdx.x; x < 6 * nbs; x += blockDim.x) sdv[x] = 0;
+		__syncthreads();
+		int ph0 = half == 0 ? 0 : 1 + P.vel_iters;
+		int ph1 = half == 0 ? 1 + P.vel_iters : 1 + P.vel_iters + P.pos_iters;
+		for (int ph = ph0; ph < ph1; ph++) {
+			int kind = ph == 0 ? 0 : (ph <= P.vel_iters ? 1 : 2);
+			for (int c = 0; c < nc; c++) {
+				colour_phase<DvLds>(C, order, e0, co[c], co[c + 1], c == MAXC, efirst, ecnt, kind, make);
+				__syncthreads();
+			}
+		}
+		// write the half back (the residual reads it) and apply it (PhysicsEngine.cpp:1113-1136)
+		for (int li = threadIdx.x; li < nbs; li += blockDim.x) {
+			int i = b0 + li;
+			for (int c = 0; c < 6; c++) b.dv[(6 * half + c) * N + i] = sdv[c * nbs + li];
+			if (b.movement[i] != 0) continue;
+			for (int c = 0; c < 3; c++) {
+				if (half == 0) {
+					b.vel[c * N + i] += sdv[c * nbs + li];
+					b.angv[c * N + i] += sdv[(3 + c) * nbs + li];
+				} else {
+					b.pvel[c * N + i] += sdv[c * nbs + li];
+					b.pang[c * N + i] += sdv[(3 + c) * nbs + li];
+				}
+			}
+		}
+		__syncthreads();
+	}
+}
+
+// Accumulated impulses back into the persistent contact cache (warm start of
+// the next substep; Contact::warmStart state, PersistentConstraint.cpp:201-215).
+__global__ void k_rows_writeback(Contacts C, int nc) {
+	int k = blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= nc || !C.live[k]) return;
+	const double* R = C.row + (size_t)k * CP_RS;
+	C.imp_n[k] = R[CP_R_IMP];
+	C.imp_t[k] = R[CP_R_FIMP];
+	C.imp_t[C.cap + k] = R[CP_R_FIMP + 1];
 }
 
 // FP64 natural-map residual of the velocity rows after the sweeps; acc[0] +=
@@ -417,39 +473,39 @@ __global__ void k_pgs_residual(Bodies b, Contacts C, int nc, double* acc) {
 	int k = blockIdx.x * blockDim.x + threadIdx.x;
 	double r2 = 0, l2 = 0, rows = 0;
 	if (k < nc && C.live[k]) {
-		const int K = C.cap, N = b.n;
+		const int N = b.n;
+		const double* R = C.row + (size_t)k * CP_RS;
 		int i1 = (int)(C.key[k] >> 32), i2 = (int)(C.key[k] & 0xFFFFFFFFu);
 		double x = 0, y = 0;
 		for (int q = 0; q < 6; q++) {
-			x += b.dv[q * N + i1] * C.ja[q * K + k];
-			y += b.dv[q * N + i2] * C.jb[q * K + k];
+			x += b.dv[q * N + i1] * R[CP_R_JA + q];
+			y += b.dv[q * N + i2] * R[CP_R_JB + q];
 		}
-		double ln = C.imp_n[k];
-		double z = ln + C.inv_n[k] * (C.tgt_n[k] - (x + y));
+		double ln = R[CP_R_IMP];
+		double z = ln + R[CP_R_INV] * (R[CP_R_TGT] - (x + y));
 		double rn = ln - (z < 0 ? 0 : z);
 		double cur[2];
 		for (int r = 0; r < 2; r++) {
 			double a = 0, c = 0;
 			for (int q = 0; q < 6; q++) {
-				a += b.dv[q * N + i1] * C.fja[(r * 6 + q) * K + k];
-				c += b.dv[q * N + i2] * C.fjb[(r * 6 + q) * K + k];
+				a += b.dv[q * N + i1] * R[CP_R_FJA + r * 6 + q];
+				c += b.dv[q * N + i2] * R[CP_R_FJB + r * 6 + q];
 			}
 			cur[r] = a + c;
 		}
-		double d0 = C.ftgt[k] - cur[0], d1 = C.ftgt[K + k] - cur[1];
-		double t0 = C.imp_t[k], t1 = C.imp_t[K + k];
-		double z0 = t0 + C.finv[k] * d0 + C.finv[K + k] * d1;
-		double z1 = t1 + C.finv[2 * K + k] * d0 + C.finv[3 * K + k] * d1;
-		double lim = C.mu[k] * (ln > 0 ? ln : 0);
+		double d0 = R[CP_R_FTGT] - cur[0], d1 = R[CP_R_FTGT + 1] - cur[1];
+		double t0 = R[CP_R_FIMP], t1 = R[CP_R_FIMP + 1];
+		double z0 = t0 + R[CP_R_FINV] * d0 + R[CP_R_FINV + 1] * d1;
+		double z1 = t1 + R[CP_R_FINV + 2] * d0 + R[CP_R_FINV + 3] * d1;
+		double lim = R[CP_R_MU] * (ln > 0 ? ln : 0);
 		double m = sqrt(z0 * z0 + z1 * z1);
 		if (m > lim && m > 0) { z0 *= lim / m; z1 *= lim / m; }
-		double r0 = t0 - z0, r1 = t1 - z1;
-		r2 = rn * rn + r0 * r0 + r1 * r1;
+		double q0 = t0 - z0, q1 = t1 - z1;
+		r2 = rn * rn + q0 * q0 + q1 * q1;
 		l2 = ln * ln + t0 * t0 + t1 * t1;
 		rows = 3;
 	}
-	// wave reduction, one atomic per wave
-	for (int off = 32; off >= 1; off >>= 1) {
+	for (int off = 32; off >= 1; off >>= 1) {  // wave reduction, one atomic per wave
 		r2 += __shfl_xor(r2, off);
 		l2 += __shfl_xor(l2, off);
 		rows += __shfl_xor(rows, off);
@@ -463,10 +519,6 @@ __global__ void k_pgs_residual(Bodies b, Contacts C, int nc, double* acc) {
 
 static inline int nblk(int n, int t) { return (n + t - 1) / t; }
 
-void launch_pgs_residual(const Bodies& b, const Contacts& C, int nc, double* acc, hipStream_t s) {
-	if (nc) hipLaunchKernelGGL(k_pgs_residual, dim3(nblk(nc, 256)), dim3(256), 0, s, b, C, nc, acc);
-}
-
 void launch_build_rows(const Bodies& b, const Contacts& C, int nc, const Params& P, hipStream_t s) {
 	if (nc) hipLaunchKernelGGL(k_build_rows, dim3(nblk(nc, 64)), dim3(64), 0, s, b, C, nc, P);
 }
@@ -476,10 +528,30 @@ void launch_pgs_ordered(const Bodies& b, const Contacts& C, const int* rows, con
 }
 void launch_pgs_colored(const Bodies& b, const Contacts& C, const int* scene_body_off, int nscenes, const int* edge_range, const int* order,
                         const int* color_off, const int* ncolors, const int* efirst, const int* ecnt, const Params& P, int threads,
-                        hipStream_t s) {
-	if (nscenes)
-		hipLaunchKernelGGL(k_pgs_colored, dim3(nscenes), dim3(threads), 0, s, b, C, scene_body_off, edge_range, order, color_off, ncolors,
-		                   efirst, ecnt, P);
+                        hipStream_t s, const int* scene_list, int nlist) {
+	int nb = scene_list ? nlist : nscenes;
+	if (nb)
+		hipLaunchKernelGGL(k_pgs_colored, dim3(nb), dim3(threads), 0, s, b, C, scene_body_off, edge_range, order, color_off, ncolors,
+		                   efirst, ecnt, P, scene_list);
+}
+void launch_pgs_colored_lds(const Bodies& b, const Contacts& C, const int* scene_list, int nlist, int max_bodies, const int* scene_body_off,
+                            const int* edge_range, const int* order, const int* color_off, const int* ncolors, const int* efirst,
+                            const int* ecnt, const Params& P, int threads, hipStream_t s) {
+	if (!nlist) return;
+	size_t lds = (size_t)6 * max_bodies * sizeof(double);
+	static bool attr = false;
+	if (!attr) {
+		(void)hipFuncSetAttribute((const void*)k_pgs_colored_lds, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+		attr = true;
+	}
+	hipLaunchKernelGGL(k_pgs_colored_lds, dim3(nlist), dim3(threads), lds, s, b, C, scene_list, scene_body_off, edge_range, order, color_off,
+	                   ncolors, efirst, ecnt, P);
+}
+void launch_rows_writeback(const Contacts& C, int nc, hipStream_t s) {
+	if (nc) hipLaunchKernelGGL(k_rows_writeback, dim3(nblk(nc, 256)), dim3(256), 0, s, C, nc);
+}
+void launch_pgs_residual(const Bodies& b, const Contacts& C, int nc, double* acc, hipStream_t s) {
+	if (nc) hipLaunchKernelGGL(k_pgs_residual, dim3(nblk(nc, 256)), dim3(256), 0, s, b, C, nc, acc);
 }
 
 }  // namespace cpx

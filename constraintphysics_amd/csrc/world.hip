@@ -97,10 +97,7 @@ struct ContactBuf {
 		c.life = al<int>(K); c.live = al<int>(K); c.sready = al<int>(K);
 		c.imp_n = al<double>(K); c.imp_t = al<double>(2 * K);
 		c.fu = al<double>(3 * K); c.fw = al<double>(3 * K);
-		c.ja = al<double>(6 * K); c.jb = al<double>(6 * K); c.ita = al<double>(6 * K); c.itb = al<double>(6 * K);
-		c.inv_n = al<double>(K); c.tgt_n = al<double>(K); c.ptgt_n = al<double>(K); c.pimp_n = al<double>(K);
-		c.fja = al<double>(12 * K); c.fjb = al<double>(12 * K); c.fita = al<double>(12 * K); c.fitb = al<double>(12 * K);
-		c.finv = al<double>(4 * K); c.ftgt = al<double>(2 * K);
+		c.row = al<double>((size_t)CP_RS * K);
 	}
 };
 
@@ -163,6 +160,9 @@ struct cp_world {
 	int nedges = 0;
 	int live_contacts = 0;
 	DArr<double> resid_acc;
+	std::vector<std::vector<int>> cls_lists;
+	int cls_max[4] = {0, 0, 0, 0};
+	DArr<int> cls_dev[4];
 	double last_resid[4] = {0, 0, 0, 0};
 	std::vector<int64_t> last_rows, last_offs;
 	hipEvent_t ev[10];
@@ -233,6 +233,7 @@ struct cp_world {
 	void edges();
 	void schedule_reference_order();
 	void record_colored_schedule();
+	void launch_pgs_colored_lds_global(const Bodies& b, const Params& P);
 	void step();
 };
 
@@ -243,6 +244,7 @@ void cp_world::free_all() {
 	                     &bp_offs, &pt_cnt, &pt_off, &keep, &keep_scan, &rank, &e_head, &e_idx, &e_first, &e_cnt, &e_live, &e_color,
 	                     &e_order, &color_off, &ncolors, &edge_range, &applied, &isl_rows, &isl_off};
 	for (auto* a : ints) a->free_();
+	for (auto& a : cls_dev) a.free_();
 	DArr<double>* dbl[] = {&s_ref_pts, &s_gnorm, &s_gmin, &s_gmax, &s_grefval, &s_radius, &s_height, &com, &q, &vel, &angv, &pvel, &pang,
 	                       &prev_com, &prev_q, &I, &Iinv, &Iref, &Irefinv, &mass, &mat, &aabb, &wpts, &wnorm, &dv, &bp_sb};
 	for (auto* a : dbl) a->free_();
@@ -303,6 +305,20 @@ void cp_world::upload() {
 		for (int i = sbo[s]; i < sbo[s + 1]; i++) sob[i] = s;
 	up(scene_body_off, sbo);
 	up(scene_of_body, sob);
+	// PGS size classes: scenes whose velocity deltas fit in LDS run the LDS
+	// kernel (largest first within a class); larger scenes use global memory.
+	cls_lists.clear();
+	cls_lists.resize(4);
+	for (int s = 0; s < nscenes; s++) {
+		int m = sbo[s + 1] - sbo[s];
+		int c = m <= 192 ? 0 : m <= 768 ? 1 : m <= 3072 ? 2 : 3;
+		cls_lists[c].push_back(s);
+	}
+	for (int c = 0; c < 4; c++) {
+		std::sort(cls_lists[c].begin(), cls_lists[c].end(), [&](int a, int b2) { return sbo[a + 1] - sbo[a] > sbo[b2 + 1] - sbo[b2]; });
+		cls_max[c] = cls_lists[c].empty() ? 0 : sbo[cls_lists[c][0] + 1] - sbo[cls_lists[c][0]];
+		up(cls_dev[c], cls_lists[c]);
+	}
 	DArr<double>* per3[] = {&com, &vel, &angv, &pvel, &pang, &prev_com};
 	for (auto* a : per3) {
 		a->reserve(3 * (size_t)n);
@@ -543,6 +559,12 @@ void cp_world::record_colored_schedule() {
 	}
 }
 
+// Scenes too large for LDS: the global-memory kernel over those scenes only.
+void cp_world::launch_pgs_colored_lds_global(const Bodies& b, const Params& P) {
+	launch_pgs_colored(b, cb[cur].c, scene_body_off.p, nscenes, edge_range.p, e_order.p, color_off.p, ncolors.p, e_first.p, e_cnt.p, P,
+	                   cfg.solver_threads, stream, cls_dev[3].p, (int)cls_lists[3].size());
+}
+
 void cp_world::step() {
 	Params P = params();
 	Bodies b = bodies();
@@ -578,13 +600,18 @@ void cp_world::step() {
 			HIPCHK(hipMemsetAsync(applied.p, 0, (size_t)nb * sizeof(int), stream));
 			launch_pgs_ordered(b, cb[cur].c, isl_rows.p, isl_off.p, nisl, P, applied.p, stream);
 		} else {
-			launch_pgs_colored(b, cb[cur].c, scene_body_off.p, nscenes, edge_range.p, e_order.p, color_off.p, ncolors.p, e_first.p, e_cnt.p,
-			                   P, cfg.solver_threads, stream);
+			static const int thr[3] = {64, 128, 256};
+			for (int c = 0; c < 3; c++)
+				launch_pgs_colored_lds(b, cb[cur].c, cls_dev[c].p, (int)cls_lists[c].size(), cls_max[c], scene_body_off.p, edge_range.p,
+				                       e_order.p, color_off.p, ncolors.p, e_first.p, e_cnt.p, P, thr[c], stream);
+			if (!cls_lists[3].empty())
+				launch_pgs_colored_lds_global(b, P);
 		}
 		if (tev) {
 			HIPCHK(hipEventRecord(pgs_ev[2 * npgs + 1], stream));
 			npgs++;
 		}
+		launch_rows_writeback(cb[cur].c, ncontacts, stream);
 		if (s == cfg.substeps - 1 && (cfg.diagnostics & 1)) {
 			resid_acc.reserve(4);
 			HIPCHK(hipMemsetAsync(resid_acc.p, 0, 4 * sizeof(double), stream));
