@@ -40,6 +40,20 @@ def pgs_bytes_per_contact(vel_iters, pos_iters):
     return 8 * (27 + vel_iters * (28 + 60) + pos_iters * 28)
 
 
+def profiled_traffic(dtype, scenes, seed):
+    """HBM bytes per PGS solve from the committed rocprofv3 PMC summary of the
+    same bench command (FETCH_SIZE and WRITE_SIZE, copy-kernel calibrated), or
+    None when no summary covers this workload."""
+    path = os.path.join(ROOT, "profiles", f"c3_batched_{dtype}_kernel_stats.md")
+    if scenes != 4096 or seed != 20240 or not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1]), os.path.relpath(path, ROOT)
+    return None, None
+
+
 def build_world(sizes, seed, pgs_mode=0, vel_iters=10, threads=256, scene_indices=None):
     import constraintphysics_amd as cpa
     from constraintphysics_amd import scenes
@@ -197,6 +211,7 @@ def main():
                           f"host has {os.cpu_count()} cores")
         print(f"[bench] cpu baseline: {v:.0f} body-steps/s", file=sys.stderr, flush=True)
 
+    traffic, traffic_src = profiled_traffic("fp64", args.scenes, args.seed)
     out = {
         "metric": "body-steps/s vs plain GPU path at stated FP64 residual; p50/p99 step latency",
         "value": round(value, 1),
@@ -227,12 +242,13 @@ def main():
         "speedup_vs_plain": round(value / plain["value"], 2) if plain else None,
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_pgs_colored",
+            "kernel": "k_pgs_colored_lds (one substep's PGS solve: one launch per scene size class)",
             "achieved": round(achieved, 2),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": traffic_src,
             "bytes_per_contact_per_launch": bpc,
             "live_contacts": live,
             "avg_launch_ms": round(pgs_ms_per_launch, 4),

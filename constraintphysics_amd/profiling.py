@@ -1,0 +1,105 @@
+"""Summaries of rocprofv3 output for the batched step.
+
+    python3 -m constraintphysics_amd.profiling DIR [--first-step 241] [--steps 20]
+        [--fetch-factor 2.0] [--write-factor 1.0] [--contacts N]
+
+DIR holds `c3_kernel_trace.csv` (rocprofv3 --kernel-trace) and, optionally,
+`fetch_counter_collection.csv` / `write_counter_collection.csv` (rocprofv3
+--pmc FETCH_SIZE / WRITE_SIZE) of the same bench command.  Steps are counted
+by k_gravity dispatches (one per step); the window is steps
+[first_step, first_step + steps).  The PGS solve of a substep is every PGS
+dispatch between that substep's k_build_rows and k_rows_writeback.  Counter
+values (KiB per dispatch) are scaled by the factors a plain copy kernel of
+known size gives for the same counters.
+"""
+import argparse
+import collections
+import csv
+import os
+
+PGS_KERNELS = ("k_pgs_colored_lds", "k_pgs_colored(", "k_pgs_colored32", "k_pgs_ordered")
+
+
+def _rows(path):
+    with open(path) as f:
+        return sorted(csv.DictReader(f), key=lambda r: int(r["Dispatch_Id"]))
+
+
+def _is_pgs(name):
+    return any(p in name for p in PGS_KERNELS)
+
+
+def pgs_groups(rows, value):
+    """{step: [per-substep sum of value(row) over PGS dispatches]}."""
+    step, cur = 0, None
+    out = collections.defaultdict(list)
+    for r in rows:
+        n = r["Kernel_Name"]
+        if "k_gravity" in n:
+            step += 1
+        if "k_build_rows" in n:
+            cur = 0.0
+        elif _is_pgs(n) and cur is not None:
+            cur += value(r)
+        elif "k_rows_writeback" in n and cur is not None:
+            out[step].append(cur)
+            cur = None
+    return out
+
+
+def _dur_ms(r):
+    return (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-6
+
+
+def summarize(d, first_step=241, steps=20, fetch_factor=2.0, write_factor=1.0, contacts=None, top=14):
+    lines = []
+    win = range(first_step, first_step + steps)
+    tr = _rows(os.path.join(d, "c3_kernel_trace.csv"))
+    g = pgs_groups(tr, _dur_ms)
+    vals = [v for s in win for v in g.get(s, [])]
+    pgs_ms = sum(vals) / max(1, len(vals))
+    lines.append(f"window steps {first_step}..{first_step + steps - 1}: {len(vals)} PGS solves, avg {pgs_ms:.4f} ms per solve")
+    per = collections.defaultdict(list)
+    step = 0
+    for r in tr:
+        if "k_gravity" in r["Kernel_Name"]:
+            step += 1
+        if step in win:
+            per[r["Kernel_Name"].split("(")[0]].append(_dur_ms(r))
+    tot = sum(sum(v) for v in per.values())
+    for k, v in sorted(per.items(), key=lambda x: -sum(x[1]))[:top]:
+        lines.append("%-48s calls %5d  avg %9.4f ms  per step %8.3f ms  %5.1f%%" % (k[-48:], len(v), sum(v) / len(v), sum(v) / steps,
+                                                                                    100 * sum(v) / tot))
+    lines.append("sum of kernel time per step: %.3f ms" % (tot / steps))
+    traffic = 0.0
+    for c, nm, f in (("FETCH_SIZE", "fetch", fetch_factor), ("WRITE_SIZE", "write", write_factor)):
+        p = os.path.join(d, f"{nm}_counter_collection.csv")
+        if not os.path.exists(p):
+            return lines, pgs_ms, None
+        g = pgs_groups(_rows(p), lambda r: float(r["Counter_Value"]))
+        v = [x for s in win for x in g.get(s, [])]
+        kib = sum(v) / max(1, len(v))
+        b = f * kib * 1024
+        traffic += b
+        lines.append(f"{c}: {len(v)} PGS solves, avg {kib:.1f} KiB x {f} = {b / 1e9:.4f} GB per solve")
+    lines.append(f"traffic_bytes_per_launch: {traffic:.6e}")
+    if contacts:
+        lines.append(f"per contact point: {traffic / contacts:.1f} B")
+    return lines, pgs_ms, traffic
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("dir")
+    ap.add_argument("--first-step", type=int, default=241)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--fetch-factor", type=float, default=2.0)
+    ap.add_argument("--write-factor", type=float, default=1.0)
+    ap.add_argument("--contacts", type=float, default=None)
+    a = ap.parse_args()
+    lines, _, _ = summarize(a.dir, a.first_step, a.steps, a.fetch_factor, a.write_factor, a.contacts)
+    print("\n".join(lines))
+
+
+if __name__ == "__main__":
+    main()
