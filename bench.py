@@ -30,14 +30,14 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-def pgs_bytes_per_contact(vel_iters, pos_iters):
+def pgs_bytes_per_contact(vel_iters, pos_iters, word=8):
     """Algorithmic FP64 bytes one PGS launch moves per live contact point:
     warm start (M^-1 J^T of 3 rows + 3 impulses), per velocity sweep the normal
     row (J 12, M^-1 J^T 12, inverse, target, impulse r/w = 28 doubles) and the
     friction pair (J 24, M^-1 J^T 24, inverse 4, targets 2, impulses r/w 4,
     mu, normal impulse = 60 doubles), per position sweep the normal row again
     (28 doubles)."""
-    return 8 * (27 + vel_iters * (28 + 60) + pos_iters * 28)
+    return word * (27 + vel_iters * (28 + 60) + pos_iters * 28)
 
 
 def profiled_traffic(dtype, scenes, seed):
@@ -54,11 +54,14 @@ def profiled_traffic(dtype, scenes, seed):
     return None, None
 
 
-def build_world(sizes, seed, pgs_mode=0, vel_iters=10, threads=256, scene_indices=None):
+PRECISIONS = {"fp64": 0, "split": 1}
+
+
+def build_world(sizes, seed, pgs_mode=0, vel_iters=10, threads=256, scene_indices=None, precision="split"):
     import constraintphysics_amd as cpa
     from constraintphysics_amd import scenes
     w = cpa.World(dt=1.0 / 120.0, gravity=(0.0, -6.0, 0.0), vel_iters=vel_iters, pos_iters=1, substeps=8, pgs_mode=pgs_mode,
-                  solver_threads=threads)
+                  solver_threads=threads, precision=PRECISIONS[precision])
     info = scenes.build_batched(w, sizes, seed, scene_indices=scene_indices)
     return w, info
 
@@ -134,6 +137,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--precision", choices=sorted(PRECISIONS), default="split",
+                    help="solve precision: split = FP32 rows/impulses/velocity deltas over FP64 state; fp64 = reference arithmetic")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: --gpus {args.gpus} is not supported (scenes would shard by HBM footprint "
@@ -143,7 +148,7 @@ def main():
     from constraintphysics_amd import scenes
     sizes = scenes.c3_sizes(args.scenes, args.seed)
     t_build = time.perf_counter()
-    w, info = build_world(sizes, args.seed)
+    w, info = build_world(sizes, args.seed, precision=args.precision)
     t_build = time.perf_counter() - t_build
     n_dyn = int(info["dynamic_bodies"])
     print(f"[bench] built {args.scenes} scenes, {n_dyn} dynamic bodies in {t_build:.1f}s", file=sys.stderr, flush=True)
@@ -162,10 +167,26 @@ def main():
         w.step(1)
         lat.append(time.perf_counter() - t0)
         tim.append(w.last_timing())
-    # FP64 residual of the solve, measured on one extra (untimed) step
-    w.set_config(diagnostics=1)
-    w.step(1)
-    resid = w.last_residual()
+    # Accuracy, on one extra (untimed) step from the state after the timed
+    # window, solved once in FP64 and once in the split precision: the FP64
+    # natural-map residual of each solve, and the split result's deviation
+    # from the FP64 result (the FP64 path replays the oracle's sequential
+    # order to 1e-12 in the parity tests).
+    acc_state, acc_contacts = w.state(), w.contacts()
+    acc = {}
+    for name in ("fp64", "split"):
+        w.set_state(acc_state)
+        w.set_contacts(*acc_contacts)
+        w.set_config(diagnostics=1, precision=PRECISIONS[name])
+        w.step(1)
+        acc[name] = (w.last_residual(), w.state(), w.contacts()[2])
+    resid = acc[args.precision][0]
+    dev_imp = float(np.linalg.norm(acc["split"][2][:, 12:15] - acc["fp64"][2][:, 12:15]) /
+                    max(1e-300, np.linalg.norm(acc["fp64"][2][:, 12:15])))
+    dev_vel = float(np.linalg.norm(acc["split"][1][:, 7:13] - acc["fp64"][1][:, 7:13]) /
+                    max(1e-300, np.linalg.norm(acc["fp64"][1][:, 7:13])))
+    print(f"[bench] split vs fp64: impulses {dev_imp:.3e}, velocities {dev_vel:.3e} relative; residual fp64 "
+          f"{acc['fp64'][0][2]:.3e} split {acc['split'][0][2]:.3e}", file=sys.stderr, flush=True)
     lat = np.array(lat)
     tim = np.array(tim)
     total = float(lat.sum())
@@ -173,7 +194,7 @@ def main():
     ms_step = 1e3 * total / args.steps
     pgs_ms_per_launch = float(tim[:, 5].sum() / max(1.0, tim[:, 6].sum()))
     live = float(tim[:, 7].mean())
-    bpc = pgs_bytes_per_contact(10, 1)
+    bpc = pgs_bytes_per_contact(10, 1, 8 if args.precision == "fp64" else 4)
     achieved = live * bpc / (pgs_ms_per_launch * 1e-3) / 1e9 if pgs_ms_per_launch > 0 else 0.0
     phases = dict(zip(["broadphase_ms", "narrowphase_ms", "persist_schedule_ms", "substeps_ms", "geometry_ms", "pgs_ms"],
                       [round(float(x), 3) for x in tim[:, :6].mean(0)]))
@@ -187,7 +208,7 @@ def main():
         pick = [int(order[int(q * (len(order) - 1))]) for q in np.linspace(0.02, 0.98, args.plain_scenes)]
         pb, pt = 0, 0.0
         for s in pick:
-            ws, inf = build_world(sizes, args.seed, scene_indices=[s])
+            ws, inf = build_world(sizes, args.seed, scene_indices=[s], precision=args.precision)
             st, cache = scene_snapshot(snap_state, snap_contacts, off, s)
             ws.set_state(st)
             ws.set_contacts(*cache)
@@ -211,7 +232,7 @@ def main():
                           f"host has {os.cpu_count()} cores")
         print(f"[bench] cpu baseline: {v:.0f} body-steps/s", file=sys.stderr, flush=True)
 
-    traffic, traffic_src = profiled_traffic("fp64", args.scenes, args.seed)
+    traffic, traffic_src = profiled_traffic(args.precision, args.scenes, args.seed)
     out = {
         "metric": "body-steps/s vs plain GPU path at stated FP64 residual; p50/p99 step latency",
         "value": round(value, 1),
@@ -225,19 +246,24 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == "fp64" else "f32 solve / f64 state",
         "data": "synthetic",
         "config": {
             "workload": f"batched scenes: {args.scenes} scenes, bodies log-uniform [8, 2048] (seed {args.seed}, per-scene seed base+i), "
                         f"box containers at 20k/60^3 bodies per m^3, v ~ N(0,1); dt 1/120, 8 substeps, PGS 10 vel + 1 pos sweeps, "
-                        f"graph-coloured PGS, FP64",
+                        f"graph-coloured PGS, " + ("FP64" if args.precision == "fp64" else "FP32 PGS rows/impulses over FP64 state"),
             "scenes": args.scenes,
             "dynamic_bodies": n_dyn,
             "primitive_mix": info["mix"],
             "phases_ms": phases,
         },
         "fp64_residual": {"relative": float(resid[2]), "abs": float(resid[0]), "lambda_norm": float(resid[1]), "rows": int(resid[3]),
-                          "note": "natural-map residual of the last substep's velocity rows after 10 coloured sweeps, FP64"},
+                          "note": "natural-map residual ||lambda - proj(lambda + A^-1 (t - J dv))|| / ||lambda|| of the last substep's "
+                                  "velocity rows after 10 coloured sweeps, evaluated in FP64"},
+        "fp64_oracle_deviation": {"impulses_rel": dev_imp, "velocities_rel": dev_vel,
+                                  "note": "one step from the same state solved in the split precision vs FP64 (the FP64 coloured "
+                                          "solve equals the oracle replaying its schedule to 1e-12): relative 2-norm over all "
+                                          "contact impulses / body velocities"},
         "plain_gpu_path": plain,
         "speedup_vs_plain": round(value / plain["value"], 2) if plain else None,
         "roofline": {

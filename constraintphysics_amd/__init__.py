@@ -36,6 +36,7 @@ class Config(ctypes.Structure):
         ("solver_threads", ctypes.c_int),
         ("diagnostics", ctypes.c_int),
         ("defect_fixes", ctypes.c_int),
+        ("precision", ctypes.c_int),
     ]
 
 

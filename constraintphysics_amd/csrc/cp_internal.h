@@ -17,6 +17,7 @@
 #include <stdint.h>
 
 #include "cp_math.h"
+#include "../../include/cp_abi.h"
 
 namespace cpx {
 
@@ -63,6 +64,7 @@ struct Params {
 	int vel_iters, pos_iters, substeps, contact_life;
 	int gyro_ticks;
 	int defect_fixes;
+	int precision;  // cp_precision (CP_PREC_FP64 or CP_PREC_SPLIT)
 };
 
 // Narrowphase output, one slot per candidate pair.
@@ -150,8 +152,8 @@ void launch_build_rows(const Bodies& b, const Contacts& C, int nc, const Params&
 void launch_pgs_colored_lds(const Bodies& b, const Contacts& C, const int* scene_list, int nlist, int max_bodies, const int* scene_body_off,
                             const int* edge_range, const int* order, const int* color_off, const int* ncolors, const int* efirst,
                             const int* ecnt, const Params& P, int threads, hipStream_t s);
-void launch_rows_writeback(const Contacts& C, int nc, hipStream_t s);
-void launch_pgs_residual(const Bodies& b, const Contacts& C, int nc, double* acc, hipStream_t s);
+void launch_rows_writeback(const Contacts& C, int nc, const Params& P, hipStream_t s);
+void launch_pgs_residual(const Bodies& b, const Contacts& C, int nc, const Params& P, double* acc, hipStream_t s);
 void launch_pgs_ordered(const Bodies& b, const Contacts& C, const int* rows, const int* isl_off, int nisl, const Params& P, int* applied,
                         hipStream_t s);
 void launch_pgs_colored(const Bodies& b, const Contacts& C, const int* scene_body_off, int nscenes, const int* edge_range, const int* order,

@@ -5,8 +5,11 @@
 // 240-272) and FrictionConstraint (:277-343), plus the per-solve target values
 // (:265-268 and ConstraintSolver.h:78).  Matrix products keep NMat's
 // accumulation order (NMat.h:50-60, :91-103) so FP64 rows match bit for bit.
-// Each contact's rows are one contiguous record of CP_RS doubles, so a row
+// Each contact's rows are one contiguous record of CP_RS values, so a row
 // update reads a few whole cache lines instead of one line per component.
+// Precision: FP64 records and deltas (parity), or the split path with FP32
+// records, impulses and velocity deltas over the FP64 body state (rows are
+// derived in FP64, deltas are added to FP64 velocities, integration is FP64).
 //
 // Sweep schedules over the same rows (ConstraintSolver.cpp:80-213):
 //  * k_pgs_ordered: the reference's sequential order (one thread per island,
@@ -30,11 +33,26 @@ namespace cpx {
 
 __device__ __forceinline__ V3 ld3(const double* a, int i, int n) { return V3{a[i], a[n + i], a[2 * n + i]}; }
 
+// Row record of contact k, CP_RS values of the solve precision T.
+template <class T>
+__device__ __forceinline__ T* rec(const Contacts& C, int k) {
+	return reinterpret_cast<T*>(C.row) + (size_t)k * CP_RS;
+}
+
+template <class T>
+__device__ __forceinline__ T tsqrt(T x) {
+	if constexpr (sizeof(T) == 4) return sqrtf(x);
+	else return sqrt(x);
+}
+
+// Rows are derived in FP64 from the FP64 state and stored in the solve
+// precision T (FP64, or FP32 for the split path).
+template <class T>
 __global__ void k_build_rows(Bodies b, Contacts C, int nc, Params P) {
 	int k = blockIdx.x * blockDim.x + threadIdx.x;
 	if (k >= nc || !C.live[k]) return;
 	const int N = b.n, K = C.cap;
-	double* R = C.row + (size_t)k * CP_RS;
+	T* R = rec<T>(C, k);
 	int i1 = (int)(C.key[k] >> 32), i2 = (int)(C.key[k] & 0xFFFFFFFFu);
 	Q q1{b.q[i1], b.q[N + i1], b.q[2 * N + i1], b.q[3 * N + i1]};
 	Q q2{b.q[i2], b.q[N + i2], b.q[2 * N + i2], b.q[3 * N + i2]};
@@ -173,46 +191,49 @@ __global__ void k_build_rows(Bodies b, Contacts C, int nc, Params P) {
 }
 
 // ------------------------------------------------------------- row updates
-// Velocity-delta accessors: global memory [12][n] (components 0-5 velocity,
-// 6-11 pseudo-velocity) or LDS [6][nbs] holding one half at a time.
+// Velocity-delta accessors in the solve precision T: global memory [12][n]
+// (components 0-5 velocity, 6-11 pseudo-velocity) or LDS [6][nbs] holding one
+// half at a time.
+template <class T>
 struct DvGlobal {
-	double* base;
+	T* base;
 	int i, n;
 	bool dyn;
-	__device__ double get(int c) const { return base[c * n + i]; }
-	__device__ void add(int c, double v) const {
+	__device__ T get(int c) const { return base[c * n + i]; }
+	__device__ void add(int c, T v) const {
 		if (dyn) base[c * n + i] += v;
 	}
 };
+template <class T>
 struct DvLds {
-	double* s;
+	T* s;
 	int i, nbs;  // scene-local body index
 	bool dyn;
-	__device__ double get(int c) const { return s[(c % 6) * nbs + i]; }
-	__device__ void add(int c, double v) const {
+	__device__ T get(int c) const { return s[(c % 6) * nbs + i]; }
+	__device__ void add(int c, T v) const {
 		if (dyn) s[(c % 6) * nbs + i] += v;
 	}
 };
 
 // applyWarmStartVelocityChange of the contact row (kind 0) or the friction
 // pair (kind 1) (ConstraintSolver.cpp:121-126).
-template <class D>
-__device__ __forceinline__ void warm_start(const double* R, const D& A, const D& B, int kind) {
+template <class T, class D>
+__device__ __forceinline__ void warm_start(const T* R, const D& A, const D& B, int kind) {
 	if (kind == 0) {
-		double imp = R[CP_R_IMP];
+		T imp = R[CP_R_IMP];
 		if (is_zero1(imp)) return;
 		for (int r = 0; r < 6; r++) {
-			double da = 0, db = 0;
+			T da = 0, db = 0;
 			da += imp * R[CP_R_ITA + r];
 			db += imp * R[CP_R_ITB + r];
 			A.add(r, da);
 			B.add(r, db);
 		}
 	} else {
-		double t0 = R[CP_R_FIMP], t1 = R[CP_R_FIMP + 1];
+		T t0 = R[CP_R_FIMP], t1 = R[CP_R_FIMP + 1];
 		if (is_zero2(t0, t1)) return;
 		for (int r = 0; r < 6; r++) {
-			double da = 0, db = 0;
+			T da = 0, db = 0;
 			da += t0 * R[CP_R_FITA + r * 2];
 			da += t1 * R[CP_R_FITA + r * 2 + 1];
 			db += t0 * R[CP_R_FITB + r * 2];
@@ -224,26 +245,26 @@ __device__ __forceinline__ void warm_start(const double* R, const D& A, const D&
 }
 
 // PGSConstraintStep<1> for the contact row (ConstraintSolver.cpp:12-40, :270-272).
-template <class D>
-__device__ __forceinline__ void normal_step(double* R, const D& A, const D& B, bool pseudo) {
+template <class T, class D>
+__device__ __forceinline__ void normal_step(T* R, const D& A, const D& B, bool pseudo) {
 	int off = pseudo ? 6 : 0;
-	double x = 0, y = 0;
+	T x = 0, y = 0;
 	for (int q = 0; q < 6; q++) {
 		x += A.get(off + q) * R[CP_R_JA + q];
 		y += B.get(off + q) * R[CP_R_JB + q];
 	}
-	double cur = x + y;
-	double tgt = pseudo ? R[CP_R_PTGT] : R[CP_R_TGT];
-	double acc = pseudo ? R[CP_R_PIMP] : R[CP_R_IMP];
-	double delta = tgt - cur;
-	double m = 0;
+	T cur = x + y;
+	T tgt = pseudo ? R[CP_R_PTGT] : R[CP_R_TGT];
+	T acc = pseudo ? R[CP_R_PIMP] : R[CP_R_IMP];
+	T delta = tgt - cur;
+	T m = 0;
 	m += delta * R[CP_R_INV];
-	double nw = acc + m;
-	double pr = (nw < 0) ? 0 : nw;  // std::max<double>(nw, 0)
-	double diff = pr - acc;
+	T nw = acc + m;
+	T pr = (nw < 0) ? T(0) : nw;  // std::max<double>(nw, 0)
+	T diff = pr - acc;
 	if (!is_zero1(diff)) {
 		for (int r = 0; r < 6; r++) {
-			double da = 0, db = 0;
+			T da = 0, db = 0;
 			da += diff * R[CP_R_ITA + r];
 			db += diff * R[CP_R_ITB + r];
 			A.add(off + r, da);
@@ -254,33 +275,34 @@ __device__ __forceinline__ void normal_step(double* R, const D& A, const D& B, b
 }
 
 // PGSConstraintStep<2> with FrictionConstraint::projectValidImpulse (ConstraintSolver.cpp:331-343).
-template <class D>
-__device__ __forceinline__ void friction_step(double* R, int* sready, const D& A, const D& B) {
-	double cur[2];
+template <class T, class D>
+__device__ __forceinline__ void friction_step(T* R, int* sready, const D& A, const D& B) {
+	T cur[2];
 	for (int r = 0; r < 2; r++) {
-		double x = 0, y = 0;
+		T x = 0, y = 0;
 		for (int q = 0; q < 6; q++) {
 			x += A.get(q) * R[CP_R_FJA + r * 6 + q];
 			y += B.get(q) * R[CP_R_FJB + r * 6 + q];
 		}
 		cur[r] = x + y;
 	}
-	double d0 = R[CP_R_FTGT] - cur[0], d1 = R[CP_R_FTGT + 1] - cur[1];
-	double m0 = 0, m1 = 0;
+	T d0 = R[CP_R_FTGT] - cur[0], d1 = R[CP_R_FTGT + 1] - cur[1];
+	T m0 = 0, m1 = 0;
 	m0 += d0 * R[CP_R_FINV];
 	m0 += d1 * R[CP_R_FINV + 1];
 	m1 += d0 * R[CP_R_FINV + 2];
 	m1 += d1 * R[CP_R_FINV + 3];
-	double i0 = R[CP_R_FIMP], i1 = R[CP_R_FIMP + 1];
-	double n0 = i0 + m0, n1 = i1 + m1;
-	double ni = R[CP_R_IMP];
-	double lim = (ni < __builtin_inf()) ? ni : __builtin_inf();
-	double maxm = R[CP_R_MU] * lim;
-	double cm = sqrt(n0 * n0 + n1 * n1);
-	double p0, p1;
+	T i0 = R[CP_R_FIMP], i1 = R[CP_R_FIMP + 1];
+	T n0 = i0 + m0, n1 = i1 + m1;
+	T ni = R[CP_R_IMP];
+	const T inf = (T)__builtin_inf();
+	T lim = (ni < inf) ? ni : inf;
+	T maxm = R[CP_R_MU] * lim;
+	T cm = tsqrt(n0 * n0 + n1 * n1);
+	T p0, p1;
 	if (cm > maxm) {
 		*sready = 0;
-		double q = maxm / cm;
+		T q = maxm / cm;
 		p0 = n0 * q;
 		p1 = n1 * q;
 	} else {
@@ -288,10 +310,10 @@ __device__ __forceinline__ void friction_step(double* R, int* sready, const D& A
 		p0 = n0;
 		p1 = n1;
 	}
-	double e0 = p0 - i0, e1 = p1 - i1;
+	T e0 = p0 - i0, e1 = p1 - i1;
 	if (!is_zero2(e0, e1)) {
 		for (int r = 0; r < 6; r++) {
-			double da = 0, db = 0;
+			T da = 0, db = 0;
 			da += e0 * R[CP_R_FITA + r * 2];
 			da += e1 * R[CP_R_FITA + r * 2 + 1];
 			db += e0 * R[CP_R_FITB + r * 2];
@@ -304,38 +326,42 @@ __device__ __forceinline__ void friction_step(double* R, int* sready, const D& A
 	}
 }
 
-__device__ __forceinline__ void apply_dv(Bodies& b, int i) {  // PhysicsEngine.cpp:1113-1136
+// applyVelocityChange (PhysicsEngine.cpp:1113-1136): velocity deltas in T
+// added to the FP64 velocities.
+template <class T>
+__device__ __forceinline__ void apply_dv(Bodies& b, const T* dv, int i) {
 	const int N = b.n;
 	for (int c = 0; c < 3; c++) {
-		b.vel[c * N + i] += b.dv[c * N + i];
-		b.angv[c * N + i] += b.dv[(3 + c) * N + i];
-		b.pvel[c * N + i] += b.dv[(6 + c) * N + i];
-		b.pang[c * N + i] += b.dv[(9 + c) * N + i];
+		b.vel[c * N + i] += (double)dv[c * N + i];
+		b.angv[c * N + i] += (double)dv[(3 + c) * N + i];
+		b.pvel[c * N + i] += (double)dv[(6 + c) * N + i];
+		b.pang[c * N + i] += (double)dv[(9 + c) * N + i];
 	}
 }
 
-// Sequential island solve: rows[2*r] = contact index, rows[2*r+1] = 0 normal / 1 friction.
+// Sequential island solve (FP64 only): rows[2*r] = contact index, rows[2*r+1] = 0 normal / 1 friction.
 __global__ void k_pgs_ordered(Bodies b, Contacts C, const int* rows, const int* isl_off, int nisl, Params P, int* applied) {
 	int t = blockIdx.x * blockDim.x + threadIdx.x;
 	if (t >= nisl) return;
 	int r0 = isl_off[t], r1 = isl_off[t + 1];
-	auto refs = [&](int k, DvGlobal& A, DvGlobal& B) {
+	using DG = DvGlobal<double>;
+	auto refs = [&](int k, DG& A, DG& B) {
 		int i1 = (int)(C.key[k] >> 32), i2 = (int)(C.key[k] & 0xFFFFFFFFu);
-		A = DvGlobal{b.dv, i1, b.n, b.movement[i1] == 0};
-		B = DvGlobal{b.dv, i2, b.n, b.movement[i2] == 0};
+		A = DG{b.dv, i1, b.n, b.movement[i1] == 0};
+		B = DG{b.dv, i2, b.n, b.movement[i2] == 0};
 	};
 	for (int r = r0; r < r1; r++) {
 		int k = rows[2 * r];
-		DvGlobal A, B;
+		DG A, B;
 		refs(k, A, B);
-		warm_start(C.row + (size_t)k * CP_RS, A, B, rows[2 * r + 1]);
+		warm_start(rec<double>(C, k), A, B, rows[2 * r + 1]);
 	}
 	for (int it = 0; it < P.vel_iters; it++)
 		for (int r = r0; r < r1; r++) {
 			int k = rows[2 * r];
-			DvGlobal A, B;
+			DG A, B;
 			refs(k, A, B);
-			double* R = C.row + (size_t)k * CP_RS;
+			double* R = rec<double>(C, k);
 			if (rows[2 * r + 1] == 0) normal_step(R, A, B, false);
 			else friction_step(R, &C.sready[k], A, B);
 		}
@@ -343,9 +369,9 @@ __global__ void k_pgs_ordered(Bodies b, Contacts C, const int* rows, const int* 
 		for (int r = r0; r < r1; r++) {
 			if (rows[2 * r + 1] != 0) continue;
 			int k = rows[2 * r];
-			DvGlobal A, B;
+			DG A, B;
 			refs(k, A, B);
-			normal_step(C.row + (size_t)k * CP_RS, A, B, true);
+			normal_step(rec<double>(C, k), A, B, true);
 		}
 	for (int r = r0; r < r1; r++) {
 		int k = rows[2 * r];
@@ -354,7 +380,7 @@ __global__ void k_pgs_ordered(Bodies b, Contacts C, const int* rows, const int* 
 			int i = ii[q];
 			if (b.movement[i] != 0 || applied[i]) continue;
 			applied[i] = 1;
-			apply_dv(b, i);
+			apply_dv(b, b.dv, i);
 		}
 	}
 }
@@ -362,7 +388,7 @@ __global__ void k_pgs_ordered(Bodies b, Contacts C, const int* rows, const int* 
 // One colour of one sweep: each thread runs the contacts of its edges in cache
 // order; the colour index MAXC holds edges that did not fit a colour and runs
 // on thread 0 alone.  kind 0 warm start, 1 velocity sweep, 2 position sweep.
-template <class D, class Make>
+template <class T, class D, class Make>
 __device__ __forceinline__ void colour_phase(Contacts& C, const int* order, int e0, int beg, int end, bool serial, const int* efirst,
                                              const int* ecnt, int kind, Make make) {
 	int stride = serial ? 1 : blockDim.x;
@@ -374,7 +400,7 @@ __device__ __forceinline__ void colour_phase(Contacts& C, const int* order, int 
 		D A = make(i1), B = make(i2);
 		for (int k = k0; k < k1; k++) {
 			if (!C.live[k]) continue;
-			double* R = C.row + (size_t)k * CP_RS;
+			T* R = rec<T>(C, k);
 			if (kind == 0) {
 				warm_start(R, A, B, 0);
 				warm_start(R, A, B, 1);
@@ -386,44 +412,50 @@ __device__ __forceinline__ void colour_phase(Contacts& C, const int* order, int 
 	}
 }
 
-// Global-memory variant (scenes whose velocity deltas do not fit LDS).
+// Global-memory variant (scenes whose velocity deltas do not fit LDS).  The
+// deltas live in b.dv viewed as [12][n] values of T.
+template <class T>
 __global__ void k_pgs_colored(Bodies b, Contacts C, const int* scene_body_off, const int* edge_range, const int* order,
                               const int* color_off, const int* ncolors, const int* efirst, const int* ecnt, Params P, const int* scene_list) {
 	const int s = scene_list ? scene_list[blockIdx.x] : blockIdx.x;
 	const int b0 = scene_body_off[s], b1 = scene_body_off[s + 1];
 	const int N = b.n;
+	T* dv = reinterpret_cast<T*>(b.dv);
 	for (int i = b0 + threadIdx.x; i < b1; i += blockDim.x)
-		for (int c = 0; c < 12; c++) b.dv[c * N + i] = 0;
+		for (int c = 0; c < 12; c++) dv[c * N + i] = 0;
 	__syncthreads();
 	const int e0 = edge_range[2 * s];
 	const int* co = color_off + (size_t)s * (MAXC + 2);
 	const int nc = ncolors[s];
-	auto make = [&](int i) { return DvGlobal{b.dv, i, N, b.movement[i] == 0}; };
+	auto make = [&](int i) { return DvGlobal<T>{dv, i, N, b.movement[i] == 0}; };
 	int total = 1 + P.vel_iters + P.pos_iters;
 	for (int ph = 0; ph < total; ph++) {
 		int kind = ph == 0 ? 0 : (ph <= P.vel_iters ? 1 : 2);
 		for (int c = 0; c < nc; c++) {
-			colour_phase<DvGlobal>(C, order, e0, co[c], co[c + 1], c == MAXC, efirst, ecnt, kind, make);
+			colour_phase<T, DvGlobal<T>>(C, order, e0, co[c], co[c + 1], c == MAXC, efirst, ecnt, kind, make);
 			__syncthreads();
 		}
 	}
 	for (int i = b0 + threadIdx.x; i < b1; i += blockDim.x)
-		if (b.movement[i] == 0) apply_dv(b, i);
+		if (b.movement[i] == 0) apply_dv(b, dv, i);
 }
 
-// LDS variant: 6 doubles per body, the velocity half during warm start and
+// LDS variant: 6 values of T per body, the velocity half during warm start and
 // velocity sweeps, then the pseudo-velocity half during position sweeps.
 // One workgroup per scene of the launch's size class (scene_list).
+template <class T>
 __global__ void k_pgs_colored_lds(Bodies b, Contacts C, const int* scene_list, const int* scene_body_off, const int* edge_range,
                                   const int* order, const int* color_off, const int* ncolors, const int* efirst, const int* ecnt, Params P) {
-	extern __shared__ double sdv[];
+	extern __shared__ double lds_raw[];
+	T* sdv = reinterpret_cast<T*>(lds_raw);
 	const int s = scene_list[blockIdx.x];
 	const int b0 = scene_body_off[s], nbs = scene_body_off[s + 1] - b0;
 	const int N = b.n;
+	T* dv = reinterpret_cast<T*>(b.dv);
 	const int e0 = edge_range[2 * s];
 	const int* co = color_off + (size_t)s * (MAXC + 2);
 	const int nc = ncolors[s];
-	auto make = [&](int i) { return DvLds{sdv, i - b0, nbs, b.movement[i] == 0}; };
+	auto make = [&](int i) { return DvLds<T>{sdv, i - b0, nbs, b.movement[i] == 0}; };
 	for (int half = 0; half < 2; half++) {
 		for (int x = threadIdx.x; x < 6 * nbs; x += blockDim.x) sdv[x] = 0;
 		__syncthreads();
@@ -432,22 +464,22 @@ __global__ void k_pgs_colored_lds(Bodies b, Contacts C, const int* scene_list, c
 		for (int ph = ph0; ph < ph1; ph++) {
 			int kind = ph == 0 ? 0 : (ph <= P.vel_iters ? 1 : 2);
 			for (int c = 0; c < nc; c++) {
-				colour_phase<DvLds>(C, order, e0, co[c], co[c + 1], c == MAXC, efirst, ecnt, kind, make);
+				colour_phase<T, DvLds<T>>(C, order, e0, co[c], co[c + 1], c == MAXC, efirst, ecnt, kind, make);
 				__syncthreads();
 			}
 		}
 		// write the half back (the residual reads it) and apply it (PhysicsEngine.cpp:1113-1136)
 		for (int li = threadIdx.x; li < nbs; li += blockDim.x) {
 			int i = b0 + li;
-			for (int c = 0; c < 6; c++) b.dv[(6 * half + c) * N + i] = sdv[c * nbs + li];
+			for (int c = 0; c < 6; c++) dv[(6 * half + c) * N + i] = sdv[c * nbs + li];
 			if (b.movement[i] != 0) continue;
 			for (int c = 0; c < 3; c++) {
 				if (half == 0) {
-					b.vel[c * N + i] += sdv[c * nbs + li];
-					b.angv[c * N + i] += sdv[(3 + c) * nbs + li];
+					b.vel[c * N + i] += (double)sdv[c * nbs + li];
+					b.angv[c * N + i] += (double)sdv[(3 + c) * nbs + li];
 				} else {
-					b.pvel[c * N + i] += sdv[c * nbs + li];
-					b.pang[c * N + i] += sdv[(3 + c) * nbs + li];
+					b.pvel[c * N + i] += (double)sdv[c * nbs + li];
+					b.pang[c * N + i] += (double)sdv[(3 + c) * nbs + li];
 				}
 			}
 		}
@@ -455,49 +487,52 @@ __global__ void k_pgs_colored_lds(Bodies b, Contacts C, const int* scene_list, c
 	}
 }
 
-// Accumulated impulses back into the persistent contact cache (warm start of
-// the next substep; Contact::warmStart state, PersistentConstraint.cpp:201-215).
+// Accumulated impulses back into the persistent FP64 contact cache (warm start
+// of the next substep; Contact::warmStart state, PersistentConstraint.cpp:201-215).
+template <class T>
 __global__ void k_rows_writeback(Contacts C, int nc) {
 	int k = blockIdx.x * blockDim.x + threadIdx.x;
 	if (k >= nc || !C.live[k]) return;
-	const double* R = C.row + (size_t)k * CP_RS;
-	C.imp_n[k] = R[CP_R_IMP];
-	C.imp_t[k] = R[CP_R_FIMP];
-	C.imp_t[C.cap + k] = R[CP_R_FIMP + 1];
+	const T* R = rec<T>(C, k);
+	C.imp_n[k] = (double)R[CP_R_IMP];
+	C.imp_t[k] = (double)R[CP_R_FIMP];
+	C.imp_t[C.cap + k] = (double)R[CP_R_FIMP + 1];
 }
 
-// FP64 natural-map residual of the velocity rows after the sweeps; acc[0] +=
-// sum r^2, acc[1] += sum lambda^2, acc[2] += rows.  Reads the final velocity
-// deltas (b.dv) of the solve, so it runs right after the PGS launch.
+// FP64 natural-map residual of the velocity rows after the sweeps, evaluated
+// in FP64 from the stored rows and deltas: acc[0] += sum r^2, acc[1] += sum
+// lambda^2, acc[2] += rows.  Runs right after the PGS launch (reads b.dv).
+template <class T>
 __global__ void k_pgs_residual(Bodies b, Contacts C, int nc, double* acc) {
 	int k = blockIdx.x * blockDim.x + threadIdx.x;
 	double r2 = 0, l2 = 0, rows = 0;
 	if (k < nc && C.live[k]) {
 		const int N = b.n;
-		const double* R = C.row + (size_t)k * CP_RS;
+		const T* R = rec<T>(C, k);
+		const T* dv = reinterpret_cast<const T*>(b.dv);
 		int i1 = (int)(C.key[k] >> 32), i2 = (int)(C.key[k] & 0xFFFFFFFFu);
 		double x = 0, y = 0;
 		for (int q = 0; q < 6; q++) {
-			x += b.dv[q * N + i1] * R[CP_R_JA + q];
-			y += b.dv[q * N + i2] * R[CP_R_JB + q];
+			x += (double)dv[q * N + i1] * (double)R[CP_R_JA + q];
+			y += (double)dv[q * N + i2] * (double)R[CP_R_JB + q];
 		}
 		double ln = R[CP_R_IMP];
-		double z = ln + R[CP_R_INV] * (R[CP_R_TGT] - (x + y));
+		double z = ln + (double)R[CP_R_INV] * ((double)R[CP_R_TGT] - (x + y));
 		double rn = ln - (z < 0 ? 0 : z);
 		double cur[2];
 		for (int r = 0; r < 2; r++) {
 			double a = 0, c = 0;
 			for (int q = 0; q < 6; q++) {
-				a += b.dv[q * N + i1] * R[CP_R_FJA + r * 6 + q];
-				c += b.dv[q * N + i2] * R[CP_R_FJB + r * 6 + q];
+				a += (double)dv[q * N + i1] * (double)R[CP_R_FJA + r * 6 + q];
+				c += (double)dv[q * N + i2] * (double)R[CP_R_FJB + r * 6 + q];
 			}
 			cur[r] = a + c;
 		}
-		double d0 = R[CP_R_FTGT] - cur[0], d1 = R[CP_R_FTGT + 1] - cur[1];
+		double d0 = (double)R[CP_R_FTGT] - cur[0], d1 = (double)R[CP_R_FTGT + 1] - cur[1];
 		double t0 = R[CP_R_FIMP], t1 = R[CP_R_FIMP + 1];
-		double z0 = t0 + R[CP_R_FINV] * d0 + R[CP_R_FINV + 1] * d1;
-		double z1 = t1 + R[CP_R_FINV + 2] * d0 + R[CP_R_FINV + 3] * d1;
-		double lim = R[CP_R_MU] * (ln > 0 ? ln : 0);
+		double z0 = t0 + (double)R[CP_R_FINV] * d0 + (double)R[CP_R_FINV + 1] * d1;
+		double z1 = t1 + (double)R[CP_R_FINV + 2] * d0 + (double)R[CP_R_FINV + 3] * d1;
+		double lim = (double)R[CP_R_MU] * (ln > 0 ? ln : 0);
 		double m = sqrt(z0 * z0 + z1 * z1);
 		if (m > lim && m > 0) { z0 *= lim / m; z1 *= lim / m; }
 		double q0 = t0 - z0, q1 = t1 - z1;
@@ -520,7 +555,9 @@ __global__ void k_pgs_residual(Bodies b, Contacts C, int nc, double* acc) {
 static inline int nblk(int n, int t) { return (n + t - 1) / t; }
 
 void launch_build_rows(const Bodies& b, const Contacts& C, int nc, const Params& P, hipStream_t s) {
-	if (nc) hipLaunchKernelGGL(k_build_rows, dim3(nblk(nc, 64)), dim3(64), 0, s, b, C, nc, P);
+	if (!nc) return;
+	if (P.precision == CP_PREC_SPLIT) hipLaunchKernelGGL(k_build_rows<float>, dim3(nblk(nc, 64)), dim3(64), 0, s, b, C, nc, P);
+	else hipLaunchKernelGGL(k_build_rows<double>, dim3(nblk(nc, 64)), dim3(64), 0, s, b, C, nc, P);
 }
 void launch_pgs_ordered(const Bodies& b, const Contacts& C, const int* rows, const int* isl_off, int nisl, const Params& P, int* applied,
                         hipStream_t s) {
@@ -530,28 +567,40 @@ void launch_pgs_colored(const Bodies& b, const Contacts& C, const int* scene_bod
                         const int* color_off, const int* ncolors, const int* efirst, const int* ecnt, const Params& P, int threads,
                         hipStream_t s, const int* scene_list, int nlist) {
 	int nb = scene_list ? nlist : nscenes;
-	if (nb)
-		hipLaunchKernelGGL(k_pgs_colored, dim3(nb), dim3(threads), 0, s, b, C, scene_body_off, edge_range, order, color_off, ncolors,
+	if (!nb) return;
+	if (P.precision == CP_PREC_SPLIT)
+		hipLaunchKernelGGL(k_pgs_colored<float>, dim3(nb), dim3(threads), 0, s, b, C, scene_body_off, edge_range, order, color_off, ncolors,
 		                   efirst, ecnt, P, scene_list);
+	else
+		hipLaunchKernelGGL(k_pgs_colored<double>, dim3(nb), dim3(threads), 0, s, b, C, scene_body_off, edge_range, order, color_off,
+		                   ncolors, efirst, ecnt, P, scene_list);
 }
 void launch_pgs_colored_lds(const Bodies& b, const Contacts& C, const int* scene_list, int nlist, int max_bodies, const int* scene_body_off,
                             const int* edge_range, const int* order, const int* color_off, const int* ncolors, const int* efirst,
                             const int* ecnt, const Params& P, int threads, hipStream_t s) {
 	if (!nlist) return;
-	size_t lds = (size_t)6 * max_bodies * sizeof(double);
 	static bool attr = false;
 	if (!attr) {
-		(void)hipFuncSetAttribute((const void*)k_pgs_colored_lds, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+		(void)hipFuncSetAttribute((const void*)k_pgs_colored_lds<double>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+		(void)hipFuncSetAttribute((const void*)k_pgs_colored_lds<float>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
 		attr = true;
 	}
-	hipLaunchKernelGGL(k_pgs_colored_lds, dim3(nlist), dim3(threads), lds, s, b, C, scene_list, scene_body_off, edge_range, order, color_off,
-	                   ncolors, efirst, ecnt, P);
+	if (P.precision == CP_PREC_SPLIT)
+		hipLaunchKernelGGL(k_pgs_colored_lds<float>, dim3(nlist), dim3(threads), (size_t)6 * max_bodies * sizeof(float), s, b, C, scene_list,
+		                   scene_body_off, edge_range, order, color_off, ncolors, efirst, ecnt, P);
+	else
+		hipLaunchKernelGGL(k_pgs_colored_lds<double>, dim3(nlist), dim3(threads), (size_t)6 * max_bodies * sizeof(double), s, b, C,
+		                   scene_list, scene_body_off, edge_range, order, color_off, ncolors, efirst, ecnt, P);
 }
-void launch_rows_writeback(const Contacts& C, int nc, hipStream_t s) {
-	if (nc) hipLaunchKernelGGL(k_rows_writeback, dim3(nblk(nc, 256)), dim3(256), 0, s, C, nc);
+void launch_rows_writeback(const Contacts& C, int nc, const Params& P, hipStream_t s) {
+	if (!nc) return;
+	if (P.precision == CP_PREC_SPLIT) hipLaunchKernelGGL(k_rows_writeback<float>, dim3(nblk(nc, 256)), dim3(256), 0, s, C, nc);
+	else hipLaunchKernelGGL(k_rows_writeback<double>, dim3(nblk(nc, 256)), dim3(256), 0, s, C, nc);
 }
-void launch_pgs_residual(const Bodies& b, const Contacts& C, int nc, double* acc, hipStream_t s) {
-	if (nc) hipLaunchKernelGGL(k_pgs_residual, dim3(nblk(nc, 256)), dim3(256), 0, s, b, C, nc, acc);
+void launch_pgs_residual(const Bodies& b, const Contacts& C, int nc, const Params& P, double* acc, hipStream_t s) {
+	if (!nc) return;
+	if (P.precision == CP_PREC_SPLIT) hipLaunchKernelGGL(k_pgs_residual<float>, dim3(nblk(nc, 256)), dim3(256), 0, s, b, C, nc, acc);
+	else hipLaunchKernelGGL(k_pgs_residual<double>, dim3(nblk(nc, 256)), dim3(256), 0, s, b, C, nc, acc);
 }
 
 }  // namespace cpx

@@ -219,6 +219,7 @@ struct cp_world {
 		p.contact_life = cfg.contact_life;
 		p.gyro_ticks = cfg.gyro_ticks;
 		p.defect_fixes = cfg.defect_fixes;
+		p.precision = cfg.pgs_mode == CP_PGS_REFERENCE_ORDER ? CP_PREC_FP64 : cfg.precision;
 		return p;
 	}
 
@@ -611,11 +612,11 @@ void cp_world::step() {
 			HIPCHK(hipEventRecord(pgs_ev[2 * npgs + 1], stream));
 			npgs++;
 		}
-		launch_rows_writeback(cb[cur].c, ncontacts, stream);
+		launch_rows_writeback(cb[cur].c, ncontacts, P, stream);
 		if (s == cfg.substeps - 1 && (cfg.diagnostics & 1)) {
 			resid_acc.reserve(4);
 			HIPCHK(hipMemsetAsync(resid_acc.p, 0, 4 * sizeof(double), stream));
-			launch_pgs_residual(b, cb[cur].c, ncontacts, resid_acc.p, stream);
+			launch_pgs_residual(b, cb[cur].c, ncontacts, P, resid_acc.p, stream);
 		}
 		if (s == cfg.substeps - 1) launch_age(cb[cur].c, ncontacts, stream);
 		launch_integrate(b, P, stream);
@@ -681,6 +682,7 @@ void cp_config_default(cp_config* c) {
 	c->solver_threads = 256;
 	c->diagnostics = 0;
 	c->defect_fixes = 1;
+	c->precision = 0;
 }
 
 const char* cp_status_string(int s) {
@@ -697,7 +699,8 @@ const char* cp_status_string(int s) {
 
 static bool valid_cfg(const cp_config* c) {
 	return c && c->dt > 0 && c->substeps >= 1 && c->vel_iters >= 0 && c->pos_iters >= 0 && c->gyro_ticks >= 1 && c->solver_threads >= 64 &&
-	       c->solver_threads <= 1024 && (c->solver_threads % 64) == 0 && (c->pgs_mode == 0 || c->pgs_mode == 1);
+	       c->solver_threads <= 1024 && (c->solver_threads % 64) == 0 && (c->pgs_mode == 0 || c->pgs_mode == 1) &&
+	       (c->precision == CP_PREC_FP64 || c->precision == CP_PREC_SPLIT);
 }
 
 int cp_world_create(const cp_config* cfg, cp_world** out) {

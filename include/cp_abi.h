@@ -39,6 +39,12 @@ typedef enum {
 
 typedef enum { CP_PGS_COLORED = 0, CP_PGS_REFERENCE_ORDER = 1 } cp_pgs_mode;
 
+/* CP_PREC_FP64: rows, impulses and velocity deltas in FP64 (the reference's
+ * arithmetic).  CP_PREC_SPLIT: rows derived in FP64 and stored in FP32, PGS
+ * sweeps, impulses within the solve and velocity deltas in FP32; body state,
+ * integration, contact cache and narrowphase stay FP64. */
+typedef enum { CP_PREC_FP64 = 0, CP_PREC_SPLIT = 1 } cp_precision;
+
 /* Engine parameters; defaults of PhysicsEngine.h:174-216 via cp_config_default. */
 typedef struct {
 	double dt;               /* setStep_time (PhysicsEngine.h:98) */
@@ -58,6 +64,8 @@ typedef struct {
 	int defect_fixes;        /* bit 0: drop non-finite manifold points (the reference's clip
 	                            intersection divides by n.d, which rounds to 0 for an edge parallel to
 	                            the clipping edge, CollisionDetect.cpp:444-467); 0 = bug-compatible */
+	int precision;           /* cp_precision: solve precision of CP_PGS_COLORED (CP_PGS_REFERENCE_ORDER is
+	                            always FP64) */
 } cp_config;
 
 typedef struct cp_world cp_world;

@@ -105,3 +105,24 @@ def test_batched_scenes_match_individual_scenes():
         for _ in range(60):
             ws.step(1)
         assert np.array_equal(ws.state(), sb[off[s]:off[s + 1]])
+
+
+def test_state_and_contact_snapshot_restores_the_step():
+    """cp_world_get_state/get_contacts then set_state/set_contacts reproduce
+    the next step: a snapshot is a complete resume point (world inertia is
+    recomputed from the orientation, so agreement is to rounding)."""
+    sizes = scenes.c3_sizes(8, 21, lo=8, hi=120)
+    w = cpa.World(dt=1.0 / 120.0, vel_iters=10, pos_iters=1, substeps=8, pgs_mode=0, diagnostics=1)
+    scenes.build_batched(w, sizes, 21, hull_lib_size=16)
+    for _ in range(120):
+        w.step(1)
+    st, cache = w.state(), w.contacts()
+    w.step(1)
+    a, ra = w.state(), w.last_residual()
+    w.set_state(st)
+    w.set_contacts(*cache)
+    w.step(1)
+    b, rb = w.state(), w.last_residual()
+    print(f"residual continuous {ra[2]:.3e} restored {rb[2]:.3e}; max state diff {np.max(np.abs(a - b)):.3e}", flush=True)
+    assert np.max(np.abs(a - b)) <= 1e-9
+    assert abs(ra[2] - rb[2]) <= 1e-6 + 1e-3 * ra[2]
