@@ -221,10 +221,15 @@ struct Body {
 	V3 com, vel, ang_vel, pvel, pang;
 	Q q;
 	Hull hull;
-	int type = 0;        // 0 polyhedron, 1 sphere
+	int type = 0;        // 0 polyhedron, 1 sphere, 3 cylinder (ConvexGeometryType, ConvexPrimitive.h:36)
 	V3 sph_ref;          // sphere centre in the body frame (Sphere::recomputeFromReference)
 	double radius = 0;
 	V3 sph_c;            // world centre
+	// cylinder (ConvexPrimitive.h:159-210): body-frame reference and world copy;
+	// 14 Gauss vertices (caps, then the 12-gon ring), 8-gon cap polygons
+	V3 cyl_c_ref, cyl_ax_ref, cyl_gv_ref[14], cyl_top_ref[8], cyl_bot_ref[8];
+	double cyl_r = 0, cyl_h = 0;
+	V3 cyl_c, cyl_ax, cyl_gv[14], cyl_top[8], cyl_bot[8];
 	double aabb[6];
 	V3 prev_com;
 	Q prev_q;
@@ -252,6 +257,24 @@ static void worldGeometry(Body& b) {
 		V3 mn = b.sph_c - r, mx = b.sph_c + r;
 		b.aabb[0] = mn.x; b.aabb[1] = mn.y; b.aabb[2] = mn.z;
 		b.aabb[3] = mx.x; b.aabb[4] = mx.y; b.aabb[5] = mx.z;
+		return;
+	}
+	if (b.type == 3) {  // Cylinder::recomputeFromReference / gen_AABB (ConvexPrimitive.cpp:272-298)
+		b.cyl_c = R * b.cyl_c_ref + b.com;
+		b.cyl_ax = R * b.cyl_ax_ref;
+		for (int i = 0; i < 14; i++) b.cyl_gv[i] = R * b.cyl_gv_ref[i];
+		for (int i = 0; i < 8; i++) {
+			b.cyl_top[i] = R * b.cyl_top_ref[i] + b.com;
+			b.cyl_bot[i] = b.cyl_top[i] - b.cyl_ax * b.cyl_h;
+		}
+		const double inf = std::numeric_limits<double>::infinity();
+		double mn[3] = {inf, inf, inf}, mx[3] = {-inf, -inf, -inf};
+		for (int i = 0; i < 16; i++) {  // point cloud order: top[0], bot[0], top[1], ...
+			const V3& p = (i & 1) ? b.cyl_bot[i >> 1] : b.cyl_top[i >> 1];
+			mn[0] = std::min<double>(mn[0], p.x); mn[1] = std::min<double>(mn[1], p.y); mn[2] = std::min<double>(mn[2], p.z);
+			mx[0] = std::max<double>(mx[0], p.x); mx[1] = std::max<double>(mx[1], p.y); mx[2] = std::max<double>(mx[2], p.z);
+		}
+		for (int i = 0; i < 3; i++) { b.aabb[i] = mn[i]; b.aabb[3 + i] = mx[i]; }
 		return;
 	}
 	Hull& h = b.hull;
@@ -309,7 +332,7 @@ static Extrema recentered(const Hull& h, int g) {  // CollisionDetect.cpp:622-62
 	return e;
 }
 
-enum Origin { FACE, EDGE, VERTEX };
+enum Origin { FACE, EDGE, VERTEX, CYLINDER_BARREL };
 struct P2 { double x, y; };
 static P2 sub2(P2 a, P2 b) { return {a.x - b.x, a.y - b.y}; }
 static double dot2(P2 a, P2 b) { double s = 0; s += a.x * b.x; s += a.y * b.y; return s; }
@@ -633,6 +656,305 @@ static Manifold satPolySphere(const Body& A, uint64_t a_key, const Body& B, uint
 	c.bid = feat;
 	out.pts.push_back(c);
 	out.max_pen = mp.pen;
+	return out;
+}
+
+
+// ---------------------------------------------------------------- cylinders
+// Cylinder ids used in MagicIDs (ConvexPrimitive.h:174-179, 8-gon caps).
+static const int CYL_TOP_PT = 0, CYL_BOT_PT = 8, CYL_TOP_SURF = 16, CYL_BOT_SURF = 17, CYL_TOP_EDGE = 18, CYL_BOT_EDGE = 19;
+
+static V3 cylTopCenter(const Body& c) { return c.cyl_c + (0.5 * c.cyl_h) * c.cyl_ax; }  // ConvexPrimitive.h:182
+static V3 cylBotCenter(const Body& c) { return c.cyl_c - (0.5 * c.cyl_h) * c.cyl_ax; }  // ConvexPrimitive.h:183
+
+// Cylinder::getExtremaOfDisk (ConvexPrimitive.cpp:319-355).
+static V3 diskExtrema(V3 dc, V3 dn, double r, V3 dir) {
+	if (std::abs(dn.dot(dir)) > 0.99999999) return dc;
+	if (std::abs(dn.dot(dir)) < 0.00000001) return dc + dir * r;
+	V3 u, v, w;
+	w = dir;
+	dir.perpBasis(&u, &v);
+	V3 un(dn.dot(u), dn.dot(v), dn.dot(w));
+	double dwdu = -un.x / un.z;
+	double dwdv = -un.y / un.z;
+	double gm = std::sqrt(dwdu * dwdu + dwdv * dwdv);
+	double gu = dwdu / gm;
+	double gv = dwdv / gm;
+	double uv2 = r * r / (1 + gm * gm);
+	double dw = std::sqrt(r * r - uv2);
+	double du = gu * std::sqrt(uv2);
+	double dv = gv * std::sqrt(uv2);
+	return dc + u * du + v * dv + w * dw;
+}
+
+// getCylinderExtrema (CollisionDetect.cpp:658-684).
+static Extrema cylExtrema(const Body& c, V3 dir) {
+	V3 tc = cylTopCenter(c);
+	V3 tmax = diskExtrema(tc, c.cyl_ax, c.cyl_r, dir);
+	V3 tmin = 2.0 * tc - tmax;
+	V3 bmax = 2.0 * c.cyl_c - tmin;
+	V3 bmin = 2.0 * c.cyl_c - tmax;
+	double tmaxv = dir.dot(tmax), tminv = dir.dot(tmin), bmaxv = dir.dot(bmax), bminv = dir.dot(bmin);
+	Extrema e;
+	e.max_val = std::max<double>(tmaxv, bmaxv);
+	e.min_val = std::min<double>(tminv, bminv);
+	e.min_pid = -1;
+	e.max_pid = -1;
+	return e;
+}
+
+struct Seg { V3 p1, p2; };
+// cylinderLengthwiseLineInDirection (CollisionDetect.cpp:347-352).
+static Seg cylLine(V3 center, V3 ax, double r, double h, V3 n) {
+	V3 barrel = (n - ax * ax.dot(n)).normalize();
+	V3 p1 = center + barrel * r + (0.5 * h) * ax;
+	V3 p2 = p1 - h * ax;
+	return {p1, p2};
+}
+
+// findCylinderContactArea (CollisionDetect.cpp:354-374) with projectCylinderFace (:331-340).
+static Area cylContactArea(const Body& c, V3 n, V3 u, V3 w) {
+	V3 ax = c.cyl_ax;
+	Area a;
+	auto face = [&](const V3* fv, int surf, int off) {
+		for (int i = 0; i < 8; i++) {
+			a.ps.push_back({fv[i].dot(u), fv[i].dot(w)});
+			a.ids.push_back(i + off);
+		}
+		a.surface = surf;
+		a.origin = FACE;
+	};
+	if (1 - std::abs(ax.dot(n)) <= COS_TOL && n.dot(ax) > 0) face(c.cyl_top, CYL_TOP_SURF, CYL_TOP_PT);
+	else if (1 - std::abs(ax.dot(n)) <= COS_TOL) face(c.cyl_bot, CYL_BOT_SURF, CYL_BOT_PT);
+	else if (std::abs(ax.dot(n)) <= SIN_TOL) {
+		Seg l = cylLine(c.cyl_c, ax, c.cyl_r, c.cyl_h, n);
+		a.ps = {{l.p1.dot(u), l.p1.dot(w)}, {l.p2.dot(u), l.p2.dot(w)}};
+		a.ids = {CYL_TOP_EDGE, CYL_BOT_EDGE};
+		a.surface = -1;
+		a.origin = CYLINDER_BARREL;
+	} else if (n.dot(ax) > 0) {
+		V3 p = diskExtrema(cylTopCenter(c), ax, c.cyl_r, n);
+		a.ps = {{p.dot(u), p.dot(w)}};
+		a.ids = {CYL_TOP_EDGE};
+		a.surface = -1;
+		a.origin = EDGE;
+	} else {
+		V3 p = diskExtrema(cylBotCenter(c), ax, c.cyl_r, n);
+		a.ps = {{p.dot(u), p.dot(w)}};
+		a.ids = {CYL_BOT_EDGE};
+		a.surface = -1;
+		a.origin = EDGE;
+	}
+	return a;
+}
+
+// Contact points from a projected pool (the lift shared by the SAT routines,
+// e.g. CollisionDetect.cpp:1685-1708).
+static void liftPool(Manifold& out, const std::vector<CEP>& pool, V3 n, V3 u, V3 w, V3 amax, double pen, const Body& A, uint64_t a_key,
+                     const Body& B, uint64_t b_key) {
+	double a_dot = amax.dot(n);
+	V3 noff = n * a_dot;
+	uint64_t cid = 0;
+	cid |= 0x00000000FFFFFFFFull & a_key;
+	cid |= 0xFFFFFFFF00000000ull & (b_key << 32);
+	for (const CEP& p : pool) {
+		CP c;
+		c.pos = u * p.pos.x + w * p.pos.y + noff;
+		c.pen = c.pos.dot(n) - a_dot + pen;
+		c.restitution = std::max<double>(A.restitution, B.restitution);
+		c.mu_k = (A.mu_k + B.mu_k) / 2.0;
+		c.mu_s = (A.mu_s + B.mu_s) / 2.0;
+		c.cid = cid;
+		c.bid = p.magic;
+		out.pts.push_back(c);
+	}
+	out.max_pen = pen;
+}
+
+static CEP poolPoint(P2 p, uint64_t magic) {
+	CEP c{};
+	c.pos = p;
+	c.magic = magic;
+	return c;
+}
+
+// Gauss-arc pair direction (the arc loop body of CollisionDetect.cpp:1617-1650 and
+// :1021-1054); false when the pair is pruned.
+static bool arcAxis(V3 a1, V3 a2, V3 b1, V3 b2, V3* out) {
+	V3 aa = a1 + a2;
+	if (aa.dot(b1) + aa.dot(b2) <= 0) return false;
+	V3 ap = a1.cross(a2), bp = b1.cross(b2);
+	if (ap.dot(b1) * ap.dot(b2) > 0 || bp.dot(a1) * bp.dot(a2) > 0) return false;
+	V3 n = ap.cross(bp);
+	if (n.magSqrd() == 0) return false;
+	n = n.normalize();
+	if (aa.dot(n) < 0) n = n * -1.0;
+	*out = n;
+	return true;
+}
+
+// Cylinder Gauss arcs as the reference builds them: {0, i} and {1, i} for
+// i < 12 (ConvexPrimitive.cpp:239-245; the second index is the loop counter).
+static void cylArc(int k, int* v1, int* v2) {
+	*v1 = k & 1;
+	*v2 = k >> 1;
+}
+
+// SAT_PolyCylinder (CollisionDetect.cpp:1563-1709).
+static Manifold satPolyCylinder(const Body& A, uint64_t a_key, const Body& B, uint64_t b_key) {
+	Manifold out;
+	out.max_pen = -1;
+	const Hull& a = A.hull;
+	CheckNorm mp{-1, -1, V3(), std::numeric_limits<double>::infinity()};
+	auto test = [&](const CheckNorm& x) {
+		if (x.pen < 0) return true;
+		if (x.pen < mp.pen) mp = x;
+		return false;
+	};
+	for (size_t g = 0; g < a.wnorm.size(); g++) {
+		if (a.gred[g]) continue;
+		if (test(satCheck(recentered(a, (int)g), cylExtrema(B, a.wnorm[g]), a.wnorm[g]))) return Manifold();
+	}
+	V3 bax = B.cyl_ax;
+	if (test(satCheck(findExtrema(a, bax), cylExtrema(B, bax), bax))) return Manifold();
+	for (auto& e : a.edges) {
+		V3 ed = a.pts[e.second] - a.pts[e.first];
+		V3 dir = ed.cross(bax);
+		if (dir.mag() < 0.00000000001) continue;
+		V3 dn = dir.normalize();
+		if (test(satCheck(findExtrema(a, dn), cylExtrema(B, dn), dn))) return Manifold();
+	}
+	for (const V3& p : a.pts) {
+		V3 diff = p - B.cyl_c;
+		V3 n = (diff - bax * bax.dot(diff)).normalize();
+		if (test(satCheck(findExtrema(a, n), cylExtrema(B, n), n))) return Manifold();
+	}
+	for (auto arc1 : a.arcs)
+		for (int k = 0; k < 24; k++) {
+			int v1, v2;
+			cylArc(k, &v1, &v2);
+			V3 n;
+			if (!arcAxis(a.wnorm[arc1.first], a.wnorm[arc1.second], -B.cyl_gv[v1], -B.cyl_gv[v2], &n)) continue;
+			if (test(satCheck(findExtrema(a, n), cylExtrema(B, n), n))) return Manifold();
+		}
+	out.n = mp.n;
+	V3 n = mp.n;
+	V3 amax = a.pts[mp.a_pid];
+	V3 u, w;
+	n.perpBasis(&u, &w);
+	Area ac = contactArea(a, n, amax, mp.a_pid, u, w);
+	Area bc = cylContactArea(B, -n, u, w);
+	std::vector<CEP> pool;
+	if (bc.origin == EDGE) pool = {poolPoint(bc.ps[0], 0)};
+	else pool = clipContacts(ac, bc);
+	liftPool(out, pool, n, u, w, amax, mp.pen, A, a_key, B, b_key);
+	return out;
+}
+
+// detectSphereCylinder (CollisionDetect.cpp:1712-1792).  The second disk
+// centre repeats the top one (+h/2), as in the reference.
+static Manifold sphereCylinder(const Body& A, uint64_t a_key, const Body& B, uint64_t b_key) {
+	Manifold out;
+	out.max_pen = -1;
+	CheckNorm mp{-1, -1, V3(), std::numeric_limits<double>::infinity()};
+	auto test = [&](V3 n) {
+		CheckNorm x = satCheck(sphereExtrema(A, n), cylExtrema(B, n), n);
+		if (x.pen < 0) return true;
+		if (x.pen < mp.pen) mp = x;
+		return false;
+	};
+	V3 ax = B.cyl_ax;
+	if (test(ax)) return Manifold();
+	V3 diff = B.cyl_c - A.sph_c;
+	V3 barrel = (diff - ax * ax.dot(diff)).normalize();
+	if (test(barrel)) return Manifold();
+	V3 tdc = B.cyl_c + ax * 0.5 * B.cyl_h;
+	V3 tcp = diskExtrema(tdc, ax, B.cyl_r, -diff.normalize());
+	if (test((tcp - A.sph_c).normalize())) return Manifold();
+	V3 bdc = B.cyl_c + ax * 0.5 * B.cyl_h;
+	V3 bcp = diskExtrema(bdc, ax, B.cyl_r, -diff.normalize());
+	if (test((bcp - A.sph_c).normalize())) return Manifold();
+	out.n = mp.n;
+	CP c;
+	c.pos = A.sph_c + out.n * A.radius;
+	c.pen = mp.pen;
+	c.restitution = std::max<double>(A.restitution, B.restitution);
+	c.mu_k = (A.mu_k + B.mu_k) / 2.0;
+	c.mu_s = (A.mu_s + B.mu_s) / 2.0;
+	c.cid = (0x00000000FFFFFFFFull & a_key) | (0xFFFFFFFF00000000ull & (b_key << 32));
+	c.bid = 0;
+	out.pts.push_back(c);
+	out.max_pen = mp.pen;
+	return out;
+}
+
+// checkCylinderEdgeApproxVsBarrel (CollisionDetect.cpp:938-956); true = separated.
+static bool cylEdgeBarrel(const Body& A, const Body& B, V3 bax, const V3* edge, CheckNorm* mp) {
+	for (int i = 0; i < 8; i++) {
+		V3 ed = edge[(i + 1) % 8] - edge[i];
+		V3 dir = ed.cross(bax);
+		if (dir.mag() < 0.00000000001) continue;
+		V3 dn = dir.normalize();
+		CheckNorm x = satCheck(cylExtrema(A, dn), cylExtrema(B, dn), dn);
+		if (x.pen < 0) return true;
+		if (x.pen < mp->pen) *mp = x;
+	}
+	return false;
+}
+
+// detectCylinderCylinder (CollisionDetect.cpp:958-1137).
+static Manifold cylCyl(const Body& A, uint64_t a_key, const Body& B, uint64_t b_key) {
+	Manifold out;
+	out.max_pen = -1;
+	CheckNorm mp{-1, -1, V3(), std::numeric_limits<double>::infinity()};
+	auto test = [&](V3 n) {
+		CheckNorm x = satCheck(cylExtrema(A, n), cylExtrema(B, n), n);
+		if (x.pen < 0) return true;
+		if (x.pen < mp.pen) mp = x;
+		return false;
+	};
+	V3 aax = A.cyl_ax, bax = B.cyl_ax;
+	if (test(aax)) return Manifold();
+	if (test(bax)) return Manifold();
+	if (cylEdgeBarrel(A, B, bax, A.cyl_top, &mp)) return Manifold();
+	if (cylEdgeBarrel(A, B, bax, A.cyl_bot, &mp)) return Manifold();
+	if (cylEdgeBarrel(A, B, aax, B.cyl_top, &mp)) return Manifold();
+	if (cylEdgeBarrel(A, B, aax, B.cyl_bot, &mp)) return Manifold();
+	V3 bb = aax.cross(bax);
+	if (bb.magSqrd() > 0.00000001) {
+		bb = bb.normalize();
+		if (test(bb)) return Manifold();
+	}
+	for (int k1 = 0; k1 < 24; k1++)
+		for (int k2 = 0; k2 < 24; k2++) {
+			int a1, a2, b1, b2;
+			cylArc(k1, &a1, &a2);
+			cylArc(k2, &b1, &b2);
+			V3 n;
+			if (!arcAxis(A.cyl_gv[a1], A.cyl_gv[a2], -B.cyl_gv[b1], -B.cyl_gv[b2], &n)) continue;
+			if (test(n)) return Manifold();
+		}
+	out.n = mp.n;
+	V3 n = mp.n;
+	V3 u, w;
+	n.perpBasis(&u, &w);
+	Area ac = cylContactArea(A, n, u, w);
+	Area bc = cylContactArea(B, -n, u, w);
+	std::vector<CEP> pool;
+	if (ac.origin == EDGE) pool = {poolPoint(ac.ps[0], 0)};
+	else if (bc.origin == EDGE) pool = {poolPoint(bc.ps[0], 0)};
+	else if (ac.origin == CYLINDER_BARREL && bc.origin == CYLINDER_BARREL && std::abs(aax.dot(bax)) > 0.995) {
+		// parallel barrels (:1082-1104): overlap of the two lines along a's axis
+		Seg la = cylLine(A.cyl_c, aax, A.cyl_r, A.cyl_h, n);
+		Seg lb = cylLine(B.cyl_c, bax, B.cyl_r, B.cyl_h, -n);
+		if (bax.dot(aax) < 0) std::swap(lb.p1, lb.p2);
+		double amaxv = la.p1.dot(aax), aminv = la.p2.dot(aax), bmaxv = lb.p1.dot(aax), bminv = lb.p2.dot(aax);
+		V3 imax = (amaxv > bmaxv) ? lb.p1 : la.p1;
+		V3 imin = (aminv > bminv) ? la.p2 : lb.p2;
+		pool = {poolPoint({u.dot(imax), w.dot(imax)}, 0x0), poolPoint({u.dot(imin), w.dot(imin)}, 0x1)};
+	} else pool = clipContacts(ac, bc);
+	V3 amax = aax.dot(n) > 0 ? diskExtrema(cylTopCenter(A), aax, A.cyl_r, n) : diskExtrema(cylBotCenter(A), aax, A.cyl_r, n);
+	liftPool(out, pool, n, u, w, amax, mp.pen, A, a_key, B, b_key);
 	return out;
 }
 
@@ -973,14 +1295,23 @@ static Manifold narrow(const World& W, int i, int j) {
 static Manifold narrowRaw(const World& W, int i, int j) {
 	const Body& A = W.bodies[i];
 	const Body& B = W.bodies[j];
+	// detectCollision dispatch (CollisionDetect.cpp:57-152): the swapped forms
+	// negate the normal and swap both MagicID halves.
 	Manifold m;
+	auto swapped = [&](Manifold r) {
+		r.n = -r.n;
+		for (CP& c : r.pts) { c.cid = swapHalves(c.cid); c.bid = swapHalves(c.bid); }
+		return r;
+	};
 	if (A.type == 0 && B.type == 0) m = satPolyPoly(A, primKey(A), B, primKey(B));
 	else if (A.type == 0 && B.type == 1) m = satPolySphere(A, primKey(A), B, primKey(B));
-	else if (A.type == 1 && B.type == 0) {
-		m = satPolySphere(B, primKey(B), A, primKey(A));
-		m.n = -m.n;
-		for (CP& c : m.pts) { c.cid = swapHalves(c.cid); c.bid = swapHalves(c.bid); }
-	} else m = sphereSphere(A, primKey(A), B, primKey(B));
+	else if (A.type == 1 && B.type == 0) m = swapped(satPolySphere(B, primKey(B), A, primKey(A)));
+	else if (A.type == 1 && B.type == 1) m = sphereSphere(A, primKey(A), B, primKey(B));
+	else if (A.type == 0 && B.type == 3) m = satPolyCylinder(A, primKey(A), B, primKey(B));
+	else if (A.type == 3 && B.type == 0) m = swapped(satPolyCylinder(B, primKey(B), A, primKey(A)));
+	else if (A.type == 1 && B.type == 3) m = sphereCylinder(A, primKey(A), B, primKey(B));
+	else if (A.type == 3 && B.type == 1) m = swapped(sphereCylinder(B, primKey(B), A, primKey(A)));
+	else if (A.type == 3 && B.type == 3) m = cylCyl(A, primKey(A), B, primKey(B));
 	if (!(m.max_pen > 0)) return Manifold();
 	return cullManifold(m, 4);
 }
@@ -1252,6 +1583,34 @@ int orc_add_sphere_body(orc_world* w, int movement, const double* mp, const doub
 	b.mu_s = mat[3];
 	b.sph_ref = V3(center[0], center[1], center[2]);
 	b.radius = radius;
+	w->bodies.push_back(b);
+	return (int)w->bodies.size() - 1;
+}
+
+int orc_add_cylinder_body(orc_world* w, int movement, const double* mp, const double* mat, const double* params8, const double* gauss42,
+                          const double* top24, const double* bot24) {
+	Body b{};
+	b.id = (int)w->bodies.size() + 1;
+	b.movement = movement;
+	b.type = 3;
+	b.mass = mp[0];
+	for (int r = 0; r < 3; r++)
+		for (int c = 0; c < 3; c++) {
+			b.I_ref.v[r][c] = mp[1 + 3 * r + c];
+			b.I_ref_inv.v[r][c] = mp[10 + 3 * r + c];
+		}
+	b.restitution = mat[1];
+	b.mu_k = mat[2];
+	b.mu_s = mat[3];
+	b.cyl_c_ref = V3(params8[0], params8[1], params8[2]);
+	b.cyl_ax_ref = V3(params8[3], params8[4], params8[5]);
+	b.cyl_r = params8[6];
+	b.cyl_h = params8[7];
+	for (int i = 0; i < 14; i++) b.cyl_gv_ref[i] = V3(gauss42[3 * i], gauss42[3 * i + 1], gauss42[3 * i + 2]);
+	for (int i = 0; i < 8; i++) {
+		b.cyl_top_ref[i] = V3(top24[3 * i], top24[3 * i + 1], top24[3 * i + 2]);
+		b.cyl_bot_ref[i] = V3(bot24[3 * i], bot24[3 * i + 1], bot24[3 * i + 2]);
+	}
 	w->bodies.push_back(b);
 	return (int)w->bodies.size() - 1;
 }

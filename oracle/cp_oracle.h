@@ -49,6 +49,12 @@ int orc_add_hull_body(orc_world* w, int movement, const double* mass_props, cons
 int orc_add_sphere_body(orc_world* w, int movement, const double* mass_props, const double* material, const double* center,
                         double radius);
 
+// Cylinder body (Cylinder, ConvexPrimitive.h:159-210) as the reference stores its
+// body-frame reference: params8 = centre xyz, height axis xyz, radius, height;
+// gauss42 = 14 Gauss vertices; top24/bot24 = the 8-gon cap polygons.
+int orc_add_cylinder_body(orc_world* w, int movement, const double* mass_props, const double* material, const double* params8,
+                          const double* gauss42, const double* top24, const double* bot24);
+
 // Body state rows of 13 doubles: com 3, q (r,i,j,k), vel 3, ang_vel 3.
 void orc_set_state(orc_world* w, const double* state);
 void orc_get_state(const orc_world* w, double* state);

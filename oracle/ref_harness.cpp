@@ -179,7 +179,11 @@ static void build_pile(PhysicsEngine& E, int nx, int ny, int nz, uint64_t seed, 
 // Mixed pile: like build_pile but every other body is a sphere of radius
 // U[0.3, 0.5] (draw after the yaw), so sphere-sphere, sphere-box, box-box and
 // body-ground pairs all occur.
-static void build_mixed(PhysicsEngine& E, int nx, int ny, int nz, uint64_t seed, double spacing, double jitter) {
+// kinds = 3 cycles box, sphere, cylinder: cylinders have r U[0.2, 0.4], h
+// U[0.5, 1.0] (draws after the radius) and a random tilt: rotation by U[0, pi)
+// about the horizontal axis (cos yaw, 0, sin yaw), so caps, rims and barrels
+// all touch.
+static void build_mixed(PhysicsEngine& E, int nx, int ny, int nz, uint64_t seed, double spacing, double jitter, int kinds = 2) {
 	phyz::Material m = c0_material();
 	RigidBody* g = E.createRigidBody(phyz::ConvexUnionGeometry::box(mthz::Vec3(-100, -1, -100), 200, 1, 200, m), RigidBody::FIXED);
 	pin_body(g);
@@ -194,11 +198,18 @@ static void build_mixed(PhysicsEngine& E, int nx, int ny, int nz, uint64_t seed,
 				double yaw = rng.uniform(0.0, 2.0 * 3.14159265358979323846);
 				double r = rng.uniform(0.3, 0.5);
 				RigidBody* b;
-				if ((idx++ & 1) == 0) {
+				int kind = idx++ % kinds;
+				if (kind == 0) {
 					b = E.createRigidBody(phyz::ConvexUnionGeometry::box(mthz::Vec3(cx - 0.5, cy - 0.5, cz - 0.5), 1, 1, 1, m));
 					b->setOrientation(mthz::Quaternion(yaw, mthz::Vec3(0, 1, 0)));
-				} else {
+				} else if (kind == 1) {
 					b = E.createRigidBody(phyz::ConvexUnionGeometry::sphere(mthz::Vec3(cx, cy, cz), r, m));
+				} else {
+					double cr = 0.2 + (r - 0.3);  // U[0.2, 0.4]
+					double h = rng.uniform(0.5, 1.0);
+					double tilt = rng.uniform(0.0, 3.14159265358979323846);
+					b = E.createRigidBody(phyz::ConvexUnionGeometry::cylinder(mthz::Vec3(cx, cy - h / 2.0, cz), cr, h, m));
+					b->setOrientation(mthz::Quaternion(tilt, mthz::Vec3(std::cos(yaw), 0, std::sin(yaw))));
 				}
 				pin_body(b);
 			}
@@ -351,6 +362,9 @@ static void dump_static(PhysicsEngine& E, Out& o) {
 	std::vector<double> pts, gv, gx;
 	std::vector<int64_t> poff{0}, foff{0}, fidx{0}, fverts, gred, gint, arcs, aoff{0}, edges, eoff{0}, surfid, ncalc, ndir;
 	std::vector<double> mat, sph;  // sphere: body-frame centre xyz, radius
+	// cylinder (body frame, as the reference stores it): centre xyz, height axis
+	// xyz, radius, height; Gauss vertices [14][3]; cap polygons [8][3] each
+	std::vector<double> cyl, cyl_g, cyl_t, cyl_b;
 	for (RigidBody* b : E.bodies) {
 		mp.push_back(b->mass);
 		for (int r = 0; r < 3; r++) for (int c = 0; c < 3; c++) mp.push_back(b->reference_tensor.v[r][c]);
@@ -365,6 +379,19 @@ static void dump_static(PhysicsEngine& E, Out& o) {
 			const phyz::Sphere& rs = (const phyz::Sphere&)*b->reference_geometry[0].getGeometry();
 			sph.insert(sph.end(), {rs.center.x, rs.center.y, rs.center.z, rs.radius});
 		} else sph.insert(sph.end(), {0, 0, 0, 0});
+		if (ptype == phyz::CYLINDER && b->geometry.size() == 1) {
+			const phyz::Cylinder& rc = (const phyz::Cylinder&)*b->reference_geometry[0].getGeometry();
+			cyl.insert(cyl.end(), {rc.center.x, rc.center.y, rc.center.z, rc.height_axis.x, rc.height_axis.y, rc.height_axis.z, rc.radius, rc.height});
+			for (auto& v : rc.gauss_verts) cyl_g.insert(cyl_g.end(), {v.x, v.y, v.z});
+			for (auto& v : rc.top_face_approximation) cyl_t.insert(cyl_t.end(), {v.x, v.y, v.z});
+			for (auto& v : rc.bot_face_approximation) cyl_b.insert(cyl_b.end(), {v.x, v.y, v.z});
+			if (rc.gauss_verts.size() != 14 || rc.top_face_approximation.size() != 8) { fprintf(stderr, "unexpected cylinder detail\n"); exit(1); }
+		} else {
+			cyl.insert(cyl.end(), 8, 0.0);
+			cyl_g.insert(cyl_g.end(), 42, 0.0);
+			cyl_t.insert(cyl_t.end(), 24, 0.0);
+			cyl_b.insert(cyl_b.end(), 24, 0.0);
+		}
 		if (ptype == phyz::POLYHEDRON && b->geometry.size() == 1) {
 			const phyz::Polyhedron& ref = (const phyz::Polyhedron&)*b->reference_geometry[0].getGeometry();
 			const phyz::Polyhedron& wrl = (const phyz::Polyhedron&)*b->geometry[0].getGeometry();
@@ -398,6 +425,10 @@ static void dump_static(PhysicsEngine& E, Out& o) {
 	o.i64("body_info", bi, {N, 5});
 	o.f64("material", mat, {N, 4});
 	o.f64("sphere", sph, {N, 4});
+	o.f64("cylinder", cyl, {N, 8});
+	o.f64("cyl_gauss", cyl_g, {N, 14, 3});
+	o.f64("cyl_top", cyl_t, {N, 8, 3});
+	o.f64("cyl_bot", cyl_b, {N, 8, 3});
 	o.f64("hull_points", pts, {pts.size() / 3, 3});
 	o.i64("hull_point_off", poff, {N + 1});
 	o.i64("hull_face_off", foff, {N + 1});
@@ -521,7 +552,8 @@ static int cmd_pile(int argc, char** argv) {
 	PhysicsEngine E;
 	configure(E, cfg);
 	if (arg_d(argc, argv, "--mixed", 0) != 0)
-		build_mixed(E, n, ny, n, seed, arg_d(argc, argv, "--spacing", 1.1), arg_d(argc, argv, "--jitter", 0.05));
+		build_mixed(E, n, ny, n, seed, arg_d(argc, argv, "--spacing", 1.1), arg_d(argc, argv, "--jitter", 0.05),
+		            arg_d(argc, argv, "--mixed", 0) >= 2 ? 3 : 2);
 	else
 		build_pile(E, n, ny, n, seed, arg_d(argc, argv, "--spacing", 1.1), arg_d(argc, argv, "--jitter", 0.05));
 	Out o(arg_s(argc, argv, "--out", "out"));

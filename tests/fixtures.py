@@ -42,9 +42,14 @@ class Fixture:
         return [int(k) for k in self["recorded_steps"]]
 
     def shape(self, b):
-        """Hull dict for polyhedra, dict(sphere=[cx, cy, cz, r]) for spheres."""
-        if int(self["body_info"][b, 4]) == 1:
+        """Hull dict for polyhedra, dict(sphere=[cx, cy, cz, r]) for spheres,
+        dict(cylinder=[c xyz, axis xyz, r, h], gauss=[14,3], top=[8,3], bot=[8,3])
+        for cylinders."""
+        kind = int(self["body_info"][b, 4])
+        if kind == 1:
             return dict(sphere=self["sphere"][b])
+        if kind == 3:
+            return dict(cylinder=self["cylinder"][b], gauss=self["cyl_gauss"][b], top=self["cyl_top"][b], bot=self["cyl_bot"][b])
         return self.hull(b)
 
     def hull(self, b):
@@ -115,6 +120,8 @@ def oracle_lib():
         lib.orc_add_hull_body.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                                           ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 9 + [ctypes.c_int, ctypes.c_void_p,
                                                                                                     ctypes.c_int, ctypes.c_void_p]
+        lib.orc_add_cylinder_body.restype = ctypes.c_int
+        lib.orc_add_cylinder_body.argtypes = [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 6
         lib.orc_add_sphere_body.restype = ctypes.c_int
         lib.orc_add_sphere_body.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double]
         lib.orc_set_cache.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
@@ -155,7 +162,7 @@ class Oracle:
             movement = fx["body_info"][:, 1]
             mp, mat = fx["mass_props"], fx["material"]
             kinds = fx["body_info"][:, 4]
-            hulls = [fx.hull(b) if kinds[b] == 0 else dict(sphere=fx["sphere"][b]) for b in range(n)]
+            hulls = [fx.shape(b) for b in range(n)]
         else:
             n = len(desc["hulls"])
             movement, mp, mat, hulls = desc["movement"], desc["mass_props"], desc["material"], desc["hulls"]
@@ -165,6 +172,12 @@ class Oracle:
         self._keep = []
         for b in range(self.n):
             h = hulls[b]
+            if "cylinder" in h:
+                arrs = [_c(mp[b], np.float64), _c(mat[b], np.float64), _c(h["cylinder"], np.float64), _c(h["gauss"], np.float64),
+                        _c(h["top"], np.float64), _c(h["bot"], np.float64)]
+                self._keep.append(arrs)
+                lib.orc_add_cylinder_body(self.w, int(movement[b]), *[a.ctypes.data for a in arrs])
+                continue
             if "sphere" in h:
                 arrs = [_c(mp[b], np.float64), _c(mat[b], np.float64), _c(h["sphere"][:3], np.float64)]
                 self._keep.append(arrs)

@@ -8,10 +8,14 @@ ROOT=$(cd "$HERE/../.." && pwd)
 make -C "$ROOT/oracle" ref REF="${REF:-/root/reference}"
 H="$ROOT/oracle/_ref/cp_ref_harness"
 cd "$HERE"
-rm -rf pile3 c0_full mix3
+rm -rf pile3 c0_full mix3 cyl3 cyl4
 # C0 pattern, 3x3x3 cubes: full per-step dumps (state, caches, manifolds, islands).
 "$H" pile --n 3 --steps 40 --record 0,1,2,10,39 --out pile3
 # C0 in full (10x10x10 cubes): first step and a settled step (no mid/post caches).
 "$H" pile --n 10 --steps 121 --record 0,120 --lite 1 --out c0_full
 # Mixed pile (cubes and spheres alternating, 3x3x3): sphere-sphere, sphere-box, box-box, ground pairs.
 "$H" pile --mixed 1 --n 3 --steps 60 --record 0,5,30,59 --out mix3
+# Boxes, spheres and tilted cylinders (3x3x3 and 4x3x4): poly-cylinder (both
+# orders), sphere-cylinder, cylinder-cylinder and cylinder-ground pairs.
+"$H" pile --mixed 2 --n 3 --steps 80 --record 0,10,30,60,79 --out cyl3
+"$H" pile --mixed 2 --n 4 --ny 3 --steps 150 --record 40,90,149 --seed 23 --out cyl4

@@ -71,9 +71,12 @@ def test_c0_full_first_and_settled_step():
         assert np.array_equal(o.state(), fx[f"s{k}_post"])
 
 
-@pytest.mark.parametrize("k", [0, 5, 30, 59])
-def test_mixed_spheres_boxes_full_step(k):
-    fx = golden("mix3")
+@pytest.mark.parametrize("name,k", [("mix3", 0), ("mix3", 5), ("mix3", 30), ("mix3", 59), ("cyl3", 0), ("cyl3", 10), ("cyl3", 30),
+                                    ("cyl3", 60), ("cyl3", 79), ("cyl4", 40), ("cyl4", 90), ("cyl4", 149)])
+def test_mixed_primitives_full_step(name, k):
+    """mix3: boxes and spheres; cyl3: boxes, spheres and tilted cylinders
+    (poly-cylinder, sphere-cylinder, cylinder-cylinder and cylinder-ground pairs)."""
+    fx = golden(name)
     o = Oracle(fx)
     _load(o, fx, k)
     assert np.array_equal(o.broadphase(), fx[f"s{k}_pairs"])
