@@ -62,6 +62,8 @@ def lib():
         "cp_shape_add_hull_exact": (_I, [_P, _I, _P, _I] + [_P] * 9 + [_I, _P, _I, _P, _P]),
         "cp_shape_add_polyhedron": (_I, [_P, _I, _P, _I, _P, _P, _P]),
         "cp_shape_add_sphere": (_I, [_P, _P, ctypes.c_double, _P]),
+        "cp_shape_add_cylinder": (_I, [_P, _P, _P, _P, _P, _P]),
+        "cp_cylinder_build": (_I, [_P, ctypes.c_double, ctypes.c_double, _P, _P, _P, _P]),
         "cp_polyhedron_mass_properties": (_I, [_I, _P, _I, _P, _P, ctypes.c_double, _P]),
         "cp_polyhedron_build": (_I, [_I, _P, _I, _P, _P] + [_P] * 8 + [_I, _P, _P, _P, _P]),
         "cp_body_add_many": (_I, [_P, _I, _P, _P, _P, _P, _P, _P]),
@@ -146,6 +148,25 @@ def polyhedron_mass_properties(pts, faces, density):
     return out[0], out[1:10].reshape(3, 3), out[10:13]
 
 
+def cylinder_build(radius, height, center=(0.0, 0.0, 0.0)):
+    """Reference geometry of Cylinder(center, radius, height) with height axis +y."""
+    c = _a(center, np.float64)
+    out = dict(cylinder=np.zeros(8), gauss=np.zeros((14, 3)), top=np.zeros((8, 3)), bot=np.zeros((8, 3)))
+    _chk(lib().cp_cylinder_build(c.ctypes.data, float(radius), float(height), out["cylinder"].ctypes.data, out["gauss"].ctypes.data,
+                                 out["top"].ctypes.data, out["bot"].ctypes.data), "cp_cylinder_build")
+    return out
+
+
+def cylinder_mass_properties(radius, height, density=1.0):
+    """Mass and inertia tensor about the centre of a solid cylinder with height
+    axis +y (RigidBody.cpp:603-631)."""
+    m = 3.1415926535 * radius * radius * height * density
+    t = np.zeros((3, 3))
+    t[0, 0] = t[2, 2] = m * (height * height / 12 + radius * radius / 4)
+    t[1, 1] = m * radius * radius / 2
+    return m, t
+
+
 def polyhedron_build(pts, faces):
     """Host-built polyhedron (surfaces, winding, Gauss map) in the exact-hull layout."""
     pts = _a(pts, np.float64)
@@ -213,6 +234,13 @@ class World:
         c = _a(center, np.float64)
         sid = ctypes.c_int()
         _chk(self.L.cp_shape_add_sphere(self.w, c.ctypes.data, float(radius), ctypes.byref(sid)), "cp_shape_add_sphere")
+        return sid.value
+
+    def add_cylinder(self, cyl):
+        """cyl: dict(cylinder=[centre xyz, axis xyz, r, h], gauss=[14, 3], top=[8, 3], bot=[8, 3])."""
+        arr = [_a(cyl["cylinder"], np.float64), _a(cyl["gauss"], np.float64), _a(cyl["top"], np.float64), _a(cyl["bot"], np.float64)]
+        sid = ctypes.c_int()
+        _chk(self.L.cp_shape_add_cylinder(self.w, *[a.ctypes.data for a in arr], ctypes.byref(sid)), "cp_shape_add_cylinder")
         return sid.value
 
     def add_polyhedron(self, pts, faces):

@@ -92,25 +92,29 @@ struct Contacts {
 	double* row;
 };
 
-#define CP_RS 88
+// Record of CP_RS values (FP64: 768 B = 6 lines of 128 B; FP32: 384 B = 3
+// lines).  The fields a sweep writes (impulses, static flag) share the first
+// line, so one velocity sweep reads the whole record and dirties one line.
+#define CP_RS 96
 enum RowField {
-	CP_R_JA = 0,     // normal row J_a [6]
-	CP_R_JB = 6,     // J_b [6]
-	CP_R_ITA = 12,   // M_a^-1 J_a^T [6]
-	CP_R_ITB = 18,   // M_b^-1 J_b^T [6]
-	CP_R_INV = 24,   // (J M^-1 J^T (1+cfm))^-1
-	CP_R_TGT = 25,   // velocity target
-	CP_R_PTGT = 26,  // pseudo-velocity target
-	CP_R_IMP = 27,   // accumulated normal impulse
-	CP_R_PIMP = 28,  // accumulated pseudo impulse
-	CP_R_MU = 29,    // friction coefficient
-	CP_R_FJA = 30,   // friction J_a [2][6]
-	CP_R_FJB = 42,   // friction J_b [2][6]
-	CP_R_FITA = 54,  // M_a^-1 J_a^T [6][2]
-	CP_R_FITB = 66,  // M_b^-1 J_b^T [6][2]
-	CP_R_FINV = 78,  // 2x2 inverse
-	CP_R_FTGT = 82,  // targets [2]
-	CP_R_FIMP = 84   // accumulated friction impulses [2]
+	CP_R_IMP = 0,     // accumulated normal impulse
+	CP_R_PIMP = 1,    // accumulated pseudo impulse
+	CP_R_FIMP = 2,    // accumulated friction impulses [2]
+	CP_R_SREADY = 4,  // friction cone not saturated in the last sweep (0/1)
+	CP_R_MU = 5,      // friction coefficient
+	CP_R_INV = 6,     // (J M^-1 J^T (1+cfm))^-1
+	CP_R_TGT = 7,     // velocity target
+	CP_R_PTGT = 8,    // pseudo-velocity target
+	CP_R_FTGT = 9,    // friction targets [2]
+	CP_R_FINV = 11,   // friction 2x2 inverse [4]
+	CP_R_JA = 15,     // normal row J_a [6]
+	CP_R_JB = 21,     // J_b [6]
+	CP_R_ITA = 27,    // M_a^-1 J_a^T [6]
+	CP_R_ITB = 33,    // M_b^-1 J_b^T [6]
+	CP_R_FJA = 39,    // friction J_a [2][6]
+	CP_R_FJB = 51,    // friction J_b [2][6]
+	CP_R_FITA = 63,   // M_a^-1 J_a^T [6][2]
+	CP_R_FITB = 75    // M_b^-1 J_b^T [6][2]; 87..95 padding
 };
 
 // A step's manifold points, compacted and sorted by pair key.

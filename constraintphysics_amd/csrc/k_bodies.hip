@@ -147,6 +147,36 @@ __global__ void k_refresh_geometry(Bodies b, Shapes sh, int only_moved) {
 			double* wp = b.wnorm + 3 * (g0 + k);
 			wp[0] = w.x; wp[1] = w.y; wp[2] = w.z;
 		}
+	} else if (sh.type[s] == SHAPE_CYLINDER) {
+		// Cylinder::recomputeFromReference and gen_AABB (ConvexPrimitive.cpp:272-298):
+		// world points [centre, top 8-gon, bottom 8-gon] with bottom = top - h * axis,
+		// normals [axis, 14 Gauss vertices]; AABB over top[0], bot[0], top[1], ...
+		const double* rp = sh.ref_pts + 3 * sh.pt_off[s];
+		const double* gn = sh.g_norm + 3 * sh.face_off[s];
+		double* wp = b.wpts + 3 * b.wpt_off[i];
+		double* wn = b.wnorm + 3 * b.wgv_off[i];
+		V3 c = R * V3{rp[0], rp[1], rp[2]} + com;
+		V3 ax = R * V3{gn[0], gn[1], gn[2]};
+		double h = sh.height[s];
+		wp[0] = c.x; wp[1] = c.y; wp[2] = c.z;
+		wn[0] = ax.x; wn[1] = ax.y; wn[2] = ax.z;
+		for (int k = 1; k < 15; k++) {
+			V3 g = R * V3{gn[3 * k], gn[3 * k + 1], gn[3 * k + 2]};
+			wn[3 * k] = g.x; wn[3 * k + 1] = g.y; wn[3 * k + 2] = g.z;
+		}
+		for (int k = 0; k < 8; k++) {
+			const double* t = rp + 3 * (1 + k);
+			V3 top = R * V3{t[0], t[1], t[2]} + com;
+			V3 bot = top - ax * h;
+			double* wt = wp + 3 * (1 + k);
+			double* wb = wp + 3 * (9 + k);
+			wt[0] = top.x; wt[1] = top.y; wt[2] = top.z;
+			wb[0] = bot.x; wb[1] = bot.y; wb[2] = bot.z;
+			mn0 = dmin(mn0, top.x); mn1 = dmin(mn1, top.y); mn2 = dmin(mn2, top.z);
+			mx0 = dmax(mx0, top.x); mx1 = dmax(mx1, top.y); mx2 = dmax(mx2, top.z);
+			mn0 = dmin(mn0, bot.x); mn1 = dmin(mn1, bot.y); mn2 = dmin(mn2, bot.z);
+			mx0 = dmax(mx0, bot.x); mx1 = dmax(mx1, bot.y); mx2 = dmax(mx2, bot.z);
+		}
 	} else if (sh.type[s] == SHAPE_SPHERE) {
 		// Sphere::recomputeFromReference and gen_AABB (ConvexPrimitive.cpp:433-443)
 		const double* rp = sh.ref_pts + 3 * sh.pt_off[s];

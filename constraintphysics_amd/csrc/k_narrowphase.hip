@@ -336,86 +336,107 @@ __device__ CEP* clip_all(CEP* bufA, CEP* bufB, int n1, const CEP* c2, int n2, bo
 	return cur;
 }
 
-// Serial contact generation for a hull pair after SAT (lane 0).
-__device__ void hull_contacts(const Hull& A, const Hull& B, const Check& mp, const Params& P, uint64_t cid, CEP* lds, int slot,
-                              Manifolds& M) {
+__device__ __forceinline__ uint64_t swap_halves(uint64_t v) {  // swapOrder (CollisionDetect.cpp:28-40)
+	uint64_t lo = v & 0x00000000FFFFFFFFull, hi = (v & 0xFFFFFFFF00000000ull) >> 32;
+	return (0x00000000FFFFFFFFull & hi) + (0xFFFFFFFF00000000ull & (lo << 32));
+}
+
+// clipContacts (CollisionDetect.cpp:578-620) on staged areas: b0 = lds[0..)
+// holds A's na points (surface sa), b1 = lds[MAXP..) B's nb points (surface
+// sb); the rest of lds is scratch.  Returns the buffer holding the pool.
+__device__ CEP* clip_areas(CEP* lds, int na, int sa, int nb, int sb, int* nr) {
 	CEP* b0 = lds;
 	CEP* b1 = lds + MAXP;
 	CEP* b2 = lds + 2 * MAXP;
-	CEP* b3 = lds + 3 * MAXP;
-	V3 n = mp.n;
-	V3 amax = ldv(A.wp, mp.a_pid), bmax = ldv(B.wp, mp.b_pid);
-	V3 u, w;
-	perp_basis(n, &u, &w);
-	int sa, sb;
-	int na = contact_area(A, n, amax, mp.a_pid, u, w, P.cos_tol, P.sin_tol, b0, &sa);
-	int nb = contact_area(B, n * -1.0, bmax, mp.b_pid, u, w, P.cos_tol, P.sin_tol, b1, &sb);
 	clip_poly(b0, na, (uint32_t)sb, false);
 	clip_poly(b1, nb, (uint32_t)sa, true);
-	CEP* res;
-	int nr;
-	if (na == 1) { res = b0; nr = 1; }
-	else if (nb == 1) { res = b1; nr = 1; }
-	else if (na == 2 && nb == 2) {
+	if (na == 1) { *nr = 1; return b0; }
+	if (nb == 1) { *nr = 1; return b1; }
+	if (na == 2 && nb == 2) {
 		double dx, dy;
 		in_dir(b1[0].x, b1[0].y, b1[1].x, b1[1].y, &dx, &dy);
 		b2[0] = intersect(b0[0], b0[1], dx, dy, b1[0].x, b1[0].y, (int32_t)edge_id((uint16_t)b1[0].src, (uint16_t)b1[1].src), false);
-		res = b2;
-		nr = 1;
-	} else if (na == 2) {
-		res = clip_all(b0, b2, na, b1, nb, false, &nr);
-	} else if (nb == 2) {
-		res = clip_all(b1, b2, nb, b0, na, true, &nr);
-	} else {
-		// poly v poly: clip A by B's edges, then B by the clipped A's edges
-		// (an empty clipped A leaves B unclipped, as in the reference).
-		int n1;
-		CEP* r1 = clip_all(b0, b2, na, b1, nb, false, &n1);
-		CEP* scratch = (r1 == b0) ? b2 : b0;
-		res = clip_all(b1, scratch, nb, r1, n1, true, &nr);
+		*nr = 1;
+		return b2;
 	}
+	if (na == 2) return clip_all(b0, b2, na, b1, nb, false, nr);
+	if (nb == 2) return clip_all(b1, b2, nb, b0, na, true, nr);
+	// poly v poly: clip A by B's edges, then B by the clipped A's edges
+	// (an empty clipped A leaves B unclipped, as in the reference).
+	int n1;
+	CEP* r1 = clip_all(b0, b2, na, b1, nb, false, &n1);
+	CEP* scratch = (r1 == b0) ? b2 : b0;
+	return clip_all(b1, scratch, nb, r1, n1, true, nr);
+}
+
+// Lift the pool to 3-D (e.g. CollisionDetect.cpp:809-831) and write the slot,
+// culled to four points (cull_manifold, :2795-2830, with the reference's own
+// cos/sin of 2*pi*i/4).  n is the SAT normal the pool was projected with;
+// flip applies detectCollision's swap for reversed pairs (normal negated,
+// MagicID halves swapped) before the cull, which works on the final normal.
+__device__ void emit_manifold(const CEP* res, int nr, V3 n, V3 u, V3 w, V3 amax, double pen, uint64_t cid, bool flip, const Params& P,
+                              int slot, Manifolds& M, int cap) {
 	double a_dot = dot(amax, n);
 	V3 noff = n * a_dot;
+	V3 nf = flip ? -n : n;
+	if (flip) cid = swap_halves(cid);
+	M.normal[slot] = nf.x;
+	M.normal[cap + slot] = nf.y;
+	M.normal[2 * cap + slot] = nf.z;
 	double* mpts = M.pts + (size_t)slot * 16;
 	uint64_t* mmag = M.magic + (size_t)slot * 8;
 	if (nr <= 4) {
 		for (int k = 0; k < nr; k++) {
 			V3 pos = u * res[k].x + w * res[k].y + noff;
 			mpts[4 * k] = pos.x; mpts[4 * k + 1] = pos.y; mpts[4 * k + 2] = pos.z;
-			mpts[4 * k + 3] = dot(pos, n) - a_dot + mp.pen;
+			mpts[4 * k + 3] = dot(pos, n) - a_dot + pen;
 			mmag[2 * k] = cid;
-			mmag[2 * k + 1] = res[k].magic;
+			mmag[2 * k + 1] = flip ? swap_halves(res[k].magic) : res[k].magic;
 		}
 		M.count[slot] = nr;
-		M.maxpen[slot] = mp.pen;
-	} else {
-		// cull_manifold (:2795-2830) with the reference's own cos/sin of 2*pi*i/4.
-		uint64_t avail = (nr >= 64) ? ~0ull : ((1ull << nr) - 1);
-		V3 cu, cw;
-		perp_basis(n, &cu, &cw);
-		double maxpen = 0;
-		for (int i = 0; i < 4; i++) {
-			V3 t = cu * P.cull_c[i] + cw * P.cull_s[i];
-			double bv = -__builtin_inf();
-			int bi = -1;
-			for (int j = 0; j < nr; j++) {
-				if (!((avail >> j) & 1)) continue;
-				V3 pos = u * res[j].x + w * res[j].y + noff;
-				double v = dot(pos, t);
-				if (v > bv) { bv = v; bi = j; }
-			}
-			if (bi < 0) bi = 0;
-			V3 pos = u * res[bi].x + w * res[bi].y + noff;
-			double pen = dot(pos, n) - a_dot + mp.pen;
-			mpts[4 * i] = pos.x; mpts[4 * i + 1] = pos.y; mpts[4 * i + 2] = pos.z; mpts[4 * i + 3] = pen;
-			mmag[2 * i] = cid;
-			mmag[2 * i + 1] = res[bi].magic;
-			maxpen = (maxpen < pen) ? pen : maxpen;
-			avail &= ~(1ull << bi);
-		}
-		M.count[slot] = 4;
-		M.maxpen[slot] = maxpen;
+		M.maxpen[slot] = pen;
+		return;
 	}
+	uint64_t avail = (nr >= 64) ? ~0ull : ((1ull << nr) - 1);
+	V3 cu, cw;
+	perp_basis(nf, &cu, &cw);
+	double maxpen = 0;
+	for (int i = 0; i < 4; i++) {
+		V3 t = cu * P.cull_c[i] + cw * P.cull_s[i];
+		double bv = -__builtin_inf();
+		int bi = -1;
+		for (int j = 0; j < nr; j++) {
+			if (!((avail >> j) & 1)) continue;
+			V3 pos = u * res[j].x + w * res[j].y + noff;
+			double v = dot(pos, t);
+			if (v > bv) { bv = v; bi = j; }
+		}
+		if (bi < 0) bi = 0;
+		V3 pos = u * res[bi].x + w * res[bi].y + noff;
+		double pp = dot(pos, n) - a_dot + pen;
+		mpts[4 * i] = pos.x; mpts[4 * i + 1] = pos.y; mpts[4 * i + 2] = pos.z; mpts[4 * i + 3] = pp;
+		mmag[2 * i] = cid;
+		mmag[2 * i + 1] = flip ? swap_halves(res[bi].magic) : res[bi].magic;
+		maxpen = (maxpen < pp) ? pp : maxpen;
+		avail &= ~(1ull << bi);
+	}
+	M.count[slot] = 4;
+	M.maxpen[slot] = maxpen;
+}
+
+// Serial contact generation for a hull pair after SAT (lane 0).
+__device__ void hull_contacts(const Hull& A, const Hull& B, const Check& mp, const Params& P, uint64_t cid, CEP* lds, int slot,
+                              Manifolds& M, int cap) {
+	V3 n = mp.n;
+	V3 amax = ldv(A.wp, mp.a_pid), bmax = ldv(B.wp, mp.b_pid);
+	V3 u, w;
+	perp_basis(n, &u, &w);
+	int sa, sb;
+	int na = contact_area(A, n, amax, mp.a_pid, u, w, P.cos_tol, P.sin_tol, lds, &sa);
+	int nb = contact_area(B, n * -1.0, bmax, mp.b_pid, u, w, P.cos_tol, P.sin_tol, lds + MAXP, &sb);
+	int nr;
+	CEP* res = clip_areas(lds, na, sa, nb, sb, &nr);
+	emit_manifold(res, nr, n, u, w, amax, mp.pen, cid, false, P, slot, M, cap);
 }
 
 // ------------------------------------------------------------- spheres
@@ -448,11 +469,6 @@ __device__ Check eval_axis_sphere(const Hull& A, V3 c, double r, int t, bool* va
 	V3 n = normalize(sample - dir * dot(dir, sample));
 	*feat = edge_id((uint16_t)i1, (uint16_t)i2);
 	return sat_check(find_extrema(A, n), sphere_extrema(c, r, n), n);
-}
-
-__device__ __forceinline__ uint64_t swap_halves(uint64_t v) {  // swapOrder (CollisionDetect.cpp:28-40)
-	uint64_t lo = v & 0x00000000FFFFFFFFull, hi = (v & 0xFFFFFFFF00000000ull) >> 32;
-	return (0x00000000FFFFFFFFull & hi) + (0xFFFFFFFF00000000ull & (lo << 32));
 }
 
 __device__ void write_mat(const Bodies& b, int ia, int ib, Manifolds& M, int slot, int cap) {  // CollisionDetect.cpp:820-824
@@ -537,6 +553,352 @@ __device__ void sphere_sphere(const Bodies& b, const Shapes& sh, int ia, int ib,
 	write_mat(b, ia, ib, M, slot, cap);
 }
 
+
+// ------------------------------------------------------------- cylinders
+// World cylinder of a body: points [centre, top cap 8-gon, bottom cap 8-gon],
+// normals [height axis, 14 Gauss vertices] (k_refresh_geometry writes them).
+struct Cyl {
+	V3 c, ax;
+	double r, h;
+	const double *top, *bot, *gv;
+};
+__device__ Cyl make_cyl(const Bodies& b, const Shapes& sh, int body) {
+	int s = b.shape[body];
+	const double* wp = b.wpts + 3 * b.wpt_off[body];
+	const double* wn = b.wnorm + 3 * b.wgv_off[body];
+	Cyl c;
+	c.c = ldv(wp, 0);
+	c.top = wp + 3;
+	c.bot = wp + 27;
+	c.ax = ldv(wn, 0);
+	c.gv = wn + 3;
+	c.r = sh.radius[s];
+	c.h = sh.height[s];
+	return c;
+}
+// MagicID ids of an 8-gon cylinder (ConvexPrimitive.h:174-179).
+#define CYL_TOP_PT 0
+#define CYL_BOT_PT 8
+#define CYL_TOP_SURF 16
+#define CYL_BOT_SURF 17
+#define CYL_TOP_EDGE 18
+#define CYL_BOT_EDGE 19
+
+__device__ __forceinline__ V3 cyl_top_center(const Cyl& c) { return c.c + c.ax * (0.5 * c.h); }  // ConvexPrimitive.h:182
+__device__ __forceinline__ V3 cyl_bot_center(const Cyl& c) { return c.c - c.ax * (0.5 * c.h); }  // :183
+
+// Cylinder::getExtremaOfDisk (ConvexPrimitive.cpp:319-355).
+__device__ V3 disk_extrema(V3 dc, V3 dn, double r, V3 dir) {
+	double nd = dot(dn, dir);
+	if (fabs(nd) > 0.99999999) return dc;
+	if (fabs(nd) < 0.00000001) return dc + dir * r;
+	V3 u, v;
+	V3 w = dir;
+	perp_basis(dir, &u, &v);
+	double nu = dot(dn, u), nv = dot(dn, v), nw = dot(dn, w);
+	double dwdu = -nu / nw;
+	double dwdv = -nv / nw;
+	double gm = sqrt(dwdu * dwdu + dwdv * dwdv);
+	double gu = dwdu / gm;
+	double gvv = dwdv / gm;
+	double uv2 = r * r / (1 + gm * gm);
+	double dw = sqrt(r * r - uv2);
+	double du = gu * sqrt(uv2);
+	double dv = gvv * sqrt(uv2);
+	return dc + u * du + v * dv + w * dw;
+}
+
+// getCylinderExtrema (CollisionDetect.cpp:658-684).
+__device__ Ext cyl_extrema(const Cyl& c, V3 dir) {
+	V3 tc = cyl_top_center(c);
+	V3 tmax = disk_extrema(tc, c.ax, c.r, dir);
+	V3 tmin = tc * 2.0 - tmax;
+	V3 bmax = c.c * 2.0 - tmin;
+	V3 bmin = c.c * 2.0 - tmax;
+	double tmaxv = dot(dir, tmax), tminv = dot(dir, tmin), bmaxv = dot(dir, bmax), bminv = dot(dir, bmin);
+	Ext e;
+	e.max_val = (tmaxv < bmaxv) ? bmaxv : tmaxv;  // std::max<double>
+	e.min_val = (bminv < tminv) ? bminv : tminv;  // std::min<double>
+	e.min_pid = -1;
+	e.max_pid = -1;
+	return e;
+}
+
+// cylinderLengthwiseLineInDirection (CollisionDetect.cpp:347-352).
+__device__ void cyl_line(V3 center, V3 ax, double r, double h, V3 n, V3* p1, V3* p2) {
+	V3 barrel = normalize(n - ax * dot(ax, n));
+	*p1 = center + barrel * r + ax * (0.5 * h);
+	*p2 = *p1 - ax * h;
+}
+
+enum { AREA_FACE = 0, AREA_EDGE = 1, AREA_VERTEX = 2, AREA_BARREL = 3 };
+// findCylinderContactArea (CollisionDetect.cpp:354-374); returns the count,
+// *surface and *origin.
+__device__ int cyl_contact_area(const Cyl& c, V3 n, V3 u, V3 w, double cos_tol, double sin_tol, CEP* out, int* surface, int* origin) {
+	double an = dot(c.ax, n);
+	if (1 - fabs(an) <= cos_tol) {
+		bool top = dot(n, c.ax) > 0;
+		const double* fv = top ? c.top : c.bot;
+		for (int i = 0; i < 8; i++) {
+			V3 v = ldv(fv, i);
+			out[i].x = dot(v, u);
+			out[i].y = dot(v, w);
+			out[i].src = i + (top ? CYL_TOP_PT : CYL_BOT_PT);
+		}
+		*surface = top ? CYL_TOP_SURF : CYL_BOT_SURF;
+		*origin = AREA_FACE;
+		return 8;
+	}
+	*surface = -1;
+	if (fabs(an) <= sin_tol) {
+		V3 p1, p2;
+		cyl_line(c.c, c.ax, c.r, c.h, n, &p1, &p2);
+		out[0].x = dot(p1, u); out[0].y = dot(p1, w); out[0].src = CYL_TOP_EDGE;
+		out[1].x = dot(p2, u); out[1].y = dot(p2, w); out[1].src = CYL_BOT_EDGE;
+		*origin = AREA_BARREL;
+		return 2;
+	}
+	bool top = dot(n, c.ax) > 0;
+	V3 p = disk_extrema(top ? cyl_top_center(c) : cyl_bot_center(c), c.ax, c.r, n);
+	out[0].x = dot(p, u); out[0].y = dot(p, w); out[0].src = top ? CYL_TOP_EDGE : CYL_BOT_EDGE;
+	*origin = AREA_EDGE;
+	return 1;
+}
+
+// Gauss-arc pair direction (arc loop body of CollisionDetect.cpp:1617-1650,
+// :1021-1054); false when pruned.
+__device__ bool arc_axis(V3 a1, V3 a2, V3 b1, V3 b2, V3* out) {
+	V3 aa = a1 + a2;
+	if (dot(aa, b1) + dot(aa, b2) <= 0) return false;
+	V3 ap = cross(a1, a2), bp = cross(b1, b2);
+	if (dot(ap, b1) * dot(ap, b2) > 0 || dot(bp, a1) * dot(bp, a2) > 0) return false;
+	V3 n = cross(ap, bp);
+	if (mag2(n) == 0) return false;
+	n = normalize(n);
+	if (dot(aa, n) < 0) n = n * -1.0;
+	*out = n;
+	return true;
+}
+// Cylinder Gauss arc k as the reference builds it: {0, k/2} or {1, k/2}
+// (ConvexPrimitive.cpp:239-245, second index = loop counter); 24 arcs.
+__device__ __forceinline__ V3 cyl_gv(const Cyl& c, int i) { return ldv(c.gv, i); }
+
+// Candidate axis t of SAT_PolyCylinder (CollisionDetect.cpp:1563-1650): poly
+// face normals, the cylinder axis, poly edge x axis, vertex-to-axis
+// directions, then poly arc x cylinder arc pairs.
+__device__ Check eval_axis_pc(const Hull& A, const Cyl& B, int t, bool* valid) {
+	*valid = true;
+	if (t < A.nf) {
+		if (A.red[t]) { *valid = false; return Check{}; }
+		V3 n = ldv(A.wn, t);
+		return sat_check(recentered(A, t), cyl_extrema(B, n), n);
+	}
+	t -= A.nf;
+	if (t == 0) return sat_check(find_extrema(A, B.ax), cyl_extrema(B, B.ax), B.ax);
+	t -= 1;
+	if (t < A.ne) {
+		V3 ed = ldv(A.wp, A.edges[2 * t + 1]) - ldv(A.wp, A.edges[2 * t]);
+		V3 dir = cross(ed, B.ax);
+		if (mag(dir) < 0.00000000001) { *valid = false; return Check{}; }
+		V3 dn = normalize(dir);
+		return sat_check(find_extrema(A, dn), cyl_extrema(B, dn), dn);
+	}
+	t -= A.ne;
+	if (t < A.np) {
+		V3 diff = ldv(A.wp, t) - B.c;
+		V3 n = normalize(diff - B.ax * dot(B.ax, diff));
+		return sat_check(find_extrema(A, n), cyl_extrema(B, n), n);
+	}
+	t -= A.np;
+	int i1 = t / 24, k = t - i1 * 24;
+	V3 n;
+	if (!arc_axis(ldv(A.wn, A.arcs[2 * i1]), ldv(A.wn, A.arcs[2 * i1 + 1]), -cyl_gv(B, k & 1), -cyl_gv(B, k >> 1), &n)) {
+		*valid = false;
+		return Check{};
+	}
+	return sat_check(find_extrema(A, n), cyl_extrema(B, n), n);
+}
+
+// Candidate axis t of detectCylinderCylinder (CollisionDetect.cpp:958-1054):
+// both axes, cap-polygon edges x the other axis (A top, A bottom, B top, B
+// bottom), the barrel-barrel direction, then 24 x 24 arc pairs.
+__device__ Check eval_axis_cc(const Cyl& A, const Cyl& B, int t, bool* valid) {
+	*valid = true;
+	if (t < 2) {
+		V3 n = t == 0 ? A.ax : B.ax;
+		return sat_check(cyl_extrema(A, n), cyl_extrema(B, n), n);
+	}
+	t -= 2;
+	if (t < 32) {
+		int g = t >> 3, i = t & 7;
+		const double* poly = g == 0 ? A.top : g == 1 ? A.bot : g == 2 ? B.top : B.bot;
+		V3 bax = g < 2 ? B.ax : A.ax;
+		V3 ed = ldv(poly, (i + 1) % 8) - ldv(poly, i);
+		V3 dir = cross(ed, bax);
+		if (mag(dir) < 0.00000000001) { *valid = false; return Check{}; }
+		V3 dn = normalize(dir);
+		return sat_check(cyl_extrema(A, dn), cyl_extrema(B, dn), dn);
+	}
+	t -= 32;
+	if (t == 0) {
+		V3 bb = cross(A.ax, B.ax);
+		if (!(mag2(bb) > 0.00000001)) { *valid = false; return Check{}; }
+		bb = normalize(bb);
+		return sat_check(cyl_extrema(A, bb), cyl_extrema(B, bb), bb);
+	}
+	t -= 1;
+	int k1 = t / 24, k2 = t - k1 * 24;
+	V3 n;
+	if (!arc_axis(cyl_gv(A, k1 & 1), cyl_gv(A, k1 >> 1), -cyl_gv(B, k2 & 1), -cyl_gv(B, k2 >> 1), &n)) {
+		*valid = false;
+		return Check{};
+	}
+	return sat_check(cyl_extrema(A, n), cyl_extrema(B, n), n);
+}
+
+// Wave-wide SAT over naxes candidates: a separating axis anywhere ends the
+// test (returns false); otherwise *bi is the first strict minimum.
+template <class Eval>
+__device__ bool wave_sat(int naxes, int lane, Eval eval, int* bi_out) {
+	double bp = __builtin_inf();
+	int bi = 0x7FFFFFFF;
+	int sep = 0;
+	for (int base = 0; base < naxes; base += 64) {
+		int t = base + lane;
+		if (t < naxes) {
+			bool valid;
+			Check x = eval(t, &valid);
+			if (valid) {
+				if (x.pen < 0) sep = 1;
+				if (x.pen < bp) { bp = x.pen; bi = t; }
+			}
+		}
+		if (__ballot(sep) != 0) return false;
+	}
+	for (int off = 32; off >= 1; off >>= 1) {
+		double obp = __shfl_xor(bp, off);
+		int obi = __shfl_xor(bi, off);
+		if (obp < bp || (obp == bp && obi < bi)) { bp = obp; bi = obi; }
+	}
+	*bi_out = bi;
+	return bp > 0 && bi != 0x7FFFFFFF;  // PhysicsEngine.cpp:213: keep only max_pen_depth > 0
+}
+
+__device__ __forceinline__ uint64_t pair_cid(int ia, int ib) {
+	uint64_t ka = (uint64_t)(ia + 1) * 16, kb = (uint64_t)(ib + 1) * 16;
+	return (0x00000000FFFFFFFFull & ka) | (0xFFFFFFFF00000000ull & (kb << 32));
+}
+
+// SAT_PolyCylinder (CollisionDetect.cpp:1563-1709); poly = body ip, cylinder =
+// body ic; swapped when the cylinder is the pair's first body.
+__device__ void poly_cyl(const Bodies& b, const Shapes& sh, const Params& P, int ip, int ic, bool swapped, int lane, CEP* lds, int slot,
+                         Manifolds& M, int cap) {
+	Hull A = make_hull(b, sh, ip);
+	Cyl B = make_cyl(b, sh, ic);
+	int naxes = A.nf + 1 + A.ne + A.np + A.na * 24;
+	int bi;
+	if (!wave_sat(naxes, lane, [&](int t, bool* v) { return eval_axis_pc(A, B, t, v); }, &bi) || lane != 0) return;
+	bool valid;
+	Check mp = eval_axis_pc(A, B, bi, &valid);
+	V3 n = mp.n;
+	V3 amax = ldv(A.wp, mp.a_pid);
+	V3 u, w;
+	perp_basis(n, &u, &w);
+	int sa, sb, ob;
+	int na = contact_area(A, n, amax, mp.a_pid, u, w, P.cos_tol, P.sin_tol, lds, &sa);
+	int nb = cyl_contact_area(B, -n, u, w, P.cos_tol, P.sin_tol, lds + MAXP, &sb, &ob);
+	CEP* res;
+	int nr;
+	if (ob == AREA_EDGE) {
+		res = lds + MAXP;
+		res[0].magic = 0;
+		nr = 1;
+	} else res = clip_areas(lds, na, sa, nb, sb, &nr);
+	emit_manifold(res, nr, n, u, w, amax, mp.pen, pair_cid(ip, ic), swapped, P, slot, M, cap);
+	write_mat(b, ip, ic, M, slot, cap);
+}
+
+// detectCylinderCylinder (CollisionDetect.cpp:958-1137).
+__device__ void cyl_cyl(const Bodies& b, const Shapes& sh, const Params& P, int ia, int ib, int lane, CEP* lds, int slot, Manifolds& M,
+                        int cap) {
+	Cyl A = make_cyl(b, sh, ia), B = make_cyl(b, sh, ib);
+	int bi;
+	if (!wave_sat(35 + 24 * 24, lane, [&](int t, bool* v) { return eval_axis_cc(A, B, t, v); }, &bi) || lane != 0) return;
+	bool valid;
+	Check mp = eval_axis_cc(A, B, bi, &valid);
+	V3 n = mp.n;
+	V3 u, w;
+	perp_basis(n, &u, &w);
+	int sa, sb, oa, ob;
+	int na = cyl_contact_area(A, n, u, w, P.cos_tol, P.sin_tol, lds, &sa, &oa);
+	int nb = cyl_contact_area(B, -n, u, w, P.cos_tol, P.sin_tol, lds + MAXP, &sb, &ob);
+	CEP* res;
+	int nr;
+	if (oa == AREA_EDGE) {
+		res = lds;
+		res[0].magic = 0;
+		nr = 1;
+	} else if (ob == AREA_EDGE) {
+		res = lds + MAXP;
+		res[0].magic = 0;
+		nr = 1;
+	} else if (oa == AREA_BARREL && ob == AREA_BARREL && fabs(dot(A.ax, B.ax)) > 0.995) {
+		// parallel barrels (:1082-1104): overlap of the two lines along A's axis
+		V3 a1, a2, b1, b2;
+		cyl_line(A.c, A.ax, A.r, A.h, n, &a1, &a2);
+		cyl_line(B.c, B.ax, B.r, B.h, -n, &b1, &b2);
+		if (dot(B.ax, A.ax) < 0) { V3 t = b1; b1 = b2; b2 = t; }
+		double amaxv = dot(a1, A.ax), aminv = dot(a2, A.ax), bmaxv = dot(b1, A.ax), bminv = dot(b2, A.ax);
+		V3 imax = (amaxv > bmaxv) ? b1 : a1;
+		V3 imin = (aminv > bminv) ? a2 : b2;
+		res = lds + 2 * MAXP;
+		res[0].x = dot(u, imax); res[0].y = dot(w, imax); res[0].magic = 0x0;
+		res[1].x = dot(u, imin); res[1].y = dot(w, imin); res[1].magic = 0x1;
+		nr = 2;
+	} else res = clip_areas(lds, na, sa, nb, sb, &nr);
+	V3 amax = dot(A.ax, n) > 0 ? disk_extrema(cyl_top_center(A), A.ax, A.r, n) : disk_extrema(cyl_bot_center(A), A.ax, A.r, n);
+	emit_manifold(res, nr, n, u, w, amax, mp.pen, pair_cid(ia, ib), false, P, slot, M, cap);
+	write_mat(b, ia, ib, M, slot, cap);
+}
+
+// detectSphereCylinder (CollisionDetect.cpp:1712-1792), lane 0.  The second
+// disk centre repeats the top one (+h/2), as in the reference.
+__device__ void sphere_cyl(const Bodies& b, const Shapes& sh, int is_, int ic, bool swapped, int slot, Manifolds& M, int cap) {
+	V3 c = ldv(b.wpts + 3 * b.wpt_off[is_], 0);
+	double r = sh.radius[b.shape[is_]];
+	Cyl B = make_cyl(b, sh, ic);
+	Check mp{-1, -1, V3{0, 0, 0}, __builtin_inf()};
+	V3 dirs[4];
+	V3 diff = B.c - c;
+	dirs[0] = B.ax;
+	dirs[1] = normalize(diff - B.ax * dot(B.ax, diff));
+	V3 tdc = B.c + B.ax * 0.5 * B.h;
+	V3 tcp = disk_extrema(tdc, B.ax, B.r, -normalize(diff));
+	dirs[2] = normalize(tcp - c);
+	dirs[3] = dirs[2];  // bottom disk centre evaluates to the top one
+	for (int i = 0; i < 4; i++) {
+		Check x = sat_check(sphere_extrema(c, r, dirs[i]), cyl_extrema(B, dirs[i]), dirs[i]);
+		if (x.pen < 0) return;
+		if (x.pen < mp.pen) mp = x;
+	}
+	if (!(mp.pen > 0)) return;
+	V3 n = mp.n;
+	V3 pos = c + n * r;
+	uint64_t cid = pair_cid(is_, ic), bid = 0;
+	if (swapped) {
+		n = -n;
+		cid = swap_halves(cid);
+	}
+	M.normal[slot] = n.x; M.normal[cap + slot] = n.y; M.normal[2 * cap + slot] = n.z;
+	double* mpts = M.pts + (size_t)slot * 16;
+	mpts[0] = pos.x; mpts[1] = pos.y; mpts[2] = pos.z; mpts[3] = mp.pen;
+	M.magic[(size_t)slot * 8] = cid;
+	M.magic[(size_t)slot * 8 + 1] = bid;
+	M.maxpen[slot] = mp.pen;
+	M.count[slot] = 1;
+	write_mat(b, is_, ic, M, slot, cap);
+}
+
 // Removes non-finite points from a written manifold slot (lane 0); a
 // non-finite normal drops the whole manifold.
 __device__ void sanitize_slot(Manifolds& M, int slot, int cap) {
@@ -591,6 +953,31 @@ __global__ __launch_bounds__(64 * WAVES_PER_BLOCK) void k_narrowphase(Bodies b, 
 			if (lane == 0 && (P.defect_fixes & 1)) sanitize_slot(M, slot, cap);
 			continue;
 		}
+		if (ta == SHAPE_HULL && tb == SHAPE_CYLINDER) {
+			poly_cyl(b, sh, P, ia, ib, false, lane, lds[wib], slot, M, cap);
+			if (lane == 0 && (P.defect_fixes & 1)) sanitize_slot(M, slot, cap);
+			continue;
+		}
+		if (ta == SHAPE_CYLINDER && tb == SHAPE_HULL) {
+			poly_cyl(b, sh, P, ib, ia, true, lane, lds[wib], slot, M, cap);
+			if (lane == 0 && (P.defect_fixes & 1)) sanitize_slot(M, slot, cap);
+			continue;
+		}
+		if (ta == SHAPE_CYLINDER && tb == SHAPE_CYLINDER) {
+			cyl_cyl(b, sh, P, ia, ib, lane, lds[wib], slot, M, cap);
+			if (lane == 0 && (P.defect_fixes & 1)) sanitize_slot(M, slot, cap);
+			continue;
+		}
+		if (ta == SHAPE_SPHERE && tb == SHAPE_CYLINDER) {
+			if (lane == 0) sphere_cyl(b, sh, ia, ib, false, slot, M, cap);
+			if (lane == 0 && (P.defect_fixes & 1)) sanitize_slot(M, slot, cap);
+			continue;
+		}
+		if (ta == SHAPE_CYLINDER && tb == SHAPE_SPHERE) {
+			if (lane == 0) sphere_cyl(b, sh, ib, ia, true, slot, M, cap);
+			if (lane == 0 && (P.defect_fixes & 1)) sanitize_slot(M, slot, cap);
+			continue;
+		}
 		if (ta != SHAPE_HULL || tb != SHAPE_HULL) continue;
 		Hull A = make_hull(b, sh, ia), B = make_hull(b, sh, ib);
 		// Stages in the reference's axis order (A faces, B faces, then arc pairs in
@@ -629,10 +1016,7 @@ __global__ __launch_bounds__(64 * WAVES_PER_BLOCK) void k_narrowphase(Bodies b, 
 			M.mat[slot] = (ra < rb) ? rb : ra;
 			M.mat[cap + slot] = (b.mat[b.n + ia] + b.mat[b.n + ib]) / 2.0;
 			M.mat[2 * cap + slot] = (b.mat[2 * b.n + ia] + b.mat[2 * b.n + ib]) / 2.0;
-			M.normal[slot] = mp.n.x;
-			M.normal[cap + slot] = mp.n.y;
-			M.normal[2 * cap + slot] = mp.n.z;
-			hull_contacts(A, B, mp, P, cid, lds[wib], slot, M);
+			hull_contacts(A, B, mp, P, cid, lds[wib], slot, M, cap);
 			if (P.defect_fixes & 1) sanitize_slot(M, slot, cap);
 		}
 	}

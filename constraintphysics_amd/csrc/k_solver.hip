@@ -182,6 +182,7 @@ __global__ void k_build_rows(Bodies b, Contacts C, int nc, Params P) {
 		C.fu[k] = u.x; C.fu[K + k] = u.y; C.fu[2 * K + k] = u.z;
 		C.fw[k] = w.x; C.fw[K + k] = w.y; C.fw[2 * K + k] = w.z;
 		C.sready[k] = 0;
+		R[CP_R_SREADY] = 0;
 		for (int r = 0; r < 2; r++) {
 			double x = 0, y = 0;
 			for (int q = 0; q < 6; q++) { x += va[q] * ja[r][q]; y += vb[q] * jb[r][q]; }
@@ -276,7 +277,7 @@ __device__ __forceinline__ void normal_step(T* R, const D& A, const D& B, bool p
 
 // PGSConstraintStep<2> with FrictionConstraint::projectValidImpulse (ConstraintSolver.cpp:331-343).
 template <class T, class D>
-__device__ __forceinline__ void friction_step(T* R, int* sready, const D& A, const D& B) {
+__device__ __forceinline__ void friction_step(T* R, const D& A, const D& B) {
 	T cur[2];
 	for (int r = 0; r < 2; r++) {
 		T x = 0, y = 0;
@@ -301,12 +302,12 @@ __device__ __forceinline__ void friction_step(T* R, int* sready, const D& A, con
 	T cm = tsqrt(n0 * n0 + n1 * n1);
 	T p0, p1;
 	if (cm > maxm) {
-		*sready = 0;
+		R[CP_R_SREADY] = 0;
 		T q = maxm / cm;
 		p0 = n0 * q;
 		p1 = n1 * q;
 	} else {
-		*sready = 1;
+		R[CP_R_SREADY] = 1;
 		p0 = n0;
 		p1 = n1;
 	}
@@ -363,7 +364,7 @@ __global__ void k_pgs_ordered(Bodies b, Contacts C, const int* rows, const int* 
 			refs(k, A, B);
 			double* R = rec<double>(C, k);
 			if (rows[2 * r + 1] == 0) normal_step(R, A, B, false);
-			else friction_step(R, &C.sready[k], A, B);
+			else friction_step(R, A, B);
 		}
 	for (int it = 0; it < P.pos_iters; it++)
 		for (int r = r0; r < r1; r++) {
@@ -406,7 +407,7 @@ __device__ __forceinline__ void colour_phase(Contacts& C, const int* order, int 
 				warm_start(R, A, B, 1);
 			} else if (kind == 1) {
 				normal_step(R, A, B, false);
-				friction_step(R, &C.sready[k], A, B);
+				friction_step(R, A, B);
 			} else normal_step(R, A, B, true);
 		}
 	}
@@ -487,8 +488,8 @@ __global__ void k_pgs_colored_lds(Bodies b, Contacts C, const int* scene_list, c
 	}
 }
 
-// Accumulated impulses back into the persistent FP64 contact cache (warm start
-// of the next substep; Contact::warmStart state, PersistentConstraint.cpp:201-215).
+// Accumulated impulses and the static flag back into the persistent FP64
+// contact cache (warm start of the next substep; PersistentConstraint.cpp:201-215).
 template <class T>
 __global__ void k_rows_writeback(Contacts C, int nc) {
 	int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -497,6 +498,7 @@ __global__ void k_rows_writeback(Contacts C, int nc) {
 	C.imp_n[k] = (double)R[CP_R_IMP];
 	C.imp_t[k] = (double)R[CP_R_FIMP];
 	C.imp_t[C.cap + k] = (double)R[CP_R_FIMP + 1];
+	C.sready[k] = R[CP_R_SREADY] != T(0);
 }
 
 // FP64 natural-map residual of the velocity rows after the sweeps, evaluated

@@ -74,6 +74,65 @@ int HostShapes::add_sphere(const double* center, double r) {
 	return id;
 }
 
+int HostShapes::add_cylinder(const double* params8, const double* gauss42, const double* top24, const double* bot24) {
+	int id = (int)type.size();
+	type.push_back(2);
+	ref_pts.insert(ref_pts.end(), params8, params8 + 3);
+	ref_pts.insert(ref_pts.end(), top24, top24 + 24);
+	ref_pts.insert(ref_pts.end(), bot24, bot24 + 24);
+	pt_off.push_back(pt_off.back() + 17);
+	for (int v = 0; v < 17; v++) {
+		vaf_off.push_back((int)vaf.size());
+		vae_off.push_back((int)vae.size());
+	}
+	// 15 normal slots (axis, Gauss vertices); the hull face fields stay empty
+	for (int f = 0; f < 15; f++) {
+		const double* g = f == 0 ? params8 + 3 : gauss42 + 3 * (f - 1);
+		gnorm.insert(gnorm.end(), {g[0], g[1], g[2]});
+		fv_off.push_back((int)fv.size());
+		ncalc.push_back(0);
+		ndir.push_back(0);
+		sid.push_back(0);
+		red.push_back(1);
+		minpid.push_back(0);
+		maxpid.push_back(0);
+		refidx.push_back(0);
+		gmin.push_back(0);
+		gmax.push_back(0);
+		grefval.push_back(0);
+	}
+	face_off.push_back(face_off.back() + 15);
+	arc_off.push_back(arc_off.back());
+	edge_off.push_back(edge_off.back());
+	radius.push_back(params8[6]);
+	height.push_back(params8[7]);
+	return id;
+}
+
+void build_cylinder(const double* center, double radius, double height, double* params8, double* gauss42, double* top24, double* bot24) {
+	const double PI = 3.1415926535;  // the reference's PI macro
+	params8[0] = center[0]; params8[1] = center[1]; params8[2] = center[2];
+	params8[3] = 0; params8[4] = 1; params8[5] = 0;
+	params8[6] = radius;
+	params8[7] = height;
+	double g[14][3] = {{0, 1, 0}, {0, -1, 0}};
+	double dt = 2 * PI / 12;
+	for (int i = 0; i < 12; i++) {
+		double th = i * dt;
+		g[i + 2][0] = std::sin(th); g[i + 2][1] = 0; g[i + 2][2] = std::cos(th);
+	}
+	for (int i = 0; i < 14; i++)
+		for (int c = 0; c < 3; c++) gauss42[3 * i + c] = g[i][c];
+	double df = 2 * PI / 8;
+	double er = radius / std::cos(PI / 8);
+	for (int i = 0; i < 8; i++) {
+		double th = i * df;
+		double x = er * std::sin(th), z = er * std::cos(th);
+		top24[3 * i] = center[0] + x; top24[3 * i + 1] = center[1] + height / 2.0; top24[3 * i + 2] = center[2] + z;
+		bot24[3 * i] = center[0] + x; bot24[3 * i + 1] = center[1] + -height / 2.0; bot24[3 * i + 2] = center[2] + z;
+	}
+}
+
 namespace {
 
 struct Surf {

@@ -29,10 +29,19 @@ struct HostShapes {
 
 	int add_hull(const HullData& d);
 	int add_sphere(const double* center, double r);
+	// Cylinder as the reference stores it: params8 = centre xyz, height axis
+	// xyz, radius, height; gauss42 = 14 Gauss vertices; top24 = top 8-gon.
+	// Stored as points [centre, top 8-gon, bottom 8-gon] and normals [axis,
+	// Gauss vertices] (the bottom polygon is derived per step).
+	int add_cylinder(const double* params8, const double* gauss42, const double* top24, const double* bot24);
 };
 
 // Polyhedron(points, surface_vertex_indices) (ConvexPrimitive.cpp:582-658)
 // with Surface construction (:783-858) and computeGaussMap (:861-896).
+// Cylinder(center, radius, height, axis (0,1,0), 12, 8) (ConvexPrimitive.cpp:222-256)
+// in the layout add_cylinder takes.
+void build_cylinder(const double* center, double radius, double height, double* params8, double* gauss42, double* top24, double* bot24);
+
 HullData build_polyhedron(const std::vector<double>& pts, const std::vector<std::vector<int>>& faces);
 
 // Mass, inertia tensor about the centre of mass and the centre of mass of a

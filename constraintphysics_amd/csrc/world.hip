@@ -282,9 +282,10 @@ void cp_world::upload() {
 		}
 		for (int c = 0; c < 3; c++) mt[c * (size_t)n + i] = b_mat[4 * (size_t)i + 1 + c];
 		int s = b_shape[i];
-		bool hull = h.type[s] == SHAPE_HULL;
+		// world points and normals per body: hull vertices and Gauss normals;
+		// sphere centre; cylinder centre + cap polygons and axis + Gauss vertices
 		wpo[i + 1] = wpo[i] + (h.pt_off[s + 1] - h.pt_off[s]);
-		wgo[i + 1] = wgo[i] + (hull ? h.face_off[s + 1] - h.face_off[s] : 0);
+		wgo[i + 1] = wgo[i] + (h.face_off[s + 1] - h.face_off[s]);
 	}
 	up(Iref, Ir); up(Irefinv, Iri); up(mass, ms); up(mat, mt);
 	up(movement, b_move); up(shape, b_shape); up(scene, b_scene);
@@ -780,6 +781,21 @@ int cp_shape_add_sphere(cp_world* w, const double* center, double radius, int* s
 	if (!w || !center || !shape_id || !(radius > 0)) return CP_ERR_INVALID;
 	*shape_id = w->hs.add_sphere(center, radius);
 	w->uploaded = false;
+	return CP_OK;
+}
+
+int cp_shape_add_cylinder(cp_world* w, const double* params8, const double* gauss42, const double* top24, const double* bot24,
+                          int* shape_id) {
+	if (!w || !params8 || !gauss42 || !top24 || !bot24 || !shape_id || !(params8[6] > 0) || !(params8[7] > 0)) return CP_ERR_INVALID;
+	*shape_id = w->hs.add_cylinder(params8, gauss42, top24, bot24);
+	w->uploaded = false;
+	return CP_OK;
+}
+
+int cp_cylinder_build(const double* center, double radius, double height, double* params8, double* gauss42, double* top24,
+                      double* bot24) {
+	if (!center || !params8 || !gauss42 || !top24 || !bot24 || !(radius > 0) || !(height > 0)) return CP_ERR_INVALID;
+	build_cylinder(center, radius, height, params8, gauss42, top24, bot24);
 	return CP_OK;
 }
 

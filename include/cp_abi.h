@@ -93,6 +93,17 @@ int cp_shape_add_hull_exact(cp_world* w, int n_pts, const double* pts, int n_fac
 /* Sphere (Sphere(center, radius), ConvexPrimitive.h:76-102): centre in the body
  * frame (the reference keeps the primitive's offset from the centre of mass). */
 int cp_shape_add_sphere(cp_world* w, const double* center, double radius, int* shape_id);
+/* Cylinder (Cylinder(center, radius, height, axis, 12, 8), ConvexPrimitive.cpp:222-256)
+ * exactly as the reference stores its body-frame reference geometry:
+ * params8 = centre xyz, height axis xyz, radius, height; gauss42 = 14 Gauss
+ * vertices (cap normals, then the 12-gon ring); top24/bot24 = the 8-gon cap
+ * polygons (radius r / cos(pi/8)). */
+int cp_shape_add_cylinder(cp_world* w, const double* params8, const double* gauss42, const double* top24, const double* bot24,
+                          int* shape_id);
+/* Host-only: the reference geometry of Cylinder(center, radius, height) with
+ * height axis +y, in the cp_shape_add_cylinder layout. */
+int cp_cylinder_build(const double* center, double radius, double height, double* params8, double* gauss42, double* top24,
+                      double* bot24);
 /* Convex polyhedron from points and face vertex lists (Polyhedron(points,
  * surface_vertex_indices), ConvexPrimitive.h:216): builds surfaces, edges and
  * the Gauss map on the host the way the reference constructor does.  Points

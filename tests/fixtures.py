@@ -284,6 +284,27 @@ def world_from_fixture(fx, pgs_mode=1, **cfg):
     info, mp, mat = fx["body_info"], fx["mass_props"], fx["material"]
     for b in range(int(m["n_bodies"])):
         sh = fx.shape(b)
-        sid = w.add_sphere(sh["sphere"][:3], sh["sphere"][3]) if "sphere" in sh else w.add_hull_exact(sh)
+        if "sphere" in sh:
+            sid = w.add_sphere(sh["sphere"][:3], sh["sphere"][3])
+        elif "cylinder" in sh:
+            sid = w.add_cylinder(sh)
+        else:
+            sid = w.add_hull_exact(sh)
         w.add_body(sid, int(info[b, 1]), mp[b], mat[b], 0)
+    return w
+
+
+def world_from_description(desc, **cfg):
+    """Device world holding the bodies of an oracle description (one scene)."""
+    import constraintphysics_amd as cpa
+    w = cpa.World(**cfg)
+    for b, sh in enumerate(desc["hulls"]):
+        if "sphere" in sh:
+            sid = w.add_sphere(sh["sphere"][:3], sh["sphere"][3])
+        elif "cylinder" in sh:
+            sid = w.add_cylinder(sh)
+        else:
+            sid = w.add_hull_exact(sh)
+        w.add_body(sid, int(desc["movement"][b]), desc["mass_props"][b], desc["material"][b], 0)
+    w.set_state(desc["state"])
     return w

@@ -53,7 +53,7 @@ def _compare(make, load):
     return imp, vel, pos, r64, r32
 
 
-@pytest.mark.parametrize("name,k", [("pile3", 10), ("mix3", 30), ("mix3", 59)])
+@pytest.mark.parametrize("name,k", [("pile3", 10), ("mix3", 30), ("mix3", 59), ("cyl4", 149)])
 def test_split_matches_fp64_on_reference_states(name, k):
     fx = golden(name)
 
