@@ -211,9 +211,9 @@ _LIBS = {}
 SPHERE_PI = 3.1415926535  # the reference's PI macro (Quaternion.h:4), used for sphere mass (RigidBody.cpp:556)
 
 # Primitive mix of the mixed-primitive configuration: 40% spheres, 30% cylinders,
-# 30% hulls.  Kinds the narrowphase does not support yet are left out and the
-# remaining weights renormalised (reported in the bench line).
-MIX = (("sphere", 0.4), ("hull", 0.3))
+# 30% hulls.  Body kinds: 0 hull, 1 sphere, 2 cylinder.
+MIX = (("sphere", 0.4), ("cylinder", 0.3), ("hull", 0.3))
+KIND = {"hull": 0, "sphere": 1, "cylinder": 2}
 
 
 def hull_library(seed, count=256):
@@ -233,16 +233,30 @@ def sphere_library(seed, count=256, density=1.0):
     return r, mp
 
 
-def scene_bodies(sizes, seed, s, lib, sph_r):
-    """Dynamic bodies of scene s: kind (0 hull, 1 sphere), library pick,
-    container side, positions, orientations, velocities."""
+def cylinder_library(seed, count=256, density=1.0):
+    """Seeded cylinders r ~ U[0.2, 0.5], h ~ U[0.5, 1.5] (height axis +y in the
+    body frame): reference geometry, mass properties, bounding radius."""
+    rng = np.random.default_rng(seed ^ 0xC711)
+    r = rng.uniform(0.2, 0.5, count)
+    h = rng.uniform(0.5, 1.5, count)
+    geo = [cpa.cylinder_build(ri, hi) for ri, hi in zip(r, h)]
+    mp = np.array([mass_props_row(*cpa.cylinder_mass_properties(ri, hi, density)) for ri, hi in zip(r, h)])
+    re = r / math.cos(3.1415926535 / 8)  # the cap polygons' encapsulating radius
+    return geo, mp, np.sqrt(re * re + (h / 2) ** 2)
+
+
+def scene_bodies(sizes, seed, s, lib, sph_r, cyl_r=None):
+    """Dynamic bodies of scene s: kind (0 hull, 1 sphere, 2 cylinder), library
+    pick, container side, positions, orientations, velocities."""
     n = int(sizes[s])
     rng = np.random.default_rng(seed + s)
     w = np.array([m[1] for m in MIX])
     kinds = rng.choice(len(MIX), size=n, p=w / w.sum())
-    kind = np.array([0 if MIX[k][0] == "hull" else 1 for k in kinds])
+    kind = np.array([KIND[MIX[k][0]] for k in kinds])
     picks = rng.integers(0, len(lib.items), n)
-    radii = np.where(kind == 0, lib.r[picks], sph_r[picks] + 0.05)
+    if cyl_r is None:
+        cyl_r = cylinder_library(seed, len(lib.items))[2]
+    radii = np.select([kind == 0, kind == 1], [lib.r[picks], sph_r[picks] + 0.05], cyl_r[picks] + 0.05)
     pitch = 2.0 * float(radii.max())
     L = max((n / C1_DENSITY) ** (1.0 / 3.0), pitch * math.ceil(n ** (1.0 / 3.0)))
     pos = place_bodies(rng, radii, L)
@@ -264,8 +278,8 @@ def container_walls(L, t=1.0):
 def mix_description():
     tot = sum(m[1] for m in MIX)
     parts = ", ".join(f"{100 * m[1] / tot:.0f}% {m[0]}s" for m in MIX)
-    return (f"{parts} (cylinders pending); hulls: 8-32 points on ellipsoids, semi-axes U[0.3,0.8] (256-shape seeded library); "
-            f"spheres r~U[0.3,0.7] (256-radius seeded library); box walls")
+    return (f"{parts}; hulls: 8-32 points on ellipsoids, semi-axes U[0.3,0.8] (256-shape seeded library); "
+            f"spheres r~U[0.3,0.7], cylinders r~U[0.2,0.5] h~U[0.5,1.5] (256-entry seeded libraries); box walls")
 
 
 def build_batched(world, sizes, seed, scene_indices=None, hull_lib_size=256):
@@ -277,16 +291,18 @@ def build_batched(world, sizes, seed, scene_indices=None, hull_lib_size=256):
     sph_r, sph_mp = sphere_library(seed, hull_lib_size)
     lib_sid = np.array([world.add_polyhedron(it["pts"], it["faces"]) for it in lib.items])
     sph_sid = np.array([world.add_sphere((0.0, 0.0, 0.0), r) for r in sph_r])
+    cyl_geo, cyl_mp, cyl_r = cylinder_library(seed, hull_lib_size)
+    cyl_sid = np.array([world.add_cylinder(g) for g in cyl_geo])
     idx = range(len(sizes)) if scene_indices is None else scene_indices
     total = 0
     for out_s, s in enumerate(idx):
-        kind, picks, L, pos, q, vel = scene_bodies(sizes, seed, s, lib, sph_r)
+        kind, picks, L, pos, q, vel = scene_bodies(sizes, seed, s, lib, sph_r, cyl_r)
         for (dx, dy, dz), c in container_walls(L):
             sid, mp = b.box_shape(dx, dy, dz, 1.0)
             b.add(sid, mp, 1, DEFAULT_MATERIAL, out_s, np.array(c))
         n = len(picks)
-        shp = np.where(kind == 0, lib_sid[picks], sph_sid[picks])
-        mps = np.where((kind == 0)[:, None], lib.mp[picks], sph_mp[picks])
+        shp = np.select([kind == 0, kind == 1], [lib_sid[picks], sph_sid[picks]], cyl_sid[picks])
+        mps = np.select([(kind == 0)[:, None], (kind == 1)[:, None]], [lib.mp[picks], sph_mp[picks]], cyl_mp[picks])
         b.add_many(shp, mps, np.zeros(n, np.int32), np.tile(DEFAULT_MATERIAL, (n, 1)), np.full(n, out_s), pos, q, vel, np.zeros((n, 3)))
         total += n
     b.finish()
@@ -298,7 +314,8 @@ def scene_description(sizes, seed, s, hull_lib_size=256):
     exact-hull layout, movement, mass properties, material, and the state."""
     lib = hull_library(seed, hull_lib_size)
     sph_r, sph_mp = sphere_library(seed, hull_lib_size)
-    kind, picks, L, pos, q, vel = scene_bodies(sizes, seed, s, lib, sph_r)
+    cyl_geo, cyl_mp, cyl_r = cylinder_library(seed, hull_lib_size)
+    kind, picks, L, pos, q, vel = scene_bodies(sizes, seed, s, lib, sph_r, cyl_r)
     hulls, move, mps, mats, state = [], [], [], [], []
     cache = {}
     for (dx, dy, dz), c in container_walls(L):
@@ -310,6 +327,13 @@ def scene_description(sizes, seed, s, hull_lib_size=256):
         mats.append(DEFAULT_MATERIAL)
         state.append(np.concatenate([c, [1, 0, 0, 0], [0, 0, 0], [0, 0, 0]]))
     for i, p in enumerate(picks):
+        if kind[i] == 2:
+            hulls.append(cyl_geo[p])
+            move.append(0)
+            mps.append(cyl_mp[p])
+            mats.append(DEFAULT_MATERIAL)
+            state.append(np.concatenate([pos[i], q[i], vel[i], [0, 0, 0]]))
+            continue
         if kind[i] == 1:
             hulls.append(dict(sphere=np.array([0.0, 0.0, 0.0, sph_r[p]])))
             move.append(0)
