@@ -136,8 +136,9 @@ void launch_bp_keys(const Bodies& b, uint64_t* keys, int* vals, hipStream_t s);
 void launch_bp_gather(const Bodies& b, const int* sorted, double* sb, hipStream_t s);
 void launch_bp_sweep(int n, const int* sorted, const double* sb, const int* scene_of_body, const int* scene_body_off, int* counts,
                      const int* offs, uint64_t* out, int write, hipStream_t s);
+// lists: 6 * npairs ints (per-class pair lists), counts: 8 ints.
 void launch_narrowphase(const Bodies& b, const Shapes& sh, const Params& p, const uint64_t* pairs, int npairs, const Manifolds& m,
-                        int cap, hipStream_t s);
+                        int cap, int* lists, int* counts, hipStream_t s);
 void launch_np_gather(const Manifolds& M, int mcap, const uint64_t* pairs, int npairs, const int* off, const NewPts& np, int cap,
                       hipStream_t s);
 void launch_np_match(const NewPts& np, int nnew, int cap, const Contacts& C, int nold, const Bodies& b, const Params& P,

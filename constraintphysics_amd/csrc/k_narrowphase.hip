@@ -756,50 +756,15 @@ __device__ Check eval_axis_cc(const Cyl& A, const Cyl& B, int t, bool* valid) {
 	return sat_check(cyl_extrema(A, n), cyl_extrema(B, n), n);
 }
 
-// Wave-wide SAT over naxes candidates: a separating axis anywhere ends the
-// test (returns false); otherwise *bi is the first strict minimum.
-template <class Eval>
-__device__ bool wave_sat(int naxes, int lane, Eval eval, int* bi_out) {
-	double bp = __builtin_inf();
-	int bi = 0x7FFFFFFF;
-	int sep = 0;
-	for (int base = 0; base < naxes; base += 64) {
-		int t = base + lane;
-		if (t < naxes) {
-			bool valid;
-			Check x = eval(t, &valid);
-			if (valid) {
-				if (x.pen < 0) sep = 1;
-				if (x.pen < bp) { bp = x.pen; bi = t; }
-			}
-		}
-		if (__ballot(sep) != 0) return false;
-	}
-	for (int off = 32; off >= 1; off >>= 1) {
-		double obp = __shfl_xor(bp, off);
-		int obi = __shfl_xor(bi, off);
-		if (obp < bp || (obp == bp && obi < bi)) { bp = obp; bi = obi; }
-	}
-	*bi_out = bi;
-	return bp > 0 && bi != 0x7FFFFFFF;  // PhysicsEngine.cpp:213: keep only max_pen_depth > 0
-}
-
 __device__ __forceinline__ uint64_t pair_cid(int ia, int ib) {
 	uint64_t ka = (uint64_t)(ia + 1) * 16, kb = (uint64_t)(ib + 1) * 16;
 	return (0x00000000FFFFFFFFull & ka) | (0xFFFFFFFF00000000ull & (kb << 32));
 }
 
-// SAT_PolyCylinder (CollisionDetect.cpp:1563-1709); poly = body ip, cylinder =
-// body ic; swapped when the cylinder is the pair's first body.
-__device__ void poly_cyl(const Bodies& b, const Shapes& sh, const Params& P, int ip, int ic, bool swapped, int lane, CEP* lds, int slot,
-                         Manifolds& M, int cap) {
-	Hull A = make_hull(b, sh, ip);
-	Cyl B = make_cyl(b, sh, ic);
-	int naxes = A.nf + 1 + A.ne + A.np + A.na * 24;
-	int bi;
-	if (!wave_sat(naxes, lane, [&](int t, bool* v) { return eval_axis_pc(A, B, t, v); }, &bi) || lane != 0) return;
-	bool valid;
-	Check mp = eval_axis_pc(A, B, bi, &valid);
+// Contact generation of SAT_PolyCylinder (CollisionDetect.cpp:1652-1709) after
+// SAT, lane 0; swapped when the cylinder is the pair's first body.
+__device__ void poly_cyl_contacts(const Hull& A, const Cyl& B, const Check& mp, const Params& P, int ip, int ic, bool swapped, CEP* lds,
+                                  int slot, Manifolds& M, int cap) {
 	V3 n = mp.n;
 	V3 amax = ldv(A.wp, mp.a_pid);
 	V3 u, w;
@@ -815,17 +780,12 @@ __device__ void poly_cyl(const Bodies& b, const Shapes& sh, const Params& P, int
 		nr = 1;
 	} else res = clip_areas(lds, na, sa, nb, sb, &nr);
 	emit_manifold(res, nr, n, u, w, amax, mp.pen, pair_cid(ip, ic), swapped, P, slot, M, cap);
-	write_mat(b, ip, ic, M, slot, cap);
 }
 
-// detectCylinderCylinder (CollisionDetect.cpp:958-1137).
-__device__ void cyl_cyl(const Bodies& b, const Shapes& sh, const Params& P, int ia, int ib, int lane, CEP* lds, int slot, Manifolds& M,
-                        int cap) {
-	Cyl A = make_cyl(b, sh, ia), B = make_cyl(b, sh, ib);
-	int bi;
-	if (!wave_sat(35 + 24 * 24, lane, [&](int t, bool* v) { return eval_axis_cc(A, B, t, v); }, &bi) || lane != 0) return;
-	bool valid;
-	Check mp = eval_axis_cc(A, B, bi, &valid);
+// Contact generation of detectCylinderCylinder (CollisionDetect.cpp:1056-1137)
+// after SAT, lane 0.
+__device__ void cyl_cyl_contacts(const Cyl& A, const Cyl& B, const Check& mp, const Params& P, int ia, int ib, CEP* lds, int slot,
+                                 Manifolds& M, int cap) {
 	V3 n = mp.n;
 	V3 u, w;
 	perp_basis(n, &u, &w);
@@ -858,7 +818,6 @@ __device__ void cyl_cyl(const Bodies& b, const Shapes& sh, const Params& P, int 
 	} else res = clip_areas(lds, na, sa, nb, sb, &nr);
 	V3 amax = dot(A.ax, n) > 0 ? disk_extrema(cyl_top_center(A), A.ax, A.r, n) : disk_extrema(cyl_bot_center(A), A.ax, A.r, n);
 	emit_manifold(res, nr, n, u, w, amax, mp.pen, pair_cid(ia, ib), false, P, slot, M, cap);
-	write_mat(b, ia, ib, M, slot, cap);
 }
 
 // detectSphereCylinder (CollisionDetect.cpp:1712-1792), lane 0.  The second
@@ -921,113 +880,310 @@ __device__ void sanitize_slot(Manifolds& M, int slot, int cap) {
 	M.count[slot] = o;
 }
 
-// Grid-stride over pair slots, one wave per pair.
-__global__ __launch_bounds__(64 * WAVES_PER_BLOCK) void k_narrowphase(Bodies b, Shapes sh, Params P, const uint64_t* pairs, int npairs,
-                                                                      Manifolds M, int cap) {
-	__shared__ CEP lds[WAVES_PER_BLOCK][4 * MAXP];
+// ------------------------------------------------------------- kernels
+// Pairs are sorted into per-class lists first, so that each class runs in a
+// kernel of its own shape: sphere-sphere and sphere-cylinder one lane per
+// pair, the SAT classes one wave per pair.
+enum { NPC_HH = 0, NPC_HS, NPC_SS, NPC_HC, NPC_SC, NPC_CC, NPC_N };
+
+__device__ __forceinline__ void wave_fence() {
+	__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+	__builtin_amdgcn_wave_barrier();
+}
+
+__global__ void k_np_zero(int* counts) {
+	if (threadIdx.x < NPC_N) counts[threadIdx.x] = 0;
+}
+
+// Initialises every slot to "no manifold" and appends the pairs that need a
+// test (PhysicsEngine.cpp:174: at least one dynamic body) to their class list;
+// one atomic per class per wave.  List order does not matter: every slot's
+// result depends on its pair only.
+__global__ void k_np_classify(Bodies b, Shapes sh, const uint64_t* pairs, int npairs, Manifolds M, int* lists, int* counts) {
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	int cls = -1;
+	if (i < npairs) {
+		M.count[i] = 0;
+		M.maxpen[i] = -1;
+		uint64_t key = pairs[i];
+		int ia = (int)(key >> 32), ib = (int)(key & 0xFFFFFFFFu);
+		if (!(b.movement[ia] != 0 && b.movement[ib] != 0)) {
+			int ta = sh.type[b.shape[ia]], tb = sh.type[b.shape[ib]];
+			int lo = ta < tb ? ta : tb, hi = ta < tb ? tb : ta;
+			if (lo == SHAPE_HULL) cls = hi == SHAPE_HULL ? NPC_HH : hi == SHAPE_SPHERE ? NPC_HS : NPC_HC;
+			else if (lo == SHAPE_SPHERE) cls = hi == SHAPE_SPHERE ? NPC_SS : NPC_SC;
+			else cls = NPC_CC;
+		}
+	}
 	const int lane = threadIdx.x & 63;
-	const int wib = threadIdx.x >> 6;
-	const int nwaves = gridDim.x * WAVES_PER_BLOCK;
-	for (int slot = blockIdx.x * WAVES_PER_BLOCK + wib; slot < npairs; slot += nwaves) {
+	const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+	for (int c = 0; c < NPC_N; c++) {
+		uint64_t m = __ballot(cls == c);
+		if (!m) continue;
+		int leader = __ffsll((long long)m) - 1;
+		int base = 0;
+		if (lane == leader) base = atomicAdd(&counts[c], __popcll(m));
+		base = __shfl(base, leader);
+		if (cls == c) lists[(size_t)c * npairs + base + __popcll(m & below)] = i;
+	}
+}
+
+#define CAND_K 4
+// Wave-wide SAT with compaction: each round prunes 64*CAND_K candidate axes
+// (cheap tests), packs the survivors into cand[] in axis order, then
+// evaluates them 64 at a time, so the extrema queries keep every lane busy.
+// A separating axis anywhere ends the test (returns false); otherwise *bi is
+// the first strict minimum (ties to the lower axis index), as in the
+// reference's sequential loops.
+template <class Prune, class Eval>
+__device__ bool wave_sat_compact(int naxes, int lane, int* cand, Prune prune, Eval eval, int* bi_out) {
+	double bp = __builtin_inf();
+	int bi = 0x7FFFFFFF;
+	const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+	for (int base = 0; base < naxes; base += 64 * CAND_K) {
+		int nc = 0;
+		for (int j = 0; j < CAND_K; j++) {
+			int t = base + j * 64 + lane;
+			bool v = t < naxes && prune(t);
+			uint64_t m = __ballot(v);
+			if (v) cand[nc + __popcll(m & below)] = t;
+			nc += __popcll(m);
+		}
+		wave_fence();
+		for (int c0 = 0; c0 < nc; c0 += 64) {
+			int idx = c0 + lane;
+			int sep = 0;
+			if (idx < nc) {
+				int t = cand[idx];
+				Check x = eval(t);
+				if (x.pen < 0) sep = 1;
+				if (x.pen < bp) { bp = x.pen; bi = t; }
+			}
+			if (__ballot(sep) != 0) return false;
+		}
+		wave_fence();
+	}
+	for (int off = 32; off >= 1; off >>= 1) {
+		double obp = __shfl_xor(bp, off);
+		int obi = __shfl_xor(bi, off);
+		if (obp < bp || (obp == bp && obi < bi)) { bp = obp; bi = obi; }
+	}
+	*bi_out = bi;
+	return bp > 0 && bi != 0x7FFFFFFF;  // PhysicsEngine.cpp:213: keep only max_pen_depth > 0
+}
+
+// Copies a hull's world points and normals into the wave's LDS slice when they
+// fit, so the extrema queries read LDS.
+#define ST_PTS 32
+#define ST_NRM 64
+__device__ void stage_hull(Hull& h, double* spts, double* snrm, int lane) {
+	if (h.np > ST_PTS || h.nf > ST_NRM) return;
+	for (int k = lane; k < 3 * h.np; k += 64) spts[k] = h.wp[k];
+	for (int k = lane; k < 3 * h.nf; k += 64) snrm[k] = h.wn[k];
+	wave_fence();
+	h.wp = spts;
+	h.wn = snrm;
+}
+
+// Arc-pair pruning of SAT_PolyPoly (CollisionDetect.cpp:750-770).
+__device__ __forceinline__ bool hh_prune(const Hull& A, const Hull& B, int t) {
+	if (t < A.nf) return !A.red[t];
+	t -= A.nf;
+	if (t < B.nf) return !B.red[t];
+	t -= B.nf;
+	int i1 = t / B.na, i2 = t - i1 * B.na;
+	V3 n;
+	return arc_axis(ldv(A.wn, A.arcs[2 * i1]), ldv(A.wn, A.arcs[2 * i1 + 1]), -ldv(B.wn, B.arcs[2 * i2]), -ldv(B.wn, B.arcs[2 * i2 + 1]), &n);
+}
+
+struct WaveLds {
+	CEP cep[4 * MAXP];
+	int cand[64 * CAND_K];
+	double pa[3 * ST_PTS], pb[3 * ST_PTS], na[3 * ST_NRM], nb[3 * ST_NRM];
+};
+
+#define NP_WAVES 4
+// Hull-hull: SAT_PolyPoly (CollisionDetect.cpp:717-835).
+__global__ __launch_bounds__(64 * NP_WAVES) void k_np_hull_hull(Bodies b, Shapes sh, Params P, const uint64_t* pairs, const int* list,
+                                                                const int* counts, Manifolds M, int cap) {
+	__shared__ WaveLds lds[NP_WAVES];
+	const int lane = threadIdx.x & 63, wib = threadIdx.x >> 6;
+	WaveLds& L = lds[wib];
+	const int cnt = counts[NPC_HH];
+	for (int w = blockIdx.x * NP_WAVES + wib; w < cnt; w += gridDim.x * NP_WAVES) {
+		int slot = list[w];
 		uint64_t key = pairs[slot];
 		int ia = (int)(key >> 32), ib = (int)(key & 0xFFFFFFFFu);
-		if (lane == 0) {
-			M.count[slot] = 0;
-			M.maxpen[slot] = -1;
-		}
-		// PhysicsEngine.cpp:174: at least one dynamic body
-		if (b.movement[ia] != 0 && b.movement[ib] != 0) continue;
-		int ta = sh.type[b.shape[ia]], tb = sh.type[b.shape[ib]];
-		if (ta == SHAPE_HULL && tb == SHAPE_SPHERE) {
-			poly_sphere(b, sh, ia, ib, false, lane, slot, M, cap);
-			if (lane == 0 && (P.defect_fixes & 1)) sanitize_slot(M, slot, cap);
-			continue;
-		}
-		if (ta == SHAPE_SPHERE && tb == SHAPE_HULL) {
-			poly_sphere(b, sh, ib, ia, true, lane, slot, M, cap);
-			if (lane == 0 && (P.defect_fixes & 1)) sanitize_slot(M, slot, cap);
-			continue;
-		}
-		if (ta == SHAPE_SPHERE && tb == SHAPE_SPHERE) {
-			if (lane == 0) sphere_sphere(b, sh, ia, ib, slot, M, cap);
-			if (lane == 0 && (P.defect_fixes & 1)) sanitize_slot(M, slot, cap);
-			continue;
-		}
-		if (ta == SHAPE_HULL && tb == SHAPE_CYLINDER) {
-			poly_cyl(b, sh, P, ia, ib, false, lane, lds[wib], slot, M, cap);
-			if (lane == 0 && (P.defect_fixes & 1)) sanitize_slot(M, slot, cap);
-			continue;
-		}
-		if (ta == SHAPE_CYLINDER && tb == SHAPE_HULL) {
-			poly_cyl(b, sh, P, ib, ia, true, lane, lds[wib], slot, M, cap);
-			if (lane == 0 && (P.defect_fixes & 1)) sanitize_slot(M, slot, cap);
-			continue;
-		}
-		if (ta == SHAPE_CYLINDER && tb == SHAPE_CYLINDER) {
-			cyl_cyl(b, sh, P, ia, ib, lane, lds[wib], slot, M, cap);
-			if (lane == 0 && (P.defect_fixes & 1)) sanitize_slot(M, slot, cap);
-			continue;
-		}
-		if (ta == SHAPE_SPHERE && tb == SHAPE_CYLINDER) {
-			if (lane == 0) sphere_cyl(b, sh, ia, ib, false, slot, M, cap);
-			if (lane == 0 && (P.defect_fixes & 1)) sanitize_slot(M, slot, cap);
-			continue;
-		}
-		if (ta == SHAPE_CYLINDER && tb == SHAPE_SPHERE) {
-			if (lane == 0) sphere_cyl(b, sh, ib, ia, true, slot, M, cap);
-			if (lane == 0 && (P.defect_fixes & 1)) sanitize_slot(M, slot, cap);
-			continue;
-		}
-		if (ta != SHAPE_HULL || tb != SHAPE_HULL) continue;
 		Hull A = make_hull(b, sh, ia), B = make_hull(b, sh, ib);
-		// Stages in the reference's axis order (A faces, B faces, then arc pairs in
-		// chunks of one axis per lane); a separating axis anywhere means "no
-		// contact", so the wave stops at the first chunk that finds one.
+		stage_hull(A, L.pa, L.na, lane);
+		stage_hull(B, L.pb, L.nb, lane);
 		int naxes = A.nf + B.nf + A.na * B.na;
-		double bp = __builtin_inf();
-		int bi = 0x7FFFFFFF;
-		int sep = 0;
-		for (int base = 0; base < naxes; base += 64) {
-			int t = base + lane;
-			if (t < naxes) {
-				bool valid;
-				Check x = eval_axis(A, B, t, &valid);
-				if (valid) {
-					if (x.pen < 0) sep = 1;
-					if (x.pen < bp) { bp = x.pen; bi = t; }
-				}
-			}
-			if (__ballot(sep) != 0) break;
-		}
-		if (__ballot(sep) != 0) continue;
-		for (int off = 32; off >= 1; off >>= 1) {
-			double obp = __shfl_xor(bp, off);
-			int obi = __shfl_xor(bi, off);
-			if (obp < bp || (obp == bp && obi < bi)) { bp = obp; bi = obi; }
-		}
-		if (!(bp > 0) || bi == 0x7FFFFFFF) continue;  // PhysicsEngine.cpp:213: keep only max_pen_depth > 0
-		if (lane == 0) {
+		int bi;
+		bool hit = wave_sat_compact(
+		    naxes, lane, L.cand, [&](int t) { return hh_prune(A, B, t); },
+		    [&](int t) {
+			    bool v;
+			    return eval_axis(A, B, t, &v);
+		    },
+		    &bi);
+		if (hit && lane == 0) {
 			bool valid;
 			Check mp = eval_axis(A, B, bi, &valid);
-			uint64_t ka = (uint64_t)(ia + 1) * 16, kb = (uint64_t)(ib + 1) * 16;
-			uint64_t cid = (0x00000000FFFFFFFFull & ka) | (0xFFFFFFFF00000000ull & (kb << 32));
-			// material mixing (CollisionDetect.cpp:820-824)
-			double ra = b.mat[ia], rb = b.mat[ib];
-			M.mat[slot] = (ra < rb) ? rb : ra;
-			M.mat[cap + slot] = (b.mat[b.n + ia] + b.mat[b.n + ib]) / 2.0;
-			M.mat[2 * cap + slot] = (b.mat[2 * b.n + ia] + b.mat[2 * b.n + ib]) / 2.0;
-			hull_contacts(A, B, mp, P, cid, lds[wib], slot, M, cap);
+			hull_contacts(A, B, mp, P, pair_cid(ia, ib), L.cep, slot, M, cap);
+			write_mat(b, ia, ib, M, slot, cap);
 			if (P.defect_fixes & 1) sanitize_slot(M, slot, cap);
 		}
+		wave_fence();
+	}
+}
+
+// Poly-cylinder pruning (degenerate edges, arc tests) for SAT_PolyCylinder.
+__device__ __forceinline__ bool hc_prune(const Hull& A, const Cyl& B, int t) {
+	if (t < A.nf) return !A.red[t];
+	t -= A.nf;
+	if (t == 0) return true;
+	t -= 1;
+	if (t < A.ne) {
+		V3 ed = ldv(A.wp, A.edges[2 * t + 1]) - ldv(A.wp, A.edges[2 * t]);
+		return !(mag(cross(ed, B.ax)) < 0.00000000001);
+	}
+	t -= A.ne;
+	if (t < A.np) return true;
+	t -= A.np;
+	int i1 = t / 24, k = t - i1 * 24;
+	V3 n;
+	return arc_axis(ldv(A.wn, A.arcs[2 * i1]), ldv(A.wn, A.arcs[2 * i1 + 1]), -cyl_gv(B, k & 1), -cyl_gv(B, k >> 1), &n);
+}
+
+__global__ __launch_bounds__(64 * NP_WAVES) void k_np_hull_cyl(Bodies b, Shapes sh, Params P, const uint64_t* pairs, const int* list,
+                                                               const int* counts, Manifolds M, int cap) {
+	__shared__ WaveLds lds[NP_WAVES];
+	const int lane = threadIdx.x & 63, wib = threadIdx.x >> 6;
+	WaveLds& L = lds[wib];
+	const int cnt = counts[NPC_HC];
+	for (int w = blockIdx.x * NP_WAVES + wib; w < cnt; w += gridDim.x * NP_WAVES) {
+		int slot = list[w];
+		uint64_t key = pairs[slot];
+		int ia = (int)(key >> 32), ib = (int)(key & 0xFFFFFFFFu);
+		bool swapped = sh.type[b.shape[ia]] == SHAPE_CYLINDER;  // detectCollision's reversed form (CollisionDetect.cpp:122-130)
+		int ip = swapped ? ib : ia, ic = swapped ? ia : ib;
+		Hull A = make_hull(b, sh, ip);
+		Cyl B = make_cyl(b, sh, ic);
+		stage_hull(A, L.pa, L.na, lane);
+		int naxes = A.nf + 1 + A.ne + A.np + A.na * 24;
+		int bi;
+		bool hit = wave_sat_compact(
+		    naxes, lane, L.cand, [&](int t) { return hc_prune(A, B, t); },
+		    [&](int t) {
+			    bool v;
+			    return eval_axis_pc(A, B, t, &v);
+		    },
+		    &bi);
+		if (hit && lane == 0) {
+			bool valid;
+			Check mp = eval_axis_pc(A, B, bi, &valid);
+			poly_cyl_contacts(A, B, mp, P, ip, ic, swapped, L.cep, slot, M, cap);
+			write_mat(b, ip, ic, M, slot, cap);
+			if (P.defect_fixes & 1) sanitize_slot(M, slot, cap);
+		}
+		wave_fence();
+	}
+}
+
+__device__ __forceinline__ bool cc_prune(const Cyl& A, const Cyl& B, int t) {
+	if (t < 2) return true;
+	t -= 2;
+	if (t < 32) {
+		int g = t >> 3, i = t & 7;
+		const double* poly = g == 0 ? A.top : g == 1 ? A.bot : g == 2 ? B.top : B.bot;
+		V3 bax = g < 2 ? B.ax : A.ax;
+		V3 ed = ldv(poly, (i + 1) % 8) - ldv(poly, i);
+		return !(mag(cross(ed, bax)) < 0.00000000001);
+	}
+	t -= 32;
+	if (t == 0) return mag2(cross(A.ax, B.ax)) > 0.00000001;
+	t -= 1;
+	int k1 = t / 24, k2 = t - k1 * 24;
+	V3 n;
+	return arc_axis(cyl_gv(A, k1 & 1), cyl_gv(A, k1 >> 1), -cyl_gv(B, k2 & 1), -cyl_gv(B, k2 >> 1), &n);
+}
+
+__global__ __launch_bounds__(64 * NP_WAVES) void k_np_cyl_cyl(Bodies b, Shapes sh, Params P, const uint64_t* pairs, const int* list,
+                                                              const int* counts, Manifolds M, int cap) {
+	__shared__ WaveLds lds[NP_WAVES];
+	const int lane = threadIdx.x & 63, wib = threadIdx.x >> 6;
+	WaveLds& L = lds[wib];
+	const int cnt = counts[NPC_CC];
+	for (int w = blockIdx.x * NP_WAVES + wib; w < cnt; w += gridDim.x * NP_WAVES) {
+		int slot = list[w];
+		uint64_t key = pairs[slot];
+		int ia = (int)(key >> 32), ib = (int)(key & 0xFFFFFFFFu);
+		Cyl A = make_cyl(b, sh, ia), B = make_cyl(b, sh, ib);
+		int bi;
+		bool hit = wave_sat_compact(
+		    35 + 24 * 24, lane, L.cand, [&](int t) { return cc_prune(A, B, t); },
+		    [&](int t) {
+			    bool v;
+			    return eval_axis_cc(A, B, t, &v);
+		    },
+		    &bi);
+		if (hit && lane == 0) {
+			bool valid;
+			Check mp = eval_axis_cc(A, B, bi, &valid);
+			cyl_cyl_contacts(A, B, mp, P, ia, ib, L.cep, slot, M, cap);
+			write_mat(b, ia, ib, M, slot, cap);
+			if (P.defect_fixes & 1) sanitize_slot(M, slot, cap);
+		}
+		wave_fence();
+	}
+}
+
+__global__ __launch_bounds__(64 * NP_WAVES) void k_np_hull_sphere(Bodies b, Shapes sh, Params P, const uint64_t* pairs, const int* list,
+                                                                  const int* counts, Manifolds M, int cap) {
+	const int lane = threadIdx.x & 63, wib = threadIdx.x >> 6;
+	const int cnt = counts[NPC_HS];
+	for (int w = blockIdx.x * NP_WAVES + wib; w < cnt; w += gridDim.x * NP_WAVES) {
+		int slot = list[w];
+		uint64_t key = pairs[slot];
+		int ia = (int)(key >> 32), ib = (int)(key & 0xFFFFFFFFu);
+		bool swapped = sh.type[b.shape[ia]] == SHAPE_SPHERE;
+		poly_sphere(b, sh, swapped ? ib : ia, swapped ? ia : ib, swapped, lane, slot, M, cap);
+		if (lane == 0 && (P.defect_fixes & 1)) sanitize_slot(M, slot, cap);
+	}
+}
+
+// Sphere-sphere and sphere-cylinder: one lane per pair.
+__global__ void k_np_sphere_pairs(Bodies b, Shapes sh, Params P, const uint64_t* pairs, const int* list_ss, const int* list_sc,
+                                  const int* counts, Manifolds M, int cap) {
+	const int nss = counts[NPC_SS], nsc = counts[NPC_SC];
+	const int stride = gridDim.x * blockDim.x;
+	for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < nss + nsc; w += stride) {
+		int slot = w < nss ? list_ss[w] : list_sc[w - nss];
+		uint64_t key = pairs[slot];
+		int ia = (int)(key >> 32), ib = (int)(key & 0xFFFFFFFFu);
+		if (w < nss) sphere_sphere(b, sh, ia, ib, slot, M, cap);
+		else if (sh.type[b.shape[ia]] == SHAPE_SPHERE) sphere_cyl(b, sh, ia, ib, false, slot, M, cap);
+		else sphere_cyl(b, sh, ib, ia, true, slot, M, cap);
+		if (P.defect_fixes & 1) sanitize_slot(M, slot, cap);
 	}
 }
 
 void launch_narrowphase(const Bodies& b, const Shapes& sh, const Params& p, const uint64_t* pairs, int npairs, const Manifolds& m,
-                        int cap, hipStream_t s) {
+                        int cap, int* lists, int* counts, hipStream_t s) {
 	if (npairs <= 0) return;
-	int blocks = (npairs + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
-	if (blocks > 8192) blocks = 8192;
-	hipLaunchKernelGGL(k_narrowphase, dim3(blocks), dim3(64 * WAVES_PER_BLOCK), 0, s, b, sh, p, pairs, npairs, m, cap);
+	hipLaunchKernelGGL(k_np_zero, dim3(1), dim3(64), 0, s, counts);
+	hipLaunchKernelGGL(k_np_classify, dim3((npairs + 255) / 256), dim3(256), 0, s, b, sh, pairs, npairs, m, lists, counts);
+	int wblocks = (npairs + NP_WAVES - 1) / NP_WAVES;
+	if (wblocks > 4096) wblocks = 4096;
+	const int* L = lists;
+	size_t n = (size_t)npairs;
+	hipLaunchKernelGGL(k_np_hull_hull, dim3(wblocks), dim3(64 * NP_WAVES), 0, s, b, sh, p, pairs, L + NPC_HH * n, counts, m, cap);
+	hipLaunchKernelGGL(k_np_hull_cyl, dim3(wblocks), dim3(64 * NP_WAVES), 0, s, b, sh, p, pairs, L + NPC_HC * n, counts, m, cap);
+	hipLaunchKernelGGL(k_np_cyl_cyl, dim3(wblocks), dim3(64 * NP_WAVES), 0, s, b, sh, p, pairs, L + NPC_CC * n, counts, m, cap);
+	hipLaunchKernelGGL(k_np_hull_sphere, dim3(wblocks), dim3(64 * NP_WAVES), 0, s, b, sh, p, pairs, L + NPC_HS * n, counts, m, cap);
+	int tblocks = (npairs + 255) / 256;
+	if (tblocks > 4096) tblocks = 4096;
+	hipLaunchKernelGGL(k_np_sphere_pairs, dim3(tblocks), dim3(256), 0, s, b, sh, p, pairs, L + NPC_SS * n, L + NPC_SC * n, counts, m, cap);
 }
 
 }  // namespace cpx

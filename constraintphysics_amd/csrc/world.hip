@@ -146,7 +146,7 @@ struct cp_world {
 	int mcap = 0;
 	Manifolds man{};
 	std::vector<void*> man_allocs;
-	DArr<int> pt_cnt, pt_off;
+	DArr<int> pt_cnt, pt_off, np_lists, np_counts;
 	int npcap = 0;
 	NewPts np{};
 	std::vector<void*> np_allocs;
@@ -242,7 +242,7 @@ void cp_world::free_all() {
 	DArr<int>* ints[] = {&s_type, &s_pt_off, &s_face_off, &s_arc_off, &s_edge_off, &s_fv_off, &s_fv, &s_ncalc, &s_ndir, &s_sid, &s_red,
 	                     &s_minpid, &s_maxpid, &s_refidx, &s_arcs, &s_edges, &s_vaf_off, &s_vaf, &s_vae_off, &s_vae, &movement, &shape,
 	                     &scene, &wpt_off, &wgv_off, &color_mask, &scene_body_off, &scene_of_body, &bp_vals, &bp_vals2, &bp_counts,
-	                     &bp_offs, &pt_cnt, &pt_off, &keep, &keep_scan, &rank, &e_head, &e_idx, &e_first, &e_cnt, &e_live, &e_color,
+	                     &bp_offs, &pt_cnt, &pt_off, &np_lists, &np_counts, &keep, &keep_scan, &rank, &e_head, &e_idx, &e_first, &e_cnt, &e_live, &e_color,
 	                     &e_order, &color_off, &ncolors, &edge_range, &applied, &isl_rows, &isl_off};
 	for (auto* a : ints) a->free_();
 	for (auto& a : cls_dev) a.free_();
@@ -420,7 +420,9 @@ int cp_world::broadphase() {
 
 int cp_world::narrowphase(int npairs) {
 	ensure_manifolds(npairs);
-	launch_narrowphase(bodies(), shapes(), params(), pairs.p, npairs, man, mcap, stream);
+	np_lists.reserve(6 * (size_t)std::max(npairs, 1));
+	np_counts.reserve(8);
+	launch_narrowphase(bodies(), shapes(), params(), pairs.p, npairs, man, mcap, np_lists.p, np_counts.p, stream);
 	return npairs;
 }
 
