@@ -17,6 +17,7 @@
 #include <limits>
 #include <map>
 #include <set>
+#include <climits>
 #include <vector>
 
 namespace {
@@ -1023,10 +1024,36 @@ struct Contact {
 
 static double posCorrectCoeff(double s, double dt) { return std::min<double>(s * dt, 1.0 / dt); }  // PersistentConstraint.cpp:199
 
+struct Joint {
+	int type;       // 3 ball-socket, 5 hinge
+	int a, b;       // body indices (b1, b2 as created)
+	V3 tp1, tp2;    // tracked points in the body frames (RigidBody.cpp:134-144)
+	V3 ax1, ax2;    // hinge axes in the body frames
+	double ps, rs;  // position / rotation correction strengths
+	double imp[5] = {}, pimp[5] = {}, himp[5] = {};
+	V3 u, w;        // hinge basis of the current solver object
+	int sys = -1;   // holonomic system, -1 none
+	NM<5, 6> ja, jb;
+	NM<6, 5> ita, itb;
+	NM<5, 5> inv;
+	double tgt[5] = {}, ptgt[5] = {};
+};
+
+struct HSys {
+	std::vector<int> og, order;  // joint indices
+	std::vector<int> table, vloc;
+	int degree = 0, cap = 0;
+	std::vector<double> buf, dbuf;
+	bool pending = true;  // rows and factor to build at the next step start
+};
+
 struct World {
 	orc_config cfg;
 	double cutoff_vel;
 	std::vector<Body> bodies;
+	std::vector<Joint> joints;
+	std::vector<HSys> systems;
+	std::set<std::pair<int, int>> nocol;  // disallowCollision pairs (lo, hi body index)
 	std::vector<Contact> cache;  // grouped by edge (b1 < b2 by id), edge order = insertion
 	uint64_t seq = 0;
 	std::vector<int64_t> last_rows, last_offs;
@@ -1118,18 +1145,398 @@ static void buildRows(World& W, Contact& c, double sub_dt) {
 }
 
 // ---------------------------------------------------------------- PGS (ConstraintSolver.cpp:80-213)
+// ---------------------------------------------------------------- joints
+// BallSocket / Hinge (PersistentConstraint.cpp:225-231, :274-285) with their
+// solver constraints (ConstraintSolver.cpp:382-405, :468-550).  Degree n = 3
+// (ball-socket) or 5 (hinge); rows use the first n of the fixed-size arrays.
+
+// rowMajorOrderInverse<3> (NMat.cpp:24-36) and the generic Gauss-Jordan form
+// (NMat.h:121-147) for n = 4..6.
+static void inverseN(int n, const double* src, double* dst) {
+	if (n == 1) { dst[0] = 1.0 / src[0]; return; }
+	if (n == 3) {
+		double a = src[0], b = src[1], c = src[2], d = src[3], e = src[4], f = src[5], g = src[6], h = src[7], i = src[8];
+		double det = a * (e * i - f * h) - b * (d * i - f * g) + c * (d * h - e * g);
+		double di = 1.0 / det;
+		dst[0] = (e * i - f * h) * di; dst[1] = (c * h - b * i) * di; dst[2] = (b * f - c * e) * di;
+		dst[3] = (f * g - d * i) * di; dst[4] = (a * i - c * g) * di; dst[5] = (c * d - a * f) * di;
+		dst[6] = (d * h - e * g) * di; dst[7] = (b * g - a * h) * di; dst[8] = (a * e - b * d) * di;
+		return;
+	}
+	double cp[36], t[36];
+	for (int k = 0; k < n * n; k++) { cp[k] = src[k]; t[k] = 0; }
+	for (int k = 0; k < n; k++) t[k * (n + 1)] = 1.0;
+	for (int piv = 0; piv < n; piv++)
+		for (int r = 0; r < n; r++) {
+			if (piv == r) continue;
+			double ratio = cp[n * r + piv] / cp[n * piv + piv];
+			for (int c = 0; c < n; c++) {
+				cp[n * r + c] -= ratio * cp[n * piv + c];
+				t[n * r + c] -= ratio * t[n * piv + c];
+			}
+		}
+	for (int piv = 0; piv < n; piv++)
+		for (int c = 0; c < n; c++) t[n * piv + c] /= cp[n * piv + piv];
+	for (int k = 0; k < n * n; k++) dst[k] = t[k];
+}
+
+// The solver constraint of a joint from the current body state.  source
+// impulse / basis: the previous object's (warm start with coefficient 1.0,
+// hinge angular impulses re-projected onto the new basis, ConstraintSolver.cpp:475-476).
+static void buildJoint(World& W, Joint& j, double sub_dt, bool reproject) {
+	const Body& A = W.bodies[j.a];
+	const Body& B = W.bodies[j.b];
+	V3 pa = A.com + A.q.rotate(j.tp1);  // getTrackedP
+	V3 pb = B.com + B.q.rotate(j.tp2);
+	double cfm = W.cfg.cfm;
+	int n = j.type;
+	double start[5];
+	for (int k = 0; k < n; k++) start[k] = j.imp[k] * 1.0;  // warm_start_coefficient * impulse
+	NM<6, 6> Ma = invMass6(A.invMass(), A.invTensor()), Mb = invMass6(B.invMass(), B.invTensor());
+	NM<5, 6> ja, jb;
+	double pcc = posCorrectCoeff(j.ps, sub_dt);
+	if (n == 3) {
+		V3 rA = pa - A.com, rB = pb - B.com;
+		NM<3, 3> sa = crossNM(rA), sb = crossNM(rB), nsa = neg(sa);
+		for (int r = 0; r < 3; r++) {
+			for (int c = 0; c < 3; c++) {
+				ja.v[r][c] = (r == c) ? 1.0 : 0.0;
+				jb.v[r][c] = -((r == c) ? 1.0 : 0.0);
+				ja.v[r][3 + c] = nsa.v[r][c];
+				jb.v[r][3 + c] = sb.v[r][c];
+			}
+		}
+		V3 err = pb - pa;
+		j.ptgt[0] = pcc * err.x; j.ptgt[1] = pcc * err.y; j.ptgt[2] = pcc * err.z;
+		for (int k = 0; k < 3; k++) j.imp[k] = start[k];
+	} else {
+		V3 axa = A.q.rotate(j.ax1), axb = B.q.rotate(j.ax2);
+		double rcc = posCorrectCoeff(j.rs, sub_dt);
+		V3 su = j.u, sw = j.w;
+		V3 rA = pa - A.com, rB = pa - B.com;  // both from the b1 point (ConstraintSolver.cpp:469)
+		V3 u, w;
+		axa.perpBasis(&u, &w);
+		V3 nn = axb;
+		for (int k = 0; k < 5; k++) j.imp[k] = start[k];
+		if (reproject) {
+			j.imp[3] = u.dot(su) * start[3] + u.dot(sw) * start[4];
+			j.imp[4] = w.dot(su) * start[3] + w.dot(sw) * start[4];
+		}
+		j.u = u;
+		j.w = w;
+		NM<3, 3> sa = crossNM(rA), sb = crossNM(rB), nsa = neg(sa);
+		NM<3, 3> us = crossNM(u), ws = crossNM(w), ns = crossNM(nn);
+		NM<1, 3> ud, wd, nd;
+		ud.v[0][0] = u.x; ud.v[0][1] = u.y; ud.v[0][2] = u.z;
+		wd.v[0][0] = w.x; wd.v[0][1] = w.y; wd.v[0][2] = w.z;
+		nd.v[0][0] = nn.x; nd.v[0][1] = nn.y; nd.v[0][2] = nn.z;
+		NM<1, 3> a3 = mul(nd, us), b3 = mul(ud, ns), a4 = mul(nd, ws), b4 = mul(wd, ns);
+		for (int r = 0; r < 3; r++)
+			for (int c = 0; c < 3; c++) {
+				ja.v[r][c] = (r == c) ? 1.0 : 0.0;
+				jb.v[r][c] = -((r == c) ? 1.0 : 0.0);
+				ja.v[r][3 + c] = nsa.v[r][c];
+				jb.v[r][3 + c] = sb.v[r][c];
+			}
+		for (int c = 0; c < 3; c++) {
+			ja.v[3][c] = 0; jb.v[3][c] = 0; ja.v[4][c] = 0; jb.v[4][c] = 0;
+			ja.v[3][3 + c] = a3.v[0][c]; jb.v[3][3 + c] = b3.v[0][c];
+			ja.v[4][3 + c] = a4.v[0][c]; jb.v[4][3 + c] = b4.v[0][c];
+		}
+		V3 pc = (pb - pa) * pcc;
+		j.ptgt[0] = pc.x; j.ptgt[1] = pc.y; j.ptgt[2] = pc.z;
+		j.ptgt[3] = u.dot(nn) * rcc;
+		j.ptgt[4] = w.dot(nn) * rcc;
+	}
+	// impulse_to_*_velocity = M^-1 J^T, impulse_to_value = J_a M_a^-1 J_a^T + J_b ...
+	for (int r = 0; r < 6; r++)
+		for (int c = 0; c < n; c++) {
+			double s1 = 0, s2 = 0;
+			for (int k = 0; k < 6; k++) { s1 += Ma.v[r][k] * ja.v[c][k]; s2 += Mb.v[r][k] * jb.v[c][k]; }
+			j.ita.v[r][c] = s1;
+			j.itb.v[r][c] = s2;
+		}
+	double itv[25];
+	for (int r = 0; r < n; r++)
+		for (int c = 0; c < n; c++) {
+			double s1 = 0, s2 = 0;
+			for (int k = 0; k < 6; k++) { s1 += ja.v[r][k] * j.ita.v[k][c]; s2 += jb.v[r][k] * j.itb.v[k][c]; }
+			itv[r * n + c] = s1 + s2;
+		}
+	for (int r = 0; r < n; r++) itv[r * n + r] *= (1 + cfm);  // applyCFM (ConstraintSolver.cpp:919-926)
+	double inv[25];
+	inverseN(n, itv, inv);
+	for (int r = 0; r < n; r++)
+		for (int c = 0; c < n; c++) j.inv.v[r][c] = inv[r * n + c];
+	j.ja = ja;
+	j.jb = jb;
+	for (int k = 0; k < n; k++) { j.pimp[k] = 0; j.himp[k] = 0; }
+}
+
+// Degreed constraint value J_a va + J_b vb (ConstraintSolver.h:86-88).
+static void jointValue(const Joint& j, const double* va, const double* vb, double* out) {
+	for (int r = 0; r < j.type; r++) {
+		double x = 0, y = 0;
+		for (int k = 0; k < 6; k++) { x += va[k] * j.ja.v[r][k]; y += vb[k] * j.jb.v[r][k]; }
+		out[r] = x + y;
+	}
+}
+// computeAndApplyVelocityChange (ConstraintSolver.h:71-74).
+static void jointApply(const Joint& j, const double* imp, double* va, double* vb) {
+	double da[6], db[6];
+	for (int r = 0; r < 6; r++) {
+		da[r] = 0; db[r] = 0;
+		for (int k = 0; k < j.type; k++) { da[r] += imp[k] * j.ita.v[r][k]; db[r] += imp[k] * j.itb.v[r][k]; }
+	}
+	for (int r = 0; r < 6; r++) { va[r] += da[r]; vb[r] += db[r]; }
+}
+
+// ---------------------------------------------------------------- holonomic systems
+// HolonomicSystem (HolonomicBlockSolver.cpp): min-fill elimination order,
+// block-lower-triangular layout, block LDL factor and the solve d = A^-1 (t - c).
+static const int HS_EMPTY = -1;
+
+static bool jointsShareBody(const World& W, int c1, int c2) {
+	const Joint& a = W.joints[c1];
+	const Joint& b = W.joints[c2];
+	return a.a == b.a || a.a == b.b || a.b == b.a || a.b == b.b;
+}
+
+// HolonomicSystem::HolonomicSystem (HolonomicBlockSolver.cpp:11-132).
+static void hsysLayout(const World& W, HSys& s) {
+	int n = (int)s.og.size();
+	std::vector<char> edge(n * n, 0), empty(n * n, 0);
+	std::vector<int> rem(n);
+	for (int i = 0; i < n; i++) {
+		rem[i] = i;
+		for (int k = 0; k < n; k++) {
+			if (jointsShareBody(W, s.og[i], s.og[k])) {
+				empty[i * n + k] = empty[k * n + i] = 0;
+				edge[i + k * n] = edge[k + i * n] = 1;
+			} else {
+				empty[i * n + k] = empty[k * n + i] = 1;
+			}
+		}
+	}
+	s.order.clear();
+	while (!rem.empty()) {
+		int best = -1, best_cnt = INT_MAX;
+		std::vector<int> best_nb;
+		for (int i = 0; i < (int)rem.size(); i++) {
+			std::vector<int> nb;
+			for (int k = 0; k < n; k++)
+				if (k != rem[i] && edge[rem[i] + k * n]) nb.push_back(k);
+			int cnt = 0;
+			for (size_t p = 0; p < nb.size(); p++)
+				for (size_t q = p + 1; q < nb.size(); q++)
+					if (!edge[nb[p] + nb[q] * n]) cnt++;
+			if (cnt < best_cnt) {
+				best_cnt = cnt;
+				best = i;
+				best_nb = nb;
+				if (cnt == 0) break;
+			}
+		}
+		int el = rem[best];
+		s.order.push_back(el);
+		rem.erase(rem.begin() + best);
+		for (int k = 0; k < n; k++) edge[el + k * n] = edge[k + el * n] = 0;
+		for (size_t p = 0; p < best_nb.size(); p++)
+			for (size_t q = p + 1; q < best_nb.size(); q++) {
+				int a = best_nb[p], b = best_nb[q];
+				empty[a * n + b] = empty[b * n + a] = 0;
+				edge[a + b * n] = edge[b + a * n] = 1;
+			}
+	}
+	// order holds og positions; convert to joint indices and build the table
+	std::vector<int> pos = s.order;
+	for (int& x : s.order) x = s.og[x];
+	s.table.clear();
+	s.vloc.clear();
+	s.degree = 0;
+	s.cap = 0;
+	for (int col = 0; col < n; col++) {
+		s.vloc.push_back(s.degree);
+		s.degree += W.joints[s.order[col]].type;
+		for (int row = col; row < n; row++) {
+			if (empty[pos[row] * n + pos[col]]) s.table.push_back(HS_EMPTY);
+			else {
+				s.table.push_back(s.cap);
+				s.cap += W.joints[s.order[row]].type * W.joints[s.order[col]].type;
+			}
+		}
+	}
+	s.buf.assign(s.cap, 0.0);
+	s.dbuf.assign(6 * s.degree, 0.0);
+}
+
+static int hsLoc(const HSys& s, int row, int col) {  // getBlockBufferLocation (:603-614)
+	int n = (int)s.order.size();
+	return s.table[row - col + ((2 * n + 1) * col - col * col) / 2];
+}
+static int hsFirstBelow(const HSys& s, int col) {  // :616-622
+	for (int row = col + 1; row < (int)s.order.size(); row++) {
+		int l = hsLoc(s, row, col);
+		if (l != HS_EMPTY) return l;
+	}
+	return HS_EMPTY;
+}
+
+// computeBlockInitialValue (:799-825, :714-744) for joints c1 (row) and c2 (col).
+static void hsBlockInit(const World& W, double cfm, int c1, int c2, double* t) {
+	const Joint& A = W.joints[c1];
+	const Joint& B = W.joints[c2];
+	int n1 = A.type, n2 = B.type;
+	if (c1 == c2) {
+		for (int r = 0; r < n1; r++)
+			for (int c = 0; c < n1; c++) {
+				double s1 = 0, s2 = 0;
+				for (int k = 0; k < 6; k++) { s1 += A.ja.v[r][k] * A.ita.v[k][c]; s2 += A.jb.v[r][k] * A.itb.v[k][c]; }
+				t[r * n1 + c] = s1 + s2;
+			}
+		for (int i = 0; i < n1; i++) t[i + n1 * i] *= (1 + cfm);
+		return;
+	}
+	const NM<5, 6>* jac = nullptr;
+	const NM<6, 5>* it = nullptr;
+	if (A.a == B.a) { jac = &A.ja; it = &B.ita; }
+	else if (A.a == B.b) { jac = &A.ja; it = &B.itb; }
+	else if (A.b == B.a) { jac = &A.jb; it = &B.ita; }
+	else if (A.b == B.b) { jac = &A.jb; it = &B.itb; }
+	if (!jac) { for (int i = 0; i < n1 * n2; i++) t[i] = 0; return; }
+	for (int i = 0; i < n1 * n2; i++) {  // matMult (:701-712)
+		int row = i / n2, col = i % n2;
+		t[i] = 0;
+		for (int k = 0; k < 6; k++) t[i] += jac->v[row][k] * it->v[k][col];
+	}
+}
+
+static double dotp(int n, const double* a, const double* b) {  // dotProd (:760-768)
+	double o = 0;
+	for (int i = 0; i < n; i++) o += a[i] * b[i];
+	return o;
+}
+
+// HolonomicSystem::computeInverse (:459-543): block LDL in place.
+static void hsFactor(const World& W, HSys& s, double cfm) {
+	int nc = (int)s.order.size();
+	for (int col = 0; col < nc; col++)
+		for (int row = col; row < nc; row++) {
+			int l = hsLoc(s, row, col);
+			if (l == HS_EMPTY) continue;
+			hsBlockInit(W, cfm, s.order[row], s.order[col], s.buf.data() + l);
+		}
+	for (int col = 0; col < nc; col++) {
+		int dl = hsLoc(s, col, col);
+		double* diag = s.buf.data() + dl;
+		int bw = W.joints[s.order[col]].type;
+		double tmp[36];
+		inverseN(bw, diag, tmp);
+		for (int k = 0; k < bw * bw; k++) diag[k] = tmp[k];
+		if (col + 1 == nc) continue;
+		int first = hsFirstBelow(s, col);
+		for (int row = col + 1; row < nc; row++) {
+			int l = hsLoc(s, row, col);
+			if (l == HS_EMPTY) continue;
+			double* tgt = s.dbuf.data() + (l - first);
+			int bh = W.joints[s.order[row]].type;
+			const double* src = s.buf.data() + l;
+			for (int r = 0; r < bh; r++)
+				for (int c = 0; c < bw; c++) tgt[c + bw * r] = dotp(bw, src + bw * r, diag + bw * c);
+		}
+		for (int ldc = col + 1; ldc < nc; ldc++)
+			for (int ldr = ldc; ldr < nc; ldr++) {
+				int l = hsLoc(s, ldr, col);
+				if (l == HS_EMPTY || hsLoc(s, ldc, col) == HS_EMPTY) continue;
+				const double* src = s.buf.data() + l;
+				const double* lsrc = s.dbuf.data() + (hsLoc(s, ldc, col) - first);
+				double* tgt = s.buf.data() + hsLoc(s, ldr, ldc);
+				int w2 = W.joints[s.order[ldc]].type, h2 = W.joints[s.order[ldr]].type;
+				for (int r = 0; r < h2; r++)
+					for (int c = 0; c < w2; c++) tgt[r * w2 + c] -= dotp(bw, src + r * bw, lsrc + c * bw);
+			}
+		if (first == HS_EMPTY) continue;
+		int total = hsLoc(s, col + 1, col + 1) - first;
+		for (int k = 0; k < total; k++) s.buf[first + k] = s.dbuf[k];
+	}
+}
+
+// multInverseWithVector (:223-296): v <- L^-T D^-1 L^-1 v.
+static void hsSolve(const World& W, const HSys& s, std::vector<double>& v) {
+	int nc = (int)s.order.size();
+	int deg = s.degree;
+	std::vector<double> eff(deg);
+	for (int col = 0; col + 1 < nc; col++) {
+		eff.assign(deg, 0.0);
+		for (int row = col + 1; row < nc; row++) {
+			int l = hsLoc(s, row, col);
+			if (l == HS_EMPTY) continue;
+			int bw = W.joints[s.order[col]].type, bh = W.joints[s.order[row]].type;
+			const double* m = s.buf.data() + l;
+			const double* x = v.data() + s.vloc[col];
+			for (int i = 0; i < bh; i++) eff[s.vloc[row] + i] = dotp(bw, m + i * bw, x);
+		}
+		for (int i = 0; i < deg; i++) v[i] -= eff[i];
+	}
+	std::vector<double> d(deg);
+	for (int col = 0; col < nc; col++) {
+		int bw = W.joints[s.order[col]].type;
+		const double* m = s.buf.data() + hsLoc(s, col, col);
+		const double* x = v.data() + s.vloc[col];
+		for (int i = 0; i < bw; i++) d[s.vloc[col] + i] = dotp(bw, m + i * bw, x);
+	}
+	v = d;
+	std::vector<double> lt(6);
+	for (int col = nc - 2; col >= 0; col--) {
+		lt.assign(6, 0.0);
+		int bw = W.joints[s.order[col]].type;
+		for (int row = col + 1; row < nc; row++) {
+			int l = hsLoc(s, row, col);
+			if (l == HS_EMPTY) continue;
+			int bh = W.joints[s.order[row]].type;
+			const double* m = s.buf.data() + l;
+			int si = s.vloc[row];
+			for (int i = 0; i < bw; i++) {
+				double dp = 0;
+				for (int k = 0; k < bh; k++) dp += v[si + k] * m[i + bw * k];
+				lt[i] += dp;
+			}
+		}
+		for (int i = 0; i < bw; i++) v[s.vloc[col] + i] -= lt[i];
+	}
+}
+
 struct DV {
 	double v[6] = {0, 0, 0, 0, 0, 0};
 	double p[6] = {0, 0, 0, 0, 0, 0};
+	double l[6] = {0, 0, 0, 0, 0, 0};  // linearization-error detection (HolonomicBlockSolver.cpp:197-221)
 };
 static void addv(double* acc, const double* d) { for (int i = 0; i < 6; i++) acc[i] += d[i]; }
 
-static void pgsIsland(World& W, const std::vector<std::pair<Contact*, int>>& rows) {
+// Joint rows are (nullptr, joint index); contact rows (contact, 0 normal / 1 friction).
+static void pgsIsland(World& W, const std::vector<std::pair<Contact*, int>>& rows, const std::vector<int>& systems, int hol_iters) {
 	std::map<int, DV> dv;
-	for (auto& r : rows) { dv[r.first->b1]; dv[r.first->b2]; }
+	for (auto& r : rows) {
+		if (r.first) { dv[r.first->b1]; dv[r.first->b2]; }
+		else { dv[W.joints[r.second].a]; dv[W.joints[r.second].b]; }
+	}
+	auto bodyVel = [&](int i, double* v) {
+		const Body& B = W.bodies[i];
+		V3 l = B.getVel(), a = B.getAngVel();
+		v[0] = l.x; v[1] = l.y; v[2] = l.z; v[3] = a.x; v[4] = a.y; v[5] = a.z;
+	};
 	// targets (ConstraintSolver.cpp:265-268 / ConstraintSolver.h:78)
 	for (auto& r : rows) {
 		Contact* c = r.first;
+		if (!c) {
+			Joint& j = W.joints[r.second];
+			double va[6], vb[6], val[5];
+			bodyVel(j.a, va);
+			bodyVel(j.b, vb);
+			jointValue(j, va, vb, val);
+			for (int k = 0; k < j.type; k++) j.tgt[k] = -val[k];
+			continue;
+		}
 		const Body& A = W.bodies[c->b1];
 		const Body& B = W.bodies[c->b2];
 		double va[6] = {A.getVel().x, A.getVel().y, A.getVel().z, A.getAngVel().x, A.getAngVel().y, A.getAngVel().z};
@@ -1149,6 +1556,11 @@ static void pgsIsland(World& W, const std::vector<std::pair<Contact*, int>>& row
 	// warm start
 	for (auto& r : rows) {
 		Contact* c = r.first;
+		if (!c) {
+			Joint& j = W.joints[r.second];
+			if (!isZero(j.imp, j.type)) jointApply(j, j.imp, dv[j.a].v, dv[j.b].v);
+			continue;
+		}
 		DV& a = dv[c->b1];
 		DV& b = dv[c->b2];
 		if (r.second == 0) {
@@ -1163,7 +1575,28 @@ static void pgsIsland(World& W, const std::vector<std::pair<Contact*, int>>& row
 			addv(a.v, da); addv(b.v, db);
 		}
 	}
+	// PGSConstraintStep<n> of a joint (equality constraint, no projection).
+	auto jstep = [&](Joint& j, bool pseudo) {
+		double* va = pseudo ? dv[j.a].p : dv[j.a].v;
+		double* vb = pseudo ? dv[j.b].p : dv[j.b].v;
+		double* acc = pseudo ? j.pimp : j.imp;
+		const double* tgt = pseudo ? j.ptgt : j.tgt;
+		int n = j.type;
+		double cur[5], delta[5], m[5], diff[5];
+		jointValue(j, va, vb, cur);
+		for (int k = 0; k < n; k++) delta[k] = tgt[k] - cur[k];
+		for (int i = 0; i < n; i++) {
+			m[i] = 0;
+			for (int k = 0; k < n; k++) m[i] += delta[k] * j.inv.v[i][k];
+		}
+		for (int k = 0; k < n; k++) diff[k] = (acc[k] + m[k]) - acc[k];
+		if (!isZero(diff, n)) {
+			jointApply(j, diff, va, vb);
+			for (int k = 0; k < n; k++) acc[k] += diff[k];
+		}
+	};
 	auto step = [&](Contact* c, int kind, bool pseudo) {
+		if (!c) { jstep(W.joints[kind], pseudo); return; }
 		DV& A = dv[c->b1];
 		DV& B = dv[c->b2];
 		double* va = pseudo ? A.p : A.v;
@@ -1219,7 +1652,55 @@ static void pgsIsland(World& W, const std::vector<std::pair<Contact*, int>>& row
 		for (auto& r : rows) step(r.first, r.second, false);
 	for (int it = 0; it < W.cfg.pos_iters; it++)
 		for (auto& r : rows)
-			if (r.second == 0) step(r.first, 0, true);
+			if (!r.first || r.second == 0) step(r.first, r.second, true);
+	// block solve of the island's holonomic systems (ConstraintSolver.cpp:163-196)
+	if (hol_iters > 0 && !systems.empty()) {
+		auto apply = [&](HSys& hs, bool pseudo, double thr) {  // computeAndApplyImpulses (:298-355)
+			std::vector<double> d(hs.degree);
+			for (size_t i = 0; i < hs.order.size(); i++) {
+				Joint& j = W.joints[hs.order[i]];
+				double cur[5];
+				jointValue(j, pseudo ? dv[j.a].p : dv[j.a].v, pseudo ? dv[j.b].p : dv[j.b].v, cur);
+				const double* tgt = pseudo ? j.ptgt : j.tgt;
+				for (int k = 0; k < j.type; k++) d[hs.vloc[i] + k] = tgt[k] - cur[k];
+			}
+			hsSolve(W, hs, d);
+			if (thr > 0) {
+				for (size_t i = 0; i < hs.order.size(); i++) {
+					Joint& j = W.joints[hs.order[i]];
+					const double* ch = d.data() + hs.vloc[i];
+					if (isZero(ch, j.type)) continue;
+					jointApply(j, ch, dv[j.a].l, dv[j.b].l);
+				}
+				for (size_t i = 0; i < hs.order.size(); i++) {
+					Joint& j = W.joints[hs.order[i]];
+					double r2 = 0;
+					for (int k = 3; k < 6; k++) r2 += dv[j.a].l[k] * dv[j.a].l[k] + dv[j.b].l[k] * dv[j.b].l[k];
+					if (r2 > thr * thr) return;  // discard the solution
+				}
+			}
+			for (size_t i = 0; i < hs.order.size(); i++) {
+				Joint& j = W.joints[hs.order[i]];
+				const double* ch = d.data() + hs.vloc[i];
+				if (isZero(ch, j.type)) continue;
+				jointApply(j, ch, pseudo ? dv[j.a].p : dv[j.a].v, pseudo ? dv[j.b].p : dv[j.b].v);
+				double* acc = pseudo ? j.pimp : j.himp;
+				for (int k = 0; k < j.type; k++) acc[k] += ch[k];
+			}
+		};
+		for (int it = 0; it < hol_iters; it++)
+			for (int si : systems) {
+				apply(W.systems[si], false, -1.0);
+				apply(W.systems[si], true, (double)15.0f);  // excessive_linear_error_torque_threshold (PhysicsEngine.h:217)
+			}
+		// addScaledHolonomicImpulseToNextWarmStart with factor 0 (PhysicsEngine.cpp:373)
+		for (auto& r : rows) {
+			if (r.first) continue;
+			Joint& j = W.joints[r.second];
+			if (j.sys < 0) continue;
+			for (int k = 0; k < j.type; k++) j.imp[k] += (double)0.0f * j.himp[k];
+		}
+	}
 	// applyVelocityChange (PhysicsEngine.cpp:1113-1136)
 	for (auto& kv : dv) {
 		Body& b = W.bodies[kv.first];
@@ -1374,37 +1855,70 @@ static void step(World& W, const int64_t* node_order, const int64_t* sched_rows 
 		const Body& A = W.bodies[pr.first];
 		const Body& B = W.bodies[pr.second];
 		if (!(A.dynamic() || B.dynamic())) continue;
+		if (W.nocol.count({std::min(pr.first, pr.second), std::max(pr.first, pr.second)})) continue;  // collisionAllowed (:570-572)
 		Manifold m = narrow(W, pr.first, pr.second);
 		for (const CP& p : m.pts) addContact(W, pr.first, pr.second, p, m.n);
 	}
 	sortCache(W);
 	W.last_mid = W.cache;
+	const int hol = W.cfg.hol_iters;
+	// holonomic systems whose rows and factor are not current (created, or
+	// loaded from a state): rows from the current state with the stored
+	// impulses as they are, then the factor (PhysicsEngine.cpp:1507-1522)
+	if (hol > 0)
+		for (HSys& hs : W.systems) {
+			if (!hs.pending) continue;
+			for (int ji : hs.og) buildJoint(W, W.joints[ji], dt / W.cfg.substeps, false);
+			hsFactor(W, hs, W.cfg.cfm);
+			hs.pending = false;
+		}
 	// island search (PhysicsEngine.cpp:1536-1613, :1263-1283)
-	std::vector<std::vector<std::pair<int, int>>> adj(N);  // (other, first cache index) per edge, sorted by other id
-	{
-		std::map<std::pair<int, int>, int> first;
-		for (size_t k = 0; k < W.cache.size(); k++) {
-			auto key = std::make_pair(W.cache[k].b1, W.cache[k].b2);
-			if (!first.count(key)) first[key] = (int)k;
-		}
-		for (auto& kv : first) {
-			adj[kv.first.first].push_back({kv.first.second, kv.second});
-			adj[kv.first.second].push_back({kv.first.first, kv.second});
-		}
-		for (auto& a : adj)
-			std::sort(a.begin(), a.end(), [&](const std::pair<int, int>& x, const std::pair<int, int>& y) {
-				return W.bodies[x.first].id < W.bodies[y.first].id;
-			});
+	// edges: body pairs with contacts or joints; a node's edges sorted by the
+	// other body's id (ConstraintGraphNode::insertNewEdge, PhysicsEngine.cpp:1984-1995);
+	// an edge's rows are its joints (creation order), then its live contacts.
+	struct Edge {
+		int first = -1;            // first cache index of the pair, -1 none
+		std::vector<int> joints;   // joint indices
+	};
+	std::map<std::pair<int, int>, Edge> edges;
+	for (size_t k = 0; k < W.cache.size(); k++) {
+		Edge& e = edges[{W.cache[k].b1, W.cache[k].b2}];
+		if (e.first < 0) e.first = (int)k;
 	}
+	for (size_t ji = 0; ji < W.joints.size(); ji++) {
+		int a = W.joints[ji].a, b = W.joints[ji].b;
+		edges[{std::min(a, b), std::max(a, b)}].joints.push_back((int)ji);
+	}
+	std::vector<std::vector<std::pair<int, const Edge*>>> adj(N);
+	for (auto& kv : edges) {
+		adj[kv.first.first].push_back({kv.first.second, &kv.second});
+		adj[kv.first.second].push_back({kv.first.first, &kv.second});
+	}
+	for (auto& a : adj)
+		std::sort(a.begin(), a.end(), [&](const std::pair<int, const Edge*>& x, const std::pair<int, const Edge*>& y) {
+			return W.bodies[x.first].id < W.bodies[y.first].id;
+		});
 	std::vector<std::vector<std::pair<Contact*, int>>> islands;
+	std::vector<std::vector<int>> isl_sys;
 	W.last_rows.clear();
 	W.last_offs.assign(1, 0);
 	if (sched_rows) {
 		// explicit sweep order over the post-narrowphase cache (one row list per group)
 		for (int g = 0; g < sched_n; g++) {
 			std::vector<std::pair<Contact*, int>> rows;
-			for (int64_t r = sched_offs[g]; r < sched_offs[g + 1]; r++) rows.push_back({&W.cache[sched_rows[2 * r]], (int)sched_rows[2 * r + 1]});
-			if (!rows.empty()) islands.push_back(rows);
+			for (int64_t r = sched_offs[g]; r < sched_offs[g + 1]; r++) {
+				if (sched_rows[2 * r + 1] == 2) rows.push_back({nullptr, (int)sched_rows[2 * r]});
+				else rows.push_back({&W.cache[sched_rows[2 * r]], (int)sched_rows[2 * r + 1]});
+			}
+			if (!rows.empty()) {
+				islands.push_back(rows);
+				std::vector<int> sys;
+				for (auto& r : rows)
+					if (!r.first && W.joints[r.second].sys >= 0 &&
+					    std::find(sys.begin(), sys.end(), W.joints[r.second].sys) == sys.end())
+						sys.push_back(W.joints[r.second].sys);
+				isl_sys.push_back(sys);
+			}
 		}
 	} else {
 		std::vector<char> visited(N, 0);
@@ -1414,6 +1928,7 @@ static void step(World& W, const int64_t* node_order, const int64_t* sched_rows 
 		for (int root : roots) {
 			if (!W.bodies[root].dynamic() || visited[root]) continue;
 			std::vector<std::pair<Contact*, int>> rows;
+			std::vector<int> sys;
 			std::vector<int> st{root};
 			while (!st.empty()) {
 				int cur = st.back();
@@ -1424,7 +1939,18 @@ static void step(World& W, const int64_t* node_order, const int64_t* sched_rows 
 					for (auto& e : adj[cur]) {
 						if (visited[e.first]) continue;
 						int lo = std::min(cur, e.first), hi = std::max(cur, e.first);
-						for (size_t k = e.second; k < W.cache.size() && W.cache[k].b1 == lo && W.cache[k].b2 == hi; k++) {
+						if (hol > 0)
+							for (int ji : e.second->joints) {
+								int si = W.joints[ji].sys;
+								if (si >= 0 && std::find(sys.begin(), sys.end(), si) == sys.end()) sys.push_back(si);
+							}
+						for (int ji : e.second->joints) {
+							rows.push_back({nullptr, ji});
+							W.last_rows.push_back(ji);
+							W.last_rows.push_back(2);
+						}
+						if (e.second->first < 0) continue;
+						for (size_t k = e.second->first; k < W.cache.size() && W.cache[k].b1 == lo && W.cache[k].b2 == hi; k++) {
 							if (!W.cache[k].live) continue;
 							rows.push_back({&W.cache[k], 0});
 							rows.push_back({&W.cache[k], 1});
@@ -1440,6 +1966,7 @@ static void step(World& W, const int64_t* node_order, const int64_t* sched_rows 
 			}
 			if (!rows.empty()) {
 				islands.push_back(rows);
+				isl_sys.push_back(sys);
 				W.last_offs.push_back((int64_t)W.last_rows.size() / 2);
 			}
 		}
@@ -1456,11 +1983,16 @@ static void step(World& W, const int64_t* node_order, const int64_t* sched_rows 
 			if (skip) continue;
 			buildRows(W, c, sdt);
 		}
+		// joints; holonomic ones are current at the first substep (PhysicsEngine.cpp:1396-1403)
+		for (Joint& j : W.joints) {
+			if (first && j.sys >= 0 && hol > 0) continue;
+			buildJoint(W, j, sdt, true);
+		}
 		if (first) {
 			// removeExpiredContactConstraints (PhysicsEngine.cpp:1434-1446); island pointers stay valid
 			// because expired contacts are never live and the vector is compacted only after the solve.
 		}
-		for (auto& isl : islands) pgsIsland(W, isl);
+		for (size_t i = 0; i < islands.size(); i++) pgsIsland(W, islands[i], isl_sys[i], first ? hol : 0);
 		// update_positions (PhysicsEngine.cpp:390-410)
 		for (Body& b : W.bodies) {
 			if (!b.dynamic()) continue;
@@ -1488,10 +2020,12 @@ static void step(World& W, const int64_t* node_order, const int64_t* sched_rows 
 				if (!(c.life < 0)) remap[&c] = k++;
 			}
 			for (auto& isl : islands)
-				for (auto& r : isl) r.first = (Contact*)(uintptr_t)(remap[r.first] + 1);
+				for (auto& r : isl)
+					if (r.first) r.first = (Contact*)(uintptr_t)(remap[r.first] + 1);
 			W.cache = keep;
 			for (auto& isl : islands)
-				for (auto& r : isl) r.first = &W.cache[(size_t)(uintptr_t)r.first - 1];
+				for (auto& r : isl)
+					if (r.first) r.first = &W.cache[(size_t)(uintptr_t)r.first - 1];
 		}
 	}
 	// update_geometry (PhysicsEngine.cpp:424-438)
@@ -1502,6 +2036,13 @@ static void step(World& W, const int64_t* node_order, const int64_t* sched_rows 
 			worldGeometry(b);
 		}
 	}
+	// calculateHolonomicSystemInversesAsync (PhysicsEngine.cpp:1615-1648): the
+	// holonomic rows and factors the next step's first substep uses
+	if (hol > 0)
+		for (HSys& hs : W.systems) {
+			for (int ji : hs.og) buildJoint(W, W.joints[ji], dt / W.cfg.substeps, true);
+			hsFactor(W, hs, W.cfg.cfm);
+		}
 }
 
 }  // namespace
@@ -1613,6 +2154,92 @@ int orc_add_cylinder_body(orc_world* w, int movement, const double* mp, const do
 	}
 	w->bodies.push_back(b);
 	return (int)w->bodies.size() - 1;
+}
+
+int orc_add_joint(orc_world* w, int type, int b1, int b2, const double* g) {
+	Joint j;
+	j.type = type;
+	j.a = b1;
+	j.b = b2;
+	j.tp1 = V3(g[0], g[1], g[2]);
+	j.tp2 = V3(g[3], g[4], g[5]);
+	j.ax1 = V3(g[6], g[7], g[8]);
+	j.ax2 = V3(g[9], g[10], g[11]);
+	j.ps = g[12];
+	j.rs = g[13];
+	w->joints.push_back(j);
+	w->nocol.insert({std::min(b1, b2), std::max(b1, b2)});
+	return (int)w->joints.size() - 1;
+}
+
+void orc_set_joint_state(orc_world* w, const double* s) {
+	for (size_t i = 0; i < w->joints.size(); i++) {
+		Joint& j = w->joints[i];
+		const double* r = s + 11 * i;
+		for (int k = 0; k < 5; k++) j.imp[k] = r[k];
+		j.u = V3(r[5], r[6], r[7]);
+		j.w = V3(r[8], r[9], r[10]);
+	}
+	for (HSys& hs : w->systems) hs.pending = true;
+}
+
+void orc_get_joint_state(const orc_world* w, double* s) {
+	for (size_t i = 0; i < w->joints.size(); i++) {
+		const Joint& j = w->joints[i];
+		double* r = s + 11 * i;
+		for (int k = 0; k < 5; k++) r[k] = k < j.type ? j.imp[k] : 0.0;
+		r[5] = j.u.x; r[6] = j.u.y; r[7] = j.u.z;
+		r[8] = j.w.x; r[9] = j.w.y; r[10] = j.w.z;
+	}
+}
+
+void orc_set_hsys(orc_world* w, int n_sys, const int64_t* off, const int64_t* og) {
+	w->systems.clear();
+	for (Joint& j : w->joints) j.sys = -1;
+	for (int k = 0; k < n_sys; k++) {
+		HSys hs;
+		for (int64_t i = off[k]; i < off[k + 1]; i++) {
+			hs.og.push_back((int)og[i]);
+			w->joints[og[i]].sys = k;
+		}
+		hsysLayout(*w, hs);
+		w->systems.push_back(hs);
+	}
+}
+
+int orc_hsys_count(const orc_world* w) { return (int)w->systems.size(); }
+
+int orc_get_joint_rows(const orc_world* w, int k, double* ja, double* jb, double* ita, double* itb) {
+	const Joint& j = w->joints[k];
+	for (int r = 0; r < j.type; r++)
+		for (int c = 0; c < 6; c++) {
+			ja[r * 6 + c] = j.ja.v[r][c];
+			jb[r * 6 + c] = j.jb.v[r][c];
+			ita[c * j.type + r] = j.ita.v[c][r];
+			itb[c * j.type + r] = j.itb.v[c][r];
+		}
+	return j.type;
+}
+
+void orc_hsys_prepare(orc_world* w) {
+	for (HSys& hs : w->systems) {
+		if (!hs.pending) continue;
+		for (int ji : hs.og) buildJoint(*w, w->joints[ji], w->cfg.dt / w->cfg.substeps, false);
+		hsFactor(*w, hs, w->cfg.cfm);
+		hs.pending = false;
+	}
+}
+
+void orc_get_hsys(const orc_world* w, int k, int64_t* order, int64_t* table, int64_t* vloc, double* buf, int64_t* sizes4) {
+	const HSys& hs = w->systems[k];
+	sizes4[0] = (int64_t)hs.order.size();
+	sizes4[1] = (int64_t)hs.table.size();
+	sizes4[2] = hs.degree;
+	sizes4[3] = hs.cap;
+	if (order) for (size_t i = 0; i < hs.order.size(); i++) order[i] = hs.order[i];
+	if (table) for (size_t i = 0; i < hs.table.size(); i++) table[i] = hs.table[i];
+	if (vloc) for (size_t i = 0; i < hs.vloc.size(); i++) vloc[i] = hs.vloc[i];
+	if (buf) for (int i = 0; i < hs.cap; i++) buf[i] = hs.buf[i];
 }
 
 void orc_set_state(orc_world* w, const double* s) {

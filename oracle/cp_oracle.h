@@ -27,6 +27,7 @@ typedef struct {
 	double contact_hardness;
 	int contact_life;
 	int drop_nonfinite;   // 1: drop non-finite manifold points (the product's defect fix); 0: as the reference
+	int hol_iters;        // holonomic block-solver iterations (setPGSIterations' third argument); 0 = none
 } orc_config;
 
 orc_world* orc_create(const orc_config* cfg);
@@ -54,6 +55,29 @@ int orc_add_sphere_body(orc_world* w, int movement, const double* mass_props, co
 // gauss42 = 14 Gauss vertices; top24/bot24 = the 8-gon cap polygons.
 int orc_add_cylinder_body(orc_world* w, int movement, const double* mass_props, const double* material, const double* params8,
                           const double* gauss42, const double* top24, const double* bot24);
+
+// Joints (PhysicsEngine::addBallSocketConstraint / addHingeConstraint,
+// PhysicsEngine.cpp:664-720).  type 3 ball-socket, 5 hinge; b1/b2 body
+// indices; geo14 = tracked point in b1's frame (3), in b2's frame (3), hinge
+// axis in b1's frame (3), in b2's frame (3), position and rotation correction
+// strengths.  The pair stops colliding.  Returns the joint index.
+int orc_add_joint(orc_world* w, int type, int b1, int b2, const double* geo14);
+// Joint solver state rows of 11 doubles: impulse (5, zero padded), hinge u, w.
+void orc_set_joint_state(orc_world* w, const double* s11);
+void orc_get_joint_state(const orc_world* w, double* s11);
+// Holonomic systems: n_sys systems, system k holds joints og[off[k]..off[k+1])
+// in the order the reference's revaluateSystem collects them.  Rows and factors
+// are built at the next step start.
+void orc_set_hsys(orc_world* w, int n_sys, const int64_t* off, const int64_t* og);
+int orc_hsys_count(const orc_world* w);
+// Joint k's current rows: J_a, J_b [n][6] and M_a^-1 J_a^T, M_b^-1 J_b^T [6][n]; returns n.
+int orc_get_joint_rows(const orc_world* w, int k, double* ja, double* jb, double* ita, double* itb);
+// Builds pending systems' rows and factors now (as the next step would).
+void orc_hsys_prepare(orc_world* w);
+// System k's elimination order, block table, vector offsets and factor buffer;
+// sizes4 = {n joints, table entries, degree, buffer doubles}.  Null arrays
+// return sizes only.
+void orc_get_hsys(const orc_world* w, int k, int64_t* order, int64_t* table, int64_t* vloc, double* buf, int64_t* sizes4);
 
 // Body state rows of 13 doubles: com 3, q (r,i,j,k), vel 3, ang_vel 3.
 void orc_set_state(orc_world* w, const double* state);

@@ -215,6 +215,113 @@ static void build_mixed(PhysicsEngine& E, int nx, int ny, int nz, uint64_t seed,
 			}
 }
 
+// Chains of cylinder links lying along +x, consecutive links joined end to end
+// by ball-socket joints, every hinge_every-th joint a hinge about +z (default
+// correction stiffnesses, PhysicsEngine.h:15-16).  Links: r 0.15, h 0.8, gap
+// 0.1; chain c at z = 1.2 c, height U[0.4, 1.2], x offset U[-0.5, 0.5]; every link
+// gets v ~ U[-0.5, 0.5]^3 so the chain bends as it falls onto the ground box.
+struct JointRec {
+	int type;  // 3 ball-socket, 5 hinge
+	RigidBody *b1, *b2;
+	phyz::PersistentConstraint* pc;
+	phyz::Constraint* solver;
+};
+static std::vector<JointRec> g_joints;
+
+static void build_chains(PhysicsEngine& E, int n_chains, int n_links, uint64_t seed, int hinge_every) {
+	phyz::Material m = c0_material();
+	RigidBody* g = E.createRigidBody(phyz::ConvexUnionGeometry::box(mthz::Vec3(-100, -1, -100), 200, 1, 200, m), RigidBody::FIXED);
+	pin_body(g);
+	SplitMix64 rng(seed);
+	const double r = 0.15, h = 0.8, gap = 0.1, PI_ = 3.14159265358979323846;
+	mthz::Quaternion q(-PI_ / 2.0, mthz::Vec3(0, 0, 1));  // link axis +y -> +x
+	for (int c = 0; c < n_chains; c++) {
+		double y0 = rng.uniform(0.4, 1.2), x0 = rng.uniform(-0.5, 0.5), z0 = 1.2 * c;
+		RigidBody* prev = nullptr;
+		mthz::Vec3 prev_c;
+		for (int i = 0; i < n_links; i++) {
+			mthz::Vec3 cc(x0 + i * (h + gap), y0, z0);
+			RigidBody* b = E.createRigidBody(phyz::ConvexUnionGeometry::cylinder(cc - mthz::Vec3(0, h / 2.0, 0), r, h, m));
+			b->setOrientation(q);
+			b->setVel(mthz::Vec3(rng.uniform(-0.5, 0.5), rng.uniform(-0.5, 0.5), rng.uniform(-0.5, 0.5)));
+			pin_body(b);
+			if (prev) {
+				mthz::Vec3 a1 = prev_c + mthz::Vec3(0, h / 2.0 + gap / 2.0, 0);
+				mthz::Vec3 a2 = cc - mthz::Vec3(0, h / 2.0 + gap / 2.0, 0);
+				bool hinge = hinge_every > 0 && i % hinge_every == 0;
+				phyz::ConstraintID id = hinge ? E.addHingeConstraint(prev, b, a1, a2, mthz::Vec3(0, 0, 1), mthz::Vec3(0, 0, 1))
+				                              : E.addBallSocketConstraint(prev, b, a1, a2);
+				auto* e = E.constraint_map[id.getID()];
+				phyz::PersistentConstraint* pc = nullptr;
+				for (auto* x : e->constraints)
+					if (x->getId() == id) pc = x;
+				phyz::Constraint* sc = hinge ? (phyz::Constraint*)&static_cast<phyz::Hinge*>(pc)->constraint
+				                             : (phyz::Constraint*)&static_cast<phyz::BallSocket*>(pc)->constraint;
+				g_joints.push_back({hinge ? 5 : 3, prev, b, pc, sc});
+			}
+			prev = b;
+			prev_c = cc;
+		}
+	}
+}
+
+// Joint solver state: impulse (5, zero padded), hinge u and w (zeros for ball-sockets).
+static void joint_state(std::vector<double>& s) {
+	for (auto& j : g_joints) {
+		if (j.type == 5) {
+			auto* h = static_cast<phyz::Hinge*>(j.pc);
+			for (int k = 0; k < 5; k++) s.push_back(h->constraint.impulse.v[k]);
+			s.insert(s.end(), {h->constraint.u.x, h->constraint.u.y, h->constraint.u.z, h->constraint.w.x, h->constraint.w.y, h->constraint.w.z});
+		} else {
+			auto* b = static_cast<phyz::BallSocket*>(j.pc);
+			for (int k = 0; k < 3; k++) s.push_back(b->constraint.impulse.v[k]);
+			s.insert(s.end(), {0, 0, 0, 0, 0, 0, 0, 0});
+		}
+	}
+}
+
+static int joint_index(const phyz::Constraint* c) {
+	for (size_t i = 0; i < g_joints.size(); i++)
+		if (g_joints[i].solver == c) return (int)i;
+	return -1;
+}
+
+// Holonomic systems at this moment: per system the constraint order of
+// revaluateSystem (member edges), the elimination order, the block location
+// table, vector offsets and the factor buffer (PhysicsEngine.cpp:1785-1798,
+// HolonomicBlockSolver.cpp:11-132, :459-543).
+struct HsysDump {
+	std::vector<int64_t> off{0}, og, order, table_off{0}, table, vloc;
+	std::vector<int64_t> buf_off{0};
+	std::vector<double> buf;
+};
+static void dump_hsys(PhysicsEngine& E, HsysDump& d) {
+	std::set<PhysicsEngine::HolonomicSystemNodes*> seen;
+	std::vector<PhysicsEngine::HolonomicSystemNodes*> list;
+	std::vector<unsigned> ids;
+	for (auto& kv : E.constraint_graph_nodes) ids.push_back(kv.first);
+	std::sort(ids.begin(), ids.end());
+	for (unsigned id : ids)
+		for (auto* e : E.constraint_graph_nodes[id]->constraints)
+			if (e->h && seen.insert(e->h).second) list.push_back(e->h);
+	for (auto* h : list) {
+		for (auto* e : h->member_edges)
+			for (auto* pc : e->constraints)
+				if (pc->isHolonomicConstraint()) {
+					std::vector<phyz::Constraint*> v;
+					pc->addSolverConstraints(&v);
+					d.og.push_back(joint_index(v[0]));
+				}
+		for (auto* c : h->system.constraints) d.order.push_back(joint_index(c));
+		d.off.push_back((int64_t)d.order.size());
+		for (int t : h->system.block_location_table) d.table.push_back(t);
+		d.table_off.push_back((int64_t)d.table.size());
+		for (int v : h->system.vector_location_lookup) d.vloc.push_back(v);
+		for (int k = 0; k < h->system.buffer_capacity; k++) d.buf.push_back(h->system.buffer[k]);
+		d.buf_off.push_back((int64_t)d.buf.size());
+	}
+}
+
 // ---------------------------------------------------------------------------
 // Snapshot helpers.
 
@@ -326,7 +433,7 @@ static void on_collision(RigidBody* b1, RigidBody* b2, const std::vector<phyz::M
 		for (auto& isl : a.island_systems) {
 			for (phyz::Constraint* c : isl.constraints) {
 				auto it = d->mid_cache.rowmap.find(c);
-				if (it == d->mid_cache.rowmap.end()) d->island_rows.insert(d->island_rows.end(), {-1, -1});
+				if (it == d->mid_cache.rowmap.end()) d->island_rows.insert(d->island_rows.end(), {(int64_t)joint_index(c), joint_index(c) >= 0 ? 2 : -1});
 				else d->island_rows.insert(d->island_rows.end(), {it->second.first, it->second.second});
 			}
 			d->island_off.push_back((int64_t)d->island_rows.size() / 2);
@@ -445,6 +552,24 @@ static void dump_static(PhysicsEngine& E, Out& o) {
 	o.i64("gauss_arc_off", aoff, {N + 1});
 	o.i64("hull_edges", edges, {edges.size() / 2, 2});
 	o.i64("hull_edge_off", eoff, {N + 1});
+	std::vector<int64_t> ji;
+	std::vector<double> jg;
+	for (auto& j : g_joints) {
+		ji.insert(ji.end(), {(int64_t)j.type, (int64_t)j.b1->getID(), (int64_t)j.b2->getID()});
+		if (j.type == 5) {
+			auto* h = static_cast<phyz::Hinge*>(j.pc);
+			mthz::Vec3 t1 = j.b1->track_p[h->b1_point_key], t2 = j.b2->track_p[h->b2_point_key];
+			jg.insert(jg.end(), {t1.x, t1.y, t1.z, t2.x, t2.y, t2.z, h->b1_rot_axis_body_space.x, h->b1_rot_axis_body_space.y,
+			                     h->b1_rot_axis_body_space.z, h->b2_rot_axis_body_space.x, h->b2_rot_axis_body_space.y,
+			                     h->b2_rot_axis_body_space.z, h->pos_correct_hardness, h->rot_correct_hardness});
+		} else {
+			auto* b = static_cast<phyz::BallSocket*>(j.pc);
+			mthz::Vec3 t1 = j.b1->track_p[b->b1_point_key], t2 = j.b2->track_p[b->b2_point_key];
+			jg.insert(jg.end(), {t1.x, t1.y, t1.z, t2.x, t2.y, t2.z, 0, 0, 0, 0, 0, 0, b->pos_correct_hardness, 0});
+		}
+	}
+	o.i64("joint_info", ji, {g_joints.size(), 3});
+	o.f64("joint_geo", jg, {g_joints.size(), 14});
 }
 
 static void put_cache(Out& o, const std::string& p, const CacheDump& c) {
@@ -468,6 +593,7 @@ static void run_trace(PhysicsEngine& E, const SceneCfg& cfg, int steps, const st
 	o.meta("vel_iters", cfg.vel_iters);
 	o.meta("pos_iters", cfg.pos_iters);
 	o.meta("substeps", cfg.substeps);
+	o.meta("hol_iters", cfg.hol_iters);
 	o.meta("steps", steps);
 	std::vector<int64_t> rec_list;
 	g_eng = &E;
@@ -475,7 +601,11 @@ static void run_trace(PhysicsEngine& E, const SceneCfg& cfg, int steps, const st
 	for (int k = 0; k < steps; k++) {
 		bool rec = record.count(k) > 0;
 		StepDump d;
+		std::vector<double> jpre;
+		HsysDump hs;
 		if (rec) {
+			joint_state(jpre);
+			dump_hsys(E, hs);
 			body_state(E, d.pre_state);
 			body_inertia(E, d.pre_inertia);
 			dump_cache(E, d.pre_cache);
@@ -508,6 +638,20 @@ static void run_trace(PhysicsEngine& E, const SceneCfg& cfg, int steps, const st
 		if (d.island_off.empty()) d.island_off.push_back(0);
 		o.i64(p + "_island_off", d.island_off, {d.island_off.size()});
 		o.i64(p + "_node_order", d.node_order, {d.node_order.size()});
+		if (!g_joints.empty()) {
+			std::vector<double> jpost;
+			joint_state(jpost);
+			o.f64(p + "_joint_pre", jpre, {g_joints.size(), 11});
+			o.f64(p + "_joint_post", jpost, {g_joints.size(), 11});
+			o.i64(p + "_hsys_off", hs.off, {hs.off.size()});
+			o.i64(p + "_hsys_og", hs.og, {hs.og.size()});
+			o.i64(p + "_hsys_order", hs.order, {hs.order.size()});
+			o.i64(p + "_hsys_table_off", hs.table_off, {hs.table_off.size()});
+			o.i64(p + "_hsys_table", hs.table, {hs.table.size()});
+			o.i64(p + "_hsys_vloc", hs.vloc, {hs.vloc.size()});
+			o.i64(p + "_hsys_buf_off", hs.buf_off, {hs.buf_off.size()});
+			o.f64(p + "_hsys_buf", hs.buf, {hs.buf.size()});
+		}
 	}
 	o.i64("recorded_steps", rec_list, {rec_list.size()});
 	{
@@ -565,6 +709,24 @@ static int cmd_pile(int argc, char** argv) {
 	return 0;
 }
 
+// chain: jointed cylinder chains on a ground box, trace dump.
+static int cmd_chain(int argc, char** argv) {
+	int chains = (int)arg_d(argc, argv, "--chains", 3);
+	int links = (int)arg_d(argc, argv, "--links", 6);
+	int steps = (int)arg_d(argc, argv, "--steps", 10);
+	uint64_t seed = (uint64_t)arg_d(argc, argv, "--seed", 4242);
+	SceneCfg cfg;
+	cfg.hol_iters = (int)arg_d(argc, argv, "--hol", 1);
+	PhysicsEngine E;
+	configure(E, cfg);
+	build_chains(E, chains, links, seed, (int)arg_d(argc, argv, "--hinge-every", 3));
+	Out o(arg_s(argc, argv, "--out", "out"));
+	o.meta("seed", (double)seed);
+	run_trace(E, cfg, steps, arg_steps(argc, argv, "--record"), o);
+	o.finish();
+	return 0;
+}
+
 // time: single-threaded wall time of the reference's own timeStep on a C0 pile.
 static int cmd_time(int argc, char** argv) {
 	int n = (int)arg_d(argc, argv, "--n", 10);
@@ -605,6 +767,7 @@ int main(int argc, char** argv) {
 	}
 	std::string cmd = argv[1];
 	if (cmd == "pile") return cmd_pile(argc, argv);
+	if (cmd == "chain") return cmd_chain(argc, argv);
 	if (cmd == "time") return cmd_time(argc, argv);
 	fprintf(stderr, "unknown command %s\n", cmd.c_str());
 	return 2;

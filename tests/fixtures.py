@@ -94,6 +94,7 @@ class OrcConfig(ctypes.Structure):
         ("contact_hardness", ctypes.c_double),
         ("contact_life", ctypes.c_int),
         ("drop_nonfinite", ctypes.c_int),
+        ("hol_iters", ctypes.c_int),
     ]
 
 
@@ -138,6 +139,17 @@ def oracle_lib():
         lib.orc_narrowphase.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         lib.orc_step.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         lib.orc_step_scheduled.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+        lib.orc_add_joint.restype = ctypes.c_int
+        lib.orc_add_joint.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+        lib.orc_set_joint_state.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        lib.orc_get_joint_state.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        lib.orc_set_hsys.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+        lib.orc_hsys_count.restype = ctypes.c_int
+        lib.orc_hsys_count.argtypes = [ctypes.c_void_p]
+        lib.orc_hsys_prepare.argtypes = [ctypes.c_void_p]
+        lib.orc_get_joint_rows.restype = ctypes.c_int
+        lib.orc_get_joint_rows.argtypes = [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 4
+        lib.orc_get_hsys.argtypes = [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 5
         lib.orc_last_islands.restype = ctypes.c_int
         lib.orc_last_islands.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
         _lib = lib
@@ -152,12 +164,17 @@ class Oracle:
     """FP64 CPU restatement of the reference step, loaded with a fixture's scene."""
 
     def __init__(self, fx=None, cfm=0.0000025, hardness=1000.0, life=6, desc=None, dt=None, gravity_y=None, vel_iters=None,
-                 pos_iters=None, substeps=None, drop_nonfinite=0):
+                 pos_iters=None, substeps=None, drop_nonfinite=0, hol_iters=None):
         lib = oracle_lib()
         self.lib = lib
+        joints = None
         if fx is not None:
             m = fx.meta
             dt, gravity_y, vel_iters, pos_iters, substeps = m["dt"], m["gravity_y"], m["vel_iters"], m["pos_iters"], m["substeps"]
+            if hol_iters is None:
+                hol_iters = int(m.get("hol_iters", 1))
+            if "joint_info" in fx:
+                joints = (fx["joint_info"], fx["joint_geo"])
             n = int(m["n_bodies"])
             movement = fx["body_info"][:, 1]
             mp, mat = fx["mass_props"], fx["material"]
@@ -166,7 +183,12 @@ class Oracle:
         else:
             n = len(desc["hulls"])
             movement, mp, mat, hulls = desc["movement"], desc["mass_props"], desc["material"], desc["hulls"]
-        cfg = OrcConfig(dt, (ctypes.c_double * 3)(0.0, gravity_y, 0.0), int(vel_iters), int(pos_iters), int(substeps), cfm, hardness, life, drop_nonfinite)
+        if desc is not None and "joints" in desc:
+            joints = desc["joints"]
+        if hol_iters is None:
+            hol_iters = 1
+        cfg = OrcConfig(dt, (ctypes.c_double * 3)(0.0, gravity_y, 0.0), int(vel_iters), int(pos_iters), int(substeps), cfm, hardness, life,
+                        drop_nonfinite, int(hol_iters))
         self.w = ctypes.c_void_p(lib.orc_create(ctypes.byref(cfg)))
         self.n = n
         self._keep = []
@@ -192,6 +214,45 @@ class Oracle:
             lib.orc_add_hull_body(self.w, int(movement[b]), arrs[0].ctypes.data, arrs[1].ctypes.data, len(h["pts"]), arrs[2].ctypes.data,
                                   len(h["face_off"]) - 1, *[a.ctypes.data for a in arrs[3:12]], len(h["arcs"]), arrs[12].ctypes.data,
                                   len(h["edges"]), arrs[13].ctypes.data)
+
+        self.n_joints = 0
+        if joints is not None:
+            info, geo = joints
+            for k in range(len(info)):
+                g = _c(geo[k], np.float64)
+                self._keep.append(g)
+                lib.orc_add_joint(self.w, int(info[k][0]), int(info[k][1]) - 1, int(info[k][2]) - 1, g.ctypes.data)
+            self.n_joints = len(info)
+
+    def load_joints(self, state11, hsys_off=None, hsys_og=None):
+        st = _c(state11, np.float64)
+        if hsys_off is not None:
+            off, og = _c(hsys_off, np.int64), _c(hsys_og, np.int64)
+            self.lib.orc_set_hsys(self.w, len(off) - 1, off.ctypes.data, og.ctypes.data)
+        self.lib.orc_set_joint_state(self.w, st.ctypes.data)
+
+    def joint_state(self):
+        s = np.zeros((self.n_joints, 11))
+        self.lib.orc_get_joint_state(self.w, s.ctypes.data)
+        return s
+
+    def joint_rows(self, k):
+        a = [np.zeros(30) for _ in range(4)]
+        n = self.lib.orc_get_joint_rows(self.w, k, *[x.ctypes.data for x in a])
+        return a[0][:6 * n].reshape(n, 6), a[1][:6 * n].reshape(n, 6), a[2][:6 * n].reshape(6, n), a[3][:6 * n].reshape(6, n)
+
+    def hsys(self, prepare=False):
+        """Per system (order, table, vloc, factor buffer)."""
+        if prepare:
+            self.lib.orc_hsys_prepare(self.w)
+        out = []
+        for k in range(self.lib.orc_hsys_count(self.w)):
+            sz = np.zeros(4, np.int64)
+            self.lib.orc_get_hsys(self.w, k, None, None, None, None, sz.ctypes.data)
+            o, t, v, b = np.zeros(sz[0], np.int64), np.zeros(sz[1], np.int64), np.zeros(sz[0], np.int64), np.zeros(sz[3])
+            self.lib.orc_get_hsys(self.w, k, o.ctypes.data, t.ctypes.data, v.ctypes.data, b.ctypes.data, sz.ctypes.data)
+            out.append((o, t, v, b))
+        return out
 
     @classmethod
     def from_description(cls, desc, **kw):

@@ -8,7 +8,7 @@ ROOT=$(cd "$HERE/../.." && pwd)
 make -C "$ROOT/oracle" ref REF="${REF:-/root/reference}"
 H="$ROOT/oracle/_ref/cp_ref_harness"
 cd "$HERE"
-rm -rf pile3 c0_full mix3 cyl3 cyl4
+rm -rf pile3 c0_full mix3 cyl3 cyl4 chain3 chain3h0
 # C0 pattern, 3x3x3 cubes: full per-step dumps (state, caches, manifolds, islands).
 "$H" pile --n 3 --steps 40 --record 0,1,2,10,39 --out pile3
 # C0 in full (10x10x10 cubes): first step and a settled step (no mid/post caches).
@@ -19,3 +19,8 @@ rm -rf pile3 c0_full mix3 cyl3 cyl4
 # orders), sphere-cylinder, cylinder-cylinder and cylinder-ground pairs.
 "$H" pile --mixed 2 --n 3 --steps 80 --record 0,10,30,60,79 --out cyl3
 "$H" pile --mixed 2 --n 4 --ny 3 --steps 150 --record 40,90,149 --seed 23 --out cyl4
+# Jointed chains (3 chains x 6 cylinder links, ball-sockets with every third
+# joint a hinge) falling onto the ground box; with and without the holonomic
+# block solver.
+"$H" chain --chains 3 --links 6 --steps 100 --record 1,2,45,70,99 --seed 4242 --out chain3
+"$H" chain --chains 3 --links 6 --steps 100 --record 1,45,70,99 --seed 4242 --hol 0 --out chain3h0

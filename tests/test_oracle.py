@@ -94,3 +94,28 @@ def test_mixed_primitives_full_step(name, k):
     pi, pm, pf = o.cache()
     gi, gm, gf = fx.cache(k, "post")
     assert np.array_equal(pi, gi) and np.array_equal(pm, gm) and np.array_equal(pf, gf)
+
+
+@pytest.mark.parametrize("name", ["chain3", "chain3h0"])
+def test_jointed_chains_full_step(name):
+    """Ball-socket and hinge joints (PGS rows) and, for chain3, the holonomic
+    block solver: elimination order, block table and LDL factor buffer, then
+    the full step (state, joint impulses and hinge bases, island rows)."""
+    import constraintphysics_amd as cpa
+    fx = golden(name)
+    for k in fx.recorded:
+        o = Oracle(fx)
+        _load(o, fx, k)
+        o.load_joints(fx[f"s{k}_joint_pre"], fx[f"s{k}_hsys_off"], fx[f"s{k}_hsys_og"])
+        off, toff, boff = fx[f"s{k}_hsys_off"], fx[f"s{k}_hsys_table_off"], fx[f"s{k}_hsys_buf_off"]
+        for si, (order, table, vloc, buf) in enumerate(o.hsys(prepare=True)):
+            assert np.array_equal(order, fx[f"s{k}_hsys_order"][off[si]:off[si + 1]])
+            assert np.array_equal(table, fx[f"s{k}_hsys_table"][toff[si]:toff[si + 1]])
+            assert np.array_equal(buf, fx[f"s{k}_hsys_buf"][boff[si]:boff[si + 1]])
+        no = fx[f"s{k}_node_order"]
+        o.step(no if len(no) else cpa.reference_node_order(int(fx.meta["n_bodies"])))
+        rows, offs = o.islands()
+        if len(fx[f"s{k}_island_rows"]):
+            assert np.array_equal(rows, fx[f"s{k}_island_rows"])
+        assert np.array_equal(o.state(), fx[f"s{k}_post"])
+        assert np.array_equal(o.joint_state(), fx[f"s{k}_joint_post"])

@@ -315,3 +315,87 @@ def test_resting_box_contacts_persist_with_warm_start():
         o.step()
     st = o.state()[1]
     assert np.linalg.norm(st[7:13]) < 5e-3
+
+
+# ----------------------------------------------------------------------------- joints and the holonomic block solver
+def _chain_oracle(k=45):
+    fx = golden("chain3")
+    o = Oracle(fx)
+    o.load(fx[f"s{k}_pre"], fx[f"s{k}_pre_inertia"], fx.cache(k, "pre"))
+    o.load_joints(fx[f"s{k}_joint_pre"], fx[f"s{k}_hsys_off"], fx[f"s{k}_hsys_og"])
+    return fx, o
+
+
+def test_joint_rows_measure_relative_velocity_of_attach_points():
+    """Rows 0-2 of a ball-socket or hinge: J_a [v_a; w_a] + J_b [v_b; w_b] =
+    (v_a + w_a x r_a) - (v_b + w_b x r_b), the relative velocity of the joint
+    points (r from each centre of mass to the b1 / b2 point; the hinge takes
+    both arms from the b1 point)."""
+    fx, o = _chain_oracle()
+    o.hsys(prepare=True)
+    st = fx["s45_pre"]
+    info, geo = fx["joint_info"], fx["joint_geo"]
+    rng = np.random.default_rng(3)
+    for k in range(len(info)):
+        ja, jb, _, _ = o.joint_rows(k)
+        a, b = int(info[k][1]) - 1, int(info[k][2]) - 1
+        pa = st[a, :3] + qrot(st[a, 3:7]) @ geo[k, 0:3]
+        pb = st[b, :3] + qrot(st[b, 3:7]) @ geo[k, 3:6]
+        ra = pa - st[a, :3]
+        rb = (pa if info[k][0] == 5 else pb) - st[b, :3]
+        va, wa, vb, wb = rng.normal(size=(4, 3))
+        got = ja[:3] @ np.concatenate([va, wa]) + jb[:3] @ np.concatenate([vb, wb])
+        ref = (va + np.cross(wa, ra)) - (vb + np.cross(wb, rb))
+        np.testing.assert_allclose(got, ref, atol=1e-12)
+
+
+def test_holonomic_factor_reproduces_the_coupled_system():
+    """The block LDL factor (D^-1 on the diagonal, L below) satisfies
+    L D L^T = A, with A assembled densely from the joint rows: A_ij =
+    J_i(shared body) M^-1 J_j^T, diagonal blocks with (1 + cfm); and the
+    factor's solve agrees with numpy.linalg.solve."""
+    fx, o = _chain_oracle()
+    systems = o.hsys(prepare=True)
+    info = fx["joint_info"]
+    cfm = 2.5e-6
+    for order, table, vloc, buf in systems:
+        n = len(order)
+        deg = [int(info[j][0]) for j in order]
+        tot = sum(deg)
+        rows = {j: o.joint_rows(int(j)) for j in order}
+        A = np.zeros((tot, tot))
+        for r in range(n):
+            for c in range(n):
+                jr, jc = int(order[r]), int(order[c])
+                ja_r, jb_r, _, _ = rows[jr]
+                _, _, ita_c, itb_c = rows[jc]
+                ar, br = int(info[jr][1]), int(info[jr][2])
+                ac, bc = int(info[jc][1]), int(info[jc][2])
+                blk = np.zeros((deg[r], deg[c]))
+                if ar == ac: blk += ja_r @ ita_c
+                if ar == bc: blk += ja_r @ itb_c
+                if br == ac: blk += jb_r @ ita_c
+                if br == bc: blk += jb_r @ itb_c
+                if r == c:
+                    blk = blk * 1.0
+                    blk[np.diag_indices(deg[r])] *= (1 + cfm)
+                A[vloc[r]:vloc[r] + deg[r], vloc[c]:vloc[c] + deg[c]] = blk
+        L = np.eye(tot)
+        Dinv = np.zeros((tot, tot))
+        t = 0
+        for c in range(n):
+            for r in range(c, n):
+                loc = table[t]
+                t += 1
+                if loc < 0:
+                    continue
+                blk = buf[loc:loc + deg[r] * deg[c]].reshape(deg[r], deg[c])
+                if r == c:
+                    Dinv[vloc[c]:vloc[c] + deg[c], vloc[c]:vloc[c] + deg[c]] = blk
+                else:
+                    L[vloc[r]:vloc[r] + deg[r], vloc[c]:vloc[c] + deg[c]] = blk
+        rec = L @ np.linalg.inv(Dinv) @ L.T
+        assert np.max(np.abs(rec - A)) <= 1e-9 * np.max(np.abs(A))
+        b = np.random.default_rng(1).normal(size=tot)
+        x = np.linalg.solve(L.T, Dinv @ np.linalg.solve(L, b))
+        np.testing.assert_allclose(x, np.linalg.solve(A, b), rtol=1e-8, atol=1e-12)
