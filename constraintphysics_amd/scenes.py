@@ -245,14 +245,14 @@ def cylinder_library(seed, count=256, density=1.0):
     return geo, mp, np.sqrt(re * re + (h / 2) ** 2)
 
 
-def scene_bodies(sizes, seed, s, lib, sph_r, cyl_r=None):
+def scene_bodies(sizes, seed, s, lib, sph_r, cyl_r=None, mix=MIX):
     """Dynamic bodies of scene s: kind (0 hull, 1 sphere, 2 cylinder), library
     pick, container side, positions, orientations, velocities."""
     n = int(sizes[s])
     rng = np.random.default_rng(seed + s)
-    w = np.array([m[1] for m in MIX])
-    kinds = rng.choice(len(MIX), size=n, p=w / w.sum())
-    kind = np.array([KIND[MIX[k][0]] for k in kinds])
+    w = np.array([m[1] for m in mix])
+    kinds = rng.choice(len(mix), size=n, p=w / w.sum())
+    kind = np.array([KIND[mix[k][0]] for k in kinds])
     picks = rng.integers(0, len(lib.items), n)
     if cyl_r is None:
         cyl_r = cylinder_library(seed, len(lib.items))[2]
@@ -309,13 +309,13 @@ def build_batched(world, sizes, seed, scene_indices=None, hull_lib_size=256):
     return dict(dynamic_bodies=total, mix=mix_description())
 
 
-def scene_description(sizes, seed, s, hull_lib_size=256):
+def scene_description(sizes, seed, s, hull_lib_size=256, mix=MIX):
     """Scene s as plain arrays (for the CPU oracle): per body the hull in the
     exact-hull layout, movement, mass properties, material, and the state."""
     lib = hull_library(seed, hull_lib_size)
     sph_r, sph_mp = sphere_library(seed, hull_lib_size)
     cyl_geo, cyl_mp, cyl_r = cylinder_library(seed, hull_lib_size)
-    kind, picks, L, pos, q, vel = scene_bodies(sizes, seed, s, lib, sph_r, cyl_r)
+    kind, picks, L, pos, q, vel = scene_bodies(sizes, seed, s, lib, sph_r, cyl_r, mix)
     hulls, move, mps, mats, state = [], [], [], [], []
     cache = {}
     for (dx, dy, dz), c in container_walls(L):

@@ -16,13 +16,14 @@ HERE = os.path.join(GOLDEN, "clip_nan")
 
 @pytest.mark.parametrize("fix", [0, 1])
 def test_clip_nan_case(fix):
+    from fixtures import world_from_description
     sizes = scenes.c3_sizes(4096, 20240)
-    w = cpa.World(dt=1 / 120, vel_iters=10, pos_iters=1, substeps=8, defect_fixes=fix)
-    scenes.build_batched(w, sizes, 20240, scene_indices=[1503])
+    # the spheres-and-hulls mix the batched scenes used when this state was recorded
+    desc = scenes.scene_description(sizes, 20240, 1503, mix=(("sphere", 0.4), ("hull", 0.3)))
     ld = lambda k: np.load(os.path.join(HERE, k + ".npy"))
-    w.set_state(ld("st"))
+    desc["state"] = ld("st")
+    w = world_from_description(desc, dt=1 / 120, vel_iters=10, pos_iters=1, substeps=8, defect_fixes=fix)
     counts, heads, pts, mg = w.narrowphase(np.array([[1, 495]]))
-    desc = scenes.scene_description(sizes, 20240, 1503)
     o = Oracle(desc=desc, dt=1 / 120, gravity_y=-6.0, vel_iters=10, pos_iters=1, substeps=8, drop_nonfinite=fix)
     o.load(ld("st"), None, (ld("ii"), ld("mg"), ld("ff")))
     h, p, m = o.narrowphase(1, 495)
@@ -31,3 +32,4 @@ def test_clip_nan_case(fix):
     np.testing.assert_array_equal(heads[0], h)
     np.testing.assert_array_equal(pts[0, :n], p)
     np.testing.assert_array_equal(mg[0, :n], m)
+    assert (np.isnan(pts[0, :n]).any(axis=1).sum() == 1) if fix == 0 else np.all(np.isfinite(pts[0, :n]))
