@@ -83,6 +83,14 @@ def lib():
         "cp_world_last_islands": (_I, [_P, _I, _P, _I, _P, _P, _P]),
         "cp_world_last_timing": (_I, [_P, _P]),
         "cp_world_last_residual": (_I, [_P, _P]),
+        "cp_joint_add": (_I, [_P, _I, _I, _I, _P, _P]),
+        "cp_world_joint_count": (_I, [_P, _P]),
+        "cp_world_set_joint_state": (_I, [_P, _P]),
+        "cp_world_get_joint_state": (_I, [_P, _P]),
+        "cp_world_get_joint_rows": (_I, [_P, _I, _P, _P, _P, _P, _P]),
+        "cp_world_set_holonomic_systems": (_I, [_P, _I, _P, _P]),
+        "cp_world_holonomic_system_count": (_I, [_P, _P]),
+        "cp_world_get_holonomic_system": (_I, [_P, _I, _P, _P, _P, _P, _P, _P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -341,6 +349,57 @@ class World:
         r = np.zeros(4)
         _chk(self.L.cp_world_last_residual(self.w, r.ctypes.data), "cp_world_last_residual")
         return r
+
+    def add_joint(self, kind, b1, b2, geo14):
+        """Ball-socket (kind 3) or hinge (kind 5) between body indices b1, b2;
+        geo14 = attach points in the body frames, hinge axes in the body
+        frames, position and rotation correction strengths."""
+        g = _a(geo14, np.float64).reshape(14)
+        k = ctypes.c_int()
+        _chk(self.L.cp_joint_add(self.w, int(kind), int(b1), int(b2), g.ctypes.data, ctypes.byref(k)), "cp_joint_add")
+        return k.value
+
+    def joint_count(self):
+        n = ctypes.c_int()
+        _chk(self.L.cp_world_joint_count(self.w, ctypes.byref(n)), "cp_world_joint_count")
+        return n.value
+
+    def set_joint_state(self, state11):
+        s = _a(state11, np.float64)
+        assert s.shape == (self.joint_count(), 11)
+        _chk(self.L.cp_world_set_joint_state(self.w, s.ctypes.data), "cp_world_set_joint_state")
+
+    def joint_state(self):
+        s = np.zeros((self.joint_count(), 11))
+        _chk(self.L.cp_world_get_joint_state(self.w, s.ctypes.data), "cp_world_get_joint_state")
+        return s
+
+    def joint_rows(self, k):
+        a = [np.zeros(30) for _ in range(4)]
+        n = ctypes.c_int()
+        _chk(self.L.cp_world_get_joint_rows(self.w, int(k), *[x.ctypes.data for x in a], ctypes.byref(n)), "cp_world_get_joint_rows")
+        t = n.value
+        return a[0][:6 * t].reshape(t, 6), a[1][:6 * t].reshape(t, 6), a[2][:6 * t].reshape(6, t), a[3][:6 * t].reshape(6, t)
+
+    def set_holonomic_systems(self, off, joints):
+        off, joints = _a(off, np.int64), _a(joints, np.int64)
+        _chk(self.L.cp_world_set_holonomic_systems(self.w, len(off) - 1, off.ctypes.data, joints.ctypes.data),
+             "cp_world_set_holonomic_systems")
+
+    def holonomic_systems(self):
+        """Per system (og, order, table, vloc, factor buffer)."""
+        n = ctypes.c_int()
+        _chk(self.L.cp_world_holonomic_system_count(self.w, ctypes.byref(n)), "cp_world_holonomic_system_count")
+        out = []
+        for k in range(n.value):
+            sz = np.zeros(4, np.int64)
+            _chk(self.L.cp_world_get_holonomic_system(self.w, k, None, None, None, None, None, sz.ctypes.data), "cp_world_get_holonomic_system")
+            og, o, t, v = (np.zeros(sz[0], np.int64), np.zeros(sz[0], np.int64), np.zeros(sz[1], np.int64), np.zeros(sz[0], np.int64))
+            b = np.zeros(max(int(sz[3]), 1))
+            _chk(self.L.cp_world_get_holonomic_system(self.w, k, og.ctypes.data, o.ctypes.data, t.ctypes.data, v.ctypes.data, b.ctypes.data,
+                                                      sz.ctypes.data), "cp_world_get_holonomic_system")
+            out.append((og, o, t, v, b[:sz[3]]))
+        return out
 
     def last_timing(self):
         ms = np.zeros(8)

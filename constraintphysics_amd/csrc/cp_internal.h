@@ -65,6 +65,39 @@ struct Params {
 	int gyro_ticks;
 	int defect_fixes;
 	int precision;  // cp_precision (CP_PREC_FP64 or CP_PREC_SPLIT)
+	int hol_iters;  // holonomic block-solver iterations (first substep only)
+};
+
+// Joints (ball-socket n = 3, hinge n = 5), SoA over n joints; state arrays
+// are component-major ([5][n], [3][n]).
+struct Joints {
+	int n;
+	int *type, *a, *b, *sys;  // sys: holonomic system, -1 none
+	double *tp1, *tp2, *ax1, *ax2;  // [3][n] body-frame attach points and hinge axes
+	double *ps, *rs;                // correction strengths
+	double *imp, *pimp, *himp;      // [5][n]
+	double *u, *w;                  // [3][n] hinge basis of the current rows
+	double* row;                    // CP_JRS doubles per joint
+};
+#define CP_JRS 160
+enum JointRowField {
+	CP_J_JA = 0,      // J_a [5][6]
+	CP_J_JB = 30,     // J_b [5][6]
+	CP_J_ITA = 60,    // M_a^-1 J_a^T [6][5]
+	CP_J_ITB = 90,    // M_b^-1 J_b^T [6][5]
+	CP_J_INV = 120,   // (J M^-1 J^T (1+cfm))^-1 [n][n]
+	CP_J_TGT = 145,   // velocity targets [5]
+	CP_J_PTGT = 150   // pseudo-velocity targets [5]
+};
+
+// Holonomic systems: per system the elimination order (joint indices), the
+// block location table (lower triangle, column major, -1 = empty block),
+// vector offsets, and slices of the factor buffer and its column scratch.
+struct HSysDev {
+	int nsys;
+	int *nc, *ord_off, *order, *vloc, *tab_off, *table, *degree, *buf_off, *dbuf_off;
+	double *buf, *dbuf;
+	double* scratch;  // 3 * degree per system (at dbuf_off / 2): solve vectors
 };
 
 // Narrowphase output, one slot per candidate pair.
@@ -137,8 +170,9 @@ void launch_bp_gather(const Bodies& b, const int* sorted, double* sb, hipStream_
 void launch_bp_sweep(int n, const int* sorted, const double* sb, const int* scene_of_body, const int* scene_body_off, int* counts,
                      const int* offs, uint64_t* out, int write, hipStream_t s);
 // lists: 6 * npairs ints (per-class pair lists), counts: 8 ints.
+// nocol: sorted pair keys (lo << 32 | hi) that never collide (jointed bodies).
 void launch_narrowphase(const Bodies& b, const Shapes& sh, const Params& p, const uint64_t* pairs, int npairs, const Manifolds& m,
-                        int cap, int* lists, int* counts, hipStream_t s);
+                        int cap, int* lists, int* counts, const uint64_t* nocol, int n_nocol, hipStream_t s);
 void launch_np_gather(const Manifolds& M, int mcap, const uint64_t* pairs, int npairs, const int* off, const NewPts& np, int cap,
                       hipStream_t s);
 void launch_np_match(const NewPts& np, int nnew, int cap, const Contacts& C, int nold, const Bodies& b, const Params& P,
@@ -154,12 +188,21 @@ void launch_edge_build(const Contacts& C, int n, const int* head, const int* eid
 void launch_color(const Bodies& b, const int* scene_body_off, int nscenes, const uint64_t* ekey, const int* elive, int nedges,
                   int* ecolor, int* order, int* color_off, int* ncolors, int* edge_range, hipStream_t s);
 void launch_build_rows(const Bodies& b, const Contacts& C, int nc, const Params& P, hipStream_t s);
+// Joint rows: mode 0 all joints (holonomic ones skipped when first), 1 the
+// holonomic joints with warm-start re-projection, 2 the holonomic joints
+// without re-projection.
+void launch_joint_rows(const Bodies& b, const Joints& J, const Params& P, double sub_dt, int mode, int first, hipStream_t s);
+// Block LDL factor of every holonomic system.
+void launch_hsys_factor(const Joints& J, const HSysDev& H, double cfm, hipStream_t s);
 void launch_pgs_colored_lds(const Bodies& b, const Contacts& C, const int* scene_list, int nlist, int max_bodies, const int* scene_body_off,
                             const int* edge_range, const int* order, const int* color_off, const int* ncolors, const int* efirst,
                             const int* ecnt, const Params& P, int threads, hipStream_t s);
 void launch_rows_writeback(const Contacts& C, int nc, const Params& P, hipStream_t s);
 void launch_pgs_residual(const Bodies& b, const Contacts& C, int nc, const Params& P, double* acc, hipStream_t s);
-void launch_pgs_ordered(const Bodies& b, const Contacts& C, const int* rows, const int* isl_off, int nisl, const Params& P, int* applied,
+// rows: (index, kind) pairs, kind 0 contact normal, 1 friction, 2 joint;
+// isl_sys/isl_sys_off: holonomic systems per island; lin: [6][n] scratch.
+void launch_pgs_ordered(const Bodies& b, const Contacts& C, const Joints& J, const HSysDev& H, const int* rows, const int* isl_off,
+                        const int* isl_sys, const int* isl_sys_off, int nisl, const Params& P, int* applied, double* lin, int first,
                         hipStream_t s);
 void launch_pgs_colored(const Bodies& b, const Contacts& C, const int* scene_body_off, int nscenes, const int* edge_range, const int* order,
                         const int* color_off, const int* ncolors, const int* efirst, const int* ecnt, const Params& P, int threads,

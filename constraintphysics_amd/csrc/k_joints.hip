@@ -1,0 +1,310 @@
+// Joints (ball-socket, hinge) and the holonomic block solver on the device.
+//
+// Rows: BallSocket / Hinge::updateSolverConstraints (PersistentConstraint.cpp:
+// 225-231, :274-285) -> BallSocketConstraint / HingeConstraint (ConstraintSolver.cpp:
+// 382-405, :468-550); degree n = 3 or 5, NMat accumulation order (NMat.h), so
+// FP64 rows match the reference bit for bit.  One thread per joint.
+//
+// Holonomic systems (HolonomicBlockSolver.cpp): the symbolic part (elimination
+// order, block table) is host work done when the joint topology changes; the
+// numeric block LDL factor (:459-543) runs one thread per system, and the solve
+// d = L^-T D^-1 L^-1 (t - c) (:223-355) runs inside the island's PGS thread
+// after the sweeps of the first substep.
+//
+// Roofline: per joint per substep ~600 B of body state in, 160 doubles of rows
+// out; per system the factor touches sum(d_i^2 + d_i d_(i+1)) doubles (5-7 KB
+// per 16-link chain).  Latency bound at these sizes.
+#include "k_hsys.h"
+
+namespace cpx {
+
+__device__ __forceinline__ V3 jld3(const double* a, int i, int n) { return V3{a[i], a[n + i], a[2 * n + i]}; }
+__device__ __forceinline__ void jst3(double* a, int i, int n, V3 v) {
+	a[i] = v.x;
+	a[n + i] = v.y;
+	a[2 * n + i] = v.z;
+}
+
+__device__ __forceinline__ double pos_coeff(double s, double dt) {  // posCorrectCoeff (PersistentConstraint.cpp:199)
+	double a = s * dt, b = 1.0 / dt;
+	return (b < a) ? b : a;
+}
+
+// rowMajorOrderInverse: 1x1, the 3x3 closed form (NMat.cpp:24-36), otherwise
+// unpivoted Gauss-Jordan (NMat.h:121-147).
+__device__ void inverse_n(int n, const double* src, double* dst) {
+	if (n == 1) {
+		dst[0] = 1.0 / src[0];
+		return;
+	}
+	if (n == 3) {
+		double a = src[0], b = src[1], c = src[2], d = src[3], e = src[4], f = src[5], g = src[6], h = src[7], i = src[8];
+		double det = a * (e * i - f * h) - b * (d * i - f * g) + c * (d * h - e * g);
+		double di = 1.0 / det;
+		dst[0] = (e * i - f * h) * di; dst[1] = (c * h - b * i) * di; dst[2] = (b * f - c * e) * di;
+		dst[3] = (f * g - d * i) * di; dst[4] = (a * i - c * g) * di; dst[5] = (c * d - a * f) * di;
+		dst[6] = (d * h - e * g) * di; dst[7] = (b * g - a * h) * di; dst[8] = (a * e - b * d) * di;
+		return;
+	}
+	double cp[36], t[36];
+	for (int k = 0; k < n * n; k++) {
+		cp[k] = src[k];
+		t[k] = 0;
+	}
+	for (int k = 0; k < n; k++) t[k * (n + 1)] = 1.0;
+	for (int piv = 0; piv < n; piv++)
+		for (int r = 0; r < n; r++) {
+			if (piv == r) continue;
+			double ratio = cp[n * r + piv] / cp[n * piv + piv];
+			for (int c = 0; c < n; c++) {
+				cp[n * r + c] -= ratio * cp[n * piv + c];
+				t[n * r + c] -= ratio * t[n * piv + c];
+			}
+		}
+	for (int piv = 0; piv < n; piv++)
+		for (int c = 0; c < n; c++) t[n * piv + c] /= cp[n * piv + piv];
+	for (int k = 0; k < n * n; k++) dst[k] = t[k];
+}
+
+// Solver rows of joint k from the current body state (warm start with
+// coefficient 1.0; hinge angular impulses re-projected onto the new basis
+// when reproject, ConstraintSolver.cpp:475-476).
+__device__ void build_joint(const Bodies& b, Joints& J, int k, double sub_dt, double cfm, bool reproject) {
+	const int N = b.n, K = J.n;
+	const int n = J.type[k];
+	const int ia = J.a[k], ib = J.b[k];
+	double* R = J.row + (size_t)k * CP_JRS;
+	Q qa{b.q[ia], b.q[N + ia], b.q[2 * N + ia], b.q[3 * N + ia]};
+	Q qb{b.q[ib], b.q[N + ib], b.q[2 * N + ib], b.q[3 * N + ib]};
+	V3 ca = jld3(b.com, ia, N), cb = jld3(b.com, ib, N);
+	V3 pa = ca + q_rotate(qa, jld3(J.tp1, k, K));  // getTrackedP (RigidBody.cpp:139-144)
+	V3 pb = cb + q_rotate(qb, jld3(J.tp2, k, K));
+	bool da = b.movement[ia] == 0, db = b.movement[ib] == 0;
+	double Ma[6][6], Mb[6][6];
+	{
+		double ima = da ? 1.0 / b.mass[ia] : 0, imb = db ? 1.0 / b.mass[ib] : 0;
+		for (int r = 0; r < 6; r++) for (int c = 0; c < 6; c++) { Ma[r][c] = 0; Mb[r][c] = 0; }
+		for (int r = 0; r < 3; r++)
+			for (int c = 0; c < 3; c++) {
+				Ma[r][c] = (r == c ? 1.0 : 0.0) * ima;
+				Mb[r][c] = (r == c ? 1.0 : 0.0) * imb;
+				Ma[3 + r][3 + c] = da ? b.Iinv[(3 * r + c) * N + ia] : 0.0;
+				Mb[3 + r][3 + c] = db ? b.Iinv[(3 * r + c) * N + ib] : 0.0;
+			}
+	}
+	double start[5];
+	for (int c = 0; c < n; c++) start[c] = J.imp[c * K + k] * 1.0;
+	double ja[5][6], jb[5][6];
+	double pcc = pos_coeff(J.ps[k], sub_dt);
+	V3 rA, rB;
+	if (n == 3) {
+		rA = pa - ca;
+		rB = pb - cb;
+	} else {
+		rA = pa - ca;
+		rB = pa - cb;  // both arms from the b1 point (ConstraintSolver.cpp:469)
+	}
+	// crossMat (NMat.cpp:62-70); the -I and -crossMat blocks are negations
+	double sa[3][3] = {{0, -rA.z, rA.y}, {rA.z, 0, -rA.x}, {-rA.y, rA.x, 0}};
+	double sb[3][3] = {{0, -rB.z, rB.y}, {rB.z, 0, -rB.x}, {-rB.y, rB.x, 0}};
+	for (int r = 0; r < 3; r++)
+		for (int c = 0; c < 3; c++) {
+			ja[r][c] = (r == c) ? 1.0 : 0.0;
+			jb[r][c] = -((r == c) ? 1.0 : 0.0);
+			ja[r][3 + c] = -sa[r][c];
+			jb[r][3 + c] = sb[r][c];
+		}
+	if (n == 3) {
+		V3 err = pb - pa;
+		R[CP_J_PTGT] = pcc * err.x;
+		R[CP_J_PTGT + 1] = pcc * err.y;
+		R[CP_J_PTGT + 2] = pcc * err.z;
+		for (int c = 0; c < 3; c++) J.imp[c * K + k] = start[c];
+	} else {
+		V3 axa = q_rotate(qa, jld3(J.ax1, k, K)), axb = q_rotate(qb, jld3(J.ax2, k, K));
+		double rcc = pos_coeff(J.rs[k], sub_dt);
+		V3 su = jld3(J.u, k, K), sw = jld3(J.w, k, K);
+		V3 u, w;
+		perp_basis(axa, &u, &w);
+		V3 nn = axb;
+		for (int c = 0; c < 5; c++) J.imp[c * K + k] = start[c];
+		if (reproject) {
+			J.imp[3 * K + k] = dot(u, su) * start[3] + dot(u, sw) * start[4];
+			J.imp[4 * K + k] = dot(w, su) * start[3] + dot(w, sw) * start[4];
+		}
+		jst3(J.u, k, K, u);
+		jst3(J.w, k, K, w);
+		double us[3][3] = {{0, -u.z, u.y}, {u.z, 0, -u.x}, {-u.y, u.x, 0}};
+		double ws[3][3] = {{0, -w.z, w.y}, {w.z, 0, -w.x}, {-w.y, w.x, 0}};
+		double ns[3][3] = {{0, -nn.z, nn.y}, {nn.z, 0, -nn.x}, {-nn.y, nn.x, 0}};
+		double ud[3] = {u.x, u.y, u.z}, wd[3] = {w.x, w.y, w.z}, nd[3] = {nn.x, nn.y, nn.z};
+		for (int c = 0; c < 3; c++) {
+			double a3 = 0, b3 = 0, a4 = 0, b4 = 0;
+			for (int q = 0; q < 3; q++) {
+				a3 += nd[q] * us[q][c];
+				b3 += ud[q] * ns[q][c];
+				a4 += nd[q] * ws[q][c];
+				b4 += wd[q] * ns[q][c];
+			}
+			ja[3][c] = 0; jb[3][c] = 0; ja[4][c] = 0; jb[4][c] = 0;
+			ja[3][3 + c] = a3; jb[3][3 + c] = b3;
+			ja[4][3 + c] = a4; jb[4][3 + c] = b4;
+		}
+		V3 pc = (pb - pa) * pcc;
+		R[CP_J_PTGT] = pc.x;
+		R[CP_J_PTGT + 1] = pc.y;
+		R[CP_J_PTGT + 2] = pc.z;
+		R[CP_J_PTGT + 3] = dot(u, nn) * rcc;
+		R[CP_J_PTGT + 4] = dot(w, nn) * rcc;
+	}
+	double ita[6][5], itb[6][5];
+	for (int r = 0; r < 6; r++)
+		for (int c = 0; c < n; c++) {
+			double s1 = 0, s2 = 0;
+			for (int q = 0; q < 6; q++) { s1 += Ma[r][q] * ja[c][q]; s2 += Mb[r][q] * jb[c][q]; }
+			ita[r][c] = s1;
+			itb[r][c] = s2;
+		}
+	double itv[25], inv[25];
+	for (int r = 0; r < n; r++)
+		for (int c = 0; c < n; c++) {
+			double s1 = 0, s2 = 0;
+			for (int q = 0; q < 6; q++) { s1 += ja[r][q] * ita[q][c]; s2 += jb[r][q] * itb[q][c]; }
+			itv[r * n + c] = s1 + s2;
+		}
+	for (int r = 0; r < n; r++) itv[r * n + r] *= (1 + cfm);  // applyCFM (ConstraintSolver.cpp:919-926)
+	inverse_n(n, itv, inv);
+	for (int r = 0; r < n; r++)
+		for (int c = 0; c < 6; c++) {
+			R[CP_J_JA + r * 6 + c] = ja[r][c];
+			R[CP_J_JB + r * 6 + c] = jb[r][c];
+		}
+	for (int r = 0; r < 6; r++)
+		for (int c = 0; c < n; c++) {
+			R[CP_J_ITA + r * 5 + c] = ita[r][c];
+			R[CP_J_ITB + r * 5 + c] = itb[r][c];
+		}
+	for (int r = 0; r < n * n; r++) R[CP_J_INV + r] = inv[r];
+	for (int c = 0; c < n; c++) {
+		J.pimp[c * K + k] = 0;
+		J.himp[c * K + k] = 0;
+	}
+}
+
+// mode 0: every joint except holonomic ones at the first substep; 1: holonomic
+// joints with re-projection (end of step); 2: holonomic joints as stored
+// (state loaded, or systems just created).
+__global__ void k_joint_rows(Bodies b, Joints J, Params P, double sub_dt, int mode, int first) {
+	int k = blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= J.n) return;
+	bool in_sys = J.sys[k] >= 0 && P.hol_iters > 0;
+	if (mode == 0) {
+		if (first && in_sys) return;
+		build_joint(b, J, k, sub_dt, P.cfm, true);
+	} else {
+		if (!in_sys) return;
+		build_joint(b, J, k, sub_dt, P.cfm, mode == 1);
+	}
+}
+
+// ------------------------------------------------------------- holonomic factor
+__device__ __forceinline__ int hs_first_below(const HSysDev& H, int s, int col) {  // :616-622
+	for (int row = col + 1; row < H.nc[s]; row++) {
+		int l = hs_loc(H, s, row, col);
+		if (l != -1) return l;
+	}
+	return -1;
+}
+// computeBlockInitialValue (:799-825, :714-744), row joint c1, column joint c2.
+__device__ void hs_block_init(const Joints& J, double cfm, int c1, int c2, double* t) {
+	int n1 = J.type[c1], n2 = J.type[c2];
+	const double* R1 = J.row + (size_t)c1 * CP_JRS;
+	const double* R2 = J.row + (size_t)c2 * CP_JRS;
+	if (c1 == c2) {
+		for (int r = 0; r < n1; r++)
+			for (int c = 0; c < n1; c++) {
+				double s1 = 0, s2 = 0;
+				for (int q = 0; q < 6; q++) {
+					s1 += R1[CP_J_JA + r * 6 + q] * R1[CP_J_ITA + q * 5 + c];
+					s2 += R1[CP_J_JB + r * 6 + q] * R1[CP_J_ITB + q * 5 + c];
+				}
+				t[r * n1 + c] = s1 + s2;
+			}
+		for (int i = 0; i < n1; i++) t[i + n1 * i] *= (1 + cfm);
+		return;
+	}
+	int jac = -1, it = -1;
+	if (J.a[c1] == J.a[c2]) { jac = CP_J_JA; it = CP_J_ITA; }
+	else if (J.a[c1] == J.b[c2]) { jac = CP_J_JA; it = CP_J_ITB; }
+	else if (J.b[c1] == J.a[c2]) { jac = CP_J_JB; it = CP_J_ITA; }
+	else if (J.b[c1] == J.b[c2]) { jac = CP_J_JB; it = CP_J_ITB; }
+	if (jac < 0) {
+		for (int i = 0; i < n1 * n2; i++) t[i] = 0;
+		return;
+	}
+	for (int i = 0; i < n1 * n2; i++) {  // matMult (:701-712)
+		int row = i / n2, col = i % n2;
+		t[i] = 0;
+		for (int q = 0; q < 6; q++) t[i] += R1[jac + row * 6 + q] * R2[it + q * 5 + col];
+	}
+}
+
+// HolonomicSystem::computeInverse (:459-543), one thread per system.
+__global__ void k_hsys_factor(Joints J, HSysDev H, double cfm) {
+	int s = blockIdx.x * blockDim.x + threadIdx.x;
+	if (s >= H.nsys) return;
+	const int nc = H.nc[s];
+	const int* ord = H.order + H.ord_off[s];
+	double* buf = H.buf + H.buf_off[s];
+	double* dbuf = H.dbuf + H.dbuf_off[s];
+	for (int col = 0; col < nc; col++)
+		for (int row = col; row < nc; row++) {
+			int l = hs_loc(H, s, row, col);
+			if (l == -1) continue;
+			hs_block_init(J, cfm, ord[row], ord[col], buf + l);
+		}
+	for (int col = 0; col < nc; col++) {
+		double* diag = buf + hs_loc(H, s, col, col);
+		int bw = J.type[ord[col]];
+		double tmp[36];
+		inverse_n(bw, diag, tmp);
+		for (int q = 0; q < bw * bw; q++) diag[q] = tmp[q];
+		if (col + 1 == nc) continue;
+		int first = hs_first_below(H, s, col);
+		for (int row = col + 1; row < nc; row++) {
+			int l = hs_loc(H, s, row, col);
+			if (l == -1) continue;
+			double* tgt = dbuf + (l - first);
+			int bh = J.type[ord[row]];
+			const double* src = buf + l;
+			for (int r = 0; r < bh; r++)
+				for (int c = 0; c < bw; c++) tgt[c + bw * r] = dotp(bw, src + bw * r, diag + bw * c);
+		}
+		for (int ldc = col + 1; ldc < nc; ldc++)
+			for (int ldr = ldc; ldr < nc; ldr++) {
+				int l = hs_loc(H, s, ldr, col);
+				int lc = hs_loc(H, s, ldc, col);
+				if (l == -1 || lc == -1) continue;
+				const double* src = buf + l;
+				const double* lsrc = dbuf + (lc - first);
+				double* tgt = buf + hs_loc(H, s, ldr, ldc);
+				int w2 = J.type[ord[ldc]], h2 = J.type[ord[ldr]];
+				for (int r = 0; r < h2; r++)
+					for (int c = 0; c < w2; c++) tgt[r * w2 + c] -= dotp(bw, src + r * bw, lsrc + c * bw);
+			}
+		if (first == -1) continue;
+		int total = hs_loc(H, s, col + 1, col + 1) - first;
+		for (int q = 0; q < total; q++) buf[first + q] = dbuf[q];
+	}
+}
+
+static inline int nblk(int n, int t) { return (n + t - 1) / t; }
+
+void launch_joint_rows(const Bodies& b, const Joints& J, const Params& P, double sub_dt, int mode, int first, hipStream_t s) {
+	if (J.n) hipLaunchKernelGGL(k_joint_rows, dim3(nblk(J.n, 64)), dim3(64), 0, s, b, J, P, sub_dt, mode, first);
+}
+void launch_hsys_factor(const Joints& J, const HSysDev& H, double cfm, hipStream_t s) {
+	if (H.nsys) hipLaunchKernelGGL(k_hsys_factor, dim3(nblk(H.nsys, 64)), dim3(64), 0, s, J, H, cfm);
+}
+
+}  // namespace cpx

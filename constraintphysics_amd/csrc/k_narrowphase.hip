@@ -899,7 +899,18 @@ __global__ void k_np_zero(int* counts) {
 // test (PhysicsEngine.cpp:174: at least one dynamic body) to their class list;
 // one atomic per class per wave.  List order does not matter: every slot's
 // result depends on its pair only.
-__global__ void k_np_classify(Bodies b, Shapes sh, const uint64_t* pairs, int npairs, Manifolds M, int* lists, int* counts) {
+__device__ __forceinline__ bool sorted_has(const uint64_t* v, int n, uint64_t x) {
+	int lo = 0, hi = n;
+	while (lo < hi) {
+		int mid = (lo + hi) >> 1;
+		if (v[mid] < x) lo = mid + 1;
+		else hi = mid;
+	}
+	return lo < n && v[lo] == x;
+}
+
+__global__ void k_np_classify(Bodies b, Shapes sh, const uint64_t* pairs, int npairs, Manifolds M, int* lists, int* counts,
+                              const uint64_t* nocol, int n_nocol) {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
 	int cls = -1;
 	if (i < npairs) {
@@ -907,7 +918,9 @@ __global__ void k_np_classify(Bodies b, Shapes sh, const uint64_t* pairs, int np
 		M.maxpen[i] = -1;
 		uint64_t key = pairs[i];
 		int ia = (int)(key >> 32), ib = (int)(key & 0xFFFFFFFFu);
-		if (!(b.movement[ia] != 0 && b.movement[ib] != 0)) {
+		// collisionAllowed (PhysicsEngine.cpp:570-572): jointed pairs never collide
+		bool allowed = n_nocol == 0 || !sorted_has(nocol, n_nocol, key);
+		if (allowed && !(b.movement[ia] != 0 && b.movement[ib] != 0)) {
 			int ta = sh.type[b.shape[ia]], tb = sh.type[b.shape[ib]];
 			int lo = ta < tb ? ta : tb, hi = ta < tb ? tb : ta;
 			if (lo == SHAPE_HULL) cls = hi == SHAPE_HULL ? NPC_HH : hi == SHAPE_SPHERE ? NPC_HS : NPC_HC;
@@ -1169,10 +1182,10 @@ __global__ void k_np_sphere_pairs(Bodies b, Shapes sh, Params P, const uint64_t*
 }
 
 void launch_narrowphase(const Bodies& b, const Shapes& sh, const Params& p, const uint64_t* pairs, int npairs, const Manifolds& m,
-                        int cap, int* lists, int* counts, hipStream_t s) {
+                        int cap, int* lists, int* counts, const uint64_t* nocol, int n_nocol, hipStream_t s) {
 	if (npairs <= 0) return;
 	hipLaunchKernelGGL(k_np_zero, dim3(1), dim3(64), 0, s, counts);
-	hipLaunchKernelGGL(k_np_classify, dim3((npairs + 255) / 256), dim3(256), 0, s, b, sh, pairs, npairs, m, lists, counts);
+	hipLaunchKernelGGL(k_np_classify, dim3((npairs + 255) / 256), dim3(256), 0, s, b, sh, pairs, npairs, m, lists, counts, nocol, n_nocol);
 	int wblocks = (npairs + NP_WAVES - 1) / NP_WAVES;
 	if (wblocks > 4096) wblocks = 4096;
 	const int* L = lists;

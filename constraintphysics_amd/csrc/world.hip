@@ -14,10 +14,12 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <map>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -169,6 +171,38 @@ struct cp_world {
 	hipEvent_t pgs_ev[2 * 64];
 	double last_ms[8] = {0};
 	bool timing = true;
+	// joints (host copies of static data; device SoA)
+	std::vector<int> j_type, j_a, j_b;
+	std::vector<double> j_geo;  // 14 per joint: tp1, tp2, ax1, ax2, ps, rs
+	int j_dev_n = 0;            // joints on the device
+	bool joints_dirty = false;  // static joint data or systems changed since the upload
+	DArr<int> dj_type, dj_a, dj_b, dj_sys;
+	DArr<double> dj_tp1, dj_tp2, dj_ax1, dj_ax2, dj_ps, dj_rs, dj_imp, dj_pimp, dj_himp, dj_u, dj_w, dj_row, lin;
+	DArr<uint64_t> nocol;
+	int n_nocol = 0;
+	// Joint edges (body pairs with joints) of the constraint graph: n1 is b1 of
+	// the pair's first joint (getOrCreateEdgeTo, PhysicsEngine.cpp:1958-1971).
+	struct JEdge {
+		int n1, n2;
+		std::vector<int> joints;  // creation order
+		bool scan = true;         // holonomic_system_scan_needed
+		int sys = -1;
+	};
+	std::map<uint64_t, JEdge> jedges;
+	std::vector<std::vector<uint64_t>> sys_edges;  // member edges per system
+	bool hs_explicit = false;  // systems set by the caller: no discovery
+	bool hs_changed = false;   // a joint joined an edge already in a system
+	// holonomic systems: constraint order (og), then the symbolic layout
+	std::vector<std::vector<int>> hs_og;
+	struct HLayout {
+		std::vector<int> order, table, vloc;
+		int degree = 0, cap = 0;
+	};
+	std::vector<HLayout> hs_lay;
+	DArr<int> dh_nc, dh_ord_off, dh_order, dh_vloc, dh_tab_off, dh_table, dh_degree, dh_buf_off, dh_dbuf_off, isl_sys, isl_sys_off;
+	DArr<double> dh_buf, dh_dbuf, dh_scratch;
+	bool hs_pending = true;
+	std::vector<int> last_isl_sys, last_isl_sys_off;
 
 	Bodies bodies() {
 		Bodies b{};
@@ -220,9 +254,32 @@ struct cp_world {
 		p.gyro_ticks = cfg.gyro_ticks;
 		p.defect_fixes = cfg.defect_fixes;
 		p.precision = cfg.pgs_mode == CP_PGS_REFERENCE_ORDER ? CP_PREC_FP64 : cfg.precision;
+		p.hol_iters = cfg.hol_iters;
 		return p;
 	}
 
+	Joints joints() {
+		Joints J{};
+		J.n = (int)j_type.size();
+		J.type = dj_type.p; J.a = dj_a.p; J.b = dj_b.p; J.sys = dj_sys.p;
+		J.tp1 = dj_tp1.p; J.tp2 = dj_tp2.p; J.ax1 = dj_ax1.p; J.ax2 = dj_ax2.p; J.ps = dj_ps.p; J.rs = dj_rs.p;
+		J.imp = dj_imp.p; J.pimp = dj_pimp.p; J.himp = dj_himp.p; J.u = dj_u.p; J.w = dj_w.p; J.row = dj_row.p;
+		return J;
+	}
+	HSysDev hsys() {
+		HSysDev H{};
+		H.nsys = (int)hs_lay.size();
+		H.nc = dh_nc.p; H.ord_off = dh_ord_off.p; H.order = dh_order.p; H.vloc = dh_vloc.p; H.tab_off = dh_tab_off.p; H.table = dh_table.p;
+		H.degree = dh_degree.p; H.buf_off = dh_buf_off.p; H.dbuf_off = dh_dbuf_off.p; H.buf = dh_buf.p; H.dbuf = dh_dbuf.p;
+		H.scratch = dh_scratch.p;
+		return H;
+	}
+	void upload_joints();
+	void read_joint_state(double* s, int n);
+	void write_joint_state(const double* s);
+	void layout_systems();
+	void discover_systems();
+	std::vector<int> root_order();
 	void free_all();
 	void upload();
 	void ensure_manifolds(int n);
@@ -252,6 +309,14 @@ void cp_world::free_all() {
 	resid_acc.free_();
 	DArr<uint64_t>* u64s[] = {&bp_keys, &bp_keys2, &pairs, &pairs2, &nk, &e_key};
 	for (auto* a : u64s) a->free_();
+	DArr<int>* jints[] = {&dj_type, &dj_a, &dj_b, &dj_sys, &dh_nc, &dh_ord_off, &dh_order, &dh_vloc, &dh_tab_off, &dh_table, &dh_degree,
+	                      &dh_buf_off, &dh_dbuf_off, &isl_sys, &isl_sys_off};
+	for (auto* a : jints) a->free_();
+	DArr<double>* jdbl[] = {&dj_tp1, &dj_tp2, &dj_ax1, &dj_ax2, &dj_ps, &dj_rs, &dj_imp, &dj_pimp, &dj_himp, &dj_u, &dj_w, &dj_row, &lin,
+	                        &dh_buf, &dh_dbuf, &dh_scratch};
+	for (auto* a : jdbl) a->free_();
+	nocol.free_();
+	j_dev_n = 0;
 	tmp.free_();
 	for (void* a : man_allocs) (void)hipFree(a);
 	man_allocs.clear();
@@ -335,6 +400,254 @@ void cp_world::upload() {
 	color_mask.reserve(4 * (size_t)n);
 	applied.reserve(n);
 	uploaded = true;
+}
+
+// ---------------------------------------------------------------- joints
+// Joint state per joint: 5 accumulated impulses (unused components 0), then
+// the hinge basis u, w of the current rows.
+void cp_world::read_joint_state(double* s, int n) {
+	int K = j_dev_n;
+	std::vector<double> imp, u, w;
+	down(imp, dj_imp.p, 5 * (size_t)K);
+	down(u, dj_u.p, 3 * (size_t)K);
+	down(w, dj_w.p, 3 * (size_t)K);
+	for (int k = 0; k < n && k < K; k++) {
+		double* r = s + 11 * (size_t)k;
+		for (int c = 0; c < 5; c++) r[c] = c < j_type[k] ? imp[c * (size_t)K + k] : 0.0;
+		for (int c = 0; c < 3; c++) {
+			r[5 + c] = u[c * (size_t)K + k];
+			r[8 + c] = w[c * (size_t)K + k];
+		}
+	}
+}
+void cp_world::write_joint_state(const double* s) {
+	int K = j_dev_n;
+	std::vector<double> imp(5 * (size_t)K), u(3 * (size_t)K), w(3 * (size_t)K), zero(5 * (size_t)K, 0.0);
+	for (int k = 0; k < K; k++) {
+		const double* r = s + 11 * (size_t)k;
+		for (int c = 0; c < 5; c++) imp[c * (size_t)K + k] = r[c];
+		for (int c = 0; c < 3; c++) {
+			u[c * (size_t)K + k] = r[5 + c];
+			w[c * (size_t)K + k] = r[8 + c];
+		}
+	}
+	up(dj_imp, imp); up(dj_u, u); up(dj_w, w); up(dj_pimp, zero); up(dj_himp, zero);
+	hs_pending = true;  // rows and factors are rebuilt from this state at the next step
+}
+
+// Static joint data, system membership and the no-collision pair list; the
+// accumulated state of joints already on the device is kept.
+void cp_world::upload_joints() {
+	int K = (int)j_type.size();
+	std::vector<double> st(11 * (size_t)K, 0.0);
+	if (j_dev_n > 0) read_joint_state(st.data(), j_dev_n);
+	std::vector<int> sys(K, -1);
+	for (size_t q = 0; q < hs_og.size(); q++)
+		for (int j : hs_og[q]) sys[j] = (int)q;
+	up(dj_type, j_type); up(dj_a, j_a); up(dj_b, j_b); up(dj_sys, sys);
+	std::vector<double> g[4], ps(K), rs(K);
+	for (auto& v : g) v.resize(3 * (size_t)K);
+	for (int k = 0; k < K; k++) {
+		const double* r = &j_geo[14 * (size_t)k];
+		for (int v = 0; v < 4; v++)
+			for (int c = 0; c < 3; c++) g[v][c * (size_t)K + k] = r[3 * v + c];
+		ps[k] = r[12];
+		rs[k] = r[13];
+	}
+	up(dj_tp1, g[0]); up(dj_tp2, g[1]); up(dj_ax1, g[2]); up(dj_ax2, g[3]); up(dj_ps, ps); up(dj_rs, rs);
+	dj_row.reserve((size_t)CP_JRS * std::max(K, 1));
+	j_dev_n = K;
+	write_joint_state(st.data());
+	// disallowCollision pairs of jointed bodies (PhysicsEngine.cpp:570-572), sorted
+	std::vector<uint64_t> nc;
+	for (int k = 0; k < K; k++) {
+		uint64_t lo = (uint64_t)std::min(j_a[k], j_b[k]), hi = (uint64_t)std::max(j_a[k], j_b[k]);
+		nc.push_back(lo << 32 | hi);
+	}
+	std::sort(nc.begin(), nc.end());
+	nc.erase(std::unique(nc.begin(), nc.end()), nc.end());
+	up(nocol, nc);
+	n_nocol = (int)nc.size();
+	joints_dirty = false;
+}
+
+// HolonomicSystem::HolonomicSystem (HolonomicBlockSolver.cpp:11-132): the
+// elimination order that greedily adds the fewest fill-in blocks (ties to the
+// first remaining constraint; a zero-fill candidate ends the search), then
+// the block-lower-triangular layout of the factor in column-major order.
+void cp_world::layout_systems() {
+	hs_lay.assign(hs_og.size(), HLayout{});
+	std::vector<int> nc, ord_off{0}, order, vloc, tab_off{0}, table, degree, buf_off{0}, dbuf_off{0};
+	for (size_t q = 0; q < hs_og.size(); q++) {
+		const std::vector<int>& og = hs_og[q];
+		HLayout& L = hs_lay[q];
+		const int n = (int)og.size();
+		auto share = [&](int x, int y) {
+			int ax = j_a[og[x]], bx = j_b[og[x]], ay = j_a[og[y]], by = j_b[og[y]];
+			return ax == ay || ax == by || bx == ay || bx == by;
+		};
+		std::vector<char> adj((size_t)n * n), filled((size_t)n * n);  // elimination graph; non-empty blocks
+		for (int x = 0; x < n; x++)
+			for (int y = 0; y < n; y++) adj[(size_t)x * n + y] = filled[(size_t)x * n + y] = share(x, y);
+		std::vector<int> remaining(n), pos;
+		for (int x = 0; x < n; x++) remaining[x] = x;
+		while (!remaining.empty()) {
+			int best = 0, best_fill = INT_MAX;
+			std::vector<int> best_nb;
+			for (int i = 0; i < (int)remaining.size(); i++) {
+				int v = remaining[i];
+				std::vector<int> nb;
+				for (int y = 0; y < n; y++)
+					if (y != v && adj[(size_t)v * n + y]) nb.push_back(y);
+				int fill = 0;
+				for (size_t x = 0; x < nb.size(); x++)
+					for (size_t y = x + 1; y < nb.size(); y++) fill += !adj[(size_t)nb[x] * n + nb[y]];
+				if (fill < best_fill) {
+					best_fill = fill;
+					best = i;
+					best_nb.swap(nb);
+					if (fill == 0) break;
+				}
+			}
+			int v = remaining[best];
+			pos.push_back(v);
+			remaining.erase(remaining.begin() + best);
+			for (int y = 0; y < n; y++) adj[(size_t)v * n + y] = adj[(size_t)y * n + v] = 0;
+			for (size_t x = 0; x < best_nb.size(); x++)
+				for (size_t y = x + 1; y < best_nb.size(); y++) {
+					int a = best_nb[x], c = best_nb[y];
+					adj[(size_t)a * n + c] = adj[(size_t)c * n + a] = 1;
+					filled[(size_t)a * n + c] = filled[(size_t)c * n + a] = 1;
+				}
+		}
+		for (int x : pos) L.order.push_back(og[x]);
+		for (int col = 0; col < n; col++) {
+			L.vloc.push_back(L.degree);
+			L.degree += j_type[L.order[col]];
+			for (int row = col; row < n; row++) {
+				if (!filled[(size_t)pos[row] * n + pos[col]]) L.table.push_back(-1);
+				else {
+					L.table.push_back(L.cap);
+					L.cap += j_type[L.order[row]] * j_type[L.order[col]];
+				}
+			}
+		}
+		nc.push_back(n);
+		order.insert(order.end(), L.order.begin(), L.order.end());
+		vloc.insert(vloc.end(), L.vloc.begin(), L.vloc.end());
+		table.insert(table.end(), L.table.begin(), L.table.end());
+		degree.push_back(L.degree);
+		ord_off.push_back((int)order.size());
+		tab_off.push_back((int)table.size());
+		buf_off.push_back(buf_off.back() + L.cap);
+		dbuf_off.push_back(dbuf_off.back() + 6 * L.degree);
+	}
+	up(dh_nc, nc); up(dh_ord_off, ord_off); up(dh_order, order); up(dh_vloc, vloc); up(dh_tab_off, tab_off); up(dh_table, table);
+	up(dh_degree, degree); up(dh_buf_off, buf_off); up(dh_dbuf_off, dbuf_off);
+	dh_buf.reserve(std::max(buf_off.back(), 1));
+	dh_dbuf.reserve(std::max(dbuf_off.back(), 1));
+	dh_scratch.reserve(std::max(dbuf_off.back() / 2, 1));
+	HIPCHK(hipMemset(dh_buf.p, 0, std::max(buf_off.back(), 1) * sizeof(double)));
+	hs_pending = true;
+	joints_dirty = true;  // system membership changed
+}
+
+// Body indices in the iteration order of the engine's node map (given by the
+// caller, else libstdc++'s order for ids 1..n).
+std::vector<int> cp_world::root_order() {
+	std::vector<int> roots;
+	if ((int)node_order.size() == nb)
+		for (int64_t id : node_order) roots.push_back((int)id - 1);
+	else {
+		std::vector<int64_t> ord(nb);
+		cp_reference_node_order(nb, ord.data());
+		for (int64_t id : ord) roots.push_back((int)id - 1);
+	}
+	return roots;
+}
+
+// maintainHolonomicSystems (PhysicsEngine.cpp:1292-1307) for joints added
+// since the last step: in node-map order, each node's edges by the other
+// body's id; an edge whose scan is pending collects the edges connected to it
+// through holonomic joints (getAllEdgesConnectedHolonomically, :1746-1779:
+// depth first, through dynamic bodies only) and, when there are at least two,
+// founds a system or merges the systems met (:1466-1512).  Where the
+// reference keeps the system that sorts first by address, the lowest system
+// index is kept.  A system's constraint order is its member edges' joints
+// (revaluateSystem, :1785-1798).
+void cp_world::discover_systems() {
+	std::vector<std::vector<std::pair<int, uint64_t>>> adj(nb);
+	for (auto& kv : jedges) {
+		adj[kv.second.n1].push_back({kv.second.n2, kv.first});
+		adj[kv.second.n2].push_back({kv.second.n1, kv.first});
+	}
+	for (auto& a : adj) std::sort(a.begin(), a.end());
+	bool changed = hs_changed;
+	for (int node : root_order()) {
+		for (auto& ne : adj[node]) {
+			JEdge& e = jedges[ne.second];
+			if (!e.scan) continue;
+			e.scan = false;
+			std::vector<uint64_t> out, st{ne.second};
+			std::vector<uint64_t> visited;
+			auto seen = [&](uint64_t k) { return std::find(visited.begin(), visited.end(), k) != visited.end(); };
+			while (!st.empty()) {
+				uint64_t k = st.back();
+				st.pop_back();
+				if (seen(k)) continue;
+				visited.push_back(k);
+				out.push_back(k);
+				const JEdge& v = jedges[k];
+				std::vector<uint64_t> nbrs;
+				if (b_move[v.n1] == 0)
+					for (auto& x : adj[v.n1]) nbrs.push_back(x.second);
+				if (b_move[v.n2] == 0)
+					for (auto& x : adj[v.n2]) nbrs.push_back(x.second);
+				for (uint64_t x : nbrs)
+					if (!seen(x)) st.push_back(x);
+			}
+			if (out.size() <= 1) continue;
+			int rep = INT_MAX;
+			bool any_null = false;
+			std::vector<int> met;
+			for (uint64_t k : out) {
+				int q = jedges[k].sys;
+				if (q < 0) any_null = true;
+				else {
+					rep = std::min(rep, q);
+					if (std::find(met.begin(), met.end(), q) == met.end()) met.push_back(q);
+				}
+			}
+			if (met.size() == 1 && !any_null) continue;  // already one system, nothing new
+			if (met.empty()) {
+				rep = (int)sys_edges.size();
+				sys_edges.emplace_back();
+			}
+			for (uint64_t k : out) {
+				JEdge& x = jedges[k];
+				if (x.sys != rep) sys_edges[rep].push_back(k);
+				x.scan = false;
+			}
+			for (uint64_t k : out) jedges[k].sys = rep;
+			// merged systems are dropped; renumber the rest
+			std::vector<std::vector<uint64_t>> keep;
+			for (int q = 0; q < (int)sys_edges.size(); q++)
+				if (q == rep || std::find(met.begin(), met.end(), q) == met.end()) keep.push_back(sys_edges[q]);
+			sys_edges.swap(keep);
+			for (int q = 0; q < (int)sys_edges.size(); q++)
+				for (uint64_t k : sys_edges[q]) jedges[k].sys = q;
+			changed = true;
+		}
+	}
+	hs_changed = false;
+	if (!changed) return;
+	hs_og.clear();
+	for (auto& edges : sys_edges) {
+		hs_og.emplace_back();
+		for (uint64_t k : edges)
+			for (int j : jedges[k].joints) hs_og.back().push_back(j);
+	}
+	layout_systems();
 }
 
 void cp_world::scan(const int* in, int* out, int n) {
@@ -422,7 +735,7 @@ int cp_world::narrowphase(int npairs) {
 	ensure_manifolds(npairs);
 	np_lists.reserve(6 * (size_t)std::max(npairs, 1));
 	np_counts.reserve(8);
-	launch_narrowphase(bodies(), shapes(), params(), pairs.p, npairs, man, mcap, np_lists.p, np_counts.p, stream);
+	launch_narrowphase(bodies(), shapes(), params(), pairs.p, npairs, man, mcap, np_lists.p, np_counts.p, nocol.p, n_nocol, stream);
 	return npairs;
 }
 
@@ -478,34 +791,58 @@ void cp_world::edges() {
 
 // Island search of PhysicsEngine::sleepOrSolveIslands (PhysicsEngine.cpp:1536-1613)
 // with bfsVisitAll (:1263-1283), on the host over the post-narrowphase cache.
+// Edges are body pairs with contacts or joints; a node's edges are ordered by
+// the other body's id (ConstraintGraphNode::insertNewEdge, :1984-1995); an
+// edge's rows are its joints in creation order, then its live contacts.  Each
+// island also lists the holonomic systems its joints belong to, in the order
+// they are first met.
 void cp_world::schedule_reference_order() {
 	int n = ncontacts;
 	std::vector<uint64_t> key;
 	std::vector<int> live;
 	down(key, cb[cur].c.key, n);
 	down(live, cb[cur].c.live, n);
-	std::vector<std::vector<std::pair<int, int>>> adj(nb);  // (other, first contact index), by other id
+	struct Edge {
+		int first = -1;
+		std::vector<int> joints;
+	};
+	std::vector<Edge> edges;
+	std::unordered_map<uint64_t, int> eid;
+	auto edge_of = [&](uint64_t k) {
+		auto it = eid.find(k);
+		if (it != eid.end()) return it->second;
+		eid.emplace(k, (int)edges.size());
+		edges.emplace_back();
+		return (int)edges.size() - 1;
+	};
 	for (int k = 0; k < n; k++) {
 		if (k > 0 && key[k] == key[k - 1]) continue;
-		int lo = (int)(key[k] >> 32), hi = (int)(key[k] & 0xFFFFFFFFu);
-		adj[lo].push_back({hi, k});
-		adj[hi].push_back({lo, k});
+		edges[edge_of(key[k])].first = k;
+	}
+	for (int j = 0; j < (int)j_type.size(); j++) {
+		uint64_t lo = (uint64_t)std::min(j_a[j], j_b[j]), hi = (uint64_t)std::max(j_a[j], j_b[j]);
+		edges[edge_of(lo << 32 | hi)].joints.push_back(j);
+	}
+	std::vector<std::vector<std::pair<int, int>>> adj(nb);  // (other, edge), by other id
+	for (auto& kv : eid) {
+		int lo = (int)(kv.first >> 32), hi = (int)(kv.first & 0xFFFFFFFFu);
+		adj[lo].push_back({hi, kv.second});
+		adj[hi].push_back({lo, kv.second});
 	}
 	for (auto& a : adj) std::sort(a.begin(), a.end());
+	std::vector<int> sys_of(j_type.size(), -1);
+	if (cfg.hol_iters > 0)
+		for (size_t q = 0; q < hs_og.size(); q++)
+			for (int j : hs_og[q]) sys_of[j] = (int)q;
 	std::vector<char> visited(nb, 0);
-	std::vector<int> roots;
-	if ((int)node_order.size() == nb)
-		for (int64_t id : node_order) roots.push_back((int)id - 1);
-	else {
-		std::vector<int64_t> ord(nb);
-		cp_reference_node_order(nb, ord.data());
-		for (int64_t id : ord) roots.push_back((int)id - 1);
-	}
+	std::vector<int> roots = root_order();
 	last_rows.clear();
 	last_offs.assign(1, 0);
+	last_isl_sys.clear();
+	last_isl_sys_off.assign(1, 0);
 	for (int root : roots) {
 		if (b_move[root] != 0 || visited[root]) continue;
-		size_t start = last_rows.size();
+		size_t start = last_rows.size(), sys_start = last_isl_sys.size();
 		std::vector<int> st{root};
 		while (!st.empty()) {
 			int c = st.back();
@@ -515,7 +852,15 @@ void cp_world::schedule_reference_order() {
 			if (b_move[c] == 0) {
 				for (auto& e : adj[c]) {
 					if (visited[e.first]) continue;
-					for (int k = e.second; k < n && key[k] == key[e.second]; k++) {
+					const Edge& E = edges[e.second];
+					for (int j : E.joints) {
+						int q = sys_of[j];
+						if (q >= 0 && std::find(last_isl_sys.begin() + sys_start, last_isl_sys.end(), q) == last_isl_sys.end())
+							last_isl_sys.push_back(q);
+						last_rows.push_back(j); last_rows.push_back(2);
+					}
+					if (E.first < 0) continue;
+					for (int k = E.first; k < n && key[k] == key[E.first]; k++) {
 						if (!live[k]) continue;
 						last_rows.push_back(k); last_rows.push_back(0);
 						last_rows.push_back(k); last_rows.push_back(1);
@@ -525,11 +870,18 @@ void cp_world::schedule_reference_order() {
 			for (auto& e : adj[c])
 				if (b_move[e.first] == 0 && !visited[e.first]) st.push_back(e.first);
 		}
-		if (last_rows.size() > start) last_offs.push_back((int64_t)last_rows.size() / 2);
+		if (last_rows.size() > start) {
+			last_offs.push_back((int64_t)last_rows.size() / 2);
+			last_isl_sys_off.push_back((int)last_isl_sys.size());
+		} else {
+			last_isl_sys.resize(sys_start);
+		}
 	}
 	std::vector<int> rows(last_rows.begin(), last_rows.end()), offs(last_offs.begin(), last_offs.end());
 	up(isl_rows, rows);
 	up(isl_off, offs);
+	up(isl_sys, last_isl_sys);
+	up(isl_sys_off, last_isl_sys_off);
 }
 
 // The colour schedule as one sequential row list per scene (colours in order,
@@ -570,6 +922,10 @@ void cp_world::launch_pgs_colored_lds_global(const Bodies& b, const Params& P) {
 }
 
 void cp_world::step() {
+	if (!j_type.empty() && cfg.pgs_mode != CP_PGS_REFERENCE_ORDER) throw ApiErr{CP_ERR_STATE};  // joints: sequential order only
+	if (!hs_explicit && cfg.hol_iters > 0) discover_systems();
+	if (joints_dirty) upload_joints();
+	if (!j_type.empty()) lin.reserve(6 * (size_t)nb);
 	Params P = params();
 	Bodies b = bodies();
 	if (timing) HIPCHK(hipEventRecord(ev[0], stream));
@@ -581,6 +937,16 @@ void cp_world::step() {
 	persist(npairs);
 	edges();
 	bool ordered = cfg.pgs_mode == CP_PGS_REFERENCE_ORDER;
+	Joints J = joints();
+	HSysDev H = hsys();
+	const bool hol = cfg.hol_iters > 0 && H.nsys > 0;
+	// holonomic systems created or loaded from a state: rows from the current
+	// state with the stored impulses, then the factor (PhysicsEngine.cpp:1507-1522)
+	if (hol && hs_pending) {
+		launch_joint_rows(b, J, P, P.sub_dt, 2, 0, stream);
+		launch_hsys_factor(J, H, cfg.cfm, stream);
+		hs_pending = false;
+	}
 	int nisl = 0;
 	if (ordered) {
 		schedule_reference_order();
@@ -597,12 +963,16 @@ void cp_world::step() {
 	int npgs = 0;
 	for (int s = 0; s < cfg.substeps; s++) {
 		launch_build_rows(b, cb[cur].c, ncontacts, P, stream);
+		// joints; holonomic ones are current at the first substep (PhysicsEngine.cpp:1396-1403)
+		launch_joint_rows(b, J, P, P.sub_dt, 0, s == 0, stream);
 		bool tev = timing && npgs < 64;
 		if (tev) HIPCHK(hipEventRecord(pgs_ev[2 * npgs], stream));
 		if (ordered) {
 			HIPCHK(hipMemsetAsync(dv.p, 0, 12 * (size_t)nb * sizeof(double), stream));
 			HIPCHK(hipMemsetAsync(applied.p, 0, (size_t)nb * sizeof(int), stream));
-			launch_pgs_ordered(b, cb[cur].c, isl_rows.p, isl_off.p, nisl, P, applied.p, stream);
+			if (hol && s == 0) HIPCHK(hipMemsetAsync(lin.p, 0, 6 * (size_t)nb * sizeof(double), stream));
+			launch_pgs_ordered(b, cb[cur].c, J, H, isl_rows.p, isl_off.p, isl_sys.p, isl_sys_off.p, nisl, P, applied.p, lin.p, s == 0,
+			                   stream);
 		} else {
 			static const int thr[3] = {64, 128, 256};
 			for (int c = 0; c < 3; c++)
@@ -626,6 +996,12 @@ void cp_world::step() {
 	}
 	if (timing) HIPCHK(hipEventRecord(ev[4], stream));
 	launch_refresh_geometry(b, shapes(), 1, stream);
+	// calculateHolonomicSystemInversesAsync (PhysicsEngine.cpp:1615-1648): the
+	// rows and factors the next step's first substep uses
+	if (hol) {
+		launch_joint_rows(b, J, P, P.sub_dt, 1, 0, stream);
+		launch_hsys_factor(J, H, cfg.cfm, stream);
+	}
 	if (cfg.diagnostics & 1) {
 		double h[4];
 		HIPCHK(hipMemcpy(h, resid_acc.p, 3 * sizeof(double), hipMemcpyDeviceToHost));
@@ -1106,6 +1482,122 @@ int cp_world_set_node_order(cp_world* w, int n, const int64_t* ids) {
 		if (ids[k] < 1 || ids[k] > w->nb) return CP_ERR_INVALID;
 	w->node_order.assign(ids, ids + n);
 	return CP_OK;
+}
+
+int cp_joint_add(cp_world* w, int type, int b1, int b2, const double* geo14, int* joint_index) {
+	if (!w || !geo14 || !joint_index || (type != 3 && type != 5)) return CP_ERR_INVALID;
+	if (b1 < 0 || b2 < 0 || b1 >= w->nb || b2 >= w->nb || b1 == b2 || w->b_scene[b1] != w->b_scene[b2]) return CP_ERR_INVALID;
+	for (int c = 0; c < 14; c++)
+		if (!std::isfinite(geo14[c])) return CP_ERR_INVALID;
+	int k = (int)w->j_type.size();
+	w->j_type.push_back(type);
+	w->j_a.push_back(b1);
+	w->j_b.push_back(b2);
+	w->j_geo.insert(w->j_geo.end(), geo14, geo14 + 14);
+	uint64_t key = (uint64_t)std::min(b1, b2) << 32 | (uint64_t)std::max(b1, b2);
+	auto it = w->jedges.find(key);
+	if (it == w->jedges.end()) {
+		cp_world::JEdge e;
+		e.n1 = b1;
+		e.n2 = b2;
+		e.joints.push_back(k);
+		w->jedges.emplace(key, e);
+	} else {
+		it->second.joints.push_back(k);
+		if (it->second.sys >= 0) w->hs_changed = true;  // constraints_changed_flag (PhysicsEngine.cpp:679-681)
+		else it->second.scan = true;
+	}
+	w->joints_dirty = true;
+	*joint_index = k;
+	return CP_OK;
+}
+
+int cp_world_joint_count(cp_world* w, int* n) {
+	if (!w || !n) return CP_ERR_INVALID;
+	*n = (int)w->j_type.size();
+	return CP_OK;
+}
+
+int cp_world_set_joint_state(cp_world* w, const double* s11) {
+	if (!w || (!s11 && !w->j_type.empty())) return CP_ERR_INVALID;
+	return guard(w, [&] {
+		if (w->joints_dirty || w->j_dev_n != (int)w->j_type.size()) w->upload_joints();
+		if (w->j_dev_n) w->write_joint_state(s11);
+	});
+}
+
+int cp_world_get_joint_state(cp_world* w, double* s11) {
+	if (!w || (!s11 && !w->j_type.empty())) return CP_ERR_INVALID;
+	return guard(w, [&] {
+		if (w->joints_dirty || w->j_dev_n != (int)w->j_type.size()) w->upload_joints();
+		HIPCHK(hipStreamSynchronize(w->stream));
+		if (w->j_dev_n) w->read_joint_state(s11, w->j_dev_n);
+	});
+}
+
+int cp_world_get_joint_rows(cp_world* w, int k, double* ja, double* jb, double* ita, double* itb, int* n) {
+	if (!w || !ja || !jb || !ita || !itb || !n || k < 0 || k >= w->j_dev_n) return CP_ERR_INVALID;
+	return guard(w, [&] {
+		double R[CP_JRS];
+		HIPCHK(hipStreamSynchronize(w->stream));
+		HIPCHK(hipMemcpy(R, w->dj_row.p + (size_t)k * CP_JRS, sizeof(R), hipMemcpyDeviceToHost));
+		int t = w->j_type[k];
+		for (int r = 0; r < t; r++)
+			for (int c = 0; c < 6; c++) {
+				ja[r * 6 + c] = R[CP_J_JA + r * 6 + c];
+				jb[r * 6 + c] = R[CP_J_JB + r * 6 + c];
+				ita[c * t + r] = R[CP_J_ITA + c * 5 + r];
+				itb[c * t + r] = R[CP_J_ITB + c * 5 + r];
+			}
+		*n = t;
+	});
+}
+
+int cp_world_set_holonomic_systems(cp_world* w, int n_sys, const int64_t* off, const int64_t* joints) {
+	if (!w || n_sys < 0 || (n_sys > 0 && (!off || !joints))) return CP_ERR_INVALID;
+	const int K = (int)w->j_type.size();
+	std::vector<char> used(K, 0);
+	std::vector<std::vector<int>> og(n_sys);
+	for (int q = 0; q < n_sys; q++) {
+		if (off[q] > off[q + 1] || (q == 0 && off[0] != 0)) return CP_ERR_INVALID;
+		for (int64_t i = off[q]; i < off[q + 1]; i++) {
+			if (joints[i] < 0 || joints[i] >= K || used[joints[i]]) return CP_ERR_INVALID;
+			used[joints[i]] = 1;
+			og[q].push_back((int)joints[i]);
+		}
+	}
+	return guard(w, [&] {
+		w->hs_og.swap(og);
+		w->hs_explicit = true;
+		w->layout_systems();
+	});
+}
+
+int cp_world_holonomic_system_count(cp_world* w, int* n) {
+	if (!w || !n) return CP_ERR_INVALID;
+	*n = (int)w->hs_og.size();
+	return CP_OK;
+}
+
+int cp_world_get_holonomic_system(cp_world* w, int k, int64_t* og, int64_t* order, int64_t* table, int64_t* vloc, double* factor,
+                                  int64_t* sizes4) {
+	if (!w || !sizes4 || k < 0 || k >= (int)w->hs_lay.size()) return CP_ERR_INVALID;
+	const cp_world::HLayout& L = w->hs_lay[k];
+	sizes4[0] = (int64_t)L.order.size();
+	sizes4[1] = (int64_t)L.table.size();
+	sizes4[2] = L.degree;
+	sizes4[3] = L.cap;
+	for (size_t i = 0; og && i < w->hs_og[k].size(); i++) og[i] = w->hs_og[k][i];
+	for (size_t i = 0; order && i < L.order.size(); i++) order[i] = L.order[i];
+	for (size_t i = 0; table && i < L.table.size(); i++) table[i] = L.table[i];
+	for (size_t i = 0; vloc && i < L.vloc.size(); i++) vloc[i] = L.vloc[i];
+	if (!factor || L.cap == 0) return CP_OK;
+	return guard(w, [&] {
+		std::vector<int> bo;
+		down(bo, w->dh_buf_off.p, (size_t)k + 1);
+		HIPCHK(hipStreamSynchronize(w->stream));
+		HIPCHK(hipMemcpy(factor, w->dh_buf.p + bo[k], (size_t)L.cap * sizeof(double), hipMemcpyDeviceToHost));
+	});
 }
 
 // Iteration order of std::unordered_map<unsigned, T> after inserting 1..n in

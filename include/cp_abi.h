@@ -153,6 +153,44 @@ int cp_world_get_contacts(cp_world* w, int cap, int64_t* ints6, uint64_t* magic2
 int cp_world_set_node_order(cp_world* w, int n, const int64_t* ids);
 int cp_reference_node_order(int n, int64_t* ids);
 
+/* Joints.  Replace PhysicsEngine::addBallSocketConstraint (type 3,
+ * PhysicsEngine.cpp:664-687) and addHingeConstraint (type 5, :693-725) between
+ * bodies b1 and b2 of one scene; the pair stops colliding (disallowCollision,
+ * :666).  geo14 holds the joint as the reference stores it: the tracked
+ * attach points in the body frames of b1 and b2 (RigidBody::trackPoint,
+ * RigidBody.cpp:134-137: local_coord_origin + attach point), the hinge axes in
+ * the body frames (Hinge::b1_rot_axis_body_space, b2_rot_axis_body_space; 0
+ * for ball-sockets), the position and rotation correction strengths.
+ * Joints are solved in CP_PGS_REFERENCE_ORDER mode; a step of a world with
+ * joints in CP_PGS_COLORED mode returns CP_ERR_STATE. */
+int cp_joint_add(cp_world* w, int type, int b1, int b2, const double* geo14, int* joint_index);
+int cp_world_joint_count(cp_world* w, int* n);
+/* Joint solver state, 11 doubles per joint: accumulated impulse (5, unused
+ * components 0), hinge basis u and w of the current rows (BallSocket /
+ * Hinge ::constraint, ConstraintSolver.h).  Setting it marks the holonomic
+ * systems for a rebuild of their rows and factors at the next step. */
+int cp_world_set_joint_state(cp_world* w, const double* state11);
+int cp_world_get_joint_state(cp_world* w, double* state11);
+/* Rows of joint k built by the last step: J_a, J_b [n][6], M^-1 J^T of
+ * each body [6][n] (BallSocketConstraint / HingeConstraint,
+ * ConstraintSolver.cpp:382-466, :468-560); n = 3 or 5 is returned. */
+int cp_world_get_joint_rows(cp_world* w, int k, double* ja30, double* jb30, double* ita30, double* itb30, int* n);
+/* Holonomic systems (HolonomicSystem, HolonomicBlockSolver.cpp), solved by
+ * block LDL at the first substep when hol_iters > 0.  By default the world
+ * finds them as maintainHolonomicSystems does (PhysicsEngine.cpp:1292-1307,
+ * :1466-1512, :1746-1798) at the first step after joints are added;
+ * cp_world_set_holonomic_systems sets them explicitly instead (joint indices
+ * of system q at joints[off[q] .. off[q+1]), in constraint order) and turns
+ * that search off. */
+int cp_world_set_holonomic_systems(cp_world* w, int n_sys, const int64_t* off, const int64_t* joints);
+int cp_world_holonomic_system_count(cp_world* w, int* n);
+/* System k: constraint order og[n], elimination order[n] (joint indices),
+ * block location table (lower triangle, column major, -1 empty), vector
+ * offsets vloc[n] and the factor buffer; sizes4 = n, table size, degree,
+ * factor size.  Any output pointer but sizes4 may be null. */
+int cp_world_get_holonomic_system(cp_world* w, int k, int64_t* og, int64_t* order, int64_t* table, int64_t* vloc, double* factor,
+                                  int64_t* sizes4);
+
 /* One or more full steps.  Replaces PhysicsEngine::timeStep (PhysicsEngine.cpp:50-471). */
 int cp_world_step(cp_world* w, int n_steps);
 
@@ -168,7 +206,7 @@ int cp_narrowphase(cp_world* w, int n, const int64_t* pairs2, int* counts, doubl
 /* Sweep schedule of the last step: in CP_PGS_REFERENCE_ORDER mode the island
  * rows; in CP_PGS_COLORED mode (with diagnostics bit 1) one row list per scene
  * in colour order.  Rows are (contact index in the post-narrowphase cache,
- * kind 0 normal / 1 friction), with group offsets. */
+ * kind 0 normal / 1 friction; or joint index, kind 2), with group offsets. */
 int cp_world_last_islands(cp_world* w, int cap_rows, int64_t* rows2, int cap_offs, int64_t* offs, int* n_rows, int* n_islands);
 
 /* FP64 residual of the last substep's velocity solve (needs diagnostics bit 0):

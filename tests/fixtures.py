@@ -339,7 +339,7 @@ def world_from_fixture(fx, pgs_mode=1, **cfg):
     import constraintphysics_amd as cpa
     m = fx.meta
     kw = dict(dt=m["dt"], gravity=(0.0, m["gravity_y"], 0.0), vel_iters=int(m["vel_iters"]), pos_iters=int(m["pos_iters"]),
-              substeps=int(m["substeps"]), pgs_mode=pgs_mode)
+              substeps=int(m["substeps"]), pgs_mode=pgs_mode, hol_iters=int(m.get("hol_iters", 1)))
     kw.update(cfg)
     w = cpa.World(**kw)
     info, mp, mat = fx["body_info"], fx["mass_props"], fx["material"]
@@ -352,6 +352,9 @@ def world_from_fixture(fx, pgs_mode=1, **cfg):
         else:
             sid = w.add_hull_exact(sh)
         w.add_body(sid, int(info[b, 1]), mp[b], mat[b], 0)
+    if "joint_info" in fx:
+        for (kind, id1, id2), geo in zip(fx["joint_info"], fx["joint_geo"]):
+            w.add_joint(int(kind), int(id1) - 1, int(id2) - 1, geo)
     return w
 
 
@@ -367,5 +370,9 @@ def world_from_description(desc, **cfg):
         else:
             sid = w.add_hull_exact(sh)
         w.add_body(sid, int(desc["movement"][b]), desc["mass_props"][b], desc["material"][b], 0)
+    if "joints" in desc:
+        info, geo = desc["joints"]
+        for (kind, id1, id2), g in zip(info, geo):
+            w.add_joint(int(kind), int(id1) - 1, int(id2) - 1, g)
     w.set_state(desc["state"])
     return w
