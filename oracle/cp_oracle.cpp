@@ -18,6 +18,8 @@
 #include <map>
 #include <set>
 #include <climits>
+#include <memory>
+#include <unordered_map>
 #include <vector>
 
 namespace {
@@ -213,6 +215,8 @@ struct Hull {
 	std::vector<V3> wnorm;
 };
 
+struct Mesh;
+
 struct Body {
 	int id;
 	int movement;  // 0 dynamic, 1 fixed
@@ -222,7 +226,8 @@ struct Body {
 	V3 com, vel, ang_vel, pvel, pang;
 	Q q;
 	Hull hull;
-	int type = 0;        // 0 polyhedron, 1 sphere, 3 cylinder (ConvexGeometryType, ConvexPrimitive.h:36)
+	int type = 0;        // 0 polyhedron, 1 sphere, 3 cylinder (ConvexGeometryType, ConvexPrimitive.h:36), 5 static mesh
+	std::shared_ptr<Mesh> mesh;  // static triangle mesh (RigidBody::mesh / reference_mesh, RigidBody.h)
 	V3 sph_ref;          // sphere centre in the body frame (Sphere::recomputeFromReference)
 	double radius = 0;
 	V3 sph_c;            // world centre
@@ -248,10 +253,316 @@ static V3 faceNormal(const Hull& h, int f) {
 	return (p2 - p1).cross(p3 - p2).normalize() * (double)h.ndir[f];
 }
 
+// ---------------------------------------------------------------- static triangle mesh
+// StaticMeshFace (Geometry.h:61-86): world triangle with its edges, Gauss
+// region (directions a contact normal may take against this triangle) and
+// material.
+struct MeshTri {
+	V3 n;
+	V3 v[3];
+	int vid[3];
+	V3 ep1[3], ep2[3], eout[3];
+	int eid[3];
+	std::vector<V3> gauss;
+	int concave = 0;
+	double rest = 0, mu_k = 0, mu_s = 0;
+	double aabb[6];
+	int id;
+};
+// AABBTree<unsigned int> node (AABB_Tree.h:272-350); leaves hold a triangle.
+struct BVNode {
+	double box[6], tbox[6];
+	double cost = 0;
+	int parent = -1, left = -1, right = -1, leaf = -1;
+};
+struct Mesh {
+	std::vector<MeshTri> ref, tris;  // reference (body frame) and world triangles
+	std::vector<BVNode> nodes;
+	int root = -1;
+};
+
+static void boxCombine(const double* a, const double* b, double* o) {  // AABB::combine (AABB.h:16-26)
+	for (int i = 0; i < 3; i++) {
+		o[i] = std::min<double>(a[i], b[i]);
+		o[3 + i] = std::max<double>(a[3 + i], b[3 + i]);
+	}
+}
+static double boxSurface(const double* a) {  // AABB::surfaceArea (AABB.h:112-118), as written
+	double dx = a[3] - a[0], dy = a[4] - a[1], dz = a[5] - a[2];
+	return 2 * (dx * dy) + (dy * dz) + (dz * dx);
+}
+static double boxLongest(const double* a) {  // AABB::longestDimension (AABB.h:120-122)
+	return std::max<double>(a[3] - a[0], std::max<double>(a[4] - a[1], a[5] - a[2]));
+}
+static void triBox(MeshTri& t) {  // AABB::encapsulatePointCloud (AABB.h:52-70)
+	const double inf = std::numeric_limits<double>::infinity();
+	double mn[3] = {inf, inf, inf}, mx[3] = {-inf, -inf, -inf};
+	for (int i = 0; i < 3; i++) {
+		mn[0] = std::min<double>(mn[0], t.v[i].x); mn[1] = std::min<double>(mn[1], t.v[i].y); mn[2] = std::min<double>(mn[2], t.v[i].z);
+		mx[0] = std::max<double>(mx[0], t.v[i].x); mx[1] = std::max<double>(mx[1], t.v[i].y); mx[2] = std::max<double>(mx[2], t.v[i].z);
+	}
+	for (int i = 0; i < 3; i++) { t.aabb[i] = mn[i]; t.aabb[3 + i] = mx[i]; }
+}
+
+// StaticMeshGeometry(const MeshInput&) (Geometry.cpp:678-820): triangle
+// normals, neighbours through shared edges, ids (triangles 0..T-1, vertices
+// T + point index, edges from T + P upwards, three fresh ids per triangle)
+// and the Gauss region of each triangle from the convexity of its edges.
+static void meshBuild(Mesh& M, int n_pts, const double* pts, int n_tris, const int64_t* idx, const double* mat3) {
+	std::vector<V3> P(n_pts);
+	for (int i = 0; i < n_pts; i++) P[i] = V3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
+	struct E { int a, b, opp; uint64_t key; };
+	struct Node { int p[3]; int nb[3]; E e[3]; V3 n; };
+	std::vector<Node> g(n_tris);
+	for (int t = 0; t < n_tris; t++) {
+		int i1 = (int)idx[3 * t], i2 = (int)idx[3 * t + 1], i3 = (int)idx[3 * t + 2];
+		Node& nd = g[t];
+		nd.p[0] = i1; nd.p[1] = i2; nd.p[2] = i3;
+		nd.n = (P[i2] - P[i1]).cross(P[i3] - P[i1]).normalize();
+		int ev[3][3] = {{i1, i2, i3}, {i2, i3, i1}, {i3, i1, i2}};
+		for (int k = 0; k < 3; k++) {
+			uint64_t mx = (uint64_t)std::max(ev[k][0], ev[k][1]), mn = (uint64_t)std::min(ev[k][0], ev[k][1]);
+			nd.e[k] = {ev[k][0], ev[k][1], ev[k][2], ((mx << 32) & 0xFFFFFFFF00000000ull) + mn};
+			nd.nb[k] = -1;
+		}
+	}
+	std::unordered_map<uint64_t, int> open;
+	for (int t = 0; t < n_tris; t++)
+		for (int k = 0; k < 3; k++) {
+			auto it = open.find(g[t].e[k].key);
+			if (it != open.end()) {
+				g[t].nb[k] = it->second;
+				for (int q = 0; q < 3; q++)
+					if (g[it->second].e[q].key == g[t].e[k].key) g[it->second].nb[q] = t;
+				open.erase(it);
+			} else {
+				open[g[t].e[k].key] = t;
+			}
+		}
+	int next_edge = n_tris + n_pts;
+	M.ref.resize(n_tris);
+	for (int t = 0; t < n_tris; t++) {
+		const Node& nd = g[t];
+		MeshTri& tr = M.ref[t];
+		tr.n = nd.n;
+		for (int k = 0; k < 3; k++) {
+			tr.v[k] = P[nd.p[k]];
+			tr.vid[k] = nd.p[k] + n_tris;
+		}
+		tr.rest = mat3[3 * t];
+		tr.mu_k = mat3[3 * t + 1];
+		tr.mu_s = mat3[3 * t + 2];
+		tr.id = t;
+		tr.concave = 0;
+		for (int k = 0; k < 3; k++) {
+			tr.ep1[k] = tr.v[k];
+			tr.ep2[k] = tr.v[(k + 1) % 3];
+			tr.eout[k] = (tr.ep2[k] - tr.ep1[k]).cross(tr.n).normalize();
+			tr.eid[k] = next_edge++;
+			if (nd.nb[k] == -1) {
+				tr.gauss.push_back(tr.eout[k]);
+				continue;
+			}
+			const Node& o = g[nd.nb[k]];
+			int opp = -1;
+			for (int q = 0; q < 3; q++)
+				if (nd.e[k].a == o.e[q].b && nd.e[k].b == o.e[q].a) opp = o.e[q].opp;
+			V3 tip = P[nd.e[k].opp], otip = P[opp];
+			const double EPS = 0.00001;
+			if ((otip - tip).normalize().dot(nd.n) >= -EPS) {
+				tr.concave++;
+				if (tr.concave <= 1) tr.gauss.push_back(tr.n);
+			} else {
+				tr.gauss.push_back(o.n);
+			}
+		}
+		triBox(tr);
+	}
+}
+
+// StaticMeshFace::getTransformed (Geometry.cpp:827-847), centre of rotation 0.
+static MeshTri triTransform(const MeshTri& r, const M3& R, V3 tr) {
+	MeshTri o = r;
+	V3 c;
+	o.n = R * r.n;
+	for (size_t j = 0; j < r.gauss.size(); j++) o.gauss[j] = R * r.gauss[j];
+	for (int j = 0; j < 3; j++) {
+		o.v[j] = tr + R * (r.v[j] - c) + c;
+		o.ep1[j] = tr + R * (r.ep1[j] - c) + c;
+		o.ep2[j] = tr + R * (r.ep2[j] - c) + c;
+		o.eout[j] = R * r.eout[j];
+	}
+	triBox(o);
+	return o;
+}
+
+// AABBTree::add (AABB_Tree.h:39-103) for a static leaf (no margin), surface
+// area cost, with haveAncestorsRecalculateAABB and rebalanceAncestors (:357-424).
+static void bvSet(BVNode& n, const double* box) {
+	for (int i = 0; i < 6; i++) n.box[i] = box[i];
+	n.cost = boxSurface(box);
+}
+static void bvSwap(Mesh& M, int n1, int n2) {  // Node::swpNodes (:306-319)
+	int p1 = M.nodes[n1].parent, p2 = M.nodes[n2].parent;
+	int* s1 = M.nodes[p1].left == n1 ? &M.nodes[p1].left : &M.nodes[p1].right;
+	int* s2 = M.nodes[p2].left == n2 ? &M.nodes[p2].left : &M.nodes[p2].right;
+	*s1 = n2;
+	*s2 = n1;
+	M.nodes[n1].parent = p2;
+	M.nodes[n2].parent = p1;
+}
+static void bvAdd(Mesh& M, int tri) {
+	BVNode leaf;
+	leaf.leaf = tri;
+	for (int i = 0; i < 6; i++) leaf.tbox[i] = M.tris[tri].aabb[i];
+	double mb[6];
+	for (int i = 0; i < 3; i++) {
+		mb[i] = M.tris[tri].aabb[i] - 0.0;
+		mb[3 + i] = M.tris[tri].aabb[3 + i] + 0.0;
+	}
+	bvSet(leaf, mb);
+	int nl = (int)M.nodes.size();
+	M.nodes.push_back(leaf);
+	if (M.root < 0) {
+		M.root = nl;
+		return;
+	}
+	int best = -1;
+	double best_cost = std::numeric_limits<double>::infinity();
+	std::vector<std::pair<int, double>> cand{{M.root, 0.0}};
+	while (!cand.empty()) {
+		auto c = cand.back();
+		cand.pop_back();
+		if (c.second >= best_cost) continue;
+		double cb[6];
+		boxCombine(M.nodes[nl].box, M.nodes[c.first].box, cb);
+		double direct = boxSurface(cb);
+		double cost = c.second + direct;
+		if (cost < best_cost) {
+			best = c.first;
+			best_cost = cost;
+		}
+		if (M.nodes[c.first].leaf < 0) {
+			double inh = c.second + direct - M.nodes[c.first].cost;
+			cand.push_back({M.nodes[c.first].left, inh});
+			cand.push_back({M.nodes[c.first].right, inh});
+		}
+	}
+	int np = (int)M.nodes.size();
+	M.nodes.push_back(BVNode());
+	if (best == M.root) {
+		M.root = np;
+	} else {
+		int op = M.nodes[best].parent;
+		if (M.nodes[op].left == best) M.nodes[op].left = np;
+		else M.nodes[op].right = np;
+		M.nodes[np].parent = op;
+	}
+	M.nodes[np].left = best;
+	M.nodes[best].parent = np;
+	M.nodes[np].right = nl;
+	M.nodes[nl].parent = np;
+	for (int cur = M.nodes[nl].parent; cur >= 0; cur = M.nodes[cur].parent) {
+		double cb[6];
+		boxCombine(M.nodes[M.nodes[cur].left].box, M.nodes[M.nodes[cur].right].box, cb);
+		bvSet(M.nodes[cur], cb);
+	}
+	const double inf = std::numeric_limits<double>::infinity();
+	int cur = M.nodes[nl].parent;
+	while (cur >= 0 && M.nodes[cur].parent >= 0) {
+		int par = M.nodes[cur].parent;
+		int sib = M.nodes[par].left == cur ? M.nodes[par].right : M.nodes[par].left;
+		if (sib >= 0) {
+			double cl[6], cr[6], sl[6], sr[6], clc, crc, slc, src;
+			const BVNode& C = M.nodes[cur];
+			const BVNode& S = M.nodes[sib];
+			boxCombine(M.nodes[C.right].box, S.box, sl);
+			boxCombine(M.nodes[C.left].box, S.box, sr);
+			slc = boxSurface(sl);
+			src = boxSurface(sr);
+			if (S.leaf >= 0) {
+				clc = inf;
+				crc = inf;
+			} else {
+				boxCombine(M.nodes[S.right].box, C.box, cl);
+				boxCombine(M.nodes[S.left].box, C.box, cr);
+				clc = boxSurface(cl);
+				crc = boxSurface(cr);
+			}
+			double dcl = clc - S.cost, dcr = crc - S.cost, dsl = slc - C.cost, dsr = src - C.cost;
+			double mn = std::min<double>(dcl, std::min<double>(dcr, std::min<double>(dsl, dsr)));
+			if (mn >= 0) {
+			} else if (mn == dcl) {
+				int sl_node = M.nodes[sib].left;
+				bvSwap(M, cur, sl_node);
+				for (int i = 0; i < 6; i++) M.nodes[sib].box[i] = cl[i];
+				M.nodes[sib].cost = clc;
+			} else if (mn == dcr) {
+				int sr_node = M.nodes[sib].right;
+				bvSwap(M, cur, sr_node);
+				for (int i = 0; i < 6; i++) M.nodes[sib].box[i] = cr[i];
+				M.nodes[sib].cost = crc;
+			} else if (mn == dsl) {
+				int cl_node = M.nodes[cur].left;
+				bvSwap(M, sib, cl_node);
+				for (int i = 0; i < 6; i++) M.nodes[cur].box[i] = sl[i];
+				M.nodes[cur].cost = slc;
+			} else {
+				int cr_node = M.nodes[cur].right;
+				bvSwap(M, sib, cr_node);
+				for (int i = 0; i < 6; i++) M.nodes[cur].box[i] = sr[i];
+				M.nodes[cur].cost = src;
+			}
+		}
+		cur = M.nodes[cur].parent;
+	}
+}
+
+static bool aabbOverlap(const double* a, const double* b) {  // AABB.h:10-14
+	return !(a[3] < b[0] || a[0] > b[3] || a[4] < b[1] || a[1] > b[4] || a[5] < b[2] || a[2] > b[5]);
+}
+
+// AABBTree::getCollisionCandidatesWith (AABB_Tree.h:226-247): depth first,
+// right child popped first.
+static std::vector<int> bvQuery(const Mesh& M, const double* target) {
+	std::vector<int> out;
+	if (M.root < 0) return out;
+	std::vector<int> st{M.root};
+	while (!st.empty()) {
+		int n = st.back();
+		st.pop_back();
+		const BVNode& N = M.nodes[n];
+		if (N.leaf >= 0) {
+			if (aabbOverlap(N.tbox, target)) out.push_back(N.leaf);
+		} else if (aabbOverlap(N.box, target)) {
+			st.push_back(N.left);
+			st.push_back(N.right);
+		}
+	}
+	return out;
+}
+
 // RigidBody::updateGeometry -> Polyhedron::recomputeFromReference + gen_AABB
 // (RigidBody.cpp:281-296, ConvexPrimitive.cpp:660-676, :715-729).
 static void worldGeometry(Body& b) {
 	M3 R = b.q.rot();
+	if (b.type == 5) {  // StaticMeshGeometry::recomputeFromReference + genAABB (Geometry.cpp:849-868)
+		Mesh& M = *b.mesh;
+		M.tris.resize(M.ref.size());
+		M.nodes.clear();
+		M.root = -1;
+		for (size_t i = 0; i < M.ref.size(); i++) {
+			M.tris[i] = triTransform(M.ref[i], R, b.com);
+			bvAdd(M, (int)i);
+		}
+		for (int i = 0; i < 6; i++) b.aabb[i] = M.tris[0].aabb[i];
+		for (size_t i = 1; i < M.tris.size(); i++) {
+			double o[6];
+			boxCombine(b.aabb, M.tris[i].aabb, o);
+			for (int k = 0; k < 6; k++) b.aabb[k] = o[k];
+		}
+		return;
+	}
 	if (b.type == 1) {  // Sphere::recomputeFromReference / gen_AABB (ConvexPrimitive.cpp:433-443)
 		b.sph_c = b.com + R * b.sph_ref;
 		V3 r(b.radius, b.radius, b.radius);
@@ -991,6 +1302,274 @@ static Manifold cullManifold(const Manifold& m, int ns) {
 	}
 	return out;
 }
+
+// ---------------------------------------------------------------- convex vs triangle
+static Extrema triExtrema(const MeshTri& t, V3 dir) {  // findTriangleExtrema (CollisionDetect.cpp:2004-2021)
+	Extrema e;
+	for (int i = 0; i < 3; i++) {
+		double val = t.v[i].dot(dir);
+		if (val < e.min_val) { e.min_pid = i; e.min_val = val; }
+		if (val > e.max_val) { e.max_pid = i; e.max_val = val; }
+	}
+	return e;
+}
+
+// normalDirectionValid (CollisionDetect.cpp:2024-2060).
+static bool normalValid(const MeshTri& t, V3 n) {
+	const double EPS = 0.0001;
+	switch (t.concave) {
+	case 0:
+	case 1:
+		for (size_t i = 0; i < t.gauss.size(); i++) {
+			V3 inner = t.gauss[i].cross(t.gauss[(i + 1) % t.gauss.size()]);
+			if (n.dot(inner) < -EPS) return false;
+		}
+		break;
+	case 2: {
+		V3 an = t.gauss[0].cross(t.gauss[1]);
+		if (std::abs(n.dot(an)) > EPS) return false;
+		V3 up = an.cross(t.gauss[0]);
+		if (n.dot(up) < -EPS) return false;
+		V3 down = t.gauss[1].cross(an);
+		if (n.dot(down) < -EPS) return false;
+		break;
+	}
+	case 3:
+		if (n.dot(t.n) < 1 - EPS) return false;
+		break;
+	}
+	return true;
+}
+
+// findTriangleContactArea (CollisionDetect.cpp:282-329).
+static Area triContactArea(const MeshTri& t, V3 n, V3 p, int pid, V3 u, V3 w) {
+	Area a;
+	if (1 - t.n.dot(n) <= COS_TOL) {
+		for (int i = 0; i < 3; i++) {
+			a.ps.push_back({t.v[i].dot(u), t.v[i].dot(w)});
+			a.ids.push_back(i);
+		}
+		a.surface = t.id;
+		a.origin = FACE;
+		return a;
+	}
+	for (int i = 0; i < 3; i++) {
+		if (i != pid && (i + 1) % 3 != pid) continue;
+		if (std::abs((t.ep2[i] - t.ep1[i]).normalize().dot(n)) <= SIN_TOL) {
+			a.ps = {{t.ep1[i].dot(u), t.ep1[i].dot(w)}, {t.ep2[i].dot(u), t.ep2[i].dot(w)}};
+			a.ids = {t.eid[i], t.eid[i]};
+			a.surface = -1;
+			a.origin = EDGE;
+			return a;
+		}
+	}
+	a.ps = {{p.dot(u), p.dot(w)}};
+	a.ids = {pid};
+	a.surface = -1;
+	a.origin = VERTEX;
+	return a;
+}
+
+// Minimum-penetration bookkeeping shared by the triangle tests: the overall
+// minimum and the minimum over Gauss-valid directions (normalDirectionValid
+// of the reversed axis).
+struct TriMin {
+	CheckNorm all{-1, -1, V3(), std::numeric_limits<double>::infinity()};
+	CheckNorm valid{-1, -1, V3(), std::numeric_limits<double>::infinity()};
+	// false = separating axis
+	bool test(const CheckNorm& x, const MeshTri& t, bool* improved = nullptr) {
+		if (x.pen < 0) return false;
+		if (improved) *improved = false;
+		if (x.pen < all.pen) {
+			all = x;
+			if (improved) *improved = true;
+		}
+		if (x.pen < valid.pen && normalValid(t, -x.n)) valid = x;
+		return true;
+	}
+	// the Gauss-valid minimum wins unless it is worse by more than the penalty
+	CheckNorm pick(double penalty) const { return valid.pen < all.pen + penalty ? valid : all; }
+};
+
+// The six arcs of a triangle's Gauss map (face normal to each edge's outward
+// direction and on to the reversed normal), CollisionDetect.cpp:2097-2103.
+static void triArcs(const MeshTri& t, V3 g1[6], V3 g2[6]) {
+	for (int e = 0; e < 3; e++) {
+		g1[2 * e] = t.n;
+		g2[2 * e] = t.eout[e];
+		g1[2 * e + 1] = t.eout[e];
+		g2[2 * e + 1] = -t.n;
+	}
+}
+
+static void triPoint(CP& c, const Body& A, const MeshTri& t) {
+	c.restitution = std::max<double>(A.restitution, t.rest);
+	c.mu_k = (A.mu_k + t.mu_k) / 2.0;
+	c.mu_s = (A.mu_s + t.mu_s) / 2.0;
+}
+
+// SAT_PolyTriangle (CollisionDetect.cpp:2062-2194).
+static Manifold satPolyTriangle(const Body& A, uint64_t a_key, const MeshTri& t, double penalty) {
+	Manifold out;
+	out.max_pen = -1;
+	const Hull& a = A.hull;
+	TriMin m;
+	if (!m.test(satCheck(findExtrema(a, t.n), triExtrema(t, t.n), t.n), t)) return Manifold();
+	for (size_t g = 0; g < a.wnorm.size(); g++) {
+		if (a.gred[g]) continue;
+		if (!m.test(satCheck(recentered(a, (int)g), triExtrema(t, a.wnorm[g]), a.wnorm[g]), t)) return Manifold();
+	}
+	V3 g1[6], g2[6];
+	triArcs(t, g1, g2);
+	for (auto arc1 : a.arcs)
+		for (int k = 0; k < 6; k++) {
+			V3 n;
+			if (!arcAxis(a.wnorm[arc1.first], a.wnorm[arc1.second], -g1[k], -g2[k], &n)) continue;
+			if (!m.test(satCheck(findExtrema(a, n), triExtrema(t, n), n), t)) return Manifold();
+		}
+	CheckNorm raw = m.all, mp = m.pick(penalty);
+	out.n = mp.n;
+	V3 n = mp.n;
+	V3 amax = a.pts[raw.a_pid], bmax = t.v[raw.b_pid];
+	V3 u, w;
+	n.perpBasis(&u, &w);
+	Area ac = contactArea(a, raw.n, amax, raw.a_pid, u, w);
+	Area bc = triContactArea(t, raw.n * -1.0, bmax, raw.b_pid, u, w);
+	std::vector<CEP> pool = clipContacts(ac, bc);
+	double a_dot = amax.dot(n);
+	V3 noff = n * a_dot;
+	uint64_t cid = (0x00000000FFFFFFFFull & a_key) | (0xFFFFFFFF00000000ull & ((uint64_t)t.id << 32));
+	for (const CEP& p : pool) {
+		CP c;
+		c.pos = u * p.pos.x + w * p.pos.y + noff;
+		c.pen = c.pos.dot(n) - a_dot + mp.pen;
+		triPoint(c, A, t);
+		c.cid = cid;
+		c.bid = p.magic;
+		out.pts.push_back(c);
+	}
+	out.max_pen = mp.pen;
+	return out;
+}
+
+// SAT_SphereTriangle (CollisionDetect.cpp:2196-2276).
+static Manifold satSphereTriangle(const Body& A, uint64_t a_key, const MeshTri& t, double penalty) {
+	TriMin m;
+	uint32_t feat = (uint32_t)-1;
+	bool imp;
+	if (!m.test(satCheck(sphereExtrema(A, t.n), triExtrema(t, t.n), t.n), t)) return Manifold();
+	for (int i = 0; i < 3; i++) {
+		V3 n = (t.v[i] - A.sph_c).normalize();
+		if (!m.test(satCheck(sphereExtrema(A, n), triExtrema(t, n), n), t, &imp)) return Manifold();
+		if (imp) feat = (uint32_t)i;
+	}
+	for (int e = 0; e < 3; e++) {
+		V3 dir = (t.ep2[e] - t.ep1[e]).normalize();
+		V3 sample = t.ep1[e] - A.sph_c;
+		V3 n = (sample - dir * dir.dot(sample)).normalize();
+		if (!m.test(satCheck(sphereExtrema(A, n), triExtrema(t, n), n), t, &imp)) return Manifold();
+		if (imp) feat = (uint32_t)t.eid[e];
+	}
+	CheckNorm mp = m.pick(penalty);
+	Manifold out;
+	out.n = mp.n;
+	CP c;
+	c.pos = A.sph_c + mp.n * A.radius;
+	c.pen = mp.pen;
+	triPoint(c, A, t);
+	c.cid = (0x00000000FFFFFFFFull & a_key) | (0xFFFFFFFF00000000ull & ((uint64_t)t.id << 32));
+	c.bid = feat;
+	out.pts.push_back(c);
+	out.max_pen = mp.pen;
+	return out;
+}
+
+// SAT_CylinderTriangle (CollisionDetect.cpp:2466-2645).
+static Manifold satCylinderTriangle(const Body& A, uint64_t a_key, const MeshTri& t, double penalty) {
+	TriMin m;
+	V3 ax = A.cyl_ax;
+	if (!m.test(satCheck(cylExtrema(A, ax), triExtrema(t, ax), ax), t)) return Manifold();
+	if (!m.test(satCheck(cylExtrema(A, t.n), triExtrema(t, t.n), t.n), t)) return Manifold();
+	for (int e = 0; e < 3; e++) {
+		V3 dir = (t.ep2[e] - t.ep1[e]).cross(ax);
+		if (dir.mag() < 0.00000000001) continue;
+		V3 dn = dir.normalize();
+		if (!m.test(satCheck(cylExtrema(A, dn), triExtrema(t, dn), dn), t)) return Manifold();
+	}
+	for (int i = 0; i < 3; i++) {
+		V3 diff = t.v[i] - A.cyl_c;
+		V3 n = (diff - ax * ax.dot(diff)).normalize();
+		if (!m.test(satCheck(cylExtrema(A, n), triExtrema(t, n), n), t)) return Manifold();
+	}
+	V3 g1[6], g2[6];
+	triArcs(t, g1, g2);
+	for (int k1 = 0; k1 < 24; k1++)
+		for (int k = 0; k < 6; k++) {
+			int v1, v2;
+			cylArc(k1, &v1, &v2);
+			V3 n;
+			if (!arcAxis(A.cyl_gv[v1], A.cyl_gv[v2], -g1[k], -g2[k], &n)) continue;
+			if (!m.test(satCheck(cylExtrema(A, n), triExtrema(t, n), n), t)) return Manifold();
+		}
+	CheckNorm raw = m.all, mp = m.pick(penalty);
+	Manifold out;
+	out.n = mp.n;
+	V3 n = mp.n;
+	V3 amax = ax.dot(n) > 0 ? diskExtrema(cylTopCenter(A), ax, A.cyl_r, n) : diskExtrema(cylBotCenter(A), ax, A.cyl_r, n);
+	V3 bmax = t.v[raw.b_pid];
+	V3 u, w;
+	n.perpBasis(&u, &w);
+	Area ac = cylContactArea(A, raw.n, u, w);
+	Area bc = triContactArea(t, -raw.n, bmax, raw.b_pid, u, w);
+	std::vector<CEP> pool;
+	if (ac.origin == EDGE) pool = {poolPoint(ac.ps[0], 0x0)};
+	else pool = clipContacts(ac, bc);
+	double a_dot = amax.dot(n);
+	V3 noff = n * a_dot;
+	uint64_t cid = (0x00000000FFFFFFFFull & a_key) | (0xFFFFFFFF00000000ull & ((uint64_t)t.id << 32));
+	for (const CEP& p : pool) {
+		CP c;
+		c.pos = u * p.pos.x + w * p.pos.y + noff;
+		c.pen = c.pos.dot(n) - a_dot + mp.pen;
+		triPoint(c, A, t);
+		c.cid = cid;
+		c.bid = p.magic;
+		out.pts.push_back(c);
+	}
+	out.max_pen = mp.pen;
+	return out;
+}
+
+// SAT_{Poly,Sphere,Cylinder}Mesh (CollisionDetect.cpp:2647-2778) for a mesh
+// held in world coordinates: the triangles whose boxes meet the convex body's
+// box, in tree order, one manifold per colliding triangle.
+static std::vector<Manifold> convexMesh(const Body& C, const Body& Mb) {
+	std::vector<Manifold> out;
+	const Mesh& M = *Mb.mesh;
+	uint64_t key = (uint64_t)C.id * 16;
+	for (int ti : bvQuery(M, C.aabb)) {
+		const MeshTri& t = M.tris[ti];
+		double pen = C.type == 0 ? 0 * std::min<double>(boxLongest(C.aabb), boxLongest(t.aabb))
+		                         : 0.15 * std::min<double>(boxLongest(C.aabb), boxLongest(t.aabb));
+		Manifold m;
+		if (C.type == 0) m = satPolyTriangle(C, key, t, pen);
+		else if (C.type == 1) m = satSphereTriangle(C, key, t, pen);
+		else if (C.type == 3) m = satCylinderTriangle(C, key, t, pen);
+		if (m.max_pen > 0 && !m.pts.empty()) out.push_back(m);
+	}
+	return out;
+}
+
+// merge_manifold (CollisionDetect.cpp:2780-2793).
+static Manifold mergeManifold(const Manifold& a, const Manifold& b) {
+	Manifold o;
+	o.pts = a.pts;
+	o.pts.insert(o.pts.end(), b.pts.begin(), b.pts.end());
+	o.n = (a.n + b.n).normalize();
+	o.max_pen = std::max<double>(a.max_pen, b.max_pen);
+	return o;
+}
+
 
 // ---------------------------------------------------------------- constraints
 struct ContactRow {  // ContactConstraint (ConstraintSolver.cpp:240-272)
@@ -1746,10 +2325,6 @@ static void refreshAll(World& W) {
 	}
 }
 
-static bool aabbOverlap(const double* a, const double* b) {  // AABB.h:10-14
-	return !(a[3] < b[0] || a[0] > b[3] || a[4] < b[1] || a[1] > b[4] || a[5] < b[2] || a[2] > b[5]);
-}
-
 static std::vector<std::pair<int, int>> broadphase(const World& W) {
 	std::vector<std::pair<int, int>> v;
 	for (size_t i = 0; i < W.bodies.size(); i++)
@@ -1795,6 +2370,48 @@ static Manifold narrowRaw(const World& W, int i, int j) {
 	else if (A.type == 3 && B.type == 3) m = cylCyl(A, primKey(A), B, primKey(B));
 	if (!(m.max_pen > 0)) return Manifold();
 	return cullManifold(m, 4);
+}
+
+// The manifolds of a body pair with a static mesh as the engine adds them
+// (PhysicsEngine.cpp:223-277): per-triangle manifolds (normal flipped and
+// MagicIDs swapped when the mesh is the first body, CollisionDetect.cpp:171-195),
+// near-parallel ones merged into the first, each culled to 4 points.
+static std::vector<Manifold> meshPair(const World& W, int i, int j) {
+	const Body& A = W.bodies[i];
+	const Body& B = W.bodies[j];
+	std::vector<Manifold> ms;
+	if (A.type == 5 && B.type == 5) return ms;
+	if (B.type == 5) ms = convexMesh(A, B);
+	else {
+		ms = convexMesh(B, A);
+		for (Manifold& m : ms) {
+			m.n = -m.n;
+			for (CP& c : m.pts) { c.cid = swapHalves(c.cid); c.bid = swapHalves(c.bid); }
+		}
+	}
+	std::vector<Manifold> out;
+	std::vector<bool> merged(ms.size(), false);
+	for (size_t a = 0; a < ms.size(); a++) {
+		if (merged[a]) continue;
+		for (size_t b = a + 1; b < ms.size(); b++) {
+			if (merged[b]) continue;
+			double v = ms[a].n.dot(ms[b].n);
+			if (1 - v < COS_TOL) {
+				ms[a] = mergeManifold(ms[a], ms[b]);
+				merged[b] = true;
+			}
+		}
+		Manifold m = cullManifold(ms[a], 4);
+		if (W.cfg.drop_nonfinite) {
+			if (!std::isfinite(m.n.x) || !std::isfinite(m.n.y) || !std::isfinite(m.n.z)) continue;
+			std::vector<CP> keep;
+			for (const CP& c : m.pts)
+				if (std::isfinite(c.pos.x) && std::isfinite(c.pos.y) && std::isfinite(c.pos.z) && std::isfinite(c.pen)) keep.push_back(c);
+			m.pts = keep;
+		}
+		out.push_back(m);
+	}
+	return out;
 }
 
 // addContact (PhysicsEngine.cpp:1062-1111).
@@ -1856,6 +2473,11 @@ static void step(World& W, const int64_t* node_order, const int64_t* sched_rows 
 		const Body& B = W.bodies[pr.second];
 		if (!(A.dynamic() || B.dynamic())) continue;
 		if (W.nocol.count({std::min(pr.first, pr.second), std::max(pr.first, pr.second)})) continue;  // collisionAllowed (:570-572)
+		if (A.type == 5 || B.type == 5) {
+			for (const Manifold& m : meshPair(W, pr.first, pr.second))
+				for (const CP& p : m.pts) addContact(W, pr.first, pr.second, p, m.n);
+			continue;
+		}
 		Manifold m = narrow(W, pr.first, pr.second);
 		for (const CP& p : m.pts) addContact(W, pr.first, pr.second, p, m.n);
 	}
@@ -2170,6 +2792,67 @@ int orc_add_joint(orc_world* w, int type, int b1, int b2, const double* g) {
 	w->joints.push_back(j);
 	w->nocol.insert({std::min(b1, b2), std::max(b1, b2)});
 	return (int)w->joints.size() - 1;
+}
+
+int orc_add_mesh_body(orc_world* w, int n_pts, const double* pts, int n_tris, const int64_t* tri_idx, const double* tri_mat3) {
+	Body b{};
+	b.id = (int)w->bodies.size() + 1;
+	b.movement = 1;
+	b.mass = 0;
+	b.type = 5;
+	b.q = Q(1, 0, 0, 0);
+	b.mesh = std::make_shared<Mesh>();
+	meshBuild(*b.mesh, n_pts, pts, n_tris, tri_idx, tri_mat3);
+	w->bodies.push_back(b);
+	worldGeometry(w->bodies.back());
+	return (int)w->bodies.size() - 1;
+}
+
+int orc_mesh_query(const orc_world* w, int body, const double* aabb6, int64_t* out, int cap) {
+	std::vector<int> c = bvQuery(*w->bodies[body].mesh, aabb6);
+	for (int i = 0; i < (int)c.size() && i < cap; i++) out[i] = c[i];
+	return (int)c.size();
+}
+
+void orc_mesh_triangle(const orc_world* w, int body, int t, double* f40, int64_t* i8) {
+	const MeshTri& r = w->bodies[body].mesh->tris[t];
+	double* o = f40;
+	auto put = [&](V3 v) { *o++ = v.x; *o++ = v.y; *o++ = v.z; };
+	put(r.n);
+	for (int k = 0; k < 3; k++) put(r.v[k]);
+	for (int k = 0; k < 3; k++) put(r.eout[k]);
+	for (int k = 0; k < 3; k++) put(k < (int)r.gauss.size() ? r.gauss[k] : V3());
+	for (int k = 0; k < 6; k++) *o++ = r.aabb[k];
+	*o++ = r.rest; *o++ = r.mu_k; *o++ = r.mu_s;
+	*o++ = 0;
+	for (int k = 0; k < 3; k++) { i8[k] = r.vid[k]; i8[3 + k] = r.eid[k]; }
+	i8[6] = (int64_t)r.gauss.size();
+	i8[7] = r.concave;
+}
+
+int orc_narrowphase_multi(const orc_world* w, int id_lo, int id_hi, int cap, double* heads, int64_t* counts, double* pts,
+                          uint64_t* magic) {
+	int i = id_lo - 1, j = id_hi - 1;
+	std::vector<Manifold> ms;
+	if (w->nocol.count({std::min(i, j), std::max(i, j)})) return 0;  // collisionAllowed (PhysicsEngine.cpp:570-572)
+	if (!(w->bodies[i].dynamic() || w->bodies[j].dynamic())) return 0;
+	if (w->bodies[i].type == 5 || w->bodies[j].type == 5) ms = meshPair(*w, i, j);
+	else {
+		Manifold m = narrow(*w, i, j);
+		if (!m.pts.empty()) ms.push_back(m);
+	}
+	int off = 0;
+	for (int k = 0; k < (int)ms.size() && k < cap; k++) {
+		const Manifold& m = ms[k];
+		heads[4 * k] = m.n.x; heads[4 * k + 1] = m.n.y; heads[4 * k + 2] = m.n.z; heads[4 * k + 3] = m.max_pen;
+		counts[k] = (int64_t)m.pts.size();
+		for (const CP& c : m.pts) {
+			pts[4 * off] = c.pos.x; pts[4 * off + 1] = c.pos.y; pts[4 * off + 2] = c.pos.z; pts[4 * off + 3] = c.pen;
+			magic[2 * off] = c.cid; magic[2 * off + 1] = c.bid;
+			off++;
+		}
+	}
+	return (int)ms.size();
 }
 
 void orc_set_joint_state(orc_world* w, const double* s) {

@@ -56,6 +56,25 @@ int orc_add_sphere_body(orc_world* w, int movement, const double* mass_props, co
 int orc_add_cylinder_body(orc_world* w, int movement, const double* mass_props, const double* material, const double* params8,
                           const double* gauss42, const double* top24, const double* bot24);
 
+// Static triangle mesh body (RigidBody(StaticMeshGeometry), RigidBody.cpp:40-58,
+// built by StaticMeshGeometry(const MeshInput&), Geometry.cpp:678-820): points,
+// triangles (3 point indices each, counter-clockwise seen from the front) and
+// per-triangle restitution, kinetic and static friction.  FIXED, centre of mass
+// at the origin, identity orientation.  Returns the body index.
+int orc_add_mesh_body(orc_world* w, int n_pts, const double* pts, int n_tris, const int64_t* tri_idx, const double* tri_mat3);
+// Triangles of mesh body `body` whose boxes meet aabb6, in the order of
+// AABBTree::getCollisionCandidatesWith (AABB_Tree.h:226-247); returns the count.
+int orc_mesh_query(const orc_world* w, int body, const double* aabb6, int64_t* out, int cap);
+// World triangle t of a mesh body: f40 = normal, 3 vertices, 3 edge out
+// directions, up to 3 Gauss-region vectors, aabb 6, restitution, mu_k, mu_s, 0;
+// i8 = 3 vertex ids, 3 edge ids, Gauss-region size, concave neighbour count.
+void orc_mesh_triangle(const orc_world* w, int body, int t, double* f40, int64_t* i8);
+// Narrowphase of one body pair returning every manifold the engine adds
+// (several for a convex body on a mesh); returns the manifold count, up to cap
+// written: heads[cap][4], counts[cap], pts / magic rows in manifold order.
+int orc_narrowphase_multi(const orc_world* w, int id_lo, int id_hi, int cap, double* heads, int64_t* counts, double* pts,
+                          uint64_t* magic);
+
 // Joints (PhysicsEngine::addBallSocketConstraint / addHingeConstraint,
 // PhysicsEngine.cpp:664-720).  type 3 ball-socket, 5 hinge; b1/b2 body
 // indices; geo14 = tracked point in b1's frame (3), in b2's frame (3), hinge

@@ -227,6 +227,7 @@ struct JointRec {
 	phyz::Constraint* solver;
 };
 static std::vector<JointRec> g_joints;
+static RigidBody* g_mesh = nullptr;  // the terrain body of the terrain scene
 
 static void build_chains(PhysicsEngine& E, int n_chains, int n_links, uint64_t seed, int hinge_every) {
 	phyz::Material m = c0_material();
@@ -334,12 +335,21 @@ static std::map<int, std::pair<int, int>> prim_owner(PhysicsEngine& E) {
 
 // cID packs two global primitive ids (CollisionDetect.cpp:827-829); remap each
 // half to body_id * 16 + primitive index so that it is stable across runs.
-static uint64_t remap_cid(uint64_t cid, const std::map<int, std::pair<int, int>>& own) {
+// The half that belongs to a static mesh is a triangle id (CollisionDetect.cpp:
+// 2178-2180) and stays as it is: raw_half 0 (low), 1 (high) or -1 (none).
+static uint64_t remap_cid(uint64_t cid, const std::map<int, std::pair<int, int>>& own, int raw_half = -1) {
 	int a = (int)(cid & 0xFFFFFFFFull), b = (int)(cid >> 32);
 	auto fa = own.find(a), fb = own.find(b);
 	uint64_t ra = fa == own.end() ? 0xFFFFFFFFull : (uint64_t)(fa->second.first * 16 + fa->second.second);
 	uint64_t rb = fb == own.end() ? 0xFFFFFFFFull : (uint64_t)(fb->second.first * 16 + fb->second.second);
+	if (raw_half == 0) ra = cid & 0xFFFFFFFFull;
+	if (raw_half == 1) rb = cid >> 32;
 	return ra | (rb << 32);
+}
+static int mesh_half(const RigidBody* b1, const RigidBody* b2) {
+	if (b1->geometry_type == RigidBody::STATIC_MESH) return 0;
+	if (b2->geometry_type == RigidBody::STATIC_MESH) return 1;
+	return -1;
 }
 
 static void body_state(PhysicsEngine& E, std::vector<double>& s) {
@@ -382,7 +392,7 @@ static void dump_cache(PhysicsEngine& E, CacheDump& d) {
 				auto* c = static_cast<phyz::Contact*>(pc);
 				d.ints.insert(d.ints.end(), {(int64_t)c->b1->getID(), (int64_t)c->b2->getID(), c->memory_life, c->is_live_contact ? 1 : 0,
 				                             c->friction.static_ready ? 1 : 0, pos});
-				d.magic.push_back(remap_cid(c->magic.cID, own));
+				d.magic.push_back(remap_cid(c->magic.cID, own, mesh_half(c->b1, c->b2)));
 				d.magic.push_back(c->magic.bID);
 				d.f.insert(d.f.end(), {c->contact_pos_local_b1.x, c->contact_pos_local_b1.y, c->contact_pos_local_b1.z,
 				                       c->contact_pos_local_b2.x, c->contact_pos_local_b2.y, c->contact_pos_local_b2.z,
@@ -439,12 +449,16 @@ static void on_collision(RigidBody* b1, RigidBody* b2, const std::vector<phyz::M
 			d->island_off.push_back((int64_t)d->island_rows.size() / 2);
 		}
 	}
-	for (const phyz::Manifold& m : ms) {
+	// One call per culled manifold; the list repeats that manifold
+	// (PhysicsEngine.cpp:325-333 pushes `man` once per unmerged manifold), so
+	// only its first entry is recorded.
+	if (!ms.empty()) {
+		const phyz::Manifold& m = ms[0];
 		d->man.pair.insert(d->man.pair.end(), {(int64_t)b1->getID(), (int64_t)b2->getID(), (int64_t)m.points.size()});
 		d->man.head.insert(d->man.head.end(), {m.normal.x, m.normal.y, m.normal.z, m.max_pen_depth});
 		for (const phyz::ContactP& p : m.points) {
 			d->man.pts.insert(d->man.pts.end(), {p.pos.x, p.pos.y, p.pos.z, p.pen_depth});
-			d->man.magic.push_back(remap_cid(p.magicID.cID, own));
+			d->man.magic.push_back(remap_cid(p.magicID.cID, own, mesh_half(b1, b2)));
 			d->man.magic.push_back(p.magicID.bID);
 		}
 	}
@@ -603,6 +617,16 @@ static void run_trace(PhysicsEngine& E, const SceneCfg& cfg, int steps, const st
 		StepDump d;
 		std::vector<double> jpre;
 		HsysDump hs;
+		std::vector<int64_t> c_off{0}, c_list;
+		std::vector<double> c_box;
+		if (rec && g_mesh) {  // mesh candidates for each body's box before the step
+			for (RigidBody* b : E.bodies) {
+				if (b != g_mesh)
+					for (unsigned t : g_mesh->mesh.getAABBTree().getCollisionCandidatesWith(b->aabb)) c_list.push_back(t);
+				c_off.push_back((int64_t)c_list.size());
+				c_box.insert(c_box.end(), {b->aabb.min.x, b->aabb.min.y, b->aabb.min.z, b->aabb.max.x, b->aabb.max.y, b->aabb.max.z});
+			}
+		}
 		if (rec) {
 			joint_state(jpre);
 			dump_hsys(E, hs);
@@ -638,6 +662,11 @@ static void run_trace(PhysicsEngine& E, const SceneCfg& cfg, int steps, const st
 		if (d.island_off.empty()) d.island_off.push_back(0);
 		o.i64(p + "_island_off", d.island_off, {d.island_off.size()});
 		o.i64(p + "_node_order", d.node_order, {d.node_order.size()});
+		if (g_mesh) {
+			o.i64(p + "_mesh_cand_off", c_off, {c_off.size()});
+			o.i64(p + "_mesh_cand", c_list, {c_list.size()});
+			o.f64(p + "_mesh_cand_box", c_box, {N, 6});
+		}
 		if (!g_joints.empty()) {
 			std::vector<double> jpost;
 			joint_state(jpost);
@@ -727,6 +756,153 @@ static int cmd_chain(int argc, char** argv) {
 	return 0;
 }
 
+// Heightfield terrain: generateGridMeshInput(grid, grid, cell) (Geometry.cpp:
+// 589-611) centred on the origin, point heights amp * value noise (a lattice
+// of U[-1, 1] values every 4 cells, bilinear with smoothstep weights).  The
+// mesh body is FIXED with its centre of mass pinned at the origin (the
+// reference leaves it unset for meshes, RigidBody.cpp:40-58).  Above it a
+// mixed pile (box, sphere, tilted cylinder in turn; nx x ny x nz, spacing 1.1)
+// and `chains` chains of `links` cylinder links as in build_chains.
+static phyz::MeshInput g_mesh_input;
+static void build_terrain(PhysicsEngine& E, int grid, double cell, double amp, int nx, int ny, int nz, int n_chains, int n_links,
+                          uint64_t seed) {
+	phyz::Material m = c0_material();
+	double half = 0.5 * grid * cell;
+	phyz::MeshInput in = phyz::generateGridMeshInput(grid, grid, cell, mthz::Vec3(-half, 0, -half), m);
+	SplitMix64 rng(seed);
+	int L = grid / 4 + 2;
+	std::vector<double> lat(L * L);
+	for (double& v : lat) v = rng.uniform(-1.0, 1.0);
+	auto smooth = [](double t) { return t * t * (3 - 2 * t); };
+	for (int j = 0; j <= grid; j++)
+		for (int i = 0; i <= grid; i++) {
+			double gx = i / 4.0, gz = j / 4.0;
+			int ix = (int)gx, iz = (int)gz;
+			double fx = smooth(gx - ix), fz = smooth(gz - iz);
+			double a = lat[ix + L * iz], b = lat[ix + 1 + L * iz], c = lat[ix + L * (iz + 1)], d = lat[ix + 1 + L * (iz + 1)];
+			double h = (a * (1 - fx) + b * fx) * (1 - fz) + (c * (1 - fx) + d * fx) * fz;
+			in.points[i + (grid + 1) * j].y = amp * h;
+		}
+	g_mesh_input = in;
+	RigidBody* g = E.createRigidBody(phyz::StaticMeshGeometry(in), true);
+	g->setCOMType(RigidBody::PHYSICALLY_BASED);
+	g->translateSoCOMAtPosition(mthz::Vec3(0, 0, 0));
+	pin_body(g);
+	g_mesh = g;
+	double y0 = amp + 0.6;
+	int idx = 0;
+	for (int j = 0; j < ny; j++)
+		for (int i = 0; i < nx; i++)
+			for (int k = 0; k < nz; k++) {
+				double cx = -0.5 * 1.1 * (nx - 1) + 1.1 * i + rng.uniform(-0.05, 0.05);
+				double cy = y0 + 1.1 * j + rng.uniform(-0.05, 0.05);
+				double cz = -0.5 * 1.1 * (nz - 1) + 1.1 * k + rng.uniform(-0.05, 0.05);
+				double yaw = rng.uniform(0.0, 2.0 * 3.14159265358979323846);
+				double r = rng.uniform(0.3, 0.5);
+				RigidBody* b;
+				int kind = idx++ % 3;
+				if (kind == 0) {
+					b = E.createRigidBody(phyz::ConvexUnionGeometry::box(mthz::Vec3(cx - 0.5, cy - 0.5, cz - 0.5), 1, 1, 1, m));
+					b->setOrientation(mthz::Quaternion(yaw, mthz::Vec3(0, 1, 0)));
+				} else if (kind == 1) {
+					b = E.createRigidBody(phyz::ConvexUnionGeometry::sphere(mthz::Vec3(cx, cy, cz), r, m));
+				} else {
+					double cr = 0.2 + (r - 0.3);
+					double h = rng.uniform(0.5, 1.0);
+					double tilt = rng.uniform(0.0, 3.14159265358979323846);
+					b = E.createRigidBody(phyz::ConvexUnionGeometry::cylinder(mthz::Vec3(cx, cy - h / 2.0, cz), cr, h, m));
+					b->setOrientation(mthz::Quaternion(tilt, mthz::Vec3(std::cos(yaw), 0, std::sin(yaw))));
+				}
+				pin_body(b);
+			}
+	const double r = 0.15, h = 0.8, gap = 0.1, PI_ = 3.14159265358979323846;
+	mthz::Quaternion q(-PI_ / 2.0, mthz::Vec3(0, 0, 1));
+	for (int c = 0; c < n_chains; c++) {
+		double yc = y0 + 1.1 * ny + rng.uniform(0.0, 0.4), x0 = -0.5 * n_links * (h + gap) + rng.uniform(-0.3, 0.3);
+		double z0 = -0.5 * 1.2 * (n_chains - 1) + 1.2 * c;
+		RigidBody* prev = nullptr;
+		mthz::Vec3 prev_c;
+		for (int i = 0; i < n_links; i++) {
+			mthz::Vec3 cc(x0 + i * (h + gap), yc, z0);
+			RigidBody* b = E.createRigidBody(phyz::ConvexUnionGeometry::cylinder(cc - mthz::Vec3(0, h / 2.0, 0), r, h, m));
+			b->setOrientation(q);
+			b->setVel(mthz::Vec3(rng.uniform(-0.5, 0.5), rng.uniform(-0.5, 0.5), rng.uniform(-0.5, 0.5)));
+			pin_body(b);
+			if (prev) {
+				mthz::Vec3 a1 = prev_c + mthz::Vec3(0, h / 2.0 + gap / 2.0, 0);
+				mthz::Vec3 a2 = cc - mthz::Vec3(0, h / 2.0 + gap / 2.0, 0);
+				bool hinge = i % 3 == 0;
+				phyz::ConstraintID id = hinge ? E.addHingeConstraint(prev, b, a1, a2, mthz::Vec3(0, 0, 1), mthz::Vec3(0, 0, 1))
+				                              : E.addBallSocketConstraint(prev, b, a1, a2);
+				auto* e = E.constraint_map[id.getID()];
+				phyz::PersistentConstraint* pc = nullptr;
+				for (auto* x : e->constraints)
+					if (x->getId() == id) pc = x;
+				phyz::Constraint* sc = hinge ? (phyz::Constraint*)&static_cast<phyz::Hinge*>(pc)->constraint
+				                             : (phyz::Constraint*)&static_cast<phyz::BallSocket*>(pc)->constraint;
+				g_joints.push_back({hinge ? 5 : 3, prev, b, pc, sc});
+			}
+			prev = b;
+			prev_c = cc;
+		}
+	}
+}
+
+// The mesh as built by StaticMeshGeometry: input points and triangles, then per
+// world triangle the derived data (normal, vertices, edge out directions,
+// Gauss region; vertex and edge ids, region size, concave neighbours).
+static void dump_mesh(Out& o, const phyz::MeshInput& in) {
+	std::vector<double> pts, tmat, tf;
+	std::vector<int64_t> tidx, ti;
+	for (auto& p : in.points) pts.insert(pts.end(), {p.x, p.y, p.z});
+	for (auto& t : in.triangle_indices) {
+		tidx.insert(tidx.end(), {(int64_t)t.i1, (int64_t)t.i2, (int64_t)t.i3});
+		tmat.insert(tmat.end(), {t.material.restitution, t.material.kinetic_friction_coeff, t.material.static_friction_coeff});
+	}
+	for (const phyz::StaticMeshFace& f : g_mesh->mesh.getTriangles()) {
+		auto put = [&](mthz::Vec3 v) { tf.insert(tf.end(), {v.x, v.y, v.z}); };
+		put(f.normal);
+		for (int k = 0; k < 3; k++) put(f.vertices[k].p);
+		for (int k = 0; k < 3; k++) put(f.edges[k].out_direction);
+		for (int k = 0; k < 3; k++) put(k < (int)f.gauss_region.size() ? f.gauss_region[k] : mthz::Vec3(0, 0, 0));
+		tf.insert(tf.end(), {f.aabb.min.x, f.aabb.min.y, f.aabb.min.z, f.aabb.max.x, f.aabb.max.y, f.aabb.max.z});
+		for (int k = 0; k < 3; k++) ti.push_back(f.vertices[k].id);
+		for (int k = 0; k < 3; k++) ti.push_back(f.edges[k].id);
+		ti.insert(ti.end(), {(int64_t)f.gauss_region.size(), (int64_t)f.concave_neighbor_count, (int64_t)f.id});
+	}
+	size_t T = in.triangle_indices.size();
+	o.f64("mesh_points", pts, {in.points.size(), 3});
+	o.i64("mesh_tris", tidx, {T, 3});
+	o.f64("mesh_tri_mat", tmat, {T, 3});
+	o.f64("mesh_tri_f", tf, {T, 36});
+	o.i64("mesh_tri_i", ti, {T, 9});
+}
+
+
+// terrain: mixed bodies and jointed chains on a heightfield mesh, trace dump.
+static int cmd_terrain(int argc, char** argv) {
+	int grid = (int)arg_d(argc, argv, "--grid", 12);
+	double cell = arg_d(argc, argv, "--cell", 0.5);
+	double amp = arg_d(argc, argv, "--amp", 0.3);
+	int n = (int)arg_d(argc, argv, "--n", 2);
+	int ny = (int)arg_d(argc, argv, "--ny", n);
+	int steps = (int)arg_d(argc, argv, "--steps", 10);
+	uint64_t seed = (uint64_t)arg_d(argc, argv, "--seed", 777);
+	SceneCfg cfg;
+	cfg.hol_iters = (int)arg_d(argc, argv, "--hol", 1);
+	PhysicsEngine E;
+	configure(E, cfg);
+	build_terrain(E, grid, cell, amp, n, ny, n, (int)arg_d(argc, argv, "--chains", 2), (int)arg_d(argc, argv, "--links", 4), seed);
+	Out o(arg_s(argc, argv, "--out", "out"));
+	o.meta("seed", (double)seed);
+	o.meta("grid", grid);
+	dump_mesh(o, g_mesh_input);
+	std::set<int> rec = arg_steps(argc, argv, "--record");
+	run_trace(E, cfg, steps, rec, o);
+	o.finish();
+	return 0;
+}
+
 // time: single-threaded wall time of the reference's own timeStep on a C0 pile.
 static int cmd_time(int argc, char** argv) {
 	int n = (int)arg_d(argc, argv, "--n", 10);
@@ -768,6 +944,7 @@ int main(int argc, char** argv) {
 	std::string cmd = argv[1];
 	if (cmd == "pile") return cmd_pile(argc, argv);
 	if (cmd == "chain") return cmd_chain(argc, argv);
+	if (cmd == "terrain") return cmd_terrain(argc, argv);
 	if (cmd == "time") return cmd_time(argc, argv);
 	fprintf(stderr, "unknown command %s\n", cmd.c_str());
 	return 2;

@@ -46,6 +46,8 @@ class Fixture:
         dict(cylinder=[c xyz, axis xyz, r, h], gauss=[14,3], top=[8,3], bot=[8,3])
         for cylinders."""
         kind = int(self["body_info"][b, 4])
+        if kind == -1 and "mesh_points" in self:
+            return dict(mesh=self["mesh_points"], tris=self["mesh_tris"], tri_mat=self["mesh_tri_mat"])
         if kind == 1:
             return dict(sphere=self["sphere"][b])
         if kind == 3:
@@ -150,6 +152,13 @@ def oracle_lib():
         lib.orc_get_joint_rows.restype = ctypes.c_int
         lib.orc_get_joint_rows.argtypes = [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 4
         lib.orc_get_hsys.argtypes = [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 5
+        lib.orc_add_mesh_body.restype = ctypes.c_int
+        lib.orc_add_mesh_body.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+        lib.orc_mesh_query.restype = ctypes.c_int
+        lib.orc_mesh_query.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+        lib.orc_mesh_triangle.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+        lib.orc_narrowphase_multi.restype = ctypes.c_int
+        lib.orc_narrowphase_multi.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 4
         lib.orc_last_islands.restype = ctypes.c_int
         lib.orc_last_islands.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
         _lib = lib
@@ -194,6 +203,11 @@ class Oracle:
         self._keep = []
         for b in range(self.n):
             h = hulls[b]
+            if "mesh" in h:
+                arrs = [_c(h["mesh"], np.float64), _c(h["tris"], np.int64), _c(h["tri_mat"], np.float64)]
+                self._keep.append(arrs)
+                lib.orc_add_mesh_body(self.w, len(arrs[0]), arrs[0].ctypes.data, len(arrs[1]), arrs[1].ctypes.data, arrs[2].ctypes.data)
+                continue
             if "cylinder" in h:
                 arrs = [_c(mp[b], np.float64), _c(mat[b], np.float64), _c(h["cylinder"], np.float64), _c(h["gauss"], np.float64),
                         _c(h["top"], np.float64), _c(h["bot"], np.float64)]
@@ -295,6 +309,32 @@ class Oracle:
         k = self.lib.orc_broadphase(self.w, out.ctypes.data, cap)
         assert k <= cap
         return out[:k]
+
+    def mesh_query(self, body, box):
+        box = _c(box, np.float64)
+        out = np.zeros(1 << 16, np.int64)
+        k = self.lib.orc_mesh_query(self.w, body, box.ctypes.data, out.ctypes.data, len(out))
+        return out[:k]
+
+    def mesh_triangle(self, body, t):
+        f = np.zeros(40)
+        i = np.zeros(8, np.int64)
+        self.lib.orc_mesh_triangle(self.w, body, t, f.ctypes.data, i.ctypes.data)
+        return f, i
+
+    def narrowphase_multi(self, lo, hi, cap=64):
+        heads = np.zeros((cap, 4))
+        counts = np.zeros(cap, np.int64)
+        pts = np.zeros((cap * 16, 4))
+        mg = np.zeros((cap * 16, 2), np.uint64)
+        k = self.lib.orc_narrowphase_multi(self.w, lo, hi, cap, heads.ctypes.data, counts.ctypes.data, pts.ctypes.data, mg.ctypes.data)
+        assert k <= cap
+        out, off = [], 0
+        for j in range(k):
+            n = int(counts[j])
+            out.append((heads[j], pts[off:off + n], mg[off:off + n]))
+            off += n
+        return out
 
     def narrowphase(self, lo, hi):
         head = np.zeros(4)

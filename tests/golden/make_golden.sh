@@ -8,7 +8,7 @@ ROOT=$(cd "$HERE/../.." && pwd)
 make -C "$ROOT/oracle" ref REF="${REF:-/root/reference}"
 H="$ROOT/oracle/_ref/cp_ref_harness"
 cd "$HERE"
-rm -rf pile3 c0_full mix3 cyl3 cyl4 chain3 chain3h0
+rm -rf pile3 c0_full mix3 cyl3 cyl4 chain3 chain3h0 terrain
 # C0 pattern, 3x3x3 cubes: full per-step dumps (state, caches, manifolds, islands).
 "$H" pile --n 3 --steps 40 --record 0,1,2,10,39 --out pile3
 # C0 in full (10x10x10 cubes): first step and a settled step (no mid/post caches).
@@ -24,3 +24,7 @@ rm -rf pile3 c0_full mix3 cyl3 cyl4 chain3 chain3h0
 # block solver.
 "$H" chain --chains 3 --links 6 --steps 100 --record 1,2,45,70,99 --seed 4242 --out chain3
 "$H" chain --chains 3 --links 6 --steps 100 --record 1,45,70,99 --seed 4242 --hol 0 --out chain3h0
+# Heightfield terrain (12 x 12 cells, 288 triangles, value-noise heights)
+# under a box/sphere/cylinder pile and two jointed chains: convex-triangle SAT,
+# several manifolds per body pair, the mesh AABB tree's candidate order.
+"$H" terrain --grid 12 --cell 0.5 --amp 0.3 --n 2 --chains 2 --links 4 --steps 240 --record 1,60,120,180,239 --out terrain
