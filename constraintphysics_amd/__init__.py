@@ -83,6 +83,8 @@ def lib():
         "cp_world_last_islands": (_I, [_P, _I, _P, _I, _P, _P, _P]),
         "cp_world_last_timing": (_I, [_P, _P]),
         "cp_world_last_residual": (_I, [_P, _P]),
+        "cp_shape_add_mesh": (_I, [_P, _I, _P, _I, _P, _P, _P]),
+        "cp_narrowphase_multi": (_I, [_P, _I, _P, _I, _P, _P, _P, _P, _P, _P]),
         "cp_joint_add": (_I, [_P, _I, _I, _I, _P, _P]),
         "cp_world_joint_count": (_I, [_P, _P]),
         "cp_world_set_joint_state": (_I, [_P, _P]),
@@ -250,6 +252,32 @@ class World:
         sid = ctypes.c_int()
         _chk(self.L.cp_shape_add_cylinder(self.w, *[a.ctypes.data for a in arr], ctypes.byref(sid)), "cp_shape_add_cylinder")
         return sid.value
+
+    def add_mesh(self, points, tris, tri_mat):
+        """Static triangle mesh: points [P,3], triangles [T,3] (point indices),
+        per-triangle (restitution, kinetic mu, static mu) [T,3]."""
+        p = _a(points, np.float64).reshape(-1, 3)
+        t = _a(tris, np.int64).reshape(-1, 3)
+        m = _a(tri_mat, np.float64).reshape(-1, 3)
+        sid = ctypes.c_int()
+        _chk(self.L.cp_shape_add_mesh(self.w, len(p), p.ctypes.data, len(t), t.ctypes.data, m.ctypes.data, ctypes.byref(sid)),
+             "cp_shape_add_mesh")
+        return sid.value
+
+    def narrowphase_multi(self, pairs, cap=None):
+        """Every manifold of the given (id_lo, id_hi) pairs: list of
+        (pair index, normal+depth[4], points[n,4], magic[n,2])."""
+        pairs = _a(pairs, np.int64).reshape(-1, 2)
+        cap = cap or max(64, 4 * len(pairs))
+        idx = np.zeros(cap, np.int64)
+        counts = np.zeros(cap, np.int32)
+        heads = np.zeros((cap, 4))
+        pts = np.zeros((cap, 4, 4))
+        mg = np.zeros((cap, 4, 2), np.uint64)
+        n = ctypes.c_int()
+        _chk(self.L.cp_narrowphase_multi(self.w, len(pairs), pairs.ctypes.data, cap, idx.ctypes.data, counts.ctypes.data, heads.ctypes.data,
+                                         pts.ctypes.data, mg.ctypes.data, ctypes.byref(n)), "cp_narrowphase_multi")
+        return [(int(idx[k]), heads[k], pts[k, :counts[k]], mg[k, :counts[k]]) for k in range(n.value)]
 
     def add_polyhedron(self, pts, faces):
         pts = _a(pts, np.float64)

@@ -21,7 +21,7 @@
 
 namespace cpx {
 
-enum ShapeType { SHAPE_HULL = 0, SHAPE_SPHERE = 1, SHAPE_CYLINDER = 2 };
+enum ShapeType { SHAPE_HULL = 0, SHAPE_SPHERE = 1, SHAPE_CYLINDER = 2, SHAPE_MESH = 3 };
 
 struct Shapes {
 	int n;
@@ -107,7 +107,39 @@ struct Manifolds {
 	double* maxpen;   // [cap]
 	double* pts;      // [cap*4*4]: pos xyz, pen
 	uint64_t* magic;  // [cap*4*2]: cid, bid
-	double* mat;      // [3][cap]: restitution, mu_k, mu_s of the pair
+	double* mat;      // [cap*4][3]: restitution, mu_k, mu_s of each point
+};
+
+// Static triangle meshes in world coordinates, one instance per mesh body
+// (records as laid out by place_mesh: triangle MT_REC doubles / MT_IREC ints,
+// tree nodes box[6] + (left, right), a leaf being (-1, triangle)); global
+// triangle and node indices, per instance the root node.
+struct MeshDev {
+	int n_inst;
+	const int* inst_of_body;  // [nb], -1 for other bodies
+	const int* root;          // [n_inst]
+	const double* tri;        // [T][MT_REC]
+	const int* tri_i;         // [T][MT_IREC]
+	const double* node_box;   // [N][6]
+	const int* node_lr;       // [N][2]
+};
+#define MT_REC 40
+#define MT_IREC 8
+#define MESH_SLOTS 16   // manifold slots per convex-body / mesh pair
+#define MESH_STACK 64   // tree traversal stack (depth + 1 entries suffice)
+
+// Per (mesh pair, candidate triangle) manifold before merging.
+#define MESH_PTS 40     // points of one triangle manifold (the clip pool bound)
+struct MeshItems {
+	int* slot;        // [C] first manifold slot of the pair
+	int* tri;         // [C] global triangle index
+	int* count;       // [C] points (0 = no manifold)
+	double* head;     // [C][4] normal, max depth
+	double* pts;      // [C][MESH_PTS][4]
+	uint64_t* magic;  // [C][MESH_PTS][2]
+	double* mat;      // [C][3]
+	int* owner;       // [C] merge scratch
+	uint64_t* avail;  // [C] cull scratch
 };
 
 // Contact cache (SoA).  Row data is rebuilt each substep for live contacts.
@@ -170,9 +202,24 @@ void launch_bp_gather(const Bodies& b, const int* sorted, double* sb, hipStream_
 void launch_bp_sweep(int n, const int* sorted, const double* sb, const int* scene_of_body, const int* scene_body_off, int* counts,
                      const int* offs, uint64_t* out, int write, hipStream_t s);
 // lists: 6 * npairs ints (per-class pair lists), counts: 8 ints.
-// nocol: sorted pair keys (lo << 32 | hi) that never collide (jointed bodies).
+// nocol: sorted pair keys (lo << 32 | hi) that never collide (jointed bodies);
+// rank (nullable): slot rank of each pair (mesh pairs own MESH_SLOTS
+// consecutive slots, rank 0 runs the test, the rest start empty).
 void launch_narrowphase(const Bodies& b, const Shapes& sh, const Params& p, const uint64_t* pairs, int npairs, const Manifolds& m,
-                        int cap, int* lists, int* counts, const uint64_t* nocol, int n_nocol, hipStream_t s);
+                        int cap, int* lists, int* counts, const uint64_t* nocol, int n_nocol, const int* rank, hipStream_t s);
+// Mesh pairs: slot expansion (count pass, then write pass at the scanned
+// offsets), candidate triangles per pair (same two passes), one wave per
+// candidate, then per pair the merge and cull into its slots.
+void launch_mesh_expand(const Bodies& b, const MeshDev& md, const uint64_t* pairs, int npairs, const uint64_t* nocol, int n_nocol,
+                        int* cnt, const int* off, uint64_t* out, int* rank, hipStream_t s);
+void launch_mesh_candidates(const Bodies& b, const MeshDev& md, const uint64_t* pairs, const int* list, const int* counts, int nlist,
+                            int* cnt, const int* off, MeshItems it, int* overflow, hipStream_t s);
+void launch_mesh_triangles(const Bodies& b, const Shapes& sh, const MeshDev& md, const Params& p, const uint64_t* pairs, MeshItems it,
+                           int nitems, hipStream_t s);
+void launch_mesh_merge(const Params& p, const uint64_t* pairs, const MeshDev& md, const int* list, int nlist, const int* off, MeshItems it,
+                       Manifolds m, int cap, int* overflow, hipStream_t s);
+// Class list index of mesh pairs in the narrowphase lists / counts.
+#define NPC_MESH_LIST 6
 void launch_np_gather(const Manifolds& M, int mcap, const uint64_t* pairs, int npairs, const int* off, const NewPts& np, int cap,
                       hipStream_t s);
 void launch_np_match(const NewPts& np, int nnew, int cap, const Contacts& C, int nold, const Bodies& b, const Params& P,

@@ -125,6 +125,7 @@ __global__ void k_refresh_geometry(Bodies b, Shapes sh, int only_moved) {
 		stq(b.q, i, n, q);
 	}
 	int s = b.shape[i];
+	if (sh.type[s] == SHAPE_MESH) return;  // placed on the host with the mesh tree
 	M3 R = q_rot(q);
 	const double inf = __builtin_inf();
 	double mn0 = inf, mn1 = inf, mn2 = inf, mx0 = -inf, mx1 = -inf, mx2 = -inf;

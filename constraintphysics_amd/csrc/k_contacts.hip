@@ -37,7 +37,7 @@ __global__ void k_np_gather(Manifolds M, int mcap, const uint64_t* pairs, int np
 		for (int d = 0; d < 3; d++) {
 			np.pos[d * cap + j] = M.pts[(size_t)s * 16 + 4 * k + d];
 			np.nrm[d * cap + j] = M.normal[d * mcap + s];
-			np.mat[d * cap + j] = M.mat[d * mcap + s];
+			np.mat[d * cap + j] = M.mat[(size_t)s * 12 + 3 * k + d];
 		}
 		np.pen[j] = M.pts[(size_t)s * 16 + 4 * k + 3];
 	}

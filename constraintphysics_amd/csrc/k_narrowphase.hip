@@ -473,9 +473,15 @@ __device__ Check eval_axis_sphere(const Hull& A, V3 c, double r, int t, bool* va
 
 __device__ void write_mat(const Bodies& b, int ia, int ib, Manifolds& M, int slot, int cap) {  // CollisionDetect.cpp:820-824
 	double ra = b.mat[ia], rb = b.mat[ib];
-	M.mat[slot] = (ra < rb) ? rb : ra;
-	M.mat[cap + slot] = (b.mat[b.n + ia] + b.mat[b.n + ib]) / 2.0;
-	M.mat[2 * cap + slot] = (b.mat[2 * b.n + ia] + b.mat[2 * b.n + ib]) / 2.0;
+	double r = (ra < rb) ? rb : ra;
+	double mk = (b.mat[b.n + ia] + b.mat[b.n + ib]) / 2.0;
+	double ms = (b.mat[2 * b.n + ia] + b.mat[2 * b.n + ib]) / 2.0;
+	double* o = M.mat + (size_t)slot * 12;
+	for (int k = 0; k < 4; k++) {
+		o[3 * k] = r;
+		o[3 * k + 1] = mk;
+		o[3 * k + 2] = ms;
+	}
 }
 
 // Wave-parallel SAT_PolySphere; poly = body ip, sphere = body is_.  When the
@@ -871,7 +877,9 @@ __device__ void sanitize_slot(Manifolds& M, int slot, int cap) {
 	for (int k = 0; k < n; k++) {
 		if (!(isfinite(p[4 * k]) && isfinite(p[4 * k + 1]) && isfinite(p[4 * k + 2]) && isfinite(p[4 * k + 3]))) continue;
 		if (o != k) {
+			double* mt = M.mat + (size_t)slot * 12;
 			for (int c = 0; c < 4; c++) p[4 * o + c] = p[4 * k + c];
+			for (int c = 0; c < 3; c++) mt[3 * o + c] = mt[3 * k + c];
 			g[2 * o] = g[2 * k];
 			g[2 * o + 1] = g[2 * k + 1];
 		}
@@ -884,7 +892,7 @@ __device__ void sanitize_slot(Manifolds& M, int slot, int cap) {
 // Pairs are sorted into per-class lists first, so that each class runs in a
 // kernel of its own shape: sphere-sphere and sphere-cylinder one lane per
 // pair, the SAT classes one wave per pair.
-enum { NPC_HH = 0, NPC_HS, NPC_SS, NPC_HC, NPC_SC, NPC_CC, NPC_N };
+enum { NPC_HH = 0, NPC_HS, NPC_SS, NPC_HC, NPC_SC, NPC_CC, NPC_MESH, NPC_N };
 
 __device__ __forceinline__ void wave_fence() {
 	__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -910,7 +918,7 @@ __device__ __forceinline__ bool sorted_has(const uint64_t* v, int n, uint64_t x)
 }
 
 __global__ void k_np_classify(Bodies b, Shapes sh, const uint64_t* pairs, int npairs, Manifolds M, int* lists, int* counts,
-                              const uint64_t* nocol, int n_nocol) {
+                              const uint64_t* nocol, int n_nocol, const int* rank) {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
 	int cls = -1;
 	if (i < npairs) {
@@ -923,7 +931,8 @@ __global__ void k_np_classify(Bodies b, Shapes sh, const uint64_t* pairs, int np
 		if (allowed && !(b.movement[ia] != 0 && b.movement[ib] != 0)) {
 			int ta = sh.type[b.shape[ia]], tb = sh.type[b.shape[ib]];
 			int lo = ta < tb ? ta : tb, hi = ta < tb ? tb : ta;
-			if (lo == SHAPE_HULL) cls = hi == SHAPE_HULL ? NPC_HH : hi == SHAPE_SPHERE ? NPC_HS : NPC_HC;
+			if (hi == SHAPE_MESH) cls = (lo != SHAPE_MESH && (!rank || rank[i] == 0)) ? NPC_MESH : -1;
+			else if (lo == SHAPE_HULL) cls = hi == SHAPE_HULL ? NPC_HH : hi == SHAPE_SPHERE ? NPC_HS : NPC_HC;
 			else if (lo == SHAPE_SPHERE) cls = hi == SHAPE_SPHERE ? NPC_SS : NPC_SC;
 			else cls = NPC_CC;
 		}
@@ -1181,11 +1190,511 @@ __global__ void k_np_sphere_pairs(Bodies b, Shapes sh, Params P, const uint64_t*
 	}
 }
 
+// ------------------------------------------------------------- static meshes
+// A convex body against a static triangle mesh (SAT_{Poly,Sphere,Cylinder}Mesh,
+// CollisionDetect.cpp:2647-2778, with the engine's merge and cull,
+// PhysicsEngine.cpp:223-277): the triangles whose boxes meet the body's box,
+// in the order of the mesh AABB tree's query; one wave per triangle runs the
+// SAT (keeping both the overall and the Gauss-valid minimum) and lane 0 the
+// contact generation; then per pair the near-parallel manifolds are merged
+// into the first and each is culled to four points into the pair's slots.
+struct Tri {
+	const double* r;  // MT_REC record
+	const int* ri;    // MT_IREC record
+};
+__device__ __forceinline__ Tri make_tri(const MeshDev& md, int t) {
+	return Tri{md.tri + (size_t)t * MT_REC, md.tri_i + (size_t)t * MT_IREC};
+}
+__device__ __forceinline__ V3 tri_n(const Tri& t) { return ldv(t.r, 0); }
+__device__ __forceinline__ V3 tri_v(const Tri& t, int k) { return ldv(t.r + 3, k); }
+__device__ __forceinline__ V3 tri_out(const Tri& t, int k) { return ldv(t.r + 12, k); }
+__device__ __forceinline__ V3 tri_g(const Tri& t, int k) { return ldv(t.r + 21, k); }
+
+__device__ Ext tri_extrema(const Tri& t, V3 dir) {  // findTriangleExtrema (CollisionDetect.cpp:2004-2021)
+	Ext e{-1, -1, __builtin_inf(), -__builtin_inf()};
+	for (int i = 0; i < 3; i++) {
+		double v = dot(tri_v(t, i), dir);
+		if (v < e.min_val) { e.min_pid = i; e.min_val = v; }
+		if (v > e.max_val) { e.max_pid = i; e.max_val = v; }
+	}
+	return e;
+}
+
+// normalDirectionValid (CollisionDetect.cpp:2024-2060).
+__device__ bool tri_normal_valid(const Tri& t, V3 n) {
+	const double EPS = 0.0001;
+	int ng = t.ri[3];
+	switch (t.ri[4]) {
+	case 0:
+	case 1:
+		for (int i = 0; i < ng; i++) {
+			V3 inner = cross(tri_g(t, i), tri_g(t, (i + 1) % ng));
+			if (dot(n, inner) < -EPS) return false;
+		}
+		break;
+	case 2: {
+		V3 an = cross(tri_g(t, 0), tri_g(t, 1));
+		if (fabs(dot(n, an)) > EPS) return false;
+		if (dot(n, cross(an, tri_g(t, 0))) < -EPS) return false;
+		if (dot(n, cross(tri_g(t, 1), an)) < -EPS) return false;
+		break;
+	}
+	case 3:
+		if (dot(n, tri_n(t)) < 1 - EPS) return false;
+		break;
+	}
+	return true;
+}
+
+// findTriangleContactArea (CollisionDetect.cpp:282-329).
+__device__ int tri_contact_area(const Tri& t, V3 n, V3 p, int pid, V3 u, V3 w, double cos_tol, double sin_tol, CEP* out, int* surface) {
+	if (1 - dot(tri_n(t), n) <= cos_tol) {
+		for (int i = 0; i < 3; i++) {
+			V3 v = tri_v(t, i);
+			out[i].x = dot(v, u);
+			out[i].y = dot(v, w);
+			out[i].src = i;
+		}
+		*surface = t.ri[5];
+		return 3;
+	}
+	*surface = -1;
+	for (int i = 0; i < 3; i++) {
+		if (i != pid && (i + 1) % 3 != pid) continue;
+		V3 p1 = tri_v(t, i), p2 = tri_v(t, (i + 1) % 3);
+		if (fabs(dot(normalize(p2 - p1), n)) <= sin_tol) {
+			out[0].x = dot(p1, u); out[0].y = dot(p1, w); out[0].src = t.ri[i];
+			out[1].x = dot(p2, u); out[1].y = dot(p2, w); out[1].src = t.ri[i];
+			return 2;
+		}
+	}
+	out[0].x = dot(p, u);
+	out[0].y = dot(p, w);
+	out[0].src = pid;
+	return 1;
+}
+
+// Arc k of a triangle's Gauss map: normal to edge e's outward direction, then
+// on to the reversed normal (CollisionDetect.cpp:2097-2103), e = k / 2.
+__device__ __forceinline__ void tri_arc(const Tri& t, int k, V3* g1, V3* g2) {
+	int e = k >> 1;
+	if ((k & 1) == 0) { *g1 = tri_n(t); *g2 = tri_out(t, e); }
+	else { *g1 = tri_out(t, e); *g2 = -tri_n(t); }
+}
+
+// Candidate axis t of SAT_PolyTriangle (:2062-2160): the triangle normal, the
+// poly's non-redundant face normals, then poly arc x triangle arc pairs.
+__device__ Check eval_axis_pt(const Hull& A, const Tri& T, int t, bool* valid) {
+	*valid = true;
+	if (t == 0) {
+		V3 n = tri_n(T);
+		return sat_check(find_extrema(A, n), tri_extrema(T, n), n);
+	}
+	t -= 1;
+	if (t < A.nf) {
+		if (A.red[t]) { *valid = false; return Check{}; }
+		V3 n = ldv(A.wn, t);
+		return sat_check(recentered(A, t), tri_extrema(T, n), n);
+	}
+	t -= A.nf;
+	int i1 = t / 6, k = t - 6 * i1;
+	V3 g1, g2, n;
+	tri_arc(T, k, &g1, &g2);
+	if (!arc_axis(ldv(A.wn, A.arcs[2 * i1]), ldv(A.wn, A.arcs[2 * i1 + 1]), -g1, -g2, &n)) { *valid = false; return Check{}; }
+	return sat_check(find_extrema(A, n), tri_extrema(T, n), n);
+}
+// Candidate axis t of SAT_SphereTriangle (:2196-2250): the triangle normal,
+// vertex directions, edge normals; *feat = the axis' sphere feature id.
+__device__ Check eval_axis_st(V3 c, double r, const Tri& T, int t, uint32_t* feat) {
+	V3 n;
+	if (t == 0) {
+		n = tri_n(T);
+		*feat = 0xFFFFFFFFu;
+	} else if (t < 4) {
+		n = normalize(tri_v(T, t - 1) - c);
+		*feat = (uint32_t)(t - 1);
+	} else {
+		int e = t - 4;
+		V3 p1 = tri_v(T, e), p2 = tri_v(T, (e + 1) % 3);
+		V3 dir = normalize(p2 - p1);
+		V3 sample = p1 - c;
+		n = normalize(sample - dir * dot(dir, sample));
+		*feat = (uint32_t)T.ri[e];
+	}
+	return sat_check(sphere_extrema(c, r, n), tri_extrema(T, n), n);
+}
+// Candidate axis t of SAT_CylinderTriangle (:2466-2596): the height axis, the
+// triangle normal, triangle edge x axis, vertex-to-axis directions, then
+// cylinder arc x triangle arc pairs.
+__device__ Check eval_axis_ct(const Cyl& A, const Tri& T, int t, bool* valid) {
+	*valid = true;
+	V3 n;
+	if (t == 0) n = A.ax;
+	else if (t == 1) n = tri_n(T);
+	else if (t < 5) {
+		int e = t - 2;
+		V3 dir = cross(tri_v(T, (e + 1) % 3) - tri_v(T, e), A.ax);
+		if (mag(dir) < 0.00000000001) { *valid = false; return Check{}; }
+		n = normalize(dir);
+	} else if (t < 8) {
+		V3 diff = tri_v(T, t - 5) - A.c;
+		n = normalize(diff - A.ax * dot(A.ax, diff));
+	} else {
+		t -= 8;
+		int k1 = t / 6, k = t - 6 * k1;
+		V3 g1, g2;
+		tri_arc(T, k, &g1, &g2);
+		if (!arc_axis(cyl_gv(A, k1 & 1), cyl_gv(A, k1 >> 1), -g1, -g2, &n)) { *valid = false; return Check{}; }
+	}
+	return sat_check(cyl_extrema(A, n), tri_extrema(T, n), n);
+}
+
+// Wave-parallel triangle SAT: any separating axis ends the test (false);
+// otherwise *bi is the first strict minimum over all axes and *vi the first
+// strict minimum over the axes whose reversed normal is Gauss-valid for the
+// triangle (0x7FFFFFFF if none).
+template <class Eval>
+__device__ bool wave_sat_tri(int naxes, int lane, const Tri& T, Eval eval, int* bi_out, double* bp_out, int* vi_out, double* vp_out) {
+	double bp = __builtin_inf(), vp = __builtin_inf();
+	int bi = 0x7FFFFFFF, vi = 0x7FFFFFFF;
+	for (int base = 0; base < naxes; base += 64) {
+		int t = base + lane;
+		int sep = 0;
+		if (t < naxes) {
+			bool v;
+			Check x = eval(t, &v);
+			if (v) {
+				if (x.pen < 0) sep = 1;
+				if (x.pen < bp) { bp = x.pen; bi = t; }
+				if (x.pen < vp && tri_normal_valid(T, -x.n)) { vp = x.pen; vi = t; }
+			}
+		}
+		if (__ballot(sep) != 0) return false;
+	}
+	for (int off = 32; off >= 1; off >>= 1) {
+		double obp = __shfl_xor(bp, off), ovp = __shfl_xor(vp, off);
+		int obi = __shfl_xor(bi, off), ovi = __shfl_xor(vi, off);
+		if (obp < bp || (obp == bp && obi < bi)) { bp = obp; bi = obi; }
+		if (ovp < vp || (ovp == vp && ovi < vi)) { vp = ovp; vi = ovi; }
+	}
+	*bi_out = bi;
+	*bp_out = bp;
+	*vi_out = vi;
+	*vp_out = vp;
+	return bi != 0x7FFFFFFF;
+}
+
+__device__ __forceinline__ bool box_overlap(const double* a, const double* b6, int bs) {  // AABB::intersects (AABB.h:10-14)
+	return !(a[3] < b6[0] || a[0] > b6[3 * bs] || a[4] < b6[bs] || a[1] > b6[4 * bs] || a[5] < b6[2 * bs] || a[2] > b6[5 * bs]);
+}
+
+// Slots of each pair: MESH_SLOTS for a convex body against a mesh that may
+// collide (one dynamic body, not a jointed pair), else 1.
+__global__ void k_mesh_expand(Bodies b, MeshDev md, const uint64_t* pairs, int npairs, const uint64_t* nocol, int n_nocol, int* cnt,
+                              const int* off, uint64_t* out, int* rank) {
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= npairs) return;
+	uint64_t key = pairs[i];
+	int c = 1;
+	if (!off) {
+		int ia = (int)(key >> 32), ib = (int)(key & 0xFFFFFFFFu);
+		bool ma = md.inst_of_body[ia] >= 0, mb = md.inst_of_body[ib] >= 0;
+		bool allowed = n_nocol == 0 || !sorted_has(nocol, n_nocol, key);
+		if (ma != mb && allowed && (b.movement[ia] == 0 || b.movement[ib] == 0)) c = MESH_SLOTS;
+		cnt[i] = c;
+		return;
+	}
+	c = cnt[i];
+	for (int r = 0; r < c; r++) {
+		out[off[i] + r] = key;
+		rank[off[i] + r] = r;
+	}
+}
+
+// Candidate triangles of each mesh pair (AABBTree::getCollisionCandidatesWith,
+// AABB_Tree.h:226-247: depth first, right child first), counted (off null)
+// or written at off[j].
+__global__ void k_mesh_candidates(Bodies b, MeshDev md, const uint64_t* pairs, const int* list, const int* counts, int* cnt, const int* off,
+                                  MeshItems it, int* overflow) {
+	int j = blockIdx.x * blockDim.x + threadIdx.x;
+	if (j >= counts[NPC_MESH]) return;
+	int slot = list[j];
+	uint64_t key = pairs[slot];
+	int ia = (int)(key >> 32), ib = (int)(key & 0xFFFFFFFFu);
+	int m = md.inst_of_body[ia] >= 0 ? ia : ib, c = m == ia ? ib : ia;
+	const int N = b.n;
+	int inst = md.inst_of_body[m];
+	int stk[MESH_STACK];
+	int sp = 0, k = 0;
+	stk[sp++] = md.root[inst];
+	while (sp > 0) {
+		int n = stk[--sp];
+		const double* nb = md.node_box + 6 * (size_t)n;
+		bool hit = box_overlap(nb, b.aabb + c, N);
+		if (md.node_lr[2 * n] < 0) {
+			if (hit) {
+				if (off) {
+					it.slot[off[j] + k] = slot;
+					it.tri[off[j] + k] = md.node_lr[2 * n + 1];
+				}
+				k++;
+			}
+		} else if (hit) {
+			if (sp + 2 > MESH_STACK) {
+				atomicAdd(overflow, 1);
+				break;
+			}
+			stk[sp++] = md.node_lr[2 * n];
+			stk[sp++] = md.node_lr[2 * n + 1];
+		}
+	}
+	if (!off) cnt[j] = k;
+}
+
+// One wave per candidate triangle: the convex body's SAT against the triangle
+// and, on lane 0, its contact points (convex body as the first shape).
+__global__ __launch_bounds__(64 * NP_WAVES) void k_mesh_triangles(Bodies b, Shapes sh, MeshDev md, Params P, const uint64_t* pairs,
+                                                                  MeshItems it, int nitems) {
+	__shared__ WaveLds lds[NP_WAVES];
+	const int lane = threadIdx.x & 63, wib = threadIdx.x >> 6;
+	WaveLds& L = lds[wib];
+	const int N = b.n;
+	for (int w = blockIdx.x * NP_WAVES + wib; w < nitems; w += gridDim.x * NP_WAVES) {
+		int slot = it.slot[w];
+		uint64_t key = pairs[slot];
+		int ia = (int)(key >> 32), ib = (int)(key & 0xFFFFFFFFu);
+		int c = md.inst_of_body[ia] >= 0 ? ib : ia;
+		Tri T = make_tri(md, it.tri[w]);
+		int ct = sh.type[b.shape[c]];
+		// non_gauss_valid_normal_penalty (CollisionDetect.cpp:2673, :2707, :2770)
+		double tl = T.r[33] - T.r[30], t2 = T.r[34] - T.r[31], t3 = T.r[35] - T.r[32];
+		double cl = b.aabb[3 * N + c] - b.aabb[c], c2 = b.aabb[4 * N + c] - b.aabb[N + c], c3 = b.aabb[5 * N + c] - b.aabb[2 * N + c];
+		double tlong = (tl < ((t2 < t3) ? t3 : t2)) ? ((t2 < t3) ? t3 : t2) : tl;
+		double clong = (cl < ((c2 < c3) ? c3 : c2)) ? ((c2 < c3) ? c3 : c2) : cl;
+		double small = (tlong < clong) ? tlong : clong;
+		double penalty = (ct == SHAPE_HULL ? 0 : 0.15) * small;
+		uint64_t ck = (uint64_t)(c + 1) * 16;
+		uint64_t cid = (0x00000000FFFFFFFFull & ck) | (0xFFFFFFFF00000000ull & ((uint64_t)T.ri[5] << 32));
+		int bi, vi;
+		double bp, vp;
+		bool hit;
+		Hull A;
+		Cyl C;
+		V3 sc;
+		double sr = 0;
+		if (ct == SHAPE_HULL) {
+			A = make_hull(b, sh, c);
+			stage_hull(A, L.pa, L.na, lane);
+			hit = wave_sat_tri(1 + A.nf + 6 * A.na, lane, T, [&](int t, bool* v) { return eval_axis_pt(A, T, t, v); }, &bi, &bp, &vi, &vp);
+		} else if (ct == SHAPE_CYLINDER) {
+			C = make_cyl(b, sh, c);
+			hit = wave_sat_tri(8 + 24 * 6, lane, T, [&](int t, bool* v) { return eval_axis_ct(C, T, t, v); }, &bi, &bp, &vi, &vp);
+		} else {
+			sc = ldv(b.wpts + 3 * b.wpt_off[c], 0);
+			sr = sh.radius[b.shape[c]];
+			hit = wave_sat_tri(7, lane, T, [&](int t, bool* v) {
+				uint32_t f;
+				*v = true;
+				return eval_axis_st(sc, sr, T, t, &f);
+			}, &bi, &bp, &vi, &vp);
+		}
+		if (lane == 0) {
+			int cnt = 0;
+			bool gauss = vp < bp + penalty;
+			double pen = gauss ? vp : bp;
+			if (hit && pen > 0) {
+				double* hp = it.head + 4 * (size_t)w;
+				double* pp = it.pts + (size_t)w * MESH_PTS * 4;
+				uint64_t* mg = it.magic + (size_t)w * MESH_PTS * 2;
+				bool vv;
+				if (ct == SHAPE_SPHERE) {
+					uint32_t fr, fm;
+					Check raw = eval_axis_st(sc, sr, T, bi, &fr);
+					Check mp = gauss ? eval_axis_st(sc, sr, T, vi, &fm) : raw;
+					V3 pos = sc + mp.n * sr;
+					pp[0] = pos.x; pp[1] = pos.y; pp[2] = pos.z; pp[3] = mp.pen;
+					mg[0] = cid;
+					mg[1] = (uint64_t)fr;
+					hp[0] = mp.n.x; hp[1] = mp.n.y; hp[2] = mp.n.z; hp[3] = mp.pen;
+					cnt = 1;
+				} else {
+					Check raw = ct == SHAPE_HULL ? eval_axis_pt(A, T, bi, &vv) : eval_axis_ct(C, T, bi, &vv);
+					Check mp = !gauss ? raw : (ct == SHAPE_HULL ? eval_axis_pt(A, T, vi, &vv) : eval_axis_ct(C, T, vi, &vv));
+					V3 n = mp.n;
+					V3 u, wv;
+					perp_basis(n, &u, &wv);
+					V3 bmax = tri_v(T, raw.b_pid);
+					V3 amax;
+					int na, sa, nb, sb, nr;
+					CEP* res;
+					nb = tri_contact_area(T, raw.n * -1.0, bmax, raw.b_pid, u, wv, P.cos_tol, P.sin_tol, L.cep + MAXP, &sb);
+					if (ct == SHAPE_HULL) {
+						amax = ldv(A.wp, raw.a_pid);
+						na = contact_area(A, raw.n, amax, raw.a_pid, u, wv, P.cos_tol, P.sin_tol, L.cep, &sa);
+						res = clip_areas(L.cep, na, sa, nb, sb, &nr);
+					} else {
+						amax = dot(C.ax, n) > 0 ? disk_extrema(cyl_top_center(C), C.ax, C.r, n) : disk_extrema(cyl_bot_center(C), C.ax, C.r, n);
+						int oa;
+						na = cyl_contact_area(C, raw.n, u, wv, P.cos_tol, P.sin_tol, L.cep, &sa, &oa);
+						if (oa == AREA_EDGE) {
+							res = L.cep;
+							res[0].magic = 0x0;
+							nr = 1;
+						} else res = clip_areas(L.cep, na, sa, nb, sb, &nr);
+					}
+					double a_dot = dot(amax, n);
+					V3 noff = n * a_dot;
+					if (nr > MESH_PTS) nr = MESH_PTS;
+					for (int q = 0; q < nr; q++) {
+						V3 pos = u * res[q].x + wv * res[q].y + noff;
+						pp[4 * q] = pos.x; pp[4 * q + 1] = pos.y; pp[4 * q + 2] = pos.z;
+						pp[4 * q + 3] = dot(pos, n) - a_dot + mp.pen;
+						mg[2 * q] = cid;
+						mg[2 * q + 1] = res[q].magic;
+					}
+					hp[0] = n.x; hp[1] = n.y; hp[2] = n.z; hp[3] = mp.pen;
+					cnt = nr;
+				}
+				// material of the points (CollisionDetect.cpp:2183-2187)
+				double ra = b.mat[c], rt = T.r[36];
+				it.mat[3 * (size_t)w] = (ra < rt) ? rt : ra;
+				it.mat[3 * (size_t)w + 1] = (b.mat[N + c] + T.r[37]) / 2.0;
+				it.mat[3 * (size_t)w + 2] = (b.mat[2 * N + c] + T.r[38]) / 2.0;
+			}
+			it.count[w] = cnt;
+		}
+		wave_fence();
+	}
+}
+
+// Per mesh pair: manifolds in triangle order, flipped when the mesh is the
+// pair's first body (CollisionDetect.cpp:171-195); each unmerged manifold
+// absorbs the later ones whose normal is within the cos tolerance of its
+// running normal (merge_manifold, :2780-2793: normal (n1 + n2) normalised,
+// points appended, deepest depth), then is culled to four points
+// (cull_manifold, :2795-2830) into the pair's next slot.
+__global__ void k_mesh_merge(Params P, const uint64_t* pairs, MeshDev md, const int* list, int nlist, const int* off, MeshItems it, Manifolds M,
+                             int cap, int* overflow) {
+	int j = blockIdx.x * blockDim.x + threadIdx.x;
+	if (j >= nlist) return;
+	int slot0 = list[j];
+	uint64_t key = pairs[slot0];
+	bool flip = md.inst_of_body[(int)(key >> 32)] >= 0;
+	int i0 = off[j], i1 = off[j + 1];
+	auto nrm = [&](int i) {
+		V3 n{it.head[4 * (size_t)i], it.head[4 * (size_t)i + 1], it.head[4 * (size_t)i + 2]};
+		return flip ? -n : n;
+	};
+	for (int i = i0; i < i1; i++) {
+		it.owner[i] = it.count[i] > 0 ? -1 : -2;
+		int c = it.count[i];
+		it.avail[i] = c >= 64 ? ~0ull : ((1ull << c) - 1);
+	}
+	int q = 0;
+	for (int a = i0; a < i1; a++) {
+		if (it.owner[a] != -1) continue;
+		it.owner[a] = a;
+		V3 n = nrm(a);
+		double mp = it.head[4 * (size_t)a + 3];
+		int total = it.count[a];
+		for (int bb = a + 1; bb < i1; bb++) {
+			if (it.owner[bb] != -1) continue;
+			V3 nb = nrm(bb);
+			if (1 - dot(n, nb) < P.cos_tol) {
+				n = normalize(n + nb);
+				double mb = it.head[4 * (size_t)bb + 3];
+				mp = (mp < mb) ? mb : mp;
+				it.owner[bb] = a;
+				total += it.count[bb];
+			}
+		}
+		if (q >= MESH_SLOTS) {
+			atomicAdd(overflow, 1);
+			continue;
+		}
+		int slot = slot0 + q++;
+		M.normal[slot] = n.x;
+		M.normal[cap + slot] = n.y;
+		M.normal[2 * cap + slot] = n.z;
+		double* mpts = M.pts + (size_t)slot * 16;
+		uint64_t* mmag = M.magic + (size_t)slot * 8;
+		double* mmat = M.mat + (size_t)slot * 12;
+		auto put = [&](int k, int i, int p) {
+			const double* src = it.pts + ((size_t)i * MESH_PTS + p) * 4;
+			for (int d = 0; d < 4; d++) mpts[4 * k + d] = src[d];
+			uint64_t c1 = it.magic[((size_t)i * MESH_PTS + p) * 2], c2 = it.magic[((size_t)i * MESH_PTS + p) * 2 + 1];
+			mmag[2 * k] = flip ? swap_halves(c1) : c1;
+			mmag[2 * k + 1] = flip ? swap_halves(c2) : c2;
+			for (int d = 0; d < 3; d++) mmat[3 * k + d] = it.mat[3 * (size_t)i + d];
+		};
+		if (total <= 4) {
+			int k = 0;
+			for (int i = a; i < i1; i++) {
+				if (it.owner[i] != a) continue;
+				for (int p = 0; p < it.count[i]; p++) put(k++, i, p);
+			}
+			M.count[slot] = total;
+			M.maxpen[slot] = mp;
+		} else {
+			V3 cu, cw;
+			perp_basis(n, &cu, &cw);
+			double maxpen = 0;
+			for (int k = 0; k < 4; k++) {
+				V3 t = cu * P.cull_c[k] + cw * P.cull_s[k];
+				double bv = -__builtin_inf();
+				int bi = -1, bpnt = -1;
+				for (int i = a; i < i1; i++) {
+					if (it.owner[i] != a) continue;
+					for (int p = 0; p < it.count[i]; p++) {
+						if (!((it.avail[i] >> p) & 1)) continue;
+						const double* src = it.pts + ((size_t)i * MESH_PTS + p) * 4;
+						double v = dot(V3{src[0], src[1], src[2]}, t);
+						if (v > bv) { bv = v; bi = i; bpnt = p; }
+					}
+				}
+				if (bi < 0) { bi = a; bpnt = 0; }
+				put(k, bi, bpnt);
+				double pp = it.pts[((size_t)bi * MESH_PTS + bpnt) * 4 + 3];
+				maxpen = (maxpen < pp) ? pp : maxpen;
+				it.avail[bi] &= ~(1ull << bpnt);
+			}
+			M.count[slot] = 4;
+			M.maxpen[slot] = maxpen;
+		}
+		if (P.defect_fixes & 1) sanitize_slot(M, slot, cap);
+	}
+}
+
+void launch_mesh_expand(const Bodies& b, const MeshDev& md, const uint64_t* pairs, int npairs, const uint64_t* nocol, int n_nocol,
+                        int* cnt, const int* off, uint64_t* out, int* rank, hipStream_t s) {
+	if (npairs > 0)
+		hipLaunchKernelGGL(k_mesh_expand, dim3((npairs + 255) / 256), dim3(256), 0, s, b, md, pairs, npairs, nocol, n_nocol, cnt, off, out,
+		                   rank);
+}
+void launch_mesh_candidates(const Bodies& b, const MeshDev& md, const uint64_t* pairs, const int* list, const int* counts, int nlist,
+                            int* cnt, const int* off, MeshItems it, int* overflow, hipStream_t s) {
+	if (nlist > 0)
+		hipLaunchKernelGGL(k_mesh_candidates, dim3((nlist + 127) / 128), dim3(128), 0, s, b, md, pairs, list, counts, cnt, off, it, overflow);
+}
+void launch_mesh_triangles(const Bodies& b, const Shapes& sh, const MeshDev& md, const Params& p, const uint64_t* pairs, MeshItems it,
+                           int nitems, hipStream_t s) {
+	if (nitems <= 0) return;
+	int blocks = (nitems + NP_WAVES - 1) / NP_WAVES;
+	if (blocks > 8192) blocks = 8192;
+	hipLaunchKernelGGL(k_mesh_triangles, dim3(blocks), dim3(64 * NP_WAVES), 0, s, b, sh, md, p, pairs, it, nitems);
+}
+void launch_mesh_merge(const Params& p, const uint64_t* pairs, const MeshDev& md, const int* list, int nlist, const int* off, MeshItems it,
+                       Manifolds m, int cap, int* overflow, hipStream_t s) {
+	if (nlist > 0)
+		hipLaunchKernelGGL(k_mesh_merge, dim3((nlist + 127) / 128), dim3(128), 0, s, p, pairs, md, list, nlist, off, it, m, cap, overflow);
+}
+
 void launch_narrowphase(const Bodies& b, const Shapes& sh, const Params& p, const uint64_t* pairs, int npairs, const Manifolds& m,
-                        int cap, int* lists, int* counts, const uint64_t* nocol, int n_nocol, hipStream_t s) {
+                        int cap, int* lists, int* counts, const uint64_t* nocol, int n_nocol, const int* rank, hipStream_t s) {
 	if (npairs <= 0) return;
 	hipLaunchKernelGGL(k_np_zero, dim3(1), dim3(64), 0, s, counts);
-	hipLaunchKernelGGL(k_np_classify, dim3((npairs + 255) / 256), dim3(256), 0, s, b, sh, pairs, npairs, m, lists, counts, nocol, n_nocol);
+	hipLaunchKernelGGL(k_np_classify, dim3((npairs + 255) / 256), dim3(256), 0, s, b, sh, pairs, npairs, m, lists, counts, nocol, n_nocol,
+	                   rank);
 	int wblocks = (npairs + NP_WAVES - 1) / NP_WAVES;
 	if (wblocks > 4096) wblocks = 4096;
 	const int* L = lists;

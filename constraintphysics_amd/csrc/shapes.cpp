@@ -6,6 +6,9 @@
 
 #include <cmath>
 #include <limits>
+#include <unordered_map>
+#include <utility>
+#include <vector>
 
 #include "cp_math.h"
 
@@ -289,6 +292,327 @@ void polyhedron_mass_properties(const std::vector<double>& pts, const std::vecto
 	com3[0] = c[0] + ctr.x;
 	com3[1] = c[1] + ctr.y;
 	com3[2] = c[2] + ctr.z;
+}
+
+}  // namespace cpx
+
+// ---------------------------------------------------------------- static meshes
+namespace cpx {
+
+int HostShapes::add_mesh(HostMesh m) {
+	int id = (int)type.size();
+	type.push_back(3);
+	pt_off.push_back(pt_off.back());
+	face_off.push_back(face_off.back());
+	arc_off.push_back(arc_off.back());
+	edge_off.push_back(edge_off.back());
+	radius.push_back(0);
+	height.push_back(0);
+	mesh_of.resize(type.size(), -1);
+	mesh_of[id] = (int)meshes.size();
+	meshes.push_back(std::move(m));
+	return id;
+}
+
+// StaticMeshGeometry(const MeshInput&) (Geometry.cpp:678-820).  Edges are
+// keyed (max index << 32) + min index; the second triangle met on a key
+// becomes the neighbour of the first (and the key is retired).  Each triangle
+// takes three fresh edge ids from T + P on (the ids are assigned on a copy of
+// the graph node, so shared edges get one id per side).  An edge's Gauss
+// region contribution is its outward direction on a border, the triangle's
+// own normal once for concave edges, else the neighbour's normal.
+HostMesh build_mesh(int n_pts, const double* pts, int n_tris, const int64_t* tri_idx, const double* tri_mat3) {
+	auto P = [&](int i) { return V3{pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]}; };
+	struct Side {
+		int a, b, opp;
+		uint64_t key;
+	};
+	struct Node {
+		int nb[3];
+		Side e[3];
+		V3 n;
+	};
+	std::vector<Node> g(n_tris);
+	for (int t = 0; t < n_tris; t++) {
+		int i[3] = {(int)tri_idx[3 * t], (int)tri_idx[3 * t + 1], (int)tri_idx[3 * t + 2]};
+		g[t].n = normalize(cross(P(i[1]) - P(i[0]), P(i[2]) - P(i[0])));
+		for (int k = 0; k < 3; k++) {
+			int a = i[k], b = i[(k + 1) % 3], o = i[(k + 2) % 3];
+			uint64_t hi = (uint64_t)(a > b ? a : b), lo = (uint64_t)(a > b ? b : a);
+			g[t].e[k] = Side{a, b, o, ((hi << 32) & 0xFFFFFFFF00000000ull) + lo};
+			g[t].nb[k] = -1;
+		}
+	}
+	std::unordered_map<uint64_t, int> open;
+	for (int t = 0; t < n_tris; t++)
+		for (int k = 0; k < 3; k++) {
+			auto it = open.find(g[t].e[k].key);
+			if (it == open.end()) {
+				open[g[t].e[k].key] = t;
+				continue;
+			}
+			int o = it->second;
+			g[t].nb[k] = o;
+			for (int q = 0; q < 3; q++)
+				if (g[o].e[q].key == g[t].e[k].key) g[o].nb[q] = t;
+			open.erase(it);
+		}
+	HostMesh m;
+	m.tris.resize(n_tris);
+	int next_edge = n_tris + n_pts;
+	for (int t = 0; t < n_tris; t++) {
+		const Node& nd = g[t];
+		MeshTriangle& r = m.tris[t];
+		V3 v[3] = {P((int)tri_idx[3 * t]), P((int)tri_idx[3 * t + 1]), P((int)tri_idx[3 * t + 2])};
+		r.n[0] = nd.n.x; r.n[1] = nd.n.y; r.n[2] = nd.n.z;
+		for (int k = 0; k < 3; k++) {
+			r.v[3 * k] = v[k].x; r.v[3 * k + 1] = v[k].y; r.v[3 * k + 2] = v[k].z;
+			r.mat[k] = tri_mat3[3 * t + k];
+		}
+		r.ngauss = 0;
+		r.concave = 0;
+		auto push_gauss = [&](V3 d) {
+			r.gauss[3 * r.ngauss] = d.x; r.gauss[3 * r.ngauss + 1] = d.y; r.gauss[3 * r.ngauss + 2] = d.z;
+			r.ngauss++;
+		};
+		for (int k = 0; k < 3; k++) {
+			V3 out = normalize(cross(v[(k + 1) % 3] - v[k], nd.n));
+			r.eout[3 * k] = out.x; r.eout[3 * k + 1] = out.y; r.eout[3 * k + 2] = out.z;
+			r.eid[k] = next_edge++;
+			if (nd.nb[k] < 0) {
+				push_gauss(out);
+				continue;
+			}
+			const Node& o = g[nd.nb[k]];
+			int opp = nd.e[k].opp;
+			for (int q = 0; q < 3; q++)
+				if (o.e[q].a == nd.e[k].b && o.e[q].b == nd.e[k].a) opp = o.e[q].opp;
+			const double EPS = 0.00001;
+			if (dot(normalize(P(opp) - P(nd.e[k].opp)), nd.n) >= -EPS) {
+				r.concave++;
+				if (r.concave <= 1) push_gauss(nd.n);
+			} else {
+				push_gauss(o.n);
+			}
+		}
+	}
+	return m;
+}
+
+namespace {
+// AABB::combine / surfaceArea (AABB.h:16-26, :112-118, as written).
+void box_combine(const double* a, const double* b, double* o) {
+	for (int i = 0; i < 3; i++) {
+		o[i] = (b[i] < a[i]) ? b[i] : a[i];
+		o[3 + i] = (a[3 + i] < b[3 + i]) ? b[3 + i] : a[3 + i];
+	}
+}
+double box_cost(const double* a) {
+	double dx = a[3] - a[0], dy = a[4] - a[1], dz = a[5] - a[2];
+	return 2 * (dx * dy) + (dy * dz) + (dz * dx);
+}
+struct TreeBuild {
+	std::vector<double> box, cost;
+	std::vector<int> parent, left, right, leaf;
+	int root = -1;
+	int add_node() {
+		int n = (int)parent.size();
+		box.resize(6 * (n + 1));
+		cost.push_back(0);
+		parent.push_back(-1);
+		left.push_back(-1);
+		right.push_back(-1);
+		leaf.push_back(-1);
+		return n;
+	}
+	void set_box(int n, const double* b) {
+		for (int i = 0; i < 6; i++) box[6 * n + i] = b[i];
+		cost[n] = box_cost(b);
+	}
+	void swap_nodes(int n1, int n2) {  // Node::swpNodes (AABB_Tree.h:306-319)
+		int p1 = parent[n1], p2 = parent[n2];
+		int* s1 = left[p1] == n1 ? &left[p1] : &right[p1];
+		int* s2 = left[p2] == n2 ? &left[p2] : &right[p2];
+		*s1 = n2;
+		*s2 = n1;
+		parent[n1] = p2;
+		parent[n2] = p1;
+	}
+	// AABBTree::add for a static object (AABB_Tree.h:39-103): branch and bound
+	// over insertion siblings, a new parent, box refresh up the tree, then
+	// rebalanceAncestors (:377-424).
+	void add(int tri, const double* tb) {
+		int nl = add_node();
+		leaf[nl] = tri;
+		double mb[6];
+		for (int i = 0; i < 3; i++) {
+			mb[i] = tb[i] - 0.0;
+			mb[3 + i] = tb[3 + i] + 0.0;
+		}
+		set_box(nl, mb);
+		if (root < 0) {
+			root = nl;
+			return;
+		}
+		int best = -1;
+		double best_cost = std::numeric_limits<double>::infinity();
+		std::vector<std::pair<int, double>> cand{{root, 0.0}};
+		while (!cand.empty()) {
+			std::pair<int, double> c = cand.back();
+			cand.pop_back();
+			if (c.second >= best_cost) continue;
+			double cb[6];
+			box_combine(&box[6 * nl], &box[6 * c.first], cb);
+			double direct = box_cost(cb);
+			if (c.second + direct < best_cost) {
+				best = c.first;
+				best_cost = c.second + direct;
+			}
+			if (leaf[c.first] < 0) {
+				double inherited = c.second + direct - cost[c.first];
+				cand.push_back({left[c.first], inherited});
+				cand.push_back({right[c.first], inherited});
+			}
+		}
+		int np = add_node();
+		if (best == root) {
+			root = np;
+		} else {
+			int op = parent[best];
+			if (left[op] == best) left[op] = np;
+			else right[op] = np;
+			parent[np] = op;
+		}
+		left[np] = best;
+		parent[best] = np;
+		right[np] = nl;
+		parent[nl] = np;
+		for (int c = parent[nl]; c >= 0; c = parent[c]) {
+			double cb[6];
+			box_combine(&box[6 * left[c]], &box[6 * right[c]], cb);
+			set_box(c, cb);
+		}
+		for (int c = parent[nl]; c >= 0 && parent[c] >= 0; c = parent[c]) {
+			int par = parent[c];
+			int sib = left[par] == c ? right[par] : left[par];
+			double sl[6], sr[6], cl[6], cr[6], clc, crc;
+			box_combine(&box[6 * right[c]], &box[6 * sib], sl);
+			box_combine(&box[6 * left[c]], &box[6 * sib], sr);
+			double slc = box_cost(sl), src = box_cost(sr);
+			if (leaf[sib] >= 0) {
+				clc = std::numeric_limits<double>::infinity();
+				crc = clc;
+			} else {
+				box_combine(&box[6 * right[sib]], &box[6 * c], cl);
+				box_combine(&box[6 * left[sib]], &box[6 * c], cr);
+				clc = box_cost(cl);
+				crc = box_cost(cr);
+			}
+			double dcl = clc - cost[sib], dcr = crc - cost[sib], dsl = slc - cost[c], dsr = src - cost[c];
+			double m1 = (dsr < dsl) ? dsr : dsl;
+			double m2 = (m1 < dcr) ? m1 : dcr;
+			double mn = (m2 < dcl) ? m2 : dcl;  // std::min<double>(cl, min(cr, min(sl, sr)))
+			if (mn >= 0) continue;
+			if (mn == dcl) {
+				swap_nodes(c, left[sib]);
+				for (int i = 0; i < 6; i++) box[6 * sib + i] = cl[i];
+				cost[sib] = clc;
+			} else if (mn == dcr) {
+				swap_nodes(c, right[sib]);
+				for (int i = 0; i < 6; i++) box[6 * sib + i] = cr[i];
+				cost[sib] = crc;
+			} else if (mn == dsl) {
+				swap_nodes(sib, left[c]);
+				for (int i = 0; i < 6; i++) box[6 * c + i] = sl[i];
+				cost[c] = slc;
+			} else {
+				swap_nodes(sib, right[c]);
+				for (int i = 0; i < 6; i++) box[6 * c + i] = sr[i];
+				cost[c] = src;
+			}
+		}
+	}
+};
+}  // namespace
+
+// recomputeFromReference with the body's pose (getTransformed, Geometry.cpp:
+// 827-847, centre of rotation 0), the AABB tree over the world triangles and
+// genAABB.  Leaves keep the triangle's exact box; the stack depth of a query
+// is recorded for the device traversal.
+WorldMesh place_mesh(const HostMesh& m, const double* com3, const double* q4) {
+	Q q = q_normalize(Q{q4[0], q4[1], q4[2], q4[3]});  // RigidBody::updateGeometry (RigidBody.cpp:283)
+	M3 R = q_rot(q);
+	V3 t{com3[0], com3[1], com3[2]}, c{0, 0, 0};
+	int T = (int)m.tris.size();
+	WorldMesh w;
+	w.tri.assign((size_t)T * MT_REC, 0.0);
+	w.tri_i.assign((size_t)T * MT_IREC, 0);
+	auto ld = [](const double* a, int k) { return V3{a[3 * k], a[3 * k + 1], a[3 * k + 2]}; };
+	auto st = [](double* a, int k, V3 v) { a[3 * k] = v.x; a[3 * k + 1] = v.y; a[3 * k + 2] = v.z; };
+	TreeBuild tb;
+	for (int i = 0; i < T; i++) {
+		const MeshTriangle& r = m.tris[i];
+		double* o = &w.tri[(size_t)i * MT_REC];
+		st(o, 0, R * ld(r.n, 0));
+		const double inf = std::numeric_limits<double>::infinity();
+		double mn[3] = {inf, inf, inf}, mx[3] = {-inf, -inf, -inf};
+		for (int k = 0; k < 3; k++) {
+			V3 v = t + R * (ld(r.v, k) - c) + c;
+			st(o + 3, k, v);
+			st(o + 12, k, R * ld(r.eout, k));
+			double p[3] = {v.x, v.y, v.z};
+			for (int d = 0; d < 3; d++) {
+				mn[d] = (p[d] < mn[d]) ? p[d] : mn[d];
+				mx[d] = (mx[d] < p[d]) ? p[d] : mx[d];
+			}
+		}
+		for (int k = 0; k < r.ngauss; k++) st(o + 21, k, R * ld(r.gauss, k));
+		for (int d = 0; d < 3; d++) {
+			o[30 + d] = mn[d];
+			o[33 + d] = mx[d];
+			o[36 + d] = r.mat[d];
+		}
+		int* oi = &w.tri_i[(size_t)i * MT_IREC];
+		for (int k = 0; k < 3; k++) oi[k] = r.eid[k];
+		oi[3] = r.ngauss;
+		oi[4] = r.concave;
+		oi[5] = i;
+		tb.add(i, o + 30);
+		if (i == 0) for (int d = 0; d < 6; d++) w.aabb[d] = o[30 + d];
+		else {
+			double cb[6];
+			box_combine(w.aabb, o + 30, cb);
+			for (int d = 0; d < 6; d++) w.aabb[d] = cb[d];
+		}
+	}
+	int N = (int)tb.parent.size();
+	w.box.resize((size_t)N * 6);
+	w.lr.resize((size_t)N * 2);
+	for (int n = 0; n < N; n++) {
+		if (tb.leaf[n] >= 0) {
+			for (int d = 0; d < 6; d++) w.box[6 * n + d] = w.tri[(size_t)tb.leaf[n] * MT_REC + 30 + d];
+			w.lr[2 * n] = -1;
+			w.lr[2 * n + 1] = tb.leaf[n];
+		} else {
+			for (int d = 0; d < 6; d++) w.box[6 * n + d] = tb.box[6 * n + d];
+			w.lr[2 * n] = tb.left[n];
+			w.lr[2 * n + 1] = tb.right[n];
+		}
+	}
+	w.root = tb.root;
+	// depth first from the root: a query's stack never holds more than depth + 1 nodes
+	std::vector<std::pair<int, int>> stk;
+	if (tb.root >= 0) stk.push_back({tb.root, 0});
+	while (!stk.empty()) {
+		auto e = stk.back();
+		stk.pop_back();
+		if (e.second > w.depth) w.depth = e.second;
+		if (tb.leaf[e.first] < 0) {
+			stk.push_back({tb.left[e.first], e.second + 1});
+			stk.push_back({tb.right[e.first], e.second + 1});
+		}
+	}
+	return w;
 }
 
 }  // namespace cpx

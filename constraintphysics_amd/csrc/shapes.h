@@ -1,6 +1,8 @@
 // Host-side shape tables and the polyhedron constructor (surfaces, edges,
 // Gauss map) that the reference runs when a convex primitive is created.
 #pragma once
+#include <stdint.h>
+
 #include <utility>
 #include <vector>
 
@@ -15,6 +17,38 @@ struct HullData {
 	std::vector<double> gmin, gmax, grefval;
 	std::vector<std::pair<int, int>> arcs, edges;
 };
+
+// Static triangle mesh as StaticMeshGeometry(const MeshInput&) builds it
+// (Geometry.cpp:678-820), in the mesh body's frame.  Per triangle: normal,
+// vertices, edge outward directions, Gauss region (up to 3 directions),
+// edge ids, concave neighbour count, restitution / kinetic / static friction.
+struct MeshTriangle {
+	double n[3], v[9], eout[9], gauss[9], mat[3];
+	int eid[3], ngauss, concave;
+};
+struct HostMesh {
+	std::vector<MeshTriangle> tris;
+};
+HostMesh build_mesh(int n_pts, const double* pts, int n_tris, const int64_t* tri_idx, const double* tri_mat3);
+
+// World copy of a mesh body (StaticMeshGeometry::recomputeFromReference,
+// Geometry.cpp:849-857) with its AABB tree built as AABBTree<unsigned int>
+// builds it: leaves added in triangle order, surface-area cost, no margin
+// (AABB_Tree.h:39-103, :357-424).  Device record layouts: triangle MT_REC
+// doubles (normal, vertices, edge out directions, Gauss region, aabb, mat)
+// and MT_IREC ints (edge ids, Gauss region size, concave count, local id);
+// node box[6] and (left, right), a leaf being (-1, triangle).
+#define MT_REC 40
+#define MT_IREC 8
+struct WorldMesh {
+	std::vector<double> tri;   // [T][MT_REC]
+	std::vector<int> tri_i;    // [T][MT_IREC]
+	std::vector<double> box;   // [N][6]
+	std::vector<int> lr;       // [N][2]
+	int root = -1, depth = 0;
+	double aabb[6];            // StaticMeshGeometry::genAABB (Geometry.cpp:859-868)
+};
+WorldMesh place_mesh(const HostMesh& m, const double* com3, const double* q4);
 
 struct HostShapes {
 	std::vector<int> type, pt_off{0}, face_off{0}, arc_off{0}, edge_off{0};
@@ -34,6 +68,10 @@ struct HostShapes {
 	// Stored as points [centre, top 8-gon, bottom 8-gon] and normals [axis,
 	// Gauss vertices] (the bottom polygon is derived per step).
 	int add_cylinder(const double* params8, const double* gauss42, const double* top24, const double* bot24);
+	// Static mesh: no convex geometry; the triangles live in meshes[mesh_of[shape]].
+	std::vector<int> mesh_of;
+	std::vector<HostMesh> meshes;
+	int add_mesh(HostMesh m);
 };
 
 // Polyhedron(points, surface_vertex_indices) (ConvexPrimitive.cpp:582-658)

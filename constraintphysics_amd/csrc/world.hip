@@ -203,6 +203,17 @@ struct cp_world {
 	DArr<double> dh_buf, dh_dbuf, dh_scratch;
 	bool hs_pending = true;
 	std::vector<int> last_isl_sys, last_isl_sys_off;
+	// static meshes: one world instance per mesh body (triangles and AABB tree
+	// placed on the host from the body's pose), device copies, step scratch
+	std::vector<int> mesh_inst_of_body, mesh_body;
+	bool meshes_placed = false;
+	DArr<int> dm_inst_of_body, dm_root, dm_tri_i, dm_lr, pair_rank, mesh_cnt, mesh_off, mesh_ccnt, mesh_coff, mesh_overflow;
+	DArr<double> dm_tri, dm_box;
+	DArr<int> mi_slot, mi_tri, mi_count, mi_owner;
+	DArr<double> mi_head, mi_pts, mi_mat;
+	DArr<uint64_t> mi_magic, mi_avail;
+	bool last_expanded = false;
+	int mesh_depth = 0;
 
 	Bodies bodies() {
 		Bodies b{};
@@ -274,6 +285,22 @@ struct cp_world {
 		H.scratch = dh_scratch.p;
 		return H;
 	}
+	MeshDev meshdev() {
+		MeshDev m{};
+		m.n_inst = (int)mesh_body.size();
+		m.inst_of_body = dm_inst_of_body.p;
+		m.root = dm_root.p;
+		m.tri = dm_tri.p;
+		m.tri_i = dm_tri_i.p;
+		m.node_box = dm_box.p;
+		m.node_lr = dm_lr.p;
+		return m;
+	}
+	MeshItems meshitems() {
+		return MeshItems{mi_slot.p, mi_tri.p, mi_count.p, mi_head.p, mi_pts.p, mi_magic.p, mi_mat.p, mi_owner.p, mi_avail.p};
+	}
+	void place_meshes(const double* state13);
+	int mesh_narrowphase(int npairs);
 	void upload_joints();
 	void read_joint_state(double* s, int n);
 	void write_joint_state(const double* s);
@@ -317,6 +344,13 @@ void cp_world::free_all() {
 	for (auto* a : jdbl) a->free_();
 	nocol.free_();
 	j_dev_n = 0;
+	DArr<int>* mints[] = {&dm_inst_of_body, &dm_root, &dm_tri_i, &dm_lr, &pair_rank, &mesh_cnt, &mesh_off, &mesh_ccnt, &mesh_coff,
+	                      &mesh_overflow, &mi_slot, &mi_tri, &mi_count, &mi_owner};
+	for (auto* a : mints) a->free_();
+	DArr<double>* mdbl[] = {&dm_tri, &dm_box, &mi_head, &mi_pts, &mi_mat};
+	for (auto* a : mdbl) a->free_();
+	mi_magic.free_();
+	mi_avail.free_();
 	tmp.free_();
 	for (void* a : man_allocs) (void)hipFree(a);
 	man_allocs.clear();
@@ -399,6 +433,16 @@ void cp_world::upload() {
 	dv.reserve(12 * (size_t)n);
 	color_mask.reserve(4 * (size_t)n);
 	applied.reserve(n);
+	// mesh bodies (placed when their pose is set)
+	mesh_inst_of_body.assign(n, -1);
+	mesh_body.clear();
+	for (int i = 0; i < n; i++)
+		if (hs.type[b_shape[i]] == SHAPE_MESH) {
+			mesh_inst_of_body[i] = (int)mesh_body.size();
+			mesh_body.push_back(i);
+		}
+	up(dm_inst_of_body, mesh_inst_of_body);
+	meshes_placed = mesh_body.empty();
 	uploaded = true;
 }
 
@@ -552,6 +596,60 @@ void cp_world::layout_systems() {
 	joints_dirty = true;  // system membership changed
 }
 
+// StaticMeshGeometry::recomputeFromReference for every mesh body at its pose
+// (RigidBody::updateGeometry, RigidBody.cpp:297-301): world triangles and the
+// AABB tree built on the host, uploaded; the body's AABB is the mesh's.
+void cp_world::place_meshes(const double* state13) {
+	if (mesh_body.empty()) return;
+	std::vector<double> tri, box;
+	std::vector<int> tri_i, lr, root;
+	mesh_depth = 0;
+	std::vector<double> ab;
+	down(ab, aabb.p, 6 * (size_t)nb);
+	for (int body : mesh_body) {
+		const double* r = state13 + 13 * (size_t)body;
+		WorldMesh wm = place_mesh(hs.meshes[hs.mesh_of[b_shape[body]]], r, r + 3);
+		int t0 = (int)(tri.size() / MT_REC), n0 = (int)(lr.size() / 2);
+		tri.insert(tri.end(), wm.tri.begin(), wm.tri.end());
+		tri_i.insert(tri_i.end(), wm.tri_i.begin(), wm.tri_i.end());
+		box.insert(box.end(), wm.box.begin(), wm.box.end());
+		for (size_t k = 0; k < wm.lr.size() / 2; k++) {
+			bool leaf = wm.lr[2 * k] < 0;
+			lr.push_back(leaf ? -1 : wm.lr[2 * k] + n0);
+			lr.push_back(leaf ? wm.lr[2 * k + 1] + t0 : wm.lr[2 * k + 1] + n0);
+		}
+		root.push_back(wm.root + n0);
+		mesh_depth = std::max(mesh_depth, wm.depth);
+		for (int d = 0; d < 6; d++) ab[d * (size_t)nb + body] = wm.aabb[d];
+	}
+	if (mesh_depth + 1 > MESH_STACK) throw ApiErr{CP_ERR_INVALID};  // tree too deep for the device traversal
+	up(dm_tri, tri); up(dm_tri_i, tri_i); up(dm_box, box); up(dm_lr, lr); up(dm_root, root);
+	HIPCHK(hipMemcpy(aabb.p, ab.data(), ab.size() * sizeof(double), hipMemcpyHostToDevice));
+	meshes_placed = true;
+}
+
+// Pairs with a static mesh: each takes MESH_SLOTS consecutive slots (same key)
+// so that its several manifolds flow through persistence in order.
+int cp_world::mesh_narrowphase(int npairs) {
+	last_expanded = false;
+	if (mesh_body.empty() || npairs <= 0) return npairs;
+	if (!meshes_placed) throw ApiErr{CP_ERR_STATE};
+	Bodies b = bodies();
+	MeshDev md = meshdev();
+	mesh_cnt.reserve(npairs + 1);
+	mesh_off.reserve(npairs + 1);
+	launch_mesh_expand(b, md, pairs.p, npairs, nocol.p, n_nocol, mesh_cnt.p, nullptr, nullptr, nullptr, stream);
+	HIPCHK(hipMemsetAsync(mesh_cnt.p + npairs, 0, sizeof(int), stream));
+	scan(mesh_cnt.p, mesh_off.p, npairs + 1);
+	int total = read1(mesh_off.p + npairs);
+	pairs2.reserve(total);
+	pair_rank.reserve(total);
+	launch_mesh_expand(b, md, pairs.p, npairs, nocol.p, n_nocol, mesh_cnt.p, mesh_off.p, pairs2.p, pair_rank.p, stream);
+	std::swap(pairs, pairs2);
+	last_expanded = true;
+	return total;
+}
+
 // Body indices in the iteration order of the engine's node map (given by the
 // caller, else libstdc++'s order for ids 1..n).
 std::vector<int> cp_world::root_order() {
@@ -673,7 +771,7 @@ void cp_world::ensure_manifolds(int n) {
 	man.maxpen = (double*)al(sizeof(double) * K);
 	man.pts = (double*)al(sizeof(double) * 16 * K);
 	man.magic = (uint64_t*)al(sizeof(uint64_t) * 8 * K);
-	man.mat = (double*)al(sizeof(double) * 3 * K);
+	man.mat = (double*)al(sizeof(double) * 12 * K);
 	mcap = K;
 }
 
@@ -732,10 +830,39 @@ int cp_world::broadphase() {
 }
 
 int cp_world::narrowphase(int npairs) {
+	npairs = mesh_narrowphase(npairs);
 	ensure_manifolds(npairs);
-	np_lists.reserve(6 * (size_t)std::max(npairs, 1));
+	np_lists.reserve(7 * (size_t)std::max(npairs, 1));
 	np_counts.reserve(8);
-	launch_narrowphase(bodies(), shapes(), params(), pairs.p, npairs, man, mcap, np_lists.p, np_counts.p, nocol.p, n_nocol, stream);
+	launch_narrowphase(bodies(), shapes(), params(), pairs.p, npairs, man, mcap, np_lists.p, np_counts.p, nocol.p, n_nocol,
+	                   last_expanded ? pair_rank.p : nullptr, stream);
+	if (!last_expanded) return npairs;
+	// convex body against mesh: candidate triangles, per-triangle SAT, merge + cull
+	int nlist = read1(np_counts.p + NPC_MESH_LIST);
+	if (nlist == 0) return npairs;
+	const int* list = np_lists.p + (size_t)NPC_MESH_LIST * npairs;
+	Bodies b = bodies();
+	MeshDev md = meshdev();
+	mesh_ccnt.reserve(nlist + 1);
+	mesh_coff.reserve(nlist + 1);
+	mesh_overflow.reserve(2);
+	HIPCHK(hipMemsetAsync(mesh_overflow.p, 0, 2 * sizeof(int), stream));
+	MeshItems it = meshitems();
+	launch_mesh_candidates(b, md, pairs.p, list, np_counts.p, nlist, mesh_ccnt.p, nullptr, it, mesh_overflow.p, stream);
+	HIPCHK(hipMemsetAsync(mesh_ccnt.p + nlist, 0, sizeof(int), stream));
+	scan(mesh_ccnt.p, mesh_coff.p, nlist + 1);
+	int C = read1(mesh_coff.p + nlist);
+	size_t c = std::max(C, 1);
+	mi_slot.reserve(c); mi_tri.reserve(c); mi_count.reserve(c); mi_owner.reserve(c); mi_avail.reserve(c);
+	mi_head.reserve(4 * c); mi_pts.reserve(4 * MESH_PTS * c); mi_magic.reserve(2 * MESH_PTS * c); mi_mat.reserve(3 * c);
+	it = meshitems();
+	launch_mesh_candidates(b, md, pairs.p, list, np_counts.p, nlist, mesh_ccnt.p, mesh_coff.p, it, mesh_overflow.p, stream);
+	launch_mesh_triangles(b, shapes(), md, params(), pairs.p, it, C, stream);
+	launch_mesh_merge(params(), pairs.p, md, list, nlist, mesh_coff.p, it, man, mcap, mesh_overflow.p + 1, stream);
+	int ov[2];
+	HIPCHK(hipMemcpyAsync(ov, mesh_overflow.p, sizeof(ov), hipMemcpyDeviceToHost, stream));
+	HIPCHK(hipStreamSynchronize(stream));
+	if (ov[0] || ov[1]) throw ApiErr{CP_ERR_CAPACITY};  // deeper tree or more manifolds per pair than the slots hold
 	return npairs;
 }
 
@@ -932,7 +1059,7 @@ void cp_world::step() {
 	launch_gravity(b, P, stream);
 	int npairs = broadphase();
 	if (timing) HIPCHK(hipEventRecord(ev[1], stream));
-	narrowphase(npairs);
+	npairs = narrowphase(npairs);
 	if (timing) HIPCHK(hipEventRecord(ev[2], stream));
 	persist(npairs);
 	edges();
@@ -1177,6 +1304,17 @@ int cp_cylinder_build(const double* center, double radius, double height, double
 	return CP_OK;
 }
 
+int cp_shape_add_mesh(cp_world* w, int n_pts, const double* pts, int n_tris, const int64_t* tri_idx, const double* tri_mat3, int* shape_id) {
+	if (!w || n_pts < 3 || n_tris < 1 || !pts || !tri_idx || !tri_mat3 || !shape_id) return CP_ERR_INVALID;
+	for (int k = 0; k < 3 * n_tris; k++)
+		if (tri_idx[k] < 0 || tri_idx[k] >= n_pts) return CP_ERR_INVALID;
+	for (int k = 0; k < 3 * n_pts; k++)
+		if (!std::isfinite(pts[k])) return CP_ERR_INVALID;
+	*shape_id = w->hs.add_mesh(build_mesh(n_pts, pts, n_tris, tri_idx, tri_mat3));
+	w->uploaded = false;
+	return CP_OK;
+}
+
 int cp_shape_add_polyhedron(cp_world* w, int n_pts, const double* pts, int n_faces, const int64_t* face_off, const int64_t* face_verts,
                             int* shape_id) {
 	if (!w || !shape_id || n_pts < 4 || n_faces < 4 || !pts || !face_off || !face_verts) return CP_ERR_INVALID;
@@ -1249,6 +1387,7 @@ int cp_polyhedron_mass_properties(int n_pts, const double* pts, int n_faces, con
 int cp_body_add(cp_world* w, int shape_id, int movement, const double* mp, const double* material, int scene, int* body_index) {
 	if (!w || !mp || !material || !body_index) return CP_ERR_INVALID;
 	if (shape_id < 0 || shape_id >= (int)w->hs.type.size() || (movement != 0 && movement != 1)) return CP_ERR_INVALID;
+	if (w->hs.type[shape_id] == SHAPE_MESH && movement != 1) return CP_ERR_INVALID;  // meshes are static (RigidBody.cpp:121)
 	if (!w->b_scene.empty() && scene < w->b_scene.back()) return CP_ERR_INVALID;  // scenes are contiguous and ascending
 	if (w->uploaded && w->ncontacts > 0) return CP_ERR_STATE;
 	w->b_shape.push_back(shape_id);
@@ -1268,6 +1407,7 @@ int cp_body_add_many(cp_world* w, int n, const int* shape_ids, const int* moveme
 	int last = w->b_scene.empty() ? -1 : w->b_scene.back();
 	for (int i = 0; i < n; i++) {
 		if (shape_ids[i] < 0 || shape_ids[i] >= (int)w->hs.type.size() || (movement[i] != 0 && movement[i] != 1)) return CP_ERR_INVALID;
+		if (w->hs.type[shape_ids[i]] == SHAPE_MESH && movement[i] != 1) return CP_ERR_INVALID;
 		if (scene[i] < last) return CP_ERR_INVALID;
 		last = scene[i];
 	}
@@ -1325,6 +1465,7 @@ int cp_world_set_state(cp_world* w, const double* s, const double* inertia) {
 		}
 		launch_refresh_geometry(w->bodies(), w->shapes(), 0, w->stream);
 		HIPCHK(hipStreamSynchronize(w->stream));
+		w->place_meshes(s);
 	});
 }
 
@@ -1641,33 +1782,87 @@ int cp_broadphase(cp_world* w, int cap, int64_t* out, int* n) {
 	return g != CP_OK ? g : status;
 }
 
+// Runs the narrowphase on the given pairs; returns per slot the manifold and
+// the input pair it belongs to (a pair with a static mesh owns several slots).
+static void narrowphase_slots(cp_world* w, int n, const int64_t* pairs2, std::vector<int>& pair_of, std::vector<int>& cnt,
+                              std::vector<double>& nrm, std::vector<double>& mp, std::vector<double>& pp, std::vector<uint64_t>& mg) {
+	std::vector<uint64_t> keys(n);
+	for (int k = 0; k < n; k++) keys[k] = ((uint64_t)(pairs2[2 * k] - 1) << 32) | (uint64_t)(pairs2[2 * k + 1] - 1);
+	up(w->pairs, keys);
+	int ns = w->narrowphase(n);
+	HIPCHK(hipStreamSynchronize(w->stream));
+	int K = w->mcap;
+	down(cnt, w->man.count, ns);
+	down(nrm, w->man.normal, 3 * (size_t)K);
+	down(mp, w->man.maxpen, ns);
+	down(pp, w->man.pts, 16 * (size_t)ns);
+	down(mg, w->man.magic, 8 * (size_t)ns);
+	pair_of.resize(ns);
+	if (w->last_expanded) {
+		std::vector<int> rank;
+		down(rank, w->pair_rank.p, ns);
+		int k = -1;
+		for (int s = 0; s < ns; s++) {
+			if (rank[s] == 0) k++;
+			pair_of[s] = k;
+		}
+	} else {
+		for (int s = 0; s < ns; s++) pair_of[s] = s;
+	}
+}
+
 int cp_narrowphase(cp_world* w, int n, const int64_t* pairs2, int* counts, double* heads, double* pts, uint64_t* magic) {
 	if (!w || n < 0 || !w->uploaded || (n > 0 && (!pairs2 || !counts || !heads || !pts || !magic))) return CP_ERR_INVALID;
 	for (int k = 0; k < n; k++)
 		if (pairs2[2 * k] < 1 || pairs2[2 * k + 1] <= pairs2[2 * k] || pairs2[2 * k + 1] > w->nb) return CP_ERR_INVALID;
 	return guard(w, [&] {
-		std::vector<uint64_t> keys(n);
-		for (int k = 0; k < n; k++) keys[k] = ((uint64_t)(pairs2[2 * k] - 1) << 32) | (uint64_t)(pairs2[2 * k + 1] - 1);
-		up(w->pairs, keys);
-		w->narrowphase(n);
-		HIPCHK(hipStreamSynchronize(w->stream));
-		int K = w->mcap;
-		std::vector<int> cnt;
+		std::vector<int> pair_of, cnt;
 		std::vector<double> nrm, mp, pp;
 		std::vector<uint64_t> mg;
-		down(cnt, w->man.count, n);
-		down(nrm, w->man.normal, 3 * (size_t)K);
-		down(mp, w->man.maxpen, n);
-		down(pp, w->man.pts, 16 * (size_t)n);
-		down(mg, w->man.magic, 8 * (size_t)n);
-		for (int k = 0; k < n; k++) {
-			counts[k] = cnt[k];
-			for (int c = 0; c < 3; c++) heads[4 * k + c] = cnt[k] ? nrm[c * (size_t)K + k] : 0;
-			heads[4 * k + 3] = mp[k];
-			for (int j = 0; j < 16; j++) pts[16 * (size_t)k + j] = pp[16 * (size_t)k + j];
-			for (int j = 0; j < 8; j++) magic[8 * (size_t)k + j] = mg[8 * (size_t)k + j];
+		narrowphase_slots(w, n, pairs2, pair_of, cnt, nrm, mp, pp, mg);
+		int K = w->mcap;
+		std::vector<char> done(n, 0);
+		for (int s = 0; s < (int)pair_of.size(); s++) {
+			int k = pair_of[s];
+			if (done[k]) continue;  // first manifold of each pair
+			done[k] = 1;
+			counts[k] = cnt[s];
+			for (int c = 0; c < 3; c++) heads[4 * k + c] = cnt[s] ? nrm[c * (size_t)K + s] : 0;
+			heads[4 * k + 3] = mp[s];
+			for (int j = 0; j < 16; j++) pts[16 * (size_t)k + j] = pp[16 * (size_t)s + j];
+			for (int j = 0; j < 8; j++) magic[8 * (size_t)k + j] = mg[8 * (size_t)s + j];
 		}
 	});
+}
+
+int cp_narrowphase_multi(cp_world* w, int n, const int64_t* pairs2, int cap, int64_t* pair_index, int* counts, double* heads, double* pts,
+                         uint64_t* magic, int* n_out) {
+	if (!w || n < 0 || cap < 0 || !n_out || !w->uploaded || (n > 0 && !pairs2) || (cap > 0 && (!pair_index || !counts || !heads || !pts || !magic)))
+		return CP_ERR_INVALID;
+	for (int k = 0; k < n; k++)
+		if (pairs2[2 * k] < 1 || pairs2[2 * k + 1] <= pairs2[2 * k] || pairs2[2 * k + 1] > w->nb) return CP_ERR_INVALID;
+	int st = guard(w, [&] {
+		std::vector<int> pair_of, cnt;
+		std::vector<double> nrm, mp, pp;
+		std::vector<uint64_t> mg;
+		narrowphase_slots(w, n, pairs2, pair_of, cnt, nrm, mp, pp, mg);
+		int K = w->mcap, o = 0;
+		for (int s = 0; s < (int)pair_of.size(); s++) {
+			if (cnt[s] == 0) continue;
+			if (o < cap) {
+				pair_index[o] = pair_of[s];
+				counts[o] = cnt[s];
+				for (int c = 0; c < 3; c++) heads[4 * o + c] = nrm[c * (size_t)K + s];
+				heads[4 * o + 3] = mp[s];
+				for (int j = 0; j < 16; j++) pts[16 * (size_t)o + j] = pp[16 * (size_t)s + j];
+				for (int j = 0; j < 8; j++) magic[8 * (size_t)o + j] = mg[8 * (size_t)s + j];
+			}
+			o++;
+		}
+		*n_out = o;
+	});
+	if (st == CP_OK && *n_out > cap) return CP_ERR_CAPACITY;
+	return st;
 }
 
 int cp_world_last_islands(cp_world* w, int cap_rows, int64_t* rows2, int cap_offs, int64_t* offs, int* n_rows, int* n_islands) {

@@ -104,6 +104,15 @@ int cp_shape_add_cylinder(cp_world* w, const double* params8, const double* gaus
  * height axis +y, in the cp_shape_add_cylinder layout. */
 int cp_cylinder_build(const double* center, double radius, double height, double* params8, double* gauss42, double* top24,
                       double* bot24);
+/* Static triangle mesh (StaticMeshGeometry(const MeshInput&), Geometry.cpp:
+ * 678-820): n_pts points (xyz, body frame), n_tris triangles as 3 point
+ * indices each (counter-clockwise seen from the side the normal faces) and
+ * per-triangle restitution, kinetic and static friction (TriIndices::material).
+ * A body with this shape must be FIXED (createRigidBody(StaticMeshGeometry),
+ * PhysicsEngine.cpp:498-513); its triangles and AABB tree are placed at its
+ * pose when cp_world_set_state is called. */
+int cp_shape_add_mesh(cp_world* w, int n_pts, const double* pts, int n_tris, const int64_t* tri_idx, const double* tri_mat3,
+                      int* shape_id);
 /* Convex polyhedron from points and face vertex lists (Polyhedron(points,
  * surface_vertex_indices), ConvexPrimitive.h:216): builds surfaces, edges and
  * the Gauss map on the host the way the reference constructor does.  Points
@@ -203,6 +212,13 @@ int cp_broadphase(cp_world* w, int cap, int64_t* pairs2, int* n);
  * counts[n]; heads[n][4] = normal xyz, max depth; pts[n][4][4] = pos xyz, depth;
  * magic[n][4][2]. */
 int cp_narrowphase(cp_world* w, int n, const int64_t* pairs2, int* counts, double* heads, double* pts, uint64_t* magic);
+/* Every manifold the engine adds for the given pairs, in pair order (a convex
+ * body on a static mesh gets one per group of near-parallel triangle contacts,
+ * PhysicsEngine.cpp:223-277): pair_index[cap] (index into pairs2), counts[cap],
+ * heads[cap][4], pts[cap][4][4], magic[cap][4][2]; *n_out = manifolds found
+ * (CP_ERR_CAPACITY when above cap). */
+int cp_narrowphase_multi(cp_world* w, int n, const int64_t* pairs2, int cap, int64_t* pair_index, int* counts, double* heads,
+                         double* pts, uint64_t* magic, int* n_out);
 /* Sweep schedule of the last step: in CP_PGS_REFERENCE_ORDER mode the island
  * rows; in CP_PGS_COLORED mode (with diagnostics bit 1) one row list per scene
  * in colour order.  Rows are (contact index in the post-narrowphase cache,

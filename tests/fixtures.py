@@ -173,7 +173,7 @@ class Oracle:
     """FP64 CPU restatement of the reference step, loaded with a fixture's scene."""
 
     def __init__(self, fx=None, cfm=0.0000025, hardness=1000.0, life=6, desc=None, dt=None, gravity_y=None, vel_iters=None,
-                 pos_iters=None, substeps=None, drop_nonfinite=0, hol_iters=None):
+                 pos_iters=None, substeps=None, drop_nonfinite=0, hol_iters=None, with_joints=True):
         lib = oracle_lib()
         self.lib = lib
         joints = None
@@ -182,7 +182,7 @@ class Oracle:
             dt, gravity_y, vel_iters, pos_iters, substeps = m["dt"], m["gravity_y"], m["vel_iters"], m["pos_iters"], m["substeps"]
             if hol_iters is None:
                 hol_iters = int(m.get("hol_iters", 1))
-            if "joint_info" in fx:
+            if with_joints and "joint_info" in fx:
                 joints = (fx["joint_info"], fx["joint_geo"])
             n = int(m["n_bodies"])
             movement = fx["body_info"][:, 1]
@@ -373,7 +373,7 @@ class Oracle:
 
 
 # ----------------------------------------------------------------------------- product world from a fixture
-def world_from_fixture(fx, pgs_mode=1, **cfg):
+def world_from_fixture(fx, pgs_mode=1, joints=True, **cfg):
     """Device world holding exactly the reference scene of a fixture
     (body-frame hulls, Gauss maps and mass properties as the reference stores them)."""
     import constraintphysics_amd as cpa
@@ -385,6 +385,10 @@ def world_from_fixture(fx, pgs_mode=1, **cfg):
     info, mp, mat = fx["body_info"], fx["mass_props"], fx["material"]
     for b in range(int(m["n_bodies"])):
         sh = fx.shape(b)
+        if "mesh" in sh:
+            # the reference leaves a mesh body's mass NaN; a fixed body's mass is never read
+            w.add_body(w.add_mesh(sh["mesh"], sh["tris"], sh["tri_mat"]), 1, np.zeros(19), mat[b], 0)
+            continue
         if "sphere" in sh:
             sid = w.add_sphere(sh["sphere"][:3], sh["sphere"][3])
         elif "cylinder" in sh:
@@ -392,7 +396,7 @@ def world_from_fixture(fx, pgs_mode=1, **cfg):
         else:
             sid = w.add_hull_exact(sh)
         w.add_body(sid, int(info[b, 1]), mp[b], mat[b], 0)
-    if "joint_info" in fx:
+    if joints and "joint_info" in fx:
         for (kind, id1, id2), geo in zip(fx["joint_info"], fx["joint_geo"]):
             w.add_joint(int(kind), int(id1) - 1, int(id2) - 1, geo)
     return w

@@ -1,0 +1,136 @@
+"""Static triangle meshes on the GPU, through the C ABI, against
+tests/golden/terrain (boxes, spheres, tilted cylinders and two jointed chains
+of cylinder links on a 12 x 12 heightfield, dumped from the compiled
+reference).
+
+Bars: every manifold of every pair bit-exact (the reference reports a mesh
+pair's manifolds through an unstable sort, so within a pair they are compared
+as a set; their order is pinned by the step's contact cache); a full step in
+the reference-order mode with island rows bit-exact, body state within 1e-12
+relative and the contact cache within 1e-10 (scale max(1, |x|)); 40 steps
+against the oracle; the coloured mode against the oracle replaying its
+colour schedule.
+"""
+import numpy as np
+import pytest
+
+import constraintphysics_amd as cpa
+from fixtures import Oracle, golden, world_from_fixture
+
+pytestmark = pytest.mark.gpu
+
+STEPS = [1, 60, 120, 180, 239]
+
+
+def _rel(a, b):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    if a.size == 0:
+        return 0.0
+    return float(np.max(np.abs(a - b) / np.maximum(1.0, np.abs(b))))
+
+
+def _node_order(fx, k):
+    no = fx[f"s{k}_node_order"]
+    return no if len(no) else cpa.reference_node_order(int(fx.meta["n_bodies"]))
+
+
+def _load(w, fx, k, joints=True):
+    w.set_state(fx[f"s{k}_pre"], fx[f"s{k}_pre_inertia"])
+    w.set_contacts(*fx.cache(k, "pre"))
+    if joints:
+        w.set_holonomic_systems(fx[f"s{k}_hsys_off"], fx[f"s{k}_hsys_og"])
+        w.set_joint_state(fx[f"s{k}_joint_pre"])
+    w.set_node_order(_node_order(fx, k))
+
+
+def _key(head, pts, mg):
+    return (tuple(head), tuple(map(tuple, pts)), tuple(map(tuple, np.asarray(mg).astype(np.int64))))
+
+
+@pytest.mark.parametrize("k", STEPS)
+def test_mesh_narrowphase_bit_exact(k):
+    fx = golden("terrain")
+    print(f"terrain narrowphase step {k}", flush=True)
+    w = world_from_fixture(fx)
+    _load(w, fx, k)
+    pairs = fx[f"s{k}_pairs"]
+    assert np.array_equal(w.broadphase(), pairs)
+    mp, mh, mpts, mm = (fx[f"s{k}_man_{x}"] for x in ("pair", "head", "pts", "magic"))
+    ref, off = {}, 0
+    for i, (a, b, n) in enumerate(mp):
+        ref.setdefault((int(a), int(b)), []).append(_key(mh[i], mpts[off:off + n], mm[off:off + n]))
+        off += n
+    got = {}
+    for pi, head, pts, mg in w.narrowphase_multi(pairs):
+        a, b = pairs[pi]
+        got.setdefault((int(a), int(b)), []).append(_key(head, pts, mg))
+    assert sorted(got) == sorted(ref)
+    for key in ref:
+        assert sorted(got[key]) == sorted(ref[key]), key
+
+
+@pytest.mark.parametrize("k", STEPS)
+def test_mesh_full_step_reference_order(k):
+    fx = golden("terrain")
+    print(f"terrain step {k}", flush=True)
+    w = world_from_fixture(fx, pgs_mode=1)
+    _load(w, fx, k)
+    w.step(1)
+    rows, offs = w.last_islands()
+    if len(fx[f"s{k}_island_rows"]):
+        assert np.array_equal(rows, fx[f"s{k}_island_rows"])
+    err = _rel(w.state(), fx[f"s{k}_post"])
+    jerr = _rel(w.joint_state(), fx[f"s{k}_joint_post"])
+    gi, gm, gf = fx.cache(k, "post")
+    ii, mg, ff = w.contacts()
+    print(f"  state rel {err:.2e}, joints rel {jerr:.2e}, contacts {len(ii)}", flush=True)
+    assert np.array_equal(ii[:, [0, 1, 2, 3, 5]], gi[:, [0, 1, 2, 3, 5]])
+    assert np.array_equal(mg, gm)
+    assert _rel(ff, gf) <= 1e-10
+    assert err <= 1e-12 and jerr <= 1e-10
+
+
+def test_mesh_run_tracks_oracle():
+    fx = golden("terrain")
+    k = 120
+    w = world_from_fixture(fx, pgs_mode=1)
+    _load(w, fx, k)
+    o = Oracle(fx)
+    o.load(fx[f"s{k}_pre"], fx[f"s{k}_pre_inertia"], fx.cache(k, "pre"))
+    o.load_joints(fx[f"s{k}_joint_pre"], fx[f"s{k}_hsys_off"], fx[f"s{k}_hsys_og"])
+    no = _node_order(fx, k)
+    worst = 0.0
+    for _ in range(40):
+        w.step(1)
+        o.step(no)
+        worst = max(worst, _rel(w.state(), o.state()))
+    print(f"40 steps on the terrain: worst state rel {worst:.2e}", flush=True)
+    assert worst <= 1e-9
+
+
+@pytest.mark.parametrize("k", [120, 239])
+def test_mesh_colored_schedule_replayed_by_oracle(k):
+    """The coloured sweep on a terrain scene without joints equals the
+    oracle's sequential replay of the same colour schedule."""
+    fx = golden("terrain")
+    w = world_from_fixture(fx, pgs_mode=0, joints=False, diagnostics=2)
+    _load(w, fx, k, joints=False)
+    w.step(1)
+    rows, offs = w.last_islands()
+    o = Oracle(fx, with_joints=False)
+    o.load(fx[f"s{k}_pre"], fx[f"s{k}_pre_inertia"], fx.cache(k, "pre"))
+    o.step_scheduled(rows, offs)
+    assert _rel(w.state(), o.state()) <= 1e-12
+    ii, mg, ff = w.contacts()
+    oi, om, of = o.cache()
+    assert np.array_equal(ii, oi) and np.array_equal(mg, om)
+    assert _rel(ff, of) <= 1e-10
+
+
+def test_mesh_body_must_be_fixed():
+    fx = golden("terrain")
+    w = cpa.World()
+    sh = fx.shape(0)
+    sid = w.add_mesh(sh["mesh"], sh["tris"], sh["tri_mat"])
+    with pytest.raises(cpa.CpError):
+        w.add_body(sid, 0, np.zeros(19), (1.0, 0.0, 0.5, 0.5), 0)
