@@ -917,7 +917,8 @@ __device__ __forceinline__ bool sorted_has(const uint64_t* v, int n, uint64_t x)
 	return lo < n && v[lo] == x;
 }
 
-__global__ void k_np_classify(Bodies b, Shapes sh, const uint64_t* pairs, int npairs, Manifolds M, int* lists, int* counts,
+#define CLS_WAVES 16  // waves per classify block
+__global__ __launch_bounds__(64 * CLS_WAVES) void k_np_classify(Bodies b, Shapes sh, const uint64_t* pairs, int npairs, Manifolds M, int* lists, int* counts,
                               const uint64_t* nocol, int n_nocol, const int* rank) {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
 	int cls = -1;
@@ -937,17 +938,30 @@ __global__ void k_np_classify(Bodies b, Shapes sh, const uint64_t* pairs, int np
 			else cls = NPC_CC;
 		}
 	}
-	const int lane = threadIdx.x & 63;
+	// list positions: wave ballots, a block-wide prefix per class in LDS, one
+	// global atomic per class per block
+	__shared__ int wcnt[NPC_N][CLS_WAVES];
+	__shared__ int bbase[NPC_N];
+	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 	const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+	uint64_t mine = 0;
 	for (int c = 0; c < NPC_N; c++) {
 		uint64_t m = __ballot(cls == c);
-		if (!m) continue;
-		int leader = __ffsll((long long)m) - 1;
-		int base = 0;
-		if (lane == leader) base = atomicAdd(&counts[c], __popcll(m));
-		base = __shfl(base, leader);
-		if (cls == c) lists[(size_t)c * npairs + base + __popcll(m & below)] = i;
+		if (cls == c) mine = m;
+		if (lane == 0) wcnt[c][wid] = __popcll(m);
 	}
+	__syncthreads();
+	if (threadIdx.x < NPC_N) {
+		int c = threadIdx.x, run = 0;
+		for (int w = 0; w < CLS_WAVES; w++) {
+			int v = wcnt[c][w];
+			wcnt[c][w] = run;
+			run += v;
+		}
+		bbase[c] = run ? atomicAdd(&counts[c], run) : 0;
+	}
+	__syncthreads();
+	if (cls >= 0) lists[(size_t)cls * npairs + bbase[cls] + wcnt[cls][wid] + __popcll(mine & below)] = i;
 }
 
 #define CAND_K 4
@@ -1693,8 +1707,8 @@ void launch_narrowphase(const Bodies& b, const Shapes& sh, const Params& p, cons
                         int cap, int* lists, int* counts, const uint64_t* nocol, int n_nocol, const int* rank, hipStream_t s) {
 	if (npairs <= 0) return;
 	hipLaunchKernelGGL(k_np_zero, dim3(1), dim3(64), 0, s, counts);
-	hipLaunchKernelGGL(k_np_classify, dim3((npairs + 255) / 256), dim3(256), 0, s, b, sh, pairs, npairs, m, lists, counts, nocol, n_nocol,
-	                   rank);
+	hipLaunchKernelGGL(k_np_classify, dim3((npairs + 64 * CLS_WAVES - 1) / (64 * CLS_WAVES)), dim3(64 * CLS_WAVES), 0, s, b, sh, pairs, npairs,
+	                   m, lists, counts, nocol, n_nocol, rank);
 	int wblocks = (npairs + NP_WAVES - 1) / NP_WAVES;
 	if (wblocks > 4096) wblocks = 4096;
 	const int* L = lists;

@@ -192,6 +192,7 @@ struct cp_world {
 	std::vector<std::vector<uint64_t>> sys_edges;  // member edges per system
 	bool hs_explicit = false;  // systems set by the caller: no discovery
 	bool hs_changed = false;   // a joint joined an edge already in a system
+	bool hs_scan = false;      // joints added since the last search
 	// holonomic systems: constraint order (og), then the symbolic layout
 	std::vector<std::vector<int>> hs_og;
 	struct HLayout {
@@ -1050,7 +1051,10 @@ void cp_world::launch_pgs_colored_lds_global(const Bodies& b, const Params& P) {
 
 void cp_world::step() {
 	if (!j_type.empty() && cfg.pgs_mode != CP_PGS_REFERENCE_ORDER) throw ApiErr{CP_ERR_STATE};  // joints: sequential order only
-	if (!hs_explicit && cfg.hol_iters > 0) discover_systems();
+	if (!hs_explicit && cfg.hol_iters > 0 && (hs_scan || hs_changed)) {
+		discover_systems();
+		hs_scan = false;
+	}
 	if (joints_dirty) upload_joints();
 	if (!j_type.empty()) lin.reserve(6 * (size_t)nb);
 	Params P = params();
@@ -1649,6 +1653,7 @@ int cp_joint_add(cp_world* w, int type, int b1, int b2, const double* geo14, int
 		else it->second.scan = true;
 	}
 	w->joints_dirty = true;
+	w->hs_scan = true;
 	*joint_index = k;
 	return CP_OK;
 }
