@@ -2,8 +2,10 @@
 reference build: brute-force AABB overlap, closed-form sphere contacts, the
 penetration depth of two convex polyhedra as the minimum overlap over every
 face normal and edge-pair direction, a resting box's four-corner manifold,
-free fall and torque-free rotation in closed form, and Coulomb friction on a
-sliding sphere.  Each test is chosen so that a plausible slip (a transposed
+free fall and torque-free rotation in closed form, Coulomb friction on a
+sliding sphere, joint rows and the block LDL solve against a dense solve, and
+static meshes (tree query = brute-force box overlap, Gauss regions from edge
+convexity, resting box and sphere on a flat patch).  Each test is chosen so that a plausible slip (a transposed
 rotation, a sign, a wrong index, gravity per substep instead of per step, a
 friction cone on the wrong impulse) fails it.
 """
@@ -399,3 +401,126 @@ def test_holonomic_factor_reproduces_the_coupled_system():
         b = np.random.default_rng(1).normal(size=tot)
         x = np.linalg.solve(L.T, Dinv @ np.linalg.solve(L, b))
         np.testing.assert_allclose(x, np.linalg.solve(A, b), rtol=1e-8, atol=1e-12)
+
+
+# ----------------------------------------------------------------------------- static meshes
+def grid_mesh(n, cell, heights=None, origin=(0.0, 0.0, 0.0)):
+    """n x n cells split as generateGridMeshInput splits them (Geometry.cpp:589-611)."""
+    ox, oy, oz = origin
+    pts = np.zeros(((n + 1) * (n + 1), 3))
+    for j in range(n + 1):
+        for i in range(n + 1):
+            h = 0.0 if heights is None else heights[j, i]
+            pts[i + (n + 1) * j] = (ox + cell * i, oy + h, oz + cell * j)
+    tris = []
+    for j in range(1, n + 1):
+        for i in range(1, n + 1):
+            i1, i2 = i - 1 + (n + 1) * (j - 1), i + (n + 1) * (j - 1)
+            i3, i4 = i + (n + 1) * j, i - 1 + (n + 1) * j
+            tris += [(i3, i2, i1), (i4, i3, i1)]
+    tris = np.array(tris, np.int64)
+    return pts, tris, np.tile([0.0, 0.5, 0.5], (len(tris), 1))
+
+
+def add_mesh(sc, pts, tris, tmat):
+    return sc._add(dict(mesh=pts, tris=tris, tri_mat=tmat), 1, np.zeros(19), MAT, np.zeros(3), (1, 0, 0, 0), np.zeros(3), np.zeros(3))
+
+
+def test_mesh_tree_candidates_are_exactly_the_overlapping_triangles():
+    """The AABB tree's query returns each triangle whose closed box meets the
+    query box, once (brute force over all triangles)."""
+    rng = np.random.default_rng(3)
+    pts, tris, tmat = grid_mesh(16, 0.5, heights=rng.uniform(-0.4, 0.4, size=(17, 17)), origin=(-4.0, 0.0, -4.0))
+    sc = Scene()
+    add_mesh(sc, pts, tris, tmat)
+    sc.sphere(0.3, (0.0, 2.0, 0.0))
+    o = sc.oracle()
+    tb = np.stack([pts[tris].min(axis=1), pts[tris].max(axis=1)], axis=1).reshape(-1, 6)
+    for _ in range(200):
+        c = rng.uniform(-4.5, 4.5, size=3)
+        c[1] = rng.uniform(-0.6, 0.6)
+        half = rng.uniform(0.01, 1.2, size=3)
+        box = np.concatenate([c - half, c + half])
+        got = o.mesh_query(0, box)
+        want = np.nonzero(~((tb[:, 3] < box[0]) | (tb[:, 0] > box[3]) | (tb[:, 4] < box[1]) | (tb[:, 1] > box[4]) |
+                            (tb[:, 5] < box[2]) | (tb[:, 2] > box[5])))[0]
+        assert len(set(got.tolist())) == len(got)
+        assert sorted(got.tolist()) == want.tolist()
+
+
+def test_mesh_gauss_regions_follow_edge_convexity():
+    """A flat patch: every interior triangle's edges are concave-or-flat, so its
+    only valid direction is its normal (3 concave neighbours, one region
+    vector); border edges contribute their outward direction.  A ridge: the
+    triangles on either side see the ridge edge as convex and take the
+    neighbour's normal."""
+    sc = Scene()
+    pts, tris, tmat = grid_mesh(4, 1.0)
+    add_mesh(sc, pts, tris, tmat)
+    h = np.zeros((5, 5))
+    h[:, 2] = 0.5  # ridge along the middle column of points
+    pts2, tris2, _ = grid_mesh(4, 1.0, heights=h, origin=(10.0, 0.0, 0.0))
+    add_mesh(sc, pts2, tris2, tmat)
+    o = sc.oracle()
+    for t in range(len(tris)):
+        f, i = o.mesh_triangle(0, t)
+        np.testing.assert_allclose(f[:3], (0, 1, 0), atol=1e-15)
+        border = sum(1 for k in range(3) if _is_border(pts, tris, t, k, 4))
+        assert i[7] == 3 - border
+        assert i[6] == border + min(1, 3 - border)  # outward directions, then the normal once
+    convex = 0
+    for t in range(len(tris2)):
+        f, i = o.mesh_triangle(1, t)
+        convex += int(i[7] < 3 - sum(1 for k in range(3) if _is_border(pts2, tris2, t, k, 4)))
+    assert convex > 0
+
+
+def _is_border(pts, tris, t, k, n):
+    a, b = tris[t][k], tris[t][(k + 1) % 3]
+    cnt = 0
+    for u in tris:
+        s = set(u.tolist())
+        if a in s and b in s:
+            cnt += 1
+    return cnt == 1
+
+
+def test_box_on_flat_mesh_patch_has_four_corner_points():
+    """A small box whose footprint lies inside one triangle of a flat patch:
+    one manifold, the patch normal, the four bottom corners at the set depth."""
+    sc = Scene()
+    pts, tris, tmat = grid_mesh(4, 2.0, origin=(-4.0, 0.0, -4.0))
+    add_mesh(sc, pts, tris, tmat)
+    pen, yaw = 0.01, 0.37
+    q = qaxis(yaw, (0, 1, 0))
+    pos = np.array([0.9, 0.1 - pen, 0.45])  # inside the triangle of cell (2, 2) below the diagonal
+    sc.box((0.2, 0.2, 0.2), pos, q=q)
+    o = sc.oracle()
+    ms = o.narrowphase_multi(1, 2)
+    assert len(ms) == 1
+    head, p, mg = ms[0]
+    np.testing.assert_allclose(head[:3], (0, 1, 0), atol=1e-12)
+    assert abs(head[3] - pen) < 1e-12
+    corners = world_points(dict(pts=box_points(0.2, 0.2, 0.2)), np.concatenate([pos, q]))
+    bottom = corners[corners[:, 1] < pos[1]]
+    assert len(p) == 4
+    for x in p:
+        assert abs(x[3] - pen) < 1e-12
+        assert np.min(np.linalg.norm(bottom[:, [0, 2]] - x[[0, 2]], axis=1)) < 1e-12
+
+
+def test_sphere_on_flat_mesh_patch_closed_form():
+    sc = Scene()
+    pts, tris, tmat = grid_mesh(4, 2.0, origin=(-4.0, 0.0, -4.0))
+    add_mesh(sc, pts, tris, tmat)
+    r, pen = 0.5, 0.01
+    c = np.array([0.9, r - pen, 0.45])
+    sc.sphere(r, c)
+    ms = sc.oracle().narrowphase_multi(1, 2)
+    assert len(ms) == 1
+    head, p, _ = ms[0]
+    np.testing.assert_allclose(head[:3], (0, 1, 0), atol=1e-12)
+    assert abs(head[3] - pen) < 1e-12
+    for x in p:
+        np.testing.assert_allclose(x[:3], c - np.array([0, r, 0]), atol=1e-12)
+        assert abs(x[3] - pen) < 1e-12
