@@ -85,6 +85,12 @@ def lib():
         "cp_world_last_residual": (_I, [_P, _P]),
         "cp_shape_add_mesh": (_I, [_P, _I, _P, _I, _P, _P, _P]),
         "cp_narrowphase_multi": (_I, [_P, _I, _P, _I, _P, _P, _P, _P, _P, _P]),
+        "cp_world_serving_init": (_I, [_P, _I]),
+        "cp_world_admit_scene": (_I, [_P, _I, _P, _P, _P, _P, _P, _P]),
+        "cp_world_retire_scene": (_I, [_P, _I]),
+        "cp_world_serving_commit": (_I, [_P]),
+        "cp_world_get_scene_state": (_I, [_P, _I, _P, _P]),
+        "cp_world_serving_info": (_I, [_P, _P]),
         "cp_joint_add": (_I, [_P, _I, _I, _I, _P, _P]),
         "cp_world_joint_count": (_I, [_P, _P]),
         "cp_world_set_joint_state": (_I, [_P, _P]),
@@ -428,6 +434,44 @@ class World:
                                                       sz.ctypes.data), "cp_world_get_holonomic_system")
             out.append((og, o, t, v, b[:sz[3]]))
         return out
+
+    # ---- serving (continuous batching)
+    def serving_init(self, capacity):
+        """Turns this (empty) world into a serving world of `capacity` body slots."""
+        _chk(self.L.cp_world_serving_init(self.w, int(capacity)), "cp_world_serving_init")
+        self.n = int(capacity)
+
+    def admit_scene(self, shape_ids, movement, mass_props, material, state):
+        """Stages a scene into a free body range; returns its handle, or None
+        when no free range is large enough (the request waits)."""
+        sid = _a(shape_ids, np.int32)
+        mv = _a(movement, np.int32)
+        mp = _a(mass_props, np.float64).reshape(-1, 19)
+        mt = _a(material, np.float64).reshape(-1, 4)
+        st = _a(state, np.float64).reshape(-1, 13)
+        h = ctypes.c_int()
+        rc = self.L.cp_world_admit_scene(self.w, len(sid), sid.ctypes.data, mv.ctypes.data, mp.ctypes.data, mt.ctypes.data, st.ctypes.data,
+                                         ctypes.byref(h))
+        if rc == 4:
+            return None
+        _chk(rc, "cp_world_admit_scene")
+        return h.value
+
+    def retire_scene(self, handle):
+        _chk(self.L.cp_world_retire_scene(self.w, int(handle)), "cp_world_retire_scene")
+
+    def scene_state(self, handle):
+        n = ctypes.c_int()
+        _chk(self.L.cp_world_get_scene_state(self.w, int(handle), None, ctypes.byref(n)), "cp_world_get_scene_state")
+        out = np.zeros((n.value, 13))
+        _chk(self.L.cp_world_get_scene_state(self.w, int(handle), out.ctypes.data, ctypes.byref(n)), "cp_world_get_scene_state")
+        return out
+
+    def serving_info(self):
+        """(capacity, free bodies, largest free range, admitted scenes)."""
+        info = np.zeros(4, np.int32)
+        _chk(self.L.cp_world_serving_info(self.w, info.ctypes.data), "cp_world_serving_info")
+        return tuple(int(x) for x in info)
 
     def last_timing(self):
         ms = np.zeros(8)

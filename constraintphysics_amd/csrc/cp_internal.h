@@ -196,6 +196,17 @@ struct NewPts {
 void launch_gravity(const Bodies& b, const Params& p, hipStream_t s);
 void launch_refresh_geometry(const Bodies& b, const Shapes& sh, int only_moved, hipStream_t s);
 void launch_update_inertia(const Bodies& b, hipStream_t s);
+// Serving: body records (3 ints + BODY_REC doubles, see k_body_scatter)
+// written into the SoA arrays with world inertia and geometry derived; list =
+// the records' body indices.  Per-body scene index from scene offsets.
+#define BODY_REC 35
+#define SLOT_PTS 32  // world points reserved per body in a serving world
+#define SLOT_NRM 64  // world Gauss normals reserved per body in a serving world
+void launch_body_scatter(const Bodies& b, const Shapes& sh, int count, const int* ri, const double* rd, const int* list, hipStream_t s);
+void launch_scene_of_body(int n, const int* sbo, int nscenes, int* out, hipStream_t s);
+// Contacts whose first body lies in one of the ranges [r[2k], r[2k] + r[2k+1])
+// are retired: not live, removed at the next merge.
+void launch_kill_contacts(const Contacts& C, int n, const int* ranges, int nranges, hipStream_t s);
 void launch_integrate(const Bodies& b, const Params& p, hipStream_t s);
 void launch_bp_keys(const Bodies& b, uint64_t* keys, int* vals, hipStream_t s);
 void launch_bp_gather(const Bodies& b, const int* sorted, double* sb, hipStream_t s);

@@ -94,8 +94,10 @@ __global__ void k_integrate(Bodies b, Params p) {
 
 // updateInertiaTensor for every body without touching the stored orientation
 // (used when a state is uploaded without its world inertia).
-__global__ void k_update_inertia(Bodies b) {
-	int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_update_inertia(Bodies b, int first, int count, const int* list) {
+	int t = blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= count) return;
+	int i = list ? list[t] : first + t;
 	const int n = b.n;
 	if (i >= n) return;
 	Q q = q_normalize(ldq(b.q, i, n));
@@ -112,8 +114,10 @@ __device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b 
 // and the AABB (ConvexPrimitive.cpp:660-676).  only_moved: the end-of-step
 // refresh (PhysicsEngine.cpp:424-438), which normalises the orientation first
 // and skips bodies whose pose did not change in the last substep.
-__global__ void k_refresh_geometry(Bodies b, Shapes sh, int only_moved) {
-	int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_refresh_geometry(Bodies b, Shapes sh, int only_moved, int first, int count, const int* list) {
+	int t = blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= count) return;
+	int i = list ? list[t] : first + t;
 	const int n = b.n;
 	if (i >= n) return;
 	Q q = ldq(b.q, i, n);
@@ -202,10 +206,64 @@ void launch_integrate(const Bodies& b, const Params& p, hipStream_t s) {
 	if (b.n) hipLaunchKernelGGL(k_integrate, dim3(nblk(b.n, 128)), dim3(128), 0, s, b, p);
 }
 void launch_update_inertia(const Bodies& b, hipStream_t s) {
-	if (b.n) hipLaunchKernelGGL(k_update_inertia, dim3(nblk(b.n, 128)), dim3(128), 0, s, b);
+	if (b.n) hipLaunchKernelGGL(k_update_inertia, dim3(nblk(b.n, 128)), dim3(128), 0, s, b, 0, b.n, nullptr);
 }
 void launch_refresh_geometry(const Bodies& b, const Shapes& sh, int only_moved, hipStream_t s) {
-	if (b.n) hipLaunchKernelGGL(k_refresh_geometry, dim3(nblk(b.n, 128)), dim3(128), 0, s, b, sh, only_moved);
+	if (b.n) hipLaunchKernelGGL(k_refresh_geometry, dim3(nblk(b.n, 128)), dim3(128), 0, s, b, sh, only_moved, 0, b.n, nullptr);
+}
+
+// Writes body records (ints: index, shape, movement; doubles: mass, reference
+// I[9], I^-1[9], restitution, mu_k, mu_s, com[3], q[4], vel[3], angv[3]) into
+// the SoA arrays, then derives world inertia and geometry for those bodies.
+__global__ void k_body_scatter(Bodies b, int count, const int* ri, const double* rd) {
+	int t = blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= count) return;
+	const int n = b.n;
+	int i = ri[3 * t];
+	if (i < 0 || i >= n) return;
+	const double* r = rd + (size_t)BODY_REC * t;
+	b.shape[i] = ri[3 * t + 1];
+	b.movement[i] = ri[3 * t + 2];
+	b.mass[i] = r[0];
+	for (int c = 0; c < 9; c++) {
+		b.Iref[c * n + i] = r[1 + c];
+		b.Irefinv[c * n + i] = r[10 + c];
+	}
+	for (int c = 0; c < 3; c++) {
+		b.mat[c * n + i] = r[19 + c];
+		b.com[c * n + i] = r[22 + c];
+		b.prev_com[c * n + i] = r[22 + c];
+		b.vel[c * n + i] = r[29 + c];
+		b.angv[c * n + i] = r[32 + c];
+		b.pvel[c * n + i] = 0;
+		b.pang[c * n + i] = 0;
+	}
+	for (int c = 0; c < 4; c++) {
+		b.q[c * n + i] = r[25 + c];
+		b.prev_q[c * n + i] = r[25 + c];
+	}
+}
+void launch_body_scatter(const Bodies& b, const Shapes& sh, int count, const int* ri, const double* rd, const int* list, hipStream_t s) {
+	if (count <= 0) return;
+	hipLaunchKernelGGL(k_body_scatter, dim3(nblk(count, 128)), dim3(128), 0, s, b, count, ri, rd);
+	hipLaunchKernelGGL(k_update_inertia, dim3(nblk(count, 128)), dim3(128), 0, s, b, 0, count, list);
+	hipLaunchKernelGGL(k_refresh_geometry, dim3(nblk(count, 128)), dim3(128), 0, s, b, sh, 0, 0, count, list);
+}
+
+// Per-body scene index from the scene offsets (binary search).
+__global__ void k_scene_of_body(int n, const int* sbo, int nscenes, int* out) {
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	int lo = 0, hi = nscenes;
+	while (hi - lo > 1) {
+		int m = (lo + hi) >> 1;
+		if (sbo[m] <= i) lo = m;
+		else hi = m;
+	}
+	out[i] = lo;
+}
+void launch_scene_of_body(int n, const int* sbo, int nscenes, int* out, hipStream_t s) {
+	if (n) hipLaunchKernelGGL(k_scene_of_body, dim3(nblk(n, 256)), dim3(256), 0, s, n, sbo, nscenes, out);
 }
 
 }  // namespace cpx

@@ -294,6 +294,21 @@ void launch_edge_build(const Contacts& C, int n, const int* head, const int* eid
                        hipStream_t s) {
 	if (n) hipLaunchKernelGGL(k_edge_build, dim3(nblk(n, 256)), dim3(256), 0, s, C, n, head, eidx, ekey, efirst, ecnt, elive);
 }
+__global__ void k_kill_contacts(Contacts C, int n, const int* ranges, int nranges) {
+	int k = blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= n) return;
+	int lo = (int)(C.key[k] >> 32);
+	for (int r = 0; r < nranges; r++)
+		if (lo >= ranges[2 * r] && lo < ranges[2 * r] + ranges[2 * r + 1]) {
+			C.life[k] = -100;
+			C.live[k] = 0;
+			return;
+		}
+}
+void launch_kill_contacts(const Contacts& C, int n, const int* ranges, int nranges, hipStream_t s) {
+	if (n > 0 && nranges > 0) hipLaunchKernelGGL(k_kill_contacts, dim3(nblk(n, 256)), dim3(256), 0, s, C, n, ranges, nranges);
+}
+
 void launch_color(const Bodies& b, const int* scene_body_off, int nscenes, const uint64_t* ekey, const int* elive, int nedges,
                   int* ecolor, int* order, int* color_off, int* ncolors, int* edge_range, hipStream_t s) {
 	hipLaunchKernelGGL(k_color, dim3(nblk(nscenes, 64)), dim3(64), 0, s, b, scene_body_off, nscenes, ekey, elive, nedges, ecolor, order,

@@ -120,6 +120,26 @@ T read1(const T* d) {
 	return v;
 }
 
+// Grows d to hold n elements keeping its first `keep` elements.
+template <class T>
+void grow_keep(DArr<T>& d, size_t n, size_t keep) {
+	if (n <= d.cap && d.p) return;
+	size_t m = std::max<size_t>(n + n / 2, 16);
+	T* p = nullptr;
+	if (hipMalloc(&p, m * sizeof(T)) != hipSuccess) throw ApiErr{CP_ERR_OOM};
+	if (keep && d.p) HIPCHK(hipMemcpy(p, d.p, keep * sizeof(T), hipMemcpyDeviceToDevice));
+	if (d.p) HIPCHK(hipFree(d.p));
+	d.p = p;
+	d.cap = m;
+}
+// Appends h[done..] to the device copy (host tables only ever grow).
+template <class T>
+void up_tail(DArr<T>& d, const std::vector<T>& h, size_t& done) {
+	grow_keep(d, h.size(), done);
+	if (h.size() > done) HIPCHK(hipMemcpy(d.p + done, h.data() + done, (h.size() - done) * sizeof(T), hipMemcpyHostToDevice));
+	done = h.size();
+}
+
 }  // namespace
 
 struct cp_world {
@@ -215,6 +235,28 @@ struct cp_world {
 	DArr<uint64_t> mi_magic, mi_avail;
 	bool last_expanded = false;
 	int mesh_depth = 0;
+	// serving: a fixed body capacity; scenes are admitted into free ranges and
+	// retired back to parked bodies without moving other scenes' bodies, so a
+	// scene steps exactly as it would alone.  Changes are staged and applied
+	// at the next step (or commit) in one scatter.
+	bool serving = false;
+	int park_shape = -1;
+	struct Range {
+		int start, len, handle;  // handle -1: free (parked bodies)
+	};
+	std::vector<Range> ranges;
+	int next_handle = 0;
+	std::vector<int> stage_i, kill_ranges, stage_mark;
+	std::vector<double> stage_d;
+	int commit_id = 0;
+	bool serving_dirty = false;
+	size_t sh_done[32] = {0};
+	DArr<int> st_i, st_list, st_kill;
+	DArr<double> st_d;
+	void sync_shapes();
+	void stage_body(int i, int shape, int move, const double* mp19, const double* mat4, const double* st13);
+	void park_range(int first, int count);
+	void serving_commit();
 
 	Bodies bodies() {
 		Bodies b{};
@@ -384,8 +426,8 @@ void cp_world::upload() {
 		int s = b_shape[i];
 		// world points and normals per body: hull vertices and Gauss normals;
 		// sphere centre; cylinder centre + cap polygons and axis + Gauss vertices
-		wpo[i + 1] = wpo[i] + (h.pt_off[s + 1] - h.pt_off[s]);
-		wgo[i + 1] = wgo[i] + (h.face_off[s + 1] - h.face_off[s]);
+		wpo[i + 1] = wpo[i] + (serving ? SLOT_PTS : h.pt_off[s + 1] - h.pt_off[s]);
+		wgo[i + 1] = wgo[i] + (serving ? SLOT_NRM : h.face_off[s + 1] - h.face_off[s]);
 	}
 	up(Iref, Ir); up(Irefinv, Iri); up(mass, ms); up(mat, mt);
 	up(movement, b_move); up(shape, b_shape); up(scene, b_scene);
@@ -749,6 +791,99 @@ void cp_world::discover_systems() {
 	layout_systems();
 }
 
+// ---------------------------------------------------------------- serving
+// Shape tables are append-only on the host; only their new tails go to the device.
+void cp_world::sync_shapes() {
+	HostShapes& h = hs;
+	int k = 0;
+	auto ints = [&](DArr<int>& d, std::vector<int>& v) { up_tail(d, v, sh_done[k++]); };
+	auto dbls = [&](DArr<double>& d, std::vector<double>& v) { up_tail(d, v, sh_done[k++]); };
+	ints(s_type, h.type); ints(s_pt_off, h.pt_off); ints(s_face_off, h.face_off); ints(s_arc_off, h.arc_off); ints(s_edge_off, h.edge_off);
+	dbls(s_ref_pts, h.ref_pts); ints(s_fv_off, h.fv_off); ints(s_fv, h.fv); ints(s_ncalc, h.ncalc); ints(s_ndir, h.ndir); ints(s_sid, h.sid);
+	dbls(s_gnorm, h.gnorm); ints(s_red, h.red); ints(s_minpid, h.minpid); ints(s_maxpid, h.maxpid); ints(s_refidx, h.refidx);
+	dbls(s_gmin, h.gmin); dbls(s_gmax, h.gmax); dbls(s_grefval, h.grefval); ints(s_arcs, h.arcs); ints(s_edges, h.edges);
+	ints(s_vaf_off, h.vaf_off); ints(s_vaf, h.vaf); ints(s_vae_off, h.vae_off); ints(s_vae, h.vae);
+	dbls(s_radius, h.radius); dbls(s_height, h.height);
+}
+
+void cp_world::stage_body(int i, int shape_id, int move, const double* mp19, const double* mat4, const double* st13) {
+	b_shape[i] = shape_id;
+	b_move[i] = move;
+	for (int c = 0; c < 19; c++) b_massprops[19 * (size_t)i + c] = mp19[c];
+	for (int c = 0; c < 4; c++) b_mat[4 * (size_t)i + c] = mat4[c];
+	stage_i.insert(stage_i.end(), {i, shape_id, move});
+	stage_d.insert(stage_d.end(), mp19, mp19 + 19);
+	stage_d.insert(stage_d.end(), mat4 + 1, mat4 + 4);
+	stage_d.insert(stage_d.end(), st13, st13 + 13);
+}
+
+// Parked bodies: FIXED unit spheres far from everything, 3 apart.
+void cp_world::park_range(int first, int count) {
+	double mp[19] = {1, 1, 0, 0, 0, 1, 0, 0, 0, 1, 1, 0, 0, 0, 1, 0, 0, 0, 1};
+	double mat4[4] = {0, 0, 0, 0};
+	for (int i = first; i < first + count; i++) {
+		double st[13] = {1.0e7 + 3.0 * i, -1.0e7, 0, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+		stage_body(i, park_shape, 1, mp, mat4, st);
+	}
+}
+
+// Applies staged admissions and retirements: new shape-table tails, one body
+// scatter, contact retirement, scene offsets and size-class lists.
+void cp_world::serving_commit() {
+	if (!serving_dirty) return;
+	sync_shapes();
+	int nrec = (int)stage_i.size() / 3;
+	if (nrec) {
+		// a body staged more than once (retired, then admitted again before this
+		// commit) keeps its last record: the scatter writes each body once
+		if ((int)stage_mark.size() < nb) stage_mark.assign(nb, -1);
+		commit_id++;
+		std::vector<int> ri, list;
+		std::vector<double> rd;
+		for (int k = nrec - 1; k >= 0; k--) {
+			int i = stage_i[3 * k];
+			if (stage_mark[i] == commit_id) continue;
+			stage_mark[i] = commit_id;
+			ri.insert(ri.end(), stage_i.begin() + 3 * k, stage_i.begin() + 3 * k + 3);
+			rd.insert(rd.end(), stage_d.begin() + (size_t)BODY_REC * k, stage_d.begin() + (size_t)BODY_REC * (k + 1));
+			list.push_back(i);
+		}
+		up(st_i, ri);
+		up(st_d, rd);
+		up(st_list, list);
+		launch_body_scatter(bodies(), shapes(), (int)list.size(), st_i.p, st_d.p, st_list.p, stream);
+	}
+	if (!kill_ranges.empty()) {
+		up(st_kill, kill_ranges);
+		launch_kill_contacts(cb[cur].c, ncontacts, st_kill.p, (int)kill_ranges.size() / 2, stream);
+	}
+	stage_i.clear();
+	stage_d.clear();
+	kill_ranges.clear();
+	// every range is a scene (free ranges hold parked bodies and never pair);
+	// only admitted scenes enter the solver's size-class lists
+	std::vector<int> sbo;
+	for (const Range& r : ranges) sbo.push_back(r.start);
+	sbo.push_back(nb);
+	nscenes = (int)ranges.size();
+	up(scene_body_off, sbo);
+	scene_of_body.reserve(nb);
+	launch_scene_of_body(nb, scene_body_off.p, nscenes, scene_of_body.p, stream);
+	cls_lists.assign(4, std::vector<int>());
+	for (int q = 0; q < nscenes; q++) {
+		if (ranges[q].handle < 0) continue;
+		int m = ranges[q].len;
+		cls_lists[m <= 192 ? 0 : m <= 768 ? 1 : m <= 3072 ? 2 : 3].push_back(q);
+	}
+	for (int c = 0; c < 4; c++) {
+		std::sort(cls_lists[c].begin(), cls_lists[c].end(), [&](int a, int b2) { return ranges[a].len > ranges[b2].len; });
+		cls_max[c] = cls_lists[c].empty() ? 0 : ranges[cls_lists[c][0]].len;
+		up(cls_dev[c], cls_lists[c]);
+	}
+	HIPCHK(hipStreamSynchronize(stream));
+	serving_dirty = false;
+}
+
 void cp_world::scan(const int* in, int* out, int n) {
 	size_t bytes = 0;
 	HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, n, stream));
@@ -1050,6 +1185,7 @@ void cp_world::launch_pgs_colored_lds_global(const Bodies& b, const Params& P) {
 }
 
 void cp_world::step() {
+	if (serving) serving_commit();
 	if (!j_type.empty() && cfg.pgs_mode != CP_PGS_REFERENCE_ORDER) throw ApiErr{CP_ERR_STATE};  // joints: sequential order only
 	if (!hs_explicit && cfg.hol_iters > 0 && (hs_scan || hs_changed)) {
 		discover_systems();
@@ -1282,14 +1418,14 @@ int cp_shape_add_hull_exact(cp_world* w, int n_pts, const double* pts, int n_fac
 	for (int a = 0; a < n_arcs; a++) d.arcs.push_back({(int)arcs[2 * a], (int)arcs[2 * a + 1]});
 	for (int e = 0; e < n_edges; e++) d.edges.push_back({(int)edges[2 * e], (int)edges[2 * e + 1]});
 	*shape_id = w->hs.add_hull(d);
-	w->uploaded = false;
+	if (!w->serving) w->uploaded = false;  // a serving world appends shape tables at its next commit
 	return CP_OK;
 }
 
 int cp_shape_add_sphere(cp_world* w, const double* center, double radius, int* shape_id) {
 	if (!w || !center || !shape_id || !(radius > 0)) return CP_ERR_INVALID;
 	*shape_id = w->hs.add_sphere(center, radius);
-	w->uploaded = false;
+	if (!w->serving) w->uploaded = false;  // a serving world appends shape tables at its next commit
 	return CP_OK;
 }
 
@@ -1297,7 +1433,7 @@ int cp_shape_add_cylinder(cp_world* w, const double* params8, const double* gaus
                           int* shape_id) {
 	if (!w || !params8 || !gauss42 || !top24 || !bot24 || !shape_id || !(params8[6] > 0) || !(params8[7] > 0)) return CP_ERR_INVALID;
 	*shape_id = w->hs.add_cylinder(params8, gauss42, top24, bot24);
-	w->uploaded = false;
+	if (!w->serving) w->uploaded = false;  // a serving world appends shape tables at its next commit
 	return CP_OK;
 }
 
@@ -1315,7 +1451,7 @@ int cp_shape_add_mesh(cp_world* w, int n_pts, const double* pts, int n_tris, con
 	for (int k = 0; k < 3 * n_pts; k++)
 		if (!std::isfinite(pts[k])) return CP_ERR_INVALID;
 	*shape_id = w->hs.add_mesh(build_mesh(n_pts, pts, n_tris, tri_idx, tri_mat3));
-	w->uploaded = false;
+	if (!w->serving) w->uploaded = false;  // a serving world appends shape tables at its next commit
 	return CP_OK;
 }
 
@@ -1334,7 +1470,7 @@ int cp_shape_add_polyhedron(cp_world* w, int n_pts, const double* pts, int n_fac
 	}
 	HullData d = build_polyhedron(std::vector<double>(pts, pts + 3 * (size_t)n_pts), faces);
 	*shape_id = w->hs.add_hull(d);
-	w->uploaded = false;
+	if (!w->serving) w->uploaded = false;  // a serving world appends shape tables at its next commit
 	return CP_OK;
 }
 
@@ -1390,6 +1526,7 @@ int cp_polyhedron_mass_properties(int n_pts, const double* pts, int n_faces, con
 
 int cp_body_add(cp_world* w, int shape_id, int movement, const double* mp, const double* material, int scene, int* body_index) {
 	if (!w || !mp || !material || !body_index) return CP_ERR_INVALID;
+	if (w->serving) return CP_ERR_STATE;  // serving worlds take whole scenes (cp_world_admit_scene)
 	if (shape_id < 0 || shape_id >= (int)w->hs.type.size() || (movement != 0 && movement != 1)) return CP_ERR_INVALID;
 	if (w->hs.type[shape_id] == SHAPE_MESH && movement != 1) return CP_ERR_INVALID;  // meshes are static (RigidBody.cpp:121)
 	if (!w->b_scene.empty() && scene < w->b_scene.back()) return CP_ERR_INVALID;  // scenes are contiguous and ascending
@@ -1400,13 +1537,14 @@ int cp_body_add(cp_world* w, int shape_id, int movement, const double* mp, const
 	w->b_massprops.insert(w->b_massprops.end(), mp, mp + 19);
 	w->b_mat.insert(w->b_mat.end(), material, material + 4);
 	*body_index = w->nb++;
-	w->uploaded = false;
+	if (!w->serving) w->uploaded = false;  // a serving world appends shape tables at its next commit
 	return CP_OK;
 }
 
 int cp_body_add_many(cp_world* w, int n, const int* shape_ids, const int* movement, const double* mp, const double* material,
                      const int* scene, int* first_index) {
 	if (!w || n < 0 || !first_index || (n > 0 && (!shape_ids || !movement || !mp || !material || !scene))) return CP_ERR_INVALID;
+	if (w->serving) return CP_ERR_STATE;
 	if (w->uploaded && w->ncontacts > 0) return CP_ERR_STATE;
 	int last = w->b_scene.empty() ? -1 : w->b_scene.back();
 	for (int i = 0; i < n; i++) {
@@ -1422,7 +1560,7 @@ int cp_body_add_many(cp_world* w, int n, const int* shape_ids, const int* moveme
 	w->b_massprops.insert(w->b_massprops.end(), mp, mp + 19 * (size_t)n);
 	w->b_mat.insert(w->b_mat.end(), material, material + 4 * (size_t)n);
 	w->nb += n;
-	w->uploaded = false;
+	if (!w->serving) w->uploaded = false;  // a serving world appends shape tables at its next commit
 	return CP_OK;
 }
 
@@ -1755,6 +1893,127 @@ int cp_reference_node_order(int n, int64_t* ids) {
 	for (int i = 1; i <= n; i++) m[(unsigned)i] = i;
 	int k = 0;
 	for (auto& kv : m) ids[k++] = kv.first;
+	return CP_OK;
+}
+
+int cp_world_serving_init(cp_world* w, int capacity) {
+	if (!w || capacity < 1) return CP_ERR_INVALID;
+	if (w->nb != 0 || !w->j_type.empty() || w->serving) return CP_ERR_STATE;
+	return guard(w, [&] {
+		double c[3] = {0, 0, 0};
+		w->park_shape = w->hs.add_sphere(c, 0.5);
+		w->nb = capacity;
+		w->b_shape.assign(capacity, w->park_shape);
+		w->b_move.assign(capacity, 1);
+		w->b_scene.assign(capacity, 0);
+		w->b_massprops.assign(19 * (size_t)capacity, 0.0);
+		w->b_mat.assign(4 * (size_t)capacity, 0.0);
+		w->serving = true;
+		w->upload();
+		for (size_t& d : w->sh_done) d = 0;
+		w->ranges = {{0, capacity, -1}};
+		w->park_range(0, capacity);
+		w->serving_dirty = true;
+		w->serving_commit();
+	});
+}
+
+int cp_world_admit_scene(cp_world* w, int n, const int* shape_ids, const int* movement, const double* mp, const double* material,
+                         const double* state13, int* handle) {
+	if (!w || !w->serving || n < 1 || !shape_ids || !movement || !mp || !material || !state13 || !handle) return CP_ERR_INVALID;
+	for (int i = 0; i < n; i++) {
+		int sid = shape_ids[i];
+		if (sid < 0 || sid >= (int)w->hs.type.size() || (movement[i] != 0 && movement[i] != 1)) return CP_ERR_INVALID;
+		if (w->hs.type[sid] == SHAPE_MESH) return CP_ERR_INVALID;
+		if (w->hs.pt_off[sid + 1] - w->hs.pt_off[sid] > SLOT_PTS || w->hs.face_off[sid + 1] - w->hs.face_off[sid] > SLOT_NRM)
+			return CP_ERR_INVALID;  // larger than a serving slot
+	}
+	int r = -1;
+	for (int k = 0; k < (int)w->ranges.size(); k++)
+		if (w->ranges[k].handle < 0 && w->ranges[k].len >= n) { r = k; break; }
+	if (r < 0) return CP_ERR_CAPACITY;  // no free range this large: the caller queues the request
+	cp_world::Range fr = w->ranges[r];
+	int h = w->next_handle++;
+	w->ranges[r] = {fr.start, n, h};
+	if (fr.len > n) w->ranges.insert(w->ranges.begin() + r + 1, cp_world::Range{fr.start + n, fr.len - n, -1});
+	for (int i = 0; i < n; i++)
+		w->stage_body(fr.start + i, shape_ids[i], movement[i], mp + 19 * (size_t)i, material + 4 * (size_t)i, state13 + 13 * (size_t)i);
+	w->serving_dirty = true;
+	*handle = h;
+	return CP_OK;
+}
+
+static int find_range(cp_world* w, int handle) {
+	for (int k = 0; k < (int)w->ranges.size(); k++)
+		if (w->ranges[k].handle == handle) return k;
+	return -1;
+}
+
+int cp_world_retire_scene(cp_world* w, int handle) {
+	if (!w || !w->serving) return CP_ERR_INVALID;
+	int r = find_range(w, handle);
+	if (r < 0) return CP_ERR_INVALID;
+	cp_world::Range& R = w->ranges[r];
+	w->park_range(R.start, R.len);
+	w->kill_ranges.insert(w->kill_ranges.end(), {R.start, R.len});
+	R.handle = -1;
+	// merge free neighbours
+	if (r + 1 < (int)w->ranges.size() && w->ranges[r + 1].handle < 0) {
+		w->ranges[r].len += w->ranges[r + 1].len;
+		w->ranges.erase(w->ranges.begin() + r + 1);
+	}
+	if (r > 0 && w->ranges[r - 1].handle < 0) {
+		w->ranges[r - 1].len += w->ranges[r].len;
+		w->ranges.erase(w->ranges.begin() + r);
+	}
+	w->serving_dirty = true;
+	return CP_OK;
+}
+
+int cp_world_serving_commit(cp_world* w) {
+	if (!w || !w->serving) return CP_ERR_INVALID;
+	return guard(w, [&] { w->serving_commit(); });
+}
+
+int cp_world_get_scene_state(cp_world* w, int handle, double* state13, int* n) {
+	if (!w || !w->serving || !n) return CP_ERR_INVALID;
+	int r = find_range(w, handle);
+	if (r < 0) return CP_ERR_INVALID;
+	return guard(w, [&] {
+		w->serving_commit();
+		HIPCHK(hipStreamSynchronize(w->stream));
+		const cp_world::Range& R = w->ranges[r];
+		*n = R.len;
+		if (!state13) return;
+		int N = w->nb;
+		std::vector<double> buf((size_t)R.len);
+		auto get = [&](DArr<double>& a, int comps, int col) {
+			for (int c = 0; c < comps; c++) {
+				HIPCHK(hipMemcpy(buf.data(), a.p + (size_t)c * N + R.start, R.len * sizeof(double), hipMemcpyDeviceToHost));
+				for (int i = 0; i < R.len; i++) state13[13 * (size_t)i + col + c] = buf[i];
+			}
+		};
+		get(w->com, 3, 0);
+		get(w->q, 4, 3);
+		get(w->vel, 3, 7);
+		get(w->angv, 3, 10);
+	});
+}
+
+// info4: capacity, free bodies, largest free range, admitted scenes.
+int cp_world_serving_info(cp_world* w, int* info4) {
+	if (!w || !w->serving || !info4) return CP_ERR_INVALID;
+	int fr = 0, big = 0, act = 0;
+	for (const cp_world::Range& r : w->ranges) {
+		if (r.handle < 0) {
+			fr += r.len;
+			big = std::max(big, r.len);
+		} else act++;
+	}
+	info4[0] = w->nb;
+	info4[1] = fr;
+	info4[2] = big;
+	info4[3] = act;
 	return CP_OK;
 }
 
