@@ -126,6 +126,38 @@ def cpu_baseline(sizes, seed, budget_s, steps, state, contacts, off):
     return bodies / spent, used, spent
 
 
+def serving_leg(args):
+    """Continuous batching (constraintphysics_amd.serving): Poisson arrivals at
+    2000 requests/s, each a scene of the same size distribution with a step
+    count log-uniform in [1, 240]; admission into a serving world of
+    --serve-capacity body slots."""
+    import constraintphysics_amd as cpa
+    from constraintphysics_amd import serving
+    rate = 2000.0
+    reqs, sizes = serving.poisson_requests(args.serve_requests, rate, args.seed + 1)
+    w = cpa.World(dt=1.0 / 120.0, gravity=(0.0, -6.0, 0.0), vel_iters=10, pos_iters=1, substeps=8, pgs_mode=0, solver_threads=256,
+                  precision=PRECISIONS[args.precision])
+    w.serving_init(args.serve_capacity)
+    src = serving.SceneSource(w, sizes, args.seed + 1)
+    out = serving.serve(w, src, reqs, log=lambda m: print(m, file=sys.stderr, flush=True))
+    w.close()
+    out.pop("states")
+    offered = sum(r.bodies * r.steps for r in reqs) / (reqs[-1].arrival + 1e-9)
+    res = dict(value=round(out["value"], 1), unit="body-steps/s", steady_state_value=round(out["steady_value"], 1),
+               requests=out["requests"], arrival_rate_per_s=rate,
+               offered_body_steps_per_s=round(offered, 1), capacity_bodies=args.serve_capacity,
+               p50_request_ms=round(out["p50_request_ms"], 2), p99_request_ms=round(out["p99_request_ms"], 2),
+               p50_step_ms=round(out["p50_step_ms"], 3), p99_step_ms=round(out["p99_step_ms"], 3), ticks=out["ticks"],
+               mean_queue_wait_ms=round(out["mean_wait_ms"], 2), wall_s=round(out["wall_s"], 3),
+               note="requests arrive in real time; each admitted as soon as a free body range holds it, stepped with every other "
+                    "admitted scene in one batched step per tick, read back and retired when its steps are done; latency = "
+                    "arrival to result; value = dynamic body-steps completed / wall time of the whole run (including the drain "
+                    "after the last arrival); steady_state_value = the same over the ticks that start before the last arrival")
+    print(f"[bench] serving: {res['value']:.0f} body-steps/s (steady {res['steady_state_value']:.0f}), request p50 {res['p50_request_ms']:.0f} ms p99 "
+          f"{res['p99_request_ms']:.0f} ms, step p50 {res['p50_step_ms']:.1f} ms", file=sys.stderr, flush=True)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -137,6 +169,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=6000, help="requests of the serving leg (Poisson arrivals at 2000/s)")
+    ap.add_argument("--serve-capacity", type=int, default=1600000, help="body slots of the serving world (its HBM budget)")
     ap.add_argument("--precision", choices=sorted(PRECISIONS), default="split",
                     help="solve precision: split = FP32 rows/impulses/velocity deltas over FP64 state; fp64 = reference arithmetic")
     args = ap.parse_args()
@@ -223,6 +258,10 @@ def main():
                           "batched run's settled state and contact cache, 2 untimed steps then the same number of timed steps")
         print(f"[bench] plain single-scene path: {plain['value']:.0f} body-steps/s", file=sys.stderr, flush=True)
 
+    served = None
+    if not args.no_serve:
+        served = serving_leg(args)
+
     cpu = None
     if not args.no_cpu:
         v, used, spent = cpu_baseline(sizes, args.seed, args.cpu_budget, args.steps, snap_state, snap_contacts, off)
@@ -266,6 +305,8 @@ def main():
                                           "contact impulses / body velocities"},
         "plain_gpu_path": plain,
         "speedup_vs_plain": round(value / plain["value"], 2) if plain else None,
+        "serving": served,
+        "serving_speedup_vs_plain": round(served["value"] / plain["value"], 2) if (plain and served) else None,
         "roofline": {
             "bound": "hbm",
             "kernel": "k_pgs_colored_lds (one substep's PGS solve: one launch per scene size class)",

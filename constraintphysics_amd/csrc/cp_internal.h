@@ -42,7 +42,8 @@ struct Shapes {
 };
 
 struct Bodies {
-	int n;
+	int n;   // bodies (the stride of every per-body array)
+	int nl;  // bodies the per-step kernels visit: all, or a serving world's occupied prefix
 	double *com, *q, *vel, *angv, *pvel, *pang, *prev_com, *prev_q;
 	double *I, *Iinv, *Iref, *Irefinv;
 	double* mass;

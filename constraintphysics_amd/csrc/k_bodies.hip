@@ -29,7 +29,7 @@ __device__ __forceinline__ void stm(double* a, int i, int n, const M3& m) {
 // PhysicsEngine.cpp:84-86: v += g * dt once per step for awake dynamic bodies.
 __global__ void k_gravity(Bodies b, Params p) {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= b.n || b.movement[i] != 0) return;
+	if (i >= b.nl || b.movement[i] != 0) return;
 	V3 v = LD3(b.vel, i, b.n);
 	v += V3{p.gx, p.gy, p.gz} * p.dt;
 	ST3(b.vel, i, b.n, v);
@@ -66,7 +66,7 @@ __device__ void gyro_rotate(V3& angv, Q& q, const M3& I_world, const M3& I_ref, 
 __global__ void k_integrate(Bodies b, Params p) {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
 	const int n = b.n;
-	if (i >= n || b.movement[i] != 0) return;
+	if (i >= b.nl || b.movement[i] != 0) return;
 	V3 com = LD3(b.com, i, n), vel = LD3(b.vel, i, n), angv = LD3(b.angv, i, n);
 	V3 pvel = LD3(b.pvel, i, n), pang = LD3(b.pang, i, n);
 	Q q = ldq(b.q, i, n);
@@ -200,16 +200,17 @@ __global__ void k_refresh_geometry(Bodies b, Shapes sh, int only_moved, int firs
 static inline int nblk(int n, int t) { return (n + t - 1) / t; }
 
 void launch_gravity(const Bodies& b, const Params& p, hipStream_t s) {
-	if (b.n) hipLaunchKernelGGL(k_gravity, dim3(nblk(b.n, 256)), dim3(256), 0, s, b, p);
+	if (b.nl) hipLaunchKernelGGL(k_gravity, dim3(nblk(b.nl, 256)), dim3(256), 0, s, b, p);
 }
 void launch_integrate(const Bodies& b, const Params& p, hipStream_t s) {
-	if (b.n) hipLaunchKernelGGL(k_integrate, dim3(nblk(b.n, 128)), dim3(128), 0, s, b, p);
+	if (b.nl) hipLaunchKernelGGL(k_integrate, dim3(nblk(b.nl, 128)), dim3(128), 0, s, b, p);
 }
 void launch_update_inertia(const Bodies& b, hipStream_t s) {
 	if (b.n) hipLaunchKernelGGL(k_update_inertia, dim3(nblk(b.n, 128)), dim3(128), 0, s, b, 0, b.n, nullptr);
 }
 void launch_refresh_geometry(const Bodies& b, const Shapes& sh, int only_moved, hipStream_t s) {
-	if (b.n) hipLaunchKernelGGL(k_refresh_geometry, dim3(nblk(b.n, 128)), dim3(128), 0, s, b, sh, only_moved, 0, b.n, nullptr);
+	int cnt = only_moved ? b.nl : b.n;
+	if (cnt) hipLaunchKernelGGL(k_refresh_geometry, dim3(nblk(cnt, 128)), dim3(128), 0, s, b, sh, only_moved, 0, cnt, nullptr);
 }
 
 // Writes body records (ints: index, shape, movement; doubles: mass, reference

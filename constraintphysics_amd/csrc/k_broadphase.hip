@@ -18,18 +18,19 @@ namespace cpx {
 
 __global__ void k_bp_keys(Bodies b, uint64_t* keys, int* vals) {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= b.n) return;
+	if (i >= b.nl) return;
 	uint64_t u = (uint64_t)__double_as_longlong(b.aabb[i]);
 	keys[i] = (u >> 63) ? ~u : (u | 0x8000000000000000ull);
 	vals[i] = i;
 }
 
+// sb: sorted bounds of the b.nl visited bodies, stride b.nl.
 __global__ void k_bp_gather(Bodies b, const int* sorted, double* sb) {
 	int p = blockIdx.x * blockDim.x + threadIdx.x;
-	const int n = b.n;
-	if (p >= n) return;
+	const int m = b.nl;
+	if (p >= m) return;
 	int i = sorted[p];
-	for (int c = 0; c < 6; c++) sb[c * n + p] = b.aabb[c * n + i];
+	for (int c = 0; c < 6; c++) sb[c * m + p] = b.aabb[c * b.n + i];
 }
 
 // scene_end[p]: one past the last sorted position of p's scene.
@@ -63,15 +64,15 @@ __global__ void k_bp_sweep(int n, const int* sorted, const double* sb, const int
 static inline int nblk(int n, int t) { return (n + t - 1) / t; }
 
 void launch_bp_keys(const Bodies& b, uint64_t* keys, int* vals, hipStream_t s) {
-	hipLaunchKernelGGL(k_bp_keys, dim3(nblk(b.n, 256)), dim3(256), 0, s, b, keys, vals);
+	if (b.nl) hipLaunchKernelGGL(k_bp_keys, dim3(nblk(b.nl, 256)), dim3(256), 0, s, b, keys, vals);
 }
 void launch_bp_gather(const Bodies& b, const int* sorted, double* sb, hipStream_t s) {
-	hipLaunchKernelGGL(k_bp_gather, dim3(nblk(b.n, 256)), dim3(256), 0, s, b, sorted, sb);
+	if (b.nl) hipLaunchKernelGGL(k_bp_gather, dim3(nblk(b.nl, 256)), dim3(256), 0, s, b, sorted, sb);
 }
 void launch_bp_sweep(int n, const int* sorted, const double* sb, const int* scene_of_body, const int* scene_body_off, int* counts,
                      const int* offs, uint64_t* out, int write, hipStream_t s) {
-	hipLaunchKernelGGL(k_bp_sweep, dim3(nblk(n, 128)), dim3(128), 0, s, n, sorted, sb, scene_of_body, scene_body_off, counts, offs,
-	                   out, write);
+	if (n) hipLaunchKernelGGL(k_bp_sweep, dim3(nblk(n, 128)), dim3(128), 0, s, n, sorted, sb, scene_of_body, scene_body_off, counts, offs,
+	                          out, write);
 }
 
 }  // namespace cpx

@@ -215,8 +215,12 @@ __global__ void k_color(Bodies b, const int* scene_body_off, int nscenes, const 
 	if (s >= nscenes) return;
 	int b0 = scene_body_off[s], b1 = scene_body_off[s + 1];
 	uint64_t* mask = (uint64_t*)b.color_mask;
-	for (int i = b0; i < b1; i++) { mask[2 * i] = 0; mask[2 * i + 1] = 0; }
 	int e0 = lower_bound_u64(ekey, nedges, (uint64_t)b0 << 32), e1 = lower_bound_u64(ekey, nedges, (uint64_t)b1 << 32);
+	for (int e = e0; e < e1; e++) {  // only bodies on an edge need a clear colour set
+		int lo = (int)(ekey[e] >> 32), hi = (int)(ekey[e] & 0xFFFFFFFFu);
+		mask[2 * lo] = 0; mask[2 * lo + 1] = 0;
+		mask[2 * hi] = 0; mask[2 * hi + 1] = 0;
+	}
 	edge_range[2 * s] = e0;
 	edge_range[2 * s + 1] = e1;
 	int cnt[MAXC + 1];

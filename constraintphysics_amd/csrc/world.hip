@@ -258,9 +258,11 @@ struct cp_world {
 	void park_range(int first, int count);
 	void serving_commit();
 
+	int live_n = 0, live_scenes = 0;  // serving: occupied prefix (bodies, scenes)
 	Bodies bodies() {
 		Bodies b{};
 		b.n = nb;
+		b.nl = serving ? live_n : nb;
 		b.com = com.p; b.q = q.p; b.vel = vel.p; b.angv = angv.p; b.pvel = pvel.p; b.pang = pang.p;
 		b.prev_com = prev_com.p; b.prev_q = prev_q.p;
 		b.I = I.p; b.Iinv = Iinv.p; b.Iref = Iref.p; b.Irefinv = Irefinv.p;
@@ -869,6 +871,13 @@ void cp_world::serving_commit() {
 	up(scene_body_off, sbo);
 	scene_of_body.reserve(nb);
 	launch_scene_of_body(nb, scene_body_off.p, nscenes, scene_of_body.p, stream);
+	live_n = 0;
+	live_scenes = 0;
+	for (int q = 0; q < nscenes; q++)
+		if (ranges[q].handle >= 0) {
+			live_n = ranges[q].start + ranges[q].len;
+			live_scenes = q + 1;
+		}
 	cls_lists.assign(4, std::vector<int>());
 	for (int q = 0; q < nscenes; q++) {
 		if (ranges[q].handle < 0) continue;
@@ -932,21 +941,23 @@ void cp_world::ensure_points(int n) {
 // Returns the number of candidate pairs now in `pairs` (sorted keys).
 int cp_world::broadphase() {
 	Bodies b = bodies();
-	int n = nb;
+	int n = b.nl;
+	int ns = serving ? live_scenes : nscenes;
+	if (n == 0) return 0;
 	bp_keys.reserve(n); bp_keys2.reserve(n); bp_vals.reserve(n); bp_vals2.reserve(n);
 	bp_sb.reserve(6 * (size_t)n);
 	bp_counts.reserve(n + 1); bp_offs.reserve(n + 1);
 	launch_bp_keys(b, bp_keys.p, bp_vals.p, stream);
 	size_t bytes = 0;
-	if (nscenes == 1) {
+	if (ns == 1) {
 		HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, bp_keys.p, bp_keys2.p, bp_vals.p, bp_vals2.p, n, 0, 64, stream));
 		tmp.reserve(bytes + 16);
 		HIPCHK(hipcub::DeviceRadixSort::SortPairs(tmp.p, bytes, bp_keys.p, bp_keys2.p, bp_vals.p, bp_vals2.p, n, 0, 64, stream));
 	} else {
-		HIPCHK(hipcub::DeviceSegmentedRadixSort::SortPairs(nullptr, bytes, bp_keys.p, bp_keys2.p, bp_vals.p, bp_vals2.p, n, nscenes,
+		HIPCHK(hipcub::DeviceSegmentedRadixSort::SortPairs(nullptr, bytes, bp_keys.p, bp_keys2.p, bp_vals.p, bp_vals2.p, n, ns,
 		                                                   scene_body_off.p, scene_body_off.p + 1, 0, 64, stream));
 		tmp.reserve(bytes + 16);
-		HIPCHK(hipcub::DeviceSegmentedRadixSort::SortPairs(tmp.p, bytes, bp_keys.p, bp_keys2.p, bp_vals.p, bp_vals2.p, n, nscenes,
+		HIPCHK(hipcub::DeviceSegmentedRadixSort::SortPairs(tmp.p, bytes, bp_keys.p, bp_keys2.p, bp_vals.p, bp_vals2.p, n, ns,
 		                                                   scene_body_off.p, scene_body_off.p + 1, 0, 64, stream));
 	}
 	launch_bp_gather(b, bp_vals2.p, bp_sb.p, stream);
@@ -1222,7 +1233,8 @@ void cp_world::step() {
 		color_off.reserve((size_t)nscenes * (CP_MAXC + 2));
 		ncolors.reserve(nscenes);
 		edge_range.reserve(2 * (size_t)nscenes);
-		launch_color(b, scene_body_off.p, nscenes, e_key.p, e_live.p, nedges, e_color.p, e_order.p, color_off.p, ncolors.p, edge_range.p,
+		launch_color(b, scene_body_off.p, serving ? live_scenes : nscenes, e_key.p, e_live.p, nedges, e_color.p, e_order.p, color_off.p,
+		             ncolors.p, edge_range.p,
 		             stream);
 		if (cfg.diagnostics & 2) record_colored_schedule();
 	}

@@ -92,6 +92,7 @@ def serve(world, source, requests, keep_states=False, log=None):
     lat, ticks, waits = {}, [], {}
     states = {}
     body_steps = 0
+    tick_log = []  # (start time, body-steps of the tick)
     t0 = time.perf_counter()
     while pending or queue or active:
         now = time.perf_counter() - t0
@@ -114,11 +115,14 @@ def serve(world, source, requests, keep_states=False, log=None):
         world.step(1)
         ticks.append(time.perf_counter() - t)
         done = []
+        tick_bs = 0
         for h, a in active.items():
-            body_steps += payloads[a[0].rid][5]
+            tick_bs += payloads[a[0].rid][5]
             a[1] -= 1
             if a[1] == 0:
                 done.append(h)
+        body_steps += tick_bs
+        tick_log.append((t - t0, tick_bs))
         for h in done:
             r = active.pop(h)[0]
             st = world.scene_state(h)
@@ -131,7 +135,11 @@ def serve(world, source, requests, keep_states=False, log=None):
     wall = time.perf_counter() - t0
     l = np.array([lat[r.rid] for r in requests])
     tk = np.array(ticks)
-    return dict(requests=len(requests), wall_s=wall, body_steps=body_steps, value=body_steps / wall,
+    # steady state: the ticks that start while requests are still arriving
+    last = max(r.arrival for r in requests)
+    win = [(ts, bs) for ts, bs in tick_log if ts <= last]
+    steady = sum(bs for _, bs in win) / max(1e-9, (win[-1][0] + ticks[len(win) - 1]) if win else 1.0)
+    return dict(requests=len(requests), wall_s=wall, body_steps=body_steps, value=body_steps / wall, steady_value=steady,
                 p50_request_ms=1e3 * float(np.percentile(l, 50)), p99_request_ms=1e3 * float(np.percentile(l, 99)),
                 p50_step_ms=1e3 * float(np.percentile(tk, 50)), p99_step_ms=1e3 * float(np.percentile(tk, 99)),
                 ticks=len(ticks), mean_wait_ms=1e3 * float(np.mean(list(waits.values()))), states=states)
