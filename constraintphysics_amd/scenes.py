@@ -349,3 +349,81 @@ def scene_description(sizes, seed, s, hull_lib_size=256, mix=MIX):
         mats.append(DEFAULT_MATERIAL)
         state.append(np.concatenate([pos[i], q[i], vel[i], [0, 0, 0]]))
     return dict(hulls=hulls, movement=np.array(move), mass_props=np.array(mps), material=np.array(mats), state=np.array(state))
+
+
+# ----------------------------------------------------------------------------- holonomic chains on a heightfield
+C2_MATERIAL = (1.0, 0.0, 0.5, 0.5)
+
+
+def heightfield(grid, cell, amp, seed, lattice=8):
+    """Points and triangles of a grid x grid heightfield centred on the origin,
+    split as generateGridMeshInput splits cells (Geometry.cpp:589-611), heights
+    amp * value noise (U[-1, 1] lattice every `lattice` cells, smoothstep
+    bilinear)."""
+    rng = np.random.default_rng(seed ^ 0x7E22)
+    L = grid // lattice + 2
+    lat = rng.uniform(-1.0, 1.0, (L, L))
+    g = np.arange(grid + 1) / lattice
+    ix = np.floor(g).astype(int)
+    f = g - ix
+    f = f * f * (3 - 2 * f)
+    a = lat[ix[:, None], ix[None, :]]
+    b = lat[ix[:, None], ix[None, :] + 1]
+    c = lat[ix[:, None] + 1, ix[None, :]]
+    d = lat[ix[:, None] + 1, ix[None, :] + 1]
+    h = (a * (1 - f[None, :]) + b * f[None, :]) * (1 - f[:, None]) + (c * (1 - f[None, :]) + d * f[None, :]) * f[:, None]
+    half = 0.5 * grid * cell
+    jj, ii = np.meshgrid(np.arange(grid + 1), np.arange(grid + 1), indexing="ij")
+    pts = np.stack([-half + cell * ii, amp * h, -half + cell * jj], axis=-1).reshape(-1, 3)
+    i, j = np.meshgrid(np.arange(1, grid + 1), np.arange(1, grid + 1), indexing="xy")
+    i, j = i.ravel(), j.ravel()
+    i1, i2 = i - 1 + (grid + 1) * (j - 1), i + (grid + 1) * (j - 1)
+    i3, i4 = i + (grid + 1) * j, i - 1 + (grid + 1) * j
+    tris = np.empty((2 * len(i), 3), np.int64)
+    tris[0::2] = np.stack([i3, i2, i1], 1)
+    tris[1::2] = np.stack([i4, i3, i1], 1)
+    return pts, tris
+
+
+def c2_description(n_chains=512, links=16, grid=256, cell=0.5, amp=2.0, seed=2024, hinge_at=8):
+    """Holonomic chains on a heightfield: n_chains chains of `links` cylinder
+    links (r 0.15, h 0.8, 0.1 apart, laid along +x) joined end to end by
+    ball-sockets, the hinge_at-th joint of each chain a hinge about +z (so a
+    16-link chain's system has degree 14 * 3 + 5 = 47), dropped onto a static
+    grid x grid heightfield of value noise with amplitude amp.  Body 0 is the
+    mesh; joints as (type, id1, id2) and geo14 rows."""
+    rng = np.random.default_rng(seed)
+    pts, tris = heightfield(grid, cell, amp, seed)
+    r, h, gap = 0.15, 0.8, 0.1
+    geo = cpa.cylinder_build(r, h)
+    m, t = cpa.cylinder_mass_properties(r, h, 1.0)
+    mp = mass_props_row(m, t)
+    span = grid * cell
+    length = links * (h + gap)
+    per_row = max(1, int((span - 2.0) // (length + 1.0)))
+    rows = (n_chains + per_row - 1) // per_row
+    dz = (span - 2.0) / max(rows, 1)
+    q = quat_axis_angle(-math.pi / 2.0, (0, 0, 1))
+    hulls = [dict(mesh=pts, tris=tris, tri_mat=np.tile(C2_MATERIAL[1:], (len(tris), 1)))]
+    move, mps, mats, state = [1], [np.zeros(19)], [np.array(C2_MATERIAL)], [np.array([0, 0, 0, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0], float)]
+    info, jgeo = [], []
+    a = h / 2.0 + gap / 2.0
+    for c in range(n_chains):
+        row, col = divmod(c, per_row)
+        x0 = -span / 2 + 1.0 + col * (length + 1.0)
+        z0 = -span / 2 + 1.0 + (row + 0.5) * dz
+        y0 = amp + 1.0 + rng.uniform(0.0, 0.5)
+        for i in range(links):
+            hulls.append(geo)
+            move.append(0)
+            mps.append(mp)
+            mats.append(np.array(C2_MATERIAL))
+            v = rng.uniform(-0.5, 0.5, 3)
+            state.append(np.concatenate([[x0 + i * (h + gap), y0, z0], q, v, [0, 0, 0]]))
+            if i:
+                idx = len(hulls) - 1  # body index of this link
+                hinge = i == hinge_at
+                info.append((5 if hinge else 3, idx, idx + 1))  # ids are index + 1: previous link id = idx, this one idx + 1
+                jgeo.append([0, a, 0, 0, -a, 0] + ([0, 0, 1, 0, 0, 1] if hinge else [0] * 6) + [15000.0, 15000.0 if hinge else 0.0])
+    return dict(hulls=hulls, movement=np.array(move), mass_props=np.array(mps), material=np.array(mats), state=np.array(state),
+                joints=(np.array(info, np.int64), np.array(jgeo, float)))

@@ -407,6 +407,9 @@ def world_from_description(desc, **cfg):
     import constraintphysics_amd as cpa
     w = cpa.World(**cfg)
     for b, sh in enumerate(desc["hulls"]):
+        if "mesh" in sh:
+            w.add_body(w.add_mesh(sh["mesh"], sh["tris"], sh["tri_mat"]), 1, np.zeros(19), desc["material"][b], 0)
+            continue
         if "sphere" in sh:
             sid = w.add_sphere(sh["sphere"][:3], sh["sphere"][3])
         elif "cylinder" in sh:
