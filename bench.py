@@ -158,6 +158,46 @@ def serving_leg(args):
     return res
 
 
+def c2_leg(warmup=150, steps=30):
+    """Holonomic chains on a heightfield (512 chains x 16 cylinder links, ball-
+    sockets and one hinge per chain, 256 x 256-cell value-noise terrain of
+    amplitude 2 m): reference-order PGS with the holonomic LDL block solve at
+    the first substep, FP64 throughout."""
+    import constraintphysics_amd as cpa
+    from constraintphysics_amd import scenes
+    t = time.perf_counter()
+    desc = scenes.c2_description()
+    w = cpa.World(dt=1.0 / 120.0, gravity=(0.0, -6.0, 0.0), vel_iters=10, pos_iters=1, substeps=8, pgs_mode=1, hol_iters=1)
+    scenes.build_description(w, desc)
+    n_dyn = int((desc["movement"] == 0).sum())
+    print(f"[bench] c2: {n_dyn} links, {len(desc['joints'][0])} joints, {len(desc['hulls'][0]['tris'])} triangles, "
+          f"built in {time.perf_counter() - t:.1f}s", file=sys.stderr, flush=True)
+    for i in range(warmup):
+        w.step(1)
+    lat, tim = [], []
+    for i in range(steps):
+        t0 = time.perf_counter()
+        w.step(1)
+        lat.append(time.perf_counter() - t0)
+        tim.append(w.last_timing())
+    lat, tim = np.array(lat), np.array(tim)
+    contacts = len(w.contacts()[0])
+    sy = w.holonomic_systems()
+    w.close()
+    phases = dict(zip(["broadphase_ms", "narrowphase_ms", "persist_schedule_ms", "substeps_ms", "geometry_ms", "pgs_ms"],
+                      [round(float(x), 3) for x in tim[:, :6].mean(0)]))
+    res = dict(value=round(n_dyn * steps / float(lat.sum()), 1), unit="body-steps/s", ms_per_step=round(1e3 * float(lat.mean()), 3),
+               p50_step_ms=round(1e3 * float(np.percentile(lat, 50)), 3), p99_step_ms=round(1e3 * float(np.percentile(lat, 99)), 3),
+               steps=steps, warmup=warmup, links=n_dyn, joints=int(len(desc["joints"][0])), holonomic_systems=len(sy),
+               triangles=int(len(desc["hulls"][0]["tris"])), live_contacts=contacts, phases_ms=phases, dtype="f64",
+               workload="512 chains x 16 cylinder links (r 0.15, h 0.8), ball-sockets + 1 hinge per chain (LDL block degree 47), "
+                        "256x256-cell heightfield (cell 0.5 m, value noise, amplitude 2 m); dt 1/120, 8 substeps, PGS 10 vel + 1 "
+                        "pos sweeps, 1 holonomic iteration; reference-order PGS (one lane per island)")
+    print(f"[bench] c2: {res['ms_per_step']:.2f} ms/step, {res['value']:.0f} body-steps/s, {contacts} contacts; phases {phases}",
+          file=sys.stderr, flush=True)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -170,6 +210,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-c2", action="store_true")
+    ap.add_argument("--c2-only", action="store_true", help="run only the holonomic-chains-on-terrain leg and print its JSON")
     ap.add_argument("--serve-requests", type=int, default=6000, help="requests of the serving leg (Poisson arrivals at 2000/s)")
     ap.add_argument("--serve-capacity", type=int, default=1600000, help="body slots of the serving world (its HBM budget)")
     ap.add_argument("--precision", choices=sorted(PRECISIONS), default="split",
@@ -179,6 +221,10 @@ def main():
         print(f"bench.py runs on one GPU only: --gpus {args.gpus} is not supported (scenes would shard by HBM footprint "
               f"across GPUs with no per-step exchange; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+
+    if args.c2_only:
+        print(json.dumps(c2_leg()), flush=True)
+        return
 
     from constraintphysics_amd import scenes
     sizes = scenes.c3_sizes(args.scenes, args.seed)
@@ -261,6 +307,7 @@ def main():
     served = None
     if not args.no_serve:
         served = serving_leg(args)
+    c2 = None if args.no_c2 else c2_leg()
 
     cpu = None
     if not args.no_cpu:
@@ -306,6 +353,7 @@ def main():
         "plain_gpu_path": plain,
         "speedup_vs_plain": round(value / plain["value"], 2) if plain else None,
         "serving": served,
+        "holonomic_chains_on_terrain": c2,
         "serving_speedup_vs_plain": round(served["value"] / plain["value"], 2) if (plain and served) else None,
         "roofline": {
             "bound": "hbm",

@@ -1,9 +1,9 @@
 """Summaries of rocprofv3 output for the batched step.
 
     python3 -m constraintphysics_amd.profiling DIR [--first-step 241] [--steps 20]
-        [--fetch-factor 2.0] [--write-factor 1.0] [--contacts N]
+        [--fetch-factor 2.0] [--write-factor 1.0] [--contacts N] [--prefix c3]
 
-DIR holds `c3_kernel_trace.csv` (rocprofv3 --kernel-trace) and, optionally,
+DIR holds `<prefix>_kernel_trace.csv` (rocprofv3 --kernel-trace -o <prefix>) and, optionally,
 `fetch_counter_collection.csv` / `write_counter_collection.csv` (rocprofv3
 --pmc FETCH_SIZE / WRITE_SIZE) of the same bench command.  Steps are counted
 by k_gravity dispatches (one per step); the window is steps
@@ -51,10 +51,10 @@ def _dur_ms(r):
     return (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-6
 
 
-def summarize(d, first_step=241, steps=20, fetch_factor=2.0, write_factor=1.0, contacts=None, top=14):
+def summarize(d, first_step=241, steps=20, fetch_factor=2.0, write_factor=1.0, contacts=None, top=14, prefix="c3"):
     lines = []
     win = range(first_step, first_step + steps)
-    tr = _rows(os.path.join(d, "c3_kernel_trace.csv"))
+    tr = _rows(os.path.join(d, f"{prefix}_kernel_trace.csv"))
     g = pgs_groups(tr, _dur_ms)
     vals = [v for s in win for v in g.get(s, [])]
     pgs_ms = sum(vals) / max(1, len(vals))
@@ -96,8 +96,9 @@ def main():
     ap.add_argument("--fetch-factor", type=float, default=2.0)
     ap.add_argument("--write-factor", type=float, default=1.0)
     ap.add_argument("--contacts", type=float, default=None)
+    ap.add_argument("--prefix", default="c3", help="rocprofv3 -o name of the kernel trace")
     a = ap.parse_args()
-    lines, _, _ = summarize(a.dir, a.first_step, a.steps, a.fetch_factor, a.write_factor, a.contacts)
+    lines, _, _ = summarize(a.dir, a.first_step, a.steps, a.fetch_factor, a.write_factor, a.contacts, prefix=a.prefix)
     print("\n".join(lines))
 
 

@@ -427,3 +427,25 @@ def c2_description(n_chains=512, links=16, grid=256, cell=0.5, amp=2.0, seed=202
                 jgeo.append([0, a, 0, 0, -a, 0] + ([0, 0, 1, 0, 0, 1] if hinge else [0] * 6) + [15000.0, 15000.0 if hinge else 0.0])
     return dict(hulls=hulls, movement=np.array(move), mass_props=np.array(mps), material=np.array(mats), state=np.array(state),
                 joints=(np.array(info, np.int64), np.array(jgeo, float)))
+
+
+def build_description(world, desc):
+    """Adds a description's bodies (static mesh, spheres, cylinders, exact
+    hulls) and joints to a world as one scene and sets the state."""
+    for b, sh in enumerate(desc["hulls"]):
+        if "mesh" in sh:
+            world.add_body(world.add_mesh(sh["mesh"], sh["tris"], sh["tri_mat"]), 1, np.zeros(19), desc["material"][b], 0)
+            continue
+        if "sphere" in sh:
+            sid = world.add_sphere(sh["sphere"][:3], sh["sphere"][3])
+        elif "cylinder" in sh:
+            sid = world.add_cylinder(sh)
+        else:
+            sid = world.add_hull_exact(sh)
+        world.add_body(sid, int(desc["movement"][b]), desc["mass_props"][b], desc["material"][b], 0)
+    if "joints" in desc:
+        info, geo = desc["joints"]
+        for (kind, id1, id2), g in zip(info, geo):
+            world.add_joint(int(kind), int(id1) - 1, int(id2) - 1, g)
+    world.set_state(desc["state"])
+    return world
