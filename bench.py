@@ -158,16 +158,20 @@ def serving_leg(args):
     return res
 
 
-def c2_leg(warmup=150, steps=30):
+C2_SOLVES = {1: "reference-order PGS (one lane per island)", 2: "coloured-island PGS (one wave per island)"}
+
+
+def c2_leg(mode=2, warmup=150, steps=30):
     """Holonomic chains on a heightfield (512 chains x 16 cylinder links, ball-
     sockets and one hinge per chain, 256 x 256-cell value-noise terrain of
-    amplitude 2 m): reference-order PGS with the holonomic LDL block solve at
-    the first substep, FP64 throughout."""
+    amplitude 2 m): island PGS (mode 1 reference order, mode 2 coloured
+    islands) with the holonomic LDL block solve at the first substep, FP64
+    throughout."""
     import constraintphysics_amd as cpa
     from constraintphysics_amd import scenes
     t = time.perf_counter()
     desc = scenes.c2_description()
-    w = cpa.World(dt=1.0 / 120.0, gravity=(0.0, -6.0, 0.0), vel_iters=10, pos_iters=1, substeps=8, pgs_mode=1, hol_iters=1)
+    w = cpa.World(dt=1.0 / 120.0, gravity=(0.0, -6.0, 0.0), vel_iters=10, pos_iters=1, substeps=8, pgs_mode=mode, hol_iters=1)
     scenes.build_description(w, desc)
     n_dyn = int((desc["movement"] == 0).sum())
     print(f"[bench] c2: {n_dyn} links, {len(desc['joints'][0])} joints, {len(desc['hulls'][0]['tris'])} triangles, "
@@ -192,8 +196,8 @@ def c2_leg(warmup=150, steps=30):
                triangles=int(len(desc["hulls"][0]["tris"])), live_contacts=contacts, phases_ms=phases, dtype="f64",
                workload="512 chains x 16 cylinder links (r 0.15, h 0.8), ball-sockets + 1 hinge per chain (LDL block degree 47), "
                         "256x256-cell heightfield (cell 0.5 m, value noise, amplitude 2 m); dt 1/120, 8 substeps, PGS 10 vel + 1 "
-                        "pos sweeps, 1 holonomic iteration; reference-order PGS (one lane per island)")
-    print(f"[bench] c2: {res['ms_per_step']:.2f} ms/step, {res['value']:.0f} body-steps/s, {contacts} contacts; phases {phases}",
+                        "pos sweeps, 1 holonomic iteration; " + C2_SOLVES[mode])
+    print(f"[bench] c2 mode {mode}: {res['ms_per_step']:.2f} ms/step, {res['value']:.0f} body-steps/s, {contacts} contacts; phases {phases}",
           file=sys.stderr, flush=True)
     return res
 
@@ -223,7 +227,7 @@ def main():
         sys.exit(2)
 
     if args.c2_only:
-        print(json.dumps(c2_leg()), flush=True)
+        print(json.dumps({"island_colored": c2_leg(2), "reference_order": c2_leg(1)}), flush=True)
         return
 
     from constraintphysics_amd import scenes

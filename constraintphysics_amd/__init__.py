@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(HERE, "libcp_hip.so")
 
 CP_PGS_COLORED = 0
 CP_PGS_REFERENCE_ORDER = 1
+CP_PGS_ISLAND_COLORED = 2
 
 
 class CpError(RuntimeError):

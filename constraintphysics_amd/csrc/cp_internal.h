@@ -101,6 +101,20 @@ struct HSysDev {
 	double* scratch;  // 3 * degree per system (at dbuf_off / 2): solve vectors
 };
 
+// Island schedule (reference-order and coloured-island solves).  rows holds
+// (index, kind) pairs, kind 0 contact normal, 1 friction, 2 joint; an
+// island's rows are [off[i], off[i+1]).  Coloured islands: an edge's rows are
+// [eoff[e], eoff[e+1]), a colour's edges [coff[c], coff[c+1]), an island's
+// colours [icoff[i], icoff[i+1]), the last one a serial bucket when serial[i].
+// jlin[2k], jlin[2k+1]: columns of joint k's bodies in the linearization-error
+// vectors (stride nl): the body index for a dynamic body, one column per
+// (island, body) beyond the bodies for the others.
+struct IslandSched {
+	int n, nl;
+	const int *rows, *off, *sys, *sys_off, *jlin;
+	const int *eoff, *coff, *icoff, *serial;
+};
+
 // Narrowphase output, one slot per candidate pair.
 struct Manifolds {
 	int* count;       // [cap] points in the culled manifold (0 = none)
@@ -260,9 +274,8 @@ void launch_rows_writeback(const Contacts& C, int nc, const Params& P, hipStream
 void launch_pgs_residual(const Bodies& b, const Contacts& C, int nc, const Params& P, double* acc, hipStream_t s);
 // rows: (index, kind) pairs, kind 0 contact normal, 1 friction, 2 joint;
 // isl_sys/isl_sys_off: holonomic systems per island; lin: [6][n] scratch.
-void launch_pgs_ordered(const Bodies& b, const Contacts& C, const Joints& J, const HSysDev& H, const int* rows, const int* isl_off,
-                        const int* isl_sys, const int* isl_sys_off, int nisl, const Params& P, int* applied, double* lin, int first,
-                        hipStream_t s);
+void launch_pgs_islands(const Bodies& b, const Contacts& C, const Joints& J, const HSysDev& H, const IslandSched& S, const Params& P,
+                        int* applied, double* lin, int first, bool colored, hipStream_t s);
 void launch_pgs_colored(const Bodies& b, const Contacts& C, const int* scene_body_off, int nscenes, const int* edge_range, const int* order,
                         const int* color_off, const int* ncolors, const int* efirst, const int* ecnt, const Params& P, int threads,
                         hipStream_t s, const int* scene_list = nullptr, int nlist = 0);

@@ -14,6 +14,8 @@
 // Sweep schedules over the same rows (ConstraintSolver.cpp:80-213):
 //  * k_pgs_ordered: the reference's sequential order (one thread per island,
 //    rows in the island-search order supplied by the host) — the parity mode.
+//  * k_pgs_islands: the same islands with their edges graph-coloured, one wave
+//    per island (joints and contacts; FP64).
 //  * k_pgs_colored_lds / k_pgs_colored: one workgroup per scene; edges (body
 //    pairs) are coloured so that no two edges of a colour share a dynamic
 //    body, each thread runs one edge's contacts in cache order, and colours
@@ -340,163 +342,235 @@ __device__ __forceinline__ void apply_dv(Bodies& b, const T* dv, int i) {
 	}
 }
 
-// Sequential island solve (FP64 only): rows[2*r] = index, rows[2*r+1] = kind
-// (0 contact normal, 1 friction, 2 joint).  At the first substep the island's
-// holonomic systems are solved after the sweeps (ConstraintSolver.cpp:163-196).
-__global__ void k_pgs_ordered(Bodies b, Contacts C, Joints J, HSysDev H, const int* rows, const int* isl_off, const int* isl_sys,
-                              const int* isl_sys_off, int nisl, Params P, int* applied, double* lin, int first) {
-	int t = blockIdx.x * blockDim.x + threadIdx.x;
-	if (t >= nisl) return;
-	const int N = b.n, K = J.n;
-	int r0 = isl_off[t], r1 = isl_off[t + 1];
-	using DG = DvGlobal<double>;
-	auto refs = [&](int k, DG& A, DG& B) {
-		int i1 = (int)(C.key[k] >> 32), i2 = (int)(C.key[k] & 0xFFFFFFFFu);
-		A = DG{b.dv, i1, N, b.movement[i1] == 0};
-		B = DG{b.dv, i2, N, b.movement[i2] == 0};
-	};
-	// joint targets at solve time (findTargetConstraintValue, ConstraintSolver.h:78)
-	for (int r = r0; r < r1; r++) {
-		if (rows[2 * r + 1] != 2) continue;
-		int k = rows[2 * r];
-		double* R = J.row + (size_t)k * CP_JRS;
-		int ia = J.a[k], ib = J.b[k];
-		double va[6], vb[6], val[5];
-		bool fa = b.movement[ia] == 1, fb = b.movement[ib] == 1;  // getVel / getAngVel (RigidBody.cpp:174-181)
-		for (int c = 0; c < 3; c++) {
-			va[c] = fa ? 0.0 : b.vel[c * N + ia];
-			va[3 + c] = fa ? 0.0 : b.angv[c * N + ia];
-			vb[c] = fb ? 0.0 : b.vel[c * N + ib];
-			vb[3 + c] = fb ? 0.0 : b.angv[c * N + ib];
-		}
-		joint_value(R, J.type[k], va, vb, 1, val);
-		for (int c = 0; c < J.type[k]; c++) R[CP_J_TGT + c] = -val[c];
+// Island solves (FP64 only) over a host-built island schedule: rows[2*r] =
+// index, rows[2*r+1] = kind (0 contact normal, 1 friction, 2 joint).  At the
+// first substep each island's holonomic systems are solved after the sweeps
+// (ConstraintSolver.cpp:163-196).  The reference keeps one set of velocity
+// changes per island solve (ConstraintSolver.cpp:83-114); dynamic bodies
+// belong to one island, so their deltas live in b.dv, and the
+// linearization-error vectors of non-dynamic joint bodies get one column per
+// (island, body) (S.jlin).
+using DG = DvGlobal<double>;
+
+__device__ __forceinline__ void contact_refs(const Bodies& b, const Contacts& C, int k, DG& A, DG& B) {
+	int i1 = (int)(C.key[k] >> 32), i2 = (int)(C.key[k] & 0xFFFFFFFFu);
+	A = DG{b.dv, i1, b.n, b.movement[i1] == 0};
+	B = DG{b.dv, i2, b.n, b.movement[i2] == 0};
+}
+
+// Joint target at solve time (findTargetConstraintValue, ConstraintSolver.h:78)
+// from getVel / getAngVel (RigidBody.cpp:174-181).
+__device__ __forceinline__ void joint_target(const Bodies& b, const Joints& J, int k) {
+	const int N = b.n;
+	double* R = J.row + (size_t)k * CP_JRS;
+	int ia = J.a[k], ib = J.b[k];
+	double va[6], vb[6], val[5];
+	bool fa = b.movement[ia] == 1, fb = b.movement[ib] == 1;
+	for (int c = 0; c < 3; c++) {
+		va[c] = fa ? 0.0 : b.vel[c * N + ia];
+		va[3 + c] = fa ? 0.0 : b.angv[c * N + ia];
+		vb[c] = fb ? 0.0 : b.vel[c * N + ib];
+		vb[3 + c] = fb ? 0.0 : b.angv[c * N + ib];
 	}
-	auto jstep = [&](int k, bool pseudo) {  // PGSConstraintStep<n>, equality constraint (ConstraintSolver.cpp:12-40)
-		double* R = J.row + (size_t)k * CP_JRS;
-		int n = J.type[k], ia = J.a[k], ib = J.b[k];
-		double* va = b.dv + (pseudo ? 6 * N : 0) + ia;
-		double* vb = b.dv + (pseudo ? 6 * N : 0) + ib;
-		double* acc = pseudo ? J.pimp : J.imp;
-		const double* tgt = R + (pseudo ? CP_J_PTGT : CP_J_TGT);
-		double cur[5], delta[5], m[5], diff[5], a[5];
-		joint_value(R, n, va, vb, N, cur);
-		for (int c = 0; c < n; c++) {
-			delta[c] = tgt[c] - cur[c];
-			a[c] = acc[c * K + k];
-		}
-		for (int i = 0; i < n; i++) {
-			m[i] = 0;
-			for (int c = 0; c < n; c++) m[i] += delta[c] * R[CP_J_INV + i * n + c];
-		}
-		for (int c = 0; c < n; c++) diff[c] = (a[c] + m[c]) - a[c];
-		if (!is_zero_n(diff, n)) {
-			joint_apply(R, n, diff, va, vb, N, b.movement[ia] == 0, b.movement[ib] == 0);
-			for (int c = 0; c < n; c++) acc[c * K + k] = a[c] + diff[c];
-		}
-	};
-	for (int r = r0; r < r1; r++) {
-		int k = rows[2 * r], kind = rows[2 * r + 1];
-		if (kind == 2) {
+	joint_value(R, J.type[k], va, vb, 1, val);
+	for (int c = 0; c < J.type[k]; c++) R[CP_J_TGT + c] = -val[c];
+}
+
+// PGSConstraintStep<n> of an equality constraint (ConstraintSolver.cpp:12-40).
+__device__ __forceinline__ void joint_step(const Bodies& b, const Joints& J, int k, bool pseudo) {
+	const int N = b.n, K = J.n;
+	double* R = J.row + (size_t)k * CP_JRS;
+	int n = J.type[k], ia = J.a[k], ib = J.b[k];
+	double* va = b.dv + (pseudo ? 6 * N : 0) + ia;
+	double* vb = b.dv + (pseudo ? 6 * N : 0) + ib;
+	double* acc = pseudo ? J.pimp : J.imp;
+	const double* tgt = R + (pseudo ? CP_J_PTGT : CP_J_TGT);
+	double cur[5], delta[5], m[5], diff[5], a[5];
+	joint_value(R, n, va, vb, N, cur);
+	for (int c = 0; c < n; c++) {
+		delta[c] = tgt[c] - cur[c];
+		a[c] = acc[c * K + k];
+	}
+	for (int i = 0; i < n; i++) {
+		m[i] = 0;
+		for (int c = 0; c < n; c++) m[i] += delta[c] * R[CP_J_INV + i * n + c];
+	}
+	for (int c = 0; c < n; c++) diff[c] = (a[c] + m[c]) - a[c];
+	if (!is_zero_n(diff, n)) {
+		joint_apply(R, n, diff, va, vb, N, b.movement[ia] == 0, b.movement[ib] == 0);
+		for (int c = 0; c < n; c++) acc[c * K + k] = a[c] + diff[c];
+	}
+}
+
+// One row of a phase: 0 warm start, 1 velocity sweep, 2 position sweep.
+__device__ __forceinline__ void island_row(const Bodies& b, const Contacts& C, const Joints& J, int k, int kind, int phase) {
+	if (kind == 2) {
+		if (phase == 0) {
+			const int K = J.n;
 			double imp[5];
 			int n = J.type[k];
 			for (int c = 0; c < n; c++) imp[c] = J.imp[c * K + k];
 			if (!is_zero_n(imp, n))
-				joint_apply(J.row + (size_t)k * CP_JRS, n, imp, b.dv + J.a[k], b.dv + J.b[k], N, b.movement[J.a[k]] == 0,
+				joint_apply(J.row + (size_t)k * CP_JRS, n, imp, b.dv + J.a[k], b.dv + J.b[k], b.n, b.movement[J.a[k]] == 0,
 				            b.movement[J.b[k]] == 0);
-			continue;
-		}
-		DG A, B;
-		refs(k, A, B);
-		warm_start(rec<double>(C, k), A, B, kind);
+		} else joint_step(b, J, k, phase == 2);
+		return;
 	}
+	if (phase == 2 && kind != 0) return;
+	DG A, B;
+	contact_refs(b, C, k, A, B);
+	double* R = rec<double>(C, k);
+	if (phase == 0) warm_start(R, A, B, kind);
+	else if (phase == 1) {
+		if (kind == 0) normal_step(R, A, B, false);
+		else friction_step(R, A, B);
+	} else normal_step(R, A, B, true);
+}
+
+// computeAndApplyImpulses (HolonomicBlockSolver.cpp:298-355) for the island's
+// systems in order, hol_iters times.
+__device__ void island_holonomic(const Bodies& b, const Joints& J, const HSysDev& H, const IslandSched& S, int t, const Params& P,
+                                 double* lin) {
+	const int N = b.n, K = J.n, NL = S.nl;
+	auto apply = [&](int s, bool pseudo, double thr) {
+		const int nc = H.nc[s];
+		const int* ord = H.order + H.ord_off[s];
+		const int* vl = H.vloc + H.ord_off[s];
+		int deg = H.degree[s];
+		double* d = H.scratch + H.dbuf_off[s] / 2;  // 3 * degree doubles per system
+		double* eff = d + deg;
+		double* tmp = eff + deg;
+		const int off = pseudo ? 6 * N : 0;
+		for (int i = 0; i < nc; i++) {
+			int k = ord[i];
+			const double* R = J.row + (size_t)k * CP_JRS;
+			double cur[5];
+			joint_value(R, J.type[k], b.dv + off + J.a[k], b.dv + off + J.b[k], N, cur);
+			const double* tg = R + (pseudo ? CP_J_PTGT : CP_J_TGT);
+			for (int c = 0; c < J.type[k]; c++) d[vl[i] + c] = tg[c] - cur[c];
+		}
+		hs_solve(J, H, s, d, eff, tmp);
+		if (thr > 0) {
+			for (int i = 0; i < nc; i++) {
+				int k = ord[i];
+				if (is_zero_n(d + vl[i], J.type[k])) continue;
+				joint_apply(J.row + (size_t)k * CP_JRS, J.type[k], d + vl[i], lin + S.jlin[2 * k], lin + S.jlin[2 * k + 1], NL, true, true);
+			}
+			for (int i = 0; i < nc; i++) {
+				int k = ord[i];
+				double r2 = 0;
+				for (int c = 3; c < 6; c++) {
+					double la = lin[c * NL + S.jlin[2 * k]], lb = lin[c * NL + S.jlin[2 * k + 1]];
+					r2 += la * la + lb * lb;
+				}
+				if (r2 > thr * thr) return;  // discard the solution
+			}
+		}
+		for (int i = 0; i < nc; i++) {
+			int k = ord[i];
+			int n = J.type[k];
+			if (is_zero_n(d + vl[i], n)) continue;
+			joint_apply(J.row + (size_t)k * CP_JRS, n, d + vl[i], b.dv + off + J.a[k], b.dv + off + J.b[k], N, b.movement[J.a[k]] == 0,
+			            b.movement[J.b[k]] == 0);
+			double* acc = pseudo ? J.pimp : J.himp;
+			for (int c = 0; c < n; c++) acc[c * K + k] += d[vl[i] + c];
+		}
+	};
+	for (int it = 0; it < P.hol_iters; it++)
+		for (int q = S.sys_off[t]; q < S.sys_off[t + 1]; q++) {
+			apply(S.sys[q], false, -1.0);
+			apply(S.sys[q], true, (double)15.0f);  // excessive_linear_error_torque_threshold (PhysicsEngine.h:217)
+		}
+}
+
+// addScaledHolonomicImpulseToNextWarmStart with factor 0 (PhysicsEngine.cpp:373).
+__device__ __forceinline__ void holonomic_warm_scale(const Joints& J, int k) {
+	if (J.sys[k] < 0) return;
+	for (int c = 0; c < J.type[k]; c++) J.imp[c * J.n + k] += (double)0.0f * J.himp[c * J.n + k];
+}
+
+__device__ __forceinline__ void row_bodies(const Contacts& C, const Joints& J, int k, int kind, int* ii) {
+	if (kind == 2) {
+		ii[0] = J.a[k];
+		ii[1] = J.b[k];
+	} else {
+		ii[0] = (int)(C.key[k] >> 32);
+		ii[1] = (int)(C.key[k] & 0xFFFFFFFFu);
+	}
+}
+
+// Reference order: one thread per island walks its rows sequentially.
+__global__ void k_pgs_ordered(Bodies b, Contacts C, Joints J, HSysDev H, IslandSched S, Params P, int* applied, double* lin, int first) {
+	int t = blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= S.n) return;
+	const int* rows = S.rows;
+	int r0 = S.off[t], r1 = S.off[t + 1];
+	for (int r = r0; r < r1; r++)
+		if (rows[2 * r + 1] == 2) joint_target(b, J, rows[2 * r]);
+	for (int r = r0; r < r1; r++) island_row(b, C, J, rows[2 * r], rows[2 * r + 1], 0);
 	for (int it = 0; it < P.vel_iters; it++)
-		for (int r = r0; r < r1; r++) {
-			int k = rows[2 * r], kind = rows[2 * r + 1];
-			if (kind == 2) { jstep(k, false); continue; }
-			DG A, B;
-			refs(k, A, B);
-			double* R = rec<double>(C, k);
-			if (kind == 0) normal_step(R, A, B, false);
-			else friction_step(R, A, B);
-		}
+		for (int r = r0; r < r1; r++) island_row(b, C, J, rows[2 * r], rows[2 * r + 1], 1);
 	for (int it = 0; it < P.pos_iters; it++)
-		for (int r = r0; r < r1; r++) {
-			int k = rows[2 * r], kind = rows[2 * r + 1];
-			if (kind == 2) { jstep(k, true); continue; }
-			if (kind != 0) continue;
-			DG A, B;
-			refs(k, A, B);
-			normal_step(rec<double>(C, k), A, B, true);
-		}
-	if (first && P.hol_iters > 0 && isl_sys_off[t + 1] > isl_sys_off[t]) {
-		// computeAndApplyImpulses (HolonomicBlockSolver.cpp:298-355)
-		auto apply = [&](int s, bool pseudo, double thr) {
-			const int nc = H.nc[s];
-			const int* ord = H.order + H.ord_off[s];
-			const int* vl = H.vloc + H.ord_off[s];
-			int deg = H.degree[s];
-			double* d = H.scratch + H.dbuf_off[s] / 2;  // 3 * degree doubles per system
-			double* eff = d + deg;
-			double* tmp = eff + deg;
-			const int off = pseudo ? 6 * N : 0;
-			for (int i = 0; i < nc; i++) {
-				int k = ord[i];
-				const double* R = J.row + (size_t)k * CP_JRS;
-				double cur[5];
-				joint_value(R, J.type[k], b.dv + off + J.a[k], b.dv + off + J.b[k], N, cur);
-				const double* tg = R + (pseudo ? CP_J_PTGT : CP_J_TGT);
-				for (int c = 0; c < J.type[k]; c++) d[vl[i] + c] = tg[c] - cur[c];
-			}
-			hs_solve(J, H, s, d, eff, tmp);
-			if (thr > 0) {
-				for (int i = 0; i < nc; i++) {
-					int k = ord[i];
-					if (is_zero_n(d + vl[i], J.type[k])) continue;
-					joint_apply(J.row + (size_t)k * CP_JRS, J.type[k], d + vl[i], lin + J.a[k], lin + J.b[k], N, true, true);
-				}
-				for (int i = 0; i < nc; i++) {
-					int k = ord[i];
-					double r2 = 0;
-					for (int c = 3; c < 6; c++) {
-						double la = lin[c * N + J.a[k]], lb = lin[c * N + J.b[k]];
-						r2 += la * la + lb * lb;
-					}
-					if (r2 > thr * thr) return;  // discard the solution
-				}
-			}
-			for (int i = 0; i < nc; i++) {
-				int k = ord[i];
-				int n = J.type[k];
-				if (is_zero_n(d + vl[i], n)) continue;
-				joint_apply(J.row + (size_t)k * CP_JRS, n, d + vl[i], b.dv + off + J.a[k], b.dv + off + J.b[k], N,
-				            b.movement[J.a[k]] == 0, b.movement[J.b[k]] == 0);
-				double* acc = pseudo ? J.pimp : J.himp;
-				for (int c = 0; c < n; c++) acc[c * K + k] += d[vl[i] + c];
-			}
-		};
-		for (int it = 0; it < P.hol_iters; it++)
-			for (int q = isl_sys_off[t]; q < isl_sys_off[t + 1]; q++) {
-				apply(isl_sys[q], false, -1.0);
-				apply(isl_sys[q], true, (double)15.0f);  // excessive_linear_error_torque_threshold (PhysicsEngine.h:217)
-			}
-		// addScaledHolonomicImpulseToNextWarmStart with factor 0 (PhysicsEngine.cpp:373)
-		for (int r = r0; r < r1; r++) {
-			if (rows[2 * r + 1] != 2) continue;
-			int k = rows[2 * r];
-			if (J.sys[k] < 0) continue;
-			for (int c = 0; c < J.type[k]; c++) J.imp[c * K + k] += (double)0.0f * J.himp[c * K + k];
-		}
+		for (int r = r0; r < r1; r++) island_row(b, C, J, rows[2 * r], rows[2 * r + 1], 2);
+	if (first && P.hol_iters > 0 && S.sys_off[t + 1] > S.sys_off[t]) {
+		island_holonomic(b, J, H, S, t, P, lin);
+		for (int r = r0; r < r1; r++)
+			if (rows[2 * r + 1] == 2) holonomic_warm_scale(J, rows[2 * r]);
 	}
 	for (int r = r0; r < r1; r++) {
-		int k = rows[2 * r], kind = rows[2 * r + 1];
 		int ii[2];
-		if (kind == 2) { ii[0] = J.a[k]; ii[1] = J.b[k]; }
-		else { ii[0] = (int)(C.key[k] >> 32); ii[1] = (int)(C.key[k] & 0xFFFFFFFFu); }
+		row_bodies(C, J, rows[2 * r], rows[2 * r + 1], ii);
 		for (int q = 0; q < 2; q++) {
 			int i = ii[q];
 			if (b.movement[i] != 0 || applied[i]) continue;
 			applied[i] = 1;
+			apply_dv(b, b.dv, i);
+		}
+	}
+}
+
+// Coloured islands: one wave per island.  The island's edges (body pairs with
+// rows: joints, then live contacts) are coloured on the host so that no two
+// edges of a colour share a dynamic body; a lane runs one edge's rows in
+// order and colours are separated by a workgroup barrier.  Per body the
+// deltas accumulate in the order of the island's row list (colour-major), so
+// the solve equals the sequential sweep over that list.  The island's last
+// colour may be a serial bucket (edges that found no colour), run by lane 0.
+__global__ __launch_bounds__(64) void k_pgs_islands(Bodies b, Contacts C, Joints J, HSysDev H, IslandSched S, Params P, int* applied,
+                                                     double* lin, int first) {
+	const int t = blockIdx.x, lane = threadIdx.x;
+	const int* rows = S.rows;
+	const int r0 = S.off[t], r1 = S.off[t + 1];
+	const int c0 = S.icoff[t], c1 = S.icoff[t + 1];
+	const bool ser = S.serial[t] != 0;
+	for (int r = r0 + lane; r < r1; r += 64)
+		if (rows[2 * r + 1] == 2) joint_target(b, J, rows[2 * r]);
+	__syncthreads();
+	auto sweep = [&](int phase) {
+		for (int c = c0; c < c1; c++) {
+			int e0 = S.coff[c], e1 = S.coff[c + 1];
+			bool serial = ser && c == c1 - 1;
+			int e = serial ? (lane == 0 ? e0 : e1) : e0 + lane;
+			for (; e < e1; e += serial ? 1 : 64)
+				for (int r = S.eoff[e]; r < S.eoff[e + 1]; r++) island_row(b, C, J, rows[2 * r], rows[2 * r + 1], phase);
+			__syncthreads();
+		}
+	};
+	sweep(0);
+	for (int it = 0; it < P.vel_iters; it++) sweep(1);
+	for (int it = 0; it < P.pos_iters; it++) sweep(2);
+	if (first && P.hol_iters > 0 && S.sys_off[t + 1] > S.sys_off[t]) {
+		if (lane == 0) island_holonomic(b, J, H, S, t, P, lin);
+		__syncthreads();
+		for (int r = r0 + lane; r < r1; r += 64)
+			if (rows[2 * r + 1] == 2) holonomic_warm_scale(J, rows[2 * r]);
+	}
+	for (int r = r0 + lane; r < r1; r += 64) {
+		int ii[2];
+		row_bodies(C, J, rows[2 * r], rows[2 * r + 1], ii);
+		for (int q = 0; q < 2; q++) {
+			int i = ii[q];
+			if (b.movement[i] != 0 || atomicExch(applied + i, 1) != 0) continue;
 			apply_dv(b, b.dv, i);
 		}
 	}
@@ -677,12 +751,11 @@ void launch_build_rows(const Bodies& b, const Contacts& C, int nc, const Params&
 	if (P.precision == CP_PREC_SPLIT) hipLaunchKernelGGL(k_build_rows<float>, dim3(nblk(nc, 64)), dim3(64), 0, s, b, C, nc, P);
 	else hipLaunchKernelGGL(k_build_rows<double>, dim3(nblk(nc, 64)), dim3(64), 0, s, b, C, nc, P);
 }
-void launch_pgs_ordered(const Bodies& b, const Contacts& C, const Joints& J, const HSysDev& H, const int* rows, const int* isl_off,
-                        const int* isl_sys, const int* isl_sys_off, int nisl, const Params& P, int* applied, double* lin, int first,
-                        hipStream_t s) {
-	if (nisl)
-		hipLaunchKernelGGL(k_pgs_ordered, dim3(nblk(nisl, 64)), dim3(64), 0, s, b, C, J, H, rows, isl_off, isl_sys, isl_sys_off, nisl, P,
-		                   applied, lin, first);
+void launch_pgs_islands(const Bodies& b, const Contacts& C, const Joints& J, const HSysDev& H, const IslandSched& S, const Params& P,
+                        int* applied, double* lin, int first, bool colored, hipStream_t s) {
+	if (!S.n) return;
+	if (colored) hipLaunchKernelGGL(k_pgs_islands, dim3(S.n), dim3(64), 0, s, b, C, J, H, S, P, applied, lin, first);
+	else hipLaunchKernelGGL(k_pgs_ordered, dim3(nblk(S.n, 64)), dim3(64), 0, s, b, C, J, H, S, P, applied, lin, first);
 }
 void launch_pgs_colored(const Bodies& b, const Contacts& C, const int* scene_body_off, int nscenes, const int* edge_range, const int* order,
                         const int* color_off, const int* ncolors, const int* efirst, const int* ecnt, const Params& P, int threads,

@@ -14,6 +14,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <array>
 #include <climits>
 #include <cmath>
 #include <cstdio>
@@ -224,6 +225,9 @@ struct cp_world {
 	DArr<double> dh_buf, dh_dbuf, dh_scratch;
 	bool hs_pending = true;
 	std::vector<int> last_isl_sys, last_isl_sys_off;
+	DArr<int> isl_jlin, isl_eoff, isl_coff, isl_icoff, isl_serial;
+	int lin_nl = 0;
+	std::vector<std::array<uint64_t, 2>> col_used;
 	// static meshes: one world instance per mesh body (triangles and AABB tree
 	// placed on the host from the body's pose), device copies, step scratch
 	std::vector<int> mesh_inst_of_body, mesh_body;
@@ -309,7 +313,7 @@ struct cp_world {
 		p.contact_life = cfg.contact_life;
 		p.gyro_ticks = cfg.gyro_ticks;
 		p.defect_fixes = cfg.defect_fixes;
-		p.precision = cfg.pgs_mode == CP_PGS_REFERENCE_ORDER ? CP_PREC_FP64 : cfg.precision;
+		p.precision = cfg.pgs_mode != CP_PGS_COLORED ? CP_PREC_FP64 : cfg.precision;
 		p.hol_iters = cfg.hol_iters;
 		return p;
 	}
@@ -361,7 +365,7 @@ struct cp_world {
 	int narrowphase(int npairs);
 	void persist(int npairs);
 	void edges();
-	void schedule_reference_order();
+	void schedule_islands(bool colored);
 	void record_colored_schedule();
 	void launch_pgs_colored_lds_global(const Bodies& b, const Params& P);
 	void step();
@@ -382,7 +386,8 @@ void cp_world::free_all() {
 	DArr<uint64_t>* u64s[] = {&bp_keys, &bp_keys2, &pairs, &pairs2, &nk, &e_key};
 	for (auto* a : u64s) a->free_();
 	DArr<int>* jints[] = {&dj_type, &dj_a, &dj_b, &dj_sys, &dh_nc, &dh_ord_off, &dh_order, &dh_vloc, &dh_tab_off, &dh_table, &dh_degree,
-	                      &dh_buf_off, &dh_dbuf_off, &isl_sys, &isl_sys_off};
+	                      &dh_buf_off, &dh_dbuf_off, &isl_sys, &isl_sys_off, &isl_jlin, &isl_eoff, &isl_coff, &isl_icoff,
+	                      &isl_serial};
 	for (auto* a : jints) a->free_();
 	DArr<double>* jdbl[] = {&dj_tp1, &dj_tp2, &dj_ax1, &dj_ax2, &dj_ps, &dj_rs, &dj_imp, &dj_pimp, &dj_himp, &dj_u, &dj_w, &dj_row, &lin,
 	                        &dh_buf, &dh_dbuf, &dh_scratch};
@@ -1070,7 +1075,7 @@ void cp_world::edges() {
 // edge's rows are its joints in creation order, then its live contacts.  Each
 // island also lists the holonomic systems its joints belong to, in the order
 // they are first met.
-void cp_world::schedule_reference_order() {
+void cp_world::schedule_islands(bool colored) {
 	int n = ncontacts;
 	std::vector<uint64_t> key;
 	std::vector<int> live;
@@ -1078,6 +1083,7 @@ void cp_world::schedule_reference_order() {
 	down(live, cb[cur].c.live, n);
 	struct Edge {
 		int first = -1;
+		int lo = 0, hi = 0;
 		std::vector<int> joints;
 	};
 	std::vector<Edge> edges;
@@ -1087,6 +1093,8 @@ void cp_world::schedule_reference_order() {
 		if (it != eid.end()) return it->second;
 		eid.emplace(k, (int)edges.size());
 		edges.emplace_back();
+		edges.back().lo = (int)(k >> 32);
+		edges.back().hi = (int)(k & 0xFFFFFFFFu);
 		return (int)edges.size() - 1;
 	};
 	for (int k = 0; k < n; k++) {
@@ -1108,54 +1116,123 @@ void cp_world::schedule_reference_order() {
 	if (cfg.hol_iters > 0)
 		for (size_t q = 0; q < hs_og.size(); q++)
 			for (int j : hs_og[q]) sys_of[j] = (int)q;
+	// an edge's rows: its joints, then its live contacts (normal, friction)
+	auto emit = [&](int e) {
+		const Edge& E = edges[e];
+		for (int j : E.joints) { last_rows.push_back(j); last_rows.push_back(2); }
+		if (E.first < 0) return;
+		for (int k = E.first; k < n && key[k] == key[E.first]; k++) {
+			if (!live[k]) continue;
+			last_rows.push_back(k); last_rows.push_back(0);
+			last_rows.push_back(k); last_rows.push_back(1);
+		}
+	};
+	auto has_rows = [&](int e) {
+		const Edge& E = edges[e];
+		if (!E.joints.empty()) return true;
+		if (E.first < 0) return false;
+		for (int k = E.first; k < n && key[k] == key[E.first]; k++)
+			if (live[k]) return true;
+		return false;
+	};
 	std::vector<char> visited(nb, 0);
 	std::vector<int> roots = root_order();
+	std::vector<int> eoff{0}, coff{0}, icoff{0}, serial, jlin(2 * j_type.size(), 0), isl_edges, col;
+	std::vector<std::pair<int, int>> slots;  // (non-dynamic body, lin column) of the current island
+	int nl = nb;
+	if (colored) col_used.assign(nb, {0, 0});
 	last_rows.clear();
 	last_offs.assign(1, 0);
 	last_isl_sys.clear();
 	last_isl_sys_off.assign(1, 0);
 	for (int root : roots) {
 		if (b_move[root] != 0 || visited[root]) continue;
-		size_t start = last_rows.size(), sys_start = last_isl_sys.size();
+		isl_edges.clear();
 		std::vector<int> st{root};
 		while (!st.empty()) {
 			int c = st.back();
 			st.pop_back();
 			if (visited[c]) continue;
 			visited[c] = 1;
-			if (b_move[c] == 0) {
-				for (auto& e : adj[c]) {
-					if (visited[e.first]) continue;
-					const Edge& E = edges[e.second];
-					for (int j : E.joints) {
-						int q = sys_of[j];
-						if (q >= 0 && std::find(last_isl_sys.begin() + sys_start, last_isl_sys.end(), q) == last_isl_sys.end())
-							last_isl_sys.push_back(q);
-						last_rows.push_back(j); last_rows.push_back(2);
-					}
-					if (E.first < 0) continue;
-					for (int k = E.first; k < n && key[k] == key[E.first]; k++) {
-						if (!live[k]) continue;
-						last_rows.push_back(k); last_rows.push_back(0);
-						last_rows.push_back(k); last_rows.push_back(1);
-					}
-				}
-			}
+			if (b_move[c] == 0)
+				for (auto& e : adj[c])
+					if (!visited[e.first] && has_rows(e.second)) isl_edges.push_back(e.second);
 			for (auto& e : adj[c])
 				if (b_move[e.first] == 0 && !visited[e.first]) st.push_back(e.first);
 		}
-		if (last_rows.size() > start) {
-			last_offs.push_back((int64_t)last_rows.size() / 2);
-			last_isl_sys_off.push_back((int)last_isl_sys.size());
+		if (isl_edges.empty()) continue;
+		size_t start = last_rows.size() / 2;
+		if (!colored) {
+			for (int e : isl_edges) emit(e);
 		} else {
-			last_isl_sys.resize(sys_start);
+			// greedy colouring in visit order: lowest colour free at both
+			// dynamic ends, CP_MAXC = the serial bucket
+			col.assign(isl_edges.size(), CP_MAXC);
+			int ncol = 0;
+			bool any_serial = false;
+			for (size_t x = 0; x < isl_edges.size(); x++) {
+				const Edge& E = edges[isl_edges[x]];
+				uint64_t m0 = 0, m1 = 0;
+				for (int i : {E.lo, E.hi})
+					if (b_move[i] == 0) { m0 |= col_used[i][0]; m1 |= col_used[i][1]; }
+				int c = ~m0 ? __builtin_ctzll(~m0) : (~m1 ? 64 + __builtin_ctzll(~m1) : CP_MAXC);
+				if (c == CP_MAXC) { any_serial = true; continue; }
+				col[x] = c;
+				ncol = std::max(ncol, c + 1);
+				for (int i : {E.lo, E.hi})
+					if (b_move[i] == 0) col_used[i][c >> 6] |= 1ull << (c & 63);
+			}
+			for (int e : isl_edges)
+				for (int i : {edges[e].lo, edges[e].hi}) col_used[i] = {0, 0};
+			for (int c = 0; c < ncol + (any_serial ? 1 : 0); c++) {
+				int cc = c < ncol ? c : CP_MAXC;
+				for (size_t x = 0; x < isl_edges.size(); x++) {
+					if (col[x] != cc) continue;
+					emit(isl_edges[x]);
+					eoff.push_back((int)(last_rows.size() / 2));
+				}
+				coff.push_back((int)eoff.size() - 1);
+			}
+			icoff.push_back((int)coff.size() - 1);
+			serial.push_back(any_serial ? 1 : 0);
 		}
+		// systems in order of first appearance; linearization-error columns
+		size_t sys_start = last_isl_sys.size();
+		slots.clear();
+		for (size_t r = start; r < last_rows.size() / 2; r++) {
+			if (last_rows[2 * r + 1] != 2) continue;
+			int j = (int)last_rows[2 * r], q = sys_of[j];
+			if (q >= 0 && std::find(last_isl_sys.begin() + sys_start, last_isl_sys.end(), q) == last_isl_sys.end())
+				last_isl_sys.push_back(q);
+			int ends[2] = {j_a[j], j_b[j]};
+			for (int h = 0; h < 2; h++) {
+				int i = ends[h], cidx = i;
+				if (b_move[i] != 0) {
+					auto it = std::find_if(slots.begin(), slots.end(), [&](const std::pair<int, int>& p) { return p.first == i; });
+					if (it == slots.end()) {
+						slots.push_back({i, nl++});
+						cidx = nl - 1;
+					} else cidx = it->second;
+				}
+				jlin[2 * j + h] = cidx;
+			}
+		}
+		last_offs.push_back((int64_t)last_rows.size() / 2);
+		last_isl_sys_off.push_back((int)last_isl_sys.size());
 	}
+	lin_nl = nl;
 	std::vector<int> rows(last_rows.begin(), last_rows.end()), offs(last_offs.begin(), last_offs.end());
 	up(isl_rows, rows);
 	up(isl_off, offs);
 	up(isl_sys, last_isl_sys);
 	up(isl_sys_off, last_isl_sys_off);
+	up(isl_jlin, jlin);
+	if (colored) {
+		up(isl_eoff, eoff);
+		up(isl_coff, coff);
+		up(isl_icoff, icoff);
+		up(isl_serial, serial);
+	}
 }
 
 // The colour schedule as one sequential row list per scene (colours in order,
@@ -1197,13 +1274,12 @@ void cp_world::launch_pgs_colored_lds_global(const Bodies& b, const Params& P) {
 
 void cp_world::step() {
 	if (serving) serving_commit();
-	if (!j_type.empty() && cfg.pgs_mode != CP_PGS_REFERENCE_ORDER) throw ApiErr{CP_ERR_STATE};  // joints: sequential order only
+	if (!j_type.empty() && cfg.pgs_mode == CP_PGS_COLORED) throw ApiErr{CP_ERR_STATE};  // joints: island solves only
 	if (!hs_explicit && cfg.hol_iters > 0 && (hs_scan || hs_changed)) {
 		discover_systems();
 		hs_scan = false;
 	}
 	if (joints_dirty) upload_joints();
-	if (!j_type.empty()) lin.reserve(6 * (size_t)nb);
 	Params P = params();
 	Bodies b = bodies();
 	if (timing) HIPCHK(hipEventRecord(ev[0], stream));
@@ -1214,7 +1290,7 @@ void cp_world::step() {
 	if (timing) HIPCHK(hipEventRecord(ev[2], stream));
 	persist(npairs);
 	edges();
-	bool ordered = cfg.pgs_mode == CP_PGS_REFERENCE_ORDER;
+	bool ordered = cfg.pgs_mode != CP_PGS_COLORED;  // island solves (reference order or coloured islands)
 	Joints J = joints();
 	HSysDev H = hsys();
 	const bool hol = cfg.hol_iters > 0 && H.nsys > 0;
@@ -1227,8 +1303,9 @@ void cp_world::step() {
 	}
 	int nisl = 0;
 	if (ordered) {
-		schedule_reference_order();
+		schedule_islands(cfg.pgs_mode == CP_PGS_ISLAND_COLORED);
 		nisl = (int)last_offs.size() - 1;
+		if (hol) lin.reserve(6 * (size_t)lin_nl);
 	} else {
 		color_off.reserve((size_t)nscenes * (CP_MAXC + 2));
 		ncolors.reserve(nscenes);
@@ -1249,9 +1326,10 @@ void cp_world::step() {
 		if (ordered) {
 			HIPCHK(hipMemsetAsync(dv.p, 0, 12 * (size_t)nb * sizeof(double), stream));
 			HIPCHK(hipMemsetAsync(applied.p, 0, (size_t)nb * sizeof(int), stream));
-			if (hol && s == 0) HIPCHK(hipMemsetAsync(lin.p, 0, 6 * (size_t)nb * sizeof(double), stream));
-			launch_pgs_ordered(b, cb[cur].c, J, H, isl_rows.p, isl_off.p, isl_sys.p, isl_sys_off.p, nisl, P, applied.p, lin.p, s == 0,
-			                   stream);
+			if (hol && s == 0) HIPCHK(hipMemsetAsync(lin.p, 0, 6 * (size_t)lin_nl * sizeof(double), stream));
+			IslandSched S{nisl, lin_nl, isl_rows.p, isl_off.p, isl_sys.p, isl_sys_off.p, isl_jlin.p,
+			              isl_eoff.p, isl_coff.p, isl_icoff.p, isl_serial.p};
+			launch_pgs_islands(b, cb[cur].c, J, H, S, P, applied.p, lin.p, s == 0, cfg.pgs_mode == CP_PGS_ISLAND_COLORED, stream);
 		} else {
 			static const int thr[3] = {64, 128, 256};
 			for (int c = 0; c < 3; c++)
@@ -1357,7 +1435,7 @@ const char* cp_status_string(int s) {
 
 static bool valid_cfg(const cp_config* c) {
 	return c && c->dt > 0 && c->substeps >= 1 && c->vel_iters >= 0 && c->pos_iters >= 0 && c->gyro_ticks >= 1 && c->solver_threads >= 64 &&
-	       c->solver_threads <= 1024 && (c->solver_threads % 64) == 0 && (c->pgs_mode == 0 || c->pgs_mode == 1) &&
+	       c->solver_threads <= 1024 && (c->solver_threads % 64) == 0 && (c->pgs_mode >= 0 && c->pgs_mode <= 2) &&
 	       (c->precision == CP_PREC_FP64 || c->precision == CP_PREC_SPLIT);
 }
 

@@ -37,7 +37,13 @@ typedef enum {
 	CP_ERR_STATE = 5      /* call not valid in the current world state */
 } cp_status;
 
-typedef enum { CP_PGS_COLORED = 0, CP_PGS_REFERENCE_ORDER = 1 } cp_pgs_mode;
+/* CP_PGS_COLORED: contacts only, graph-coloured per scene (FP64 or split).
+ * CP_PGS_REFERENCE_ORDER: the reference's island search and sequential row
+ * order (PhysicsEngine.cpp:1536-1613, ConstraintSolver.cpp:80-213), joints and
+ * holonomic systems included; FP64.  CP_PGS_ISLAND_COLORED: the same islands,
+ * rows and holonomic solve, with each island's edges graph-coloured so that a
+ * wave sweeps an island in parallel; FP64. */
+typedef enum { CP_PGS_COLORED = 0, CP_PGS_REFERENCE_ORDER = 1, CP_PGS_ISLAND_COLORED = 2 } cp_pgs_mode;
 
 /* CP_PREC_FP64: rows, impulses and velocity deltas in FP64 (the reference's
  * arithmetic).  CP_PREC_SPLIT: rows derived in FP64 and stored in FP32, PGS

@@ -54,6 +54,34 @@ def test_small_chains_on_terrain_match_oracle():
     assert worst <= 1e-12
 
 
+def test_small_chains_island_colored_replayed_by_oracle():
+    """The coloured-island solve through the fall: each step equals the
+    oracle's replay of the step's recorded row order."""
+    desc = scenes.c2_description(n_chains=6, links=16, grid=40, cell=0.5, amp=1.0, seed=7)
+    n = len(desc["hulls"])
+    no = cpa.reference_node_order(n)
+    cfg = dict(CFG, pgs_mode=2)
+    probe = world_from_description(desc, **cfg)
+    probe.set_node_order(no)
+    probe.step(1)
+    off, og = _systems(probe)
+    probe.close()
+    w = world_from_description(desc, **cfg)
+    w.set_holonomic_systems(off, og)
+    w.set_node_order(no)
+    o = Oracle.from_description(desc, dt=CFG["dt"], gravity_y=-6.0, vel_iters=10, pos_iters=1, substeps=8, hol_iters=1)
+    o.load_joints(np.zeros((len(desc["joints"][0]), 11)), off, og)
+    worst = 0.0
+    for k in range(150):
+        w.step(1)
+        o.step_scheduled(*w.last_islands())
+        a, b = w.state(), o.state()
+        worst = max(worst, float(np.max(np.abs(a - b) / np.maximum(1.0, np.abs(b)))))
+    print(f"150 steps coloured islands, worst state rel {worst:.2e}, contacts {len(w.contacts()[0])}", flush=True)
+    assert len(w.contacts()[0]) > 0
+    assert worst <= 1e-12
+
+
 def _attach_error(state, joints):
     info, geo = joints
     err = 0.0
@@ -69,9 +97,10 @@ def _attach_error(state, joints):
     return err
 
 
-def test_full_size_chains_on_terrain_properties():
+@pytest.mark.parametrize("mode", [1, 2])
+def test_full_size_chains_on_terrain_properties(mode):
     desc = scenes.c2_description()
-    w = world_from_description(desc, **CFG)
+    w = world_from_description(desc, **dict(CFG, pgs_mode=mode))
     for k in range(120):
         w.step(1)
     st = w.state()

@@ -127,6 +127,29 @@ def test_mesh_colored_schedule_replayed_by_oracle(k):
     assert _rel(ff, of) <= 1e-10
 
 
+@pytest.mark.parametrize("k", [60, 120, 239])
+def test_mesh_island_colored_replayed_by_oracle(k):
+    """Coloured islands with the jointed chains and mesh contacts: five steps,
+    each equal to the oracle's replay of the recorded row order."""
+    fx = golden("terrain")
+    w = world_from_fixture(fx, pgs_mode=2)
+    _load(w, fx, k)
+    o = Oracle(fx)
+    o.load(fx[f"s{k}_pre"], fx[f"s{k}_pre_inertia"], fx.cache(k, "pre"))
+    o.load_joints(fx[f"s{k}_joint_pre"], fx[f"s{k}_hsys_off"], fx[f"s{k}_hsys_og"])
+    worst = 0.0
+    for _ in range(5):
+        w.step(1)
+        o.step_scheduled(*w.last_islands())
+        worst = max(worst, _rel(w.state(), o.state()))
+    ii, mg, ff = w.contacts()
+    oi, om, of = o.cache()
+    assert np.array_equal(ii, oi) and np.array_equal(mg, om)
+    assert _rel(ff, of) <= 1e-10
+    assert _rel(w.joint_state(), o.joint_state()) <= 1e-10
+    assert worst <= 1e-12
+
+
 def test_mesh_body_must_be_fixed():
     fx = golden("terrain")
     w = cpa.World()
