@@ -225,7 +225,11 @@ void launch_kill_contacts(const Contacts& C, int n, const int* ranges, int nrang
 void launch_integrate(const Bodies& b, const Params& p, hipStream_t s);
 void launch_bp_keys(const Bodies& b, uint64_t* keys, int* vals, hipStream_t s);
 void launch_bp_gather(const Bodies& b, const int* sorted, double* sb, hipStream_t s);
-void launch_bp_sweep(int n, const int* sorted, const double* sb, const int* scene_of_body, const int* scene_body_off, int* counts,
+#define BP_CH 64  // sweep positions per broadphase thread
+void launch_bp_extent(int n, const int* sorted, const double* sb, const int* scene_of_body, const int* scene_body_off, int* ext,
+                      int* nchunk, hipStream_t s);
+void launch_bp_owner(int n, const int* choff, int* owner, hipStream_t s);
+void launch_bp_sweep(int n, int nch, const int* sorted, const double* sb, const int* ext, const int* choff, const int* owner, int* counts,
                      const int* offs, uint64_t* out, int write, hipStream_t s);
 // lists: 6 * npairs ints (per-class pair lists), counts: 8 ints.
 // nocol: sorted pair keys (lo << 32 | hi) that never collide (jointed bodies);

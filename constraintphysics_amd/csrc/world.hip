@@ -163,7 +163,7 @@ struct cp_world {
 	bool uploaded = false;
 	// step temporaries
 	DArr<uint64_t> bp_keys, bp_keys2, pairs, pairs2, nk;
-	DArr<int> bp_vals, bp_vals2, bp_counts, bp_offs;
+	DArr<int> bp_vals, bp_vals2, bp_counts, bp_ext, bp_choff, bp_owner, bp_ccount, bp_coffs;
 	DArr<double> bp_sb;
 	DArr<char> tmp;
 	int mcap = 0;
@@ -355,7 +355,9 @@ struct cp_world {
 	void write_joint_state(const double* s);
 	void layout_systems();
 	void discover_systems();
-	std::vector<int> root_order();
+	const std::vector<int>& root_order();  // the reference's node-map order (cached per body count)
+	std::vector<int> root_cache;
+	int root_cache_nb = -1;
 	void free_all();
 	void upload();
 	void ensure_manifolds(int n);
@@ -375,7 +377,7 @@ void cp_world::free_all() {
 	DArr<int>* ints[] = {&s_type, &s_pt_off, &s_face_off, &s_arc_off, &s_edge_off, &s_fv_off, &s_fv, &s_ncalc, &s_ndir, &s_sid, &s_red,
 	                     &s_minpid, &s_maxpid, &s_refidx, &s_arcs, &s_edges, &s_vaf_off, &s_vaf, &s_vae_off, &s_vae, &movement, &shape,
 	                     &scene, &wpt_off, &wgv_off, &color_mask, &scene_body_off, &scene_of_body, &bp_vals, &bp_vals2, &bp_counts,
-	                     &bp_offs, &pt_cnt, &pt_off, &np_lists, &np_counts, &keep, &keep_scan, &rank, &e_head, &e_idx, &e_first, &e_cnt, &e_live, &e_color,
+	                     &bp_ext, &bp_choff, &bp_owner, &bp_ccount, &bp_coffs, &pt_cnt, &pt_off, &np_lists, &np_counts, &keep, &keep_scan, &rank, &e_head, &e_idx, &e_first, &e_cnt, &e_live, &e_color,
 	                     &e_order, &color_off, &ncolors, &edge_range, &applied, &isl_rows, &isl_off};
 	for (auto* a : ints) a->free_();
 	for (auto& a : cls_dev) a.free_();
@@ -702,16 +704,19 @@ int cp_world::mesh_narrowphase(int npairs) {
 
 // Body indices in the iteration order of the engine's node map (given by the
 // caller, else libstdc++'s order for ids 1..n).
-std::vector<int> cp_world::root_order() {
-	std::vector<int> roots;
-	if ((int)node_order.size() == nb)
-		for (int64_t id : node_order) roots.push_back((int)id - 1);
-	else {
+const std::vector<int>& cp_world::root_order() {
+	if ((int)node_order.size() == nb) {
+		root_cache.clear();
+		for (int64_t id : node_order) root_cache.push_back((int)id - 1);
+		root_cache_nb = -1;
+	} else if (root_cache_nb != nb) {
 		std::vector<int64_t> ord(nb);
 		cp_reference_node_order(nb, ord.data());
-		for (int64_t id : ord) roots.push_back((int)id - 1);
+		root_cache.assign(ord.begin(), ord.end());
+		for (int& r : root_cache) r -= 1;
+		root_cache_nb = nb;
 	}
-	return roots;
+	return root_cache;
 }
 
 // maintainHolonomicSystems (PhysicsEngine.cpp:1292-1307) for joints added
@@ -951,7 +956,7 @@ int cp_world::broadphase() {
 	if (n == 0) return 0;
 	bp_keys.reserve(n); bp_keys2.reserve(n); bp_vals.reserve(n); bp_vals2.reserve(n);
 	bp_sb.reserve(6 * (size_t)n);
-	bp_counts.reserve(n + 1); bp_offs.reserve(n + 1);
+	bp_counts.reserve(n + 1);
 	launch_bp_keys(b, bp_keys.p, bp_vals.p, stream);
 	size_t bytes = 0;
 	if (ns == 1) {
@@ -966,12 +971,20 @@ int cp_world::broadphase() {
 		                                                   scene_body_off.p, scene_body_off.p + 1, 0, 64, stream));
 	}
 	launch_bp_gather(b, bp_vals2.p, bp_sb.p, stream);
+	// sweep ends and chunks, then count / scan / write per chunk
+	bp_ext.reserve(n); bp_choff.reserve(n + 1);
 	HIPCHK(hipMemsetAsync(bp_counts.p + n, 0, sizeof(int), stream));
-	launch_bp_sweep(n, bp_vals2.p, bp_sb.p, scene_of_body.p, scene_body_off.p, bp_counts.p, nullptr, nullptr, 0, stream);
-	scan(bp_counts.p, bp_offs.p, n + 1);
-	int np_ = read1(bp_offs.p + n);
+	launch_bp_extent(n, bp_vals2.p, bp_sb.p, scene_of_body.p, scene_body_off.p, bp_ext.p, bp_counts.p, stream);
+	scan(bp_counts.p, bp_choff.p, n + 1);
+	int nch = read1(bp_choff.p + n);
+	bp_owner.reserve(nch); bp_ccount.reserve(nch + 1); bp_coffs.reserve(nch + 1);
+	launch_bp_owner(n, bp_choff.p, bp_owner.p, stream);
+	HIPCHK(hipMemsetAsync(bp_ccount.p + nch, 0, sizeof(int), stream));
+	launch_bp_sweep(n, nch, bp_vals2.p, bp_sb.p, bp_ext.p, bp_choff.p, bp_owner.p, bp_ccount.p, nullptr, nullptr, 0, stream);
+	scan(bp_ccount.p, bp_coffs.p, nch + 1);
+	int np_ = read1(bp_coffs.p + nch);
 	pairs.reserve(np_); pairs2.reserve(np_);
-	launch_bp_sweep(n, bp_vals2.p, bp_sb.p, scene_of_body.p, scene_body_off.p, bp_counts.p, bp_offs.p, pairs2.p, 1, stream);
+	launch_bp_sweep(n, nch, bp_vals2.p, bp_sb.p, bp_ext.p, bp_choff.p, bp_owner.p, bp_ccount.p, bp_coffs.p, pairs2.p, 1, stream);
 	if (np_ > 0) {
 		bytes = 0;
 		HIPCHK(hipcub::DeviceRadixSort::SortKeys(nullptr, bytes, pairs2.p, pairs.p, np_, 0, 64, stream));
@@ -1081,63 +1094,74 @@ void cp_world::schedule_islands(bool colored) {
 	std::vector<int> live;
 	down(key, cb[cur].c.key, n);
 	down(live, cb[cur].c.live, n);
+	// edges in key order (lo << 32 | hi): the contact pairs (the cache is
+	// sorted by key) merged with the joint pairs (stable: creation order)
 	struct Edge {
-		int first = -1;
+		int first = -1, nlive = 0;  // first cache index of the pair, live contacts
 		int lo = 0, hi = 0;
-		std::vector<int> joints;
+		int j0 = 0, j1 = 0;  // joints jord[j0, j1)
 	};
-	std::vector<Edge> edges;
-	std::unordered_map<uint64_t, int> eid;
-	auto edge_of = [&](uint64_t k) {
-		auto it = eid.find(k);
-		if (it != eid.end()) return it->second;
-		eid.emplace(k, (int)edges.size());
-		edges.emplace_back();
-		edges.back().lo = (int)(k >> 32);
-		edges.back().hi = (int)(k & 0xFFFFFFFFu);
-		return (int)edges.size() - 1;
-	};
-	for (int k = 0; k < n; k++) {
-		if (k > 0 && key[k] == key[k - 1]) continue;
-		edges[edge_of(key[k])].first = k;
-	}
-	for (int j = 0; j < (int)j_type.size(); j++) {
+	const int nj = (int)j_type.size();
+	std::vector<uint64_t> jkey(nj);
+	std::vector<int> jord(nj);
+	for (int j = 0; j < nj; j++) {
 		uint64_t lo = (uint64_t)std::min(j_a[j], j_b[j]), hi = (uint64_t)std::max(j_a[j], j_b[j]);
-		edges[edge_of(lo << 32 | hi)].joints.push_back(j);
+		jkey[j] = lo << 32 | hi;
+		jord[j] = j;
 	}
-	std::vector<std::vector<std::pair<int, int>>> adj(nb);  // (other, edge), by other id
-	for (auto& kv : eid) {
-		int lo = (int)(kv.first >> 32), hi = (int)(kv.first & 0xFFFFFFFFu);
-		adj[lo].push_back({hi, kv.second});
-		adj[hi].push_back({lo, kv.second});
+	std::stable_sort(jord.begin(), jord.end(), [&](int x, int y) { return jkey[x] < jkey[y]; });
+	std::vector<Edge> edges;
+	edges.reserve(n / 2 + nj + 1);
+	{
+		int k = 0, jj = 0;
+		while (k < n || jj < nj) {
+			uint64_t kk = k < n ? key[k] : UINT64_MAX, kj = jj < nj ? jkey[jord[jj]] : UINT64_MAX;
+			uint64_t K = std::min(kk, kj);
+			Edge E;
+			E.lo = (int)(K >> 32);
+			E.hi = (int)(K & 0xFFFFFFFFu);
+			E.j0 = jj;
+			while (jj < nj && jkey[jord[jj]] == K) jj++;
+			E.j1 = jj;
+			if (k < n && key[k] == K) {
+				E.first = k;
+				for (; k < n && key[k] == K; k++) E.nlive += live[k] != 0;
+			}
+			edges.push_back(E);
+		}
 	}
-	for (auto& a : adj) std::sort(a.begin(), a.end());
-	std::vector<int> sys_of(j_type.size(), -1);
+	// adjacency (CSR): appending in key order lists each body's neighbours by
+	// id (those below it, as hi, come first, each group ascending)
+	std::vector<int> aoff(nb + 1, 0), adj_o(2 * edges.size()), adj_e(2 * edges.size());
+	for (const Edge& E : edges) { aoff[E.lo + 1]++; aoff[E.hi + 1]++; }
+	for (int i = 0; i < nb; i++) aoff[i + 1] += aoff[i];
+	{
+		std::vector<int> fill(aoff.begin(), aoff.end() - 1);
+		for (int e = 0; e < (int)edges.size(); e++) {
+			const Edge& E = edges[e];
+			adj_o[fill[E.lo]] = E.hi; adj_e[fill[E.lo]++] = e;
+			adj_o[fill[E.hi]] = E.lo; adj_e[fill[E.hi]++] = e;
+		}
+	}
+	std::vector<int> sys_of(nj, -1), sys_seen(hs_og.size(), -1);
 	if (cfg.hol_iters > 0)
 		for (size_t q = 0; q < hs_og.size(); q++)
 			for (int j : hs_og[q]) sys_of[j] = (int)q;
 	// an edge's rows: its joints, then its live contacts (normal, friction)
 	auto emit = [&](int e) {
 		const Edge& E = edges[e];
-		for (int j : E.joints) { last_rows.push_back(j); last_rows.push_back(2); }
-		if (E.first < 0) return;
+		for (int x = E.j0; x < E.j1; x++) { last_rows.push_back(jord[x]); last_rows.push_back(2); }
+		if (!E.nlive) return;
 		for (int k = E.first; k < n && key[k] == key[E.first]; k++) {
 			if (!live[k]) continue;
 			last_rows.push_back(k); last_rows.push_back(0);
 			last_rows.push_back(k); last_rows.push_back(1);
 		}
 	};
-	auto has_rows = [&](int e) {
-		const Edge& E = edges[e];
-		if (!E.joints.empty()) return true;
-		if (E.first < 0) return false;
-		for (int k = E.first; k < n && key[k] == key[E.first]; k++)
-			if (live[k]) return true;
-		return false;
-	};
+	auto has_rows = [&](int e) { return edges[e].j1 > edges[e].j0 || edges[e].nlive > 0; };
 	std::vector<char> visited(nb, 0);
-	std::vector<int> roots = root_order();
-	std::vector<int> eoff{0}, coff{0}, icoff{0}, serial, jlin(2 * j_type.size(), 0), isl_edges, col;
+	const std::vector<int>& roots = root_order();
+	std::vector<int> eoff{0}, coff{0}, icoff{0}, serial, jlin(2 * nj, 0), isl_edges, col;
 	std::vector<std::pair<int, int>> slots;  // (non-dynamic body, lin column) of the current island
 	int nl = nb;
 	if (colored) col_used.assign(nb, {0, 0});
@@ -1155,10 +1179,10 @@ void cp_world::schedule_islands(bool colored) {
 			if (visited[c]) continue;
 			visited[c] = 1;
 			if (b_move[c] == 0)
-				for (auto& e : adj[c])
-					if (!visited[e.first] && has_rows(e.second)) isl_edges.push_back(e.second);
-			for (auto& e : adj[c])
-				if (b_move[e.first] == 0 && !visited[e.first]) st.push_back(e.first);
+				for (int x = aoff[c]; x < aoff[c + 1]; x++)
+					if (!visited[adj_o[x]] && has_rows(adj_e[x])) isl_edges.push_back(adj_e[x]);
+			for (int x = aoff[c]; x < aoff[c + 1]; x++)
+				if (b_move[adj_o[x]] == 0 && !visited[adj_o[x]]) st.push_back(adj_o[x]);
 		}
 		if (isl_edges.empty()) continue;
 		size_t start = last_rows.size() / 2;
@@ -1197,13 +1221,15 @@ void cp_world::schedule_islands(bool colored) {
 			serial.push_back(any_serial ? 1 : 0);
 		}
 		// systems in order of first appearance; linearization-error columns
-		size_t sys_start = last_isl_sys.size();
+		const int isl_id = (int)last_offs.size();
 		slots.clear();
 		for (size_t r = start; r < last_rows.size() / 2; r++) {
 			if (last_rows[2 * r + 1] != 2) continue;
 			int j = (int)last_rows[2 * r], q = sys_of[j];
-			if (q >= 0 && std::find(last_isl_sys.begin() + sys_start, last_isl_sys.end(), q) == last_isl_sys.end())
+			if (q >= 0 && sys_seen[q] != isl_id) {
+				sys_seen[q] = isl_id;
 				last_isl_sys.push_back(q);
+			}
 			int ends[2] = {j_a[j], j_b[j]};
 			for (int h = 0; h < 2; h++) {
 				int i = ends[h], cidx = i;
