@@ -70,7 +70,7 @@ typedef struct {
 	int defect_fixes;        /* bit 0: drop non-finite manifold points (the reference's clip
 	                            intersection divides by n.d, which rounds to 0 for an edge parallel to
 	                            the clipping edge, CollisionDetect.cpp:444-467); 0 = bug-compatible */
-	int precision;           /* cp_precision: solve precision of CP_PGS_COLORED (CP_PGS_REFERENCE_ORDER is
+	int precision;           /* cp_precision: solve precision of CP_PGS_COLORED (the island modes are
 	                            always FP64) */
 } cp_config;
 
@@ -226,8 +226,9 @@ int cp_narrowphase(cp_world* w, int n, const int64_t* pairs2, int* counts, doubl
 int cp_narrowphase_multi(cp_world* w, int n, const int64_t* pairs2, int cap, int64_t* pair_index, int* counts, double* heads,
                          double* pts, uint64_t* magic, int* n_out);
 /* Sweep schedule of the last step: in CP_PGS_REFERENCE_ORDER mode the island
- * rows; in CP_PGS_COLORED mode (with diagnostics bit 1) one row list per scene
- * in colour order.  Rows are (contact index in the post-narrowphase cache,
+ * rows; in CP_PGS_ISLAND_COLORED mode the same islands with each island's rows
+ * in colour order; in CP_PGS_COLORED mode (with diagnostics bit 1) one row
+ * list per scene in colour order.  Rows are (contact index in the post-narrowphase cache,
  * kind 0 normal / 1 friction; or joint index, kind 2), with group offsets. */
 int cp_world_last_islands(cp_world* w, int cap_rows, int64_t* rows2, int cap_offs, int64_t* offs, int* n_rows, int* n_islands);
 
@@ -236,6 +237,32 @@ int cp_world_last_islands(cp_world* w, int cap_rows, int64_t* rows2, int cap_off
  * A^-1 (target - J dv)) (contact: lambda >= 0; friction: disk of radius
  * mu * lambda_n), out[1] = ||lambda||, out[2] = out[0] / out[1], out[3] = rows. */
 int cp_world_last_residual(cp_world* w, double* out4);
+
+/* Serving world (continuous batching of independent scenes).  The nearest
+ * reference interfaces are createRigidBody / removeRigidBody
+ * (PhysicsEngine.cpp:486-513, :515-518) called on a running engine; scenes in
+ * one serving world never interact.
+ * cp_world_serving_init: turns an empty world (no bodies, no joints) into a
+ *   serving world of `capacity` body slots (its HBM budget).
+ * cp_world_admit_scene: stages a scene of n bodies into the first free slot
+ *   range that holds it: shape_ids[n] (no meshes; at most 32 points and 64
+ *   faces each), movement[n] (0 dynamic, 1 fixed), mp[n][19] (mass, I_ref,
+ *   I_ref^-1), material[n][4], state13[n][13]; *handle names it.
+ *   CP_ERR_CAPACITY when no free range is large enough (the caller queues).
+ * cp_world_retire_scene: frees the scene's range and drops its contacts.
+ * cp_world_serving_commit: applies staged admissions and retirements (the next
+ *   cp_world_step does so itself).
+ * cp_world_get_scene_state: the scene's bodies, state13[n][13]; with
+ *   state13 == NULL only *n.
+ * cp_world_serving_info: info4 = capacity, free slots, largest free range,
+ *   admitted scenes. */
+int cp_world_serving_init(cp_world* w, int capacity);
+int cp_world_admit_scene(cp_world* w, int n, const int* shape_ids, const int* movement, const double* mp, const double* material,
+                         const double* state13, int* handle);
+int cp_world_retire_scene(cp_world* w, int handle);
+int cp_world_serving_commit(cp_world* w);
+int cp_world_get_scene_state(cp_world* w, int handle, double* state13, int* n);
+int cp_world_serving_info(cp_world* w, int* info4);
 
 /* Timing of the last cp_world_step call, per phase, in milliseconds (HIP events):
  * [0] gravity + broadphase, [1] narrowphase, [2] persistence + schedule,
