@@ -175,6 +175,8 @@ struct Contacts {
 // Record of CP_RS values (FP64: 768 B = 6 lines of 128 B; FP32: 384 B = 3
 // lines).  The fields a sweep writes (impulses, static flag) share the first
 // line, so one velocity sweep reads the whole record and dirties one line.
+// Blocks start at multiples of 4 values and records are 128-byte aligned, so
+// a row's coefficients load as 16-byte vectors.
 #define CP_RS 96
 enum RowField {
 	CP_R_IMP = 0,     // accumulated normal impulse
@@ -185,16 +187,16 @@ enum RowField {
 	CP_R_INV = 6,     // (J M^-1 J^T (1+cfm))^-1
 	CP_R_TGT = 7,     // velocity target
 	CP_R_PTGT = 8,    // pseudo-velocity target
-	CP_R_FTGT = 9,    // friction targets [2]
-	CP_R_FINV = 11,   // friction 2x2 inverse [4]
-	CP_R_JA = 15,     // normal row J_a [6]
-	CP_R_JB = 21,     // J_b [6]
-	CP_R_ITA = 27,    // M_a^-1 J_a^T [6]
-	CP_R_ITB = 33,    // M_b^-1 J_b^T [6]
-	CP_R_FJA = 39,    // friction J_a [2][6]
-	CP_R_FJB = 51,    // friction J_b [2][6]
-	CP_R_FITA = 63,   // M_a^-1 J_a^T [6][2]
-	CP_R_FITB = 75    // M_b^-1 J_b^T [6][2]; 87..95 padding
+	CP_R_FTGT = 9,    // friction targets [2]; 11 padding
+	CP_R_FINV = 12,   // friction 2x2 inverse [4]
+	CP_R_JA = 16,     // normal row J_a [6]
+	CP_R_JB = 22,     // J_b [6]
+	CP_R_ITA = 28,    // M_a^-1 J_a^T [6]
+	CP_R_ITB = 34,    // M_b^-1 J_b^T [6]
+	CP_R_FJA = 40,    // friction J_a [2][6]
+	CP_R_FJB = 52,    // friction J_b [2][6]
+	CP_R_FITA = 64,   // M_a^-1 J_a^T [6][2]
+	CP_R_FITB = 76    // M_b^-1 J_b^T [6][2]; 88..95 padding
 };
 
 // A step's manifold points, compacted and sorted by pair key.

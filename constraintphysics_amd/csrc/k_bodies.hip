@@ -63,7 +63,7 @@ __device__ void gyro_rotate(V3& angv, Q& q, const M3& I_world, const M3& I_ref, 
 
 // update_positions (PhysicsEngine.cpp:390-410) followed by
 // RigidBody::updateInertiaTensor (RigidBody.cpp:270-279).
-__global__ void k_integrate(Bodies b, Params p) {
+__global__ __launch_bounds__(128) void k_integrate(Bodies b, Params p) {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
 	const int n = b.n;
 	if (i >= b.nl || b.movement[i] != 0) return;
@@ -94,7 +94,7 @@ __global__ void k_integrate(Bodies b, Params p) {
 
 // updateInertiaTensor for every body without touching the stored orientation
 // (used when a state is uploaded without its world inertia).
-__global__ void k_update_inertia(Bodies b, int first, int count, const int* list) {
+__global__ __launch_bounds__(128) void k_update_inertia(Bodies b, int first, int count, const int* list) {
 	int t = blockIdx.x * blockDim.x + threadIdx.x;
 	if (t >= count) return;
 	int i = list ? list[t] : first + t;
@@ -114,7 +114,7 @@ __device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b 
 // and the AABB (ConvexPrimitive.cpp:660-676).  only_moved: the end-of-step
 // refresh (PhysicsEngine.cpp:424-438), which normalises the orientation first
 // and skips bodies whose pose did not change in the last substep.
-__global__ void k_refresh_geometry(Bodies b, Shapes sh, int only_moved, int first, int count, const int* list) {
+__global__ __launch_bounds__(128) void k_refresh_geometry(Bodies b, Shapes sh, int only_moved, int first, int count, const int* list) {
 	int t = blockIdx.x * blockDim.x + threadIdx.x;
 	if (t >= count) return;
 	int i = list ? list[t] : first + t;

@@ -209,7 +209,7 @@ __global__ void k_edge_build(const Contacts C, int n, const int* head, const int
 // One thread per scene: greedy edge colouring, then a counting sort of the
 // scene's live edges by colour.  Colour MAXC collects edges that found no free
 // colour; the solver runs that bucket on a single thread.
-__global__ void k_color(Bodies b, const int* scene_body_off, int nscenes, const uint64_t* ekey, const int* elive, int nedges,
+__global__ __launch_bounds__(64) void k_color(Bodies b, const int* scene_body_off, int nscenes, const uint64_t* ekey, const int* elive, int nedges,
                         int* ecolor, int* order, int* color_off, int* ncolors, int* edge_range) {
 	int s = blockIdx.x * blockDim.x + threadIdx.x;
 	if (s >= nscenes) return;

@@ -194,7 +194,7 @@ __device__ void build_joint(const Bodies& b, Joints& J, int k, double sub_dt, do
 // mode 0: every joint except holonomic ones at the first substep; 1: holonomic
 // joints with re-projection (end of step); 2: holonomic joints as stored
 // (state loaded, or systems just created).
-__global__ void k_joint_rows(Bodies b, Joints J, Params P, double sub_dt, int mode, int first) {
+__global__ __launch_bounds__(64) void k_joint_rows(Bodies b, Joints J, Params P, double sub_dt, int mode, int first) {
 	int k = blockIdx.x * blockDim.x + threadIdx.x;
 	if (k >= J.n) return;
 	bool in_sys = J.sys[k] >= 0 && P.hol_iters > 0;
@@ -250,7 +250,7 @@ __device__ void hs_block_init(const Joints& J, double cfm, int c1, int c2, doubl
 }
 
 // HolonomicSystem::computeInverse (:459-543), one thread per system.
-__global__ void k_hsys_factor(Joints J, HSysDev H, double cfm) {
+__global__ __launch_bounds__(64) void k_hsys_factor(Joints J, HSysDev H, double cfm) {
 	int s = blockIdx.x * blockDim.x + threadIdx.x;
 	if (s >= H.nsys) return;
 	const int nc = H.nc[s];

@@ -38,7 +38,7 @@ __device__ __forceinline__ V3 ld3(const double* a, int i, int n) { return V3{a[i
 // Row record of contact k, CP_RS values of the solve precision T.
 template <class T>
 __device__ __forceinline__ T* rec(const Contacts& C, int k) {
-	return reinterpret_cast<T*>(C.row) + (size_t)k * CP_RS;
+	return static_cast<T*>(__builtin_assume_aligned(reinterpret_cast<T*>(C.row) + (size_t)k * CP_RS, 128));
 }
 
 template <class T>
@@ -50,7 +50,7 @@ __device__ __forceinline__ T tsqrt(T x) {
 // Rows are derived in FP64 from the FP64 state and stored in the solve
 // precision T (FP64, or FP32 for the split path).
 template <class T>
-__global__ void k_build_rows(Bodies b, Contacts C, int nc, Params P) {
+__global__ __launch_bounds__(64) void k_build_rows(Bodies b, Contacts C, int nc, Params P) {
 	int k = blockIdx.x * blockDim.x + threadIdx.x;
 	if (k >= nc || !C.live[k]) return;
 	const int N = b.n, K = C.cap;
@@ -218,89 +218,135 @@ struct DvLds {
 	}
 };
 
+// Record blocks are loaded into registers as 16-byte vectors before any
+// update is written, so a row step waits for memory once instead of once per
+// coefficient (the deltas it writes are LDS or other bodies' memory, which the
+// compiler cannot always prove disjoint from the record).
+template <class T, int N>
+__device__ __forceinline__ void ldv(const T* p, T (&o)[N]) {
+	constexpr int W = 16 / sizeof(T);
+	static_assert(N % W == 0, "whole 16-byte vectors");
+	if constexpr (sizeof(T) == 4) {
+		const float4* q = reinterpret_cast<const float4*>(p);
+#pragma unroll
+		for (int i = 0; i < N / 4; i++) {
+			float4 x = q[i];
+			o[4 * i] = x.x; o[4 * i + 1] = x.y; o[4 * i + 2] = x.z; o[4 * i + 3] = x.w;
+		}
+	} else {
+		const double2* q = reinterpret_cast<const double2*>(p);
+#pragma unroll
+		for (int i = 0; i < N / 2; i++) {
+			double2 x = q[i];
+			o[2 * i] = x.x; o[2 * i + 1] = x.y;
+		}
+	}
+}
+
+// Register copies of a record: h = fields 0..15, j = J_a|J_b, it = ITA|ITB,
+// fj = friction J_a|J_b, fit = friction ITA|ITB (blocks are adjacent).
+template <class T>
+struct RowRegs {
+	T h[16], j[12], it[12], fj[24], fit[24];
+};
+
 // applyWarmStartVelocityChange of the contact row (kind 0) or the friction
 // pair (kind 1) (ConstraintSolver.cpp:121-126).
 template <class T, class D>
-__device__ __forceinline__ void warm_start(const T* R, const D& A, const D& B, int kind) {
-	if (kind == 0) {
-		T imp = R[CP_R_IMP];
-		if (is_zero1(imp)) return;
-		for (int r = 0; r < 6; r++) {
-			T da = 0, db = 0;
-			da += imp * R[CP_R_ITA + r];
-			db += imp * R[CP_R_ITB + r];
-			A.add(r, da);
-			B.add(r, db);
-		}
-	} else {
-		T t0 = R[CP_R_FIMP], t1 = R[CP_R_FIMP + 1];
-		if (is_zero2(t0, t1)) return;
-		for (int r = 0; r < 6; r++) {
-			T da = 0, db = 0;
-			da += t0 * R[CP_R_FITA + r * 2];
-			da += t1 * R[CP_R_FITA + r * 2 + 1];
-			db += t0 * R[CP_R_FITB + r * 2];
-			db += t1 * R[CP_R_FITB + r * 2 + 1];
-			A.add(r, da);
-			B.add(r, db);
-		}
+__device__ __forceinline__ void warm_normal(const RowRegs<T>& g, const D& A, const D& B) {
+	T imp = g.h[CP_R_IMP];
+	if (is_zero1(imp)) return;
+#pragma unroll
+	for (int r = 0; r < 6; r++) {
+		T da = 0, db = 0;
+		da += imp * g.it[r];
+		db += imp * g.it[6 + r];
+		A.add(r, da);
+		B.add(r, db);
+	}
+}
+template <class T, class D>
+__device__ __forceinline__ void warm_friction(const RowRegs<T>& g, const D& A, const D& B) {
+	T t0 = g.h[CP_R_FIMP], t1 = g.h[CP_R_FIMP + 1];
+	if (is_zero2(t0, t1)) return;
+#pragma unroll
+	for (int r = 0; r < 6; r++) {
+		T da = 0, db = 0;
+		da += t0 * g.fit[r * 2];
+		da += t1 * g.fit[r * 2 + 1];
+		db += t0 * g.fit[12 + r * 2];
+		db += t1 * g.fit[12 + r * 2 + 1];
+		A.add(r, da);
+		B.add(r, db);
 	}
 }
 
 // PGSConstraintStep<1> for the contact row (ConstraintSolver.cpp:12-40, :270-272).
 template <class T, class D>
-__device__ __forceinline__ void normal_step(T* R, const D& A, const D& B, bool pseudo) {
+__device__ __forceinline__ void normal_core(T* R, RowRegs<T>& g, const D& A, const D& B, bool pseudo) {
 	int off = pseudo ? 6 : 0;
 	T x = 0, y = 0;
+#pragma unroll
 	for (int q = 0; q < 6; q++) {
-		x += A.get(off + q) * R[CP_R_JA + q];
-		y += B.get(off + q) * R[CP_R_JB + q];
+		x += A.get(off + q) * g.j[q];
+		y += B.get(off + q) * g.j[6 + q];
 	}
 	T cur = x + y;
-	T tgt = pseudo ? R[CP_R_PTGT] : R[CP_R_TGT];
-	T acc = pseudo ? R[CP_R_PIMP] : R[CP_R_IMP];
+	T tgt = pseudo ? g.h[CP_R_PTGT] : g.h[CP_R_TGT];
+	T acc = pseudo ? g.h[CP_R_PIMP] : g.h[CP_R_IMP];
 	T delta = tgt - cur;
 	T m = 0;
-	m += delta * R[CP_R_INV];
+	m += delta * g.h[CP_R_INV];
 	T nw = acc + m;
 	T pr = (nw < 0) ? T(0) : nw;  // std::max<double>(nw, 0)
 	T diff = pr - acc;
 	if (!is_zero1(diff)) {
+#pragma unroll
 		for (int r = 0; r < 6; r++) {
 			T da = 0, db = 0;
-			da += diff * R[CP_R_ITA + r];
-			db += diff * R[CP_R_ITB + r];
+			da += diff * g.it[r];
+			db += diff * g.it[6 + r];
 			A.add(off + r, da);
 			B.add(off + r, db);
 		}
-		R[pseudo ? CP_R_PIMP : CP_R_IMP] = acc + diff;
+		T v = acc + diff;
+		if (pseudo) {
+			g.h[CP_R_PIMP] = v;
+			R[CP_R_PIMP] = v;
+		} else {
+			g.h[CP_R_IMP] = v;
+			R[CP_R_IMP] = v;
+		}
 	}
 }
 
-// PGSConstraintStep<2> with FrictionConstraint::projectValidImpulse (ConstraintSolver.cpp:331-343).
+// PGSConstraintStep<2> with FrictionConstraint::projectValidImpulse (ConstraintSolver.cpp:331-343);
+// the normal impulse is the one the contact row holds now.
 template <class T, class D>
-__device__ __forceinline__ void friction_step(T* R, const D& A, const D& B) {
+__device__ __forceinline__ void friction_core(T* R, RowRegs<T>& g, const D& A, const D& B) {
 	T cur[2];
+#pragma unroll
 	for (int r = 0; r < 2; r++) {
 		T x = 0, y = 0;
+#pragma unroll
 		for (int q = 0; q < 6; q++) {
-			x += A.get(q) * R[CP_R_FJA + r * 6 + q];
-			y += B.get(q) * R[CP_R_FJB + r * 6 + q];
+			x += A.get(q) * g.fj[r * 6 + q];
+			y += B.get(q) * g.fj[12 + r * 6 + q];
 		}
 		cur[r] = x + y;
 	}
-	T d0 = R[CP_R_FTGT] - cur[0], d1 = R[CP_R_FTGT + 1] - cur[1];
+	T d0 = g.h[CP_R_FTGT] - cur[0], d1 = g.h[CP_R_FTGT + 1] - cur[1];
 	T m0 = 0, m1 = 0;
-	m0 += d0 * R[CP_R_FINV];
-	m0 += d1 * R[CP_R_FINV + 1];
-	m1 += d0 * R[CP_R_FINV + 2];
-	m1 += d1 * R[CP_R_FINV + 3];
-	T i0 = R[CP_R_FIMP], i1 = R[CP_R_FIMP + 1];
+	m0 += d0 * g.h[CP_R_FINV];
+	m0 += d1 * g.h[CP_R_FINV + 1];
+	m1 += d0 * g.h[CP_R_FINV + 2];
+	m1 += d1 * g.h[CP_R_FINV + 3];
+	T i0 = g.h[CP_R_FIMP], i1 = g.h[CP_R_FIMP + 1];
 	T n0 = i0 + m0, n1 = i1 + m1;
-	T ni = R[CP_R_IMP];
+	T ni = g.h[CP_R_IMP];
 	const T inf = (T)__builtin_inf();
 	T lim = (ni < inf) ? ni : inf;
-	T maxm = R[CP_R_MU] * lim;
+	T maxm = g.h[CP_R_MU] * lim;
 	T cm = tsqrt(n0 * n0 + n1 * n1);
 	T p0, p1;
 	if (cm > maxm) {
@@ -315,17 +361,67 @@ __device__ __forceinline__ void friction_step(T* R, const D& A, const D& B) {
 	}
 	T e0 = p0 - i0, e1 = p1 - i1;
 	if (!is_zero2(e0, e1)) {
+#pragma unroll
 		for (int r = 0; r < 6; r++) {
 			T da = 0, db = 0;
-			da += e0 * R[CP_R_FITA + r * 2];
-			da += e1 * R[CP_R_FITA + r * 2 + 1];
-			db += e0 * R[CP_R_FITB + r * 2];
-			db += e1 * R[CP_R_FITB + r * 2 + 1];
+			da += e0 * g.fit[r * 2];
+			da += e1 * g.fit[r * 2 + 1];
+			db += e0 * g.fit[12 + r * 2];
+			db += e1 * g.fit[12 + r * 2 + 1];
 			A.add(r, da);
 			B.add(r, db);
 		}
+		g.h[CP_R_FIMP] = i0 + e0;
+		g.h[CP_R_FIMP + 1] = i1 + e1;
 		R[CP_R_FIMP] = i0 + e0;
 		R[CP_R_FIMP + 1] = i1 + e1;
+	}
+}
+
+// One contact in one phase: 0 warm start (both rows), 1 velocity sweep
+// (normal row, then friction rows), 2 position sweep (normal row).
+template <class T, class D>
+__device__ __forceinline__ void contact_phase(T* R, const D& A, const D& B, int phase) {
+	RowRegs<T> g;
+	ldv(R, g.h);
+	if (phase == 0) {
+		ldv(R + CP_R_ITA, g.it);
+		ldv(R + CP_R_FITA, g.fit);
+		warm_normal(g, A, B);
+		warm_friction(g, A, B);
+	} else if (phase == 1) {
+		ldv(R + CP_R_JA, g.j);
+		ldv(R + CP_R_ITA, g.it);
+		ldv(R + CP_R_FJA, g.fj);
+		ldv(R + CP_R_FITA, g.fit);
+		normal_core(R, g, A, B, false);
+		friction_core(R, g, A, B);
+	} else {
+		ldv(R + CP_R_JA, g.j);
+		ldv(R + CP_R_ITA, g.it);
+		normal_core(R, g, A, B, true);
+	}
+}
+
+// One row (kind 0 contact normal, 1 friction) of an island's sequential list.
+template <class T, class D>
+__device__ __forceinline__ void contact_row(T* R, const D& A, const D& B, int kind, int phase) {
+	RowRegs<T> g;
+	ldv(R, g.h);
+	if (kind == 0) {
+		ldv(R + CP_R_ITA, g.it);
+		if (phase == 0) warm_normal(g, A, B);
+		else {
+			ldv(R + CP_R_JA, g.j);
+			normal_core(R, g, A, B, phase == 2);
+		}
+	} else {
+		ldv(R + CP_R_FITA, g.fit);
+		if (phase == 0) warm_friction(g, A, B);
+		else {
+			ldv(R + CP_R_FJA, g.fj);
+			friction_core(R, g, A, B);
+		}
 	}
 }
 
@@ -419,12 +515,7 @@ __device__ __forceinline__ void island_row(const Bodies& b, const Contacts& C, c
 	if (phase == 2 && kind != 0) return;
 	DG A, B;
 	contact_refs(b, C, k, A, B);
-	double* R = rec<double>(C, k);
-	if (phase == 0) warm_start(R, A, B, kind);
-	else if (phase == 1) {
-		if (kind == 0) normal_step(R, A, B, false);
-		else friction_step(R, A, B);
-	} else normal_step(R, A, B, true);
+	contact_row(rec<double>(C, k), A, B, kind, phase);
 }
 
 // computeAndApplyImpulses (HolonomicBlockSolver.cpp:298-355) for the island's
@@ -500,7 +591,7 @@ __device__ __forceinline__ void row_bodies(const Contacts& C, const Joints& J, i
 }
 
 // Reference order: one thread per island walks its rows sequentially.
-__global__ void k_pgs_ordered(Bodies b, Contacts C, Joints J, HSysDev H, IslandSched S, Params P, int* applied, double* lin, int first) {
+__global__ __launch_bounds__(64) void k_pgs_ordered(Bodies b, Contacts C, Joints J, HSysDev H, IslandSched S, Params P, int* applied, double* lin, int first) {
 	int t = blockIdx.x * blockDim.x + threadIdx.x;
 	if (t >= S.n) return;
 	const int* rows = S.rows;
@@ -591,14 +682,7 @@ __device__ __forceinline__ void colour_phase(Contacts& C, const int* order, int 
 		D A = make(i1), B = make(i2);
 		for (int k = k0; k < k1; k++) {
 			if (!C.live[k]) continue;
-			T* R = rec<T>(C, k);
-			if (kind == 0) {
-				warm_start(R, A, B, 0);
-				warm_start(R, A, B, 1);
-			} else if (kind == 1) {
-				normal_step(R, A, B, false);
-				friction_step(R, A, B);
-			} else normal_step(R, A, B, true);
+			contact_phase(rec<T>(C, k), A, B, kind);
 		}
 	}
 }
@@ -606,7 +690,7 @@ __device__ __forceinline__ void colour_phase(Contacts& C, const int* order, int 
 // Global-memory variant (scenes whose velocity deltas do not fit LDS).  The
 // deltas live in b.dv viewed as [12][n] values of T.
 template <class T>
-__global__ void k_pgs_colored(Bodies b, Contacts C, const int* scene_body_off, const int* edge_range, const int* order,
+__global__ __launch_bounds__(256) void k_pgs_colored(Bodies b, Contacts C, const int* scene_body_off, const int* edge_range, const int* order,
                               const int* color_off, const int* ncolors, const int* efirst, const int* ecnt, Params P, const int* scene_list) {
 	const int s = scene_list ? scene_list[blockIdx.x] : blockIdx.x;
 	const int b0 = scene_body_off[s], b1 = scene_body_off[s + 1];
@@ -635,7 +719,7 @@ __global__ void k_pgs_colored(Bodies b, Contacts C, const int* scene_body_off, c
 // velocity sweeps, then the pseudo-velocity half during position sweeps.
 // One workgroup per scene of the launch's size class (scene_list).
 template <class T>
-__global__ void k_pgs_colored_lds(Bodies b, Contacts C, const int* scene_list, const int* scene_body_off, const int* edge_range,
+__global__ __launch_bounds__(256) void k_pgs_colored_lds(Bodies b, Contacts C, const int* scene_list, const int* scene_body_off, const int* edge_range,
                                   const int* order, const int* color_off, const int* ncolors, const int* efirst, const int* ecnt, Params P) {
 	extern __shared__ double lds_raw[];
 	T* sdv = reinterpret_cast<T*>(lds_raw);

@@ -1461,7 +1461,7 @@ const char* cp_status_string(int s) {
 
 static bool valid_cfg(const cp_config* c) {
 	return c && c->dt > 0 && c->substeps >= 1 && c->vel_iters >= 0 && c->pos_iters >= 0 && c->gyro_ticks >= 1 && c->solver_threads >= 64 &&
-	       c->solver_threads <= 1024 && (c->solver_threads % 64) == 0 && (c->pgs_mode >= 0 && c->pgs_mode <= 2) &&
+	       c->solver_threads <= 256 && (c->solver_threads % 64) == 0 && (c->pgs_mode >= 0 && c->pgs_mode <= 2) &&
 	       (c->precision == CP_PREC_FP64 || c->precision == CP_PREC_SPLIT);
 }
 

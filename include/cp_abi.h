@@ -64,7 +64,8 @@ typedef struct {
 	int contact_life;        /* contact_life (PhysicsEngine.h:215) */
 	int gyro_ticks;          /* setAngleVelUpdateTickCount */
 	int pgs_mode;            /* cp_pgs_mode */
-	int solver_threads;      /* threads per scene workgroup in colored mode (64..1024) */
+	int solver_threads;      /* threads per scene workgroup in colored mode for scenes whose velocity
+	                            deltas exceed LDS (64..256, a multiple of 64) */
 	int diagnostics;         /* bit 0: measure the FP64 PGS residual of every step's last substep;
 	                            bit 1: record the coloured sweep schedule (cp_world_last_islands) */
 	int defect_fixes;        /* bit 0: drop non-finite manifold points (the reference's clip
