@@ -146,6 +146,10 @@ void up_tail(DArr<T>& d, const std::vector<T>& h, size_t& done) {
 struct cp_world {
 	cp_config cfg;
 	hipStream_t stream = nullptr;
+	// the PGS size classes run concurrently: two side streams forked from and
+	// joined back into `stream` around each solve (default priority)
+	hipStream_t side[2] = {nullptr, nullptr};
+	hipEvent_t fork_ev = nullptr, join_ev[2] = {nullptr, nullptr};
 	int device = 0;
 	// host copies of static data
 	HostShapes hs;
@@ -1357,12 +1361,26 @@ void cp_world::step() {
 			              isl_eoff.p, isl_coff.p, isl_icoff.p, isl_serial.p};
 			launch_pgs_islands(b, cb[cur].c, J, H, S, P, applied.p, lin.p, s == 0, cfg.pgs_mode == CP_PGS_ISLAND_COLORED, stream);
 		} else {
+			// the largest class (and the global-memory one) on the world's
+			// stream, the two smaller classes beside it on the side streams
 			static const int thr[3] = {64, 128, 256};
-			for (int c = 0; c < 3; c++)
-				launch_pgs_colored_lds(b, cb[cur].c, cls_dev[c].p, (int)cls_lists[c].size(), cls_max[c], scene_body_off.p, edge_range.p,
-				                       e_order.p, color_off.p, ncolors.p, e_first.p, e_cnt.p, P, thr[c], stream);
+			bool fork = !cls_lists[0].empty() || !cls_lists[1].empty();
+			if (fork) {
+				HIPCHK(hipEventRecord(fork_ev, stream));
+				for (auto& x : side) HIPCHK(hipStreamWaitEvent(x, fork_ev, 0));
+			}
+			launch_pgs_colored_lds(b, cb[cur].c, cls_dev[2].p, (int)cls_lists[2].size(), cls_max[2], scene_body_off.p, edge_range.p,
+			                       e_order.p, color_off.p, ncolors.p, e_first.p, e_cnt.p, P, thr[2], stream);
 			if (!cls_lists[3].empty())
 				launch_pgs_colored_lds_global(b, P);
+			for (int c = 0; c < 2 && fork; c++)
+				launch_pgs_colored_lds(b, cb[cur].c, cls_dev[c].p, (int)cls_lists[c].size(), cls_max[c], scene_body_off.p, edge_range.p,
+				                       e_order.p, color_off.p, ncolors.p, e_first.p, e_cnt.p, P, thr[c], side[c]);
+			if (fork)
+				for (int c = 0; c < 2; c++) {
+					HIPCHK(hipEventRecord(join_ev[c], side[c]));
+					HIPCHK(hipStreamWaitEvent(stream, join_ev[c], 0));
+				}
 		}
 		if (tev) {
 			HIPCHK(hipEventRecord(pgs_ev[2 * npgs + 1], stream));
@@ -1475,6 +1493,9 @@ int cp_world_create(const cp_config* cfg, cp_world** out) {
 	int st = guard(w, [&] {
 		HIPCHK(hipGetDevice(&w->device));
 		HIPCHK(hipStreamCreate(&w->stream));
+		for (auto& x : w->side) HIPCHK(hipStreamCreate(&x));
+		HIPCHK(hipEventCreateWithFlags(&w->fork_ev, hipEventDisableTiming));
+		for (auto& e : w->join_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
 		for (auto& e : w->ev) HIPCHK(hipEventCreate(&e));
 		for (auto& e : w->pgs_ev) HIPCHK(hipEventCreate(&e));
 	});
@@ -1493,6 +1514,14 @@ int cp_world_destroy(cp_world* w) {
 	w->free_all();
 	for (auto& e : w->ev) (void)hipEventDestroy(e);
 	for (auto& e : w->pgs_ev) (void)hipEventDestroy(e);
+	for (auto& x : w->side)
+		if (x) {
+			(void)hipStreamSynchronize(x);
+			(void)hipStreamDestroy(x);
+		}
+	if (w->fork_ev) (void)hipEventDestroy(w->fork_ev);
+	for (auto& e : w->join_ev)
+		if (e) (void)hipEventDestroy(e);
 	(void)hipStreamDestroy(w->stream);
 	delete w;
 	return CP_OK;

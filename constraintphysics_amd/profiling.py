@@ -29,8 +29,10 @@ def _is_pgs(name):
     return any(p in name for p in PGS_KERNELS)
 
 
-def pgs_groups(rows, value):
-    """{step: [per-substep sum of value(row) over PGS dispatches]}."""
+def pgs_groups(rows, value=None):
+    """{step: [per substep: the sum of value(row) over the PGS dispatches, or
+    with value=None the span from the first PGS start to the last PGS end (the
+    size classes run concurrently)]}."""
     step, cur = 0, None
     out = collections.defaultdict(list)
     for r in rows:
@@ -38,11 +40,16 @@ def pgs_groups(rows, value):
         if "k_gravity" in n:
             step += 1
         if "k_build_rows" in n:
-            cur = 0.0
+            cur = []
         elif _is_pgs(n) and cur is not None:
-            cur += value(r)
+            cur.append(r)
         elif "k_rows_writeback" in n and cur is not None:
-            out[step].append(cur)
+            if value is None:
+                t0 = min(int(x["Start_Timestamp"]) for x in cur) if cur else 0
+                t1 = max(int(x["End_Timestamp"]) for x in cur) if cur else 0
+                out[step].append((t1 - t0) * 1e-6)
+            else:
+                out[step].append(sum(value(x) for x in cur))
             cur = None
     return out
 
@@ -55,10 +62,11 @@ def summarize(d, first_step=241, steps=20, fetch_factor=2.0, write_factor=1.0, c
     lines = []
     win = range(first_step, first_step + steps)
     tr = _rows(os.path.join(d, f"{prefix}_kernel_trace.csv"))
-    g = pgs_groups(tr, _dur_ms)
+    g = pgs_groups(tr)
     vals = [v for s in win for v in g.get(s, [])]
     pgs_ms = sum(vals) / max(1, len(vals))
-    lines.append(f"window steps {first_step}..{first_step + steps - 1}: {len(vals)} PGS solves, avg {pgs_ms:.4f} ms per solve")
+    lines.append(f"window steps {first_step}..{first_step + steps - 1}: {len(vals)} PGS solves, avg {pgs_ms:.4f} ms per solve "
+                 "(first PGS dispatch start to last end)")
     per = collections.defaultdict(list)
     step = 0
     for r in tr:
