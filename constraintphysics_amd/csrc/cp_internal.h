@@ -209,6 +209,16 @@ struct NewPts {
 
 #define CP_MAXC 128  // colours per scene (plus one serial bucket)
 
+// Coloured edge slots: one descriptor per position of a scene's colour-sorted
+// edge list, so a sweep reads one 16-byte record per edge instead of a chain
+// of dependent loads.  x = first cache index, y = contact count (low 16 bits)
+// | ED_LONG (more than 32 contacts: live flags are read per contact) | dynamic
+// flags of the two bodies, z / w = body indices; lmask bit j = contact x + j
+// is live.
+#define ED_LONG (1 << 29)
+#define ED_DYN_A (1 << 30)
+#define ED_DYN_B (1u << 31)
+
 // Launchers (defined in the kernel translation units).
 void launch_gravity(const Bodies& b, const Params& p, hipStream_t s);
 void launch_refresh_geometry(const Bodies& b, const Shapes& sh, int only_moved, hipStream_t s);
@@ -264,6 +274,8 @@ void launch_age(const Contacts& C, int n, hipStream_t s);
 void launch_edge_heads(const Contacts& C, int n, int* head, hipStream_t s);
 void launch_edge_build(const Contacts& C, int n, const int* head, const int* eidx, uint64_t* ekey, int* efirst, int* ecnt, int* elive,
                        hipStream_t s);
+void launch_edge_slots(const Bodies& b, const Contacts& C, const int* order, int nslots, const int* efirst, const int* ecnt, int4* desc,
+                       uint32_t* lmask, hipStream_t s);
 void launch_color(const Bodies& b, const int* scene_body_off, int nscenes, const uint64_t* ekey, const int* elive, int nedges,
                   int* ecolor, int* order, int* color_off, int* ncolors, int* edge_range, hipStream_t s);
 void launch_build_rows(const Bodies& b, const Contacts& C, int nc, const Params& P, hipStream_t s);
@@ -276,6 +288,18 @@ void launch_hsys_factor(const Joints& J, const HSysDev& H, double cfm, hipStream
 void launch_pgs_colored_lds(const Bodies& b, const Contacts& C, const int* scene_list, int nlist, int max_bodies, const int* scene_body_off,
                             const int* edge_range, const int* order, const int* color_off, const int* ncolors, const int* efirst,
                             const int* ecnt, const Params& P, int threads, hipStream_t s);
+void launch_rows_writeback(const Contacts& C, int nc, const Params& P, hipStream_t s);
+void launch_pgs_residual(const Bodies& b, const Contacts& C, int nc, const Params& P, double* acc, hipStream_t s);
+// rows: (index, kind) pairs, kind 0 contact normal, 1 friction, 2 joint;
+// isl_sys/isl_sys_off: holonomic systems per island; lin: [6][n] scratch.
+void launch_pgs_islands(const Bodies& b, const Contacts& C, const Joints& J, const HSysDev& H, const IslandSched& S, const Params& P,
+                        int* applied, double* lin, int first, bool colored, hipStream_t s);
+void launch_pgs_colored(const Bodies& b, const Contacts& C, const int* scene_body_off, int nscenes, const int* edge_range, const int4* desc,
+                        const uint32_t* lmask, const int* color_off, const int* ncolors, const Params& P, int threads, hipStream_t s,
+                        const int* scene_list, int nlist);
+void launch_pgs_colored_lds(const Bodies& b, const Contacts& C, const int* scene_list, int nlist, int max_bodies, const int* scene_body_off,
+                            const int* edge_range, const int4* desc, const uint32_t* lmask, const int* color_off, const int* ncolors,
+                            const Params& P, int threads, hipStream_t s);
 void launch_rows_writeback(const Contacts& C, int nc, const Params& P, hipStream_t s);
 void launch_pgs_residual(const Bodies& b, const Contacts& C, int nc, const Params& P, double* acc, hipStream_t s);
 // rows: (index, kind) pairs, kind 0 contact normal, 1 friction, 2 joint;

@@ -266,7 +266,30 @@ __global__ __launch_bounds__(64) void k_color(Bodies b, const int* scene_body_of
 	ncolors[s] = maxc + 1;
 }
 
+// Descriptor of each colour-sorted edge slot (order[x] = -1: unused slot).
+__global__ void k_edge_slots(Bodies b, Contacts C, const int* order, int nslots, const int* efirst, const int* ecnt, int4* desc,
+                             uint32_t* lmask) {
+	int x = blockIdx.x * blockDim.x + threadIdx.x;
+	if (x >= nslots) return;
+	int e = order[x];
+	if (e < 0) return;
+	int k0 = efirst[e], cnt = ecnt[e];
+	int i1 = (int)(C.key[k0] >> 32), i2 = (int)(C.key[k0] & 0xFFFFFFFFu);
+	uint32_t m = 0;
+	for (int j = 0; j < cnt && j < 32; j++)
+		if (C.live[k0 + j]) m |= 1u << j;
+	uint32_t y = (uint32_t)(cnt & 0xFFFF) | (cnt > 32 ? (uint32_t)ED_LONG : 0u) | (b.movement[i1] == 0 ? (uint32_t)ED_DYN_A : 0u) |
+	             (b.movement[i2] == 0 ? ED_DYN_B : 0u);
+	desc[x] = make_int4(k0, (int)y, i1, i2);
+	lmask[x] = m;
+}
+
 static inline int nblk(int n, int t) { return (n + t - 1) / t; }
+
+void launch_edge_slots(const Bodies& b, const Contacts& C, const int* order, int nslots, const int* efirst, const int* ecnt, int4* desc,
+                       uint32_t* lmask, hipStream_t s) {
+	if (nslots) hipLaunchKernelGGL(k_edge_slots, dim3(nblk(nslots, 256)), dim3(256), 0, s, b, C, order, nslots, efirst, ecnt, desc, lmask);
+}
 
 void launch_np_gather(const Manifolds& M, int mcap, const uint64_t* pairs, int npairs, const int* off, const NewPts& np, int cap,
                       hipStream_t s) {

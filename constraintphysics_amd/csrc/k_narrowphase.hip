@@ -673,6 +673,13 @@ __device__ int cyl_contact_area(const Cyl& c, V3 n, V3 u, V3 w, double cos_tol, 
 
 // Gauss-arc pair direction (arc loop body of CollisionDetect.cpp:1617-1650,
 // :1021-1054); false when pruned.
+// The acceptance tests of arc_axis alone (no axis), written as negations of
+// the reference's `continue` conditions so NaNs take the same branch.
+__device__ __forceinline__ bool arc_test(V3 a1, V3 a2, V3 aa, V3 ap, V3 b1, V3 b2, V3 bp) {
+	if (dot(aa, b1) + dot(aa, b2) <= 0) return false;
+	if (dot(ap, b1) * dot(ap, b2) > 0 || dot(bp, a1) * dot(bp, a2) > 0) return false;
+	return !(mag2(cross(ap, bp)) == 0);
+}
 __device__ bool arc_axis(V3 a1, V3 a2, V3 b1, V3 b2, V3* out) {
 	V3 aa = a1 + a2;
 	if (dot(aa, b1) + dot(aa, b2) <= 0) return false;
@@ -1021,6 +1028,22 @@ __device__ void stage_hull(Hull& h, double* spts, double* snrm, int lane) {
 	h.wn = snrm;
 }
 
+// Copies a hull's Gauss arc list (vertex index pairs) into LDS when it fits.
+#define ST_ARC 128
+__device__ void stage_arcs(Hull& h, int* sarc, int lane) {
+	if (h.na > ST_ARC) return;
+	for (int k = lane; k < 2 * h.na; k += 64) sarc[k] = h.arcs[k];
+	wave_fence();
+	h.arcs = sarc;
+}
+
+// Copies a cylinder's 14 world Gauss vertices into LDS.
+__device__ void stage_cyl(Cyl& c, double* sgv, int lane) {
+	if (lane < 42) sgv[lane] = c.gv[lane];
+	wave_fence();
+	c.gv = sgv;
+}
+
 // Arc-pair pruning of SAT_PolyPoly (CollisionDetect.cpp:750-770).
 __device__ __forceinline__ bool hh_prune(const Hull& A, const Hull& B, int t) {
 	if (t < A.nf) return !A.red[t];
@@ -1035,8 +1058,81 @@ __device__ __forceinline__ bool hh_prune(const Hull& A, const Hull& B, int t) {
 struct WaveLds {
 	CEP cep[4 * MAXP];
 	int cand[64 * CAND_K];
+	int arca[2 * ST_ARC], arcb[2 * ST_ARC];
 	double pa[3 * ST_PTS], pb[3 * ST_PTS], na[3 * ST_NRM], nb[3 * ST_NRM];
 };
+
+// SAT_PolyPoly as a wave (CollisionDetect.cpp:717-788), same result as
+// wave_sat_compact over eval_axis: the face axes of A then B, 64 at a time;
+// then the arc pairs with each lane holding one or two of B's arcs in
+// registers while A's arcs stream past uniformly (i1 outer, i2 inner, as the
+// reference loops), surviving pairs compacted in axis order and evaluated 64
+// at a time.  Any separating axis ends the test (returns false); otherwise
+// *bi is the lowest-index axis of minimum depth.  Needs B.na <= 128.
+__device__ bool wave_sat_hh(const Hull& A, const Hull& B, int lane, int* cand, int* bi_out) {
+	double bp = __builtin_inf();
+	int bi = 0x7FFFFFFF;
+	const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+	auto consider = [&](int t) {
+		bool v;
+		Check x = eval_axis(A, B, t, &v);
+		if (!v) return false;
+		if (x.pen < bp || (x.pen == bp && t < bi)) {
+			bp = x.pen;
+			bi = t;
+		}
+		return x.pen < 0;
+	};
+	const int nfa = A.nf + B.nf;
+	for (int base = 0; base < nfa; base += 64) {
+		int t = base + lane;
+		bool sep = false;
+		if (t < nfa && !(t < A.nf ? A.red[t] : B.red[t - A.nf])) sep = consider(t);
+		if (__ballot(sep) != 0) return false;
+	}
+	V3 b1[2], b2[2], bq[2];
+	const int nh = (B.na + 63) >> 6;
+#pragma unroll
+	for (int h = 0; h < 2; h++) {
+		int i2 = lane + 64 * h;
+		if (h < nh && i2 < B.na) {
+			b1[h] = -ldv(B.wn, B.arcs[2 * i2]);
+			b2[h] = -ldv(B.wn, B.arcs[2 * i2 + 1]);
+			bq[h] = cross(b1[h], b2[h]);
+		}
+	}
+	int nc = 0;
+	for (int i1 = 0; i1 < A.na; i1++) {
+		V3 a1 = ldv(A.wn, A.arcs[2 * i1]), a2 = ldv(A.wn, A.arcs[2 * i1 + 1]);
+		V3 aa = a1 + a2, ap = cross(a1, a2);
+#pragma unroll
+		for (int h = 0; h < 2; h++) {
+			if (h >= nh) break;
+			int i2 = lane + 64 * h;
+			bool v = i2 < B.na && arc_test(a1, a2, aa, ap, b1[h], b2[h], bq[h]);
+			uint64_t m = __ballot(v);
+			if (v) cand[nc + __popcll(m & below)] = nfa + i1 * B.na + i2;
+			nc += __popcll(m);
+		}
+		if (nc > 64 * CAND_K - 128 || (i1 == A.na - 1 && nc > 0)) {
+			wave_fence();
+			for (int c0 = 0; c0 < nc; c0 += 64) {
+				int idx = c0 + lane;
+				bool sep = idx < nc && consider(cand[idx]);
+				if (__ballot(sep) != 0) return false;
+			}
+			nc = 0;
+			wave_fence();
+		}
+	}
+	for (int off = 32; off >= 1; off >>= 1) {
+		double obp = __shfl_xor(bp, off);
+		int obi = __shfl_xor(bi, off);
+		if (obp < bp || (obp == bp && obi < bi)) { bp = obp; bi = obi; }
+	}
+	*bi_out = bi;
+	return bp > 0 && bi != 0x7FFFFFFF;  // PhysicsEngine.cpp:213: keep only max_pen_depth > 0
+}
 
 #define NP_WAVES 4
 // Hull-hull: SAT_PolyPoly (CollisionDetect.cpp:717-835).
@@ -1053,15 +1149,18 @@ __global__ __launch_bounds__(64 * NP_WAVES) void k_np_hull_hull(Bodies b, Shapes
 		Hull A = make_hull(b, sh, ia), B = make_hull(b, sh, ib);
 		stage_hull(A, L.pa, L.na, lane);
 		stage_hull(B, L.pb, L.nb, lane);
+		stage_arcs(A, L.arca, lane);
+		stage_arcs(B, L.arcb, lane);
 		int naxes = A.nf + B.nf + A.na * B.na;
 		int bi;
-		bool hit = wave_sat_compact(
-		    naxes, lane, L.cand, [&](int t) { return hh_prune(A, B, t); },
-		    [&](int t) {
-			    bool v;
-			    return eval_axis(A, B, t, &v);
-		    },
-		    &bi);
+		bool hit = B.na <= 128 ? wave_sat_hh(A, B, lane, L.cand, &bi)
+		                       : wave_sat_compact(
+		                             naxes, lane, L.cand, [&](int t) { return hh_prune(A, B, t); },
+		                             [&](int t) {
+			                             bool v;
+			                             return eval_axis(A, B, t, &v);
+		                             },
+		                             &bi);
 		if (hit && lane == 0) {
 			bool valid;
 			Check mp = eval_axis(A, B, bi, &valid);
@@ -1106,6 +1205,8 @@ __global__ __launch_bounds__(64 * NP_WAVES) void k_np_hull_cyl(Bodies b, Shapes 
 		Hull A = make_hull(b, sh, ip);
 		Cyl B = make_cyl(b, sh, ic);
 		stage_hull(A, L.pa, L.na, lane);
+		stage_arcs(A, L.arca, lane);
+		stage_cyl(B, L.nb, lane);
 		int naxes = A.nf + 1 + A.ne + A.np + A.na * 24;
 		int bi;
 		bool hit = wave_sat_compact(
@@ -1155,6 +1256,8 @@ __global__ __launch_bounds__(64 * NP_WAVES) void k_np_cyl_cyl(Bodies b, Shapes s
 		uint64_t key = pairs[slot];
 		int ia = (int)(key >> 32), ib = (int)(key & 0xFFFFFFFFu);
 		Cyl A = make_cyl(b, sh, ia), B = make_cyl(b, sh, ib);
+		stage_cyl(A, L.na, lane);
+		stage_cyl(B, L.nb, lane);
 		int bi;
 		bool hit = wave_sat_compact(
 		    35 + 24 * 24, lane, L.cand, [&](int t) { return cc_prune(A, B, t); },

@@ -671,17 +671,20 @@ __global__ __launch_bounds__(64) void k_pgs_islands(Bodies b, Contacts C, Joints
 // order; the colour index MAXC holds edges that did not fit a colour and runs
 // on thread 0 alone.  kind 0 warm start, 1 velocity sweep, 2 position sweep.
 template <class T, class D, class Make>
-__device__ __forceinline__ void colour_phase(Contacts& C, const int* order, int e0, int beg, int end, bool serial, const int* efirst,
-                                             const int* ecnt, int kind, Make make) {
+__device__ __forceinline__ void colour_phase(Contacts& C, const int4* desc, const uint32_t* lmask, int e0, int beg, int end, bool serial,
+                                             int kind, Make make) {
 	int stride = serial ? 1 : blockDim.x;
 	int start = serial ? (threadIdx.x == 0 ? beg : end) : beg + threadIdx.x;
 	for (int x = start; x < end; x += stride) {
-		int e = order[e0 + x];
-		int k0 = efirst[e], k1 = k0 + ecnt[e];
-		int i1 = (int)(C.key[k0] >> 32), i2 = (int)(C.key[k0] & 0xFFFFFFFFu);
-		D A = make(i1), B = make(i2);
-		for (int k = k0; k < k1; k++) {
-			if (!C.live[k]) continue;
+		int4 d = desc[e0 + x];
+		uint32_t lm = lmask[e0 + x];
+		const int cnt = d.y & 0xFFFF;
+		const bool lng = (d.y & ED_LONG) != 0;
+		D A = make(d.z, (d.y & ED_DYN_A) != 0), B = make(d.w, ((uint32_t)d.y & ED_DYN_B) != 0);
+		for (int j = 0; j < cnt; j++) {
+			int k = d.x + j;
+			bool live = lng ? C.live[k] != 0 : ((lm >> j) & 1u) != 0;
+			if (!live) continue;
 			contact_phase(rec<T>(C, k), A, B, kind);
 		}
 	}
@@ -690,8 +693,9 @@ __device__ __forceinline__ void colour_phase(Contacts& C, const int* order, int 
 // Global-memory variant (scenes whose velocity deltas do not fit LDS).  The
 // deltas live in b.dv viewed as [12][n] values of T.
 template <class T>
-__global__ __launch_bounds__(256) void k_pgs_colored(Bodies b, Contacts C, const int* scene_body_off, const int* edge_range, const int* order,
-                              const int* color_off, const int* ncolors, const int* efirst, const int* ecnt, Params P, const int* scene_list) {
+__global__ __launch_bounds__(256) void k_pgs_colored(Bodies b, Contacts C, const int* scene_body_off, const int* edge_range, const int4* desc,
+                                                     const uint32_t* lmask, const int* color_off, const int* ncolors, Params P,
+                                                     const int* scene_list) {
 	const int s = scene_list ? scene_list[blockIdx.x] : blockIdx.x;
 	const int b0 = scene_body_off[s], b1 = scene_body_off[s + 1];
 	const int N = b.n;
@@ -702,12 +706,12 @@ __global__ __launch_bounds__(256) void k_pgs_colored(Bodies b, Contacts C, const
 	const int e0 = edge_range[2 * s];
 	const int* co = color_off + (size_t)s * (MAXC + 2);
 	const int nc = ncolors[s];
-	auto make = [&](int i) { return DvGlobal<T>{dv, i, N, b.movement[i] == 0}; };
+	auto make = [&](int i, bool dyn) { return DvGlobal<T>{dv, i, N, dyn}; };
 	int total = 1 + P.vel_iters + P.pos_iters;
 	for (int ph = 0; ph < total; ph++) {
 		int kind = ph == 0 ? 0 : (ph <= P.vel_iters ? 1 : 2);
 		for (int c = 0; c < nc; c++) {
-			colour_phase<T, DvGlobal<T>>(C, order, e0, co[c], co[c + 1], c == MAXC, efirst, ecnt, kind, make);
+			colour_phase<T, DvGlobal<T>>(C, desc, lmask, e0, co[c], co[c + 1], c == MAXC, kind, make);
 			__syncthreads();
 		}
 	}
@@ -720,7 +724,8 @@ __global__ __launch_bounds__(256) void k_pgs_colored(Bodies b, Contacts C, const
 // One workgroup per scene of the launch's size class (scene_list).
 template <class T>
 __global__ __launch_bounds__(256) void k_pgs_colored_lds(Bodies b, Contacts C, const int* scene_list, const int* scene_body_off, const int* edge_range,
-                                  const int* order, const int* color_off, const int* ncolors, const int* efirst, const int* ecnt, Params P) {
+                                                         const int4* desc, const uint32_t* lmask, const int* color_off, const int* ncolors,
+                                                         Params P) {
 	extern __shared__ double lds_raw[];
 	T* sdv = reinterpret_cast<T*>(lds_raw);
 	const int s = scene_list[blockIdx.x];
@@ -730,7 +735,7 @@ __global__ __launch_bounds__(256) void k_pgs_colored_lds(Bodies b, Contacts C, c
 	const int e0 = edge_range[2 * s];
 	const int* co = color_off + (size_t)s * (MAXC + 2);
 	const int nc = ncolors[s];
-	auto make = [&](int i) { return DvLds<T>{sdv, i - b0, nbs, b.movement[i] == 0}; };
+	auto make = [&](int i, bool dyn) { return DvLds<T>{sdv, i - b0, nbs, dyn}; };
 	for (int half = 0; half < 2; half++) {
 		for (int x = threadIdx.x; x < 6 * nbs; x += blockDim.x) sdv[x] = 0;
 		__syncthreads();
@@ -739,7 +744,7 @@ __global__ __launch_bounds__(256) void k_pgs_colored_lds(Bodies b, Contacts C, c
 		for (int ph = ph0; ph < ph1; ph++) {
 			int kind = ph == 0 ? 0 : (ph <= P.vel_iters ? 1 : 2);
 			for (int c = 0; c < nc; c++) {
-				colour_phase<T, DvLds<T>>(C, order, e0, co[c], co[c + 1], c == MAXC, efirst, ecnt, kind, make);
+				colour_phase<T, DvLds<T>>(C, desc, lmask, e0, co[c], co[c + 1], c == MAXC, kind, make);
 				__syncthreads();
 			}
 		}
@@ -841,21 +846,21 @@ void launch_pgs_islands(const Bodies& b, const Contacts& C, const Joints& J, con
 	if (colored) hipLaunchKernelGGL(k_pgs_islands, dim3(S.n), dim3(64), 0, s, b, C, J, H, S, P, applied, lin, first);
 	else hipLaunchKernelGGL(k_pgs_ordered, dim3(nblk(S.n, 64)), dim3(64), 0, s, b, C, J, H, S, P, applied, lin, first);
 }
-void launch_pgs_colored(const Bodies& b, const Contacts& C, const int* scene_body_off, int nscenes, const int* edge_range, const int* order,
-                        const int* color_off, const int* ncolors, const int* efirst, const int* ecnt, const Params& P, int threads,
-                        hipStream_t s, const int* scene_list, int nlist) {
+void launch_pgs_colored(const Bodies& b, const Contacts& C, const int* scene_body_off, int nscenes, const int* edge_range, const int4* desc,
+                        const uint32_t* lmask, const int* color_off, const int* ncolors, const Params& P, int threads, hipStream_t s,
+                        const int* scene_list, int nlist) {
 	int nb = scene_list ? nlist : nscenes;
 	if (!nb) return;
 	if (P.precision == CP_PREC_SPLIT)
-		hipLaunchKernelGGL(k_pgs_colored<float>, dim3(nb), dim3(threads), 0, s, b, C, scene_body_off, edge_range, order, color_off, ncolors,
-		                   efirst, ecnt, P, scene_list);
+		hipLaunchKernelGGL(k_pgs_colored<float>, dim3(nb), dim3(threads), 0, s, b, C, scene_body_off, edge_range, desc, lmask, color_off,
+		                   ncolors, P, scene_list);
 	else
-		hipLaunchKernelGGL(k_pgs_colored<double>, dim3(nb), dim3(threads), 0, s, b, C, scene_body_off, edge_range, order, color_off,
-		                   ncolors, efirst, ecnt, P, scene_list);
+		hipLaunchKernelGGL(k_pgs_colored<double>, dim3(nb), dim3(threads), 0, s, b, C, scene_body_off, edge_range, desc, lmask, color_off,
+		                   ncolors, P, scene_list);
 }
 void launch_pgs_colored_lds(const Bodies& b, const Contacts& C, const int* scene_list, int nlist, int max_bodies, const int* scene_body_off,
-                            const int* edge_range, const int* order, const int* color_off, const int* ncolors, const int* efirst,
-                            const int* ecnt, const Params& P, int threads, hipStream_t s) {
+                            const int* edge_range, const int4* desc, const uint32_t* lmask, const int* color_off, const int* ncolors,
+                            const Params& P, int threads, hipStream_t s) {
 	if (!nlist) return;
 	static bool attr = false;
 	if (!attr) {
@@ -865,10 +870,10 @@ void launch_pgs_colored_lds(const Bodies& b, const Contacts& C, const int* scene
 	}
 	if (P.precision == CP_PREC_SPLIT)
 		hipLaunchKernelGGL(k_pgs_colored_lds<float>, dim3(nlist), dim3(threads), (size_t)6 * max_bodies * sizeof(float), s, b, C, scene_list,
-		                   scene_body_off, edge_range, order, color_off, ncolors, efirst, ecnt, P);
+		                   scene_body_off, edge_range, desc, lmask, color_off, ncolors, P);
 	else
 		hipLaunchKernelGGL(k_pgs_colored_lds<double>, dim3(nlist), dim3(threads), (size_t)6 * max_bodies * sizeof(double), s, b, C,
-		                   scene_list, scene_body_off, edge_range, order, color_off, ncolors, efirst, ecnt, P);
+		                   scene_list, scene_body_off, edge_range, desc, lmask, color_off, ncolors, P);
 }
 void launch_rows_writeback(const Contacts& C, int nc, const Params& P, hipStream_t s) {
 	if (!nc) return;

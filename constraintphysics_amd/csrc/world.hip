@@ -184,6 +184,8 @@ struct cp_world {
 	uint64_t seq = 1ull << 32;
 	DArr<int> e_head, e_idx, e_first, e_cnt, e_live, e_color, e_order, color_off, ncolors, edge_range, applied, isl_rows, isl_off;
 	DArr<uint64_t> e_key;
+	DArr<int4> e_desc;
+	DArr<uint32_t> e_lmask;
 	int nedges = 0;
 	int live_contacts = 0;
 	DArr<double> resid_acc;
@@ -391,6 +393,8 @@ void cp_world::free_all() {
 	resid_acc.free_();
 	DArr<uint64_t>* u64s[] = {&bp_keys, &bp_keys2, &pairs, &pairs2, &nk, &e_key};
 	for (auto* a : u64s) a->free_();
+	e_desc.free_();
+	e_lmask.free_();
 	DArr<int>* jints[] = {&dj_type, &dj_a, &dj_b, &dj_sys, &dh_nc, &dh_ord_off, &dh_order, &dh_vloc, &dh_tab_off, &dh_table, &dh_degree,
 	                      &dh_buf_off, &dh_dbuf_off, &isl_sys, &isl_sys_off, &isl_jlin, &isl_eoff, &isl_coff, &isl_icoff,
 	                      &isl_serial};
@@ -1298,8 +1302,8 @@ void cp_world::record_colored_schedule() {
 
 // Scenes too large for LDS: the global-memory kernel over those scenes only.
 void cp_world::launch_pgs_colored_lds_global(const Bodies& b, const Params& P) {
-	launch_pgs_colored(b, cb[cur].c, scene_body_off.p, nscenes, edge_range.p, e_order.p, color_off.p, ncolors.p, e_first.p, e_cnt.p, P,
-	                   cfg.solver_threads, stream, cls_dev[3].p, (int)cls_lists[3].size());
+	launch_pgs_colored(b, cb[cur].c, scene_body_off.p, nscenes, edge_range.p, e_desc.p, e_lmask.p, color_off.p, ncolors.p, P, cfg.solver_threads,
+	                   stream, cls_dev[3].p, (int)cls_lists[3].size());
 }
 
 void cp_world::step() {
@@ -1340,9 +1344,13 @@ void cp_world::step() {
 		color_off.reserve((size_t)nscenes * (CP_MAXC + 2));
 		ncolors.reserve(nscenes);
 		edge_range.reserve(2 * (size_t)nscenes);
+		e_desc.reserve(nedges);
+		e_lmask.reserve(nedges);
+		if (nedges) HIPCHK(hipMemsetAsync(e_order.p, 0xFF, (size_t)nedges * sizeof(int), stream));  // unused slots read -1
 		launch_color(b, scene_body_off.p, serving ? live_scenes : nscenes, e_key.p, e_live.p, nedges, e_color.p, e_order.p, color_off.p,
 		             ncolors.p, edge_range.p,
 		             stream);
+		launch_edge_slots(b, cb[cur].c, e_order.p, nedges, e_first.p, e_cnt.p, e_desc.p, e_lmask.p, stream);
 		if (cfg.diagnostics & 2) record_colored_schedule();
 	}
 	if (timing) HIPCHK(hipEventRecord(ev[3], stream));
@@ -1370,12 +1378,12 @@ void cp_world::step() {
 				for (auto& x : side) HIPCHK(hipStreamWaitEvent(x, fork_ev, 0));
 			}
 			launch_pgs_colored_lds(b, cb[cur].c, cls_dev[2].p, (int)cls_lists[2].size(), cls_max[2], scene_body_off.p, edge_range.p,
-			                       e_order.p, color_off.p, ncolors.p, e_first.p, e_cnt.p, P, thr[2], stream);
+			                       e_desc.p, e_lmask.p, color_off.p, ncolors.p, P, thr[2], stream);
 			if (!cls_lists[3].empty())
 				launch_pgs_colored_lds_global(b, P);
 			for (int c = 0; c < 2 && fork; c++)
 				launch_pgs_colored_lds(b, cb[cur].c, cls_dev[c].p, (int)cls_lists[c].size(), cls_max[c], scene_body_off.p, edge_range.p,
-				                       e_order.p, color_off.p, ncolors.p, e_first.p, e_cnt.p, P, thr[c], side[c]);
+				                       e_desc.p, e_lmask.p, color_off.p, ncolors.p, P, thr[c], side[c]);
 			if (fork)
 				for (int c = 0; c < 2; c++) {
 					HIPCHK(hipEventRecord(join_ev[c], side[c]));
