@@ -215,6 +215,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-c2", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default PGS size classes and workgroup widths")
     ap.add_argument("--c2-only", action="store_true", help="run only the holonomic-chains-on-terrain leg and print its JSON")
     ap.add_argument("--serve-requests", type=int, default=6000, help="requests of the serving leg (Poisson arrivals at 2000/s)")
     ap.add_argument("--serve-capacity", type=int, default=1600000, help="body slots of the serving world (its HBM budget)")
@@ -237,10 +238,23 @@ def main():
     t_build = time.perf_counter() - t_build
     n_dyn = int(info["dynamic_bodies"])
     print(f"[bench] built {args.scenes} scenes, {n_dyn} dynamic bodies in {t_build:.1f}s", file=sys.stderr, flush=True)
-    for i in range(args.warmup):
+    # the autotuner runs inside the warm-up (its trials are steps of the same
+    # run), so the timed window starts at the same step with or without it
+    tuned = None
+    i = 0
+    while i < args.warmup:
+        if tuned is None and not args.no_tune and i >= max(0, args.warmup - 80):
+            from constraintphysics_amd import autotune
+            tuned = autotune.tune(w, log=None)
+            i += tuned["steps"]
+            print(f"[bench] autotune: classes {tuned['pgs_class_bodies']} threads {tuned['pgs_class_threads']}, PGS "
+                  f"{tuned['default_pgs_ms']:.3f} -> {tuned['pgs_ms']:.3f} ms/step over {tuned['trials']} trials", file=sys.stderr,
+                  flush=True)
+            continue
         w.step(1)
-        if (i + 1) % 40 == 0:
-            print(f"[bench] warmup {i + 1}/{args.warmup}", file=sys.stderr, flush=True)
+        i += 1
+        if i % 40 == 0:
+            print(f"[bench] warmup {i}/{args.warmup}", file=sys.stderr, flush=True)
     # settled state at the start of the timed window, for the plain and CPU legs
     snap_state = w.state()
     snap_contacts = w.contacts()
@@ -356,6 +370,7 @@ def main():
                                           "contact impulses / body velocities"},
         "plain_gpu_path": plain,
         "speedup_vs_plain": round(value / plain["value"], 2) if plain else None,
+        "autotune": tuned,
         "serving": served,
         "holonomic_chains_on_terrain": c2,
         "serving_speedup_vs_plain": round(served["value"] / plain["value"], 2) if (plain and served) else None,

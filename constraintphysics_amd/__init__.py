@@ -38,6 +38,8 @@ class Config(ctypes.Structure):
         ("diagnostics", ctypes.c_int),
         ("defect_fixes", ctypes.c_int),
         ("precision", ctypes.c_int),
+        ("pgs_class_bodies", ctypes.c_int * 3),
+        ("pgs_class_threads", ctypes.c_int * 3),
     ]
 
 
@@ -141,11 +143,17 @@ def default_config(**kw):
     c = Config()
     lib().cp_config_default(ctypes.byref(c))
     for k, v in kw.items():
-        if k == "gravity":
-            c.gravity = (ctypes.c_double * 3)(*v)
-        else:
-            setattr(c, k, v)
+        _set_field(c, k, v)
     return c
+
+
+def _set_field(c, k, v):
+    if k == "gravity":
+        c.gravity = (ctypes.c_double * 3)(*v)
+    elif k in ("pgs_class_bodies", "pgs_class_threads"):
+        setattr(c, k, (ctypes.c_int * 3)(*[int(x) for x in v]))
+    else:
+        setattr(c, k, v)
 
 
 def reference_node_order(n):
@@ -232,7 +240,7 @@ class World:
 
     def set_config(self, **kw):
         for k, v in kw.items():
-            setattr(self.cfg, k, v)
+            _set_field(self.cfg, k, v)
         _chk(self.L.cp_world_set_config(self.w, ctypes.byref(self.cfg)), "cp_world_set_config")
 
     # ---- shapes / bodies

@@ -190,6 +190,9 @@ struct cp_world {
 	int live_contacts = 0;
 	DArr<double> resid_acc;
 	std::vector<std::vector<int>> cls_lists;
+	std::vector<std::pair<int, int>> cls_sizes;  // (scene, bodies) the classes are built from
+	int cls_built[3] = {0, 0, 0};                 // class bounds the lists were built with
+	void build_classes();
 	int cls_max[4] = {0, 0, 0, 0};
 	DArr<int> cls_dev[4];
 	double last_resid[4] = {0, 0, 0, 0};
@@ -466,20 +469,9 @@ void cp_world::upload() {
 		for (int i = sbo[s]; i < sbo[s + 1]; i++) sob[i] = s;
 	up(scene_body_off, sbo);
 	up(scene_of_body, sob);
-	// PGS size classes: scenes whose velocity deltas fit in LDS run the LDS
-	// kernel (largest first within a class); larger scenes use global memory.
-	cls_lists.clear();
-	cls_lists.resize(4);
-	for (int s = 0; s < nscenes; s++) {
-		int m = sbo[s + 1] - sbo[s];
-		int c = m <= 192 ? 0 : m <= 768 ? 1 : m <= 3072 ? 2 : 3;
-		cls_lists[c].push_back(s);
-	}
-	for (int c = 0; c < 4; c++) {
-		std::sort(cls_lists[c].begin(), cls_lists[c].end(), [&](int a, int b2) { return sbo[a + 1] - sbo[a] > sbo[b2 + 1] - sbo[b2]; });
-		cls_max[c] = cls_lists[c].empty() ? 0 : sbo[cls_lists[c][0] + 1] - sbo[cls_lists[c][0]];
-		up(cls_dev[c], cls_lists[c]);
-	}
+	cls_sizes.clear();
+	for (int s = 0; s < nscenes; s++) cls_sizes.push_back({s, sbo[s + 1] - sbo[s]});
+	build_classes();
 	DArr<double>* per3[] = {&com, &vel, &angv, &pvel, &pang, &prev_com};
 	for (auto* a : per3) {
 		a->reserve(3 * (size_t)n);
@@ -896,19 +888,32 @@ void cp_world::serving_commit() {
 			live_n = ranges[q].start + ranges[q].len;
 			live_scenes = q + 1;
 		}
-	cls_lists.assign(4, std::vector<int>());
-	for (int q = 0; q < nscenes; q++) {
-		if (ranges[q].handle < 0) continue;
-		int m = ranges[q].len;
-		cls_lists[m <= 192 ? 0 : m <= 768 ? 1 : m <= 3072 ? 2 : 3].push_back(q);
-	}
-	for (int c = 0; c < 4; c++) {
-		std::sort(cls_lists[c].begin(), cls_lists[c].end(), [&](int a, int b2) { return ranges[a].len > ranges[b2].len; });
-		cls_max[c] = cls_lists[c].empty() ? 0 : ranges[cls_lists[c][0]].len;
-		up(cls_dev[c], cls_lists[c]);
-	}
+	cls_sizes.clear();
+	for (int q = 0; q < nscenes; q++)
+		if (ranges[q].handle >= 0) cls_sizes.push_back({q, ranges[q].len});
+	build_classes();
 	HIPCHK(hipStreamSynchronize(stream));
 	serving_dirty = false;
+}
+
+// PGS size classes: scenes whose velocity deltas fit in LDS run the LDS
+// kernel of their class (largest first within a class); larger scenes use
+// global memory.
+void cp_world::build_classes() {
+	const int* lim = cfg.pgs_class_bodies;
+	cls_lists.assign(4, std::vector<int>());
+	std::vector<int> size_of(nscenes + 1, 0);
+	for (auto& q : cls_sizes) {
+		int m = q.second;
+		cls_lists[m <= lim[0] ? 0 : m <= lim[1] ? 1 : m <= lim[2] ? 2 : 3].push_back(q.first);
+		size_of[q.first] = m;
+	}
+	for (int c = 0; c < 4; c++) {
+		std::sort(cls_lists[c].begin(), cls_lists[c].end(), [&](int a, int b2) { return size_of[a] > size_of[b2]; });
+		cls_max[c] = cls_lists[c].empty() ? 0 : size_of[cls_lists[c][0]];
+		up(cls_dev[c], cls_lists[c]);
+	}
+	for (int c = 0; c < 3; c++) cls_built[c] = lim[c];
 }
 
 void cp_world::scan(const int* in, int* out, int n) {
@@ -1371,7 +1376,7 @@ void cp_world::step() {
 		} else {
 			// the largest class (and the global-memory one) on the world's
 			// stream, the two smaller classes beside it on the side streams
-			static const int thr[3] = {64, 128, 256};
+			const int* thr = cfg.pgs_class_threads;
 			bool fork = !cls_lists[0].empty() || !cls_lists[1].empty();
 			if (fork) {
 				HIPCHK(hipEventRecord(fork_ev, stream));
@@ -1470,6 +1475,12 @@ void cp_config_default(cp_config* c) {
 	c->solver_threads = 256;
 	c->diagnostics = 0;
 	c->defect_fixes = 1;
+	c->pgs_class_bodies[0] = 192;
+	c->pgs_class_bodies[1] = 768;
+	c->pgs_class_bodies[2] = 3072;
+	c->pgs_class_threads[0] = 64;
+	c->pgs_class_threads[1] = 128;
+	c->pgs_class_threads[2] = 256;
 	c->precision = 0;
 }
 
@@ -1485,10 +1496,14 @@ const char* cp_status_string(int s) {
 	return "unknown";
 }
 
+static bool valid_threads(int t) { return t >= 64 && t <= 256 && (t % 64) == 0; }
 static bool valid_cfg(const cp_config* c) {
-	return c && c->dt > 0 && c->substeps >= 1 && c->vel_iters >= 0 && c->pos_iters >= 0 && c->gyro_ticks >= 1 && c->solver_threads >= 64 &&
-	       c->solver_threads <= 256 && (c->solver_threads % 64) == 0 && (c->pgs_mode >= 0 && c->pgs_mode <= 2) &&
-	       (c->precision == CP_PREC_FP64 || c->precision == CP_PREC_SPLIT);
+	if (!c) return false;
+	const int* lim = c->pgs_class_bodies;
+	bool classes = lim[0] >= 1 && lim[0] < lim[1] && lim[1] < lim[2] && lim[2] <= 3072;
+	for (int k = 0; k < 3; k++) classes = classes && valid_threads(c->pgs_class_threads[k]);
+	return c->dt > 0 && c->substeps >= 1 && c->vel_iters >= 0 && c->pos_iters >= 0 && c->gyro_ticks >= 1 && valid_threads(c->solver_threads) &&
+	       (c->pgs_mode >= 0 && c->pgs_mode <= 2) && (c->precision == CP_PREC_FP64 || c->precision == CP_PREC_SPLIT) && classes;
 }
 
 int cp_world_create(const cp_config* cfg, cp_world** out) {
@@ -1538,6 +1553,9 @@ int cp_world_destroy(cp_world* w) {
 int cp_world_set_config(cp_world* w, const cp_config* cfg) {
 	if (!w || !valid_cfg(cfg)) return CP_ERR_INVALID;
 	w->cfg = *cfg;
+	if (w->uploaded && !w->cls_sizes.empty() &&
+	    !std::equal(w->cls_built, w->cls_built + 3, w->cfg.pgs_class_bodies))
+		return guard(w, [&] { w->build_classes(); });
 	return CP_OK;
 }
 

@@ -73,6 +73,11 @@ typedef struct {
 	                            the clipping edge, CollisionDetect.cpp:444-467); 0 = bug-compatible */
 	int precision;           /* cp_precision: solve precision of CP_PGS_COLORED (the island modes are
 	                            always FP64) */
+	int pgs_class_bodies[3]; /* CP_PGS_COLORED size classes: scenes of at most [0], [1], [2] bodies
+	                            (ascending, [2] <= 3072) keep their velocity deltas in LDS; larger
+	                            scenes use global memory.  Defaults 192, 768, 3072. */
+	int pgs_class_threads[3]; /* threads per scene workgroup of each LDS class (64..256, a multiple of
+	                            64).  Defaults 64, 128, 256.  Neither setting changes results. */
 } cp_config;
 
 typedef struct cp_world cp_world;
