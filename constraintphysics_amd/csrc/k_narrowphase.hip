@@ -1139,13 +1139,14 @@ __device__ bool wave_sat_hh(const Hull& A, const Hull& B, int lane, int* cand, i
 __global__ __launch_bounds__(64 * NP_WAVES) void k_np_hull_hull(Bodies b, Shapes sh, Params P, const uint64_t* pairs, const int* list,
                                                                 const int* counts, Manifolds M, int cap) {
 	__shared__ WaveLds lds[NP_WAVES];
-	const int lane = threadIdx.x & 63, wib = threadIdx.x >> 6;
+	const int lane = threadIdx.x & 63, wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
 	WaveLds& L = lds[wib];
 	const int cnt = counts[NPC_HH];
 	for (int w = blockIdx.x * NP_WAVES + wib; w < cnt; w += gridDim.x * NP_WAVES) {
-		int slot = list[w];
+		// one pair per wave: its indices are wave-uniform (scalar registers)
+		int slot = __builtin_amdgcn_readfirstlane(list[w]);
 		uint64_t key = pairs[slot];
-		int ia = (int)(key >> 32), ib = (int)(key & 0xFFFFFFFFu);
+		int ia = __builtin_amdgcn_readfirstlane((int)(key >> 32)), ib = __builtin_amdgcn_readfirstlane((int)(key & 0xFFFFFFFFu));
 		Hull A = make_hull(b, sh, ia), B = make_hull(b, sh, ib);
 		stage_hull(A, L.pa, L.na, lane);
 		stage_hull(B, L.pb, L.nb, lane);
@@ -1193,13 +1194,14 @@ __device__ __forceinline__ bool hc_prune(const Hull& A, const Cyl& B, int t) {
 __global__ __launch_bounds__(64 * NP_WAVES) void k_np_hull_cyl(Bodies b, Shapes sh, Params P, const uint64_t* pairs, const int* list,
                                                                const int* counts, Manifolds M, int cap) {
 	__shared__ WaveLds lds[NP_WAVES];
-	const int lane = threadIdx.x & 63, wib = threadIdx.x >> 6;
+	const int lane = threadIdx.x & 63, wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
 	WaveLds& L = lds[wib];
 	const int cnt = counts[NPC_HC];
 	for (int w = blockIdx.x * NP_WAVES + wib; w < cnt; w += gridDim.x * NP_WAVES) {
-		int slot = list[w];
+		// one pair per wave: its indices are wave-uniform (scalar registers)
+		int slot = __builtin_amdgcn_readfirstlane(list[w]);
 		uint64_t key = pairs[slot];
-		int ia = (int)(key >> 32), ib = (int)(key & 0xFFFFFFFFu);
+		int ia = __builtin_amdgcn_readfirstlane((int)(key >> 32)), ib = __builtin_amdgcn_readfirstlane((int)(key & 0xFFFFFFFFu));
 		bool swapped = sh.type[b.shape[ia]] == SHAPE_CYLINDER;  // detectCollision's reversed form (CollisionDetect.cpp:122-130)
 		int ip = swapped ? ib : ia, ic = swapped ? ia : ib;
 		Hull A = make_hull(b, sh, ip);
@@ -1248,13 +1250,14 @@ __device__ __forceinline__ bool cc_prune(const Cyl& A, const Cyl& B, int t) {
 __global__ __launch_bounds__(64 * NP_WAVES) void k_np_cyl_cyl(Bodies b, Shapes sh, Params P, const uint64_t* pairs, const int* list,
                                                               const int* counts, Manifolds M, int cap) {
 	__shared__ WaveLds lds[NP_WAVES];
-	const int lane = threadIdx.x & 63, wib = threadIdx.x >> 6;
+	const int lane = threadIdx.x & 63, wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
 	WaveLds& L = lds[wib];
 	const int cnt = counts[NPC_CC];
 	for (int w = blockIdx.x * NP_WAVES + wib; w < cnt; w += gridDim.x * NP_WAVES) {
-		int slot = list[w];
+		// one pair per wave: its indices are wave-uniform (scalar registers)
+		int slot = __builtin_amdgcn_readfirstlane(list[w]);
 		uint64_t key = pairs[slot];
-		int ia = (int)(key >> 32), ib = (int)(key & 0xFFFFFFFFu);
+		int ia = __builtin_amdgcn_readfirstlane((int)(key >> 32)), ib = __builtin_amdgcn_readfirstlane((int)(key & 0xFFFFFFFFu));
 		Cyl A = make_cyl(b, sh, ia), B = make_cyl(b, sh, ib);
 		stage_cyl(A, L.na, lane);
 		stage_cyl(B, L.nb, lane);
@@ -1279,12 +1282,13 @@ __global__ __launch_bounds__(64 * NP_WAVES) void k_np_cyl_cyl(Bodies b, Shapes s
 
 __global__ __launch_bounds__(64 * NP_WAVES) void k_np_hull_sphere(Bodies b, Shapes sh, Params P, const uint64_t* pairs, const int* list,
                                                                   const int* counts, Manifolds M, int cap) {
-	const int lane = threadIdx.x & 63, wib = threadIdx.x >> 6;
+	const int lane = threadIdx.x & 63, wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
 	const int cnt = counts[NPC_HS];
 	for (int w = blockIdx.x * NP_WAVES + wib; w < cnt; w += gridDim.x * NP_WAVES) {
-		int slot = list[w];
+		// one pair per wave: its indices are wave-uniform (scalar registers)
+		int slot = __builtin_amdgcn_readfirstlane(list[w]);
 		uint64_t key = pairs[slot];
-		int ia = (int)(key >> 32), ib = (int)(key & 0xFFFFFFFFu);
+		int ia = __builtin_amdgcn_readfirstlane((int)(key >> 32)), ib = __builtin_amdgcn_readfirstlane((int)(key & 0xFFFFFFFFu));
 		bool swapped = sh.type[b.shape[ia]] == SHAPE_SPHERE;
 		poly_sphere(b, sh, swapped ? ib : ia, swapped ? ia : ib, swapped, lane, slot, M, cap);
 		if (lane == 0 && (P.defect_fixes & 1)) sanitize_slot(M, slot, cap);
@@ -1573,15 +1577,16 @@ __global__ void k_mesh_candidates(Bodies b, MeshDev md, const uint64_t* pairs, c
 __global__ __launch_bounds__(64 * NP_WAVES) void k_mesh_triangles(Bodies b, Shapes sh, MeshDev md, Params P, const uint64_t* pairs,
                                                                   MeshItems it, int nitems) {
 	__shared__ WaveLds lds[NP_WAVES];
-	const int lane = threadIdx.x & 63, wib = threadIdx.x >> 6;
+	const int lane = threadIdx.x & 63, wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
 	WaveLds& L = lds[wib];
 	const int N = b.n;
 	for (int w = blockIdx.x * NP_WAVES + wib; w < nitems; w += gridDim.x * NP_WAVES) {
-		int slot = it.slot[w];
+		// one (body, triangle) item per wave: its indices are wave-uniform
+		int slot = __builtin_amdgcn_readfirstlane(it.slot[w]);
 		uint64_t key = pairs[slot];
-		int ia = (int)(key >> 32), ib = (int)(key & 0xFFFFFFFFu);
+		int ia = __builtin_amdgcn_readfirstlane((int)(key >> 32)), ib = __builtin_amdgcn_readfirstlane((int)(key & 0xFFFFFFFFu));
 		int c = md.inst_of_body[ia] >= 0 ? ib : ia;
-		Tri T = make_tri(md, it.tri[w]);
+		Tri T = make_tri(md, __builtin_amdgcn_readfirstlane(it.tri[w]));
 		int ct = sh.type[b.shape[c]];
 		// non_gauss_valid_normal_penalty (CollisionDetect.cpp:2673, :2707, :2770)
 		double tl = T.r[33] - T.r[30], t2 = T.r[34] - T.r[31], t3 = T.r[35] - T.r[32];
