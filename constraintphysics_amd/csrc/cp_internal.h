@@ -277,7 +277,7 @@ void launch_edge_build(const Contacts& C, int n, const int* head, const int* eid
 void launch_edge_slots(const Bodies& b, const Contacts& C, const int* order, int nslots, const int* efirst, const int* ecnt, int4* desc,
                        uint32_t* lmask, hipStream_t s);
 void launch_color(const Bodies& b, const int* scene_body_off, int nscenes, const uint64_t* ekey, const int* elive, int nedges,
-                  int* ecolor, int* order, int* color_off, int* ncolors, int* edge_range, hipStream_t s);
+                  int* ecolor, int* order, int* color_off, int* ncolors, int* edge_range, int max_bodies, hipStream_t s);
 void launch_build_rows(const Bodies& b, const Contacts& C, int nc, const Params& P, hipStream_t s);
 // Joint rows: mode 0 all joints (holonomic ones skipped when first), 1 the
 // holonomic joints with warm-start re-projection, 2 the holonomic joints

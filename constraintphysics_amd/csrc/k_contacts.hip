@@ -206,64 +206,95 @@ __global__ void k_edge_build(const Contacts C, int n, const int* head, const int
 
 #define MAXC CP_MAXC
 
-// One thread per scene: greedy edge colouring, then a counting sort of the
-// scene's live edges by colour.  Colour MAXC collects edges that found no free
-// colour; the solver runs that bucket on a single thread.
-__global__ __launch_bounds__(64) void k_color(Bodies b, const int* scene_body_off, int nscenes, const uint64_t* ekey, const int* elive, int nedges,
-                        int* ecolor, int* order, int* color_off, int* ncolors, int* edge_range) {
-	int s = blockIdx.x * blockDim.x + threadIdx.x;
-	if (s >= nscenes) return;
-	int b0 = scene_body_off[s], b1 = scene_body_off[s + 1];
-	uint64_t* mask = (uint64_t*)b.color_mask;
-	int e0 = lower_bound_u64(ekey, nedges, (uint64_t)b0 << 32), e1 = lower_bound_u64(ekey, nedges, (uint64_t)b1 << 32);
-	for (int e = e0; e < e1; e++) {  // only bodies on an edge need a clear colour set
-		int lo = (int)(ekey[e] >> 32), hi = (int)(ekey[e] & 0xFFFFFFFFu);
-		mask[2 * lo] = 0; mask[2 * lo + 1] = 0;
-		mask[2 * hi] = 0; mask[2 * hi + 1] = 0;
-	}
-	edge_range[2 * s] = e0;
-	edge_range[2 * s + 1] = e1;
-	int cnt[MAXC + 1];
-	for (int c = 0; c <= MAXC; c++) cnt[c] = 0;
-	int maxc = -1;
-	for (int e = e0; e < e1; e++) {
-		if (!elive[e]) { ecolor[e] = -1; continue; }
-		int lo = (int)(ekey[e] >> 32), hi = (int)(ekey[e] & 0xFFFFFFFFu);
-		bool dl = b.movement[lo] == 0, dh = b.movement[hi] == 0;
-		uint64_t m0 = (dl ? mask[2 * lo] : 0) | (dh ? mask[2 * hi] : 0);
-		uint64_t m1 = (dl ? mask[2 * lo + 1] : 0) | (dh ? mask[2 * hi + 1] : 0);
-		int c;
-		if (~m0) c = __builtin_ctzll(~m0);
-		else if (~m1) c = 64 + __builtin_ctzll(~m1);
-		else c = MAXC;
-		if (c < MAXC) {
-			uint64_t bit = 1ull << (c & 63);
-			if (c < 64) {
-				if (dl) mask[2 * lo] |= bit;
-				if (dh) mask[2 * hi] |= bit;
-			} else {
-				if (dl) mask[2 * lo + 1] |= bit;
-				if (dh) mask[2 * hi + 1] |= bit;
-			}
+// One wave per scene: greedy edge colouring in edge (cache key) order, then a
+// counting sort of the scene's live edges by colour.  All lanes stage 64
+// edges at a time (bodies, dynamic and live flags) and lane 0 colours them:
+// each edge takes the lowest colour free at its dynamic bodies.  The colour
+// sets live in LDS for scenes of at most `lds_bodies` bodies, else in
+// b.color_mask.  Colour MAXC collects edges that found no free colour; the
+// solver runs that bucket on a single thread.  ecolor holds colour | rank
+// within the colour << 8 (-1: no live contact).
+__global__ __launch_bounds__(64) void k_color(Bodies b, const int* scene_body_off, int nscenes, const uint64_t* ekey, const int* elive,
+                                              int nedges, int* ecolor, int* order, int* color_off, int* ncolors, int* edge_range,
+                                              int lds_bodies) {
+	extern __shared__ uint64_t smask_lds[];
+	__shared__ int sdat[64], shi[64];
+	__shared__ int cnt[MAXC + 1];
+	const int s = blockIdx.x, lane = threadIdx.x;
+	const int b0 = scene_body_off[s], b1 = scene_body_off[s + 1], nbs = b1 - b0;
+	const int e0 = lower_bound_u64(ekey, nedges, (uint64_t)b0 << 32), e1 = lower_bound_u64(ekey, nedges, (uint64_t)b1 << 32);
+	const bool in_lds = nbs <= lds_bodies;
+	uint64_t* mask = in_lds ? smask_lds : reinterpret_cast<uint64_t*>(b.color_mask) + 2 * (size_t)b0;
+	if (in_lds)
+		for (int i = lane; i < 2 * nbs; i += 64) mask[i] = 0;
+	else
+		for (int e = e0 + lane; e < e1; e += 64) {  // only bodies on an edge need a clear colour set
+			int lo = (int)(ekey[e] >> 32) - b0, hi = (int)(ekey[e] & 0xFFFFFFFFu) - b0;
+			mask[2 * lo] = 0; mask[2 * lo + 1] = 0;
+			mask[2 * hi] = 0; mask[2 * hi + 1] = 0;
 		}
-		ecolor[e] = c;
-		cnt[c]++;
-		if (c > maxc) maxc = c;
+	for (int c = lane; c <= MAXC; c += 64) cnt[c] = 0;
+	if (lane == 0) {
+		edge_range[2 * s] = e0;
+		edge_range[2 * s + 1] = e1;
+	}
+	__syncthreads();
+	int maxc = -1;
+	for (int base = e0; base < e1; base += 64) {
+		const int e = base + lane, n = min(64, e1 - base);
+		if (e < e1) {
+			uint64_t k = ekey[e];
+			int lo = (int)(k >> 32), hi = (int)(k & 0xFFFFFFFFu);
+			int fl = (elive[e] ? 1 : 0) | (b.movement[lo] == 0 ? 2 : 0) | (b.movement[hi] == 0 ? 4 : 0);
+			sdat[lane] = (lo - b0) | (fl << 28);
+			shi[lane] = hi - b0;
+		}
+		__syncthreads();
+		if (lane == 0)
+			for (int j = 0; j < n; j++) {
+				const int d = sdat[j], lo = d & 0x0FFFFFFF, fl = d >> 28, hi = shi[j];
+				if (!(fl & 1)) {
+					sdat[j] = -1;
+					continue;
+				}
+				const bool dl = (fl & 2) != 0, dh = (fl & 4) != 0;
+				uint64_t m0 = (dl ? mask[2 * lo] : 0) | (dh ? mask[2 * hi] : 0);
+				uint64_t m1 = (dl ? mask[2 * lo + 1] : 0) | (dh ? mask[2 * hi + 1] : 0);
+				int c;
+				if (~m0) c = __builtin_ctzll(~m0);
+				else if (~m1) c = 64 + __builtin_ctzll(~m1);
+				else c = MAXC;
+				if (c < MAXC) {
+					uint64_t bit = 1ull << (c & 63);
+					int h = c >> 6;
+					if (dl) mask[2 * lo + h] |= bit;
+					if (dh) mask[2 * hi + h] |= bit;
+				}
+				sdat[j] = c | (cnt[c]++ << 8);
+				if (c > maxc) maxc = c;
+			}
+		__syncthreads();
+		if (e < e1) ecolor[e] = sdat[lane];
+		__syncthreads();
 	}
 	int* co = color_off + (size_t)s * (MAXC + 2);
-	int acc = 0;
-	for (int c = 0; c <= MAXC; c++) {
-		co[c] = acc;
-		acc += cnt[c];
-		cnt[c] = co[c];
+	if (lane == 0) {
+		int acc = 0;
+		for (int c = 0; c <= MAXC; c++) {
+			co[c] = acc;
+			int v = cnt[c];
+			cnt[c] = acc;
+			acc += v;
+		}
+		co[MAXC + 1] = acc;
+		ncolors[s] = maxc + 1;
 	}
-	co[MAXC + 1] = acc;
-	for (int e = e0; e < e1; e++) {
-		int c = ecolor[e];
-		if (c < 0) continue;
-		order[e0 + cnt[c]++] = e;
+	__syncthreads();
+	for (int e = e0 + lane; e < e1; e += 64) {
+		int v = ecolor[e];
+		if (v < 0) continue;
+		order[e0 + cnt[v & 0xFF] + (v >> 8)] = e;
 	}
-	ncolors[s] = maxc + 1;
 }
 
 // Descriptor of each colour-sorted edge slot (order[x] = -1: unused slot).
@@ -337,9 +368,17 @@ void launch_kill_contacts(const Contacts& C, int n, const int* ranges, int nrang
 }
 
 void launch_color(const Bodies& b, const int* scene_body_off, int nscenes, const uint64_t* ekey, const int* elive, int nedges,
-                  int* ecolor, int* order, int* color_off, int* ncolors, int* edge_range, hipStream_t s) {
-	hipLaunchKernelGGL(k_color, dim3(nblk(nscenes, 64)), dim3(64), 0, s, b, scene_body_off, nscenes, ekey, elive, nedges, ecolor, order,
-	                   color_off, ncolors, edge_range);
+                  int* ecolor, int* order, int* color_off, int* ncolors, int* edge_range, int max_bodies, hipStream_t s) {
+	if (!nscenes) return;
+	// colour sets of scenes up to 4096 bodies in LDS (64 KB), larger ones in global memory
+	int lds_bodies = max_bodies <= 4096 ? max_bodies : 0;
+	static bool attr = false;
+	if (!attr) {
+		(void)hipFuncSetAttribute((const void*)k_color, hipFuncAttributeMaxDynamicSharedMemorySize, 64 * 1024);
+		attr = true;
+	}
+	hipLaunchKernelGGL(k_color, dim3(nscenes), dim3(64), (size_t)16 * lds_bodies, s, b, scene_body_off, nscenes, ekey, elive, nedges, ecolor,
+	                   order, color_off, ncolors, edge_range, lds_bodies);
 }
 
 }  // namespace cpx

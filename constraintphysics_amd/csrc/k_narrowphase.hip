@@ -680,6 +680,7 @@ __device__ __forceinline__ bool arc_test(V3 a1, V3 a2, V3 aa, V3 ap, V3 b1, V3 b
 	if (dot(ap, b1) * dot(ap, b2) > 0 || dot(bp, a1) * dot(bp, a2) > 0) return false;
 	return !(mag2(cross(ap, bp)) == 0);
 }
+__device__ __forceinline__ bool arc_pair(V3 a1, V3 a2, V3 b1, V3 b2) { return arc_test(a1, a2, a1 + a2, cross(a1, a2), b1, b2, cross(b1, b2)); }
 __device__ bool arc_axis(V3 a1, V3 a2, V3 b1, V3 b2, V3* out) {
 	V3 aa = a1 + a2;
 	if (dot(aa, b1) + dot(aa, b2) <= 0) return false;
@@ -1051,8 +1052,7 @@ __device__ __forceinline__ bool hh_prune(const Hull& A, const Hull& B, int t) {
 	if (t < B.nf) return !B.red[t];
 	t -= B.nf;
 	int i1 = t / B.na, i2 = t - i1 * B.na;
-	V3 n;
-	return arc_axis(ldv(A.wn, A.arcs[2 * i1]), ldv(A.wn, A.arcs[2 * i1 + 1]), -ldv(B.wn, B.arcs[2 * i2]), -ldv(B.wn, B.arcs[2 * i2 + 1]), &n);
+	return arc_pair(ldv(A.wn, A.arcs[2 * i1]), ldv(A.wn, A.arcs[2 * i1 + 1]), -ldv(B.wn, B.arcs[2 * i2]), -ldv(B.wn, B.arcs[2 * i2 + 1]));
 }
 
 struct WaveLds {
@@ -1187,8 +1187,7 @@ __device__ __forceinline__ bool hc_prune(const Hull& A, const Cyl& B, int t) {
 	if (t < A.np) return true;
 	t -= A.np;
 	int i1 = t / 24, k = t - i1 * 24;
-	V3 n;
-	return arc_axis(ldv(A.wn, A.arcs[2 * i1]), ldv(A.wn, A.arcs[2 * i1 + 1]), -cyl_gv(B, k & 1), -cyl_gv(B, k >> 1), &n);
+	return arc_pair(ldv(A.wn, A.arcs[2 * i1]), ldv(A.wn, A.arcs[2 * i1 + 1]), -cyl_gv(B, k & 1), -cyl_gv(B, k >> 1));
 }
 
 __global__ __launch_bounds__(64 * NP_WAVES) void k_np_hull_cyl(Bodies b, Shapes sh, Params P, const uint64_t* pairs, const int* list,
@@ -1243,8 +1242,7 @@ __device__ __forceinline__ bool cc_prune(const Cyl& A, const Cyl& B, int t) {
 	if (t == 0) return mag2(cross(A.ax, B.ax)) > 0.00000001;
 	t -= 1;
 	int k1 = t / 24, k2 = t - k1 * 24;
-	V3 n;
-	return arc_axis(cyl_gv(A, k1 & 1), cyl_gv(A, k1 >> 1), -cyl_gv(B, k2 & 1), -cyl_gv(B, k2 >> 1), &n);
+	return arc_pair(cyl_gv(A, k1 & 1), cyl_gv(A, k1 >> 1), -cyl_gv(B, k2 & 1), -cyl_gv(B, k2 >> 1));
 }
 
 __global__ __launch_bounds__(64 * NP_WAVES) void k_np_cyl_cyl(Bodies b, Shapes sh, Params P, const uint64_t* pairs, const int* list,

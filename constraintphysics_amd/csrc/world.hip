@@ -192,6 +192,7 @@ struct cp_world {
 	std::vector<std::vector<int>> cls_lists;
 	std::vector<std::pair<int, int>> cls_sizes;  // (scene, bodies) the classes are built from
 	int cls_built[3] = {0, 0, 0};                 // class bounds the lists were built with
+	int max_scene_bodies = 0;                     // largest scene the classes hold
 	void build_classes();
 	int cls_max[4] = {0, 0, 0, 0};
 	DArr<int> cls_dev[4];
@@ -914,6 +915,8 @@ void cp_world::build_classes() {
 		up(cls_dev[c], cls_lists[c]);
 	}
 	for (int c = 0; c < 3; c++) cls_built[c] = lim[c];
+	max_scene_bodies = 0;
+	for (auto& q : cls_sizes) max_scene_bodies = std::max(max_scene_bodies, q.second);
 }
 
 void cp_world::scan(const int* in, int* out, int n) {
@@ -1353,8 +1356,7 @@ void cp_world::step() {
 		e_lmask.reserve(nedges);
 		if (nedges) HIPCHK(hipMemsetAsync(e_order.p, 0xFF, (size_t)nedges * sizeof(int), stream));  // unused slots read -1
 		launch_color(b, scene_body_off.p, serving ? live_scenes : nscenes, e_key.p, e_live.p, nedges, e_color.p, e_order.p, color_off.p,
-		             ncolors.p, edge_range.p,
-		             stream);
+		             ncolors.p, edge_range.p, max_scene_bodies, stream);
 		launch_edge_slots(b, cb[cur].c, e_order.p, nedges, e_first.p, e_cnt.p, e_desc.p, e_lmask.p, stream);
 		if (cfg.diagnostics & 2) record_colored_schedule();
 	}
