@@ -487,8 +487,8 @@ __device__ void write_mat(const Bodies& b, int ia, int ib, Manifolds& M, int slo
 // Wave-parallel SAT_PolySphere; poly = body ip, sphere = body is_.  When the
 // sphere is the pair's first body the manifold is flipped as detectCollision
 // does (CollisionDetect.cpp:91-99): normal negated, MagicID halves swapped.
-__device__ void poly_sphere(const Bodies& b, const Shapes& sh, int ip, int is_, bool swapped, int lane, int slot, Manifolds& M, int cap) {
-	Hull A = make_hull(b, sh, ip);
+__device__ void poly_sphere(const Bodies& b, const Shapes& sh, const Hull& A, int ip, int is_, bool swapped, int lane, int slot, Manifolds& M,
+                            int cap) {
 	V3 c = ldv(b.wpts + 3 * b.wpt_off[is_], 0);
 	double r = sh.radius[b.shape[is_]];
 	int naxes = A.nf + A.np + A.ne;
@@ -1280,6 +1280,7 @@ __global__ __launch_bounds__(64 * NP_WAVES) void k_np_cyl_cyl(Bodies b, Shapes s
 
 __global__ __launch_bounds__(64 * NP_WAVES) void k_np_hull_sphere(Bodies b, Shapes sh, Params P, const uint64_t* pairs, const int* list,
                                                                   const int* counts, Manifolds M, int cap) {
+	__shared__ double sp[NP_WAVES][3 * ST_PTS], sn[NP_WAVES][3 * ST_NRM];
 	const int lane = threadIdx.x & 63, wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
 	const int cnt = counts[NPC_HS];
 	for (int w = blockIdx.x * NP_WAVES + wib; w < cnt; w += gridDim.x * NP_WAVES) {
@@ -1288,7 +1289,11 @@ __global__ __launch_bounds__(64 * NP_WAVES) void k_np_hull_sphere(Bodies b, Shap
 		uint64_t key = pairs[slot];
 		int ia = __builtin_amdgcn_readfirstlane((int)(key >> 32)), ib = __builtin_amdgcn_readfirstlane((int)(key & 0xFFFFFFFFu));
 		bool swapped = sh.type[b.shape[ia]] == SHAPE_SPHERE;
-		poly_sphere(b, sh, swapped ? ib : ia, swapped ? ia : ib, swapped, lane, slot, M, cap);
+		const int ip = swapped ? ib : ia;
+		Hull A = make_hull(b, sh, ip);
+		stage_hull(A, sp[wib], sn[wib], lane);
+		poly_sphere(b, sh, A, ip, swapped ? ia : ib, swapped, lane, slot, M, cap);
+		wave_fence();
 		if (lane == 0 && (P.defect_fixes & 1)) sanitize_slot(M, slot, cap);
 	}
 }
