@@ -115,6 +115,13 @@ struct IslandSched {
 	const int *eoff, *coff, *icoff, *serial;
 };
 
+// Two side streams forked from and joined back into a main stream with events
+// (independent launches run beside each other).
+struct Fork {
+	hipStream_t side[2];
+	hipEvent_t fork, join[2];
+};
+
 // Narrowphase output, one slot per candidate pair.
 struct Manifolds {
 	int* count;       // [cap] points in the culled manifold (0 = none)
@@ -248,7 +255,7 @@ void launch_bp_sweep(int n, int nch, const int* sorted, const double* sb, const 
 // rank (nullable): slot rank of each pair (mesh pairs own MESH_SLOTS
 // consecutive slots, rank 0 runs the test, the rest start empty).
 void launch_narrowphase(const Bodies& b, const Shapes& sh, const Params& p, const uint64_t* pairs, int npairs, const Manifolds& m,
-                        int cap, int* lists, int* counts, const uint64_t* nocol, int n_nocol, const int* rank, hipStream_t s);
+                        int cap, int* lists, int* counts, const uint64_t* nocol, int n_nocol, const int* rank, hipStream_t s, const Fork* fk = nullptr);
 // Mesh pairs: slot expansion (count pass, then write pass at the scanned
 // offsets), candidate triangles per pair (same two passes), one wave per
 // candidate, then per pair the merge and cull into its slots.

@@ -1815,7 +1815,7 @@ void launch_mesh_merge(const Params& p, const uint64_t* pairs, const MeshDev& md
 }
 
 void launch_narrowphase(const Bodies& b, const Shapes& sh, const Params& p, const uint64_t* pairs, int npairs, const Manifolds& m,
-                        int cap, int* lists, int* counts, const uint64_t* nocol, int n_nocol, const int* rank, hipStream_t s) {
+                        int cap, int* lists, int* counts, const uint64_t* nocol, int n_nocol, const int* rank, hipStream_t s, const Fork* fk) {
 	if (npairs <= 0) return;
 	hipLaunchKernelGGL(k_np_zero, dim3(1), dim3(64), 0, s, counts);
 	hipLaunchKernelGGL(k_np_classify, dim3((npairs + 64 * CLS_WAVES - 1) / (64 * CLS_WAVES)), dim3(64 * CLS_WAVES), 0, s, b, sh, pairs, npairs,
@@ -1824,13 +1824,27 @@ void launch_narrowphase(const Bodies& b, const Shapes& sh, const Params& p, cons
 	if (wblocks > 4096) wblocks = 4096;
 	const int* L = lists;
 	size_t n = (size_t)npairs;
-	hipLaunchKernelGGL(k_np_hull_hull, dim3(wblocks), dim3(64 * NP_WAVES), 0, s, b, sh, p, pairs, L + NPC_HH * n, counts, m, cap);
+	// the pair classes touch disjoint slots: hull-cylinder on the world's
+	// stream, the others beside it on the two side streams
+	hipStream_t s0 = fk ? fk->side[0] : s, s1 = fk ? fk->side[1] : s;
+	if (fk) {
+		(void)(hipEventRecord(fk->fork, s));
+		(void)(hipStreamWaitEvent(s0, fk->fork, 0));
+		(void)(hipStreamWaitEvent(s1, fk->fork, 0));
+	}
 	hipLaunchKernelGGL(k_np_hull_cyl, dim3(wblocks), dim3(64 * NP_WAVES), 0, s, b, sh, p, pairs, L + NPC_HC * n, counts, m, cap);
-	hipLaunchKernelGGL(k_np_cyl_cyl, dim3(wblocks), dim3(64 * NP_WAVES), 0, s, b, sh, p, pairs, L + NPC_CC * n, counts, m, cap);
-	hipLaunchKernelGGL(k_np_hull_sphere, dim3(wblocks), dim3(64 * NP_WAVES), 0, s, b, sh, p, pairs, L + NPC_HS * n, counts, m, cap);
+	hipLaunchKernelGGL(k_np_hull_hull, dim3(wblocks), dim3(64 * NP_WAVES), 0, s0, b, sh, p, pairs, L + NPC_HH * n, counts, m, cap);
+	hipLaunchKernelGGL(k_np_cyl_cyl, dim3(wblocks), dim3(64 * NP_WAVES), 0, s1, b, sh, p, pairs, L + NPC_CC * n, counts, m, cap);
+	hipLaunchKernelGGL(k_np_hull_sphere, dim3(wblocks), dim3(64 * NP_WAVES), 0, s1, b, sh, p, pairs, L + NPC_HS * n, counts, m, cap);
 	int tblocks = (npairs + 255) / 256;
 	if (tblocks > 4096) tblocks = 4096;
-	hipLaunchKernelGGL(k_np_sphere_pairs, dim3(tblocks), dim3(256), 0, s, b, sh, p, pairs, L + NPC_SS * n, L + NPC_SC * n, counts, m, cap);
+	hipLaunchKernelGGL(k_np_sphere_pairs, dim3(tblocks), dim3(256), 0, s0, b, sh, p, pairs, L + NPC_SS * n, L + NPC_SC * n, counts, m, cap);
+	if (fk) {
+		(void)(hipEventRecord(fk->join[0], s0));
+		(void)(hipEventRecord(fk->join[1], s1));
+		(void)(hipStreamWaitEvent(s, fk->join[0], 0));
+		(void)(hipStreamWaitEvent(s, fk->join[1], 0));
+	}
 }
 
 }  // namespace cpx

@@ -150,6 +150,7 @@ struct cp_world {
 	// joined back into `stream` around each solve (default priority)
 	hipStream_t side[2] = {nullptr, nullptr};
 	hipEvent_t fork_ev = nullptr, join_ev[2] = {nullptr, nullptr};
+	Fork fk{};
 	int device = 0;
 	// host copies of static data
 	HostShapes hs;
@@ -1016,7 +1017,7 @@ int cp_world::narrowphase(int npairs) {
 	np_lists.reserve(7 * (size_t)std::max(npairs, 1));
 	np_counts.reserve(8);
 	launch_narrowphase(bodies(), shapes(), params(), pairs.p, npairs, man, mcap, np_lists.p, np_counts.p, nocol.p, n_nocol,
-	                   last_expanded ? pair_rank.p : nullptr, stream);
+	                   last_expanded ? pair_rank.p : nullptr, stream, &fk);
 	if (!last_expanded) return npairs;
 	// convex body against mesh: candidate triangles, per-triangle SAT, merge + cull
 	int nlist = read1(np_counts.p + NPC_MESH_LIST);
@@ -1521,6 +1522,7 @@ int cp_world_create(const cp_config* cfg, cp_world** out) {
 		for (auto& x : w->side) HIPCHK(hipStreamCreate(&x));
 		HIPCHK(hipEventCreateWithFlags(&w->fork_ev, hipEventDisableTiming));
 		for (auto& e : w->join_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+		w->fk = Fork{{w->side[0], w->side[1]}, w->fork_ev, {w->join_ev[0], w->join_ev[1]}};
 		for (auto& e : w->ev) HIPCHK(hipEventCreate(&e));
 		for (auto& e : w->pgs_ev) HIPCHK(hipEventCreate(&e));
 	});
