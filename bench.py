@@ -217,6 +217,8 @@ def main():
     ap.add_argument("--no-c2", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default PGS size classes and workgroup widths")
     ap.add_argument("--c2-only", action="store_true", help="run only the holonomic-chains-on-terrain leg and print its JSON")
+    ap.add_argument("--c2-mode", type=int, choices=[1, 2], default=None,
+                    help="with --c2-only: only this island solve (1 reference order, 2 coloured islands; default both)")
     ap.add_argument("--serve-requests", type=int, default=6000, help="requests of the serving leg (Poisson arrivals at 2000/s)")
     ap.add_argument("--serve-capacity", type=int, default=1600000, help="body slots of the serving world (its HBM budget)")
     ap.add_argument("--precision", choices=sorted(PRECISIONS), default="split",
@@ -228,7 +230,9 @@ def main():
         sys.exit(2)
 
     if args.c2_only:
-        print(json.dumps({"island_colored": c2_leg(2), "reference_order": c2_leg(1)}), flush=True)
+        modes = [args.c2_mode] if args.c2_mode else [2, 1]
+        names = {1: "reference_order", 2: "island_colored"}
+        print(json.dumps({names[m]: c2_leg(m) for m in modes}), flush=True)
         return
 
     from constraintphysics_amd import scenes

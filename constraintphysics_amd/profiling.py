@@ -17,7 +17,7 @@ import collections
 import csv
 import os
 
-PGS_KERNELS = ("k_pgs_colored_lds", "k_pgs_colored(", "k_pgs_colored32", "k_pgs_ordered")
+PGS_KERNELS = ("k_pgs_colored_lds", "k_pgs_colored(", "k_pgs_colored32", "k_pgs_ordered", "k_pgs_islands")
 
 
 def _rows(path):
