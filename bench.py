@@ -216,6 +216,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-c2", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default PGS size classes and workgroup widths")
+    ap.add_argument("--no-graph", action="store_true", help="skip the HIP-graph replay leg")
     ap.add_argument("--c2-only", action="store_true", help="run only the holonomic-chains-on-terrain leg and print its JSON")
     ap.add_argument("--c2-mode", type=int, choices=[1, 2], default=None,
                     help="with --c2-only: only this island solve (1 reference order, 2 coloured islands; default both)")
@@ -290,6 +291,21 @@ def main():
                     max(1e-300, np.linalg.norm(acc["fp64"][1][:, 7:13])))
     print(f"[bench] split vs fp64: impulses {dev_imp:.3e}, velocities {dev_vel:.3e} relative; residual fp64 "
           f"{acc['fp64'][0][2]:.3e} split {acc['split'][0][2]:.3e}", file=sys.stderr, flush=True)
+    # The same batch from the end of the timed window with the substep loop
+    # replayed from a HIP graph (same results, tested bit for bit); reported
+    # beside the eager figure, which is the one `value` uses.
+    graph_leg = None
+    if not args.no_graph:
+        w.set_config(diagnostics=0, precision=PRECISIONS[args.precision], graph=1)
+        w.set_state(acc_state)
+        w.set_contacts(*acc_contacts)
+        w.step(1)
+        gl = time_steps(w, args.steps)
+        w.set_config(graph=0)
+        graph_leg = dict(ms_per_step=round(1e3 * float(gl.mean()), 3), value=round(n_dyn * args.steps / float(gl.sum()), 1),
+                         note="steps after the timed window, substep loop (rows, PGS, write-back, integration) replayed "
+                              "from one captured HIP graph per contact buffer")
+        print(f"[bench] graph replay: {graph_leg['ms_per_step']:.2f} ms/step", file=sys.stderr, flush=True)
     lat = np.array(lat)
     tim = np.array(tim)
     total = float(lat.sum())
@@ -309,7 +325,7 @@ def main():
     if not args.no_plain:
         order = np.argsort(sizes)
         pick = [int(order[int(q * (len(order) - 1))]) for q in np.linspace(0.02, 0.98, args.plain_scenes)]
-        pb, pt = 0, 0.0
+        pb, pt, pg = 0, 0.0, 0.0
         for s in pick:
             ws, inf = build_world(sizes, args.seed, scene_indices=[s], precision=args.precision)
             st, cache = scene_snapshot(snap_state, snap_contacts, off, s)
@@ -320,11 +336,17 @@ def main():
             l = time_steps(ws, args.steps)
             pb += int(inf["dynamic_bodies"]) * args.steps
             pt += float(l.sum())
+            # the same scene's substep loop replayed from a HIP graph (not the baseline; reported beside it)
+            ws.set_config(graph=1)
+            ws.step(1)
+            pg += float(time_steps(ws, args.steps).sum())
             ws.close()
-        plain = dict(value=pb / pt, scenes=len(pick),
+        plain = dict(value=pb / pt, scenes=len(pick), graph_replay_value=round(pb / pg, 1),
                      note="each sampled scene (spread over the size distribution) stepped alone in its own world from the "
-                          "batched run's settled state and contact cache, 2 untimed steps then the same number of timed steps")
-        print(f"[bench] plain single-scene path: {plain['value']:.0f} body-steps/s", file=sys.stderr, flush=True)
+                          "batched run's settled state and contact cache, 2 untimed steps then the same number of timed steps; "
+                          "graph_replay_value: the same scenes continued with the substep loop replayed from a HIP graph")
+        print(f"[bench] plain single-scene path: {plain['value']:.0f} body-steps/s (graph replay {pb / pg:.0f})", file=sys.stderr,
+              flush=True)
 
     served = None
     if not args.no_serve:
@@ -375,6 +397,7 @@ def main():
         "plain_gpu_path": plain,
         "speedup_vs_plain": round(value / plain["value"], 2) if plain else None,
         "autotune": tuned,
+        "graph_replay": graph_leg,
         "serving": served,
         "holonomic_chains_on_terrain": c2,
         "serving_speedup_vs_plain": round(served["value"] / plain["value"], 2) if (plain and served) else None,

@@ -40,6 +40,7 @@ class Config(ctypes.Structure):
         ("precision", ctypes.c_int),
         ("pgs_class_bodies", ctypes.c_int * 3),
         ("pgs_class_threads", ctypes.c_int * 3),
+        ("graph", ctypes.c_int),
     ]
 
 

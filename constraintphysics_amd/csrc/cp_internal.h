@@ -277,7 +277,8 @@ void launch_old_keep(const Contacts& C, int nold, int* keep, hipStream_t s);
 void launch_new_keys(const NewPts& np, int nnew, const int* rank, uint64_t* nk, hipStream_t s);
 void launch_merge(const Contacts& S, int nold, const int* keep_scan, const uint64_t* nk, int nnewc, const NewPts& np, int nnew, int cap,
                   const int* rank, const Contacts& D, const Bodies& b, const Params& P, uint64_t seq0, hipStream_t s);
-void launch_age(const Contacts& C, int n, hipStream_t s);
+void launch_age(const Contacts& C, int n, hipStream_t s, const int* ncp = nullptr);
+void launch_set_count(int* p, int v, hipStream_t s);
 void launch_edge_heads(const Contacts& C, int n, int* head, hipStream_t s);
 void launch_edge_build(const Contacts& C, int n, const int* head, const int* eidx, uint64_t* ekey, int* efirst, int* ecnt, int* elive,
                        hipStream_t s);
@@ -285,7 +286,7 @@ void launch_edge_slots(const Bodies& b, const Contacts& C, const int* order, int
                        uint32_t* lmask, hipStream_t s);
 void launch_color(const Bodies& b, const int* scene_body_off, int nscenes, const uint64_t* ekey, const int* elive, int nedges,
                   int* ecolor, int* order, int* color_off, int* ncolors, int* edge_range, int max_bodies, hipStream_t s);
-void launch_build_rows(const Bodies& b, const Contacts& C, int nc, const Params& P, hipStream_t s);
+void launch_build_rows(const Bodies& b, const Contacts& C, int nc, const Params& P, hipStream_t s, const int* ncp = nullptr);
 // Joint rows: mode 0 all joints (holonomic ones skipped when first), 1 the
 // holonomic joints with warm-start re-projection, 2 the holonomic joints
 // without re-projection.
@@ -295,8 +296,6 @@ void launch_hsys_factor(const Joints& J, const HSysDev& H, double cfm, hipStream
 void launch_pgs_colored_lds(const Bodies& b, const Contacts& C, const int* scene_list, int nlist, int max_bodies, const int* scene_body_off,
                             const int* edge_range, const int* order, const int* color_off, const int* ncolors, const int* efirst,
                             const int* ecnt, const Params& P, int threads, hipStream_t s);
-void launch_rows_writeback(const Contacts& C, int nc, const Params& P, hipStream_t s);
-void launch_pgs_residual(const Bodies& b, const Contacts& C, int nc, const Params& P, double* acc, hipStream_t s);
 // rows: (index, kind) pairs, kind 0 contact normal, 1 friction, 2 joint;
 // isl_sys/isl_sys_off: holonomic systems per island; lin: [6][n] scratch.
 void launch_pgs_islands(const Bodies& b, const Contacts& C, const Joints& J, const HSysDev& H, const IslandSched& S, const Params& P,
@@ -307,7 +306,7 @@ void launch_pgs_colored(const Bodies& b, const Contacts& C, const int* scene_bod
 void launch_pgs_colored_lds(const Bodies& b, const Contacts& C, const int* scene_list, int nlist, int max_bodies, const int* scene_body_off,
                             const int* edge_range, const int4* desc, const uint32_t* lmask, const int* color_off, const int* ncolors,
                             const Params& P, int threads, hipStream_t s);
-void launch_rows_writeback(const Contacts& C, int nc, const Params& P, hipStream_t s);
+void launch_rows_writeback(const Contacts& C, int nc, const Params& P, hipStream_t s, const int* ncp = nullptr);
 void launch_pgs_residual(const Bodies& b, const Contacts& C, int nc, const Params& P, double* acc, hipStream_t s);
 // rows: (index, kind) pairs, kind 0 contact normal, 1 friction, 2 joint;
 // isl_sys/isl_sys_off: holonomic systems per island; lin: [6][n] scratch.

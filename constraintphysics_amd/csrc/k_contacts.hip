@@ -179,11 +179,16 @@ __global__ void k_merge_new(NewPts np, int nnew, int cap, const int* rank, Conta
 }
 
 // Last substep (PhysicsEngine.cpp:1387-1392): memory_life--, is_live = false.
-__global__ void k_age(Contacts C, int n) {
-	int k = blockIdx.x * blockDim.x + threadIdx.x;
-	if (k >= n) return;
-	C.life[k] -= 1;
-	C.live[k] = 0;
+__global__ void k_age(Contacts C, int nc, const int* ncp) {
+	const int n = ncp ? *ncp : nc;
+	for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+		C.life[k] -= 1;
+		C.live[k] = 0;
+	}
+}
+
+__global__ void k_set_count(int* p, int v) {
+	if (threadIdx.x == 0) *p = v;
 }
 
 // ------------------------------------------------------------- edges
@@ -342,9 +347,12 @@ void launch_merge(const Contacts& S, int nold, const int* keep_scan, const uint6
 	if (nnew)
 		hipLaunchKernelGGL(k_merge_new, dim3(nblk(nnew, 128)), dim3(128), 0, s, np, nnew, cap, rank, S, nold, keep_scan, D, b, P, seq0);
 }
-void launch_age(const Contacts& C, int n, hipStream_t s) {
-	if (n) hipLaunchKernelGGL(k_age, dim3(nblk(n, 256)), dim3(256), 0, s, C, n);
+void launch_age(const Contacts& C, int n, hipStream_t s, const int* ncp) {
+	if (!n) return;
+	int g = ncp ? std::max(1, std::min(nblk(n, 256), 4096)) : nblk(n, 256);
+	hipLaunchKernelGGL(k_age, dim3(g), dim3(256), 0, s, C, n, ncp);
 }
+void launch_set_count(int* p, int v, hipStream_t s) { hipLaunchKernelGGL(k_set_count, dim3(1), dim3(64), 0, s, p, v); }
 void launch_edge_heads(const Contacts& C, int n, int* head, hipStream_t s) {
 	if (n) hipLaunchKernelGGL(k_edge_heads, dim3(nblk(n, 256)), dim3(256), 0, s, C, n, head);
 }

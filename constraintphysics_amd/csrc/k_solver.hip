@@ -49,10 +49,20 @@ __device__ __forceinline__ T tsqrt(T x) {
 
 // Rows are derived in FP64 from the FP64 state and stored in the solve
 // precision T (FP64, or FP32 for the split path).
+// Contacts [0, n): n = *ncp when the count lives on the device (a captured
+// graph replays with each step's count), else nc; grid-stride.
 template <class T>
-__global__ __launch_bounds__(64) void k_build_rows(Bodies b, Contacts C, int nc, Params P) {
-	int k = blockIdx.x * blockDim.x + threadIdx.x;
-	if (k >= nc || !C.live[k]) return;
+__device__ void build_row(const Bodies& b, const Contacts& C, int k, const Params& P);
+
+template <class T>
+__global__ __launch_bounds__(64) void k_build_rows(Bodies b, Contacts C, int nc, const int* ncp, Params P) {
+	const int n = ncp ? *ncp : nc;
+	for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
+		if (C.live[k]) build_row<T>(b, C, k, P);
+}
+
+template <class T>
+__device__ void build_row(const Bodies& b, const Contacts& C, int k, const Params& P) {
 	const int N = b.n, K = C.cap;
 	T* R = rec<T>(C, k);
 	int i1 = (int)(C.key[k] >> 32), i2 = (int)(C.key[k] & 0xFFFFFFFFu);
@@ -770,14 +780,16 @@ __global__ __launch_bounds__(256) void k_pgs_colored_lds(Bodies b, Contacts C, c
 // Accumulated impulses and the static flag back into the persistent FP64
 // contact cache (warm start of the next substep; PersistentConstraint.cpp:201-215).
 template <class T>
-__global__ void k_rows_writeback(Contacts C, int nc) {
-	int k = blockIdx.x * blockDim.x + threadIdx.x;
-	if (k >= nc || !C.live[k]) return;
-	const T* R = rec<T>(C, k);
-	C.imp_n[k] = (double)R[CP_R_IMP];
-	C.imp_t[k] = (double)R[CP_R_FIMP];
-	C.imp_t[C.cap + k] = (double)R[CP_R_FIMP + 1];
-	C.sready[k] = R[CP_R_SREADY] != T(0);
+__global__ void k_rows_writeback(Contacts C, int nc, const int* ncp) {
+	const int n = ncp ? *ncp : nc;
+	for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+		if (!C.live[k]) continue;
+		const T* R = rec<T>(C, k);
+		C.imp_n[k] = (double)R[CP_R_IMP];
+		C.imp_t[k] = (double)R[CP_R_FIMP];
+		C.imp_t[C.cap + k] = (double)R[CP_R_FIMP + 1];
+		C.sready[k] = R[CP_R_SREADY] != T(0);
+	}
 }
 
 // FP64 natural-map residual of the velocity rows after the sweeps, evaluated
@@ -835,10 +847,14 @@ __global__ void k_pgs_residual(Bodies b, Contacts C, int nc, double* acc) {
 
 static inline int nblk(int n, int t) { return (n + t - 1) / t; }
 
-void launch_build_rows(const Bodies& b, const Contacts& C, int nc, const Params& P, hipStream_t s) {
+// With ncp, nc is an upper bound on the device count (the grid is sized for
+// it, capped; threads stride over the rest).
+static inline int grid_for(int n, int t) { return std::max(1, std::min(nblk(n, t), 4096)); }
+void launch_build_rows(const Bodies& b, const Contacts& C, int nc, const Params& P, hipStream_t s, const int* ncp) {
 	if (!nc) return;
-	if (P.precision == CP_PREC_SPLIT) hipLaunchKernelGGL(k_build_rows<float>, dim3(nblk(nc, 64)), dim3(64), 0, s, b, C, nc, P);
-	else hipLaunchKernelGGL(k_build_rows<double>, dim3(nblk(nc, 64)), dim3(64), 0, s, b, C, nc, P);
+	int g = ncp ? grid_for(nc, 64) : nblk(nc, 64);
+	if (P.precision == CP_PREC_SPLIT) hipLaunchKernelGGL(k_build_rows<float>, dim3(g), dim3(64), 0, s, b, C, nc, ncp, P);
+	else hipLaunchKernelGGL(k_build_rows<double>, dim3(g), dim3(64), 0, s, b, C, nc, ncp, P);
 }
 void launch_pgs_islands(const Bodies& b, const Contacts& C, const Joints& J, const HSysDev& H, const IslandSched& S, const Params& P,
                         int* applied, double* lin, int first, bool colored, hipStream_t s) {
@@ -875,10 +891,11 @@ void launch_pgs_colored_lds(const Bodies& b, const Contacts& C, const int* scene
 		hipLaunchKernelGGL(k_pgs_colored_lds<double>, dim3(nlist), dim3(threads), (size_t)6 * max_bodies * sizeof(double), s, b, C,
 		                   scene_list, scene_body_off, edge_range, desc, lmask, color_off, ncolors, P);
 }
-void launch_rows_writeback(const Contacts& C, int nc, const Params& P, hipStream_t s) {
+void launch_rows_writeback(const Contacts& C, int nc, const Params& P, hipStream_t s, const int* ncp) {
 	if (!nc) return;
-	if (P.precision == CP_PREC_SPLIT) hipLaunchKernelGGL(k_rows_writeback<float>, dim3(nblk(nc, 256)), dim3(256), 0, s, C, nc);
-	else hipLaunchKernelGGL(k_rows_writeback<double>, dim3(nblk(nc, 256)), dim3(256), 0, s, C, nc);
+	int g = ncp ? grid_for(nc, 256) : nblk(nc, 256);
+	if (P.precision == CP_PREC_SPLIT) hipLaunchKernelGGL(k_rows_writeback<float>, dim3(g), dim3(256), 0, s, C, nc, ncp);
+	else hipLaunchKernelGGL(k_rows_writeback<double>, dim3(g), dim3(256), 0, s, C, nc, ncp);
 }
 void launch_pgs_residual(const Bodies& b, const Contacts& C, int nc, const Params& P, double* acc, hipStream_t s) {
 	if (!nc) return;

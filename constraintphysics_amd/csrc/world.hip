@@ -151,6 +151,13 @@ struct cp_world {
 	hipStream_t side[2] = {nullptr, nullptr};
 	hipEvent_t fork_ev = nullptr, join_ev[2] = {nullptr, nullptr};
 	Fork fk{};
+	// captured substep loop of the coloured mode, one per contact buffer
+	struct Captured {
+		hipGraphExec_t exec = nullptr;
+		std::vector<uint64_t> key;
+	};
+	Captured graphs[2];
+	DArr<int> d_nc;  // the step's contact count on the device (read by the captured kernels)
 	int device = 0;
 	// host copies of static data
 	HostShapes hs;
@@ -382,6 +389,9 @@ struct cp_world {
 	void record_colored_schedule();
 	void launch_pgs_colored_lds_global(const Bodies& b, const Params& P);
 	void step();
+	void colored_pgs(const Bodies& b, const Params& P);
+	void colored_substep(const Bodies& b, const Params& P, int s, const int* ncp);
+	void graph_substeps(const Bodies& b, const Params& P);
 };
 
 void cp_world::free_all() {
@@ -400,6 +410,7 @@ void cp_world::free_all() {
 	for (auto* a : u64s) a->free_();
 	e_desc.free_();
 	e_lmask.free_();
+	d_nc.free_();
 	DArr<int>* jints[] = {&dj_type, &dj_a, &dj_b, &dj_sys, &dh_nc, &dh_ord_off, &dh_order, &dh_vloc, &dh_tab_off, &dh_table, &dh_degree,
 	                      &dh_buf_off, &dh_dbuf_off, &isl_sys, &isl_sys_off, &isl_jlin, &isl_eoff, &isl_coff, &isl_icoff,
 	                      &isl_serial};
@@ -1315,6 +1326,89 @@ void cp_world::launch_pgs_colored_lds_global(const Bodies& b, const Params& P) {
 	                   stream, cls_dev[3].p, (int)cls_lists[3].size());
 }
 
+// One coloured PGS solve: the largest size class (and the global-memory one)
+// on the world's stream, the two smaller classes beside it on the side
+// streams.
+void cp_world::colored_pgs(const Bodies& b, const Params& P) {
+	const int* thr = cfg.pgs_class_threads;
+	bool fork = !cls_lists[0].empty() || !cls_lists[1].empty();
+	if (fork) {
+		HIPCHK(hipEventRecord(fork_ev, stream));
+		for (auto& x : side) HIPCHK(hipStreamWaitEvent(x, fork_ev, 0));
+	}
+	launch_pgs_colored_lds(b, cb[cur].c, cls_dev[2].p, (int)cls_lists[2].size(), cls_max[2], scene_body_off.p, edge_range.p, e_desc.p,
+	                       e_lmask.p, color_off.p, ncolors.p, P, thr[2], stream);
+	if (!cls_lists[3].empty()) launch_pgs_colored_lds_global(b, P);
+	for (int c = 0; c < 2 && fork; c++)
+		launch_pgs_colored_lds(b, cb[cur].c, cls_dev[c].p, (int)cls_lists[c].size(), cls_max[c], scene_body_off.p, edge_range.p, e_desc.p,
+		                       e_lmask.p, color_off.p, ncolors.p, P, thr[c], side[c]);
+	if (fork)
+		for (int c = 0; c < 2; c++) {
+			HIPCHK(hipEventRecord(join_ev[c], side[c]));
+			HIPCHK(hipStreamWaitEvent(stream, join_ev[c], 0));
+		}
+}
+
+// Substep s of the coloured mode with the contact count read on the device.
+void cp_world::colored_substep(const Bodies& b, const Params& P, int s, const int* ncp) {
+	const int cap = cb[cur].c.cap;
+	launch_build_rows(b, cb[cur].c, cap, P, stream, ncp);
+	colored_pgs(b, P);
+	launch_rows_writeback(cb[cur].c, cap, P, stream, ncp);
+	if (s == cfg.substeps - 1) launch_age(cb[cur].c, cap, stream, ncp);
+	launch_integrate(b, P, stream);
+}
+
+// The substep loop as a HIP graph: captured the first time (and whenever a
+// buffer, size or setting it depends on changes), then replayed; the step's
+// contact count reaches the kernels through d_nc.
+void cp_world::graph_substeps(const Bodies& b, const Params& P) {
+	d_nc.reserve(1);
+	launch_set_count(d_nc.p, ncontacts, stream);
+	const Contacts& C = cb[cur].c;
+	std::vector<uint64_t> key = {(uint64_t)(uintptr_t)C.key, (uint64_t)(uintptr_t)C.row, (uint64_t)C.cap, (uint64_t)(uintptr_t)C.live,
+	                             (uint64_t)b.n, (uint64_t)b.nl, (uint64_t)(uintptr_t)b.com, (uint64_t)(uintptr_t)b.dv,
+	                             (uint64_t)(uintptr_t)e_desc.p, (uint64_t)(uintptr_t)e_lmask.p, (uint64_t)(uintptr_t)color_off.p,
+	                             (uint64_t)(uintptr_t)ncolors.p, (uint64_t)(uintptr_t)edge_range.p, (uint64_t)(uintptr_t)scene_body_off.p,
+	                             (uint64_t)nscenes, (uint64_t)cfg.substeps, (uint64_t)cfg.solver_threads, (uint64_t)(uintptr_t)d_nc.p};
+	for (int c = 0; c < 4; c++) {
+		key.push_back((uint64_t)(uintptr_t)cls_dev[c].p);
+		key.push_back((uint64_t)cls_lists[c].size());
+		key.push_back((uint64_t)cls_max[c]);
+	}
+	for (int c = 0; c < 3; c++) key.push_back((uint64_t)cfg.pgs_class_threads[c]);
+	{
+		uint64_t pw[(sizeof(Params) + 7) / 8] = {};
+		memcpy(pw, &P, sizeof(Params));
+		key.insert(key.end(), pw, pw + (sizeof(Params) + 7) / 8);
+	}
+	Captured& g = graphs[cur];
+	if (!g.exec || g.key != key) {
+		if (g.exec) {
+			HIPCHK(hipGraphExecDestroy(g.exec));
+			g.exec = nullptr;
+		}
+		hipGraph_t gr = nullptr;
+		HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
+		try {
+			for (int s = 0; s < cfg.substeps; s++) colored_substep(b, P, s, d_nc.p);
+		} catch (...) {
+			(void)hipStreamEndCapture(stream, &gr);
+			if (gr) (void)hipGraphDestroy(gr);
+			throw;
+		}
+		HIPCHK(hipStreamEndCapture(stream, &gr));
+		hipError_t e = hipGraphInstantiate(&g.exec, gr, nullptr, nullptr, 0);
+		(void)hipGraphDestroy(gr);
+		if (e != hipSuccess) {
+			g.exec = nullptr;
+			throw ApiErr{CP_ERR_DEVICE};
+		}
+		g.key = key;
+	}
+	HIPCHK(hipGraphLaunch(g.exec, stream));
+}
+
 void cp_world::step() {
 	if (serving) serving_commit();
 	if (!j_type.empty() && cfg.pgs_mode == CP_PGS_COLORED) throw ApiErr{CP_ERR_STATE};  // joints: island solves only
@@ -1363,7 +1457,9 @@ void cp_world::step() {
 	}
 	if (timing) HIPCHK(hipEventRecord(ev[3], stream));
 	int npgs = 0;
-	for (int s = 0; s < cfg.substeps; s++) {
+	const bool use_graph = cfg.graph && !ordered && !(cfg.diagnostics & 1) && ncontacts > 0;
+	if (use_graph) graph_substeps(b, P);
+	for (int s = 0; s < cfg.substeps && !use_graph; s++) {
 		launch_build_rows(b, cb[cur].c, ncontacts, P, stream);
 		// joints; holonomic ones are current at the first substep (PhysicsEngine.cpp:1396-1403)
 		launch_joint_rows(b, J, P, P.sub_dt, 0, s == 0, stream);
@@ -1376,28 +1472,8 @@ void cp_world::step() {
 			IslandSched S{nisl, lin_nl, isl_rows.p, isl_off.p, isl_sys.p, isl_sys_off.p, isl_jlin.p,
 			              isl_eoff.p, isl_coff.p, isl_icoff.p, isl_serial.p};
 			launch_pgs_islands(b, cb[cur].c, J, H, S, P, applied.p, lin.p, s == 0, cfg.pgs_mode == CP_PGS_ISLAND_COLORED, stream);
-		} else {
-			// the largest class (and the global-memory one) on the world's
-			// stream, the two smaller classes beside it on the side streams
-			const int* thr = cfg.pgs_class_threads;
-			bool fork = !cls_lists[0].empty() || !cls_lists[1].empty();
-			if (fork) {
-				HIPCHK(hipEventRecord(fork_ev, stream));
-				for (auto& x : side) HIPCHK(hipStreamWaitEvent(x, fork_ev, 0));
-			}
-			launch_pgs_colored_lds(b, cb[cur].c, cls_dev[2].p, (int)cls_lists[2].size(), cls_max[2], scene_body_off.p, edge_range.p,
-			                       e_desc.p, e_lmask.p, color_off.p, ncolors.p, P, thr[2], stream);
-			if (!cls_lists[3].empty())
-				launch_pgs_colored_lds_global(b, P);
-			for (int c = 0; c < 2 && fork; c++)
-				launch_pgs_colored_lds(b, cb[cur].c, cls_dev[c].p, (int)cls_lists[c].size(), cls_max[c], scene_body_off.p, edge_range.p,
-				                       e_desc.p, e_lmask.p, color_off.p, ncolors.p, P, thr[c], side[c]);
-			if (fork)
-				for (int c = 0; c < 2; c++) {
-					HIPCHK(hipEventRecord(join_ev[c], side[c]));
-					HIPCHK(hipStreamWaitEvent(stream, join_ev[c], 0));
-				}
-		}
+		} else
+			colored_pgs(b, P);
 		if (tev) {
 			HIPCHK(hipEventRecord(pgs_ev[2 * npgs + 1], stream));
 			npgs++;
@@ -1484,6 +1560,7 @@ void cp_config_default(cp_config* c) {
 	c->pgs_class_threads[0] = 64;
 	c->pgs_class_threads[1] = 128;
 	c->pgs_class_threads[2] = 256;
+	c->graph = 0;
 	c->precision = 0;
 }
 
@@ -1506,7 +1583,8 @@ static bool valid_cfg(const cp_config* c) {
 	bool classes = lim[0] >= 1 && lim[0] < lim[1] && lim[1] < lim[2] && lim[2] <= 3072;
 	for (int k = 0; k < 3; k++) classes = classes && valid_threads(c->pgs_class_threads[k]);
 	return c->dt > 0 && c->substeps >= 1 && c->vel_iters >= 0 && c->pos_iters >= 0 && c->gyro_ticks >= 1 && valid_threads(c->solver_threads) &&
-	       (c->pgs_mode >= 0 && c->pgs_mode <= 2) && (c->precision == CP_PREC_FP64 || c->precision == CP_PREC_SPLIT) && classes;
+	       (c->pgs_mode >= 0 && c->pgs_mode <= 2) && (c->precision == CP_PREC_FP64 || c->precision == CP_PREC_SPLIT) && classes &&
+	       (c->graph == 0 || c->graph == 1);
 }
 
 int cp_world_create(const cp_config* cfg, cp_world** out) {
@@ -1541,6 +1619,8 @@ int cp_world_destroy(cp_world* w) {
 	w->free_all();
 	for (auto& e : w->ev) (void)hipEventDestroy(e);
 	for (auto& e : w->pgs_ev) (void)hipEventDestroy(e);
+	for (auto& g : w->graphs)
+		if (g.exec) (void)hipGraphExecDestroy(g.exec);
 	for (auto& x : w->side)
 		if (x) {
 			(void)hipStreamSynchronize(x);

@@ -78,6 +78,11 @@ typedef struct {
 	                            scenes use global memory.  Defaults 192, 768, 3072. */
 	int pgs_class_threads[3]; /* threads per scene workgroup of each LDS class (64..256, a multiple of
 	                            64).  Defaults 64, 128, 256.  Neither setting changes results. */
+	int graph;               /* 1: in CP_PGS_COLORED mode without the residual diagnostics, the
+	                            substep loop (rows, PGS, write-back, integration) is captured once in
+	                            a HIP graph and replayed each step (recaptured when buffers, sizes or
+	                            settings change); the per-solve PGS time is then not measured.
+	                            Default 0.  Does not change results. */
 } cp_config;
 
 typedef struct cp_world cp_world;
