@@ -113,6 +113,13 @@ struct IslandSched {
 	int n, nl;
 	const int *rows, *off, *sys, *sys_off, *jlin;
 	const int *eoff, *coff, *icoff, *serial;
+	// coloured islands with island-local velocity deltas in LDS: the island's
+	// dynamic bodies bod[boff[i], boff[i+1]) (local index = position), contact
+	// k's bodies as local indices cloc[2k], cloc[2k+1], joint k's jloc[2k],
+	// jloc[2k+1] (the zero column boff[i+1] - boff[i] for non-dynamic bodies);
+	// islands with more than lds_bodies - 1 bodies use b.dv.  cloc null: none.
+	const int *bod, *boff, *cloc, *jloc;
+	int lds_bodies;
 };
 
 // Two side streams forked from and joined back into a main stream with events

@@ -458,10 +458,25 @@ __device__ __forceinline__ void apply_dv(Bodies& b, const T* dv, int i) {
 // (island, body) (S.jlin).
 using DG = DvGlobal<double>;
 
-__device__ __forceinline__ void contact_refs(const Bodies& b, const Contacts& C, int k, DG& A, DG& B) {
+// Where an island solve keeps its velocity deltas: b.dv ([12][n], global body
+// indices) or an island-local LDS array ([12][stride], local indices from the
+// host's cloc / jloc maps; non-dynamic bodies share the never-written zero
+// column stride - 1).
+struct DvMap {
+	double* base;
+	int stride;
+	const int* cloc;  // contact k -> local (body 1, body 2); null: global indices
+	const int* jloc;  // joint k -> local (a, b)
+};
+
+__device__ __forceinline__ void contact_refs(const Bodies& b, const Contacts& C, const DvMap& m, int k, DG& A, DG& B) {
 	int i1 = (int)(C.key[k] >> 32), i2 = (int)(C.key[k] & 0xFFFFFFFFu);
-	A = DG{b.dv, i1, b.n, b.movement[i1] == 0};
-	B = DG{b.dv, i2, b.n, b.movement[i2] == 0};
+	int l1 = m.cloc ? m.cloc[2 * k] : i1, l2 = m.cloc ? m.cloc[2 * k + 1] : i2;
+	A = DG{m.base, l1, m.stride, b.movement[i1] == 0};
+	B = DG{m.base, l2, m.stride, b.movement[i2] == 0};
+}
+__device__ __forceinline__ int jl(const Joints& J, const DvMap& m, int k, int h) {
+	return m.jloc ? m.jloc[2 * k + h] : (h ? J.b[k] : J.a[k]);
 }
 
 // Joint target at solve time (findTargetConstraintValue, ConstraintSolver.h:78)
@@ -483,33 +498,33 @@ __device__ __forceinline__ void joint_target(const Bodies& b, const Joints& J, i
 }
 
 // PGSConstraintStep<n> of an equality constraint (ConstraintSolver.cpp:12-40).
-__device__ __forceinline__ void joint_step(const Bodies& b, const Joints& J, int k, bool pseudo) {
-	const int N = b.n, K = J.n;
+__device__ __forceinline__ void joint_step(const Bodies& b, const Joints& J, const DvMap& m, int k, bool pseudo) {
+	const int K = J.n, S = m.stride;
 	double* R = J.row + (size_t)k * CP_JRS;
 	int n = J.type[k], ia = J.a[k], ib = J.b[k];
-	double* va = b.dv + (pseudo ? 6 * N : 0) + ia;
-	double* vb = b.dv + (pseudo ? 6 * N : 0) + ib;
+	double* va = m.base + (pseudo ? 6 * S : 0) + jl(J, m, k, 0);
+	double* vb = m.base + (pseudo ? 6 * S : 0) + jl(J, m, k, 1);
 	double* acc = pseudo ? J.pimp : J.imp;
 	const double* tgt = R + (pseudo ? CP_J_PTGT : CP_J_TGT);
-	double cur[5], delta[5], m[5], diff[5], a[5];
-	joint_value(R, n, va, vb, N, cur);
+	double cur[5], delta[5], mm[5], diff[5], a[5];
+	joint_value(R, n, va, vb, S, cur);
 	for (int c = 0; c < n; c++) {
 		delta[c] = tgt[c] - cur[c];
 		a[c] = acc[c * K + k];
 	}
 	for (int i = 0; i < n; i++) {
-		m[i] = 0;
-		for (int c = 0; c < n; c++) m[i] += delta[c] * R[CP_J_INV + i * n + c];
+		mm[i] = 0;
+		for (int c = 0; c < n; c++) mm[i] += delta[c] * R[CP_J_INV + i * n + c];
 	}
-	for (int c = 0; c < n; c++) diff[c] = (a[c] + m[c]) - a[c];
+	for (int c = 0; c < n; c++) diff[c] = (a[c] + mm[c]) - a[c];
 	if (!is_zero_n(diff, n)) {
-		joint_apply(R, n, diff, va, vb, N, b.movement[ia] == 0, b.movement[ib] == 0);
+		joint_apply(R, n, diff, va, vb, S, b.movement[ia] == 0, b.movement[ib] == 0);
 		for (int c = 0; c < n; c++) acc[c * K + k] = a[c] + diff[c];
 	}
 }
 
 // One row of a phase: 0 warm start, 1 velocity sweep, 2 position sweep.
-__device__ __forceinline__ void island_row(const Bodies& b, const Contacts& C, const Joints& J, int k, int kind, int phase) {
+__device__ __forceinline__ void island_row(const Bodies& b, const Contacts& C, const Joints& J, const DvMap& m, int k, int kind, int phase) {
 	if (kind == 2) {
 		if (phase == 0) {
 			const int K = J.n;
@@ -517,22 +532,22 @@ __device__ __forceinline__ void island_row(const Bodies& b, const Contacts& C, c
 			int n = J.type[k];
 			for (int c = 0; c < n; c++) imp[c] = J.imp[c * K + k];
 			if (!is_zero_n(imp, n))
-				joint_apply(J.row + (size_t)k * CP_JRS, n, imp, b.dv + J.a[k], b.dv + J.b[k], b.n, b.movement[J.a[k]] == 0,
-				            b.movement[J.b[k]] == 0);
-		} else joint_step(b, J, k, phase == 2);
+				joint_apply(J.row + (size_t)k * CP_JRS, n, imp, m.base + jl(J, m, k, 0), m.base + jl(J, m, k, 1), m.stride,
+				            b.movement[J.a[k]] == 0, b.movement[J.b[k]] == 0);
+		} else joint_step(b, J, m, k, phase == 2);
 		return;
 	}
 	if (phase == 2 && kind != 0) return;
 	DG A, B;
-	contact_refs(b, C, k, A, B);
+	contact_refs(b, C, m, k, A, B);
 	contact_row(rec<double>(C, k), A, B, kind, phase);
 }
 
 // computeAndApplyImpulses (HolonomicBlockSolver.cpp:298-355) for the island's
 // systems in order, hol_iters times.
 __device__ void island_holonomic(const Bodies& b, const Joints& J, const HSysDev& H, const IslandSched& S, int t, const Params& P,
-                                 double* lin) {
-	const int N = b.n, K = J.n, NL = S.nl;
+                                 double* lin, const DvMap& m) {
+	const int K = J.n, NL = S.nl, DS = m.stride;
 	auto apply = [&](int s, bool pseudo, double thr) {
 		const int nc = H.nc[s];
 		const int* ord = H.order + H.ord_off[s];
@@ -541,12 +556,12 @@ __device__ void island_holonomic(const Bodies& b, const Joints& J, const HSysDev
 		double* d = H.scratch + H.dbuf_off[s] / 2;  // 3 * degree doubles per system
 		double* eff = d + deg;
 		double* tmp = eff + deg;
-		const int off = pseudo ? 6 * N : 0;
+		double* dv = m.base + (pseudo ? 6 * DS : 0);
 		for (int i = 0; i < nc; i++) {
 			int k = ord[i];
 			const double* R = J.row + (size_t)k * CP_JRS;
 			double cur[5];
-			joint_value(R, J.type[k], b.dv + off + J.a[k], b.dv + off + J.b[k], N, cur);
+			joint_value(R, J.type[k], dv + jl(J, m, k, 0), dv + jl(J, m, k, 1), DS, cur);
 			const double* tg = R + (pseudo ? CP_J_PTGT : CP_J_TGT);
 			for (int c = 0; c < J.type[k]; c++) d[vl[i] + c] = tg[c] - cur[c];
 		}
@@ -571,7 +586,7 @@ __device__ void island_holonomic(const Bodies& b, const Joints& J, const HSysDev
 			int k = ord[i];
 			int n = J.type[k];
 			if (is_zero_n(d + vl[i], n)) continue;
-			joint_apply(J.row + (size_t)k * CP_JRS, n, d + vl[i], b.dv + off + J.a[k], b.dv + off + J.b[k], N, b.movement[J.a[k]] == 0,
+			joint_apply(J.row + (size_t)k * CP_JRS, n, d + vl[i], dv + jl(J, m, k, 0), dv + jl(J, m, k, 1), DS, b.movement[J.a[k]] == 0,
 			            b.movement[J.b[k]] == 0);
 			double* acc = pseudo ? J.pimp : J.himp;
 			for (int c = 0; c < n; c++) acc[c * K + k] += d[vl[i] + c];
@@ -606,15 +621,16 @@ __global__ __launch_bounds__(64) void k_pgs_ordered(Bodies b, Contacts C, Joints
 	if (t >= S.n) return;
 	const int* rows = S.rows;
 	int r0 = S.off[t], r1 = S.off[t + 1];
+	const DvMap m{b.dv, b.n, nullptr, nullptr};
 	for (int r = r0; r < r1; r++)
 		if (rows[2 * r + 1] == 2) joint_target(b, J, rows[2 * r]);
-	for (int r = r0; r < r1; r++) island_row(b, C, J, rows[2 * r], rows[2 * r + 1], 0);
+	for (int r = r0; r < r1; r++) island_row(b, C, J, m, rows[2 * r], rows[2 * r + 1], 0);
 	for (int it = 0; it < P.vel_iters; it++)
-		for (int r = r0; r < r1; r++) island_row(b, C, J, rows[2 * r], rows[2 * r + 1], 1);
+		for (int r = r0; r < r1; r++) island_row(b, C, J, m, rows[2 * r], rows[2 * r + 1], 1);
 	for (int it = 0; it < P.pos_iters; it++)
-		for (int r = r0; r < r1; r++) island_row(b, C, J, rows[2 * r], rows[2 * r + 1], 2);
+		for (int r = r0; r < r1; r++) island_row(b, C, J, m, rows[2 * r], rows[2 * r + 1], 2);
 	if (first && P.hol_iters > 0 && S.sys_off[t + 1] > S.sys_off[t]) {
-		island_holonomic(b, J, H, S, t, P, lin);
+		island_holonomic(b, J, H, S, t, P, lin, m);
 		for (int r = r0; r < r1; r++)
 			if (rows[2 * r + 1] == 2) holonomic_warm_scale(J, rows[2 * r]);
 	}
@@ -639,11 +655,19 @@ __global__ __launch_bounds__(64) void k_pgs_ordered(Bodies b, Contacts C, Joints
 // colour may be a serial bucket (edges that found no colour), run by lane 0.
 __global__ __launch_bounds__(64) void k_pgs_islands(Bodies b, Contacts C, Joints J, HSysDev H, IslandSched S, Params P, int* applied,
                                                      double* lin, int first) {
+	extern __shared__ double sdv[];
 	const int t = blockIdx.x, lane = threadIdx.x;
 	const int* rows = S.rows;
 	const int r0 = S.off[t], r1 = S.off[t + 1];
 	const int c0 = S.icoff[t], c1 = S.icoff[t + 1];
 	const bool ser = S.serial[t] != 0;
+	// island-local velocity deltas in LDS when the island fits (its bodies
+	// plus a zero column for the non-dynamic ones), else b.dv
+	const int bo0 = S.boff[t], nbi = S.boff[t + 1] - bo0;
+	const bool local = S.cloc && nbi + 1 <= S.lds_bodies;
+	const DvMap m = local ? DvMap{sdv, nbi + 1, S.cloc, S.jloc} : DvMap{b.dv, b.n, nullptr, nullptr};
+	if (local)
+		for (int x = lane; x < 12 * (nbi + 1); x += 64) sdv[x] = 0;
 	for (int r = r0 + lane; r < r1; r += 64)
 		if (rows[2 * r + 1] == 2) joint_target(b, J, rows[2 * r]);
 	__syncthreads();
@@ -653,7 +677,7 @@ __global__ __launch_bounds__(64) void k_pgs_islands(Bodies b, Contacts C, Joints
 			bool serial = ser && c == c1 - 1;
 			int e = serial ? (lane == 0 ? e0 : e1) : e0 + lane;
 			for (; e < e1; e += serial ? 1 : 64)
-				for (int r = S.eoff[e]; r < S.eoff[e + 1]; r++) island_row(b, C, J, rows[2 * r], rows[2 * r + 1], phase);
+				for (int r = S.eoff[e]; r < S.eoff[e + 1]; r++) island_row(b, C, J, m, rows[2 * r], rows[2 * r + 1], phase);
 			__syncthreads();
 		}
 	};
@@ -661,10 +685,24 @@ __global__ __launch_bounds__(64) void k_pgs_islands(Bodies b, Contacts C, Joints
 	for (int it = 0; it < P.vel_iters; it++) sweep(1);
 	for (int it = 0; it < P.pos_iters; it++) sweep(2);
 	if (first && P.hol_iters > 0 && S.sys_off[t + 1] > S.sys_off[t]) {
-		if (lane == 0) island_holonomic(b, J, H, S, t, P, lin);
+		if (lane == 0) island_holonomic(b, J, H, S, t, P, lin, m);
 		__syncthreads();
 		for (int r = r0 + lane; r < r1; r += 64)
 			if (rows[2 * r + 1] == 2) holonomic_warm_scale(J, rows[2 * r]);
+	}
+	if (local) {
+		// applyVelocityChange (PhysicsEngine.cpp:1113-1136) for the island's dynamic bodies
+		const int N = b.n, DS = nbi + 1;
+		for (int j = lane; j < nbi; j += 64) {
+			int i = S.bod[bo0 + j];
+			for (int c = 0; c < 3; c++) {
+				b.vel[c * N + i] += sdv[c * DS + j];
+				b.angv[c * N + i] += sdv[(3 + c) * DS + j];
+				b.pvel[c * N + i] += sdv[(6 + c) * DS + j];
+				b.pang[c * N + i] += sdv[(9 + c) * DS + j];
+			}
+		}
+		return;
 	}
 	for (int r = r0 + lane; r < r1; r += 64) {
 		int ii[2];
@@ -859,7 +897,15 @@ void launch_build_rows(const Bodies& b, const Contacts& C, int nc, const Params&
 void launch_pgs_islands(const Bodies& b, const Contacts& C, const Joints& J, const HSysDev& H, const IslandSched& S, const Params& P,
                         int* applied, double* lin, int first, bool colored, hipStream_t s) {
 	if (!S.n) return;
-	if (colored) hipLaunchKernelGGL(k_pgs_islands, dim3(S.n), dim3(64), 0, s, b, C, J, H, S, P, applied, lin, first);
+	if (colored) {
+		static bool attr = false;
+		if (!attr) {
+			(void)hipFuncSetAttribute((const void*)k_pgs_islands, hipFuncAttributeMaxDynamicSharedMemorySize, 64 * 1024);
+			attr = true;
+		}
+		size_t lds = S.cloc ? (size_t)12 * S.lds_bodies * sizeof(double) : 0;
+		hipLaunchKernelGGL(k_pgs_islands, dim3(S.n), dim3(64), lds, s, b, C, J, H, S, P, applied, lin, first);
+	}
 	else hipLaunchKernelGGL(k_pgs_ordered, dim3(nblk(S.n, 64)), dim3(64), 0, s, b, C, J, H, S, P, applied, lin, first);
 }
 void launch_pgs_colored(const Bodies& b, const Contacts& C, const int* scene_body_off, int nscenes, const int* edge_range, const int4* desc,

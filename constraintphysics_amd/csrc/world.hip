@@ -143,6 +143,10 @@ void up_tail(DArr<T>& d, const std::vector<T>& h, size_t& done) {
 
 }  // namespace
 
+// Largest island (bodies + the zero column) whose velocity deltas the
+// coloured-island solve keeps in LDS: 12 doubles per column, 48 KB.
+#define ISL_LDS_BODIES 512
+
 struct cp_world {
 	cp_config cfg;
 	hipStream_t stream = nullptr;
@@ -243,7 +247,9 @@ struct cp_world {
 	DArr<double> dh_buf, dh_dbuf, dh_scratch;
 	bool hs_pending = true;
 	std::vector<int> last_isl_sys, last_isl_sys_off;
-	DArr<int> isl_jlin, isl_eoff, isl_coff, isl_icoff, isl_serial;
+	DArr<int> isl_jlin, isl_eoff, isl_coff, isl_icoff, isl_serial, isl_bod, isl_boff, isl_cloc, isl_jloc;
+	std::vector<int> loc_of, loc_stamp;  // schedule scratch: island-local body indices
+	int isl_lds_bodies = 0;              // LDS columns per island solve (largest fitting island + 1)
 	int lin_nl = 0;
 	std::vector<std::array<uint64_t, 2>> col_used;
 	// static meshes: one world instance per mesh body (triangles and AABB tree
@@ -413,7 +419,7 @@ void cp_world::free_all() {
 	d_nc.free_();
 	DArr<int>* jints[] = {&dj_type, &dj_a, &dj_b, &dj_sys, &dh_nc, &dh_ord_off, &dh_order, &dh_vloc, &dh_tab_off, &dh_table, &dh_degree,
 	                      &dh_buf_off, &dh_dbuf_off, &isl_sys, &isl_sys_off, &isl_jlin, &isl_eoff, &isl_coff, &isl_icoff,
-	                      &isl_serial};
+	                      &isl_serial, &isl_bod, &isl_boff, &isl_cloc, &isl_jloc};
 	for (auto* a : jints) a->free_();
 	DArr<double>* jdbl[] = {&dj_tp1, &dj_tp2, &dj_ax1, &dj_ax2, &dj_ps, &dj_rs, &dj_imp, &dj_pimp, &dj_himp, &dj_u, &dj_w, &dj_row, &lin,
 	                        &dh_buf, &dh_dbuf, &dh_scratch};
@@ -1192,7 +1198,15 @@ void cp_world::schedule_islands(bool colored) {
 	std::vector<int> eoff{0}, coff{0}, icoff{0}, serial, jlin(2 * nj, 0), isl_edges, col;
 	std::vector<std::pair<int, int>> slots;  // (non-dynamic body, lin column) of the current island
 	int nl = nb;
-	if (colored) col_used.assign(nb, {0, 0});
+	std::vector<int> bod, boff{0}, cloc, jloc;
+	int max_isl_bodies = 0;
+	if (colored) {
+		col_used.assign(nb, {0, 0});
+		loc_of.assign(nb, 0);
+		loc_stamp.assign(nb, -1);
+		cloc.assign(2 * (size_t)std::max(n, 1), 0);
+		jloc.assign(2 * (size_t)std::max(nj, 1), 0);
+	}
 	last_rows.clear();
 	last_offs.assign(1, 0);
 	last_isl_sys.clear();
@@ -1248,6 +1262,48 @@ void cp_world::schedule_islands(bool colored) {
 			icoff.push_back((int)coff.size() - 1);
 			serial.push_back(any_serial ? 1 : 0);
 		}
+		if (colored) {
+			// island-local indices of the dynamic bodies, in order of first
+			// appearance; non-dynamic bodies get the zero column (set below)
+			size_t bstart = bod.size();
+			std::vector<std::pair<int*, int>> fix;  // (slot, global body) of non-dynamic ends
+			for (size_t r = start; r < last_rows.size() / 2; r++) {
+				int k = (int)last_rows[2 * r], kind = (int)last_rows[2 * r + 1];
+				int ends[2];
+				int* slot;
+				if (kind == 2) {
+					ends[0] = j_a[k]; ends[1] = j_b[k];
+					slot = &jloc[2 * k];
+				} else {
+					if (kind == 1) continue;  // the normal row covered the contact
+					ends[0] = (int)(key[k] >> 32); ends[1] = (int)(key[k] & 0xFFFFFFFFu);
+					slot = &cloc[2 * k];
+				}
+				for (int h = 0; h < 2; h++) {
+					int i = ends[h];
+					if (b_move[i] != 0) {
+						slot[h] = -1;
+						continue;
+					}
+					if (loc_stamp[i] != (int)last_offs.size()) {
+						loc_stamp[i] = (int)last_offs.size();
+						loc_of[i] = (int)(bod.size() - bstart);
+						bod.push_back(i);
+					}
+					slot[h] = loc_of[i];
+				}
+			}
+			const int nbi = (int)(bod.size() - bstart);
+			for (size_t r = start; r < last_rows.size() / 2; r++) {
+				int k = (int)last_rows[2 * r], kind = (int)last_rows[2 * r + 1];
+				if (kind == 1) continue;
+				int* slot = kind == 2 ? &jloc[2 * k] : &cloc[2 * k];
+				for (int h = 0; h < 2; h++)
+					if (slot[h] < 0) slot[h] = nbi;
+			}
+			boff.push_back((int)bod.size());
+			max_isl_bodies = std::max(max_isl_bodies, nbi + 1);
+		}
 		// systems in order of first appearance; linearization-error columns
 		const int isl_id = (int)last_offs.size();
 		slots.clear();
@@ -1286,6 +1342,12 @@ void cp_world::schedule_islands(bool colored) {
 		up(isl_coff, coff);
 		up(isl_icoff, icoff);
 		up(isl_serial, serial);
+		if (bod.empty()) bod.push_back(0);
+		up(isl_bod, bod);
+		up(isl_boff, boff);
+		up(isl_cloc, cloc);
+		up(isl_jloc, jloc);
+		isl_lds_bodies = std::min(max_isl_bodies, ISL_LDS_BODIES);
 	}
 }
 
@@ -1469,8 +1531,11 @@ void cp_world::step() {
 			HIPCHK(hipMemsetAsync(dv.p, 0, 12 * (size_t)nb * sizeof(double), stream));
 			HIPCHK(hipMemsetAsync(applied.p, 0, (size_t)nb * sizeof(int), stream));
 			if (hol && s == 0) HIPCHK(hipMemsetAsync(lin.p, 0, 6 * (size_t)lin_nl * sizeof(double), stream));
+			const bool icol = cfg.pgs_mode == CP_PGS_ISLAND_COLORED;
 			IslandSched S{nisl, lin_nl, isl_rows.p, isl_off.p, isl_sys.p, isl_sys_off.p, isl_jlin.p,
-			              isl_eoff.p, isl_coff.p, isl_icoff.p, isl_serial.p};
+			              isl_eoff.p, isl_coff.p, isl_icoff.p, isl_serial.p,
+			              icol ? isl_bod.p : nullptr, icol ? isl_boff.p : nullptr, icol ? isl_cloc.p : nullptr,
+			              icol ? isl_jloc.p : nullptr, icol ? isl_lds_bodies : 0};
 			launch_pgs_islands(b, cb[cur].c, J, H, S, P, applied.p, lin.p, s == 0, cfg.pgs_mode == CP_PGS_ISLAND_COLORED, stream);
 		} else
 			colored_pgs(b, P);
