@@ -99,6 +99,10 @@ struct HSysDev {
 	int *nc, *ord_off, *order, *vloc, *tab_off, *table, *degree, *buf_off, *dbuf_off;
 	double *buf, *dbuf;
 	double* scratch;  // 3 * degree per system (at dbuf_off / 2): solve vectors
+	// per system s and column c (in elimination order): the non-empty blocks
+	// below the diagonal, rows ascending, as (cs_row, cs_loc)[cs_ptr[cp_off[s]
+	// + c], cs_ptr[cp_off[s] + c + 1]); the diagonal block at dg_loc[ord_off[s] + c]
+	int *cp_off, *cs_ptr, *cs_row, *cs_loc, *dg_loc;
 };
 
 // Island schedule (reference-order and coloured-island solves).  rows holds

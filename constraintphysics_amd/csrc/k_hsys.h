@@ -50,36 +50,39 @@ __device__ __forceinline__ bool is_zero_n(const double* v, int n) {  // NVec::is
 	return true;
 }
 
-// multInverseWithVector (:223-296) in place on v (degree entries); eff and d
-// are scratch vectors of the same length.
+// multInverseWithVector (:223-296) in place on v (degree entries); d is a
+// scratch vector of the same length.  The reference zeroes a full-degree
+// vector per column, fills the rows of the column's non-empty blocks and
+// subtracts the whole vector; here only the non-empty blocks are visited (the
+// skipped entries subtract an exact zero, which leaves every value as it is).
 __device__ void hs_solve(const Joints& J, const HSysDev& H, int s, double* v, double* eff, double* d) {
+	(void)eff;
 	const int nc = H.nc[s];
 	const int deg = H.degree[s];
 	const int* ord = H.order + H.ord_off[s];
 	const int* vl = H.vloc + H.ord_off[s];
+	const int* cp = H.cs_ptr + H.cp_off[s];
+	const int* dg = H.dg_loc + H.ord_off[s];
 	const double* buf = H.buf + H.buf_off[s];
 	for (int col = 0; col + 1 < nc; col++) {
-		for (int i = 0; i < deg; i++) eff[i] = 0;
-		for (int row = col + 1; row < nc; row++) {
-			int l = hs_loc(H, s, row, col);
-			if (l == -1) continue;
-			int bw = J.type[ord[col]], bh = J.type[ord[row]];
-			for (int i = 0; i < bh; i++) eff[vl[row] + i] = dotp(bw, buf + l + i * bw, v + vl[col]);
+		int bw = J.type[ord[col]];
+		for (int e = cp[col]; e < cp[col + 1]; e++) {
+			int row = H.cs_row[e], l = H.cs_loc[e];
+			int bh = J.type[ord[row]];
+			for (int i = 0; i < bh; i++) v[vl[row] + i] -= dotp(bw, buf + l + i * bw, v + vl[col]);
 		}
-		for (int i = 0; i < deg; i++) v[i] -= eff[i];
 	}
 	for (int col = 0; col < nc; col++) {
 		int bw = J.type[ord[col]];
-		const double* m = buf + hs_loc(H, s, col, col);
+		const double* m = buf + dg[col];
 		for (int i = 0; i < bw; i++) d[vl[col] + i] = dotp(bw, m + i * bw, v + vl[col]);
 	}
 	for (int i = 0; i < deg; i++) v[i] = d[i];
 	for (int col = nc - 2; col >= 0; col--) {
 		double lt[6] = {0, 0, 0, 0, 0, 0};
 		int bw = J.type[ord[col]];
-		for (int row = col + 1; row < nc; row++) {
-			int l = hs_loc(H, s, row, col);
-			if (l == -1) continue;
+		for (int e = cp[col]; e < cp[col + 1]; e++) {
+			int row = H.cs_row[e], l = H.cs_loc[e];
 			int bh = J.type[ord[row]];
 			for (int i = 0; i < bw; i++) {
 				double dp = 0;

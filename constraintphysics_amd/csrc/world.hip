@@ -244,6 +244,7 @@ struct cp_world {
 	};
 	std::vector<HLayout> hs_lay;
 	DArr<int> dh_nc, dh_ord_off, dh_order, dh_vloc, dh_tab_off, dh_table, dh_degree, dh_buf_off, dh_dbuf_off, isl_sys, isl_sys_off;
+	DArr<int> dh_cp_off, dh_cs_ptr, dh_cs_row, dh_cs_loc, dh_dg_loc;
 	DArr<double> dh_buf, dh_dbuf, dh_scratch;
 	bool hs_pending = true;
 	std::vector<int> last_isl_sys, last_isl_sys_off;
@@ -356,6 +357,7 @@ struct cp_world {
 		H.nc = dh_nc.p; H.ord_off = dh_ord_off.p; H.order = dh_order.p; H.vloc = dh_vloc.p; H.tab_off = dh_tab_off.p; H.table = dh_table.p;
 		H.degree = dh_degree.p; H.buf_off = dh_buf_off.p; H.dbuf_off = dh_dbuf_off.p; H.buf = dh_buf.p; H.dbuf = dh_dbuf.p;
 		H.scratch = dh_scratch.p;
+		H.cp_off = dh_cp_off.p; H.cs_ptr = dh_cs_ptr.p; H.cs_row = dh_cs_row.p; H.cs_loc = dh_cs_loc.p; H.dg_loc = dh_dg_loc.p;
 		return H;
 	}
 	MeshDev meshdev() {
@@ -419,7 +421,8 @@ void cp_world::free_all() {
 	d_nc.free_();
 	DArr<int>* jints[] = {&dj_type, &dj_a, &dj_b, &dj_sys, &dh_nc, &dh_ord_off, &dh_order, &dh_vloc, &dh_tab_off, &dh_table, &dh_degree,
 	                      &dh_buf_off, &dh_dbuf_off, &isl_sys, &isl_sys_off, &isl_jlin, &isl_eoff, &isl_coff, &isl_icoff,
-	                      &isl_serial, &isl_bod, &isl_boff, &isl_cloc, &isl_jloc};
+	                      &isl_serial, &isl_bod, &isl_boff, &isl_cloc, &isl_jloc, &dh_cp_off, &dh_cs_ptr,
+	                      &dh_cs_row, &dh_cs_loc, &dh_dg_loc};
 	for (auto* a : jints) a->free_();
 	DArr<double>* jdbl[] = {&dj_tp1, &dj_tp2, &dj_ax1, &dj_ax2, &dj_ps, &dj_rs, &dj_imp, &dj_pimp, &dj_himp, &dj_u, &dj_w, &dj_row, &lin,
 	                        &dh_buf, &dh_dbuf, &dh_scratch};
@@ -593,6 +596,7 @@ void cp_world::upload_joints() {
 void cp_world::layout_systems() {
 	hs_lay.assign(hs_og.size(), HLayout{});
 	std::vector<int> nc, ord_off{0}, order, vloc, tab_off{0}, table, degree, buf_off{0}, dbuf_off{0};
+	std::vector<int> cp_off, cs_ptr, cs_row, cs_loc, dg_loc;  // per column: the non-empty blocks below the diagonal
 	for (size_t q = 0; q < hs_og.size(); q++) {
 		const std::vector<int>& og = hs_og[q];
 		HLayout& L = hs_lay[q];
@@ -636,17 +640,26 @@ void cp_world::layout_systems() {
 				}
 		}
 		for (int x : pos) L.order.push_back(og[x]);
+		cp_off.push_back((int)cs_ptr.size());
 		for (int col = 0; col < n; col++) {
 			L.vloc.push_back(L.degree);
 			L.degree += j_type[L.order[col]];
+			cs_ptr.push_back((int)cs_row.size());
 			for (int row = col; row < n; row++) {
-				if (!filled[(size_t)pos[row] * n + pos[col]]) L.table.push_back(-1);
-				else {
-					L.table.push_back(L.cap);
+				int l = -1;
+				if (filled[(size_t)pos[row] * n + pos[col]]) {
+					l = L.cap;
 					L.cap += j_type[L.order[row]] * j_type[L.order[col]];
+				}
+				L.table.push_back(l);
+				if (row == col) dg_loc.push_back(l);
+				else if (l != -1) {
+					cs_row.push_back(row);
+					cs_loc.push_back(l);
 				}
 			}
 		}
+		cs_ptr.push_back((int)cs_row.size());
 		nc.push_back(n);
 		order.insert(order.end(), L.order.begin(), L.order.end());
 		vloc.insert(vloc.end(), L.vloc.begin(), L.vloc.end());
@@ -659,6 +672,11 @@ void cp_world::layout_systems() {
 	}
 	up(dh_nc, nc); up(dh_ord_off, ord_off); up(dh_order, order); up(dh_vloc, vloc); up(dh_tab_off, tab_off); up(dh_table, table);
 	up(dh_degree, degree); up(dh_buf_off, buf_off); up(dh_dbuf_off, dbuf_off);
+	if (cs_row.empty()) { cs_row.push_back(0); cs_loc.push_back(0); }
+	if (cp_off.empty()) cp_off.push_back(0);
+	if (cs_ptr.empty()) cs_ptr.push_back(0);
+	if (dg_loc.empty()) dg_loc.push_back(0);
+	up(dh_cp_off, cp_off); up(dh_cs_ptr, cs_ptr); up(dh_cs_row, cs_row); up(dh_cs_loc, cs_loc); up(dh_dg_loc, dg_loc);
 	dh_buf.reserve(std::max(buf_off.back(), 1));
 	dh_dbuf.reserve(std::max(dbuf_off.back(), 1));
 	dh_scratch.reserve(std::max(dbuf_off.back() / 2, 1));
