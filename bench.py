@@ -1,7 +1,7 @@
 """Benchmark: batched contact-pipeline steps on one MI355X.
 
-Workload (BASELINE metric "body-steps/s vs plain GPU path at stated FP64
-residual; p50/p99 step latency", batched-scenes configuration): independent
+Workload (metric "body-steps/s vs plain GPU path at stated FP64 residual;
+p50/p99 step latency", batched-scenes configuration): independent
 scenes with body counts log-uniform in [8, 2048], each in its own fixed box
 container, seeded per scene (base + i); 10 PGS velocity sweeps, 1 position
 sweep, 8 substeps, dt = 1/120 s.  The primitive kinds drawn are the ones the
@@ -227,7 +227,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: --gpus {args.gpus} is not supported (scenes would shard by HBM footprint "
-              f"across GPUs with no per-step exchange; see DESIGN.md)", file=sys.stderr)
+              f"across GPUs with no per-step exchange)", file=sys.stderr)
         sys.exit(2)
 
     if args.c2_only:

@@ -1,6 +1,6 @@
 """Builds libcp_hip.so (all HIP kernels + host runtime) in-tree for gfx950.
 
-Usage: python -m constraintphysics_amd.build   (or __graft_entry__.build()).
+Usage: python -m constraintphysics_amd.build
 """
 import os
 import subprocess
