@@ -9,9 +9,11 @@
 // test" is reproduced exactly by a wave ballot for separation and a
 // lexicographic (depth, axis index) minimum across lanes.
 //
-// The contact feature search, 2-D Sutherland-Hodgman clipping with feature-id
-// (MagicID) bookkeeping, lift to 3-D and cull to four points (CollisionDetect.cpp
-// :237-280, :392-620, :809-831, :2795-2830) are inherently serial and short;
+// The contact feature search of a hull (CollisionDetect.cpp:237-280) runs on
+// the whole wave (adjacent faces and edges tested side by side, a first-minimum
+// reduction picks the reference's feature).  The 2-D Sutherland-Hodgman
+// clipping with feature-id (MagicID) bookkeeping, lift to 3-D and cull to four
+// points (:392-620, :809-831, :2795-2830) are inherently serial and short;
 // lane 0 runs them on polygons staged in LDS.
 //
 // Roofline: per pair the algorithmic traffic is 2 poses + pair key + the
@@ -189,6 +191,79 @@ __device__ int contact_area(const Hull& h, V3 n, V3 p, int pid, V3 u, V3 w, doub
 	out[0].x = dot(p, u);
 	out[0].y = dot(p, w);
 	out[0].src = pid;
+	return 1;
+}
+
+__device__ __forceinline__ void wave_fence() {
+	__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+	__builtin_amdgcn_wave_barrier();
+}
+
+// (value, position) minimum across the wave; every lane ends with the result.
+__device__ __forceinline__ void wave_first_min(double* v, int* k) {
+	for (int off = 32; off >= 1; off >>= 1) {
+		double ov = __shfl_xor(*v, off);
+		int ok = __shfl_xor(*k, off);
+		if (ov < *v || (ov == *v && ok < *k)) { *v = ov; *k = ok; }
+	}
+}
+
+// contact_area with the whole wave: every lane calls it with the same
+// arguments and gets the same count and surface; out[] is complete after a
+// wave_fence.  Lanes test the vertex's adjacent faces (then edges) side by
+// side, each keeping its first strict minimum over its own positions; the
+// (value, position) minimum across lanes is then the feature the sequential
+// loop picks (first strict minimum below 1, NaNs never taken).  The chosen
+// face's points are projected one per lane.
+__device__ int contact_area_wave(const Hull& h, V3 n, V3 p, int pid, V3 u, V3 w, double cos_tol, double sin_tol, CEP* out, int* surface,
+                                 int lane) {
+	const int gv = h.p0 + pid;
+	double bt = 1.0;
+	int bk = 0x7FFFFFFF;
+	for (int k = h.vaf_off[gv] + lane; k < h.vaf_off[gv + 1]; k += 64) {
+		double t = 1 - dot(face_normal(h, h.vaf[k]), n);
+		if (t < bt) { bt = t; bk = k; }
+	}
+	wave_first_min(&bt, &bk);
+	if (bk != 0x7FFFFFFF && bt <= cos_tol) {
+		int f = h.vaf[bk];
+		int o = h.fv_off[h.f0 + f], e = h.fv_off[h.f0 + f + 1];
+		int cnt = e - o < MAXP ? e - o : MAXP;
+		for (int k = lane; k < cnt; k += 64) {
+			int vi = h.fv[o + k];
+			V3 v = ldv(h.wp, vi);
+			out[k].x = dot(v, u);
+			out[k].y = dot(v, w);
+			out[k].src = vi;
+		}
+		*surface = h.sid[f];
+		return cnt;
+	}
+	double be = 1.0;
+	int ke = 0x7FFFFFFF;
+	for (int k = h.vae_off[gv] + lane; k < h.vae_off[gv + 1]; k += 64) {
+		int e = h.vae[k];
+		V3 e1 = ldv(h.wp, h.edges[2 * e]), e2 = ldv(h.wp, h.edges[2 * e + 1]);
+		double s = fabs(dot(normalize(e2 - e1), n));
+		if (s < be) { be = s; ke = k; }
+	}
+	wave_first_min(&be, &ke);
+	*surface = -1;
+	if (ke != 0x7FFFFFFF && be <= sin_tol) {
+		if (lane < 2) {
+			int vi = h.edges[2 * h.vae[ke] + lane];
+			V3 v = ldv(h.wp, vi);
+			out[lane].x = dot(v, u);
+			out[lane].y = dot(v, w);
+			out[lane].src = vi;
+		}
+		return 2;
+	}
+	if (lane == 0) {
+		out[0].x = dot(p, u);
+		out[0].y = dot(p, w);
+		out[0].src = pid;
+	}
 	return 1;
 }
 
@@ -424,19 +499,24 @@ __device__ void emit_manifold(const CEP* res, int nr, V3 n, V3 u, V3 w, V3 amax,
 	M.maxpen[slot] = maxpen;
 }
 
-// Serial contact generation for a hull pair after SAT (lane 0).
+// Contact generation for a hull pair after SAT, called by the whole wave with
+// the same mp: the two contact areas are found by the wave, the clipping and
+// the manifold write run on lane 0.
 __device__ void hull_contacts(const Hull& A, const Hull& B, const Check& mp, const Params& P, uint64_t cid, CEP* lds, int slot,
-                              Manifolds& M, int cap) {
+                              Manifolds& M, int cap, int lane) {
 	V3 n = mp.n;
 	V3 amax = ldv(A.wp, mp.a_pid), bmax = ldv(B.wp, mp.b_pid);
 	V3 u, w;
 	perp_basis(n, &u, &w);
 	int sa, sb;
-	int na = contact_area(A, n, amax, mp.a_pid, u, w, P.cos_tol, P.sin_tol, lds, &sa);
-	int nb = contact_area(B, n * -1.0, bmax, mp.b_pid, u, w, P.cos_tol, P.sin_tol, lds + MAXP, &sb);
-	int nr;
-	CEP* res = clip_areas(lds, na, sa, nb, sb, &nr);
-	emit_manifold(res, nr, n, u, w, amax, mp.pen, cid, false, P, slot, M, cap);
+	int na = contact_area_wave(A, n, amax, mp.a_pid, u, w, P.cos_tol, P.sin_tol, lds, &sa, lane);
+	int nb = contact_area_wave(B, n * -1.0, bmax, mp.b_pid, u, w, P.cos_tol, P.sin_tol, lds + MAXP, &sb, lane);
+	wave_fence();
+	if (lane == 0) {
+		int nr;
+		CEP* res = clip_areas(lds, na, sa, nb, sb, &nr);
+		emit_manifold(res, nr, n, u, w, amax, mp.pen, cid, false, P, slot, M, cap);
+	}
 }
 
 // ------------------------------------------------------------- spheres
@@ -776,15 +856,19 @@ __device__ __forceinline__ uint64_t pair_cid(int ia, int ib) {
 }
 
 // Contact generation of SAT_PolyCylinder (CollisionDetect.cpp:1652-1709) after
-// SAT, lane 0; swapped when the cylinder is the pair's first body.
+// SAT, called by the whole wave with the same mp: the hull's contact area is
+// found by the wave, the rest runs on lane 0; swapped when the cylinder is the
+// pair's first body.
 __device__ void poly_cyl_contacts(const Hull& A, const Cyl& B, const Check& mp, const Params& P, int ip, int ic, bool swapped, CEP* lds,
-                                  int slot, Manifolds& M, int cap) {
+                                  int slot, Manifolds& M, int cap, int lane) {
 	V3 n = mp.n;
 	V3 amax = ldv(A.wp, mp.a_pid);
 	V3 u, w;
 	perp_basis(n, &u, &w);
 	int sa, sb, ob;
-	int na = contact_area(A, n, amax, mp.a_pid, u, w, P.cos_tol, P.sin_tol, lds, &sa);
+	int na = contact_area_wave(A, n, amax, mp.a_pid, u, w, P.cos_tol, P.sin_tol, lds, &sa, lane);
+	wave_fence();
+	if (lane != 0) return;
 	int nb = cyl_contact_area(B, -n, u, w, P.cos_tol, P.sin_tol, lds + MAXP, &sb, &ob);
 	CEP* res;
 	int nr;
@@ -902,11 +986,6 @@ __device__ void sanitize_slot(Manifolds& M, int slot, int cap) {
 // pair, the SAT classes one wave per pair.
 enum { NPC_HH = 0, NPC_HS, NPC_SS, NPC_HC, NPC_SC, NPC_CC, NPC_MESH, NPC_N };
 
-__device__ __forceinline__ void wave_fence() {
-	__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-	__builtin_amdgcn_wave_barrier();
-}
-
 __global__ void k_np_zero(int* counts) {
 	if (threadIdx.x < NPC_N) counts[threadIdx.x] = 0;
 }
@@ -973,38 +1052,46 @@ __global__ __launch_bounds__(64 * CLS_WAVES) void k_np_classify(Bodies b, Shapes
 }
 
 #define CAND_K 4
-// Wave-wide SAT with compaction: each round prunes 64*CAND_K candidate axes
-// (cheap tests), packs the survivors into cand[] in axis order, then
-// evaluates them 64 at a time, so the extrema queries keep every lane busy.
-// A separating axis anywhere ends the test (returns false); otherwise *bi is
+// Wave-wide SAT with compaction: the cheap tests (pruning) run on 64 axes at
+// a time and the survivors are appended to cand[] in axis order; whenever 64
+// or more are waiting, whole batches of 64 are evaluated (the remainder moves
+// to the front), and the tail is evaluated at the end, so the extrema queries
+// keep every lane busy.  Each lane sees its axes in increasing order.  A
+// separating axis anywhere ends the test (returns false); otherwise *bi is
 // the first strict minimum (ties to the lower axis index), as in the
-// reference's sequential loops.
+// reference's sequential loops.  cand[] needs 128 entries.
 template <class Prune, class Eval>
 __device__ bool wave_sat_compact(int naxes, int lane, int* cand, Prune prune, Eval eval, int* bi_out) {
 	double bp = __builtin_inf();
 	int bi = 0x7FFFFFFF;
 	const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-	for (int base = 0; base < naxes; base += 64 * CAND_K) {
-		int nc = 0;
-		for (int j = 0; j < CAND_K; j++) {
-			int t = base + j * 64 + lane;
-			bool v = t < naxes && prune(t);
-			uint64_t m = __ballot(v);
-			if (v) cand[nc + __popcll(m & below)] = t;
-			nc += __popcll(m);
-		}
+	int nc = 0;
+	for (int base = 0; base < naxes; base += 64) {
+		int t = base + lane;
+		bool v = t < naxes && prune(t);
+		uint64_t m = __ballot(v);
+		if (v) cand[nc + __popcll(m & below)] = t;
+		nc += __popcll(m);
+		const bool last = base + 64 >= naxes;
+		if (nc < 64 && !(last && nc > 0)) continue;
 		wave_fence();
-		for (int c0 = 0; c0 < nc; c0 += 64) {
+		const int full = last ? nc : (nc & ~63);
+		for (int c0 = 0; c0 < full; c0 += 64) {
 			int idx = c0 + lane;
 			int sep = 0;
-			if (idx < nc) {
-				int t = cand[idx];
-				Check x = eval(t);
+			if (idx < full) {
+				int a = cand[idx];
+				Check x = eval(a);
 				if (x.pen < 0) sep = 1;
-				if (x.pen < bp) { bp = x.pen; bi = t; }
+				if (x.pen < bp) { bp = x.pen; bi = a; }
 			}
 			if (__ballot(sep) != 0) return false;
 		}
+		const int rem = nc - full;  // < 64
+		int r = lane < rem ? cand[full + lane] : 0;
+		wave_fence();
+		if (lane < rem) cand[lane] = r;
+		nc = rem;
 		wave_fence();
 	}
 	for (int off = 32; off >= 1; off >>= 1) {
@@ -1055,10 +1142,19 @@ __device__ __forceinline__ bool hh_prune(const Hull& A, const Hull& B, int t) {
 	return arc_pair(ldv(A.wn, A.arcs[2 * i1]), ldv(A.wn, A.arcs[2 * i1 + 1]), -ldv(B.wn, B.arcs[2 * i2]), -ldv(B.wn, B.arcs[2 * i2 + 1]));
 }
 
-struct WaveLds {
-	CEP cep[4 * MAXP];
+// A wave's LDS slice.  The axis search (candidate list, staged arcs) and the
+// contact generation (clip polygons) never overlap in time, so they share
+// storage; a wave_fence separates the last axis-search read from the first
+// clip-polygon write.
+struct SatLds {
 	int cand[64 * CAND_K];
 	int arca[2 * ST_ARC], arcb[2 * ST_ARC];
+};
+struct WaveLds {
+	union {
+		SatLds sat;
+		CEP cep[3 * MAXP];
+	};
 	double pa[3 * ST_PTS], pb[3 * ST_PTS], na[3 * ST_NRM], nb[3 * ST_NRM];
 };
 
@@ -1150,24 +1246,27 @@ __global__ __launch_bounds__(64 * NP_WAVES) void k_np_hull_hull(Bodies b, Shapes
 		Hull A = make_hull(b, sh, ia), B = make_hull(b, sh, ib);
 		stage_hull(A, L.pa, L.na, lane);
 		stage_hull(B, L.pb, L.nb, lane);
-		stage_arcs(A, L.arca, lane);
-		stage_arcs(B, L.arcb, lane);
+		stage_arcs(A, L.sat.arca, lane);
+		stage_arcs(B, L.sat.arcb, lane);
 		int naxes = A.nf + B.nf + A.na * B.na;
 		int bi;
-		bool hit = B.na <= 128 ? wave_sat_hh(A, B, lane, L.cand, &bi)
+		bool hit = B.na <= 128 ? wave_sat_hh(A, B, lane, L.sat.cand, &bi)
 		                       : wave_sat_compact(
-		                             naxes, lane, L.cand, [&](int t) { return hh_prune(A, B, t); },
+		                             naxes, lane, L.sat.cand, [&](int t) { return hh_prune(A, B, t); },
 		                             [&](int t) {
 			                             bool v;
 			                             return eval_axis(A, B, t, &v);
 		                             },
 		                             &bi);
-		if (hit && lane == 0) {
+		if (hit) {  // bi is wave-uniform
 			bool valid;
 			Check mp = eval_axis(A, B, bi, &valid);
-			hull_contacts(A, B, mp, P, pair_cid(ia, ib), L.cep, slot, M, cap);
-			write_mat(b, ia, ib, M, slot, cap);
-			if (P.defect_fixes & 1) sanitize_slot(M, slot, cap);
+			wave_fence();
+			hull_contacts(A, B, mp, P, pair_cid(ia, ib), L.cep, slot, M, cap, lane);
+			if (lane == 0) {
+				write_mat(b, ia, ib, M, slot, cap);
+				if (P.defect_fixes & 1) sanitize_slot(M, slot, cap);
+			}
 		}
 		wave_fence();
 	}
@@ -1190,7 +1289,7 @@ __device__ __forceinline__ bool hc_prune(const Hull& A, const Cyl& B, int t) {
 	return arc_pair(ldv(A.wn, A.arcs[2 * i1]), ldv(A.wn, A.arcs[2 * i1 + 1]), -cyl_gv(B, k & 1), -cyl_gv(B, k >> 1));
 }
 
-__global__ __launch_bounds__(64 * NP_WAVES) void k_np_hull_cyl(Bodies b, Shapes sh, Params P, const uint64_t* pairs, const int* list,
+__global__ __launch_bounds__(64 * NP_WAVES) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_np_hull_cyl(Bodies b, Shapes sh, Params P, const uint64_t* pairs, const int* list,
                                                                const int* counts, Manifolds M, int cap) {
 	__shared__ WaveLds lds[NP_WAVES];
 	const int lane = threadIdx.x & 63, wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
@@ -1206,23 +1305,26 @@ __global__ __launch_bounds__(64 * NP_WAVES) void k_np_hull_cyl(Bodies b, Shapes 
 		Hull A = make_hull(b, sh, ip);
 		Cyl B = make_cyl(b, sh, ic);
 		stage_hull(A, L.pa, L.na, lane);
-		stage_arcs(A, L.arca, lane);
+		stage_arcs(A, L.sat.arca, lane);
 		stage_cyl(B, L.nb, lane);
 		int naxes = A.nf + 1 + A.ne + A.np + A.na * 24;
 		int bi;
 		bool hit = wave_sat_compact(
-		    naxes, lane, L.cand, [&](int t) { return hc_prune(A, B, t); },
+		    naxes, lane, L.sat.cand, [&](int t) { return hc_prune(A, B, t); },
 		    [&](int t) {
 			    bool v;
 			    return eval_axis_pc(A, B, t, &v);
 		    },
 		    &bi);
-		if (hit && lane == 0) {
+		if (hit) {  // bi is wave-uniform
 			bool valid;
 			Check mp = eval_axis_pc(A, B, bi, &valid);
-			poly_cyl_contacts(A, B, mp, P, ip, ic, swapped, L.cep, slot, M, cap);
-			write_mat(b, ip, ic, M, slot, cap);
-			if (P.defect_fixes & 1) sanitize_slot(M, slot, cap);
+			wave_fence();
+			poly_cyl_contacts(A, B, mp, P, ip, ic, swapped, L.cep, slot, M, cap, lane);
+			if (lane == 0) {
+				write_mat(b, ip, ic, M, slot, cap);
+				if (P.defect_fixes & 1) sanitize_slot(M, slot, cap);
+			}
 		}
 		wave_fence();
 	}
@@ -1245,7 +1347,7 @@ __device__ __forceinline__ bool cc_prune(const Cyl& A, const Cyl& B, int t) {
 	return arc_pair(cyl_gv(A, k1 & 1), cyl_gv(A, k1 >> 1), -cyl_gv(B, k2 & 1), -cyl_gv(B, k2 >> 1));
 }
 
-__global__ __launch_bounds__(64 * NP_WAVES) void k_np_cyl_cyl(Bodies b, Shapes sh, Params P, const uint64_t* pairs, const int* list,
+__global__ __launch_bounds__(64 * NP_WAVES) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_np_cyl_cyl(Bodies b, Shapes sh, Params P, const uint64_t* pairs, const int* list,
                                                               const int* counts, Manifolds M, int cap) {
 	__shared__ WaveLds lds[NP_WAVES];
 	const int lane = threadIdx.x & 63, wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
@@ -1261,12 +1363,13 @@ __global__ __launch_bounds__(64 * NP_WAVES) void k_np_cyl_cyl(Bodies b, Shapes s
 		stage_cyl(B, L.nb, lane);
 		int bi;
 		bool hit = wave_sat_compact(
-		    35 + 24 * 24, lane, L.cand, [&](int t) { return cc_prune(A, B, t); },
+		    35 + 24 * 24, lane, L.sat.cand, [&](int t) { return cc_prune(A, B, t); },
 		    [&](int t) {
 			    bool v;
 			    return eval_axis_cc(A, B, t, &v);
 		    },
 		    &bi);
+		wave_fence();
 		if (hit && lane == 0) {
 			bool valid;
 			Check mp = eval_axis_cc(A, B, bi, &valid);
