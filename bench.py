@@ -31,13 +31,17 @@ HBM_PEAK_GBS = 8000.0
 
 
 def pgs_bytes_per_contact(vel_iters, pos_iters, word=8):
-    """Algorithmic FP64 bytes one PGS launch moves per live contact point:
-    warm start (M^-1 J^T of 3 rows + 3 impulses), per velocity sweep the normal
-    row (J 12, M^-1 J^T 12, inverse, target, impulse r/w = 28 doubles) and the
-    friction pair (J 24, M^-1 J^T 24, inverse 4, targets 2, impulses r/w 4,
-    mu, normal impulse = 60 doubles), per position sweep the normal row again
-    (28 doubles)."""
-    return word * (27 + vel_iters * (28 + 60) + pos_iters * 28)
+    """Algorithmic bytes one PGS launch moves per live contact point, in words
+    of the record precision.  A contact's three rows are determined by the
+    basis n, u, w, the two inverse masses and the angular parts of J and
+    M^-1 J^T (the linear parts are +-n, +-u, +-w and the inverse masses times
+    those).  Warm start: basis 11, angular M^-1 J^T of the three rows 18,
+    3 impulses = 32 words.  Per velocity sweep: basis 11, angular J 18 and
+    M^-1 J^T 18, normal inverse + target, friction inverse 4 + targets 2, mu,
+    impulses read and written 3 + 3 = 62 words.  Per position sweep: n and the
+    inverse masses 5, angular J 6 and M^-1 J^T 6, inverse, target, pseudo
+    impulse read and written = 21 words."""
+    return word * (32 + vel_iters * 62 + pos_iters * 21)
 
 
 def profiled_traffic(dtype, scenes, seed):

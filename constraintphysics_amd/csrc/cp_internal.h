@@ -190,12 +190,16 @@ struct Contacts {
 	double* row;
 };
 
-// Record of CP_RS values (FP64: 768 B = 6 lines of 128 B; FP32: 384 B = 3
+// Record of CP_RS values (FP64: 512 B = 4 lines of 128 B; FP32: 256 B = 2
 // lines).  The fields a sweep writes (impulses, static flag) share the first
-// line, so one velocity sweep reads the whole record and dirties one line.
+// line.  Only the independent coefficients are stored: the linear parts of
+// the Jacobians are -n | n (normal row) and u | -u, w | -w (friction rows),
+// and the linear parts of M^-1 J^T are the inverse masses times those, so a
+// row step rebuilds them from the basis block as 0 + (1/m) j, the value the
+// full 6x6 product M^-1 J^T gives (its other five terms are signed zeros).
 // Blocks start at multiples of 4 values and records are 128-byte aligned, so
-// a row's coefficients load as 16-byte vectors.
-#define CP_RS 96
+// the blocks load as 16-byte vectors.
+#define CP_RS 64
 enum RowField {
 	CP_R_IMP = 0,     // accumulated normal impulse
 	CP_R_PIMP = 1,    // accumulated pseudo impulse
@@ -207,14 +211,9 @@ enum RowField {
 	CP_R_PTGT = 8,    // pseudo-velocity target
 	CP_R_FTGT = 9,    // friction targets [2]; 11 padding
 	CP_R_FINV = 12,   // friction 2x2 inverse [4]
-	CP_R_JA = 16,     // normal row J_a [6]
-	CP_R_JB = 22,     // J_b [6]
-	CP_R_ITA = 28,    // M_a^-1 J_a^T [6]
-	CP_R_ITB = 34,    // M_b^-1 J_b^T [6]
-	CP_R_FJA = 40,    // friction J_a [2][6]
-	CP_R_FJB = 52,    // friction J_b [2][6]
-	CP_R_FITA = 64,   // M_a^-1 J_a^T [6][2]
-	CP_R_FITB = 76    // M_b^-1 J_b^T [6][2]; 88..95 padding
+	CP_R_BASIS = 16,  // n[3], 1/m_a, u[3], 1/m_b, w[3], padding (1/m = 0 unless dynamic)
+	CP_R_ANG = 28,    // normal row angular parts: J_a[3], J_b[3], (M_a^-1 J_a^T)[3], (M_b^-1 J_b^T)[3]
+	CP_R_FANG = 40    // friction angular parts: J_a[2][3], J_b[2][3], (M_a^-1 J_a^T)[3][2], (M_b^-1 J_b^T)[3][2]
 };
 
 // A step's manifold points, compacted and sorted by pair key.

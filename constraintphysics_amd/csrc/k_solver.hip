@@ -80,8 +80,8 @@ __device__ void build_row(const Bodies& b, const Contacts& C, int k, const Param
 	double pcc = (a2 < a1) ? a2 : a1;  // posCorrectCoeff, PersistentConstraint.cpp:199
 	bool d1 = b.movement[i1] == 0, d2 = b.movement[i2] == 0;
 	double Ma[6][6], Mb[6][6];
+	const double ima = d1 ? 1.0 / b.mass[i1] : 0, imb = d2 ? 1.0 / b.mass[i2] : 0;
 	{
-		double ima = d1 ? 1.0 / b.mass[i1] : 0, imb = d2 ? 1.0 / b.mass[i2] : 0;
 		for (int r = 0; r < 6; r++) for (int c = 0; c < 6; c++) { Ma[r][c] = 0; Mb[r][c] = 0; }
 		for (int r = 0; r < 3; r++)
 			for (int c = 0; c < 3; c++) {
@@ -125,9 +125,14 @@ __device__ void build_row(const Bodies& b, const Contacts& C, int k, const Param
 		for (int q = 0; q < 6; q++) { t1 += ja[q] * ita[q]; t2 += jb[q] * itb[q]; }
 		double itv = t1 + t2;
 		itv *= (1 + P.cfm);
-		for (int r = 0; r < 6; r++) {
-			R[CP_R_JA + r] = ja[r]; R[CP_R_JB + r] = jb[r];
-			R[CP_R_ITA + r] = ita[r]; R[CP_R_ITB + r] = itb[r];
+		// stored: the basis and the angular parts (linear parts are -n | n and
+		// ima, imb times those)
+		R[CP_R_BASIS] = n.x; R[CP_R_BASIS + 1] = n.y; R[CP_R_BASIS + 2] = n.z; R[CP_R_BASIS + 3] = ima;
+		R[CP_R_BASIS + 7] = imb;
+		R[CP_R_BASIS + 11] = 0;
+		for (int r = 0; r < 3; r++) {
+			R[CP_R_ANG + r] = ja[3 + r]; R[CP_R_ANG + 3 + r] = jb[3 + r];
+			R[CP_R_ANG + 6 + r] = ita[3 + r]; R[CP_R_ANG + 9 + r] = itb[3 + r];
 		}
 		R[CP_R_INV] = 1.0 / itv;
 		double sp = adj - 0.05;
@@ -185,10 +190,12 @@ __device__ void build_row(const Bodies& b, const Contacts& C, int k, const Param
 		double dt = a * d - bq * cq;
 		double di = 1.0 / dt;
 		R[CP_R_FINV] = d * di; R[CP_R_FINV + 1] = -bq * di; R[CP_R_FINV + 2] = -cq * di; R[CP_R_FINV + 3] = a * di;
+		R[CP_R_BASIS + 4] = u.x; R[CP_R_BASIS + 5] = u.y; R[CP_R_BASIS + 6] = u.z;
+		R[CP_R_BASIS + 8] = w.x; R[CP_R_BASIS + 9] = w.y; R[CP_R_BASIS + 10] = w.z;
 		for (int r = 0; r < 2; r++)
-			for (int c = 0; c < 6; c++) { R[CP_R_FJA + r * 6 + c] = ja[r][c]; R[CP_R_FJB + r * 6 + c] = jb[r][c]; }
-		for (int r = 0; r < 6; r++)
-			for (int c = 0; c < 2; c++) { R[CP_R_FITA + r * 2 + c] = ita[r][c]; R[CP_R_FITB + r * 2 + c] = itb[r][c]; }
+			for (int c = 0; c < 3; c++) { R[CP_R_FANG + r * 3 + c] = ja[r][3 + c]; R[CP_R_FANG + 6 + r * 3 + c] = jb[r][3 + c]; }
+		for (int r = 0; r < 3; r++)
+			for (int c = 0; c < 2; c++) { R[CP_R_FANG + 12 + r * 2 + c] = ita[3 + r][c]; R[CP_R_FANG + 18 + r * 2 + c] = itb[3 + r][c]; }
 		R[CP_R_FIMP] = i0;
 		R[CP_R_FIMP + 1] = i1v;
 		C.fu[k] = u.x; C.fu[K + k] = u.y; C.fu[2 * K + k] = u.z;
@@ -259,6 +266,61 @@ template <class T>
 struct RowRegs {
 	T h[16], j[12], it[12], fj[24], fit[24];
 };
+
+// The normal row's J_a|J_b (g.j) and M^-1 J^T (g.it) from the basis block bs
+// (n, 1/m_a, u, 1/m_b, w) and the angular block a.  The linear parts of M^-1
+// J^T are 0 + (1/m) j: the full product's sum starts at +0 and adds five
+// signed zeros, which leaves exactly this value.
+template <class T>
+__device__ __forceinline__ void expand_normal(const T (&bs)[12], const T (&a)[12], RowRegs<T>& g) {
+#pragma unroll
+	for (int q = 0; q < 3; q++) {
+		g.j[q] = -bs[q];
+		g.j[3 + q] = a[q];
+		g.j[6 + q] = bs[q];
+		g.j[9 + q] = a[3 + q];
+		g.it[q] = T(0) + bs[3] * g.j[q];
+		g.it[3 + q] = a[6 + q];
+		g.it[6 + q] = T(0) + bs[7] * g.j[6 + q];
+		g.it[9 + q] = a[9 + q];
+	}
+}
+// The friction rows' J_a|J_b (g.fj: [dir][6] for a, then b) and M^-1 J^T
+// (g.fit: [6][dir] for a, then b): J_a = (u | w), J_b = (-u | -w) in the
+// linear part, angular parts from block f.
+template <class T>
+__device__ __forceinline__ void expand_friction(const T (&bs)[12], const T (&f)[24], RowRegs<T>& g) {
+#pragma unroll
+	for (int q = 0; q < 3; q++) {
+		g.fj[q] = bs[4 + q];
+		g.fj[6 + q] = bs[8 + q];
+		g.fj[12 + q] = -bs[4 + q];
+		g.fj[18 + q] = -bs[8 + q];
+		g.fj[3 + q] = f[q];
+		g.fj[9 + q] = f[3 + q];
+		g.fj[15 + q] = f[6 + q];
+		g.fj[21 + q] = f[9 + q];
+#pragma unroll
+		for (int c = 0; c < 2; c++) {
+			g.fit[q * 2 + c] = T(0) + bs[3] * g.fj[c * 6 + q];
+			g.fit[12 + q * 2 + c] = T(0) + bs[7] * g.fj[12 + c * 6 + q];
+			g.fit[(3 + q) * 2 + c] = f[12 + q * 2 + c];
+			g.fit[12 + (3 + q) * 2 + c] = f[18 + q * 2 + c];
+		}
+	}
+}
+template <class T>
+__device__ __forceinline__ void load_normal(const T* R, const T (&bs)[12], RowRegs<T>& g) {
+	T a[12];
+	ldv(R + CP_R_ANG, a);
+	expand_normal(bs, a, g);
+}
+template <class T>
+__device__ __forceinline__ void load_friction(const T* R, const T (&bs)[12], RowRegs<T>& g) {
+	T f[24];
+	ldv(R + CP_R_FANG, f);
+	expand_friction(bs, f, g);
+}
 
 // applyWarmStartVelocityChange of the contact row (kind 0) or the friction
 // pair (kind 1) (ConstraintSolver.cpp:121-126).
@@ -393,22 +455,19 @@ __device__ __forceinline__ void friction_core(T* R, RowRegs<T>& g, const D& A, c
 template <class T, class D>
 __device__ __forceinline__ void contact_phase(T* R, const D& A, const D& B, int phase) {
 	RowRegs<T> g;
+	T bs[12];
 	ldv(R, g.h);
+	ldv(R + CP_R_BASIS, bs);
+	load_normal(R, bs, g);
 	if (phase == 0) {
-		ldv(R + CP_R_ITA, g.it);
-		ldv(R + CP_R_FITA, g.fit);
+		load_friction(R, bs, g);
 		warm_normal(g, A, B);
 		warm_friction(g, A, B);
 	} else if (phase == 1) {
-		ldv(R + CP_R_JA, g.j);
-		ldv(R + CP_R_ITA, g.it);
-		ldv(R + CP_R_FJA, g.fj);
-		ldv(R + CP_R_FITA, g.fit);
+		load_friction(R, bs, g);
 		normal_core(R, g, A, B, false);
 		friction_core(R, g, A, B);
 	} else {
-		ldv(R + CP_R_JA, g.j);
-		ldv(R + CP_R_ITA, g.it);
 		normal_core(R, g, A, B, true);
 	}
 }
@@ -417,21 +476,17 @@ __device__ __forceinline__ void contact_phase(T* R, const D& A, const D& B, int 
 template <class T, class D>
 __device__ __forceinline__ void contact_row(T* R, const D& A, const D& B, int kind, int phase) {
 	RowRegs<T> g;
+	T bs[12];
 	ldv(R, g.h);
+	ldv(R + CP_R_BASIS, bs);
 	if (kind == 0) {
-		ldv(R + CP_R_ITA, g.it);
+		load_normal(R, bs, g);
 		if (phase == 0) warm_normal(g, A, B);
-		else {
-			ldv(R + CP_R_JA, g.j);
-			normal_core(R, g, A, B, phase == 2);
-		}
+		else normal_core(R, g, A, B, phase == 2);
 	} else {
-		ldv(R + CP_R_FITA, g.fit);
+		load_friction(R, bs, g);
 		if (phase == 0) warm_friction(g, A, B);
-		else {
-			ldv(R + CP_R_FJA, g.fj);
-			friction_core(R, g, A, B);
-		}
+		else friction_core(R, g, A, B);
 	}
 }
 
@@ -842,10 +897,15 @@ __global__ void k_pgs_residual(Bodies b, Contacts C, int nc, double* acc) {
 		const T* R = rec<T>(C, k);
 		const T* dv = reinterpret_cast<const T*>(b.dv);
 		int i1 = (int)(C.key[k] >> 32), i2 = (int)(C.key[k] & 0xFFFFFFFFu);
+		RowRegs<T> g;
+		T bs[12];
+		ldv(R + CP_R_BASIS, bs);
+		load_normal(R, bs, g);
+		load_friction(R, bs, g);
 		double x = 0, y = 0;
 		for (int q = 0; q < 6; q++) {
-			x += (double)dv[q * N + i1] * (double)R[CP_R_JA + q];
-			y += (double)dv[q * N + i2] * (double)R[CP_R_JB + q];
+			x += (double)dv[q * N + i1] * (double)g.j[q];
+			y += (double)dv[q * N + i2] * (double)g.j[6 + q];
 		}
 		double ln = R[CP_R_IMP];
 		double z = ln + (double)R[CP_R_INV] * ((double)R[CP_R_TGT] - (x + y));
@@ -854,8 +914,8 @@ __global__ void k_pgs_residual(Bodies b, Contacts C, int nc, double* acc) {
 		for (int r = 0; r < 2; r++) {
 			double a = 0, c = 0;
 			for (int q = 0; q < 6; q++) {
-				a += (double)dv[q * N + i1] * (double)R[CP_R_FJA + r * 6 + q];
-				c += (double)dv[q * N + i2] * (double)R[CP_R_FJB + r * 6 + q];
+				a += (double)dv[q * N + i1] * (double)g.fj[r * 6 + q];
+				c += (double)dv[q * N + i2] * (double)g.fj[12 + r * 6 + q];
 			}
 			cur[r] = a + c;
 		}
