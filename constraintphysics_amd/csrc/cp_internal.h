@@ -21,6 +21,17 @@
 
 namespace cpx {
 
+// Workgroups of a dispatch go to the eight XCDs round-robin (block b to XCD
+// b mod 8), and each XCD has its own L2.  xcd_block() renumbers the blocks so
+// that each XCD runs one contiguous range of logical blocks; work items next
+// to each other (a scene's pairs or contacts, which share bodies) then share
+// an L2.  The grid must be a multiple of 8 (xcd_grid rounds it up).
+__device__ __forceinline__ int xcd_block() {
+	const int per = gridDim.x >> 3;
+	return (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+}
+static inline int xcd_grid(int blocks) { return (blocks + 7) & ~7; }
+
 enum ShapeType { SHAPE_HULL = 0, SHAPE_SPHERE = 1, SHAPE_CYLINDER = 2, SHAPE_MESH = 3 };
 
 struct Shapes {

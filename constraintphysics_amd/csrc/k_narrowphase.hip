@@ -85,13 +85,24 @@ struct Ext {
 	double min_val, max_val;
 };
 
+// The points are read four at a time so that their loads are in flight
+// together; the comparisons still run in point order (first extreme wins).
 __device__ Ext find_extrema(const Hull& h, V3 axis) {  // CollisionDetect.cpp:627-644
 	Ext e{-1, -1, __builtin_inf(), -__builtin_inf()};
-	for (int i = 0; i < h.np; i++) {
-		double v = dot(ldv(h.wp, i), axis);
+	auto take = [&](double v, int i) {
 		if (v < e.min_val) { e.min_pid = i; e.min_val = v; }
 		if (v > e.max_val) { e.max_pid = i; e.max_val = v; }
+	};
+	int i = 0;
+	for (; i + 4 <= h.np; i += 4) {
+		V3 p0 = ldv(h.wp, i), p1 = ldv(h.wp, i + 1), p2 = ldv(h.wp, i + 2), p3 = ldv(h.wp, i + 3);
+		double v0 = dot(p0, axis), v1 = dot(p1, axis), v2 = dot(p2, axis), v3 = dot(p3, axis);
+		take(v0, i);
+		take(v1, i + 1);
+		take(v2, i + 2);
+		take(v3, i + 3);
 	}
+	for (; i < h.np; i++) take(dot(ldv(h.wp, i), axis), i);
 	return e;
 }
 
@@ -1232,13 +1243,13 @@ __device__ bool wave_sat_hh(const Hull& A, const Hull& B, int lane, int* cand, i
 
 #define NP_WAVES 4
 // Hull-hull: SAT_PolyPoly (CollisionDetect.cpp:717-835).
-__global__ __launch_bounds__(64 * NP_WAVES) void k_np_hull_hull(Bodies b, Shapes sh, Params P, const uint64_t* pairs, const int* list,
+__global__ __launch_bounds__(64 * NP_WAVES) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_np_hull_hull(Bodies b, Shapes sh, Params P, const uint64_t* pairs, const int* list,
                                                                 const int* counts, Manifolds M, int cap) {
 	__shared__ WaveLds lds[NP_WAVES];
 	const int lane = threadIdx.x & 63, wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
 	WaveLds& L = lds[wib];
 	const int cnt = counts[NPC_HH];
-	for (int w = blockIdx.x * NP_WAVES + wib; w < cnt; w += gridDim.x * NP_WAVES) {
+	for (int w = xcd_block() * NP_WAVES + wib; w < cnt; w += gridDim.x * NP_WAVES) {
 		// one pair per wave: its indices are wave-uniform (scalar registers)
 		int slot = __builtin_amdgcn_readfirstlane(list[w]);
 		uint64_t key = pairs[slot];
@@ -1295,7 +1306,7 @@ __global__ __launch_bounds__(64 * NP_WAVES) __attribute__((amdgpu_waves_per_eu(4
 	const int lane = threadIdx.x & 63, wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
 	WaveLds& L = lds[wib];
 	const int cnt = counts[NPC_HC];
-	for (int w = blockIdx.x * NP_WAVES + wib; w < cnt; w += gridDim.x * NP_WAVES) {
+	for (int w = xcd_block() * NP_WAVES + wib; w < cnt; w += gridDim.x * NP_WAVES) {
 		// one pair per wave: its indices are wave-uniform (scalar registers)
 		int slot = __builtin_amdgcn_readfirstlane(list[w]);
 		uint64_t key = pairs[slot];
@@ -1353,7 +1364,7 @@ __global__ __launch_bounds__(64 * NP_WAVES) __attribute__((amdgpu_waves_per_eu(3
 	const int lane = threadIdx.x & 63, wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
 	WaveLds& L = lds[wib];
 	const int cnt = counts[NPC_CC];
-	for (int w = blockIdx.x * NP_WAVES + wib; w < cnt; w += gridDim.x * NP_WAVES) {
+	for (int w = xcd_block() * NP_WAVES + wib; w < cnt; w += gridDim.x * NP_WAVES) {
 		// one pair per wave: its indices are wave-uniform (scalar registers)
 		int slot = __builtin_amdgcn_readfirstlane(list[w]);
 		uint64_t key = pairs[slot];
@@ -1386,7 +1397,7 @@ __global__ __launch_bounds__(64 * NP_WAVES) void k_np_hull_sphere(Bodies b, Shap
 	__shared__ double sp[NP_WAVES][3 * ST_PTS], sn[NP_WAVES][3 * ST_NRM];
 	const int lane = threadIdx.x & 63, wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
 	const int cnt = counts[NPC_HS];
-	for (int w = blockIdx.x * NP_WAVES + wib; w < cnt; w += gridDim.x * NP_WAVES) {
+	for (int w = xcd_block() * NP_WAVES + wib; w < cnt; w += gridDim.x * NP_WAVES) {
 		// one pair per wave: its indices are wave-uniform (scalar registers)
 		int slot = __builtin_amdgcn_readfirstlane(list[w]);
 		uint64_t key = pairs[slot];
@@ -1686,7 +1697,7 @@ __global__ __launch_bounds__(64 * NP_WAVES) void k_mesh_triangles(Bodies b, Shap
 	const int lane = threadIdx.x & 63, wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
 	WaveLds& L = lds[wib];
 	const int N = b.n;
-	for (int w = blockIdx.x * NP_WAVES + wib; w < nitems; w += gridDim.x * NP_WAVES) {
+	for (int w = xcd_block() * NP_WAVES + wib; w < nitems; w += gridDim.x * NP_WAVES) {
 		// one (body, triangle) item per wave: its indices are wave-uniform
 		int slot = __builtin_amdgcn_readfirstlane(it.slot[w]);
 		uint64_t key = pairs[slot];
@@ -1909,6 +1920,7 @@ void launch_mesh_triangles(const Bodies& b, const Shapes& sh, const MeshDev& md,
 	if (nitems <= 0) return;
 	int blocks = (nitems + NP_WAVES - 1) / NP_WAVES;
 	if (blocks > 8192) blocks = 8192;
+	blocks = xcd_grid(blocks);
 	hipLaunchKernelGGL(k_mesh_triangles, dim3(blocks), dim3(64 * NP_WAVES), 0, s, b, sh, md, p, pairs, it, nitems);
 }
 void launch_mesh_merge(const Params& p, const uint64_t* pairs, const MeshDev& md, const int* list, int nlist, const int* off, MeshItems it,
@@ -1925,6 +1937,7 @@ void launch_narrowphase(const Bodies& b, const Shapes& sh, const Params& p, cons
 	                   m, lists, counts, nocol, n_nocol, rank);
 	int wblocks = (npairs + NP_WAVES - 1) / NP_WAVES;
 	if (wblocks > 4096) wblocks = 4096;
+	wblocks = xcd_grid(wblocks);
 	const int* L = lists;
 	size_t n = (size_t)npairs;
 	// the pair classes touch disjoint slots: hull-cylinder on the world's

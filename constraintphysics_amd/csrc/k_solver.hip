@@ -54,10 +54,14 @@ __device__ __forceinline__ T tsqrt(T x) {
 template <class T>
 __device__ void build_row(const Bodies& b, const Contacts& C, int k, const Params& P);
 
+// Each logical block (xcd_block) takes one contiguous chunk of contacts, so
+// the contacts of a scene, which gather the same bodies, run on one XCD.
 template <class T>
 __global__ __launch_bounds__(64) void k_build_rows(Bodies b, Contacts C, int nc, const int* ncp, Params P) {
 	const int n = ncp ? *ncp : nc;
-	for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
+	const int chunk = (((n + gridDim.x - 1) / gridDim.x) + 63) & ~63;
+	const int k0 = xcd_block() * chunk, k1 = min(n, k0 + chunk);
+	for (int k = k0 + threadIdx.x; k < k1; k += 64)
 		if (C.live[k]) build_row<T>(b, C, k, P);
 }
 
@@ -950,7 +954,7 @@ static inline int nblk(int n, int t) { return (n + t - 1) / t; }
 static inline int grid_for(int n, int t) { return std::max(1, std::min(nblk(n, t), 4096)); }
 void launch_build_rows(const Bodies& b, const Contacts& C, int nc, const Params& P, hipStream_t s, const int* ncp) {
 	if (!nc) return;
-	int g = ncp ? grid_for(nc, 64) : nblk(nc, 64);
+	int g = xcd_grid(ncp ? grid_for(nc, 64) : nblk(nc, 64));
 	if (P.precision == CP_PREC_SPLIT) hipLaunchKernelGGL(k_build_rows<float>, dim3(g), dim3(64), 0, s, b, C, nc, ncp, P);
 	else hipLaunchKernelGGL(k_build_rows<double>, dim3(g), dim3(64), 0, s, b, C, nc, ncp, P);
 }
