@@ -211,32 +211,95 @@ __global__ void k_edge_build(const Contacts C, int n, const int* head, const int
 
 #define MAXC CP_MAXC
 
-// One wave per scene: greedy edge colouring in edge (cache key) order, then a
-// counting sort of the scene's live edges by colour.  All lanes stage 64
-// edges at a time (bodies, dynamic and live flags) and lane 0 colours them:
-// each edge takes the lowest colour free at its dynamic bodies.  The colour
-// sets live in LDS for scenes of at most `lds_bodies` bodies, else in
-// b.color_mask.  Colour MAXC collects edges that found no free colour; the
-// solver runs that bucket on a single thread.  ecolor holds colour | rank
-// within the colour << 8 (-1: no live contact).
+// Greedy colouring of one scene's edges [e0, e1) in order, 64 at a time:
+// each lane loads one edge (scene-local bodies, live and dynamic flags), then
+// the wave walks the 64 edges in order with the edge's fields read from its
+// lane (readlane, so they are wave-uniform) and gives each the lowest colour
+// free at its dynamic bodies (colour sets `mask`, two words per body, updated
+// by lane 0).  The rank of an edge within its colour (edges of that colour
+// before it) is then found for all 64 at once: lanes of equal colour are
+// matched bit by bit with ballots and counted below, on top of the running
+// count cnt[c].  ecolor = colour | rank << 8, -1 for an edge with no live
+// contact.  Returns the largest colour used.
+template <class MaskPtr>
+__device__ int color_edges(const Bodies& b, int b0, int e0, int e1, const uint64_t* ekey, const int* elive, MaskPtr mask, int* cnt,
+                           int* ecolor, int lane) {
+	const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+	int maxc = -1;
+	for (int base = e0; base < e1; base += 64) {
+		const int e = base + lane, n = min(64, e1 - base);
+		int lo = 0, hi = 0, fl = 0;
+		if (e < e1) {
+			uint64_t k = ekey[e];
+			int glo = (int)(k >> 32), ghi = (int)(k & 0xFFFFFFFFu);
+			fl = (elive[e] ? 1 : 0) | (b.movement[glo] == 0 ? 2 : 0) | (b.movement[ghi] == 0 ? 4 : 0);
+			lo = glo - b0;
+			hi = ghi - b0;
+		}
+		int myc = -1;
+		for (int j = 0; j < n; j++) {
+			const int jfl = __builtin_amdgcn_readlane(fl, j);
+			if (!(jfl & 1)) continue;
+			const int jlo = __builtin_amdgcn_readlane(lo, j), jhi = __builtin_amdgcn_readlane(hi, j);
+			const bool dl = (jfl & 2) != 0, dh = (jfl & 4) != 0;
+			uint64_t m0 = (dl ? mask[2 * jlo] : 0) | (dh ? mask[2 * jhi] : 0);
+			uint64_t m1 = (dl ? mask[2 * jlo + 1] : 0) | (dh ? mask[2 * jhi + 1] : 0);
+			int c;
+			if (~m0) c = __builtin_ctzll(~m0);
+			else if (~m1) c = 64 + __builtin_ctzll(~m1);
+			else c = MAXC;
+			if (c < MAXC && lane == 0) {
+				uint64_t bit = 1ull << (c & 63);
+				int h = c >> 6;
+				if (dl) mask[2 * jlo + h] |= bit;
+				if (dh) mask[2 * jhi + h] |= bit;
+			}
+			if (lane == j) myc = c;
+			maxc = c > maxc ? c : maxc;
+		}
+		// lanes with the same colour (colours 0..MAXC fit in 8 bits; -1 has
+		// all 8 set and is never ranked)
+		uint64_t peers = ~0ull;
+#pragma unroll
+		for (int bit = 0; bit < 8; bit++) {
+			const bool set = (myc >> bit) & 1;
+			const uint64_t m = __ballot(set);
+			peers &= set ? m : ~m;
+		}
+		int rank = 0;
+		if (myc >= 0) rank = cnt[myc] + __popcll(peers & below);
+		__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+		__builtin_amdgcn_wave_barrier();
+		if (myc >= 0 && (peers & ~below & ~(1ull << lane)) == 0) cnt[myc] += __popcll(peers);
+		__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+		__builtin_amdgcn_wave_barrier();
+		if (e < e1) ecolor[e] = myc < 0 ? -1 : (myc | (rank << 8));
+	}
+	return maxc;
+}
+
+// One wave per scene: greedy edge colouring in edge (cache key) order
+// (color_edges), then a counting sort of the scene's live edges by colour.
+// The colour sets live in LDS for scenes of at most `lds_bodies` bodies, else
+// in b.color_mask.  Colour MAXC collects edges that found no free colour; the
+// solver runs that bucket on a single thread.
 __global__ __launch_bounds__(64) void k_color(Bodies b, const int* scene_body_off, int nscenes, const uint64_t* ekey, const int* elive,
                                               int nedges, int* ecolor, int* order, int* color_off, int* ncolors, int* edge_range,
                                               int lds_bodies) {
 	extern __shared__ uint64_t smask_lds[];
-	__shared__ int sdat[64], shi[64];
 	__shared__ int cnt[MAXC + 1];
 	const int s = blockIdx.x, lane = threadIdx.x;
 	const int b0 = scene_body_off[s], b1 = scene_body_off[s + 1], nbs = b1 - b0;
 	const int e0 = lower_bound_u64(ekey, nedges, (uint64_t)b0 << 32), e1 = lower_bound_u64(ekey, nedges, (uint64_t)b1 << 32);
 	const bool in_lds = nbs <= lds_bodies;
-	uint64_t* mask = in_lds ? smask_lds : reinterpret_cast<uint64_t*>(b.color_mask) + 2 * (size_t)b0;
+	uint64_t* gmask = reinterpret_cast<uint64_t*>(b.color_mask) + 2 * (size_t)b0;
 	if (in_lds)
-		for (int i = lane; i < 2 * nbs; i += 64) mask[i] = 0;
+		for (int i = lane; i < 2 * nbs; i += 64) smask_lds[i] = 0;
 	else
 		for (int e = e0 + lane; e < e1; e += 64) {  // only bodies on an edge need a clear colour set
 			int lo = (int)(ekey[e] >> 32) - b0, hi = (int)(ekey[e] & 0xFFFFFFFFu) - b0;
-			mask[2 * lo] = 0; mask[2 * lo + 1] = 0;
-			mask[2 * hi] = 0; mask[2 * hi + 1] = 0;
+			gmask[2 * lo] = 0; gmask[2 * lo + 1] = 0;
+			gmask[2 * hi] = 0; gmask[2 * hi + 1] = 0;
 		}
 	for (int c = lane; c <= MAXC; c += 64) cnt[c] = 0;
 	if (lane == 0) {
@@ -244,44 +307,10 @@ __global__ __launch_bounds__(64) void k_color(Bodies b, const int* scene_body_of
 		edge_range[2 * s + 1] = e1;
 	}
 	__syncthreads();
-	int maxc = -1;
-	for (int base = e0; base < e1; base += 64) {
-		const int e = base + lane, n = min(64, e1 - base);
-		if (e < e1) {
-			uint64_t k = ekey[e];
-			int lo = (int)(k >> 32), hi = (int)(k & 0xFFFFFFFFu);
-			int fl = (elive[e] ? 1 : 0) | (b.movement[lo] == 0 ? 2 : 0) | (b.movement[hi] == 0 ? 4 : 0);
-			sdat[lane] = (lo - b0) | (fl << 28);
-			shi[lane] = hi - b0;
-		}
-		__syncthreads();
-		if (lane == 0)
-			for (int j = 0; j < n; j++) {
-				const int d = sdat[j], lo = d & 0x0FFFFFFF, fl = d >> 28, hi = shi[j];
-				if (!(fl & 1)) {
-					sdat[j] = -1;
-					continue;
-				}
-				const bool dl = (fl & 2) != 0, dh = (fl & 4) != 0;
-				uint64_t m0 = (dl ? mask[2 * lo] : 0) | (dh ? mask[2 * hi] : 0);
-				uint64_t m1 = (dl ? mask[2 * lo + 1] : 0) | (dh ? mask[2 * hi + 1] : 0);
-				int c;
-				if (~m0) c = __builtin_ctzll(~m0);
-				else if (~m1) c = 64 + __builtin_ctzll(~m1);
-				else c = MAXC;
-				if (c < MAXC) {
-					uint64_t bit = 1ull << (c & 63);
-					int h = c >> 6;
-					if (dl) mask[2 * lo + h] |= bit;
-					if (dh) mask[2 * hi + h] |= bit;
-				}
-				sdat[j] = c | (cnt[c]++ << 8);
-				if (c > maxc) maxc = c;
-			}
-		__syncthreads();
-		if (e < e1) ecolor[e] = sdat[lane];
-		__syncthreads();
-	}
+	// two instances, so that the LDS one addresses LDS directly
+	const int maxc = in_lds ? color_edges(b, b0, e0, e1, ekey, elive, smask_lds, cnt, ecolor, lane)
+	                        : color_edges(b, b0, e0, e1, ekey, elive, gmask, cnt, ecolor, lane);
+	__syncthreads();
 	int* co = color_off + (size_t)s * (MAXC + 2);
 	if (lane == 0) {
 		int acc = 0;
