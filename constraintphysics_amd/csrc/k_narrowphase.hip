@@ -1940,8 +1940,11 @@ void launch_narrowphase(const Bodies& b, const Shapes& sh, const Params& p, cons
 	wblocks = xcd_grid(wblocks);
 	const int* L = lists;
 	size_t n = (size_t)npairs;
-	// the pair classes touch disjoint slots: hull-cylinder on the world's
-	// stream, the others beside it on the two side streams
+	// the pair classes touch disjoint slots and run side by side on three
+	// streams, each stream a long SAT class followed by a short one:
+	// hull-cylinder then hull-sphere, hull-hull then the sphere pairs, and
+	// cylinder-cylinder alone (it keeps fewer waves per SIMD, so it runs long
+	// while it shares the chip)
 	hipStream_t s0 = fk ? fk->side[0] : s, s1 = fk ? fk->side[1] : s;
 	if (fk) {
 		(void)(hipEventRecord(fk->fork, s));
@@ -1951,7 +1954,7 @@ void launch_narrowphase(const Bodies& b, const Shapes& sh, const Params& p, cons
 	hipLaunchKernelGGL(k_np_hull_cyl, dim3(wblocks), dim3(64 * NP_WAVES), 0, s, b, sh, p, pairs, L + NPC_HC * n, counts, m, cap);
 	hipLaunchKernelGGL(k_np_hull_hull, dim3(wblocks), dim3(64 * NP_WAVES), 0, s0, b, sh, p, pairs, L + NPC_HH * n, counts, m, cap);
 	hipLaunchKernelGGL(k_np_cyl_cyl, dim3(wblocks), dim3(64 * NP_WAVES), 0, s1, b, sh, p, pairs, L + NPC_CC * n, counts, m, cap);
-	hipLaunchKernelGGL(k_np_hull_sphere, dim3(wblocks), dim3(64 * NP_WAVES), 0, s1, b, sh, p, pairs, L + NPC_HS * n, counts, m, cap);
+	hipLaunchKernelGGL(k_np_hull_sphere, dim3(wblocks), dim3(64 * NP_WAVES), 0, s, b, sh, p, pairs, L + NPC_HS * n, counts, m, cap);
 	int tblocks = (npairs + 255) / 256;
 	if (tblocks > 4096) tblocks = 4096;
 	hipLaunchKernelGGL(k_np_sphere_pairs, dim3(tblocks), dim3(256), 0, s0, b, sh, p, pairs, L + NPC_SS * n, L + NPC_SC * n, counts, m, cap);
