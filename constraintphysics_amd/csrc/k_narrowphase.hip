@@ -1358,7 +1358,7 @@ __device__ __forceinline__ bool cc_prune(const Cyl& A, const Cyl& B, int t) {
 	return arc_pair(cyl_gv(A, k1 & 1), cyl_gv(A, k1 >> 1), -cyl_gv(B, k2 & 1), -cyl_gv(B, k2 >> 1));
 }
 
-__global__ __launch_bounds__(64 * NP_WAVES) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_np_cyl_cyl(Bodies b, Shapes sh, Params P, const uint64_t* pairs, const int* list,
+__global__ __launch_bounds__(64 * NP_WAVES) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_np_cyl_cyl(Bodies b, Shapes sh, Params P, const uint64_t* pairs, const int* list,
                                                               const int* counts, Manifolds M, int cap) {
 	__shared__ WaveLds lds[NP_WAVES];
 	const int lane = threadIdx.x & 63, wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
@@ -1380,11 +1380,34 @@ __global__ __launch_bounds__(64 * NP_WAVES) __attribute__((amdgpu_waves_per_eu(3
 			    return eval_axis_cc(A, B, t, &v);
 		    },
 		    &bi);
+		// a hit leaves its axis for k_np_cyl_cyl_contacts (count -2 - axis)
+		if (hit && lane == 0) M.count[slot] = -2 - bi;
 		wave_fence();
-		if (hit && lane == 0) {
+	}
+}
+
+// Contact generation of the cylinder pairs k_np_cyl_cyl found touching, one
+// wave per pair of the same list, lane 0 (kept out of the axis-search kernel
+// so that kernel needs fewer registers and holds more waves).
+__global__ __launch_bounds__(64 * NP_WAVES) void k_np_cyl_cyl_contacts(Bodies b, Shapes sh, Params P, const uint64_t* pairs, const int* list,
+                                                                       const int* counts, Manifolds M, int cap) {
+	__shared__ CEP cep[NP_WAVES][3 * MAXP];
+	__shared__ double sgv[NP_WAVES][2][42];
+	const int lane = threadIdx.x & 63, wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
+	const int cnt = counts[NPC_CC];
+	for (int w = xcd_block() * NP_WAVES + wib; w < cnt; w += gridDim.x * NP_WAVES) {
+		int slot = __builtin_amdgcn_readfirstlane(list[w]);
+		const int code = __builtin_amdgcn_readfirstlane(M.count[slot]);
+		if (code >= -1) continue;
+		uint64_t key = pairs[slot];
+		int ia = __builtin_amdgcn_readfirstlane((int)(key >> 32)), ib = __builtin_amdgcn_readfirstlane((int)(key & 0xFFFFFFFFu));
+		Cyl A = make_cyl(b, sh, ia), B = make_cyl(b, sh, ib);
+		stage_cyl(A, sgv[wib][0], lane);
+		stage_cyl(B, sgv[wib][1], lane);
+		if (lane == 0) {
 			bool valid;
-			Check mp = eval_axis_cc(A, B, bi, &valid);
-			cyl_cyl_contacts(A, B, mp, P, ia, ib, L.cep, slot, M, cap);
+			Check mp = eval_axis_cc(A, B, -2 - code, &valid);
+			cyl_cyl_contacts(A, B, mp, P, ia, ib, cep[wib], slot, M, cap);
 			write_mat(b, ia, ib, M, slot, cap);
 			if (P.defect_fixes & 1) sanitize_slot(M, slot, cap);
 		}
@@ -1954,6 +1977,7 @@ void launch_narrowphase(const Bodies& b, const Shapes& sh, const Params& p, cons
 	hipLaunchKernelGGL(k_np_hull_cyl, dim3(wblocks), dim3(64 * NP_WAVES), 0, s, b, sh, p, pairs, L + NPC_HC * n, counts, m, cap);
 	hipLaunchKernelGGL(k_np_hull_hull, dim3(wblocks), dim3(64 * NP_WAVES), 0, s0, b, sh, p, pairs, L + NPC_HH * n, counts, m, cap);
 	hipLaunchKernelGGL(k_np_cyl_cyl, dim3(wblocks), dim3(64 * NP_WAVES), 0, s1, b, sh, p, pairs, L + NPC_CC * n, counts, m, cap);
+	hipLaunchKernelGGL(k_np_cyl_cyl_contacts, dim3(wblocks), dim3(64 * NP_WAVES), 0, s1, b, sh, p, pairs, L + NPC_CC * n, counts, m, cap);
 	hipLaunchKernelGGL(k_np_hull_sphere, dim3(wblocks), dim3(64 * NP_WAVES), 0, s, b, sh, p, pairs, L + NPC_HS * n, counts, m, cap);
 	int tblocks = (npairs + 255) / 256;
 	if (tblocks > 4096) tblocks = 4096;
