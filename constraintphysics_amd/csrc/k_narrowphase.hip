@@ -1241,7 +1241,10 @@ __device__ bool wave_sat_hh(const Hull& A, const Hull& B, int lane, int* cand, i
 	return bp > 0 && bi != 0x7FFFFFFF;  // PhysicsEngine.cpp:213: keep only max_pen_depth > 0
 }
 
-#define NP_WAVES 4
+// Waves per workgroup of the wave-per-pair kernels: single-wave groups let
+// the concurrent pair classes pack the CUs most tightly (measured against 2
+// and 4 on C3).
+#define NP_WAVES 1
 // Hull-hull: SAT_PolyPoly (CollisionDetect.cpp:717-835).
 __global__ __launch_bounds__(64 * NP_WAVES) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_np_hull_hull(Bodies b, Shapes sh, Params P, const uint64_t* pairs, const int* list,
                                                                 const int* counts, Manifolds M, int cap) {
@@ -1942,7 +1945,7 @@ void launch_mesh_triangles(const Bodies& b, const Shapes& sh, const MeshDev& md,
                            int nitems, hipStream_t s) {
 	if (nitems <= 0) return;
 	int blocks = (nitems + NP_WAVES - 1) / NP_WAVES;
-	if (blocks > 8192) blocks = 8192;
+	if (blocks > 32768 / NP_WAVES) blocks = 32768 / NP_WAVES;
 	blocks = xcd_grid(blocks);
 	hipLaunchKernelGGL(k_mesh_triangles, dim3(blocks), dim3(64 * NP_WAVES), 0, s, b, sh, md, p, pairs, it, nitems);
 }
@@ -1959,7 +1962,7 @@ void launch_narrowphase(const Bodies& b, const Shapes& sh, const Params& p, cons
 	hipLaunchKernelGGL(k_np_classify, dim3((npairs + 64 * CLS_WAVES - 1) / (64 * CLS_WAVES)), dim3(64 * CLS_WAVES), 0, s, b, sh, pairs, npairs,
 	                   m, lists, counts, nocol, n_nocol, rank);
 	int wblocks = (npairs + NP_WAVES - 1) / NP_WAVES;
-	if (wblocks > 4096) wblocks = 4096;
+	if (wblocks > 16384 / NP_WAVES) wblocks = 16384 / NP_WAVES;
 	wblocks = xcd_grid(wblocks);
 	const int* L = lists;
 	size_t n = (size_t)npairs;
