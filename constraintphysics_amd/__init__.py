@@ -69,6 +69,7 @@ def lib():
         "cp_shape_add_cylinder": (_I, [_P, _P, _P, _P, _P, _P]),
         "cp_cylinder_build": (_I, [_P, ctypes.c_double, ctypes.c_double, _P, _P, _P, _P]),
         "cp_polyhedron_mass_properties": (_I, [_I, _P, _I, _P, _P, ctypes.c_double, _P]),
+        "cp_polyhedron_body": (_I, [_I, _P, _I, _P, _P, ctypes.c_double, _P, _P, _P]),
         "cp_polyhedron_build": (_I, [_I, _P, _I, _P, _P] + [_P] * 8 + [_I, _P, _P, _P, _P]),
         "cp_body_add_many": (_I, [_P, _I, _P, _P, _P, _P, _P, _P]),
         "cp_body_add": (_I, [_P, _I, _I, _P, _P, _I, _P]),
@@ -172,6 +173,42 @@ def polyhedron_mass_properties(pts, faces, density):
     _chk(lib().cp_polyhedron_mass_properties(len(pts), pts.ctypes.data, len(faces), off.ctypes.data, fv.ctypes.data, float(density),
                                              out.ctypes.data), "cp_polyhedron_mass_properties")
     return out[0], out[1:10].reshape(3, 3), out[10:13]
+
+
+def polyhedron_body(pts, faces, density):
+    """The polyhedron of a body created from Polyhedron(points, faces) with
+    the given density (RigidBody ctor, RigidBody.cpp:13-38): the exact-hull
+    layout (Gauss map of the points as given, body-frame points shifted to
+    the centre of mass) plus ``mass_props`` (mass, reference tensor, its
+    Mat3::inverse) and ``com`` (centre of mass in the input frame)."""
+    h = polyhedron_build(pts, faces)
+    pts = _a(pts, np.float64)
+    off = np.zeros(len(faces) + 1, np.int64)
+    off[1:] = np.cumsum([len(f) for f in faces])
+    fv = _a(np.concatenate([np.asarray(f) for f in faces]), np.int64)
+    body = np.zeros_like(pts)
+    mp = np.zeros(19)
+    com = np.zeros(3)
+    _chk(lib().cp_polyhedron_body(len(pts), pts.ctypes.data, len(faces), off.ctypes.data, fv.ctypes.data, float(density),
+                                  body.ctypes.data, mp.ctypes.data, com.ctypes.data), "cp_polyhedron_body")
+    h["pts"] = body
+    # reference_geometry's Gauss normals: the identity rotation applied (Mat3 * Vec3, Mat3.h:70-74)
+    g = np.asarray(h["gauss_normal"], np.float64)
+    x, y, z = g[:, 0], g[:, 1], g[:, 2]
+    h["gauss_normal"] = np.stack([(1.0 * x + 0.0 * y) + 0.0 * z, (0.0 * x + 1.0 * y) + 0.0 * z, (0.0 * x + 0.0 * y) + 1.0 * z], axis=1)
+    h["mass_props"] = mp
+    h["com"] = com
+    return h
+
+
+def mat3_inverse(m):
+    """Mat3::inverse (Mat3.h:57-68): cofactors divided by the determinant, in
+    the reference's operation order (the inverse tensor a body stores)."""
+    (a, b, c), (d, e, f), (g, h, i) = [[float(x) for x in row] for row in np.asarray(m, float).reshape(3, 3)]
+    det = a * (e * i - f * h) - b * (d * i - f * g) + c * (d * h - e * g)
+    o = [(e * i - f * h), (c * h - b * i), (b * f - c * e), (f * g - d * i), (a * i - c * g), (c * d - a * f), (d * h - e * g),
+         (b * g - a * h), (a * e - b * d)]
+    return np.array([x / det for x in o]).reshape(3, 3)
 
 
 def cylinder_build(radius, height, center=(0.0, 0.0, 0.0)):

@@ -252,46 +252,152 @@ HullData build_polyhedron(const std::vector<double>& pts, const std::vector<std:
 	return d;
 }
 
+namespace {
+// getAxisVal (RigidBody.cpp:368-402): the projection plane's axes A, B and
+// its normal axis C for a surface projected along X, Y or Z.
+enum ProjAxis { PX, PY, PZ };
+double axis_val(V3 v, int ax, ProjAxis p) {
+	static const int map[3][3] = {{1, 2, 0}, {2, 0, 1}, {0, 1, 2}};
+	int c = map[p][ax];
+	return c == 0 ? v.x : c == 1 ? v.y : v.z;
+}
+struct Intrg {
+	double v = 0, v2 = 0, v2t = 0, v3 = 0;
+};
+}  // namespace
+
+// calculateMassProperties for one POLYHEDRON primitive of a convex union
+// (RigidBody.cpp:427-532, then :633-637): projection integrals per surface
+// after Mirtich, in the reference's operation order, over the surfaces as the
+// Polyhedron constructor leaves them (winding fixed, normal from
+// Surface::normal).  The tensor is returned about the centre of mass
+// (recenterTensor, RigidBody.cpp:406-425).
 void polyhedron_mass_properties(const std::vector<double>& pts, const std::vector<std::vector<int>>& faces, double density,
                                 double* mass, double* tensor9, double* com3) {
-	double m = 0, c[3] = {0, 0, 0}, C[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-	// orient faces outward with respect to the vertex centroid
-	int np = (int)pts.size() / 3;
-	V3 ctr{0, 0, 0};
-	for (int i = 0; i < np; i++) ctr += P(pts, i);
-	ctr = ctr * (1.0 / np);
-	for (const auto& f : faces) {
-		V3 a0 = P(pts, f[0]);
-		V3 fn{0, 0, 0};
-		for (size_t k = 1; k + 1 < f.size(); k++) fn += cross(P(pts, f[k]) - a0, P(pts, f[k + 1]) - a0);
-		double sgn = dot(fn, a0 - ctr) < 0 ? -1.0 : 1.0;
-		for (size_t k = 1; k + 1 < f.size(); k++) {
-			V3 a = a0 - ctr, b = P(pts, f[k]) - ctr, cc = P(pts, f[k + 1]) - ctr;
-			double v = sgn * dot(a, cross(b, cc)) / 6.0;
-			double dm = density * v;
-			m += dm;
-			V3 s = a + b + cc;
-			c[0] += dm * s.x / 4.0; c[1] += dm * s.y / 4.0; c[2] += dm * s.z / 4.0;
-			double A[3] = {a.x, a.y, a.z}, B[3] = {b.x, b.y, b.z}, Cc[3] = {cc.x, cc.y, cc.z}, S[3] = {s.x, s.y, s.z};
-			for (int i = 0; i < 3; i++)
-				for (int j = 0; j < 3; j++) C[i][j] += dm / 20.0 * (A[i] * A[j] + B[i] * B[j] + Cc[i] * Cc[j] + S[i] * S[j]);
+	HullData d = build_polyhedron(pts, faces);
+	double vol = 0, vol_x = 0, vol_y = 0, vol_z = 0, vol_xy = 0, vol_yz = 0, vol_zx = 0, vol_x2 = 0, vol_y2 = 0, vol_z2 = 0;
+	for (size_t f = 0; f < d.faces.size(); f++) {
+		const std::vector<int>& s = d.faces[f];
+		V3 n{d.gnorm[3 * f], d.gnorm[3 * f + 1], d.gnorm[3 * f + 2]};  // Surface::normal()
+		ProjAxis pa;
+		if (std::abs(n.z) > 0.01) pa = PZ;
+		else if (std::abs(n.x) >= std::abs(n.y) && std::abs(n.x) >= std::abs(n.z)) pa = PX;
+		else if (std::abs(n.y) >= std::abs(n.x) && std::abs(n.y) >= std::abs(n.z)) pa = PY;
+		else pa = PZ;
+		double g1 = 0, ga = 0, gb = 0, gab = 0, ga2 = 0, gb2 = 0, ga2b = 0, gab2 = 0, ga3 = 0, gb3 = 0;
+		int m = (int)s.size();
+		for (int i = 0; i < m; i++) {
+			V3 v = P(pts, s[i]), v2 = P(pts, s[(i + 1) % m]);
+			double va = axis_val(v, 0, pa), vb = axis_val(v, 1, pa);
+			double dA = axis_val(v2, 0, pa) - va, dB = axis_val(v2, 1, pa) - vb;
+			g1 += dB * (va + dA / 2.0);
+			ga += dB * (dA * dA / 3.0 + va * dA + va * va) / 2.0;
+			gb -= dA * (dB * dB / 3.0 + vb * dB + vb * vb) / 2.0;
+			gab -= dA * (va * vb * vb + dA * vb * vb / 2.0 + va * vb * dB + (va * dB * dB + 2 * vb * dA * dB) / 3.0 + dA * dB * dB / 4.0) / 2.0;
+			ga2 += dB * (va * va * va + 3 * va * va * dA / 2.0 + va * dA * dA + dA * dA * dA / 4.0) / 3.0;
+			gb2 -= dA * (vb * vb * vb + 3 * vb * vb * dB / 2.0 + vb * dB * dB + dB * dB * dB / 4.0) / 3.0;
+			double mixed = va * va * vb * vb + va * va * vb * dB + va * vb * vb * dA + (va * va * dB * dB + vb * vb * dA * dA + 4 * va * vb * dA * dB) / 3.0 +
+			               (va * dA * dB * dB + vb * dB * dA * dA) / 2.0 + dA * dA * dB * dB / 5.0;
+			ga2b -= dA * mixed / 2.0;
+			gab2 += dB * mixed / 2.0;
+			ga3 += dB * (va * va * va * va + 2 * va * va * va * dA + 2 * va * va * dA * dA + va * dA * dA * dA + dA * dA * dA * dA / 5.0) / 4.0;
+			gb3 -= dA * (vb * vb * vb * vb + 2 * vb * vb * vb * dB + 2 * vb * vb * dB * dB + vb * dB * dB * dB + dB * dB * dB * dB / 5.0) / 4.0;
 		}
+		double na = axis_val(n, 0, pa), nb = axis_val(n, 1, pa), nc = axis_val(n, 2, pa);
+		double nc_i = 1.0 / nc;
+		V3 sp = P(pts, s[0]);
+		double k = nc * axis_val(sp, 2, pa) + nb * axis_val(sp, 1, pa) + na * axis_val(sp, 0, pa);
+		Intrg a, b, c;
+		a.v = nc_i * ga;
+		b.v = nc_i * gb;
+		c.v = nc_i * nc_i * (k * g1 - na * ga - nb * gb);
+		a.v2 = nc_i * ga2;
+		b.v2 = nc_i * gb2;
+		c.v2 = nc_i * nc_i * nc_i * (k * k * g1 + na * na * ga2 + nb * nb * gb2 + 2 * (na * nb * gab - k * na * ga - k * nb * gb));
+		a.v2t = nc_i * ga2b;
+		b.v2t = nc_i * nc_i * (k * gb2 - na * gab2 - nb * gb3);
+		c.v2t = nc_i * nc_i * nc_i * (k * k * ga + na * na * ga3 + nb * nb * gab2 + 2 * (na * nb * ga2b - k * na * ga2 - k * nb * gab));
+		a.v3 = nc_i * ga3;
+		b.v3 = nc_i * gb3;
+		c.v3 = nc_i * nc_i * nc_i * nc_i *
+		       (k * k * k * g1 - na * na * na * ga3 - nb * nb * nb * gb3 - 3 * k * k * na * ga + 3 * k * na * na * ga2 - 3 * na * na * nb * ga2b -
+		        3 * na * nb * nb * gab2 + 3 * k * nb * nb * gb2 - 3 * k * k * nb * gb + 6 * k * na * nb * gab);
+		Intrg x, y, z;
+		if (pa == PX) { x = c; y = a; z = b; }
+		else if (pa == PY) { x = b; y = c; z = a; }
+		else { x = a; y = b; z = c; }
+		vol += n.x * x.v;
+		vol_x += n.x * x.v2 / 2.0;
+		vol_y += n.y * y.v2 / 2.0;
+		vol_z += n.z * z.v2 / 2.0;
+		vol_xy += n.x * x.v2t / 2.0;
+		vol_yz += n.y * y.v2t / 2.0;
+		vol_zx += n.z * z.v2t / 2.0;
+		vol_x2 += n.x * x.v3 / 3.0;
+		vol_y2 += n.y * y.v3 / 3.0;
+		vol_z2 += n.z * z.v3 / 3.0;
 	}
-	for (int i = 0; i < 3; i++) c[i] /= m;
-	double tr = C[0][0] + C[1][1] + C[2][2];
-	double I[3][3];
+	double ms = 0;
+	ms += density * vol;
+	V3 com{0, 0, 0};
+	com += V3{vol_x, vol_y, vol_z} * density;
+	double t[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+	t[0][0] += (vol_y2 + vol_z2) * density;
+	t[0][1] -= vol_xy * density;
+	t[0][2] -= vol_zx * density;
+	t[1][1] += (vol_x2 + vol_z2) * density;
+	t[1][2] -= vol_yz * density;
+	t[2][2] += (vol_x2 + vol_y2) * density;
+	com = V3{com.x / ms, com.y / ms, com.z / ms};
+	// recenterTensor(mass, tensor, com, 0, com)
+	V3 dd = com - V3{0, 0, 0}, cc = com - V3{0, 0, 0};
+	t[0][0] += ms * (dd.y * dd.y + dd.z * dd.z - 2 * dd.y * cc.y - 2 * dd.z * cc.z);
+	t[0][1] += ms * (dd.x * cc.y + dd.y * cc.x - dd.x * dd.y);
+	t[0][2] += ms * (dd.x * cc.z + dd.z * cc.x - dd.x * dd.z);
+	t[1][1] += ms * (dd.x * dd.x + dd.z * dd.z - 2 * dd.x * cc.x - 2 * dd.z * cc.z);
+	t[1][2] += ms * (dd.y * cc.z + dd.z * cc.y - dd.y * dd.z);
+	t[2][2] += ms * (dd.x * dd.x + dd.y * dd.y - 2 * dd.x * cc.x - 2 * dd.y * cc.y);
+	t[1][0] = t[0][1];
+	t[2][0] = t[0][2];
+	t[2][1] = t[1][2];
+	*mass = ms;
 	for (int i = 0; i < 3; i++)
-		for (int j = 0; j < 3; j++) I[i][j] = (i == j ? tr : 0.0) - C[i][j];
-	// shift from the centroid to the centre of mass
-	double d2 = c[0] * c[0] + c[1] * c[1] + c[2] * c[2];
-	for (int i = 0; i < 3; i++)
-		for (int j = 0; j < 3; j++) I[i][j] -= m * ((i == j ? d2 : 0.0) - c[i] * c[j]);
-	*mass = m;
-	for (int i = 0; i < 3; i++)
-		for (int j = 0; j < 3; j++) tensor9[3 * i + j] = I[i][j];
-	com3[0] = c[0] + ctr.x;
-	com3[1] = c[1] + ctr.y;
-	com3[2] = c[2] + ctr.z;
+		for (int j = 0; j < 3; j++) tensor9[3 * i + j] = t[i][j];
+	com3[0] = com.x;
+	com3[1] = com.y;
+	com3[2] = com.z;
+}
+
+// Mat3::inverse (Mat3.h:57-68): cofactors divided by the determinant.
+void mat3_inverse(const double* m9, double* out9) {
+	double a = m9[0], b = m9[1], c = m9[2], d = m9[3], e = m9[4], f = m9[5], g = m9[6], h = m9[7], i = m9[8];
+	double det = a * (e * i - f * h) - b * (d * i - f * g) + c * (d * h - e * g);
+	double o[9] = {(e * i - f * h), (c * h - b * i), (b * f - c * e), (f * g - d * i), (a * i - c * g), (c * d - a * f),
+	               (d * h - e * g), (b * g - a * h), (a * e - b * d)};
+	for (int k = 0; k < 9; k++) out9[k] = o[k] / det;
+}
+
+// A dynamic body built from Polyhedron(points, faces) (RigidBody ctor,
+// RigidBody.cpp:13-38): mass properties of the points as given, the Gauss map
+// (with its cached self-extrema) of the points as given, and the body-frame
+// points recomputeFromReference(identity, -com) leaves: iden * p + (-com).
+HullData polyhedron_body(const std::vector<double>& pts, const std::vector<std::vector<int>>& faces, double density, double* mp19) {
+	double com[3];
+	polyhedron_mass_properties(pts, faces, density, mp19, mp19 + 1, com);
+	mat3_inverse(mp19 + 1, mp19 + 10);
+	HullData d = build_polyhedron(pts, faces);
+	for (size_t i = 0; i + 2 < d.pts.size(); i += 3) {
+		double x = pts[i], y = pts[i + 1], z = pts[i + 2];
+		double r[3] = {1 * x + 0 * y + 0 * z, 0 * x + 1 * y + 0 * z, 0 * x + 0 * y + 1 * z};
+		for (int c = 0; c < 3; c++) d.pts[i + c] = -com[c] + r[c];
+	}
+	for (size_t f = 0; f + 2 < d.gnorm.size(); f += 3) {
+		double x = d.gnorm[f], y = d.gnorm[f + 1], z = d.gnorm[f + 2];
+		d.gnorm[f] = 1 * x + 0 * y + 0 * z;
+		d.gnorm[f + 1] = 0 * x + 1 * y + 0 * z;
+		d.gnorm[f + 2] = 0 * x + 0 * y + 1 * z;
+	}
+	return d;
 }
 
 }  // namespace cpx

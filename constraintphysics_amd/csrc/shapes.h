@@ -83,9 +83,15 @@ void build_cylinder(const double* center, double radius, double height, double* 
 HullData build_polyhedron(const std::vector<double>& pts, const std::vector<std::vector<int>>& faces);
 
 // Mass, inertia tensor about the centre of mass and the centre of mass of a
-// closed polyhedron of uniform density (divergence theorem over the
-// tetrahedra spanned by the origin and a fan triangulation of each face).
+// closed polyhedron of uniform density as calculateMassProperties computes
+// them (projection integrals, RigidBody.cpp:427-640).
 void polyhedron_mass_properties(const std::vector<double>& pts, const std::vector<std::vector<int>>& faces, double density,
                                 double* mass, double* tensor9, double* com3);
+// Mat3::inverse (Mat3.h:57-68).
+void mat3_inverse(const double* m9, double* out9);
+// The polyhedron of a body created from Polyhedron(points, faces): Gauss map
+// of the points as given, body-frame points shifted to the centre of mass;
+// mp19 = mass, reference tensor, its inverse.
+HullData polyhedron_body(const std::vector<double>& pts, const std::vector<std::vector<int>>& faces, double density, double* mp19);
 
 }  // namespace cpx

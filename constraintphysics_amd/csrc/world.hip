@@ -1862,6 +1862,28 @@ int cp_polyhedron_mass_properties(int n_pts, const double* pts, int n_faces, con
 	return CP_OK;
 }
 
+int cp_polyhedron_body(int n_pts, const double* pts, int n_faces, const int64_t* face_off, const int64_t* face_verts, double density,
+                       double* pts_out, double* mp19, double* com3) {
+	if (n_pts < 4 || n_faces < 4 || !pts || !face_off || !face_verts || !pts_out || !mp19 || !com3 || !(density > 0))
+		return CP_ERR_INVALID;
+	std::vector<std::vector<int>> faces;
+	for (int f = 0; f < n_faces; f++) {
+		std::vector<int> fv;
+		for (int64_t k = face_off[f]; k < face_off[f + 1]; k++) {
+			if (face_verts[k] < 0 || face_verts[k] >= n_pts) return CP_ERR_INVALID;
+			fv.push_back((int)face_verts[k]);
+		}
+		if (fv.size() < 3) return CP_ERR_INVALID;
+		faces.push_back(fv);
+	}
+	std::vector<double> p(pts, pts + 3 * (size_t)n_pts);
+	double t9[9];
+	polyhedron_mass_properties(p, faces, density, mp19, t9, com3);
+	HullData d = polyhedron_body(p, faces, density, mp19);
+	for (size_t k = 0; k < d.pts.size(); k++) pts_out[k] = d.pts[k];
+	return CP_OK;
+}
+
 int cp_body_add(cp_world* w, int shape_id, int movement, const double* mp, const double* material, int scene, int* body_index) {
 	if (!w || !mp || !material || !body_index) return CP_ERR_INVALID;
 	if (w->serving) return CP_ERR_STATE;  // serving worlds take whole scenes (cp_world_admit_scene)

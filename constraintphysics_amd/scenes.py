@@ -42,12 +42,20 @@ def box_points(dx, dy, dz):
     return p - np.array([dx, dy, dz]) / 2.0
 
 
-def inv3(m):
-    return np.linalg.inv(m)
-
-
 def mass_props_row(mass, tensor):
-    return np.concatenate([[mass], np.asarray(tensor).ravel(), inv3(tensor).ravel()])
+    """mass, reference tensor, its inverse as the reference stores it (Mat3::inverse)."""
+    return np.concatenate([[mass], np.asarray(tensor).ravel(), cpa.mat3_inverse(tensor).ravel()])
+
+
+_BOX_BODIES = {}
+
+
+def box_body(dx, dy, dz, density):
+    """The exact-hull layout and mass properties of a box body created centred on the origin."""
+    key = (dx, dy, dz, density)
+    if key not in _BOX_BODIES:
+        _BOX_BODIES[key] = cpa.polyhedron_body(box_points(dx, dy, dz), BOX_FACES, density)
+    return _BOX_BODIES[key]
 
 
 def quat_axis_angle(theta, axis):
@@ -67,20 +75,16 @@ class SceneBuilder:
         self.state = []
 
     def box_shape(self, dx, dy, dz, density):
+        """ConvexUnionGeometry::box centred on the origin (Geometry.cpp:14-37) as a body's polyhedron."""
         key = ("box", dx, dy, dz, density)
         if key not in self.shape_cache:
-            pts = box_points(dx, dy, dz)
-            sid = self.w.add_polyhedron(pts, BOX_FACES)
-            m, t, _ = cpa.polyhedron_mass_properties(pts, BOX_FACES, density)
-            self.shape_cache[key] = (sid, mass_props_row(m, t))
+            h = box_body(dx, dy, dz, density)
+            self.shape_cache[key] = (self.w.add_hull_exact(h), h["mass_props"])
         return self.shape_cache[key]
 
     def hull_shape(self, pts, faces, density):
-        m, t, com = cpa.polyhedron_mass_properties(pts, faces, density)
-        pts = np.asarray(pts, float) - com
-        m, t, _ = cpa.polyhedron_mass_properties(pts, faces, density)
-        sid = self.w.add_polyhedron(pts, faces)
-        return sid, mass_props_row(m, t)
+        h = cpa.polyhedron_body(pts, faces, density)
+        return self.w.add_hull_exact(h), h["mass_props"]
 
     def add(self, shape, mp, movement, material, scene, com, q=(1, 0, 0, 0), vel=(0, 0, 0), angv=(0, 0, 0)):
         self.add_many([shape], [mp], [movement], [material], [scene], [com], [q], [vel], [angv])
@@ -167,11 +171,9 @@ class HullLibrary:
         self.items = []
         for _ in range(count):
             pts, faces = random_hull(rng)
-            m, t, com = cpa.polyhedron_mass_properties(pts, faces, density)
-            pts = np.asarray(pts, float) - com
-            m, t, _ = cpa.polyhedron_mass_properties(pts, faces, density)
-            r = float(np.max(np.linalg.norm(pts, axis=1))) + 0.05
-            self.items.append(dict(pts=pts, faces=faces, mp=mass_props_row(m, t), r=r))
+            body = cpa.polyhedron_body(pts, faces, density)  # the body createRigidBody makes of these points
+            r = float(np.max(np.linalg.norm(body["pts"], axis=1))) + 0.05
+            self.items.append(dict(pts=pts, faces=faces, body=body, mp=body["mass_props"], r=r))
         self.r = np.array([it["r"] for it in self.items])
         self.mp = np.array([it["mp"] for it in self.items])
 
@@ -289,7 +291,7 @@ def build_batched(world, sizes, seed, scene_indices=None, hull_lib_size=256):
     b = SceneBuilder(world)
     lib = hull_library(seed, hull_lib_size)
     sph_r, sph_mp = sphere_library(seed, hull_lib_size)
-    lib_sid = np.array([world.add_polyhedron(it["pts"], it["faces"]) for it in lib.items])
+    lib_sid = np.array([world.add_hull_exact(it["body"]) for it in lib.items])
     sph_sid = np.array([world.add_sphere((0.0, 0.0, 0.0), r) for r in sph_r])
     cyl_geo, cyl_mp, cyl_r = cylinder_library(seed, hull_lib_size)
     cyl_sid = np.array([world.add_cylinder(g) for g in cyl_geo])
@@ -317,13 +319,11 @@ def scene_description(sizes, seed, s, hull_lib_size=256, mix=MIX):
     cyl_geo, cyl_mp, cyl_r = cylinder_library(seed, hull_lib_size)
     kind, picks, L, pos, q, vel = scene_bodies(sizes, seed, s, lib, sph_r, cyl_r, mix)
     hulls, move, mps, mats, state = [], [], [], [], []
-    cache = {}
     for (dx, dy, dz), c in container_walls(L):
-        pts = box_points(dx, dy, dz)
-        hulls.append(cpa.polyhedron_build(pts, BOX_FACES))
-        m, t, _ = cpa.polyhedron_mass_properties(pts, BOX_FACES, 1.0)
+        h = box_body(dx, dy, dz, 1.0)
+        hulls.append(h)
         move.append(1)
-        mps.append(mass_props_row(m, t))
+        mps.append(h["mass_props"])
         mats.append(DEFAULT_MATERIAL)
         state.append(np.concatenate([c, [1, 0, 0, 0], [0, 0, 0], [0, 0, 0]]))
     for i, p in enumerate(picks):
@@ -341,9 +341,7 @@ def scene_description(sizes, seed, s, hull_lib_size=256, mix=MIX):
             mats.append(DEFAULT_MATERIAL)
             state.append(np.concatenate([pos[i], q[i], vel[i], [0, 0, 0]]))
             continue
-        if p not in cache:
-            cache[p] = cpa.polyhedron_build(lib.items[p]["pts"], lib.items[p]["faces"])
-        hulls.append(cache[p])
+        hulls.append(lib.items[p]["body"])
         move.append(0)
         mps.append(lib.mp[p])
         mats.append(DEFAULT_MATERIAL)

@@ -43,17 +43,15 @@ class SceneSource:
         self.lib = scenes.hull_library(seed, hull_lib_size)
         self.sph_r, self.sph_mp = scenes.sphere_library(seed, hull_lib_size)
         geo, self.cyl_mp, self.cyl_r = scenes.cylinder_library(seed, hull_lib_size)
-        self.lib_sid = np.array([world.add_polyhedron(it["pts"], it["faces"]) for it in self.lib.items])
+        self.lib_sid = np.array([world.add_hull_exact(it["body"]) for it in self.lib.items])
         self.sph_sid = np.array([world.add_sphere((0.0, 0.0, 0.0), r) for r in self.sph_r])
         self.cyl_sid = np.array([world.add_cylinder(g) for g in geo])
         self.boxes = {}
 
     def _box(self, dims):
         if dims not in self.boxes:
-            pts = scenes.box_points(*dims)
-            sid = self.w.add_polyhedron(pts, scenes.BOX_FACES)
-            m, t, _ = scenes.cpa.polyhedron_mass_properties(pts, scenes.BOX_FACES, 1.0)
-            self.boxes[dims] = (sid, scenes.mass_props_row(m, t))
+            h = scenes.box_body(*dims, 1.0)
+            self.boxes[dims] = (self.w.add_hull_exact(h), h["mass_props"])
         return self.boxes[dims]
 
     def payload(self, s):

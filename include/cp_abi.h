@@ -150,6 +150,17 @@ int cp_polyhedron_build(int n_pts, const double* pts, int n_faces, const int64_t
  * (calculateMassProperties, RigidBody.cpp:427-640). */
 int cp_polyhedron_mass_properties(int n_pts, const double* pts, int n_faces, const int64_t* face_off,
                                   const int64_t* face_verts, double density, double* out13);
+/* Host-only: the polyhedron of a body created as createRigidBody(
+ * ConvexUnionGeometry(Polyhedron(points, faces), material)) (RigidBody ctor,
+ * RigidBody.cpp:13-38): pts_out[n_pts][3] = body-frame points (identity
+ * rotation and -com applied to the points as given, recomputeFromReference,
+ * ConvexPrimitive.cpp:715-729); mp19 = mass, reference tensor about the
+ * centre of mass, its inverse (Mat3::inverse, Mat3.h:57-68); com3 = centre of
+ * mass in the input frame.  The body's Gauss map, with the cached
+ * self-extrema of the points as given, is what cp_polyhedron_build returns
+ * for the same input. */
+int cp_polyhedron_body(int n_pts, const double* pts, int n_faces, const int64_t* face_off, const int64_t* face_verts,
+                       double density, double* pts_out, double* mp19, double* com3);
 
 /* Bodies.  Replaces PhysicsEngine::createRigidBody (PhysicsEngine.cpp:486-496).
  * movement: 0 DYNAMIC, 1 FIXED (RigidBody.h:22).  mass_props: mass,

@@ -903,6 +903,115 @@ static int cmd_terrain(int argc, char** argv) {
 	return 0;
 }
 
+// hulls: irregular convex hulls (points and faces read from a text file: the
+// count, then per hull "n_pts n_faces", the points, and each face as its
+// vertex count and indices) built as createRigidBody(ConvexUnionGeometry(
+// Polyhedron(points, faces))) does (RigidBody.cpp:13-38), mixed with spheres
+// and cylinders created centred on the origin, in an nx x ny x nz grid of
+// pitch `spacing` above the ground box; kinds cycle hull, hull, sphere,
+// cylinder; each body gets a random orientation (angle U[0, 2 pi) about a
+// normalised U[-1, 1]^3 axis) and velocity U[-0.5, 0.5]^3.  Spheres r U[0.3,
+// 0.7], cylinders r U[0.2, 0.5], h U[0.5, 1.5]; default material.
+struct HullIn {
+	std::vector<mthz::Vec3> pts;
+	std::vector<std::vector<uint32_t>> faces;
+};
+static std::vector<HullIn> read_hulls(const std::string& path) {
+	FILE* f = fopen(path.c_str(), "r");
+	if (!f) { perror(path.c_str()); exit(2); }
+	int k = 0;
+	if (fscanf(f, "%d", &k) != 1) exit(2);
+	std::vector<HullIn> out(k);
+	for (auto& h : out) {
+		int np = 0, nf = 0;
+		if (fscanf(f, "%d %d", &np, &nf) != 2) exit(2);
+		for (int i = 0; i < np; i++) {
+			char b[3][64];
+			if (fscanf(f, "%63s %63s %63s", b[0], b[1], b[2]) != 3) exit(2);
+			h.pts.push_back(mthz::Vec3(strtod(b[0], nullptr), strtod(b[1], nullptr), strtod(b[2], nullptr)));
+		}
+		for (int i = 0; i < nf; i++) {
+			int m = 0;
+			if (fscanf(f, "%d", &m) != 1) exit(2);
+			std::vector<uint32_t> fv(m);
+			for (auto& x : fv)
+				if (fscanf(f, "%u", &x) != 1) exit(2);
+			h.faces.push_back(fv);
+		}
+	}
+	fclose(f);
+	return out;
+}
+
+static int cmd_hulls(int argc, char** argv) {
+	int nx = (int)arg_d(argc, argv, "--n", 3);
+	int ny = (int)arg_d(argc, argv, "--ny", nx);
+	int steps = (int)arg_d(argc, argv, "--steps", 10);
+	double spacing = arg_d(argc, argv, "--spacing", 1.8);
+	uint64_t seed = (uint64_t)arg_d(argc, argv, "--seed", 31337);
+	std::vector<HullIn> in = read_hulls(arg_s(argc, argv, "--in", "hull_input.txt"));
+	SceneCfg cfg;
+	PhysicsEngine E;
+	configure(E, cfg);
+	phyz::Material m = phyz::Material::default_material();
+	RigidBody* g = E.createRigidBody(phyz::ConvexUnionGeometry::box(mthz::Vec3(-100, -1, -100), 200, 1, 200, m), RigidBody::FIXED);
+	pin_body(g);
+	SplitMix64 rng(seed);
+	std::vector<int64_t> hull_of{-1};  // per body: the input hull it was built from
+	int idx = 0, next_hull = 0;
+	const double PI_ = 3.14159265358979323846;
+	for (int j = 0; j < ny; j++)
+		for (int i = 0; i < nx; i++)
+			for (int k = 0; k < nx; k++) {
+				mthz::Vec3 c(spacing * (i - 0.5 * (nx - 1)) + rng.uniform(-0.05, 0.05), 1.0 + spacing * j + rng.uniform(-0.05, 0.05),
+				             spacing * (k - 0.5 * (nx - 1)) + rng.uniform(-0.05, 0.05));
+				double th = rng.uniform(0.0, 2.0 * PI_);
+				mthz::Vec3 ax(rng.uniform(-1.0, 1.0), rng.uniform(-1.0, 1.0), rng.uniform(-1.0, 1.0));
+				mthz::Vec3 v(rng.uniform(-0.5, 0.5), rng.uniform(-0.5, 0.5), rng.uniform(-0.5, 0.5));
+				double r = rng.uniform(0.3, 0.7), cr = rng.uniform(0.2, 0.5), h = rng.uniform(0.5, 1.5);
+				int kind = idx++ % 4;
+				RigidBody* b;
+				if (kind < 2) {
+					const HullIn& hi = in[next_hull % in.size()];
+					hull_of.push_back(next_hull % (int)in.size());
+					next_hull++;
+					b = E.createRigidBody(phyz::ConvexUnionGeometry(phyz::ConvexPrimitive((const phyz::ConvexGeometry&)phyz::Polyhedron(hi.pts, hi.faces), m)));
+				} else if (kind == 2) {
+					hull_of.push_back(-1);
+					b = E.createRigidBody(phyz::ConvexUnionGeometry::sphere(mthz::Vec3(0, 0, 0), r, m));
+				} else {
+					hull_of.push_back(-1);
+					b = E.createRigidBody(phyz::ConvexUnionGeometry::cylinder(mthz::Vec3(0, -h / 2.0, 0), cr, h, m));
+				}
+				b->setToPosition(c);
+				b->setOrientation(mthz::Quaternion(th, ax.normalize()));
+				b->setVel(v);
+				pin_body(b);
+			}
+	Out o(arg_s(argc, argv, "--out", "out"));
+	o.meta("seed", (double)seed);
+	std::vector<double> ip;
+	std::vector<int64_t> ipo{0}, ifo{0}, ifv, ifoff{0};
+	for (auto& h : in) {
+		for (auto& p : h.pts) ip.insert(ip.end(), {p.x, p.y, p.z});
+		ipo.push_back((int64_t)ip.size() / 3);
+		for (auto& fv : h.faces) {
+			for (uint32_t x : fv) ifv.push_back(x);
+			ifoff.push_back((int64_t)ifv.size());
+		}
+		ifo.push_back((int64_t)ifoff.size() - 1);
+	}
+	o.f64("input_hull_points", ip, {ip.size() / 3, 3});
+	o.i64("input_hull_point_off", ipo, {ipo.size()});
+	o.i64("input_hull_face_off", ifo, {ifo.size()});
+	o.i64("input_hull_face_idx", ifoff, {ifoff.size()});
+	o.i64("input_hull_face_verts", ifv, {ifv.size()});
+	o.i64("input_hull_of_body", hull_of, {hull_of.size()});
+	run_trace(E, cfg, steps, arg_steps(argc, argv, "--record"), o);
+	o.finish();
+	return 0;
+}
+
 // time: single-threaded wall time of the reference's own timeStep on a C0 pile.
 static int cmd_time(int argc, char** argv) {
 	int n = (int)arg_d(argc, argv, "--n", 10);
@@ -946,6 +1055,7 @@ int main(int argc, char** argv) {
 	if (cmd == "chain") return cmd_chain(argc, argv);
 	if (cmd == "terrain") return cmd_terrain(argc, argv);
 	if (cmd == "time") return cmd_time(argc, argv);
+	if (cmd == "hulls") return cmd_hulls(argc, argv);
 	fprintf(stderr, "unknown command %s\n", cmd.c_str());
 	return 2;
 }

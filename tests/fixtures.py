@@ -423,3 +423,21 @@ def world_from_description(desc, **cfg):
             w.add_joint(int(kind), int(id1) - 1, int(id2) - 1, g)
     w.set_state(desc["state"])
     return w
+
+
+CLIP_NAN_KEYS = ("pts", "face_off", "face_verts", "surface_id", "normal_calc", "normal_dir", "gauss_normal", "gauss_red", "gauss_ints",
+                 "gauss_vals", "arcs", "edges")
+
+
+def clip_nan_description():
+    """The scene of the clip-NaN state (tests/golden/clip_nan): its bodies as
+    they were built when the state was recorded (exact-hull layouts, spheres,
+    mass properties, materials), stored with the state."""
+    z = np.load(os.path.join(GOLDEN, "clip_nan", "desc.npz"))
+    hulls = []
+    for i, k in enumerate(z["kind"]):
+        if k == 1:
+            hulls.append(dict(sphere=z[f"b{i}_sphere"]))
+        else:
+            hulls.append({key: z[f"b{i}_{key}"] for key in CLIP_NAN_KEYS})
+    return dict(hulls=hulls, movement=z["movement"], mass_props=z["mass_props"], material=z["material"], state=z["state"])

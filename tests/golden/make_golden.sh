@@ -8,7 +8,7 @@ ROOT=$(cd "$HERE/../.." && pwd)
 make -C "$ROOT/oracle" ref REF="${REF:-/root/reference}"
 H="$ROOT/oracle/_ref/cp_ref_harness"
 cd "$HERE"
-rm -rf pile3 c0_full mix3 cyl3 cyl4 chain3 chain3h0 terrain
+rm -rf pile3 c0_full mix3 cyl3 cyl4 chain3 chain3h0 terrain hulls
 # C0 pattern, 3x3x3 cubes: full per-step dumps (state, caches, manifolds, islands).
 "$H" pile --n 3 --steps 40 --record 0,1,2,10,39 --out pile3
 # C0 in full (10x10x10 cubes): first step and a settled step (no mid/post caches).
@@ -28,3 +28,9 @@ rm -rf pile3 c0_full mix3 cyl3 cyl4 chain3 chain3h0 terrain
 # under a box/sphere/cylinder pile and two jointed chains: convex-triangle SAT,
 # several manifolds per body pair, the mesh AABB tree's candidate order.
 "$H" terrain --grid 12 --cell 0.5 --amp 0.3 --n 2 --chains 2 --links 4 --steps 240 --record 1,60,120,180,239 --out terrain
+# Irregular convex hulls (8-32 points on ellipsoids, the mixed-primitive
+# library's generator) built by the reference from points and faces, with
+# spheres and cylinders, falling onto the ground box: shape construction
+# (Gauss maps, mass properties), irregular hull-hull SAT and clipping.
+python3 make_hull_input.py hull_input.txt 24 20240
+"$H" hulls --in hull_input.txt --n 4 --ny 3 --steps 240 --record 0,60,100,149,239 --out hulls

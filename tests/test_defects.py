@@ -13,16 +13,13 @@ import os
 import numpy as np
 import pytest
 
-from fixtures import GOLDEN, Oracle
+from fixtures import GOLDEN, Oracle, clip_nan_description
 
 HERE = os.path.join(GOLDEN, "clip_nan")
 
 
 def _oracle(drop):
-    from constraintphysics_amd import scenes
-    sizes = scenes.c3_sizes(4096, 20240)
-    # the spheres-and-hulls mix the batched scenes used when this state was recorded
-    desc = scenes.scene_description(sizes, 20240, 1503, mix=(("sphere", 0.4), ("hull", 0.3)))
+    desc = clip_nan_description()
     o = Oracle(desc=desc, dt=1 / 120, gravity_y=-6.0, vel_iters=10, pos_iters=1, substeps=8, drop_nonfinite=drop)
     ld = lambda k: np.load(os.path.join(HERE, k + ".npy"))
     o.load(ld("st"), None, (ld("ii"), ld("mg"), ld("ff")))

@@ -1,8 +1,11 @@
-"""Cylinder narrowphase on the GPU against the oracle, bit for bit, on seeded
-random pairs of every order the dispatch distinguishes (CollisionDetect.cpp:
-57-152): hull-cylinder, cylinder-hull (normal negated, MagicID halves
-swapped), sphere-cylinder, cylinder-sphere and cylinder-cylinder, including
-near-parallel and near-perpendicular axes (cap faces, rims and barrels)."""
+"""Cylinder and irregular-hull narrowphase on the GPU against the oracle, bit
+for bit, on seeded random pairs of every order the dispatch distinguishes
+(CollisionDetect.cpp:57-152): hull-cylinder, cylinder-hull (normal negated,
+MagicID halves swapped), sphere-cylinder, cylinder-sphere, cylinder-cylinder
+and hull-hull (8-32-point hulls of the mixed-primitive library and boxes:
+the Gauss-arc pair search, its compaction in batches, the wave-parallel
+contact-area search and the clipping), including near-parallel and
+near-perpendicular axes (cap faces, rims and barrels)."""
 import math
 
 import numpy as np
@@ -38,17 +41,19 @@ def _body(kind, rng, lib):
         return dict(sphere=np.array([0, 0, 0, r])), mass_props_row(m, np.eye(3) * 0.4 * m * r * r), r
     if rng.uniform() < 0.5:  # box: face contacts against caps and barrels
         dims = rng.uniform(0.5, 1.2, 3)
-        pts = box_points(*dims)
-        m, t, _ = cpa.polyhedron_mass_properties(pts, BOX_FACES, 1.0)
-        return cpa.polyhedron_build(pts, BOX_FACES), mass_props_row(m, t), 0.5 * float(np.linalg.norm(dims))
+        h = cpa.polyhedron_body(box_points(*dims), BOX_FACES, 1.0)
+        return h, h["mass_props"], 0.5 * float(np.linalg.norm(dims))
     i = int(rng.integers(0, len(lib.items)))
-    return cpa.polyhedron_build(lib.items[i]["pts"], lib.items[i]["faces"]), lib.mp[i], lib.r[i]
+    return lib.items[i]["body"], lib.mp[i], lib.r[i]
 
 
-def _scene(seed, n_pairs=60):
+KINDS = [("hull", "cyl"), ("cyl", "hull"), ("sphere", "cyl"), ("cyl", "sphere"), ("cyl", "cyl"), ("hull", "hull")]
+
+
+def _scene(seed, n_pairs=72):
     rng = np.random.default_rng(seed)
     lib = hull_library(20240, 32)
-    kinds = [("hull", "cyl"), ("cyl", "hull"), ("sphere", "cyl"), ("cyl", "sphere"), ("cyl", "cyl")]
+    kinds = KINDS
     hulls, mps, state, pairs = [], [], [], []
     for p in range(n_pairs):
         ka, kb = kinds[p % len(kinds)]
@@ -77,16 +82,20 @@ def test_cylinder_pairs_bit_exact(seed):
     w = world_from_description(desc, pgs_mode=1)
     counts, heads, pts, mg = w.narrowphase(pairs)
     hit = 0
+    hits = {k: 0 for k in KINDS}
     for i, (a, b) in enumerate(pairs):
         head, opts, omg = o.narrowphase(int(a), int(b))
         assert counts[i] == len(opts), (seed, i, counts[i], len(opts))
         if len(opts) == 0:
             continue
         hit += 1
+        hits[KINDS[i % len(KINDS)]] += 1
         assert np.array_equal(heads[i], head), (seed, i)
         assert np.array_equal(pts[i, :len(opts)], opts), (seed, i)
         assert np.array_equal(mg[i, :len(opts)], omg), (seed, i)
-    assert hit >= 20
+    print(f"seed {seed}: contacts per kind {hits}", flush=True)
+    assert hit >= 24
+    assert hits[("hull", "hull")] >= 5
 
 
 def test_cylinder_pile_full_step_matches_oracle():

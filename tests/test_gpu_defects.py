@@ -6,9 +6,7 @@ import os
 import numpy as np
 import pytest
 
-import constraintphysics_amd as cpa
-from constraintphysics_amd import scenes
-from fixtures import GOLDEN, Oracle
+from fixtures import GOLDEN, Oracle, clip_nan_description
 
 pytestmark = pytest.mark.gpu
 HERE = os.path.join(GOLDEN, "clip_nan")
@@ -17,9 +15,7 @@ HERE = os.path.join(GOLDEN, "clip_nan")
 @pytest.mark.parametrize("fix", [0, 1])
 def test_clip_nan_case(fix):
     from fixtures import world_from_description
-    sizes = scenes.c3_sizes(4096, 20240)
-    # the spheres-and-hulls mix the batched scenes used when this state was recorded
-    desc = scenes.scene_description(sizes, 20240, 1503, mix=(("sphere", 0.4), ("hull", 0.3)))
+    desc = clip_nan_description()
     ld = lambda k: np.load(os.path.join(HERE, k + ".npy"))
     desc["state"] = ld("st")
     w = world_from_description(desc, dt=1 / 120, vel_iters=10, pos_iters=1, substeps=8, defect_fixes=fix)

@@ -40,7 +40,7 @@ def c0():
 
 
 CASES = ([("pile3", k) for k in PILE_STEPS] + [("mix3", k) for k in (0, 5, 30, 59)] + [("cyl3", k) for k in (0, 10, 30, 60, 79)] +
-         [("cyl4", k) for k in (40, 90, 149)])
+         [("cyl4", k) for k in (40, 90, 149)] + [("hulls", k) for k in (60, 100, 149, 239)])
 
 
 @pytest.mark.parametrize("name,k", CASES)
@@ -160,7 +160,8 @@ def test_colored_mode_converges_to_reference_order_solution(pile3):
     assert d_hi < 0.25 * d_lo, (d_hi, d_lo)
 
 
-@pytest.mark.parametrize("name,k", [("pile3", 10), ("pile3", 39), ("mix3", 30), ("mix3", 59), ("cyl3", 79), ("cyl4", 149)])
+@pytest.mark.parametrize("name,k", [("pile3", 10), ("pile3", 39), ("mix3", 30), ("mix3", 59), ("cyl3", 79), ("cyl4", 149),
+                                    ("hulls", 149), ("hulls", 239)])
 def test_colored_schedule_replayed_by_oracle(name, k):
     """The coloured sweep is a sequential Gauss-Seidel order in disguise (edges
     of one colour share no dynamic body).  The oracle replaying the GPU's

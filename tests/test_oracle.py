@@ -72,10 +72,13 @@ def test_c0_full_first_and_settled_step():
 
 
 @pytest.mark.parametrize("name,k", [("mix3", 0), ("mix3", 5), ("mix3", 30), ("mix3", 59), ("cyl3", 0), ("cyl3", 10), ("cyl3", 30),
-                                    ("cyl3", 60), ("cyl3", 79), ("cyl4", 40), ("cyl4", 90), ("cyl4", 149)])
+                                    ("cyl3", 60), ("cyl3", 79), ("cyl4", 40), ("cyl4", 90), ("cyl4", 149), ("hulls", 60),
+                                    ("hulls", 100), ("hulls", 149), ("hulls", 239)])
 def test_mixed_primitives_full_step(name, k):
     """mix3: boxes and spheres; cyl3: boxes, spheres and tilted cylinders
-    (poly-cylinder, sphere-cylinder, cylinder-cylinder and cylinder-ground pairs)."""
+    (poly-cylinder, sphere-cylinder, cylinder-cylinder and cylinder-ground pairs);
+    hulls: irregular 8-32-point hulls with spheres and cylinders (hull-hull
+    Gauss-arc SAT on irregular polyhedra)."""
     fx = golden(name)
     o = Oracle(fx)
     _load(o, fx, k)
