@@ -44,18 +44,27 @@ def pgs_bytes_per_contact(vel_iters, pos_iters, word=8):
     return word * (32 + vel_iters * 62 + pos_iters * 21)
 
 
-def profiled_traffic(dtype, scenes, seed):
+def profiled_traffic(dtype, scenes, seed, settle, live):
     """HBM bytes per PGS solve from the committed rocprofv3 PMC summary of the
-    same bench command (FETCH_SIZE and WRITE_SIZE, copy-kernel calibrated), or
-    None when no summary covers this workload."""
+    same bench workload (FETCH_SIZE and WRITE_SIZE, copy-kernel calibrated),
+    or (None, None) unless the summary states the same scenes, seed and settle
+    steps and a live-contact count within 10% of this run's (the traffic per
+    solve scales with the contacts in the window)."""
     path = os.path.join(ROOT, "profiles", f"c3_batched_{dtype}_kernel_stats.md")
-    if scenes != 4096 or seed != 20240 or not os.path.exists(path):
+    if not os.path.exists(path):
         return None, None
+    meta = {}
     with open(path) as f:
         for line in f:
-            if line.startswith("traffic_bytes_per_launch:"):
-                return float(line.split(":")[1]), os.path.relpath(path, ROOT)
-    return None, None
+            if ":" in line and not line.startswith((" ", "#", "`")):
+                k, v = line.split(":", 1)
+                meta[k.strip()] = v.strip()
+    try:
+        ok = (int(meta["scenes"]) == scenes and int(meta["seed"]) == seed and int(meta["settle_steps"]) == settle and
+              abs(float(meta["live_contacts"]) - live) <= 0.1 * float(meta["live_contacts"]))
+        return (float(meta["traffic_bytes_per_launch"]), os.path.relpath(path, ROOT)) if ok else (None, None)
+    except (KeyError, ValueError):
+        return None, None
 
 
 PRECISIONS = {"fp64": 0, "split": 1}
@@ -80,24 +89,13 @@ def time_steps(w, k):
 
 
 def scene_offsets(sizes):
-    """First body index of each scene in the batched world (6 walls + n bodies per scene)."""
     from constraintphysics_amd import scenes
-    cnt = np.asarray(sizes) + scenes.CONTAINER_WALLS
-    return np.concatenate([[0], np.cumsum(cnt)])
+    return scenes.scene_offsets(sizes)
 
 
 def scene_snapshot(state, contacts, off, s):
-    """State rows and contact cache of scene s, renumbered to a world holding only that scene."""
-    a, b = int(off[s]), int(off[s + 1])
-    ii, mg, ff = contacts
-    sel = (ii[:, 0] > a) & (ii[:, 0] <= b)
-    ii, mg, ff = ii[sel].copy(), mg[sel].copy(), ff[sel].copy()
-    ii[:, :2] -= a
-    shift = np.uint64(16 * a)
-    lo = (mg[:, 0] & np.uint64(0xFFFFFFFF)) - shift
-    hi = (mg[:, 0] >> np.uint64(32)) - shift
-    mg[:, 0] = lo | (hi << np.uint64(32))
-    return state[a:b], (ii, mg, ff)
+    from constraintphysics_amd import scenes
+    return scenes.scene_snapshot(state, contacts, off, s)
 
 
 def cpu_baseline(sizes, seed, budget_s, steps, state, contacts, off):
@@ -206,11 +204,76 @@ def c2_leg(mode=2, warmup=150, steps=30):
     return res
 
 
+def c0_leg(steps=300, cpu_runs=3, cpu_steps=10, cpu_from=240):
+    """The plain CPU-runnable configuration: 1000 unit cubes (10 x 10 x 10,
+    1.1 m pitch, U[-0.05, 0.05] jitter, random yaw) on a ground box, density 1,
+    restitution 0, friction 0.5, dt 1/120, 8 substeps, 10 + 1 PGS sweeps, FP64,
+    300 steps.  GPU: the coloured single-scene path and the reference-order
+    path (the reference's sequential sweep order, bit-compatible), each for
+    all 300 steps from the same start.  CPU: the FP64 oracle (a bit-exact
+    restatement of the reference step, one thread) continuing the
+    reference-order run's state and contact cache at step `cpu_from`: median
+    of `cpu_runs` runs of `cpu_steps` steps, p50/p99 over all their steps."""
+    import constraintphysics_amd as cpa
+    from constraintphysics_amd import scenes
+    desc = scenes.c0_description()
+    n_dyn = int((desc["movement"] == 0).sum())
+    out = {}
+    snap = None
+    for mode, name in ((0, "gpu_colored"), (1, "gpu_reference_order")):
+        w = cpa.World(dt=1.0 / 120.0, gravity=(0.0, -6.0, 0.0), vel_iters=10, pos_iters=1, substeps=8, pgs_mode=mode, precision=0)
+        scenes.build_description(w, desc)
+        lat = []
+        for k in range(steps):
+            if mode == 1 and k == cpu_from:
+                snap = (w.state(), w.inertia(), w.contacts())
+            t0 = time.perf_counter()
+            w.step(1)
+            lat.append(time.perf_counter() - t0)
+        lat = np.array(lat)
+        st = w.state()
+        w.close()
+        out[name] = dict(value=round(n_dyn * steps / float(lat.sum()), 1), ms_per_step=round(1e3 * float(lat.mean()), 3),
+                         p50_step_ms=round(1e3 * float(np.percentile(lat, 50)), 3),
+                         p99_step_ms=round(1e3 * float(np.percentile(lat, 99)), 3), finite=bool(np.all(np.isfinite(st))),
+                         min_height=round(float(st[1:, 1].min()), 4))
+        print(f"[bench] c0 {name}: {out[name]['ms_per_step']:.2f} ms/step, {out[name]['value']:.0f} body-steps/s", file=sys.stderr,
+              flush=True)
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from fixtures import Oracle  # the checker, used here only for the reported CPU baseline
+    rates, lat = [], []
+    for _ in range(cpu_runs):
+        o = Oracle(desc=desc, dt=1.0 / 120.0, gravity_y=-6.0, vel_iters=10, pos_iters=1, substeps=8, drop_nonfinite=1)
+        o.load(snap[0], snap[1], snap[2])
+        t = []
+        for _ in range(cpu_steps):
+            t0 = time.perf_counter()
+            o.step()
+            t.append(time.perf_counter() - t0)
+        rates.append(n_dyn * cpu_steps / sum(t))
+        lat += t
+    lat = np.array(lat)
+    out["cpu_oracle"] = dict(value=round(float(np.median(rates)), 1), runs=[round(r, 1) for r in rates], cores=1, kind="port",
+                             p50_step_ms=round(1e3 * float(np.percentile(lat, 50)), 2),
+                             p99_step_ms=round(1e3 * float(np.percentile(lat, 99)), 2),
+                             sample=f"median of {cpu_runs} runs of {cpu_steps} steps continuing the reference-order GPU run's "
+                                    f"state and contact cache at step {cpu_from} (settled pile), one thread")
+    print(f"[bench] c0 cpu oracle: {out['cpu_oracle']['value']:.0f} body-steps/s (p50 {out['cpu_oracle']['p50_step_ms']:.0f} ms)",
+          file=sys.stderr, flush=True)
+    out["workload"] = ("1000 unit cubes (10x10x10, pitch 1.1 m, jitter U[-0.05,0.05], random yaw) on a fixed ground box, density 1, "
+                       "restitution 0, friction 0.5; dt 1/120, 8 substeps, PGS 10 vel + 1 pos sweeps, FP64, 300 steps")
+    out["unit"] = "body-steps/s"
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=240)
+    ap.add_argument("--warmup", type=int, default=5, help="untimed steps right before the timed window")
+    ap.add_argument("--settle", type=int, default=240,
+                    help="steps from the initial state before the warm-up, so that the timed window sees settled scenes "
+                         "(the autotuner's trial steps are taken inside them)")
     ap.add_argument("--scenes", type=int, default=4096)
     ap.add_argument("--seed", type=int, default=20240)
     ap.add_argument("--plain-scenes", type=int, default=16)
@@ -219,9 +282,11 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-c2", action="store_true")
+    ap.add_argument("--no-c0", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default PGS size classes and workgroup widths")
     ap.add_argument("--no-graph", action="store_true", help="skip the HIP-graph replay leg")
     ap.add_argument("--c2-only", action="store_true", help="run only the holonomic-chains-on-terrain leg and print its JSON")
+    ap.add_argument("--c0-only", action="store_true", help="run only the 1000-cube pile leg and print its JSON")
     ap.add_argument("--c2-mode", type=int, choices=[1, 2], default=None,
                     help="with --c2-only: only this island solve (1 reference order, 2 coloured islands; default both)")
     ap.add_argument("--serve-requests", type=int, default=6000, help="requests of the serving leg (Poisson arrivals at 2000/s)")
@@ -239,6 +304,9 @@ def main():
         names = {1: "reference_order", 2: "island_colored"}
         print(json.dumps({names[m]: c2_leg(m) for m in modes}), flush=True)
         return
+    if args.c0_only:
+        print(json.dumps(c0_leg()), flush=True)
+        return
 
     from constraintphysics_amd import scenes
     sizes = scenes.c3_sizes(args.scenes, args.seed)
@@ -247,24 +315,30 @@ def main():
     t_build = time.perf_counter() - t_build
     n_dyn = int(info["dynamic_bodies"])
     print(f"[bench] built {args.scenes} scenes, {n_dyn} dynamic bodies in {t_build:.1f}s", file=sys.stderr, flush=True)
-    # the autotuner runs inside the warm-up (its trials are steps of the same
-    # run), so the timed window starts at the same step with or without it
+    # Settle: `--settle` steps from the initial state (free flight at first,
+    # then piles), so that the timed window measures the contact pipeline on
+    # settled scenes.  The autotuner's trials are steps of this phase, taken
+    # near its end where the contact state changes slowly.
     tuned = None
     i = 0
-    while i < args.warmup:
-        if tuned is None and not args.no_tune and i >= max(0, args.warmup - 80):
+    tune_at = max(0, args.settle - 60)
+    while i < args.settle:
+        if tuned is None and not args.no_tune and i >= tune_at:
             from constraintphysics_amd import autotune
             tuned = autotune.tune(w, log=None)
             i += tuned["steps"]
-            print(f"[bench] autotune: classes {tuned['pgs_class_bodies']} threads {tuned['pgs_class_threads']}, PGS "
-                  f"{tuned['default_pgs_ms']:.3f} -> {tuned['pgs_ms']:.3f} ms/step over {tuned['trials']} trials", file=sys.stderr,
-                  flush=True)
+            print(f"[bench] autotune at step {tune_at}: classes {tuned['pgs_class_bodies']} threads {tuned['pgs_class_threads']}, PGS "
+                  f"{tuned['default_pgs_ms']:.3f} -> {tuned['pgs_ms']:.3f} ms/step over {tuned['trials']} trials ({tuned['steps']} steps)",
+                  file=sys.stderr, flush=True)
             continue
         w.step(1)
         i += 1
         if i % 40 == 0:
-            print(f"[bench] warmup {i}/{args.warmup}", file=sys.stderr, flush=True)
-    # settled state at the start of the timed window, for the plain and CPU legs
+            print(f"[bench] settle {i}/{args.settle}", file=sys.stderr, flush=True)
+    settled = i
+    for _ in range(args.warmup):
+        w.step(1)
+    # state at the start of the timed window, for the plain and CPU legs
     snap_state = w.state()
     snap_contacts = w.contacts()
     off = scene_offsets(sizes)
@@ -278,12 +352,12 @@ def main():
     # Accuracy, on one extra (untimed) step from the state after the timed
     # window, solved once in FP64 and once in the split precision: the FP64
     # natural-map residual of each solve, and the split result's deviation
-    # from the FP64 result (the FP64 path replays the oracle's sequential
-    # order to 1e-12 in the parity tests).
-    acc_state, acc_contacts = w.state(), w.contacts()
+    # from the FP64 result (the FP64 coloured solve equals the oracle
+    # replaying its colour schedule to 1e-12; tests/test_gpu_headline.py).
+    acc_state, acc_inertia, acc_contacts = w.state(), w.inertia(), w.contacts()
     acc = {}
     for name in ("fp64", "split"):
-        w.set_state(acc_state)
+        w.set_state(acc_state, acc_inertia)
         w.set_contacts(*acc_contacts)
         w.set_config(diagnostics=1, precision=PRECISIONS[name])
         w.step(1)
@@ -301,7 +375,7 @@ def main():
     graph_leg = None
     if not args.no_graph:
         w.set_config(diagnostics=0, precision=PRECISIONS[args.precision], graph=1)
-        w.set_state(acc_state)
+        w.set_state(acc_state, acc_inertia)
         w.set_contacts(*acc_contacts)
         w.step(1)
         gl = time_steps(w, args.steps)
@@ -321,8 +395,8 @@ def main():
     achieved = live * bpc / (pgs_ms_per_launch * 1e-3) / 1e9 if pgs_ms_per_launch > 0 else 0.0
     phases = dict(zip(["broadphase_ms", "narrowphase_ms", "persist_schedule_ms", "substeps_ms", "geometry_ms", "pgs_ms"],
                       [round(float(x), 3) for x in tim[:, :6].mean(0)]))
-    print(f"[bench] timed {args.steps} steps: {ms_step:.2f} ms/step, {value:.0f} body-steps/s; phases {phases}", file=sys.stderr,
-          flush=True)
+    print(f"[bench] timed {args.steps} steps: {ms_step:.2f} ms/step, {value:.0f} body-steps/s, {live:.0f} live contacts; phases {phases}",
+          file=sys.stderr, flush=True)
     w.close()
 
     plain = None
@@ -347,8 +421,8 @@ def main():
             ws.close()
         plain = dict(value=pb / pt, scenes=len(pick), graph_replay_value=round(pb / pg, 1),
                      note="each sampled scene (spread over the size distribution) stepped alone in its own world from the "
-                          "batched run's settled state and contact cache, 2 untimed steps then the same number of timed steps; "
-                          "graph_replay_value: the same scenes continued with the substep loop replayed from a HIP graph")
+                          "batched run's state at the start of the timed window, 2 untimed steps then the same number of timed "
+                          "steps; graph_replay_value: the same scenes continued with the substep loop replayed from a HIP graph")
         print(f"[bench] plain single-scene path: {plain['value']:.0f} body-steps/s (graph replay {pb / pg:.0f})", file=sys.stderr,
               flush=True)
 
@@ -356,17 +430,18 @@ def main():
     if not args.no_serve:
         served = serving_leg(args)
     c2 = None if args.no_c2 else c2_leg()
+    c0 = None if args.no_c0 else c0_leg()
 
     cpu = None
     if not args.no_cpu:
         v, used, spent = cpu_baseline(sizes, args.seed, args.cpu_budget, args.steps, snap_state, snap_contacts, off)
         cpu = dict(value=round(v, 1), unit="body-steps/s", cores=1, kind="port",
-                   sample=f"FP64 oracle (bit-exact restatement of the reference step, single thread) continuing the settled "
-                          f"state and contact cache of sampled scenes (index, bodies, steps) {used}: {spent:.1f}s of CPU; "
-                          f"host has {os.cpu_count()} cores")
+                   sample=f"FP64 oracle (bit-exact restatement of the reference step, single thread) continuing the state and "
+                          f"contact cache at the start of the timed window of sampled scenes (index, bodies, steps) {used}: "
+                          f"{spent:.1f}s of CPU; host has {os.cpu_count()} cores")
         print(f"[bench] cpu baseline: {v:.0f} body-steps/s", file=sys.stderr, flush=True)
 
-    traffic, traffic_src = profiled_traffic(args.precision, args.scenes, args.seed)
+    traffic, traffic_src = profiled_traffic(args.precision, args.scenes, args.seed, settled, live)
     out = {
         "metric": "body-steps/s vs plain GPU path at stated FP64 residual; p50/p99 step latency",
         "value": round(value, 1),
@@ -385,25 +460,33 @@ def main():
         "config": {
             "workload": f"batched scenes: {args.scenes} scenes, bodies log-uniform [8, 2048] (seed {args.seed}, per-scene seed base+i), "
                         f"box containers at 20k/60^3 bodies per m^3, v ~ N(0,1); dt 1/120, 8 substeps, PGS 10 vel + 1 pos sweeps, "
-                        f"graph-coloured PGS, " + ("FP64" if args.precision == "fp64" else "FP32 PGS rows/impulses over FP64 state"),
+                        f"graph-coloured PGS, " + ("FP64" if args.precision == "fp64" else "FP32 PGS rows/impulses over FP64 state") +
+                        f"; timed after {settled} settle steps (autotuner trials included) and {args.warmup} warm-up steps",
             "scenes": args.scenes,
             "dynamic_bodies": n_dyn,
+            "settle_steps": settled,
+            "steps_before_timing": settled + args.warmup,
+            "live_contacts": round(live, 1),
             "primitive_mix": info["mix"],
             "phases_ms": phases,
         },
+        "accuracy_bar": "<=1e-4 relative is met by split_vs_fp64_deviation.velocities_rel (the split solve against the FP64 solve, "
+                        "which equals the oracle's replay of the colour schedule to 1e-12); fp64_residual is the natural-map "
+                        "residual of 10 unconverged PGS sweeps, reported as a property of the solve, not held to 1e-4",
         "fp64_residual": {"relative": float(resid[2]), "abs": float(resid[0]), "lambda_norm": float(resid[1]), "rows": int(resid[3]),
+                          "split_relative": float(acc["split"][0][2]), "fp64_relative": float(acc["fp64"][0][2]),
                           "note": "natural-map residual ||lambda - proj(lambda + A^-1 (t - J dv))|| / ||lambda|| of the last substep's "
                                   "velocity rows after 10 coloured sweeps, evaluated in FP64"},
-        "fp64_oracle_deviation": {"impulses_rel": dev_imp, "velocities_rel": dev_vel,
-                                  "note": "one step from the same state solved in the split precision vs FP64 (the FP64 coloured "
-                                          "solve equals the oracle replaying its schedule to 1e-12): relative 2-norm over all "
-                                          "contact impulses / body velocities"},
+        "split_vs_fp64_deviation": {"impulses_rel": dev_imp, "velocities_rel": dev_vel,
+                                    "note": "one step from the same state solved in the split precision vs FP64: relative 2-norm "
+                                            "over all contact impulses / body velocities"},
         "plain_gpu_path": plain,
         "speedup_vs_plain": round(value / plain["value"], 2) if plain else None,
         "autotune": tuned,
         "graph_replay": graph_leg,
         "serving": served,
         "holonomic_chains_on_terrain": c2,
+        "plain_cube_pile": c0,
         "serving_speedup_vs_plain": round(served["value"] / plain["value"], 2) if (plain and served) else None,
         "roofline": {
             "bound": "hbm",

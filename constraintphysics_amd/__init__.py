@@ -77,6 +77,7 @@ def lib():
         "cp_world_set_state": (_I, [_P, _P, _P]),
         "cp_world_get_state": (_I, [_P, _P]),
         "cp_world_get_aabbs": (_I, [_P, _P]),
+        "cp_world_get_inertia": (_I, [_P, _P]),
         "cp_world_set_contacts": (_I, [_P, _I, _P, _P, _P]),
         "cp_world_get_contact_count": (_I, [_P, _P]),
         "cp_world_get_contacts": (_I, [_P, _I, _P, _P, _P, _P]),
@@ -88,6 +89,7 @@ def lib():
         "cp_world_last_islands": (_I, [_P, _I, _P, _I, _P, _P, _P]),
         "cp_world_last_timing": (_I, [_P, _P]),
         "cp_world_last_residual": (_I, [_P, _P]),
+        "cp_world_last_scene_contacts": (_I, [_P, _I, _P, _P]),
         "cp_shape_add_mesh": (_I, [_P, _I, _P, _I, _P, _P, _P]),
         "cp_narrowphase_multi": (_I, [_P, _I, _P, _I, _P, _P, _P, _P, _P, _P]),
         "cp_world_serving_init": (_I, [_P, _I]),
@@ -366,6 +368,12 @@ class World:
         _chk(self.L.cp_world_get_state(self.w, s.ctypes.data), "cp_world_get_state")
         return s
 
+    def inertia(self):
+        """World inertia tensor and inverse per body [n, 18] (exact restore with set_state)."""
+        a = np.zeros((self.n, 18))
+        _chk(self.L.cp_world_get_inertia(self.w, a.ctypes.data), "cp_world_get_inertia")
+        return a
+
     def aabbs(self):
         a = np.zeros((self.n, 6))
         _chk(self.L.cp_world_get_aabbs(self.w, a.ctypes.data), "cp_world_get_aabbs")
@@ -425,6 +433,14 @@ class World:
         _chk(self.L.cp_world_last_islands(self.w, nr.value, rows.ctypes.data, ni.value + 1, offs.ctypes.data, ctypes.byref(nr),
                                           ctypes.byref(ni)), "cp_world_last_islands")
         return rows[:nr.value], offs
+
+    def last_scene_contacts(self):
+        """First post-narrowphase cache index of each scene's contacts (n_scenes + 1)."""
+        n = ctypes.c_int()
+        self.L.cp_world_last_scene_contacts(self.w, 0, None, ctypes.byref(n))
+        out = np.zeros(max(n.value, 1), np.int64)
+        _chk(self.L.cp_world_last_scene_contacts(self.w, n.value, out.ctypes.data, ctypes.byref(n)), "cp_world_last_scene_contacts")
+        return out[:n.value]
 
     def last_residual(self):
         r = np.zeros(4)

@@ -19,13 +19,25 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=o
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-result"]
 
 
-def build(verbose=False):
+def _stale(src, obj):
+    """True when obj is missing or older than its source or any header in csrc/."""
+    if not os.path.exists(obj):
+        return True
+    t = os.path.getmtime(obj)
+    deps = [src] + [os.path.join(CSRC, h) for h in os.listdir(CSRC) if h.endswith(".h")]
+    return any(os.path.getmtime(d) > t for d in deps)
+
+
+def build(verbose=False, incremental=False):
+    """Compiles every translation unit (incremental=True: only those older
+    than their source or a header) and links libcp_hip.so."""
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     objs, cmds = [], []
     for src in SOURCES:
         obj = os.path.join(CSRC, "_obj", src + ".o")
         os.makedirs(os.path.dirname(obj), exist_ok=True)
-        cmds.append([hipcc] + FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj])
+        if not incremental or _stale(os.path.join(CSRC, src), obj):
+            cmds.append([hipcc] + FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
     # one compiler process per translation unit, a few at a time
     jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", "4")), os.cpu_count() or 1))
@@ -42,4 +54,4 @@ def build(verbose=False):
 
 
 if __name__ == "__main__":
-    build(verbose="-v" in sys.argv)
+    build(verbose="-v" in sys.argv, incremental="--incremental" in sys.argv)

@@ -209,7 +209,7 @@ struct cp_world {
 	int cls_max[4] = {0, 0, 0, 0};
 	DArr<int> cls_dev[4];
 	double last_resid[4] = {0, 0, 0, 0};
-	std::vector<int64_t> last_rows, last_offs;
+	std::vector<int64_t> last_rows, last_offs, last_scene_coff;
 	hipEvent_t ev[10];
 	hipEvent_t pgs_ev[2 * 64];
 	double last_ms[8] = {0};
@@ -1384,6 +1384,14 @@ void cp_world::record_colored_schedule() {
 	down(live, cb[cur].c.live, ncontacts);
 	last_rows.clear();
 	last_offs.assign(1, 0);
+	// first cache index of each scene's contacts (keys are lo << 32 | hi, sorted)
+	std::vector<uint64_t> keys;
+	down(keys, cb[cur].c.key, ncontacts);
+	last_scene_coff.assign(nscenes + 1, ncontacts);
+	for (int k = ncontacts - 1; k >= 0; k--) {
+		int sc = b_scene[(size_t)(keys[k] >> 32)];
+		for (int t = sc; t >= 0 && last_scene_coff[t] > k; t--) last_scene_coff[t] = k;
+	}
 	for (int s = 0; s < nscenes; s++) {
 		for (int c = 0; c < nc[s]; c++) {
 			const int* o = &co[(size_t)s * (CP_MAXC + 2)];
@@ -1993,6 +2001,22 @@ int cp_world_get_state(cp_world* w, double* s) {
 	});
 }
 
+int cp_world_get_inertia(cp_world* w, double* inertia) {
+	if (!w || !inertia || !w->uploaded) return CP_ERR_INVALID;
+	return guard(w, [&] {
+		HIPCHK(hipStreamSynchronize(w->stream));
+		int n = w->nb;
+		std::vector<double> I, Ii;
+		down(I, w->I.p, 9 * (size_t)n);
+		down(Ii, w->Iinv.p, 9 * (size_t)n);
+		for (int i = 0; i < n; i++)
+			for (int c = 0; c < 9; c++) {
+				inertia[18 * (size_t)i + c] = I[c * (size_t)n + i];
+				inertia[18 * (size_t)i + 9 + c] = Ii[c * (size_t)n + i];
+			}
+	});
+}
+
 int cp_world_get_aabbs(cp_world* w, double* a) {
 	if (!w || !a || !w->uploaded) return CP_ERR_INVALID;
 	return guard(w, [&] {
@@ -2498,6 +2522,15 @@ int cp_world_last_islands(cp_world* w, int cap_rows, int64_t* rows2, int cap_off
 	memcpy(rows2, w->last_rows.data(), w->last_rows.size() * sizeof(int64_t));
 	memcpy(offs, w->last_offs.data(), w->last_offs.size() * sizeof(int64_t));
 	return CP_OK;
+}
+
+int cp_world_last_scene_contacts(cp_world* w, int cap, int64_t* first, int* n) {
+	if (!w || !n) return CP_ERR_INVALID;
+	return guard(w, [&] {
+		*n = (int)w->last_scene_coff.size();
+		if (*n > cap || !first) throw ApiErr{CP_ERR_CAPACITY};
+		std::copy(w->last_scene_coff.begin(), w->last_scene_coff.end(), first);
+	});
 }
 
 int cp_world_last_residual(cp_world* w, double* out) {

@@ -137,6 +137,29 @@ def build_c0(world, n=10, seed=12345, scene=0, builder=None, offset=(0.0, 0.0, 0
     return None
 
 
+def c0_description(n=10, seed=12345):
+    """The build_c0 pile as plain arrays (one scene; for the CPU oracle): the
+    same shapes, mass properties, materials and seeded placement."""
+    ground = box_body(200.0, 1.0, 200.0, 1.0)
+    cube = box_body(1.0, 1.0, 1.0, 1.0)
+    hulls, move, mps, state = [ground], [1], [ground["mass_props"]], [np.array([0.0, -0.5, 0.0, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0])]
+    rng = SplitMix64(seed)
+    for j in range(n):
+        for i in range(n):
+            for k in range(n):
+                cx = 1.1 * i + rng.uniform(-0.05, 0.05)
+                cy = 0.6 + 1.1 * j + rng.uniform(-0.05, 0.05)
+                cz = 1.1 * k + rng.uniform(-0.05, 0.05)
+                yaw = rng.uniform(0.0, 2.0 * 3.14159265358979323846)
+                hulls.append(cube)
+                move.append(0)
+                mps.append(cube["mass_props"])
+                state.append(np.concatenate([[cx, cy, cz], quat_axis_angle(yaw, (0, 1, 0)), np.zeros(6)]))
+    nb = len(hulls)
+    return dict(hulls=hulls, movement=np.array(move), mass_props=np.array(mps), material=np.tile(C0_MATERIAL, (nb, 1)),
+                state=np.array(state))
+
+
 # ----------------------------------------------------------------------------- C1 / C3 mix
 C1_DENSITY = 20000.0 / 60.0 ** 3
 
@@ -269,6 +292,35 @@ def scene_bodies(sizes, seed, s, lib, sph_r, cyl_r=None, mix=MIX):
 
 
 CONTAINER_WALLS = 6
+
+
+def scene_offsets(sizes):
+    """First body index of each scene of a batched world (container walls, then the scene's bodies)."""
+    cnt = np.asarray(sizes) + CONTAINER_WALLS
+    return np.concatenate([[0], np.cumsum(cnt)])
+
+
+def scene_contacts(contacts, off, s):
+    """Contact cache rows of scene s of a batched world, renumbered to a world holding only that scene
+    (body ids id - first, MagicID primitive halves body_id * 16 + primitive shifted with them)."""
+    a, b = int(off[s]), int(off[s + 1])
+    ii, mg, ff = contacts
+    sel = (ii[:, 0] > a) & (ii[:, 0] <= b)
+    ii, mg, ff = ii[sel].copy(), mg[sel].copy(), ff[sel].copy()
+    ii[:, :2] -= a
+    shift = np.uint64(16 * a)
+    lo = (mg[:, 0] & np.uint64(0xFFFFFFFF)) - shift
+    hi = (mg[:, 0] >> np.uint64(32)) - shift
+    mg[:, 0] = lo | (hi << np.uint64(32))
+    return ii, mg, ff
+
+
+def scene_snapshot(state, contacts, off, s, inertia=None):
+    """State rows (and world inertia rows when given) and contact cache of scene s, renumbered to a
+    world holding only that scene."""
+    a, b = int(off[s]), int(off[s + 1])
+    out = (state[a:b], scene_contacts(contacts, off, s))
+    return out + ((inertia[a:b],) if inertia is not None else ())
 
 
 def container_walls(L, t=1.0):

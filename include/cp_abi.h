@@ -178,6 +178,10 @@ int cp_world_body_count(cp_world* w, int* n);
  * recomputed from the orientation (RigidBody::updateInertiaTensor). */
 int cp_world_set_state(cp_world* w, const double* state13, const double* inertia18);
 int cp_world_get_state(cp_world* w, double* state13);
+/* World inertia tensor[9] and its inverse[9] per body (RigidBody::tensor,
+ * invTensor, RigidBody.h), the values cp_world_set_state takes back: state
+ * and inertia together restore a body bit for bit. */
+int cp_world_get_inertia(cp_world* w, double* inertia18);
 int cp_world_get_aabbs(cp_world* w, double* aabb6);
 int cp_world_set_contacts(cp_world* w, int n, const int64_t* ints6, const uint64_t* magic2, const double* f21);
 int cp_world_get_contact_count(cp_world* w, int* n);
@@ -253,6 +257,14 @@ int cp_narrowphase_multi(cp_world* w, int n, const int64_t* pairs2, int cap, int
  * list per scene in colour order.  Rows are (contact index in the post-narrowphase cache,
  * kind 0 normal / 1 friction; or joint index, kind 2), with group offsets. */
 int cp_world_last_islands(cp_world* w, int cap_rows, int64_t* rows2, int cap_offs, int64_t* offs, int* n_rows, int* n_islands);
+
+/* With the coloured schedule recorded (CP_PGS_COLORED, diagnostics bit 1):
+ * first[s] = index of scene s's first contact in the post-narrowphase cache
+ * the schedule's rows index, first[n_scenes] = cache size; *n = n_scenes + 1.
+ * Lets a caller replay one scene of a batched world on its own (the scene's
+ * contacts are one contiguous range of that cache).  No counterpart in the
+ * reference, which steps one scene per engine. */
+int cp_world_last_scene_contacts(cp_world* w, int cap, int64_t* first, int* n);
 
 /* FP64 residual of the last substep's velocity solve (needs diagnostics bit 0):
  * out[0] = ||r|| over all live rows of the natural map r = lambda - proj(lambda +

@@ -56,3 +56,37 @@ def test_admission_waits_for_capacity():
     assert w.admit_scene(sid, mv, mp, mat, st) is not None
     w.step(1)
     w.close()
+
+
+def test_serving_full_request_distribution_equals_solo_runs():
+    """Requests drawn as the serving configuration draws them — scenes of
+    8-2048 bodies (log-uniform), step counts log-uniform in [1, 240] —
+    arriving faster than a 40,000-slot serving world holds them.  A sample
+    spanning the sizes and step counts (the largest scene, the longest run,
+    and quantiles of both) equals its scene stepped alone, bit for bit."""
+    seed = 2024
+    reqs, sizes = serving.poisson_requests(40, rate=2000.0, seed=seed)
+    w = cpa.World(precision=1, **CFG)
+    w.serving_init(40000)
+    src = serving.SceneSource(w, sizes, seed)
+    out = serving.serve(w, src, reqs, keep_states=True)
+    print(f"{out['requests']} requests ({sum(r.bodies for r in reqs)} bodies, steps {min(r.steps for r in reqs)}-"
+          f"{max(r.steps for r in reqs)}), {out['ticks']} ticks, request p50 {out['p50_request_ms']:.1f} ms p99 "
+          f"{out['p99_request_ms']:.1f} ms", flush=True)
+    assert max(r.bodies for r in reqs) > 1000 and max(r.steps for r in reqs) > 100
+    assert out["ticks"] < sum(r.steps for r in reqs)
+    w.close()
+    by_size = sorted(reqs, key=lambda r: r.bodies)
+    by_steps = sorted(reqs, key=lambda r: r.steps)
+    sample = {by_size[-1].rid: by_size[-1], by_steps[-1].rid: by_steps[-1]}
+    for q in (0.25, 0.5, 0.75):
+        for lst in (by_size, by_steps):
+            r = lst[int(q * (len(lst) - 1))]
+            sample[r.rid] = r
+    for r in sample.values():
+        a = cpa.World(precision=1, **CFG)
+        scenes.build_batched(a, sizes, seed, scene_indices=[r.scene])
+        a.step(r.steps)
+        solo = a.state()
+        a.close()
+        assert np.array_equal(out["states"][r.rid], solo), r
