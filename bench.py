@@ -204,41 +204,55 @@ def c2_leg(mode=2, warmup=150, steps=30):
     return res
 
 
-def c0_leg(steps=300, cpu_runs=3, cpu_steps=10, cpu_from=240):
+def c0_leg(steps=300, ref_steps=3, cpu_runs=3, cpu_steps=5, snap_at=240):
     """The plain CPU-runnable configuration: 1000 unit cubes (10 x 10 x 10,
     1.1 m pitch, U[-0.05, 0.05] jitter, random yaw) on a ground box, density 1,
     restitution 0, friction 0.5, dt 1/120, 8 substeps, 10 + 1 PGS sweeps, FP64,
-    300 steps.  GPU: the coloured single-scene path and the reference-order
-    path (the reference's sequential sweep order, bit-compatible), each for
-    all 300 steps from the same start.  CPU: the FP64 oracle (a bit-exact
-    restatement of the reference step, one thread) continuing the
-    reference-order run's state and contact cache at step `cpu_from`: median
-    of `cpu_runs` runs of `cpu_steps` steps, p50/p99 over all their steps."""
+    300 steps.  GPU: the coloured single-scene path timed over all 300 steps
+    from the initial state, with the state and contact cache at step
+    `snap_at` (a settled pile) kept; from that state, `ref_steps` steps of the
+    reference-order path (the reference's sequential sweep order, one lane per
+    island; bit-compatible and slow, so a bounded sample) and the CPU oracle (a
+    bit-exact restatement of the reference step, one thread): median of
+    `cpu_runs` runs of `cpu_steps` steps, p50/p99 over all their steps."""
     import constraintphysics_amd as cpa
     from constraintphysics_amd import scenes
     desc = scenes.c0_description()
     n_dyn = int((desc["movement"] == 0).sum())
     out = {}
+    mk = lambda mode: cpa.World(dt=1.0 / 120.0, gravity=(0.0, -6.0, 0.0), vel_iters=10, pos_iters=1, substeps=8, pgs_mode=mode, precision=0)
+    w = mk(0)
+    scenes.build_description(w, desc)
+    lat, tim = [], []
     snap = None
-    for mode, name in ((0, "gpu_colored"), (1, "gpu_reference_order")):
-        w = cpa.World(dt=1.0 / 120.0, gravity=(0.0, -6.0, 0.0), vel_iters=10, pos_iters=1, substeps=8, pgs_mode=mode, precision=0)
-        scenes.build_description(w, desc)
-        lat = []
-        for k in range(steps):
-            if mode == 1 and k == cpu_from:
-                snap = (w.state(), w.inertia(), w.contacts())
-            t0 = time.perf_counter()
-            w.step(1)
-            lat.append(time.perf_counter() - t0)
-        lat = np.array(lat)
-        st = w.state()
-        w.close()
-        out[name] = dict(value=round(n_dyn * steps / float(lat.sum()), 1), ms_per_step=round(1e3 * float(lat.mean()), 3),
-                         p50_step_ms=round(1e3 * float(np.percentile(lat, 50)), 3),
-                         p99_step_ms=round(1e3 * float(np.percentile(lat, 99)), 3), finite=bool(np.all(np.isfinite(st))),
-                         min_height=round(float(st[1:, 1].min()), 4))
-        print(f"[bench] c0 {name}: {out[name]['ms_per_step']:.2f} ms/step, {out[name]['value']:.0f} body-steps/s", file=sys.stderr,
-              flush=True)
+    for k in range(steps):
+        if k == snap_at:
+            snap = (w.state(), w.inertia(), w.contacts())
+        t0 = time.perf_counter()
+        w.step(1)
+        lat.append(time.perf_counter() - t0)
+        tim.append(w.last_timing())
+    lat, tim = np.array(lat), np.array(tim)
+    st = w.state()
+    w.close()
+    phases = dict(zip(["broadphase_ms", "narrowphase_ms", "persist_schedule_ms", "substeps_ms", "geometry_ms", "pgs_ms"],
+                      [round(float(x), 3) for x in tim[snap_at:, :6].mean(0)]))
+    out["gpu_colored"] = dict(value=round(n_dyn * steps / float(lat.sum()), 1), ms_per_step=round(1e3 * float(lat.mean()), 3),
+                              p50_step_ms=round(1e3 * float(np.percentile(lat, 50)), 3),
+                              p99_step_ms=round(1e3 * float(np.percentile(lat, 99)), 3), finite=bool(np.all(np.isfinite(st))),
+                              min_height=round(float(st[1:, 1].min()), 4), live_contacts_settled=round(float(tim[snap_at:, 7].mean()), 1),
+                              phases_ms_settled=phases)
+    print(f"[bench] c0 gpu coloured: {out['gpu_colored']['ms_per_step']:.2f} ms/step, {out['gpu_colored']['value']:.0f} body-steps/s; "
+          f"settled phases {phases}", file=sys.stderr, flush=True)
+    w = mk(1)
+    scenes.build_description(w, desc)
+    w.set_state(snap[0], snap[1])
+    w.set_contacts(*snap[2])
+    lat = time_steps(w, ref_steps)
+    w.close()
+    out["gpu_reference_order"] = dict(value=round(n_dyn * ref_steps / float(lat.sum()), 1), ms_per_step=round(1e3 * float(lat.mean()), 3),
+                                      steps=ref_steps, sample=f"{ref_steps} steps from the coloured run's state at step {snap_at}")
+    print(f"[bench] c0 gpu reference order: {out['gpu_reference_order']['ms_per_step']:.1f} ms/step", file=sys.stderr, flush=True)
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from fixtures import Oracle  # the checker, used here only for the reported CPU baseline
     rates, lat = [], []
@@ -256,14 +270,52 @@ def c0_leg(steps=300, cpu_runs=3, cpu_steps=10, cpu_from=240):
     out["cpu_oracle"] = dict(value=round(float(np.median(rates)), 1), runs=[round(r, 1) for r in rates], cores=1, kind="port",
                              p50_step_ms=round(1e3 * float(np.percentile(lat, 50)), 2),
                              p99_step_ms=round(1e3 * float(np.percentile(lat, 99)), 2),
-                             sample=f"median of {cpu_runs} runs of {cpu_steps} steps continuing the reference-order GPU run's "
-                                    f"state and contact cache at step {cpu_from} (settled pile), one thread")
+                             sample=f"median of {cpu_runs} runs of {cpu_steps} steps from the coloured run's state and contact cache "
+                                    f"at step {snap_at} (settled pile), one thread")
     print(f"[bench] c0 cpu oracle: {out['cpu_oracle']['value']:.0f} body-steps/s (p50 {out['cpu_oracle']['p50_step_ms']:.0f} ms)",
           file=sys.stderr, flush=True)
     out["workload"] = ("1000 unit cubes (10x10x10, pitch 1.1 m, jitter U[-0.05,0.05], random yaw) on a fixed ground box, density 1, "
                        "restitution 0, friction 0.5; dt 1/120, 8 substeps, PGS 10 vel + 1 pos sweeps, FP64, 300 steps")
     out["unit"] = "body-steps/s"
     return out
+
+
+def c1_leg(steps=300, seed=20241, precision="split"):
+    """Mixed primitives in one scene: 20,000 bodies in a 60 m box container
+    (40% spheres r ~ U[0.3, 0.7], 30% cylinders r ~ U[0.2, 0.5] h ~ U[0.5,
+    1.5], 30% hulls of 8-32 points on ellipsoids with semi-axes U[0.3, 0.8]),
+    placed without overlap, v ~ N(0, 1); dt 1/120, 8 substeps, 8 PGS velocity
+    sweeps + 1 position sweep, graph-coloured PGS; `steps` steps from the
+    initial state, all timed."""
+    import constraintphysics_amd as cpa
+    from constraintphysics_amd import scenes
+    w = cpa.World(dt=1.0 / 120.0, gravity=(0.0, -6.0, 0.0), vel_iters=8, pos_iters=1, substeps=8, pgs_mode=0,
+                  precision=PRECISIONS[precision])
+    info = scenes.build_batched(w, np.array([20000]), seed)
+    n_dyn = int(info["dynamic_bodies"])
+    lat, tim = [], []
+    for k in range(steps):
+        t0 = time.perf_counter()
+        w.step(1)
+        lat.append(time.perf_counter() - t0)
+        tim.append(w.last_timing())
+        if (k + 1) % 50 == 0:
+            print(f"[bench] c1 step {k + 1}: {1e3 * float(np.mean(lat[-50:])):.2f} ms/step, {tim[-1][7]:.0f} contacts", file=sys.stderr,
+                  flush=True)
+    lat, tim = np.array(lat), np.array(tim)
+    st = w.state()
+    w.close()
+    phases = dict(zip(["broadphase_ms", "narrowphase_ms", "persist_schedule_ms", "substeps_ms", "geometry_ms", "pgs_ms"],
+                      [round(float(x), 3) for x in tim[-50:, :6].mean(0)]))
+    res = dict(value=round(n_dyn * steps / float(lat.sum()), 1), unit="body-steps/s", ms_per_step=round(1e3 * float(lat.mean()), 3),
+               p50_step_ms=round(1e3 * float(np.percentile(lat, 50)), 3), p99_step_ms=round(1e3 * float(np.percentile(lat, 99)), 3),
+               steps=steps, dynamic_bodies=n_dyn, live_contacts_last=round(float(tim[-1, 7]), 1), phases_ms_last50=phases,
+               finite=bool(np.all(np.isfinite(st))), dtype="f64" if precision == "fp64" else "f32 solve / f64 state",
+               workload=f"one scene of 20000 bodies (seed {seed}) in a 60 m box, C1 primitive mix, v ~ N(0,1); dt 1/120, 8 substeps, "
+                        f"PGS 8 vel + 1 pos sweeps, graph-coloured PGS; {steps} steps from the initial state")
+    print(f"[bench] c1: {res['ms_per_step']:.2f} ms/step, {res['value']:.0f} body-steps/s; last-50 phases {phases}", file=sys.stderr,
+          flush=True)
+    return res
 
 
 def main():
@@ -287,6 +339,8 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="skip the HIP-graph replay leg")
     ap.add_argument("--c2-only", action="store_true", help="run only the holonomic-chains-on-terrain leg and print its JSON")
     ap.add_argument("--c0-only", action="store_true", help="run only the 1000-cube pile leg and print its JSON")
+    ap.add_argument("--c1-only", action="store_true", help="run only the 20,000-body single-scene leg and print its JSON")
+    ap.add_argument("--c1-steps", type=int, default=300)
     ap.add_argument("--c2-mode", type=int, choices=[1, 2], default=None,
                     help="with --c2-only: only this island solve (1 reference order, 2 coloured islands; default both)")
     ap.add_argument("--serve-requests", type=int, default=6000, help="requests of the serving leg (Poisson arrivals at 2000/s)")
@@ -306,6 +360,9 @@ def main():
         return
     if args.c0_only:
         print(json.dumps(c0_leg()), flush=True)
+        return
+    if args.c1_only:
+        print(json.dumps(c1_leg(args.c1_steps, precision=args.precision)), flush=True)
         return
 
     from constraintphysics_amd import scenes

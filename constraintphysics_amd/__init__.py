@@ -17,6 +17,9 @@ CP_PGS_REFERENCE_ORDER = 1
 CP_PGS_ISLAND_COLORED = 2
 
 
+CP_ERR_NONFINITE = 6
+
+
 class CpError(RuntimeError):
     pass
 
@@ -90,6 +93,7 @@ def lib():
         "cp_world_last_timing": (_I, [_P, _P]),
         "cp_world_last_residual": (_I, [_P, _P]),
         "cp_world_last_scene_contacts": (_I, [_P, _I, _P, _P]),
+        "cp_world_scene_flags": (_I, [_P, _I, _P, _P]),
         "cp_shape_add_mesh": (_I, [_P, _I, _P, _I, _P, _P, _P]),
         "cp_narrowphase_multi": (_I, [_P, _I, _P, _I, _P, _P, _P, _P, _P, _P]),
         "cp_world_serving_init": (_I, [_P, _I]),
@@ -399,7 +403,22 @@ class World:
         _chk(self.L.cp_world_set_node_order(self.w, len(ids), ids.ctypes.data), "cp_world_set_node_order")
 
     def step(self, n=1):
-        _chk(self.L.cp_world_step(self.w, n), "cp_world_step")
+        """Advances n steps.  Returns False when a dynamic body's pose or
+        velocity went NaN or infinite (CP_ERR_NONFINITE: the steps completed;
+        scene_flags() names the scenes), else True."""
+        st = self.L.cp_world_step(self.w, n)
+        if st == CP_ERR_NONFINITE:
+            return False
+        _chk(st, "cp_world_step")
+        return True
+
+    def scene_flags(self):
+        """Per scene 1 when one of its dynamic bodies has gone non-finite since its state was set."""
+        n = ctypes.c_int()
+        self.L.cp_world_scene_flags(self.w, 0, None, ctypes.byref(n))
+        out = np.zeros(max(n.value, 1), np.int32)
+        _chk(self.L.cp_world_scene_flags(self.w, n.value, out.ctypes.data, ctypes.byref(n)), "cp_world_scene_flags")
+        return out[:n.value]
 
     def broadphase(self):
         cap = max(64, self.n * 16)
@@ -523,12 +542,19 @@ class World:
     def retire_scene(self, handle):
         _chk(self.L.cp_world_retire_scene(self.w, int(handle)), "cp_world_retire_scene")
 
-    def scene_state(self, handle):
+    def scene_state(self, handle, with_status=False):
+        """State rows of an admitted scene; with_status: (rows, finite) where
+        finite is False when the scene holds a body whose pose or velocity went
+        NaN or infinite (CP_ERR_NONFINITE)."""
         n = ctypes.c_int()
         _chk(self.L.cp_world_get_scene_state(self.w, int(handle), None, ctypes.byref(n)), "cp_world_get_scene_state")
         out = np.zeros((n.value, 13))
-        _chk(self.L.cp_world_get_scene_state(self.w, int(handle), out.ctypes.data, ctypes.byref(n)), "cp_world_get_scene_state")
-        return out
+        st = self.L.cp_world_get_scene_state(self.w, int(handle), out.ctypes.data, ctypes.byref(n))
+        if st != CP_ERR_NONFINITE:
+            _chk(st, "cp_world_get_scene_state")
+        elif not with_status:
+            raise CpError(f"cp_world_get_scene_state: scene {handle} holds a non-finite body state ({st})")
+        return (out, st != CP_ERR_NONFINITE) if with_status else out
 
     def serving_info(self):
         """(capacity, free bodies, largest free range, admitted scenes)."""

@@ -263,8 +263,15 @@ void launch_scene_of_body(int n, const int* sbo, int nscenes, int* out, hipStrea
 // are retired: not live, removed at the next merge.
 void launch_kill_contacts(const Contacts& C, int n, const int* ranges, int nranges, hipStream_t s);
 void launch_integrate(const Bodies& b, const Params& p, hipStream_t s);
+// Flags dynamic bodies (of the visited prefix) with a non-finite pose or
+// velocity: body_flag[i] = 1 and any[0] = 1.  Fill: p[0..n) = v.
+void launch_check_finite(const Bodies& b, int* body_flag, int* any, hipStream_t s);
+void launch_fill_int(int* p, int n, int v, hipStream_t s);
+// p[list[k]] = 0 for k < n.
+void launch_zero_list(int* p, const int* list, int n, hipStream_t s);
 void launch_bp_keys(const Bodies& b, uint64_t* keys, int* vals, hipStream_t s);
-void launch_bp_gather(const Bodies& b, const int* sorted, double* sb, hipStream_t s);
+// skip_nonfinite: store non-finite boxes empty (they overlap nothing).
+void launch_bp_gather(const Bodies& b, const int* sorted, double* sb, int skip_nonfinite, hipStream_t s);
 #define BP_CH 64  // sweep positions per broadphase thread
 void launch_bp_extent(int n, const int* sorted, const double* sb, const int* scene_of_body, const int* scene_body_off, int* ext,
                       int* nchunk, hipStream_t s);

@@ -213,6 +213,38 @@ void launch_refresh_geometry(const Bodies& b, const Shapes& sh, int only_moved, 
 	if (cnt) hipLaunchKernelGGL(k_refresh_geometry, dim3(nblk(cnt, 128)), dim3(128), 0, s, b, sh, only_moved, 0, cnt, nullptr);
 }
 
+// Non-finite detection (the reference only asserts, PhysicsEngine.cpp:
+// 1121-1122): a dynamic body whose pose or velocity holds a NaN or an
+// infinity sets its flag and the world's `any` flag.
+__global__ void k_check_finite(Bodies b, int* body_flag, int* any) {
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= b.nl || b.movement[i] != 0) return;
+	const int n = b.n;
+	bool ok = true;
+	for (int c = 0; c < 3; c++) ok = ok && isfinite(b.com[c * n + i]) && isfinite(b.vel[c * n + i]) && isfinite(b.angv[c * n + i]);
+	for (int c = 0; c < 4; c++) ok = ok && isfinite(b.q[c * n + i]);
+	if (ok) return;
+	body_flag[i] = 1;
+	any[0] = 1;
+}
+__global__ void k_fill_int(int* p, int n, int v) {
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) p[i] = v;
+}
+__global__ void k_zero_list(int* p, const int* list, int n) {
+	int k = blockIdx.x * blockDim.x + threadIdx.x;
+	if (k < n) p[list[k]] = 0;
+}
+void launch_zero_list(int* p, const int* list, int n, hipStream_t s) {
+	if (n > 0) hipLaunchKernelGGL(k_zero_list, dim3(nblk(n, 256)), dim3(256), 0, s, p, list, n);
+}
+void launch_check_finite(const Bodies& b, int* body_flag, int* any, hipStream_t s) {
+	if (b.nl) hipLaunchKernelGGL(k_check_finite, dim3(nblk(b.nl, 256)), dim3(256), 0, s, b, body_flag, any);
+}
+void launch_fill_int(int* p, int n, int v, hipStream_t s) {
+	if (n > 0) hipLaunchKernelGGL(k_fill_int, dim3(nblk(n, 256)), dim3(256), 0, s, p, n, v);
+}
+
 // Writes body records (ints: index, shape, movement; doubles: mass, reference
 // I[9], I^-1[9], restitution, mu_k, mu_s, com[3], q[4], vel[3], angv[3]) into
 // the SoA arrays, then derives world inertia and geometry for those bodies.

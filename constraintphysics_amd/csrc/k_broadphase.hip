@@ -27,13 +27,26 @@ __global__ void k_bp_keys(Bodies b, uint64_t* keys, int* vals) {
 	vals[i] = i;
 }
 
-// sb: sorted bounds of the b.nl visited bodies, stride b.nl.
-__global__ void k_bp_gather(Bodies b, const int* sorted, double* sb) {
+// sb: sorted bounds of the b.nl visited bodies, stride b.nl.  With
+// skip_nonfinite, a box holding a NaN or an infinity is stored empty (min
+// +inf, max -inf), so it overlaps nothing; the reference's test
+// (AABB::intersects, AABB.h:10-14) lets a NaN box overlap every box.  The
+// sweep order is unaffected: such a box's sort key sits at an end of its
+// scene, where no other position's scan reads it as a stop.
+__global__ void k_bp_gather(Bodies b, const int* sorted, double* sb, int skip_nonfinite) {
 	int p = blockIdx.x * blockDim.x + threadIdx.x;
 	const int m = b.nl;
 	if (p >= m) return;
 	int i = sorted[p];
-	for (int c = 0; c < 6; c++) sb[c * m + p] = b.aabb[c * b.n + i];
+	double v[6];
+	bool fin = true;
+	for (int c = 0; c < 6; c++) {
+		v[c] = b.aabb[c * b.n + i];
+		fin = fin && isfinite(v[c]);
+	}
+	if (skip_nonfinite && !fin)
+		for (int c = 0; c < 6; c++) v[c] = c < 3 ? __builtin_inf() : -__builtin_inf();
+	for (int c = 0; c < 6; c++) sb[c * m + p] = v[c];
 }
 
 // Sweep end of sorted position p: the first q > p of p's scene with
@@ -101,8 +114,8 @@ static inline int nblk(int n, int t) { return (n + t - 1) / t; }
 void launch_bp_keys(const Bodies& b, uint64_t* keys, int* vals, hipStream_t s) {
 	if (b.nl) hipLaunchKernelGGL(k_bp_keys, dim3(nblk(b.nl, 256)), dim3(256), 0, s, b, keys, vals);
 }
-void launch_bp_gather(const Bodies& b, const int* sorted, double* sb, hipStream_t s) {
-	if (b.nl) hipLaunchKernelGGL(k_bp_gather, dim3(nblk(b.nl, 256)), dim3(256), 0, s, b, sorted, sb);
+void launch_bp_gather(const Bodies& b, const int* sorted, double* sb, int skip_nonfinite, hipStream_t s) {
+	if (b.nl) hipLaunchKernelGGL(k_bp_gather, dim3(nblk(b.nl, 256)), dim3(256), 0, s, b, sorted, sb, skip_nonfinite);
 }
 void launch_bp_extent(int n, const int* sorted, const double* sb, const int* scene_of_body, const int* scene_body_off, int* ext,
                       int* nchunk, hipStream_t s) {

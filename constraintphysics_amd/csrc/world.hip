@@ -201,6 +201,11 @@ struct cp_world {
 	int nedges = 0;
 	int live_contacts = 0;
 	DArr<double> resid_acc;
+	// non-finite detection: per-body flags (sticky until the state is set),
+	// the step's any-flag and its pinned host copy
+	DArr<int> nf_body, nf_any;
+	int* nf_host = nullptr;
+	bool nf_step = false;  // the last cp_world_step call saw a non-finite body
 	std::vector<std::vector<int>> cls_lists;
 	std::vector<std::pair<int, int>> cls_sizes;  // (scene, bodies) the classes are built from
 	int cls_built[3] = {0, 0, 0};                 // class bounds the lists were built with
@@ -419,6 +424,8 @@ void cp_world::free_all() {
 	e_desc.free_();
 	e_lmask.free_();
 	d_nc.free_();
+	nf_body.free_();
+	nf_any.free_();
 	DArr<int>* jints[] = {&dj_type, &dj_a, &dj_b, &dj_sys, &dh_nc, &dh_ord_off, &dh_order, &dh_vloc, &dh_tab_off, &dh_table, &dh_degree,
 	                      &dh_buf_off, &dh_dbuf_off, &isl_sys, &isl_sys_off, &isl_jlin, &isl_eoff, &isl_coff, &isl_icoff,
 	                      &isl_serial, &isl_bod, &isl_boff, &isl_cloc, &isl_jloc, &dh_cp_off, &dh_cs_ptr,
@@ -507,6 +514,9 @@ void cp_world::upload() {
 	dv.reserve(12 * (size_t)n);
 	color_mask.reserve(4 * (size_t)n);
 	applied.reserve(n);
+	nf_body.reserve(n);
+	HIPCHK(hipMemset(nf_body.p, 0, (size_t)n * sizeof(int)));
+	nf_any.reserve(1);
 	// mesh bodies (placed when their pose is set)
 	mesh_inst_of_body.assign(n, -1);
 	mesh_body.clear();
@@ -901,6 +911,7 @@ void cp_world::serving_commit() {
 		up(st_d, rd);
 		up(st_list, list);
 		launch_body_scatter(bodies(), shapes(), (int)list.size(), st_i.p, st_d.p, st_list.p, stream);
+		launch_zero_list(nf_body.p, st_list.p, (int)list.size(), stream);  // admitted bodies start unflagged
 	}
 	if (!kill_ranges.empty()) {
 		up(st_kill, kill_ranges);
@@ -1022,7 +1033,7 @@ int cp_world::broadphase() {
 		HIPCHK(hipcub::DeviceSegmentedRadixSort::SortPairs(tmp.p, bytes, bp_keys.p, bp_keys2.p, bp_vals.p, bp_vals2.p, n, ns,
 		                                                   scene_body_off.p, scene_body_off.p + 1, 0, 64, stream));
 	}
-	launch_bp_gather(b, bp_vals2.p, bp_sb.p, stream);
+	launch_bp_gather(b, bp_vals2.p, bp_sb.p, (cfg.defect_fixes >> 1) & 1, stream);
 	// sweep ends and chunks, then count / scan / write per chunk
 	bp_ext.reserve(n); bp_choff.reserve(n + 1);
 	HIPCHK(hipMemsetAsync(bp_counts.p + n, 0, sizeof(int), stream));
@@ -1586,6 +1597,7 @@ void cp_world::step() {
 		launch_joint_rows(b, J, P, P.sub_dt, 1, 0, stream);
 		launch_hsys_factor(J, H, cfg.cfm, stream);
 	}
+	launch_check_finite(b, nf_body.p, nf_any.p, stream);
 	if (cfg.diagnostics & 1) {
 		double h[4];
 		HIPCHK(hipMemcpy(h, resid_acc.p, 3 * sizeof(double), hipMemcpyDeviceToHost));
@@ -1644,7 +1656,7 @@ void cp_config_default(cp_config* c) {
 	c->pgs_mode = CP_PGS_COLORED;
 	c->solver_threads = 256;
 	c->diagnostics = 0;
-	c->defect_fixes = 1;
+	c->defect_fixes = 3;
 	c->pgs_class_bodies[0] = 192;
 	c->pgs_class_bodies[1] = 768;
 	c->pgs_class_bodies[2] = 3072;
@@ -1663,6 +1675,7 @@ const char* cp_status_string(int s) {
 	case CP_ERR_DEVICE: return "HIP runtime error";
 	case CP_ERR_CAPACITY: return "buffer too small";
 	case CP_ERR_STATE: return "invalid world state";
+	case CP_ERR_NONFINITE: return "non-finite body state (NaN or infinity)";
 	}
 	return "unknown";
 }
@@ -1721,6 +1734,7 @@ int cp_world_destroy(cp_world* w) {
 	for (auto& e : w->join_ev)
 		if (e) (void)hipEventDestroy(e);
 	(void)hipStreamDestroy(w->stream);
+	if (w->nf_host) (void)hipHostFree(w->nf_host);
 	delete w;
 	return CP_OK;
 }
@@ -1974,6 +1988,7 @@ int cp_world_set_state(cp_world* w, const double* s, const double* inertia) {
 			launch_update_inertia(w->bodies(), w->stream);
 		}
 		launch_refresh_geometry(w->bodies(), w->shapes(), 0, w->stream);
+		launch_fill_int(w->nf_body.p, w->nb, 0, w->stream);  // a new state: no body has gone non-finite yet
 		HIPCHK(hipStreamSynchronize(w->stream));
 		w->place_meshes(s);
 	});
@@ -2381,6 +2396,11 @@ int cp_world_get_scene_state(cp_world* w, int handle, double* state13, int* n) {
 		get(w->q, 4, 3);
 		get(w->vel, 3, 7);
 		get(w->angv, 3, 10);
+		std::vector<int> f;
+		f.resize((size_t)R.len);
+		HIPCHK(hipMemcpy(f.data(), w->nf_body.p + R.start, R.len * sizeof(int), hipMemcpyDeviceToHost));
+		for (int v : f)
+			if (v) throw ApiErr{CP_ERR_NONFINITE};  // the state is returned; the status names the divergence
 	});
 }
 
@@ -2404,9 +2424,31 @@ int cp_world_serving_info(cp_world* w, int* info4) {
 int cp_world_step(cp_world* w, int n_steps) {
 	if (!w || n_steps < 0 || w->nb == 0) return CP_ERR_INVALID;
 	if (!w->uploaded) return CP_ERR_STATE;
-	return guard(w, [&] {
+	int st = guard(w, [&] {
+		if (!w->nf_host) HIPCHK(hipHostMalloc((void**)&w->nf_host, sizeof(int), hipHostMallocDefault));
+		launch_fill_int(w->nf_any.p, 1, 0, w->stream);
 		for (int i = 0; i < n_steps; i++) w->step();
+		HIPCHK(hipMemcpyAsync(w->nf_host, w->nf_any.p, sizeof(int), hipMemcpyDeviceToHost, w->stream));
 		HIPCHK(hipStreamSynchronize(w->stream));
+		w->nf_step = *w->nf_host != 0;
+	});
+	if (st == CP_OK && w->nf_step) return CP_ERR_NONFINITE;
+	return st;
+}
+
+int cp_world_scene_flags(cp_world* w, int cap, int* flags, int* n) {
+	if (!w || !n || !w->uploaded) return CP_ERR_INVALID;
+	return guard(w, [&] {
+		*n = w->nscenes;
+		if (w->nscenes > cap || !flags) throw ApiErr{CP_ERR_CAPACITY};
+		HIPCHK(hipStreamSynchronize(w->stream));
+		std::vector<int> f, sbo;
+		down(f, w->nf_body.p, w->nb);
+		down(sbo, w->scene_body_off.p, (size_t)w->nscenes + 1);
+		for (int s2 = 0; s2 < w->nscenes; s2++) {
+			flags[s2] = 0;
+			for (int i = sbo[s2]; i < sbo[s2 + 1]; i++) flags[s2] |= f[i];
+		}
 	});
 }
 

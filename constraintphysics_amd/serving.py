@@ -89,6 +89,7 @@ def serve(world, source, requests, keep_states=False, log=None):
     active = {}
     lat, ticks, waits = {}, [], {}
     states = {}
+    diverged = []  # requests whose scene went non-finite (reported, not retried)
     body_steps = 0
     tick_log = []  # (start time, body-steps of the tick)
     t0 = time.perf_counter()
@@ -123,7 +124,9 @@ def serve(world, source, requests, keep_states=False, log=None):
         tick_log.append((t - t0, tick_bs))
         for h in done:
             r = active.pop(h)[0]
-            st = world.scene_state(h)
+            st, finite = world.scene_state(h, with_status=True)
+            if not finite:
+                diverged.append(r.rid)
             if keep_states:
                 states[r.rid] = st
             world.retire_scene(h)
@@ -140,4 +143,4 @@ def serve(world, source, requests, keep_states=False, log=None):
     return dict(requests=len(requests), wall_s=wall, body_steps=body_steps, value=body_steps / wall, steady_value=steady,
                 p50_request_ms=1e3 * float(np.percentile(l, 50)), p99_request_ms=1e3 * float(np.percentile(l, 99)),
                 p50_step_ms=1e3 * float(np.percentile(tk, 50)), p99_step_ms=1e3 * float(np.percentile(tk, 99)),
-                ticks=len(ticks), mean_wait_ms=1e3 * float(np.mean(list(waits.values()))), states=states)
+                ticks=len(ticks), mean_wait_ms=1e3 * float(np.mean(list(waits.values()))), states=states, diverged=diverged)

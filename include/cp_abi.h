@@ -34,7 +34,10 @@ typedef enum {
 	CP_ERR_OOM = 2,       /* device allocation failed */
 	CP_ERR_DEVICE = 3,    /* HIP runtime error */
 	CP_ERR_CAPACITY = 4,  /* caller buffer too small (required size returned) */
-	CP_ERR_STATE = 5      /* call not valid in the current world state */
+	CP_ERR_STATE = 5,     /* call not valid in the current world state */
+	CP_ERR_NONFINITE = 6  /* cp_world_step: a dynamic body's pose or velocity became NaN or infinite
+	                         (the step completed; cp_world_scene_flags names the scenes).
+	                         cp_world_get_scene_state: that scene holds such a body. */
 } cp_status;
 
 /* CP_PGS_COLORED: contacts only, graph-coloured per scene (FP64 or split).
@@ -70,7 +73,11 @@ typedef struct {
 	                            bit 1: record the coloured sweep schedule (cp_world_last_islands) */
 	int defect_fixes;        /* bit 0: drop non-finite manifold points (the reference's clip
 	                            intersection divides by n.d, which rounds to 0 for an edge parallel to
-	                            the clipping edge, CollisionDetect.cpp:444-467); 0 = bug-compatible */
+	                            the clipping edge, CollisionDetect.cpp:444-467); bit 1: a body whose
+	                            bounding box holds a NaN or an infinity collides with nothing (the
+	                            reference's AABB::intersects, AABB.h:10-14, lets a NaN box overlap every
+	                            box, so one diverged body would pair with its whole scene).  Default 3;
+	                            0 = bug-compatible */
 	int precision;           /* cp_precision: solve precision of CP_PGS_COLORED (the island modes are
 	                            always FP64) */
 	int pgs_class_bodies[3]; /* CP_PGS_COLORED size classes: scenes of at most [0], [1], [2] bodies
@@ -265,6 +272,15 @@ int cp_world_last_islands(cp_world* w, int cap_rows, int64_t* rows2, int cap_off
  * contacts are one contiguous range of that cache).  No counterpart in the
  * reference, which steps one scene per engine. */
 int cp_world_last_scene_contacts(cp_world* w, int cap, int64_t* first, int* n);
+
+/* Per-scene non-finite flags (the reference's NaN asserts, PhysicsEngine.cpp:
+ * 1121-1122, compiled out in its Release build): flags[s] = 1 when a dynamic
+ * body of scene s has held a NaN or infinite pose or velocity at the end of a
+ * step since its state was last set (cp_world_set_state clears every flag,
+ * admission clears the admitted bodies').  Scenes are the world's scenes in
+ * order, or a serving world's body ranges in order.  Other scenes are not
+ * affected: scenes never interact.  *n = number of scenes. */
+int cp_world_scene_flags(cp_world* w, int cap, int* flags, int* n);
 
 /* FP64 residual of the last substep's velocity solve (needs diagnostics bit 0):
  * out[0] = ||r|| over all live rows of the natural map r = lambda - proj(lambda +

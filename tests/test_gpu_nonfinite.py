@@ -1,0 +1,89 @@
+"""Per-scene non-finite detection through the C ABI.
+
+The reference only asserts on NaN (PhysicsEngine.cpp:1121-1122, compiled out
+in its Release build) and otherwise propagates it silently.  Here a step that
+leaves a dynamic body with a NaN or infinite pose or velocity returns
+CP_ERR_NONFINITE after completing, cp_world_scene_flags names the scenes, and
+a serving world reports the scene when it is read back.  With the default
+defect fixes a body whose bounding box is non-finite collides with nothing,
+so a diverged body stays alone and the other scenes — and the other bodies of
+its own scene — step exactly as without it.  The NaN is planted from the host
+through cp_world_set_state.
+"""
+import numpy as np
+import pytest
+
+import constraintphysics_amd as cpa
+from constraintphysics_amd import scenes, serving
+
+pytestmark = pytest.mark.gpu
+
+CFG = dict(dt=1.0 / 120.0, gravity=(0.0, -6.0, 0.0), vel_iters=10, pos_iters=1, substeps=8, pgs_mode=0)
+
+
+@pytest.mark.parametrize("precision", [1, 0])
+@pytest.mark.parametrize("poison", ["nan_velocity", "inf_position"])
+def test_only_the_poisoned_scene_is_flagged(precision, poison):
+    sizes = scenes.c3_sizes(12, 31, lo=8, hi=300)
+    off = scenes.scene_offsets(sizes)
+    w = cpa.World(precision=precision, **CFG)
+    scenes.build_batched(w, sizes, 31, hull_lib_size=32)
+    assert w.step(40)
+    st, inertia, cache = w.state(), w.inertia(), w.contacts()
+    w.set_state(st, inertia)
+    w.set_contacts(*cache)
+    assert w.step(3)
+    clean = w.state()
+    assert not w.scene_flags().any()
+    bad_scene = 5
+    body = int(off[bad_scene]) + scenes.CONTAINER_WALLS + 1  # a dynamic body of scene 5
+    p = st.copy()
+    if poison == "nan_velocity":
+        p[body, 7] = np.nan
+    else:
+        p[body, 0] = np.inf
+    w.set_state(p, inertia)
+    w.set_contacts(*cache)
+    assert not w.scene_flags().any()  # setting a state clears the flags
+    assert not w.step(3)  # CP_ERR_NONFINITE: the steps ran, a body diverged
+    flags = w.scene_flags()
+    assert list(np.nonzero(flags)[0]) == [bad_scene], flags
+    got = w.state()
+    other = np.ones(len(got), bool)
+    other[off[bad_scene]:off[bad_scene + 1]] = False
+    assert np.array_equal(got[other], clean[other])  # the other scenes are bit-identical
+    assert not np.all(np.isfinite(got[body]))
+    if poison == "inf_position":
+        # a body whose box is non-finite from the start collides with nothing,
+        # so the rest of its scene stays finite (a NaN velocity in a finite
+        # box reaches its contacts in the first step, as in the reference)
+        rest = np.arange(off[bad_scene], off[bad_scene + 1]) != body
+        assert np.all(np.isfinite(got[off[bad_scene]:off[bad_scene + 1]][rest]))
+    w.close()
+
+
+def test_serving_reports_the_diverged_scene():
+    seed = 77
+    w = cpa.World(precision=1, **CFG)
+    w.serving_init(2000)
+    reqs, sizes = serving.poisson_requests(3, rate=1e6, seed=seed, max_steps=5, lo=40, hi=120)
+    src = serving.SceneSource(w, sizes, seed, hull_lib_size=32)
+    handles = []
+    for i in range(3):
+        sid, mv, mp, mat, st, n = src.payload(i)
+        if i == 1:
+            st = st.copy()
+            st[scenes.CONTAINER_WALLS + 2, 10] = np.nan  # angular velocity of a dynamic body
+        handles.append(w.admit_scene(sid, mv, mp, mat, st))
+    assert not w.step(2)
+    status = [w.scene_state(h, with_status=True)[1] for h in handles]
+    assert status == [True, False, True], status
+    with pytest.raises(cpa.CpError):
+        w.scene_state(handles[1])
+    # retiring the scene and admitting a fresh one into its range clears the flag
+    w.retire_scene(handles[1])
+    sid, mv, mp, mat, st, n = src.payload(1)
+    h = w.admit_scene(sid, mv, mp, mat, st)
+    assert w.step(2)
+    assert w.scene_state(h, with_status=True)[1]
+    w.close()
