@@ -70,7 +70,7 @@ def profiled_traffic(dtype, scenes, seed, settle, live):
 PRECISIONS = {"fp64": 0, "split": 1}
 
 
-def build_world(sizes, seed, pgs_mode=0, vel_iters=10, threads=256, scene_indices=None, precision="split"):
+def build_world(sizes, seed, pgs_mode=0, vel_iters=10, threads=1024, scene_indices=None, precision="split"):
     import constraintphysics_amd as cpa
     from constraintphysics_amd import scenes
     w = cpa.World(dt=1.0 / 120.0, gravity=(0.0, -6.0, 0.0), vel_iters=vel_iters, pos_iters=1, substeps=8, pgs_mode=pgs_mode,

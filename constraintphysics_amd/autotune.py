@@ -17,7 +17,7 @@ applied.
 """
 import numpy as np
 
-THREADS = (64, 128, 256)
+THREADS = {"T0": (64, 128, 256), "T1": (64, 128, 256, 512), "T2": (128, 256, 512, 1024)}
 BOUND0 = (96, 128, 192, 256, 384)
 BOUND1 = (512, 640, 768, 1024, 1536)
 BOUND2 = 3072
@@ -30,7 +30,7 @@ def _config(p):
 
 def _candidates(knob, p):
     if knob.startswith("T"):
-        return THREADS
+        return THREADS[knob]
     if knob == "B0":
         return tuple(v for v in BOUND0 if v < p["B1"])
     return tuple(v for v in BOUND1 if v > p["B0"])

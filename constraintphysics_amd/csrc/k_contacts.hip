@@ -211,18 +211,30 @@ __global__ void k_edge_build(const Contacts C, int n, const int* head, const int
 
 #define MAXC CP_MAXC
 
+// Colour sets of a scene's bodies: two 64-bit words per body (colours 0-63
+// and 64-127), word w of body i at w{0,1}[i].  Word 0 decides almost every
+// edge (word 1 is read only when a body already holds 64 colours), so it
+// lives in LDS for scenes of up to CP_COLOR_LDS_WORDS bodies; word 1 lives in
+// LDS too when both fit, else in global memory.
+#define CP_COLOR_LDS_WORDS 20352  // 159 KB of LDS (with the kernel's own colour counts, within the 160 KB limit)
+struct ColorSets {
+	uint64_t* w0;
+	uint64_t* w1;
+	bool g0, g1;  // the word is in global memory
+};
+
 // Greedy colouring of one scene's edges [e0, e1) in order, 64 at a time:
 // each lane loads one edge (scene-local bodies, live and dynamic flags), then
 // the wave walks the 64 edges in order with the edge's fields read from its
 // lane (readlane, so they are wave-uniform) and gives each the lowest colour
-// free at its dynamic bodies (colour sets `mask`, two words per body, updated
-// by lane 0).  The rank of an edge within its colour (edges of that colour
-// before it) is then found for all 64 at once: lanes of equal colour are
-// matched bit by bit with ballots and counted below, on top of the running
-// count cnt[c].  ecolor = colour | rank << 8, -1 for an edge with no live
-// contact.  Returns the largest colour used.
-template <class MaskPtr>
-__device__ int color_edges(const Bodies& b, int b0, int e0, int e1, const uint64_t* ekey, const int* elive, MaskPtr mask, int* cnt,
+// free at its dynamic bodies (colour sets, updated by lane 0; an update to a
+// word in global memory is fenced before the next edge reads it).  The rank of
+// an edge within its colour (edges of that colour before it) is then found for
+// all 64 at once: lanes of equal colour are matched bit by bit with ballots
+// and counted below, on top of the running count cnt[c].  ecolor = colour |
+// rank << 8, -1 for an edge with no live contact.  Returns the largest colour
+// used.
+__device__ int color_edges(const Bodies& b, int b0, int e0, int e1, const uint64_t* ekey, const int* elive, const ColorSets& M, int* cnt,
                            int* ecolor, int lane) {
 	const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
 	int maxc = -1;
@@ -242,17 +254,24 @@ __device__ int color_edges(const Bodies& b, int b0, int e0, int e1, const uint64
 			if (!(jfl & 1)) continue;
 			const int jlo = __builtin_amdgcn_readlane(lo, j), jhi = __builtin_amdgcn_readlane(hi, j);
 			const bool dl = (jfl & 2) != 0, dh = (jfl & 4) != 0;
-			uint64_t m0 = (dl ? mask[2 * jlo] : 0) | (dh ? mask[2 * jhi] : 0);
-			uint64_t m1 = (dl ? mask[2 * jlo + 1] : 0) | (dh ? mask[2 * jhi + 1] : 0);
+			uint64_t m0 = (dl ? M.w0[jlo] : 0) | (dh ? M.w0[jhi] : 0);
 			int c;
 			if (~m0) c = __builtin_ctzll(~m0);
-			else if (~m1) c = 64 + __builtin_ctzll(~m1);
-			else c = MAXC;
-			if (c < MAXC && lane == 0) {
-				uint64_t bit = 1ull << (c & 63);
-				int h = c >> 6;
-				if (dl) mask[2 * jlo + h] |= bit;
-				if (dh) mask[2 * jhi + h] |= bit;
+			else {
+				uint64_t m1 = (dl ? M.w1[jlo] : 0) | (dh ? M.w1[jhi] : 0);
+				c = (~m1) ? 64 + __builtin_ctzll(~m1) : MAXC;
+			}
+			if (c < MAXC) {
+				uint64_t* w = c < 64 ? M.w0 : M.w1;
+				if (lane == 0) {
+					uint64_t bit = 1ull << (c & 63);
+					if (dl) w[jlo] |= bit;
+					if (dh) w[jhi] |= bit;
+				}
+				if (c < 64 ? M.g0 : M.g1) {  // wave-uniform: the next edge's loads see lane 0's stores
+					__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+					__builtin_amdgcn_wave_barrier();
+				}
 			}
 			if (lane == j) myc = c;
 			maxc = c > maxc ? c : maxc;
@@ -280,26 +299,38 @@ __device__ int color_edges(const Bodies& b, int b0, int e0, int e1, const uint64
 
 // One wave per scene: greedy edge colouring in edge (cache key) order
 // (color_edges), then a counting sort of the scene's live edges by colour.
-// The colour sets live in LDS for scenes of at most `lds_bodies` bodies, else
-// in b.color_mask.  Colour MAXC collects edges that found no free colour; the
-// solver runs that bucket on a single thread.
+// lds_words: 64-bit words of LDS per workgroup (at most CP_COLOR_LDS_WORDS).
+// A scene keeps both colour-set words in LDS when 2 x its bodies fit, word 0
+// when its bodies fit, else both words in b.color_mask.  Colour MAXC collects
+// edges that found no free colour; the solver runs that bucket on a single
+// thread.
 __global__ __launch_bounds__(64) void k_color(Bodies b, const int* scene_body_off, int nscenes, const uint64_t* ekey, const int* elive,
                                               int nedges, int* ecolor, int* order, int* color_off, int* ncolors, int* edge_range,
-                                              int lds_bodies) {
+                                              int lds_words) {
 	extern __shared__ uint64_t smask_lds[];
 	__shared__ int cnt[MAXC + 1];
 	const int s = blockIdx.x, lane = threadIdx.x;
 	const int b0 = scene_body_off[s], b1 = scene_body_off[s + 1], nbs = b1 - b0;
 	const int e0 = lower_bound_u64(ekey, nedges, (uint64_t)b0 << 32), e1 = lower_bound_u64(ekey, nedges, (uint64_t)b1 << 32);
-	const bool in_lds = nbs <= lds_bodies;
-	uint64_t* gmask = reinterpret_cast<uint64_t*>(b.color_mask) + 2 * (size_t)b0;
-	if (in_lds)
-		for (int i = lane; i < 2 * nbs; i += 64) smask_lds[i] = 0;
-	else
+	uint64_t* gmask = reinterpret_cast<uint64_t*>(b.color_mask) + 2 * (size_t)b0;  // [2][nbs] words of this scene
+	ColorSets M;
+	M.g0 = nbs > lds_words;
+	M.g1 = 2 * nbs > lds_words;
+	M.w0 = M.g0 ? gmask : smask_lds;
+	M.w1 = M.g1 ? gmask + nbs : smask_lds + nbs;
+	for (int i = lane; i < nbs; i += 64) {
+		if (!M.g0) M.w0[i] = 0;
+		if (!M.g1) M.w1[i] = 0;
+	}
+	if (M.g1)
 		for (int e = e0 + lane; e < e1; e += 64) {  // only bodies on an edge need a clear colour set
 			int lo = (int)(ekey[e] >> 32) - b0, hi = (int)(ekey[e] & 0xFFFFFFFFu) - b0;
-			gmask[2 * lo] = 0; gmask[2 * lo + 1] = 0;
-			gmask[2 * hi] = 0; gmask[2 * hi + 1] = 0;
+			M.w1[lo] = 0;
+			M.w1[hi] = 0;
+			if (M.g0) {
+				M.w0[lo] = 0;
+				M.w0[hi] = 0;
+			}
 		}
 	for (int c = lane; c <= MAXC; c += 64) cnt[c] = 0;
 	if (lane == 0) {
@@ -307,9 +338,7 @@ __global__ __launch_bounds__(64) void k_color(Bodies b, const int* scene_body_of
 		edge_range[2 * s + 1] = e1;
 	}
 	__syncthreads();
-	// two instances, so that the LDS one addresses LDS directly
-	const int maxc = in_lds ? color_edges(b, b0, e0, e1, ekey, elive, smask_lds, cnt, ecolor, lane)
-	                        : color_edges(b, b0, e0, e1, ekey, elive, gmask, cnt, ecolor, lane);
+	const int maxc = color_edges(b, b0, e0, e1, ekey, elive, M, cnt, ecolor, lane);
 	__syncthreads();
 	int* co = color_off + (size_t)s * (MAXC + 2);
 	if (lane == 0) {
@@ -407,15 +436,17 @@ void launch_kill_contacts(const Contacts& C, int n, const int* ranges, int nrang
 void launch_color(const Bodies& b, const int* scene_body_off, int nscenes, const uint64_t* ekey, const int* elive, int nedges,
                   int* ecolor, int* order, int* color_off, int* ncolors, int* edge_range, int max_bodies, hipStream_t s) {
 	if (!nscenes) return;
-	// colour sets of scenes up to 4096 bodies in LDS (64 KB), larger ones in global memory
-	int lds_bodies = max_bodies <= 4096 ? max_bodies : 0;
+	// colour-set words in LDS: both words of every scene when the largest fits,
+	// else word 0 of scenes up to CP_COLOR_LDS_WORDS bodies (159 KB)
+	int lds_words = std::min(2 * max_bodies, CP_COLOR_LDS_WORDS);
 	static bool attr = false;
 	if (!attr) {
-		(void)hipFuncSetAttribute((const void*)k_color, hipFuncAttributeMaxDynamicSharedMemorySize, 64 * 1024);
+		if (hipFuncSetAttribute((const void*)k_color, hipFuncAttributeMaxDynamicSharedMemorySize, 8 * CP_COLOR_LDS_WORDS) != hipSuccess)
+			(void)hipGetLastError();  // the launch then reports the error itself
 		attr = true;
 	}
-	hipLaunchKernelGGL(k_color, dim3(nscenes), dim3(64), (size_t)16 * lds_bodies, s, b, scene_body_off, nscenes, ekey, elive, nedges, ecolor,
-	                   order, color_off, ncolors, edge_range, lds_bodies);
+	hipLaunchKernelGGL(k_color, dim3(nscenes), dim3(64), (size_t)8 * lds_words, s, b, scene_body_off, nscenes, ekey, elive, nedges, ecolor,
+	                   order, color_off, ncolors, edge_range, lds_words);
 }
 
 }  // namespace cpx

@@ -227,6 +227,7 @@ struct DvGlobal {
 	__device__ void add(int c, T v) const {
 		if (dyn) base[c * n + i] += v;
 	}
+	__device__ void set(int c, T v) const { base[c * n + i] = v; }
 };
 template <class T>
 struct DvLds {
@@ -236,6 +237,21 @@ struct DvLds {
 	__device__ T get(int c) const { return s[(c % 6) * nbs + i]; }
 	__device__ void add(int c, T v) const {
 		if (dyn) s[(c % 6) * nbs + i] += v;
+	}
+	__device__ void set(int c, T v) const { s[(c % 6) * nbs + i] = v; }
+};
+// One body's six deltas of the phase's half (velocity, or pseudo-velocity
+// for position sweeps) held in registers while a thread runs the contacts of
+// one coloured edge: no other thread touches a dynamic body of the edge during
+// its colour, so the edge's contacts update the registers in the same order
+// and rounding as they would the memory copy, which is written back once.
+template <class T>
+struct DvReg {
+	mutable T v[6];
+	bool dyn;
+	__device__ T get(int c) const { return v[c % 6]; }
+	__device__ void add(int c, T x) const {
+		if (dyn) v[c % 6] += x;
 	}
 };
 
@@ -782,25 +798,39 @@ __device__ __forceinline__ void colour_phase(Contacts& C, const int4* desc, cons
                                              int kind, Make make) {
 	int stride = serial ? 1 : blockDim.x;
 	int start = serial ? (threadIdx.x == 0 ? beg : end) : beg + threadIdx.x;
+	const int off = kind == 2 ? 6 : 0;
 	for (int x = start; x < end; x += stride) {
 		int4 d = desc[e0 + x];
 		uint32_t lm = lmask[e0 + x];
 		const int cnt = d.y & 0xFFFF;
 		const bool lng = (d.y & ED_LONG) != 0;
-		D A = make(d.z, (d.y & ED_DYN_A) != 0), B = make(d.w, ((uint32_t)d.y & ED_DYN_B) != 0);
+		D MA = make(d.z, (d.y & ED_DYN_A) != 0), MB = make(d.w, ((uint32_t)d.y & ED_DYN_B) != 0);
+		DvReg<T> A, B;
+		A.dyn = MA.dyn;
+		B.dyn = MB.dyn;
+#pragma unroll
+		for (int c = 0; c < 6; c++) {
+			A.v[c] = MA.get(off + c);
+			B.v[c] = MB.get(off + c);
+		}
 		for (int j = 0; j < cnt; j++) {
 			int k = d.x + j;
 			bool live = lng ? C.live[k] != 0 : ((lm >> j) & 1u) != 0;
 			if (!live) continue;
 			contact_phase(rec<T>(C, k), A, B, kind);
 		}
+#pragma unroll
+		for (int c = 0; c < 6; c++) {
+			if (A.dyn) MA.set(off + c, A.v[c]);
+			if (B.dyn) MB.set(off + c, B.v[c]);
+		}
 	}
 }
 
 // Global-memory variant (scenes whose velocity deltas do not fit LDS).  The
 // deltas live in b.dv viewed as [12][n] values of T.
-template <class T>
-__global__ __launch_bounds__(256) void k_pgs_colored(Bodies b, Contacts C, const int* scene_body_off, const int* edge_range, const int4* desc,
+template <class T, int MAXT>
+__global__ __launch_bounds__(MAXT) void k_pgs_colored(Bodies b, Contacts C, const int* scene_body_off, const int* edge_range, const int4* desc,
                                                      const uint32_t* lmask, const int* color_off, const int* ncolors, Params P,
                                                      const int* scene_list) {
 	const int s = scene_list ? scene_list[blockIdx.x] : blockIdx.x;
@@ -829,8 +859,8 @@ __global__ __launch_bounds__(256) void k_pgs_colored(Bodies b, Contacts C, const
 // LDS variant: 6 values of T per body, the velocity half during warm start and
 // velocity sweeps, then the pseudo-velocity half during position sweeps.
 // One workgroup per scene of the launch's size class (scene_list).
-template <class T>
-__global__ __launch_bounds__(256) void k_pgs_colored_lds(Bodies b, Contacts C, const int* scene_list, const int* scene_body_off, const int* edge_range,
+template <class T, int MAXT>
+__global__ __launch_bounds__(MAXT) void k_pgs_colored_lds(Bodies b, Contacts C, const int* scene_list, const int* scene_body_off, const int* edge_range,
                                                          const int4* desc, const uint32_t* lmask, const int* color_off, const int* ncolors,
                                                          Params P) {
 	extern __shared__ double lds_raw[];
@@ -972,34 +1002,60 @@ void launch_pgs_islands(const Bodies& b, const Contacts& C, const Joints& J, con
 	}
 	else hipLaunchKernelGGL(k_pgs_ordered, dim3(nblk(S.n, 64)), dim3(64), 0, s, b, C, J, H, S, P, applied, lin, first);
 }
+// Workgroups of more than 256 threads (up to 1024: a scene's colour then runs
+// on 16 waves) use the kernels instantiated with a wider launch bound (512:
+// the FP64 kernels keep their registers; 1024: 4 waves per SIMD).
+template <class T>
+static void pgs_colored(const Bodies& b, const Contacts& C, const int* scene_body_off, int nb, const int* edge_range, const int4* desc,
+                        const uint32_t* lmask, const int* color_off, const int* ncolors, const Params& P, int threads, hipStream_t s,
+                        const int* scene_list) {
+	if (threads > 512)
+		hipLaunchKernelGGL((k_pgs_colored<T, 1024>), dim3(nb), dim3(threads), 0, s, b, C, scene_body_off, edge_range, desc, lmask, color_off,
+		                   ncolors, P, scene_list);
+	else if (threads > 256)
+		hipLaunchKernelGGL((k_pgs_colored<T, 512>), dim3(nb), dim3(threads), 0, s, b, C, scene_body_off, edge_range, desc, lmask, color_off,
+		                   ncolors, P, scene_list);
+	else
+		hipLaunchKernelGGL((k_pgs_colored<T, 256>), dim3(nb), dim3(threads), 0, s, b, C, scene_body_off, edge_range, desc, lmask, color_off,
+		                   ncolors, P, scene_list);
+}
 void launch_pgs_colored(const Bodies& b, const Contacts& C, const int* scene_body_off, int nscenes, const int* edge_range, const int4* desc,
                         const uint32_t* lmask, const int* color_off, const int* ncolors, const Params& P, int threads, hipStream_t s,
                         const int* scene_list, int nlist) {
 	int nb = scene_list ? nlist : nscenes;
 	if (!nb) return;
 	if (P.precision == CP_PREC_SPLIT)
-		hipLaunchKernelGGL(k_pgs_colored<float>, dim3(nb), dim3(threads), 0, s, b, C, scene_body_off, edge_range, desc, lmask, color_off,
-		                   ncolors, P, scene_list);
+		pgs_colored<float>(b, C, scene_body_off, nb, edge_range, desc, lmask, color_off, ncolors, P, threads, s, scene_list);
 	else
-		hipLaunchKernelGGL(k_pgs_colored<double>, dim3(nb), dim3(threads), 0, s, b, C, scene_body_off, edge_range, desc, lmask, color_off,
-		                   ncolors, P, scene_list);
+		pgs_colored<double>(b, C, scene_body_off, nb, edge_range, desc, lmask, color_off, ncolors, P, threads, s, scene_list);
+}
+template <class T, int MAXT>
+static void pgs_colored_lds(const Bodies& b, const Contacts& C, const int* scene_list, int nlist, int max_bodies, const int* scene_body_off,
+                            const int* edge_range, const int4* desc, const uint32_t* lmask, const int* color_off, const int* ncolors,
+                            const Params& P, int threads, hipStream_t s) {
+	static bool attr = false;
+	if (!attr) {
+		(void)hipFuncSetAttribute((const void*)k_pgs_colored_lds<T, MAXT>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+		attr = true;
+	}
+	hipLaunchKernelGGL((k_pgs_colored_lds<T, MAXT>), dim3(nlist), dim3(threads), (size_t)6 * max_bodies * sizeof(T), s, b, C, scene_list,
+	                   scene_body_off, edge_range, desc, lmask, color_off, ncolors, P);
 }
 void launch_pgs_colored_lds(const Bodies& b, const Contacts& C, const int* scene_list, int nlist, int max_bodies, const int* scene_body_off,
                             const int* edge_range, const int4* desc, const uint32_t* lmask, const int* color_off, const int* ncolors,
                             const Params& P, int threads, hipStream_t s) {
 	if (!nlist) return;
-	static bool attr = false;
-	if (!attr) {
-		(void)hipFuncSetAttribute((const void*)k_pgs_colored_lds<double>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-		(void)hipFuncSetAttribute((const void*)k_pgs_colored_lds<float>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-		attr = true;
+#define CP_LDS_ARGS b, C, scene_list, nlist, max_bodies, scene_body_off, edge_range, desc, lmask, color_off, ncolors, P, threads, s
+	if (P.precision == CP_PREC_SPLIT) {
+		if (threads > 512) pgs_colored_lds<float, 1024>(CP_LDS_ARGS);
+		else if (threads > 256) pgs_colored_lds<float, 512>(CP_LDS_ARGS);
+		else pgs_colored_lds<float, 256>(CP_LDS_ARGS);
+	} else {
+		if (threads > 512) pgs_colored_lds<double, 1024>(CP_LDS_ARGS);
+		else if (threads > 256) pgs_colored_lds<double, 512>(CP_LDS_ARGS);
+		else pgs_colored_lds<double, 256>(CP_LDS_ARGS);
 	}
-	if (P.precision == CP_PREC_SPLIT)
-		hipLaunchKernelGGL(k_pgs_colored_lds<float>, dim3(nlist), dim3(threads), (size_t)6 * max_bodies * sizeof(float), s, b, C, scene_list,
-		                   scene_body_off, edge_range, desc, lmask, color_off, ncolors, P);
-	else
-		hipLaunchKernelGGL(k_pgs_colored_lds<double>, dim3(nlist), dim3(threads), (size_t)6 * max_bodies * sizeof(double), s, b, C,
-		                   scene_list, scene_body_off, edge_range, desc, lmask, color_off, ncolors, P);
+#undef CP_LDS_ARGS
 }
 void launch_rows_writeback(const Contacts& C, int nc, const Params& P, hipStream_t s, const int* ncp) {
 	if (!nc) return;

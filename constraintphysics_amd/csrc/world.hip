@@ -1654,7 +1654,7 @@ void cp_config_default(cp_config* c) {
 	c->contact_life = 6;
 	c->gyro_ticks = 4;
 	c->pgs_mode = CP_PGS_COLORED;
-	c->solver_threads = 256;
+	c->solver_threads = 1024;
 	c->diagnostics = 0;
 	c->defect_fixes = 3;
 	c->pgs_class_bodies[0] = 192;
@@ -1680,7 +1680,7 @@ const char* cp_status_string(int s) {
 	return "unknown";
 }
 
-static bool valid_threads(int t) { return t >= 64 && t <= 256 && (t % 64) == 0; }
+static bool valid_threads(int t) { return t >= 64 && t <= 1024 && (t % 64) == 0; }
 static bool valid_cfg(const cp_config* c) {
 	if (!c) return false;
 	const int* lim = c->pgs_class_bodies;

@@ -68,7 +68,7 @@ typedef struct {
 	int gyro_ticks;          /* setAngleVelUpdateTickCount */
 	int pgs_mode;            /* cp_pgs_mode */
 	int solver_threads;      /* threads per scene workgroup in colored mode for scenes whose velocity
-	                            deltas exceed LDS (64..256, a multiple of 64) */
+	                            deltas exceed LDS (64..1024, a multiple of 64) */
 	int diagnostics;         /* bit 0: measure the FP64 PGS residual of every step's last substep;
 	                            bit 1: record the coloured sweep schedule (cp_world_last_islands) */
 	int defect_fixes;        /* bit 0: drop non-finite manifold points (the reference's clip
@@ -83,7 +83,7 @@ typedef struct {
 	int pgs_class_bodies[3]; /* CP_PGS_COLORED size classes: scenes of at most [0], [1], [2] bodies
 	                            (ascending, [2] <= 3072) keep their velocity deltas in LDS; larger
 	                            scenes use global memory.  Defaults 192, 768, 3072. */
-	int pgs_class_threads[3]; /* threads per scene workgroup of each LDS class (64..256, a multiple of
+	int pgs_class_threads[3]; /* threads per scene workgroup of each LDS class (64..1024, a multiple of
 	                            64).  Defaults 64, 128, 256.  Neither setting changes results. */
 	int graph;               /* 1: in CP_PGS_COLORED mode without the residual diagnostics, the
 	                            substep loop (rows, PGS, write-back, integration) is captured once in

@@ -195,3 +195,40 @@ def test_sampled_scene_alone_equals_batched(batch, which):
     alone = w.state()
     w.close()
     assert np.array_equal(alone, b["split"]["state"][off[s]:off[s + 1]]), s
+
+
+def test_single_large_scene_matches_oracle_replay():
+    """The mixed-primitive single scene: 20,000 bodies in a 60 m box, 8 PGS
+    velocity sweeps.  After 200 steps (the lowest bodies resting on the floor
+    and on each other), one FP64 step and one split step from the same state
+    against the oracle's replay of the recorded colour schedule; same bars as
+    the batched scenes."""
+    cfg = dict(CFG, vel_iters=8)
+    sizes = np.array([20000])
+    w = cpa.World(precision=1, **cfg)
+    scenes.build_batched(w, sizes, 20241)
+    for _ in range(200):
+        assert w.step(1)
+    snap = (w.state(), w.inertia(), w.contacts())
+    rec64 = _recorded_step(w, snap, 0)
+    rec32 = _recorded_step(w, snap, 1)
+    w.close()
+    assert np.array_equal(rec64["rows"], rec32["rows"])
+    desc = scenes.scene_description(sizes, 20241, 0)
+    o = Oracle(desc=desc, dt=cfg["dt"], gravity_y=-6.0, vel_iters=8, pos_iters=1, substeps=8, drop_nonfinite=1)
+    o.load(snap[0], snap[1], snap[2])
+    o.step_scheduled(rec64["rows"], np.array([0, len(rec64["rows"])], np.int64))
+    ref = o.state()
+    err64 = _rel_max(rec64["state"], ref)
+    oi, om, of = o.cache()
+    gi, gm, gf = rec64["contacts"]
+    assert np.array_equal(gi[:, [0, 1, 2, 3, 5]], oi[:, [0, 1, 2, 3, 5]]) and np.array_equal(gm, om)
+    cerr = _rel_max(gf, of)
+    sv = _rel2(rec32["state"][:, 7:13], ref[:, 7:13])
+    sp = float(np.max(np.abs(rec32["state"][:, :7] - ref[:, :7])))
+    si = _rel2(rec32["contacts"][2][:, 12:15], of[:, 12:15])
+    print(f"20000-body scene: {len(oi)} contacts, {len(rec64['rows']) // 2} contact rows; fp64 state {err64:.2e}, cache {cerr:.2e}; "
+          f"split velocity {sv:.2e}, pose {sp:.2e}, impulse {si:.2e}", flush=True)
+    assert len(oi) > 1000
+    assert err64 <= 1e-12 and cerr <= 1e-10
+    assert sv <= 1e-4 and sp <= 1e-6 and si <= 1e-2
