@@ -220,7 +220,10 @@ def c0_leg(steps=300, ref_steps=3, cpu_runs=3, cpu_steps=5, snap_at=240):
     desc = scenes.c0_description()
     n_dyn = int((desc["movement"] == 0).sum())
     out = {}
-    mk = lambda mode: cpa.World(dt=1.0 / 120.0, gravity=(0.0, -6.0, 0.0), vel_iters=10, pos_iters=1, substeps=8, pgs_mode=mode, precision=0)
+    # one scene of 1000 bodies: the 769-3072-body size class on 512 threads (8 waves: a colour of the
+    # pile in one pass; the FP64 kernel keeps its registers at that width)
+    mk = lambda mode: cpa.World(dt=1.0 / 120.0, gravity=(0.0, -6.0, 0.0), vel_iters=10, pos_iters=1, substeps=8, pgs_mode=mode, precision=0,
+                                pgs_class_threads=(64, 128, 512))
     w = mk(0)
     scenes.build_description(w, desc)
     lat, tim = [], []

@@ -94,6 +94,15 @@ def lib():
         "cp_world_last_residual": (_I, [_P, _P]),
         "cp_world_last_scene_contacts": (_I, [_P, _I, _P, _P]),
         "cp_world_scene_flags": (_I, [_P, _I, _P, _P]),
+        "cp_world_raycast": (_I, [_P, _I, _P, _P, _P, _P, _P]),
+        "cp_world_save_snapshot": (_I, [_P, ctypes.c_char_p]),
+        "cp_world_load_snapshot": (_I, [_P, ctypes.c_char_p]),
+        "cp_world_trajectory_open": (_I, [_P, ctypes.c_char_p]),
+        "cp_world_trajectory_close": (_I, [_P]),
+        "cp_trajectory_frames": (_I, [ctypes.c_char_p, _I, _P]),
+        "cp_trajectory_read": (_I, [ctypes.c_char_p, _I, _I, _P]),
+        "cp_world_scene_snapshot": (_I, [_P, _I, _I, _P, _P, _P, _P, _P, _P, _P]),
+        "cp_world_admit_scene_resume": (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P]),
         "cp_shape_add_mesh": (_I, [_P, _I, _P, _I, _P, _P, _P]),
         "cp_narrowphase_multi": (_I, [_P, _I, _P, _I, _P, _P, _P, _P, _P, _P]),
         "cp_world_serving_init": (_I, [_P, _I]),
@@ -179,6 +188,20 @@ def polyhedron_mass_properties(pts, faces, density):
     _chk(lib().cp_polyhedron_mass_properties(len(pts), pts.ctypes.data, len(faces), off.ctypes.data, fv.ctypes.data, float(density),
                                              out.ctypes.data), "cp_polyhedron_mass_properties")
     return out[0], out[1:10].reshape(3, 3), out[10:13]
+
+
+def trajectory_frames(path, n_bodies):
+    """Frames in a recorded trajectory file of n_bodies bodies (7 doubles per body per frame)."""
+    n = ctypes.c_int()
+    _chk(lib().cp_trajectory_frames(os.fsencode(path), int(n_bodies), ctypes.byref(n)), "cp_trajectory_frames")
+    return n.value
+
+
+def trajectory_read(path, n_bodies, frame):
+    """Frame `frame` of a recorded trajectory: [n_bodies, 7] = com xyz, orientation r i j k."""
+    out = np.zeros((int(n_bodies), 7))
+    _chk(lib().cp_trajectory_read(os.fsencode(path), int(n_bodies), int(frame), out.ctypes.data), "cp_trajectory_read")
+    return out
 
 
 def polyhedron_body(pts, faces, density):
@@ -555,6 +578,58 @@ class World:
         elif not with_status:
             raise CpError(f"cp_world_get_scene_state: scene {handle} holds a non-finite body state ({st})")
         return (out, st != CP_ERR_NONFINITE) if with_status else out
+
+    def raycast(self, origins, dirs, scenes=None):
+        """First body hit per ray: (body index [n] (-1 none), hit [n, 7] = point, normal, distance)."""
+        o = _a(origins, np.float64).reshape(-1, 3)
+        d = _a(dirs, np.float64).reshape(-1, 3)
+        n = len(o)
+        sc = None if scenes is None else _a(scenes, np.int32)
+        hb = np.zeros(n, np.int32)
+        out = np.zeros((n, 7))
+        _chk(self.L.cp_world_raycast(self.w, n, o.ctypes.data, d.ctypes.data, None if sc is None else sc.ctypes.data, hb.ctypes.data,
+                                     out.ctypes.data), "cp_world_raycast")
+        return hb, out
+
+    def save_snapshot(self, path):
+        _chk(self.L.cp_world_save_snapshot(self.w, os.fsencode(path)), "cp_world_save_snapshot")
+
+    def load_snapshot(self, path):
+        _chk(self.L.cp_world_load_snapshot(self.w, os.fsencode(path)), "cp_world_load_snapshot")
+
+    def trajectory_open(self, path):
+        _chk(self.L.cp_world_trajectory_open(self.w, os.fsencode(path)), "cp_world_trajectory_open")
+
+    def trajectory_close(self):
+        _chk(self.L.cp_world_trajectory_close(self.w), "cp_world_trajectory_close")
+
+    def scene_snapshot(self, handle):
+        """(state13 [n,13], inertia18 [n,18], contacts (ints6, magic2, f21) with scene-relative ids)."""
+        nb, nc = ctypes.c_int(), ctypes.c_int()
+        _chk(self.L.cp_world_scene_snapshot(self.w, int(handle), 0, None, None, None, None, None, ctypes.byref(nb), ctypes.byref(nc)),
+             "cp_world_scene_snapshot")
+        st, ine = np.zeros((nb.value, 13)), np.zeros((nb.value, 18))
+        ii, mg, ff = np.zeros((nc.value, 6), np.int64), np.zeros((nc.value, 2), np.uint64), np.zeros((nc.value, 21))
+        s = self.L.cp_world_scene_snapshot(self.w, int(handle), nc.value, st.ctypes.data, ine.ctypes.data, ii.ctypes.data, mg.ctypes.data,
+                                           ff.ctypes.data, ctypes.byref(nb), ctypes.byref(nc))
+        if s != CP_ERR_NONFINITE:
+            _chk(s, "cp_world_scene_snapshot")
+        return st, ine, (ii, mg, ff)
+
+    def admit_scene_resume(self, shape_ids, movement, mass_props, material, snapshot):
+        """Admits a scene evicted with scene_snapshot, resuming its inertia and contacts; None when no range holds it."""
+        st, ine, (ii, mg, ff) = snapshot
+        sid, mv = _a(shape_ids, np.int32), _a(movement, np.int32)
+        mp, mt, st, ine = _a(mass_props, np.float64), _a(material, np.float64), _a(st, np.float64), _a(ine, np.float64)
+        ii, mg, ff = _a(ii, np.int64), _a(mg, np.uint64), _a(ff, np.float64)
+        h = ctypes.c_int()
+        r = self.L.cp_world_admit_scene_resume(self.w, len(sid), sid.ctypes.data, mv.ctypes.data, mp.ctypes.data, mt.ctypes.data,
+                                               st.ctypes.data, ine.ctypes.data, len(ii), ii.ctypes.data, mg.ctypes.data, ff.ctypes.data,
+                                               ctypes.byref(h))
+        if r == 4:  # CP_ERR_CAPACITY: no free range holds it now
+            return None
+        _chk(r, "cp_world_admit_scene_resume")
+        return h.value
 
     def serving_info(self):
         """(capacity, free bodies, largest free range, admitted scenes)."""

@@ -295,6 +295,11 @@ void launch_mesh_triangles(const Bodies& b, const Shapes& sh, const MeshDev& md,
                            int nitems, hipStream_t s);
 void launch_mesh_merge(const Params& p, const uint64_t* pairs, const MeshDev& md, const int* list, int nlist, const int* off, MeshItems it,
                        Manifolds m, int cap, int* overflow, hipStream_t s);
+// Batched raycasts: per ray the first body hit among its scene's bodies
+// (ray_scene[r] = -1 or a null ray_scene: every body); hit_body[r] (-1: none)
+// and hit7[r] = point xyz, normal xyz, distance (written on a hit).
+void launch_raycast(const Bodies& b, const Shapes& sh, const MeshDev& md, const int* scene_body_off, int n_rays, const double* org,
+                    const double* dir, const int* ray_scene, int* hit_body, double* hit7, hipStream_t s);
 // Class list index of mesh pairs in the narrowphase lists / counts.
 #define NPC_MESH_LIST 6
 void launch_np_gather(const Manifolds& M, int mcap, const uint64_t* pairs, int npairs, const int* off, const NewPts& np, int cap,

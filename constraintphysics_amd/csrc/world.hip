@@ -286,7 +286,27 @@ struct cp_world {
 	bool serving_dirty = false;
 	size_t sh_done[32] = {0};
 	DArr<int> st_i, st_list, st_kill;
+	DArr<double> ray_io;  // raycast inputs (origins, directions) and outputs
+	DArr<int> ray_int;    // raycast scenes and hit bodies
 	DArr<double> st_d;
+	// admissions that resume a scene snapshot: world inertia and contact
+	// cache rows (global ids) applied at the commit that scatters the bodies
+	struct Resume {
+		int start, n;
+		std::vector<double> inertia;  // [n][18]
+		std::vector<int64_t> ci;      // [nc][6]
+		std::vector<uint64_t> cm;     // [nc][2]
+		std::vector<double> cf;       // [nc][21]
+	};
+	std::vector<Resume> resumes;
+	void apply_resumes();
+	// trajectory recording: one frame of 7 doubles (com xyz, q r i j k) per
+	// body after every step, the layout of the reference's recorded runs
+	FILE* traj = nullptr;
+	double* traj_host = nullptr;  // pinned staging (com [3][n], q [4][n])
+	size_t traj_cap = 0;
+	std::vector<double> traj_frame;
+	void record_frame();
 	void sync_shapes();
 	void stage_body(int i, int shape, int move, const double* mp19, const double* mat4, const double* st13);
 	void park_range(int first, int count);
@@ -419,6 +439,8 @@ void cp_world::free_all() {
 	                       &prev_com, &prev_q, &I, &Iinv, &Iref, &Irefinv, &mass, &mat, &aabb, &wpts, &wnorm, &dv, &bp_sb};
 	for (auto* a : dbl) a->free_();
 	resid_acc.free_();
+	ray_io.free_();
+	ray_int.free_();
 	DArr<uint64_t>* u64s[] = {&bp_keys, &bp_keys2, &pairs, &pairs2, &nk, &e_key};
 	for (auto* a : u64s) a->free_();
 	e_desc.free_();
@@ -917,6 +939,7 @@ void cp_world::serving_commit() {
 		up(st_kill, kill_ranges);
 		launch_kill_contacts(cb[cur].c, ncontacts, st_kill.p, (int)kill_ranges.size() / 2, stream);
 	}
+	apply_resumes();
 	stage_i.clear();
 	stage_d.clear();
 	kill_ranges.clear();
@@ -1626,6 +1649,92 @@ void cp_world::step() {
 	}
 }
 
+// ------------------------------------------------- snapshots and trajectories
+void cp_world::record_frame() {
+	const size_t n = (size_t)nb;
+	if (traj_cap < 7 * n) {
+		if (traj_host) HIPCHK(hipHostFree(traj_host));
+		traj_host = nullptr;
+		traj_cap = 0;
+		HIPCHK(hipHostMalloc((void**)&traj_host, 7 * n * sizeof(double), hipHostMallocDefault));
+		traj_cap = 7 * n;
+	}
+	HIPCHK(hipMemcpyAsync(traj_host, com.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost, stream));
+	HIPCHK(hipMemcpyAsync(traj_host + 3 * n, q.p, 4 * n * sizeof(double), hipMemcpyDeviceToHost, stream));
+	HIPCHK(hipStreamSynchronize(stream));
+	traj_frame.resize(7 * n);
+	for (size_t i = 0; i < n; i++) {
+		for (int c = 0; c < 3; c++) traj_frame[7 * i + c] = traj_host[c * n + i];
+		for (int c = 0; c < 4; c++) traj_frame[7 * i + 3 + c] = traj_host[(3 + c) * n + i];
+	}
+	if (fwrite(traj_frame.data(), sizeof(double), 7 * n, traj) != 7 * n) throw ApiErr{CP_ERR_IO};
+}
+
+// Resumed admissions: the snapshot's world inertia over the freshly
+// scattered bodies, then the snapshot's contacts merged into the cache (the
+// rows the reference would still hold, ordered by body pair; the new scene's
+// pairs are new to the cache, and rows keep their order within a pair).
+void cp_world::apply_resumes() {
+	if (resumes.empty()) return;
+	HIPCHK(hipStreamSynchronize(stream));
+	const size_t N = (size_t)nb;
+	for (const Resume& r : resumes) {
+		std::vector<double> comp((size_t)r.n);
+		for (int c = 0; c < 18; c++) {
+			for (int i = 0; i < r.n; i++) comp[i] = r.inertia[18 * (size_t)i + c];
+			double* dst = (c < 9 ? I.p + c * N : Iinv.p + (c - 9) * N) + r.start;
+			HIPCHK(hipMemcpy(dst, comp.data(), (size_t)r.n * sizeof(double), hipMemcpyHostToDevice));
+		}
+	}
+	int nc = 0;
+	int st = cp_world_get_contacts(this, 0, nullptr, nullptr, nullptr, &nc);
+	if (st != CP_OK && st != CP_ERR_CAPACITY) throw ApiErr{st};
+	std::vector<int64_t> ci(6 * (size_t)nc);
+	std::vector<uint64_t> cm(2 * (size_t)nc);
+	std::vector<double> cf(21 * (size_t)nc);
+	if (nc && cp_world_get_contacts(this, nc, ci.data(), cm.data(), cf.data(), &nc) != CP_OK) throw ApiErr{CP_ERR_DEVICE};
+	for (const Resume& r : resumes) {
+		ci.insert(ci.end(), r.ci.begin(), r.ci.end());
+		cm.insert(cm.end(), r.cm.begin(), r.cm.end());
+		cf.insert(cf.end(), r.cf.begin(), r.cf.end());
+	}
+	const int tot = (int)(ci.size() / 6);
+	std::vector<int> ord(tot);
+	for (int k = 0; k < tot; k++) ord[k] = k;
+	std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) {
+		return ci[6 * (size_t)a] != ci[6 * (size_t)b] ? ci[6 * (size_t)a] < ci[6 * (size_t)b] : ci[6 * (size_t)a + 1] < ci[6 * (size_t)b + 1];
+	});
+	std::vector<int64_t> ci2(ci.size());
+	std::vector<uint64_t> cm2(cm.size());
+	std::vector<double> cf2(cf.size());
+	for (int k = 0; k < tot; k++) {
+		std::copy_n(&ci[6 * (size_t)ord[k]], 6, &ci2[6 * (size_t)k]);
+		std::copy_n(&cm[2 * (size_t)ord[k]], 2, &cm2[2 * (size_t)k]);
+		std::copy_n(&cf[21 * (size_t)ord[k]], 21, &cf2[21 * (size_t)k]);
+	}
+	resumes.clear();
+	st = cp_world_set_contacts(this, tot, ci2.data(), cm2.data(), cf2.data());
+	if (st != CP_OK) throw ApiErr{st};
+}
+
+namespace {
+const char SNAP_MAGIC[8] = {'C', 'P', 'S', 'N', 'A', 'P', '1', '\n'};
+uint64_t fnv1a(uint64_t h, const void* p, size_t n) {
+	const unsigned char* b = static_cast<const unsigned char*>(p);
+	for (size_t i = 0; i < n; i++) {
+		h ^= b[i];
+		h *= 1099511628211ull;
+	}
+	return h;
+}
+struct SnapFile {  // a snapshot as written: counts, then the sections in file order
+	int64_t counts[4] = {0, 0, 0, 0};  // bodies, contacts, joints, 0
+	std::vector<double> state, inertia, cf, joint;
+	std::vector<int64_t> ci;
+	std::vector<uint64_t> cm;
+};
+}  // namespace
+
 // ============================================================================ C ABI
 static int guard(cp_world* w, const std::function<void()>& f) {
 	try {
@@ -1676,6 +1785,7 @@ const char* cp_status_string(int s) {
 	case CP_ERR_CAPACITY: return "buffer too small";
 	case CP_ERR_STATE: return "invalid world state";
 	case CP_ERR_NONFINITE: return "non-finite body state (NaN or infinity)";
+	case CP_ERR_IO: return "file could not be read or written, or is not a valid file of its kind";
 	}
 	return "unknown";
 }
@@ -1735,6 +1845,8 @@ int cp_world_destroy(cp_world* w) {
 		if (e) (void)hipEventDestroy(e);
 	(void)hipStreamDestroy(w->stream);
 	if (w->nf_host) (void)hipHostFree(w->nf_host);
+	if (w->traj) fclose(w->traj);
+	if (w->traj_host) (void)hipHostFree(w->traj_host);
 	delete w;
 	return CP_OK;
 }
@@ -2295,6 +2407,128 @@ int cp_reference_node_order(int n, int64_t* ids) {
 	return CP_OK;
 }
 
+int cp_world_save_snapshot(cp_world* w, const char* path) {
+	if (!w || !path || !w->uploaded) return CP_ERR_INVALID;
+	SnapFile f;
+	int nb = w->nb, nj = (int)w->j_type.size(), nc = 0;
+	f.state.resize(13 * (size_t)nb);
+	f.inertia.resize(18 * (size_t)nb);
+	int st = cp_world_get_state(w, f.state.data());
+	if (st == CP_OK) st = cp_world_get_inertia(w, f.inertia.data());
+	if (st != CP_OK) return st;
+	st = cp_world_get_contacts(w, 0, nullptr, nullptr, nullptr, &nc);
+	if (st != CP_OK && st != CP_ERR_CAPACITY) return st;
+	f.ci.resize(6 * (size_t)nc);
+	f.cm.resize(2 * (size_t)nc);
+	f.cf.resize(21 * (size_t)nc);
+	if (nc && (st = cp_world_get_contacts(w, nc, f.ci.data(), f.cm.data(), f.cf.data(), &nc)) != CP_OK) return st;
+	f.joint.resize(11 * (size_t)nj);
+	if (nj && (st = cp_world_get_joint_state(w, f.joint.data())) != CP_OK) return st;
+	f.counts[0] = nb;
+	f.counts[1] = nc;
+	f.counts[2] = nj;
+	FILE* fp = fopen(path, "wb");
+	if (!fp) return CP_ERR_IO;
+	uint64_t h = 14695981039346656037ull;
+	bool ok = true;
+	auto put = [&](const void* p, size_t bytes) {
+		if (!ok || !bytes) return;
+		ok = fwrite(p, 1, bytes, fp) == bytes;
+		h = fnv1a(h, p, bytes);
+	};
+	put(SNAP_MAGIC, 8);
+	put(f.counts, sizeof f.counts);
+	put(f.state.data(), f.state.size() * 8);
+	put(f.inertia.data(), f.inertia.size() * 8);
+	put(f.ci.data(), f.ci.size() * 8);
+	put(f.cm.data(), f.cm.size() * 8);
+	put(f.cf.data(), f.cf.size() * 8);
+	put(f.joint.data(), f.joint.size() * 8);
+	ok = ok && fwrite(&h, 8, 1, fp) == 1;
+	ok = (fclose(fp) == 0) && ok;
+	return ok ? CP_OK : CP_ERR_IO;
+}
+
+int cp_world_load_snapshot(cp_world* w, const char* path) {
+	if (!w || !path || w->nb == 0) return CP_ERR_INVALID;
+	FILE* fp = fopen(path, "rb");
+	if (!fp) return CP_ERR_IO;
+	SnapFile f;
+	uint64_t h = 14695981039346656037ull;
+	bool ok = true;
+	auto get = [&](void* p, size_t bytes) {
+		if (!ok || !bytes) return;
+		ok = fread(p, 1, bytes, fp) == bytes;
+		if (ok) h = fnv1a(h, p, bytes);
+	};
+	char magic[8] = {};
+	get(magic, 8);
+	ok = ok && memcmp(magic, SNAP_MAGIC, 8) == 0;
+	get(f.counts, sizeof f.counts);
+	ok = ok && f.counts[0] == w->nb && f.counts[2] == (int64_t)w->j_type.size() && f.counts[1] >= 0 && f.counts[1] < (int64_t)1 << 31;
+	if (ok) {
+		size_t nb = (size_t)f.counts[0], nc = (size_t)f.counts[1], nj = (size_t)f.counts[2];
+		f.state.resize(13 * nb); f.inertia.resize(18 * nb);
+		f.ci.resize(6 * nc); f.cm.resize(2 * nc); f.cf.resize(21 * nc);
+		f.joint.resize(11 * nj);
+		get(f.state.data(), f.state.size() * 8);
+		get(f.inertia.data(), f.inertia.size() * 8);
+		get(f.ci.data(), f.ci.size() * 8);
+		get(f.cm.data(), f.cm.size() * 8);
+		get(f.cf.data(), f.cf.size() * 8);
+		get(f.joint.data(), f.joint.size() * 8);
+		uint64_t want = 0;
+		ok = ok && fread(&want, 8, 1, fp) == 1 && want == h;
+	}
+	fclose(fp);
+	if (!ok) return CP_ERR_IO;
+	int st = cp_world_set_state(w, f.state.data(), f.inertia.data());
+	if (st == CP_OK) st = cp_world_set_contacts(w, (int)f.counts[1], f.ci.data(), f.cm.data(), f.cf.data());
+	if (st == CP_OK && f.counts[2] > 0) st = cp_world_set_joint_state(w, f.joint.data());
+	return st;
+}
+
+int cp_world_trajectory_open(cp_world* w, const char* path) {
+	if (!w || !path) return CP_ERR_INVALID;
+	if (w->traj) fclose(w->traj);
+	w->traj = fopen(path, "wb");
+	return w->traj ? CP_OK : CP_ERR_IO;
+}
+
+int cp_world_trajectory_close(cp_world* w) {
+	if (!w) return CP_ERR_INVALID;
+	int st = CP_OK;
+	if (w->traj && fclose(w->traj) != 0) st = CP_ERR_IO;
+	w->traj = nullptr;
+	return st;
+}
+
+int cp_trajectory_frames(const char* path, int n_bodies, int* n_frames) {
+	if (!path || n_bodies < 1 || !n_frames) return CP_ERR_INVALID;
+	FILE* fp = fopen(path, "rb");
+	if (!fp) return CP_ERR_IO;
+	bool ok = fseek(fp, 0, SEEK_END) == 0;
+	long bytes = ok ? ftell(fp) : -1;
+	fclose(fp);
+	const long frame = 56L * n_bodies;
+	if (bytes < 0 || bytes % frame != 0) return CP_ERR_IO;
+	*n_frames = (int)(bytes / frame);
+	return CP_OK;
+}
+
+int cp_trajectory_read(const char* path, int n_bodies, int frame, double* poses7) {
+	if (!path || n_bodies < 1 || frame < 0 || !poses7) return CP_ERR_INVALID;
+	int nf = 0;
+	int st = cp_trajectory_frames(path, n_bodies, &nf);
+	if (st != CP_OK) return st;
+	if (frame >= nf) return CP_ERR_INVALID;
+	FILE* fp = fopen(path, "rb");
+	if (!fp) return CP_ERR_IO;
+	bool ok = fseek(fp, (long)frame * 56L * n_bodies, SEEK_SET) == 0 && fread(poses7, 8, 7 * (size_t)n_bodies, fp) == 7 * (size_t)n_bodies;
+	fclose(fp);
+	return ok ? CP_OK : CP_ERR_IO;
+}
+
 int cp_world_serving_init(cp_world* w, int capacity) {
 	if (!w || capacity < 1) return CP_ERR_INVALID;
 	if (w->nb != 0 || !w->j_type.empty() || w->serving) return CP_ERR_STATE;
@@ -2405,6 +2639,93 @@ int cp_world_get_scene_state(cp_world* w, int handle, double* state13, int* n) {
 }
 
 // info4: capacity, free bodies, largest free range, admitted scenes.
+int cp_world_scene_snapshot(cp_world* w, int handle, int cap_contacts, double* state13, double* inertia18, int64_t* ci6, uint64_t* cm2,
+                            double* cf21, int* n_bodies, int* n_contacts) {
+	if (!w || !w->serving || !n_bodies || !n_contacts) return CP_ERR_INVALID;
+	int r = find_range(w, handle);
+	if (r < 0) return CP_ERR_INVALID;
+	int st = guard(w, [&] { w->serving_commit(); });
+	if (st != CP_OK) return st;
+	const cp_world::Range R = w->ranges[r];
+	*n_bodies = R.len;
+	int nc = 0;
+	st = cp_world_get_contacts(w, 0, nullptr, nullptr, nullptr, &nc);
+	if (st != CP_OK && st != CP_ERR_CAPACITY) return st;
+	std::vector<int64_t> ci(6 * (size_t)nc);
+	std::vector<uint64_t> cm(2 * (size_t)nc);
+	std::vector<double> cf(21 * (size_t)nc);
+	if (nc && (st = cp_world_get_contacts(w, nc, ci.data(), cm.data(), cf.data(), &nc)) != CP_OK) return st;
+	const int64_t lo = R.start, hi = (int64_t)R.start + R.len;  // ids (lo, hi]
+	const uint64_t shift = 16 * (uint64_t)R.start;
+	int m = 0;
+	for (int k = 0; k < nc; k++) {
+		if (ci[6 * (size_t)k] <= lo || ci[6 * (size_t)k] > hi) continue;
+		if (state13 && m < cap_contacts && ci6 && cm2 && cf21) {
+			std::copy_n(&ci[6 * (size_t)k], 6, ci6 + 6 * (size_t)m);
+			ci6[6 * (size_t)m] -= lo;
+			ci6[6 * (size_t)m + 1] -= lo;
+			uint64_t c = cm[2 * (size_t)k];
+			uint64_t clo = (c & 0xFFFFFFFFull) - shift, chi = (c >> 32) - shift;
+			cm2[2 * (size_t)m] = (clo & 0xFFFFFFFFull) | (chi << 32);
+			cm2[2 * (size_t)m + 1] = cm[2 * (size_t)k + 1];
+			std::copy_n(&cf[21 * (size_t)k], 21, cf21 + 21 * (size_t)m);
+		}
+		m++;
+	}
+	*n_contacts = m;
+	if (!state13) return CP_OK;
+	if (m > cap_contacts) return CP_ERR_CAPACITY;
+	int n = 0;
+	st = cp_world_get_scene_state(w, handle, state13, &n);
+	if (st != CP_OK && st != CP_ERR_NONFINITE) return st;
+	int st2 = CP_OK;
+	if (inertia18)
+		st2 = guard(w, [&] {
+			const size_t N = (size_t)w->nb;
+			std::vector<double> comp((size_t)R.len);
+			for (int c = 0; c < 18; c++) {
+				const double* src = (c < 9 ? w->I.p + c * N : w->Iinv.p + (c - 9) * N) + R.start;
+				HIPCHK(hipMemcpy(comp.data(), src, (size_t)R.len * sizeof(double), hipMemcpyDeviceToHost));
+				for (int i = 0; i < R.len; i++) inertia18[18 * (size_t)i + c] = comp[i];
+			}
+		});
+	return st2 != CP_OK ? st2 : st;
+}
+
+int cp_world_admit_scene_resume(cp_world* w, int n, const int* shape_ids, const int* movement, const double* mp, const double* material,
+                                const double* state13, const double* inertia18, int n_contacts, const int64_t* ci6, const uint64_t* cm2,
+                                const double* cf21, int* handle) {
+	if (!w || !w->serving || !inertia18 || n_contacts < 0 || (n_contacts > 0 && (!ci6 || !cm2 || !cf21)) || !handle) return CP_ERR_INVALID;
+	for (int k = 0; k < n_contacts; k++) {
+		const int64_t* r = ci6 + 6 * (size_t)k;
+		if (r[0] < 1 || r[1] <= r[0] || r[1] > n) return CP_ERR_INVALID;
+		if (k > 0) {
+			const int64_t* p = ci6 + 6 * (size_t)(k - 1);
+			if (p[0] > r[0] || (p[0] == r[0] && p[1] > r[1])) return CP_ERR_INVALID;  // grouped by sorted pair
+		}
+	}
+	int st = cp_world_admit_scene(w, n, shape_ids, movement, mp, material, state13, handle);
+	if (st != CP_OK) return st;
+	int r = find_range(w, *handle);
+	cp_world::Resume res;
+	res.start = w->ranges[r].start;
+	res.n = n;
+	res.inertia.assign(inertia18, inertia18 + 18 * (size_t)n);
+	res.ci.assign(ci6, ci6 + 6 * (size_t)n_contacts);
+	res.cm.assign(cm2, cm2 + 2 * (size_t)n_contacts);
+	res.cf.assign(cf21, cf21 + 21 * (size_t)n_contacts);
+	const uint64_t shift = 16 * (uint64_t)res.start;
+	for (int k = 0; k < n_contacts; k++) {
+		res.ci[6 * (size_t)k] += res.start;
+		res.ci[6 * (size_t)k + 1] += res.start;
+		uint64_t c = res.cm[2 * (size_t)k];
+		uint64_t clo = (c & 0xFFFFFFFFull) + shift, chi = (c >> 32) + shift;
+		res.cm[2 * (size_t)k] = (clo & 0xFFFFFFFFull) | (chi << 32);
+	}
+	w->resumes.push_back(std::move(res));
+	return CP_OK;
+}
+
 int cp_world_serving_info(cp_world* w, int* info4) {
 	if (!w || !w->serving || !info4) return CP_ERR_INVALID;
 	int fr = 0, big = 0, act = 0;
@@ -2427,7 +2748,10 @@ int cp_world_step(cp_world* w, int n_steps) {
 	int st = guard(w, [&] {
 		if (!w->nf_host) HIPCHK(hipHostMalloc((void**)&w->nf_host, sizeof(int), hipHostMallocDefault));
 		launch_fill_int(w->nf_any.p, 1, 0, w->stream);
-		for (int i = 0; i < n_steps; i++) w->step();
+		for (int i = 0; i < n_steps; i++) {
+			w->step();
+			if (w->traj) w->record_frame();
+		}
 		HIPCHK(hipMemcpyAsync(w->nf_host, w->nf_any.p, sizeof(int), hipMemcpyDeviceToHost, w->stream));
 		HIPCHK(hipStreamSynchronize(w->stream));
 		w->nf_step = *w->nf_host != 0;
@@ -2572,6 +2896,40 @@ int cp_world_last_scene_contacts(cp_world* w, int cap, int64_t* first, int* n) {
 		*n = (int)w->last_scene_coff.size();
 		if (*n > cap || !first) throw ApiErr{CP_ERR_CAPACITY};
 		std::copy(w->last_scene_coff.begin(), w->last_scene_coff.end(), first);
+	});
+}
+
+int cp_world_raycast(cp_world* w, int n, const double* origins3, const double* dirs3, const int* scene_of_ray, int* hit_body, double* hit7) {
+	if (!w || n < 0 || !w->uploaded || (n > 0 && (!origins3 || !dirs3 || !hit_body || !hit7))) return CP_ERR_INVALID;
+	const int ns = w->serving ? w->live_scenes : w->nscenes;
+	for (int k = 0; k < n; k++) {
+		if (scene_of_ray && (scene_of_ray[k] < -1 || scene_of_ray[k] >= ns)) return CP_ERR_INVALID;
+		for (int c = 0; c < 3; c++)
+			if (!std::isfinite(origins3[3 * k + c]) || !std::isfinite(dirs3[3 * k + c])) return CP_ERR_INVALID;
+	}
+	if (n == 0) return CP_OK;
+	if (w->mesh_body.size() && !w->meshes_placed) return CP_ERR_STATE;
+	return guard(w, [&] {
+		w->serving_commit();
+		const size_t N = (size_t)n;
+		w->ray_io.reserve(13 * N);
+		w->ray_int.reserve(2 * N);
+		double* org = w->ray_io.p;
+		double* dir = org + 3 * N;
+		double* out = dir + 3 * N;
+		int* sc = w->ray_int.p;
+		int* hb = sc + N;
+		hipStream_t st = w->stream;
+		HIPCHK(hipMemcpyAsync(org, origins3, 24 * N, hipMemcpyHostToDevice, st));
+		HIPCHK(hipMemcpyAsync(dir, dirs3, 24 * N, hipMemcpyHostToDevice, st));
+		if (scene_of_ray) HIPCHK(hipMemcpyAsync(sc, scene_of_ray, 4 * N, hipMemcpyHostToDevice, st));
+		launch_raycast(w->bodies(), w->shapes(), w->meshdev(), w->scene_body_off.p, n, org, dir, scene_of_ray ? sc : nullptr, hb, out, st);
+		HIPCHK(hipMemcpyAsync(hit_body, hb, 4 * N, hipMemcpyDeviceToHost, st));
+		HIPCHK(hipMemcpyAsync(hit7, out, 56 * N, hipMemcpyDeviceToHost, st));
+		HIPCHK(hipStreamSynchronize(st));
+		for (int k = 0; k < n; k++)
+			if (hit_body[k] < 0)
+				for (int c = 0; c < 7; c++) hit7[7 * (size_t)k + c] = 0;
 	});
 }
 

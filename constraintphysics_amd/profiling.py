@@ -96,6 +96,27 @@ def summarize(d, first_step=241, steps=20, fetch_factor=2.0, write_factor=1.0, c
     return lines, pgs_ms, traffic
 
 
+def kernel_counters(d, name, first_step=241, steps=20, factor=1.0):
+    """{kernel: (dispatches, mean counter value x factor)} over the window of a
+    counter run (`name`_counter_collection.csv), steps counted by k_gravity."""
+    p = os.path.join(d, f"{name}_counter_collection.csv")
+    if not os.path.exists(p):
+        return {}
+    win = range(first_step, first_step + steps)
+    acc = collections.defaultdict(list)
+    step = 0
+    seen = set()
+    for r in _rows(p):
+        k = r["Kernel_Name"].split("(")[0]
+        did = r["Dispatch_Id"]
+        if "k_gravity" in k and did not in seen:
+            step += 1
+        seen.add(did)
+        if step in win:
+            acc[(k, r["Counter_Name"])].append(float(r["Counter_Value"]) * factor)
+    return {key: (len(v), sum(v) / len(v)) for key, v in acc.items()}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("dir")
@@ -105,8 +126,14 @@ def main():
     ap.add_argument("--write-factor", type=float, default=1.0)
     ap.add_argument("--contacts", type=float, default=None)
     ap.add_argument("--prefix", default="c3", help="rocprofv3 -o name of the kernel trace")
+    ap.add_argument("--counters", nargs="*", default=[],
+                    help="name[:factor] of further counter runs (name_counter_collection.csv) to summarise per kernel")
     a = ap.parse_args()
     lines, _, _ = summarize(a.dir, a.first_step, a.steps, a.fetch_factor, a.write_factor, a.contacts, prefix=a.prefix)
+    for spec in a.counters:
+        name, _, f = spec.partition(":")
+        for (k, c), (n, v) in sorted(kernel_counters(a.dir, name, a.first_step, a.steps, float(f or 1.0)).items()):
+            lines.append("%-8s %-40s %-16s dispatches %5d  mean %14.3f" % (name, k[-40:], c, n, v))
     print("\n".join(lines))
 
 

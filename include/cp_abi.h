@@ -35,9 +35,11 @@ typedef enum {
 	CP_ERR_DEVICE = 3,    /* HIP runtime error */
 	CP_ERR_CAPACITY = 4,  /* caller buffer too small (required size returned) */
 	CP_ERR_STATE = 5,     /* call not valid in the current world state */
-	CP_ERR_NONFINITE = 6  /* cp_world_step: a dynamic body's pose or velocity became NaN or infinite
+	CP_ERR_NONFINITE = 6, /* cp_world_step: a dynamic body's pose or velocity became NaN or infinite
 	                         (the step completed; cp_world_scene_flags names the scenes).
 	                         cp_world_get_scene_state: that scene holds such a body. */
+	CP_ERR_IO = 7         /* a file could not be opened, read or written, or is not a valid file of
+	                         its kind (wrong magic, sizes or checksum) */
 } cp_status;
 
 /* CP_PGS_COLORED: contacts only, graph-coloured per scene (FP64 or split).
@@ -313,6 +315,64 @@ int cp_world_retire_scene(cp_world* w, int handle);
 int cp_world_serving_commit(cp_world* w);
 int cp_world_get_scene_state(cp_world* w, int handle, double* state13, int* n);
 int cp_world_serving_info(cp_world* w, int* info4);
+
+/* Snapshots and trajectories.  The reference records runs as frames of 7
+ * doubles per body (position xyz, orientation r i j k; ImageDemo.h:98-133)
+ * and resumes a step from the contact cache its constraint graph holds
+ * (PhysicsEngine.cpp:1062-1111).
+ *
+ * cp_world_save_snapshot / cp_world_load_snapshot: the whole dynamic state of
+ *   a world in one file, enough to resume it bit for bit in a world built
+ *   with the same bodies (and joints): "CPSNAP1\n", int64 counts[4] =
+ *   (bodies, contacts, joints, 0), then state13[bodies], inertia18[bodies],
+ *   contact rows (ints6, magic2, f21 per contact, the cp_world_get_contacts
+ *   layout), joint state11[joints], and a 64-bit FNV-1a checksum of every
+ *   byte before it; all little-endian.  Loading checks the magic, the
+ *   checksum and the body and joint counts (CP_ERR_IO otherwise) and sets
+ *   state, inertia, contacts and joint state.
+ * cp_world_trajectory_open: from the next cp_world_step on, appends one frame
+ *   per step to `path` (created or truncated): per body, in index order,
+ *   com xyz and orientation r i j k as doubles after the step.
+ *   cp_world_trajectory_close ends the recording (also done by destroy).
+ * cp_trajectory_frames / cp_trajectory_read (host-only): the number of
+ *   frames of a file for n_bodies bodies, and frame `frame` as poses7[n][7]. */
+int cp_world_save_snapshot(cp_world* w, const char* path);
+int cp_world_load_snapshot(cp_world* w, const char* path);
+int cp_world_trajectory_open(cp_world* w, const char* path);
+int cp_world_trajectory_close(cp_world* w);
+int cp_trajectory_frames(const char* path, int n_bodies, int* n_frames);
+int cp_trajectory_read(const char* path, int n_bodies, int frame, double* poses7);
+
+/* Serving: evicting a scene and admitting it again later, resuming exactly
+ * where it stopped.
+ * cp_world_scene_snapshot: scene `handle`'s bodies (state13[n], world
+ *   inertia18[n]) and its contact rows in the cp_world_get_contacts layout,
+ *   with body ids relative to the scene (1..n; MagicID primitive halves
+ *   shifted with them); with state13 == NULL only *n_bodies and *n_contacts.
+ *   CP_ERR_CAPACITY when the scene has more than cap_contacts contacts.
+ * cp_world_admit_scene_resume: cp_world_admit_scene with that snapshot's
+ *   inertia and contact rows applied when the admission is committed; the
+ *   scene then steps as if it had never left, wherever its new range is. */
+int cp_world_scene_snapshot(cp_world* w, int handle, int cap_contacts, double* state13, double* inertia18, int64_t* ints6,
+                            uint64_t* magic2, double* f21, int* n_bodies, int* n_contacts);
+int cp_world_admit_scene_resume(cp_world* w, int n, const int* shape_ids, const int* movement, const double* mp,
+                                const double* material, const double* state13, const double* inertia18, int n_contacts,
+                                const int64_t* ints6, const uint64_t* magic2, const double* f21, int* handle);
+
+/* Batched raycasts (PhysicsEngine::raycastFirstIntersection, PhysicsEngine.cpp:
+ * 582-609, for n rays at once) against the world geometry of the last step:
+ * ray k starts at origins3[k] with unit direction dirs3[k] and sees the
+ * bodies of scene scene_of_ray[k] (-1, or scene_of_ray NULL: every body;
+ * scenes never see each other).  hit_body[k] = index of the first body hit
+ * (least distance t >= 0; -1 none), hit7[k] = hit point xyz, surface normal
+ * xyz, t (zeros when nothing is hit).  Bodies are tested as
+ * RigidBody::checkRayIntersection tests them (RigidBody.cpp:185-204), boxes
+ * first (AABB::rayIntersectsAABB, AABB.h:143-173).  A tie in t between two
+ * bodies goes to the lower index; the reference takes the first candidate
+ * of its body tree's traversal.  The ignore list of the reference query is
+ * not taken. */
+int cp_world_raycast(cp_world* w, int n, const double* origins3, const double* dirs3, const int* scene_of_ray, int* hit_body,
+                     double* hit7);
 
 /* Timing of the last cp_world_step call, per phase, in milliseconds (HIP events):
  * [0] gravity + broadphase, [1] narrowphase, [2] persistence + schedule,

@@ -3046,6 +3046,202 @@ void orc_get_aabbs(const orc_world* w, double* a) {
 	for (size_t i = 0; i < w->bodies.size(); i++) memcpy(a + 6 * i, w->bodies[i].aabb, 6 * sizeof(double));
 }
 
+// ---------------------------------------------------------------- raycasts
+namespace {
+struct RayHit {
+	bool hit = false;
+	V3 p, n;
+	double t = 0;
+};
+
+// AABB::rayIntersectsAABB (AABB.h:143-173).
+bool rayHitsBox(const double* box, V3 o, V3 d) {
+	double t_min = -std::numeric_limits<double>::infinity(), t_max = std::numeric_limits<double>::infinity();
+	const double eps = 0.00000000001;
+	const double oo[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z};
+	for (int i = 0; i < 3; i++) {
+		if (std::abs(dd[i]) < eps) {
+			if (oo[i] < box[i] || oo[i] > box[3 + i]) return false;
+		} else {
+			double t0 = (box[i] - oo[i]) / dd[i];
+			double t1 = (box[3 + i] - oo[i]) / dd[i];
+			if (t0 > t1) std::swap(t0, t1);
+			t_min = (t0 < t_min) ? t_min : t0;  // std::max<double>(t0, t_min)
+			t_max = (t_max < t1) ? t_max : t1;  // std::min<double>(t1, t_max)
+			if (t_max < 0 || t_max < t_min) return false;
+		}
+	}
+	return true;
+}
+
+// Roots of the ray / quadric quadratic a t^2 + b t + c as the sphere and
+// cylinder-side tests take them (ConvexPrimitive.cpp:137-150, :179-192):
+// the smaller non-negative root.
+bool nearRoot(double a, double b, double c, double* t) {
+	double radical = b * b - 4 * a * c;
+	if (radical < 0) return false;
+	if (radical == 0) {
+		*t = -0.5 * b / a;
+		return !(*t < 0);
+	}
+	double t1 = (-b + std::sqrt(radical)) / (2 * a);
+	double t2 = (-b - std::sqrt(radical)) / (2 * a);
+	if (t1 < 0 && t2 < 0) return false;
+	if (t1 < 0) *t = t2;
+	else if (t2 < 0) *t = t1;
+	else *t = (t2 < t1) ? t2 : t1;  // std::min<double>(t1, t2)
+	return true;
+}
+
+// testRayAgainstSphere (ConvexPrimitive.cpp:170-202).
+RayHit raySphere(V3 o, V3 d, V3 c, double r) {
+	RayHit out;
+	V3 rel = o - c;
+	double a = d.magSqrd(), b = 2 * rel.dot(d), cc = rel.magSqrd() - r * r;
+	double t;
+	if (!nearRoot(a, b, cc, &t)) return out;
+	out.hit = true;
+	out.p = o + t * d;
+	out.n = (out.p - c).normalize();
+	out.t = t;
+	return out;
+}
+
+// testRayAgainstCylinderSide (ConvexPrimitive.cpp:113-167).
+RayHit rayCylinderSide(V3 o, V3 d, V3 c, double radius, double height, V3 ax) {
+	RayHit out;
+	V3 u = d - ax * ax.dot(d);
+	V3 w = o - c - ax * (ax.dot(o) - ax.dot(c));
+	double a = u.x * u.x + u.y * u.y + u.z * u.z;
+	double b = 2 * (u.x * w.x + u.y * w.y + u.z * w.z);
+	double cc = w.x * w.x + w.y * w.y + w.z * w.z - radius * radius;
+	double t;
+	if (!nearRoot(a, b, cc, &t)) return out;
+	V3 hp = o + t * d;
+	double x = (hp - c).dot(ax);
+	if (std::abs(x) > height / 2.0) return out;
+	out.hit = true;
+	out.p = hp;
+	out.n = ((hp - c) - ax * x).normalize();
+	out.t = t;
+	return out;
+}
+
+// Cylinder::testRayIntersection with checkDiskIntersection (ConvexPrimitive.cpp:300-317).
+RayHit rayCylinder(const Body& b, V3 o, V3 d) {
+	RayHit out = rayCylinderSide(o, d, b.cyl_c, b.cyl_r, b.cyl_h, b.cyl_ax);
+	double sgn = b.cyl_ax.dot(d) < 0 ? 1.0 : -1.0;
+	V3 dc = b.cyl_c + (sgn * b.cyl_ax) * (b.cyl_h / 2.0);
+	double t = b.cyl_ax.dot(dc - o) / b.cyl_ax.dot(d);
+	V3 ip = o + t * d;
+	if (!(t < 0 || (ip - dc).magSqrd() > b.cyl_r * b.cyl_r) && (!out.hit || t < out.t)) {
+		out.hit = true;
+		out.p = ip;
+		out.n = b.cyl_ax;
+		out.t = t;
+	}
+	return out;
+}
+
+// Polyhedron::testRayIntersection (ConvexPrimitive.cpp:731-750): the first
+// surface, in surface order, that the ray enters from outside.
+RayHit rayPolyhedron(const Hull& h, V3 o, V3 d) {
+	RayHit out;
+	for (int f = 0; f < (int)h.faces.size(); f++) {
+		const std::vector<int>& fv = h.faces[f];
+		V3 sp = h.pts[fv[0]];
+		V3 n = faceNormal(h, f);
+		const double eps = 0.0001;
+		if (d.dot(n) > -eps || (o - sp).dot(n) < 0) continue;
+		double t = n.dot(sp - o) / n.dot(d);
+		V3 ip = o + t * d;
+		bool inside = true;
+		int m = (int)fv.size();
+		for (int i = 0; i < m && inside; i++) {
+			V3 e1 = h.pts[fv[i]], e2 = h.pts[fv[i + 1 == m ? 0 : i + 1]];
+			V3 od = (e2 - e1).cross(n);
+			if (od.dot(ip - e1) > 0) inside = false;
+		}
+		if (!inside) continue;
+		out.hit = true;
+		out.p = ip;
+		out.n = n;
+		out.t = t;
+		return out;
+	}
+	return out;
+}
+
+// StaticMeshGeometry::testRayIntersection (Geometry.cpp:867-895) over the
+// candidates of AABBTree::raycastHitCandidates (AABB_Tree.h:249-270): depth
+// first, right child popped first, leaves tested on their true boxes.
+RayHit rayMesh(const Mesh& M, V3 o, V3 d) {
+	RayHit best;
+	if (M.root < 0) return best;
+	std::vector<int> st{M.root};
+	while (!st.empty()) {
+		int nd = st.back();
+		st.pop_back();
+		const BVNode& N = M.nodes[nd];
+		if (N.leaf < 0) {
+			if (rayHitsBox(N.box, o, d)) {
+				st.push_back(N.left);
+				st.push_back(N.right);
+			}
+			continue;
+		}
+		if (!rayHitsBox(N.tbox, o, d)) continue;
+		const MeshTri& tri = M.tris[N.leaf];
+		if (std::abs(tri.n.dot(d)) < 0.0000000001) continue;
+		double t = -(o - tri.v[1]).dot(tri.n) / d.dot(tri.n);
+		if (t < 0 || (best.hit && best.t < t)) continue;
+		V3 hp = o + t * d;
+		if ((hp - tri.ep1[0]).dot(tri.eout[0]) > 0 || (hp - tri.ep1[1]).dot(tri.eout[1]) > 0 || (hp - tri.ep1[2]).dot(tri.eout[2]) > 0)
+			continue;
+		best.hit = true;
+		best.p = hp;
+		best.n = tri.n;
+		best.t = t;
+	}
+	return best;
+}
+
+// RigidBody::checkRayIntersection (RigidBody.cpp:185-204) for a body of one
+// primitive (its box test is the engine's) or a static mesh.
+RayHit rayBody(const Body& b, V3 o, V3 d) {
+	switch (b.type) {
+	case 0: return rayPolyhedron(b.hull, o, d);
+	case 1: return raySphere(o, d, b.sph_c, b.radius);
+	case 3: return rayCylinder(b, o, d);
+	default: return b.mesh ? rayMesh(*b.mesh, o, d) : RayHit();
+	}
+}
+}  // namespace
+
+// PhysicsEngine::raycastFirstIntersection (PhysicsEngine.cpp:582-609) over the
+// bodies [b0, b1) in index order, each candidate a body whose box the ray
+// meets; the least distance wins, the first candidate on a tie.
+int orc_raycast(const orc_world* w, const double* o3, const double* d3, int b0, int b1, double* out7) {
+	V3 o(o3[0], o3[1], o3[2]), d(d3[0], d3[1], d3[2]);
+	int best = -1;
+	RayHit bh;
+	for (int i = b0; i < b1; i++) {
+		const Body& b = w->bodies[i];
+		if (!rayHitsBox(b.aabb, o, d)) continue;
+		RayHit h = rayBody(b, o, d);
+		if (h.hit && (best < 0 || h.t < bh.t)) {
+			bh = h;
+			best = i;
+		}
+	}
+	if (best >= 0) {
+		out7[0] = bh.p.x; out7[1] = bh.p.y; out7[2] = bh.p.z;
+		out7[3] = bh.n.x; out7[4] = bh.n.y; out7[5] = bh.n.z;
+		out7[6] = bh.t;
+	}
+	return best;
+}
+
 void orc_get_world_points(const orc_world* w, int body, double* p) {
 	const Hull& h = w->bodies[body].hull;
 	for (size_t i = 0; i < h.pts.size(); i++) {

@@ -118,6 +118,11 @@ void orc_get_cache(const orc_world* w, int64_t* ints, uint64_t* magic, double* f
 int orc_broadphase(const orc_world* w, int64_t* pairs, int cap);
 void orc_get_aabbs(const orc_world* w, double* aabbs);
 void orc_get_world_points(const orc_world* w, int body, double* pts);
+// First body hit by the ray o3 + t d3 (t >= 0) among bodies [b0, b1) at the
+// current world geometry (PhysicsEngine::raycastFirstIntersection,
+// PhysicsEngine.cpp:582-609): body index or -1; out7 = hit point, surface
+// normal, distance t.
+int orc_raycast(const orc_world* w, const double* o3, const double* d3, int b0, int b1, double* out7);
 
 // Narrowphase of one body pair (ids lo < hi): culled manifold as the engine
 // would add it (PhysicsEngine.cpp:167-277).  Returns number of points (0 if no

@@ -879,6 +879,45 @@ static void dump_mesh(Out& o, const phyz::MeshInput& in) {
 }
 
 
+// Rays cast at the state after the last step through the public query
+// (PhysicsEngine::raycastFirstIntersection, PhysicsEngine.cpp:582-609):
+// n_rays rays from points 6 m from a random body's centre (raised 3 m),
+// aimed at that centre with U[-0.3, 0.3]^3 jitter; every 8th ray straight
+// down (0, -1, 0) (the axis-parallel branch of rayIntersectsAABB), every
+// 16th in a random direction.  Dumps origins, unit directions, (hit, body id)
+// and hit point, surface normal, distance.
+static void dump_rays(PhysicsEngine& E, Out& o, int n_rays, uint64_t seed) {
+	SplitMix64 rng(seed ^ 0x5A5A5A5Aull);
+	std::vector<double> org, dir, res;
+	std::vector<int64_t> hit;
+	int N = (int)E.bodies.size();
+	for (int r = 0; r < n_rays; r++) {
+		RigidBody* tb = E.bodies[(size_t)(rng.next() % (uint64_t)N)];
+		mthz::Vec3 c = tb->getCOM();
+		if (tb->geometry_type == RigidBody::STATIC_MESH) c = mthz::Vec3(rng.uniform(-2, 2), 0, rng.uniform(-2, 2));
+		mthz::Vec3 off(rng.uniform(-1, 1), rng.uniform(-1, 1), rng.uniform(-1, 1));
+		mthz::Vec3 jit(rng.uniform(-0.3, 0.3), rng.uniform(-0.3, 0.3), rng.uniform(-0.3, 0.3));
+		mthz::Vec3 rnd(rng.uniform(-1, 1), rng.uniform(-1, 1), rng.uniform(-1, 1));
+		mthz::Vec3 origin = c + off.normalize() * 6.0 + mthz::Vec3(0, 3, 0);
+		mthz::Vec3 d = (c + jit - origin).normalize();
+		if (r % 8 == 7) d = mthz::Vec3(0, -1, 0);
+		if (r % 16 == 15) d = rnd.normalize();
+		phyz::RayHitInfo h = E.raycastFirstIntersection(origin, d);
+		org.insert(org.end(), {origin.x, origin.y, origin.z});
+		dir.insert(dir.end(), {d.x, d.y, d.z});
+		hit.insert(hit.end(), {(int64_t)h.did_hit, h.did_hit ? (int64_t)h.hit_object->getID() : 0});
+		if (h.did_hit)
+			res.insert(res.end(), {h.hit_position.x, h.hit_position.y, h.hit_position.z, h.surface_normal.x, h.surface_normal.y,
+			                       h.surface_normal.z, h.hit_distance});
+		else
+			res.insert(res.end(), 7, 0.0);
+	}
+	o.f64("ray_origin", org, {(size_t)n_rays, 3});
+	o.f64("ray_dir", dir, {(size_t)n_rays, 3});
+	o.i64("ray_hit", hit, {(size_t)n_rays, 2});
+	o.f64("ray_out", res, {(size_t)n_rays, 7});
+}
+
 // terrain: mixed bodies and jointed chains on a heightfield mesh, trace dump.
 static int cmd_terrain(int argc, char** argv) {
 	int grid = (int)arg_d(argc, argv, "--grid", 12);
@@ -899,6 +938,7 @@ static int cmd_terrain(int argc, char** argv) {
 	dump_mesh(o, g_mesh_input);
 	std::set<int> rec = arg_steps(argc, argv, "--record");
 	run_trace(E, cfg, steps, rec, o);
+	dump_rays(E, o, (int)arg_d(argc, argv, "--rays", 0), seed);
 	o.finish();
 	return 0;
 }
@@ -1008,6 +1048,7 @@ static int cmd_hulls(int argc, char** argv) {
 	o.i64("input_hull_face_verts", ifv, {ifv.size()});
 	o.i64("input_hull_of_body", hull_of, {hull_of.size()});
 	run_trace(E, cfg, steps, arg_steps(argc, argv, "--record"), o);
+	dump_rays(E, o, (int)arg_d(argc, argv, "--rays", 0), seed);
 	o.finish();
 	return 0;
 }

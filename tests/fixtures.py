@@ -137,6 +137,8 @@ def oracle_lib():
         lib.orc_broadphase.restype = ctypes.c_int
         lib.orc_broadphase.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
         lib.orc_get_world_points.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+        lib.orc_raycast.restype = ctypes.c_int
+        lib.orc_raycast.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
         lib.orc_narrowphase.restype = ctypes.c_int
         lib.orc_narrowphase.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         lib.orc_step.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
@@ -292,6 +294,13 @@ class Oracle:
         if cache is not None:
             ci, cm, cf = (_c(cache[0], np.int64), _c(cache[1], np.uint64), _c(cache[2], np.float64))
             self.lib.orc_set_cache(self.w, len(ci), ci.ctypes.data, cm.ctypes.data, cf.ctypes.data)
+
+    def raycast(self, origin, direction, b0=0, b1=None):
+        """First body hit among bodies [b0, b1): (body index or -1, [point 3, normal 3, distance])."""
+        o, d = _c(origin, np.float64), _c(direction, np.float64)
+        out = np.zeros(7)
+        k = self.lib.orc_raycast(self.w, o.ctypes.data, d.ctypes.data, int(b0), int(self.n if b1 is None else b1), out.ctypes.data)
+        return k, out
 
     def state(self):
         s = np.zeros((self.n, 13))

@@ -27,10 +27,11 @@ rm -rf pile3 c0_full mix3 cyl3 cyl4 chain3 chain3h0 terrain hulls
 # Heightfield terrain (12 x 12 cells, 288 triangles, value-noise heights)
 # under a box/sphere/cylinder pile and two jointed chains: convex-triangle SAT,
 # several manifolds per body pair, the mesh AABB tree's candidate order.
-"$H" terrain --grid 12 --cell 0.5 --amp 0.3 --n 2 --chains 2 --links 4 --steps 240 --record 1,60,120,180,239 --out terrain
+"$H" terrain --grid 12 --cell 0.5 --amp 0.3 --n 2 --chains 2 --links 4 --steps 240 --record 1,60,120,180,239 --rays 96 --out terrain
 # Irregular convex hulls (8-32 points on ellipsoids, the mixed-primitive
 # library's generator) built by the reference from points and faces, with
 # spheres and cylinders, falling onto the ground box: shape construction
-# (Gauss maps, mass properties), irregular hull-hull SAT and clipping.
+# (Gauss maps, mass properties), irregular hull-hull SAT and clipping; rays
+# cast at the final state (also on the terrain fixture).
 python3 make_hull_input.py hull_input.txt 24 20240
-"$H" hulls --in hull_input.txt --n 4 --ny 3 --steps 240 --record 0,60,100,149,239 --out hulls
+"$H" hulls --in hull_input.txt --n 4 --ny 3 --steps 240 --record 0,60,100,149,239 --rays 128 --out hulls

@@ -3,7 +3,8 @@
 The tuner only reads World.cfg, calls set_config / step and reads the PGS
 time from last_timing(); a stand-in world with a known cost surface checks
 that coordinate descent finds its minimum, keeps every trial valid (class
-bounds ascending, widths of 64..256) and leaves the best setting applied.
+bounds ascending, widths from each class's candidate list, 64..1024) and
+leaves the best setting applied.
 """
 import types
 
@@ -22,7 +23,7 @@ class FakeWorld:
     def set_config(self, pgs_class_bodies, pgs_class_threads):
         b, t = list(pgs_class_bodies), list(pgs_class_threads)
         assert b[0] < b[1] < b[2] <= 3072
-        assert all(x in (64, 128, 256) for x in t)
+        assert all(x in autotune.THREADS[k] for x, k in zip(t, ("T0", "T1", "T2")))
         self.cfg.pgs_class_bodies, self.cfg.pgs_class_threads = b, t
         self.seen.append((tuple(b), tuple(t)))
 
@@ -39,13 +40,13 @@ class FakeWorld:
 
 
 def test_coordinate_descent_finds_the_minimum():
-    best = dict(T0=128, T1=64, T2=128, B0=256, B1=1024)
+    best = dict(T0=128, T1=64, T2=1024, B0=256, B1=1024)
     w = FakeWorld(best)
     r = autotune.tune(w, steps_per_trial=3, rounds=3)
-    assert r["pgs_class_threads"] == [128, 64, 128]
+    assert r["pgs_class_threads"] == [128, 64, 1024]
     assert r["pgs_class_bodies"] == [256, 1024, 3072]
     assert r["pgs_ms"] == 10.0 and r["default_pgs_ms"] > 10.0
-    assert w.cfg.pgs_class_threads == [128, 64, 128] and w.cfg.pgs_class_bodies == [256, 1024, 3072]
+    assert w.cfg.pgs_class_threads == [128, 64, 1024] and w.cfg.pgs_class_bodies == [256, 1024, 3072]
     assert w.steps == r["steps"]
 
 
