@@ -89,6 +89,7 @@ def lib():
         "cp_world_step": (_I, [_P, _I]),
         "cp_broadphase": (_I, [_P, _I, _P, _P]),
         "cp_narrowphase": (_I, [_P, _I, _P, _P, _P, _P, _P]),
+        "cp_narrowphase_noalloc": (_I, [_P, _I, _P, _P, _P, _P, _P]),
         "cp_world_last_islands": (_I, [_P, _I, _P, _I, _P, _P, _P]),
         "cp_world_last_timing": (_I, [_P, _P]),
         "cp_world_last_residual": (_I, [_P, _P]),

@@ -21,6 +21,7 @@
 #include <cstring>
 #include <functional>
 #include <map>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -1735,6 +1736,54 @@ struct SnapFile {  // a snapshot as written: counts, then the sections in file o
 };
 }  // namespace
 
+// ------------------------------------------------- narrowphase without allocation
+// cp_narrowphase_noalloc runs the narrowphase kernels on a fixed device array
+// instead of the world's grow-only buffers: pair keys, manifold slots (stride
+// NA_MAX_PAIRS), per-class pair lists and counts, and the exported heads.
+// Every call on a device shares that device's copy; it keeps the last call's
+// contents (the class counts are cleared by the kernels, every slot of the
+// call's pairs is written), so calls that overlap on one device race.
+#define NA_MAX_PAIRS 16384
+namespace {
+constexpr size_t NA_PAIRS = 0;                               // uint64 [MAX]
+constexpr size_t NA_COUNT = NA_PAIRS + 8 * NA_MAX_PAIRS;     // int [MAX]
+constexpr size_t NA_NORMAL = NA_COUNT + 4 * NA_MAX_PAIRS;    // double [3][MAX]
+constexpr size_t NA_MAXPEN = NA_NORMAL + 24 * NA_MAX_PAIRS;  // double [MAX]
+constexpr size_t NA_PTS = NA_MAXPEN + 8 * NA_MAX_PAIRS;      // double [MAX][4][4]
+constexpr size_t NA_MAGIC = NA_PTS + 128 * NA_MAX_PAIRS;     // uint64 [MAX][4][2]
+constexpr size_t NA_MAT = NA_MAGIC + 64 * NA_MAX_PAIRS;      // double [MAX * 4][3]
+constexpr size_t NA_HEADS = NA_MAT + 96 * NA_MAX_PAIRS;      // double [MAX][4] (first the input pairs, int64 [MAX][2])
+constexpr size_t NA_LISTS = NA_HEADS + 32 * NA_MAX_PAIRS;    // int [7][MAX]
+constexpr size_t NA_CLASSCNT = NA_LISTS + 28 * NA_MAX_PAIRS; // int [8]
+constexpr size_t NA_BYTES = NA_CLASSCNT + 256;
+constexpr int NA_MAX_DEVICES = 64;
+void* na_ptr[NA_MAX_DEVICES] = {};  // per device: the address of its copy of `scratch`
+std::mutex na_mu;
+}  // namespace
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+// Once per device, with that device current: the device address of its copy
+// of `scratch` (the host-side name is only a stand-in for it).
+static void na_setup(int dev) {
+	if (dev < 0 || dev >= NA_MAX_DEVICES) return;
+	std::lock_guard<std::mutex> lk(na_mu);
+	if (na_ptr[dev]) return;
+	void* p = nullptr;
+	HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+	na_ptr[dev] = p;
+}
+
+__global__ void k_na_keys(const int64_t* ids, int n, uint64_t* keys) {
+	int k = blockIdx.x * blockDim.x + threadIdx.x;
+	if (k < n) keys[k] = ((uint64_t)(ids[2 * k] - 1) << 32) | (uint64_t)(ids[2 * k + 1] - 1);
+}
+__global__ void k_na_heads(const int* count, const double* normal, const double* maxpen, int cap, int n, double* heads) {
+	int k = blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= n) return;
+	for (int c = 0; c < 3; c++) heads[4 * k + c] = count[k] ? normal[(size_t)c * cap + k] : 0.0;
+	heads[4 * k + 3] = maxpen[k];
+}
+
 // ============================================================================ C ABI
 static int guard(cp_world* w, const std::function<void()>& f) {
 	try {
@@ -1817,6 +1866,7 @@ int cp_world_create(const cp_config* cfg, cp_world** out) {
 		w->fk = Fork{{w->side[0], w->side[1]}, w->fork_ev, {w->join_ev[0], w->join_ev[1]}};
 		for (auto& e : w->ev) HIPCHK(hipEventCreate(&e));
 		for (auto& e : w->pgs_ev) HIPCHK(hipEventCreate(&e));
+		na_setup(w->device);
 	});
 	if (st != CP_OK) {
 		delete w;
@@ -2846,6 +2896,35 @@ int cp_narrowphase(cp_world* w, int n, const int64_t* pairs2, int* counts, doubl
 			for (int j = 0; j < 16; j++) pts[16 * (size_t)k + j] = pp[16 * (size_t)s + j];
 			for (int j = 0; j < 8; j++) magic[8 * (size_t)k + j] = mg[8 * (size_t)s + j];
 		}
+	});
+}
+
+int cp_narrowphase_noalloc(cp_world* w, int n, const int64_t* pairs2, int* counts, double* heads, double* pts, uint64_t* magic) {
+	if (!w || n < 0 || !w->uploaded || (n > 0 && (!pairs2 || !counts || !heads || !pts || !magic))) return CP_ERR_INVALID;
+	if (n > NA_MAX_PAIRS) return CP_ERR_CAPACITY;
+	if (!w->mesh_body.empty()) return CP_ERR_STATE;  // mesh pairs expand into several slots each
+	for (int k = 0; k < n; k++)
+		if (pairs2[2 * k] < 1 || pairs2[2 * k + 1] <= pairs2[2 * k] || pairs2[2 * k + 1] > w->nb) return CP_ERR_INVALID;
+	return guard(w, [&] {
+		if (n == 0) return;
+		if (w->device < 0 || w->device >= NA_MAX_DEVICES || !na_ptr[w->device]) throw ApiErr{CP_ERR_STATE};
+		unsigned char* base = static_cast<unsigned char*>(na_ptr[w->device]);
+		hipStream_t st = w->stream;
+		uint64_t* keys = reinterpret_cast<uint64_t*>(base + NA_PAIRS);
+		double* hd = reinterpret_cast<double*>(base + NA_HEADS);
+		HIPCHK(hipMemcpyAsync(hd, pairs2, 16 * (size_t)n, hipMemcpyHostToDevice, st));
+		hipLaunchKernelGGL(k_na_keys, dim3((n + 255) / 256), dim3(256), 0, st, reinterpret_cast<const int64_t*>(hd), n, keys);
+		Manifolds m{reinterpret_cast<int*>(base + NA_COUNT), reinterpret_cast<double*>(base + NA_NORMAL),
+		            reinterpret_cast<double*>(base + NA_MAXPEN), reinterpret_cast<double*>(base + NA_PTS),
+		            reinterpret_cast<uint64_t*>(base + NA_MAGIC), reinterpret_cast<double*>(base + NA_MAT)};
+		launch_narrowphase(w->bodies(), w->shapes(), w->params(), keys, n, m, NA_MAX_PAIRS, reinterpret_cast<int*>(base + NA_LISTS),
+		                   reinterpret_cast<int*>(base + NA_CLASSCNT), w->nocol.p, w->n_nocol, nullptr, st, &w->fk);
+		hipLaunchKernelGGL(k_na_heads, dim3((n + 255) / 256), dim3(256), 0, st, m.count, m.normal, m.maxpen, NA_MAX_PAIRS, n, hd);
+		HIPCHK(hipMemcpyAsync(counts, m.count, 4 * (size_t)n, hipMemcpyDeviceToHost, st));
+		HIPCHK(hipMemcpyAsync(heads, hd, 32 * (size_t)n, hipMemcpyDeviceToHost, st));
+		HIPCHK(hipMemcpyAsync(pts, m.pts, 128 * (size_t)n, hipMemcpyDeviceToHost, st));
+		HIPCHK(hipMemcpyAsync(magic, m.magic, 64 * (size_t)n, hipMemcpyDeviceToHost, st));
+		HIPCHK(hipStreamSynchronize(st));
 	});
 }
 

@@ -253,6 +253,16 @@ int cp_broadphase(cp_world* w, int cap, int64_t* pairs2, int* n);
  * counts[n]; heads[n][4] = normal xyz, max depth; pts[n][4][4] = pos xyz, depth;
  * magic[n][4][2]. */
 int cp_narrowphase(cp_world* w, int n, const int64_t* pairs2, int* counts, double* heads, double* pts, uint64_t* magic);
+/* cp_narrowphase without allocation.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same inputs and
+ * outputs as cp_narrowphase, computed by the same kernels on a fixed device
+ * array of the library (about 6.4 MB per GPU, kept until the process ends)
+ * instead of the world's growing buffers.  At most 16384 pairs per call
+ * (CP_ERR_CAPACITY above); worlds holding a static mesh are refused
+ * (CP_ERR_STATE: mesh pairs expand into many slots).  Every call on one GPU
+ * uses that GPU's array, so calls that overlap on one GPU race. */
+int cp_narrowphase_noalloc(cp_world* w, int n, const int64_t* pairs2, int* counts, double* heads, double* pts, uint64_t* magic);
 /* Every manifold the engine adds for the given pairs, in pair order (a convex
  * body on a static mesh gets one per group of near-parallel triangle contacts,
  * PhysicsEngine.cpp:223-277): pair_index[cap] (index into pairs2), counts[cap],
