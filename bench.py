@@ -44,7 +44,18 @@ def pgs_bytes_per_contact(vel_iters, pos_iters, word=8):
     return word * (32 + vel_iters * 62 + pos_iters * 21)
 
 
-def profiled_traffic(dtype, scenes, seed, settle, live):
+# Algorithmic bytes of the other per-substep work, per live contact or per
+# dynamic body: the row build reads two bodies' pose and inertia (2 x 76 B) and
+# the contact's cache fields (48 B) and writes its 256-byte FP32 record; the
+# write-back reads the record's 4 impulse words and writes 3 doubles and a
+# flag; integration reads and writes com, q, v, w, pseudo v and w and reads
+# the world and reference inertia (592 B, FP64).
+ROW_BUILD_BYTES = 2 * 76 + 48 + 256
+WRITEBACK_BYTES = 16 + 28
+INTEGRATE_BYTES = 592
+
+
+def profiled_traffic(dtype, scenes, seed, settle, live, launch):
     """HBM bytes per PGS solve from the committed rocprofv3 PMC summary of the
     same bench workload (FETCH_SIZE and WRITE_SIZE, copy-kernel calibrated),
     or (None, None) unless the summary states the same scenes, seed and settle
@@ -60,7 +71,8 @@ def profiled_traffic(dtype, scenes, seed, settle, live):
                 k, v = line.split(":", 1)
                 meta[k.strip()] = v.strip()
     try:
-        ok = (int(meta["scenes"]) == scenes and int(meta["seed"]) == seed and int(meta["settle_steps"]) == settle and
+        ok = (meta.get("launch", "solve") == launch and int(meta["scenes"]) == scenes and int(meta["seed"]) == seed and
+              int(meta["settle_steps"]) == settle and
               abs(float(meta["live_contacts"]) - live) <= 0.1 * float(meta["live_contacts"]))
         return (float(meta["traffic_bytes_per_launch"]), os.path.relpath(path, ROOT)) if ok else (None, None)
     except (KeyError, ValueError):
@@ -449,10 +461,21 @@ def main():
     total = float(lat.sum())
     value = n_dyn * args.steps / total
     ms_step = 1e3 * total / args.steps
-    pgs_ms_per_launch = float(tim[:, 5].sum() / max(1.0, tim[:, 6].sum()))
     live = float(tim[:, 7].mean())
-    bpc = pgs_bytes_per_contact(10, 1, 8 if args.precision == "fp64" else 4)
-    achieved = live * bpc / (pgs_ms_per_launch * 1e-3) / 1e9 if pgs_ms_per_launch > 0 else 0.0
+    word = 8 if args.precision == "fp64" else 4
+    bpc = pgs_bytes_per_contact(10, 1, word)
+    fused = float(tim[:, 6].sum()) == 0.0  # the substep loop ran as the fused per-scene kernels
+    if fused:
+        # one launch = the step's substep loop: 8 x (per live contact: row build, solve, write-back;
+        # per dynamic body: integration)
+        launch_ms = float(tim[:, 3].mean())
+        per_contact = ROW_BUILD_BYTES + bpc + WRITEBACK_BYTES
+        launch_bytes = 8 * (live * per_contact + n_dyn * INTEGRATE_BYTES)
+    else:
+        launch_ms = float(tim[:, 5].sum() / max(1.0, tim[:, 6].sum()))
+        per_contact = bpc
+        launch_bytes = live * bpc
+    achieved = launch_bytes / (launch_ms * 1e-3) / 1e9 if launch_ms > 0 else 0.0
     phases = dict(zip(["broadphase_ms", "narrowphase_ms", "persist_schedule_ms", "substeps_ms", "geometry_ms", "pgs_ms"],
                       [round(float(x), 3) for x in tim[:, :6].mean(0)]))
     print(f"[bench] timed {args.steps} steps: {ms_step:.2f} ms/step, {value:.0f} body-steps/s, {live:.0f} live contacts; phases {phases}",
@@ -501,7 +524,7 @@ def main():
                           f"{spent:.1f}s of CPU; host has {os.cpu_count()} cores")
         print(f"[bench] cpu baseline: {v:.0f} body-steps/s", file=sys.stderr, flush=True)
 
-    traffic, traffic_src = profiled_traffic(args.precision, args.scenes, args.seed, settled, live)
+    traffic, traffic_src = profiled_traffic(args.precision, args.scenes, args.seed, settled, live, "fused" if fused else "solve")
     out = {
         "metric": "body-steps/s vs plain GPU path at stated FP64 residual; p50/p99 step latency",
         "value": round(value, 1),
@@ -550,16 +573,20 @@ def main():
         "serving_speedup_vs_plain": round(served["value"] / plain["value"], 2) if (plain and served) else None,
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_pgs_colored_lds (one substep's PGS solve: one launch per scene size class)",
+            "kernel": ("k_substeps_lds / k_substeps_global (the step's substep loop: each scene's rows, coloured PGS, "
+                       "impulse write-back and integration in one workgroup; one launch per scene size class)") if fused else
+                      "k_pgs_colored_lds (one substep's PGS solve: one launch per scene size class)",
             "achieved": round(achieved, 2),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
             "traffic": traffic,
             "traffic_source": traffic_src,
-            "bytes_per_contact_per_launch": bpc,
+            "algorithmic_bytes_per_launch": round(launch_bytes, 1),
+            "bytes_per_contact_per_launch": (8 * per_contact) if fused else bpc,
+            "bytes_per_body_per_launch": (8 * INTEGRATE_BYTES) if fused else 0,
             "live_contacts": live,
-            "avg_launch_ms": round(pgs_ms_per_launch, 4),
+            "avg_launch_ms": round(launch_ms, 4),
         },
         "cpu_baseline": cpu,
     }

@@ -44,6 +44,7 @@ class Config(ctypes.Structure):
         ("pgs_class_bodies", ctypes.c_int * 3),
         ("pgs_class_threads", ctypes.c_int * 3),
         ("graph", ctypes.c_int),
+        ("fused", ctypes.c_int),
     ]
 
 

@@ -340,6 +340,16 @@ void launch_pgs_colored_lds(const Bodies& b, const Contacts& C, const int* scene
                             const int* edge_range, const int4* desc, const uint32_t* lmask, const int* color_off, const int* ncolors,
                             const Params& P, int threads, hipStream_t s);
 void launch_rows_writeback(const Contacts& C, int nc, const Params& P, hipStream_t s, const int* ncp = nullptr);
+// The whole substep loop of each listed scene in one workgroup (rows, coloured
+// solve, impulses back into the cache, integration; P.substeps times): the LDS
+// variant for scenes of at most max_bodies bodies, the global-memory one for
+// larger scenes.  efirst / ecnt: first cache index and contact count per edge.
+void launch_substeps_lds(const Bodies& b, const Contacts& C, const int* scene_list, int nlist, int max_bodies, const int* scene_body_off,
+                         const int* edge_range, const int* efirst, const int* ecnt, const int4* desc, const uint32_t* lmask,
+                         const int* color_off, const int* ncolors, const Params& P, int threads, hipStream_t s);
+void launch_substeps_global(const Bodies& b, const Contacts& C, const int* scene_list, int nlist, const int* scene_body_off, const int* edge_range,
+                            const int* efirst, const int* ecnt, const int4* desc, const uint32_t* lmask, const int* color_off,
+                            const int* ncolors, const Params& P, int threads, hipStream_t s);
 void launch_pgs_residual(const Bodies& b, const Contacts& C, int nc, const Params& P, double* acc, hipStream_t s);
 // rows: (index, kind) pairs, kind 0 contact normal, 1 friction, 2 joint;
 // isl_sys/isl_sys_off: holonomic systems per island; lin: [6][n] scratch.

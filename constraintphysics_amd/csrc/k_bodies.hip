@@ -7,24 +7,9 @@
 // com/q/vel/angv/pvel/pang + world and reference inertia: 2*(3+4+3+3+3+3)*8 +
 // 36*8 B = 592 B per body per substep (FP64).
 #include "cp_internal.h"
+#include "k_integrate.h"
 
 namespace cpx {
-
-#define LD3(a, i, n) V3{(a)[(i)], (a)[(n) + (i)], (a)[2 * (n) + (i)]}
-#define ST3(a, i, n, v) do { (a)[(i)] = (v).x; (a)[(n) + (i)] = (v).y; (a)[2 * (n) + (i)] = (v).z; } while (0)
-
-__device__ __forceinline__ Q ldq(const double* a, int i, int n) { return Q{a[i], a[n + i], a[2 * n + i], a[3 * n + i]}; }
-__device__ __forceinline__ void stq(double* a, int i, int n, Q q) {
-	a[i] = q.r; a[n + i] = q.i; a[2 * n + i] = q.j; a[3 * n + i] = q.k;
-}
-__device__ __forceinline__ M3 ldm(const double* a, int i, int n) {
-	M3 m;
-	for (int r = 0; r < 3; r++) for (int c = 0; c < 3; c++) m.m[r][c] = a[(3 * r + c) * n + i];
-	return m;
-}
-__device__ __forceinline__ void stm(double* a, int i, int n, const M3& m) {
-	for (int r = 0; r < 3; r++) for (int c = 0; c < 3; c++) a[(3 * r + c) * n + i] = m.m[r][c];
-}
 
 // PhysicsEngine.cpp:84-86: v += g * dt once per step for awake dynamic bodies.
 __global__ void k_gravity(Bodies b, Params p) {
@@ -35,61 +20,12 @@ __global__ void k_gravity(Bodies b, Params p) {
 	ST3(b.vel, i, b.n, v);
 }
 
-// RigidBody::rotateWhileApplyingGyroAccel (RigidBody.cpp:207-255), implicit
-// Euler on the gyroscopic term with two Newton iterations per sub-tick.
-__device__ void gyro_rotate(V3& angv, Q& q, const M3& I_world, const M3& I_ref, double dt, int n_itr) {
-	if (mag2(angv) == 0) return;
-	M3 T = I_world;
-	Q o = q;
-	V3 av = angv;
-	for (int it = 0; it < n_itr; it++) {
-		V3 w = av;
-		double ts = dt / n_itr;
-		for (int s = 0; s < 2; s++) {
-			V3 f = ts * cross(w, T * w) - T * (av - w);
-			M3 jac = T + ts * (cross_mat(w) * T - cross_mat(T * w));
-			if (fabs(det(jac)) > 0.00000000001) w -= inverse(jac) * f;
-		}
-		av = w;
-		o = q_mul(q_axis_angle(ts * mag(av), av), o);
-		if (it + 1 < n_itr) {
-			M3 r = q_rot(o), rc = q_rot(q_conj(o));
-			T = r * I_ref * rc;
-		}
-	}
-	q = o;
-	angv = av;
-}
-
 // update_positions (PhysicsEngine.cpp:390-410) followed by
 // RigidBody::updateInertiaTensor (RigidBody.cpp:270-279).
 __global__ __launch_bounds__(128) void k_integrate(Bodies b, Params p) {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
-	const int n = b.n;
 	if (i >= b.nl || b.movement[i] != 0) return;
-	V3 com = LD3(b.com, i, n), vel = LD3(b.vel, i, n), angv = LD3(b.angv, i, n);
-	V3 pvel = LD3(b.pvel, i, n), pang = LD3(b.pang, i, n);
-	Q q = ldq(b.q, i, n);
-	ST3(b.prev_com, i, n, com);
-	stq(b.prev_q, i, n, q);
-	com += (vel + pvel) * p.sub_dt;
-	if (mag2(angv) != 0) {
-		M3 Iw = ldm(b.I, i, n), Ir = ldm(b.Iref, i, n);
-		gyro_rotate(angv, q, Iw, Ir, p.sub_dt, p.gyro_ticks);
-		q = q_mul(q_axis_angle(p.sub_dt * mag(pang), pang), q);
-	}
-	V3 z{0, 0, 0};
-	ST3(b.pvel, i, n, z);
-	ST3(b.pang, i, n, z);
-	q = q_normalize(q);
-	M3 r = q_rot(q), rc = q_rot(q_conj(q));
-	M3 Iw = r * ldm(b.Iref, i, n) * rc;
-	M3 Iwi = r * ldm(b.Irefinv, i, n) * rc;
-	ST3(b.com, i, n, com);
-	ST3(b.angv, i, n, angv);
-	stq(b.q, i, n, q);
-	stm(b.I, i, n, Iw);
-	stm(b.Iinv, i, n, Iwi);
+	integrate_body(b, p, i);
 }
 
 // updateInertiaTensor for every body without touching the stored orientation

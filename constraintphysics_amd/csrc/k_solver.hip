@@ -28,6 +28,7 @@
 // their impulses, per position sweep the normal row), plus two bodies'
 // velocity deltas when those are not in LDS.
 #include "k_hsys.h"
+#include "k_integrate.h"
 
 namespace cpx {
 
@@ -827,13 +828,13 @@ __device__ __forceinline__ void colour_phase(Contacts& C, const int4* desc, cons
 	}
 }
 
-// Global-memory variant (scenes whose velocity deltas do not fit LDS).  The
-// deltas live in b.dv viewed as [12][n] values of T.
-template <class T, int MAXT>
-__global__ __launch_bounds__(MAXT) void k_pgs_colored(Bodies b, Contacts C, const int* scene_body_off, const int* edge_range, const int4* desc,
-                                                     const uint32_t* lmask, const int* color_off, const int* ncolors, Params P,
-                                                     const int* scene_list) {
-	const int s = scene_list ? scene_list[blockIdx.x] : blockIdx.x;
+// The coloured solve of scene s by one workgroup.  Global-memory variant
+// (scenes whose velocity deltas do not fit LDS): the deltas live in b.dv
+// viewed as [12][n] values of T.
+template <class T>
+__device__ __forceinline__ void pgs_scene_global(Bodies& b, Contacts& C, int s, const int* scene_body_off, const int* edge_range,
+                                                 const int4* desc, const uint32_t* lmask, const int* color_off, const int* ncolors,
+                                                 const Params& P) {
 	const int b0 = scene_body_off[s], b1 = scene_body_off[s + 1];
 	const int N = b.n;
 	T* dv = reinterpret_cast<T*>(b.dv);
@@ -854,18 +855,15 @@ __global__ __launch_bounds__(MAXT) void k_pgs_colored(Bodies b, Contacts C, cons
 	}
 	for (int i = b0 + threadIdx.x; i < b1; i += blockDim.x)
 		if (b.movement[i] == 0) apply_dv(b, dv, i);
+	__syncthreads();
 }
 
 // LDS variant: 6 values of T per body, the velocity half during warm start and
 // velocity sweeps, then the pseudo-velocity half during position sweeps.
-// One workgroup per scene of the launch's size class (scene_list).
-template <class T, int MAXT>
-__global__ __launch_bounds__(MAXT) void k_pgs_colored_lds(Bodies b, Contacts C, const int* scene_list, const int* scene_body_off, const int* edge_range,
-                                                         const int4* desc, const uint32_t* lmask, const int* color_off, const int* ncolors,
-                                                         Params P) {
-	extern __shared__ double lds_raw[];
-	T* sdv = reinterpret_cast<T*>(lds_raw);
-	const int s = scene_list[blockIdx.x];
+template <class T>
+__device__ __forceinline__ void pgs_scene_lds(Bodies& b, Contacts& C, T* sdv, int s, const int* scene_body_off, const int* edge_range,
+                                              const int4* desc, const uint32_t* lmask, const int* color_off, const int* ncolors,
+                                              const Params& P) {
 	const int b0 = scene_body_off[s], nbs = scene_body_off[s + 1] - b0;
 	const int N = b.n;
 	T* dv = reinterpret_cast<T*>(b.dv);
@@ -904,19 +902,82 @@ __global__ __launch_bounds__(MAXT) void k_pgs_colored_lds(Bodies b, Contacts C, 
 	}
 }
 
+// One coloured solve, one workgroup per scene (scene_list, or every scene).
+template <class T, int MAXT>
+__global__ __launch_bounds__(MAXT) void k_pgs_colored(Bodies b, Contacts C, const int* scene_body_off, const int* edge_range, const int4* desc,
+                                                     const uint32_t* lmask, const int* color_off, const int* ncolors, Params P,
+                                                     const int* scene_list) {
+	const int s = scene_list ? scene_list[blockIdx.x] : blockIdx.x;
+	pgs_scene_global<T>(b, C, s, scene_body_off, edge_range, desc, lmask, color_off, ncolors, P);
+}
+template <class T, int MAXT>
+__global__ __launch_bounds__(MAXT) void k_pgs_colored_lds(Bodies b, Contacts C, const int* scene_list, const int* scene_body_off, const int* edge_range,
+                                                         const int4* desc, const uint32_t* lmask, const int* color_off, const int* ncolors,
+                                                         Params P) {
+	extern __shared__ double lds_raw[];
+	pgs_scene_lds<T>(b, C, reinterpret_cast<T*>(lds_raw), scene_list[blockIdx.x], scene_body_off, edge_range, desc, lmask, color_off,
+	                 ncolors, P);
+}
+
+template <class T>
+__device__ __forceinline__ void writeback_contact(Contacts& C, int k) {
+	const T* R = rec<T>(C, k);
+	C.imp_n[k] = (double)R[CP_R_IMP];
+	C.imp_t[k] = (double)R[CP_R_FIMP];
+	C.imp_t[C.cap + k] = (double)R[CP_R_FIMP + 1];
+	C.sready[k] = R[CP_R_SREADY] != T(0);
+}
+
+// The whole substep loop of one scene in one workgroup (P.substeps times:
+// rows of the scene's live contacts, the coloured solve, the impulses back
+// into the contact cache, integration of the scene's dynamic bodies).  A
+// scene's contacts are one contiguous range of the cache (its edges'), and a
+// workgroup touches only its own scene's bodies, contacts and rows, so the
+// substeps of different scenes need no synchronisation between them; the
+// arithmetic is the per-substep kernels', in the same order.
+template <class T, bool LDS>
+__device__ __forceinline__ void substeps_scene(Bodies& b, Contacts& C, T* sdv, int s, const int* scene_body_off, const int* edge_range,
+                                               const int* efirst, const int* ecnt, const int4* desc, const uint32_t* lmask,
+                                               const int* color_off, const int* ncolors, const Params& P) {
+	const int b0 = scene_body_off[s], b1 = scene_body_off[s + 1];
+	const int e0 = edge_range[2 * s], e1 = edge_range[2 * s + 1];
+	const int c0 = e1 > e0 ? efirst[e0] : 0, c1 = e1 > e0 ? efirst[e1 - 1] + ecnt[e1 - 1] : 0;
+	for (int sub = 0; sub < P.substeps; sub++) {
+		for (int k = c0 + threadIdx.x; k < c1; k += blockDim.x)
+			if (C.live[k]) build_row<T>(b, C, k, P);
+		__syncthreads();
+		if (LDS) pgs_scene_lds<T>(b, C, sdv, s, scene_body_off, edge_range, desc, lmask, color_off, ncolors, P);
+		else pgs_scene_global<T>(b, C, s, scene_body_off, edge_range, desc, lmask, color_off, ncolors, P);
+		for (int k = c0 + threadIdx.x; k < c1; k += blockDim.x)
+			if (C.live[k]) writeback_contact<T>(C, k);
+		for (int i = b0 + threadIdx.x; i < b1; i += blockDim.x)
+			if (b.movement[i] == 0) integrate_body(b, P, i);
+		__syncthreads();
+	}
+}
+template <class T, int MAXT>
+__global__ __launch_bounds__(MAXT) void k_substeps_lds(Bodies b, Contacts C, const int* scene_list, const int* scene_body_off,
+                                                      const int* edge_range, const int* efirst, const int* ecnt, const int4* desc,
+                                                      const uint32_t* lmask, const int* color_off, const int* ncolors, Params P) {
+	extern __shared__ double lds_raw[];
+	substeps_scene<T, true>(b, C, reinterpret_cast<T*>(lds_raw), scene_list[blockIdx.x], scene_body_off, edge_range, efirst, ecnt, desc,
+	                        lmask, color_off, ncolors, P);
+}
+template <class T, int MAXT>
+__global__ __launch_bounds__(MAXT) void k_substeps_global(Bodies b, Contacts C, const int* scene_list, const int* scene_body_off,
+                                                         const int* edge_range, const int* efirst, const int* ecnt, const int4* desc,
+                                                         const uint32_t* lmask, const int* color_off, const int* ncolors, Params P) {
+	substeps_scene<T, false>(b, C, nullptr, scene_list[blockIdx.x], scene_body_off, edge_range, efirst, ecnt, desc, lmask, color_off,
+	                         ncolors, P);
+}
+
 // Accumulated impulses and the static flag back into the persistent FP64
 // contact cache (warm start of the next substep; PersistentConstraint.cpp:201-215).
 template <class T>
 __global__ void k_rows_writeback(Contacts C, int nc, const int* ncp) {
 	const int n = ncp ? *ncp : nc;
-	for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-		if (!C.live[k]) continue;
-		const T* R = rec<T>(C, k);
-		C.imp_n[k] = (double)R[CP_R_IMP];
-		C.imp_t[k] = (double)R[CP_R_FIMP];
-		C.imp_t[C.cap + k] = (double)R[CP_R_FIMP + 1];
-		C.sready[k] = R[CP_R_SREADY] != T(0);
-	}
+	for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
+		if (C.live[k]) writeback_contact<T>(C, k);
 }
 
 // FP64 natural-map residual of the velocity rows after the sweeps, evaluated
@@ -1056,6 +1117,50 @@ void launch_pgs_colored_lds(const Bodies& b, const Contacts& C, const int* scene
 		else pgs_colored_lds<double, 256>(CP_LDS_ARGS);
 	}
 #undef CP_LDS_ARGS
+}
+template <class T, int MAXT>
+static void substeps_lds(const Bodies& b, const Contacts& C, const int* scene_list, int nlist, int max_bodies, const int* scene_body_off,
+                         const int* edge_range, const int* efirst, const int* ecnt, const int4* desc, const uint32_t* lmask,
+                         const int* color_off, const int* ncolors, const Params& P, int threads, hipStream_t s) {
+	static bool attr = false;
+	if (!attr) {
+		(void)hipFuncSetAttribute((const void*)k_substeps_lds<T, MAXT>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+		attr = true;
+	}
+	hipLaunchKernelGGL((k_substeps_lds<T, MAXT>), dim3(nlist), dim3(threads), (size_t)6 * max_bodies * sizeof(T), s, b, C, scene_list,
+	                   scene_body_off, edge_range, efirst, ecnt, desc, lmask, color_off, ncolors, P);
+}
+void launch_substeps_lds(const Bodies& b, const Contacts& C, const int* scene_list, int nlist, int max_bodies, const int* scene_body_off,
+                         const int* edge_range, const int* efirst, const int* ecnt, const int4* desc, const uint32_t* lmask,
+                         const int* color_off, const int* ncolors, const Params& P, int threads, hipStream_t s) {
+	if (!nlist) return;
+#define CP_SUB_ARGS b, C, scene_list, nlist, max_bodies, scene_body_off, edge_range, efirst, ecnt, desc, lmask, color_off, ncolors, P, threads, s
+	if (P.precision == CP_PREC_SPLIT) {
+		if (threads > 512) substeps_lds<float, 1024>(CP_SUB_ARGS);
+		else if (threads > 256) substeps_lds<float, 512>(CP_SUB_ARGS);
+		else substeps_lds<float, 256>(CP_SUB_ARGS);
+	} else {
+		if (threads > 512) substeps_lds<double, 1024>(CP_SUB_ARGS);
+		else if (threads > 256) substeps_lds<double, 512>(CP_SUB_ARGS);
+		else substeps_lds<double, 256>(CP_SUB_ARGS);
+	}
+#undef CP_SUB_ARGS
+}
+void launch_substeps_global(const Bodies& b, const Contacts& C, const int* scene_list, int nlist, const int* scene_body_off, const int* edge_range,
+                            const int* efirst, const int* ecnt, const int4* desc, const uint32_t* lmask, const int* color_off,
+                            const int* ncolors, const Params& P, int threads, hipStream_t s) {
+	if (!nlist) return;
+#define CP_SUB_ARGS dim3(nlist), dim3(threads), 0, s, b, C, scene_list, scene_body_off, edge_range, efirst, ecnt, desc, lmask, color_off, ncolors, P
+	if (P.precision == CP_PREC_SPLIT) {
+		if (threads > 512) hipLaunchKernelGGL((k_substeps_global<float, 1024>), CP_SUB_ARGS);
+		else if (threads > 256) hipLaunchKernelGGL((k_substeps_global<float, 512>), CP_SUB_ARGS);
+		else hipLaunchKernelGGL((k_substeps_global<float, 256>), CP_SUB_ARGS);
+	} else {
+		if (threads > 512) hipLaunchKernelGGL((k_substeps_global<double, 1024>), CP_SUB_ARGS);
+		else if (threads > 256) hipLaunchKernelGGL((k_substeps_global<double, 512>), CP_SUB_ARGS);
+		else hipLaunchKernelGGL((k_substeps_global<double, 256>), CP_SUB_ARGS);
+	}
+#undef CP_SUB_ARGS
 }
 void launch_rows_writeback(const Contacts& C, int nc, const Params& P, hipStream_t s, const int* ncp) {
 	if (!nc) return;

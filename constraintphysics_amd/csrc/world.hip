@@ -220,6 +220,7 @@ struct cp_world {
 	hipEvent_t pgs_ev[2 * 64];
 	double last_ms[8] = {0};
 	bool timing = true;
+	bool fused_step = false;  // the last step ran the fused substep kernels
 	// joints (host copies of static data; device SoA)
 	std::vector<int> j_type, j_a, j_b;
 	std::vector<double> j_geo;  // 14 per joint: tp1, tp2, ax1, ax2, ps, rs
@@ -425,6 +426,7 @@ struct cp_world {
 	void step();
 	void colored_pgs(const Bodies& b, const Params& P);
 	void colored_substep(const Bodies& b, const Params& P, int s, const int* ncp);
+	void fused_substeps(const Bodies& b, const Params& P);
 	void graph_substeps(const Bodies& b, const Params& P);
 };
 
@@ -1472,6 +1474,30 @@ void cp_world::colored_pgs(const Bodies& b, const Params& P) {
 		}
 }
 
+// The substep loop as one launch per size class (the largest class and the
+// global-memory one on the world's stream, the two smaller classes beside it).
+void cp_world::fused_substeps(const Bodies& b, const Params& P) {
+	const int* thr = cfg.pgs_class_threads;
+	bool fork = !cls_lists[0].empty() || !cls_lists[1].empty();
+	if (fork) {
+		HIPCHK(hipEventRecord(fork_ev, stream));
+		for (auto& x : side) HIPCHK(hipStreamWaitEvent(x, fork_ev, 0));
+	}
+	const Contacts& C = cb[cur].c;
+	launch_substeps_lds(b, C, cls_dev[2].p, (int)cls_lists[2].size(), cls_max[2], scene_body_off.p, edge_range.p, e_first.p, e_cnt.p,
+	                    e_desc.p, e_lmask.p, color_off.p, ncolors.p, P, thr[2], stream);
+	launch_substeps_global(b, C, cls_dev[3].p, (int)cls_lists[3].size(), scene_body_off.p, edge_range.p, e_first.p, e_cnt.p, e_desc.p,
+	                       e_lmask.p, color_off.p, ncolors.p, P, cfg.solver_threads, stream);
+	for (int c = 0; c < 2 && fork; c++)
+		launch_substeps_lds(b, C, cls_dev[c].p, (int)cls_lists[c].size(), cls_max[c], scene_body_off.p, edge_range.p, e_first.p, e_cnt.p,
+		                    e_desc.p, e_lmask.p, color_off.p, ncolors.p, P, thr[c], side[c]);
+	if (fork)
+		for (int c = 0; c < 2; c++) {
+			HIPCHK(hipEventRecord(join_ev[c], side[c]));
+			HIPCHK(hipStreamWaitEvent(stream, join_ev[c], 0));
+		}
+}
+
 // Substep s of the coloured mode with the contact count read on the device.
 void cp_world::colored_substep(const Bodies& b, const Params& P, int s, const int* ncp) {
 	const int cap = cb[cur].c.cap;
@@ -1581,8 +1607,14 @@ void cp_world::step() {
 	if (timing) HIPCHK(hipEventRecord(ev[3], stream));
 	int npgs = 0;
 	const bool use_graph = cfg.graph && !ordered && !(cfg.diagnostics & 1) && ncontacts > 0;
+	const bool use_fused = cfg.fused && !cfg.graph && !ordered && !(cfg.diagnostics & 1);
 	if (use_graph) graph_substeps(b, P);
-	for (int s = 0; s < cfg.substeps && !use_graph; s++) {
+	fused_step = use_fused;
+	if (use_fused) {
+		fused_substeps(b, P);
+		launch_age(cb[cur].c, ncontacts, stream);
+	}
+	for (int s = 0; s < cfg.substeps && !use_graph && !use_fused; s++) {
 		launch_build_rows(b, cb[cur].c, ncontacts, P, stream);
 		// joints; holonomic ones are current at the first substep (PhysicsEngine.cpp:1396-1403)
 		launch_joint_rows(b, J, P, P.sub_dt, 0, s == 0, stream);
@@ -1646,6 +1678,10 @@ void cp_world::step() {
 		}
 		last_ms[5] = pg;
 		last_ms[6] = npgs;
+		if (fused_step) {  // the solve is inside the fused substep kernels: report their span
+			last_ms[5] = last_ms[3];
+			last_ms[6] = 0;
+		}
 		last_ms[7] = live_contacts;
 	}
 }
@@ -1822,6 +1858,7 @@ void cp_config_default(cp_config* c) {
 	c->pgs_class_threads[1] = 128;
 	c->pgs_class_threads[2] = 256;
 	c->graph = 0;
+	c->fused = 0;
 	c->precision = 0;
 }
 
@@ -1847,7 +1884,7 @@ static bool valid_cfg(const cp_config* c) {
 	for (int k = 0; k < 3; k++) classes = classes && valid_threads(c->pgs_class_threads[k]);
 	return c->dt > 0 && c->substeps >= 1 && c->vel_iters >= 0 && c->pos_iters >= 0 && c->gyro_ticks >= 1 && valid_threads(c->solver_threads) &&
 	       (c->pgs_mode >= 0 && c->pgs_mode <= 2) && (c->precision == CP_PREC_FP64 || c->precision == CP_PREC_SPLIT) && classes &&
-	       (c->graph == 0 || c->graph == 1);
+	       (c->graph == 0 || c->graph == 1) && (c->fused == 0 || c->fused == 1);
 }
 
 int cp_world_create(const cp_config* cfg, cp_world** out) {

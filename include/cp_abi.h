@@ -92,6 +92,13 @@ typedef struct {
 	                            a HIP graph and replayed each step (recaptured when buffers, sizes or
 	                            settings change); the per-solve PGS time is then not measured.
 	                            Default 0.  Does not change results. */
+	int fused;               /* 1: in CP_PGS_COLORED mode without the residual diagnostics and the graph
+	                            option, the substep loop runs as one kernel per size class: each scene's
+	                            workgroup builds its rows, solves, writes the impulses back and
+	                            integrates its bodies, substep after substep, with no synchronisation
+	                            between scenes.  0: one launch per stage and substep.  Default 0 (the
+	                            fused kernel holds 2 waves per SIMD against the solve's 4, and is faster
+	                            only on small single scenes).  Does not change results. */
 } cp_config;
 
 typedef struct cp_world cp_world;
@@ -387,7 +394,9 @@ int cp_world_raycast(cp_world* w, int n, const double* origins3, const double* d
 /* Timing of the last cp_world_step call, per phase, in milliseconds (HIP events):
  * [0] gravity + broadphase, [1] narrowphase, [2] persistence + schedule,
  * [3] substeps (rows + PGS + integrate), [4] geometry refresh,
- * [5] PGS kernels only (sum over substeps), [6] PGS launches, [7] live contacts. */
+ * [5] PGS kernels only (sum over substeps), [6] PGS launches, [7] live contacts.
+ * With the fused substep kernels (cp_config.fused) the solve is not timed on
+ * its own: [5] = [3] and [6] = 0. */
 int cp_world_last_timing(cp_world* w, double* ms8);
 
 #ifdef __cplusplus
