@@ -10,7 +10,7 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libcp_hip.so")
-SOURCES = ["k_bodies.hip", "k_broadphase.hip", "k_narrowphase.hip", "k_contacts.hip", "k_solver.hip", "k_joints.hip", "k_raycast.hip", "world.hip",
+SOURCES = ["k_bodies.hip", "k_broadphase.hip", "k_narrowphase.hip", "k_contacts.hip", "k_solver.hip", "k_joints.hip", "k_raycast.hip", "k_islands.hip", "world.hip",
            "shapes.cpp"]
 
 # FP64 parity needs IEEE results in the reference's operation order: no FMA

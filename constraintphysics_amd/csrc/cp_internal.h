@@ -116,25 +116,72 @@ struct HSysDev {
 	int *cp_off, *cs_ptr, *cs_row, *cs_loc, *dg_loc;
 };
 
-// Island schedule (reference-order and coloured-island solves).  rows holds
-// (index, kind) pairs, kind 0 contact normal, 1 friction, 2 joint; an
-// island's rows are [off[i], off[i+1]).  Coloured islands: an edge's rows are
-// [eoff[e], eoff[e+1]), a colour's edges [coff[c], coff[c+1]), an island's
-// colours [icoff[i], icoff[i+1]), the last one a serial bucket when serial[i].
+// Island schedule built on the device each step.  rows: (index, kind)
+// pairs, kind 0 contact normal, 1 friction, 2 joint; an island's rows are
+// [off[i], off[i+1]).  Its holonomic systems are sys[sys_off[i], sys_off[i] +
+// sys_cnt[i]).  Coloured islands: an edge's rows are [eoff[e], eoff[e+1]),
+// a colour's edges [coff[c], coff[c+1]), an island's colours [icoff[i],
+// icoff[i] + ncol[i]), the last one a serial bucket when serial[i].
 // jlin[2k], jlin[2k+1]: columns of joint k's bodies in the linearization-error
-// vectors (stride nl): the body index for a dynamic body, one column per
-// (island, body) beyond the bodies for the others.
+// vectors (stride nl): the body index for a dynamic body, a column of the
+// island beyond the bodies for the others.  The island's bodies (every body
+// of its component, in walk order) are vlist[voff[i], voff[i] + vcnt[i]).
 struct IslandSched {
 	int n, nl;
-	const int *rows, *off, *sys, *sys_off, *jlin;
-	const int *eoff, *coff, *icoff, *serial;
+	const int *rows, *off, *sys, *sys_off, *sys_cnt, *jlin;
+	const int *eoff, *coff, *icoff, *ncol, *serial;
 	// coloured islands with island-local velocity deltas in LDS: the island's
-	// dynamic bodies bod[boff[i], boff[i+1]) (local index = position), contact
-	// k's bodies as local indices cloc[2k], cloc[2k+1], joint k's jloc[2k],
-	// jloc[2k+1] (the zero column boff[i+1] - boff[i] for non-dynamic bodies);
-	// islands with more than lds_bodies - 1 bodies use b.dv.  cloc null: none.
-	const int *bod, *boff, *cloc, *jloc;
+	// dynamic bodies with rows bod[boff[i], boff[i] + nbod[i]) (local index =
+	// position), contact k's bodies as local indices cloc[2k], cloc[2k+1],
+	// joint k's jloc[2k], jloc[2k+1] (the zero column nbod[i] for non-dynamic
+	// bodies); islands with more than lds_bodies - 1 bodies use b.dv.  cloc
+	// null: none.
+	const int *bod, *boff, *nbod, *cloc, *jloc;
 	int lds_bodies;
+	const int *vlist, *voff, *vcnt;
+};
+
+// Per-component counts of the island search (indexed by the component's
+// root body), summed by an exclusive scan over node-map positions.
+struct IslCnt {
+	int n, rows, edges, bodies, adj, joints, notready;
+};
+struct IslCntSum {
+	__host__ __device__ IslCnt operator()(const IslCnt& a, const IslCnt& b) const {
+		return IslCnt{a.n + b.n, a.rows + b.rows, a.edges + b.edges, a.bodies + b.bodies, a.adj + b.adj, a.joints + b.joints,
+		              a.notready | b.notready};
+	}
+};
+// Island ranges: root body, row offsets (n + 1), walked-edge, body, stack
+// and joint offsets, bodies per island.
+struct IslTables {
+	int *root, *off, *edge_off, *body_off, *body_cnt, *stack_off, *joint_off;
+};
+// Sleeping state (PhysicsEngine.h:218-224; RigidBody asleep,
+// sleep_ready_counter, non_sleepy_tick_count, history).
+struct SleepDev {
+	int enabled;
+	int* asleep;        // [n]
+	double* counter;    // [n] sleep_ready_counter
+	int* ticks;         // [n] non_sleepy_tick_count
+	int* hlen;          // [n] history entries held
+	int* hhead;         // [n] ring position of the oldest entry
+	double* hist;       // [hcap][6][n] (vel xyz, ang_vel xyz) per entry
+	int* wake_req;      // [n] waking actions pending (RigidBody::recievedWakingAction)
+	int* disabled;      // [n] RigidBody::sleep_disabled
+	int hcap;           // history length kept: (int)(sleep_delay / step_time)
+	double delay, step_time, vel_eps, ang_vel_eps, accel_eps, ang_accel_eps, wake_vel, hist_need;
+	int tick_threshold;
+};
+// What the per-island walk reads and writes.
+struct IslWalk {
+	IslTables T;
+	int nb, hol;
+	const int *movement, *asleep, *aoff, *adjv, *me, *jord, *live, *ja, *jb, *jsys;
+	const uint64_t *adjk, *ckey;
+	int *stack, *vlist, *elist, *vis, *rows, *ecol, *coff, *eorder, *icoff, *ncol, *serial, *eoff, *bod, *nbod, *loc, *cloc, *jloc;
+	int *sys_seen, *sys, *slots, *jlin, *sys_cnt;
+	uint64_t* col_used;
 };
 
 // Two side streams forked from and joined back into a main stream with events
@@ -358,5 +405,18 @@ void launch_pgs_islands(const Bodies& b, const Contacts& C, const Joints& J, con
 void launch_pgs_colored(const Bodies& b, const Contacts& C, const int* scene_body_off, int nscenes, const int* edge_range, const int* order,
                         const int* color_off, const int* ncolors, const int* efirst, const int* ecnt, const Params& P, int threads,
                         hipStream_t s, const int* scene_list = nullptr, int nlist = 0);
+
+// Island search (k_islands.hip), in launch order.
+void launch_ib_keys(const uint64_t* ekey, int ne, const uint64_t* jgkey, int ng, uint64_t* keys, int* vals, hipStream_t s);
+void launch_ib_heads(const uint64_t* skeys, int n, int* head, hipStream_t s);
+void launch_ib_edges(const uint64_t* skeys, const int* svals, int n, const int* head, const int* midx, int ne, const int* efirst,
+                     const int* ecnt, const int* live, const int* jgoff, int* me, uint64_t* adjk, int* adjv, hipStream_t s);
+void launch_ib_aoff(const uint64_t* adjk, int n2, int nb, int* aoff, hipStream_t s);
+void launch_ib_components(const int* me, const int* midx, int n, int nb, const int* movement, const int* rank, const int* aoff, int* parent,
+                          int* comp, IslCnt* acc, const SleepDev& S, hipStream_t s);
+void launch_ib_flags(int nb, const int* rord, const int* comp, const IslCnt* acc, int sleeping, IslCnt* in, hipStream_t s);
+void launch_ib_tables(int nb, const int* rord, const IslCnt* in, const IslCnt* out, const IslTables& T, int* info, const int* comp,
+                      const IslCnt* acc, const SleepDev& S, double* vel, double* angv, hipStream_t s);
+void launch_ib_walk(const IslWalk& W, int nisl, bool colored, hipStream_t s);
 
 }  // namespace cpx

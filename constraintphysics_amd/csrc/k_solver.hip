@@ -13,7 +13,7 @@
 //
 // Sweep schedules over the same rows (ConstraintSolver.cpp:80-213):
 //  * k_pgs_ordered: the reference's sequential order (one thread per island,
-//    rows in the island-search order supplied by the host) — the parity mode.
+//    rows in the island-search order) — the parity mode.
 //  * k_pgs_islands: the same islands with their edges graph-coloured, one wave
 //    per island (joints and contacts; FP64).
 //  * k_pgs_colored_lds / k_pgs_colored: one workgroup per scene; edges (body
@@ -524,7 +524,7 @@ __device__ __forceinline__ void apply_dv(Bodies& b, const T* dv, int i) {
 	}
 }
 
-// Island solves (FP64 only) over a host-built island schedule: rows[2*r] =
+// Island solves (FP64 only) over the island schedule: rows[2*r] =
 // index, rows[2*r+1] = kind (0 contact normal, 1 friction, 2 joint).  At the
 // first substep each island's holonomic systems are solved after the sweeps
 // (ConstraintSolver.cpp:163-196).  The reference keeps one set of velocity
@@ -536,7 +536,7 @@ using DG = DvGlobal<double>;
 
 // Where an island solve keeps its velocity deltas: b.dv ([12][n], global body
 // indices) or an island-local LDS array ([12][stride], local indices from the
-// host's cloc / jloc maps; non-dynamic bodies share the never-written zero
+// island search's cloc / jloc maps; non-dynamic bodies share the never-written zero
 // column stride - 1).
 struct DvMap {
 	double* base;
@@ -669,7 +669,7 @@ __device__ void island_holonomic(const Bodies& b, const Joints& J, const HSysDev
 		}
 	};
 	for (int it = 0; it < P.hol_iters; it++)
-		for (int q = S.sys_off[t]; q < S.sys_off[t + 1]; q++) {
+		for (int q = S.sys_off[t]; q < S.sys_off[t] + S.sys_cnt[t]; q++) {
 			apply(S.sys[q], false, -1.0);
 			apply(S.sys[q], true, (double)15.0f);  // excessive_linear_error_torque_threshold (PhysicsEngine.h:217)
 		}
@@ -705,7 +705,7 @@ __global__ __launch_bounds__(64) void k_pgs_ordered(Bodies b, Contacts C, Joints
 		for (int r = r0; r < r1; r++) island_row(b, C, J, m, rows[2 * r], rows[2 * r + 1], 1);
 	for (int it = 0; it < P.pos_iters; it++)
 		for (int r = r0; r < r1; r++) island_row(b, C, J, m, rows[2 * r], rows[2 * r + 1], 2);
-	if (first && P.hol_iters > 0 && S.sys_off[t + 1] > S.sys_off[t]) {
+	if (first && P.hol_iters > 0 && S.sys_cnt[t] > 0) {
 		island_holonomic(b, J, H, S, t, P, lin, m);
 		for (int r = r0; r < r1; r++)
 			if (rows[2 * r + 1] == 2) holonomic_warm_scale(J, rows[2 * r]);
@@ -723,7 +723,7 @@ __global__ __launch_bounds__(64) void k_pgs_ordered(Bodies b, Contacts C, Joints
 }
 
 // Coloured islands: one wave per island.  The island's edges (body pairs with
-// rows: joints, then live contacts) are coloured on the host so that no two
+// rows: joints, then live contacts) are coloured by the island search so that no two
 // edges of a colour share a dynamic body; a lane runs one edge's rows in
 // order and colours are separated by a workgroup barrier.  Per body the
 // deltas accumulate in the order of the island's row list (colour-major), so
@@ -735,11 +735,11 @@ __global__ __launch_bounds__(64) void k_pgs_islands(Bodies b, Contacts C, Joints
 	const int t = blockIdx.x, lane = threadIdx.x;
 	const int* rows = S.rows;
 	const int r0 = S.off[t], r1 = S.off[t + 1];
-	const int c0 = S.icoff[t], c1 = S.icoff[t + 1];
+	const int c0 = S.icoff[t], c1 = c0 + S.ncol[t];
 	const bool ser = S.serial[t] != 0;
 	// island-local velocity deltas in LDS when the island fits (its bodies
 	// plus a zero column for the non-dynamic ones), else b.dv
-	const int bo0 = S.boff[t], nbi = S.boff[t + 1] - bo0;
+	const int bo0 = S.boff[t], nbi = S.nbod[t];
 	const bool local = S.cloc && nbi + 1 <= S.lds_bodies;
 	const DvMap m = local ? DvMap{sdv, nbi + 1, S.cloc, S.jloc} : DvMap{b.dv, b.n, nullptr, nullptr};
 	if (local)
@@ -760,7 +760,7 @@ __global__ __launch_bounds__(64) void k_pgs_islands(Bodies b, Contacts C, Joints
 	sweep(0);
 	for (int it = 0; it < P.vel_iters; it++) sweep(1);
 	for (int it = 0; it < P.pos_iters; it++) sweep(2);
-	if (first && P.hol_iters > 0 && S.sys_off[t + 1] > S.sys_off[t]) {
+	if (first && P.hol_iters > 0 && S.sys_cnt[t] > 0) {
 		if (lane == 0) island_holonomic(b, J, H, S, t, P, lin, m);
 		__syncthreads();
 		for (int r = r0 + lane; r < r1; r += 64)

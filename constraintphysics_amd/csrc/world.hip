@@ -250,16 +250,26 @@ struct cp_world {
 		int degree = 0, cap = 0;
 	};
 	std::vector<HLayout> hs_lay;
-	DArr<int> dh_nc, dh_ord_off, dh_order, dh_vloc, dh_tab_off, dh_table, dh_degree, dh_buf_off, dh_dbuf_off, isl_sys, isl_sys_off;
+	DArr<int> dh_nc, dh_ord_off, dh_order, dh_vloc, dh_tab_off, dh_table, dh_degree, dh_buf_off, dh_dbuf_off, isl_sys;
 	DArr<int> dh_cp_off, dh_cs_ptr, dh_cs_row, dh_cs_loc, dh_dg_loc;
 	DArr<double> dh_buf, dh_dbuf, dh_scratch;
 	bool hs_pending = true;
-	std::vector<int> last_isl_sys, last_isl_sys_off;
-	DArr<int> isl_jlin, isl_eoff, isl_coff, isl_icoff, isl_serial, isl_bod, isl_boff, isl_cloc, isl_jloc;
-	std::vector<int> loc_of, loc_stamp;  // schedule scratch: island-local body indices
-	int isl_lds_bodies = 0;              // LDS columns per island solve (largest fitting island + 1)
+	// island schedule (built on the device each step of the island modes)
+	DArr<int> isl_jlin, isl_eoff, isl_coff, isl_icoff, isl_ncol, isl_serial, isl_bod, isl_nbod, isl_cloc, isl_jloc, isl_sys_cnt;
+	DArr<int> isl_root, isl_edge_off, isl_body_off, isl_body_cnt, isl_stack_off, isl_joint_off;
+	DArr<uint64_t> ib_keys, ib_keys2, ib_adjk, ib_adjk2, ib_jgkey, ib_col_used;
+	DArr<int> ib_vals, ib_vals2, ib_head, ib_midx, ib_me, ib_adjv, ib_adjv2, ib_aoff, ib_parent, ib_comp, ib_jgoff, ib_jord, ib_rank, ib_rord,
+	    ib_info, ib_stack, ib_vlist, ib_elist, ib_vis, ib_ecol, ib_eorder, ib_loc, ib_sys_seen, ib_slots;
+	DArr<IslCnt> ib_acc, ib_in, ib_out;
+	int ib_nj = -1, ib_ng = 0;        // joints the pair groups were built from, groups
+	std::vector<int> ib_rord_host;     // node-map order the ranks were uploaded from
+	int isl_n = 0;                     // islands of the last schedule
+	bool islands_on_device = false;    // last_rows / last_offs not yet read back
+	int isl_lds_bodies = 0;            // LDS columns per island solve (largest island + 1, capped)
 	int lin_nl = 0;
-	std::vector<std::array<uint64_t, 2>> col_used;
+	IslTables isl_tables();
+	void fetch_islands();
+	SleepDev sleep_dev();
 	// static meshes: one world instance per mesh body (triangles and AABB tree
 	// placed on the host from the body's pose), device copies, step scratch
 	std::vector<int> mesh_inst_of_body, mesh_body;
@@ -452,10 +462,22 @@ void cp_world::free_all() {
 	nf_body.free_();
 	nf_any.free_();
 	DArr<int>* jints[] = {&dj_type, &dj_a, &dj_b, &dj_sys, &dh_nc, &dh_ord_off, &dh_order, &dh_vloc, &dh_tab_off, &dh_table, &dh_degree,
-	                      &dh_buf_off, &dh_dbuf_off, &isl_sys, &isl_sys_off, &isl_jlin, &isl_eoff, &isl_coff, &isl_icoff,
-	                      &isl_serial, &isl_bod, &isl_boff, &isl_cloc, &isl_jloc, &dh_cp_off, &dh_cs_ptr,
-	                      &dh_cs_row, &dh_cs_loc, &dh_dg_loc};
+	                      &dh_buf_off, &dh_dbuf_off, &isl_sys, &isl_jlin, &isl_eoff, &isl_coff, &isl_icoff, &isl_ncol,
+	                      &isl_serial, &isl_bod, &isl_nbod, &isl_cloc, &isl_jloc, &isl_sys_cnt, &dh_cp_off, &dh_cs_ptr,
+	                      &dh_cs_row, &dh_cs_loc, &dh_dg_loc, &isl_root, &isl_edge_off, &isl_body_off, &isl_body_cnt, &isl_stack_off,
+	                      &isl_joint_off, &ib_vals, &ib_vals2, &ib_head, &ib_midx, &ib_me, &ib_adjv, &ib_adjv2, &ib_aoff, &ib_parent,
+	                      &ib_comp, &ib_jgoff, &ib_jord, &ib_rank, &ib_rord, &ib_info, &ib_stack, &ib_vlist, &ib_elist, &ib_vis,
+	                      &ib_ecol, &ib_eorder, &ib_loc, &ib_sys_seen, &ib_slots};
 	for (auto* a : jints) a->free_();
+	DArr<uint64_t>* ju64[] = {&ib_keys, &ib_keys2, &ib_adjk, &ib_adjk2, &ib_jgkey, &ib_col_used};
+	for (auto* a : ju64) a->free_();
+	ib_acc.free_();
+	ib_in.free_();
+	ib_out.free_();
+	ib_nj = -1;
+	ib_rord_host.clear();
+	isl_n = 0;
+	islands_on_device = false;
 	DArr<double>* jdbl[] = {&dj_tp1, &dj_tp2, &dj_ax1, &dj_ax2, &dj_ps, &dj_rs, &dj_imp, &dj_pimp, &dj_himp, &dj_u, &dj_w, &dj_row, &lin,
 	                        &dh_buf, &dh_dbuf, &dh_scratch};
 	for (auto* a : jdbl) a->free_();
@@ -1171,239 +1193,140 @@ void cp_world::edges() {
 }
 
 // Island search of PhysicsEngine::sleepOrSolveIslands (PhysicsEngine.cpp:1536-1613)
-// with bfsVisitAll (:1263-1283), on the host over the post-narrowphase cache.
-// Edges are body pairs with contacts or joints; a node's edges are ordered by
+// on the device (k_islands.hip) over the post-narrowphase cache and the
+// joints.  The host supplies the joint pair groups and the node-map order
+// when they change and reads back the island count and the largest island.
+// Edges are body pairs with contacts or joints, a node's edges ordered by
 // the other body's id (ConstraintGraphNode::insertNewEdge, :1984-1995); an
-// edge's rows are its joints in creation order, then its live contacts.  Each
-// island also lists the holonomic systems its joints belong to, in the order
-// they are first met.
+// edge's rows are its joints in creation order, then its live contacts.
 void cp_world::schedule_islands(bool colored) {
-	int n = ncontacts;
-	std::vector<uint64_t> key;
-	std::vector<int> live;
-	down(key, cb[cur].c.key, n);
-	down(live, cb[cur].c.live, n);
-	// edges in key order (lo << 32 | hi): the contact pairs (the cache is
-	// sorted by key) merged with the joint pairs (stable: creation order)
-	struct Edge {
-		int first = -1, nlive = 0;  // first cache index of the pair, live contacts
-		int lo = 0, hi = 0;
-		int j0 = 0, j1 = 0;  // joints jord[j0, j1)
-	};
-	const int nj = (int)j_type.size();
-	std::vector<uint64_t> jkey(nj);
-	std::vector<int> jord(nj);
-	for (int j = 0; j < nj; j++) {
-		uint64_t lo = (uint64_t)std::min(j_a[j], j_b[j]), hi = (uint64_t)std::max(j_a[j], j_b[j]);
-		jkey[j] = lo << 32 | hi;
-		jord[j] = j;
-	}
-	std::stable_sort(jord.begin(), jord.end(), [&](int x, int y) { return jkey[x] < jkey[y]; });
-	std::vector<Edge> edges;
-	edges.reserve(n / 2 + nj + 1);
-	{
-		int k = 0, jj = 0;
-		while (k < n || jj < nj) {
-			uint64_t kk = k < n ? key[k] : UINT64_MAX, kj = jj < nj ? jkey[jord[jj]] : UINT64_MAX;
-			uint64_t K = std::min(kk, kj);
-			Edge E;
-			E.lo = (int)(K >> 32);
-			E.hi = (int)(K & 0xFFFFFFFFu);
-			E.j0 = jj;
-			while (jj < nj && jkey[jord[jj]] == K) jj++;
-			E.j1 = jj;
-			if (k < n && key[k] == K) {
-				E.first = k;
-				for (; k < n && key[k] == K; k++) E.nlive += live[k] != 0;
+	const int n = ncontacts, nj = (int)j_type.size();
+	if (ib_nj != nj) {
+		// joint pair groups in key order (lo << 32 | hi), joints of a group in
+		// creation order
+		std::vector<uint64_t> jkey(nj), gk;
+		std::vector<int> jord(nj), goff;
+		for (int j = 0; j < nj; j++) {
+			uint64_t lo = (uint64_t)std::min(j_a[j], j_b[j]), hi = (uint64_t)std::max(j_a[j], j_b[j]);
+			jkey[j] = lo << 32 | hi;
+			jord[j] = j;
+		}
+		std::stable_sort(jord.begin(), jord.end(), [&](int x, int y) { return jkey[x] < jkey[y]; });
+		for (int x = 0; x < nj; x++)
+			if (x == 0 || jkey[jord[x]] != jkey[jord[x - 1]]) {
+				gk.push_back(jkey[jord[x]]);
+				goff.push_back(x);
 			}
-			edges.push_back(E);
-		}
+		goff.push_back(nj);
+		up(ib_jord, jord);
+		up(ib_jgkey, gk);
+		up(ib_jgoff, goff);
+		ib_ng = (int)gk.size();
+		ib_nj = nj;
 	}
-	// adjacency (CSR): appending in key order lists each body's neighbours by
-	// id (those below it, as hi, come first, each group ascending)
-	std::vector<int> aoff(nb + 1, 0), adj_o(2 * edges.size()), adj_e(2 * edges.size());
-	for (const Edge& E : edges) { aoff[E.lo + 1]++; aoff[E.hi + 1]++; }
-	for (int i = 0; i < nb; i++) aoff[i + 1] += aoff[i];
-	{
-		std::vector<int> fill(aoff.begin(), aoff.end() - 1);
-		for (int e = 0; e < (int)edges.size(); e++) {
-			const Edge& E = edges[e];
-			adj_o[fill[E.lo]] = E.hi; adj_e[fill[E.lo]++] = e;
-			adj_o[fill[E.hi]] = E.lo; adj_e[fill[E.hi]++] = e;
-		}
-	}
-	std::vector<int> sys_of(nj, -1), sys_seen(hs_og.size(), -1);
-	if (cfg.hol_iters > 0)
-		for (size_t q = 0; q < hs_og.size(); q++)
-			for (int j : hs_og[q]) sys_of[j] = (int)q;
-	// an edge's rows: its joints, then its live contacts (normal, friction)
-	auto emit = [&](int e) {
-		const Edge& E = edges[e];
-		for (int x = E.j0; x < E.j1; x++) { last_rows.push_back(jord[x]); last_rows.push_back(2); }
-		if (!E.nlive) return;
-		for (int k = E.first; k < n && key[k] == key[E.first]; k++) {
-			if (!live[k]) continue;
-			last_rows.push_back(k); last_rows.push_back(0);
-			last_rows.push_back(k); last_rows.push_back(1);
-		}
-	};
-	auto has_rows = [&](int e) { return edges[e].j1 > edges[e].j0 || edges[e].nlive > 0; };
-	std::vector<char> visited(nb, 0);
 	const std::vector<int>& roots = root_order();
-	std::vector<int> eoff{0}, coff{0}, icoff{0}, serial, jlin(2 * nj, 0), isl_edges, col;
-	std::vector<std::pair<int, int>> slots;  // (non-dynamic body, lin column) of the current island
-	int nl = nb;
-	std::vector<int> bod, boff{0}, cloc, jloc;
-	int max_isl_bodies = 0;
-	if (colored) {
-		col_used.assign(nb, {0, 0});
-		loc_of.assign(nb, 0);
-		loc_stamp.assign(nb, -1);
-		cloc.assign(2 * (size_t)std::max(n, 1), 0);
-		jloc.assign(2 * (size_t)std::max(nj, 1), 0);
+	if (ib_rord_host != roots) {
+		ib_rord_host = roots;
+		std::vector<int> rank(nb, 0);
+		for (int p = 0; p < (int)roots.size(); p++) rank[roots[p]] = p;
+		up(ib_rord, roots);
+		up(ib_rank, rank);
 	}
-	last_rows.clear();
-	last_offs.assign(1, 0);
-	last_isl_sys.clear();
-	last_isl_sys_off.assign(1, 0);
-	for (int root : roots) {
-		if (b_move[root] != 0 || visited[root]) continue;
-		isl_edges.clear();
-		std::vector<int> st{root};
-		while (!st.empty()) {
-			int c = st.back();
-			st.pop_back();
-			if (visited[c]) continue;
-			visited[c] = 1;
-			if (b_move[c] == 0)
-				for (int x = aoff[c]; x < aoff[c + 1]; x++)
-					if (!visited[adj_o[x]] && has_rows(adj_e[x])) isl_edges.push_back(adj_e[x]);
-			for (int x = aoff[c]; x < aoff[c + 1]; x++)
-				if (b_move[adj_o[x]] == 0 && !visited[adj_o[x]]) st.push_back(adj_o[x]);
-		}
-		if (isl_edges.empty()) continue;
-		size_t start = last_rows.size() / 2;
-		if (!colored) {
-			for (int e : isl_edges) emit(e);
-		} else {
-			// greedy colouring in visit order: lowest colour free at both
-			// dynamic ends, CP_MAXC = the serial bucket
-			col.assign(isl_edges.size(), CP_MAXC);
-			int ncol = 0;
-			bool any_serial = false;
-			for (size_t x = 0; x < isl_edges.size(); x++) {
-				const Edge& E = edges[isl_edges[x]];
-				uint64_t m0 = 0, m1 = 0;
-				for (int i : {E.lo, E.hi})
-					if (b_move[i] == 0) { m0 |= col_used[i][0]; m1 |= col_used[i][1]; }
-				int c = ~m0 ? __builtin_ctzll(~m0) : (~m1 ? 64 + __builtin_ctzll(~m1) : CP_MAXC);
-				if (c == CP_MAXC) { any_serial = true; continue; }
-				col[x] = c;
-				ncol = std::max(ncol, c + 1);
-				for (int i : {E.lo, E.hi})
-					if (b_move[i] == 0) col_used[i][c >> 6] |= 1ull << (c & 63);
-			}
-			for (int e : isl_edges)
-				for (int i : {edges[e].lo, edges[e].hi}) col_used[i] = {0, 0};
-			for (int c = 0; c < ncol + (any_serial ? 1 : 0); c++) {
-				int cc = c < ncol ? c : CP_MAXC;
-				for (size_t x = 0; x < isl_edges.size(); x++) {
-					if (col[x] != cc) continue;
-					emit(isl_edges[x]);
-					eoff.push_back((int)(last_rows.size() / 2));
-				}
-				coff.push_back((int)eoff.size() - 1);
-			}
-			icoff.push_back((int)coff.size() - 1);
-			serial.push_back(any_serial ? 1 : 0);
-		}
-		if (colored) {
-			// island-local indices of the dynamic bodies, in order of first
-			// appearance; non-dynamic bodies get the zero column (set below)
-			size_t bstart = bod.size();
-			std::vector<std::pair<int*, int>> fix;  // (slot, global body) of non-dynamic ends
-			for (size_t r = start; r < last_rows.size() / 2; r++) {
-				int k = (int)last_rows[2 * r], kind = (int)last_rows[2 * r + 1];
-				int ends[2];
-				int* slot;
-				if (kind == 2) {
-					ends[0] = j_a[k]; ends[1] = j_b[k];
-					slot = &jloc[2 * k];
-				} else {
-					if (kind == 1) continue;  // the normal row covered the contact
-					ends[0] = (int)(key[k] >> 32); ends[1] = (int)(key[k] & 0xFFFFFFFFu);
-					slot = &cloc[2 * k];
-				}
-				for (int h = 0; h < 2; h++) {
-					int i = ends[h];
-					if (b_move[i] != 0) {
-						slot[h] = -1;
-						continue;
-					}
-					if (loc_stamp[i] != (int)last_offs.size()) {
-						loc_stamp[i] = (int)last_offs.size();
-						loc_of[i] = (int)(bod.size() - bstart);
-						bod.push_back(i);
-					}
-					slot[h] = loc_of[i];
-				}
-			}
-			const int nbi = (int)(bod.size() - bstart);
-			for (size_t r = start; r < last_rows.size() / 2; r++) {
-				int k = (int)last_rows[2 * r], kind = (int)last_rows[2 * r + 1];
-				if (kind == 1) continue;
-				int* slot = kind == 2 ? &jloc[2 * k] : &cloc[2 * k];
-				for (int h = 0; h < 2; h++)
-					if (slot[h] < 0) slot[h] = nbi;
-			}
-			boff.push_back((int)bod.size());
-			max_isl_bodies = std::max(max_isl_bodies, nbi + 1);
-		}
-		// systems in order of first appearance; linearization-error columns
-		const int isl_id = (int)last_offs.size();
-		slots.clear();
-		for (size_t r = start; r < last_rows.size() / 2; r++) {
-			if (last_rows[2 * r + 1] != 2) continue;
-			int j = (int)last_rows[2 * r], q = sys_of[j];
-			if (q >= 0 && sys_seen[q] != isl_id) {
-				sys_seen[q] = isl_id;
-				last_isl_sys.push_back(q);
-			}
-			int ends[2] = {j_a[j], j_b[j]};
-			for (int h = 0; h < 2; h++) {
-				int i = ends[h], cidx = i;
-				if (b_move[i] != 0) {
-					auto it = std::find_if(slots.begin(), slots.end(), [&](const std::pair<int, int>& p) { return p.first == i; });
-					if (it == slots.end()) {
-						slots.push_back({i, nl++});
-						cidx = nl - 1;
-					} else cidx = it->second;
-				}
-				jlin[2 * j + h] = cidx;
-			}
-		}
-		last_offs.push_back((int64_t)last_rows.size() / 2);
-		last_isl_sys_off.push_back((int)last_isl_sys.size());
+	const int ne = nedges, ng = ib_ng, N = ne + ng;
+	const size_t NB = (size_t)nb;
+	ib_keys.reserve(N); ib_keys2.reserve(N); ib_vals.reserve(N); ib_vals2.reserve(N);
+	ib_head.reserve(N + 1); ib_midx.reserve(N + 1); ib_me.reserve(8 * (size_t)N);
+	ib_adjk.reserve(2 * (size_t)N); ib_adjk2.reserve(2 * (size_t)N); ib_adjv.reserve(2 * (size_t)N); ib_adjv2.reserve(2 * (size_t)N);
+	ib_aoff.reserve(NB + 1); ib_parent.reserve(NB); ib_comp.reserve(NB);
+	ib_acc.reserve(NB); ib_in.reserve(NB + 1); ib_out.reserve(NB + 1); ib_info.reserve(2);
+	DArr<int>* per_isl[] = {&isl_root, &isl_edge_off, &isl_body_off, &isl_body_cnt, &isl_stack_off, &isl_joint_off, &isl_icoff, &isl_ncol,
+	                        &isl_serial, &isl_nbod, &isl_sys_cnt};
+	for (auto* a : per_isl) a->reserve(NB + 1);
+	isl_off.reserve(NB + 1);
+	ib_stack.reserve(2 * (size_t)N + NB); ib_vlist.reserve(NB); ib_elist.reserve(N); ib_ecol.reserve(N); ib_eorder.reserve(N);
+	isl_eoff.reserve(N + 1); isl_coff.reserve((size_t)N + NB + 1); isl_bod.reserve(NB);
+	isl_rows.reserve(2 * ((size_t)nj + 2 * (size_t)n)); isl_cloc.reserve(2 * (size_t)n); isl_jloc.reserve(2 * (size_t)nj);
+	isl_sys.reserve(nj); ib_slots.reserve(2 * (size_t)nj); isl_jlin.reserve(2 * (size_t)nj);
+	// walk scratch that every walk leaves as it found it
+	auto keep_filled = [&](auto& a, size_t m, int byte) {
+		if (a.p && m <= a.cap) return;
+		a.reserve(m);
+		HIPCHK(hipMemsetAsync(a.p, byte, a.cap * sizeof(*a.p), stream));
+	};
+	keep_filled(ib_vis, NB, 0);
+	keep_filled(ib_loc, NB, 0xFF);
+	keep_filled(ib_col_used, 2 * NB, 0);
+	keep_filled(ib_sys_seen, hs_lay.size() + 1, 0);
+	size_t bytes = 0;
+	if (N) {
+		launch_ib_keys(e_key.p, ne, ib_jgkey.p, ng, ib_keys.p, ib_vals.p, stream);
+		HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, ib_keys.p, ib_keys2.p, ib_vals.p, ib_vals2.p, N, 0, 64, stream));
+		tmp.reserve(bytes + 16);
+		HIPCHK(hipcub::DeviceRadixSort::SortPairs(tmp.p, bytes, ib_keys.p, ib_keys2.p, ib_vals.p, ib_vals2.p, N, 0, 64, stream));
 	}
-	lin_nl = nl;
-	std::vector<int> rows(last_rows.begin(), last_rows.end()), offs(last_offs.begin(), last_offs.end());
-	up(isl_rows, rows);
-	up(isl_off, offs);
-	up(isl_sys, last_isl_sys);
-	up(isl_sys_off, last_isl_sys_off);
-	up(isl_jlin, jlin);
-	if (colored) {
-		up(isl_eoff, eoff);
-		up(isl_coff, coff);
-		up(isl_icoff, icoff);
-		up(isl_serial, serial);
-		if (bod.empty()) bod.push_back(0);
-		up(isl_bod, bod);
-		up(isl_boff, boff);
-		up(isl_cloc, cloc);
-		up(isl_jloc, jloc);
-		isl_lds_bodies = std::min(max_isl_bodies, ISL_LDS_BODIES);
+	launch_ib_heads(ib_keys2.p, N, ib_head.p, stream);
+	scan(ib_head.p, ib_midx.p, N + 1);
+	if (N) {
+		launch_ib_edges(ib_keys2.p, ib_vals2.p, N, ib_head.p, ib_midx.p, ne, e_first.p, e_cnt.p, cb[cur].c.live, ib_jgoff.p, ib_me.p,
+		                ib_adjk.p, ib_adjv.p, stream);
+		bytes = 0;
+		HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, ib_adjk.p, ib_adjk2.p, ib_adjv.p, ib_adjv2.p, 2 * N, 0, 64, stream));
+		tmp.reserve(bytes + 16);
+		HIPCHK(hipcub::DeviceRadixSort::SortPairs(tmp.p, bytes, ib_adjk.p, ib_adjk2.p, ib_adjv.p, ib_adjv2.p, 2 * N, 0, 64, stream));
 	}
+	launch_ib_aoff(ib_adjk2.p, 2 * N, nb, ib_aoff.p, stream);
+	const SleepDev S = sleep_dev();
+	launch_ib_components(ib_me.p, ib_midx.p, N, nb, movement.p, ib_rank.p, ib_aoff.p, ib_parent.p, ib_comp.p, ib_acc.p, S, stream);
+	launch_ib_flags(nb, ib_rord.p, ib_comp.p, ib_acc.p, S.enabled, ib_in.p, stream);
+	bytes = 0;
+	const IslCnt zero{0, 0, 0, 0, 0, 0, 0};
+	HIPCHK(hipcub::DeviceScan::ExclusiveScan(nullptr, bytes, ib_in.p, ib_out.p, IslCntSum(), zero, nb + 1, stream));
+	tmp.reserve(bytes + 16);
+	HIPCHK(hipcub::DeviceScan::ExclusiveScan(tmp.p, bytes, ib_in.p, ib_out.p, IslCntSum(), zero, nb + 1, stream));
+	HIPCHK(hipMemsetAsync(ib_info.p, 0, 2 * sizeof(int), stream));
+	const IslTables T = isl_tables();
+	launch_ib_tables(nb, ib_rord.p, ib_in.p, ib_out.p, T, ib_info.p, ib_comp.p, ib_acc.p, S, vel.p, angv.p, stream);
+	int info[2];
+	HIPCHK(hipMemcpyAsync(info, ib_info.p, 2 * sizeof(int), hipMemcpyDeviceToHost, stream));
+	HIPCHK(hipStreamSynchronize(stream));
+	isl_n = info[0];
+	IslWalk W{};
+	W.T = T;
+	W.nb = nb;
+	W.hol = cfg.hol_iters > 0 && !hs_lay.empty();
+	W.movement = movement.p; W.asleep = S.enabled ? S.asleep : nullptr; W.aoff = ib_aoff.p; W.adjv = ib_adjv2.p; W.me = ib_me.p;
+	W.jord = ib_jord.p; W.live = cb[cur].c.live; W.ja = dj_a.p; W.jb = dj_b.p; W.jsys = dj_sys.p; W.adjk = ib_adjk2.p;
+	W.ckey = cb[cur].c.key; W.stack = ib_stack.p; W.vlist = ib_vlist.p; W.elist = ib_elist.p; W.vis = ib_vis.p; W.rows = isl_rows.p;
+	W.ecol = ib_ecol.p; W.coff = isl_coff.p; W.eorder = ib_eorder.p; W.icoff = isl_icoff.p; W.ncol = isl_ncol.p; W.serial = isl_serial.p;
+	W.eoff = isl_eoff.p; W.bod = isl_bod.p; W.nbod = isl_nbod.p; W.loc = ib_loc.p; W.cloc = isl_cloc.p; W.jloc = isl_jloc.p;
+	W.sys_seen = ib_sys_seen.p; W.sys = isl_sys.p; W.slots = ib_slots.p; W.jlin = isl_jlin.p; W.sys_cnt = isl_sys_cnt.p;
+	W.col_used = ib_col_used.p;
+	launch_ib_walk(W, isl_n, colored, stream);
+	lin_nl = nb + 2 * nj;
+	isl_lds_bodies = std::min(info[1] + 1, ISL_LDS_BODIES);
+	islands_on_device = true;
+}
+
+SleepDev cp_world::sleep_dev() {
+	SleepDev S{};
+	S.enabled = 0;
+	return S;
+}
+
+IslTables cp_world::isl_tables() {
+	return IslTables{isl_root.p, isl_off.p, isl_edge_off.p, isl_body_off.p, isl_body_cnt.p, isl_stack_off.p, isl_joint_off.p};
+}
+
+// The last island schedule as (index, kind) rows and island offsets, read
+// back from the device.
+void cp_world::fetch_islands() {
+	if (!islands_on_device) return;
+	std::vector<int> offs, rows;
+	down(offs, isl_off.p, (size_t)isl_n + 1);
+	down(rows, isl_rows.p, 2 * (size_t)offs[isl_n]);
+	last_rows.assign(rows.begin(), rows.end());
+	last_offs.assign(offs.begin(), offs.end());
+	islands_on_device = false;
 }
 
 // The colour schedule as one sequential row list per scene (colours in order,
@@ -1419,6 +1342,7 @@ void cp_world::record_colored_schedule() {
 	down(ef, e_first.p, nedges);
 	down(ecnt, e_cnt.p, nedges);
 	down(live, cb[cur].c.live, ncontacts);
+	islands_on_device = false;
 	last_rows.clear();
 	last_offs.assign(1, 0);
 	// first cache index of each scene's contacts (keys are lo << 32 | hi, sorted)
@@ -1590,7 +1514,7 @@ void cp_world::step() {
 	int nisl = 0;
 	if (ordered) {
 		schedule_islands(cfg.pgs_mode == CP_PGS_ISLAND_COLORED);
-		nisl = (int)last_offs.size() - 1;
+		nisl = isl_n;
 		if (hol) lin.reserve(6 * (size_t)lin_nl);
 	} else {
 		color_off.reserve((size_t)nscenes * (CP_MAXC + 2));
@@ -1625,10 +1549,11 @@ void cp_world::step() {
 			HIPCHK(hipMemsetAsync(applied.p, 0, (size_t)nb * sizeof(int), stream));
 			if (hol && s == 0) HIPCHK(hipMemsetAsync(lin.p, 0, 6 * (size_t)lin_nl * sizeof(double), stream));
 			const bool icol = cfg.pgs_mode == CP_PGS_ISLAND_COLORED;
-			IslandSched S{nisl, lin_nl, isl_rows.p, isl_off.p, isl_sys.p, isl_sys_off.p, isl_jlin.p,
-			              isl_eoff.p, isl_coff.p, isl_icoff.p, isl_serial.p,
-			              icol ? isl_bod.p : nullptr, icol ? isl_boff.p : nullptr, icol ? isl_cloc.p : nullptr,
-			              icol ? isl_jloc.p : nullptr, icol ? isl_lds_bodies : 0};
+			IslandSched S{nisl, lin_nl, isl_rows.p, isl_off.p, isl_sys.p, isl_joint_off.p, isl_sys_cnt.p, isl_jlin.p,
+			              isl_eoff.p, isl_coff.p, isl_icoff.p, isl_ncol.p, isl_serial.p,
+			              icol ? isl_bod.p : nullptr, icol ? isl_body_off.p : nullptr, icol ? isl_nbod.p : nullptr,
+			              icol ? isl_cloc.p : nullptr, icol ? isl_jloc.p : nullptr, icol ? isl_lds_bodies : 0,
+			              ib_vlist.p, isl_body_off.p, isl_body_cnt.p};
 			launch_pgs_islands(b, cb[cur].c, J, H, S, P, applied.p, lin.p, s == 0, cfg.pgs_mode == CP_PGS_ISLAND_COLORED, stream);
 		} else
 			colored_pgs(b, P);
@@ -2997,6 +2922,8 @@ int cp_narrowphase_multi(cp_world* w, int n, const int64_t* pairs2, int cap, int
 
 int cp_world_last_islands(cp_world* w, int cap_rows, int64_t* rows2, int cap_offs, int64_t* offs, int* n_rows, int* n_islands) {
 	if (!w || !n_rows || !n_islands) return CP_ERR_INVALID;
+	int st = guard(w, [&] { w->fetch_islands(); });
+	if (st != CP_OK) return st;
 	int nr = (int)w->last_rows.size() / 2, no = (int)w->last_offs.size();
 	*n_rows = nr;
 	*n_islands = no - 1;
