@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <deque>
 #include <limits>
 #include <map>
 #include <set>
@@ -239,7 +240,15 @@ struct Body {
 	double aabb[6];
 	V3 prev_com;
 	Q prev_q;
+	// sleeping (RigidBody.h:88-124): asleep, sleep_ready_counter,
+	// non_sleepy_tick_count, history of (vel, ang_vel), recievedWakingAction,
+	// sleep_disabled
+	bool asleep = false, wake_req = false, sleep_disabled = false;
+	double sleep_counter = 0;
+	int sleep_ticks = 0;
+	std::deque<std::pair<V3, V3>> history;
 	bool dynamic() const { return movement == 0; }
+	bool getAsleep() const { return asleep && movement == 0; }  // RigidBody.cpp:181-183
 	double invMass() const { return movement == 0 ? 1.0 / mass : 0; }       // RigidBody.cpp:162-164
 	M3 invTensor() const { return movement == 0 ? I_inv : M3(); }          // RigidBody.cpp:170-172
 	V3 getVel() const { return movement == 1 ? V3() : vel; }
@@ -1637,7 +1646,20 @@ struct World {
 	uint64_t seq = 0;
 	std::vector<int64_t> last_rows, last_offs;
 	std::vector<Contact> last_mid;
+	// sleeping parameters (PhysicsEngine.h:218-224)
+	int sleeping = 0;
+	double sleep_delay = 0.5, vel_sleep_coeff = 0.1, ang_vel_eps = 0.1, accel_sleep_coeff = 0.044;
+	int non_sleepy_tick_threshold = 3;
 };
+
+// RigidBody::wake (RigidBody.cpp:331-338).
+static void wakeBody(Body& b) {
+	if (!b.asleep) return;
+	b.asleep = false;
+	b.sleep_counter = 0;
+	b.sleep_ticks = 0;
+	b.history.clear();
+}
 
 static void applyCFMInv1(double itv, double cfm, double* inv) { *inv = 1.0 / (itv * (1 + cfm)); }
 
@@ -2093,7 +2115,8 @@ struct DV {
 static void addv(double* acc, const double* d) { for (int i = 0; i < 6; i++) acc[i] += d[i]; }
 
 // Joint rows are (nullptr, joint index); contact rows (contact, 0 normal / 1 friction).
-static void pgsIsland(World& W, const std::vector<std::pair<Contact*, int>>& rows, const std::vector<int>& systems, int hol_iters) {
+static void pgsIsland(World& W, const std::vector<std::pair<Contact*, int>>& rows, const std::vector<int>& systems, int hol_iters,
+                      const std::vector<int>& island_bodies) {
 	std::map<int, DV> dv;
 	for (auto& r : rows) {
 		if (r.first) { dv[r.first->b1]; dv[r.first->b2]; }
@@ -2289,6 +2312,29 @@ static void pgsIsland(World& W, const std::vector<std::pair<Contact*, int>>& row
 		b.pvel += V3(kv.second.p[0], kv.second.p[1], kv.second.p[2]);
 		b.pang += V3(kv.second.p[3], kv.second.p[4], kv.second.p[5]);
 	}
+	// An asleep body whose new velocity exceeds wake_vel wakes its island
+	// (wakeupIsland, :1285-1290); otherwise asleep bodies keep a zero
+	// velocity.  The reference visits the bodies in the order of an
+	// unordered_map keyed by address (ConstraintSolver.cpp:91, :199-212), so
+	// an asleep body it visits before the waking one is zeroed first; this
+	// restatement takes the order in which every waking body comes first.
+	V3 g(W.cfg.gravity[0], W.cfg.gravity[1], W.cfg.gravity[2]);
+	double wake_vel = 0.25 * W.vel_sleep_coeff * g.mag();
+	bool wake = false;
+	for (auto& kv : dv) {
+		const Body& b = W.bodies[kv.first];
+		if (b.getAsleep() && (b.vel.mag() > wake_vel || b.ang_vel.mag() > wake_vel)) wake = true;
+	}
+	if (wake) {
+		for (int i : island_bodies) wakeBody(W.bodies[i]);
+	} else {
+		for (auto& kv : dv) {
+			Body& b = W.bodies[kv.first];
+			if (!b.getAsleep()) continue;
+			b.vel = V3();
+			b.ang_vel = V3();
+		}
+	}
 }
 
 // ---------------------------------------------------------------- integration
@@ -2458,20 +2504,92 @@ static void sortCache(World& W) {
 	});
 }
 
-static void step(World& W, const int64_t* node_order, const int64_t* sched_rows = nullptr, const int64_t* sched_offs = nullptr,
-                 int sched_n = 0) {
+// PhysicsEngine::bodySleepy (PhysicsEngine.cpp:1221-1254) over the movement
+// history, oldest entry first.
+static bool bodySleepy(const World& W, const Body& b) {
+	if (b.sleep_disabled) return false;
+	const int n = (int)b.history.size();
+	const double dt = W.cfg.dt;
+	if (n == 0 || 2 * n < (W.sleep_delay / dt)) return false;
+	V3 av, aw;
+	for (const auto& h : b.history) {
+		av += h.first;
+		aw += h.second;
+	}
+	av = V3(av.x / n, av.y / n, av.z / n);
+	aw = V3(aw.x / n, aw.y / n, aw.z / n);
+	V3 acc = (b.history.back().first - b.history.front().first) / (dt * (n - 1));
+	V3 aacc = (b.history.back().second - b.history.front().second) / (dt * (n - 1));
+	V3 g(W.cfg.gravity[0], W.cfg.gravity[1], W.cfg.gravity[2]);
+	double vel_eps = W.vel_sleep_coeff * g.mag();
+	double accel_eps = W.accel_sleep_coeff * g.mag();
+	double ang_accel_eps = W.accel_sleep_coeff * 2.5;
+	return av.mag() <= vel_eps && aw.mag() <= W.ang_vel_eps && acc.mag() <= accel_eps && aacc.mag() <= ang_accel_eps;
+}
+
+// PhysicsEngine::wakeupIsland (PhysicsEngine.cpp:1285-1290) through
+// bfsVisitAll (:1263-1283) over the constraint graph as it stands (contact
+// cache pairs and joint pairs): every dynamic body reachable through dynamic
+// bodies from body i wakes.
+static void wakeIsland(World& W, int i) {
+	const int N = (int)W.bodies.size();
+	std::vector<std::vector<int>> adj(N);
+	for (const Contact& c : W.cache) {
+		adj[c.b1].push_back(c.b2);
+		adj[c.b2].push_back(c.b1);
+	}
+	for (const Joint& j : W.joints) {
+		adj[j.a].push_back(j.b);
+		adj[j.b].push_back(j.a);
+	}
+	std::vector<char> vis(N, 0);
+	std::vector<int> st{i};
+	while (!st.empty()) {
+		int c = st.back();
+		st.pop_back();
+		if (vis[c]) continue;
+		vis[c] = 1;
+		if (W.bodies[c].dynamic()) wakeBody(W.bodies[c]);
+		for (int o : adj[c])
+			if (W.bodies[o].dynamic() && !vis[o]) st.push_back(o);
+	}
+}
+
+// Returns 0, or -1 (nothing stepped) when a schedule's groups do not match
+// the islands of a sleeping world.
+static int step(World& W, const int64_t* node_order, const int64_t* sched_rows = nullptr, const int64_t* sched_offs = nullptr,
+                int sched_n = 0) {
 	const int N = (int)W.bodies.size();
 	double dt = W.cfg.dt;
 	V3 g(W.cfg.gravity[0], W.cfg.gravity[1], W.cfg.gravity[2]);
-	// PhysicsEngine.cpp:65-87 (gravity once per step)
-	for (Body& b : W.bodies)
-		if (b.dynamic()) b.vel += g * dt;
+	// PhysicsEngine.cpp:65-87: waking actions, movement history and sleep
+	// counters, gravity once per step
+	const int hist_len = static_cast<int>(W.sleep_delay / dt);
+	for (int i = 0; i < N; i++) {
+		Body& b = W.bodies[i];
+		if (b.wake_req) wakeIsland(W, i);
+		b.wake_req = false;
+		if (!(b.dynamic() && !b.getAsleep())) continue;
+		b.history.push_back({b.vel, b.ang_vel});  // RigidBody::recordMovementState (RigidBody.cpp:347-352)
+		while ((int)b.history.size() > hist_len) b.history.pop_front();
+		if (bodySleepy(W, b)) {
+			b.sleep_counter += dt;
+		} else {
+			b.sleep_ticks++;
+			if (b.sleep_ticks >= W.non_sleepy_tick_threshold) {
+				b.sleep_counter = 0.0;
+				b.sleep_ticks = 0;
+			}
+		}
+		b.vel += g * dt;
+	}
 	// broadphase + narrowphase (PhysicsEngine.cpp:102-335)
 	auto pairs = broadphase(W);
 	for (auto& pr : pairs) {
 		const Body& A = W.bodies[pr.first];
 		const Body& B = W.bodies[pr.second];
-		if (!(A.dynamic() || B.dynamic())) continue;
+		// collision_possible (PhysicsEngine.cpp:174): a dynamic body, not both asleep
+		if (!(A.dynamic() || B.dynamic()) || (A.getAsleep() && B.getAsleep())) continue;
 		if (W.nocol.count({std::min(pr.first, pr.second), std::max(pr.first, pr.second)})) continue;  // collisionAllowed (:570-572)
 		if (A.type == 5 || B.type == 5) {
 			for (const Manifold& m : meshPair(W, pr.first, pr.second))
@@ -2521,28 +2639,10 @@ static void step(World& W, const int64_t* node_order, const int64_t* sched_rows 
 			return W.bodies[x.first].id < W.bodies[y.first].id;
 		});
 	std::vector<std::vector<std::pair<Contact*, int>>> islands;
-	std::vector<std::vector<int>> isl_sys;
+	std::vector<std::vector<int>> isl_sys, isl_bodies;
 	W.last_rows.clear();
 	W.last_offs.assign(1, 0);
-	if (sched_rows) {
-		// explicit sweep order over the post-narrowphase cache (one row list per group)
-		for (int g = 0; g < sched_n; g++) {
-			std::vector<std::pair<Contact*, int>> rows;
-			for (int64_t r = sched_offs[g]; r < sched_offs[g + 1]; r++) {
-				if (sched_rows[2 * r + 1] == 2) rows.push_back({nullptr, (int)sched_rows[2 * r]});
-				else rows.push_back({&W.cache[sched_rows[2 * r]], (int)sched_rows[2 * r + 1]});
-			}
-			if (!rows.empty()) {
-				islands.push_back(rows);
-				std::vector<int> sys;
-				for (auto& r : rows)
-					if (!r.first && W.joints[r.second].sys >= 0 &&
-					    std::find(sys.begin(), sys.end(), W.joints[r.second].sys) == sys.end())
-						sys.push_back(W.joints[r.second].sys);
-				isl_sys.push_back(sys);
-			}
-		}
-	} else {
+	{
 		std::vector<char> visited(N, 0);
 		std::vector<int> roots;
 		if (node_order) for (int k = 0; k < N; k++) roots.push_back((int)node_order[k] - 1);
@@ -2550,7 +2650,9 @@ static void step(World& W, const int64_t* node_order, const int64_t* sched_rows 
 		for (int root : roots) {
 			if (!W.bodies[root].dynamic() || visited[root]) continue;
 			std::vector<std::pair<Contact*, int>> rows;
-			std::vector<int> sys;
+			std::vector<int> sys, bodies;
+			std::vector<int64_t> lrows;
+			bool all_ready = true;
 			std::vector<int> st{root};
 			while (!st.empty()) {
 				int cur = st.back();
@@ -2558,8 +2660,11 @@ static void step(World& W, const int64_t* node_order, const int64_t* sched_rows 
 				if (visited[cur]) continue;
 				visited[cur] = 1;
 				if (W.bodies[cur].dynamic()) {
+					const Body& cb = W.bodies[cur];
+					bodies.push_back(cur);
 					for (auto& e : adj[cur]) {
-						if (visited[e.first]) continue;
+						const Body& nb = W.bodies[e.first];
+						if ((cb.getAsleep() && (nb.getAsleep() || nb.movement == 1)) || visited[e.first]) continue;
 						int lo = std::min(cur, e.first), hi = std::max(cur, e.first);
 						if (hol > 0)
 							for (int ji : e.second->joints) {
@@ -2568,30 +2673,68 @@ static void step(World& W, const int64_t* node_order, const int64_t* sched_rows 
 							}
 						for (int ji : e.second->joints) {
 							rows.push_back({nullptr, ji});
-							W.last_rows.push_back(ji);
-							W.last_rows.push_back(2);
+							lrows.push_back(ji);
+							lrows.push_back(2);
 						}
 						if (e.second->first < 0) continue;
 						for (size_t k = e.second->first; k < W.cache.size() && W.cache[k].b1 == lo && W.cache[k].b2 == hi; k++) {
 							if (!W.cache[k].live) continue;
 							rows.push_back({&W.cache[k], 0});
 							rows.push_back({&W.cache[k], 1});
-							W.last_rows.push_back((int64_t)k);
-							W.last_rows.push_back(0);
-							W.last_rows.push_back((int64_t)k);
-							W.last_rows.push_back(1);
+							lrows.insert(lrows.end(), {(int64_t)k, 0, (int64_t)k, 1});
 						}
 					}
+					// readyToSleep (PhysicsEngine.cpp:1256-1258)
+					if (!(cb.sleep_counter >= W.sleep_delay)) all_ready = false;
 				}
 				for (auto& e : adj[cur])
 					if (W.bodies[e.first].dynamic() && !visited[e.first]) st.push_back(e.first);
 			}
-			if (!rows.empty()) {
+			if (W.sleeping && all_ready) {
+				// RigidBody::sleep (RigidBody.cpp:322-329)
+				for (int i : bodies) {
+					Body& b = W.bodies[i];
+					b.asleep = true;
+					b.vel = V3();
+					b.ang_vel = V3();
+				}
+			} else if (!rows.empty()) {
 				islands.push_back(rows);
 				isl_sys.push_back(sys);
+				isl_bodies.push_back(bodies);
+				W.last_rows.insert(W.last_rows.end(), lrows.begin(), lrows.end());
 				W.last_offs.push_back((int64_t)W.last_rows.size() / 2);
 			}
 		}
+	}
+	if (sched_rows) {
+		// explicit sweep order over the post-narrowphase cache (one row list per
+		// group); with sleeping the groups are the islands above, in order
+		if (W.sleeping && sched_n != (int)islands.size()) return -1;
+		std::vector<std::vector<std::pair<Contact*, int>>> g_rows;
+		std::vector<std::vector<int>> g_sys, g_bodies;
+		W.last_rows.clear();
+		W.last_offs.assign(1, 0);
+		for (int g = 0; g < sched_n; g++) {
+			std::vector<std::pair<Contact*, int>> rows;
+			for (int64_t r = sched_offs[g]; r < sched_offs[g + 1]; r++) {
+				if (sched_rows[2 * r + 1] == 2) rows.push_back({nullptr, (int)sched_rows[2 * r]});
+				else rows.push_back({&W.cache[sched_rows[2 * r]], (int)sched_rows[2 * r + 1]});
+			}
+			if (!rows.empty()) {
+				g_rows.push_back(rows);
+				std::vector<int> sys;
+				for (auto& r : rows)
+					if (!r.first && W.joints[r.second].sys >= 0 &&
+					    std::find(sys.begin(), sys.end(), W.joints[r.second].sys) == sys.end())
+						sys.push_back(W.joints[r.second].sys);
+				g_sys.push_back(sys);
+				g_bodies.push_back(W.sleeping ? isl_bodies[g] : std::vector<int>());
+			}
+		}
+		islands.swap(g_rows);
+		isl_sys.swap(g_sys);
+		isl_bodies.swap(g_bodies);
 	}
 	// substeps (PhysicsEngine.cpp:368-420)
 	int S = W.cfg.substeps;
@@ -2610,14 +2753,19 @@ static void step(World& W, const int64_t* node_order, const int64_t* sched_rows 
 			if (first && j.sys >= 0 && hol > 0) continue;
 			buildJoint(W, j, sdt, true);
 		}
-		if (first) {
-			// removeExpiredContactConstraints (PhysicsEngine.cpp:1434-1446); island pointers stay valid
-			// because expired contacts are never live and the vector is compacted only after the solve.
-		}
-		for (size_t i = 0; i < islands.size(); i++) pgsIsland(W, islands[i], isl_sys[i], first ? hol : 0);
+		// removeExpiredContactConstraints (PhysicsEngine.cpp:1434-1446) at the
+		// first substep, before the solve: Contact::isExpired
+		// (PersistentConstraint.h:74) spares contacts of asleep bodies.  Island
+		// pointers stay valid because expired contacts are never live; the
+		// vector is compacted after this substep.
+		std::vector<char> expired;
+		if (first)
+			for (const Contact& c : W.cache)
+				expired.push_back(!W.bodies[c.b1].getAsleep() && !W.bodies[c.b2].getAsleep() && c.life < 0);
+		for (size_t i = 0; i < islands.size(); i++) pgsIsland(W, islands[i], isl_sys[i], first ? hol : 0, isl_bodies[i]);
 		// update_positions (PhysicsEngine.cpp:390-410)
 		for (Body& b : W.bodies) {
-			if (!b.dynamic()) continue;
+			if (!b.dynamic() || b.getAsleep()) continue;
 			b.prev_com = b.com;
 			b.prev_q = b.q;
 			b.com += (b.vel + b.pvel) * sdt;
@@ -2632,14 +2780,14 @@ static void step(World& W, const int64_t* node_order, const int64_t* sched_rows 
 		}
 		if (first) {
 			std::vector<Contact> keep;
-			for (Contact& c : W.cache)
-				if (!(c.life < 0)) keep.push_back(c);
+			for (size_t k = 0; k < W.cache.size(); k++)
+				if (!expired[k]) keep.push_back(W.cache[k]);
 			// re-point island rows is unnecessary after this substep only if no later substep
 			// uses them; rebuild the island vectors against the compacted cache instead.
 			std::map<const Contact*, size_t> remap;
 			size_t k = 0;
-			for (Contact& c : W.cache) {
-				if (!(c.life < 0)) remap[&c] = k++;
+			for (size_t x = 0; x < W.cache.size(); x++) {
+				if (!expired[x]) remap[&W.cache[x]] = k++;
 			}
 			for (auto& isl : islands)
 				for (auto& r : isl)
@@ -2652,7 +2800,7 @@ static void step(World& W, const int64_t* node_order, const int64_t* sched_rows 
 	}
 	// update_geometry (PhysicsEngine.cpp:424-438)
 	for (Body& b : W.bodies) {
-		if (!b.dynamic()) continue;
+		if (!b.dynamic() || b.getAsleep()) continue;
 		if (b.com != b.prev_com || b.q != b.prev_q) {
 			b.q = b.q.normalize();
 			worldGeometry(b);
@@ -2665,6 +2813,7 @@ static void step(World& W, const int64_t* node_order, const int64_t* sched_rows 
 			for (int ji : hs.og) buildJoint(W, W.joints[ji], dt / W.cfg.substeps, true);
 			hsFactor(W, hs, W.cfg.cfm);
 		}
+	return 0;
 }
 
 }  // namespace
@@ -3263,7 +3412,43 @@ int orc_narrowphase(const orc_world* w, int id_lo, int id_hi, double* head, doub
 
 void orc_step(orc_world* w, const int64_t* node_order) { step(*w, node_order); }
 
-void orc_step_scheduled(orc_world* w, int n_groups, const int64_t* rows, const int64_t* offs) { step(*w, nullptr, rows, offs, n_groups); }
+int orc_step_scheduled(orc_world* w, int n_groups, const int64_t* rows, const int64_t* offs) { return step(*w, nullptr, rows, offs, n_groups); }
+
+void orc_set_sleeping(orc_world* w, int enabled, const double* params4, int tick_threshold) {
+	w->sleeping = enabled;
+	if (params4) {
+		w->sleep_delay = params4[0];
+		w->vel_sleep_coeff = params4[1];
+		w->ang_vel_eps = params4[2];
+		w->accel_sleep_coeff = params4[3];
+	}
+	if (tick_threshold > 0) w->non_sleepy_tick_threshold = tick_threshold;
+}
+
+void orc_get_sleep_state(const orc_world* w, double* out4) {
+	for (size_t i = 0; i < w->bodies.size(); i++) {
+		const Body& b = w->bodies[i];
+		out4[4 * i] = b.asleep ? 1.0 : 0.0;
+		out4[4 * i + 1] = b.sleep_counter;
+		out4[4 * i + 2] = (double)b.sleep_ticks;
+		out4[4 * i + 3] = (double)b.history.size();
+	}
+}
+
+void orc_wake_bodies(orc_world* w, int n, const int64_t* idx) {
+	for (int k = 0; k < n; k++) {
+		Body& b = w->bodies[idx[k]];
+		// RigidBody::alertWakingAction (RigidBody.cpp:340-345)
+		b.wake_req = true;
+		b.sleep_counter = 0;
+		b.sleep_ticks = 0;
+		b.history.clear();
+	}
+}
+
+void orc_set_sleep_disabled(orc_world* w, int n, const int64_t* idx, int disabled) {
+	for (int k = 0; k < n; k++) w->bodies[idx[k]].sleep_disabled = disabled != 0;
+}
 
 int orc_last_islands(const orc_world* w, int64_t* rows, int64_t* offs, int cap) {
 	int n = (int)w->last_rows.size() / 2;

@@ -129,15 +129,31 @@ int orc_raycast(const orc_world* w, const double* o3, const double* d3, int b0, 
 // collision); head = normal 3, max_pen; pts rows = pos 3, pen; magic rows = cid, bid.
 int orc_narrowphase(const orc_world* w, int id_lo, int id_hi, double* head, double* pts, uint64_t* magic);
 
-// One full step (PhysicsEngine::timeStep for the configuration used here:
-// no sleeping, no actions, contacts only).  node_order: body ids in the order
-// the island search visits roots (PhysicsEngine.cpp:1540); NULL = ascending id.
+// One full step (PhysicsEngine::timeStep without collision actions).
+// node_order: body ids in the order the island search visits roots
+// (PhysicsEngine.cpp:1540); NULL = ascending id.
 void orc_step(orc_world* w, const int64_t* node_order);
 
 // One full step with an explicit sweep schedule: n_groups row lists
 // (rows[2*r] = index into the post-narrowphase cache, rows[2*r+1] = 0 normal /
-// 1 friction; offs[n_groups+1]), solved as PGS_solve solves one island each.
-void orc_step_scheduled(orc_world* w, int n_groups, const int64_t* rows, const int64_t* offs);
+// 1 friction, 2 joint; offs[n_groups+1]), solved as PGS_solve solves one
+// island each.  With sleeping on, the groups must be the step's islands in
+// order (their sleep decisions and wake lists come from the island search);
+// returns -1 without stepping when the count differs, else 0.
+int orc_step_scheduled(orc_world* w, int n_groups, const int64_t* rows, const int64_t* offs);
+
+// Sleeping (PhysicsEngine::setSleepingEnabled / setSleepParameters,
+// PhysicsEngine.cpp:1138-1190; defaults PhysicsEngine.h:218-224): enabled,
+// params4 = sleep_delay, vel_sleep_coeff, ang_vel_eps, accel_sleep_coeff
+// (NULL keeps them), tick_threshold = non_sleepy_tick_threshold (<= 0 keeps).
+void orc_set_sleeping(orc_world* w, int enabled, const double* params4, int tick_threshold);
+// Per body: asleep, sleep_ready_counter, non_sleepy_tick_count, history length.
+void orc_get_sleep_state(const orc_world* w, double* out4);
+// RigidBody::alertWakingAction (RigidBody.cpp:340-345) for the listed bodies:
+// their islands wake at the next step start.
+void orc_wake_bodies(orc_world* w, int n, const int64_t* idx);
+// RigidBody::setSleepDisabled for the listed bodies.
+void orc_set_sleep_disabled(orc_world* w, int n, const int64_t* idx, int disabled);
 
 // Island rows of the last step: (cache index into the post-narrowphase cache, kind 0 normal / 1 friction).
 int orc_last_islands(const orc_world* w, int64_t* rows, int64_t* offs, int cap);

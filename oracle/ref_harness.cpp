@@ -428,6 +428,7 @@ struct StepDump {
 
 static StepDump* g_cur = nullptr;
 static bool g_lite = false;  // omit the mid/post contact caches (large scenes)
+static bool g_sleep = false;  // record every body's sleep state after every step
 static PhysicsEngine* g_eng = nullptr;
 
 static void on_collision(RigidBody* b1, RigidBody* b2, const std::vector<phyz::Manifold>& ms) {
@@ -610,6 +611,7 @@ static void run_trace(PhysicsEngine& E, const SceneCfg& cfg, int steps, const st
 	o.meta("hol_iters", cfg.hol_iters);
 	o.meta("steps", steps);
 	std::vector<int64_t> rec_list;
+	std::vector<double> sleep_trace;
 	g_eng = &E;
 	E.registerCollisionAction(phyz::CollisionTarget::all(), phyz::CollisionTarget::all(), on_collision);
 	for (int k = 0; k < steps; k++) {
@@ -637,6 +639,10 @@ static void run_trace(PhysicsEngine& E, const SceneCfg& cfg, int steps, const st
 		}
 		E.timeStep();
 		g_cur = nullptr;
+		if (g_sleep)
+			for (RigidBody* b : E.bodies)
+				sleep_trace.insert(sleep_trace.end(), {b->asleep ? 1.0 : 0.0, b->sleep_ready_counter, (double)b->non_sleepy_tick_count,
+				                                       (double)b->history.size()});
 		if (!rec) continue;
 		rec_list.push_back(k);
 		broadphase_pairs(E, d.pairs);
@@ -683,6 +689,7 @@ static void run_trace(PhysicsEngine& E, const SceneCfg& cfg, int steps, const st
 		}
 	}
 	o.i64("recorded_steps", rec_list, {rec_list.size()});
+	if (g_sleep) o.f64("sleep_trace", sleep_trace, {(size_t)steps, N, 4});
 	{
 		std::vector<double> s;
 		body_state(E, s);
@@ -943,6 +950,56 @@ static int cmd_terrain(int argc, char** argv) {
 	return 0;
 }
 
+// sleep: sleeping enabled (the engine's default parameters, every body's
+// sleep_disabled pinned to false): a 2x2x2 cube pile and a lone cube settle
+// and fall asleep, a cube launched along the ground slides to rest and falls
+// asleep later, and a sphere dropped from high above lands on a sleeping
+// column of three cubes and wakes it.  Every step's sleep state is recorded
+// (asleep, sleep_ready_counter, non_sleepy_tick_count, history length).
+static int cmd_sleep(int argc, char** argv) {
+	int steps = (int)arg_d(argc, argv, "--steps", 420);
+	uint64_t seed = (uint64_t)arg_d(argc, argv, "--seed", 5150);
+	double drop = arg_d(argc, argv, "--drop", 24.0);
+	SceneCfg cfg;
+	PhysicsEngine E;
+	configure(E, cfg);
+	E.setSleepingEnabled(true);
+	phyz::Material m = c0_material();
+	auto pin = [](RigidBody* r) {
+		r->setNoCollision(false);
+		r->setSleepDisabled(false);
+	};
+	RigidBody* g = E.createRigidBody(phyz::ConvexUnionGeometry::box(mthz::Vec3(-100, -1, -100), 200, 1, 200, m), RigidBody::FIXED);
+	pin(g);
+	SplitMix64 rng(seed);
+	for (int j = 0; j < 2; j++)
+		for (int i = 0; i < 2; i++)
+			for (int k = 0; k < 2; k++) {
+				double cx = 1.1 * i + rng.uniform(-0.05, 0.05), cy = 0.6 + 1.1 * j + rng.uniform(-0.05, 0.05);
+				double cz = 1.1 * k + rng.uniform(-0.05, 0.05), yaw = rng.uniform(0.0, 2.0 * 3.14159265358979323846);
+				RigidBody* r = E.createRigidBody(phyz::ConvexUnionGeometry::box(mthz::Vec3(cx - 0.5, cy - 0.5, cz - 0.5), 1, 1, 1, m));
+				r->setOrientation(mthz::Quaternion(yaw, mthz::Vec3(0, 1, 0)));
+				pin(r);
+			}
+	RigidBody* lone = E.createRigidBody(phyz::ConvexUnionGeometry::box(mthz::Vec3(-5.5, 0.05, -0.5), 1, 1, 1, m));
+	pin(lone);
+	RigidBody* slider = E.createRigidBody(phyz::ConvexUnionGeometry::box(mthz::Vec3(-0.5, 0.0, -6.5), 1, 1, 1, m));
+	slider->setVel(mthz::Vec3(2.5, 0, 0));
+	pin(slider);
+	for (int j = 0; j < 3; j++) {
+		RigidBody* r = E.createRigidBody(phyz::ConvexUnionGeometry::box(mthz::Vec3(5.5, 0.02 + 1.02 * j, -0.5), 1, 1, 1, m));
+		pin(r);
+	}
+	RigidBody* ball = E.createRigidBody(phyz::ConvexUnionGeometry::sphere(mthz::Vec3(6.0 + 0.1, drop, 0.05), 0.5, m));
+	pin(ball);
+	Out o(arg_s(argc, argv, "--out", "out"));
+	o.meta("seed", (double)seed);
+	g_sleep = true;
+	run_trace(E, cfg, steps, arg_steps(argc, argv, "--record"), o);
+	o.finish();
+	return 0;
+}
+
 // hulls: irregular convex hulls (points and faces read from a text file: the
 // count, then per hull "n_pts n_faces", the points, and each face as its
 // vertex count and indices) built as createRigidBody(ConvexUnionGeometry(
@@ -1097,6 +1154,7 @@ int main(int argc, char** argv) {
 	if (cmd == "terrain") return cmd_terrain(argc, argv);
 	if (cmd == "time") return cmd_time(argc, argv);
 	if (cmd == "hulls") return cmd_hulls(argc, argv);
+	if (cmd == "sleep") return cmd_sleep(argc, argv);
 	fprintf(stderr, "unknown command %s\n", cmd.c_str());
 	return 2;
 }

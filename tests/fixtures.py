@@ -142,7 +142,12 @@ def oracle_lib():
         lib.orc_narrowphase.restype = ctypes.c_int
         lib.orc_narrowphase.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         lib.orc_step.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        lib.orc_step_scheduled.restype = ctypes.c_int
         lib.orc_step_scheduled.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+        lib.orc_set_sleeping.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int]
+        lib.orc_get_sleep_state.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        lib.orc_wake_bodies.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+        lib.orc_set_sleep_disabled.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int]
         lib.orc_add_joint.restype = ctypes.c_int
         lib.orc_add_joint.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
         lib.orc_set_joint_state.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
@@ -362,7 +367,22 @@ class Oracle:
     def step_scheduled(self, rows, offs):
         r = _c(rows, np.int64)
         o = _c(offs, np.int64)
-        self.lib.orc_step_scheduled(self.w, len(o) - 1, r.ctypes.data, o.ctypes.data)
+        if self.lib.orc_step_scheduled(self.w, len(o) - 1, r.ctypes.data, o.ctypes.data) != 0:
+            raise RuntimeError("schedule groups do not match the oracle's islands")
+
+    def set_sleeping(self, enabled=True, params=None, tick_threshold=0):
+        p = None if params is None else _c(params, np.float64)
+        self.lib.orc_set_sleeping(self.w, int(enabled), None if p is None else p.ctypes.data, int(tick_threshold))
+
+    def sleep_state(self):
+        """Per body: asleep, sleep_ready_counter, non_sleepy_tick_count, history length."""
+        out = np.zeros((self.n, 4))
+        self.lib.orc_get_sleep_state(self.w, out.ctypes.data)
+        return out
+
+    def wake_bodies(self, idx):
+        i = _c(idx, np.int64)
+        self.lib.orc_wake_bodies(self.w, len(i), i.ctypes.data)
 
     def cache(self, mid=False):
         n = self.lib.orc_last_mid_cache_size(self.w) if mid else self.lib.orc_cache_size(self.w)

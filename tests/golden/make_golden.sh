@@ -8,7 +8,7 @@ ROOT=$(cd "$HERE/../.." && pwd)
 make -C "$ROOT/oracle" ref REF="${REF:-/root/reference}"
 H="$ROOT/oracle/_ref/cp_ref_harness"
 cd "$HERE"
-rm -rf pile3 c0_full mix3 cyl3 cyl4 chain3 chain3h0 terrain hulls
+rm -rf pile3 c0_full mix3 cyl3 cyl4 chain3 chain3h0 terrain hulls sleep
 # C0 pattern, 3x3x3 cubes: full per-step dumps (state, caches, manifolds, islands).
 "$H" pile --n 3 --steps 40 --record 0,1,2,10,39 --out pile3
 # C0 in full (10x10x10 cubes): first step and a settled step (no mid/post caches).
@@ -35,3 +35,7 @@ rm -rf pile3 c0_full mix3 cyl3 cyl4 chain3 chain3h0 terrain hulls
 # cast at the final state (also on the terrain fixture).
 python3 make_hull_input.py hull_input.txt 24 20240
 "$H" hulls --in hull_input.txt --n 4 --ny 3 --steps 240 --record 0,60,100,149,239 --rays 128 --out hulls
+# Sleeping on (the engine's default parameters): a cube pile, a lone cube and
+# a sliding cube fall asleep; a sphere dropped on a sleeping column of three
+# cubes wakes it.  Every step's sleep state is recorded.
+"$H" sleep --steps 420 --record 0,100,135,136,153,154,200,220,221,300,313,314,315,330,419 --out sleep
