@@ -429,6 +429,7 @@ struct StepDump {
 static StepDump* g_cur = nullptr;
 static bool g_lite = false;  // omit the mid/post contact caches (large scenes)
 static bool g_sleep = false;  // record every body's sleep state after every step
+static int g_wake_step = -1, g_wake_body = -1;  // RigidBody::alertWakingAction on that body before that step
 static PhysicsEngine* g_eng = nullptr;
 
 static void on_collision(RigidBody* b1, RigidBody* b2, const std::vector<phyz::Manifold>& ms) {
@@ -637,6 +638,7 @@ static void run_trace(PhysicsEngine& E, const SceneCfg& cfg, int steps, const st
 			dump_cache(E, d.pre_cache);
 			g_cur = &d;
 		}
+		if (k == g_wake_step) E.bodies[g_wake_body]->alertWakingAction();
 		E.timeStep();
 		g_cur = nullptr;
 		if (g_sleep)
@@ -954,8 +956,10 @@ static int cmd_terrain(int argc, char** argv) {
 // sleep_disabled pinned to false): a 2x2x2 cube pile and a lone cube settle
 // and fall asleep, a cube launched along the ground slides to rest and falls
 // asleep later, and a sphere dropped from high above lands on a sleeping
-// column of three cubes and wakes it.  Every step's sleep state is recorded
-// (asleep, sleep_ready_counter, non_sleepy_tick_count, history length).
+// column of three cubes and wakes it; a waking action on one cube of the
+// sleeping pile (before step --wake-step, default 250) wakes the pile.
+// Every step's sleep state is recorded (asleep, sleep_ready_counter,
+// non_sleepy_tick_count, history length).
 static int cmd_sleep(int argc, char** argv) {
 	int steps = (int)arg_d(argc, argv, "--steps", 420);
 	uint64_t seed = (uint64_t)arg_d(argc, argv, "--seed", 5150);
@@ -995,6 +999,10 @@ static int cmd_sleep(int argc, char** argv) {
 	Out o(arg_s(argc, argv, "--out", "out"));
 	o.meta("seed", (double)seed);
 	g_sleep = true;
+	g_wake_step = (int)arg_d(argc, argv, "--wake-step", 250);
+	g_wake_body = (int)arg_d(argc, argv, "--wake-body", 3);
+	o.meta("wake_step", g_wake_step);
+	o.meta("wake_body", g_wake_body);
 	run_trace(E, cfg, steps, arg_steps(argc, argv, "--record"), o);
 	o.finish();
 	return 0;

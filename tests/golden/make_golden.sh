@@ -37,5 +37,6 @@ python3 make_hull_input.py hull_input.txt 24 20240
 "$H" hulls --in hull_input.txt --n 4 --ny 3 --steps 240 --record 0,60,100,149,239 --rays 128 --out hulls
 # Sleeping on (the engine's default parameters): a cube pile, a lone cube and
 # a sliding cube fall asleep; a sphere dropped on a sleeping column of three
-# cubes wakes it.  Every step's sleep state is recorded.
-"$H" sleep --steps 420 --record 0,100,135,136,153,154,200,220,221,300,313,314,315,330,419 --out sleep
+# cubes wakes it; a waking action on a pile cube before step 250 wakes the
+# pile.  Every step's sleep state is recorded.
+"$H" sleep --steps 420 --record 0,100,135,136,153,154,200,220,221,250,251,300,313,314,315,330,339,419 --out sleep
