@@ -45,6 +45,12 @@ class Config(ctypes.Structure):
         ("pgs_class_threads", ctypes.c_int * 3),
         ("graph", ctypes.c_int),
         ("fused", ctypes.c_int),
+        ("sleeping", ctypes.c_int),
+        ("sleep_tick_threshold", ctypes.c_int),
+        ("sleep_vel_coeff", ctypes.c_double),
+        ("sleep_ang_vel_eps", ctypes.c_double),
+        ("sleep_accel_coeff", ctypes.c_double),
+        ("sleep_delay", ctypes.c_double),
     ]
 
 
@@ -96,6 +102,9 @@ def lib():
         "cp_world_last_residual": (_I, [_P, _P]),
         "cp_world_last_scene_contacts": (_I, [_P, _I, _P, _P]),
         "cp_world_scene_flags": (_I, [_P, _I, _P, _P]),
+        "cp_world_get_sleep_state": (_I, [_P, _P]),
+        "cp_world_wake_bodies": (_I, [_P, _I, _P]),
+        "cp_world_set_sleep_disabled": (_I, [_P, _I, _P, _I]),
         "cp_world_raycast": (_I, [_P, _I, _P, _P, _P, _P, _P]),
         "cp_world_save_snapshot": (_I, [_P, ctypes.c_char_p]),
         "cp_world_load_snapshot": (_I, [_P, ctypes.c_char_p]),
@@ -436,6 +445,21 @@ class World:
             return False
         _chk(st, "cp_world_step")
         return True
+
+    def sleep_state(self):
+        """Per body: asleep, sleep_ready_counter, non_sleepy_tick_count, history length."""
+        out = np.zeros((self.n, 4))
+        _chk(self.L.cp_world_get_sleep_state(self.w, out.ctypes.data), "cp_world_get_sleep_state")
+        return out
+
+    def wake_bodies(self, idx):
+        """RigidBody::alertWakingAction for the listed bodies (their islands wake at the next step)."""
+        i = _a(idx, np.int32)
+        _chk(self.L.cp_world_wake_bodies(self.w, len(i), i.ctypes.data), "cp_world_wake_bodies")
+
+    def set_sleep_disabled(self, idx, disabled=True):
+        i = _a(idx, np.int32)
+        _chk(self.L.cp_world_set_sleep_disabled(self.w, len(i), i.ctypes.data, int(disabled)), "cp_world_set_sleep_disabled")
 
     def scene_flags(self):
         """Per scene 1 when one of its dynamic bodies has gone non-finite since its state was set."""

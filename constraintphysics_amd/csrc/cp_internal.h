@@ -67,6 +67,11 @@ struct Bodies {
 	double* wnorm;    // [3*WG]
 	double* dv;       // [12][n] PGS velocity / pseudo-velocity deltas
 	int* color_mask;  // scratch for colouring (2 x u64 per body, as 4 ints)
+	// sleeping (null while no body can be asleep): RigidBody::asleep, and the
+	// asleep flags of the last step's first substep, when the reference
+	// removes expired contacts (Contact::isExpired spares asleep bodies')
+	int* asleep;
+	const int* asleep_rm;
 };
 
 struct Params {
@@ -116,6 +121,24 @@ struct HSysDev {
 	int *cp_off, *cs_ptr, *cs_row, *cs_loc, *dg_loc;
 };
 
+// enabled: the arrays are valid and asleep bodies are honoured (some body
+// may be asleep); sleeping: islands may fall asleep this step.
+struct SleepDev {
+	int enabled, sleeping;
+	int* asleep;        // [n]
+	double* counter;    // [n] sleep_ready_counter
+	int* ticks;         // [n] non_sleepy_tick_count
+	int* hlen;          // [n] history entries held
+	int* hhead;         // [n] ring position of the oldest entry
+	double* hist;       // [hcap][6][n] (vel xyz, ang_vel xyz) per entry
+	const int* proc;    // [n] or null: whether the step-start loop treats the body as awake (set
+	                    // when waking actions changed asleep flags during that loop), else !asleep
+	const int* disabled;  // [n] RigidBody::sleep_disabled
+	int hcap;           // history length kept: (int)(sleep_delay / step_time)
+	double delay, step_time, vel_eps, ang_vel_eps, accel_eps, ang_accel_eps, wake_vel, hist_need;
+	int tick_threshold;
+};
+
 // Island schedule built on the device each step.  rows: (index, kind)
 // pairs, kind 0 contact normal, 1 friction, 2 joint; an island's rows are
 // [off[i], off[i+1]).  Its holonomic systems are sys[sys_off[i], sys_off[i] +
@@ -139,6 +162,7 @@ struct IslandSched {
 	const int *bod, *boff, *nbod, *cloc, *jloc;
 	int lds_bodies;
 	const int *vlist, *voff, *vcnt;
+	SleepDev sl;  // sleeping: the wake check after each island solve
 };
 
 // Per-component counts of the island search (indexed by the component's
@@ -159,20 +183,6 @@ struct IslTables {
 };
 // Sleeping state (PhysicsEngine.h:218-224; RigidBody asleep,
 // sleep_ready_counter, non_sleepy_tick_count, history).
-struct SleepDev {
-	int enabled;
-	int* asleep;        // [n]
-	double* counter;    // [n] sleep_ready_counter
-	int* ticks;         // [n] non_sleepy_tick_count
-	int* hlen;          // [n] history entries held
-	int* hhead;         // [n] ring position of the oldest entry
-	double* hist;       // [hcap][6][n] (vel xyz, ang_vel xyz) per entry
-	int* wake_req;      // [n] waking actions pending (RigidBody::recievedWakingAction)
-	int* disabled;      // [n] RigidBody::sleep_disabled
-	int hcap;           // history length kept: (int)(sleep_delay / step_time)
-	double delay, step_time, vel_eps, ang_vel_eps, accel_eps, ang_accel_eps, wake_vel, hist_need;
-	int tick_threshold;
-};
 // What the per-island walk reads and writes.
 struct IslWalk {
 	IslTables T;
@@ -248,6 +258,18 @@ struct Contacts {
 	double* row;
 };
 
+// A cached contact the reference removed at the previous step's first
+// substep (removeExpiredContactConstraints, PhysicsEngine.cpp:1434-1446):
+// memory_life was below 0 there (below -1 after the end-of-step decrement)
+// and neither body was asleep then (Contact::isExpired,
+// PersistentConstraint.h:74; prot: the asleep flags of that moment, null
+// when no body sleeps).
+__device__ __forceinline__ bool contact_removed(const Contacts& C, int k, const int* prot) {
+	if (C.life[k] >= -1) return false;
+	if (!prot) return true;
+	return !prot[(int)(C.key[k] >> 32)] && !prot[(int)(C.key[k] & 0xFFFFFFFFu)];
+}
+
 // Record of CP_RS values (FP64: 512 B = 4 lines of 128 B; FP32: 256 B = 2
 // lines).  The fields a sweep writes (impulses, static flag) share the first
 // line.  Only the independent coefficients are stored: the linear parts of
@@ -309,6 +331,8 @@ void launch_scene_of_body(int n, const int* sbo, int nscenes, int* out, hipStrea
 // Contacts whose first body lies in one of the ranges [r[2k], r[2k] + r[2k+1])
 // are retired: not live, removed at the next merge.
 void launch_kill_contacts(const Contacts& C, int n, const int* ranges, int nranges, hipStream_t s);
+// The step-start loop with sleeping: movement history, sleep counters, gravity.
+void launch_gravity_sleep(const Bodies& b, const Params& p, const SleepDev& S, hipStream_t s);
 void launch_integrate(const Bodies& b, const Params& p, hipStream_t s);
 // Flags dynamic bodies (of the visited prefix) with a non-finite pose or
 // velocity: body_flag[i] = 1 and any[0] = 1.  Fill: p[0..n) = v.
@@ -353,7 +377,7 @@ void launch_np_gather(const Manifolds& M, int mcap, const uint64_t* pairs, int n
                       hipStream_t s);
 void launch_np_match(const NewPts& np, int nnew, int cap, const Contacts& C, int nold, const Bodies& b, const Params& P,
                      hipStream_t s);
-void launch_old_keep(const Contacts& C, int nold, int* keep, hipStream_t s);
+void launch_old_keep(const Contacts& C, int nold, int* keep, const int* prot, hipStream_t s);
 void launch_new_keys(const NewPts& np, int nnew, const int* rank, uint64_t* nk, hipStream_t s);
 void launch_merge(const Contacts& S, int nold, const int* keep_scan, const uint64_t* nk, int nnewc, const NewPts& np, int nnew, int cap,
                   const int* rank, const Contacts& D, const Bodies& b, const Params& P, uint64_t seq0, hipStream_t s);

@@ -20,11 +20,74 @@ __global__ void k_gravity(Bodies b, Params p) {
 	ST3(b.vel, i, b.n, v);
 }
 
+// The step-start loop with sleeping (PhysicsEngine.cpp:65-87) for a dynamic
+// body the loop finds awake: RigidBody::recordMovementState (RigidBody.cpp:
+// 347-352) into the history ring, PhysicsEngine::bodySleepy (:1221-1254) and
+// the sleep counters, then gravity.  FP64 in the reference's operation order.
+__global__ void k_gravity_sleep(Bodies b, Params p, SleepDev S) {
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= b.nl || b.movement[i] != 0) return;
+	if (S.proc ? !S.proc[i] : S.asleep[i] != 0) return;
+	const int n = b.n;
+	V3 v = LD3(b.vel, i, n), w = LD3(b.angv, i, n);
+	int len = S.hlen[i], head = S.hhead[i];
+	if (S.hcap > 0) {
+		int slot;
+		if (len < S.hcap) {
+			slot = head + len;
+			if (slot >= S.hcap) slot -= S.hcap;
+			len++;
+		} else {
+			slot = head;  // the oldest entry leaves
+			head = head + 1 == S.hcap ? 0 : head + 1;
+		}
+		double* h = S.hist + (size_t)slot * 6 * n;
+		h[i] = v.x; h[n + i] = v.y; h[2 * n + i] = v.z;
+		h[3 * n + i] = w.x; h[4 * n + i] = w.y; h[5 * n + i] = w.z;
+		S.hlen[i] = len;
+		S.hhead[i] = head;
+	}
+	bool sleepy = false;
+	if (!S.disabled[i] && !(len == 0 || 2 * len < S.hist_need)) {
+		V3 av{0, 0, 0}, aw{0, 0, 0};
+		for (int j = 0; j < len; j++) {
+			int slot = head + j;
+			if (slot >= S.hcap) slot -= S.hcap;
+			const double* h = S.hist + (size_t)slot * 6 * n;
+			av += V3{h[i], h[n + i], h[2 * n + i]};
+			aw += V3{h[3 * n + i], h[4 * n + i], h[5 * n + i]};
+		}
+		const double dn = (double)len;
+		av = V3{av.x / dn, av.y / dn, av.z / dn};
+		aw = V3{aw.x / dn, aw.y / dn, aw.z / dn};
+		int last = head + len - 1;
+		if (last >= S.hcap) last -= S.hcap;
+		const double* hf = S.hist + (size_t)head * 6 * n;
+		const double* hb = S.hist + (size_t)last * 6 * n;
+		const double span = S.step_time * (len - 1);
+		V3 acc{(hb[i] - hf[i]) / span, (hb[n + i] - hf[n + i]) / span, (hb[2 * n + i] - hf[2 * n + i]) / span};
+		V3 aacc{(hb[3 * n + i] - hf[3 * n + i]) / span, (hb[4 * n + i] - hf[4 * n + i]) / span, (hb[5 * n + i] - hf[5 * n + i]) / span};
+		sleepy = mag(av) <= S.vel_eps && mag(aw) <= S.ang_vel_eps && mag(acc) <= S.accel_eps && mag(aacc) <= S.ang_accel_eps;
+	}
+	if (sleepy) {
+		S.counter[i] += S.step_time;
+	} else {
+		int t = S.ticks[i] + 1;
+		if (t >= S.tick_threshold) {
+			S.counter[i] = 0.0;
+			t = 0;
+		}
+		S.ticks[i] = t;
+	}
+	v += V3{p.gx, p.gy, p.gz} * p.dt;
+	ST3(b.vel, i, n, v);
+}
+
 // update_positions (PhysicsEngine.cpp:390-410) followed by
 // RigidBody::updateInertiaTensor (RigidBody.cpp:270-279).
 __global__ __launch_bounds__(128) void k_integrate(Bodies b, Params p) {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= b.nl || b.movement[i] != 0) return;
+	if (i >= b.nl || b.movement[i] != 0 || (b.asleep && b.asleep[i])) return;
 	integrate_body(b, p, i);
 }
 
@@ -59,7 +122,7 @@ __global__ __launch_bounds__(128) void k_refresh_geometry(Bodies b, Shapes sh, i
 	Q q = ldq(b.q, i, n);
 	V3 com = LD3(b.com, i, n);
 	if (only_moved) {
-		if (b.movement[i] != 0) return;
+		if (b.movement[i] != 0 || (b.asleep && b.asleep[i])) return;
 		if (!(neq(com, LD3(b.prev_com, i, n)) || q_neq(q, ldq(b.prev_q, i, n)))) return;
 		q = q_normalize(q);
 		stq(b.q, i, n, q);
@@ -137,6 +200,9 @@ static inline int nblk(int n, int t) { return (n + t - 1) / t; }
 
 void launch_gravity(const Bodies& b, const Params& p, hipStream_t s) {
 	if (b.nl) hipLaunchKernelGGL(k_gravity, dim3(nblk(b.nl, 256)), dim3(256), 0, s, b, p);
+}
+void launch_gravity_sleep(const Bodies& b, const Params& p, const SleepDev& S, hipStream_t s) {
+	if (b.nl) hipLaunchKernelGGL(k_gravity_sleep, dim3(nblk(b.nl, 256)), dim3(256), 0, s, b, p, S);
 }
 void launch_integrate(const Bodies& b, const Params& p, hipStream_t s) {
 	if (b.nl) hipLaunchKernelGGL(k_integrate, dim3(nblk(b.nl, 128)), dim3(128), 0, s, b, p);

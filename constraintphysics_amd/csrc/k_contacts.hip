@@ -9,10 +9,10 @@
 // of their pair.  The merge needs no sort: each element's output position is
 // its rank in its own list plus a binary-search count in the other list.
 //
-// Removal of expired contacts (memory_life < 0 at the first substep) is
-// deferred to the next step's merge: the reference removes them before any
-// later read, so dropping entries whose life fell below -1 after the end-of-
-// step decrement yields the same cache.
+// Removal of expired contacts (memory_life < 0 at the first substep, neither
+// body asleep) is deferred to the next step's merge: the reference removes
+// them before any later read, so dropping entries whose life fell below -1
+// after the end-of-step decrement (contact_removed) yields the same cache.
 //
 // Colouring: one thread per scene greedily colours the scene's edges (body
 // pairs with at least one live contact) so that no two edges of a colour share
@@ -90,7 +90,7 @@ __global__ void k_np_match(NewPts np, int nnew, int cap, Contacts C, int nold, B
 		if (np.cid[k] == cid && np.bid[k] == bid) { last = false; lastj = k; }
 	int m = -1;
 	for (int k = lower_bound_u64(C.key, nold, key); k < nold && C.key[k] == key; k++) {
-		if (C.life[k] < -1) continue;  // removed at the previous step's first substep
+		if (contact_removed(C, k, b.asleep_rm)) continue;  // removed at the previous step's first substep
 		if (C.cid[k] == cid && C.bid[k] == bid) { m = k; break; }
 	}
 	np.is_new[j] = (m < 0 && first) ? 1 : 0;
@@ -112,9 +112,9 @@ __global__ void k_np_match(NewPts np, int nnew, int cap, Contacts C, int nold, B
 	}
 }
 
-__global__ void k_old_keep(const Contacts C, int nold, int* keep) {
+__global__ void k_old_keep(const Contacts C, int nold, int* keep, const int* prot) {
 	int k = blockIdx.x * blockDim.x + threadIdx.x;
-	if (k < nold) keep[k] = C.life[k] >= -1 ? 1 : 0;
+	if (k < nold) keep[k] = contact_removed(C, k, prot) ? 0 : 1;
 	if (k == nold) keep[k] = 0;
 }
 
@@ -140,9 +140,9 @@ __device__ void copy_contact(const Contacts& S, int i, Contacts& D, int o) {
 	D.imp_t[KD + o] = S.imp_t[KS + i];
 }
 
-__global__ void k_merge_old(Contacts S, int nold, const int* keep_scan, const uint64_t* nk, int nnewc, Contacts D) {
+__global__ void k_merge_old(Contacts S, int nold, const int* keep_scan, const uint64_t* nk, int nnewc, Contacts D, const int* prot) {
 	int k = blockIdx.x * blockDim.x + threadIdx.x;
-	if (k >= nold || S.life[k] < -1) return;
+	if (k >= nold || contact_removed(S, k, prot)) return;
 	int o = keep_scan[k] + lower_bound_u64(nk, nnewc, S.key[k]);
 	copy_contact(S, k, D, o);
 }
@@ -393,15 +393,15 @@ void launch_np_match(const NewPts& np, int nnew, int cap, const Contacts& C, int
                      hipStream_t s) {
 	if (nnew) hipLaunchKernelGGL(k_np_match, dim3(nblk(nnew, 128)), dim3(128), 0, s, np, nnew, cap, C, nold, b, P);
 }
-void launch_old_keep(const Contacts& C, int nold, int* keep, hipStream_t s) {
-	hipLaunchKernelGGL(k_old_keep, dim3(nblk(nold + 1, 256)), dim3(256), 0, s, C, nold, keep);
+void launch_old_keep(const Contacts& C, int nold, int* keep, const int* prot, hipStream_t s) {
+	hipLaunchKernelGGL(k_old_keep, dim3(nblk(nold + 1, 256)), dim3(256), 0, s, C, nold, keep, prot);
 }
 void launch_new_keys(const NewPts& np, int nnew, const int* rank, uint64_t* nk, hipStream_t s) {
 	if (nnew) hipLaunchKernelGGL(k_new_keys, dim3(nblk(nnew, 256)), dim3(256), 0, s, np, nnew, rank, nk);
 }
 void launch_merge(const Contacts& S, int nold, const int* keep_scan, const uint64_t* nk, int nnewc, const NewPts& np, int nnew, int cap,
                   const int* rank, const Contacts& D, const Bodies& b, const Params& P, uint64_t seq0, hipStream_t s) {
-	if (nold) hipLaunchKernelGGL(k_merge_old, dim3(nblk(nold, 128)), dim3(128), 0, s, S, nold, keep_scan, nk, nnewc, D);
+	if (nold) hipLaunchKernelGGL(k_merge_old, dim3(nblk(nold, 128)), dim3(128), 0, s, S, nold, keep_scan, nk, nnewc, D, b.asleep_rm);
 	if (nnew)
 		hipLaunchKernelGGL(k_merge_new, dim3(nblk(nnew, 128)), dim3(128), 0, s, np, nnew, cap, rank, S, nold, keep_scan, D, b, P, seq0);
 }

@@ -152,7 +152,7 @@ __global__ void k_ib_bodies(int nb, const int* movement, const int* aoff, int* p
 	comp[i] = r;
 	atomicAdd(&acc[r].bodies, 1);
 	atomicAdd(&acc[r].adj, aoff[i + 1] - aoff[i]);
-	if (!S.enabled || !ready_to_sleep(S, i)) atomicOr(&acc[r].notready, 1);
+	if (!S.sleeping || !ready_to_sleep(S, i)) atomicOr(&acc[r].notready, 1);
 }
 
 // The rows an edge gives its island: none when neither end is an awake
@@ -236,7 +236,7 @@ __global__ void k_ib_tables(int nb, const int* rord, const IslCnt* in, const Isl
 // ready.
 __global__ void k_ib_sleep(int nb, const int* comp, const IslCnt* acc, SleepDev S, double* vel, double* angv) {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= nb || comp[i] < 0 || !S.enabled) return;
+	if (i >= nb || comp[i] < 0 || !S.sleeping) return;
 	if (acc[comp[i]].notready) return;
 	S.asleep[i] = 1;
 	for (int c = 0; c < 3; c++) {
@@ -457,7 +457,7 @@ void launch_ib_flags(int nb, const int* rord, const int* comp, const IslCnt* acc
 void launch_ib_tables(int nb, const int* rord, const IslCnt* in, const IslCnt* out, const IslTables& T, int* info, const int* comp,
                       const IslCnt* acc, const SleepDev& S, double* vel, double* angv, hipStream_t s) {
 	hipLaunchKernelGGL(k_ib_tables, dim3(nblk256(nb + 1)), dim3(256), 0, s, nb, rord, in, out, T, info);
-	if (S.enabled) hipLaunchKernelGGL(k_ib_sleep, dim3(nblk256(nb)), dim3(256), 0, s, nb, comp, acc, S, vel, angv);
+	if (S.sleeping) hipLaunchKernelGGL(k_ib_sleep, dim3(nblk256(nb)), dim3(256), 0, s, nb, comp, acc, S, vel, angv);
 }
 void launch_ib_walk(const IslWalk& W, int nisl, bool colored, hipStream_t s) {
 	if (!nisl) return;

@@ -1027,7 +1027,9 @@ __global__ __launch_bounds__(64 * CLS_WAVES) void k_np_classify(Bodies b, Shapes
 		int ia = (int)(key >> 32), ib = (int)(key & 0xFFFFFFFFu);
 		// collisionAllowed (PhysicsEngine.cpp:570-572): jointed pairs never collide
 		bool allowed = n_nocol == 0 || !sorted_has(nocol, n_nocol, key);
-		if (allowed && !(b.movement[ia] != 0 && b.movement[ib] != 0)) {
+		// collision_possible (PhysicsEngine.cpp:174): a dynamic body, not both asleep
+		bool both_asleep = b.asleep && b.asleep[ia] && b.asleep[ib];
+		if (allowed && !(b.movement[ia] != 0 && b.movement[ib] != 0) && !both_asleep) {
 			int ta = sh.type[b.shape[ia]], tb = sh.type[b.shape[ib]];
 			int lo = ta < tb ? ta : tb, hi = ta < tb ? tb : ta;
 			if (hi == SHAPE_MESH) cls = (lo != SHAPE_MESH && (!rank || rank[i] == 0)) ? NPC_MESH : -1;

@@ -691,6 +691,36 @@ __device__ __forceinline__ void row_bodies(const Contacts& C, const Joints& J, i
 	}
 }
 
+// After an island's velocity changes (applyVelocityChange, PhysicsEngine.cpp:
+// 1113-1136): an asleep body the solve moved faster than wake_vel wakes the
+// whole island (wakeupIsland, :1285-1290; RigidBody::wake, RigidBody.cpp:
+// 331-338); otherwise asleep bodies keep a zero velocity.  The reference
+// visits the island's bodies in the order of an unordered_map keyed by
+// address (ConstraintSolver.cpp:91, :199-212), so an asleep body it visits
+// before a waking one is zeroed first; here every waking body comes first,
+// one of the orders the reference can take.
+__device__ __forceinline__ bool wakes(const Bodies& b, double wake_vel, int i) {
+	const int N = b.n;
+	V3 v{b.vel[i], b.vel[N + i], b.vel[2 * N + i]}, w{b.angv[i], b.angv[N + i], b.angv[2 * N + i]};
+	return b.asleep[i] && (mag(v) > wake_vel || mag(w) > wake_vel);
+}
+__device__ __forceinline__ void wake_body(const Bodies& b, const SleepDev& sl, int i) {
+	if (!b.asleep[i]) return;
+	b.asleep[i] = 0;
+	sl.counter[i] = 0;
+	sl.ticks[i] = 0;
+	sl.hlen[i] = 0;
+	sl.hhead[i] = 0;
+}
+__device__ __forceinline__ void zero_asleep(const Bodies& b, int i) {
+	if (!b.asleep[i]) return;
+	const int N = b.n;
+	for (int c = 0; c < 3; c++) {
+		b.vel[c * N + i] = 0;
+		b.angv[c * N + i] = 0;
+	}
+}
+
 // Reference order: one thread per island walks its rows sequentially.
 __global__ __launch_bounds__(64) void k_pgs_ordered(Bodies b, Contacts C, Joints J, HSysDev H, IslandSched S, Params P, int* applied, double* lin, int first) {
 	int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -719,6 +749,24 @@ __global__ __launch_bounds__(64) void k_pgs_ordered(Bodies b, Contacts C, Joints
 			applied[i] = 1;
 			apply_dv(b, b.dv, i);
 		}
+	}
+	if (!b.asleep) return;
+	bool wake = false;
+	for (int r = r0; r < r1 && !wake; r++) {
+		int ii[2];
+		row_bodies(C, J, rows[2 * r], rows[2 * r + 1], ii);
+		for (int q = 0; q < 2; q++)
+			if (b.movement[ii[q]] == 0 && wakes(b, S.sl.wake_vel, ii[q])) wake = true;
+	}
+	if (wake) {
+		for (int x = S.voff[t]; x < S.voff[t] + S.vcnt[t]; x++) wake_body(b, S.sl, S.vlist[x]);
+		return;
+	}
+	for (int r = r0; r < r1; r++) {
+		int ii[2];
+		row_bodies(C, J, rows[2 * r], rows[2 * r + 1], ii);
+		for (int q = 0; q < 2; q++)
+			if (b.movement[ii[q]] == 0) zero_asleep(b, ii[q]);
 	}
 }
 
@@ -778,6 +826,15 @@ __global__ __launch_bounds__(64) void k_pgs_islands(Bodies b, Contacts C, Joints
 				b.pang[c * N + i] += sdv[(9 + c) * DS + j];
 			}
 		}
+		if (!b.asleep) return;
+		__syncthreads();
+		bool w = false;
+		for (int j = lane; j < nbi; j += 64) w = w || wakes(b, S.sl.wake_vel, S.bod[bo0 + j]);
+		if (__syncthreads_or(w)) {
+			for (int x = S.voff[t] + lane; x < S.voff[t] + S.vcnt[t]; x += 64) wake_body(b, S.sl, S.vlist[x]);
+		} else {
+			for (int j = lane; j < nbi; j += 64) zero_asleep(b, S.bod[bo0 + j]);
+		}
 		return;
 	}
 	for (int r = r0 + lane; r < r1; r += 64) {
@@ -787,6 +844,24 @@ __global__ __launch_bounds__(64) void k_pgs_islands(Bodies b, Contacts C, Joints
 			int i = ii[q];
 			if (b.movement[i] != 0 || atomicExch(applied + i, 1) != 0) continue;
 			apply_dv(b, b.dv, i);
+		}
+	}
+	if (!b.asleep) return;
+	__syncthreads();
+	bool w = false;
+	for (int r = r0 + lane; r < r1; r += 64) {
+		int ii[2];
+		row_bodies(C, J, rows[2 * r], rows[2 * r + 1], ii);
+		for (int q = 0; q < 2; q++) w = w || (b.movement[ii[q]] == 0 && wakes(b, S.sl.wake_vel, ii[q]));
+	}
+	if (__syncthreads_or(w)) {
+		for (int x = S.voff[t] + lane; x < S.voff[t] + S.vcnt[t]; x += 64) wake_body(b, S.sl, S.vlist[x]);
+	} else {
+		for (int r = r0 + lane; r < r1; r += 64) {
+			int ii[2];
+			row_bodies(C, J, rows[2 * r], rows[2 * r + 1], ii);
+			for (int q = 0; q < 2; q++)
+				if (b.movement[ii[q]] == 0) zero_asleep(b, ii[q]);
 		}
 	}
 }

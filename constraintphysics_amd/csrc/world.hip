@@ -263,6 +263,31 @@ struct cp_world {
 	DArr<IslCnt> ib_acc, ib_in, ib_out;
 	int ib_nj = -1, ib_ng = 0;        // joints the pair groups were built from, groups
 	std::vector<int> ib_rord_host;     // node-map order the ranks were uploaded from
+	// sleeping: per-body state (allocated with the bodies), the movement
+	// history ring, pending waking actions
+	DArr<int> sl_asleep, sl_asleep_rm, sl_ticks, sl_hlen, sl_hhead, sl_disabled, sl_proc;
+	DArr<double> sl_counter, sl_hist;
+	int sl_hcap = -1;                  // history slots the ring holds
+	bool sl_active = false;            // sleeping has been on: asleep flags are honoured
+	std::vector<int> sl_wake;          // bodies with a waking action since the last step
+	std::vector<int> sl_dis_host;      // sleep_disabled per body
+	void prepare_sleep(bool& proc);
+	// per cached contact: removed by the reference at the last step's first
+	// substep (memory_life below -1 now, neither body asleep then)
+	std::vector<char> removed_contacts(const std::vector<int>& life) {
+		std::vector<char> out(life.size(), 0);
+		std::vector<int> rm;
+		std::vector<uint64_t> key;
+		if (sl_active) {
+			down(rm, sl_asleep_rm.p, nb);
+			down(key, cb[cur].c.key, life.size());
+		}
+		for (size_t k = 0; k < life.size(); k++) {
+			if (life[k] >= -1) continue;
+			out[k] = !sl_active || (!rm[(size_t)(key[k] >> 32)] && !rm[(size_t)(key[k] & 0xFFFFFFFFu)]);
+		}
+		return out;
+	}
 	int isl_n = 0;                     // islands of the last schedule
 	bool islands_on_device = false;    // last_rows / last_offs not yet read back
 	int isl_lds_bodies = 0;            // LDS columns per island solve (largest island + 1, capped)
@@ -337,6 +362,8 @@ struct cp_world {
 		b.mat = mat.p; b.aabb = aabb.p;
 		b.wpt_off = wpt_off.p; b.wpts = wpts.p; b.wgv_off = wgv_off.p; b.wnorm = wnorm.p;
 		b.dv = dv.p; b.color_mask = color_mask.p;
+		b.asleep = sl_active ? sl_asleep.p : nullptr;
+		b.asleep_rm = sl_active ? sl_asleep_rm.p : nullptr;
 		return b;
 	}
 	Shapes shapes() {
@@ -561,6 +588,16 @@ void cp_world::upload() {
 	dv.reserve(12 * (size_t)n);
 	color_mask.reserve(4 * (size_t)n);
 	applied.reserve(n);
+	DArr<int>* sl_int[] = {&sl_asleep, &sl_asleep_rm, &sl_ticks, &sl_hlen, &sl_hhead, &sl_proc};
+	for (auto* a : sl_int) {
+		a->reserve(n);
+		HIPCHK(hipMemset(a->p, 0, (size_t)n * sizeof(int)));
+	}
+	sl_counter.reserve(n);
+	HIPCHK(hipMemset(sl_counter.p, 0, (size_t)n * sizeof(double)));
+	sl_dis_host.resize(n, 0);
+	up(sl_disabled, sl_dis_host);
+	sl_hcap = -1;
 	nf_body.reserve(n);
 	HIPCHK(hipMemset(nf_body.p, 0, (size_t)n * sizeof(int)));
 	nf_any.reserve(1);
@@ -1159,7 +1196,7 @@ void cp_world::persist(int npairs) {
 	if (nold == 0) cb[cur].reserve(1);
 	launch_np_match(np, nnew, npcap, cb[cur].c, nold, b, P, stream);
 	keep.reserve(nold + 1); keep_scan.reserve(nold + 1);
-	launch_old_keep(cb[cur].c, nold, keep.p, stream);
+	launch_old_keep(cb[cur].c, nold, keep.p, b.asleep_rm, stream);
 	scan(keep.p, keep_scan.p, nold + 1);
 	// is_new -> rank (exclusive scan over nnew+1 with a zero tail)
 	rank.reserve(nnew + 1);
@@ -1277,7 +1314,7 @@ void cp_world::schedule_islands(bool colored) {
 	launch_ib_aoff(ib_adjk2.p, 2 * N, nb, ib_aoff.p, stream);
 	const SleepDev S = sleep_dev();
 	launch_ib_components(ib_me.p, ib_midx.p, N, nb, movement.p, ib_rank.p, ib_aoff.p, ib_parent.p, ib_comp.p, ib_acc.p, S, stream);
-	launch_ib_flags(nb, ib_rord.p, ib_comp.p, ib_acc.p, S.enabled, ib_in.p, stream);
+	launch_ib_flags(nb, ib_rord.p, ib_comp.p, ib_acc.p, S.sleeping, ib_in.p, stream);
 	bytes = 0;
 	const IslCnt zero{0, 0, 0, 0, 0, 0, 0};
 	HIPCHK(hipcub::DeviceScan::ExclusiveScan(nullptr, bytes, ib_in.p, ib_out.p, IslCntSum(), zero, nb + 1, stream));
@@ -1309,8 +1346,95 @@ void cp_world::schedule_islands(bool colored) {
 
 SleepDev cp_world::sleep_dev() {
 	SleepDev S{};
-	S.enabled = 0;
+	S.enabled = sl_active ? 1 : 0;
+	S.sleeping = sl_active && cfg.sleeping ? 1 : 0;
+	if (!S.enabled) return S;
+	S.asleep = sl_asleep.p; S.counter = sl_counter.p; S.ticks = sl_ticks.p; S.hlen = sl_hlen.p; S.hhead = sl_hhead.p;
+	S.hist = sl_hist.p; S.disabled = sl_disabled.p; S.proc = nullptr;
+	S.hcap = sl_hcap;
+	// PhysicsEngine::bodySleepy (PhysicsEngine.cpp:1243-1245) and the wake
+	// threshold of applyVelocityChange (:1124), as the reference computes them
+	const double* g = cfg.gravity;
+	const double gmag = std::sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+	S.delay = cfg.sleep_delay;
+	S.step_time = cfg.dt;
+	S.vel_eps = cfg.sleep_vel_coeff * gmag;
+	S.accel_eps = cfg.sleep_accel_coeff * gmag;
+	S.ang_accel_eps = cfg.sleep_accel_coeff * 2.5;
+	S.ang_vel_eps = cfg.sleep_ang_vel_eps;
+	S.wake_vel = 0.25 * cfg.sleep_vel_coeff * gmag;
+	S.hist_need = cfg.sleep_delay / cfg.dt;
+	S.tick_threshold = cfg.sleep_tick_threshold;
 	return S;
+}
+
+// Sleep state at the step start: the history ring sized for
+// (int)(sleep_delay / dt) entries (RigidBody::recordMovementState's length,
+// PhysicsEngine.cpp:72), and the waking actions since the last step
+// (PhysicsEngine.cpp:66-69): in body order, a body with a waking action wakes
+// every body of its island in the constraint graph as it stands (the contact
+// cache without the contacts the last step removed, and the joints); a body
+// the loop reaches while still asleep gets no history entry and no gravity
+// this step, even when a later body's action wakes it.  proc is set when
+// those per-body decisions were uploaded.
+void cp_world::prepare_sleep(bool& proc) {
+	proc = false;
+	const int hcap = static_cast<int>(cfg.sleep_delay / cfg.dt);
+	if (hcap != sl_hcap) {
+		sl_hist.reserve(6 * (size_t)std::max(hcap, 1) * nb);
+		HIPCHK(hipMemsetAsync(sl_hlen.p, 0, (size_t)nb * sizeof(int), stream));
+		HIPCHK(hipMemsetAsync(sl_hhead.p, 0, (size_t)nb * sizeof(int), stream));
+		sl_hcap = hcap;
+	}
+	if (sl_wake.empty()) return;
+	HIPCHK(hipStreamSynchronize(stream));
+	std::vector<int> asleep, rm, ticks, hlen, hhead, live;
+	std::vector<double> counter;
+	std::vector<int> life;
+	std::vector<uint64_t> key;
+	down(asleep, sl_asleep.p, nb); down(rm, sl_asleep_rm.p, nb); down(ticks, sl_ticks.p, nb);
+	down(hlen, sl_hlen.p, nb); down(hhead, sl_hhead.p, nb); down(counter, sl_counter.p, nb);
+	down(key, cb[cur].c.key, ncontacts); down(life, cb[cur].c.life, ncontacts);
+	std::vector<std::vector<int>> adj(nb);
+	for (int k = 0; k < ncontacts; k++) {
+		int a = (int)(key[k] >> 32), c = (int)(key[k] & 0xFFFFFFFFu);
+		if (life[k] < -1 && !rm[a] && !rm[c]) continue;  // removed at the last step's first substep
+		adj[a].push_back(c);
+		adj[c].push_back(a);
+	}
+	for (size_t j = 0; j < j_type.size(); j++) {
+		adj[j_a[j]].push_back(j_b[j]);
+		adj[j_b[j]].push_back(j_a[j]);
+	}
+	std::vector<char> req(nb, 0), vis(nb, 0);
+	for (int i : sl_wake) req[i] = 1;
+	sl_wake.clear();
+	std::vector<int> pr(nb, 0), st;
+	for (int i = 0; i < nb; i++) {
+		if (req[i]) {
+			// wakeupIsland (PhysicsEngine.cpp:1285-1290) through bfsVisitAll (:1263-1283)
+			std::fill(vis.begin(), vis.end(), 0);
+			st.assign(1, i);
+			while (!st.empty()) {
+				int c = st.back();
+				st.pop_back();
+				if (vis[c]) continue;
+				vis[c] = 1;
+				if (b_move[c] == 0 && asleep[c]) {
+					asleep[c] = 0;
+					counter[c] = 0;
+					ticks[c] = 0;
+					hlen[c] = 0;
+					hhead[c] = 0;
+				}
+				for (int o : adj[c])
+					if (b_move[o] == 0 && !vis[o]) st.push_back(o);
+			}
+		}
+		pr[i] = b_move[i] == 0 && !asleep[i];
+	}
+	up(sl_asleep, asleep); up(sl_counter, counter); up(sl_ticks, ticks); up(sl_hlen, hlen); up(sl_hhead, hhead); up(sl_proc, pr);
+	proc = true;
 }
 
 IslTables cp_world::isl_tables() {
@@ -1485,6 +1609,16 @@ void cp_world::graph_substeps(const Bodies& b, const Params& P) {
 void cp_world::step() {
 	if (serving) serving_commit();
 	if (!j_type.empty() && cfg.pgs_mode == CP_PGS_COLORED) throw ApiErr{CP_ERR_STATE};  // joints: island solves only
+	if (cfg.sleeping && (cfg.pgs_mode == CP_PGS_COLORED || serving)) throw ApiErr{CP_ERR_STATE};  // sleeping: island solves only
+	if (sl_active && !cfg.sleeping && cfg.pgs_mode == CP_PGS_COLORED) {
+		// the coloured mode keeps no asleep bodies: all wake
+		HIPCHK(hipMemsetAsync(sl_asleep.p, 0, (size_t)nb * sizeof(int), stream));
+		HIPCHK(hipMemsetAsync(sl_asleep_rm.p, 0, (size_t)nb * sizeof(int), stream));
+		sl_active = false;
+	}
+	if (cfg.sleeping) sl_active = true;
+	bool sl_proc_set = false;
+	if (sl_active) prepare_sleep(sl_proc_set);
 	if (!hs_explicit && cfg.hol_iters > 0 && (hs_scan || hs_changed)) {
 		discover_systems();
 		hs_scan = false;
@@ -1493,7 +1627,11 @@ void cp_world::step() {
 	Params P = params();
 	Bodies b = bodies();
 	if (timing) HIPCHK(hipEventRecord(ev[0], stream));
-	launch_gravity(b, P, stream);
+	if (sl_active) {
+		SleepDev SD = sleep_dev();
+		if (sl_proc_set) SD.proc = sl_proc.p;
+		launch_gravity_sleep(b, P, SD, stream);
+	} else launch_gravity(b, P, stream);
 	int npairs = broadphase();
 	if (timing) HIPCHK(hipEventRecord(ev[1], stream));
 	npairs = narrowphase(npairs);
@@ -1515,6 +1653,10 @@ void cp_world::step() {
 	if (ordered) {
 		schedule_islands(cfg.pgs_mode == CP_PGS_ISLAND_COLORED);
 		nisl = isl_n;
+		// the asleep flags when the reference removes expired contacts (the
+		// first substep, after the island decision): the next step's merge
+		// spares the contacts of these bodies
+		if (sl_active) HIPCHK(hipMemcpyAsync(sl_asleep_rm.p, sl_asleep.p, (size_t)nb * sizeof(int), hipMemcpyDeviceToDevice, stream));
 		if (hol) lin.reserve(6 * (size_t)lin_nl);
 	} else {
 		color_off.reserve((size_t)nscenes * (CP_MAXC + 2));
@@ -1553,7 +1695,7 @@ void cp_world::step() {
 			              isl_eoff.p, isl_coff.p, isl_icoff.p, isl_ncol.p, isl_serial.p,
 			              icol ? isl_bod.p : nullptr, icol ? isl_body_off.p : nullptr, icol ? isl_nbod.p : nullptr,
 			              icol ? isl_cloc.p : nullptr, icol ? isl_jloc.p : nullptr, icol ? isl_lds_bodies : 0,
-			              ib_vlist.p, isl_body_off.p, isl_body_cnt.p};
+			              ib_vlist.p, isl_body_off.p, isl_body_cnt.p, sleep_dev()};
 			launch_pgs_islands(b, cb[cur].c, J, H, S, P, applied.p, lin.p, s == 0, cfg.pgs_mode == CP_PGS_ISLAND_COLORED, stream);
 		} else
 			colored_pgs(b, P);
@@ -1785,6 +1927,12 @@ void cp_config_default(cp_config* c) {
 	c->graph = 0;
 	c->fused = 0;
 	c->precision = 0;
+	c->sleeping = 0;
+	c->sleep_tick_threshold = 3;
+	c->sleep_vel_coeff = 0.1;
+	c->sleep_ang_vel_eps = 0.1;
+	c->sleep_accel_coeff = 0.044;
+	c->sleep_delay = 0.5;
 }
 
 const char* cp_status_string(int s) {
@@ -1809,7 +1957,9 @@ static bool valid_cfg(const cp_config* c) {
 	for (int k = 0; k < 3; k++) classes = classes && valid_threads(c->pgs_class_threads[k]);
 	return c->dt > 0 && c->substeps >= 1 && c->vel_iters >= 0 && c->pos_iters >= 0 && c->gyro_ticks >= 1 && valid_threads(c->solver_threads) &&
 	       (c->pgs_mode >= 0 && c->pgs_mode <= 2) && (c->precision == CP_PREC_FP64 || c->precision == CP_PREC_SPLIT) && classes &&
-	       (c->graph == 0 || c->graph == 1) && (c->fused == 0 || c->fused == 1);
+	       (c->graph == 0 || c->graph == 1) && (c->fused == 0 || c->fused == 1) && (c->sleeping == 0 || c->sleeping == 1) &&
+	       c->sleep_tick_threshold >= 1 && c->sleep_delay >= 0 && c->sleep_delay / c->dt < 1e5 && std::isfinite(c->sleep_vel_coeff) &&
+	       std::isfinite(c->sleep_ang_vel_eps) && std::isfinite(c->sleep_accel_coeff);
 }
 
 int cp_world_create(const cp_config* cfg, cp_world** out) {
@@ -2228,13 +2378,15 @@ int cp_world_get_contact_count(cp_world* w, int* n) {
 		HIPCHK(hipStreamSynchronize(w->stream));
 		std::vector<int> life;
 		down(life, w->cb[w->cur].c.life, w->ncontacts);
+		std::vector<char> gone = w->removed_contacts(life);
 		int c = 0;
-		for (int v : life) c += v >= -1;
+		for (char g2 : gone) c += !g2;
 		*n = c;
 	});
 }
 
-// Contacts that the reference would still hold (memory_life >= -1), in cache order.
+// Contacts that the reference would still hold (not removed at the last
+// step's first substep), in cache order.
 int cp_world_get_contacts(cp_world* w, int cap, int64_t* ii, uint64_t* mg, double* ff, int* nout) {
 	if (!w || !nout) return CP_ERR_INVALID;
 	int status = CP_OK;
@@ -2250,10 +2402,11 @@ int cp_world_get_contacts(cp_world* w, int cap, int64_t* ii, uint64_t* mg, doubl
 		down(fu, C.fu, 3 * (size_t)K); down(fw, C.fw, 3 * (size_t)K); down(imp_t, C.imp_t, 2 * (size_t)K);
 		down(pen, C.pen, n); down(bounce, C.bounce, n); down(mu, C.mu, n); down(imp_n, C.imp_n, n);
 		down(life, C.life, n); down(live, C.live, n); down(sr, C.sready, n);
+		std::vector<char> gone = w->removed_contacts(life);
 		int o = 0, pos = 0;
 		uint64_t prev = ~0ull;
 		for (int k = 0; k < n; k++) {
-			if (life[k] < -1) continue;
+			if (gone[k]) continue;
 			if (key[k] != prev) pos = 0;
 			prev = key[k];
 			if (o < cap && ii && mg && ff) {
@@ -2785,6 +2938,56 @@ int cp_world_scene_flags(cp_world* w, int cap, int* flags, int* n) {
 			flags[s2] = 0;
 			for (int i = sbo[s2]; i < sbo[s2 + 1]; i++) flags[s2] |= f[i];
 		}
+	});
+}
+
+int cp_world_get_sleep_state(cp_world* w, double* out4) {
+	if (!w || !out4) return CP_ERR_INVALID;
+	if (!w->uploaded) return CP_ERR_STATE;
+	return guard(w, [&] {
+		HIPCHK(hipStreamSynchronize(w->stream));
+		std::vector<int> a, t, h;
+		std::vector<double> c;
+		down(a, w->sl_asleep.p, w->nb); down(t, w->sl_ticks.p, w->nb); down(h, w->sl_hlen.p, w->nb); down(c, w->sl_counter.p, w->nb);
+		for (int i = 0; i < w->nb; i++) {
+			out4[4 * (size_t)i] = a[i] && w->b_move[i] == 0 ? 1.0 : 0.0;
+			out4[4 * (size_t)i + 1] = c[i];
+			out4[4 * (size_t)i + 2] = (double)t[i];
+			out4[4 * (size_t)i + 3] = (double)h[i];
+		}
+	});
+}
+
+int cp_world_wake_bodies(cp_world* w, int n, const int* idx) {
+	if (!w || n < 0 || (n > 0 && !idx)) return CP_ERR_INVALID;
+	for (int k = 0; k < n; k++)
+		if (idx[k] < 0 || idx[k] >= w->nb) return CP_ERR_INVALID;
+	return guard(w, [&] {
+		w->sl_wake.insert(w->sl_wake.end(), idx, idx + n);
+		if (!w->uploaded || !n) return;
+		// RigidBody::alertWakingAction (RigidBody.cpp:340-345): counters and
+		// history reset at once
+		HIPCHK(hipStreamSynchronize(w->stream));
+		std::vector<int> t, h;
+		std::vector<double> c;
+		down(t, w->sl_ticks.p, w->nb); down(h, w->sl_hlen.p, w->nb); down(c, w->sl_counter.p, w->nb);
+		for (int k = 0; k < n; k++) {
+			t[idx[k]] = 0;
+			h[idx[k]] = 0;
+			c[idx[k]] = 0;
+		}
+		up(w->sl_ticks, t); up(w->sl_hlen, h); up(w->sl_counter, c);
+	});
+}
+
+int cp_world_set_sleep_disabled(cp_world* w, int n, const int* idx, int disabled) {
+	if (!w || n < 0 || (n > 0 && !idx)) return CP_ERR_INVALID;
+	for (int k = 0; k < n; k++)
+		if (idx[k] < 0 || idx[k] >= w->nb) return CP_ERR_INVALID;
+	return guard(w, [&] {
+		w->sl_dis_host.resize(w->nb, 0);
+		for (int k = 0; k < n; k++) w->sl_dis_host[idx[k]] = disabled ? 1 : 0;
+		if (w->uploaded) up(w->sl_disabled, w->sl_dis_host);
 	});
 }
 

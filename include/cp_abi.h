@@ -99,6 +99,22 @@ typedef struct {
 	                            between scenes.  0: one launch per stage and substep.  Default 0 (the
 	                            fused kernel holds 2 waves per SIMD against the solve's 4, and is faster
 	                            only on small single scenes).  Does not change results. */
+	int sleeping;            /* 1: bodies fall asleep and wake as the reference's do (setSleepingEnabled,
+	                            PhysicsEngine.cpp:1138-1140): movement history and sleep counters at the
+	                            step start (:65-87, bodySleepy :1221-1254), an island whose bodies are all
+	                            ready to sleep sleeps and is not solved (:1598-1606), an asleep body that
+	                            a solve moves faster than 0.25 * sleep_vel_coeff * |g| wakes its island
+	                            (:1124-1135), asleep bodies are not integrated and their contacts do not
+	                            expire (PersistentConstraint.h:74).  Island modes only: a
+	                            CP_PGS_COLORED step with sleeping on returns CP_ERR_STATE.  Default 0
+	                            (the reference's default is on). */
+	int sleep_tick_threshold; /* non_sleepy_tick_threshold (PhysicsEngine.h:220); default 3 */
+	double sleep_vel_coeff;   /* vel_sleep_coeff (PhysicsEngine.h:222); default 0.1 */
+	double sleep_ang_vel_eps; /* ang_vel_eps; default 0.1 */
+	double sleep_accel_coeff; /* accel_sleep_coeff; default 0.044 */
+	double sleep_delay;       /* sleep_delay (s) (PhysicsEngine.h:219); default 0.5.  The reference's
+	                            setSleepParameters takes a sleep_assesment_time it never stores
+	                            (PhysicsEngine.cpp:1186-1191), so its delay stays 0.5 */
 } cp_config;
 
 typedef struct cp_world cp_world;
@@ -375,6 +391,20 @@ int cp_world_scene_snapshot(cp_world* w, int handle, int cap_contacts, double* s
 int cp_world_admit_scene_resume(cp_world* w, int n, const int* shape_ids, const int* movement, const double* mp,
                                 const double* material, const double* state13, const double* inertia18, int n_contacts,
                                 const int64_t* ints6, const uint64_t* magic2, const double* f21, int* handle);
+
+/* Sleeping (cp_config.sleeping).
+ * cp_world_get_sleep_state: per body asleep (RigidBody::getAsleep), sleep_ready_counter,
+ *   non_sleepy_tick_count and the length of its movement history, as out4[n][4].
+ * cp_world_wake_bodies: RigidBody::alertWakingAction (RigidBody.cpp:340-345) for the listed
+ *   bodies (what the reference's setters setVel, setPos, setOrientation, ... call): their
+ *   counters and histories reset now, and at the next step start each one's island wakes
+ *   (PhysicsEngine.cpp:66-69) - the island of the constraint graph as it stands then.
+ * cp_world_set_sleep_disabled: RigidBody::setSleepDisabled for the listed bodies (a body with
+ *   sleep disabled is never sleepy, so its island never sleeps).  The reference leaves the
+ *   flag uninitialised in its constructors (RigidBody.h:89); here it starts at 0. */
+int cp_world_get_sleep_state(cp_world* w, double* out4);
+int cp_world_wake_bodies(cp_world* w, int n, const int* idx);
+int cp_world_set_sleep_disabled(cp_world* w, int n, const int* idx, int disabled);
 
 /* Batched raycasts (PhysicsEngine::raycastFirstIntersection, PhysicsEngine.cpp:
  * 582-609, for n rays at once) against the world geometry of the last step:

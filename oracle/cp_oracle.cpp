@@ -2555,8 +2555,8 @@ static void wakeIsland(World& W, int i) {
 	}
 }
 
-// Returns 0, or -1 (nothing stepped) when a schedule's groups do not match
-// the islands of a sleeping world.
+// Returns 0, or -1 when a schedule's groups do not match the islands of a
+// sleeping world.
 static int step(World& W, const int64_t* node_order, const int64_t* sched_rows = nullptr, const int64_t* sched_offs = nullptr,
                 int sched_n = 0) {
 	const int N = (int)W.bodies.size();
@@ -2709,8 +2709,12 @@ static int step(World& W, const int64_t* node_order, const int64_t* sched_rows =
 	}
 	if (sched_rows) {
 		// explicit sweep order over the post-narrowphase cache (one row list per
-		// group); with sleeping the groups are the islands above, in order
+		// group); with sleeping the groups are the islands above (matched by
+		// the island of each group's first dynamic body)
 		if (W.sleeping && sched_n != (int)islands.size()) return -1;
+		std::vector<int> isl_of(N, -1);
+		for (size_t k = 0; k < isl_bodies.size(); k++)
+			for (int i : isl_bodies[k]) isl_of[i] = (int)k;
 		std::vector<std::vector<std::pair<Contact*, int>>> g_rows;
 		std::vector<std::vector<int>> g_sys, g_bodies;
 		W.last_rows.clear();
@@ -2729,7 +2733,14 @@ static int step(World& W, const int64_t* node_order, const int64_t* sched_rows =
 					    std::find(sys.begin(), sys.end(), W.joints[r.second].sys) == sys.end())
 						sys.push_back(W.joints[r.second].sys);
 				g_sys.push_back(sys);
-				g_bodies.push_back(W.sleeping ? isl_bodies[g] : std::vector<int>());
+				int first = -1;
+				for (auto& r : rows) {
+					int ends[2] = {r.first ? r.first->b1 : W.joints[r.second].a, r.first ? r.first->b2 : W.joints[r.second].b};
+					for (int e : ends)
+						if (first < 0 && W.bodies[e].dynamic()) first = e;
+				}
+				if (W.sleeping && (first < 0 || isl_of[first] < 0)) return -1;
+				g_bodies.push_back(W.sleeping ? isl_bodies[isl_of[first]] : std::vector<int>());
 			}
 		}
 		islands.swap(g_rows);

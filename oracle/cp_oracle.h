@@ -137,9 +137,10 @@ void orc_step(orc_world* w, const int64_t* node_order);
 // One full step with an explicit sweep schedule: n_groups row lists
 // (rows[2*r] = index into the post-narrowphase cache, rows[2*r+1] = 0 normal /
 // 1 friction, 2 joint; offs[n_groups+1]), solved as PGS_solve solves one
-// island each.  With sleeping on, the groups must be the step's islands in
-// order (their sleep decisions and wake lists come from the island search);
-// returns -1 without stepping when the count differs, else 0.
+// island each.  With sleeping on, the groups must be the step's islands
+// (their sleep decisions and wake lists come from the island search; a
+// group is matched to the island of its first dynamic body); returns -1 when
+// they are not, else 0.
 int orc_step_scheduled(orc_world* w, int n_groups, const int64_t* rows, const int64_t* offs);
 
 // Sleeping (PhysicsEngine::setSleepingEnabled / setSleepParameters,
