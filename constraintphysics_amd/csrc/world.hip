@@ -1832,9 +1832,9 @@ uint64_t fnv1a(uint64_t h, const void* p, size_t n) {
 	return h;
 }
 struct SnapFile {  // a snapshot as written: counts, then the sections in file order
-	int64_t counts[4] = {0, 0, 0, 0};  // bodies, contacts, joints, 0
-	std::vector<double> state, inertia, cf, joint;
-	std::vector<int64_t> ci;
+	int64_t counts[4] = {0, 0, 0, 0};  // bodies, contacts, joints, sleep (0 none, else history slots + 1)
+	std::vector<double> state, inertia, cf, joint, sl_counter, sl_hist;
+	std::vector<int64_t> ci, sl_ints;
 	std::vector<uint64_t> cm;
 };
 }  // namespace
@@ -2592,6 +2592,27 @@ int cp_world_save_snapshot(cp_world* w, const char* path) {
 	f.counts[0] = nb;
 	f.counts[1] = nc;
 	f.counts[2] = nj;
+	if (w->sl_active) {
+		// sleep state: per body asleep, asleep at the last removal, ticks,
+		// history length and head, sleep disabled; counters; the history ring
+		st = guard(w, [&] {
+			HIPCHK(hipStreamSynchronize(w->stream));
+			const int hc = std::max(w->sl_hcap, 0);
+			std::vector<int> a, rm, t, hl, hh;
+			down(a, w->sl_asleep.p, nb); down(rm, w->sl_asleep_rm.p, nb); down(t, w->sl_ticks.p, nb);
+			down(hl, w->sl_hlen.p, nb); down(hh, w->sl_hhead.p, nb);
+			down(f.sl_counter, w->sl_counter.p, nb);
+			if (hc) down(f.sl_hist, w->sl_hist.p, 6 * (size_t)hc * nb);
+			w->sl_dis_host.resize(nb, 0);
+			f.sl_ints.resize(6 * (size_t)nb);
+			for (int i = 0; i < nb; i++) {
+				int64_t* r = &f.sl_ints[6 * (size_t)i];
+				r[0] = a[i]; r[1] = rm[i]; r[2] = t[i]; r[3] = hl[i]; r[4] = hh[i]; r[5] = w->sl_dis_host[i];
+			}
+			f.counts[3] = hc + 1;
+		});
+		if (st != CP_OK) return st;
+	}
 	FILE* fp = fopen(path, "wb");
 	if (!fp) return CP_ERR_IO;
 	uint64_t h = 14695981039346656037ull;
@@ -2609,6 +2630,9 @@ int cp_world_save_snapshot(cp_world* w, const char* path) {
 	put(f.cm.data(), f.cm.size() * 8);
 	put(f.cf.data(), f.cf.size() * 8);
 	put(f.joint.data(), f.joint.size() * 8);
+	put(f.sl_ints.data(), f.sl_ints.size() * 8);
+	put(f.sl_counter.data(), f.sl_counter.size() * 8);
+	put(f.sl_hist.data(), f.sl_hist.size() * 8);
 	ok = ok && fwrite(&h, 8, 1, fp) == 1;
 	ok = (fclose(fp) == 0) && ok;
 	return ok ? CP_OK : CP_ERR_IO;
@@ -2630,7 +2654,8 @@ int cp_world_load_snapshot(cp_world* w, const char* path) {
 	get(magic, 8);
 	ok = ok && memcmp(magic, SNAP_MAGIC, 8) == 0;
 	get(f.counts, sizeof f.counts);
-	ok = ok && f.counts[0] == w->nb && f.counts[2] == (int64_t)w->j_type.size() && f.counts[1] >= 0 && f.counts[1] < (int64_t)1 << 31;
+	ok = ok && f.counts[0] == w->nb && f.counts[2] == (int64_t)w->j_type.size() && f.counts[1] >= 0 && f.counts[1] < (int64_t)1 << 31 &&
+	     f.counts[3] >= 0 && f.counts[3] <= 100001;
 	if (ok) {
 		size_t nb = (size_t)f.counts[0], nc = (size_t)f.counts[1], nj = (size_t)f.counts[2];
 		f.state.resize(13 * nb); f.inertia.resize(18 * nb);
@@ -2642,6 +2667,14 @@ int cp_world_load_snapshot(cp_world* w, const char* path) {
 		get(f.cm.data(), f.cm.size() * 8);
 		get(f.cf.data(), f.cf.size() * 8);
 		get(f.joint.data(), f.joint.size() * 8);
+		if (f.counts[3] > 0) {
+			f.sl_ints.resize(6 * nb);
+			f.sl_counter.resize(nb);
+			f.sl_hist.resize(6 * (size_t)(f.counts[3] - 1) * nb);
+			get(f.sl_ints.data(), f.sl_ints.size() * 8);
+			get(f.sl_counter.data(), f.sl_counter.size() * 8);
+			get(f.sl_hist.data(), f.sl_hist.size() * 8);
+		}
 		uint64_t want = 0;
 		ok = ok && fread(&want, 8, 1, fp) == 1 && want == h;
 	}
@@ -2650,6 +2683,24 @@ int cp_world_load_snapshot(cp_world* w, const char* path) {
 	int st = cp_world_set_state(w, f.state.data(), f.inertia.data());
 	if (st == CP_OK) st = cp_world_set_contacts(w, (int)f.counts[1], f.ci.data(), f.cm.data(), f.cf.data());
 	if (st == CP_OK && f.counts[2] > 0) st = cp_world_set_joint_state(w, f.joint.data());
+	if (st == CP_OK && f.counts[3] > 0)
+		st = guard(w, [&] {
+			const int nb = w->nb, hc = (int)f.counts[3] - 1;
+			std::vector<int> a(nb), rm(nb), t(nb), hl(nb), hh(nb);
+			w->sl_dis_host.assign(nb, 0);
+			for (int i = 0; i < nb; i++) {
+				const int64_t* r = &f.sl_ints[6 * (size_t)i];
+				a[i] = (int)r[0]; rm[i] = (int)r[1]; t[i] = (int)r[2]; hl[i] = (int)r[3]; hh[i] = (int)r[4];
+				w->sl_dis_host[i] = (int)r[5];
+			}
+			up(w->sl_asleep, a); up(w->sl_asleep_rm, rm); up(w->sl_ticks, t); up(w->sl_hlen, hl); up(w->sl_hhead, hh);
+			up(w->sl_disabled, w->sl_dis_host); up(w->sl_counter, f.sl_counter);
+			w->sl_hist.reserve(6 * (size_t)std::max(hc, 1) * nb);
+			if (hc) HIPCHK(hipMemcpy(w->sl_hist.p, f.sl_hist.data(), f.sl_hist.size() * 8, hipMemcpyHostToDevice));
+			w->sl_hcap = hc;
+			w->sl_active = true;
+			w->sl_wake.clear();
+		});
 	return st;
 }
 

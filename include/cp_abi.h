@@ -357,12 +357,17 @@ int cp_world_serving_info(cp_world* w, int* info4);
  * cp_world_save_snapshot / cp_world_load_snapshot: the whole dynamic state of
  *   a world in one file, enough to resume it bit for bit in a world built
  *   with the same bodies (and joints): "CPSNAP1\n", int64 counts[4] =
- *   (bodies, contacts, joints, 0), then state13[bodies], inertia18[bodies],
+ *   (bodies, contacts, joints, sleep), then state13[bodies], inertia18[bodies],
  *   contact rows (ints6, magic2, f21 per contact, the cp_world_get_contacts
- *   layout), joint state11[joints], and a 64-bit FNV-1a checksum of every
- *   byte before it; all little-endian.  Loading checks the magic, the
+ *   layout), joint state11[joints]; when sleep = H + 1 > 0 (the world has
+ *   had sleeping on), the sleep state: int64 [bodies][6] (asleep, asleep at
+ *   the last contact removal, non_sleepy_tick_count, history length, history
+ *   head, sleep disabled), sleep_ready_counter[bodies], and the movement
+ *   history ring as doubles [H][6][bodies]; then a 64-bit FNV-1a checksum of
+ *   every byte before it; all little-endian.  Loading checks the magic, the
  *   checksum and the body and joint counts (CP_ERR_IO otherwise) and sets
- *   state, inertia, contacts and joint state.
+ *   state, inertia, contacts, joint state and sleep state (pending waking
+ *   actions are not saved).
  * cp_world_trajectory_open: from the next cp_world_step on, appends one frame
  *   per step to `path` (created or truncated): per body, in index order,
  *   com xyz and orientation r i j k as doubles after the step.

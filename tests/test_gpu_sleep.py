@@ -105,3 +105,28 @@ def test_sleep_disabled_body_keeps_its_island_awake(fx):
     s = w.sleep_state()
     assert s[9, 0] == 0 and s[9, 1] == 0 and s[1:9, 0].all()
     w.close()
+
+
+def test_snapshot_resumes_sleep_state(fx, tmp_path):
+    """A snapshot taken mid-run (bodies asleep, histories part full) resumes
+    bit for bit: the loaded world then follows the reference's sleep trace,
+    through the column's wake at step 314."""
+    w = world_from_fixture(fx, pgs_mode=1, sleeping=1)
+    w.set_state(fx["initial_state"])
+    for k in range(300):
+        if k == int(fx.meta["wake_step"]):
+            w.wake_bodies([int(fx.meta["wake_body"])])
+        w.step(1)
+    path = str(tmp_path / "sleep.cps")
+    w.save_snapshot(path)
+    w.close()
+    v = world_from_fixture(fx, pgs_mode=1, sleeping=1)
+    v.set_state(fx["initial_state"])
+    v.load_snapshot(path)
+    tr = fx["sleep_trace"]
+    assert np.array_equal(v.sleep_state(), tr[299])
+    for k in range(300, int(fx.meta["steps"])):
+        v.step(1)
+        assert np.array_equal(v.sleep_state(), tr[k]), f"step {k}"
+    assert np.array_equal(v.state(), fx["final_state"])
+    v.close()
