@@ -1613,18 +1613,19 @@ struct Contact {
 static double posCorrectCoeff(double s, double dt) { return std::min<double>(s * dt, 1.0 / dt); }  // PersistentConstraint.cpp:199
 
 struct Joint {
-	int type;       // 3 ball-socket, 5 hinge
+	int kind;       // 1 distance, 3 ball-socket, 4 sliding hinge, 5 hinge, 6 weld, 7 slider
+	int type;       // degree (rows): 1, 3, 4, 5, 6, 5
 	int a, b;       // body indices (b1, b2 as created)
 	V3 tp1, tp2;    // tracked points in the body frames (RigidBody.cpp:134-144)
-	V3 ax1, ax2;    // hinge axes in the body frames
+	V3 ax1, ax2;    // hinge / slide axes in the body frames; distance: ax1.x = target distance
 	double ps, rs;  // position / rotation correction strengths
-	double imp[5] = {}, pimp[5] = {}, himp[5] = {};
-	V3 u, w;        // hinge basis of the current solver object
+	double imp[6] = {}, pimp[6] = {}, himp[6] = {};
+	V3 u, w;        // basis of the current solver object (hinge, slider, sliding hinge)
 	int sys = -1;   // holonomic system, -1 none
-	NM<5, 6> ja, jb;
-	NM<6, 5> ita, itb;
-	NM<5, 5> inv;
-	double tgt[5] = {}, ptgt[5] = {};
+	NM<6, 6> ja, jb;
+	NM<6, 6> ita, itb;
+	NM<6, 6> inv;
+	double tgt[6] = {}, ptgt[6] = {};
 };
 
 struct HSys {
@@ -1784,6 +1785,50 @@ static void inverseN(int n, const double* src, double* dst) {
 // The solver constraint of a joint from the current body state.  source
 // impulse / basis: the previous object's (warm start with coefficient 1.0,
 // hinge angular impulses re-projected onto the new basis, ConstraintSolver.cpp:475-476).
+// Quaternion::angleTo (Quaternion.h:35-39) and the rotation correction of
+// the weld and slider rows (ConstraintSolver.cpp:688-696, :888-896): the
+// rotation taking a's orientation to b's, as axis * angle, scaled by h.
+static V3 rotCorrect(const Q& a, const Q& b, double h) {
+	Q diff = b * a.conj();
+	double dot = std::abs(a.r * b.r + a.i * b.i + a.j * b.j + a.k * b.k);
+	double angle = dot > 0.99999999999 ? 0 : 2 * std::acos(std::abs(a.r * b.r + a.i * b.i + a.j * b.j + a.k * b.k));
+	V3 axis;
+	if (angle != 0) {  // getRotAxis (Quaternion.h:41-49)
+		V3 ax(diff.i, diff.j, diff.k);
+		axis = (diff.r < 0 ? -ax.normalize() : ax.normalize()) * h;
+	}
+	return angle * axis;
+}
+static NM<1, 3> rowNM(V3 v) {
+	NM<1, 3> o;
+	o.v[0][0] = v.x; o.v[0][1] = v.y; o.v[0][2] = v.z;
+	return o;
+}
+template <int R, int C>
+static NM<R, C> subNM(const NM<R, C>& a, const NM<R, C>& b) {  // NMat.h:64-72
+	NM<R, C> o;
+	for (int i = 0; i < R; i++) for (int k = 0; k < C; k++) o.v[i][k] = a.v[i][k] - b.v[i][k];
+	return o;
+}
+// copyInto of a 1x3 row (NMat.h:36-42)
+static void putRow(NM<6, 6>& m, int r, int c0, const NM<1, 3>& x) {
+	for (int c = 0; c < 3; c++) m.v[r][c0 + c] = x.v[0][c];
+}
+static void putBlock(NM<6, 6>& m, int r0, int c0, const NM<3, 3>& x) {
+	for (int r = 0; r < 3; r++) for (int c = 0; c < 3; c++) m.v[r0 + r][c0 + c] = x.v[r][c];
+}
+static NM<3, 3> idenNM() {  // idenMat<3> (NMat.h:110-119)
+	NM<3, 3> o;
+	for (int r = 0; r < 3; r++) o.v[r][r] = 1;
+	return o;
+}
+
+// Solver rows of one joint from the current body state: the joint's
+// updateSolverConstraints (PersistentConstraint.cpp:217-354, warm start
+// coefficient 1) and its constraint's constructor (ConstraintSolver.cpp:
+// 348-900).  reproject: a basis-carrying joint re-projects its stored
+// impulses onto the new basis (false for holonomic rows rebuilt from a
+// stored state).
 static void buildJoint(World& W, Joint& j, double sub_dt, bool reproject) {
 	const Body& A = W.bodies[j.a];
 	const Body& B = W.bodies[j.b];
@@ -1791,63 +1836,142 @@ static void buildJoint(World& W, Joint& j, double sub_dt, bool reproject) {
 	V3 pb = B.com + B.q.rotate(j.tp2);
 	double cfm = W.cfg.cfm;
 	int n = j.type;
-	double start[5];
+	double start[6];
 	for (int k = 0; k < n; k++) start[k] = j.imp[k] * 1.0;  // warm_start_coefficient * impulse
 	NM<6, 6> Ma = invMass6(A.invMass(), A.invTensor()), Mb = invMass6(B.invMass(), B.invTensor());
-	NM<5, 6> ja, jb;
+	NM<6, 6> ja, jb;
 	double pcc = posCorrectCoeff(j.ps, sub_dt);
-	if (n == 3) {
+	double rcc = posCorrectCoeff(j.rs, sub_dt);
+	for (int k = 0; k < n; k++) j.imp[k] = start[k];
+	switch (j.kind) {
+	case 1: {  // DistanceConstraint (ConstraintSolver.cpp:348-379), not in moving mode
 		V3 rA = pa - A.com, rB = pb - B.com;
-		NM<3, 3> sa = crossNM(rA), sb = crossNM(rB), nsa = neg(sa);
-		for (int r = 0; r < 3; r++) {
-			for (int c = 0; c < 3; c++) {
-				ja.v[r][c] = (r == c) ? 1.0 : 0.0;
-				jb.v[r][c] = -((r == c) ? 1.0 : 0.0);
-				ja.v[r][3 + c] = nsa.v[r][c];
-				jb.v[r][3 + c] = sb.v[r][c];
-			}
-		}
+		V3 diff = pa - pb;
+		V3 dd = diff.normalize();
+		NM<1, 3> d = rowNM(dd);
+		putRow(ja, 0, 0, d);
+		putRow(ja, 0, 3, mul(neg(d), crossNM(rA)));
+		putRow(jb, 0, 0, neg(d));
+		putRow(jb, 0, 3, mul(d, crossNM(rB)));
+		double target = j.ax1.x;
+		double error = diff.magSqrd() < 0.00001 ? target : target - diff.normalize().dot(diff);
+		j.ptgt[0] = pcc * error;
+		break;
+	}
+	case 3: {  // BallSocketConstraint (:382-408)
+		V3 rA = pa - A.com, rB = pb - B.com;
+		putBlock(ja, 0, 0, idenNM());
+		putBlock(ja, 0, 3, neg(crossNM(rA)));
+		putBlock(jb, 0, 0, neg(idenNM()));
+		putBlock(jb, 0, 3, crossNM(rB));
 		V3 err = pb - pa;
 		j.ptgt[0] = pcc * err.x; j.ptgt[1] = pcc * err.y; j.ptgt[2] = pcc * err.z;
-		for (int k = 0; k < 3; k++) j.imp[k] = start[k];
-	} else {
+		break;
+	}
+	case 5: {  // HingeConstraint (:468-553)
 		V3 axa = A.q.rotate(j.ax1), axb = B.q.rotate(j.ax2);
-		double rcc = posCorrectCoeff(j.rs, sub_dt);
 		V3 su = j.u, sw = j.w;
 		V3 rA = pa - A.com, rB = pa - B.com;  // both from the b1 point (ConstraintSolver.cpp:469)
 		V3 u, w;
 		axa.perpBasis(&u, &w);
 		V3 nn = axb;
-		for (int k = 0; k < 5; k++) j.imp[k] = start[k];
 		if (reproject) {
 			j.imp[3] = u.dot(su) * start[3] + u.dot(sw) * start[4];
 			j.imp[4] = w.dot(su) * start[3] + w.dot(sw) * start[4];
 		}
 		j.u = u;
 		j.w = w;
-		NM<3, 3> sa = crossNM(rA), sb = crossNM(rB), nsa = neg(sa);
-		NM<3, 3> us = crossNM(u), ws = crossNM(w), ns = crossNM(nn);
-		NM<1, 3> ud, wd, nd;
-		ud.v[0][0] = u.x; ud.v[0][1] = u.y; ud.v[0][2] = u.z;
-		wd.v[0][0] = w.x; wd.v[0][1] = w.y; wd.v[0][2] = w.z;
-		nd.v[0][0] = nn.x; nd.v[0][1] = nn.y; nd.v[0][2] = nn.z;
-		NM<1, 3> a3 = mul(nd, us), b3 = mul(ud, ns), a4 = mul(nd, ws), b4 = mul(wd, ns);
-		for (int r = 0; r < 3; r++)
-			for (int c = 0; c < 3; c++) {
-				ja.v[r][c] = (r == c) ? 1.0 : 0.0;
-				jb.v[r][c] = -((r == c) ? 1.0 : 0.0);
-				ja.v[r][3 + c] = nsa.v[r][c];
-				jb.v[r][3 + c] = sb.v[r][c];
-			}
-		for (int c = 0; c < 3; c++) {
-			ja.v[3][c] = 0; jb.v[3][c] = 0; ja.v[4][c] = 0; jb.v[4][c] = 0;
-			ja.v[3][3 + c] = a3.v[0][c]; jb.v[3][3 + c] = b3.v[0][c];
-			ja.v[4][3 + c] = a4.v[0][c]; jb.v[4][3 + c] = b4.v[0][c];
-		}
+		putBlock(ja, 0, 0, idenNM());
+		putBlock(ja, 0, 3, neg(crossNM(rA)));
+		putBlock(jb, 0, 0, neg(idenNM()));
+		putBlock(jb, 0, 3, crossNM(rB));
+		putRow(ja, 3, 3, mul(rowNM(nn), crossNM(u)));
+		putRow(jb, 3, 3, mul(rowNM(u), crossNM(nn)));
+		putRow(ja, 4, 3, mul(rowNM(nn), crossNM(w)));
+		putRow(jb, 4, 3, mul(rowNM(w), crossNM(nn)));
 		V3 pc = (pb - pa) * pcc;
 		j.ptgt[0] = pc.x; j.ptgt[1] = pc.y; j.ptgt[2] = pc.z;
 		j.ptgt[3] = u.dot(nn) * rcc;
 		j.ptgt[4] = w.dot(nn) * rcc;
+		break;
+	}
+	case 7: {  // SliderConstraint (:611-697); Slider passes the position coefficient for both (PersistentConstraint.cpp:297)
+		V3 axis = A.q.rotate(j.ax1).normalize();
+		V3 u, w;
+		axis.perpBasis(&u, &w);
+		V3 su = j.u, sw = j.w;
+		if (reproject) {
+			j.imp[0] = u.dot(su) * start[0] + u.dot(sw) * start[1];
+			j.imp[1] = w.dot(su) * start[0] + w.dot(sw) * start[1];
+		}
+		j.u = u;
+		j.w = w;
+		V3 rA = pa - A.com, rB = pb - B.com;
+		V3 pe = pa - pb;
+		NM<1, 3> ud = rowNM(u), wd = rowNM(w), ped = rowNM(pe);
+		putRow(ja, 0, 0, ud);
+		putRow(ja, 0, 3, subNM(mul(neg(ud), crossNM(rA)), mul(ped, crossNM(u))));
+		putRow(jb, 0, 0, neg(ud));
+		putRow(jb, 0, 3, mul(ud, crossNM(rB)));
+		putRow(ja, 1, 0, wd);
+		putRow(ja, 1, 3, subNM(mul(neg(wd), crossNM(rA)), mul(ped, crossNM(w))));
+		putRow(jb, 1, 0, neg(wd));
+		putRow(jb, 1, 3, mul(wd, crossNM(rB)));
+		putBlock(ja, 2, 3, idenNM());
+		putBlock(jb, 2, 3, neg(idenNM()));
+		V3 rc = rotCorrect(A.q, B.q, pcc);
+		j.ptgt[0] = -pe.dot(u) * pcc;
+		j.ptgt[1] = -pe.dot(w) * pcc;
+		j.ptgt[2] = rc.x; j.ptgt[3] = rc.y; j.ptgt[4] = rc.z;
+		break;
+	}
+	case 4: {  // SlidingHingeConstraint (:781-857)
+		V3 axa = A.q.rotate(j.ax1), axb = B.q.rotate(j.ax2);
+		V3 u, w;
+		axb.perpBasis(&u, &w);
+		V3 nn = axa;
+		V3 su = j.u, sw = j.w;
+		if (reproject) {
+			j.imp[2] = u.dot(su) * start[2] + u.dot(sw) * start[3];
+			j.imp[3] = w.dot(su) * start[2] + w.dot(sw) * start[3];
+		}
+		j.u = u;
+		j.w = w;
+		V3 rA = pa - A.com, rB = pb - B.com;
+		V3 pd = pa - pb;
+		NM<1, 3> ud = rowNM(u), wd = rowNM(w), nd = rowNM(nn), pdd = rowNM(pd);
+		putRow(ja, 0, 0, ud);
+		putRow(ja, 0, 3, mul(neg(ud), crossNM(rA)));
+		putRow(jb, 0, 0, neg(ud));
+		putRow(jb, 0, 3, subNM(mul(ud, crossNM(rB)), mul(pdd, crossNM(u))));
+		putRow(ja, 1, 0, wd);
+		putRow(ja, 1, 3, mul(neg(wd), crossNM(rA)));
+		putRow(jb, 1, 0, neg(wd));
+		putRow(jb, 1, 3, subNM(mul(wd, crossNM(rB)), mul(pdd, crossNM(w))));
+		putRow(ja, 2, 3, mul(neg(ud), crossNM(nn)));
+		putRow(jb, 2, 3, mul(neg(nd), crossNM(u)));
+		putRow(ja, 3, 3, mul(neg(wd), crossNM(nn)));
+		putRow(jb, 3, 3, mul(neg(nd), crossNM(w)));
+		j.ptgt[0] = -u.dot(pd) * pcc;
+		j.ptgt[1] = -w.dot(pd) * pcc;
+		j.ptgt[2] = -u.dot(nn) * rcc;
+		j.ptgt[3] = -w.dot(nn) * rcc;
+		break;
+	}
+	default: {  // 6: WeldConstraint (:861-897)
+		V3 rA = pa - A.com, rB = pb - B.com;
+		putBlock(ja, 0, 0, idenNM());
+		putBlock(ja, 0, 3, neg(crossNM(rA)));
+		putBlock(ja, 3, 3, idenNM());
+		putBlock(jb, 0, 0, neg(idenNM()));
+		putBlock(jb, 0, 3, crossNM(rB));
+		putBlock(jb, 3, 3, neg(idenNM()));
+		V3 pc = (pb - pa) * pcc;
+		V3 rc = rotCorrect(A.q, B.q, rcc);
+		j.ptgt[0] = pc.x; j.ptgt[1] = pc.y; j.ptgt[2] = pc.z;
+		j.ptgt[3] = rc.x; j.ptgt[4] = rc.y; j.ptgt[5] = rc.z;
+		break;
+	}
 	}
 	// impulse_to_*_velocity = M^-1 J^T, impulse_to_value = J_a M_a^-1 J_a^T + J_b ...
 	for (int r = 0; r < 6; r++)
@@ -1857,7 +1981,7 @@ static void buildJoint(World& W, Joint& j, double sub_dt, bool reproject) {
 			j.ita.v[r][c] = s1;
 			j.itb.v[r][c] = s2;
 		}
-	double itv[25];
+	double itv[36];
 	for (int r = 0; r < n; r++)
 		for (int c = 0; c < n; c++) {
 			double s1 = 0, s2 = 0;
@@ -1865,7 +1989,7 @@ static void buildJoint(World& W, Joint& j, double sub_dt, bool reproject) {
 			itv[r * n + c] = s1 + s2;
 		}
 	for (int r = 0; r < n; r++) itv[r * n + r] *= (1 + cfm);  // applyCFM (ConstraintSolver.cpp:919-926)
-	double inv[25];
+	double inv[36];
 	inverseN(n, itv, inv);
 	for (int r = 0; r < n; r++)
 		for (int c = 0; c < n; c++) j.inv.v[r][c] = inv[r * n + c];
@@ -1998,8 +2122,8 @@ static void hsBlockInit(const World& W, double cfm, int c1, int c2, double* t) {
 		for (int i = 0; i < n1; i++) t[i + n1 * i] *= (1 + cfm);
 		return;
 	}
-	const NM<5, 6>* jac = nullptr;
-	const NM<6, 5>* it = nullptr;
+	const NM<6, 6>* jac = nullptr;
+	const NM<6, 6>* it = nullptr;
 	if (A.a == B.a) { jac = &A.ja; it = &B.ita; }
 	else if (A.a == B.b) { jac = &A.ja; it = &B.itb; }
 	else if (A.b == B.a) { jac = &A.jb; it = &B.ita; }
@@ -2132,7 +2256,7 @@ static void pgsIsland(World& W, const std::vector<std::pair<Contact*, int>>& row
 		Contact* c = r.first;
 		if (!c) {
 			Joint& j = W.joints[r.second];
-			double va[6], vb[6], val[5];
+			double va[6], vb[6], val[6];
 			bodyVel(j.a, va);
 			bodyVel(j.b, vb);
 			jointValue(j, va, vb, val);
@@ -2184,7 +2308,7 @@ static void pgsIsland(World& W, const std::vector<std::pair<Contact*, int>>& row
 		double* acc = pseudo ? j.pimp : j.imp;
 		const double* tgt = pseudo ? j.ptgt : j.tgt;
 		int n = j.type;
-		double cur[5], delta[5], m[5], diff[5];
+		double cur[6], delta[6], m[6], diff[6];
 		jointValue(j, va, vb, cur);
 		for (int k = 0; k < n; k++) delta[k] = tgt[k] - cur[k];
 		for (int i = 0; i < n; i++) {
@@ -2261,7 +2385,7 @@ static void pgsIsland(World& W, const std::vector<std::pair<Contact*, int>>& row
 			std::vector<double> d(hs.degree);
 			for (size_t i = 0; i < hs.order.size(); i++) {
 				Joint& j = W.joints[hs.order[i]];
-				double cur[5];
+				double cur[6];
 				jointValue(j, pseudo ? dv[j.a].p : dv[j.a].v, pseudo ? dv[j.b].p : dv[j.b].v, cur);
 				const double* tgt = pseudo ? j.ptgt : j.tgt;
 				for (int k = 0; k < j.type; k++) d[hs.vloc[i] + k] = tgt[k] - cur[k];
@@ -2940,7 +3064,10 @@ int orc_add_cylinder_body(orc_world* w, int movement, const double* mp, const do
 
 int orc_add_joint(orc_world* w, int type, int b1, int b2, const double* g) {
 	Joint j;
-	j.type = type;
+	static const int degree[8] = {0, 1, 0, 3, 4, 5, 6, 5};
+	if (type < 1 || type > 7 || degree[type] == 0) return -1;
+	j.kind = type;
+	j.type = degree[type];
 	j.a = b1;
 	j.b = b2;
 	j.tp1 = V3(g[0], g[1], g[2]);
@@ -3018,10 +3145,10 @@ int orc_narrowphase_multi(const orc_world* w, int id_lo, int id_hi, int cap, dou
 void orc_set_joint_state(orc_world* w, const double* s) {
 	for (size_t i = 0; i < w->joints.size(); i++) {
 		Joint& j = w->joints[i];
-		const double* r = s + 11 * i;
-		for (int k = 0; k < 5; k++) j.imp[k] = r[k];
-		j.u = V3(r[5], r[6], r[7]);
-		j.w = V3(r[8], r[9], r[10]);
+		const double* r = s + 12 * i;
+		for (int k = 0; k < 6; k++) j.imp[k] = r[k];
+		j.u = V3(r[6], r[7], r[8]);
+		j.w = V3(r[9], r[10], r[11]);
 	}
 	for (HSys& hs : w->systems) hs.pending = true;
 }
@@ -3029,10 +3156,10 @@ void orc_set_joint_state(orc_world* w, const double* s) {
 void orc_get_joint_state(const orc_world* w, double* s) {
 	for (size_t i = 0; i < w->joints.size(); i++) {
 		const Joint& j = w->joints[i];
-		double* r = s + 11 * i;
-		for (int k = 0; k < 5; k++) r[k] = k < j.type ? j.imp[k] : 0.0;
-		r[5] = j.u.x; r[6] = j.u.y; r[7] = j.u.z;
-		r[8] = j.w.x; r[9] = j.w.y; r[10] = j.w.z;
+		double* r = s + 12 * i;
+		for (int k = 0; k < 6; k++) r[k] = k < j.type ? j.imp[k] : 0.0;
+		r[6] = j.u.x; r[7] = j.u.y; r[8] = j.u.z;
+		r[9] = j.w.x; r[10] = j.w.y; r[11] = j.w.z;
 	}
 }
 

@@ -220,8 +220,10 @@ static void build_mixed(PhysicsEngine& E, int nx, int ny, int nz, uint64_t seed,
 // correction stiffnesses, PhysicsEngine.h:15-16).  Links: r 0.15, h 0.8, gap
 // 0.1; chain c at z = 1.2 c, height U[0.4, 1.2], x offset U[-0.5, 0.5]; every link
 // gets v ~ U[-0.5, 0.5]^3 so the chain bends as it falls onto the ground box.
+// Joint kinds as the dumps number them: 1 distance, 3 ball-socket, 4 sliding
+// hinge, 5 hinge, 6 weld, 7 slider.
 struct JointRec {
-	int type;  // 3 ball-socket, 5 hinge
+	int type;
 	RigidBody *b1, *b2;
 	phyz::PersistentConstraint* pc;
 	phyz::Constraint* solver;
@@ -266,17 +268,31 @@ static void build_chains(PhysicsEngine& E, int n_chains, int n_links, uint64_t s
 	}
 }
 
-// Joint solver state: impulse (5, zero padded), hinge u and w (zeros for ball-sockets).
+// Joint solver state: impulse (6, zero padded), the basis u and w of hinges,
+// sliders and sliding hinges (zeros for the others).
+template <class C>
+static void put_impulse(std::vector<double>& s, const C& c, int n) {
+	for (int k = 0; k < 6; k++) s.push_back(k < n ? c.impulse.v[k] : 0.0);
+}
+template <class C>
+static void put_joint(std::vector<double>& s, const C& c, int n, bool) {
+	put_impulse(s, c, n);
+	s.insert(s.end(), {0, 0, 0, 0, 0, 0});
+}
+template <class C>
+static void put_joint_basis(std::vector<double>& s, const C& c, int n) {
+	put_impulse(s, c, n);
+	s.insert(s.end(), {c.u.x, c.u.y, c.u.z, c.w.x, c.w.y, c.w.z});
+}
 static void joint_state(std::vector<double>& s) {
 	for (auto& j : g_joints) {
-		if (j.type == 5) {
-			auto* h = static_cast<phyz::Hinge*>(j.pc);
-			for (int k = 0; k < 5; k++) s.push_back(h->constraint.impulse.v[k]);
-			s.insert(s.end(), {h->constraint.u.x, h->constraint.u.y, h->constraint.u.z, h->constraint.w.x, h->constraint.w.y, h->constraint.w.z});
-		} else {
-			auto* b = static_cast<phyz::BallSocket*>(j.pc);
-			for (int k = 0; k < 3; k++) s.push_back(b->constraint.impulse.v[k]);
-			s.insert(s.end(), {0, 0, 0, 0, 0, 0, 0, 0});
+		switch (j.type) {
+		case 1: put_joint(s, static_cast<phyz::Distance*>(j.pc)->constraint, 1, false); break;
+		case 3: put_joint(s, static_cast<phyz::BallSocket*>(j.pc)->constraint, 3, false); break;
+		case 4: put_joint_basis(s, static_cast<phyz::SlidingHinge*>(j.pc)->constraint, 4); break;
+		case 5: put_joint_basis(s, static_cast<phyz::Hinge*>(j.pc)->constraint, 5); break;
+		case 6: put_joint(s, static_cast<phyz::Weld*>(j.pc)->constraint, 6, false); break;
+		default: put_joint_basis(s, static_cast<phyz::Slider*>(j.pc)->constraint, 5); break;
 		}
 	}
 }
@@ -570,18 +586,49 @@ static void dump_static(PhysicsEngine& E, Out& o) {
 	o.i64("hull_edge_off", eoff, {N + 1});
 	std::vector<int64_t> ji;
 	std::vector<double> jg;
+	// geometry per joint: tracked points (body frames), two body-frame axes
+	// (hinge axes, slide axes; a distance joint's target distance in the first
+	// slot), position and rotation correction strengths
+	auto geo = [&](const JointRec& j, int k1, int k2, mthz::Vec3 a1, mthz::Vec3 a2, double ps, double rs) {
+		mthz::Vec3 t1 = j.b1->track_p[k1], t2 = j.b2->track_p[k2];
+		jg.insert(jg.end(), {t1.x, t1.y, t1.z, t2.x, t2.y, t2.z, a1.x, a1.y, a1.z, a2.x, a2.y, a2.z, ps, rs});
+	};
 	for (auto& j : g_joints) {
 		ji.insert(ji.end(), {(int64_t)j.type, (int64_t)j.b1->getID(), (int64_t)j.b2->getID()});
-		if (j.type == 5) {
-			auto* h = static_cast<phyz::Hinge*>(j.pc);
-			mthz::Vec3 t1 = j.b1->track_p[h->b1_point_key], t2 = j.b2->track_p[h->b2_point_key];
-			jg.insert(jg.end(), {t1.x, t1.y, t1.z, t2.x, t2.y, t2.z, h->b1_rot_axis_body_space.x, h->b1_rot_axis_body_space.y,
-			                     h->b1_rot_axis_body_space.z, h->b2_rot_axis_body_space.x, h->b2_rot_axis_body_space.y,
-			                     h->b2_rot_axis_body_space.z, h->pos_correct_hardness, h->rot_correct_hardness});
-		} else {
+		switch (j.type) {
+		case 1: {
+			auto* d = static_cast<phyz::Distance*>(j.pc);
+			geo(j, d->b1_point_key, d->b2_point_key, mthz::Vec3(d->target_distance, 0, 0), mthz::Vec3(), d->pos_correct_hardness, 0);
+			break;
+		}
+		case 3: {
 			auto* b = static_cast<phyz::BallSocket*>(j.pc);
-			mthz::Vec3 t1 = j.b1->track_p[b->b1_point_key], t2 = j.b2->track_p[b->b2_point_key];
-			jg.insert(jg.end(), {t1.x, t1.y, t1.z, t2.x, t2.y, t2.z, 0, 0, 0, 0, 0, 0, b->pos_correct_hardness, 0});
+			geo(j, b->b1_point_key, b->b2_point_key, mthz::Vec3(), mthz::Vec3(), b->pos_correct_hardness, 0);
+			break;
+		}
+		case 4: {
+			auto* h = static_cast<phyz::SlidingHinge*>(j.pc);
+			geo(j, h->b1_point_key, h->b2_point_key, h->b1_slide_axis_body_space, h->b2_slide_axis_body_space, h->pos_correct_hardness,
+			    h->rot_correct_hardness);
+			break;
+		}
+		case 5: {
+			auto* h = static_cast<phyz::Hinge*>(j.pc);
+			geo(j, h->b1_point_key, h->b2_point_key, h->b1_rot_axis_body_space, h->b2_rot_axis_body_space, h->pos_correct_hardness,
+			    h->rot_correct_hardness);
+			break;
+		}
+		case 6: {
+			auto* w = static_cast<phyz::Weld*>(j.pc);
+			geo(j, w->b1_point_key, w->b2_point_key, mthz::Vec3(), mthz::Vec3(), w->pos_correct_hardness, w->rot_correct_hardness);
+			break;
+		}
+		default: {
+			auto* h = static_cast<phyz::Slider*>(j.pc);
+			geo(j, h->b1_point_key, h->b2_point_key, h->b1_slide_axis_body_space, h->b2_slide_axis_body_space, h->pos_correct_hardness,
+			    h->rot_correct_hardness);
+			break;
+		}
 		}
 	}
 	o.i64("joint_info", ji, {g_joints.size(), 3});
@@ -678,8 +725,8 @@ static void run_trace(PhysicsEngine& E, const SceneCfg& cfg, int steps, const st
 		if (!g_joints.empty()) {
 			std::vector<double> jpost;
 			joint_state(jpost);
-			o.f64(p + "_joint_pre", jpre, {g_joints.size(), 11});
-			o.f64(p + "_joint_post", jpost, {g_joints.size(), 11});
+			o.f64(p + "_joint_pre", jpre, {g_joints.size(), 12});
+			o.f64(p + "_joint_post", jpost, {g_joints.size(), 12});
 			o.i64(p + "_hsys_off", hs.off, {hs.off.size()});
 			o.i64(p + "_hsys_og", hs.og, {hs.og.size()});
 			o.i64(p + "_hsys_order", hs.order, {hs.order.size()});
@@ -952,6 +999,70 @@ static int cmd_terrain(int argc, char** argv) {
 	return 0;
 }
 
+// joints: the equality joints beyond ball-sockets and hinges.  A chain hangs
+// from a FIXED anchor box: anchor -distance- box -weld- box -slider- box
+// -sliding hinge- cylinder (one holonomic system of degrees 1, 6, 5, 4); a
+// welded pair of boxes and a slider pair (a box sliding along another, axis
+// tilted) fall onto the ground box.  Every dynamic body gets a random
+// velocity U[-1, 1]^3 and angular velocity U[-1, 1]^3; default material and
+// correction strengths (PhysicsEngine.h:15-16).
+template <class T>
+static void record_joint(PhysicsEngine& E, phyz::ConstraintID id, int type, RigidBody* b1, RigidBody* b2) {
+	auto* e = E.constraint_map[id.getID()];
+	phyz::PersistentConstraint* pc = nullptr;
+	for (auto* x : e->constraints)
+		if (x->getId() == id) pc = x;
+	g_joints.push_back({type, b1, b2, pc, (phyz::Constraint*)&static_cast<T*>(pc)->constraint});
+}
+static int cmd_joints(int argc, char** argv) {
+	int steps = (int)arg_d(argc, argv, "--steps", 120);
+	uint64_t seed = (uint64_t)arg_d(argc, argv, "--seed", 909);
+	SceneCfg cfg;
+	cfg.hol_iters = (int)arg_d(argc, argv, "--hol", 1);
+	PhysicsEngine E;
+	configure(E, cfg);
+	phyz::Material m = c0_material();
+	RigidBody* g = E.createRigidBody(phyz::ConvexUnionGeometry::box(mthz::Vec3(-100, -1, -100), 200, 1, 200, m), RigidBody::FIXED);
+	pin_body(g);
+	SplitMix64 rng(seed);
+	auto kick = [&](RigidBody* b) {
+		b->setVel(mthz::Vec3(rng.uniform(-1, 1), rng.uniform(-1, 1), rng.uniform(-1, 1)));
+		b->setAngVel(mthz::Vec3(rng.uniform(-1, 1), rng.uniform(-1, 1), rng.uniform(-1, 1)));
+		pin_body(b);
+	};
+	auto box = [&](mthz::Vec3 c, double sx, double sy, double sz) {
+		return E.createRigidBody(phyz::ConvexUnionGeometry::box(c - mthz::Vec3(sx / 2, sy / 2, sz / 2), sx, sy, sz, m));
+	};
+	// the hanging chain (bodies centred on x = 0, z = 0)
+	RigidBody* anchor = E.createRigidBody(phyz::ConvexUnionGeometry::box(mthz::Vec3(-0.5, 9.5, -0.5), 1, 1, 1, m), RigidBody::FIXED);
+	pin_body(anchor);
+	RigidBody* l1 = box(mthz::Vec3(0.3, 8.2, 0), 0.6, 0.6, 0.6);
+	RigidBody* l2 = box(mthz::Vec3(0.3, 7.5, 0), 0.6, 0.6, 0.6);
+	RigidBody* l3 = box(mthz::Vec3(0.3, 6.8, 0.1), 0.5, 0.5, 0.5);
+	RigidBody* l4 = E.createRigidBody(phyz::ConvexUnionGeometry::cylinder(mthz::Vec3(0.3, 5.6, 0.1), 0.25, 0.8, m));
+	for (RigidBody* b : {l1, l2, l3, l4}) kick(b);
+	record_joint<phyz::Distance>(E, E.addDistanceConstraint(anchor, l1, mthz::Vec3(0, -0.5, 0), mthz::Vec3(-0.2, 0.3, 0)), 1, anchor, l1);
+	record_joint<phyz::Weld>(E, E.addWeldConstraint(l1, l2, mthz::Vec3(0, -0.35, 0), mthz::Vec3(0, 0.35, 0)), 6, l1, l2);
+	record_joint<phyz::Slider>(E, E.addSliderConstraint(l2, l3, mthz::Vec3(0, -0.3, 0), mthz::Vec3(0, 0.25, 0), mthz::Vec3(0, 1, 0),
+	                                                    mthz::Vec3(0, 1, 0)), 7, l2, l3);
+	record_joint<phyz::SlidingHinge>(E, E.addSlidingHingeConstraint(l3, l4, mthz::Vec3(0, -0.25, 0), mthz::Vec3(0, 0.4, 0),
+	                                                                mthz::Vec3(0, 1, 0), mthz::Vec3(0, 1, 0)), 4, l3, l4);
+	// a welded pair and a slider pair on the ground
+	RigidBody* w1 = box(mthz::Vec3(4, 1.5, 0), 1, 0.5, 1);
+	RigidBody* w2 = box(mthz::Vec3(4.9, 1.8, 0), 0.8, 0.8, 0.8);
+	RigidBody* s1 = box(mthz::Vec3(-4, 1.2, 0), 2, 0.4, 1);
+	RigidBody* s2 = box(mthz::Vec3(-4.3, 1.7, 0), 0.5, 0.5, 0.5);
+	for (RigidBody* b : {w1, w2, s1, s2}) kick(b);
+	record_joint<phyz::Weld>(E, E.addWeldConstraint(w1, w2, mthz::Vec3(0.5, 0.1, 0), mthz::Vec3(-0.4, -0.2, 0)), 6, w1, w2);
+	record_joint<phyz::Slider>(E, E.addSliderConstraint(s1, s2, mthz::Vec3(0, 0.2, 0), mthz::Vec3(0, -0.25, 0), mthz::Vec3(1, 0.1, 0),
+	                                                    mthz::Vec3(1, 0.1, 0)), 7, s1, s2);
+	Out o(arg_s(argc, argv, "--out", "out"));
+	o.meta("seed", (double)seed);
+	run_trace(E, cfg, steps, arg_steps(argc, argv, "--record"), o);
+	o.finish();
+	return 0;
+}
+
 // sleep: sleeping enabled (the engine's default parameters, every body's
 // sleep_disabled pinned to false): a 2x2x2 cube pile and a lone cube settle
 // and fall asleep, a cube launched along the ground slides to rest and falls
@@ -1163,6 +1274,7 @@ int main(int argc, char** argv) {
 	if (cmd == "time") return cmd_time(argc, argv);
 	if (cmd == "hulls") return cmd_hulls(argc, argv);
 	if (cmd == "sleep") return cmd_sleep(argc, argv);
+	if (cmd == "joints") return cmd_joints(argc, argv);
 	fprintf(stderr, "unknown command %s\n", cmd.c_str());
 	return 2;
 }

@@ -245,20 +245,20 @@ class Oracle:
                 lib.orc_add_joint(self.w, int(info[k][0]), int(info[k][1]) - 1, int(info[k][2]) - 1, g.ctypes.data)
             self.n_joints = len(info)
 
-    def load_joints(self, state11, hsys_off=None, hsys_og=None):
-        st = _c(state11, np.float64)
+    def load_joints(self, state12, hsys_off=None, hsys_og=None):
+        st = _c(state12, np.float64)
         if hsys_off is not None:
             off, og = _c(hsys_off, np.int64), _c(hsys_og, np.int64)
             self.lib.orc_set_hsys(self.w, len(off) - 1, off.ctypes.data, og.ctypes.data)
         self.lib.orc_set_joint_state(self.w, st.ctypes.data)
 
     def joint_state(self):
-        s = np.zeros((self.n_joints, 11))
+        s = np.zeros((self.n_joints, 12))
         self.lib.orc_get_joint_state(self.w, s.ctypes.data)
         return s
 
     def joint_rows(self, k):
-        a = [np.zeros(30) for _ in range(4)]
+        a = [np.zeros(36) for _ in range(4)]
         n = self.lib.orc_get_joint_rows(self.w, k, *[x.ctypes.data for x in a])
         return a[0][:6 * n].reshape(n, 6), a[1][:6 * n].reshape(n, 6), a[2][:6 * n].reshape(6, n), a[3][:6 * n].reshape(6, n)
 

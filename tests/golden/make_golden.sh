@@ -8,7 +8,7 @@ ROOT=$(cd "$HERE/../.." && pwd)
 make -C "$ROOT/oracle" ref REF="${REF:-/root/reference}"
 H="$ROOT/oracle/_ref/cp_ref_harness"
 cd "$HERE"
-rm -rf pile3 c0_full mix3 cyl3 cyl4 chain3 chain3h0 terrain hulls sleep
+rm -rf pile3 c0_full mix3 cyl3 cyl4 chain3 chain3h0 terrain hulls sleep joints joints_h0
 # C0 pattern, 3x3x3 cubes: full per-step dumps (state, caches, manifolds, islands).
 "$H" pile --n 3 --steps 40 --record 0,1,2,10,39 --out pile3
 # C0 in full (10x10x10 cubes): first step and a settled step (no mid/post caches).
@@ -40,3 +40,9 @@ python3 make_hull_input.py hull_input.txt 24 20240
 # cubes wakes it; a waking action on a pile cube before step 250 wakes the
 # pile.  Every step's sleep state is recorded.
 "$H" sleep --steps 420 --record 0,100,135,136,153,154,200,220,221,250,251,300,313,314,315,330,339,419 --out sleep
+# The other equality joints: a chain hanging from a fixed anchor through a
+# distance joint, a weld, a slider and a sliding hinge (one holonomic system
+# of degrees 1, 6, 5, 4), a welded pair and a slider pair on the ground; with
+# and without the holonomic block solver.
+"$H" joints --steps 120 --record 0,1,2,30,60,119 --out joints
+"$H" joints --steps 120 --record 0,1,30,60,119 --hol 0 --out joints_h0

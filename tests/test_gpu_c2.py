@@ -41,7 +41,7 @@ def test_small_chains_on_terrain_match_oracle():
     w.set_holonomic_systems(off, og)
     w.set_node_order(no)
     o = Oracle.from_description(desc, dt=CFG["dt"], gravity_y=-6.0, vel_iters=10, pos_iters=1, substeps=8, hol_iters=1)
-    o.load_joints(np.zeros((len(desc["joints"][0]), 11)), off, og)
+    o.load_joints(np.zeros((len(desc["joints"][0]), 12)), off, og)
     worst, touched = 0.0, False
     for k in range(150):
         w.step(1)
@@ -70,7 +70,7 @@ def test_small_chains_island_colored_replayed_by_oracle():
     w.set_holonomic_systems(off, og)
     w.set_node_order(no)
     o = Oracle.from_description(desc, dt=CFG["dt"], gravity_y=-6.0, vel_iters=10, pos_iters=1, substeps=8, hol_iters=1)
-    o.load_joints(np.zeros((len(desc["joints"][0]), 11)), off, og)
+    o.load_joints(np.zeros((len(desc["joints"][0]), 12)), off, og)
     worst = 0.0
     for k in range(150):
         w.step(1)
