@@ -516,9 +516,11 @@ class World:
         return r
 
     def add_joint(self, kind, b1, b2, geo14):
-        """Ball-socket (kind 3) or hinge (kind 5) between body indices b1, b2;
-        geo14 = attach points in the body frames, hinge axes in the body
-        frames, position and rotation correction strengths."""
+        """Joint of kind 1 distance, 3 ball-socket, 4 sliding hinge, 5 hinge,
+        6 weld or 7 slider between body indices b1, b2; geo14 = attach points
+        in the body frames, two body-frame axes (hinge or slide axes; a
+        distance joint's target distance first), position and rotation
+        correction strengths."""
         g = _a(geo14, np.float64).reshape(14)
         k = ctypes.c_int()
         _chk(self.L.cp_joint_add(self.w, int(kind), int(b1), int(b2), g.ctypes.data, ctypes.byref(k)), "cp_joint_add")
@@ -529,18 +531,19 @@ class World:
         _chk(self.L.cp_world_joint_count(self.w, ctypes.byref(n)), "cp_world_joint_count")
         return n.value
 
-    def set_joint_state(self, state11):
-        s = _a(state11, np.float64)
-        assert s.shape == (self.joint_count(), 11)
+    def set_joint_state(self, state12):
+        """Per joint: 6 accumulated impulses (zero padded), basis u, w."""
+        s = _a(state12, np.float64)
+        assert s.shape == (self.joint_count(), 12)
         _chk(self.L.cp_world_set_joint_state(self.w, s.ctypes.data), "cp_world_set_joint_state")
 
     def joint_state(self):
-        s = np.zeros((self.joint_count(), 11))
+        s = np.zeros((self.joint_count(), 12))
         _chk(self.L.cp_world_get_joint_state(self.w, s.ctypes.data), "cp_world_get_joint_state")
         return s
 
     def joint_rows(self, k):
-        a = [np.zeros(30) for _ in range(4)]
+        a = [np.zeros(36) for _ in range(4)]
         n = ctypes.c_int()
         _chk(self.L.cp_world_get_joint_rows(self.w, int(k), *[x.ctypes.data for x in a], ctypes.byref(n)), "cp_world_get_joint_rows")
         t = n.value

@@ -85,26 +85,29 @@ struct Params {
 	int hol_iters;  // holonomic block-solver iterations (first substep only)
 };
 
-// Joints (ball-socket n = 3, hinge n = 5), SoA over n joints; state arrays
-// are component-major ([5][n], [3][n]).
+// Joints, SoA over n joints.  kind: 1 distance, 3 ball-socket, 4 sliding
+// hinge, 5 hinge, 6 weld, 7 slider; type: the joint's degree (rows: 1, 3, 4,
+// 5, 6, 5).  State arrays are component-major ([CP_JMAX][n], [3][n]).
+#define CP_JMAX 6
+#define CP_JSTATE 12  // joint state per joint at the ABI: CP_JMAX impulses, u, w
 struct Joints {
 	int n;
-	int *type, *a, *b, *sys;  // sys: holonomic system, -1 none
-	double *tp1, *tp2, *ax1, *ax2;  // [3][n] body-frame attach points and hinge axes
+	int *type, *kind, *a, *b, *sys;  // sys: holonomic system, -1 none
+	double *tp1, *tp2, *ax1, *ax2;  // [3][n] body-frame attach points and axes (distance: ax1 x = target distance)
 	double *ps, *rs;                // correction strengths
-	double *imp, *pimp, *himp;      // [5][n]
-	double *u, *w;                  // [3][n] hinge basis of the current rows
+	double *imp, *pimp, *himp;      // [CP_JMAX][n]
+	double *u, *w;                  // [3][n] basis of the current rows (hinge, slider, sliding hinge)
 	double* row;                    // CP_JRS doubles per joint
 };
-#define CP_JRS 160
+#define CP_JRS 192
 enum JointRowField {
-	CP_J_JA = 0,      // J_a [5][6]
-	CP_J_JB = 30,     // J_b [5][6]
-	CP_J_ITA = 60,    // M_a^-1 J_a^T [6][5]
-	CP_J_ITB = 90,    // M_b^-1 J_b^T [6][5]
-	CP_J_INV = 120,   // (J M^-1 J^T (1+cfm))^-1 [n][n]
-	CP_J_TGT = 145,   // velocity targets [5]
-	CP_J_PTGT = 150   // pseudo-velocity targets [5]
+	CP_J_JA = 0,      // J_a [CP_JMAX][6]
+	CP_J_JB = 36,     // J_b [CP_JMAX][6]
+	CP_J_ITA = 72,    // M_a^-1 J_a^T [6][CP_JMAX]
+	CP_J_ITB = 108,   // M_b^-1 J_b^T [6][CP_JMAX]
+	CP_J_INV = 144,   // (J M^-1 J^T (1+cfm))^-1 [n][n]
+	CP_J_TGT = 180,   // velocity targets [CP_JMAX]
+	CP_J_PTGT = 186   // pseudo-velocity targets [CP_JMAX]
 };
 
 // Holonomic systems: per system the elimination order (joint indices), the

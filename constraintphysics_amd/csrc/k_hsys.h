@@ -35,8 +35,8 @@ __device__ __forceinline__ void joint_apply(const double* R, int n, const double
 		da[r] = 0;
 		db[r] = 0;
 		for (int q = 0; q < n; q++) {
-			da[r] += imp[q] * R[CP_J_ITA + r * 5 + q];
-			db[r] += imp[q] * R[CP_J_ITB + r * 5 + q];
+			da[r] += imp[q] * R[CP_J_ITA + r * CP_JMAX + q];
+			db[r] += imp[q] * R[CP_J_ITB + r * CP_JMAX + q];
 		}
 	}
 	for (int r = 0; r < 6; r++) {

@@ -66,9 +66,55 @@ __device__ void inverse_n(int n, const double* src, double* dst) {
 	for (int k = 0; k < n * n; k++) dst[k] = t[k];
 }
 
-// Solver rows of joint k from the current body state (warm start with
-// coefficient 1.0; hinge angular impulses re-projected onto the new basis
-// when reproject, ConstraintSolver.cpp:475-476).
+// x S(v) for a row vector x, S the cross-product matrix crossMat (NMat.cpp:
+// 62-70), accumulated as NMat's product accumulates (NMat.h:95-103).
+__device__ __forceinline__ void row_cross(const double* x, V3 v, double* out) {
+	const double S[3][3] = {{0, -v.z, v.y}, {v.z, 0, -v.x}, {-v.y, v.x, 0}};
+	for (int c = 0; c < 3; c++) {
+		double acc = 0;
+		for (int q = 0; q < 3; q++) acc += x[q] * S[q][c];
+		out[c] = acc;
+	}
+}
+__device__ __forceinline__ void put3(double (*m)[6], int r, int c0, const double* x) {
+	for (int c = 0; c < 3; c++) m[r][c0 + c] = x[c];
+}
+// I (idenMat<3>) or -I into rows r0.., columns c0..; -I keeps its negative zeros.
+__device__ __forceinline__ void put_iden(double (*m)[6], int r0, int c0, bool negative) {
+	for (int r = 0; r < 3; r++)
+		for (int c = 0; c < 3; c++) {
+			double e = (r == c) ? 1.0 : 0.0;
+			m[r0 + r][c0 + c] = negative ? -e : e;
+		}
+}
+// crossMat(v) or -crossMat(v) into rows r0.., columns c0...
+__device__ __forceinline__ void put_cross(double (*m)[6], int r0, int c0, V3 v, bool negative) {
+	const double S[3][3] = {{0, -v.z, v.y}, {v.z, 0, -v.x}, {-v.y, v.x, 0}};
+	for (int r = 0; r < 3; r++)
+		for (int c = 0; c < 3; c++) m[r0 + r][c0 + c] = negative ? -S[r][c] : S[r][c];
+}
+// Quaternion::angleTo (Quaternion.h:35-39) and getRotAxis (:41-49): the
+// rotation correction of the weld and slider rows (ConstraintSolver.cpp:
+// 688-696, :888-896), axis * angle scaled by h.
+__device__ V3 rot_correct(Q a, Q b, double h) {
+	Q diff = q_mul(b, q_conj(a));
+	double dot = fabs(a.r * b.r + a.i * b.i + a.j * b.j + a.k * b.k);
+	double angle = dot > 0.99999999999 ? 0 : 2 * acos(fabs(a.r * b.r + a.i * b.i + a.j * b.j + a.k * b.k));
+	V3 axis{0, 0, 0};
+	if (angle != 0) {
+		V3 ax{diff.i, diff.j, diff.k};
+		V3 nx = normalize(ax);
+		axis = (diff.r < 0 ? V3{-nx.x, -nx.y, -nx.z} : nx) * h;
+	}
+	return V3{axis.x * angle, axis.y * angle, axis.z * angle};
+}
+
+// Solver rows of joint k from the current body state: the joint's
+// updateSolverConstraints (PersistentConstraint.cpp:217-354, warm start
+// coefficient 1) and its constraint's constructor (ConstraintSolver.cpp:
+// 348-900).  reproject: hinges, sliders and sliding hinges re-project their
+// stored impulses onto the new basis (ConstraintSolver.cpp:475-476, :617-618,
+// :787-788); false for holonomic rows rebuilt from a stored state.
 __device__ void build_joint(const Bodies& b, Joints& J, int k, double sub_dt, double cfm, bool reproject) {
 	const int N = b.n, K = J.n;
 	const int n = J.type[k];
@@ -92,72 +138,141 @@ __device__ void build_joint(const Bodies& b, Joints& J, int k, double sub_dt, do
 				Mb[3 + r][3 + c] = db ? b.Iinv[(3 * r + c) * N + ib] : 0.0;
 			}
 	}
-	double start[5];
+	double start[CP_JMAX];
 	for (int c = 0; c < n; c++) start[c] = J.imp[c * K + k] * 1.0;
-	double ja[5][6], jb[5][6];
-	double pcc = pos_coeff(J.ps[k], sub_dt);
-	V3 rA, rB;
-	if (n == 3) {
-		rA = pa - ca;
-		rB = pb - cb;
-	} else {
-		rA = pa - ca;
-		rB = pa - cb;  // both arms from the b1 point (ConstraintSolver.cpp:469)
-	}
-	// crossMat (NMat.cpp:62-70); the -I and -crossMat blocks are negations
-	double sa[3][3] = {{0, -rA.z, rA.y}, {rA.z, 0, -rA.x}, {-rA.y, rA.x, 0}};
-	double sb[3][3] = {{0, -rB.z, rB.y}, {rB.z, 0, -rB.x}, {-rB.y, rB.x, 0}};
-	for (int r = 0; r < 3; r++)
-		for (int c = 0; c < 3; c++) {
-			ja[r][c] = (r == c) ? 1.0 : 0.0;
-			jb[r][c] = -((r == c) ? 1.0 : 0.0);
-			ja[r][3 + c] = -sa[r][c];
-			jb[r][3 + c] = sb[r][c];
+	for (int c = 0; c < n; c++) J.imp[c * K + k] = start[c];
+	double ja[CP_JMAX][6], jb[CP_JMAX][6];
+	for (int r = 0; r < CP_JMAX; r++)
+		for (int c = 0; c < 6; c++) {
+			ja[r][c] = 0;
+			jb[r][c] = 0;
 		}
-	if (n == 3) {
-		V3 err = pb - pa;
-		R[CP_J_PTGT] = pcc * err.x;
-		R[CP_J_PTGT + 1] = pcc * err.y;
-		R[CP_J_PTGT + 2] = pcc * err.z;
-		for (int c = 0; c < 3; c++) J.imp[c * K + k] = start[c];
-	} else {
-		V3 axa = q_rotate(qa, jld3(J.ax1, k, K)), axb = q_rotate(qb, jld3(J.ax2, k, K));
-		double rcc = pos_coeff(J.rs[k], sub_dt);
+	const double pcc = pos_coeff(J.ps[k], sub_dt), rcc = pos_coeff(J.rs[k], sub_dt);
+	double* pt = R + CP_J_PTGT;
+	const int kind = J.kind[k];
+	// new basis (u, w) of a basis-carrying joint and its re-projection of
+	// impulses i0, i0 + 1
+	auto basis = [&](V3 axis, int i0, V3* u, V3* w) {
 		V3 su = jld3(J.u, k, K), sw = jld3(J.w, k, K);
-		V3 u, w;
-		perp_basis(axa, &u, &w);
-		V3 nn = axb;
-		for (int c = 0; c < 5; c++) J.imp[c * K + k] = start[c];
+		perp_basis(axis, u, w);
 		if (reproject) {
-			J.imp[3 * K + k] = dot(u, su) * start[3] + dot(u, sw) * start[4];
-			J.imp[4 * K + k] = dot(w, su) * start[3] + dot(w, sw) * start[4];
+			J.imp[i0 * K + k] = dot(*u, su) * start[i0] + dot(*u, sw) * start[i0 + 1];
+			J.imp[(i0 + 1) * K + k] = dot(*w, su) * start[i0] + dot(*w, sw) * start[i0 + 1];
 		}
-		jst3(J.u, k, K, u);
-		jst3(J.w, k, K, w);
-		double us[3][3] = {{0, -u.z, u.y}, {u.z, 0, -u.x}, {-u.y, u.x, 0}};
-		double ws[3][3] = {{0, -w.z, w.y}, {w.z, 0, -w.x}, {-w.y, w.x, 0}};
-		double ns[3][3] = {{0, -nn.z, nn.y}, {nn.z, 0, -nn.x}, {-nn.y, nn.x, 0}};
-		double ud[3] = {u.x, u.y, u.z}, wd[3] = {w.x, w.y, w.z}, nd[3] = {nn.x, nn.y, nn.z};
-		for (int c = 0; c < 3; c++) {
-			double a3 = 0, b3 = 0, a4 = 0, b4 = 0;
-			for (int q = 0; q < 3; q++) {
-				a3 += nd[q] * us[q][c];
-				b3 += ud[q] * ns[q][c];
-				a4 += nd[q] * ws[q][c];
-				b4 += wd[q] * ns[q][c];
-			}
-			ja[3][c] = 0; jb[3][c] = 0; ja[4][c] = 0; jb[4][c] = 0;
-			ja[3][3 + c] = a3; jb[3][3 + c] = b3;
-			ja[4][3 + c] = a4; jb[4][3 + c] = b4;
-		}
+		jst3(J.u, k, K, *u);
+		jst3(J.w, k, K, *w);
+	};
+	if (kind == 1) {  // DistanceConstraint (ConstraintSolver.cpp:348-379), not in moving mode
+		V3 rA = pa - ca, rB = pb - cb;
+		V3 diff = pa - pb;
+		V3 dd = normalize(diff);
+		double d[3] = {dd.x, dd.y, dd.z}, nd[3] = {-dd.x, -dd.y, -dd.z}, t[3];
+		put3(ja, 0, 0, d);
+		row_cross(nd, rA, t);
+		put3(ja, 0, 3, t);
+		put3(jb, 0, 0, nd);
+		row_cross(d, rB, t);
+		put3(jb, 0, 3, t);
+		double target = J.ax1[k];
+		double error = dot(diff, diff) < 0.00001 ? target : target - dot(normalize(diff), diff);
+		pt[0] = pcc * error;
+	} else if (kind == 3) {  // BallSocketConstraint (:382-408)
+		V3 rA = pa - ca, rB = pb - cb;
+		put_iden(ja, 0, 0, false);
+		put_cross(ja, 0, 3, rA, true);
+		put_iden(jb, 0, 0, true);
+		put_cross(jb, 0, 3, rB, false);
+		V3 err = pb - pa;
+		pt[0] = pcc * err.x;
+		pt[1] = pcc * err.y;
+		pt[2] = pcc * err.z;
+	} else if (kind == 5) {  // HingeConstraint (:468-553)
+		V3 axa = q_rotate(qa, jld3(J.ax1, k, K)), axb = q_rotate(qb, jld3(J.ax2, k, K));
+		V3 u, w;
+		basis(axa, 3, &u, &w);
+		V3 nn = axb;
+		V3 rA = pa - ca, rB = pa - cb;  // both arms from the b1 point (ConstraintSolver.cpp:469)
+		put_iden(ja, 0, 0, false);
+		put_cross(ja, 0, 3, rA, true);
+		put_iden(jb, 0, 0, true);
+		put_cross(jb, 0, 3, rB, false);
+		double ud[3] = {u.x, u.y, u.z}, wd[3] = {w.x, w.y, w.z}, nd[3] = {nn.x, nn.y, nn.z}, t[3];
+		row_cross(nd, u, t); put3(ja, 3, 3, t);
+		row_cross(ud, nn, t); put3(jb, 3, 3, t);
+		row_cross(nd, w, t); put3(ja, 4, 3, t);
+		row_cross(wd, nn, t); put3(jb, 4, 3, t);
 		V3 pc = (pb - pa) * pcc;
-		R[CP_J_PTGT] = pc.x;
-		R[CP_J_PTGT + 1] = pc.y;
-		R[CP_J_PTGT + 2] = pc.z;
-		R[CP_J_PTGT + 3] = dot(u, nn) * rcc;
-		R[CP_J_PTGT + 4] = dot(w, nn) * rcc;
+		pt[0] = pc.x;
+		pt[1] = pc.y;
+		pt[2] = pc.z;
+		pt[3] = dot(u, nn) * rcc;
+		pt[4] = dot(w, nn) * rcc;
+	} else if (kind == 7) {  // SliderConstraint (:611-697); Slider passes the position coefficient for both (PersistentConstraint.cpp:297)
+		V3 u, w;
+		basis(normalize(q_rotate(qa, jld3(J.ax1, k, K))), 0, &u, &w);
+		V3 rA = pa - ca, rB = pb - cb;
+		V3 pe = pa - pb;
+		double ud[3] = {u.x, u.y, u.z}, wd[3] = {w.x, w.y, w.z}, nu[3] = {-u.x, -u.y, -u.z}, nw[3] = {-w.x, -w.y, -w.z};
+		double ped[3] = {pe.x, pe.y, pe.z}, t1[3], t2[3];
+		put3(ja, 0, 0, ud);
+		row_cross(nu, rA, t1); row_cross(ped, u, t2);
+		for (int c = 0; c < 3; c++) ja[0][3 + c] = t1[c] - t2[c];
+		put3(jb, 0, 0, nu);
+		row_cross(ud, rB, t1); put3(jb, 0, 3, t1);
+		put3(ja, 1, 0, wd);
+		row_cross(nw, rA, t1); row_cross(ped, w, t2);
+		for (int c = 0; c < 3; c++) ja[1][3 + c] = t1[c] - t2[c];
+		put3(jb, 1, 0, nw);
+		row_cross(wd, rB, t1); put3(jb, 1, 3, t1);
+		put_iden(ja, 2, 3, false);
+		put_iden(jb, 2, 3, true);
+		V3 rc = rot_correct(qa, qb, pcc);
+		pt[0] = -dot(pe, u) * pcc;
+		pt[1] = -dot(pe, w) * pcc;
+		pt[2] = rc.x;
+		pt[3] = rc.y;
+		pt[4] = rc.z;
+	} else if (kind == 4) {  // SlidingHingeConstraint (:781-857)
+		V3 axa = q_rotate(qa, jld3(J.ax1, k, K)), axb = q_rotate(qb, jld3(J.ax2, k, K));
+		V3 u, w;
+		basis(axb, 2, &u, &w);
+		V3 nn = axa;
+		V3 rA = pa - ca, rB = pb - cb;
+		V3 pd = pa - pb;
+		double ud[3] = {u.x, u.y, u.z}, wd[3] = {w.x, w.y, w.z}, nu[3] = {-u.x, -u.y, -u.z}, nw[3] = {-w.x, -w.y, -w.z};
+		double nnn[3] = {-nn.x, -nn.y, -nn.z}, pdd[3] = {pd.x, pd.y, pd.z}, t1[3], t2[3];
+		put3(ja, 0, 0, ud);
+		row_cross(nu, rA, t1); put3(ja, 0, 3, t1);
+		put3(jb, 0, 0, nu);
+		row_cross(ud, rB, t1); row_cross(pdd, u, t2);
+		for (int c = 0; c < 3; c++) jb[0][3 + c] = t1[c] - t2[c];
+		put3(ja, 1, 0, wd);
+		row_cross(nw, rA, t1); put3(ja, 1, 3, t1);
+		put3(jb, 1, 0, nw);
+		row_cross(wd, rB, t1); row_cross(pdd, w, t2);
+		for (int c = 0; c < 3; c++) jb[1][3 + c] = t1[c] - t2[c];
+		row_cross(nu, nn, t1); put3(ja, 2, 3, t1);
+		row_cross(nnn, u, t1); put3(jb, 2, 3, t1);
+		row_cross(nw, nn, t1); put3(ja, 3, 3, t1);
+		row_cross(nnn, w, t1); put3(jb, 3, 3, t1);
+		pt[0] = -dot(u, pd) * pcc;
+		pt[1] = -dot(w, pd) * pcc;
+		pt[2] = -dot(u, nn) * rcc;
+		pt[3] = -dot(w, nn) * rcc;
+	} else {  // 6: WeldConstraint (:861-897)
+		V3 rA = pa - ca, rB = pb - cb;
+		put_iden(ja, 0, 0, false);
+		put_cross(ja, 0, 3, rA, true);
+		put_iden(ja, 3, 3, false);
+		put_iden(jb, 0, 0, true);
+		put_cross(jb, 0, 3, rB, false);
+		put_iden(jb, 3, 3, true);
+		V3 pc = (pb - pa) * pcc;
+		V3 rc = rot_correct(qa, qb, rcc);
+		pt[0] = pc.x; pt[1] = pc.y; pt[2] = pc.z;
+		pt[3] = rc.x; pt[4] = rc.y; pt[5] = rc.z;
 	}
-	double ita[6][5], itb[6][5];
+	double ita[6][CP_JMAX], itb[6][CP_JMAX];
 	for (int r = 0; r < 6; r++)
 		for (int c = 0; c < n; c++) {
 			double s1 = 0, s2 = 0;
@@ -165,7 +280,7 @@ __device__ void build_joint(const Bodies& b, Joints& J, int k, double sub_dt, do
 			ita[r][c] = s1;
 			itb[r][c] = s2;
 		}
-	double itv[25], inv[25];
+	double itv[CP_JMAX * CP_JMAX], inv[CP_JMAX * CP_JMAX];
 	for (int r = 0; r < n; r++)
 		for (int c = 0; c < n; c++) {
 			double s1 = 0, s2 = 0;
@@ -181,8 +296,8 @@ __device__ void build_joint(const Bodies& b, Joints& J, int k, double sub_dt, do
 		}
 	for (int r = 0; r < 6; r++)
 		for (int c = 0; c < n; c++) {
-			R[CP_J_ITA + r * 5 + c] = ita[r][c];
-			R[CP_J_ITB + r * 5 + c] = itb[r][c];
+			R[CP_J_ITA + r * CP_JMAX + c] = ita[r][c];
+			R[CP_J_ITB + r * CP_JMAX + c] = itb[r][c];
 		}
 	for (int r = 0; r < n * n; r++) R[CP_J_INV + r] = inv[r];
 	for (int c = 0; c < n; c++) {
@@ -225,8 +340,8 @@ __device__ void hs_block_init(const Joints& J, double cfm, int c1, int c2, doubl
 			for (int c = 0; c < n1; c++) {
 				double s1 = 0, s2 = 0;
 				for (int q = 0; q < 6; q++) {
-					s1 += R1[CP_J_JA + r * 6 + q] * R1[CP_J_ITA + q * 5 + c];
-					s2 += R1[CP_J_JB + r * 6 + q] * R1[CP_J_ITB + q * 5 + c];
+					s1 += R1[CP_J_JA + r * 6 + q] * R1[CP_J_ITA + q * CP_JMAX + c];
+					s2 += R1[CP_J_JB + r * 6 + q] * R1[CP_J_ITB + q * CP_JMAX + c];
 				}
 				t[r * n1 + c] = s1 + s2;
 			}
@@ -245,7 +360,7 @@ __device__ void hs_block_init(const Joints& J, double cfm, int c1, int c2, doubl
 	for (int i = 0; i < n1 * n2; i++) {  // matMult (:701-712)
 		int row = i / n2, col = i % n2;
 		t[i] = 0;
-		for (int q = 0; q < 6; q++) t[i] += R1[jac + row * 6 + q] * R2[it + q * 5 + col];
+		for (int q = 0; q < 6; q++) t[i] += R1[jac + row * 6 + q] * R2[it + q * CP_JMAX + col];
 	}
 }
 

@@ -561,7 +561,7 @@ __device__ __forceinline__ void joint_target(const Bodies& b, const Joints& J, i
 	const int N = b.n;
 	double* R = J.row + (size_t)k * CP_JRS;
 	int ia = J.a[k], ib = J.b[k];
-	double va[6], vb[6], val[5];
+	double va[6], vb[6], val[CP_JMAX];
 	bool fa = b.movement[ia] == 1, fb = b.movement[ib] == 1;
 	for (int c = 0; c < 3; c++) {
 		va[c] = fa ? 0.0 : b.vel[c * N + ia];
@@ -582,7 +582,7 @@ __device__ __forceinline__ void joint_step(const Bodies& b, const Joints& J, con
 	double* vb = m.base + (pseudo ? 6 * S : 0) + jl(J, m, k, 1);
 	double* acc = pseudo ? J.pimp : J.imp;
 	const double* tgt = R + (pseudo ? CP_J_PTGT : CP_J_TGT);
-	double cur[5], delta[5], mm[5], diff[5], a[5];
+	double cur[CP_JMAX], delta[CP_JMAX], mm[CP_JMAX], diff[CP_JMAX], a[CP_JMAX];
 	joint_value(R, n, va, vb, S, cur);
 	for (int c = 0; c < n; c++) {
 		delta[c] = tgt[c] - cur[c];
@@ -604,7 +604,7 @@ __device__ __forceinline__ void island_row(const Bodies& b, const Contacts& C, c
 	if (kind == 2) {
 		if (phase == 0) {
 			const int K = J.n;
-			double imp[5];
+			double imp[CP_JMAX];
 			int n = J.type[k];
 			for (int c = 0; c < n; c++) imp[c] = J.imp[c * K + k];
 			if (!is_zero_n(imp, n))
@@ -636,7 +636,7 @@ __device__ void island_holonomic(const Bodies& b, const Joints& J, const HSysDev
 		for (int i = 0; i < nc; i++) {
 			int k = ord[i];
 			const double* R = J.row + (size_t)k * CP_JRS;
-			double cur[5];
+			double cur[CP_JMAX];
 			joint_value(R, J.type[k], dv + jl(J, m, k, 0), dv + jl(J, m, k, 1), DS, cur);
 			const double* tg = R + (pseudo ? CP_J_PTGT : CP_J_TGT);
 			for (int c = 0; c < J.type[k]; c++) d[vl[i] + c] = tg[c] - cur[c];

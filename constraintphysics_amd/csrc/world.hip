@@ -222,11 +222,11 @@ struct cp_world {
 	bool timing = true;
 	bool fused_step = false;  // the last step ran the fused substep kernels
 	// joints (host copies of static data; device SoA)
-	std::vector<int> j_type, j_a, j_b;
+	std::vector<int> j_type, j_kind, j_a, j_b;  // j_type: degree (rows); j_kind: cp_joint_add's type
 	std::vector<double> j_geo;  // 14 per joint: tp1, tp2, ax1, ax2, ps, rs
 	int j_dev_n = 0;            // joints on the device
 	bool joints_dirty = false;  // static joint data or systems changed since the upload
-	DArr<int> dj_type, dj_a, dj_b, dj_sys;
+	DArr<int> dj_type, dj_kind, dj_a, dj_b, dj_sys;
 	DArr<double> dj_tp1, dj_tp2, dj_ax1, dj_ax2, dj_ps, dj_rs, dj_imp, dj_pimp, dj_himp, dj_u, dj_w, dj_row, lin;
 	DArr<uint64_t> nocol;
 	int n_nocol = 0;
@@ -410,7 +410,7 @@ struct cp_world {
 	Joints joints() {
 		Joints J{};
 		J.n = (int)j_type.size();
-		J.type = dj_type.p; J.a = dj_a.p; J.b = dj_b.p; J.sys = dj_sys.p;
+		J.type = dj_type.p; J.kind = dj_kind.p; J.a = dj_a.p; J.b = dj_b.p; J.sys = dj_sys.p;
 		J.tp1 = dj_tp1.p; J.tp2 = dj_tp2.p; J.ax1 = dj_ax1.p; J.ax2 = dj_ax2.p; J.ps = dj_ps.p; J.rs = dj_rs.p;
 		J.imp = dj_imp.p; J.pimp = dj_pimp.p; J.himp = dj_himp.p; J.u = dj_u.p; J.w = dj_w.p; J.row = dj_row.p;
 		return J;
@@ -488,7 +488,7 @@ void cp_world::free_all() {
 	d_nc.free_();
 	nf_body.free_();
 	nf_any.free_();
-	DArr<int>* jints[] = {&dj_type, &dj_a, &dj_b, &dj_sys, &dh_nc, &dh_ord_off, &dh_order, &dh_vloc, &dh_tab_off, &dh_table, &dh_degree,
+	DArr<int>* jints[] = {&dj_type, &dj_kind, &dj_a, &dj_b, &dj_sys, &dh_nc, &dh_ord_off, &dh_order, &dh_vloc, &dh_tab_off, &dh_table, &dh_degree,
 	                      &dh_buf_off, &dh_dbuf_off, &isl_sys, &isl_jlin, &isl_eoff, &isl_coff, &isl_icoff, &isl_ncol,
 	                      &isl_serial, &isl_bod, &isl_nbod, &isl_cloc, &isl_jloc, &isl_sys_cnt, &dh_cp_off, &dh_cs_ptr,
 	                      &dh_cs_row, &dh_cs_loc, &dh_dg_loc, &isl_root, &isl_edge_off, &isl_body_off, &isl_body_cnt, &isl_stack_off,
@@ -615,32 +615,33 @@ void cp_world::upload() {
 }
 
 // ---------------------------------------------------------------- joints
-// Joint state per joint: 5 accumulated impulses (unused components 0), then
-// the hinge basis u, w of the current rows.
+// Joint state per joint: CP_JMAX accumulated impulses (unused components
+// 0), then the basis u, w of the current rows (hinges, sliders, sliding
+// hinges).
 void cp_world::read_joint_state(double* s, int n) {
 	int K = j_dev_n;
 	std::vector<double> imp, u, w;
-	down(imp, dj_imp.p, 5 * (size_t)K);
+	down(imp, dj_imp.p, CP_JMAX * (size_t)K);
 	down(u, dj_u.p, 3 * (size_t)K);
 	down(w, dj_w.p, 3 * (size_t)K);
 	for (int k = 0; k < n && k < K; k++) {
-		double* r = s + 11 * (size_t)k;
-		for (int c = 0; c < 5; c++) r[c] = c < j_type[k] ? imp[c * (size_t)K + k] : 0.0;
+		double* r = s + CP_JSTATE * (size_t)k;
+		for (int c = 0; c < CP_JMAX; c++) r[c] = c < j_type[k] ? imp[c * (size_t)K + k] : 0.0;
 		for (int c = 0; c < 3; c++) {
-			r[5 + c] = u[c * (size_t)K + k];
-			r[8 + c] = w[c * (size_t)K + k];
+			r[CP_JMAX + c] = u[c * (size_t)K + k];
+			r[CP_JMAX + 3 + c] = w[c * (size_t)K + k];
 		}
 	}
 }
 void cp_world::write_joint_state(const double* s) {
 	int K = j_dev_n;
-	std::vector<double> imp(5 * (size_t)K), u(3 * (size_t)K), w(3 * (size_t)K), zero(5 * (size_t)K, 0.0);
+	std::vector<double> imp(CP_JMAX * (size_t)K), u(3 * (size_t)K), w(3 * (size_t)K), zero(CP_JMAX * (size_t)K, 0.0);
 	for (int k = 0; k < K; k++) {
-		const double* r = s + 11 * (size_t)k;
-		for (int c = 0; c < 5; c++) imp[c * (size_t)K + k] = r[c];
+		const double* r = s + CP_JSTATE * (size_t)k;
+		for (int c = 0; c < CP_JMAX; c++) imp[c * (size_t)K + k] = r[c];
 		for (int c = 0; c < 3; c++) {
-			u[c * (size_t)K + k] = r[5 + c];
-			w[c * (size_t)K + k] = r[8 + c];
+			u[c * (size_t)K + k] = r[CP_JMAX + c];
+			w[c * (size_t)K + k] = r[CP_JMAX + 3 + c];
 		}
 	}
 	up(dj_imp, imp); up(dj_u, u); up(dj_w, w); up(dj_pimp, zero); up(dj_himp, zero);
@@ -651,12 +652,12 @@ void cp_world::write_joint_state(const double* s) {
 // accumulated state of joints already on the device is kept.
 void cp_world::upload_joints() {
 	int K = (int)j_type.size();
-	std::vector<double> st(11 * (size_t)K, 0.0);
+	std::vector<double> st(CP_JSTATE * (size_t)K, 0.0);
 	if (j_dev_n > 0) read_joint_state(st.data(), j_dev_n);
 	std::vector<int> sys(K, -1);
 	for (size_t q = 0; q < hs_og.size(); q++)
 		for (int j : hs_og[q]) sys[j] = (int)q;
-	up(dj_type, j_type); up(dj_a, j_a); up(dj_b, j_b); up(dj_sys, sys);
+	up(dj_type, j_type); up(dj_kind, j_kind); up(dj_a, j_a); up(dj_b, j_b); up(dj_sys, sys);
 	std::vector<double> g[4], ps(K), rs(K);
 	for (auto& v : g) v.resize(3 * (size_t)K);
 	for (int k = 0; k < K; k++) {
@@ -2444,12 +2445,14 @@ int cp_world_set_node_order(cp_world* w, int n, const int64_t* ids) {
 }
 
 int cp_joint_add(cp_world* w, int type, int b1, int b2, const double* geo14, int* joint_index) {
-	if (!w || !geo14 || !joint_index || (type != 3 && type != 5)) return CP_ERR_INVALID;
+	static const int degree[8] = {0, 1, 0, 3, 4, 5, 6, 5};  // rows per kind (ConstraintSolver.h:193-398)
+	if (!w || !geo14 || !joint_index || type < 1 || type > 7 || degree[type] == 0) return CP_ERR_INVALID;
 	if (b1 < 0 || b2 < 0 || b1 >= w->nb || b2 >= w->nb || b1 == b2 || w->b_scene[b1] != w->b_scene[b2]) return CP_ERR_INVALID;
 	for (int c = 0; c < 14; c++)
 		if (!std::isfinite(geo14[c])) return CP_ERR_INVALID;
 	int k = (int)w->j_type.size();
-	w->j_type.push_back(type);
+	w->j_type.push_back(degree[type]);
+	w->j_kind.push_back(type);
 	w->j_a.push_back(b1);
 	w->j_b.push_back(b2);
 	w->j_geo.insert(w->j_geo.end(), geo14, geo14 + 14);
@@ -2478,20 +2481,20 @@ int cp_world_joint_count(cp_world* w, int* n) {
 	return CP_OK;
 }
 
-int cp_world_set_joint_state(cp_world* w, const double* s11) {
-	if (!w || (!s11 && !w->j_type.empty())) return CP_ERR_INVALID;
+int cp_world_set_joint_state(cp_world* w, const double* s12) {
+	if (!w || (!s12 && !w->j_type.empty())) return CP_ERR_INVALID;
 	return guard(w, [&] {
 		if (w->joints_dirty || w->j_dev_n != (int)w->j_type.size()) w->upload_joints();
-		if (w->j_dev_n) w->write_joint_state(s11);
+		if (w->j_dev_n) w->write_joint_state(s12);
 	});
 }
 
-int cp_world_get_joint_state(cp_world* w, double* s11) {
-	if (!w || (!s11 && !w->j_type.empty())) return CP_ERR_INVALID;
+int cp_world_get_joint_state(cp_world* w, double* s12) {
+	if (!w || (!s12 && !w->j_type.empty())) return CP_ERR_INVALID;
 	return guard(w, [&] {
 		if (w->joints_dirty || w->j_dev_n != (int)w->j_type.size()) w->upload_joints();
 		HIPCHK(hipStreamSynchronize(w->stream));
-		if (w->j_dev_n) w->read_joint_state(s11, w->j_dev_n);
+		if (w->j_dev_n) w->read_joint_state(s12, w->j_dev_n);
 	});
 }
 
@@ -2506,8 +2509,8 @@ int cp_world_get_joint_rows(cp_world* w, int k, double* ja, double* jb, double* 
 			for (int c = 0; c < 6; c++) {
 				ja[r * 6 + c] = R[CP_J_JA + r * 6 + c];
 				jb[r * 6 + c] = R[CP_J_JB + r * 6 + c];
-				ita[c * t + r] = R[CP_J_ITA + c * 5 + r];
-				itb[c * t + r] = R[CP_J_ITB + c * 5 + r];
+				ita[c * t + r] = R[CP_J_ITA + c * CP_JMAX + r];
+				itb[c * t + r] = R[CP_J_ITB + c * CP_JMAX + r];
 			}
 		*n = t;
 	});
@@ -2587,7 +2590,7 @@ int cp_world_save_snapshot(cp_world* w, const char* path) {
 	f.cm.resize(2 * (size_t)nc);
 	f.cf.resize(21 * (size_t)nc);
 	if (nc && (st = cp_world_get_contacts(w, nc, f.ci.data(), f.cm.data(), f.cf.data(), &nc)) != CP_OK) return st;
-	f.joint.resize(11 * (size_t)nj);
+	f.joint.resize(CP_JSTATE * (size_t)nj);
 	if (nj && (st = cp_world_get_joint_state(w, f.joint.data())) != CP_OK) return st;
 	f.counts[0] = nb;
 	f.counts[1] = nc;
@@ -2660,7 +2663,7 @@ int cp_world_load_snapshot(cp_world* w, const char* path) {
 		size_t nb = (size_t)f.counts[0], nc = (size_t)f.counts[1], nj = (size_t)f.counts[2];
 		f.state.resize(13 * nb); f.inertia.resize(18 * nb);
 		f.ci.resize(6 * nc); f.cm.resize(2 * nc); f.cf.resize(21 * nc);
-		f.joint.resize(11 * nj);
+		f.joint.resize(CP_JSTATE * nj);
 		get(f.state.data(), f.state.size() * 8);
 		get(f.inertia.data(), f.inertia.size() * 8);
 		get(f.ci.data(), f.ci.size() * 8);

@@ -226,28 +226,42 @@ int cp_world_get_contacts(cp_world* w, int cap, int64_t* ints6, uint64_t* magic2
 int cp_world_set_node_order(cp_world* w, int n, const int64_t* ids);
 int cp_reference_node_order(int n, int64_t* ids);
 
-/* Joints.  Replace PhysicsEngine::addBallSocketConstraint (type 3,
- * PhysicsEngine.cpp:664-687) and addHingeConstraint (type 5, :693-725) between
- * bodies b1 and b2 of one scene; the pair stops colliding (disallowCollision,
- * :666).  geo14 holds the joint as the reference stores it: the tracked
- * attach points in the body frames of b1 and b2 (RigidBody::trackPoint,
- * RigidBody.cpp:134-137: local_coord_origin + attach point), the hinge axes in
- * the body frames (Hinge::b1_rot_axis_body_space, b2_rot_axis_body_space; 0
- * for ball-sockets), the position and rotation correction strengths.
- * Joints are solved in CP_PGS_REFERENCE_ORDER mode; a step of a world with
- * joints in CP_PGS_COLORED mode returns CP_ERR_STATE. */
+/* Joint kinds (cp_joint_add's type) and their degrees (rows). */
+typedef enum {
+	CP_JOINT_DISTANCE = 1,      /* addDistanceConstraint, PhysicsEngine.cpp:623-658; 1 row */
+	CP_JOINT_BALL_SOCKET = 3,   /* addBallSocketConstraint, :664-687; 3 rows */
+	CP_JOINT_SLIDING_HINGE = 4, /* addSlidingHingeConstraint, :860-890; 4 rows */
+	CP_JOINT_HINGE = 5,         /* addHingeConstraint, :693-725; 5 rows */
+	CP_JOINT_WELD = 6,          /* addWeldConstraint, :896-920; 6 rows */
+	CP_JOINT_SLIDER = 7         /* addSliderConstraint, :773-806; 5 rows */
+} cp_joint_kind;
+
+/* Joints between bodies b1 and b2 of one scene; the pair stops colliding
+ * (disallowCollision).  geo14 holds the joint as the reference stores it:
+ * the tracked attach points in the body frames of b1 and b2
+ * (RigidBody::trackPoint, RigidBody.cpp:134-137: local_coord_origin +
+ * attach point), two body-frame axes - the hinge axes (Hinge::b1/
+ * b2_rot_axis_body_space) or the normalised slide axes (Slider /
+ * SlidingHinge ::b1/b2_slide_axis_body_space); for a distance joint the
+ * first slot holds the target distance (Distance::target_distance, the
+ * current distance when the reference is given -1); 0 otherwise - and the
+ * position and rotation correction strengths.  Joints are solved in the
+ * island modes; a step of a world with joints in CP_PGS_COLORED mode returns
+ * CP_ERR_STATE.  Every kind is a holonomic constraint (isHolonomicConstraint)
+ * and joins holonomic systems. */
 int cp_joint_add(cp_world* w, int type, int b1, int b2, const double* geo14, int* joint_index);
 int cp_world_joint_count(cp_world* w, int* n);
-/* Joint solver state, 11 doubles per joint: accumulated impulse (5, unused
- * components 0), hinge basis u and w of the current rows (BallSocket /
- * Hinge ::constraint, ConstraintSolver.h).  Setting it marks the holonomic
- * systems for a rebuild of their rows and factors at the next step. */
-int cp_world_set_joint_state(cp_world* w, const double* state11);
-int cp_world_get_joint_state(cp_world* w, double* state11);
-/* Rows of joint k built by the last step: J_a, J_b [n][6], M^-1 J^T of
- * each body [6][n] (BallSocketConstraint / HingeConstraint,
- * ConstraintSolver.cpp:382-466, :468-560); n = 3 or 5 is returned. */
-int cp_world_get_joint_rows(cp_world* w, int k, double* ja30, double* jb30, double* ita30, double* itb30, int* n);
+/* Joint solver state, 12 doubles per joint: accumulated impulse (6, unused
+ * components 0), basis u and w of the current rows of hinges, sliders and
+ * sliding hinges (0 for the others) (the PersistentConstraint ::constraint
+ * objects, ConstraintSolver.h).  Setting it marks the holonomic systems for a
+ * rebuild of their rows and factors at the next step. */
+int cp_world_set_joint_state(cp_world* w, const double* state12);
+int cp_world_get_joint_state(cp_world* w, double* state12);
+/* Rows of joint k built by the last step: J_a, J_b [n][6], M^-1 J^T of each
+ * body [6][n] (the constraints' constructors, ConstraintSolver.cpp:348-900);
+ * the degree n (1 to 6) is returned. */
+int cp_world_get_joint_rows(cp_world* w, int k, double* ja36, double* jb36, double* ita36, double* itb36, int* n);
 /* Holonomic systems (HolonomicSystem, HolonomicBlockSolver.cpp), solved by
  * block LDL at the first substep when hol_iters > 0.  By default the world
  * finds them as maintainHolonomicSystems does (PhysicsEngine.cpp:1292-1307,

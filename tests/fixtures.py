@@ -360,6 +360,8 @@ class Oracle:
     def step(self, node_order=None):
         if node_order is not None:
             no = _c(node_order, np.int64)
+            if len(no) != self.n:
+                raise ValueError(f"node order holds {len(no)} ids for {self.n} bodies")
             self.lib.orc_step(self.w, no.ctypes.data)
         else:
             self.lib.orc_step(self.w, None)
