@@ -1613,8 +1613,8 @@ struct Contact {
 static double posCorrectCoeff(double s, double dt) { return std::min<double>(s * dt, 1.0 / dt); }  // PersistentConstraint.cpp:199
 
 struct Joint {
-	int kind;       // 1 distance, 3 ball-socket, 4 sliding hinge, 5 hinge, 6 weld, 7 slider
-	int type;       // degree (rows): 1, 3, 4, 5, 6, 5
+	int kind;       // 1 distance, 3 ball-socket, 4 sliding hinge, 5 hinge, 6 weld, 7 slider, 8 cone limit, 9 twist limit
+	int type;       // degree (rows): 1, 3, 4, 5, 6, 5, 1, 1
 	int a, b;       // body indices (b1, b2 as created)
 	V3 tp1, tp2;    // tracked points in the body frames (RigidBody.cpp:134-144)
 	V3 ax1, ax2;    // hinge / slide axes in the body frames; distance: ax1.x = target distance
@@ -1626,6 +1626,12 @@ struct Joint {
 	NM<6, 6> ita, itb;
 	NM<6, 6> inv;
 	double tgt[6] = {}, ptgt[6] = {};
+	// cone (8) and twist (9) limits: ConeRotationLimit / TwistLimit state
+	// (PersistentConstraint.h:145-215); is_inactive starts true
+	bool inactive = true;
+	double cur_angle = 0;
+	int above = 0;  // TwistLimitConstraint::lim_status == ABOVE_MAX
+	bool active() const { return kind < 8 || !inactive; }
 };
 
 struct HSys {
@@ -1829,6 +1835,14 @@ static NM<3, 3> idenNM() {  // idenMat<3> (NMat.h:110-119)
 // 348-900).  reproject: a basis-carrying joint re-projects its stored
 // impulses onto the new basis (false for holonomic rows rebuilt from a
 // stored state).
+// Taylor approximation of 1 - cos f used as the limits' position error
+// (ConstraintSolver.cpp:425-428, :455-457).
+static const double QUAT_PI = 3.1415926535;  // PI of Quaternion.h:4, used by the twist angle's unwrap
+
+static double limitError(double f) {
+	return -f * f * f * f * f * f * f * f / 40320.0 + f * f * f * f * f * f / 720.0 - f * f * f * f / 24.0 + f * f / 2.0;
+}
+
 static void buildJoint(World& W, Joint& j, double sub_dt, bool reproject) {
 	const Body& A = W.bodies[j.a];
 	const Body& B = W.bodies[j.b];
@@ -1842,8 +1856,54 @@ static void buildJoint(World& W, Joint& j, double sub_dt, bool reproject) {
 	NM<6, 6> ja, jb;
 	double pcc = posCorrectCoeff(j.ps, sub_dt);
 	double rcc = posCorrectCoeff(j.rs, sub_dt);
+	if (j.kind == 8) {
+		// ConeRotationLimit::updateSolverConstraints (PersistentConstraint.cpp:233-247):
+		// inactive inside the cone (impulse cleared, rows left as they were)
+		V3 d1 = A.q.rotate(j.ax1), d2 = B.q.rotate(j.ax2);
+		double angle = std::acos(d1.dot(d2));
+		j.inactive = angle < j.tp1.x;
+		if (j.inactive) {
+			j.imp[0] = 0.0;
+			return;
+		}
+		// ConeLimitConstraint (ConstraintSolver.cpp:410-430), warm started from the stored impulse
+		V3 h = d1.cross(d2).normalize();
+		NM<1, 3> hd = rowNM(h);
+		putRow(ja, 0, 3, hd);
+		putRow(jb, 0, 3, neg(hd));
+		j.ptgt[0] = limitError(angle - j.tp1.x) * j.rs;
+	} else if (j.kind == 9) {
+		// TwistLimit::updateSolverConstraints (PersistentConstraint.cpp:249-272)
+		M3 r1 = A.q.rot(), r2 = B.q.rot();
+		V3 wl, ul;
+		j.ax1.perpBasis(&wl, &ul);  // TwistLimit ctor: getPerpendicularBasis(&b1_w_axis_local, &b1_u_axis_local)
+		V3 t1 = r1 * j.ax1, u1 = r1 * ul, w1 = r1 * wl;
+		V3 t2 = r2 * j.ax2;
+		V3 r2w = r2 * wl;  // the reference rotates b1_w_axis_local here (PersistentConstraint.cpp:258)
+		// getCurrentAngle (:115-132); its b2-axis alignment is computed but not used
+		double prev = j.cur_angle;
+		double cur = std::atan2(r2w.dot(u1), r2w.dot(w1));
+		while (prev - cur > QUAT_PI) cur += 2 * QUAT_PI;
+		while (prev - cur < -QUAT_PI) cur -= 2 * QUAT_PI;
+		j.cur_angle = cur;
+		double mn = j.tp1.x, mx = j.tp1.y;
+		j.inactive = mn <= cur && cur <= mx;
+		if (j.inactive) {
+			j.imp[0] = 0.0;
+			return;
+		}
+		// TwistLimitConstraint (ConstraintSolver.cpp:439-459)
+		j.above = cur > mx ? 1 : 0;
+		putRow(ja, 0, 3, neg(rowNM(t1)));
+		putRow(jb, 0, 3, rowNM(t2));
+		double f = j.above ? cur - mx : mn - cur;
+		j.ptgt[0] = limitError(f) * j.rs;
+	}
 	for (int k = 0; k < n; k++) j.imp[k] = start[k];
 	switch (j.kind) {
+	case 8:
+	case 9:
+		break;
 	case 1: {  // DistanceConstraint (ConstraintSolver.cpp:348-379), not in moving mode
 		V3 rA = pa - A.com, rB = pb - B.com;
 		V3 diff = pa - pb;
@@ -2301,7 +2361,7 @@ static void pgsIsland(World& W, const std::vector<std::pair<Contact*, int>>& row
 			addv(a.v, da); addv(b.v, db);
 		}
 	}
-	// PGSConstraintStep<n> of a joint (equality constraint, no projection).
+	// PGSConstraintStep<n> of a joint (ConstraintSolver.cpp:12-40); the limits project.
 	auto jstep = [&](Joint& j, bool pseudo) {
 		double* va = pseudo ? dv[j.a].p : dv[j.a].v;
 		double* vb = pseudo ? dv[j.b].p : dv[j.b].v;
@@ -2316,6 +2376,11 @@ static void pgsIsland(World& W, const std::vector<std::pair<Contact*, int>>& row
 			for (int k = 0; k < n; k++) m[i] += delta[k] * j.inv.v[i][k];
 		}
 		for (int k = 0; k < n; k++) diff[k] = (acc[k] + m[k]) - acc[k];
+		if (j.kind >= 8) {  // projectValidImpulse of the limits (ConstraintSolver.cpp:432-435, :461-463)
+			double nw = acc[0] + m[0];
+			double pr = (j.kind == 9 && j.above) ? std::min<double>(nw, 0) : std::max<double>(nw, 0);
+			diff[0] = pr - acc[0];
+		}
 		if (!isZero(diff, n)) {
 			jointApply(j, diff, va, vb);
 			for (int k = 0; k < n; k++) acc[k] += diff[k];
@@ -2796,6 +2861,8 @@ static int step(World& W, const int64_t* node_order, const int64_t* sched_rows =
 								if (si >= 0 && std::find(sys.begin(), sys.end(), si) == sys.end()) sys.push_back(si);
 							}
 						for (int ji : e.second->joints) {
+							// an inactive limit adds no solver row (PersistentConstraint.h:164)
+							if (!W.joints[ji].active()) continue;
 							rows.push_back({nullptr, ji});
 							lrows.push_back(ji);
 							lrows.push_back(2);
@@ -3064,8 +3131,8 @@ int orc_add_cylinder_body(orc_world* w, int movement, const double* mp, const do
 
 int orc_add_joint(orc_world* w, int type, int b1, int b2, const double* g) {
 	Joint j;
-	static const int degree[8] = {0, 1, 0, 3, 4, 5, 6, 5};
-	if (type < 1 || type > 7 || degree[type] == 0) return -1;
+	static const int degree[10] = {0, 1, 0, 3, 4, 5, 6, 5, 1, 1};
+	if (type < 1 || type > 9 || degree[type] == 0) return -1;
 	j.kind = type;
 	j.type = degree[type];
 	j.a = b1;
@@ -3147,6 +3214,14 @@ void orc_set_joint_state(orc_world* w, const double* s) {
 		Joint& j = w->joints[i];
 		const double* r = s + 12 * i;
 		for (int k = 0; k < 6; k++) j.imp[k] = r[k];
+		if (j.kind >= 8) {
+			// limits: is_inactive, the twist's tracked angle; an active twist
+			// limit was built at that angle (lim_status)
+			j.inactive = r[6] != 0;
+			j.cur_angle = j.kind == 9 ? r[7] : 0.0;
+			j.above = j.kind == 9 && !j.inactive && j.cur_angle > j.tp1.y;
+			continue;
+		}
 		j.u = V3(r[6], r[7], r[8]);
 		j.w = V3(r[9], r[10], r[11]);
 	}
@@ -3158,6 +3233,12 @@ void orc_get_joint_state(const orc_world* w, double* s) {
 		const Joint& j = w->joints[i];
 		double* r = s + 12 * i;
 		for (int k = 0; k < 6; k++) r[k] = k < j.type ? j.imp[k] : 0.0;
+		if (j.kind >= 8) {
+			r[6] = j.inactive ? 1.0 : 0.0;
+			r[7] = j.cur_angle;
+			for (int k = 8; k < 12; k++) r[k] = 0.0;
+			continue;
+		}
 		r[6] = j.u.x; r[7] = j.u.y; r[8] = j.u.z;
 		r[9] = j.w.x; r[10] = j.w.y; r[11] = j.w.z;
 	}

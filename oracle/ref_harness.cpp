@@ -221,7 +221,7 @@ static void build_mixed(PhysicsEngine& E, int nx, int ny, int nz, uint64_t seed,
 // 0.1; chain c at z = 1.2 c, height U[0.4, 1.2], x offset U[-0.5, 0.5]; every link
 // gets v ~ U[-0.5, 0.5]^3 so the chain bends as it falls onto the ground box.
 // Joint kinds as the dumps number them: 1 distance, 3 ball-socket, 4 sliding
-// hinge, 5 hinge, 6 weld, 7 slider.
+// hinge, 5 hinge, 6 weld, 7 slider, 8 cone limit, 9 twist limit.
 struct JointRec {
 	int type;
 	RigidBody *b1, *b2;
@@ -292,6 +292,18 @@ static void joint_state(std::vector<double>& s) {
 		case 4: put_joint_basis(s, static_cast<phyz::SlidingHinge*>(j.pc)->constraint, 4); break;
 		case 5: put_joint_basis(s, static_cast<phyz::Hinge*>(j.pc)->constraint, 5); break;
 		case 6: put_joint(s, static_cast<phyz::Weld*>(j.pc)->constraint, 6, false); break;
+		case 8: {  // impulse, then is_inactive
+			auto* c = static_cast<phyz::ConeRotationLimit*>(j.pc);
+			put_impulse(s, c->constraint, 1);
+			s.insert(s.end(), {c->is_inactive ? 1.0 : 0.0, 0, 0, 0, 0, 0});
+			break;
+		}
+		case 9: {  // impulse, then is_inactive and the tracked current angle
+			auto* t = static_cast<phyz::TwistLimit*>(j.pc);
+			put_impulse(s, t->constraint, 1);
+			s.insert(s.end(), {t->is_inactive ? 1.0 : 0.0, t->current_angle, 0, 0, 0, 0});
+			break;
+		}
 		default: put_joint_basis(s, static_cast<phyz::Slider*>(j.pc)->constraint, 5); break;
 		}
 	}
@@ -616,6 +628,18 @@ static void dump_static(PhysicsEngine& E, Out& o) {
 			auto* h = static_cast<phyz::Hinge*>(j.pc);
 			geo(j, h->b1_point_key, h->b2_point_key, h->b1_rot_axis_body_space, h->b2_rot_axis_body_space, h->pos_correct_hardness,
 			    h->rot_correct_hardness);
+			break;
+		}
+		case 8: {  // tracked-point slots: the cone's max angle
+			auto* c = static_cast<phyz::ConeRotationLimit*>(j.pc);
+			mthz::Vec3 a1 = c->b1_allignment_axis_local, a2 = c->b2_allignment_axis_local;
+			jg.insert(jg.end(), {c->max_rotation_angle, 0, 0, 0, 0, 0, a1.x, a1.y, a1.z, a2.x, a2.y, a2.z, 0, c->rot_correct_hardness});
+			break;
+		}
+		case 9: {  // tracked-point slots: the twist's min and max angles
+			auto* t = static_cast<phyz::TwistLimit*>(j.pc);
+			mthz::Vec3 a1 = t->b1_twist_axis_local, a2 = t->b2_twist_axis_local;
+			jg.insert(jg.end(), {t->min_angle, t->max_angle, 0, 0, 0, 0, a1.x, a1.y, a1.z, a2.x, a2.y, a2.z, 0, t->rot_correct_hardness});
 			break;
 		}
 		case 6: {
@@ -1063,6 +1087,55 @@ static int cmd_joints(int argc, char** argv) {
 	return 0;
 }
 
+// limits: cone and twist limits.  Two pendulums hang from FIXED anchors:
+// a two-link ball-socket pendulum (one holonomic system) with a cone limit
+// (0.12 rad about the downward axis) between its links and one (0.15 rad)
+// between the anchor and the first link; and a box on a hinge about +y with a twist
+// limit [-0.3, 0.4] rad about the same axis.  Each link gets a sideways
+// velocity U[1, 2] and an angular velocity about +y U[2, 3] so the limits
+// engage and release; default correction strengths.
+static int cmd_limits(int argc, char** argv) {
+	int steps = (int)arg_d(argc, argv, "--steps", 150);
+	uint64_t seed = (uint64_t)arg_d(argc, argv, "--seed", 311);
+	SceneCfg cfg;
+	cfg.hol_iters = (int)arg_d(argc, argv, "--hol", 1);
+	PhysicsEngine E;
+	configure(E, cfg);
+	phyz::Material m = c0_material();
+	RigidBody* g = E.createRigidBody(phyz::ConvexUnionGeometry::box(mthz::Vec3(-100, -1, -100), 200, 1, 200, m), RigidBody::FIXED);
+	pin_body(g);
+	SplitMix64 rng(seed);
+	auto kick = [&](RigidBody* b) {
+		b->setVel(mthz::Vec3(rng.uniform(1, 2), 0, rng.uniform(-0.5, 0.5)));
+		b->setAngVel(mthz::Vec3(0, rng.uniform(2, 3), 0));
+		pin_body(b);
+	};
+	// attach points are given in the creation frame, i.e. as world points
+	RigidBody* a1 = E.createRigidBody(phyz::ConvexUnionGeometry::box(mthz::Vec3(-0.5, 8.5, -0.5), 1, 1, 1, m), RigidBody::FIXED);
+	pin_body(a1);
+	RigidBody* p1 = E.createRigidBody(phyz::ConvexUnionGeometry::box(mthz::Vec3(-0.2, 7.2, -0.2), 0.4, 1.2, 0.4, m));
+	RigidBody* p2 = E.createRigidBody(phyz::ConvexUnionGeometry::box(mthz::Vec3(-0.2, 5.9, -0.2), 0.4, 1.2, 0.4, m));
+	kick(p1);
+	kick(p2);
+	mthz::Vec3 down(0, -1, 0);
+	record_joint<phyz::BallSocket>(E, E.addBallSocketConstraint(a1, p1, mthz::Vec3(0, 8.45, 0), mthz::Vec3(0, 8.45, 0)), 3, a1, p1);
+	record_joint<phyz::BallSocket>(E, E.addBallSocketConstraint(p1, p2, mthz::Vec3(0, 7.15, 0), mthz::Vec3(0, 7.15, 0)), 3, p1, p2);
+	record_joint<phyz::ConeRotationLimit>(E, E.addConeLimitConstraint(a1, p1, down, down, 0.15), 8, a1, p1);
+	record_joint<phyz::ConeRotationLimit>(E, E.addConeLimitConstraint(p1, p2, down, down, 0.12), 8, p1, p2);
+	RigidBody* a2 = E.createRigidBody(phyz::ConvexUnionGeometry::box(mthz::Vec3(3.5, 4.5, -0.5), 1, 1, 1, m), RigidBody::FIXED);
+	pin_body(a2);
+	RigidBody* t1 = E.createRigidBody(phyz::ConvexUnionGeometry::box(mthz::Vec3(3.4, 3.2, -0.6), 1.2, 1.2, 1.2, m));
+	kick(t1);
+	mthz::Vec3 up(0, 1, 0);
+	record_joint<phyz::Hinge>(E, E.addHingeConstraint(a2, t1, mthz::Vec3(4, 4.45, 0), mthz::Vec3(4, 4.45, 0), up, up), 5, a2, t1);
+	record_joint<phyz::TwistLimit>(E, E.addTwistLimitConstraint(a2, t1, up, -0.3, 0.4), 9, a2, t1);
+	Out o(arg_s(argc, argv, "--out", "out"));
+	o.meta("seed", (double)seed);
+	run_trace(E, cfg, steps, arg_steps(argc, argv, "--record"), o);
+	o.finish();
+	return 0;
+}
+
 // sleep: sleeping enabled (the engine's default parameters, every body's
 // sleep_disabled pinned to false): a 2x2x2 cube pile and a lone cube settle
 // and fall asleep, a cube launched along the ground slides to rest and falls
@@ -1275,6 +1348,7 @@ int main(int argc, char** argv) {
 	if (cmd == "hulls") return cmd_hulls(argc, argv);
 	if (cmd == "sleep") return cmd_sleep(argc, argv);
 	if (cmd == "joints") return cmd_joints(argc, argv);
+	if (cmd == "limits") return cmd_limits(argc, argv);
 	fprintf(stderr, "unknown command %s\n", cmd.c_str());
 	return 2;
 }

@@ -8,7 +8,7 @@ ROOT=$(cd "$HERE/../.." && pwd)
 make -C "$ROOT/oracle" ref REF="${REF:-/root/reference}"
 H="$ROOT/oracle/_ref/cp_ref_harness"
 cd "$HERE"
-rm -rf pile3 c0_full mix3 cyl3 cyl4 chain3 chain3h0 terrain hulls sleep joints joints_h0
+rm -rf pile3 c0_full mix3 cyl3 cyl4 chain3 chain3h0 terrain hulls sleep joints joints_h0 limits limits_h0
 # C0 pattern, 3x3x3 cubes: full per-step dumps (state, caches, manifolds, islands).
 "$H" pile --n 3 --steps 40 --record 0,1,2,10,39 --out pile3
 # C0 in full (10x10x10 cubes): first step and a settled step (no mid/post caches).
@@ -46,3 +46,10 @@ python3 make_hull_input.py hull_input.txt 24 20240
 # and without the holonomic block solver.
 "$H" joints --steps 120 --record 0,1,2,30,60,119 --out joints
 "$H" joints --steps 120 --record 0,1,30,60,119 --hol 0 --out joints_h0
+# Cone and twist limits: a two-link ball-socket pendulum under cone limits
+# (0.15 and 0.12 rad) and a hinged block under a twist limit (-0.3, 0.4),
+# kicked so that the limits engage and release (steps 15, 16, 32 and 57 solve
+# a limit that the step's own update found inside its range); with and
+# without the holonomic block solver.
+"$H" limits --steps 150 --record 0,1,10,11,14,15,16,20,21,30,32,40,57,80,145,149 --out limits
+"$H" limits --steps 150 --record 0,1,10,15,16,20,30,32,57,80,149 --hol 0 --out limits_h0

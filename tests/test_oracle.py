@@ -99,13 +99,14 @@ def test_mixed_primitives_full_step(name, k):
     assert np.array_equal(pi, gi) and np.array_equal(pm, gm) and np.array_equal(pf, gf)
 
 
-@pytest.mark.parametrize("name", ["chain3", "chain3h0", "joints", "joints_h0"])
+@pytest.mark.parametrize("name", ["chain3", "chain3h0", "joints", "joints_h0", "limits", "limits_h0"])
 def test_jointed_chains_full_step(name):
     """Equality joints (PGS rows) and, for chain3 and joints, the holonomic
     block solver: elimination order, block table and LDL factor buffer, then
     the full step (state, joint impulses and bases, island rows).  chain3:
     ball-sockets and hinges; joints: distance, weld, slider and sliding
-    hinge, chained into one system of degrees 1, 6, 5, 4."""
+    hinge, chained into one system of degrees 1, 6, 5, 4; limits: cone and
+    twist limits (inequality rows, projected) that engage and release."""
     import constraintphysics_amd as cpa
     fx = golden(name)
     for k in fx.recorded:
@@ -128,17 +129,28 @@ def test_jointed_chains_full_step(name):
         assert np.array_equal(o.joint_state(), fx[f"s{k}_joint_post"])
 
 
-@pytest.mark.parametrize("name", ["joints", "joints_h0"])
+def initial_joint_state(fx):
+    """Joint state of freshly added joints: zero impulses and bases; a cone or
+    twist limit starts inactive (PersistentConstraint.h:148-151, :177-183)."""
+    s = np.zeros((len(fx["joint_info"]), 12))
+    s[fx["joint_info"][:, 0] >= 8, 6] = 1.0
+    return s
+
+
+@pytest.mark.parametrize("name", ["joints", "joints_h0", "limits", "limits_h0"])
 def test_other_joints_whole_run(name):
-    """The joints scene stepped from its initial state by the oracle alone:
-    every recorded step's state and joint state equal the reference's."""
+    """The joints and limits scenes stepped from their initial states by the
+    oracle alone: every recorded step's state and joint state equal the
+    reference's.  In the limits scene a limit found active at the island
+    search and inactive by its own update solves with the rows of its last
+    active build; the run is exact only if that is reproduced."""
     import constraintphysics_amd as cpa
     fx = golden(name)
     o = Oracle(fx)
     o.load(fx["initial_state"])
     # the holonomic system the reference founds at the first step (the oracle
     # takes systems as given)
-    o.load_joints(np.zeros((len(fx["joint_info"]), 12)), fx["s1_hsys_off"], fx["s1_hsys_og"])
+    o.load_joints(initial_joint_state(fx), fx["s1_hsys_off"], fx["s1_hsys_og"])
     no = cpa.reference_node_order(int(fx.meta["n_bodies"]))
     rec = set(fx.recorded)
     for k in range(int(fx.meta["steps"])):
@@ -147,3 +159,40 @@ def test_other_joints_whole_run(name):
             assert np.array_equal(o.state(), fx[f"s{k}_post"]), k
             assert np.array_equal(o.joint_state(), fx[f"s{k}_joint_post"]), k
     assert np.array_equal(o.state(), fx["final_state"])
+
+
+def _rotate(q, v):
+    w, u = q[0], np.asarray(q[1:4])
+    return 2 * np.dot(u, v) * u + (w * w - np.dot(u, u)) * v + 2 * w * np.cross(u, v)
+
+
+def test_limit_angles_closed_form():
+    """The limits fixture against geometry computed here from the recorded
+    post-step orientations (bodies' state rows: com, quaternion w x y z,
+    velocity, angular velocity).  The limits evaluate their angle at the last
+    substep's start, one substep (h = dt / substeps) before the post state.
+    Twist: the hinged block turns about +y under an anchor of identity
+    orientation, so the tracked angle is 2 atan2(q_y, q_w) - w_y h.  Cone: the
+    flag is_inactive holds exactly when acos(d1 . d2) < max angle, d the
+    bodies' rotated axes, wherever the angle is farther than |w| h from the
+    bound.  The oracle equals the fixture bit for bit (the tests above)."""
+    fx = golden("limits")
+    h = fx.meta["dt"] / fx.meta["substeps"]
+    info, geo = fx["joint_info"], fx["joint_geo"]
+    decided = {0.0: 0, 1.0: 0}
+    for k in fx.recorded:
+        s, j = fx[f"s{k}_post"], fx[f"s{k}_joint_post"]
+        for ji in np.nonzero(info[:, 0] == 9)[0]:
+            b = info[ji, 2] - 1
+            twist = 2 * np.arctan2(s[b, 5], s[b, 3]) - s[b, 11] * h
+            assert abs(j[ji, 7] - twist) <= 1e-9, (k, j[ji, 7], twist)
+            assert (j[ji, 6] == 1.0) == (geo[ji, 0] <= j[ji, 7] <= geo[ji, 1])
+        for ji in np.nonzero(info[:, 0] == 8)[0]:
+            a, b = info[ji, 1] - 1, info[ji, 2] - 1
+            d1, d2 = _rotate(s[a, 3:7], geo[ji, 6:9]), _rotate(s[b, 3:7], geo[ji, 9:12])
+            ang = np.arccos(np.clip(d1 @ d2, -1, 1))
+            tol = (np.linalg.norm(s[a, 10:13]) + np.linalg.norm(s[b, 10:13])) * h
+            if abs(ang - geo[ji, 0]) > tol:
+                assert (j[ji, 6] == 1.0) == (ang < geo[ji, 0]), (k, ji, ang)
+                decided[j[ji, 6]] += 1
+    assert decided[0.0] >= 3 and decided[1.0] >= 3, decided
