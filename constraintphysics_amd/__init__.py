@@ -517,10 +517,12 @@ class World:
 
     def add_joint(self, kind, b1, b2, geo14):
         """Joint of kind 1 distance, 3 ball-socket, 4 sliding hinge, 5 hinge,
-        6 weld or 7 slider between body indices b1, b2; geo14 = attach points
-        in the body frames, two body-frame axes (hinge or slide axes; a
-        distance joint's target distance first), position and rotation
-        correction strengths."""
+        6 weld, 7 slider, 8 cone limit or 9 twist limit between body indices
+        b1, b2; geo14 = attach points in the body frames, two body-frame axes
+        (hinge or slide axes; a distance joint's target distance first),
+        position and rotation correction strengths.  A cone limit holds its
+        maximum angle, a twist limit its minimum and maximum angles, in the
+        first attach-point slots."""
         g = _a(geo14, np.float64).reshape(14)
         k = ctypes.c_int()
         _chk(self.L.cp_joint_add(self.w, int(kind), int(b1), int(b2), g.ctypes.data, ctypes.byref(k)), "cp_joint_add")
@@ -532,7 +534,8 @@ class World:
         return n.value
 
     def set_joint_state(self, state12):
-        """Per joint: 6 accumulated impulses (zero padded), basis u, w."""
+        """Per joint: 6 accumulated impulses (zero padded), basis u, w (a
+        limit: is_inactive and the twist's tracked angle in place of u)."""
         s = _a(state12, np.float64)
         assert s.shape == (self.joint_count(), 12)
         _chk(self.L.cp_world_set_joint_state(self.w, s.ctypes.data), "cp_world_set_joint_state")

@@ -86,8 +86,9 @@ struct Params {
 };
 
 // Joints, SoA over n joints.  kind: 1 distance, 3 ball-socket, 4 sliding
-// hinge, 5 hinge, 6 weld, 7 slider; type: the joint's degree (rows: 1, 3, 4,
-// 5, 6, 5).  State arrays are component-major ([CP_JMAX][n], [3][n]).
+// hinge, 5 hinge, 6 weld, 7 slider, 8 cone limit, 9 twist limit; type: the
+// joint's degree (rows: 1, 3, 4, 5, 6, 5, 1, 1).  State arrays are
+// component-major ([CP_JMAX][n], [3][n]).
 #define CP_JMAX 6
 #define CP_JSTATE 12  // joint state per joint at the ABI: CP_JMAX impulses, u, w
 struct Joints {
@@ -96,9 +97,15 @@ struct Joints {
 	double *tp1, *tp2, *ax1, *ax2;  // [3][n] body-frame attach points and axes (distance: ax1 x = target distance)
 	double *ps, *rs;                // correction strengths
 	double *imp, *pimp, *himp;      // [CP_JMAX][n]
-	double *u, *w;                  // [3][n] basis of the current rows (hinge, slider, sliding hinge)
+	double *u, *w;                  // [3][n] basis of the current rows (hinge, slider, sliding hinge);
+	                                // limits: u x = is_inactive (1.0 / 0.0), u y = the twist's tracked angle
+	int* proj;                      // impulse projection of the rows: 0 none, 1 to >= 0, 2 to <= 0 (limits)
 	double* row;                    // CP_JRS doubles per joint
 };
+// Cone (8) and twist (9) limits: one projected row, present in an island only
+// while active (ConeRotationLimit / TwistLimit, PersistentConstraint.h:145-215).
+__host__ __device__ __forceinline__ bool joint_is_limit(int kind) { return kind >= 8; }
+__device__ __forceinline__ bool joint_active(const int* kind, const double* u, int k) { return !joint_is_limit(kind[k]) || u[k] == 0.0; }
 #define CP_JRS 192
 enum JointRowField {
 	CP_J_JA = 0,      // J_a [CP_JMAX][6]
@@ -190,7 +197,8 @@ struct IslTables {
 struct IslWalk {
 	IslTables T;
 	int nb, hol;
-	const int *movement, *asleep, *aoff, *adjv, *me, *jord, *live, *ja, *jb, *jsys;
+	const int *movement, *asleep, *aoff, *adjv, *me, *jord, *live, *ja, *jb, *jsys, *jkind;
+	const double* ju;  // joint state u: a limit's is_inactive flag
 	const uint64_t *adjk, *ckey;
 	int *stack, *vlist, *elist, *vis, *rows, *ecol, *coff, *eorder, *icoff, *ncol, *serial, *eoff, *bod, *nbod, *loc, *cloc, *jloc;
 	int *sys_seen, *sys, *slots, *jlin, *sys_cnt;
@@ -437,7 +445,8 @@ void launch_pgs_colored(const Bodies& b, const Contacts& C, const int* scene_bod
 void launch_ib_keys(const uint64_t* ekey, int ne, const uint64_t* jgkey, int ng, uint64_t* keys, int* vals, hipStream_t s);
 void launch_ib_heads(const uint64_t* skeys, int n, int* head, hipStream_t s);
 void launch_ib_edges(const uint64_t* skeys, const int* svals, int n, const int* head, const int* midx, int ne, const int* efirst,
-                     const int* ecnt, const int* live, const int* jgoff, int* me, uint64_t* adjk, int* adjv, hipStream_t s);
+                     const int* ecnt, const int* live, const int* jgoff, const int* jord, const int* jkind, const double* ju, int* me,
+                     uint64_t* adjk, int* adjv, hipStream_t s);
 void launch_ib_aoff(const uint64_t* adjk, int n2, int nb, int* aoff, hipStream_t s);
 void launch_ib_components(const int* me, const int* midx, int n, int nb, const int* movement, const int* rank, const int* aoff, int* parent,
                           int* comp, IslCnt* acc, const SleepDev& S, hipStream_t s);

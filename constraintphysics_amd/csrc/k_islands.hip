@@ -75,10 +75,13 @@ __global__ void k_ib_heads(const uint64_t* skeys, int n, int* head) {
 }
 
 // Merged edge e = midx[i] of head entry i: lo, hi, first contact, contacts,
-// live contacts, joints jord[j0, j1); and its two adjacency entries.  Slots
-// past the merged count get adjacency keys that sort last.
+// live contacts, joints jord[j0, j1), active joints among them (a limit adds
+// a row only while active, by the flag of its last update); and its two
+// adjacency entries.  Slots past the merged count get adjacency keys that
+// sort last.
 __global__ void k_ib_edges(const uint64_t* skeys, const int* svals, int n, const int* head, const int* midx, int ne, const int* efirst,
-                           const int* ecnt, const int* live, const int* jgoff, int* me, uint64_t* adjk, int* adjv) {
+                           const int* ecnt, const int* live, const int* jgoff, const int* jord, const int* jkind, const double* ju,
+                           int* me, uint64_t* adjk, int* adjv) {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n) return;
 	const int M = midx[n];
@@ -103,11 +106,12 @@ __global__ void k_ib_edges(const uint64_t* skeys, const int* svals, int n, const
 		} else {
 			E[5] = jgoff[v - ne];
 			E[6] = jgoff[v - ne + 1];
+			for (int x2 = E[5]; x2 < E[6]; x2++) E[7] += joint_active(jkind, ju, jord[x2]);
 		}
 	}
 	int4* o = reinterpret_cast<int4*>(me + 8 * (size_t)e);
 	o[0] = int4{E[0], E[1], E[2], E[3]};
-	o[1] = int4{E[4], E[5], E[6], 0};
+	o[1] = int4{E[4], E[5], E[6], E[7]};
 	adjk[2 * e] = (uint64_t)(uint32_t)E[0] << 32 | (uint32_t)E[1];
 	adjk[2 * e + 1] = (uint64_t)(uint32_t)E[1] << 32 | (uint32_t)E[0];
 	adjv[2 * e] = e;
@@ -157,9 +161,9 @@ __global__ void k_ib_bodies(int nb, const int* movement, const int* aoff, int* p
 
 // The rows an edge gives its island: none when neither end is an awake
 // dynamic body (bfsVisitAll skips an asleep node's edges to asleep or fixed
-// neighbours); else its joints and two rows per live contact.
+// neighbours); else its active joints and two rows per live contact.
 __device__ __forceinline__ bool edge_emitted(const int* E, const int* movement, const int* asleep) {
-	if (E[5] == E[6] && E[4] == 0) return false;
+	if (E[7] == 0 && E[4] == 0) return false;
 	bool a = movement[E[0]] == 0 && !(asleep && asleep[E[0]]);
 	bool b = movement[E[1]] == 0 && !(asleep && asleep[E[1]]);
 	return a || b;
@@ -172,7 +176,7 @@ __global__ void k_ib_edge_counts(const int* me, const int* midx, int n, const in
 	int d = movement[E[0]] == 0 ? E[0] : (movement[E[1]] == 0 ? E[1] : -1);
 	if (d < 0 || !edge_emitted(E, movement, asleep)) return;
 	int r = comp[d];
-	int nj = E[6] - E[5];
+	int nj = E[7];
 	atomicAdd(&acc[r].rows, nj + 2 * E[4]);
 	atomicAdd(&acc[r].edges, 1);
 	atomicAdd(&acc[r].joints, nj);
@@ -279,7 +283,7 @@ __global__ void k_ib_walk(IslWalk W, int nisl) {
 			if (W.vis[o]) continue;
 			if (ca && (mv[o] != 0 || (asleep && asleep[o]))) continue;
 			const int* E = W.me + 8 * (size_t)W.adjv[x];
-			if ((E[5] < E[6] || E[4] > 0) && ne < ecap) el[ne++] = W.adjv[x];
+			if ((E[7] > 0 || E[4] > 0) && ne < ecap) el[ne++] = W.adjv[x];
 		}
 		for (int x = a0; x < a1; x++) {
 			int o = (int)(W.adjk[x] & 0xFFFFFFFFu);
@@ -294,6 +298,7 @@ __global__ void k_ib_walk(IslWalk W, int nisl) {
 	auto emit = [&](int e) {
 		const int* E = W.me + 8 * (size_t)e;
 		for (int x = E[5]; x < E[6] && nr < rcap; x++) {
+			if (!joint_active(W.jkind, W.ju, W.jord[x])) continue;
 			rows[2 * nr] = W.jord[x];
 			rows[2 * nr + 1] = 2;
 			nr++;
@@ -436,10 +441,11 @@ void launch_ib_heads(const uint64_t* skeys, int n, int* head, hipStream_t s) {
 	hipLaunchKernelGGL(k_ib_heads, dim3(nblk256(n + 1)), dim3(256), 0, s, skeys, n, head);
 }
 void launch_ib_edges(const uint64_t* skeys, const int* svals, int n, const int* head, const int* midx, int ne, const int* efirst,
-                     const int* ecnt, const int* live, const int* jgoff, int* me, uint64_t* adjk, int* adjv, hipStream_t s) {
+                     const int* ecnt, const int* live, const int* jgoff, const int* jord, const int* jkind, const double* ju, int* me,
+                     uint64_t* adjk, int* adjv, hipStream_t s) {
 	if (n)
-		hipLaunchKernelGGL(k_ib_edges, dim3(nblk256(n)), dim3(256), 0, s, skeys, svals, n, head, midx, ne, efirst, ecnt, live, jgoff, me,
-		                   adjk, adjv);
+		hipLaunchKernelGGL(k_ib_edges, dim3(nblk256(n)), dim3(256), 0, s, skeys, svals, n, head, midx, ne, efirst, ecnt, live, jgoff, jord,
+		                   jkind, ju, me, adjk, adjv);
 }
 void launch_ib_aoff(const uint64_t* adjk, int n2, int nb, int* aoff, hipStream_t s) {
 	hipLaunchKernelGGL(k_ib_aoff, dim3(nblk256(n2 + 1)), dim3(256), 0, s, adjk, n2, nb, aoff);

@@ -109,6 +109,13 @@ __device__ V3 rot_correct(Q a, Q b, double h) {
 	return V3{axis.x * angle, axis.y * angle, axis.z * angle};
 }
 
+// Taylor approximation of 1 - cos f, the limits' position error
+// (ConstraintSolver.cpp:425-428, :455-457).
+__device__ __forceinline__ double limit_error(double f) {
+	return -f * f * f * f * f * f * f * f / 40320.0 + f * f * f * f * f * f / 720.0 - f * f * f * f / 24.0 + f * f / 2.0;
+}
+#define CP_QUAT_PI 3.1415926535  // PI of Quaternion.h:4, the twist angle's unwrap
+
 // Solver rows of joint k from the current body state: the joint's
 // updateSolverConstraints (PersistentConstraint.cpp:217-354, warm start
 // coefficient 1) and its constraint's constructor (ConstraintSolver.cpp:
@@ -140,6 +147,57 @@ __device__ void build_joint(const Bodies& b, Joints& J, int k, double sub_dt, do
 	}
 	double start[CP_JMAX];
 	for (int c = 0; c < n; c++) start[c] = J.imp[c * K + k] * 1.0;
+	const int kind = J.kind[k];
+	double lim_err = 0;  // a limit's position error (1 - cos of the excess, Taylor)
+	V3 lim_a{0, 0, 0}, lim_b{0, 0, 0};  // a limit's angular rows
+	if (kind == 8) {
+		// ConeRotationLimit::updateSolverConstraints (PersistentConstraint.cpp:233-247):
+		// inside the cone the limit clears its impulse and keeps its rows
+		V3 d1 = q_rotate(qa, jld3(J.ax1, k, K)), d2 = q_rotate(qb, jld3(J.ax2, k, K));
+		double angle = acos(dot(d1, d2));
+		const double lim = J.tp1[k];
+		bool inactive = angle < lim;
+		J.u[k] = inactive ? 1.0 : 0.0;
+		if (inactive) {
+			J.imp[k] = 0.0;
+			return;
+		}
+		// ConeLimitConstraint (ConstraintSolver.cpp:410-430)
+		V3 h = normalize(cross(d1, d2));
+		lim_a = h;
+		lim_b = V3{-h.x, -h.y, -h.z};
+		lim_err = limit_error(angle - lim);
+		J.proj[k] = 1;
+	} else if (kind == 9) {
+		// TwistLimit::updateSolverConstraints (PersistentConstraint.cpp:249-272);
+		// the angle as getCurrentAngle (:115-132) measures it: the reference
+		// rotates b1's local w axis by b2's orientation, and the b2-axis
+		// alignment it computes is not used
+		M3 r1 = q_rot(qa), r2 = q_rot(qb);
+		V3 axl = jld3(J.ax1, k, K), wl, ul;
+		perp_basis(axl, &wl, &ul);  // the constructor's getPerpendicularBasis(&w, &u)
+		V3 t1 = r1 * axl, u1 = r1 * ul, w1 = r1 * wl;
+		V3 t2 = r2 * jld3(J.ax2, k, K);
+		V3 r2w = r2 * wl;
+		double prev = J.u[K + k];
+		double cur = atan2(dot(r2w, u1), dot(r2w, w1));
+		while (prev - cur > CP_QUAT_PI) cur += 2 * CP_QUAT_PI;
+		while (prev - cur < -CP_QUAT_PI) cur -= 2 * CP_QUAT_PI;
+		J.u[K + k] = cur;
+		const double mn = J.tp1[k], mx = J.tp1[K + k];
+		bool inactive = mn <= cur && cur <= mx;
+		J.u[k] = inactive ? 1.0 : 0.0;
+		if (inactive) {
+			J.imp[k] = 0.0;
+			return;
+		}
+		// TwistLimitConstraint (ConstraintSolver.cpp:439-459)
+		bool above = cur > mx;
+		lim_a = V3{-t1.x, -t1.y, -t1.z};
+		lim_b = t2;
+		lim_err = limit_error(above ? cur - mx : mn - cur);
+		J.proj[k] = above ? 2 : 1;
+	}
 	for (int c = 0; c < n; c++) J.imp[c * K + k] = start[c];
 	double ja[CP_JMAX][6], jb[CP_JMAX][6];
 	for (int r = 0; r < CP_JMAX; r++)
@@ -149,7 +207,6 @@ __device__ void build_joint(const Bodies& b, Joints& J, int k, double sub_dt, do
 		}
 	const double pcc = pos_coeff(J.ps[k], sub_dt), rcc = pos_coeff(J.rs[k], sub_dt);
 	double* pt = R + CP_J_PTGT;
-	const int kind = J.kind[k];
 	// new basis (u, w) of a basis-carrying joint and its re-projection of
 	// impulses i0, i0 + 1
 	auto basis = [&](V3 axis, int i0, V3* u, V3* w) {
@@ -162,7 +219,11 @@ __device__ void build_joint(const Bodies& b, Joints& J, int k, double sub_dt, do
 		jst3(J.u, k, K, *u);
 		jst3(J.w, k, K, *w);
 	};
-	if (kind == 1) {  // DistanceConstraint (ConstraintSolver.cpp:348-379), not in moving mode
+	if (joint_is_limit(kind)) {
+		ja[0][3] = lim_a.x; ja[0][4] = lim_a.y; ja[0][5] = lim_a.z;
+		jb[0][3] = lim_b.x; jb[0][4] = lim_b.y; jb[0][5] = lim_b.z;
+		pt[0] = lim_err * J.rs[k];  // the raw rotation hardness, not posCorrectCoeff
+	} else if (kind == 1) {  // DistanceConstraint (ConstraintSolver.cpp:348-379), not in moving mode
 		V3 rA = pa - ca, rB = pb - cb;
 		V3 diff = pa - pb;
 		V3 dd = normalize(diff);

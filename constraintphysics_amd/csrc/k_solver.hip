@@ -573,7 +573,8 @@ __device__ __forceinline__ void joint_target(const Bodies& b, const Joints& J, i
 	for (int c = 0; c < J.type[k]; c++) R[CP_J_TGT + c] = -val[c];
 }
 
-// PGSConstraintStep<n> of an equality constraint (ConstraintSolver.cpp:12-40).
+// PGSConstraintStep<n> of a joint (ConstraintSolver.cpp:12-40); the limits
+// project the accumulated impulse.
 __device__ __forceinline__ void joint_step(const Bodies& b, const Joints& J, const DvMap& m, int k, bool pseudo) {
 	const int K = J.n, S = m.stride;
 	double* R = J.row + (size_t)k * CP_JRS;
@@ -593,6 +594,13 @@ __device__ __forceinline__ void joint_step(const Bodies& b, const Joints& J, con
 		for (int c = 0; c < n; c++) mm[i] += delta[c] * R[CP_J_INV + i * n + c];
 	}
 	for (int c = 0; c < n; c++) diff[c] = (a[c] + mm[c]) - a[c];
+	const int pj = J.proj[k];
+	if (pj) {  // projectValidImpulse of the limits (ConstraintSolver.cpp:432-435, :461-463), on the accumulation
+		double nw = a[0] + mm[0];
+		// std::min / std::max<double>(nw, 0): signed zeros and NaN as they pass
+		double pr = pj == 2 ? (0.0 < nw ? 0.0 : nw) : (nw < 0.0 ? 0.0 : nw);
+		diff[0] = pr - a[0];
+	}
 	if (!is_zero_n(diff, n)) {
 		joint_apply(R, n, diff, va, vb, S, b.movement[ia] == 0, b.movement[ib] == 0);
 		for (int c = 0; c < n; c++) acc[c * K + k] = a[c] + diff[c];

@@ -226,7 +226,7 @@ struct cp_world {
 	std::vector<double> j_geo;  // 14 per joint: tp1, tp2, ax1, ax2, ps, rs
 	int j_dev_n = 0;            // joints on the device
 	bool joints_dirty = false;  // static joint data or systems changed since the upload
-	DArr<int> dj_type, dj_kind, dj_a, dj_b, dj_sys;
+	DArr<int> dj_type, dj_kind, dj_a, dj_b, dj_sys, dj_proj;
 	DArr<double> dj_tp1, dj_tp2, dj_ax1, dj_ax2, dj_ps, dj_rs, dj_imp, dj_pimp, dj_himp, dj_u, dj_w, dj_row, lin;
 	DArr<uint64_t> nocol;
 	int n_nocol = 0;
@@ -235,6 +235,7 @@ struct cp_world {
 	struct JEdge {
 		int n1, n2;
 		std::vector<int> joints;  // creation order
+		int nhol = 0;             // holonomic joints among them (hasHolonomicConstraint)
 		bool scan = true;         // holonomic_system_scan_needed
 		int sys = -1;
 	};
@@ -413,6 +414,7 @@ struct cp_world {
 		J.type = dj_type.p; J.kind = dj_kind.p; J.a = dj_a.p; J.b = dj_b.p; J.sys = dj_sys.p;
 		J.tp1 = dj_tp1.p; J.tp2 = dj_tp2.p; J.ax1 = dj_ax1.p; J.ax2 = dj_ax2.p; J.ps = dj_ps.p; J.rs = dj_rs.p;
 		J.imp = dj_imp.p; J.pimp = dj_pimp.p; J.himp = dj_himp.p; J.u = dj_u.p; J.w = dj_w.p; J.row = dj_row.p;
+		J.proj = dj_proj.p;
 		return J;
 	}
 	HSysDev hsys() {
@@ -488,7 +490,7 @@ void cp_world::free_all() {
 	d_nc.free_();
 	nf_body.free_();
 	nf_any.free_();
-	DArr<int>* jints[] = {&dj_type, &dj_kind, &dj_a, &dj_b, &dj_sys, &dh_nc, &dh_ord_off, &dh_order, &dh_vloc, &dh_tab_off, &dh_table, &dh_degree,
+	DArr<int>* jints[] = {&dj_type, &dj_kind, &dj_a, &dj_b, &dj_sys, &dj_proj, &dh_nc, &dh_ord_off, &dh_order, &dh_vloc, &dh_tab_off, &dh_table, &dh_degree,
 	                      &dh_buf_off, &dh_dbuf_off, &isl_sys, &isl_jlin, &isl_eoff, &isl_coff, &isl_icoff, &isl_ncol,
 	                      &isl_serial, &isl_bod, &isl_nbod, &isl_cloc, &isl_jloc, &isl_sys_cnt, &dh_cp_off, &dh_cs_ptr,
 	                      &dh_cs_row, &dh_cs_loc, &dh_dg_loc, &isl_root, &isl_edge_off, &isl_body_off, &isl_body_cnt, &isl_stack_off,
@@ -617,7 +619,7 @@ void cp_world::upload() {
 // ---------------------------------------------------------------- joints
 // Joint state per joint: CP_JMAX accumulated impulses (unused components
 // 0), then the basis u, w of the current rows (hinges, sliders, sliding
-// hinges).
+// hinges); a limit keeps is_inactive and the twist's tracked angle in u.
 void cp_world::read_joint_state(double* s, int n) {
 	int K = j_dev_n;
 	std::vector<double> imp, u, w;
@@ -636,6 +638,7 @@ void cp_world::read_joint_state(double* s, int n) {
 void cp_world::write_joint_state(const double* s) {
 	int K = j_dev_n;
 	std::vector<double> imp(CP_JMAX * (size_t)K), u(3 * (size_t)K), w(3 * (size_t)K), zero(CP_JMAX * (size_t)K, 0.0);
+	std::vector<int> proj(K, 0);
 	for (int k = 0; k < K; k++) {
 		const double* r = s + CP_JSTATE * (size_t)k;
 		for (int c = 0; c < CP_JMAX; c++) imp[c * (size_t)K + k] = r[c];
@@ -643,8 +646,12 @@ void cp_world::write_joint_state(const double* s) {
 			u[c * (size_t)K + k] = r[CP_JMAX + c];
 			w[c * (size_t)K + k] = r[CP_JMAX + 3 + c];
 		}
+		// a limit projects to >= 0, an active twist limit built beyond its
+		// maximum to <= 0 (TwistLimitConstraint::lim_status)
+		if (joint_is_limit(j_kind[k]))
+			proj[k] = j_kind[k] == 9 && r[CP_JMAX] == 0.0 && r[CP_JMAX + 1] > j_geo[14 * (size_t)k + 1] ? 2 : 1;
 	}
-	up(dj_imp, imp); up(dj_u, u); up(dj_w, w); up(dj_pimp, zero); up(dj_himp, zero);
+	up(dj_imp, imp); up(dj_u, u); up(dj_w, w); up(dj_pimp, zero); up(dj_himp, zero); up(dj_proj, proj);
 	hs_pending = true;  // rows and factors are rebuilt from this state at the next step
 }
 
@@ -654,6 +661,8 @@ void cp_world::upload_joints() {
 	int K = (int)j_type.size();
 	std::vector<double> st(CP_JSTATE * (size_t)K, 0.0);
 	if (j_dev_n > 0) read_joint_state(st.data(), j_dev_n);
+	for (int k = std::max(j_dev_n, 0); k < K; k++)
+		if (joint_is_limit(j_kind[k])) st[CP_JSTATE * (size_t)k + CP_JMAX] = 1.0;  // limits start inactive
 	std::vector<int> sys(K, -1);
 	for (size_t q = 0; q < hs_og.size(); q++)
 		for (int j : hs_og[q]) sys[j] = (int)q;
@@ -885,9 +894,11 @@ void cp_world::discover_systems() {
 				const JEdge& v = jedges[k];
 				std::vector<uint64_t> nbrs;
 				if (b_move[v.n1] == 0)
-					for (auto& x : adj[v.n1]) nbrs.push_back(x.second);
+					for (auto& x : adj[v.n1])
+						if (jedges[x.second].nhol) nbrs.push_back(x.second);
 				if (b_move[v.n2] == 0)
-					for (auto& x : adj[v.n2]) nbrs.push_back(x.second);
+					for (auto& x : adj[v.n2])
+						if (jedges[x.second].nhol) nbrs.push_back(x.second);
 				for (uint64_t x : nbrs)
 					if (!seen(x)) st.push_back(x);
 			}
@@ -930,7 +941,8 @@ void cp_world::discover_systems() {
 	for (auto& edges : sys_edges) {
 		hs_og.emplace_back();
 		for (uint64_t k : edges)
-			for (int j : jedges[k].joints) hs_og.back().push_back(j);
+			for (int j : jedges[k].joints)
+				if (!joint_is_limit(j_kind[j])) hs_og.back().push_back(j);  // revaluateSystem (:1785-1798)
 	}
 	layout_systems();
 }
@@ -1305,8 +1317,8 @@ void cp_world::schedule_islands(bool colored) {
 	launch_ib_heads(ib_keys2.p, N, ib_head.p, stream);
 	scan(ib_head.p, ib_midx.p, N + 1);
 	if (N) {
-		launch_ib_edges(ib_keys2.p, ib_vals2.p, N, ib_head.p, ib_midx.p, ne, e_first.p, e_cnt.p, cb[cur].c.live, ib_jgoff.p, ib_me.p,
-		                ib_adjk.p, ib_adjv.p, stream);
+		launch_ib_edges(ib_keys2.p, ib_vals2.p, N, ib_head.p, ib_midx.p, ne, e_first.p, e_cnt.p, cb[cur].c.live, ib_jgoff.p, ib_jord.p,
+		                dj_kind.p, dj_u.p, ib_me.p, ib_adjk.p, ib_adjv.p, stream);
 		bytes = 0;
 		HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, ib_adjk.p, ib_adjk2.p, ib_adjv.p, ib_adjv2.p, 2 * N, 0, 64, stream));
 		tmp.reserve(bytes + 16);
@@ -1334,6 +1346,7 @@ void cp_world::schedule_islands(bool colored) {
 	W.hol = cfg.hol_iters > 0 && !hs_lay.empty();
 	W.movement = movement.p; W.asleep = S.enabled ? S.asleep : nullptr; W.aoff = ib_aoff.p; W.adjv = ib_adjv2.p; W.me = ib_me.p;
 	W.jord = ib_jord.p; W.live = cb[cur].c.live; W.ja = dj_a.p; W.jb = dj_b.p; W.jsys = dj_sys.p; W.adjk = ib_adjk2.p;
+	W.jkind = dj_kind.p; W.ju = dj_u.p;
 	W.ckey = cb[cur].c.key; W.stack = ib_stack.p; W.vlist = ib_vlist.p; W.elist = ib_elist.p; W.vis = ib_vis.p; W.rows = isl_rows.p;
 	W.ecol = ib_ecol.p; W.coff = isl_coff.p; W.eorder = ib_eorder.p; W.icoff = isl_icoff.p; W.ncol = isl_ncol.p; W.serial = isl_serial.p;
 	W.eoff = isl_eoff.p; W.bod = isl_bod.p; W.nbod = isl_nbod.p; W.loc = ib_loc.p; W.cloc = isl_cloc.p; W.jloc = isl_jloc.p;
@@ -2445,8 +2458,8 @@ int cp_world_set_node_order(cp_world* w, int n, const int64_t* ids) {
 }
 
 int cp_joint_add(cp_world* w, int type, int b1, int b2, const double* geo14, int* joint_index) {
-	static const int degree[8] = {0, 1, 0, 3, 4, 5, 6, 5};  // rows per kind (ConstraintSolver.h:193-398)
-	if (!w || !geo14 || !joint_index || type < 1 || type > 7 || degree[type] == 0) return CP_ERR_INVALID;
+	static const int degree[10] = {0, 1, 0, 3, 4, 5, 6, 5, 1, 1};  // rows per kind (ConstraintSolver.h:193-398)
+	if (!w || !geo14 || !joint_index || type < 1 || type > 9 || degree[type] == 0) return CP_ERR_INVALID;
 	if (b1 < 0 || b2 < 0 || b1 >= w->nb || b2 >= w->nb || b1 == b2 || w->b_scene[b1] != w->b_scene[b2]) return CP_ERR_INVALID;
 	for (int c = 0; c < 14; c++)
 		if (!std::isfinite(geo14[c])) return CP_ERR_INVALID;
@@ -2457,20 +2470,28 @@ int cp_joint_add(cp_world* w, int type, int b1, int b2, const double* geo14, int
 	w->j_b.push_back(b2);
 	w->j_geo.insert(w->j_geo.end(), geo14, geo14 + 14);
 	uint64_t key = (uint64_t)std::min(b1, b2) << 32 | (uint64_t)std::max(b1, b2);
+	// limits are not holonomic: they join the pair's edge and leave the
+	// holonomic systems as they are (PhysicsEngine.cpp:926-970)
+	const bool holonomic = !joint_is_limit(type);
 	auto it = w->jedges.find(key);
 	if (it == w->jedges.end()) {
 		cp_world::JEdge e;
 		e.n1 = b1;
 		e.n2 = b2;
 		e.joints.push_back(k);
+		e.nhol = holonomic;
+		e.scan = holonomic;
 		w->jedges.emplace(key, e);
 	} else {
 		it->second.joints.push_back(k);
-		if (it->second.sys >= 0) w->hs_changed = true;  // constraints_changed_flag (PhysicsEngine.cpp:679-681)
-		else it->second.scan = true;
+		it->second.nhol += holonomic;
+		if (holonomic) {
+			if (it->second.sys >= 0) w->hs_changed = true;  // constraints_changed_flag (PhysicsEngine.cpp:679-681)
+			else it->second.scan = true;
+		}
 	}
 	w->joints_dirty = true;
-	w->hs_scan = true;
+	if (holonomic) w->hs_scan = true;
 	*joint_index = k;
 	return CP_OK;
 }
@@ -2525,6 +2546,7 @@ int cp_world_set_holonomic_systems(cp_world* w, int n_sys, const int64_t* off, c
 		if (off[q] > off[q + 1] || (q == 0 && off[0] != 0)) return CP_ERR_INVALID;
 		for (int64_t i = off[q]; i < off[q + 1]; i++) {
 			if (joints[i] < 0 || joints[i] >= K || used[joints[i]]) return CP_ERR_INVALID;
+			if (joint_is_limit(w->j_kind[joints[i]])) return CP_ERR_INVALID;  // limits are not holonomic
 			used[joints[i]] = 1;
 			og[q].push_back((int)joints[i]);
 		}

@@ -233,7 +233,9 @@ typedef enum {
 	CP_JOINT_SLIDING_HINGE = 4, /* addSlidingHingeConstraint, :860-890; 4 rows */
 	CP_JOINT_HINGE = 5,         /* addHingeConstraint, :693-725; 5 rows */
 	CP_JOINT_WELD = 6,          /* addWeldConstraint, :896-920; 6 rows */
-	CP_JOINT_SLIDER = 7         /* addSliderConstraint, :773-806; 5 rows */
+	CP_JOINT_SLIDER = 7,        /* addSliderConstraint, :773-806; 5 rows */
+	CP_JOINT_CONE_LIMIT = 8,    /* addConeLimitConstraint, :926-945; 1 projected row while active */
+	CP_JOINT_TWIST_LIMIT = 9    /* addTwistLimitConstraint, :952-970; 1 projected row while active */
 } cp_joint_kind;
 
 /* Joints between bodies b1 and b2 of one scene; the pair stops colliding
@@ -245,17 +247,27 @@ typedef enum {
  * SlidingHinge ::b1/b2_slide_axis_body_space); for a distance joint the
  * first slot holds the target distance (Distance::target_distance, the
  * current distance when the reference is given -1); 0 otherwise - and the
- * position and rotation correction strengths.  Joints are solved in the
- * island modes; a step of a world with joints in CP_PGS_COLORED mode returns
- * CP_ERR_STATE.  Every kind is a holonomic constraint (isHolonomicConstraint)
- * and joins holonomic systems. */
+ * position and rotation correction strengths.  The limits keep their angles
+ * in the first attach-point slots (cone: the maximum angle; twist: the
+ * minimum and maximum angles), the body-frame directions in the axis slots,
+ * 0 as position strength and the rotation hardness as rotation strength.
+ * Joints are solved in the island modes; a step of a world with joints in
+ * CP_PGS_COLORED mode returns CP_ERR_STATE.  Kinds 1-7 are holonomic
+ * constraints (isHolonomicConstraint) and join holonomic systems; the limits
+ * are inequality rows outside them: each step a limit first re-evaluates its
+ * angle (ConeRotationLimit / TwistLimit::updateSolverConstraints,
+ * PersistentConstraint.cpp:233-272) and, inside its range, clears its
+ * impulse; an island holds a limit's row when the limit was active at its
+ * previous evaluation. */
 int cp_joint_add(cp_world* w, int type, int b1, int b2, const double* geo14, int* joint_index);
 int cp_world_joint_count(cp_world* w, int* n);
 /* Joint solver state, 12 doubles per joint: accumulated impulse (6, unused
  * components 0), basis u and w of the current rows of hinges, sliders and
  * sliding hinges (0 for the others) (the PersistentConstraint ::constraint
- * objects, ConstraintSolver.h).  Setting it marks the holonomic systems for a
- * rebuild of their rows and factors at the next step. */
+ * objects, ConstraintSolver.h); a limit holds is_inactive (1 or 0) and, for a
+ * twist limit, its tracked current_angle where u is (a new limit: 1, 0).
+ * Setting it marks the holonomic systems for a rebuild of their rows and
+ * factors at the next step. */
 int cp_world_set_joint_state(cp_world* w, const double* state12);
 int cp_world_get_joint_state(cp_world* w, double* state12);
 /* Rows of joint k built by the last step: J_a, J_b [n][6], M^-1 J^T of each
