@@ -1613,8 +1613,9 @@ struct Contact {
 static double posCorrectCoeff(double s, double dt) { return std::min<double>(s * dt, 1.0 / dt); }  // PersistentConstraint.cpp:199
 
 struct Joint {
-	int kind;       // 1 distance, 3 ball-socket, 4 sliding hinge, 5 hinge, 6 weld, 7 slider, 8 cone limit, 9 twist limit
-	int type;       // degree (rows): 1, 3, 4, 5, 6, 5, 1, 1
+	int kind;       // 1 distance, 3 ball-socket, 4 sliding hinge, 5 hinge, 6 weld, 7 slider, 8 cone limit, 9 twist limit,
+	                // 10 motor, 11 piston drive, 12 piston slide limit
+	int type;       // degree (rows): 1, 3, 4, 5, 6, 5, 1, 1, 1, 1, 1
 	int a, b;       // body indices (b1, b2 as created)
 	V3 tp1, tp2;    // tracked points in the body frames (RigidBody.cpp:134-144)
 	V3 ax1, ax2;    // hinge / slide axes in the body frames; distance: ax1.x = target distance
@@ -1631,7 +1632,23 @@ struct Joint {
 	bool inactive = true;
 	double cur_angle = 0;
 	int above = 0;  // TwistLimitConstraint::lim_status == ABOVE_MAX
-	bool active() const { return kind < 8 || !inactive; }
+	// motor (10), piston drive (11) and its slide limit (12): Motor / Piston
+	// (PersistentConstraint.h:218-290).  mode: 0 OFF, 1 constant torque or
+	// force, 2 target velocity, 3 target position (PoweredConstraitMode)
+	int mode = 0;
+	double dmax = 0, dvel = 0, dpos = 0;  // max torque / force, target velocity, target position
+	double jpos = 0;      // motor_angular_position / piston_position
+	double prev_vel = 0;  // prev_velocity (the piston never writes it)
+	int lstat = 0;        // limit status of the last build: 0 none, 1 below the minimum, 2 above the maximum
+	double bound = 0;     // max_torque_impulse / max_impulse of the last build
+	double tv = 0;        // the target velocity of the last build (real_target_velocity)
+	bool active() const {
+		if (kind == 10) return (mode != 0 && dmax != 0) || jpos < tp1.x || jpos > tp1.y;  // Motor::constraintIsActive (:110-112)
+		if (kind == 11) return mode != 0 && dmax != 0;                                    // Piston::pistonIsActive (:195-197)
+		return kind < 8 || !inactive;  // limits while active; slide limit while exceeded
+	}
+	// needsPosCorrect (ConstraintSolver.h:309, :350)
+	bool pos_correct() const { return !(kind == 11 || (kind == 10 && lstat == 0)); }
 };
 
 struct HSys {
@@ -1843,6 +1860,78 @@ static double limitError(double f) {
 	return -f * f * f * f * f * f * f * f / 40320.0 + f * f * f * f * f * f / 720.0 - f * f * f * f / 24.0 + f * f / 2.0;
 }
 
+// Motor::calculatePosition (PersistentConstraint.cpp:30-71): the angle
+// predicted from the relative angular velocity, snapped to the measured angle
+// of b2's comparison axis in b1's reference plane (the candidate nearest the
+// prediction among the measured angle shifted by -2PI, 0, +2PI).
+static double motorPosition(const Joint& j, const Body& A, const Body& B, V3 axis, V3 av1, V3 av2, double dt) {
+	V3 u1, w1, u2, tmp;
+	j.ax1.perpBasis(&u1, &w1);  // Motor ctor (:8-13)
+	j.ax2.perpBasis(&u2, &tmp);
+	V3 ru = A.q.rotate(u1), rw = A.q.rotate(w1), cmp = B.q.rotate(u2);
+	double rel = axis.dot(av1) - axis.dot(av2);
+	double pred = j.jpos + rel * dt;
+	if (std::abs(cmp.dot(axis)) > 0.999) return pred;
+	V3 cn = (cmp - axis * axis.dot(cmp)).normalize();
+	double dv = cn.dot(ru);
+	if (dv > 1.0) dv = 1.0;
+	if (dv < -1.0) dv = -1.0;
+	double relang = 2 * QUAT_PI - std::acos(dv);
+	if (cn.dot(rw) < 0) relang = 2 * QUAT_PI - relang;
+	double r = std::fmod(pred, 2 * QUAT_PI);
+	double n2 = pred - r;
+	const double cand[3] = {n2 + relang - 2 * QUAT_PI, n2 + relang, n2 + relang + 2 * QUAT_PI};
+	double closest = std::numeric_limits<double>::infinity();
+	for (double c : cand)
+		if (std::abs(pred - c) < std::abs(pred - closest)) closest = c;
+	return closest;
+}
+
+// getConstraintTargetVelocityValue of the motor (:77-108) and the piston
+// (:153-185).  vrel: the relative velocity along the axis as each measures it;
+// inv_sum: the inverse inertia (motor) or inverse mass (piston) sum along it.
+// The reference tests abs(impulse < EPS), i.e. impulse < EPS.
+static double driveTarget(const Joint& j, double vrel, double inv_sum, double imp, double pos) {
+	switch (j.mode) {
+	case 0: return 0;
+	case 1: return (j.dmax > 0) ? std::numeric_limits<double>::infinity() : -std::numeric_limits<double>::infinity();
+	case 2: return j.dvel;
+	default: {
+		double diff = pos - j.dpos;
+		double dir = (diff > 0) ? -1 : 1;
+		double vd = std::abs(vrel - j.prev_vel);
+		const double EPS = 0.0000001;
+		double inertia = (vd < EPS || imp < EPS) ? 1.0 / inv_sum : std::abs(imp) / vd;
+		return dir * std::sqrt(2 * std::abs(diff) * j.dmax / inertia);
+	}
+	}
+}
+
+// Piston position b2_pos . axis - b1_pos . axis (Piston::calculatePosition, :149-151).
+static double pistonPosition(V3 p1, V3 p2, V3 axis) { return p2.dot(axis) - p1.dot(axis); }
+
+// A joint's state as the reference's constructors leave it, from the current
+// bodies: limits inactive, a motor's angular position measured as the Motor
+// constructor does (:17-22), a piston's position from its two attach points.
+// The Piston constructor (:139-141) looks b2's point key up in b1's tracked
+// points; the value is replaced at the first update and is seen only by a
+// position query before the first step.
+static void jointInitState(World& W, Joint& j) {
+	for (int k = 0; k < 6; k++) j.imp[k] = j.pimp[k] = j.himp[k] = 0;
+	j.u = V3();
+	j.w = V3();
+	j.inactive = true;
+	j.cur_angle = 0;
+	j.above = 0;
+	j.lstat = 0;
+	j.prev_vel = 0;
+	j.jpos = 0;
+	const Body& A = W.bodies[j.a];
+	const Body& B = W.bodies[j.b];
+	if (j.kind == 10) j.jpos = motorPosition(j, A, B, A.q.rotate(j.ax1), V3(), V3(), 0);
+	if (j.kind == 11 || j.kind == 12) j.jpos = pistonPosition(A.com + A.q.rotate(j.tp1), B.com + B.q.rotate(j.tp2), A.q.rotate(j.ax1));
+}
+
 static void buildJoint(World& W, Joint& j, double sub_dt, bool reproject) {
 	const Body& A = W.bodies[j.a];
 	const Body& B = W.bodies[j.b];
@@ -1898,11 +1987,73 @@ static void buildJoint(World& W, Joint& j, double sub_dt, bool reproject) {
 		putRow(jb, 0, 3, rowNM(t2));
 		double f = j.above ? cur - mx : mn - cur;
 		j.ptgt[0] = limitError(f) * j.rs;
+	} else if (j.kind == 10) {
+		// Motor::updateSolverConstraints (PersistentConstraint.cpp:320-329) ->
+		// MotorConstraint (ConstraintSolver.cpp:555-584)
+		V3 axis = A.q.rotate(j.ax1);
+		V3 av1 = A.getAngVel(), av2 = B.getAngVel();
+		j.jpos = motorPosition(j, A, B, axis, av1, av2, sub_dt);
+		if (j.active()) {
+			double inv_sum = axis.dot(A.invTensor() * axis) + axis.dot(B.invTensor() * axis);
+			double target = 1.0 * driveTarget(j, axis.dot(av1) - axis.dot(av2), inv_sum, j.imp[0], j.jpos);
+			j.bound = std::abs(j.dmax * sub_dt);
+			double mn = j.tp1.x, mx = j.tp1.y;
+			if (j.jpos > mx) {
+				j.ptgt[0] = (mx - j.jpos) * rcc;
+				j.tv = std::min<double>(0, target);
+				j.lstat = 2;
+			} else if (j.jpos < mn) {
+				j.ptgt[0] = (mn - j.jpos) * rcc;
+				j.tv = std::max<double>(0, target);
+				j.lstat = 1;
+			} else {
+				j.ptgt[0] = 0;
+				j.tv = target;
+				j.lstat = 0;
+			}
+			NM<1, 3> ad = rowNM(axis);
+			putRow(ja, 0, 3, ad);
+			putRow(jb, 0, 3, neg(ad));
+			j.prev_vel = axis.dot(av1) - axis.dot(av2);  // writePrevVel, after the build
+		} else {
+			j.prev_vel = axis.dot(av1) - axis.dot(av2);
+			return;
+		}
+	} else if (j.kind == 11 || j.kind == 12) {
+		// Piston::updateSolverConstraints (PersistentConstraint.cpp:300-318)
+		V3 axis = A.q.rotate(j.ax1);
+		j.jpos = pistonPosition(pa, pb, axis);
+		if (j.kind == 12) {
+			double mn = j.ax2.x, mx = j.ax2.y;
+			j.inactive = !(j.jpos < mn || j.jpos > mx);  // slideLimitExceeded (:191-193)
+			if (j.inactive) return;
+			// SlideLimitConstraint (ConstraintSolver.cpp:736-766)
+			if (j.jpos > mx) {
+				j.ptgt[0] = (mx - j.jpos) * pcc;
+				j.lstat = 2;
+			} else {
+				j.ptgt[0] = (mn - j.jpos) * pcc;
+				j.lstat = 1;
+			}
+		} else {
+			if (!j.active()) return;
+			// PistonConstraint (ConstraintSolver.cpp:701-721)
+			double inv_sum = A.invMass() + B.invMass();
+			j.tv = driveTarget(j, axis.dot(A.getVel()) - axis.dot(B.getVel()), inv_sum, j.imp[0], j.jpos);
+			j.bound = j.dmax * sub_dt;
+		}
+		NM<1, 3> sd = rowNM(axis);
+		putRow(ja, 0, 0, neg(sd));
+		putRow(ja, 0, 3, mul(neg(sd), crossNM(axis)));
+		putRow(jb, 0, 0, sd);
 	}
 	for (int k = 0; k < n; k++) j.imp[k] = start[k];
 	switch (j.kind) {
 	case 8:
 	case 9:
+	case 10:
+	case 11:
+	case 12:
 		break;
 	case 1: {  // DistanceConstraint (ConstraintSolver.cpp:348-379), not in moving mode
 		V3 rA = pa - A.com, rB = pb - B.com;
@@ -2320,7 +2471,9 @@ static void pgsIsland(World& W, const std::vector<std::pair<Contact*, int>>& row
 			bodyVel(j.a, va);
 			bodyVel(j.b, vb);
 			jointValue(j, va, vb, val);
-			for (int k = 0; k < j.type; k++) j.tgt[k] = -val[k];
+			if (j.kind == 10 || j.kind == 11) j.tgt[0] = j.tv - val[0];  // Motor / PistonConstraint::findTargetConstraintValue
+			else
+				for (int k = 0; k < j.type; k++) j.tgt[k] = -val[k];
 			continue;
 		}
 		const Body& A = W.bodies[c->b1];
@@ -2376,9 +2529,12 @@ static void pgsIsland(World& W, const std::vector<std::pair<Contact*, int>>& row
 			for (int k = 0; k < n; k++) m[i] += delta[k] * j.inv.v[i][k];
 		}
 		for (int k = 0; k < n; k++) diff[k] = (acc[k] + m[k]) - acc[k];
-		if (j.kind >= 8) {  // projectValidImpulse of the limits (ConstraintSolver.cpp:432-435, :461-463)
-			double nw = acc[0] + m[0];
-			double pr = (j.kind == 9 && j.above) ? std::min<double>(nw, 0) : std::max<double>(nw, 0);
+		if (j.kind >= 8) {  // projectValidImpulse (ConstraintSolver.cpp:432-435, :461-463, :591-605, :728-731, :768-775)
+			double nw = acc[0] + m[0], pr;
+			bool clamp = j.kind == 11 || (j.kind == 10 && j.lstat == 0);
+			bool upper = (j.kind == 9 && j.above) || ((j.kind == 10 || j.kind == 12) && j.lstat == 2);
+			if (clamp) pr = nw > 0 ? std::min<double>(nw, j.bound) : std::max<double>(nw, -j.bound);
+			else pr = upper ? std::min<double>(nw, 0) : std::max<double>(nw, 0);
 			diff[0] = pr - acc[0];
 		}
 		if (!isZero(diff, n)) {
@@ -2443,7 +2599,7 @@ static void pgsIsland(World& W, const std::vector<std::pair<Contact*, int>>& row
 		for (auto& r : rows) step(r.first, r.second, false);
 	for (int it = 0; it < W.cfg.pos_iters; it++)
 		for (auto& r : rows)
-			if (!r.first || r.second == 0) step(r.first, r.second, true);
+			if (r.first ? r.second == 0 : W.joints[r.second].pos_correct()) step(r.first, r.second, true);
 	// block solve of the island's holonomic systems (ConstraintSolver.cpp:163-196)
 	if (hol_iters > 0 && !systems.empty()) {
 		auto apply = [&](HSys& hs, bool pseudo, double thr) {  // computeAndApplyImpulses (:298-355)
@@ -3131,8 +3287,8 @@ int orc_add_cylinder_body(orc_world* w, int movement, const double* mp, const do
 
 int orc_add_joint(orc_world* w, int type, int b1, int b2, const double* g) {
 	Joint j;
-	static const int degree[10] = {0, 1, 0, 3, 4, 5, 6, 5, 1, 1};
-	if (type < 1 || type > 9 || degree[type] == 0) return -1;
+	static const int degree[13] = {0, 1, 0, 3, 4, 5, 6, 5, 1, 1, 1, 1, 1};
+	if (type < 1 || type > 12 || degree[type] == 0) return -1;
 	j.kind = type;
 	j.type = degree[type];
 	j.a = b1;
@@ -3214,12 +3370,26 @@ void orc_set_joint_state(orc_world* w, const double* s) {
 		Joint& j = w->joints[i];
 		const double* r = s + 12 * i;
 		for (int k = 0; k < 6; k++) j.imp[k] = r[k];
-		if (j.kind >= 8) {
+		if (j.kind == 8 || j.kind == 9) {
 			// limits: is_inactive, the twist's tracked angle; an active twist
 			// limit was built at that angle (lim_status)
 			j.inactive = r[6] != 0;
 			j.cur_angle = j.kind == 9 ? r[7] : 0.0;
 			j.above = j.kind == 9 && !j.inactive && j.cur_angle > j.tp1.y;
+			continue;
+		}
+		if (j.kind == 10 || j.kind == 11) {
+			// motor / piston drive: position and prev_velocity; a motor row
+			// built at that position has its limit status
+			j.jpos = r[6];
+			j.prev_vel = r[7];
+			j.lstat = j.kind == 10 ? (j.jpos > j.tp1.y ? 2 : (j.jpos < j.tp1.x ? 1 : 0)) : 0;
+			continue;
+		}
+		if (j.kind == 12) {  // slide limit: not exceeded, position
+			j.inactive = r[6] != 0;
+			j.jpos = r[7];
+			j.lstat = j.jpos > j.ax2.y ? 2 : 1;
 			continue;
 		}
 		j.u = V3(r[6], r[7], r[8]);
@@ -3234,14 +3404,39 @@ void orc_get_joint_state(const orc_world* w, double* s) {
 		double* r = s + 12 * i;
 		for (int k = 0; k < 6; k++) r[k] = k < j.type ? j.imp[k] : 0.0;
 		if (j.kind >= 8) {
-			r[6] = j.inactive ? 1.0 : 0.0;
-			r[7] = j.cur_angle;
-			for (int k = 8; k < 12; k++) r[k] = 0.0;
+			for (int k = 6; k < 12; k++) r[k] = 0.0;
+			if (j.kind == 8 || j.kind == 9 || j.kind == 12) r[6] = j.inactive ? 1.0 : 0.0;
+			if (j.kind == 9) r[7] = j.cur_angle;
+			if (j.kind == 10 || j.kind == 11) {
+				r[6] = j.jpos;
+				r[7] = j.prev_vel;
+			}
+			if (j.kind == 12) r[7] = j.jpos;
 			continue;
 		}
 		r[6] = j.u.x; r[7] = j.u.y; r[8] = j.u.z;
 		r[9] = j.w.x; r[10] = j.w.y; r[11] = j.w.z;
 	}
+}
+
+void orc_init_joint_state(orc_world* w) {
+	for (Joint& j : w->joints) jointInitState(*w, j);
+	for (HSys& hs : w->systems) hs.pending = true;
+}
+
+int orc_set_drive(orc_world* w, int k, int mode, double max, double target) {
+	if (k < 0 || k >= (int)w->joints.size() || mode < 0 || mode > 3) return -1;
+	Joint& j = w->joints[k];
+	if (j.kind != 10 && j.kind != 11) return -1;
+	// setMotor* / setPiston* (PhysicsEngine.cpp:1683-1912): OFF keeps the
+	// parameters; the others set the maximum and their target
+	j.mode = mode;
+	if (mode >= 1) j.dmax = max;
+	if (mode == 2) j.dvel = target;
+	if (mode == 3) j.dpos = target;
+	const int64_t ends[2] = {j.a, j.b};  // alertWakingAction on both bodies
+	orc_wake_bodies(w, 2, ends);
+	return 0;
 }
 
 void orc_set_hsys(orc_world* w, int n_sys, const int64_t* off, const int64_t* og) {

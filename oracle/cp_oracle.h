@@ -82,6 +82,12 @@ int orc_narrowphase_multi(const orc_world* w, int id_lo, int id_hi, int cap, dou
 // strengths.  The pair stops colliding.  Returns the joint index.
 int orc_add_joint(orc_world* w, int type, int b1, int b2, const double* geo14);
 // Joint solver state rows of 11 doubles: impulse (5, zero padded), hinge u, w.
+/* Joint state as the reference's constructors leave it, from the current bodies. */
+void orc_init_joint_state(orc_world* w);
+/* Powered mode of a motor (10) or piston drive (11): 0 off, 1 constant
+ * torque / force (max), 2 target velocity (max, target), 3 target position
+ * (max, target).  Returns -1 for another kind. */
+int orc_set_drive(orc_world* w, int k, int mode, double max, double target);
 void orc_set_joint_state(orc_world* w, const double* s11);
 void orc_get_joint_state(const orc_world* w, double* s11);
 // Holonomic systems: n_sys systems, system k holds joints og[off[k]..off[k+1])

@@ -221,12 +221,15 @@ static void build_mixed(PhysicsEngine& E, int nx, int ny, int nz, uint64_t seed,
 // 0.1; chain c at z = 1.2 c, height U[0.4, 1.2], x offset U[-0.5, 0.5]; every link
 // gets v ~ U[-0.5, 0.5]^3 so the chain bends as it falls onto the ground box.
 // Joint kinds as the dumps number them: 1 distance, 3 ball-socket, 4 sliding
-// hinge, 5 hinge, 6 weld, 7 slider, 8 cone limit, 9 twist limit.
+// hinge, 5 hinge, 6 weld, 7 slider, 8 cone limit, 9 twist limit, 10 motor,
+// 11 piston drive, 12 piston slide limit (a piston is recorded as its slide
+// limit, then its drive: the order Piston::addSolverConstraints adds them).
 struct JointRec {
 	int type;
 	RigidBody *b1, *b2;
 	phyz::PersistentConstraint* pc;
 	phyz::Constraint* solver;
+	mthz::Vec3 aux = mthz::Vec3();  // a motor: b2's rotation axis of its base hinge (the motor keeps only what it yields)
 };
 static std::vector<JointRec> g_joints;
 static RigidBody* g_mesh = nullptr;  // the terrain body of the terrain scene
@@ -302,6 +305,24 @@ static void joint_state(std::vector<double>& s) {
 			auto* t = static_cast<phyz::TwistLimit*>(j.pc);
 			put_impulse(s, t->constraint, 1);
 			s.insert(s.end(), {t->is_inactive ? 1.0 : 0.0, t->current_angle, 0, 0, 0, 0});
+			break;
+		}
+		case 10: {  // impulse, then motor_angular_position and prev_velocity
+			auto* m = static_cast<phyz::Motor*>(j.pc);
+			put_impulse(s, m->motor_constraint, 1);
+			s.insert(s.end(), {m->motor_angular_position, m->prev_velocity, 0, 0, 0, 0});
+			break;
+		}
+		case 11: {  // impulse, then piston_position and prev_velocity
+			auto* p = static_cast<phyz::Piston*>(j.pc);
+			put_impulse(s, p->piston_constraint, 1);
+			s.insert(s.end(), {p->piston_position, p->prev_velocity, 0, 0, 0, 0});
+			break;
+		}
+		case 12: {  // impulse, then !slide_limit_exceeded and piston_position
+			auto* p = static_cast<phyz::Piston*>(j.pc);
+			put_impulse(s, p->slide_limit, 1);
+			s.insert(s.end(), {p->slide_limit_exceeded ? 0.0 : 1.0, p->piston_position, 0, 0, 0, 0});
 			break;
 		}
 		default: put_joint_basis(s, static_cast<phyz::Slider*>(j.pc)->constraint, 5); break;
@@ -458,6 +479,7 @@ static StepDump* g_cur = nullptr;
 static bool g_lite = false;  // omit the mid/post contact caches (large scenes)
 static bool g_sleep = false;  // record every body's sleep state after every step
 static int g_wake_step = -1, g_wake_body = -1;  // RigidBody::alertWakingAction on that body before that step
+static std::function<void(PhysicsEngine&, int)> g_pre_step;  // scene actions before step k (powered-mode changes)
 static PhysicsEngine* g_eng = nullptr;
 
 static void on_collision(RigidBody* b1, RigidBody* b2, const std::vector<phyz::Manifold>& ms) {
@@ -642,6 +664,22 @@ static void dump_static(PhysicsEngine& E, Out& o) {
 			jg.insert(jg.end(), {t->min_angle, t->max_angle, 0, 0, 0, 0, a1.x, a1.y, a1.z, a2.x, a2.y, a2.z, 0, t->rot_correct_hardness});
 			break;
 		}
+		case 10: {  // tracked-point slots: the motor's min and max angles
+			auto* m = static_cast<phyz::Motor*>(j.pc);
+			mthz::Vec3 a1 = m->b1_rot_axis_body_space, a2 = j.aux;
+			jg.insert(jg.end(), {m->min_motor_position, m->max_motor_position, 0, 0, 0, 0, a1.x, a1.y, a1.z, a2.x, a2.y, a2.z, 0,
+			                     m->rot_correct_hardness});
+			break;
+		}
+		case 11:
+		case 12: {  // axis slots: b1's slide axis, then the slide limits
+			auto* p = static_cast<phyz::Piston*>(j.pc);
+			mthz::Vec3 t1 = j.b1->track_p[p->b1_point_key], t2 = j.b2->track_p[p->b2_point_key];
+			mthz::Vec3 a1 = p->b1_slide_axis_body_space;
+			jg.insert(jg.end(), {t1.x, t1.y, t1.z, t2.x, t2.y, t2.z, a1.x, a1.y, a1.z, p->min_slide_limit, p->max_slide_limit, 0,
+			                     p->pos_correct_hardness, 0});
+			break;
+		}
 		case 6: {
 			auto* w = static_cast<phyz::Weld*>(j.pc);
 			geo(j, w->b1_point_key, w->b2_point_key, mthz::Vec3(), mthz::Vec3(), w->pos_correct_hardness, w->rot_correct_hardness);
@@ -710,6 +748,7 @@ static void run_trace(PhysicsEngine& E, const SceneCfg& cfg, int steps, const st
 			g_cur = &d;
 		}
 		if (k == g_wake_step) E.bodies[g_wake_body]->alertWakingAction();
+		if (g_pre_step) g_pre_step(E, k);
 		E.timeStep();
 		g_cur = nullptr;
 		if (g_sleep)
@@ -1136,6 +1175,90 @@ static int cmd_limits(int argc, char** argv) {
 	return 0;
 }
 
+// powered: a motor on a hinge (a pendulum arm under a fixed anchor, angle
+// limits -1.0 and 1.2) driven through constant torque, target velocity,
+// target position and off; a motor on a sliding hinge at a target velocity
+// throughout (no limits); a piston on a slider (a block beside a fixed
+// anchor, slide limits -0.4 and 0.9) driven through target velocity, target
+// position, constant force and off.  The powered-mode changes are applied
+// through the engine's setters before their steps and dumped as
+// drive_events rows (step, joint, mode, max, target; mode 0 off, 1 constant
+// torque / force, 2 target velocity, 3 target position).
+static phyz::PersistentConstraint* find_constraint(PhysicsEngine& E, phyz::ConstraintID id) {
+	for (auto* x : E.constraint_map[id.getID()]->constraints)
+		if (x->getId() == id) return x;
+	return nullptr;
+}
+static int cmd_powered(int argc, char** argv) {
+	int steps = (int)arg_d(argc, argv, "--steps", 200);
+	SceneCfg cfg;
+	cfg.hol_iters = (int)arg_d(argc, argv, "--hol", 1);
+	PhysicsEngine E;
+	configure(E, cfg);
+	phyz::Material m = c0_material();
+	RigidBody* g = E.createRigidBody(phyz::ConvexUnionGeometry::box(mthz::Vec3(-100, -1, -100), 200, 1, 200, m), RigidBody::FIXED);
+	pin_body(g);
+	mthz::Vec3 z(0, 0, 1), x(1, 0, 0);
+	// motor on a hinge
+	RigidBody* a1 = E.createRigidBody(phyz::ConvexUnionGeometry::box(mthz::Vec3(-0.5, 6.1, -0.5), 1, 1, 1, m), RigidBody::FIXED);
+	RigidBody* arm = E.createRigidBody(phyz::ConvexUnionGeometry::box(mthz::Vec3(-0.15, 4.6, -0.15), 0.3, 1.4, 0.3, m));
+	pin_body(a1);
+	pin_body(arm);
+	phyz::ConstraintID h1 = E.addHingeConstraint(a1, arm, mthz::Vec3(0, 6, 0), mthz::Vec3(0, 6, 0), z, z);
+	record_joint<phyz::Hinge>(E, h1, 5, a1, arm);
+	phyz::ConstraintID m1 = E.addMotorConstraint(h1, -1.0, 1.2);
+	auto* mp1 = static_cast<phyz::Motor*>(find_constraint(E, m1));
+	g_joints.push_back({10, a1, arm, mp1, &mp1->motor_constraint, static_cast<phyz::Hinge*>(find_constraint(E, h1))->b2_rot_axis_body_space});
+	const int jm1 = (int)g_joints.size() - 1;
+	// motor on a sliding hinge
+	RigidBody* a2 = E.createRigidBody(phyz::ConvexUnionGeometry::box(mthz::Vec3(3.5, 6.1, -0.5), 1, 1, 1, m), RigidBody::FIXED);
+	RigidBody* sw = E.createRigidBody(phyz::ConvexUnionGeometry::box(mthz::Vec3(3.85, 4.6, -0.15), 0.3, 1.4, 0.3, m));
+	pin_body(a2);
+	pin_body(sw);
+	phyz::ConstraintID h2 = E.addSlidingHingeConstraint(a2, sw, mthz::Vec3(4, 6, 0), z);
+	record_joint<phyz::SlidingHinge>(E, h2, 4, a2, sw);
+	phyz::ConstraintID m2 = E.addMotorConstraint(h2, -1e9, 1e9);
+	auto* mp2 = static_cast<phyz::Motor*>(find_constraint(E, m2));
+	g_joints.push_back({10, a2, sw, mp2, &mp2->motor_constraint, static_cast<phyz::SlidingHinge*>(find_constraint(E, h2))->b2_slide_axis_body_space});
+	const int jm2 = (int)g_joints.size() - 1;
+	// piston on a slider
+	RigidBody* a3 = E.createRigidBody(phyz::ConvexUnionGeometry::box(mthz::Vec3(7.5, 2.5, -0.5), 1, 1, 1, m), RigidBody::FIXED);
+	RigidBody* blk = E.createRigidBody(phyz::ConvexUnionGeometry::box(mthz::Vec3(8.7, 2.7, -0.3), 0.6, 0.6, 0.6, m));
+	pin_body(a3);
+	pin_body(blk);
+	phyz::ConstraintID s1 = E.addSliderConstraint(a3, blk, mthz::Vec3(8.5, 3, 0), x);
+	record_joint<phyz::Slider>(E, s1, 7, a3, blk);
+	phyz::ConstraintID p1 = E.addPistonConstraint(s1, -0.4, 0.9);
+	auto* pp = static_cast<phyz::Piston*>(find_constraint(E, p1));
+	g_joints.push_back({12, a3, blk, pp, &pp->slide_limit});
+	g_joints.push_back({11, a3, blk, pp, &pp->piston_constraint});
+	const int jp = (int)g_joints.size() - 1;
+	struct Ev { int step, joint, mode; double max, target; };
+	std::vector<Ev> ev = {{0, jm1, 1, 4.0, 0},     {40, jm1, 2, 6.0, -2.0}, {80, jm1, 3, 5.0, 0.3},   {130, jm1, 0, 0, 0},
+	                      {0, jm2, 2, 3.0, 1.5},   {0, jp, 2, 40.0, 2.0},   {70, jp, 3, 30.0, -0.1},  {120, jp, 1, -20.0, 0},
+	                      {160, jp, 0, 0, 0}};
+	g_pre_step = [&](PhysicsEngine& e, int k) {
+		for (const Ev& v : ev) {
+			if (v.step != k) continue;
+			phyz::ConstraintID id = v.joint == jm1 ? m1 : (v.joint == jm2 ? m2 : p1);
+			bool motor = v.joint != jp;
+			switch (v.mode) {
+			case 0: motor ? e.setMotorOff(id) : e.setPistonOff(id); break;
+			case 1: motor ? e.setMotorConstantTorque(id, v.max) : e.setPistonConstantForce(id, v.max); break;
+			case 2: motor ? e.setMotorTargetVelocity(id, v.max, v.target) : e.setPistonTargetVelocity(id, v.max, v.target); break;
+			default: motor ? e.setMotorTargetPosition(id, v.max, v.target) : e.setPistonTargetPosition(id, v.max, v.target); break;
+			}
+		}
+	};
+	Out o(arg_s(argc, argv, "--out", "out"));
+	std::vector<double> evd;
+	for (const Ev& v : ev) evd.insert(evd.end(), {(double)v.step, (double)v.joint, (double)v.mode, v.max, v.target});
+	o.f64("drive_events", evd, {ev.size(), 5});
+	run_trace(E, cfg, steps, arg_steps(argc, argv, "--record"), o);
+	o.finish();
+	return 0;
+}
+
 // sleep: sleeping enabled (the engine's default parameters, every body's
 // sleep_disabled pinned to false): a 2x2x2 cube pile and a lone cube settle
 // and fall asleep, a cube launched along the ground slides to rest and falls
@@ -1349,6 +1472,7 @@ int main(int argc, char** argv) {
 	if (cmd == "sleep") return cmd_sleep(argc, argv);
 	if (cmd == "joints") return cmd_joints(argc, argv);
 	if (cmd == "limits") return cmd_limits(argc, argv);
+	if (cmd == "powered") return cmd_powered(argc, argv);
 	fprintf(stderr, "unknown command %s\n", cmd.c_str());
 	return 2;
 }

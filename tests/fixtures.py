@@ -151,6 +151,9 @@ def oracle_lib():
         lib.orc_add_joint.restype = ctypes.c_int
         lib.orc_add_joint.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
         lib.orc_set_joint_state.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        lib.orc_init_joint_state.argtypes = [ctypes.c_void_p]
+        lib.orc_set_drive.restype = ctypes.c_int
+        lib.orc_set_drive.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double]
         lib.orc_get_joint_state.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         lib.orc_set_hsys.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
         lib.orc_hsys_count.restype = ctypes.c_int
@@ -251,6 +254,14 @@ class Oracle:
             off, og = _c(hsys_off, np.int64), _c(hsys_og, np.int64)
             self.lib.orc_set_hsys(self.w, len(off) - 1, off.ctypes.data, og.ctypes.data)
         self.lib.orc_set_joint_state(self.w, st.ctypes.data)
+
+    def init_joint_state(self):
+        """Joint state as the reference's constructors leave it, from the loaded bodies."""
+        self.lib.orc_init_joint_state(self.w)
+
+    def set_drive(self, k, mode, max_, target):
+        if self.lib.orc_set_drive(self.w, int(k), int(mode), float(max_), float(target)) != 0:
+            raise ValueError("not a motor or piston drive")
 
     def joint_state(self):
         s = np.zeros((self.n_joints, 12))
@@ -472,3 +483,15 @@ def clip_nan_description():
         else:
             hulls.append({key: z[f"b{i}_{key}"] for key in CLIP_NAN_KEYS})
     return dict(hulls=hulls, movement=z["movement"], mass_props=z["mass_props"], material=z["material"], state=z["state"])
+
+
+def drive_events(fx, upto=None, at=None):
+    """The fixture's powered-mode changes (step, joint, mode, max, target):
+    those before step `upto` (the state a recorded step starts from), or
+    those applied at step `at`."""
+    if "drive_events" not in fx:
+        return []
+    ev = [(int(r[0]), int(r[1]), int(r[2]), float(r[3]), float(r[4])) for r in fx["drive_events"]]
+    if upto is not None:
+        return [e for e in ev if e[0] <= upto]
+    return [e for e in ev if e[0] == at]

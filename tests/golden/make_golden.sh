@@ -8,7 +8,7 @@ ROOT=$(cd "$HERE/../.." && pwd)
 make -C "$ROOT/oracle" ref REF="${REF:-/root/reference}"
 H="$ROOT/oracle/_ref/cp_ref_harness"
 cd "$HERE"
-rm -rf pile3 c0_full mix3 cyl3 cyl4 chain3 chain3h0 terrain hulls sleep joints joints_h0 limits limits_h0
+rm -rf pile3 c0_full mix3 cyl3 cyl4 chain3 chain3h0 terrain hulls sleep joints joints_h0 limits limits_h0 powered powered_h0
 # C0 pattern, 3x3x3 cubes: full per-step dumps (state, caches, manifolds, islands).
 "$H" pile --n 3 --steps 40 --record 0,1,2,10,39 --out pile3
 # C0 in full (10x10x10 cubes): first step and a settled step (no mid/post caches).
@@ -53,3 +53,10 @@ python3 make_hull_input.py hull_input.txt 24 20240
 # without the holonomic block solver.
 "$H" limits --steps 150 --record 0,1,10,11,14,15,16,20,21,30,32,40,57,80,145,149 --out limits
 "$H" limits --steps 150 --record 0,1,10,15,16,20,30,32,57,80,149 --hol 0 --out limits_h0
+# Motors and a piston: a motor on a hinge (angle limits -1.0, 1.2) through
+# constant torque, target velocity, target position and off; a motor on a
+# sliding hinge at a target velocity; a piston on a slider (slide limits
+# -0.4, 0.9) through target velocity, target position, constant force and
+# off; the mode changes are dumped as drive_events.
+"$H" powered --steps 200 --record 0,1,27,29,32,40,54,70,80,89,93,120,130,141,160,199 --out powered
+"$H" powered --steps 200 --record 0,27,40,54,70,80,93,130,160,199 --hol 0 --out powered_h0
