@@ -124,6 +124,7 @@ def lib():
         "cp_world_serving_info": (_I, [_P, _P]),
         "cp_joint_add": (_I, [_P, _I, _I, _I, _P, _P]),
         "cp_world_joint_count": (_I, [_P, _P]),
+        "cp_joint_set_drive": (_I, [_P, _I, _I, ctypes.c_double, ctypes.c_double]),
         "cp_world_set_joint_state": (_I, [_P, _P]),
         "cp_world_get_joint_state": (_I, [_P, _P]),
         "cp_world_get_joint_rows": (_I, [_P, _I, _P, _P, _P, _P, _P]),
@@ -517,16 +518,24 @@ class World:
 
     def add_joint(self, kind, b1, b2, geo14):
         """Joint of kind 1 distance, 3 ball-socket, 4 sliding hinge, 5 hinge,
-        6 weld, 7 slider, 8 cone limit or 9 twist limit between body indices
-        b1, b2; geo14 = attach points in the body frames, two body-frame axes
-        (hinge or slide axes; a distance joint's target distance first),
-        position and rotation correction strengths.  A cone limit holds its
-        maximum angle, a twist limit its minimum and maximum angles, in the
-        first attach-point slots."""
+        6 weld, 7 slider, 8 cone limit, 9 twist limit, 10 motor, 11 piston
+        drive or 12 piston slide limit between body indices b1, b2; geo14 =
+        attach points in the body frames, two body-frame axes (hinge or slide
+        axes; a distance joint's target distance first), position and
+        rotation correction strengths.  A cone limit holds its maximum angle,
+        a twist limit and a motor their minimum and maximum angles, in the
+        first attach-point slots; a slide limit its minimum and maximum
+        positions in the second axis slot.  A piston is its slide limit, then
+        its drive."""
         g = _a(geo14, np.float64).reshape(14)
         k = ctypes.c_int()
         _chk(self.L.cp_joint_add(self.w, int(kind), int(b1), int(b2), g.ctypes.data, ctypes.byref(k)), "cp_joint_add")
         return k.value
+
+    def set_drive(self, k, mode, max_=0.0, target=0.0):
+        """Powered mode of motor or piston drive k: 0 off, 1 constant torque /
+        force max_, 2 target velocity, 3 target position (max_, target)."""
+        _chk(self.L.cp_joint_set_drive(self.w, int(k), int(mode), float(max_), float(target)), "cp_joint_set_drive")
 
     def joint_count(self):
         n = ctypes.c_int()

@@ -86,8 +86,9 @@ struct Params {
 };
 
 // Joints, SoA over n joints.  kind: 1 distance, 3 ball-socket, 4 sliding
-// hinge, 5 hinge, 6 weld, 7 slider, 8 cone limit, 9 twist limit; type: the
-// joint's degree (rows: 1, 3, 4, 5, 6, 5, 1, 1).  State arrays are
+// hinge, 5 hinge, 6 weld, 7 slider, 8 cone limit, 9 twist limit, 10 motor,
+// 11 piston drive, 12 piston slide limit; type: the joint's degree (rows: 1,
+// 3, 4, 5, 6, 5, then 1 for each inequality kind).  State arrays are
 // component-major ([CP_JMAX][n], [3][n]).
 #define CP_JMAX 6
 #define CP_JSTATE 12  // joint state per joint at the ABI: CP_JMAX impulses, u, w
@@ -98,15 +99,42 @@ struct Joints {
 	double *ps, *rs;                // correction strengths
 	double *imp, *pimp, *himp;      // [CP_JMAX][n]
 	double *u, *w;                  // [3][n] basis of the current rows (hinge, slider, sliding hinge);
-	                                // limits: u x = is_inactive (1.0 / 0.0), u y = the twist's tracked angle
-	int* proj;                      // impulse projection of the rows: 0 none, 1 to >= 0, 2 to <= 0 (limits)
+	                                // cone / twist / slide limits: u x = inactive (1.0 / 0.0); twist: u y = tracked angle;
+	                                // slide limit: u y = piston position; motor, piston drive: u x = position, u y = prev_velocity
+	double* drive;                  // [4][n] powered mode (0 off, 1 constant, 2 target velocity, 3 target position), max, target velocity, target position
+	int* proj;                      // the rows' impulse projection (CP_PROJ_*)
 	double* row;                    // CP_JRS doubles per joint
 };
-// Cone (8) and twist (9) limits: one projected row, present in an island only
-// while active (ConeRotationLimit / TwistLimit, PersistentConstraint.h:145-215).
-__host__ __device__ __forceinline__ bool joint_is_limit(int kind) { return kind >= 8; }
-__device__ __forceinline__ bool joint_active(const int* kind, const double* u, int k) { return !joint_is_limit(kind[k]) || u[k] == 0.0; }
-#define CP_JRS 192
+// Kinds 8-12 are one-row inequality constraints, present in an island only
+// while active and never part of a holonomic system: cone (8) and twist (9)
+// limits (PersistentConstraint.h:145-215), motor (10), and a piston's drive
+// (11) and slide limit (12) (:218-290).
+__host__ __device__ __forceinline__ bool joint_is_inequality(int kind) { return kind >= 8; }
+// Projection of a row's accumulated impulse (projectValidImpulse), low bits;
+// CP_PROJ_NOPOS marks a row that the position pass skips (needsPosCorrect).
+enum { CP_PROJ_NONE = 0, CP_PROJ_LOWER = 1, CP_PROJ_UPPER = 2, CP_PROJ_CLAMP = 3, CP_PROJ_NOPOS = 4 };
+// Whether joint k adds its row to an island: limits while active, a slide
+// limit while exceeded (flags of their last update), a motor while powered or
+// outside its angle range (Motor::constraintIsActive, PersistentConstraint.cpp:
+// 110-112), a piston drive while powered (Piston::pistonIsActive, :195-197).
+struct JointAct {
+	int n;
+	const int* kind;
+	const double *u, *drive, *tp1;
+	__device__ __forceinline__ bool operator()(int k) const {
+		const int kd = kind[k];
+		if (kd < 8) return true;
+		if (kd == 10) {
+			double pos = u[k];
+			return (drive[k] != 0.0 && drive[n + k] != 0) || pos < tp1[k] || pos > tp1[n + k];
+		}
+		if (kd == 11) return drive[k] != 0.0 && drive[n + k] != 0;
+		return u[k] == 0.0;
+	}
+};
+#define CP_J_BOUND 192  // a powered row's impulse bound (max_torque_impulse, max_impulse)
+#define CP_J_TV 193     // a powered row's target velocity (real_target_velocity)
+#define CP_JRS 196
 enum JointRowField {
 	CP_J_JA = 0,      // J_a [CP_JMAX][6]
 	CP_J_JB = 36,     // J_b [CP_JMAX][6]
@@ -197,8 +225,8 @@ struct IslTables {
 struct IslWalk {
 	IslTables T;
 	int nb, hol;
-	const int *movement, *asleep, *aoff, *adjv, *me, *jord, *live, *ja, *jb, *jsys, *jkind;
-	const double* ju;  // joint state u: a limit's is_inactive flag
+	const int *movement, *asleep, *aoff, *adjv, *me, *jord, *live, *ja, *jb, *jsys;
+	JointAct jact;
 	const uint64_t *adjk, *ckey;
 	int *stack, *vlist, *elist, *vis, *rows, *ecol, *coff, *eorder, *icoff, *ncol, *serial, *eoff, *bod, *nbod, *loc, *cloc, *jloc;
 	int *sys_seen, *sys, *slots, *jlin, *sys_cnt;
@@ -405,6 +433,7 @@ void launch_build_rows(const Bodies& b, const Contacts& C, int nc, const Params&
 // Joint rows: mode 0 all joints (holonomic ones skipped when first), 1 the
 // holonomic joints with warm-start re-projection, 2 the holonomic joints
 // without re-projection.
+void launch_joint_init(const Bodies& b, const Joints& J, int k0, int k1, hipStream_t s);
 void launch_joint_rows(const Bodies& b, const Joints& J, const Params& P, double sub_dt, int mode, int first, hipStream_t s);
 // Block LDL factor of every holonomic system.
 void launch_hsys_factor(const Joints& J, const HSysDev& H, double cfm, hipStream_t s);
@@ -445,7 +474,7 @@ void launch_pgs_colored(const Bodies& b, const Contacts& C, const int* scene_bod
 void launch_ib_keys(const uint64_t* ekey, int ne, const uint64_t* jgkey, int ng, uint64_t* keys, int* vals, hipStream_t s);
 void launch_ib_heads(const uint64_t* skeys, int n, int* head, hipStream_t s);
 void launch_ib_edges(const uint64_t* skeys, const int* svals, int n, const int* head, const int* midx, int ne, const int* efirst,
-                     const int* ecnt, const int* live, const int* jgoff, const int* jord, const int* jkind, const double* ju, int* me,
+                     const int* ecnt, const int* live, const int* jgoff, const int* jord, const JointAct& jact, int* me,
                      uint64_t* adjk, int* adjv, hipStream_t s);
 void launch_ib_aoff(const uint64_t* adjk, int n2, int nb, int* aoff, hipStream_t s);
 void launch_ib_components(const int* me, const int* midx, int n, int nb, const int* movement, const int* rank, const int* aoff, int* parent,

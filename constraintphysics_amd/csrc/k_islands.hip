@@ -75,13 +75,13 @@ __global__ void k_ib_heads(const uint64_t* skeys, int n, int* head) {
 }
 
 // Merged edge e = midx[i] of head entry i: lo, hi, first contact, contacts,
-// live contacts, joints jord[j0, j1), active joints among them (a limit adds
-// a row only while active, by the flag of its last update); and its two
+// live contacts, joints jord[j0, j1), active joints among them (an
+// inequality joint adds its row only while active, JointAct); and its two
 // adjacency entries.  Slots past the merged count get adjacency keys that
 // sort last.
 __global__ void k_ib_edges(const uint64_t* skeys, const int* svals, int n, const int* head, const int* midx, int ne, const int* efirst,
-                           const int* ecnt, const int* live, const int* jgoff, const int* jord, const int* jkind, const double* ju,
-                           int* me, uint64_t* adjk, int* adjv) {
+                           const int* ecnt, const int* live, const int* jgoff, const int* jord, JointAct jact, int* me,
+                           uint64_t* adjk, int* adjv) {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n) return;
 	const int M = midx[n];
@@ -106,7 +106,7 @@ __global__ void k_ib_edges(const uint64_t* skeys, const int* svals, int n, const
 		} else {
 			E[5] = jgoff[v - ne];
 			E[6] = jgoff[v - ne + 1];
-			for (int x2 = E[5]; x2 < E[6]; x2++) E[7] += joint_active(jkind, ju, jord[x2]);
+			for (int x2 = E[5]; x2 < E[6]; x2++) E[7] += jact(jord[x2]);
 		}
 	}
 	int4* o = reinterpret_cast<int4*>(me + 8 * (size_t)e);
@@ -298,7 +298,7 @@ __global__ void k_ib_walk(IslWalk W, int nisl) {
 	auto emit = [&](int e) {
 		const int* E = W.me + 8 * (size_t)e;
 		for (int x = E[5]; x < E[6] && nr < rcap; x++) {
-			if (!joint_active(W.jkind, W.ju, W.jord[x])) continue;
+			if (!W.jact(W.jord[x])) continue;
 			rows[2 * nr] = W.jord[x];
 			rows[2 * nr + 1] = 2;
 			nr++;
@@ -441,11 +441,11 @@ void launch_ib_heads(const uint64_t* skeys, int n, int* head, hipStream_t s) {
 	hipLaunchKernelGGL(k_ib_heads, dim3(nblk256(n + 1)), dim3(256), 0, s, skeys, n, head);
 }
 void launch_ib_edges(const uint64_t* skeys, const int* svals, int n, const int* head, const int* midx, int ne, const int* efirst,
-                     const int* ecnt, const int* live, const int* jgoff, const int* jord, const int* jkind, const double* ju, int* me,
+                     const int* ecnt, const int* live, const int* jgoff, const int* jord, const JointAct& jact, int* me,
                      uint64_t* adjk, int* adjv, hipStream_t s) {
 	if (n)
 		hipLaunchKernelGGL(k_ib_edges, dim3(nblk256(n)), dim3(256), 0, s, skeys, svals, n, head, midx, ne, efirst, ecnt, live, jgoff, jord,
-		                   jkind, ju, me, adjk, adjv);
+		                   jact, me, adjk, adjv);
 }
 void launch_ib_aoff(const uint64_t* adjk, int n2, int nb, int* aoff, hipStream_t s) {
 	hipLaunchKernelGGL(k_ib_aoff, dim3(nblk256(n2 + 1)), dim3(256), 0, s, adjk, n2, nb, aoff);

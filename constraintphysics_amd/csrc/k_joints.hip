@@ -116,6 +116,53 @@ __device__ __forceinline__ double limit_error(double f) {
 }
 #define CP_QUAT_PI 3.1415926535  // PI of Quaternion.h:4, the twist angle's unwrap
 
+// Motor::calculatePosition (PersistentConstraint.cpp:30-71): the angle
+// predicted from the relative angular velocity, snapped to the measured angle
+// of b2's comparison axis in b1's reference plane (the candidate nearest the
+// prediction among the measured angle shifted by -2PI, 0, +2PI).
+__device__ double motor_position(const Joints& J, int k, Q qa, Q qb, V3 axis, V3 av1, V3 av2, double dt, double prev) {
+	const int K = J.n;
+	V3 u1, w1, u2, tmp;
+	perp_basis(jld3(J.ax1, k, K), &u1, &w1);  // Motor ctor (:8-13)
+	perp_basis(jld3(J.ax2, k, K), &u2, &tmp);
+	V3 ru = q_rotate(qa, u1), rw = q_rotate(qa, w1), cmp = q_rotate(qb, u2);
+	double rel = dot(axis, av1) - dot(axis, av2);
+	double pred = prev + rel * dt;
+	if (fabs(dot(cmp, axis)) > 0.999) return pred;
+	V3 cn = normalize(cmp - axis * dot(axis, cmp));
+	double dv = dot(cn, ru);
+	if (dv > 1.0) dv = 1.0;
+	if (dv < -1.0) dv = -1.0;
+	double relang = 2 * CP_QUAT_PI - acos(dv);
+	if (dot(cn, rw) < 0) relang = 2 * CP_QUAT_PI - relang;
+	double r = fmod(pred, 2 * CP_QUAT_PI);
+	double n2 = pred - r;
+	const double cand[3] = {n2 + relang - 2 * CP_QUAT_PI, n2 + relang, n2 + relang + 2 * CP_QUAT_PI};
+	double closest = __builtin_huge_val();
+	for (int c = 0; c < 3; c++)
+		if (fabs(pred - cand[c]) < fabs(pred - closest)) closest = cand[c];
+	return closest;
+}
+
+// getConstraintTargetVelocityValue of the motor (:77-108) and the piston
+// (:153-185).  vrel: the relative velocity along the axis as each measures
+// it; inv_sum: inverse inertia (motor) or inverse mass (piston) along it;
+// the reference tests abs(impulse < EPS), i.e. impulse < EPS.
+__device__ double drive_target(const Joints& J, int k, double vrel, double inv_sum, double imp, double pos, double prev_vel) {
+	const int K = J.n;
+	const int mode = (int)J.drive[k];
+	const double mx = J.drive[K + k];
+	if (mode == 0) return 0;
+	if (mode == 1) return mx > 0 ? __builtin_huge_val() : -__builtin_huge_val();
+	if (mode == 2) return J.drive[2 * K + k];
+	double diff = pos - J.drive[3 * K + k];
+	double dir = (diff > 0) ? -1 : 1;
+	double vd = fabs(vrel - prev_vel);
+	const double EPS = 0.0000001;
+	double inertia = (vd < EPS || imp < EPS) ? 1.0 / inv_sum : fabs(imp) / vd;
+	return dir * sqrt(2 * fabs(diff) * mx / inertia);
+}
+
 // Solver rows of joint k from the current body state: the joint's
 // updateSolverConstraints (PersistentConstraint.cpp:217-354, warm start
 // coefficient 1) and its constraint's constructor (ConstraintSolver.cpp:
@@ -148,8 +195,10 @@ __device__ void build_joint(const Bodies& b, Joints& J, int k, double sub_dt, do
 	double start[CP_JMAX];
 	for (int c = 0; c < n; c++) start[c] = J.imp[c * K + k] * 1.0;
 	const int kind = J.kind[k];
-	double lim_err = 0;  // a limit's position error (1 - cos of the excess, Taylor)
-	V3 lim_a{0, 0, 0}, lim_b{0, 0, 0};  // a limit's angular rows
+	// an inequality row: linear and angular parts for each body, pseudo target
+	V3 il_a{0, 0, 0}, lim_a{0, 0, 0}, il_b{0, 0, 0}, lim_b{0, 0, 0};
+	double ipt = 0;
+	auto vel_of = [&](const double* v, int i) { return b.movement[i] == 1 ? V3{0, 0, 0} : jld3(v, i, N); };  // getVel / getAngVel
 	if (kind == 8) {
 		// ConeRotationLimit::updateSolverConstraints (PersistentConstraint.cpp:233-247):
 		// inside the cone the limit clears its impulse and keeps its rows
@@ -166,8 +215,8 @@ __device__ void build_joint(const Bodies& b, Joints& J, int k, double sub_dt, do
 		V3 h = normalize(cross(d1, d2));
 		lim_a = h;
 		lim_b = V3{-h.x, -h.y, -h.z};
-		lim_err = limit_error(angle - lim);
-		J.proj[k] = 1;
+		ipt = limit_error(angle - lim) * J.rs[k];  // the raw rotation hardness, not posCorrectCoeff
+		J.proj[k] = CP_PROJ_LOWER;
 	} else if (kind == 9) {
 		// TwistLimit::updateSolverConstraints (PersistentConstraint.cpp:249-272);
 		// the angle as getCurrentAngle (:115-132) measures it: the reference
@@ -195,8 +244,86 @@ __device__ void build_joint(const Bodies& b, Joints& J, int k, double sub_dt, do
 		bool above = cur > mx;
 		lim_a = V3{-t1.x, -t1.y, -t1.z};
 		lim_b = t2;
-		lim_err = limit_error(above ? cur - mx : mn - cur);
-		J.proj[k] = above ? 2 : 1;
+		ipt = limit_error(above ? cur - mx : mn - cur) * J.rs[k];
+		J.proj[k] = above ? CP_PROJ_UPPER : CP_PROJ_LOWER;
+	} else if (kind == 10) {
+		// Motor::updateSolverConstraints (PersistentConstraint.cpp:320-329) ->
+		// MotorConstraint (ConstraintSolver.cpp:555-584); prev_velocity is
+		// written after the build either way
+		V3 axis = q_rotate(qa, jld3(J.ax1, k, K));
+		V3 av1 = vel_of(b.angv, ia), av2 = vel_of(b.angv, ib);
+		const double pos = motor_position(J, k, qa, qb, axis, av1, av2, sub_dt, J.u[k]);
+		J.u[k] = pos;
+		const double pv = dot(axis, av1) - dot(axis, av2);
+		JointAct act{K, J.kind, J.u, J.drive, J.tp1};
+		if (!act(k)) {
+			J.u[K + k] = pv;
+			return;
+		}
+		M3 ia3 = m3_zero(), ib3 = m3_zero();
+		for (int r = 0; r < 3; r++)
+			for (int c = 0; c < 3; c++) {
+				if (da) ia3.m[r][c] = b.Iinv[(3 * r + c) * N + ia];
+				if (db) ib3.m[r][c] = b.Iinv[(3 * r + c) * N + ib];
+			}
+		const double inv_sum = dot(axis, ia3 * axis) + dot(axis, ib3 * axis);
+		const double target = 1.0 * drive_target(J, k, pv, inv_sum, start[0], pos, J.u[K + k]);
+		const double rcc0 = pos_coeff(J.rs[k], sub_dt);
+		const double mn = J.tp1[k], mx = J.tp1[K + k];
+		R[CP_J_BOUND] = fabs(J.drive[K + k] * sub_dt);
+		if (pos > mx) {
+			ipt = (mx - pos) * rcc0;
+			R[CP_J_TV] = target < 0 ? target : 0.0;  // std::min<double>(0, target)
+			J.proj[k] = CP_PROJ_UPPER;
+		} else if (pos < mn) {
+			ipt = (mn - pos) * rcc0;
+			R[CP_J_TV] = 0.0 < target ? target : 0.0;  // std::max<double>(0, target)
+			J.proj[k] = CP_PROJ_LOWER;
+		} else {
+			ipt = 0;
+			R[CP_J_TV] = target;
+			J.proj[k] = CP_PROJ_CLAMP | CP_PROJ_NOPOS;
+		}
+		lim_a = axis;
+		lim_b = V3{-axis.x, -axis.y, -axis.z};
+		J.u[K + k] = pv;
+	} else if (kind == 11 || kind == 12) {
+		// Piston::updateSolverConstraints (PersistentConstraint.cpp:300-318)
+		V3 axis = q_rotate(qa, jld3(J.ax1, k, K));
+		const double pos = dot(pb, axis) - dot(pa, axis);  // Piston::calculatePosition (:149-151)
+		if (kind == 12) {
+			J.u[K + k] = pos;
+			const double mn = J.ax2[k], mx = J.ax2[K + k];
+			bool inactive = !(pos < mn || pos > mx);  // slideLimitExceeded (:191-193)
+			J.u[k] = inactive ? 1.0 : 0.0;
+			if (inactive) return;
+			// SlideLimitConstraint (ConstraintSolver.cpp:736-766)
+			const double pcc0 = pos_coeff(J.ps[k], sub_dt);
+			if (pos > mx) {
+				ipt = (mx - pos) * pcc0;
+				J.proj[k] = CP_PROJ_UPPER;
+			} else {
+				ipt = (mn - pos) * pcc0;
+				J.proj[k] = CP_PROJ_LOWER;
+			}
+		} else {
+			J.u[k] = pos;
+			JointAct act{K, J.kind, J.u, J.drive, J.tp1};
+			if (!act(k)) return;
+			// PistonConstraint (ConstraintSolver.cpp:701-721); the piston never
+			// writes prev_velocity
+			const double inv_sum = (da ? 1.0 / b.mass[ia] : 0.0) + (db ? 1.0 / b.mass[ib] : 0.0);
+			V3 v1 = vel_of(b.vel, ia), v2 = vel_of(b.vel, ib);
+			R[CP_J_TV] = drive_target(J, k, dot(axis, v1) - dot(axis, v2), inv_sum, start[0], pos, J.u[K + k]);
+			R[CP_J_BOUND] = J.drive[K + k] * sub_dt;
+			J.proj[k] = CP_PROJ_CLAMP | CP_PROJ_NOPOS;
+		}
+		il_a = V3{-axis.x, -axis.y, -axis.z};
+		const double ns[3] = {-axis.x, -axis.y, -axis.z};
+		double t[3];
+		row_cross(ns, axis, t);
+		lim_a = V3{t[0], t[1], t[2]};
+		il_b = axis;
 	}
 	for (int c = 0; c < n; c++) J.imp[c * K + k] = start[c];
 	double ja[CP_JMAX][6], jb[CP_JMAX][6];
@@ -219,10 +346,12 @@ __device__ void build_joint(const Bodies& b, Joints& J, int k, double sub_dt, do
 		jst3(J.u, k, K, *u);
 		jst3(J.w, k, K, *w);
 	};
-	if (joint_is_limit(kind)) {
+	if (joint_is_inequality(kind)) {
+		ja[0][0] = il_a.x; ja[0][1] = il_a.y; ja[0][2] = il_a.z;
 		ja[0][3] = lim_a.x; ja[0][4] = lim_a.y; ja[0][5] = lim_a.z;
+		jb[0][0] = il_b.x; jb[0][1] = il_b.y; jb[0][2] = il_b.z;
 		jb[0][3] = lim_b.x; jb[0][4] = lim_b.y; jb[0][5] = lim_b.z;
-		pt[0] = lim_err * J.rs[k];  // the raw rotation hardness, not posCorrectCoeff
+		pt[0] = ipt;
 	} else if (kind == 1) {  // DistanceConstraint (ConstraintSolver.cpp:348-379), not in moving mode
 		V3 rA = pa - ca, rB = pb - cb;
 		V3 diff = pa - pb;
@@ -367,6 +496,30 @@ __device__ void build_joint(const Bodies& b, Joints& J, int k, double sub_dt, do
 	}
 }
 
+// Joints k0..k1-1 as their constructors leave them: a motor's angle measured
+// from the bodies (Motor ctor, PersistentConstraint.cpp:17-22) and a piston's
+// position from its two attach points.  The Piston constructor (:139-141)
+// looks b2's point key up in b1's tracked points; that value is replaced at
+// the first update and is seen only by a position query before the first step.
+__global__ __launch_bounds__(64) void k_joint_init(Bodies b, Joints J, int k0, int k1) {
+	const int k = k0 + blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= k1) return;
+	const int N = b.n, K = J.n, kind = J.kind[k];
+	if (kind < 10) return;
+	const int ia = J.a[k], ib = J.b[k];
+	Q qa{b.q[ia], b.q[N + ia], b.q[2 * N + ia], b.q[3 * N + ia]};
+	Q qb{b.q[ib], b.q[N + ib], b.q[2 * N + ib], b.q[3 * N + ib]};
+	if (kind == 10) {
+		V3 z{0, 0, 0};
+		J.u[k] = motor_position(J, k, qa, qb, q_rotate(qa, jld3(J.ax1, k, K)), z, z, 0, 0);
+		return;
+	}
+	V3 pa = jld3(b.com, ia, N) + q_rotate(qa, jld3(J.tp1, k, K));
+	V3 pb = jld3(b.com, ib, N) + q_rotate(qb, jld3(J.tp2, k, K));
+	V3 axis = q_rotate(qa, jld3(J.ax1, k, K));
+	J.u[(kind == 11 ? 0 : K) + k] = dot(pb, axis) - dot(pa, axis);
+}
+
 // mode 0: every joint except holonomic ones at the first substep; 1: holonomic
 // joints with re-projection (end of step); 2: holonomic joints as stored
 // (state loaded, or systems just created).
@@ -476,6 +629,9 @@ __global__ __launch_bounds__(64) void k_hsys_factor(Joints J, HSysDev H, double 
 
 static inline int nblk(int n, int t) { return (n + t - 1) / t; }
 
+void launch_joint_init(const Bodies& b, const Joints& J, int k0, int k1, hipStream_t s) {
+	if (k1 > k0) hipLaunchKernelGGL(k_joint_init, dim3(nblk(k1 - k0, 64)), dim3(64), 0, s, b, J, k0, k1);
+}
 void launch_joint_rows(const Bodies& b, const Joints& J, const Params& P, double sub_dt, int mode, int first, hipStream_t s) {
 	if (J.n) hipLaunchKernelGGL(k_joint_rows, dim3(nblk(J.n, 64)), dim3(64), 0, s, b, J, P, sub_dt, mode, first);
 }

@@ -570,7 +570,9 @@ __device__ __forceinline__ void joint_target(const Bodies& b, const Joints& J, i
 		vb[3 + c] = fb ? 0.0 : b.angv[c * N + ib];
 	}
 	joint_value(R, J.type[k], va, vb, 1, val);
-	for (int c = 0; c < J.type[k]; c++) R[CP_J_TGT + c] = -val[c];
+	if (J.kind[k] == 10 || J.kind[k] == 11) R[CP_J_TGT] = R[CP_J_TV] - val[0];  // Motor / PistonConstraint::findTargetConstraintValue
+	else
+		for (int c = 0; c < J.type[k]; c++) R[CP_J_TGT + c] = -val[c];
 }
 
 // PGSConstraintStep<n> of a joint (ConstraintSolver.cpp:12-40); the limits
@@ -594,11 +596,16 @@ __device__ __forceinline__ void joint_step(const Bodies& b, const Joints& J, con
 		for (int c = 0; c < n; c++) mm[i] += delta[c] * R[CP_J_INV + i * n + c];
 	}
 	for (int c = 0; c < n; c++) diff[c] = (a[c] + mm[c]) - a[c];
-	const int pj = J.proj[k];
-	if (pj) {  // projectValidImpulse of the limits (ConstraintSolver.cpp:432-435, :461-463), on the accumulation
-		double nw = a[0] + mm[0];
-		// std::min / std::max<double>(nw, 0): signed zeros and NaN as they pass
-		double pr = pj == 2 ? (0.0 < nw ? 0.0 : nw) : (nw < 0.0 ? 0.0 : nw);
+	const int pj = J.proj[k] & 3;
+	if (pj) {  // projectValidImpulse of the inequality rows (ConstraintSolver.cpp:432-775), on the accumulation
+		double nw = a[0] + mm[0], pr;
+		// std::min / std::max<double>: signed zeros and NaN as they pass
+		if (pj == CP_PROJ_CLAMP) {
+			const double bd = R[CP_J_BOUND];
+			pr = nw > 0 ? (bd < nw ? bd : nw) : (nw < -bd ? -bd : nw);
+		} else {
+			pr = pj == CP_PROJ_UPPER ? (0.0 < nw ? 0.0 : nw) : (nw < 0.0 ? 0.0 : nw);
+		}
 		diff[0] = pr - a[0];
 	}
 	if (!is_zero_n(diff, n)) {
@@ -618,7 +625,9 @@ __device__ __forceinline__ void island_row(const Bodies& b, const Contacts& C, c
 			if (!is_zero_n(imp, n))
 				joint_apply(J.row + (size_t)k * CP_JRS, n, imp, m.base + jl(J, m, k, 0), m.base + jl(J, m, k, 1), m.stride,
 				            b.movement[J.a[k]] == 0, b.movement[J.b[k]] == 0);
-		} else joint_step(b, J, m, k, phase == 2);
+		} else if (phase == 1 || !(J.proj[k] & CP_PROJ_NOPOS)) {  // needsPosCorrect (ConstraintSolver.cpp:152)
+			joint_step(b, J, m, k, phase == 2);
+		}
 		return;
 	}
 	if (phase == 2 && kind != 0) return;

@@ -224,10 +224,12 @@ struct cp_world {
 	// joints (host copies of static data; device SoA)
 	std::vector<int> j_type, j_kind, j_a, j_b;  // j_type: degree (rows); j_kind: cp_joint_add's type
 	std::vector<double> j_geo;  // 14 per joint: tp1, tp2, ax1, ax2, ps, rs
+	std::vector<double> j_drive;  // 4 per joint: powered mode, max, target velocity, target position (motors, piston drives)
+	bool drive_dirty = false;
 	int j_dev_n = 0;            // joints on the device
 	bool joints_dirty = false;  // static joint data or systems changed since the upload
 	DArr<int> dj_type, dj_kind, dj_a, dj_b, dj_sys, dj_proj;
-	DArr<double> dj_tp1, dj_tp2, dj_ax1, dj_ax2, dj_ps, dj_rs, dj_imp, dj_pimp, dj_himp, dj_u, dj_w, dj_row, lin;
+	DArr<double> dj_tp1, dj_tp2, dj_ax1, dj_ax2, dj_ps, dj_rs, dj_imp, dj_pimp, dj_himp, dj_u, dj_w, dj_row, dj_drive, lin;
 	DArr<uint64_t> nocol;
 	int n_nocol = 0;
 	// Joint edges (body pairs with joints) of the constraint graph: n1 is b1 of
@@ -414,9 +416,10 @@ struct cp_world {
 		J.type = dj_type.p; J.kind = dj_kind.p; J.a = dj_a.p; J.b = dj_b.p; J.sys = dj_sys.p;
 		J.tp1 = dj_tp1.p; J.tp2 = dj_tp2.p; J.ax1 = dj_ax1.p; J.ax2 = dj_ax2.p; J.ps = dj_ps.p; J.rs = dj_rs.p;
 		J.imp = dj_imp.p; J.pimp = dj_pimp.p; J.himp = dj_himp.p; J.u = dj_u.p; J.w = dj_w.p; J.row = dj_row.p;
-		J.proj = dj_proj.p;
+		J.proj = dj_proj.p; J.drive = dj_drive.p;
 		return J;
 	}
+	JointAct joint_act() { return JointAct{(int)j_type.size(), dj_kind.p, dj_u.p, dj_drive.p, dj_tp1.p}; }
 	HSysDev hsys() {
 		HSysDev H{};
 		H.nsys = (int)hs_lay.size();
@@ -443,6 +446,10 @@ struct cp_world {
 	void place_meshes(const double* state13);
 	int mesh_narrowphase(int npairs);
 	void upload_joints();
+	void upload_drives();
+	void init_new_joints(const Bodies& b);
+	static int state_projection(int kind, const double* s6, const double* g);
+	int j_init_from = 0;  // joints from this index on take their constructor state from the bodies
 	void read_joint_state(double* s, int n);
 	void write_joint_state(const double* s);
 	void layout_systems();
@@ -507,7 +514,7 @@ void cp_world::free_all() {
 	ib_rord_host.clear();
 	isl_n = 0;
 	islands_on_device = false;
-	DArr<double>* jdbl[] = {&dj_tp1, &dj_tp2, &dj_ax1, &dj_ax2, &dj_ps, &dj_rs, &dj_imp, &dj_pimp, &dj_himp, &dj_u, &dj_w, &dj_row, &lin,
+	DArr<double>* jdbl[] = {&dj_tp1, &dj_tp2, &dj_ax1, &dj_ax2, &dj_ps, &dj_rs, &dj_imp, &dj_pimp, &dj_himp, &dj_u, &dj_w, &dj_row, &dj_drive, &lin,
 	                        &dh_buf, &dh_dbuf, &dh_scratch};
 	for (auto* a : jdbl) a->free_();
 	nocol.free_();
@@ -646,23 +653,52 @@ void cp_world::write_joint_state(const double* s) {
 			u[c * (size_t)K + k] = r[CP_JMAX + c];
 			w[c * (size_t)K + k] = r[CP_JMAX + 3 + c];
 		}
-		// a limit projects to >= 0, an active twist limit built beyond its
-		// maximum to <= 0 (TwistLimitConstraint::lim_status)
-		if (joint_is_limit(j_kind[k]))
-			proj[k] = j_kind[k] == 9 && r[CP_JMAX] == 0.0 && r[CP_JMAX + 1] > j_geo[14 * (size_t)k + 1] ? 2 : 1;
+		if (joint_is_inequality(j_kind[k])) proj[k] = state_projection(j_kind[k], r + CP_JMAX, &j_geo[14 * (size_t)k]);
 	}
 	up(dj_imp, imp); up(dj_u, u); up(dj_w, w); up(dj_pimp, zero); up(dj_himp, zero); up(dj_proj, proj);
 	hs_pending = true;  // rows and factors are rebuilt from this state at the next step
 }
 
-// Static joint data, system membership and the no-collision pair list; the
-// accumulated state of joints already on the device is kept.
+// The projection of an inequality row whose last build left the state s6
+// (state columns 6-11) and geometry g: a limit, slide limit or motor row
+// built beyond its maximum projects to <= 0, below its minimum to >= 0; a
+// motor inside its range and a piston drive clamp to their bound and skip
+// the position pass (TwistLimitConstraint / MotorConstraint /
+// SlideLimitConstraint lim_status, ConstraintSolver.cpp:432-775).
+int cp_world::state_projection(int kind, const double* s6, const double* g) {
+	switch (kind) {
+	case 9: return s6[0] == 0.0 && s6[1] > g[1] ? CP_PROJ_UPPER : CP_PROJ_LOWER;
+	case 10: return s6[0] > g[1] ? CP_PROJ_UPPER : (s6[0] < g[0] ? CP_PROJ_LOWER : (CP_PROJ_CLAMP | CP_PROJ_NOPOS));
+	case 11: return CP_PROJ_CLAMP | CP_PROJ_NOPOS;
+	case 12: return s6[1] > g[10] ? CP_PROJ_UPPER : CP_PROJ_LOWER;
+	default: return CP_PROJ_LOWER;
+	}
+}
+
+// Static joint data, system membership, powered modes and the no-collision
+// pair list; the accumulated state of joints already on the device is kept.
+// New joints start as the reference's constructors leave them: limits
+// inactive, and the positions of motors and pistons measured from the bodies
+// at the next step (k_joint_init).
 void cp_world::upload_joints() {
 	int K = (int)j_type.size();
+	const int K0 = std::max(j_dev_n, 0);
 	std::vector<double> st(CP_JSTATE * (size_t)K, 0.0);
-	if (j_dev_n > 0) read_joint_state(st.data(), j_dev_n);
+	// solver objects of the joints already on the device: rows, their
+	// projections and pseudo impulses stay as they are (an inequality joint
+	// may solve with them again); new joints start from zeroed rows
+	std::vector<double> rows, pimp0;
+	std::vector<int> proj0;
+	if (K0 > 0) {
+		HIPCHK(hipStreamSynchronize(stream));
+		read_joint_state(st.data(), K0);
+		down(rows, dj_row.p, (size_t)CP_JRS * K0);
+		down(pimp0, dj_pimp.p, (size_t)CP_JMAX * K0);
+		down(proj0, dj_proj.p, K0);
+	}
+	rows.resize((size_t)CP_JRS * std::max(K, 1), 0.0);
 	for (int k = std::max(j_dev_n, 0); k < K; k++)
-		if (joint_is_limit(j_kind[k])) st[CP_JSTATE * (size_t)k + CP_JMAX] = 1.0;  // limits start inactive
+		if (j_kind[k] == 8 || j_kind[k] == 9 || j_kind[k] == 12) st[CP_JSTATE * (size_t)k + CP_JMAX] = 1.0;  // inactive
 	std::vector<int> sys(K, -1);
 	for (size_t q = 0; q < hs_og.size(); q++)
 		for (int j : hs_og[q]) sys[j] = (int)q;
@@ -677,9 +713,21 @@ void cp_world::upload_joints() {
 		rs[k] = r[13];
 	}
 	up(dj_tp1, g[0]); up(dj_tp2, g[1]); up(dj_ax1, g[2]); up(dj_ax2, g[3]); up(dj_ps, ps); up(dj_rs, rs);
-	dj_row.reserve((size_t)CP_JRS * std::max(K, 1));
+	upload_drives();
+	up(dj_row, rows);
 	j_dev_n = K;
 	write_joint_state(st.data());
+	if (K0 > 0) {
+		std::vector<double> pimp(CP_JMAX * (size_t)K, 0.0);
+		std::vector<int> proj;
+		down(proj, dj_proj.p, K);
+		for (int k = 0; k < K0; k++) {
+			proj[k] = proj0[k];
+			for (int c = 0; c < CP_JMAX; c++) pimp[c * (size_t)K + k] = pimp0[c * (size_t)K0 + k];
+		}
+		up(dj_pimp, pimp);
+		up(dj_proj, proj);
+	}
 	// disallowCollision pairs of jointed bodies (PhysicsEngine.cpp:570-572), sorted
 	std::vector<uint64_t> nc;
 	for (int k = 0; k < K; k++) {
@@ -691,6 +739,25 @@ void cp_world::upload_joints() {
 	up(nocol, nc);
 	n_nocol = (int)nc.size();
 	joints_dirty = false;
+}
+
+// Powered modes of motors and piston drives, component-major [4][K].
+void cp_world::upload_drives() {
+	const int K = (int)j_type.size();
+	std::vector<double> d(4 * (size_t)std::max(K, 1), 0.0);
+	for (int k = 0; k < K; k++)
+		for (int c = 0; c < 4; c++) d[c * (size_t)K + k] = j_drive[4 * (size_t)k + c];
+	up(dj_drive, d);
+	drive_dirty = false;
+}
+
+// Motor angles and piston positions of joints added since the last step,
+// measured from the current bodies (the Motor and Piston constructors).
+void cp_world::init_new_joints(const Bodies& b) {
+	const int K = (int)j_type.size();
+	if (j_init_from >= K || !uploaded) return;
+	launch_joint_init(b, joints(), j_init_from, K, stream);
+	j_init_from = K;
 }
 
 // HolonomicSystem::HolonomicSystem (HolonomicBlockSolver.cpp:11-132): the
@@ -942,7 +1009,7 @@ void cp_world::discover_systems() {
 		hs_og.emplace_back();
 		for (uint64_t k : edges)
 			for (int j : jedges[k].joints)
-				if (!joint_is_limit(j_kind[j])) hs_og.back().push_back(j);  // revaluateSystem (:1785-1798)
+				if (!joint_is_inequality(j_kind[j])) hs_og.back().push_back(j);  // revaluateSystem (:1785-1798)
 	}
 	layout_systems();
 }
@@ -1318,7 +1385,7 @@ void cp_world::schedule_islands(bool colored) {
 	scan(ib_head.p, ib_midx.p, N + 1);
 	if (N) {
 		launch_ib_edges(ib_keys2.p, ib_vals2.p, N, ib_head.p, ib_midx.p, ne, e_first.p, e_cnt.p, cb[cur].c.live, ib_jgoff.p, ib_jord.p,
-		                dj_kind.p, dj_u.p, ib_me.p, ib_adjk.p, ib_adjv.p, stream);
+		                joint_act(), ib_me.p, ib_adjk.p, ib_adjv.p, stream);
 		bytes = 0;
 		HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, ib_adjk.p, ib_adjk2.p, ib_adjv.p, ib_adjv2.p, 2 * N, 0, 64, stream));
 		tmp.reserve(bytes + 16);
@@ -1346,7 +1413,7 @@ void cp_world::schedule_islands(bool colored) {
 	W.hol = cfg.hol_iters > 0 && !hs_lay.empty();
 	W.movement = movement.p; W.asleep = S.enabled ? S.asleep : nullptr; W.aoff = ib_aoff.p; W.adjv = ib_adjv2.p; W.me = ib_me.p;
 	W.jord = ib_jord.p; W.live = cb[cur].c.live; W.ja = dj_a.p; W.jb = dj_b.p; W.jsys = dj_sys.p; W.adjk = ib_adjk2.p;
-	W.jkind = dj_kind.p; W.ju = dj_u.p;
+	W.jact = joint_act();
 	W.ckey = cb[cur].c.key; W.stack = ib_stack.p; W.vlist = ib_vlist.p; W.elist = ib_elist.p; W.vis = ib_vis.p; W.rows = isl_rows.p;
 	W.ecol = ib_ecol.p; W.coff = isl_coff.p; W.eorder = ib_eorder.p; W.icoff = isl_icoff.p; W.ncol = isl_ncol.p; W.serial = isl_serial.p;
 	W.eoff = isl_eoff.p; W.bod = isl_bod.p; W.nbod = isl_nbod.p; W.loc = ib_loc.p; W.cloc = isl_cloc.p; W.jloc = isl_jloc.p;
@@ -1638,8 +1705,10 @@ void cp_world::step() {
 		hs_scan = false;
 	}
 	if (joints_dirty) upload_joints();
+	if (drive_dirty) upload_drives();
 	Params P = params();
 	Bodies b = bodies();
+	init_new_joints(b);
 	if (timing) HIPCHK(hipEventRecord(ev[0], stream));
 	if (sl_active) {
 		SleepDev SD = sleep_dev();
@@ -1836,7 +1905,7 @@ void cp_world::apply_resumes() {
 }
 
 namespace {
-const char SNAP_MAGIC[8] = {'C', 'P', 'S', 'N', 'A', 'P', '1', '\n'};
+const char SNAP_MAGIC[8] = {'C', 'P', 'S', 'N', 'A', 'P', '2', '\n'};
 uint64_t fnv1a(uint64_t h, const void* p, size_t n) {
 	const unsigned char* b = static_cast<const unsigned char*>(p);
 	for (size_t i = 0; i < n; i++) {
@@ -1847,8 +1916,8 @@ uint64_t fnv1a(uint64_t h, const void* p, size_t n) {
 }
 struct SnapFile {  // a snapshot as written: counts, then the sections in file order
 	int64_t counts[4] = {0, 0, 0, 0};  // bodies, contacts, joints, sleep (0 none, else history slots + 1)
-	std::vector<double> state, inertia, cf, joint, sl_counter, sl_hist;
-	std::vector<int64_t> ci, sl_ints;
+	std::vector<double> state, inertia, cf, joint, jdrive, jrow, jpimp, sl_counter, sl_hist;
+	std::vector<int64_t> ci, jproj, sl_ints;
 	std::vector<uint64_t> cm;
 };
 }  // namespace
@@ -2458,8 +2527,8 @@ int cp_world_set_node_order(cp_world* w, int n, const int64_t* ids) {
 }
 
 int cp_joint_add(cp_world* w, int type, int b1, int b2, const double* geo14, int* joint_index) {
-	static const int degree[10] = {0, 1, 0, 3, 4, 5, 6, 5, 1, 1};  // rows per kind (ConstraintSolver.h:193-398)
-	if (!w || !geo14 || !joint_index || type < 1 || type > 9 || degree[type] == 0) return CP_ERR_INVALID;
+	static const int degree[13] = {0, 1, 0, 3, 4, 5, 6, 5, 1, 1, 1, 1, 1};  // rows per kind (ConstraintSolver.h:193-398)
+	if (!w || !geo14 || !joint_index || type < 1 || type > 12 || degree[type] == 0) return CP_ERR_INVALID;
 	if (b1 < 0 || b2 < 0 || b1 >= w->nb || b2 >= w->nb || b1 == b2 || w->b_scene[b1] != w->b_scene[b2]) return CP_ERR_INVALID;
 	for (int c = 0; c < 14; c++)
 		if (!std::isfinite(geo14[c])) return CP_ERR_INVALID;
@@ -2469,10 +2538,12 @@ int cp_joint_add(cp_world* w, int type, int b1, int b2, const double* geo14, int
 	w->j_a.push_back(b1);
 	w->j_b.push_back(b2);
 	w->j_geo.insert(w->j_geo.end(), geo14, geo14 + 14);
+	w->j_drive.insert(w->j_drive.end(), {0.0, 0.0, 0.0, 0.0});  // powered mode off (the Motor / Piston constructors)
 	uint64_t key = (uint64_t)std::min(b1, b2) << 32 | (uint64_t)std::max(b1, b2);
-	// limits are not holonomic: they join the pair's edge and leave the
-	// holonomic systems as they are (PhysicsEngine.cpp:926-970)
-	const bool holonomic = !joint_is_limit(type);
+	// limits, motors and pistons are not holonomic: they join the pair's edge
+	// and leave the holonomic systems as they are (PhysicsEngine.cpp:725-760,
+	// :808-846, :926-970)
+	const bool holonomic = !joint_is_inequality(type);
 	auto it = w->jedges.find(key);
 	if (it == w->jedges.end()) {
 		cp_world::JEdge e;
@@ -2496,6 +2567,26 @@ int cp_joint_add(cp_world* w, int type, int b1, int b2, const double* geo14, int
 	return CP_OK;
 }
 
+int cp_joint_set_drive(cp_world* w, int k, int mode, double max, double target) {
+	if (!w || k < 0 || k >= (int)w->j_type.size() || mode < 0 || mode > 3) return CP_ERR_INVALID;
+	if (w->j_kind[k] != CP_JOINT_MOTOR && w->j_kind[k] != CP_JOINT_PISTON_DRIVE) return CP_ERR_INVALID;
+	if (!std::isfinite(max) || !std::isfinite(target)) return CP_ERR_INVALID;
+	int st = guard(w, [&] {
+		// setMotor* / setPiston* (PhysicsEngine.cpp:1683-1912): off keeps the
+		// parameters, the other modes set the maximum and their target; both
+		// bodies receive a waking action
+		double* d = &w->j_drive[4 * (size_t)k];
+		d[0] = mode;
+		if (mode >= 1) d[1] = max;
+		if (mode == 2) d[2] = target;
+		if (mode == 3) d[3] = target;
+		w->drive_dirty = true;
+	});
+	if (st != CP_OK || !w->cfg.sleeping) return st;
+	const int ids[2] = {w->j_a[k], w->j_b[k]};
+	return cp_world_wake_bodies(w, 2, ids);
+}
+
 int cp_world_joint_count(cp_world* w, int* n) {
 	if (!w || !n) return CP_ERR_INVALID;
 	*n = (int)w->j_type.size();
@@ -2507,6 +2598,7 @@ int cp_world_set_joint_state(cp_world* w, const double* s12) {
 	return guard(w, [&] {
 		if (w->joints_dirty || w->j_dev_n != (int)w->j_type.size()) w->upload_joints();
 		if (w->j_dev_n) w->write_joint_state(s12);
+		w->j_init_from = w->j_dev_n;  // the caller's state replaces the constructors'
 	});
 }
 
@@ -2514,6 +2606,7 @@ int cp_world_get_joint_state(cp_world* w, double* s12) {
 	if (!w || (!s12 && !w->j_type.empty())) return CP_ERR_INVALID;
 	return guard(w, [&] {
 		if (w->joints_dirty || w->j_dev_n != (int)w->j_type.size()) w->upload_joints();
+		w->init_new_joints(w->bodies());
 		HIPCHK(hipStreamSynchronize(w->stream));
 		if (w->j_dev_n) w->read_joint_state(s12, w->j_dev_n);
 	});
@@ -2546,7 +2639,7 @@ int cp_world_set_holonomic_systems(cp_world* w, int n_sys, const int64_t* off, c
 		if (off[q] > off[q + 1] || (q == 0 && off[0] != 0)) return CP_ERR_INVALID;
 		for (int64_t i = off[q]; i < off[q + 1]; i++) {
 			if (joints[i] < 0 || joints[i] >= K || used[joints[i]]) return CP_ERR_INVALID;
-			if (joint_is_limit(w->j_kind[joints[i]])) return CP_ERR_INVALID;  // limits are not holonomic
+			if (joint_is_inequality(w->j_kind[joints[i]])) return CP_ERR_INVALID;  // inequality kinds are not holonomic
 			used[joints[i]] = 1;
 			og[q].push_back((int)joints[i]);
 		}
@@ -2614,6 +2707,20 @@ int cp_world_save_snapshot(cp_world* w, const char* path) {
 	if (nc && (st = cp_world_get_contacts(w, nc, f.ci.data(), f.cm.data(), f.cf.data(), &nc)) != CP_OK) return st;
 	f.joint.resize(CP_JSTATE * (size_t)nj);
 	if (nj && (st = cp_world_get_joint_state(w, f.joint.data())) != CP_OK) return st;
+	if (nj) {
+		// powered modes, and the rows, projections and pseudo impulses an
+		// inequality joint solves with when its update leaves them standing
+		f.jdrive = w->j_drive;
+		st = guard(w, [&] {
+			HIPCHK(hipStreamSynchronize(w->stream));
+			down(f.jrow, w->dj_row.p, (size_t)CP_JRS * nj);
+			down(f.jpimp, w->dj_pimp.p, (size_t)CP_JMAX * nj);
+			std::vector<int> pj;
+			down(pj, w->dj_proj.p, nj);
+			f.jproj.assign(pj.begin(), pj.end());
+		});
+		if (st != CP_OK) return st;
+	}
 	f.counts[0] = nb;
 	f.counts[1] = nc;
 	f.counts[2] = nj;
@@ -2655,6 +2762,10 @@ int cp_world_save_snapshot(cp_world* w, const char* path) {
 	put(f.cm.data(), f.cm.size() * 8);
 	put(f.cf.data(), f.cf.size() * 8);
 	put(f.joint.data(), f.joint.size() * 8);
+	put(f.jdrive.data(), f.jdrive.size() * 8);
+	put(f.jproj.data(), f.jproj.size() * 8);
+	put(f.jrow.data(), f.jrow.size() * 8);
+	put(f.jpimp.data(), f.jpimp.size() * 8);
 	put(f.sl_ints.data(), f.sl_ints.size() * 8);
 	put(f.sl_counter.data(), f.sl_counter.size() * 8);
 	put(f.sl_hist.data(), f.sl_hist.size() * 8);
@@ -2686,12 +2797,17 @@ int cp_world_load_snapshot(cp_world* w, const char* path) {
 		f.state.resize(13 * nb); f.inertia.resize(18 * nb);
 		f.ci.resize(6 * nc); f.cm.resize(2 * nc); f.cf.resize(21 * nc);
 		f.joint.resize(CP_JSTATE * nj);
+		f.jdrive.resize(4 * nj); f.jproj.resize(nj); f.jrow.resize((size_t)CP_JRS * nj); f.jpimp.resize((size_t)CP_JMAX * nj);
 		get(f.state.data(), f.state.size() * 8);
 		get(f.inertia.data(), f.inertia.size() * 8);
 		get(f.ci.data(), f.ci.size() * 8);
 		get(f.cm.data(), f.cm.size() * 8);
 		get(f.cf.data(), f.cf.size() * 8);
 		get(f.joint.data(), f.joint.size() * 8);
+		get(f.jdrive.data(), f.jdrive.size() * 8);
+		get(f.jproj.data(), f.jproj.size() * 8);
+		get(f.jrow.data(), f.jrow.size() * 8);
+		get(f.jpimp.data(), f.jpimp.size() * 8);
 		if (f.counts[3] > 0) {
 			f.sl_ints.resize(6 * nb);
 			f.sl_counter.resize(nb);
@@ -2708,6 +2824,15 @@ int cp_world_load_snapshot(cp_world* w, const char* path) {
 	int st = cp_world_set_state(w, f.state.data(), f.inertia.data());
 	if (st == CP_OK) st = cp_world_set_contacts(w, (int)f.counts[1], f.ci.data(), f.cm.data(), f.cf.data());
 	if (st == CP_OK && f.counts[2] > 0) st = cp_world_set_joint_state(w, f.joint.data());
+	if (st == CP_OK && f.counts[2] > 0)
+		st = guard(w, [&] {
+			w->j_drive = f.jdrive;
+			w->upload_drives();
+			std::vector<int> pj(f.jproj.begin(), f.jproj.end());
+			up(w->dj_proj, pj);
+			up(w->dj_pimp, f.jpimp);
+			HIPCHK(hipMemcpy(w->dj_row.p, f.jrow.data(), f.jrow.size() * 8, hipMemcpyHostToDevice));
+		});
 	if (st == CP_OK && f.counts[3] > 0)
 		st = guard(w, [&] {
 			const int nb = w->nb, hc = (int)f.counts[3] - 1;

@@ -235,8 +235,19 @@ typedef enum {
 	CP_JOINT_WELD = 6,          /* addWeldConstraint, :896-920; 6 rows */
 	CP_JOINT_SLIDER = 7,        /* addSliderConstraint, :773-806; 5 rows */
 	CP_JOINT_CONE_LIMIT = 8,    /* addConeLimitConstraint, :926-945; 1 projected row while active */
-	CP_JOINT_TWIST_LIMIT = 9    /* addTwistLimitConstraint, :952-970; 1 projected row while active */
+	CP_JOINT_TWIST_LIMIT = 9,   /* addTwistLimitConstraint, :952-970; 1 projected row while active */
+	CP_JOINT_MOTOR = 10,        /* addMotorConstraint, :725-760; 1 row while powered or outside its angle range */
+	CP_JOINT_PISTON_DRIVE = 11, /* addPistonConstraint, :808-846: the piston's drive, 1 row while powered */
+	CP_JOINT_SLIDE_LIMIT = 12   /* addPistonConstraint: the piston's slide limit, 1 row while exceeded */
 } cp_joint_kind;
+
+/* Powered modes of motors and piston drives (PoweredConstraitMode). */
+typedef enum {
+	CP_DRIVE_OFF = 0,
+	CP_DRIVE_CONSTANT = 1,        /* constant torque / force: max */
+	CP_DRIVE_TARGET_VELOCITY = 2, /* max torque / force, target velocity */
+	CP_DRIVE_TARGET_POSITION = 3  /* max torque / force, target angle / position */
+} cp_drive_mode;
 
 /* Joints between bodies b1 and b2 of one scene; the pair stops colliding
  * (disallowCollision).  geo14 holds the joint as the reference stores it:
@@ -251,21 +262,42 @@ typedef enum {
  * in the first attach-point slots (cone: the maximum angle; twist: the
  * minimum and maximum angles), the body-frame directions in the axis slots,
  * 0 as position strength and the rotation hardness as rotation strength.
+ * A motor (on a hinge or sliding hinge: Motor takes the base joint's b1 and
+ * b2 rotation axes and rotation hardness) holds its minimum and maximum
+ * angles in the first attach-point slots and the two body-frame axes.  A
+ * piston (on a slider or sliding hinge) is two joints, added in this order
+ * (the order Piston::addSolverConstraints adds its rows): its slide limit
+ * (12) and its drive (11), each with the base joint's attach points, b1's
+ * slide axis and position strength; the slide limit holds the minimum and
+ * maximum positions in the second axis slot.
  * Joints are solved in the island modes; a step of a world with joints in
  * CP_PGS_COLORED mode returns CP_ERR_STATE.  Kinds 1-7 are holonomic
- * constraints (isHolonomicConstraint) and join holonomic systems; the limits
- * are inequality rows outside them: each step a limit first re-evaluates its
- * angle (ConeRotationLimit / TwistLimit::updateSolverConstraints,
+ * constraints (isHolonomicConstraint) and join holonomic systems; kinds
+ * 8-12 are inequality rows outside them.  Each substep a limit re-evaluates
+ * its angle (ConeRotationLimit / TwistLimit::updateSolverConstraints,
  * PersistentConstraint.cpp:233-272) and, inside its range, clears its
- * impulse; an island holds a limit's row when the limit was active at its
- * previous evaluation. */
+ * impulse; a motor re-measures its angle and a piston its position (:300-329).
+ * An island holds an inequality row when the joint was active at its
+ * previous evaluation (a motor or drive also by its current powered mode). */
 int cp_joint_add(cp_world* w, int type, int b1, int b2, const double* geo14, int* joint_index);
 int cp_world_joint_count(cp_world* w, int* n);
+/* Powered mode of motor or piston drive k: setMotorOff / ConstantTorque /
+ * TargetVelocity / TargetPosition and setPistonOff / ConstantForce /
+ * TargetVelocity / TargetPosition (PhysicsEngine.cpp:1683-1912).  Off keeps
+ * the other parameters; the other modes set max and, for the targets,
+ * target.  With sleeping on, both bodies receive a waking action.  A new
+ * motor or drive is off. */
+int cp_joint_set_drive(cp_world* w, int k, int mode, double max, double target);
 /* Joint solver state, 12 doubles per joint: accumulated impulse (6, unused
  * components 0), basis u and w of the current rows of hinges, sliders and
  * sliding hinges (0 for the others) (the PersistentConstraint ::constraint
  * objects, ConstraintSolver.h); a limit holds is_inactive (1 or 0) and, for a
- * twist limit, its tracked current_angle where u is (a new limit: 1, 0).
+ * twist limit, its tracked current_angle where u is (a new limit: 1, 0); a
+ * motor its angular position and prev_velocity, a piston drive its position
+ * and prev_velocity, a slide limit !slide_limit_exceeded and the position
+ * (getMotorAngularPosition / getPistonPosition, PhysicsEngine.cpp:1914-1930).
+ * A new motor or piston measures its position from the bodies at the next
+ * step or state read (the Motor / Piston constructors).
  * Setting it marks the holonomic systems for a rebuild of their rows and
  * factors at the next step. */
 int cp_world_set_joint_state(cp_world* w, const double* state12);
@@ -382,18 +414,21 @@ int cp_world_serving_info(cp_world* w, int* info4);
  *
  * cp_world_save_snapshot / cp_world_load_snapshot: the whole dynamic state of
  *   a world in one file, enough to resume it bit for bit in a world built
- *   with the same bodies (and joints): "CPSNAP1\n", int64 counts[4] =
+ *   with the same bodies (and joints): "CPSNAP2\n", int64 counts[4] =
  *   (bodies, contacts, joints, sleep), then state13[bodies], inertia18[bodies],
  *   contact rows (ints6, magic2, f21 per contact, the cp_world_get_contacts
- *   layout), joint state11[joints]; when sleep = H + 1 > 0 (the world has
+ *   layout), joint state12[joints], powered modes [joints][4] (mode, max,
+ *   target velocity, target position), then the solver objects a joint's
+ *   update may leave standing: int64 row projection [joints], rows
+ *   [joints][196] and pseudo impulses [6][joints]; when sleep = H + 1 > 0 (the world has
  *   had sleeping on), the sleep state: int64 [bodies][6] (asleep, asleep at
  *   the last contact removal, non_sleepy_tick_count, history length, history
  *   head, sleep disabled), sleep_ready_counter[bodies], and the movement
  *   history ring as doubles [H][6][bodies]; then a 64-bit FNV-1a checksum of
  *   every byte before it; all little-endian.  Loading checks the magic, the
  *   checksum and the body and joint counts (CP_ERR_IO otherwise) and sets
- *   state, inertia, contacts, joint state and sleep state (pending waking
- *   actions are not saved).
+ *   state, inertia, contacts, joint state and solver objects, powered modes
+ *   and sleep state (pending waking actions are not saved).
  * cp_world_trajectory_open: from the next cp_world_step on, appends one frame
  *   per step to `path` (created or truncated): per body, in index order,
  *   com xyz and orientation r i j k as doubles after the step.
