@@ -1614,8 +1614,8 @@ static double posCorrectCoeff(double s, double dt) { return std::min<double>(s *
 
 struct Joint {
 	int kind;       // 1 distance, 3 ball-socket, 4 sliding hinge, 5 hinge, 6 weld, 7 slider, 8 cone limit, 9 twist limit,
-	                // 10 motor, 11 piston drive, 12 piston slide limit
-	int type;       // degree (rows): 1, 3, 4, 5, 6, 5, 1, 1, 1, 1, 1
+	                // 10 motor, 11 piston drive, 12 piston slide limit, 13 spring
+	int type;       // degree (rows): 1, 3, 4, 5, 6, 5, 1, 1, 1, 1, 1, 0
 	int a, b;       // body indices (b1, b2 as created)
 	V3 tp1, tp2;    // tracked points in the body frames (RigidBody.cpp:134-144)
 	V3 ax1, ax2;    // hinge / slide axes in the body frames; distance: ax1.x = target distance
@@ -1642,7 +1642,10 @@ struct Joint {
 	int lstat = 0;        // limit status of the last build: 0 none, 1 below the minimum, 2 above the maximum
 	double bound = 0;     // max_torque_impulse / max_impulse of the last build
 	double tv = 0;        // the target velocity of the last build (real_target_velocity)
+	double rest = 0;      // a spring's resting length
+	double ccfm = std::numeric_limits<double>::quiet_NaN();  // setConstraintUseCustomCFM, NaN: the global CFM (CFM.h:5-12)
 	bool active() const {
+		if (kind == 13) return false;  // springs add no row (Spring::isSolverConstraint)
 		if (kind == 10) return (mode != 0 && dmax != 0) || jpos < tp1.x || jpos > tp1.y;  // Motor::constraintIsActive (:110-112)
 		if (kind == 11) return mode != 0 && dmax != 0;                                    // Piston::pistonIsActive (:195-197)
 		return kind < 8 || !inactive;  // limits while active; slide limit while exceeded
@@ -1666,6 +1669,7 @@ struct World {
 	std::vector<Joint> joints;
 	std::vector<HSys> systems;
 	std::set<std::pair<int, int>> nocol;  // disallowCollision pairs (lo, hi body index)
+	std::vector<int> node_order;          // island roots for steps given none (body indices)
 	std::vector<Contact> cache;  // grouped by edge (b1 < b2 by id), edge order = insertion
 	uint64_t seq = 0;
 	std::vector<int64_t> last_rows, last_offs;
@@ -1929,15 +1933,18 @@ static void jointInitState(World& W, Joint& j) {
 	const Body& A = W.bodies[j.a];
 	const Body& B = W.bodies[j.b];
 	if (j.kind == 10) j.jpos = motorPosition(j, A, B, A.q.rotate(j.ax1), V3(), V3(), 0);
+	if (j.kind == 13)  // addSpring (PhysicsEngine.cpp:978): a negative length takes the current distance
+		j.rest = j.ax1.z >= 0 ? j.ax1.z : ((A.com + A.q.rotate(j.tp1)) - (B.com + B.q.rotate(j.tp2))).mag();
 	if (j.kind == 11 || j.kind == 12) j.jpos = pistonPosition(A.com + A.q.rotate(j.tp1), B.com + B.q.rotate(j.tp2), A.q.rotate(j.ax1));
 }
 
 static void buildJoint(World& W, Joint& j, double sub_dt, bool reproject) {
+	if (j.kind == 13) return;  // springs act through applySprings
 	const Body& A = W.bodies[j.a];
 	const Body& B = W.bodies[j.b];
 	V3 pa = A.com + A.q.rotate(j.tp1);  // getTrackedP
 	V3 pb = B.com + B.q.rotate(j.tp2);
-	double cfm = W.cfg.cfm;
+	double cfm = std::isnan(j.ccfm) ? W.cfg.cfm : j.ccfm;  // cfm.getCFMValue(global_cfm)
 	int n = j.type;
 	double start[6];
 	for (int k = 0; k < n; k++) start[k] = j.imp[k] * 1.0;  // warm_start_coefficient * impulse
@@ -2900,6 +2907,57 @@ static void wakeIsland(World& W, int i) {
 	}
 }
 
+// RigidBody::applyImpulse (RigidBody.cpp:60-66).  Its waking action only
+// matters with sleeping on, which worlds with springs do not use here.
+static void applyImpulse(Body& b, V3 imp, V3 pos) {
+	V3 torque = (pos - b.com).cross(imp);
+	b.vel += b.invMass() * imp;
+	b.ang_vel += b.invTensor() * torque;
+}
+
+// Spring forces at a substep's start (PhysicsEngine.cpp:1313-1355), in the
+// constraint graph's edge order: nodes in node-map order, a node's edges by
+// the other body's id (insertNewEdge, :1984-1995), each edge once, an edge's
+// springs in creation order.  getVelOfPoint (RigidBody.cpp:153-155) is taken
+// at the other body's attach point, as the reference does.
+static void applySprings(World& W, const std::vector<int>& roots, double dt) {
+	std::vector<int> sp;
+	for (size_t k = 0; k < W.joints.size(); k++)
+		if (W.joints[k].kind == 13) sp.push_back((int)k);
+	if (sp.empty()) return;
+	std::vector<int> rank(W.bodies.size());
+	for (size_t p = 0; p < roots.size(); p++) rank[roots[p]] = (int)p;
+	auto key = [&](int k) {
+		const Joint& j = W.joints[k];
+		int first = rank[j.a] < rank[j.b] ? j.a : j.b, other = first == j.a ? j.b : j.a;
+		return std::make_pair(rank[first], other);
+	};
+	std::stable_sort(sp.begin(), sp.end(), [&](int x, int y) { return key(x) < key(y); });
+	for (int k : sp) {
+		const Joint& j = W.joints[k];
+		Body& B1 = W.bodies[j.a];
+		Body& B2 = W.bodies[j.b];
+		const double stiffness = j.ax1.x, damping = j.ax1.y;
+		V3 p1 = B1.com + B1.q.rotate(j.tp1), p2 = B2.com + B2.q.rotate(j.tp2);
+		V3 diff = p2 - p1;
+		double distance = diff.mag();
+		V3 dir = (distance != 0) ? (p2 - p1).normalize() : V3(0, -1, 0);
+		double expand = stiffness * (distance - j.rest);
+		applyImpulse(B1, dir * expand * dt, p1);
+		applyImpulse(B2, -dir * expand * dt, p2);
+		V3 rA = p1 - B1.com, rB = p2 - B2.com;
+		V3 rdA = B1.invTensor() * dir.cross(rA), rdB = B2.invTensor() * dir.cross(rB);
+		double sens = 1.0 / (B1.invMass() + B2.invMass() + rA.cross(rdA).dot(dir) + rB.cross(rdB).dot(dir));
+		double velocity = (B2.vel + B2.ang_vel.cross(p1 - B2.com)).dot(dir) - (B1.vel + B1.ang_vel.cross(p2 - B1.com)).dot(dir);
+		double dampen = damping * velocity;
+		double max_dampen = velocity * sens / dt;
+		if (velocity > 0 && dampen > max_dampen) dampen = max_dampen;
+		else if (velocity < 0 && dampen < max_dampen) dampen = max_dampen;
+		applyImpulse(B1, dir * dampen * dt, p1);
+		applyImpulse(B2, -dir * dampen * dt, p2);
+	}
+}
+
 // Returns 0, or -1 when a schedule's groups do not match the islands of a
 // sleeping world.
 static int step(World& W, const int64_t* node_order, const int64_t* sched_rows = nullptr, const int64_t* sched_offs = nullptr,
@@ -2985,13 +3043,16 @@ static int step(World& W, const int64_t* node_order, const int64_t* sched_rows =
 		});
 	std::vector<std::vector<std::pair<Contact*, int>>> islands;
 	std::vector<std::vector<int>> isl_sys, isl_bodies;
+	std::vector<int> spring_roots;
 	W.last_rows.clear();
 	W.last_offs.assign(1, 0);
 	{
 		std::vector<char> visited(N, 0);
 		std::vector<int> roots;
 		if (node_order) for (int k = 0; k < N; k++) roots.push_back((int)node_order[k] - 1);
+		else if ((int)W.node_order.size() == N) roots = W.node_order;
 		else for (int k = 0; k < N; k++) roots.push_back(k);
+		spring_roots = roots;
 		for (int root : roots) {
 			if (!W.bodies[root].dynamic() || visited[root]) continue;
 			std::vector<std::pair<Contact*, int>> rows;
@@ -3099,6 +3160,7 @@ static int step(World& W, const int64_t* node_order, const int64_t* sched_rows =
 	for (int s = 0; s < S; s++) {
 		double sdt = dt / S;
 		bool first = s == 0, last = s == S - 1;
+		applySprings(W, spring_roots, sdt);
 		// maintainConstraintGraphApplyPoweredConstraints (PhysicsEngine.cpp:1378-1425)
 		for (Contact& c : W.cache) {
 			bool skip = !c.live;
@@ -3287,8 +3349,8 @@ int orc_add_cylinder_body(orc_world* w, int movement, const double* mp, const do
 
 int orc_add_joint(orc_world* w, int type, int b1, int b2, const double* g) {
 	Joint j;
-	static const int degree[13] = {0, 1, 0, 3, 4, 5, 6, 5, 1, 1, 1, 1, 1};
-	if (type < 1 || type > 12 || degree[type] == 0) return -1;
+	static const int degree[14] = {0, 1, 0, 3, 4, 5, 6, 5, 1, 1, 1, 1, 1, 0};
+	if (type < 1 || type > 13 || (degree[type] == 0 && type != 13)) return -1;
 	j.kind = type;
 	j.type = degree[type];
 	j.a = b1;
@@ -3300,7 +3362,7 @@ int orc_add_joint(orc_world* w, int type, int b1, int b2, const double* g) {
 	j.ps = g[12];
 	j.rs = g[13];
 	w->joints.push_back(j);
-	w->nocol.insert({std::min(b1, b2), std::max(b1, b2)});
+	if (type != 13) w->nocol.insert({std::min(b1, b2), std::max(b1, b2)});  // addSpring allows the pair to collide
 	return (int)w->joints.size() - 1;
 }
 
@@ -3392,6 +3454,10 @@ void orc_set_joint_state(orc_world* w, const double* s) {
 			j.lstat = j.jpos > j.ax2.y ? 2 : 1;
 			continue;
 		}
+		if (j.kind == 13) {  // spring: resting length
+			j.rest = r[6];
+			continue;
+		}
 		j.u = V3(r[6], r[7], r[8]);
 		j.w = V3(r[9], r[10], r[11]);
 	}
@@ -3412,11 +3478,19 @@ void orc_get_joint_state(const orc_world* w, double* s) {
 				r[7] = j.prev_vel;
 			}
 			if (j.kind == 12) r[7] = j.jpos;
+			if (j.kind == 13) r[6] = j.rest;
 			continue;
 		}
 		r[6] = j.u.x; r[7] = j.u.y; r[8] = j.u.z;
 		r[9] = j.w.x; r[10] = j.w.y; r[11] = j.w.z;
 	}
+}
+
+void orc_set_joint_cfm(orc_world* w, int k, double cfm) { w->joints[k].ccfm = cfm; }
+
+void orc_set_node_order(orc_world* w, int n, const int64_t* ids) {
+	w->node_order.clear();
+	for (int k = 0; k < n; k++) w->node_order.push_back((int)ids[k] - 1);
 }
 
 void orc_init_joint_state(orc_world* w) {

@@ -82,6 +82,10 @@ int orc_narrowphase_multi(const orc_world* w, int id_lo, int id_hi, int cap, dou
 // strengths.  The pair stops colliding.  Returns the joint index.
 int orc_add_joint(orc_world* w, int type, int b1, int b2, const double* geo14);
 // Joint solver state rows of 11 doubles: impulse (5, zero padded), hinge u, w.
+/* Custom CFM of joint k (setConstraintUseCustomCFM); NaN: the global CFM. */
+void orc_set_joint_cfm(orc_world* w, int k, double cfm);
+/* Island roots (body ids) for steps given no node order. */
+void orc_set_node_order(orc_world* w, int n, const int64_t* ids);
 /* Joint state as the reference's constructors leave it, from the current bodies. */
 void orc_init_joint_state(orc_world* w);
 /* Powered mode of a motor (10) or piston drive (11): 0 off, 1 constant

@@ -223,13 +223,16 @@ static void build_mixed(PhysicsEngine& E, int nx, int ny, int nz, uint64_t seed,
 // Joint kinds as the dumps number them: 1 distance, 3 ball-socket, 4 sliding
 // hinge, 5 hinge, 6 weld, 7 slider, 8 cone limit, 9 twist limit, 10 motor,
 // 11 piston drive, 12 piston slide limit (a piston is recorded as its slide
-// limit, then its drive: the order Piston::addSolverConstraints adds them).
+// limit, then its drive: the order Piston::addSolverConstraints adds them),
+// 13 spring.
 struct JointRec {
 	int type;
 	RigidBody *b1, *b2;
 	phyz::PersistentConstraint* pc;
 	phyz::Constraint* solver;
-	mthz::Vec3 aux = mthz::Vec3();  // a motor: b2's rotation axis of its base hinge (the motor keeps only what it yields)
+	mthz::Vec3 aux = mthz::Vec3();  // a motor: b2's rotation axis of its base hinge (the motor keeps only what it yields);
+	                                // a spring: x = the resting length as given (-1: the current distance)
+	double ccfm = std::numeric_limits<double>::quiet_NaN();  // setConstraintUseCustomCFM value, NaN: the global CFM
 };
 static std::vector<JointRec> g_joints;
 static RigidBody* g_mesh = nullptr;  // the terrain body of the terrain scene
@@ -323,6 +326,11 @@ static void joint_state(std::vector<double>& s) {
 			auto* p = static_cast<phyz::Piston*>(j.pc);
 			put_impulse(s, p->slide_limit, 1);
 			s.insert(s.end(), {p->slide_limit_exceeded ? 0.0 : 1.0, p->piston_position, 0, 0, 0, 0});
+			break;
+		}
+		case 13: {  // no solver object: the resting length
+			auto* sp = static_cast<phyz::Spring*>(j.pc);
+			s.insert(s.end(), {0, 0, 0, 0, 0, 0, sp->resting_length, 0, 0, 0, 0, 0});
 			break;
 		}
 		default: put_joint_basis(s, static_cast<phyz::Slider*>(j.pc)->constraint, 5); break;
@@ -680,6 +688,11 @@ static void dump_static(PhysicsEngine& E, Out& o) {
 			                     p->pos_correct_hardness, 0});
 			break;
 		}
+		case 13: {  // axis slots: stiffness, damping, resting length as given
+			auto* sp = static_cast<phyz::Spring*>(j.pc);
+			geo(j, sp->b1_point_key, sp->b2_point_key, mthz::Vec3(sp->stiffness, sp->damping, j.aux.x), mthz::Vec3(), 0, 0);
+			break;
+		}
 		case 6: {
 			auto* w = static_cast<phyz::Weld*>(j.pc);
 			geo(j, w->b1_point_key, w->b2_point_key, mthz::Vec3(), mthz::Vec3(), w->pos_correct_hardness, w->rot_correct_hardness);
@@ -695,6 +708,9 @@ static void dump_static(PhysicsEngine& E, Out& o) {
 	}
 	o.i64("joint_info", ji, {g_joints.size(), 3});
 	o.f64("joint_geo", jg, {g_joints.size(), 14});
+	std::vector<double> jc;
+	for (auto& j : g_joints) jc.push_back(j.ccfm);
+	o.f64("joint_cfm", jc, {g_joints.size()});
 }
 
 static void put_cache(Out& o, const std::string& p, const CacheDump& c) {
@@ -1259,6 +1275,73 @@ static int cmd_powered(int argc, char** argv) {
 	return 0;
 }
 
+// springs: a box hanging from a fixed anchor by a spring (resting length the
+// current distance), a second box hanging from it by a spring of resting
+// length 1.0 that lands on the ground, and two boxes on the ground pushed
+// apart by a compressed spring; a three-link ball-socket chain from a fixed
+// anchor (one holonomic system) whose middle joint has a custom CFM, and a
+// hinged block with a custom CFM.
+static int cmd_springs(int argc, char** argv) {
+	int steps = (int)arg_d(argc, argv, "--steps", 200);
+	SceneCfg cfg;
+	cfg.hol_iters = (int)arg_d(argc, argv, "--hol", 1);
+	PhysicsEngine E;
+	configure(E, cfg);
+	phyz::Material m = c0_material();
+	RigidBody* g = E.createRigidBody(phyz::ConvexUnionGeometry::box(mthz::Vec3(-100, -1, -100), 200, 1, 200, m), RigidBody::FIXED);
+	pin_body(g);
+	auto spring = [&](RigidBody* b1, RigidBody* b2, mthz::Vec3 p1, mthz::Vec3 p2, double damping, double stiffness, double rest) {
+		phyz::ConstraintID id = E.addSpring(b1, b2, p1, p2, damping, stiffness, rest);
+		g_joints.push_back({13, b1, b2, find_constraint(E, id), nullptr, mthz::Vec3(rest, 0, 0)});
+	};
+	RigidBody* a = E.createRigidBody(phyz::ConvexUnionGeometry::box(mthz::Vec3(-0.5, 6.0, -0.5), 1, 1, 1, m), RigidBody::FIXED);
+	RigidBody* s1 = E.createRigidBody(phyz::ConvexUnionGeometry::box(mthz::Vec3(-0.3, 3.9, -0.3), 0.6, 0.6, 0.6, m));
+	RigidBody* s2 = E.createRigidBody(phyz::ConvexUnionGeometry::box(mthz::Vec3(-0.2, 2.2, 0.0), 0.4, 0.4, 0.4, m));
+	pin_body(a);
+	pin_body(s1);
+	pin_body(s2);
+	s1->setVel(mthz::Vec3(0.4, 0, 0.2));
+	spring(a, s1, mthz::Vec3(0, 6.0, 0), mthz::Vec3(0.1, 4.5, 0), 2.0, 60.0, -1);
+	spring(s1, s2, mthz::Vec3(0, 3.9, 0), mthz::Vec3(0, 2.6, 0.2), 0.5, 40.0, 1.0);
+	RigidBody* f1 = E.createRigidBody(phyz::ConvexUnionGeometry::box(mthz::Vec3(3.0, 0.0, -0.4), 0.8, 0.8, 0.8, m));
+	RigidBody* f2 = E.createRigidBody(phyz::ConvexUnionGeometry::box(mthz::Vec3(4.2, 0.0, -0.4), 0.8, 0.8, 0.8, m));
+	pin_body(f1);
+	pin_body(f2);
+	spring(f1, f2, mthz::Vec3(3.8, 0.4, 0), mthz::Vec3(4.2, 0.4, 0), 0.2, 80.0, 1.2);
+	// custom CFM on joints
+	RigidBody* c0 = E.createRigidBody(phyz::ConvexUnionGeometry::box(mthz::Vec3(-6.5, 7.5, -0.5), 1, 1, 1, m), RigidBody::FIXED);
+	pin_body(c0);
+	RigidBody* prev = c0;
+	double y = 7.5;
+	for (int i = 0; i < 3; i++) {
+		RigidBody* l = E.createRigidBody(phyz::ConvexUnionGeometry::box(mthz::Vec3(-6.2 + 0.5 * i, y - 1.0, -0.2), 0.4, 1.0, 0.4, m));
+		pin_body(l);
+		l->setVel(mthz::Vec3(1.0, 0, 0.5));
+		phyz::ConstraintID id = E.addBallSocketConstraint(prev, l, mthz::Vec3(-6.0 + 0.5 * i, y, 0), mthz::Vec3(-6.0 + 0.5 * i, y, 0));
+		record_joint<phyz::BallSocket>(E, id, 3, prev, l);
+		if (i == 1) {
+			E.setConstraintUseCustomCFM(id, 1e-3);
+			g_joints.back().ccfm = 1e-3;
+		}
+		prev = l;
+		y -= 1.0;
+	}
+	RigidBody* ha = E.createRigidBody(phyz::ConvexUnionGeometry::box(mthz::Vec3(7.5, 4.5, -0.5), 1, 1, 1, m), RigidBody::FIXED);
+	RigidBody* hb = E.createRigidBody(phyz::ConvexUnionGeometry::box(mthz::Vec3(7.4, 3.2, -0.6), 1.2, 1.2, 1.2, m));
+	pin_body(ha);
+	pin_body(hb);
+	hb->setAngVel(mthz::Vec3(0, 2.0, 0));
+	hb->setVel(mthz::Vec3(0, 0, 1.0));
+	phyz::ConstraintID hid = E.addHingeConstraint(ha, hb, mthz::Vec3(8, 4.45, 0), mthz::Vec3(8, 4.45, 0), mthz::Vec3(0, 1, 0), mthz::Vec3(0, 1, 0));
+	record_joint<phyz::Hinge>(E, hid, 5, ha, hb);
+	E.setConstraintUseCustomCFM(hid, 5e-4);
+	g_joints.back().ccfm = 5e-4;
+	Out o(arg_s(argc, argv, "--out", "out"));
+	run_trace(E, cfg, steps, arg_steps(argc, argv, "--record"), o);
+	o.finish();
+	return 0;
+}
+
 // sleep: sleeping enabled (the engine's default parameters, every body's
 // sleep_disabled pinned to false): a 2x2x2 cube pile and a lone cube settle
 // and fall asleep, a cube launched along the ground slides to rest and falls
@@ -1473,6 +1556,7 @@ int main(int argc, char** argv) {
 	if (cmd == "joints") return cmd_joints(argc, argv);
 	if (cmd == "limits") return cmd_limits(argc, argv);
 	if (cmd == "powered") return cmd_powered(argc, argv);
+	if (cmd == "springs") return cmd_springs(argc, argv);
 	fprintf(stderr, "unknown command %s\n", cmd.c_str());
 	return 2;
 }

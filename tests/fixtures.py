@@ -152,6 +152,8 @@ def oracle_lib():
         lib.orc_add_joint.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
         lib.orc_set_joint_state.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         lib.orc_init_joint_state.argtypes = [ctypes.c_void_p]
+        lib.orc_set_joint_cfm.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
+        lib.orc_set_node_order.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
         lib.orc_set_drive.restype = ctypes.c_int
         lib.orc_set_drive.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double]
         lib.orc_get_joint_state.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
@@ -247,6 +249,9 @@ class Oracle:
                 self._keep.append(g)
                 lib.orc_add_joint(self.w, int(info[k][0]), int(info[k][1]) - 1, int(info[k][2]) - 1, g.ctypes.data)
             self.n_joints = len(info)
+            if fx is not None and "joint_cfm" in fx:  # setConstraintUseCustomCFM values (NaN: global)
+                for k, c in enumerate(fx["joint_cfm"]):
+                    lib.orc_set_joint_cfm(self.w, k, float(c))
 
     def load_joints(self, state12, hsys_off=None, hsys_og=None):
         st = _c(state12, np.float64)
@@ -254,6 +259,11 @@ class Oracle:
             off, og = _c(hsys_off, np.int64), _c(hsys_og, np.int64)
             self.lib.orc_set_hsys(self.w, len(off) - 1, off.ctypes.data, og.ctypes.data)
         self.lib.orc_set_joint_state(self.w, st.ctypes.data)
+
+    def set_node_order(self, ids):
+        """Island roots (body ids) for steps given none (step_scheduled)."""
+        a = _c(ids, np.int64)
+        self.lib.orc_set_node_order(self.w, len(a), a.ctypes.data)
 
     def init_joint_state(self):
         """Joint state as the reference's constructors leave it, from the loaded bodies."""
@@ -441,6 +451,10 @@ def world_from_fixture(fx, pgs_mode=1, joints=True, **cfg):
     if joints and "joint_info" in fx:
         for (kind, id1, id2), geo in zip(fx["joint_info"], fx["joint_geo"]):
             w.add_joint(int(kind), int(id1) - 1, int(id2) - 1, geo)
+        if "joint_cfm" in fx:
+            for k, c in enumerate(fx["joint_cfm"]):
+                if not np.isnan(c):
+                    w.set_joint_cfm(k, float(c))
     return w
 
 

@@ -8,7 +8,7 @@ ROOT=$(cd "$HERE/../.." && pwd)
 make -C "$ROOT/oracle" ref REF="${REF:-/root/reference}"
 H="$ROOT/oracle/_ref/cp_ref_harness"
 cd "$HERE"
-rm -rf pile3 c0_full mix3 cyl3 cyl4 chain3 chain3h0 terrain hulls sleep joints joints_h0 limits limits_h0 powered powered_h0
+rm -rf pile3 c0_full mix3 cyl3 cyl4 chain3 chain3h0 terrain hulls sleep joints joints_h0 limits limits_h0 powered powered_h0 springs springs_h0
 # C0 pattern, 3x3x3 cubes: full per-step dumps (state, caches, manifolds, islands).
 "$H" pile --n 3 --steps 40 --record 0,1,2,10,39 --out pile3
 # C0 in full (10x10x10 cubes): first step and a settled step (no mid/post caches).
@@ -60,3 +60,8 @@ python3 make_hull_input.py hull_input.txt 24 20240
 # off; the mode changes are dumped as drive_events.
 "$H" powered --steps 200 --record 0,1,27,29,32,40,54,70,80,89,93,120,130,141,160,199 --out powered
 "$H" powered --steps 200 --record 0,27,40,54,70,80,93,130,160,199 --hol 0 --out powered_h0
+# Springs (a box hanging from a fixed anchor, a second box hanging from it,
+# two boxes on the ground pushed apart) and joints with a custom CFM (the
+# middle joint of a ball-socket chain in a holonomic system, a hinge).
+"$H" springs --steps 200 --record 0,1,10,50,99,100,150,199 --out springs
+"$H" springs --steps 200 --record 0,1,50,100,199 --hol 0 --out springs_h0

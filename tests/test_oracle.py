@@ -99,7 +99,8 @@ def test_mixed_primitives_full_step(name, k):
     assert np.array_equal(pi, gi) and np.array_equal(pm, gm) and np.array_equal(pf, gf)
 
 
-@pytest.mark.parametrize("name", ["chain3", "chain3h0", "joints", "joints_h0", "limits", "limits_h0", "powered", "powered_h0"])
+@pytest.mark.parametrize("name", ["chain3", "chain3h0", "joints", "joints_h0", "limits", "limits_h0", "powered", "powered_h0",
+                                  "springs", "springs_h0"])
 def test_jointed_chains_full_step(name):
     """Equality joints (PGS rows) and, for chain3 and joints, the holonomic
     block solver: elimination order, block table and LDL factor buffer, then
@@ -109,7 +110,9 @@ def test_jointed_chains_full_step(name):
     twist limits (inequality rows, projected) that engage and release;
     powered: motors and a piston through every powered mode, with their angle
     and slide limits (the modes in force at each step replayed from the
-    fixture's drive events)."""
+    fixture's drive events); springs: springs between hanging and resting
+    boxes (impulses at each substep's start, in the constraint graph's edge
+    order) and joints with a custom CFM."""
     import constraintphysics_amd as cpa
     fx = golden(name)
     for k in fx.recorded:
@@ -134,10 +137,11 @@ def test_jointed_chains_full_step(name):
         assert np.array_equal(o.joint_state(), fx[f"s{k}_joint_post"])
 
 
-@pytest.mark.parametrize("name", ["joints", "limits", "powered"])
+@pytest.mark.parametrize("name", ["joints", "limits", "powered", "springs"])
 def test_initial_joint_state(name):
     """Joint state as the constructors leave it (limits inactive, a motor's
-    angle measured from the bodies, a piston's position) equals the
+    angle measured from the bodies, a piston's position, a spring's default
+    resting length) equals the
     reference's state before its first step."""
     fx = golden(name)
     o = Oracle(fx)
@@ -146,7 +150,7 @@ def test_initial_joint_state(name):
     assert np.array_equal(o.joint_state(), fx["s0_joint_pre"])
 
 
-@pytest.mark.parametrize("name", ["joints", "joints_h0", "limits", "limits_h0", "powered", "powered_h0"])
+@pytest.mark.parametrize("name", ["joints", "joints_h0", "limits", "limits_h0", "powered", "powered_h0", "springs", "springs_h0"])
 def test_other_joints_whole_run(name):
     """The joints, limits and powered scenes stepped from their initial
     states by the oracle alone, powered-mode changes applied before their
