@@ -125,6 +125,7 @@ def lib():
         "cp_joint_add": (_I, [_P, _I, _I, _I, _P, _P]),
         "cp_world_joint_count": (_I, [_P, _P]),
         "cp_joint_set_drive": (_I, [_P, _I, _I, ctypes.c_double, ctypes.c_double]),
+        "cp_joint_set_cfm": (_I, [_P, _I, ctypes.c_double]),
         "cp_world_set_joint_state": (_I, [_P, _P]),
         "cp_world_get_joint_state": (_I, [_P, _P]),
         "cp_world_get_joint_rows": (_I, [_P, _I, _P, _P, _P, _P, _P]),
@@ -519,14 +520,15 @@ class World:
     def add_joint(self, kind, b1, b2, geo14):
         """Joint of kind 1 distance, 3 ball-socket, 4 sliding hinge, 5 hinge,
         6 weld, 7 slider, 8 cone limit, 9 twist limit, 10 motor, 11 piston
-        drive or 12 piston slide limit between body indices b1, b2; geo14 =
+        drive, 12 piston slide limit or 13 spring between body indices b1, b2; geo14 =
         attach points in the body frames, two body-frame axes (hinge or slide
         axes; a distance joint's target distance first), position and
         rotation correction strengths.  A cone limit holds its maximum angle,
         a twist limit and a motor their minimum and maximum angles, in the
         first attach-point slots; a slide limit its minimum and maximum
         positions in the second axis slot.  A piston is its slide limit, then
-        its drive."""
+        its drive.  A spring holds stiffness, damping and resting length (a
+        negative length: the current distance) in the first axis slot."""
         g = _a(geo14, np.float64).reshape(14)
         k = ctypes.c_int()
         _chk(self.L.cp_joint_add(self.w, int(kind), int(b1), int(b2), g.ctypes.data, ctypes.byref(k)), "cp_joint_add")
@@ -536,6 +538,11 @@ class World:
         """Powered mode of motor or piston drive k: 0 off, 1 constant torque /
         force max_, 2 target velocity, 3 target position (max_, target)."""
         _chk(self.L.cp_joint_set_drive(self.w, int(k), int(mode), float(max_), float(target)), "cp_joint_set_drive")
+
+    def set_joint_cfm(self, k, cfm):
+        """Custom constraint force mixing of joint k's rows
+        (setConstraintUseCustomCFM); NaN returns it to the world's CFM."""
+        _chk(self.L.cp_joint_set_cfm(self.w, int(k), float(cfm)), "cp_joint_set_cfm")
 
     def joint_count(self):
         n = ctypes.c_int()

@@ -87,8 +87,8 @@ struct Params {
 
 // Joints, SoA over n joints.  kind: 1 distance, 3 ball-socket, 4 sliding
 // hinge, 5 hinge, 6 weld, 7 slider, 8 cone limit, 9 twist limit, 10 motor,
-// 11 piston drive, 12 piston slide limit; type: the joint's degree (rows: 1,
-// 3, 4, 5, 6, 5, then 1 for each inequality kind).  State arrays are
+// 11 piston drive, 12 piston slide limit, 13 spring; type: the joint's degree
+// (rows: 1, 3, 4, 5, 6, 5, then 1 for each inequality kind, 0 for a spring).  State arrays are
 // component-major ([CP_JMAX][n], [3][n]).
 #define CP_JMAX 6
 #define CP_JSTATE 12  // joint state per joint at the ABI: CP_JMAX impulses, u, w
@@ -102,13 +102,16 @@ struct Joints {
 	                                // cone / twist / slide limits: u x = inactive (1.0 / 0.0); twist: u y = tracked angle;
 	                                // slide limit: u y = piston position; motor, piston drive: u x = position, u y = prev_velocity
 	double* drive;                  // [4][n] powered mode (0 off, 1 constant, 2 target velocity, 3 target position), max, target velocity, target position
+	                                // spring: ax1 = stiffness, damping, resting length as given; u x = resting length
 	int* proj;                      // the rows' impulse projection (CP_PROJ_*)
 	double* row;                    // CP_JRS doubles per joint
+	double* cfm;                    // [n] custom CFM (setConstraintUseCustomCFM), NaN: the world's
 };
 // Kinds 8-12 are one-row inequality constraints, present in an island only
 // while active and never part of a holonomic system: cone (8) and twist (9)
 // limits (PersistentConstraint.h:145-215), motor (10), and a piston's drive
-// (11) and slide limit (12) (:218-290).
+// (11) and slide limit (12) (:218-290).  A spring (13) is not holonomic either
+// and adds no row.
 __host__ __device__ __forceinline__ bool joint_is_inequality(int kind) { return kind >= 8; }
 // Projection of a row's accumulated impulse (projectValidImpulse), low bits;
 // CP_PROJ_NOPOS marks a row that the position pass skips (needsPosCorrect).
@@ -124,6 +127,7 @@ struct JointAct {
 	__device__ __forceinline__ bool operator()(int k) const {
 		const int kd = kind[k];
 		if (kd < 8) return true;
+		if (kd == 13) return false;  // springs add no row (Spring::isSolverConstraint)
 		if (kd == 10) {
 			double pos = u[k];
 			return (drive[k] != 0.0 && drive[n + k] != 0) || pos < tp1[k] || pos > tp1[n + k];
@@ -434,6 +438,10 @@ void launch_build_rows(const Bodies& b, const Contacts& C, int nc, const Params&
 // holonomic joints with warm-start re-projection, 2 the holonomic joints
 // without re-projection.
 void launch_joint_init(const Bodies& b, const Joints& J, int k0, int k1, hipStream_t s);
+// Springs at a substep's start: groups [goff[g], goff[g+1]) of list (joint
+// indices), each group in the constraint graph's edge order, groups sharing no
+// dynamic body.
+void launch_springs(const Bodies& b, const Joints& J, int ngroups, const int* goff, const int* list, double sub_dt, hipStream_t s);
 void launch_joint_rows(const Bodies& b, const Joints& J, const Params& P, double sub_dt, int mode, int first, hipStream_t s);
 // Block LDL factor of every holonomic system.
 void launch_hsys_factor(const Joints& J, const HSysDev& H, double cfm, hipStream_t s);

@@ -509,6 +509,13 @@ __global__ __launch_bounds__(64) void k_joint_init(Bodies b, Joints J, int k0, i
 	const int ia = J.a[k], ib = J.b[k];
 	Q qa{b.q[ia], b.q[N + ia], b.q[2 * N + ia], b.q[3 * N + ia]};
 	Q qb{b.q[ib], b.q[N + ib], b.q[2 * N + ib], b.q[3 * N + ib]};
+	if (kind == 13) {  // addSpring (PhysicsEngine.cpp:978): a negative length takes the current distance
+		const double given = J.ax1[2 * K + k];
+		V3 pa = jld3(b.com, ia, N) + q_rotate(qa, jld3(J.tp1, k, K));
+		V3 pb = jld3(b.com, ib, N) + q_rotate(qb, jld3(J.tp2, k, K));
+		J.u[k] = given >= 0 ? given : mag(pa - pb);
+		return;
+	}
 	if (kind == 10) {
 		V3 z{0, 0, 0};
 		J.u[k] = motor_position(J, k, qa, qb, q_rotate(qa, jld3(J.ax1, k, K)), z, z, 0, 0);
@@ -523,16 +530,80 @@ __global__ __launch_bounds__(64) void k_joint_init(Bodies b, Joints J, int k0, i
 // mode 0: every joint except holonomic ones at the first substep; 1: holonomic
 // joints with re-projection (end of step); 2: holonomic joints as stored
 // (state loaded, or systems just created).
+// A joint's rows take its own CFM when it has one (cfm.getCFMValue(global_cfm),
+// PersistentConstraint.cpp:217-329); springs have no rows.
 __global__ __launch_bounds__(64) void k_joint_rows(Bodies b, Joints J, Params P, double sub_dt, int mode, int first) {
 	int k = blockIdx.x * blockDim.x + threadIdx.x;
-	if (k >= J.n) return;
+	if (k >= J.n || J.kind[k] == 13) return;
 	bool in_sys = J.sys[k] >= 0 && P.hol_iters > 0;
+	const double own = J.cfm[k];
+	const double cfm = isnan(own) ? P.cfm : own;
 	if (mode == 0) {
 		if (first && in_sys) return;
-		build_joint(b, J, k, sub_dt, P.cfm, true);
+		build_joint(b, J, k, sub_dt, cfm, true);
 	} else {
 		if (!in_sys) return;
-		build_joint(b, J, k, sub_dt, P.cfm, mode == 1);
+		build_joint(b, J, k, sub_dt, cfm, mode == 1);
+	}
+}
+
+// ------------------------------------------------------------------- springs
+// RigidBody::applyImpulse (RigidBody.cpp:60-66) on a body whose inverse mass
+// and world inverse inertia are im, it; only dynamic bodies are written (the
+// reference adds zero products to the others).
+__device__ __forceinline__ void spring_impulse(const Bodies& b, int i, bool dyn, double im, const M3& it, V3 com, V3 imp, V3 pos) {
+	V3 torque = cross(pos - com, imp);
+	V3 dvel = im * imp, dang = it * torque;
+	if (!dyn) return;
+	const int N = b.n;
+	b.vel[i] += dvel.x; b.vel[N + i] += dvel.y; b.vel[2 * N + i] += dvel.z;
+	b.angv[i] += dang.x; b.angv[N + i] += dang.y; b.angv[2 * N + i] += dang.z;
+}
+
+// Spring forces at a substep's start (PhysicsEngine.cpp:1313-1355): one
+// thread per group of springs that share dynamic bodies, its springs in the
+// constraint graph's edge order (nodes in node-map order, a node's edges by
+// the other body's id, an edge's springs in creation order).  Each spring
+// pushes both bodies along the attach-point direction by stiffness times the
+// stretch, then damps their relative velocity along it, at most to a
+// standstill.  getVelOfPoint (RigidBody.cpp:153-155) is taken at the other
+// body's attach point, as the reference takes it.
+__global__ __launch_bounds__(64) void k_springs(Bodies b, Joints J, int ngroups, const int* goff, const int* list, double dt) {
+	const int g = blockIdx.x * blockDim.x + threadIdx.x;
+	if (g >= ngroups) return;
+	const int N = b.n, K = J.n;
+	for (int x = goff[g]; x < goff[g + 1]; x++) {
+		const int k = list[x];
+		const int i1 = J.a[k], i2 = J.b[k];
+		const bool d1 = b.movement[i1] == 0, d2 = b.movement[i2] == 0;
+		const double im1 = d1 ? 1.0 / b.mass[i1] : 0, im2 = d2 ? 1.0 / b.mass[i2] : 0;  // getInvMass (RigidBody.cpp:161-163)
+		M3 it1, it2;  // getInvTensor (:169-171)
+		for (int r = 0; r < 3; r++)
+			for (int c = 0; c < 3; c++) {
+				it1.m[r][c] = d1 ? b.Iinv[(3 * r + c) * N + i1] : 0.0;
+				it2.m[r][c] = d2 ? b.Iinv[(3 * r + c) * N + i2] : 0.0;
+			}
+		const V3 c1 = jld3(b.com, i1, N), c2 = jld3(b.com, i2, N);
+		const Q q1{b.q[i1], b.q[N + i1], b.q[2 * N + i1], b.q[3 * N + i1]};
+		const Q q2{b.q[i2], b.q[N + i2], b.q[2 * N + i2], b.q[3 * N + i2]};
+		const double stiffness = J.ax1[k], damping = J.ax1[K + k], rest = J.u[k];
+		const V3 p1 = c1 + q_rotate(q1, jld3(J.tp1, k, K)), p2 = c2 + q_rotate(q2, jld3(J.tp2, k, K));
+		const double distance = mag(p2 - p1);
+		const V3 dir = (distance != 0) ? normalize(p2 - p1) : V3{0, -1, 0};
+		const double expand = stiffness * (distance - rest);
+		spring_impulse(b, i1, d1, im1, it1, c1, dir * expand * dt, p1);
+		spring_impulse(b, i2, d2, im2, it2, c2, -dir * expand * dt, p2);
+		const V3 rA = p1 - c1, rB = p2 - c2;
+		const V3 rdA = it1 * cross(dir, rA), rdB = it2 * cross(dir, rB);
+		const double sens = 1.0 / (im1 + im2 + dot(cross(rA, rdA), dir) + dot(cross(rB, rdB), dir));
+		const V3 v1 = jld3(b.vel, i1, N), w1 = jld3(b.angv, i1, N), v2 = jld3(b.vel, i2, N), w2 = jld3(b.angv, i2, N);
+		const double velocity = dot(v2 + cross(w2, p1 - c2), dir) - dot(v1 + cross(w1, p2 - c1), dir);
+		double dampen = damping * velocity;
+		const double max_dampen = velocity * sens / dt;
+		if (velocity > 0 && dampen > max_dampen) dampen = max_dampen;
+		else if (velocity < 0 && dampen < max_dampen) dampen = max_dampen;
+		spring_impulse(b, i1, d1, im1, it1, c1, dir * dampen * dt, p1);
+		spring_impulse(b, i2, d2, im2, it2, c2, -dir * dampen * dt, p2);
 	}
 }
 
@@ -631,6 +702,9 @@ static inline int nblk(int n, int t) { return (n + t - 1) / t; }
 
 void launch_joint_init(const Bodies& b, const Joints& J, int k0, int k1, hipStream_t s) {
 	if (k1 > k0) hipLaunchKernelGGL(k_joint_init, dim3(nblk(k1 - k0, 64)), dim3(64), 0, s, b, J, k0, k1);
+}
+void launch_springs(const Bodies& b, const Joints& J, int ngroups, const int* goff, const int* list, double sub_dt, hipStream_t s) {
+	if (ngroups) hipLaunchKernelGGL(k_springs, dim3(nblk(ngroups, 64)), dim3(64), 0, s, b, J, ngroups, goff, list, sub_dt);
 }
 void launch_joint_rows(const Bodies& b, const Joints& J, const Params& P, double sub_dt, int mode, int first, hipStream_t s) {
 	if (J.n) hipLaunchKernelGGL(k_joint_rows, dim3(nblk(J.n, 64)), dim3(64), 0, s, b, J, P, sub_dt, mode, first);

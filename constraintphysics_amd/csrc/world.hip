@@ -14,6 +14,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <limits>
 #include <array>
 #include <climits>
 #include <cmath>
@@ -225,11 +226,18 @@ struct cp_world {
 	std::vector<int> j_type, j_kind, j_a, j_b;  // j_type: degree (rows); j_kind: cp_joint_add's type
 	std::vector<double> j_geo;  // 14 per joint: tp1, tp2, ax1, ax2, ps, rs
 	std::vector<double> j_drive;  // 4 per joint: powered mode, max, target velocity, target position (motors, piston drives)
+	std::vector<double> j_cfm;    // per joint: custom CFM, NaN: the world's
+	// springs: groups of springs sharing dynamic bodies, each in the
+	// constraint graph's edge order, for the joint count and root order they
+	// were built for
+	DArr<int> sp_goff, sp_list;
+	int sp_ngroups = 0, sp_nj = -1, n_springs = 0;
+	std::vector<int> sp_roots;
 	bool drive_dirty = false;
 	int j_dev_n = 0;            // joints on the device
 	bool joints_dirty = false;  // static joint data or systems changed since the upload
 	DArr<int> dj_type, dj_kind, dj_a, dj_b, dj_sys, dj_proj;
-	DArr<double> dj_tp1, dj_tp2, dj_ax1, dj_ax2, dj_ps, dj_rs, dj_imp, dj_pimp, dj_himp, dj_u, dj_w, dj_row, dj_drive, lin;
+	DArr<double> dj_tp1, dj_tp2, dj_ax1, dj_ax2, dj_ps, dj_rs, dj_imp, dj_pimp, dj_himp, dj_u, dj_w, dj_row, dj_drive, dj_cfm, lin;
 	DArr<uint64_t> nocol;
 	int n_nocol = 0;
 	// Joint edges (body pairs with joints) of the constraint graph: n1 is b1 of
@@ -416,7 +424,7 @@ struct cp_world {
 		J.type = dj_type.p; J.kind = dj_kind.p; J.a = dj_a.p; J.b = dj_b.p; J.sys = dj_sys.p;
 		J.tp1 = dj_tp1.p; J.tp2 = dj_tp2.p; J.ax1 = dj_ax1.p; J.ax2 = dj_ax2.p; J.ps = dj_ps.p; J.rs = dj_rs.p;
 		J.imp = dj_imp.p; J.pimp = dj_pimp.p; J.himp = dj_himp.p; J.u = dj_u.p; J.w = dj_w.p; J.row = dj_row.p;
-		J.proj = dj_proj.p; J.drive = dj_drive.p;
+		J.proj = dj_proj.p; J.drive = dj_drive.p; J.cfm = dj_cfm.p;
 		return J;
 	}
 	JointAct joint_act() { return JointAct{(int)j_type.size(), dj_kind.p, dj_u.p, dj_drive.p, dj_tp1.p}; }
@@ -448,6 +456,7 @@ struct cp_world {
 	void upload_joints();
 	void upload_drives();
 	void init_new_joints(const Bodies& b);
+	void schedule_springs();
 	static int state_projection(int kind, const double* s6, const double* g);
 	int j_init_from = 0;  // joints from this index on take their constructor state from the bodies
 	void read_joint_state(double* s, int n);
@@ -503,7 +512,7 @@ void cp_world::free_all() {
 	                      &dh_cs_row, &dh_cs_loc, &dh_dg_loc, &isl_root, &isl_edge_off, &isl_body_off, &isl_body_cnt, &isl_stack_off,
 	                      &isl_joint_off, &ib_vals, &ib_vals2, &ib_head, &ib_midx, &ib_me, &ib_adjv, &ib_adjv2, &ib_aoff, &ib_parent,
 	                      &ib_comp, &ib_jgoff, &ib_jord, &ib_rank, &ib_rord, &ib_info, &ib_stack, &ib_vlist, &ib_elist, &ib_vis,
-	                      &ib_ecol, &ib_eorder, &ib_loc, &ib_sys_seen, &ib_slots};
+	                      &ib_ecol, &ib_eorder, &ib_loc, &ib_sys_seen, &ib_slots, &sp_goff, &sp_list};
 	for (auto* a : jints) a->free_();
 	DArr<uint64_t>* ju64[] = {&ib_keys, &ib_keys2, &ib_adjk, &ib_adjk2, &ib_jgkey, &ib_col_used};
 	for (auto* a : ju64) a->free_();
@@ -512,9 +521,12 @@ void cp_world::free_all() {
 	ib_out.free_();
 	ib_nj = -1;
 	ib_rord_host.clear();
+	sp_nj = -1;
+	sp_ngroups = 0;
+	sp_roots.clear();
 	isl_n = 0;
 	islands_on_device = false;
-	DArr<double>* jdbl[] = {&dj_tp1, &dj_tp2, &dj_ax1, &dj_ax2, &dj_ps, &dj_rs, &dj_imp, &dj_pimp, &dj_himp, &dj_u, &dj_w, &dj_row, &dj_drive, &lin,
+	DArr<double>* jdbl[] = {&dj_tp1, &dj_tp2, &dj_ax1, &dj_ax2, &dj_ps, &dj_rs, &dj_imp, &dj_pimp, &dj_himp, &dj_u, &dj_w, &dj_row, &dj_drive, &dj_cfm, &lin,
 	                        &dh_buf, &dh_dbuf, &dh_scratch};
 	for (auto* a : jdbl) a->free_();
 	nocol.free_();
@@ -713,6 +725,7 @@ void cp_world::upload_joints() {
 		rs[k] = r[13];
 	}
 	up(dj_tp1, g[0]); up(dj_tp2, g[1]); up(dj_ax1, g[2]); up(dj_ax2, g[3]); up(dj_ps, ps); up(dj_rs, rs);
+	up(dj_cfm, j_cfm);
 	upload_drives();
 	up(dj_row, rows);
 	j_dev_n = K;
@@ -728,9 +741,11 @@ void cp_world::upload_joints() {
 		up(dj_pimp, pimp);
 		up(dj_proj, proj);
 	}
-	// disallowCollision pairs of jointed bodies (PhysicsEngine.cpp:570-572), sorted
+	// disallowCollision pairs of jointed bodies (PhysicsEngine.cpp:570-572),
+	// sorted; springs leave their pair colliding (addSpring, :974-995)
 	std::vector<uint64_t> nc;
 	for (int k = 0; k < K; k++) {
+		if (j_kind[k] == CP_JOINT_SPRING) continue;
 		uint64_t lo = (uint64_t)std::min(j_a[k], j_b[k]), hi = (uint64_t)std::max(j_a[k], j_b[k]);
 		nc.push_back(lo << 32 | hi);
 	}
@@ -758,6 +773,66 @@ void cp_world::init_new_joints(const Bodies& b) {
 	if (j_init_from >= K || !uploaded) return;
 	launch_joint_init(b, joints(), j_init_from, K, stream);
 	j_init_from = K;
+}
+
+// The order in which the reference applies spring impulses
+// (PhysicsEngine.cpp:1313-1355): it walks the constraint graph's nodes in
+// node-map order and each node's edges by the other body's id
+// (insertNewEdge, :1984-1995), visiting an edge once, so a spring comes at
+// (rank of its first-visited body, the other body), springs of one edge in
+// creation order.  Springs that share no dynamic body commute; they are cut
+// into groups (connected through shared dynamic bodies) that the device runs
+// side by side, each in that order.
+void cp_world::schedule_springs() {
+	const int K = (int)j_type.size();
+	const std::vector<int>& roots = root_order();
+	if (sp_nj == K && sp_roots == roots) return;
+	std::vector<int> sp;
+	for (int k = 0; k < K; k++)
+		if (j_kind[k] == CP_JOINT_SPRING) sp.push_back(k);
+	std::vector<int> rank(nb, 0);
+	for (int p = 0; p < (int)roots.size(); p++) rank[roots[p]] = p;
+	auto key = [&](int k) {
+		const int a = j_a[k], c = j_b[k];
+		const int first = rank[a] < rank[c] ? a : c, other = first == a ? c : a;
+		return std::make_pair(rank[first], other);
+	};
+	std::stable_sort(sp.begin(), sp.end(), [&](int x, int y) { return key(x) < key(y); });
+	std::vector<int> parent(nb);
+	for (int i = 0; i < nb; i++) parent[i] = i;
+	auto find = [&](int i) {
+		while (parent[i] != i) i = parent[i] = parent[parent[i]];
+		return i;
+	};
+	for (int k : sp)
+		if (b_move[j_a[k]] == 0 && b_move[j_b[k]] == 0) parent[find(j_a[k])] = find(j_b[k]);
+	std::vector<int> group_of(nb, -1), gcount, gid(sp.size());
+	for (size_t x = 0; x < sp.size(); x++) {
+		const int k = sp[x];
+		const int body = b_move[j_a[k]] == 0 ? j_a[k] : (b_move[j_b[k]] == 0 ? j_b[k] : -1);
+		if (body < 0) {  // between two bodies that do not move: a group of its own
+			gid[x] = (int)gcount.size();
+			gcount.push_back(0);
+		} else {
+			int& g = group_of[find(body)];
+			if (g < 0) {
+				g = (int)gcount.size();
+				gcount.push_back(0);
+			}
+			gid[x] = g;
+		}
+		gcount[gid[x]]++;
+	}
+	std::vector<int> goff(gcount.size() + 1, 0), list(sp.size());
+	for (size_t g = 0; g < gcount.size(); g++) goff[g + 1] = goff[g] + gcount[g];
+	std::vector<int> fill(goff.begin(), goff.end() - 1);
+	for (size_t x = 0; x < sp.size(); x++) list[fill[gid[x]]++] = sp[x];
+	up(sp_goff, goff);
+	up(sp_list, list);
+	sp_ngroups = (int)gcount.size();
+	n_springs = (int)sp.size();
+	sp_nj = K;
+	sp_roots = roots;
 }
 
 // HolonomicSystem::HolonomicSystem (HolonomicBlockSolver.cpp:11-132): the
@@ -1691,6 +1766,7 @@ void cp_world::step() {
 	if (serving) serving_commit();
 	if (!j_type.empty() && cfg.pgs_mode == CP_PGS_COLORED) throw ApiErr{CP_ERR_STATE};  // joints: island solves only
 	if (cfg.sleeping && (cfg.pgs_mode == CP_PGS_COLORED || serving)) throw ApiErr{CP_ERR_STATE};  // sleeping: island solves only
+	if (cfg.sleeping && std::count(j_kind.begin(), j_kind.end(), (int)CP_JOINT_SPRING)) throw ApiErr{CP_ERR_STATE};  // springs: no sleeping
 	if (sl_active && !cfg.sleeping && cfg.pgs_mode == CP_PGS_COLORED) {
 		// the coloured mode keeps no asleep bodies: all wake
 		HIPCHK(hipMemsetAsync(sl_asleep.p, 0, (size_t)nb * sizeof(int), stream));
@@ -1709,6 +1785,7 @@ void cp_world::step() {
 	Params P = params();
 	Bodies b = bodies();
 	init_new_joints(b);
+	schedule_springs();
 	if (timing) HIPCHK(hipEventRecord(ev[0], stream));
 	if (sl_active) {
 		SleepDev SD = sleep_dev();
@@ -1764,6 +1841,8 @@ void cp_world::step() {
 		launch_age(cb[cur].c, ncontacts, stream);
 	}
 	for (int s = 0; s < cfg.substeps && !use_graph && !use_fused; s++) {
+		// spring impulses come before any row is built (PhysicsEngine.cpp:1313-1355)
+		launch_springs(b, J, sp_ngroups, sp_goff.p, sp_list.p, P.sub_dt, stream);
 		launch_build_rows(b, cb[cur].c, ncontacts, P, stream);
 		// joints; holonomic ones are current at the first substep (PhysicsEngine.cpp:1396-1403)
 		launch_joint_rows(b, J, P, P.sub_dt, 0, s == 0, stream);
@@ -2527,8 +2606,9 @@ int cp_world_set_node_order(cp_world* w, int n, const int64_t* ids) {
 }
 
 int cp_joint_add(cp_world* w, int type, int b1, int b2, const double* geo14, int* joint_index) {
-	static const int degree[13] = {0, 1, 0, 3, 4, 5, 6, 5, 1, 1, 1, 1, 1};  // rows per kind (ConstraintSolver.h:193-398)
-	if (!w || !geo14 || !joint_index || type < 1 || type > 12 || degree[type] == 0) return CP_ERR_INVALID;
+	static const int degree[14] = {0, 1, 0, 3, 4, 5, 6, 5, 1, 1, 1, 1, 1, 0};  // rows per kind (ConstraintSolver.h:193-398)
+	if (!w || !geo14 || !joint_index || type < 1 || type > CP_JOINT_SPRING || (degree[type] == 0 && type != CP_JOINT_SPRING))
+		return CP_ERR_INVALID;
 	if (b1 < 0 || b2 < 0 || b1 >= w->nb || b2 >= w->nb || b1 == b2 || w->b_scene[b1] != w->b_scene[b2]) return CP_ERR_INVALID;
 	for (int c = 0; c < 14; c++)
 		if (!std::isfinite(geo14[c])) return CP_ERR_INVALID;
@@ -2539,6 +2619,7 @@ int cp_joint_add(cp_world* w, int type, int b1, int b2, const double* geo14, int
 	w->j_b.push_back(b2);
 	w->j_geo.insert(w->j_geo.end(), geo14, geo14 + 14);
 	w->j_drive.insert(w->j_drive.end(), {0.0, 0.0, 0.0, 0.0});  // powered mode off (the Motor / Piston constructors)
+	w->j_cfm.push_back(std::numeric_limits<double>::quiet_NaN());  // CFM{USE_GLOBAL}
 	uint64_t key = (uint64_t)std::min(b1, b2) << 32 | (uint64_t)std::max(b1, b2);
 	// limits, motors and pistons are not holonomic: they join the pair's edge
 	// and leave the holonomic systems as they are (PhysicsEngine.cpp:725-760,
@@ -2585,6 +2666,14 @@ int cp_joint_set_drive(cp_world* w, int k, int mode, double max, double target) 
 	if (st != CP_OK || !w->cfg.sleeping) return st;
 	const int ids[2] = {w->j_a[k], w->j_b[k]};
 	return cp_world_wake_bodies(w, 2, ids);
+}
+
+int cp_joint_set_cfm(cp_world* w, int k, double cfm) {
+	if (!w || k < 0 || k >= (int)w->j_type.size() || (!std::isnan(cfm) && !(cfm >= 0 && std::isfinite(cfm)))) return CP_ERR_INVALID;
+	return guard(w, [&] {
+		w->j_cfm[k] = cfm;
+		w->joints_dirty = true;
+	});
 }
 
 int cp_world_joint_count(cp_world* w, int* n) {

@@ -238,7 +238,8 @@ typedef enum {
 	CP_JOINT_TWIST_LIMIT = 9,   /* addTwistLimitConstraint, :952-970; 1 projected row while active */
 	CP_JOINT_MOTOR = 10,        /* addMotorConstraint, :725-760; 1 row while powered or outside its angle range */
 	CP_JOINT_PISTON_DRIVE = 11, /* addPistonConstraint, :808-846: the piston's drive, 1 row while powered */
-	CP_JOINT_SLIDE_LIMIT = 12   /* addPistonConstraint: the piston's slide limit, 1 row while exceeded */
+	CP_JOINT_SLIDE_LIMIT = 12,  /* addPistonConstraint: the piston's slide limit, 1 row while exceeded */
+	CP_JOINT_SPRING = 13        /* addSpring, :974-995; no row: impulses at each substep's start */
 } cp_joint_kind;
 
 /* Powered modes of motors and piston drives (PoweredConstraitMode). */
@@ -270,6 +271,13 @@ typedef enum {
  * (12) and its drive (11), each with the base joint's attach points, b1's
  * slide axis and position strength; the slide limit holds the minimum and
  * maximum positions in the second axis slot.
+ * A spring (13) holds its stiffness, damping and resting length in the first
+ * axis slot (a negative length: the current distance, measured from the
+ * bodies at the next step), adds no row and leaves the pair colliding; its
+ * impulses are applied at the start of every substep, before any row is
+ * built, in the constraint graph's edge order (PhysicsEngine.cpp:1313-1355).
+ * Worlds with springs do not sleep: a step with sleeping on returns
+ * CP_ERR_STATE.
  * Joints are solved in the island modes; a step of a world with joints in
  * CP_PGS_COLORED mode returns CP_ERR_STATE.  Kinds 1-7 are holonomic
  * constraints (isHolonomicConstraint) and join holonomic systems; kinds
@@ -288,6 +296,11 @@ int cp_world_joint_count(cp_world* w, int* n);
  * target.  With sleeping on, both bodies receive a waking action.  A new
  * motor or drive is off. */
 int cp_joint_set_drive(cp_world* w, int k, int mode, double max, double target);
+/* Constraint force mixing of joint k's rows: setConstraintUseCustomCFM with
+ * cfm >= 0, setConstraintUseGlobalCFM with NaN (PhysicsEngine.cpp:1012-1026;
+ * CFM::getCFMValue, CFM.h:5-12).  The holonomic block factor keeps the
+ * world's CFM, as the reference's computeInverse does (:1527-1531). */
+int cp_joint_set_cfm(cp_world* w, int k, double cfm);
 /* Joint solver state, 12 doubles per joint: accumulated impulse (6, unused
  * components 0), basis u and w of the current rows of hinges, sliders and
  * sliding hinges (0 for the others) (the PersistentConstraint ::constraint
@@ -295,9 +308,11 @@ int cp_joint_set_drive(cp_world* w, int k, int mode, double max, double target);
  * twist limit, its tracked current_angle where u is (a new limit: 1, 0); a
  * motor its angular position and prev_velocity, a piston drive its position
  * and prev_velocity, a slide limit !slide_limit_exceeded and the position
- * (getMotorAngularPosition / getPistonPosition, PhysicsEngine.cpp:1914-1930).
- * A new motor or piston measures its position from the bodies at the next
- * step or state read (the Motor / Piston constructors).
+ * (getMotorAngularPosition / getPistonPosition, PhysicsEngine.cpp:1914-1930),
+ * a spring its resting length where u.x is.
+ * A new motor or piston measures its position, and a spring given a negative
+ * length its resting length, from the bodies at the next step or state read
+ * (the Motor / Piston constructors, addSpring).
  * Setting it marks the holonomic systems for a rebuild of their rows and
  * factors at the next step. */
 int cp_world_set_joint_state(cp_world* w, const double* state12);
