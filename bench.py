@@ -44,13 +44,27 @@ def pgs_bytes_per_contact(vel_iters, pos_iters, word=8):
     return word * (32 + vel_iters * 62 + pos_iters * 21)
 
 
+def pgs_bytes_per_contact_split(vel_iters, pos_iters):
+    """The same for the split solve's compact FP32 record (normal, u, w, the
+    two inverse masses and lever arms; the angular rows are rebuilt from the
+    lever arms and the bodies' FP32 inverse inertia, 2 x 6 words, counted per
+    contact).  Warm start: basis and arms 17, impulses 3, inertia 12 = 32
+    words.  Per velocity sweep: basis and arms 17, normal inverse + target,
+    friction inverse 4 + targets 2, mu (9), impulses read 3 and written 3 with
+    the static flag (4), inertia 12 = 45 words.  Per position sweep: n, the
+    inverse masses and arms 11, inverse, target, pseudo impulse read and
+    written 4, inertia 12 = 27 words."""
+    return 4 * (32 + vel_iters * 45 + pos_iters * 27)
+
+
 # Algorithmic bytes of the other per-substep work, per live contact or per
-# dynamic body: the row build reads two bodies' pose and inertia (2 x 76 B) and
-# the contact's cache fields (48 B) and writes its 256-byte FP32 record; the
+# dynamic body: the row build reads two bodies' pose, velocity, mass and FP32
+# inverse inertia (2 x 144 B) and the contact's cache fields (21 doubles) and
+# writes its 128-byte FP32 record and friction basis (6 doubles, a flag); the
 # write-back reads the record's 4 impulse words and writes 3 doubles and a
 # flag; integration reads and writes com, q, v, w, pseudo v and w and reads
 # the world and reference inertia (592 B, FP64).
-ROW_BUILD_BYTES = 2 * 76 + 48 + 256
+ROW_BUILD_BYTES = 2 * 144 + 168 + 128 + 52
 WRITEBACK_BYTES = 16 + 28
 INTEGRATE_BYTES = 592
 
@@ -462,8 +476,7 @@ def main():
     value = n_dyn * args.steps / total
     ms_step = 1e3 * total / args.steps
     live = float(tim[:, 7].mean())
-    word = 8 if args.precision == "fp64" else 4
-    bpc = pgs_bytes_per_contact(10, 1, word)
+    bpc = pgs_bytes_per_contact(10, 1, 8) if args.precision == "fp64" else pgs_bytes_per_contact_split(10, 1)
     fused = float(tim[:, 6].sum()) == 0.0  # the substep loop ran as the fused per-scene kernels
     if fused:
         # one launch = the step's substep loop: 8 x (per live contact: row build, solve, write-back;

@@ -66,6 +66,7 @@ struct Bodies {
 	int* wgv_off;     // [n+1] world Gauss normal offsets
 	double* wnorm;    // [3*WG]
 	double* dv;       // [12][n] PGS velocity / pseudo-velocity deltas
+	float* iinv32;    // [n][8] world inverse inertia in FP32 for the split solve: xx, xy, xz, yy, yz, zz (0 unless dynamic), 2 unused
 	int* color_mask;  // scratch for colouring (2 x u64 per body, as 4 ints)
 	// sleeping (null while no body can be asleep): RigidBody::asleep, and the
 	// asleep flags of the last step's first substep, when the reference
@@ -322,7 +323,25 @@ __device__ __forceinline__ bool contact_removed(const Contacts& C, int k, const 
 // full 6x6 product M^-1 J^T gives (its other five terms are signed zeros).
 // Blocks start at multiples of 4 values and records are 128-byte aligned, so
 // the blocks load as 16-byte vectors.
+//
+// The split solve stores a compact FP32 record of CP_RSF values (128 B, one
+// cache line): the first 16 values as below with the inverse mass of body a
+// in the padding slot, then the basis and the two lever arms (CP_F_*).  A row
+// step rebuilds the angular Jacobians as cross products of the lever arms
+// with the basis and M^-1 J^T from the bodies' FP32 inverse inertia
+// (Bodies::iinv32), so a sweep reads half the bytes and a whole batch's
+// records fit the 256 MB last-level cache.
 #define CP_RS 64
+#define CP_RSF 32
+enum RowFieldF32 {
+	CP_F_IMA = 11,  // 1/m_a (0 unless dynamic)
+	CP_F_N = 16,    // normal [3]
+	CP_F_IMB = 19,  // 1/m_b
+	CP_F_U = 20,    // friction direction u [3]
+	CP_F_W = 23,    // friction direction w [3]
+	CP_F_RA = 26,   // contact point - centre of mass of body a [3]
+	CP_F_RB = 29    // contact point - centre of mass of body b [3]
+};
 enum RowField {
 	CP_R_IMP = 0,     // accumulated normal impulse
 	CP_R_PIMP = 1,    // accumulated pseudo impulse
@@ -434,6 +453,8 @@ void launch_edge_slots(const Bodies& b, const Contacts& C, const int* order, int
 void launch_color(const Bodies& b, const int* scene_body_off, int nscenes, const uint64_t* ekey, const int* elive, int nedges,
                   int* ecolor, int* order, int* color_off, int* ncolors, int* edge_range, int max_bodies, hipStream_t s);
 void launch_build_rows(const Bodies& b, const Contacts& C, int nc, const Params& P, hipStream_t s, const int* ncp = nullptr);
+// The FP32 inverse inertia of every body (the split solve's rows read it).
+void launch_iinv32(const Bodies& b, hipStream_t s);
 // Joint rows: mode 0 all joints (holonomic ones skipped when first), 1 the
 // holonomic joints with warm-start re-projection, 2 the holonomic joints
 // without re-projection.

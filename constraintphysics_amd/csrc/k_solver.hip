@@ -36,10 +36,34 @@ namespace cpx {
 
 __device__ __forceinline__ V3 ld3(const double* a, int i, int n) { return V3{a[i], a[n + i], a[2 * n + i]}; }
 
-// Row record of contact k, CP_RS values of the solve precision T.
+// Row record of contact k: CP_RS FP64 values, or the compact CP_RSF FP32 values
+// of the split solve.
+template <class T>
+__host__ __device__ constexpr int rec_stride() { return sizeof(T) == 4 ? CP_RSF : CP_RS; }
 template <class T>
 __device__ __forceinline__ T* rec(const Contacts& C, int k) {
-	return static_cast<T*>(__builtin_assume_aligned(reinterpret_cast<T*>(C.row) + (size_t)k * CP_RS, 128));
+	return static_cast<T*>(__builtin_assume_aligned(reinterpret_cast<T*>(C.row) + (size_t)k * rec_stride<T>(), 128));
+}
+
+// The FP32 copy of body i's world inverse inertia, upper triangle (the split
+// solve's rows multiply by it; 0 unless the body is dynamic, as getInvTensor
+// returns for the others, RigidBody.cpp:169-171).
+__device__ __forceinline__ void iinv32_body(const Bodies& b, int i) {
+	const int N = b.n;
+	const bool d = b.movement[i] == 0;
+	auto e = [&](int r, int c) { return d ? (float)b.Iinv[(3 * r + c) * N + i] : 0.0f; };
+	float4* o = reinterpret_cast<float4*>(b.iinv32 + 8 * (size_t)i);
+	o[0] = float4{e(0, 0), e(0, 1), e(0, 2), e(1, 1)};
+	o[1] = float4{e(1, 2), e(2, 2), 0.0f, 0.0f};
+}
+__global__ __launch_bounds__(256) void k_iinv32(Bodies b) {
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < b.nl) iinv32_body(b, i);
+}
+__device__ __forceinline__ void ld_iinv32(const float* base, int i, float (&m)[6]) {
+	const float4* p = reinterpret_cast<const float4*>(base + 8 * (size_t)i);
+	float4 x = p[0], y = p[1];
+	m[0] = x.x; m[1] = x.y; m[2] = x.z; m[3] = x.w; m[4] = y.x; m[5] = y.y;
 }
 
 template <class T>
@@ -66,8 +90,99 @@ __global__ __launch_bounds__(64) void k_build_rows(Bodies b, Contacts C, int nc,
 		if (C.live[k]) build_row<T>(b, C, k, P);
 }
 
+// The compact FP32 record of the split solve, derived in FP64 from the same
+// state: the values the FP64 record's products give, evaluated directly
+// (J M^-1 J^T as ima n.n + j_ang . I^-1 j_ang and its friction 2x2
+// counterpart) instead of through 6x6 products, then rounded to FP32.  The
+// inverse inertia is the bodies' FP32 copy (Bodies::iinv32, current when rows
+// are built), the matrix the row steps multiply by.
+__device__ void build_row_split(const Bodies& b, const Contacts& C, int k, const Params& P) {
+	const int N = b.n, K = C.cap;
+	float* R = rec<float>(C, k);
+	const int i1 = (int)(C.key[k] >> 32), i2 = (int)(C.key[k] & 0xFFFFFFFFu);
+	const Q q1{b.q[i1], b.q[N + i1], b.q[2 * N + i1], b.q[3 * N + i1]};
+	const Q q2{b.q[i2], b.q[N + i2], b.q[2 * N + i2], b.q[3 * N + i2]};
+	const V3 c1 = ld3(b.com, i1, N), c2 = ld3(b.com, i2, N);
+	const V3 p1 = q_rotate(q1, ld3(C.lp1, k, K)) + c1;
+	const V3 p2 = q_rotate(q2, ld3(C.lp2, k, K)) + c2;
+	const V3 n = q_rotate(q1, ld3(C.ln1, k, K));
+	const double adj = dot(p2 - p1, n) + C.pen[k];
+	const double a1 = P.hardness * P.sub_dt, a2 = 1.0 / P.sub_dt;
+	const double pcc = (a2 < a1) ? a2 : a1;  // posCorrectCoeff, PersistentConstraint.cpp:199
+	const bool d1 = b.movement[i1] == 0, d2 = b.movement[i2] == 0;
+	const double ima = d1 ? 1.0 / b.mass[i1] : 0, imb = d2 ? 1.0 / b.mass[i2] : 0;
+	// the bodies' FP32 inverse inertia (upper triangles), the one the row steps multiply by
+	float fa[6], fb[6];
+	ld_iinv32(b.iinv32, i1, fa);
+	ld_iinv32(b.iinv32, i2, fb);
+	auto sym = [](const float (&m)[6], V3 v) {
+		return V3{m[0] * v.x + m[1] * v.y + m[2] * v.z, m[1] * v.x + m[3] * v.y + m[4] * v.z, m[2] * v.x + m[4] * v.y + m[5] * v.z};
+	};
+	const V3 rA = p1 - c1, rB = p1 - c2;
+	const bool x1 = b.movement[i1] == 1, x2 = b.movement[i2] == 1;  // getVel()/getAngVel() of FIXED bodies are zero
+	const V3 z{0, 0, 0};
+	const V3 va = x1 ? z : ld3(b.vel, i1, N), wa = x1 ? z : ld3(b.angv, i1, N);
+	const V3 vb = x2 ? z : ld3(b.vel, i2, N), wb = x2 ? z : ld3(b.angv, i2, N);
+	const double nn = dot(n, n);
+	// normal row: J_a = (-n | n x rA), J_b = (n | rB x n)
+	{
+		const V3 ja = cross(n, rA), jb = cross(rB, n);
+		double itv = ima * nn + dot(ja, sym(fa, ja)) + imb * nn + dot(jb, sym(fb, jb));
+		itv *= (1 + P.cfm);
+		R[CP_R_INV] = (float)(1.0 / itv);
+		const double cur = (dot(va, -n) + dot(wa, ja)) + (dot(vb, n) + dot(wb, jb));
+		R[CP_R_TGT] = (float)((cur < -P.cutoff_vel) ? -(1 + C.bounce[k]) * cur : -cur);
+		double sp = adj - 0.05;
+		sp = (sp < 0.0) ? 0.0 : sp;
+		R[CP_R_PTGT] = (float)(sp * pcc);
+		R[CP_R_PIMP] = 0;
+		R[CP_R_IMP] = (float)C.imp_n[k];
+		R[CP_R_MU] = (float)C.mu[k];
+	}
+	// friction rows: J = (u | rA x u), (-u | u x rB) and the same for w
+	{
+		const V3 su = ld3(C.fu, k, K), sw = ld3(C.fw, k, K);
+		const double f0 = C.imp_t[k], f1 = C.imp_t[K + k];
+		V3 u, w;
+		perp_basis(n, &u, &w);
+		const V3 au = cross(rA, u), aw = cross(rA, w), bu = cross(u, rB), bw = cross(w, rB);
+		const V3 tau = sym(fa, au), taw = sym(fa, aw), tbu = sym(fb, bu), tbw = sym(fb, bw);
+		const double m = ima + imb;
+		double k00 = m * dot(u, u) + dot(au, tau) + dot(bu, tbu);
+		const double k01 = m * dot(u, w) + dot(au, taw) + dot(bu, tbw);
+		const double k10 = m * dot(w, u) + dot(aw, tau) + dot(bw, tbu);
+		double k11 = m * dot(w, w) + dot(aw, taw) + dot(bw, tbw);
+		k00 *= (1 + P.cfm);
+		k11 *= (1 + P.cfm);
+		const double di = 1.0 / (k00 * k11 - k01 * k10);
+		R[CP_R_FINV] = (float)(k11 * di);
+		R[CP_R_FINV + 1] = (float)(-k01 * di);
+		R[CP_R_FINV + 2] = (float)(-k10 * di);
+		R[CP_R_FINV + 3] = (float)(k00 * di);
+		R[CP_R_FIMP] = (float)(dot(u, su) * f0 + dot(u, sw) * f1);
+		R[CP_R_FIMP + 1] = (float)(dot(w, su) * f0 + dot(w, sw) * f1);
+		R[CP_R_FTGT] = (float)(-((dot(va, u) + dot(wa, au)) + (dot(vb, -u) + dot(wb, bu))));
+		R[CP_R_FTGT + 1] = (float)(-((dot(va, w) + dot(wa, aw)) + (dot(vb, -w) + dot(wb, bw))));
+		R[CP_R_SREADY] = 0;
+		C.fu[k] = u.x; C.fu[K + k] = u.y; C.fu[2 * K + k] = u.z;
+		C.fw[k] = w.x; C.fw[K + k] = w.y; C.fw[2 * K + k] = w.z;
+		C.sready[k] = 0;
+		R[CP_F_U] = (float)u.x; R[CP_F_U + 1] = (float)u.y; R[CP_F_U + 2] = (float)u.z;
+		R[CP_F_W] = (float)w.x; R[CP_F_W + 1] = (float)w.y; R[CP_F_W + 2] = (float)w.z;
+	}
+	R[CP_F_N] = (float)n.x; R[CP_F_N + 1] = (float)n.y; R[CP_F_N + 2] = (float)n.z;
+	R[CP_F_IMA] = (float)ima;
+	R[CP_F_IMB] = (float)imb;
+	R[CP_F_RA] = (float)rA.x; R[CP_F_RA + 1] = (float)rA.y; R[CP_F_RA + 2] = (float)rA.z;
+	R[CP_F_RB] = (float)rB.x; R[CP_F_RB + 1] = (float)rB.y; R[CP_F_RB + 2] = (float)rB.z;
+}
+
 template <class T>
 __device__ void build_row(const Bodies& b, const Contacts& C, int k, const Params& P) {
+	if constexpr (sizeof(T) == 4) {
+		build_row_split(b, C, k, P);
+		return;
+	}
 	const int N = b.n, K = C.cap;
 	T* R = rec<T>(C, k);
 	int i1 = (int)(C.key[k] >> 32), i2 = (int)(C.key[k] & 0xFFFFFFFFu);
@@ -343,6 +458,76 @@ __device__ __forceinline__ void load_friction(const T* R, const T (&bs)[12], Row
 	expand_friction(bs, f, g);
 }
 
+// The same rows from the compact FP32 record r and the bodies' FP32 inverse
+// inertia ia, ib (upper triangles): J_a = (-n | n x r_a), J_b = (n | r_b x n)
+// for the normal row, J_a = (u | r_a x u), J_b = (-u | u x r_b) (and w) for
+// the friction rows, the angular parts of M^-1 J^T the inverse inertia times
+// those: the products the FP64 record stores, evaluated in FP32.
+__device__ __forceinline__ void cross3(const float* a, const float* b, float* o) {
+	o[0] = a[1] * b[2] - a[2] * b[1];
+	o[1] = a[2] * b[0] - a[0] * b[2];
+	o[2] = a[0] * b[1] - a[1] * b[0];
+}
+__device__ __forceinline__ void sym3(const float (&m)[6], const float* v, float* o) {
+	o[0] = m[0] * v[0] + m[1] * v[1] + m[2] * v[2];
+	o[1] = m[1] * v[0] + m[3] * v[1] + m[4] * v[2];
+	o[2] = m[2] * v[0] + m[4] * v[1] + m[5] * v[2];
+}
+__device__ __forceinline__ void compact_normal(const float (&r)[CP_RSF], const float (&ia)[6], const float (&ib)[6], RowRegs<float>& g) {
+	const float* n = r + CP_F_N;
+	float ja[3], jb[3], ta[3], tb[3];
+	cross3(n, r + CP_F_RA, ja);
+	cross3(r + CP_F_RB, n, jb);
+	sym3(ia, ja, ta);
+	sym3(ib, jb, tb);
+#pragma unroll
+	for (int q = 0; q < 3; q++) {
+		g.j[q] = -n[q];
+		g.j[3 + q] = ja[q];
+		g.j[6 + q] = n[q];
+		g.j[9 + q] = jb[q];
+		g.it[q] = 0.0f + r[CP_F_IMA] * g.j[q];
+		g.it[3 + q] = ta[q];
+		g.it[6 + q] = 0.0f + r[CP_F_IMB] * g.j[6 + q];
+		g.it[9 + q] = tb[q];
+	}
+}
+__device__ __forceinline__ void compact_friction(const float (&r)[CP_RSF], const float (&ia)[6], const float (&ib)[6], RowRegs<float>& g) {
+	const float *u = r + CP_F_U, *w = r + CP_F_W, *ra = r + CP_F_RA, *rb = r + CP_F_RB;
+	float au[3], aw[3], bu[3], bw[3], tau[3], taw[3], tbu[3], tbw[3];
+	cross3(ra, u, au);
+	cross3(ra, w, aw);
+	cross3(u, rb, bu);
+	cross3(w, rb, bw);
+	sym3(ia, au, tau);
+	sym3(ia, aw, taw);
+	sym3(ib, bu, tbu);
+	sym3(ib, bw, tbw);
+#pragma unroll
+	for (int q = 0; q < 3; q++) {
+		g.fj[q] = u[q];
+		g.fj[3 + q] = au[q];
+		g.fj[6 + q] = w[q];
+		g.fj[9 + q] = aw[q];
+		g.fj[12 + q] = -u[q];
+		g.fj[15 + q] = bu[q];
+		g.fj[18 + q] = -w[q];
+		g.fj[21 + q] = bw[q];
+	}
+#pragma unroll
+	for (int q = 0; q < 3; q++) {
+#pragma unroll
+		for (int c = 0; c < 2; c++) {
+			g.fit[q * 2 + c] = 0.0f + r[CP_F_IMA] * g.fj[c * 6 + q];
+			g.fit[12 + q * 2 + c] = 0.0f + r[CP_F_IMB] * g.fj[12 + c * 6 + q];
+		}
+		g.fit[(3 + q) * 2] = tau[q];
+		g.fit[(3 + q) * 2 + 1] = taw[q];
+		g.fit[12 + (3 + q) * 2] = tbu[q];
+		g.fit[12 + (3 + q) * 2 + 1] = tbw[q];
+	}
+}
+
 // applyWarmStartVelocityChange of the contact row (kind 0) or the friction
 // pair (kind 1) (ConstraintSolver.cpp:121-126).
 template <class T, class D>
@@ -472,20 +657,180 @@ __device__ __forceinline__ void friction_core(T* R, RowRegs<T>& g, const D& A, c
 }
 
 // One contact in one phase: 0 warm start (both rows), 1 velocity sweep
-// (normal row, then friction rows), 2 position sweep (normal row).
+// (normal row, then friction rows), 2 position sweep (normal row).  ia, ib:
+// the bodies' FP32 inverse inertia (the compact FP32 record; unused in FP64).
+// The compact FP32 record already in registers (r); R receives the updates.
+// The row steps of warm_*, normal_core and friction_core with the rows
+// formed as they are used (compact_normal / compact_friction), so that no
+// expanded copy of the record stays live.
+template <class D>
+__device__ __forceinline__ void contact_phase_f32(float* R, const float (&r)[CP_RSF], const D& A, const D& B, int phase, const float (&ia)[6],
+                                                  const float (&ib)[6]) {
+	const float* n = r + CP_F_N;
+	const float ima = r[CP_F_IMA], imb = r[CP_F_IMB];
+	float ja[3], jb[3], ta[3], tb[3];
+	cross3(n, r + CP_F_RA, ja);
+	cross3(r + CP_F_RB, n, jb);
+	sym3(ia, ja, ta);
+	sym3(ib, jb, tb);
+	// impulse x along the normal row into the deltas of half off
+	auto push_normal = [&](float x, int off) {
+#pragma unroll
+		for (int q = 0; q < 3; q++) {
+			float da = 0, db = 0;
+			da += x * (0.0f + ima * -n[q]);
+			db += x * (0.0f + imb * n[q]);
+			A.add(off + q, da);
+			B.add(off + q, db);
+		}
+#pragma unroll
+		for (int q = 0; q < 3; q++) {
+			float da = 0, db = 0;
+			da += x * ta[q];
+			db += x * tb[q];
+			A.add(off + 3 + q, da);
+			B.add(off + 3 + q, db);
+		}
+	};
+	// returns the row's accumulated impulse after the step
+	auto normal_step = [&](bool pseudo) -> float {
+		const int off = pseudo ? 6 : 0;
+		float x = 0, y = 0;
+#pragma unroll
+		for (int q = 0; q < 3; q++) {
+			x += A.get(off + q) * -n[q];
+			y += B.get(off + q) * n[q];
+		}
+#pragma unroll
+		for (int q = 0; q < 3; q++) {
+			x += A.get(off + 3 + q) * ja[q];
+			y += B.get(off + 3 + q) * jb[q];
+		}
+		const float cur = x + y;
+		const float tgt = pseudo ? r[CP_R_PTGT] : r[CP_R_TGT];
+		const float acc = pseudo ? r[CP_R_PIMP] : r[CP_R_IMP];
+		const float delta = tgt - cur;
+		float m = 0;
+		m += delta * r[CP_R_INV];
+		const float nw = acc + m;
+		const float pr = (nw < 0) ? 0.0f : nw;
+		const float diff = pr - acc;
+		if (is_zero1(diff)) return acc;
+		push_normal(diff, off);
+		const float v = acc + diff;
+		R[pseudo ? CP_R_PIMP : CP_R_IMP] = v;
+		return v;
+	};
+	if (phase == 2) {
+		(void)normal_step(true);
+		return;
+	}
+	const float *u = r + CP_F_U, *w = r + CP_F_W, *ra = r + CP_F_RA, *rb = r + CP_F_RB;
+	float au[3], aw[3], bu[3], bw[3];
+	cross3(ra, u, au);
+	cross3(ra, w, aw);
+	cross3(u, rb, bu);
+	cross3(w, rb, bw);
+	float tau[3], taw[3], tbu[3], tbw[3];
+	sym3(ia, au, tau);
+	sym3(ia, aw, taw);
+	sym3(ib, bu, tbu);
+	sym3(ib, bw, tbw);
+	auto push_friction = [&](float t0, float t1) {
+#pragma unroll
+		for (int q = 0; q < 3; q++) {
+			float da = 0, db = 0;
+			da += t0 * (0.0f + ima * u[q]);
+			da += t1 * (0.0f + ima * w[q]);
+			db += t0 * (0.0f + imb * -u[q]);
+			db += t1 * (0.0f + imb * -w[q]);
+			A.add(q, da);
+			B.add(q, db);
+		}
+#pragma unroll
+		for (int q = 0; q < 3; q++) {
+			float da = 0, db = 0;
+			da += t0 * tau[q];
+			da += t1 * taw[q];
+			db += t0 * tbu[q];
+			db += t1 * tbw[q];
+			A.add(3 + q, da);
+			B.add(3 + q, db);
+		}
+	};
+	if (phase == 0) {  // warm start: the stored impulses (applyWarmStartVelocityChange)
+		const float imp = r[CP_R_IMP];
+		if (!is_zero1(imp)) push_normal(imp, 0);
+		const float t0 = r[CP_R_FIMP], t1 = r[CP_R_FIMP + 1];
+		if (!is_zero2(t0, t1)) push_friction(t0, t1);
+		return;
+	}
+	const float ni = normal_step(false);
+	// friction pair, projected into the cone of the normal impulse held now
+	float c0x = 0, c0y = 0, c1x = 0, c1y = 0;
+#pragma unroll
+	for (int q = 0; q < 3; q++) {
+		c0x += A.get(q) * u[q];
+		c0y += B.get(q) * -u[q];
+		c1x += A.get(q) * w[q];
+		c1y += B.get(q) * -w[q];
+	}
+#pragma unroll
+	for (int q = 0; q < 3; q++) {
+		c0x += A.get(3 + q) * au[q];
+		c0y += B.get(3 + q) * bu[q];
+		c1x += A.get(3 + q) * aw[q];
+		c1y += B.get(3 + q) * bw[q];
+	}
+	const float d0 = r[CP_R_FTGT] - (c0x + c0y), d1 = r[CP_R_FTGT + 1] - (c1x + c1y);
+	float m0 = 0, m1 = 0;
+	m0 += d0 * r[CP_R_FINV];
+	m0 += d1 * r[CP_R_FINV + 1];
+	m1 += d0 * r[CP_R_FINV + 2];
+	m1 += d1 * r[CP_R_FINV + 3];
+	const float i0 = r[CP_R_FIMP], i1 = r[CP_R_FIMP + 1];
+	const float n0 = i0 + m0, n1 = i1 + m1;
+	const float inf = __builtin_inff();
+	const float lim = (ni < inf) ? ni : inf;
+	const float maxm = r[CP_R_MU] * lim;
+	const float cm = sqrtf(n0 * n0 + n1 * n1);
+	float p0, p1;
+	if (cm > maxm) {
+		R[CP_R_SREADY] = 0;
+		const float q = maxm / cm;
+		p0 = n0 * q;
+		p1 = n1 * q;
+	} else {
+		R[CP_R_SREADY] = 1;
+		p0 = n0;
+		p1 = n1;
+	}
+	const float e0 = p0 - i0, e1 = p1 - i1;
+	if (!is_zero2(e0, e1)) {
+		push_friction(e0, e1);
+		R[CP_R_FIMP] = i0 + e0;
+		R[CP_R_FIMP + 1] = i1 + e1;
+	}
+}
 template <class T, class D>
-__device__ __forceinline__ void contact_phase(T* R, const D& A, const D& B, int phase) {
+__device__ __forceinline__ void contact_phase(T* R, const D& A, const D& B, int phase, const float (&ia)[6], const float (&ib)[6]) {
 	RowRegs<T> g;
-	T bs[12];
-	ldv(R, g.h);
-	ldv(R + CP_R_BASIS, bs);
-	load_normal(R, bs, g);
+	if constexpr (sizeof(T) == 4) {
+		float r[CP_RSF];
+		ldv(R, r);
+		contact_phase_f32(R, r, A, B, phase, ia, ib);
+		return;
+	} else {
+		T bs[12];
+		ldv(R, g.h);
+		ldv(R + CP_R_BASIS, bs);
+		load_normal(R, bs, g);
+		if (phase != 2) load_friction(R, bs, g);
+	}
 	if (phase == 0) {
-		load_friction(R, bs, g);
 		warm_normal(g, A, B);
 		warm_friction(g, A, B);
 	} else if (phase == 1) {
-		load_friction(R, bs, g);
 		normal_core(R, g, A, B, false);
 		friction_core(R, g, A, B);
 	} else {
@@ -887,8 +1232,8 @@ __global__ __launch_bounds__(64) void k_pgs_islands(Bodies b, Contacts C, Joints
 // order; the colour index MAXC holds edges that did not fit a colour and runs
 // on thread 0 alone.  kind 0 warm start, 1 velocity sweep, 2 position sweep.
 template <class T, class D, class Make>
-__device__ __forceinline__ void colour_phase(Contacts& C, const int4* desc, const uint32_t* lmask, int e0, int beg, int end, bool serial,
-                                             int kind, Make make) {
+__device__ __forceinline__ void colour_phase(Contacts& C, const float* iinv32, const int4* desc, const uint32_t* lmask, int e0, int beg, int end,
+                                             bool serial, int kind, Make make) {
 	int stride = serial ? 1 : blockDim.x;
 	int start = serial ? (threadIdx.x == 0 ? beg : end) : beg + threadIdx.x;
 	const int off = kind == 2 ? 6 : 0;
@@ -901,6 +1246,11 @@ __device__ __forceinline__ void colour_phase(Contacts& C, const int4* desc, cons
 		DvReg<T> A, B;
 		A.dyn = MA.dyn;
 		B.dyn = MB.dyn;
+		float ia[6] = {0, 0, 0, 0, 0, 0}, ib[6] = {0, 0, 0, 0, 0, 0};
+		if constexpr (sizeof(T) == 4) {
+			ld_iinv32(iinv32, d.z, ia);
+			ld_iinv32(iinv32, d.w, ib);
+		}
 #pragma unroll
 		for (int c = 0; c < 6; c++) {
 			A.v[c] = MA.get(off + c);
@@ -910,7 +1260,7 @@ __device__ __forceinline__ void colour_phase(Contacts& C, const int4* desc, cons
 			int k = d.x + j;
 			bool live = lng ? C.live[k] != 0 : ((lm >> j) & 1u) != 0;
 			if (!live) continue;
-			contact_phase(rec<T>(C, k), A, B, kind);
+			contact_phase(rec<T>(C, k), A, B, kind, ia, ib);
 		}
 #pragma unroll
 		for (int c = 0; c < 6; c++) {
@@ -941,7 +1291,7 @@ __device__ __forceinline__ void pgs_scene_global(Bodies& b, Contacts& C, int s, 
 	for (int ph = 0; ph < total; ph++) {
 		int kind = ph == 0 ? 0 : (ph <= P.vel_iters ? 1 : 2);
 		for (int c = 0; c < nc; c++) {
-			colour_phase<T, DvGlobal<T>>(C, desc, lmask, e0, co[c], co[c + 1], c == MAXC, kind, make);
+			colour_phase<T, DvGlobal<T>>(C, b.iinv32, desc, lmask, e0, co[c], co[c + 1], c == MAXC, kind, make);
 			__syncthreads();
 		}
 	}
@@ -971,7 +1321,7 @@ __device__ __forceinline__ void pgs_scene_lds(Bodies& b, Contacts& C, T* sdv, in
 		for (int ph = ph0; ph < ph1; ph++) {
 			int kind = ph == 0 ? 0 : (ph <= P.vel_iters ? 1 : 2);
 			for (int c = 0; c < nc; c++) {
-				colour_phase<T, DvLds<T>>(C, desc, lmask, e0, co[c], co[c + 1], c == MAXC, kind, make);
+				colour_phase<T, DvLds<T>>(C, b.iinv32, desc, lmask, e0, co[c], co[c + 1], c == MAXC, kind, make);
 				__syncthreads();
 			}
 		}
@@ -1035,6 +1385,10 @@ __device__ __forceinline__ void substeps_scene(Bodies& b, Contacts& C, T* sdv, i
 	const int e0 = edge_range[2 * s], e1 = edge_range[2 * s + 1];
 	const int c0 = e1 > e0 ? efirst[e0] : 0, c1 = e1 > e0 ? efirst[e1 - 1] + ecnt[e1 - 1] : 0;
 	for (int sub = 0; sub < P.substeps; sub++) {
+		if constexpr (sizeof(T) == 4) {
+			for (int i = b0 + threadIdx.x; i < b1; i += blockDim.x) iinv32_body(b, i);
+			__syncthreads();
+		}
 		for (int k = c0 + threadIdx.x; k < c1; k += blockDim.x)
 			if (C.live[k]) build_row<T>(b, C, k, P);
 		__syncthreads();
@@ -1085,10 +1439,18 @@ __global__ void k_pgs_residual(Bodies b, Contacts C, int nc, double* acc) {
 		const T* dv = reinterpret_cast<const T*>(b.dv);
 		int i1 = (int)(C.key[k] >> 32), i2 = (int)(C.key[k] & 0xFFFFFFFFu);
 		RowRegs<T> g;
-		T bs[12];
-		ldv(R + CP_R_BASIS, bs);
-		load_normal(R, bs, g);
-		load_friction(R, bs, g);
+		if constexpr (sizeof(T) == 4) {  // the Jacobians do not involve the inertia
+			float r[CP_RSF];
+			const float z[6] = {0, 0, 0, 0, 0, 0};
+			ldv(R, r);
+			compact_normal(r, z, z, g);
+			compact_friction(r, z, z, g);
+		} else {
+			T bs[12];
+			ldv(R + CP_R_BASIS, bs);
+			load_normal(R, bs, g);
+			load_friction(R, bs, g);
+		}
 		double x = 0, y = 0;
 		for (int q = 0; q < 6; q++) {
 			x += (double)dv[q * N + i1] * (double)g.j[q];
@@ -1140,6 +1502,9 @@ void launch_build_rows(const Bodies& b, const Contacts& C, int nc, const Params&
 	int g = xcd_grid(ncp ? grid_for(nc, 64) : nblk(nc, 64));
 	if (P.precision == CP_PREC_SPLIT) hipLaunchKernelGGL(k_build_rows<float>, dim3(g), dim3(64), 0, s, b, C, nc, ncp, P);
 	else hipLaunchKernelGGL(k_build_rows<double>, dim3(g), dim3(64), 0, s, b, C, nc, ncp, P);
+}
+void launch_iinv32(const Bodies& b, hipStream_t s) {
+	if (b.nl) hipLaunchKernelGGL(k_iinv32, dim3(nblk(b.nl, 256)), dim3(256), 0, s, b);
 }
 void launch_pgs_islands(const Bodies& b, const Contacts& C, const Joints& J, const HSysDev& H, const IslandSched& S, const Params& P,
                         int* applied, double* lin, int first, bool colored, hipStream_t s) {

@@ -176,6 +176,7 @@ struct cp_world {
 	DArr<double> s_ref_pts, s_gnorm, s_gmin, s_gmax, s_grefval, s_radius, s_height;
 	// device: bodies
 	DArr<double> com, q, vel, angv, pvel, pang, prev_com, prev_q, I, Iinv, Iref, Irefinv, mass, mat, aabb, wpts, wnorm, dv;
+	DArr<float> iinv32;
 	DArr<int> movement, shape, scene, wpt_off, wgv_off, color_mask, scene_body_off, scene_of_body;
 	int nb = 0, nscenes = 0;
 	bool uploaded = false;
@@ -372,7 +373,7 @@ struct cp_world {
 		b.movement = movement.p; b.shape = shape.p; b.scene = scene.p;
 		b.mat = mat.p; b.aabb = aabb.p;
 		b.wpt_off = wpt_off.p; b.wpts = wpts.p; b.wgv_off = wgv_off.p; b.wnorm = wnorm.p;
-		b.dv = dv.p; b.color_mask = color_mask.p;
+		b.dv = dv.p; b.iinv32 = iinv32.p; b.color_mask = color_mask.p;
 		b.asleep = sl_active ? sl_asleep.p : nullptr;
 		b.asleep_rm = sl_active ? sl_asleep_rm.p : nullptr;
 		return b;
@@ -496,6 +497,7 @@ void cp_world::free_all() {
 	DArr<double>* dbl[] = {&s_ref_pts, &s_gnorm, &s_gmin, &s_gmax, &s_grefval, &s_radius, &s_height, &com, &q, &vel, &angv, &pvel, &pang,
 	                       &prev_com, &prev_q, &I, &Iinv, &Iref, &Irefinv, &mass, &mat, &aabb, &wpts, &wnorm, &dv, &bp_sb};
 	for (auto* a : dbl) a->free_();
+	iinv32.free_();
 	resid_acc.free_();
 	ray_io.free_();
 	ray_int.free_();
@@ -607,6 +609,7 @@ void cp_world::upload() {
 	Iinv.reserve(9 * (size_t)n);
 	aabb.reserve(6 * (size_t)n);
 	dv.reserve(12 * (size_t)n);
+	iinv32.reserve(8 * (size_t)n);
 	color_mask.reserve(4 * (size_t)n);
 	applied.reserve(n);
 	DArr<int>* sl_int[] = {&sl_asleep, &sl_asleep_rm, &sl_ticks, &sl_hlen, &sl_hhead, &sl_proc};
@@ -1705,6 +1708,7 @@ void cp_world::fused_substeps(const Bodies& b, const Params& P) {
 // Substep s of the coloured mode with the contact count read on the device.
 void cp_world::colored_substep(const Bodies& b, const Params& P, int s, const int* ncp) {
 	const int cap = cb[cur].c.cap;
+	if (P.precision == CP_PREC_SPLIT) launch_iinv32(b, stream);
 	launch_build_rows(b, cb[cur].c, cap, P, stream, ncp);
 	colored_pgs(b, P);
 	launch_rows_writeback(cb[cur].c, cap, P, stream, ncp);
@@ -1720,7 +1724,7 @@ void cp_world::graph_substeps(const Bodies& b, const Params& P) {
 	launch_set_count(d_nc.p, ncontacts, stream);
 	const Contacts& C = cb[cur].c;
 	std::vector<uint64_t> key = {(uint64_t)(uintptr_t)C.key, (uint64_t)(uintptr_t)C.row, (uint64_t)C.cap, (uint64_t)(uintptr_t)C.live,
-	                             (uint64_t)b.n, (uint64_t)b.nl, (uint64_t)(uintptr_t)b.com, (uint64_t)(uintptr_t)b.dv,
+	                             (uint64_t)b.n, (uint64_t)b.nl, (uint64_t)(uintptr_t)b.com, (uint64_t)(uintptr_t)b.dv, (uint64_t)(uintptr_t)b.iinv32,
 	                             (uint64_t)(uintptr_t)e_desc.p, (uint64_t)(uintptr_t)e_lmask.p, (uint64_t)(uintptr_t)color_off.p,
 	                             (uint64_t)(uintptr_t)ncolors.p, (uint64_t)(uintptr_t)edge_range.p, (uint64_t)(uintptr_t)scene_body_off.p,
 	                             (uint64_t)nscenes, (uint64_t)cfg.substeps, (uint64_t)cfg.solver_threads, (uint64_t)(uintptr_t)d_nc.p};
@@ -1843,6 +1847,7 @@ void cp_world::step() {
 	for (int s = 0; s < cfg.substeps && !use_graph && !use_fused; s++) {
 		// spring impulses come before any row is built (PhysicsEngine.cpp:1313-1355)
 		launch_springs(b, J, sp_ngroups, sp_goff.p, sp_list.p, P.sub_dt, stream);
+		if (P.precision == CP_PREC_SPLIT) launch_iinv32(b, stream);
 		launch_build_rows(b, cb[cur].c, ncontacts, P, stream);
 		// joints; holonomic ones are current at the first substep (PhysicsEngine.cpp:1396-1403)
 		launch_joint_rows(b, J, P, P.sub_dt, 0, s == 0, stream);
