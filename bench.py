@@ -73,8 +73,9 @@ def profiled_traffic(dtype, scenes, seed, settle, live, launch):
     """HBM bytes per PGS solve from the committed rocprofv3 PMC summary of the
     same bench workload (FETCH_SIZE and WRITE_SIZE, copy-kernel calibrated),
     or (None, None) unless the summary states the same scenes, seed and settle
-    steps and a live-contact count within 10% of this run's (the traffic per
-    solve scales with the contacts in the window)."""
+    steps, the same row-record size and a live-contact count within 10% of
+    this run's (the traffic per solve scales with the contacts in the
+    window)."""
     path = os.path.join(ROOT, "profiles", f"c3_batched_{dtype}_kernel_stats.md")
     if not os.path.exists(path):
         return None, None
@@ -84,8 +85,10 @@ def profiled_traffic(dtype, scenes, seed, settle, live, launch):
             if ":" in line and not line.startswith((" ", "#", "`")):
                 k, v = line.split(":", 1)
                 meta[k.strip()] = v.strip()
+    record = 512 if dtype == "fp64" else 128  # bytes of one contact's row record in the solve this build runs
     try:
         ok = (meta.get("launch", "solve") == launch and int(meta["scenes"]) == scenes and int(meta["seed"]) == seed and
+              int(meta.get("record_bytes", 512 if dtype == "fp64" else 256)) == record and
               int(meta["settle_steps"]) == settle and
               abs(float(meta["live_contacts"]) - live) <= 0.1 * float(meta["live_contacts"]))
         return (float(meta["traffic_bytes_per_launch"]), os.path.relpath(path, ROOT)) if ok else (None, None)
