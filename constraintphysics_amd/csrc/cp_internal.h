@@ -162,6 +162,7 @@ struct HSysDev {
 	// below the diagonal, rows ascending, as (cs_row, cs_loc)[cs_ptr[cp_off[s]
 	// + c], cs_ptr[cp_off[s] + c + 1]); the diagonal block at dg_loc[ord_off[s] + c]
 	int *cp_off, *cs_ptr, *cs_row, *cs_loc, *dg_loc;
+	int lds_doubles;  // the largest system's block buffer plus column scratch (the wave factor stages them in LDS)
 };
 
 // enabled: the arrays are valid and asleep bodies are honoured (some body

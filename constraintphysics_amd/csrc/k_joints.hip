@@ -7,7 +7,8 @@
 //
 // Holonomic systems (HolonomicBlockSolver.cpp): the symbolic part (elimination
 // order, block table) is host work done when the joint topology changes; the
-// numeric block LDL factor (:459-543) runs one thread per system, and the solve
+// numeric block LDL factor (:459-543) runs one wave per system on its blocks
+// staged in LDS (one thread per system for systems too large for LDS), and the solve
 // d = L^-T D^-1 L^-1 (t - c) (:223-355) runs inside the island's PGS thread
 // after the sweeps of the first substep.
 //
@@ -698,6 +699,106 @@ __global__ __launch_bounds__(64) void k_hsys_factor(Joints J, HSysDev H, double 
 	}
 }
 
+// One element (r, c) of block (c1 row joint, c2 column joint) of the initial
+// values: computeBlockInitialValue's arithmetic for that element alone.
+__device__ double hs_block_elem(const Joints& J, double cfm, int c1, int c2, int r, int c) {
+	const double* R1 = J.row + (size_t)c1 * CP_JRS;
+	const double* R2 = J.row + (size_t)c2 * CP_JRS;
+	if (c1 == c2) {
+		double s1 = 0, s2 = 0;
+		for (int q = 0; q < 6; q++) {
+			s1 += R1[CP_J_JA + r * 6 + q] * R1[CP_J_ITA + q * CP_JMAX + c];
+			s2 += R1[CP_J_JB + r * 6 + q] * R1[CP_J_ITB + q * CP_JMAX + c];
+		}
+		double t = s1 + s2;
+		if (r == c) t *= (1 + cfm);
+		return t;
+	}
+	int jac = -1, it = -1;
+	if (J.a[c1] == J.a[c2]) { jac = CP_J_JA; it = CP_J_ITA; }
+	else if (J.a[c1] == J.b[c2]) { jac = CP_J_JA; it = CP_J_ITB; }
+	else if (J.b[c1] == J.a[c2]) { jac = CP_J_JB; it = CP_J_ITA; }
+	else if (J.b[c1] == J.b[c2]) { jac = CP_J_JB; it = CP_J_ITB; }
+	if (jac < 0) return 0;
+	double t = 0;
+	for (int q = 0; q < 6; q++) t += R1[jac + r * 6 + q] * R2[it + q * CP_JMAX + c];
+	return t;
+}
+
+// HolonomicSystem::computeInverse (:459-543), one wave per system on its block
+// buffer and column scratch staged in LDS: every step's block elements (the
+// initial values, the lower-triangle blocks of a column, the updates of the
+// blocks below and right of it, the copy back) run side by side, each element
+// with the reference's operation order, and each diagonal block is inverted
+// by one lane, so the factor equals k_hsys_factor's bit for bit.  A column's
+// blocks below the diagonal lie contiguously in row order from its first one,
+// each row-major.
+__global__ __launch_bounds__(64) void k_hsys_factor_wave(Joints J, HSysDev H, double cfm) {
+	extern __shared__ double sm[];
+	const int s = blockIdx.x, lane = threadIdx.x;
+	const int nc = H.nc[s];
+	const int* ord = H.order + H.ord_off[s];
+	const int cap = H.buf_off[s + 1] - H.buf_off[s];
+	double* buf = sm;
+	double* dbuf = sm + cap;
+	const int* cs_ptr = H.cs_ptr + H.cp_off[s];  // per column: its blocks below the diagonal, rows ascending
+	auto sync = [] {
+		__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+		__builtin_amdgcn_wave_barrier();
+	};
+	// initial values of every non-empty block, column by column
+	for (int col = 0; col < nc; col++) {
+		const int bw = J.type[ord[col]];
+		const int dl = H.dg_loc[H.ord_off[s] + col];
+		for (int e = lane; e < bw * bw; e += 64) buf[dl + e] = hs_block_elem(J, cfm, ord[col], ord[col], e / bw, e % bw);
+		for (int x = cs_ptr[col]; x < cs_ptr[col + 1]; x++) {
+			const int row = H.cs_row[x], l = H.cs_loc[x], bh = J.type[ord[row]];
+			for (int e = lane; e < bh * bw; e += 64) buf[l + e] = hs_block_elem(J, cfm, ord[row], ord[col], e / bw, e % bw);
+		}
+	}
+	sync();
+	for (int col = 0; col < nc; col++) {
+		const int bw = J.type[ord[col]];
+		double* diag = buf + H.dg_loc[H.ord_off[s] + col];
+		if (lane == 0) {
+			double tmp[36];
+			inverse_n(bw, diag, tmp);
+			for (int q = 0; q < bw * bw; q++) diag[q] = tmp[q];
+		}
+		sync();
+		const int x0 = cs_ptr[col], x1 = cs_ptr[col + 1];
+		if (x0 == x1) continue;
+		const int first = H.cs_loc[x0];
+		const int total = hs_loc(H, s, col + 1, col + 1) - first;
+		// lower-triangle blocks of the column into the scratch
+		for (int e = lane; e < total; e += 64) {
+			int x = x0;
+			while (x + 1 < x1 && H.cs_loc[x + 1] - first <= e) x++;
+			const int rel = e - (H.cs_loc[x] - first), r = rel / bw, c = rel - r * bw;
+			dbuf[e] = dotp(bw, buf + H.cs_loc[x] + bw * r, diag + bw * c);
+		}
+		sync();
+		// the blocks below and right of the column
+		for (int xc = x0; xc < x1; xc++)
+			for (int xr = xc; xr < x1; xr++) {
+				const int ldc = H.cs_row[xc], ldr = H.cs_row[xr];
+				const int w2 = J.type[ord[ldc]], h2 = J.type[ord[ldr]];
+				const double* src = buf + H.cs_loc[xr];
+				const double* lsrc = dbuf + (H.cs_loc[xc] - first);
+				double* tgt = buf + hs_loc(H, s, ldr, ldc);
+				for (int e = lane; e < h2 * w2; e += 64) {
+					const int r = e / w2, c = e - r * w2;
+					tgt[r * w2 + c] -= dotp(bw, src + r * bw, lsrc + c * bw);
+				}
+			}
+		sync();
+		for (int e = lane; e < total; e += 64) buf[first + e] = dbuf[e];
+		sync();
+	}
+	double* out = H.buf + H.buf_off[s];
+	for (int e = lane; e < cap; e += 64) out[e] = buf[e];
+}
+
 static inline int nblk(int n, int t) { return (n + t - 1) / t; }
 
 void launch_joint_init(const Bodies& b, const Joints& J, int k0, int k1, hipStream_t s) {
@@ -710,7 +811,17 @@ void launch_joint_rows(const Bodies& b, const Joints& J, const Params& P, double
 	if (J.n) hipLaunchKernelGGL(k_joint_rows, dim3(nblk(J.n, 64)), dim3(64), 0, s, b, J, P, sub_dt, mode, first);
 }
 void launch_hsys_factor(const Joints& J, const HSysDev& H, double cfm, hipStream_t s) {
-	if (H.nsys) hipLaunchKernelGGL(k_hsys_factor, dim3(nblk(H.nsys, 64)), dim3(64), 0, s, J, H, cfm);
+	if (!H.nsys) return;
+	const size_t lds = (size_t)H.lds_doubles * sizeof(double);
+	if (lds > 0 && lds <= 64 * 1024) {
+		static bool attr = false;
+		if (!attr) {
+			(void)hipFuncSetAttribute((const void*)k_hsys_factor_wave, hipFuncAttributeMaxDynamicSharedMemorySize, 64 * 1024);
+			attr = true;
+		}
+		hipLaunchKernelGGL(k_hsys_factor_wave, dim3(H.nsys), dim3(64), lds, s, J, H, cfm);
+	} else
+		hipLaunchKernelGGL(k_hsys_factor, dim3(nblk(H.nsys, 64)), dim3(64), 0, s, J, H, cfm);
 }
 
 }  // namespace cpx

@@ -266,6 +266,7 @@ struct cp_world {
 	DArr<int> dh_cp_off, dh_cs_ptr, dh_cs_row, dh_cs_loc, dh_dg_loc;
 	DArr<double> dh_buf, dh_dbuf, dh_scratch;
 	bool hs_pending = true;
+	int hs_lds_doubles = 0;  // largest system's buffer + column scratch, in doubles
 	// island schedule (built on the device each step of the island modes)
 	DArr<int> isl_jlin, isl_eoff, isl_coff, isl_icoff, isl_ncol, isl_serial, isl_bod, isl_nbod, isl_cloc, isl_jloc, isl_sys_cnt;
 	DArr<int> isl_root, isl_edge_off, isl_body_off, isl_body_cnt, isl_stack_off, isl_joint_off;
@@ -436,6 +437,7 @@ struct cp_world {
 		H.degree = dh_degree.p; H.buf_off = dh_buf_off.p; H.dbuf_off = dh_dbuf_off.p; H.buf = dh_buf.p; H.dbuf = dh_dbuf.p;
 		H.scratch = dh_scratch.p;
 		H.cp_off = dh_cp_off.p; H.cs_ptr = dh_cs_ptr.p; H.cs_row = dh_cs_row.p; H.cs_loc = dh_cs_loc.p; H.dg_loc = dh_dg_loc.p;
+		H.lds_doubles = hs_lds_doubles;
 		return H;
 	}
 	MeshDev meshdev() {
@@ -921,6 +923,9 @@ void cp_world::layout_systems() {
 	}
 	up(dh_nc, nc); up(dh_ord_off, ord_off); up(dh_order, order); up(dh_vloc, vloc); up(dh_tab_off, tab_off); up(dh_table, table);
 	up(dh_degree, degree); up(dh_buf_off, buf_off); up(dh_dbuf_off, dbuf_off);
+	hs_lds_doubles = 0;
+	for (size_t q = 0; q + 1 < buf_off.size(); q++)
+		hs_lds_doubles = std::max(hs_lds_doubles, (buf_off[q + 1] - buf_off[q]) + (dbuf_off[q + 1] - dbuf_off[q]));
 	if (cs_row.empty()) { cs_row.push_back(0); cs_loc.push_back(0); }
 	if (cp_off.empty()) cp_off.push_back(0);
 	if (cs_ptr.empty()) cs_ptr.push_back(0);
