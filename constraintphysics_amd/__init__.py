@@ -121,6 +121,7 @@ def lib():
         "cp_world_retire_scene": (_I, [_P, _I]),
         "cp_world_serving_commit": (_I, [_P]),
         "cp_world_get_scene_state": (_I, [_P, _I, _P, _P]),
+        "cp_world_get_scene_states": (_I, [_P, _I, _P, _I, _P, _P, _P]),
         "cp_world_serving_info": (_I, [_P, _P]),
         "cp_joint_add": (_I, [_P, _I, _I, _I, _P, _P]),
         "cp_world_joint_count": (_I, [_P, _P]),
@@ -612,6 +613,20 @@ class World:
 
     def retire_scene(self, handle):
         _chk(self.L.cp_world_retire_scene(self.w, int(handle)), "cp_world_retire_scene")
+
+    def scene_states(self, handles, rows):
+        """Results of several admitted scenes in one call: a list of (rows, finite)
+        per handle; rows = an upper bound on their total body count."""
+        hs = _a(list(handles), np.int32)
+        if len(hs) == 0:
+            return []
+        out = np.zeros((int(rows), 13))
+        cnt = np.zeros(len(hs), np.int32)
+        st = np.zeros(len(hs), np.int32)
+        _chk(self.L.cp_world_get_scene_states(self.w, len(hs), hs.ctypes.data, int(rows), out.ctypes.data, cnt.ctypes.data, st.ctypes.data),
+             "cp_world_get_scene_states")
+        off = np.concatenate([[0], np.cumsum(cnt)])
+        return [(out[off[k]:off[k + 1]], st[k] != CP_ERR_NONFINITE) for k in range(len(hs))]
 
     def scene_state(self, handle, with_status=False):
         """State rows of an admitted scene; with_status: (rows, finite) where

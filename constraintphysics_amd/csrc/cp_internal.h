@@ -401,6 +401,8 @@ void launch_integrate(const Bodies& b, const Params& p, hipStream_t s);
 // velocity: body_flag[i] = 1 and any[0] = 1.  Fill: p[0..n) = v.
 void launch_check_finite(const Bodies& b, int* body_flag, int* any, hipStream_t s);
 void launch_fill_int(int* p, int n, int v, hipStream_t s);
+void launch_gather_states(const Bodies& b, const int* start, const int* off, int nsc, int rows, const int* body_flag, double* out, int* flag,
+                          hipStream_t s);
 // p[list[k]] = 0 for k < n.
 void launch_zero_list(int* p, const int* list, int n, hipStream_t s);
 void launch_bp_keys(const Bodies& b, uint64_t* keys, int* vals, hipStream_t s);

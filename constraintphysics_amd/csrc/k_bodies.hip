@@ -229,6 +229,29 @@ __global__ void k_check_finite(Bodies b, int* body_flag, int* any) {
 	body_flag[i] = 1;
 	any[0] = 1;
 }
+// Rows of many scenes gathered back to back: output row r belongs to scene k
+// with off[k] <= r < off[k+1] and is body start[k] + r - off[k]; 13 doubles
+// (com, q, vel, angv) per row; flag[k] set when one of its bodies was found
+// non-finite at the end of the last step.
+__global__ void k_gather_states(Bodies b, const int* start, const int* off, int nsc, const int* body_flag, double* out, int* flag) {
+	const int r = blockIdx.x * blockDim.x + threadIdx.x;
+	if (r >= off[nsc]) return;
+	int lo = 0, hi = nsc - 1;
+	while (lo < hi) {  // last scene with off <= r
+		const int mid = (lo + hi + 1) >> 1;
+		if (off[mid] <= r) lo = mid;
+		else hi = mid - 1;
+	}
+	const int i = start[lo] + r - off[lo], n = b.n;
+	double* o = out + 13 * (size_t)r;
+	for (int c = 0; c < 3; c++) {
+		o[c] = b.com[c * n + i];
+		o[7 + c] = b.vel[c * n + i];
+		o[10 + c] = b.angv[c * n + i];
+	}
+	for (int c = 0; c < 4; c++) o[3 + c] = b.q[c * n + i];
+	if (body_flag[i]) flag[lo] = 1;
+}
 __global__ void k_fill_int(int* p, int n, int v) {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i < n) p[i] = v;
@@ -242,6 +265,10 @@ void launch_zero_list(int* p, const int* list, int n, hipStream_t s) {
 }
 void launch_check_finite(const Bodies& b, int* body_flag, int* any, hipStream_t s) {
 	if (b.nl) hipLaunchKernelGGL(k_check_finite, dim3(nblk(b.nl, 256)), dim3(256), 0, s, b, body_flag, any);
+}
+void launch_gather_states(const Bodies& b, const int* start, const int* off, int nsc, int rows, const int* body_flag, double* out, int* flag,
+                          hipStream_t s) {
+	if (rows > 0) hipLaunchKernelGGL(k_gather_states, dim3(nblk(rows, 256)), dim3(256), 0, s, b, start, off, nsc, body_flag, out, flag);
 }
 void launch_fill_int(int* p, int n, int v, hipStream_t s) {
 	if (n > 0) hipLaunchKernelGGL(k_fill_int, dim3(nblk(n, 256)), dim3(256), 0, s, p, n, v);

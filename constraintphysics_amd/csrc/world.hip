@@ -330,6 +330,8 @@ struct cp_world {
 	};
 	std::vector<Range> ranges;
 	int next_handle = 0;
+	DArr<int> gs_idx, gs_flag;  // cp_world_get_scene_states: scene starts and row offsets, non-finite flags
+	DArr<double> gs_rows;       // its gathered rows
 	std::vector<int> stage_i, kill_ranges, stage_mark;
 	std::vector<double> stage_d;
 	int commit_id = 0;
@@ -500,6 +502,9 @@ void cp_world::free_all() {
 	                       &prev_com, &prev_q, &I, &Iinv, &Iref, &Irefinv, &mass, &mat, &aabb, &wpts, &wnorm, &dv, &bp_sb};
 	for (auto* a : dbl) a->free_();
 	iinv32.free_();
+	gs_idx.free_();
+	gs_flag.free_();
+	gs_rows.free_();
 	resid_acc.free_();
 	ray_io.free_();
 	ray_int.free_();
@@ -3100,6 +3105,36 @@ int cp_world_get_scene_state(cp_world* w, int handle, double* state13, int* n) {
 		HIPCHK(hipMemcpy(f.data(), w->nf_body.p + R.start, R.len * sizeof(int), hipMemcpyDeviceToHost));
 		for (int v : f)
 			if (v) throw ApiErr{CP_ERR_NONFINITE};  // the state is returned; the status names the divergence
+	});
+}
+
+int cp_world_get_scene_states(cp_world* w, int n, const int* handles, int cap_rows, double* state13, int* counts, int* status) {
+	if (!w || !w->serving || n < 0 || (n > 0 && (!handles || !counts || !status)) || cap_rows < 0) return CP_ERR_INVALID;
+	std::vector<int> start(n), off(n + 1, 0);
+	for (int k = 0; k < n; k++) {
+		const int r = find_range(w, handles[k]);
+		if (r < 0) return CP_ERR_INVALID;
+		start[k] = w->ranges[r].start;
+		counts[k] = w->ranges[r].len;
+		off[k + 1] = off[k] + counts[k];
+	}
+	if (off[n] > cap_rows || (off[n] > 0 && !state13)) return CP_ERR_CAPACITY;
+	return guard(w, [&] {
+		w->serving_commit();
+		if (!n) return;
+		// one gather on the device, one copy back
+		w->gs_idx.reserve(2 * (size_t)n + 1);
+		std::vector<int> idx(start);
+		idx.insert(idx.end(), off.begin(), off.end());
+		HIPCHK(hipMemcpyAsync(w->gs_idx.p, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice, w->stream));
+		w->gs_rows.reserve(13 * (size_t)std::max(off[n], 1));
+		w->gs_flag.reserve((size_t)n);
+		HIPCHK(hipMemsetAsync(w->gs_flag.p, 0, (size_t)n * sizeof(int), w->stream));
+		launch_gather_states(w->bodies(), w->gs_idx.p, w->gs_idx.p + n, n, off[n], w->nf_body.p, w->gs_rows.p, w->gs_flag.p, w->stream);
+		HIPCHK(hipMemcpyAsync(state13, w->gs_rows.p, 13 * (size_t)off[n] * sizeof(double), hipMemcpyDeviceToHost, w->stream));
+		HIPCHK(hipMemcpyAsync(status, w->gs_flag.p, (size_t)n * sizeof(int), hipMemcpyDeviceToHost, w->stream));
+		HIPCHK(hipStreamSynchronize(w->stream));
+		for (int k = 0; k < n; k++) status[k] = status[k] ? CP_ERR_NONFINITE : CP_OK;
 	});
 }
 

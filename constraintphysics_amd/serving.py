@@ -122,9 +122,10 @@ def serve(world, source, requests, keep_states=False, log=None):
                 done.append(h)
         body_steps += tick_bs
         tick_log.append((t - t0, tick_bs))
-        for h in done:
+        # the finished scenes' results in one read
+        results = world.scene_states(done, sum(len(payloads[active[h][0].rid][0]) for h in done))
+        for h, (st, finite) in zip(done, results):
             r = active.pop(h)[0]
-            st, finite = world.scene_state(h, with_status=True)
             if not finite:
                 diverged.append(r.rid)
             if keep_states:

@@ -420,6 +420,14 @@ int cp_world_admit_scene(cp_world* w, int n, const int* shape_ids, const int* mo
 int cp_world_retire_scene(cp_world* w, int handle);
 int cp_world_serving_commit(cp_world* w);
 int cp_world_get_scene_state(cp_world* w, int handle, double* state13, int* n);
+/* The results of many admitted scenes in one call (one synchronisation, one
+ * gather on the device, one copy back), the serving loop's per-tick read of
+ * the scenes it retires: state13 receives their rows back to back in handle
+ * order, counts[k] scene k's body count, status[k] CP_OK or CP_ERR_NONFINITE
+ * (a body of that scene went non-finite).  CP_ERR_CAPACITY when the scenes
+ * hold more than cap_rows rows.  No counterpart in the reference, which reads
+ * its bodies one by one (RigidBody getters, RigidBody.h:25-68). */
+int cp_world_get_scene_states(cp_world* w, int n, const int* handles, int cap_rows, double* state13, int* counts, int* status);
 int cp_world_serving_info(cp_world* w, int* info4);
 
 /* Snapshots and trajectories.  The reference records runs as frames of 7

@@ -78,6 +78,15 @@ def test_serving_reports_the_diverged_scene():
     assert not w.step(2)
     status = [w.scene_state(h, with_status=True)[1] for h in handles]
     assert status == [True, False, True], status
+    # the batched read returns the same rows and flags, in the order asked for
+    order = [handles[2], handles[0], handles[1]]
+    batch = w.scene_states(order, 3 * 200)
+    for h, (rows, finite) in zip(order, batch):
+        one, ok = w.scene_state(h, with_status=True)
+        assert finite == ok
+        assert np.array_equal(rows, one, equal_nan=True)
+    with pytest.raises(cpa.CpError):
+        w.scene_states(order, 10)  # fewer rows than the scenes hold
     with pytest.raises(cpa.CpError):
         w.scene_state(handles[1])
     # retiring the scene and admitting a fresh one into its range clears the flag
