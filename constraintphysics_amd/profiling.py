@@ -2,6 +2,11 @@
 
     python3 -m constraintphysics_amd.profiling DIR [--first-step 241] [--steps 20]
         [--fetch-factor 2.0] [--write-factor 1.0] [--contacts N] [--prefix c3]
+    python3 -m constraintphysics_amd.profiling --calibrate
+
+--calibrate runs the plain copy kernel the counter factors come from: an
+elementwise float32 product over 64 Mi elements (256 MiB read, 256 MiB
+written), five times; profile it with the same counters as the bench.
 
 DIR holds `<prefix>_kernel_trace.csv` (rocprofv3 --kernel-trace -o <prefix>) and, optionally,
 `fetch_counter_collection.csv` / `write_counter_collection.csv` (rocprofv3
@@ -117,7 +122,23 @@ def kernel_counters(d, name, first_step=241, steps=20, factor=1.0):
     return {key: (len(v), sum(v) / len(v)) for key, v in acc.items()}
 
 
+def calibrate(reps=5, n=64 * 1024 * 1024):
+    """The copy kernel of known size: y = 2 x over n float32 values (4n bytes
+    read, 4n bytes written per dispatch)."""
+    import torch
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        torch.mul(x, 2.0, out=y)
+    torch.cuda.synchronize()
+    print(f"copy kernel: {reps} dispatches of {4 * n / 2**20:.0f} MiB read + {4 * n / 2**20:.0f} MiB written")
+
+
 def main():
+    import sys
+    if "--calibrate" in sys.argv:
+        calibrate()
+        return
     ap = argparse.ArgumentParser()
     ap.add_argument("dir")
     ap.add_argument("--first-step", type=int, default=241)
