@@ -67,6 +67,8 @@ struct Bodies {
 	double* wnorm;    // [3*WG]
 	double* dv;       // [12][n] PGS velocity / pseudo-velocity deltas
 	float* iinv32;    // [n][8] world inverse inertia in FP32 for the split solve: xx, xy, xz, yy, yz, zz (0 unless dynamic), 2 unused
+	int4* geo;        // [n][3] per body, rebuilt at each narrowphase: (shape type, shape, world point offset, world normal
+	                  // offset), (first point, points, first face, faces), (first arc, arcs, first edge, edges)
 	int* color_mask;  // scratch for colouring (2 x u64 per body, as 4 ints)
 	// sleeping (null while no body can be asleep): RigidBody::asleep, and the
 	// asleep flags of the last step's first substep, when the reference

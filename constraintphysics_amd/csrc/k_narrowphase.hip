@@ -80,6 +80,53 @@ __device__ Hull make_hull(const Bodies& b, const Shapes& sh, int body) {
 	return h;
 }
 
+// The same hull from the body's geometry record (Bodies::geo, three
+// independent 16-byte loads instead of the chain body -> shape -> offsets).
+__device__ Hull make_hull_g(const Bodies& b, const Shapes& sh, int body) {
+	const int4* g = b.geo + 3 * (size_t)body;
+	const int4 g0 = g[0], g1 = g[1], g2 = g[2];
+	Hull h;
+	h.wp = b.wpts + 3 * (size_t)g0.z;
+	h.wn = b.wnorm + 3 * (size_t)g0.w;
+	h.p0 = g1.x;
+	h.np = g1.y;
+	h.f0 = g1.z;
+	h.nf = g1.w;
+	h.na = g2.y;
+	h.ne = g2.w;
+	h.fv_off = sh.fv_off;
+	h.fv = sh.fv;
+	h.ncalc = sh.f_ncalc + h.f0;
+	h.ndir = sh.f_ndir + h.f0;
+	h.sid = sh.f_sid + h.f0;
+	h.red = sh.g_red + h.f0;
+	h.minpid = sh.g_minpid + h.f0;
+	h.maxpid = sh.g_maxpid + h.f0;
+	h.refidx = sh.g_refidx + h.f0;
+	h.gmin = sh.g_min + h.f0;
+	h.gmax = sh.g_max + h.f0;
+	h.grefval = sh.g_refval + h.f0;
+	h.arcs = sh.arcs + 2 * g2.x;
+	h.edges = sh.edges + 2 * g2.z;
+	h.vaf_off = sh.va_face_off;
+	h.vaf = sh.va_faces;
+	h.vae_off = sh.va_edge_off;
+	h.vae = sh.va_edges;
+	return h;
+}
+
+// Geometry records of the bodies the step visits.
+__global__ void k_body_geo(Bodies b, Shapes sh) {
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= b.nl) return;
+	const int s = b.shape[i];
+	const int p0 = sh.pt_off[s], f0 = sh.face_off[s], a0 = sh.arc_off[s], e0 = sh.edge_off[s];
+	int4* g = b.geo + 3 * (size_t)i;
+	g[0] = int4{sh.type[s], s, b.wpt_off[i], b.wgv_off[i]};
+	g[1] = int4{p0, sh.pt_off[s + 1] - p0, f0, sh.face_off[s + 1] - f0};
+	g[2] = int4{a0, sh.arc_off[s + 1] - a0, e0, sh.edge_off[s + 1] - e0};
+}
+
 struct Ext {
 	int min_pid, max_pid;
 	double min_val, max_val;
@@ -659,6 +706,20 @@ struct Cyl {
 	double r, h;
 	const double *top, *bot, *gv;
 };
+__device__ Cyl make_cyl_g(const Bodies& b, const Shapes& sh, int body) {
+	const int4 g0 = b.geo[3 * (size_t)body];
+	const double* wp = b.wpts + 3 * (size_t)g0.z;
+	const double* wn = b.wnorm + 3 * (size_t)g0.w;
+	Cyl c;
+	c.c = ldv(wp, 0);
+	c.top = wp + 3;
+	c.bot = wp + 27;
+	c.ax = ldv(wn, 0);
+	c.gv = wn + 3;
+	c.r = sh.radius[g0.y];
+	c.h = sh.height[g0.y];
+	return c;
+}
 __device__ Cyl make_cyl(const Bodies& b, const Shapes& sh, int body) {
 	int s = b.shape[body];
 	const double* wp = b.wpts + 3 * b.wpt_off[body];
@@ -934,7 +995,7 @@ __device__ void cyl_cyl_contacts(const Cyl& A, const Cyl& B, const Check& mp, co
 __device__ void sphere_cyl(const Bodies& b, const Shapes& sh, int is_, int ic, bool swapped, int slot, Manifolds& M, int cap) {
 	V3 c = ldv(b.wpts + 3 * b.wpt_off[is_], 0);
 	double r = sh.radius[b.shape[is_]];
-	Cyl B = make_cyl(b, sh, ic);
+	Cyl B = make_cyl_g(b, sh, ic);
 	Check mp{-1, -1, V3{0, 0, 0}, __builtin_inf()};
 	V3 dirs[4];
 	V3 diff = B.c - c;
@@ -1259,7 +1320,7 @@ __global__ __launch_bounds__(64 * NP_WAVES) __attribute__((amdgpu_waves_per_eu(4
 		int slot = __builtin_amdgcn_readfirstlane(list[w]);
 		uint64_t key = pairs[slot];
 		int ia = __builtin_amdgcn_readfirstlane((int)(key >> 32)), ib = __builtin_amdgcn_readfirstlane((int)(key & 0xFFFFFFFFu));
-		Hull A = make_hull(b, sh, ia), B = make_hull(b, sh, ib);
+		Hull A = make_hull_g(b, sh, ia), B = make_hull_g(b, sh, ib);
 		stage_hull(A, L.pa, L.na, lane);
 		stage_hull(B, L.pb, L.nb, lane);
 		stage_arcs(A, L.sat.arca, lane);
@@ -1316,10 +1377,10 @@ __global__ __launch_bounds__(64 * NP_WAVES) __attribute__((amdgpu_waves_per_eu(4
 		int slot = __builtin_amdgcn_readfirstlane(list[w]);
 		uint64_t key = pairs[slot];
 		int ia = __builtin_amdgcn_readfirstlane((int)(key >> 32)), ib = __builtin_amdgcn_readfirstlane((int)(key & 0xFFFFFFFFu));
-		bool swapped = sh.type[b.shape[ia]] == SHAPE_CYLINDER;  // detectCollision's reversed form (CollisionDetect.cpp:122-130)
+		bool swapped = __builtin_amdgcn_readfirstlane(b.geo[3 * (size_t)ia].x) == SHAPE_CYLINDER;  // detectCollision's reversed form (CollisionDetect.cpp:122-130)
 		int ip = swapped ? ib : ia, ic = swapped ? ia : ib;
-		Hull A = make_hull(b, sh, ip);
-		Cyl B = make_cyl(b, sh, ic);
+		Hull A = make_hull_g(b, sh, ip);
+		Cyl B = make_cyl_g(b, sh, ic);
 		stage_hull(A, L.pa, L.na, lane);
 		stage_arcs(A, L.sat.arca, lane);
 		stage_cyl(B, L.nb, lane);
@@ -1374,7 +1435,7 @@ __global__ __launch_bounds__(64 * NP_WAVES) __attribute__((amdgpu_waves_per_eu(4
 		int slot = __builtin_amdgcn_readfirstlane(list[w]);
 		uint64_t key = pairs[slot];
 		int ia = __builtin_amdgcn_readfirstlane((int)(key >> 32)), ib = __builtin_amdgcn_readfirstlane((int)(key & 0xFFFFFFFFu));
-		Cyl A = make_cyl(b, sh, ia), B = make_cyl(b, sh, ib);
+		Cyl A = make_cyl_g(b, sh, ia), B = make_cyl_g(b, sh, ib);
 		stage_cyl(A, L.na, lane);
 		stage_cyl(B, L.nb, lane);
 		int bi;
@@ -1406,7 +1467,7 @@ __global__ __launch_bounds__(64 * NP_WAVES) void k_np_cyl_cyl_contacts(Bodies b,
 		if (code >= -1) continue;
 		uint64_t key = pairs[slot];
 		int ia = __builtin_amdgcn_readfirstlane((int)(key >> 32)), ib = __builtin_amdgcn_readfirstlane((int)(key & 0xFFFFFFFFu));
-		Cyl A = make_cyl(b, sh, ia), B = make_cyl(b, sh, ib);
+		Cyl A = make_cyl_g(b, sh, ia), B = make_cyl_g(b, sh, ib);
 		stage_cyl(A, sgv[wib][0], lane);
 		stage_cyl(B, sgv[wib][1], lane);
 		if (lane == 0) {
@@ -1430,9 +1491,9 @@ __global__ __launch_bounds__(64 * NP_WAVES) void k_np_hull_sphere(Bodies b, Shap
 		int slot = __builtin_amdgcn_readfirstlane(list[w]);
 		uint64_t key = pairs[slot];
 		int ia = __builtin_amdgcn_readfirstlane((int)(key >> 32)), ib = __builtin_amdgcn_readfirstlane((int)(key & 0xFFFFFFFFu));
-		bool swapped = sh.type[b.shape[ia]] == SHAPE_SPHERE;
+		bool swapped = __builtin_amdgcn_readfirstlane(b.geo[3 * (size_t)ia].x) == SHAPE_SPHERE;
 		const int ip = swapped ? ib : ia;
-		Hull A = make_hull(b, sh, ip);
+		Hull A = make_hull_g(b, sh, ip);
 		stage_hull(A, sp[wib], sn[wib], lane);
 		poly_sphere(b, sh, A, ip, swapped ? ia : ib, swapped, lane, slot, M, cap);
 		wave_fence();
@@ -1961,6 +2022,7 @@ void launch_narrowphase(const Bodies& b, const Shapes& sh, const Params& p, cons
                         int cap, int* lists, int* counts, const uint64_t* nocol, int n_nocol, const int* rank, hipStream_t s, const Fork* fk) {
 	if (npairs <= 0) return;
 	hipLaunchKernelGGL(k_np_zero, dim3(1), dim3(64), 0, s, counts);
+	if (b.nl) hipLaunchKernelGGL(k_body_geo, dim3((b.nl + 255) / 256), dim3(256), 0, s, b, sh);
 	hipLaunchKernelGGL(k_np_classify, dim3((npairs + 64 * CLS_WAVES - 1) / (64 * CLS_WAVES)), dim3(64 * CLS_WAVES), 0, s, b, sh, pairs, npairs,
 	                   m, lists, counts, nocol, n_nocol, rank);
 	int wblocks = (npairs + NP_WAVES - 1) / NP_WAVES;

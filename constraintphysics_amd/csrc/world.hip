@@ -177,6 +177,7 @@ struct cp_world {
 	// device: bodies
 	DArr<double> com, q, vel, angv, pvel, pang, prev_com, prev_q, I, Iinv, Iref, Irefinv, mass, mat, aabb, wpts, wnorm, dv;
 	DArr<float> iinv32;
+	DArr<int4> bgeo;
 	DArr<int> movement, shape, scene, wpt_off, wgv_off, color_mask, scene_body_off, scene_of_body;
 	int nb = 0, nscenes = 0;
 	bool uploaded = false;
@@ -376,7 +377,7 @@ struct cp_world {
 		b.movement = movement.p; b.shape = shape.p; b.scene = scene.p;
 		b.mat = mat.p; b.aabb = aabb.p;
 		b.wpt_off = wpt_off.p; b.wpts = wpts.p; b.wgv_off = wgv_off.p; b.wnorm = wnorm.p;
-		b.dv = dv.p; b.iinv32 = iinv32.p; b.color_mask = color_mask.p;
+		b.dv = dv.p; b.iinv32 = iinv32.p; b.geo = bgeo.p; b.color_mask = color_mask.p;
 		b.asleep = sl_active ? sl_asleep.p : nullptr;
 		b.asleep_rm = sl_active ? sl_asleep_rm.p : nullptr;
 		return b;
@@ -502,6 +503,7 @@ void cp_world::free_all() {
 	                       &prev_com, &prev_q, &I, &Iinv, &Iref, &Irefinv, &mass, &mat, &aabb, &wpts, &wnorm, &dv, &bp_sb};
 	for (auto* a : dbl) a->free_();
 	iinv32.free_();
+	bgeo.free_();
 	gs_idx.free_();
 	gs_flag.free_();
 	gs_rows.free_();
@@ -617,6 +619,7 @@ void cp_world::upload() {
 	aabb.reserve(6 * (size_t)n);
 	dv.reserve(12 * (size_t)n);
 	iinv32.reserve(8 * (size_t)n);
+	bgeo.reserve(3 * (size_t)n);
 	color_mask.reserve(4 * (size_t)n);
 	applied.reserve(n);
 	DArr<int>* sl_int[] = {&sl_asleep, &sl_asleep_rm, &sl_ticks, &sl_hlen, &sl_hhead, &sl_proc};
