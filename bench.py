@@ -196,13 +196,16 @@ def c2_leg(mode=2, warmup=150, steps=30):
     """Holonomic chains on a heightfield (512 chains x 16 cylinder links, ball-
     sockets and one hinge per chain, 256 x 256-cell value-noise terrain of
     amplitude 2 m): island PGS (mode 1 reference order, mode 2 coloured
-    islands) with the holonomic LDL block solve at the first substep, FP64
-    throughout."""
+    islands) with the holonomic LDL block solve at the first substep.  Mode 2
+    solves the contacts in the split precision (FP32 contact rows; joint
+    rows, the LDL and the deltas FP64), as the configuration states; mode 1,
+    the parity mode, is FP64 throughout."""
     import constraintphysics_amd as cpa
     from constraintphysics_amd import scenes
     t = time.perf_counter()
     desc = scenes.c2_description()
-    w = cpa.World(dt=1.0 / 120.0, gravity=(0.0, -6.0, 0.0), vel_iters=10, pos_iters=1, substeps=8, pgs_mode=mode, hol_iters=1)
+    w = cpa.World(dt=1.0 / 120.0, gravity=(0.0, -6.0, 0.0), vel_iters=10, pos_iters=1, substeps=8, pgs_mode=mode, hol_iters=1,
+                  precision=PRECISIONS["split" if mode == 2 else "fp64"])
     scenes.build_description(w, desc)
     n_dyn = int((desc["movement"] == 0).sum())
     print(f"[bench] c2: {n_dyn} links, {len(desc['joints'][0])} joints, {len(desc['hulls'][0]['tris'])} triangles, "
@@ -224,7 +227,8 @@ def c2_leg(mode=2, warmup=150, steps=30):
     res = dict(value=round(n_dyn * steps / float(lat.sum()), 1), unit="body-steps/s", ms_per_step=round(1e3 * float(lat.mean()), 3),
                p50_step_ms=round(1e3 * float(np.percentile(lat, 50)), 3), p99_step_ms=round(1e3 * float(np.percentile(lat, 99)), 3),
                steps=steps, warmup=warmup, links=n_dyn, joints=int(len(desc["joints"][0])), holonomic_systems=len(sy),
-               triangles=int(len(desc["hulls"][0]["tris"])), live_contacts=contacts, phases_ms=phases, dtype="f64",
+               triangles=int(len(desc["hulls"][0]["tris"])), live_contacts=contacts, phases_ms=phases,
+               dtype="f32 contact rows / f64 joint rows, LDL and state" if mode == 2 else "f64",
                workload="512 chains x 16 cylinder links (r 0.15, h 0.8), ball-sockets + 1 hinge per chain (LDL block degree 47), "
                         "256x256-cell heightfield (cell 0.5 m, value noise, amplitude 2 m); dt 1/120, 8 substeps, PGS 10 vel + 1 "
                         "pos sweeps, 1 holonomic iteration; " + C2_SOLVES[mode])

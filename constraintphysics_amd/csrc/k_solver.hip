@@ -888,6 +888,17 @@ struct DvMap {
 	int stride;
 	const int* cloc;  // contact k -> local (body 1, body 2); null: global indices
 	const int* jloc;  // joint k -> local (a, b)
+	int split;        // contact rows are the split solve's compact FP32 records
+};
+
+// An FP64 delta accessor seen in FP32 by the split contact rows: the row
+// reads the shared deltas rounded to FP32 and adds its FP32 changes back.
+template <class D>
+struct DvAsF32 {
+	const D& d;
+	bool dyn;
+	__device__ float get(int c) const { return (float)d.get(c); }
+	__device__ void add(int c, float v) const { d.add(c, (double)v); }
 };
 
 __device__ __forceinline__ void contact_refs(const Bodies& b, const Contacts& C, const DvMap& m, int k, DG& A, DG& B) {
@@ -978,6 +989,19 @@ __device__ __forceinline__ void island_row(const Bodies& b, const Contacts& C, c
 	if (phase == 2 && kind != 0) return;
 	DG A, B;
 	contact_refs(b, C, m, k, A, B);
+	if (m.split) {
+		// a contact's normal and friction rows are adjacent in every island row
+		// list (Contact::addSolverConstraints): both run at the normal row
+		if (kind != 0) return;
+		float ia[6], ib[6];
+		ld_iinv32(b.iinv32, (int)(C.key[k] >> 32), ia);
+		ld_iinv32(b.iinv32, (int)(C.key[k] & 0xFFFFFFFFu), ib);
+		const DvAsF32<DG> fa{A, A.dyn}, fb{B, B.dyn};
+		float r[CP_RSF];
+		ldv(rec<float>(C, k), r);
+		contact_phase_f32(rec<float>(C, k), r, fa, fb, phase, ia, ib);
+		return;
+	}
 	contact_row(rec<double>(C, k), A, B, kind, phase);
 }
 
@@ -1151,7 +1175,8 @@ __global__ __launch_bounds__(64) void k_pgs_islands(Bodies b, Contacts C, Joints
 	// plus a zero column for the non-dynamic ones), else b.dv
 	const int bo0 = S.boff[t], nbi = S.nbod[t];
 	const bool local = S.cloc && nbi + 1 <= S.lds_bodies;
-	const DvMap m = local ? DvMap{sdv, nbi + 1, S.cloc, S.jloc} : DvMap{b.dv, b.n, nullptr, nullptr};
+	const int split = P.precision == CP_PREC_SPLIT;
+	const DvMap m = local ? DvMap{sdv, nbi + 1, S.cloc, S.jloc, split} : DvMap{b.dv, b.n, nullptr, nullptr, split};
 	if (local)
 		for (int x = lane; x < 12 * (nbi + 1); x += 64) sdv[x] = 0;
 	for (int r = r0 + lane; r < r1; r += 64)

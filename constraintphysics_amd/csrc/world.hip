@@ -418,7 +418,7 @@ struct cp_world {
 		p.contact_life = cfg.contact_life;
 		p.gyro_ticks = cfg.gyro_ticks;
 		p.defect_fixes = cfg.defect_fixes;
-		p.precision = cfg.pgs_mode != CP_PGS_COLORED ? CP_PREC_FP64 : cfg.precision;
+		p.precision = cfg.pgs_mode == CP_PGS_REFERENCE_ORDER ? CP_PREC_FP64 : cfg.precision;  // the parity mode stays FP64
 		p.hol_iters = cfg.hol_iters;
 		return p;
 	}

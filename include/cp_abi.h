@@ -51,9 +51,11 @@ typedef enum {
 typedef enum { CP_PGS_COLORED = 0, CP_PGS_REFERENCE_ORDER = 1, CP_PGS_ISLAND_COLORED = 2 } cp_pgs_mode;
 
 /* CP_PREC_FP64: rows, impulses and velocity deltas in FP64 (the reference's
- * arithmetic).  CP_PREC_SPLIT: rows derived in FP64 and stored in FP32, PGS
- * sweeps, impulses within the solve and velocity deltas in FP32; body state,
- * integration, contact cache and narrowphase stay FP64. */
+ * arithmetic).  CP_PREC_SPLIT: contact rows derived in FP64 and stored in a
+ * compact FP32 record, PGS sweeps, impulses within the solve and (in the
+ * batched coloured mode) velocity deltas in FP32; body state, integration,
+ * contact cache, narrowphase, joint rows and the holonomic block solve stay
+ * FP64. */
 typedef enum { CP_PREC_FP64 = 0, CP_PREC_SPLIT = 1 } cp_precision;
 
 /* Engine parameters; defaults of PhysicsEngine.h:174-216 via cp_config_default. */
@@ -80,8 +82,10 @@ typedef struct {
 	                            reference's AABB::intersects, AABB.h:10-14, lets a NaN box overlap every
 	                            box, so one diverged body would pair with its whole scene).  Default 3;
 	                            0 = bug-compatible */
-	int precision;           /* cp_precision: solve precision of CP_PGS_COLORED (the island modes are
-	                            always FP64) */
+	int precision;           /* cp_precision: solve precision of CP_PGS_COLORED and of the contact rows
+	                            of CP_PGS_ISLAND_COLORED (joint rows, the holonomic block solve and the
+	                            island solve's velocity deltas stay FP64 there); CP_PGS_REFERENCE_ORDER,
+	                            the parity mode, is always FP64 */
 	int pgs_class_bodies[3]; /* CP_PGS_COLORED size classes: scenes of at most [0], [1], [2] bodies
 	                            (ascending, [2] <= 3072) keep their velocity deltas in LDS; larger
 	                            scenes use global memory.  Defaults 192, 768, 3072. */

@@ -82,6 +82,44 @@ def test_small_chains_island_colored_replayed_by_oracle():
     assert worst <= 1e-12
 
 
+def test_small_chains_island_colored_split_contacts():
+    """Coloured islands with the split precision (FP32 contact rows; joint
+    rows, the holonomic block solve and the deltas FP64) through the fall:
+    each step, from the GPU's own state, against the oracle's FP64 replay of
+    the recorded row order.  Bars: velocities within 1e-4 and poses within
+    1e-6 relative (scale max(1, |x|))."""
+    desc = scenes.c2_description(n_chains=6, links=16, grid=40, cell=0.5, amp=1.0, seed=7)
+    n = len(desc["hulls"])
+    no = cpa.reference_node_order(n)
+    cfg = dict(CFG, pgs_mode=2, precision=1)
+    probe = world_from_description(desc, **cfg)
+    probe.set_node_order(no)
+    probe.step(1)
+    off, og = _systems(probe)
+    probe.close()
+    w = world_from_description(desc, **cfg)
+    w.set_holonomic_systems(off, og)
+    w.set_node_order(no)
+    o = Oracle.from_description(desc, dt=CFG["dt"], gravity_y=-6.0, vel_iters=10, pos_iters=1, substeps=8, hol_iters=1)
+    rel = lambda a, b: float(np.max(np.abs(a - b) / np.maximum(1.0, np.abs(b)))) if a.size else 0.0
+    wv = wp = 0.0
+    for k in range(120):
+        pre = (w.state(), w.inertia(), w.contacts(), w.joint_state())
+        w.step(1)
+        if k < 60 and k % 10:  # check every tenth step through the free fall, every step once on the terrain
+            continue
+        o.load(pre[0], pre[1], pre[2])
+        o.load_joints(pre[3], off, og)
+        o.step_scheduled(*w.last_islands())
+        a, b = w.state(), o.state()
+        wp = max(wp, rel(a[:, :7], b[:, :7]))
+        wv = max(wv, rel(a[:, 7:], b[:, 7:]))
+    print(f"split contacts in coloured islands: worst pose rel {wp:.2e}, velocity rel {wv:.2e}, contacts {len(w.contacts()[0])}",
+          flush=True)
+    assert len(w.contacts()[0]) > 0
+    assert wv <= 1e-4 and wp <= 1e-6
+
+
 def _attach_error(state, joints):
     info, geo = joints
     err = 0.0
