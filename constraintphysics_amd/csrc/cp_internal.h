@@ -471,9 +471,6 @@ void launch_springs(const Bodies& b, const Joints& J, int ngroups, const int* go
 void launch_joint_rows(const Bodies& b, const Joints& J, const Params& P, double sub_dt, int mode, int first, hipStream_t s);
 // Block LDL factor of every holonomic system.
 void launch_hsys_factor(const Joints& J, const HSysDev& H, double cfm, hipStream_t s);
-void launch_pgs_colored_lds(const Bodies& b, const Contacts& C, const int* scene_list, int nlist, int max_bodies, const int* scene_body_off,
-                            const int* edge_range, const int* order, const int* color_off, const int* ncolors, const int* efirst,
-                            const int* ecnt, const Params& P, int threads, hipStream_t s);
 // rows: (index, kind) pairs, kind 0 contact normal, 1 friction, 2 joint;
 // isl_sys/isl_sys_off: holonomic systems per island; lin: [6][n] scratch.
 void launch_pgs_islands(const Bodies& b, const Contacts& C, const Joints& J, const HSysDev& H, const IslandSched& S, const Params& P,
