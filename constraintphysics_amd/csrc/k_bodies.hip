@@ -84,11 +84,13 @@ __global__ void k_gravity_sleep(Bodies b, Params p, SleepDev S) {
 }
 
 // update_positions (PhysicsEngine.cpp:390-410) followed by
-// RigidBody::updateInertiaTensor (RigidBody.cpp:270-279).
+// RigidBody::updateInertiaTensor (RigidBody.cpp:270-279).  For the FP32
+// solve it refreshes the body's FP32 inverse inertia too, so only a step's
+// first substep needs k_iinv32 (bodies it skips keep their inertia).
 __global__ __launch_bounds__(128) void k_integrate(Bodies b, Params p) {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= b.nl || b.movement[i] != 0 || (b.asleep && b.asleep[i])) return;
-	integrate_body(b, p, i);
+	integrate_body(b, p, i, p.precision == CP_PREC_SPLIT ? b.iinv32 : nullptr);
 }
 
 // updateInertiaTensor for every body without touching the stored orientation

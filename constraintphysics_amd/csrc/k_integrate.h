@@ -51,7 +51,10 @@ static __device__ void gyro_rotate(V3& angv, Q& q, const M3& I_world, const M3& 
 
 // update_positions (PhysicsEngine.cpp:390-410) followed by
 // RigidBody::updateInertiaTensor (RigidBody.cpp:270-279) for dynamic body i.
-static __device__ void integrate_body(const Bodies& b, const Params& p, int i) {
+// iinv32 (optional): also stores the new world inverse inertia's upper
+// triangle in FP32 there ([n][8]: xx xy xz yy yz zz, 2 pad), the values the
+// FP32 solve reads.
+static __device__ void integrate_body(const Bodies& b, const Params& p, int i, float* iinv32 = nullptr) {
 	const int n = b.n;
 	V3 com = LD3(b.com, i, n), vel = LD3(b.vel, i, n), angv = LD3(b.angv, i, n);
 	V3 pvel = LD3(b.pvel, i, n), pang = LD3(b.pang, i, n);
@@ -76,6 +79,11 @@ static __device__ void integrate_body(const Bodies& b, const Params& p, int i) {
 	stq(b.q, i, n, q);
 	stm(b.I, i, n, Iw);
 	stm(b.Iinv, i, n, Iwi);
+	if (iinv32) {
+		float4* o = reinterpret_cast<float4*>(iinv32 + 8 * (size_t)i);
+		o[0] = float4{(float)Iwi.m[0][0], (float)Iwi.m[0][1], (float)Iwi.m[0][2], (float)Iwi.m[1][1]};
+		o[1] = float4{(float)Iwi.m[1][2], (float)Iwi.m[2][2], 0.0f, 0.0f};
+	}
 }
 
 }  // namespace cpx

@@ -1721,7 +1721,7 @@ void cp_world::fused_substeps(const Bodies& b, const Params& P) {
 // Substep s of the coloured mode with the contact count read on the device.
 void cp_world::colored_substep(const Bodies& b, const Params& P, int s, const int* ncp) {
 	const int cap = cb[cur].c.cap;
-	if (P.precision == CP_PREC_SPLIT) launch_iinv32(b, stream);
+	if (P.precision == CP_PREC_SPLIT && s == 0) launch_iinv32(b, stream);  // later substeps: k_integrate refreshed it
 	launch_build_rows(b, cb[cur].c, cap, P, stream, ncp);
 	colored_pgs(b, P);
 	launch_rows_writeback(cb[cur].c, cap, P, stream, ncp);
@@ -1860,7 +1860,7 @@ void cp_world::step() {
 	for (int s = 0; s < cfg.substeps && !use_graph && !use_fused; s++) {
 		// spring impulses come before any row is built (PhysicsEngine.cpp:1313-1355)
 		launch_springs(b, J, sp_ngroups, sp_goff.p, sp_list.p, P.sub_dt, stream);
-		if (P.precision == CP_PREC_SPLIT) launch_iinv32(b, stream);
+		if (P.precision == CP_PREC_SPLIT && s == 0) launch_iinv32(b, stream);  // later substeps: k_integrate refreshed it
 		launch_build_rows(b, cb[cur].c, ncontacts, P, stream);
 		// joints; holonomic ones are current at the first substep (PhysicsEngine.cpp:1396-1403)
 		launch_joint_rows(b, J, P, P.sub_dt, 0, s == 0, stream);
