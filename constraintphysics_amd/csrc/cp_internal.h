@@ -455,8 +455,11 @@ void launch_edge_build(const Contacts& C, int n, const int* head, const int* eid
                        hipStream_t s);
 void launch_edge_slots(const Bodies& b, const Contacts& C, const int* order, int nslots, const int* efirst, const int* ecnt, int4* desc,
                        uint32_t* lmask, hipStream_t s);
+// Edge colouring of nscenes scenes (scene_list, or scenes 0..nscenes-1);
+// max_bodies: the largest of them.
 void launch_color(const Bodies& b, const int* scene_body_off, int nscenes, const uint64_t* ekey, const int* elive, int nedges,
-                  int* ecolor, int* order, int* color_off, int* ncolors, int* edge_range, int max_bodies, hipStream_t s);
+                  int* ecolor, int* order, int* color_off, int* ncolors, int* edge_range, int max_bodies, hipStream_t s,
+                  const int* scene_list = nullptr);
 void launch_build_rows(const Bodies& b, const Contacts& C, int nc, const Params& P, hipStream_t s, const int* ncp = nullptr);
 // The FP32 inverse inertia of every body (the split solve's rows read it).
 void launch_iinv32(const Bodies& b, hipStream_t s);

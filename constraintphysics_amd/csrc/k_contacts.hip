@@ -297,19 +297,20 @@ __device__ int color_edges(const Bodies& b, int b0, int e0, int e1, const uint64
 	return maxc;
 }
 
-// One wave per scene: greedy edge colouring in edge (cache key) order
-// (color_edges), then a counting sort of the scene's live edges by colour.
+// One wave per scene (scene_list[blockIdx.x], or blockIdx.x without a list):
+// greedy edge colouring in edge (cache key) order (color_edges), then a
+// counting sort of the scene's live edges by colour.
 // lds_words: 64-bit words of LDS per workgroup (at most CP_COLOR_LDS_WORDS).
 // A scene keeps both colour-set words in LDS when 2 x its bodies fit, word 0
 // when its bodies fit, else both words in b.color_mask.  Colour MAXC collects
 // edges that found no free colour; the solver runs that bucket on a single
 // thread.
-__global__ __launch_bounds__(64) void k_color(Bodies b, const int* scene_body_off, int nscenes, const uint64_t* ekey, const int* elive,
-                                              int nedges, int* ecolor, int* order, int* color_off, int* ncolors, int* edge_range,
-                                              int lds_words) {
+__global__ __launch_bounds__(64) void k_color(Bodies b, const int* scene_body_off, const int* scene_list, const uint64_t* ekey,
+                                              const int* elive, int nedges, int* ecolor, int* order, int* color_off, int* ncolors,
+                                              int* edge_range, int lds_words) {
 	extern __shared__ uint64_t smask_lds[];
 	__shared__ int cnt[MAXC + 1];
-	const int s = blockIdx.x, lane = threadIdx.x;
+	const int s = scene_list ? scene_list[blockIdx.x] : blockIdx.x, lane = threadIdx.x;
 	const int b0 = scene_body_off[s], b1 = scene_body_off[s + 1], nbs = b1 - b0;
 	const int e0 = lower_bound_u64(ekey, nedges, (uint64_t)b0 << 32), e1 = lower_bound_u64(ekey, nedges, (uint64_t)b1 << 32);
 	uint64_t* gmask = reinterpret_cast<uint64_t*>(b.color_mask) + 2 * (size_t)b0;  // [2][nbs] words of this scene
@@ -434,10 +435,13 @@ void launch_kill_contacts(const Contacts& C, int n, const int* ranges, int nrang
 }
 
 void launch_color(const Bodies& b, const int* scene_body_off, int nscenes, const uint64_t* ekey, const int* elive, int nedges,
-                  int* ecolor, int* order, int* color_off, int* ncolors, int* edge_range, int max_bodies, hipStream_t s) {
+                  int* ecolor, int* order, int* color_off, int* ncolors, int* edge_range, int max_bodies, hipStream_t s,
+                  const int* scene_list) {
 	if (!nscenes) return;
 	// colour-set words in LDS: both words of every scene when the largest fits,
-	// else word 0 of scenes up to CP_COLOR_LDS_WORDS bodies (159 KB)
+	// else word 0 of scenes up to CP_COLOR_LDS_WORDS bodies (159 KB).  The LDS a
+	// workgroup takes bounds how many run on a CU, so callers launch scenes of
+	// similar size together.
 	int lds_words = std::min(2 * max_bodies, CP_COLOR_LDS_WORDS);
 	static bool attr = false;
 	if (!attr) {
@@ -445,7 +449,7 @@ void launch_color(const Bodies& b, const int* scene_body_off, int nscenes, const
 			(void)hipGetLastError();  // the launch then reports the error itself
 		attr = true;
 	}
-	hipLaunchKernelGGL(k_color, dim3(nscenes), dim3(64), (size_t)8 * lds_words, s, b, scene_body_off, nscenes, ekey, elive, nedges, ecolor,
+	hipLaunchKernelGGL(k_color, dim3(nscenes), dim3(64), (size_t)8 * lds_words, s, b, scene_body_off, scene_list, ekey, elive, nedges, ecolor,
 	                   order, color_off, ncolors, edge_range, lds_words);
 }
 

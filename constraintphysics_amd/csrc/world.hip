@@ -484,6 +484,7 @@ struct cp_world {
 	void schedule_islands(bool colored);
 	void record_colored_schedule();
 	void launch_pgs_colored_lds_global(const Bodies& b, const Params& P);
+	void color_by_class(const Bodies& b);
 	void step();
 	void colored_pgs(const Bodies& b, const Params& P);
 	void colored_substep(const Bodies& b, const Params& P, int s, const int* ncp);
@@ -1694,6 +1695,23 @@ void cp_world::colored_pgs(const Bodies& b, const Params& P) {
 		}
 }
 
+// The edge colouring as one launch per PGS size class, each with LDS for its
+// largest scene (one launch sized for the largest scene of all would fit a
+// few small scenes' workgroups on a CU), the two smaller classes beside the
+// others on the side streams.
+void cp_world::color_by_class(const Bodies& b) {
+	HIPCHK(hipEventRecord(fork_ev, stream));
+	for (auto& x : side) HIPCHK(hipStreamWaitEvent(x, fork_ev, 0));
+	hipStream_t st[4] = {side[0], side[1], stream, stream};
+	for (int c = 3; c >= 0; c--)
+		launch_color(b, scene_body_off.p, (int)cls_lists[c].size(), e_key.p, e_live.p, nedges, e_color.p, e_order.p, color_off.p, ncolors.p,
+		             edge_range.p, cls_max[c], st[c], cls_dev[c].p);
+	for (int c = 0; c < 2; c++) {
+		HIPCHK(hipEventRecord(join_ev[c], side[c]));
+		HIPCHK(hipStreamWaitEvent(stream, join_ev[c], 0));
+	}
+}
+
 // The substep loop as one launch per size class (the largest class and the
 // global-memory one on the world's stream, the two smaller classes beside it).
 void cp_world::fused_substeps(const Bodies& b, const Params& P) {
@@ -1842,8 +1860,11 @@ void cp_world::step() {
 		e_desc.reserve(nedges);
 		e_lmask.reserve(nedges);
 		if (nedges) HIPCHK(hipMemsetAsync(e_order.p, 0xFF, (size_t)nedges * sizeof(int), stream));  // unused slots read -1
-		launch_color(b, scene_body_off.p, serving ? live_scenes : nscenes, e_key.p, e_live.p, nedges, e_color.p, e_order.p, color_off.p,
-		             ncolors.p, edge_range.p, max_scene_bodies, stream);
+		if (serving || cls_lists.size() != 4)
+			launch_color(b, scene_body_off.p, live_scenes, e_key.p, e_live.p, nedges, e_color.p, e_order.p, color_off.p, ncolors.p,
+			             edge_range.p, max_scene_bodies, stream);
+		else
+			color_by_class(b);
 		launch_edge_slots(b, cb[cur].c, e_order.p, nedges, e_first.p, e_cnt.p, e_desc.p, e_lmask.p, stream);
 		if (cfg.diagnostics & 2) record_colored_schedule();
 	}
